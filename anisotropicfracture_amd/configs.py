@@ -1,0 +1,164 @@
+"""Seeded synthetic inputs for the nodal elasticity workloads.
+
+Every generator here is this package's own and is deterministic given its
+seed.  The reference builds its microstructures with ``rand()``
+(src/IC/Voronoi.H:39-107, src/Set/Set.cpp:25-28), which is not reproducible
+across C libraries, so these replace it.
+
+Voronoi seeds are quantized to 1/1024 of a cell so that the nearest-seed
+search is exact integer arithmetic (the GPU kernel and the checker agree bit
+for bit on every node's grain, ties going to the lowest seed index).
+"""
+import numpy as np
+
+SEED = 20261203
+VORONOI_Q = 1024  # seed positions are integers in units of (cell size / VORONOI_Q)
+
+DISP, TRAC, PERIODIC, NEUMANN = 0, 1, 2, 3
+
+FACES3 = ["XLO", "YLO", "ZLO", "XHI", "YHI", "ZHI",
+          "YLO_ZLO", "YLO_ZHI", "YHI_ZLO", "YHI_ZHI",
+          "ZLO_XLO", "ZLO_XHI", "ZHI_XLO", "ZHI_XHI",
+          "XLO_YLO", "XLO_YHI", "XHI_YLO", "XHI_YHI",
+          "XLO_YLO_ZLO", "XLO_YLO_ZHI", "XLO_YHI_ZLO", "XLO_YHI_ZHI",
+          "XHI_YLO_ZLO", "XHI_YLO_ZHI", "XHI_YHI_ZLO", "XHI_YHI_ZHI"]
+FACES2 = ["XLO", "YLO", "XHI", "YHI", "XLO_YLO", "XLO_YHI", "XHI_YLO", "XHI_YHI"]
+
+# Cubic constants used by the cubic workloads (the reference's defaults, Cubic.H:97).
+C11, C12, C44 = 1.68, 1.21, 0.75
+
+
+def faces(dim):
+    return FACES3 if dim == 3 else FACES2
+
+
+def bc_tables(dim, spec):
+    """(type[nfaces,3] int32, value[nfaces,3] f64) from {face name: {comp: (type, value)}}.
+
+    Every face class not named defaults to Displacement 0, as in the reference
+    (BC/Operator/Elastic.H:46-55).
+    """
+    names = faces(dim)
+    t = np.zeros((len(names), 3), dtype=np.int32)
+    v = np.zeros((len(names), 3))
+    for face, comps in spec.items():
+        fi = names.index(face)
+        for c, (ty, val) in comps.items():
+            t[fi, c] = ty
+            v[fi, c] = val
+    return t, v
+
+
+def lame(E, nu):
+    """(lambda, mu) from (E, nu) in the reference's arithmetic order (Affine/Isotropic.H:67-74)."""
+    lam = E * nu / (1.0 + nu) / (1.0 - 2.0 * nu)
+    mu = E / 2.0 / (1.0 + nu)
+    return lam, mu
+
+
+def node_coords(n):
+    """Nodal coordinates on [0,1]^dim for n cells per side; arrays of shape reversed(nodes)."""
+    dim = len(n)
+    axes = [np.arange(n[d] + 1) / n[d] for d in range(dim)]
+    mesh = np.meshgrid(*reversed(axes), indexing="ij")
+    return list(reversed(mesh))
+
+
+# ------------------------------------------------------------------ config 1
+def cfg1(n=64):
+    """2D, n x n cells on [0,1]^2, isotropic E=210 nu=0.3 with a centred r=0.2 inclusion of 10x stiffness.
+
+    BCs: YLO displacement (0,0) (its corners too, the reference default);
+    YHI displacement (0, 0.01) including the YHI corners; XLO/XHI traction 0.
+    Returns (n, C[2, ny+1, nx+1] as (lambda, mu) planes, bc_type, bc_value).
+    """
+    lam, mu = lame(210.0, 0.3)
+    x, y = node_coords((n, n))
+    inc = (x - 0.5) ** 2 + (y - 0.5) ** 2 < 0.2 ** 2
+    scale = np.where(inc, 10.0, 1.0)
+    C = np.stack([lam * scale, mu * scale])
+    spec = {
+        "YHI": {0: (DISP, 0.0), 1: (DISP, 0.01)},
+        "XLO_YHI": {0: (DISP, 0.0), 1: (DISP, 0.01)},
+        "XHI_YHI": {0: (DISP, 0.0), 1: (DISP, 0.01)},
+        "XLO": {0: (TRAC, 0.0), 1: (TRAC, 0.0)},
+        "XHI": {0: (TRAC, 0.0), 1: (TRAC, 0.0)},
+    }
+    t, v = bc_tables(2, spec)
+    return (n, n), C, t, v
+
+
+# ------------------------------------------------------------------ grains
+def haar_rotations(rng, count):
+    """Haar-uniform rotation matrices from normalized Gaussian quaternions; shape (count, 3, 3)."""
+    q = rng.standard_normal((count, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    R = np.empty((count, 3, 3))
+    R[:, 0, 0] = 1 - 2 * (y * y + z * z)
+    R[:, 0, 1] = 2 * (x * y - z * w)
+    R[:, 0, 2] = 2 * (x * z + y * w)
+    R[:, 1, 0] = 2 * (x * y + z * w)
+    R[:, 1, 1] = 1 - 2 * (x * x + z * z)
+    R[:, 1, 2] = 2 * (y * z - x * w)
+    R[:, 2, 0] = 2 * (x * z - y * w)
+    R[:, 2, 1] = 2 * (y * z + x * w)
+    R[:, 2, 2] = 1 - 2 * (x * x + y * y)
+    return R
+
+
+def voronoi_seeds(rng, count, n):
+    """Uniform seeds in [0,1]^3 quantized to 1/VORONOI_Q of a cell: int64 array (count, 3)."""
+    u = rng.random((count, 3))
+    scale = np.array([n[0], n[1], n[2]], dtype=np.float64) * VORONOI_Q
+    return np.rint(u * scale).astype(np.int64)
+
+
+def voronoi_grain_ids(seeds_q, n, chunk=1 << 16):
+    """Nearest-seed grain id per node by exact integer distance (ties -> lowest index).
+
+    Returns int32 array of shape (nz+1, ny+1, nx+1).  Used by tests as the
+    checker of the GPU grain assignment.
+    """
+    nx, ny, nz = (v + 1 for v in n)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    P = np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1).astype(np.int64) * VORONOI_Q
+    out = np.empty(P.shape[0], dtype=np.int32)
+    S = seeds_q.astype(np.int64)
+    for s in range(0, P.shape[0], chunk):
+        d = ((P[s:s + chunk, None, :] - S[None, :, :]) ** 2).sum(axis=2)
+        out[s:s + chunk] = np.argmin(d, axis=1)
+    return out.reshape(nz, ny, nx)
+
+
+def uniaxial_z_bcs(strain=0.01):
+    """ZLO fixed (all comps), ZHI u_z = strain with x/y traction free, lateral faces traction free.
+
+    Edges and corners follow the face whose z-condition applies.
+    """
+    zlo = {0: (DISP, 0.0), 1: (DISP, 0.0), 2: (DISP, 0.0)}
+    zhi = {0: (TRAC, 0.0), 1: (TRAC, 0.0), 2: (DISP, strain)}
+    free = {0: (TRAC, 0.0), 1: (TRAC, 0.0), 2: (TRAC, 0.0)}
+    spec = {}
+    for f in FACES3:
+        parts = f.split("_")
+        if "ZLO" in parts:
+            spec[f] = zlo
+        elif "ZHI" in parts:
+            spec[f] = zhi
+        else:
+            spec[f] = free
+    return bc_tables(3, spec)
+
+
+def cfg2(n=256, grains=64, seed=SEED):
+    """3D, n^3 cells, cubic (1.68,1.21,0.75) Voronoi polycrystal with Haar orientations, uniaxial z strain 0.01.
+
+    Returns a dict with the grid size, quantized seeds, rotations and BC tables.
+    """
+    rng = np.random.default_rng(seed)
+    size = (n, n, n)
+    seeds = voronoi_seeds(rng, grains, size)
+    R = haar_rotations(rng, grains)
+    t, v = uniaxial_z_bcs(0.01)
+    return {"n": size, "seeds": seeds, "R": R, "C11": C11, "C12": C12, "C44": C44, "bc_type": t, "bc_value": v}

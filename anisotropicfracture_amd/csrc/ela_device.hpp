@@ -1,0 +1,387 @@
+// ela_device.hpp -- per-node math of the nodal elastic operator for gfx950.
+//
+// The discrete operator is the reference's (src/Operator/Elastic.cpp:151-260):
+// on interior nodes f = C : grad grad u + sum_d (d_d C : grad u) e_d with
+// central differences; on domain-boundary nodes the boundary row of
+// BC::Operator::Elastic (src/BC/Operator/Elastic.H:222-303), with one-sided
+// first differences on the boundary (src/Numeric/Stencil.H:16-102).
+//
+// Second and mixed differences are evaluated differences-first,
+// ((u+ - u) + (u- - u)) / h^2 and ((u++ - u+-) - (u-+ - u--)) / (4 h_a h_b):
+// the same stencil as u+ - 2u + u- but without the cancellation of two
+// O(|u|) terms, which keeps the FP64 residual floor near eps * |h grad u|
+// rather than eps * |u|.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ela {
+
+enum { K_ISO = 0, K_MM = 1, K_DIAG = 2 };
+enum { BC_DISP = 0, BC_TRAC = 1, BC_PER = 2, BC_NEU = 3 };
+
+// One MG level: a single box of nodes, component-major, x fastest.
+struct Lay {
+    int nx, ny, nz;   // node counts (nz = 1 in 2D)
+    int dim;
+    long long px;     // pitch between j rows
+    long long pl;     // pitch between k planes
+    long long cs;     // component stride
+};
+
+// Boundary-condition types per face class and component (faces numbered as
+// the reference's BC::Operator::Elastic::Face enum).
+struct BCTab {
+    unsigned char t[26][3];
+};
+
+// Inverse spacings used by the stencils.
+template <typename T>
+struct Inv {
+    T ih[3];     // 1 / h
+    T ih2[3];    // 0.5 / h       (central first difference)
+    T ihh[3];    // 1 / h^2
+    T imix[3];   // 0.25 / (h_a h_b) for the pairs (x,y), (x,z), (y,z)
+};
+
+template <int DIM, int K>
+struct NComp;
+template <int DIM> struct NComp<DIM, K_ISO> { static constexpr int v = 2; };
+template <> struct NComp<3, K_MM> { static constexpr int v = 21; };
+template <> struct NComp<2, K_MM> { static constexpr int v = 6; };
+template <int DIM> struct NComp<DIM, K_DIAG> { static constexpr int v = DIM * DIM; };
+
+// Voigt index of a symmetric pair, order (00,01,02,11,12,22) in 3D and (00,01,11) in 2D.
+template <int DIM>
+__host__ __device__ constexpr int voigt(int i, int j) {
+    return DIM == 3 ? ((i <= j) ? (i == 0 ? j : (i == 1 ? 2 + j : 5)) : (j == 0 ? i : (j == 1 ? 2 + i : 5)))
+                    : (i + j);
+}
+
+// Slot of C_ijkl in the upper-triangular Voigt storage.
+template <int DIM>
+__host__ __device__ constexpr int mm_slot(int i, int j, int k, int l) {
+    const int nv = DIM == 3 ? 6 : 3;
+    const int a = voigt<DIM>(i, j), b = voigt<DIM>(k, l);
+    const int I = a < b ? a : b, J = a < b ? b : a;
+    return I * nv - I * (I - 1) / 2 + (J - I);
+}
+
+// Face classification with the reference's precedence (corners, then the YZ,
+// ZX and XY edges, then faces; BC/Operator/Elastic.H:239-278).  Returns -1 for
+// interior nodes.
+template <int DIM>
+__device__ __forceinline__ int face_of(int i, int j, int k, int hx, int hy, int hz) {
+    const bool xl = i == 0, xh = i == hx, yl = j == 0, yh = j == hy;
+    if (DIM == 2) {
+        if (xl && yl) return 4;
+        if (xl && yh) return 5;
+        if (xh && yl) return 6;
+        if (xh && yh) return 7;
+        if (xl) return 0;
+        if (xh) return 2;
+        if (yl) return 1;
+        if (yh) return 3;
+        return -1;
+    }
+    const bool zl = k == 0, zh = k == hz;
+    if ((xl || xh) && (yl || yh) && (zl || zh)) return 18 + (xh ? 4 : 0) + (yh ? 2 : 0) + (zh ? 1 : 0);
+    if ((yl || yh) && (zl || zh)) return 6 + (yh ? 2 : 0) + (zh ? 1 : 0);
+    if ((zl || zh) && (xl || xh)) return 10 + (zh ? 2 : 0) + (xh ? 1 : 0);
+    if ((xl || xh) && (yl || yh)) return 14 + (xh ? 2 : 0) + (yh ? 1 : 0);
+    if (xl) return 0;
+    if (xh) return 3;
+    if (yl) return 1;
+    if (yh) return 4;
+    if (zl) return 2;
+    if (zh) return 5;
+    return -1;
+}
+
+// Outward normal of a face class; the reference's truncated 1/sqrt(3) and
+// 1/sqrt(2) (BC/Operator/Elastic.H:215-216) on corners and edges.
+template <typename T, int DIM>
+__device__ __forceinline__ void normal_of(int face, T n[3]) {
+    const T s3 = T(0.57735026919), s2 = T(0.70710678118);
+    n[0] = n[1] = n[2] = T(0);
+    if (DIM == 2) {
+        switch (face) {
+            case 0: n[0] = -1; break;
+            case 1: n[1] = -1; break;
+            case 2: n[0] = 1; break;
+            case 3: n[1] = 1; break;
+            default: {
+                const int c = face - 4;  // XLO_YLO, XLO_YHI, XHI_YLO, XHI_YHI
+                n[0] = (c & 2) ? s2 : -s2;
+                n[1] = (c & 1) ? s2 : -s2;
+            }
+        }
+        return;
+    }
+    if (face < 6) {
+        n[face % 3] = face < 3 ? T(-1) : T(1);
+    } else if (face < 10) {  // Y?_Z?
+        const int c = face - 6;
+        n[1] = (c & 2) ? s2 : -s2;
+        n[2] = (c & 1) ? s2 : -s2;
+    } else if (face < 14) {  // Z?_X?
+        const int c = face - 10;
+        n[2] = (c & 2) ? s2 : -s2;
+        n[0] = (c & 1) ? s2 : -s2;
+    } else if (face < 18) {  // X?_Y?
+        const int c = face - 14;
+        n[0] = (c & 2) ? s2 : -s2;
+        n[1] = (c & 1) ? s2 : -s2;
+    } else {
+        const int c = face - 18;
+        n[0] = (c & 4) ? s3 : -s3;
+        n[1] = (c & 2) ? s3 : -s3;
+        n[2] = (c & 1) ? s3 : -s3;
+    }
+}
+
+// sigma_ij = C_ijkl g_kl at one node, stiffness planes c[q * cs].
+template <typename T, int K, int DIM>
+__device__ __forceinline__ void stress(const T* __restrict__ c, long long cs, const T g[DIM][DIM], T s[DIM][DIM]) {
+    if (K == K_ISO) {
+        const T lam = c[0], mu = c[cs];
+        T tr = 0;
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) tr += g[a][a];
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+#pragma unroll
+            for (int b = 0; b < DIM; ++b) s[a][b] = mu * (g[a][b] + g[b][a]) + (a == b ? lam * tr : T(0));
+    } else if (K == K_MM) {
+        T gs[DIM][DIM];
+#pragma unroll
+        for (int k = 0; k < DIM; ++k)
+#pragma unroll
+            for (int l = 0; l < DIM; ++l) gs[k][l] = (k == l) ? g[k][k] : g[k][l] + g[l][k];
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+#pragma unroll
+            for (int b = a; b < DIM; ++b) {
+                T acc = 0;
+#pragma unroll
+                for (int k = 0; k < DIM; ++k)
+#pragma unroll
+                    for (int l = k; l < DIM; ++l) acc += c[mm_slot<DIM>(a, b, k, l) * cs] * gs[k][l];
+                s[a][b] = acc;
+                s[b][a] = acc;
+            }
+    } else {
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+#pragma unroll
+            for (int b = 0; b < DIM; ++b) s[a][b] = c[(a * DIM + b) * cs] * g[a][b];
+    }
+}
+
+// Boundary row (BC/Operator/Elastic.H:286-303).
+template <typename T, int DIM>
+__device__ __forceinline__ void bc_row(const BCTab& bc, int face, const T u[DIM], const T g[DIM][DIM],
+                                       const T s[DIM][DIM], T f[DIM]) {
+    T n[3];
+    normal_of<T, DIM>(face, n);
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+        const int t = bc.t[face][a];
+        T v = 0;
+        if (t == BC_DISP) {
+            v = u[a];
+        } else if (t == BC_TRAC) {
+#pragma unroll
+            for (int b = 0; b < DIM; ++b) v += s[a][b] * n[b];
+        } else if (t == BC_NEU) {
+#pragma unroll
+            for (int b = 0; b < DIM; ++b) v += g[a][b] * n[b];
+        }
+        f[a] = v;
+    }
+}
+
+// f = A u at node (i,j,k).  C points at plane 0 of the stiffness, u at comp 0.
+template <typename T, int K, int DIM>
+__device__ __forceinline__ void node_apply(const Lay& L, const T* __restrict__ C, const T* __restrict__ u, int i,
+                                           int j, int k, const BCTab& bc, const Inv<T>& iv, bool uniform, T f[DIM]) {
+    const long long o = (long long)i + L.px * j + L.pl * k;
+    const int hx = L.nx - 1, hy = L.ny - 1, hz = L.nz - 1;
+    const bool bnd = (i == 0 || i == hx || j == 0 || j == hy || (DIM == 3 && (k == 0 || k == hz)));
+    const long long st[3] = {1, L.px, L.pl};
+    const int idx[3] = {i, j, k};
+    const int top[3] = {hx, hy, hz};
+    if (bnd) {
+        T uc[DIM], g[DIM][DIM], s[DIM][DIM];
+#pragma unroll
+        for (int p = 0; p < DIM; ++p) {
+            const T* up = u + p * L.cs + o;
+            uc[p] = up[0];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                if (idx[d] == 0) g[p][d] = (up[st[d]] - up[0]) * iv.ih[d];
+                else if (idx[d] == top[d]) g[p][d] = (up[0] - up[-st[d]]) * iv.ih[d];
+                else g[p][d] = (up[st[d]] - up[-st[d]]) * iv.ih2[d];
+            }
+        }
+        stress<T, K, DIM>(C + o, L.cs, g, s);
+        bc_row<T, DIM>(bc, face_of<DIM>(i, j, k, hx, hy, hz), uc, g, s, f);
+        return;
+    }
+    T g[DIM][DIM], gg[DIM][DIM][DIM];  // gg[p][a][b] = u_{p,ab}
+#pragma unroll
+    for (int p = 0; p < DIM; ++p) {
+        const T* up = u + p * L.cs + o;
+        const T c0 = up[0];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const T vp = up[st[d]], vm = up[-st[d]];
+            g[p][d] = (vp - vm) * iv.ih2[d];
+            gg[p][d][d] = ((vp - c0) + (vm - c0)) * iv.ihh[d];
+        }
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+#pragma unroll
+            for (int b = a + 1; b < DIM; ++b) {
+                const long long sa = st[a], sb = st[b];
+                const T pp = up[sa + sb], pm = up[sa - sb], mp = up[-sa + sb], mm = up[-sa - sb];
+                const T m = ((pp - pm) - (mp - mm)) * iv.imix[a + b - 1];
+                gg[p][a][b] = m;
+                gg[p][b][a] = m;
+            }
+    }
+    const T* c = C + o;
+    const long long cs = L.cs;
+    if (K == K_ISO) {
+        const T lam = c[0], mu = c[cs];
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) {
+            T lap = 0, gd = 0;
+#pragma unroll
+            for (int b = 0; b < DIM; ++b) {
+                lap += gg[a][b][b];
+                gd += gg[b][a][b];
+            }
+            f[a] = mu * lap + (mu + lam) * gd;
+        }
+        if (!uniform) {
+            T tr = 0;
+#pragma unroll
+            for (int a = 0; a < DIM; ++a) tr += g[a][a];
+            T dl[DIM], dm[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                dl[d] = (c[st[d]] - c[-st[d]]) * iv.ih2[d];
+                dm[d] = (c[cs + st[d]] - c[cs - st[d]]) * iv.ih2[d];
+            }
+#pragma unroll
+            for (int a = 0; a < DIM; ++a) {
+                T add = dl[a] * tr;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) add += dm[d] * (g[a][d] + g[d][a]);
+                f[a] += add;
+            }
+        }
+    } else if (K == K_MM) {
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) {
+            T acc = 0;
+#pragma unroll
+            for (int b = 0; b < DIM; ++b)
+#pragma unroll
+                for (int kk = 0; kk < DIM; ++kk)
+#pragma unroll
+                    for (int l = 0; l < DIM; ++l) acc += c[mm_slot<DIM>(a, b, kk, l) * cs] * gg[kk][l][b];
+            f[a] = acc;
+        }
+        if (!uniform) {
+            T gs[DIM][DIM];
+#pragma unroll
+            for (int kk = 0; kk < DIM; ++kk)
+#pragma unroll
+                for (int l = 0; l < DIM; ++l) gs[kk][l] = (kk == l) ? g[kk][kk] : g[kk][l] + g[l][kk];
+#pragma unroll
+            for (int a = 0; a < DIM; ++a) {
+                T add = 0;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    T acc = 0;
+#pragma unroll
+                    for (int kk = 0; kk < DIM; ++kk)
+#pragma unroll
+                        for (int l = kk; l < DIM; ++l) {
+                            const long long q = mm_slot<DIM>(a, d, kk, l) * cs;
+                            acc += (c[q + st[d]] - c[q - st[d]]) * gs[kk][l];
+                        }
+                    add += acc * iv.ih2[d];
+                }
+                f[a] += add;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) {
+            T acc = 0;
+#pragma unroll
+            for (int b = 0; b < DIM; ++b) acc += c[(a * DIM + b) * cs] * gg[a][b][b];
+            f[a] = acc;
+        }
+        if (!uniform) {
+#pragma unroll
+            for (int a = 0; a < DIM; ++a) {
+                T add = 0;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    const long long q = (a * DIM + d) * cs;
+                    add += (c[q + st[d]] - c[q - st[d]]) * iv.ih2[d] * g[a][d];
+                }
+                f[a] += add;
+            }
+        }
+    }
+}
+
+// Diagonal of A at node (i,j,k) (Elastic.cpp:285-349), FP64.
+template <int K, int DIM>
+__device__ __forceinline__ void node_diag(const Lay& L, const double* __restrict__ C, int i, int j, int k,
+                                          const BCTab& bc, const double h[3], double dg[DIM]) {
+    const long long o = (long long)i + L.px * j + L.pl * k;
+    const int hx = L.nx - 1, hy = L.ny - 1, hz = L.nz - 1;
+    const bool mn[3] = {i == 0, j == 0, DIM == 3 && k == 0};
+    const bool mx[3] = {i == hx, j == hy, DIM == 3 && k == hz};
+    const bool bnd = mn[0] || mx[0] || mn[1] || mx[1] || mn[2] || mx[2];
+    const double* c = C + o;
+    const long long cs = L.cs;
+#pragma unroll
+    for (int p = 0; p < DIM; ++p) {
+        if (bnd) {
+            double g[DIM][DIM], s[DIM][DIM], uu[DIM], f[DIM];
+#pragma unroll
+            for (int q = 0; q < DIM; ++q) {
+                uu[q] = (q == p) ? 1.0 : 0.0;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    const double e = (p == q) ? 1.0 : 0.0;
+                    g[q][d] = ((!mx[d] ? 0.0 : e) - (!mn[d] ? 0.0 : e)) / ((mn[d] || mx[d] ? 1.0 : 2.0) * h[d]);
+                }
+            }
+            stress<double, K, DIM>(c, cs, g, s);
+            bc_row<double, DIM>(bc, face_of<DIM>(i, j, k, hx, hy, hz), uu, g, s, f);
+            dg[p] = f[p];
+        } else {
+            // f_p = sum_j C_{p j p j} (-2 / h_j^2): the only entries of C : grad grad e_p at the node.
+            double acc = 0;
+#pragma unroll
+            for (int b = 0; b < DIM; ++b) {
+                double cpp;
+                if (K == K_ISO) cpp = c[cs] + (b == p ? c[cs] + c[0] : 0.0);
+                else if (K == K_MM) cpp = c[mm_slot<DIM>(p, b, p, b) * cs];
+                else cpp = c[(p * DIM + b) * cs];
+                acc += cpp * (-2.0 / h[b] / h[b]);
+            }
+            dg[p] = acc;
+        }
+    }
+}
+
+}  // namespace ela

@@ -1,0 +1,580 @@
+// ela_kernels.hip -- gfx950 kernels of the nodal elasticity multigrid and
+// their launchers.  The operator math lives in ela_device.hpp.
+//
+// Launch shapes: node kernels run one thread per node on 64 x 4 thread
+// blocks (one wavefront per x row of 64 nodes, so loads along x coalesce),
+// grid (ceil(nx/64), ceil(ny/4), nz).  Every kernel checks its node indices
+// against the level's node counts before touching memory.
+#include "ela_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+namespace ela {
+
+namespace {
+
+constexpr int BX = 64, BY = 4;
+
+__device__ __forceinline__ bool node_index(const Lay& L, int& i, int& j, int& k) {
+    i = blockIdx.x * BX + threadIdx.x;
+    j = blockIdx.y * BY + threadIdx.y;
+    k = blockIdx.z;
+    return i < L.nx && j < L.ny && k < L.nz;
+}
+
+inline dim3 node_grid(const Lay& L) { return dim3((L.nx + BX - 1) / BX, (L.ny + BY - 1) / BY, L.nz); }
+
+// Wavefront max of a non-negative value, then one atomicMax per wavefront on
+// the bit pattern (ordered like the value for non-negative IEEE numbers; a
+// NaN pattern orders above +inf so it propagates).
+__device__ __forceinline__ void wave_max_atomic(unsigned long long* dst, double v) {
+    unsigned long long b = __double_as_longlong(v);
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(b, off, 64);
+        b = o > b ? o : b;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(dst, b);
+}
+
+template <typename T, int K, int DIM>
+__global__ __launch_bounds__(256) void k_fapply(Lay L, const T* __restrict__ C, const T* __restrict__ u,
+                                                T* __restrict__ f, BCTab bc, Inv<T> iv, int uniform) {
+    int i, j, k;
+    if (!node_index(L, i, j, k)) return;
+    T r[DIM];
+    node_apply<T, K, DIM>(L, C, u, i, j, k, bc, iv, uniform != 0, r);
+    const long long o = (long long)i + L.px * j + L.pl * k;
+#pragma unroll
+    for (int p = 0; p < DIM; ++p) f[p * L.cs + o] = r[p];
+}
+
+// r = b - A x; optional outputs: r in T, r * scale in TO (the next cycle's
+// right-hand side), and max |r| via atomicMax.  All-lanes participate in the
+// wavefront reduction (out-of-range lanes contribute 0).
+template <typename T, typename TO, int K, int DIM>
+__global__ __launch_bounds__(256) void k_residual(Lay L, const T* __restrict__ C, const T* __restrict__ x,
+                                                  const T* __restrict__ b, T* __restrict__ r, TO* __restrict__ rs,
+                                                  const double* __restrict__ scale, unsigned long long* norm,
+                                                  BCTab bc, Inv<T> iv, int uniform) {
+    int i, j, k;
+    const bool in = node_index(L, i, j, k);
+    double m = 0.0;
+    if (in) {
+        T f[DIM];
+        node_apply<T, K, DIM>(L, C, x, i, j, k, bc, iv, uniform != 0, f);
+        const long long o = (long long)i + L.px * j + L.pl * k;
+        const double s = rs ? *scale : 1.0;
+#pragma unroll
+        for (int p = 0; p < DIM; ++p) {
+            const T v = b[p * L.cs + o] - f[p];
+            if (r) r[p * L.cs + o] = v;
+            if (rs) rs[p * L.cs + o] = (TO)((double)v * s);
+            const double a = fabs((double)v);
+            m = (a > m || a != a) ? a : m;
+        }
+    }
+    if (norm) wave_max_atomic(norm, m);
+}
+
+// One weighted-Jacobi sweep x_out = x_in + omega (b - A x_in) / D.
+template <typename T, int K, int DIM>
+__global__ __launch_bounds__(256) void k_jacobi(Lay L, const T* __restrict__ C, const T* __restrict__ dg,
+                                                const T* __restrict__ xin, const T* __restrict__ b,
+                                                T* __restrict__ xout, BCTab bc, Inv<T> iv, int uniform, T omega) {
+    int i, j, k;
+    if (!node_index(L, i, j, k)) return;
+    T f[DIM];
+    node_apply<T, K, DIM>(L, C, xin, i, j, k, bc, iv, uniform != 0, f);
+    const long long o = (long long)i + L.px * j + L.pl * k;
+#pragma unroll
+    for (int p = 0; p < DIM; ++p) {
+        const long long q = p * L.cs + o;
+        xout[q] = xin[q] + omega * (b[q] - f[p]) / dg[q];
+    }
+}
+
+// The first Jacobi sweep from x = 0: x = omega b / D.
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void k_jacobi0(Lay L, const T* __restrict__ dg, const T* __restrict__ b,
+                                                 T* __restrict__ xout, T omega) {
+    int i, j, k;
+    if (!node_index(L, i, j, k)) return;
+    const long long o = (long long)i + L.px * j + L.pl * k;
+#pragma unroll
+    for (int p = 0; p < DIM; ++p) {
+        const long long q = p * L.cs + o;
+        xout[q] = omega * b[q] / dg[q];
+    }
+}
+
+template <int K, int DIM>
+__global__ __launch_bounds__(256) void k_diag(Lay L, const double* __restrict__ C, double* __restrict__ d64,
+                                              float* __restrict__ d32, BCTab bc, double hx, double hy, double hz,
+                                              unsigned long long* nan_flag) {
+    int i, j, k;
+    if (!node_index(L, i, j, k)) return;
+    const double h[3] = {hx, hy, hz};
+    double dg[DIM];
+    node_diag<K, DIM>(L, C, i, j, k, bc, h, dg);
+    const long long o = (long long)i + L.px * j + L.pl * k;
+#pragma unroll
+    for (int p = 0; p < DIM; ++p) {
+        if (d64) d64[p * L.cs + o] = dg[p];
+        if (d32) d32[p * L.cs + o] = (float)dg[p];
+        if (!(dg[p] == dg[p]) || dg[p] == 0.0) atomicMax(nan_flag, 1ull);
+    }
+}
+
+// Nodal full weighting from the fine level (Lf) onto the coarse level (Lc),
+// nc components (Operator.cpp:486-543; the same stencil coarsens the stiffness,
+// Elastic.cpp:759-803).  In 2D the k direction counts as "on the boundary"
+// (the reference's degenerate z box), which selects the 2D branches.
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void k_restrict(Lay Lc, Lay Lf, int nc, const T* __restrict__ fine,
+                                                  T* __restrict__ crse) {
+    int I, J, K;
+    if (!node_index(Lc, I, J, K)) return;
+    const int i = 2 * I, j = 2 * J, k = 2 * K;
+    const bool bI = I == 0 || I == Lc.nx - 1, bJ = J == 0 || J == Lc.ny - 1;
+    const bool bK = DIM == 2 ? true : (K == 0 || K == Lc.nz - 1);
+    const long long sx = 1, sy = Lf.px, sz = Lf.pl;
+    const long long of = (long long)i + Lf.px * j + Lf.pl * k;
+    const long long oc = (long long)I + Lc.px * J + Lc.pl * K;
+    for (int n = 0; n < nc; ++n) {
+        const T* F = fine + n * Lf.cs + of;
+        T v;
+        if (bI && bJ && bK) {
+            v = F[0];
+        } else if (bJ && bK) {
+            v = T(0.25) * F[-sx] + T(0.5) * F[0] + T(0.25) * F[sx];
+        } else if (bK && bI) {
+            v = T(0.25) * F[-sy] + T(0.5) * F[0] + T(0.25) * F[sy];
+        } else if (bI && bJ) {
+            v = T(0.25) * F[-sz] + T(0.5) * F[0] + T(0.25) * F[sz];
+        } else if (bI || bJ || bK) {
+            // face: (1 2 1) x (1 2 1) / 16 over the two tangential directions
+            const long long u1 = bI ? sy : sx, u2 = bK ? sy : sz;
+            v = (F[-u1 - u2] + T(2) * F[-u2] + F[u1 - u2] + T(2) * F[-u1] + T(4) * F[0] + T(2) * F[u1] +
+                 F[-u1 + u2] + T(2) * F[u2] + F[u1 + u2]) /
+                T(16);
+        } else {
+            T corners = 0, edges = 0, faces = 0;
+#pragma unroll
+            for (int a = -1; a <= 1; a += 2)
+#pragma unroll
+                for (int bb = -1; bb <= 1; bb += 2)
+#pragma unroll
+                    for (int c = -1; c <= 1; c += 2) corners += F[a * sx + bb * sy + c * sz];
+#pragma unroll
+            for (int a = -1; a <= 1; a += 2)
+#pragma unroll
+                for (int bb = -1; bb <= 1; bb += 2) edges += F[a * sy + bb * sz] + F[a * sx + bb * sz] + F[a * sx + bb * sy];
+#pragma unroll
+            for (int a = -1; a <= 1; a += 2) faces += F[a * sx] + F[a * sy] + F[a * sz];
+            v = corners / T(64) + edges / T(32) + faces / T(16) + F[0] / T(8);
+        }
+        crse[n * Lc.cs + oc] = v;
+    }
+}
+
+// Trilinear nodal prolongation (Operator.cpp:589-618): fine (+)= P crse.
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void k_prolong(Lay Lf, Lay Lc, int nc, const T* __restrict__ crse,
+                                                 T* __restrict__ fine, int add) {
+    int i, j, k;
+    if (!node_index(Lf, i, j, k)) return;
+    const int I = i >> 1, J = j >> 1, K = k >> 1;
+    const bool ei = (i & 1) == 0, ej = (j & 1) == 0, ek = (k & 1) == 0;
+    const long long sx = 1, sy = Lc.px, sz = Lc.pl;
+    const long long oc = (long long)I + Lc.px * J + Lc.pl * K;
+    const long long of = (long long)i + Lf.px * j + Lf.pl * k;
+    for (int n = 0; n < nc; ++n) {
+        const T* c = crse + n * Lc.cs + oc;
+        T v;
+        if (ei && ej && ek) v = c[0];
+        else if (ej && ek) v = T(0.5) * (c[0] + c[sx]);
+        else if (ek && ei) v = T(0.5) * (c[0] + c[sy]);
+        else if (ei && ej) v = T(0.5) * (c[0] + c[sz]);
+        else if (ei) v = T(0.25) * (c[0] + c[sy] + c[sz] + c[sy + sz]);
+        else if (ej) v = T(0.25) * (c[0] + c[sz] + c[sx] + c[sx + sz]);
+        else if (ek) v = T(0.25) * (c[0] + c[sx] + c[sy] + c[sx + sy]);
+        else v = T(0.125) * (c[0] + c[sx] + c[sy] + c[sz] + c[sy + sz] + c[sx + sz] + c[sx + sy] + c[sx + sy + sz]);
+        T* fp = fine + n * Lf.cs + of;
+        *fp = add ? (*fp + v) : v;
+    }
+}
+
+// y (+)= a * x over nc components of a level (elementwise on valid nodes).
+template <typename TX, typename TY>
+__global__ __launch_bounds__(256) void k_axpy(Lay L, int nc, const TX* __restrict__ x, TY* __restrict__ y,
+                                              const double* __restrict__ ascale, double a, int assign) {
+    int i, j, k;
+    if (!node_index(L, i, j, k)) return;
+    const long long o = (long long)i + L.px * j + L.pl * k;
+    const double s = ascale ? a / *ascale : a;
+    for (int n = 0; n < nc; ++n) {
+        const long long q = n * L.cs + o;
+        const double v = (double)x[q] * s;
+        y[q] = assign ? (TY)v : (TY)((double)y[q] + v);
+    }
+}
+
+template <typename TX, typename TY>
+__global__ __launch_bounds__(256) void k_convert(long long n, const TX* __restrict__ x, TY* __restrict__ y) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) y[q] = (TY)x[q];
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) p[q] = 0ull;
+}
+
+// scale = 2^-e with 2^e the power of two at or above |r|inf (exact in any precision).
+__global__ void k_scale_from_norm(const unsigned long long* norm, double* scale) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const double r = __longlong_as_double((long long)*norm);
+        int e = 0;
+        if (r > 0.0 && r < 1e300) frexp(r, &e);
+        *scale = ldexp(1.0, -e);
+    }
+}
+
+// ---------------------------------------------------------------- dense coarsest level
+// Dense (row-major) operator of the coarsest level: A[r][q] = (A e_q)[r],
+// unknowns numbered comp * nodes + dense node index.  One thread per (node, column q).
+__global__ __launch_bounds__(256) void k_assemble_col(Lay L, int nodes, const double* __restrict__ F, int q,
+                                                      int nunk, double* __restrict__ A, int dim) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nodes) return;
+    const int i = t % L.nx, j = (t / L.nx) % L.ny, k = t / (L.nx * L.ny);
+    const long long o = (long long)i + L.px * j + L.pl * k;
+    for (int p = 0; p < dim; ++p) A[(long long)(p * nodes + t) * nunk + q] = F[p * L.cs + o];
+}
+
+// Gauss-Jordan inversion with partial pivoting, one workgroup; M is n x 2n
+// row-major ([A | I] on entry, [I | A^-1] on exit).  flag != 0 on a zero pivot.
+__global__ __launch_bounds__(1024) void k_gauss_jordan(double* __restrict__ M, int n, int* flag) {
+    __shared__ double sval[1024];
+    __shared__ int sidx[1024];
+    __shared__ int piv;
+    const int tid = threadIdx.x, nt = blockDim.x, w = 2 * n;
+    for (int c = 0; c < n; ++c) {
+        double best = -1.0;
+        int bi = c;
+        for (int r = c + tid; r < n; r += nt) {
+            const double a = fabs(M[(long long)r * w + c]);
+            if (a > best) { best = a; bi = r; }
+        }
+        sval[tid] = best;
+        sidx[tid] = bi;
+        __syncthreads();
+        for (int s = nt / 2; s > 0; s >>= 1) {
+            if (tid < s) {
+                if (sval[tid + s] > sval[tid] || (sval[tid + s] == sval[tid] && sidx[tid + s] < sidx[tid])) {
+                    sval[tid] = sval[tid + s];
+                    sidx[tid] = sidx[tid + s];
+                }
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            piv = sidx[0];
+            if (!(sval[0] > 0.0)) *flag = 1;
+        }
+        __syncthreads();
+        const int p = piv;
+        if (p != c)
+            for (int q = tid; q < w; q += nt) {
+                const double t = M[(long long)c * w + q];
+                M[(long long)c * w + q] = M[(long long)p * w + q];
+                M[(long long)p * w + q] = t;
+            }
+        __syncthreads();
+        const double d = M[(long long)c * w + c];
+        __syncthreads();
+        for (int q = tid; q < w; q += nt) M[(long long)c * w + q] /= d;
+        __syncthreads();
+        for (long long e = tid; e < (long long)n * w; e += nt) {
+            const int r = (int)(e / w), q = (int)(e % w);
+            if (r == c || q == c) continue;
+            const double m = M[(long long)r * w + c];
+            if (m != 0.0) M[e] -= m * M[(long long)c * w + q];
+        }
+        __syncthreads();
+        for (int r = tid; r < n; r += nt)
+            if (r != c) M[(long long)r * w + c] = 0.0;
+        __syncthreads();
+    }
+}
+
+// x = A^-1 b on the coarsest level (b, x in the level layout, precision T).
+template <typename T>
+__global__ __launch_bounds__(256) void k_bottom(Lay L, int nodes, int dim, const double* __restrict__ M,
+                                                const T* __restrict__ b, T* __restrict__ x) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nunk = nodes * dim;
+    if (r >= nunk) return;
+    const double* row = M + (long long)r * (2 * nunk) + nunk;
+    double acc = 0.0;
+    for (int p = 0; p < dim; ++p)
+        for (int t = 0; t < nodes; ++t) {
+            const int i = t % L.nx, j = (t / L.nx) % L.ny, k = t / (L.nx * L.ny);
+            acc += row[p * nodes + t] * (double)b[p * L.cs + i + L.px * j + L.pl * k];
+        }
+    const int p = r / nodes, t = r % nodes;
+    const int i = t % L.nx, j = (t / L.nx) % L.ny, k = t / (L.nx * L.ny);
+    x[p * L.cs + i + L.px * j + L.pl * k] = (T)acc;
+}
+
+// ---------------------------------------------------------------- stiffness evaluation
+// Per-grain cubic stiffness (Linear::Cubic::Define, Cubic.H:30-59): the
+// unrotated Ctmp, rotated R_pi R_sk Ctmp_ijkl R_qj R_tl, written slot by slot
+// in p,q,s,t order so the last tuple of each symmetry class sets the value.
+// One thread per grain.
+__global__ void k_cubic_table(int ng, const double* __restrict__ R, double C11, double C12, double C44,
+                              double* __restrict__ table) {
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= ng) return;
+    const double* r = R + 9 * gi;
+    double* out = table + 21 * gi;
+    for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q)
+            for (int s = 0; s < 3; ++s)
+                for (int t = 0; t < 3; ++t) {
+                    double acc = 0.0;
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j)
+                            for (int k = 0; k < 3; ++k)
+                                for (int l = 0; l < 3; ++l) {
+                                    double ct;
+                                    if (i == j && j == k && k == l) ct = C11;
+                                    else if (i == k && j == l) ct = C44;
+                                    else if (i == j && k == l) ct = C12;
+                                    else ct = 0.0;
+                                    acc += r[3 * p + i] * r[3 * s + k] * ct * r[3 * q + j] * r[3 * t + l];
+                                }
+                    out[mm_slot<3>(p, q, s, t)] = acc;
+                }
+}
+
+// Nearest Voronoi seed per node by exact int64 distance (ties -> lowest
+// index), then the grain's 21 values into the stiffness planes.
+__global__ __launch_bounds__(256) void k_voronoi_cubic(Lay L, int nseeds, const long long* __restrict__ seeds, int q,
+                                                       const double* __restrict__ table, double* __restrict__ C,
+                                                       int* __restrict__ gid) {
+    int i, j, k;
+    if (!node_index(L, i, j, k)) return;
+    const long long x = (long long)i * q, y = (long long)j * q, z = (long long)k * q;
+    long long best = -1;
+    int bi = 0;
+    for (int s = 0; s < nseeds; ++s) {
+        const long long dx = x - seeds[3 * s], dy = y - seeds[3 * s + 1], dz = z - seeds[3 * s + 2];
+        const long long d = dx * dx + dy * dy + dz * dz;
+        if (best < 0 || d < best) { best = d; bi = s; }
+    }
+    const long long o = (long long)i + L.px * j + L.pl * k;
+    for (int c = 0; c < 21; ++c) C[c * L.cs + o] = table[21 * bi + c];
+    if (gid) gid[o] = bi;
+}
+
+__global__ __launch_bounds__(256) void k_degraded_iso(Lay L, const double* __restrict__ eta, double lambda0,
+                                                      double mu0, double a, double b, double c,
+                                                      double* __restrict__ C) {
+    int i, j, k;
+    if (!node_index(L, i, j, k)) return;
+    const long long o = (long long)i + L.px * j + L.pl * k;
+    const double e = eta[o];
+    const double g = (a * e + b) * e + c;
+    C[o] = lambda0 * g;
+    C[L.cs + o] = mu0 * g;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_pack(Lay L, int nc, const double* __restrict__ dense, T* __restrict__ dev,
+                                              int to_dev) {
+    int i, j, k;
+    if (!node_index(L, i, j, k)) return;
+    const long long o = (long long)i + L.px * j + L.pl * k;
+    const long long d = (long long)i + (long long)L.nx * (j + (long long)L.ny * k);
+    const long long nn = (long long)L.nx * L.ny * L.nz;
+    for (int n = 0; n < nc; ++n) {
+        if (to_dev) dev[n * L.cs + o] = (T)dense[n * nn + d];
+        else ((double*)dense)[n * nn + d] = (double)dev[n * L.cs + o];
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+#define ELA_DISPATCH(KIND, DIMV, ...)                                              \
+    do {                                                                           \
+        if ((DIMV) == 3) {                                                         \
+            if ((KIND) == K_ISO) { constexpr int KK = K_ISO, DD = 3; __VA_ARGS__; } \
+            else if ((KIND) == K_MM) { constexpr int KK = K_MM, DD = 3; __VA_ARGS__; } \
+            else { constexpr int KK = K_DIAG, DD = 3; __VA_ARGS__; }              \
+        } else {                                                                   \
+            if ((KIND) == K_ISO) { constexpr int KK = K_ISO, DD = 2; __VA_ARGS__; } \
+            else if ((KIND) == K_MM) { constexpr int KK = K_MM, DD = 2; __VA_ARGS__; } \
+            else { constexpr int KK = K_DIAG, DD = 2; __VA_ARGS__; }              \
+        }                                                                          \
+    } while (0)
+
+template <typename T>
+Inv<T> make_inv(const double h[3]) {
+    Inv<T> iv;
+    for (int d = 0; d < 3; ++d) {
+        iv.ih[d] = (T)(1.0 / h[d]);
+        iv.ih2[d] = (T)(0.5 / h[d]);
+        iv.ihh[d] = (T)(1.0 / (h[d] * h[d]));
+    }
+    iv.imix[0] = (T)(0.25 / (h[0] * h[1]));
+    iv.imix[1] = (T)(0.25 / (h[0] * h[2]));
+    iv.imix[2] = (T)(0.25 / (h[1] * h[2]));
+    return iv;
+}
+
+template <typename T>
+void launch_fapply(const Lay& L, int kind, const T* C, const T* u, T* f, const BCTab& bc, const double h[3],
+                   int uniform, hipStream_t s) {
+    const Inv<T> iv = make_inv<T>(h);
+    ELA_DISPATCH(kind, L.dim,
+                 hipLaunchKernelGGL((k_fapply<T, KK, DD>), node_grid(L), dim3(BX, BY), 0, s, L, C, u, f, bc, iv,
+                                    uniform));
+}
+
+template <typename T, typename TO>
+void launch_residual(const Lay& L, int kind, const T* C, const T* x, const T* b, T* r, TO* rs, const double* scale,
+                     unsigned long long* norm, const BCTab& bc, const double h[3], int uniform, hipStream_t s) {
+    const Inv<T> iv = make_inv<T>(h);
+    ELA_DISPATCH(kind, L.dim,
+                 hipLaunchKernelGGL((k_residual<T, TO, KK, DD>), node_grid(L), dim3(BX, BY), 0, s, L, C, x, b, r,
+                                    rs, scale, norm, bc, iv, uniform));
+}
+
+template <typename T>
+void launch_jacobi(const Lay& L, int kind, const T* C, const T* dg, const T* xin, const T* b, T* xout,
+                   const BCTab& bc, const double h[3], int uniform, double omega, hipStream_t s) {
+    const Inv<T> iv = make_inv<T>(h);
+    ELA_DISPATCH(kind, L.dim,
+                 hipLaunchKernelGGL((k_jacobi<T, KK, DD>), node_grid(L), dim3(BX, BY), 0, s, L, C, dg, xin, b,
+                                    xout, bc, iv, uniform, (T)omega));
+}
+
+template <typename T>
+void launch_jacobi0(const Lay& L, const T* dg, const T* b, T* xout, double omega, hipStream_t s) {
+    if (L.dim == 3) hipLaunchKernelGGL((k_jacobi0<T, 3>), node_grid(L), dim3(BX, BY), 0, s, L, dg, b, xout, (T)omega);
+    else hipLaunchKernelGGL((k_jacobi0<T, 2>), node_grid(L), dim3(BX, BY), 0, s, L, dg, b, xout, (T)omega);
+}
+
+void launch_diag(const Lay& L, int kind, const double* C, double* d64, float* d32, const BCTab& bc,
+                 const double h[3], unsigned long long* flag, hipStream_t s) {
+    ELA_DISPATCH(kind, L.dim,
+                 hipLaunchKernelGGL((k_diag<KK, DD>), node_grid(L), dim3(BX, BY), 0, s, L, C, d64, d32, bc, h[0],
+                                    h[1], h[2], flag));
+}
+
+template <typename T>
+void launch_restrict(const Lay& Lc, const Lay& Lf, int nc, const T* fine, T* crse, hipStream_t s) {
+    if (Lc.dim == 3) hipLaunchKernelGGL((k_restrict<T, 3>), node_grid(Lc), dim3(BX, BY), 0, s, Lc, Lf, nc, fine, crse);
+    else hipLaunchKernelGGL((k_restrict<T, 2>), node_grid(Lc), dim3(BX, BY), 0, s, Lc, Lf, nc, fine, crse);
+}
+
+template <typename T>
+void launch_prolong(const Lay& Lf, const Lay& Lc, int nc, const T* crse, T* fine, int add, hipStream_t s) {
+    if (Lf.dim == 3) hipLaunchKernelGGL((k_prolong<T, 3>), node_grid(Lf), dim3(BX, BY), 0, s, Lf, Lc, nc, crse, fine, add);
+    else hipLaunchKernelGGL((k_prolong<T, 2>), node_grid(Lf), dim3(BX, BY), 0, s, Lf, Lc, nc, crse, fine, add);
+}
+
+template <typename TX, typename TY>
+void launch_axpy(const Lay& L, int nc, const TX* x, TY* y, const double* ascale, double a, int assign, hipStream_t s) {
+    hipLaunchKernelGGL((k_axpy<TX, TY>), node_grid(L), dim3(BX, BY), 0, s, L, nc, x, y, ascale, a, assign);
+}
+
+template <typename TX, typename TY>
+void launch_convert(long long n, const TX* x, TY* y, hipStream_t s) {
+    const int bs = 256;
+    hipLaunchKernelGGL((k_convert<TX, TY>), dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, n, x, y);
+}
+
+void launch_zero_u64(unsigned long long* p, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u64, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+}
+
+void launch_scale_from_norm(const unsigned long long* norm, double* scale, hipStream_t s) {
+    hipLaunchKernelGGL(k_scale_from_norm, dim3(1), dim3(64), 0, s, norm, scale);
+}
+
+void launch_assemble_col(const Lay& L, int nodes, const double* F, int q, int nunk, double* A, int dim,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_assemble_col, dim3((nodes + 255) / 256), dim3(256), 0, s, L, nodes, F, q, nunk, A, dim);
+}
+
+void launch_gauss_jordan(double* M, int n, int* flag, hipStream_t s) {
+    hipLaunchKernelGGL(k_gauss_jordan, dim3(1), dim3(1024), 0, s, M, n, flag);
+}
+
+template <typename T>
+void launch_bottom(const Lay& L, int nodes, int dim, const double* M, const T* b, T* x, hipStream_t s) {
+    const int nunk = nodes * dim;
+    hipLaunchKernelGGL((k_bottom<T>), dim3((nunk + 255) / 256), dim3(256), 0, s, L, nodes, dim, M, b, x);
+}
+
+void launch_cubic_table(int ng, const double* R, double C11, double C12, double C44, double* table, hipStream_t s) {
+    hipLaunchKernelGGL(k_cubic_table, dim3((ng + 63) / 64), dim3(64), 0, s, ng, R, C11, C12, C44, table);
+}
+
+void launch_voronoi_cubic(const Lay& L, int nseeds, const long long* seeds, int q, const double* table, double* C,
+                          int* gid, hipStream_t s) {
+    hipLaunchKernelGGL(k_voronoi_cubic, node_grid(L), dim3(BX, BY), 0, s, L, nseeds, seeds, q, table, C, gid);
+}
+
+void launch_degraded_iso(const Lay& L, const double* eta, double lambda0, double mu0, double a, double b, double c,
+                         double* C, hipStream_t s) {
+    hipLaunchKernelGGL(k_degraded_iso, node_grid(L), dim3(BX, BY), 0, s, L, eta, lambda0, mu0, a, b, c, C);
+}
+
+template <typename T>
+void launch_pack(const Lay& L, int nc, const double* dense, T* dev, int to_dev, hipStream_t s) {
+    hipLaunchKernelGGL((k_pack<T>), node_grid(L), dim3(BX, BY), 0, s, L, nc, dense, dev, to_dev);
+}
+
+// explicit instantiations
+template void launch_fapply<double>(const Lay&, int, const double*, const double*, double*, const BCTab&,
+                                    const double*, int, hipStream_t);
+template void launch_fapply<float>(const Lay&, int, const float*, const float*, float*, const BCTab&, const double*,
+                                   int, hipStream_t);
+template void launch_residual<double, double>(const Lay&, int, const double*, const double*, const double*, double*,
+                                              double*, const double*, unsigned long long*, const BCTab&,
+                                              const double*, int, hipStream_t);
+template void launch_residual<double, float>(const Lay&, int, const double*, const double*, const double*, double*,
+                                             float*, const double*, unsigned long long*, const BCTab&, const double*,
+                                             int, hipStream_t);
+template void launch_residual<float, float>(const Lay&, int, const float*, const float*, const float*, float*,
+                                            float*, const double*, unsigned long long*, const BCTab&, const double*,
+                                            int, hipStream_t);
+template void launch_jacobi<double>(const Lay&, int, const double*, const double*, const double*, const double*,
+                                    double*, const BCTab&, const double*, int, double, hipStream_t);
+template void launch_jacobi<float>(const Lay&, int, const float*, const float*, const float*, const float*, float*,
+                                   const BCTab&, const double*, int, double, hipStream_t);
+template void launch_jacobi0<double>(const Lay&, const double*, const double*, double*, double, hipStream_t);
+template void launch_jacobi0<float>(const Lay&, const float*, const float*, float*, double, hipStream_t);
+template void launch_restrict<double>(const Lay&, const Lay&, int, const double*, double*, hipStream_t);
+template void launch_restrict<float>(const Lay&, const Lay&, int, const float*, float*, hipStream_t);
+template void launch_prolong<double>(const Lay&, const Lay&, int, const double*, double*, int, hipStream_t);
+template void launch_prolong<float>(const Lay&, const Lay&, int, const float*, float*, int, hipStream_t);
+template void launch_axpy<double, double>(const Lay&, int, const double*, double*, const double*, double, int,
+                                          hipStream_t);
+template void launch_axpy<float, double>(const Lay&, int, const float*, double*, const double*, double, int,
+                                         hipStream_t);
+template void launch_axpy<float, float>(const Lay&, int, const float*, float*, const double*, double, int,
+                                        hipStream_t);
+template void launch_convert<double, float>(long long, const double*, float*, hipStream_t);
+template void launch_bottom<double>(const Lay&, int, int, const double*, const double*, double*, hipStream_t);
+template void launch_bottom<float>(const Lay&, int, int, const double*, const float*, float*, hipStream_t);
+template void launch_pack<double>(const Lay&, int, const double*, double*, int, hipStream_t);
+template void launch_pack<float>(const Lay&, int, const double*, float*, int, hipStream_t);
+
+}  // namespace ela

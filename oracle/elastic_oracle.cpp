@@ -1,0 +1,791 @@
+// elastic_oracle.cpp -- CPU FP64 restatement of the reference's nodal
+// elasticity path.  TEST INFRASTRUCTURE ONLY: this file is the checker the
+// parity tests, smoke() and bench.py's cpu_baseline leg compare against.  The
+// product (anisotropicfracture_amd/) never links, loads or calls it.
+//
+// What it restates (reference file:line, relative to the reference checkout):
+//   * Numeric::GetStencil / Stencil<...>::D     src/Numeric/Stencil.H:16-176
+//   * Operator::Elastic<T>::Fapply              src/Operator/Elastic.cpp:151-260
+//   * Operator::Elastic<T>::Diagonal            src/Operator/Elastic.cpp:264-352
+//   * averageDownCoeffsSameAmrLevel             src/Operator/Elastic.cpp:722-809
+//   * Operator<Node>::Fsmooth (Jacobi)          src/Operator/Operator.cpp:90-162
+//   * Operator<Node>::restriction               src/Operator/Operator.cpp:454-554
+//   * Operator<Node>::interpolation             src/Operator/Operator.cpp:556-624
+//   * Operator<Node>::reflux                    src/Operator/Operator.cpp:728-839
+//   * BC::Operator::Elastic operator()/set/Init src/BC/Operator/Elastic.H:106-303
+//   * Matrix4<.,Isotropic|MajorMinor|Diagonal>  src/Set/Matrix4_*.H (storage + contractions)
+//   * Linear::Cubic::Define                     src/Model/Solid/Linear/Cubic.H:21-59
+//   * MLMG F/V-cycle driver (AMReX, not vendored; called from
+//     src/Solver/Nonlocal/Linear.H:23-29,102-112): restated from AMReX's
+//     published MLMG design -- nu1 = nu2 = 2 smoother calls, F-cycles while
+//     iter < max_fmg_iter, convergence |r|inf <= max(tol_abs, tol_rel*|r0|inf).
+//     The coarsest level is solved either exactly (dense LU) or by BiCGStab to
+//     a relative 1e-4 (the reference selects BiCGStab, Linear.H:25).
+//
+// Storage conventions (shared with the product's C ABI):
+//   * a field with nc components on a grid of (n+1)^dim nodes is stored
+//     component-major, x fastest: v[c][k][j][i].
+//   * stiffness kinds: 0 = isotropic (lambda, mu)             Matrix4_Isotropic.H:15
+//                      1 = major/minor symmetric, 21 (3D) or 6 (2D) values in
+//                          the reference's slot order (Matrix4_MajorMinor.H:19-31,148-189)
+//                      2 = diagonal, dim*dim values A(i,j) row-major (Matrix4_Diagonal.H:12-21)
+//   * boundary conditions: type[face][comp] (0 displacement, 1 traction,
+//     2 periodic, 3 neumann) and value[face][comp], faces numbered as the
+//     reference's BC::Operator::Elastic::Face enum (BC/Operator/Elastic.H:24-42).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+extern "C" {
+
+struct or_grid {
+    int dim;       // 2 or 3
+    int n[3];      // cells per direction (n[2] ignored in 2D)
+    double dx[3];  // cell size per direction
+};
+
+}  // extern "C"
+
+namespace {
+
+enum { BC_DISP = 0, BC_TRAC = 1, BC_PERIODIC = 2, BC_NEUMANN = 3 };
+enum { K_ISO = 0, K_MM = 1, K_DIAG = 2 };
+
+// Truncated constants exactly as the reference writes them (BC/Operator/Elastic.H:215-216).
+const double kSqrt3Inv = 0.57735026919;
+const double kSqrt2Inv = 0.70710678118;
+
+struct G {
+    int dim;
+    int n[3];
+    int nn[3];
+    double dx[3];
+    long nodes;
+    explicit G(const or_grid* g) {
+        dim = g->dim;
+        for (int d = 0; d < 3; ++d) {
+            n[d] = (d < dim) ? g->n[d] : 0;
+            nn[d] = n[d] + 1;
+            dx[d] = (d < dim) ? g->dx[d] : 1.0;
+        }
+        nodes = (long)nn[0] * nn[1] * nn[2];
+    }
+    long id(int i, int j, int k) const { return i + (long)nn[0] * (j + (long)nn[1] * k); }
+};
+
+int ncomp_of(int dim, int kind) {
+    if (kind == K_ISO) return 2;
+    if (kind == K_MM) return dim == 3 ? 21 : 6;
+    return dim * dim;
+}
+
+// Voigt index of the symmetric pair (i,j): 3D order (00,01,02,11,12,22), 2D (00,01,11).
+int voigt(int dim, int i, int j) {
+    if (i > j) std::swap(i, j);
+    if (dim == 3) {
+        static const int t[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+        return t[i][j];
+    }
+    static const int t2[2][2] = {{0, 1}, {1, 2}};
+    return t2[i][j];
+}
+
+// Slot of C_ijkl in the upper-triangular Voigt storage.
+int mm_slot(int dim, int i, int j, int k, int l) {
+    int I = voigt(dim, i, j), J = voigt(dim, k, l);
+    if (I > J) std::swap(I, J);
+    int nv = dim == 3 ? 6 : 3;
+    return I * nv - I * (I - 1) / 2 + (J - I);
+}
+
+// Full-tensor accessor for one node's stiffness record c[0..ncomp).
+double Cijkl(int dim, int kind, const double* c, int i, int j, int k, int l) {
+    if (kind == K_ISO) {
+        double lambda = c[0], mu = c[1], r = 0.0;
+        if (i == k && j == l) r += mu;
+        if (i == l && j == k) r += mu;
+        if (i == j && k == l) r += lambda;
+        return r;
+    }
+    if (kind == K_MM) return c[mm_slot(dim, i, j, k, l)];
+    return (i == k && j == l) ? c[i * dim + j] : 0.0;
+}
+
+// sigma_ij = C_ijkl g_kl
+void contract2(int dim, int kind, const double* c, const double g[3][3], double s[3][3]) {
+    for (int i = 0; i < dim; ++i)
+        for (int j = 0; j < dim; ++j) {
+            double acc = 0.0;
+            for (int k = 0; k < dim; ++k)
+                for (int l = 0; l < dim; ++l) acc += Cijkl(dim, kind, c, i, j, k, l) * g[k][l];
+            s[i][j] = acc;
+        }
+}
+
+// f_i = C_ijkl gg[k][l][j]   with gg[k][l][j] = u_{k,lj}   (Set::Matrix3 convention, Matrix3.H:8-47)
+void contract3(int dim, int kind, const double* c, const double gg[3][3][3], double f[3]) {
+    for (int i = 0; i < dim; ++i) {
+        double acc = 0.0;
+        for (int j = 0; j < dim; ++j)
+            for (int k = 0; k < dim; ++k)
+                for (int l = 0; l < dim; ++l) acc += Cijkl(dim, kind, c, i, j, k, l) * gg[k][l][j];
+        f[i] = acc;
+    }
+}
+
+// Face classification with the reference's precedence: corners, then edges
+// (YZ, ZX, XY), then faces (BC/Operator/Elastic.H:239-278).  Returns the face
+// enum value, or -1 for an interior node.  n_out receives the outward normal.
+int classify(const G& g, int i, int j, int k, double n_out[3]) {
+    const int hx = g.n[0], hy = g.n[1], hz = g.n[2];
+    n_out[0] = n_out[1] = n_out[2] = 0.0;
+    if (g.dim == 2) {
+        struct R { bool hit; int face; double nx, ny; };
+        const R rows[] = {
+            {i == 0 && j == 0, 4, -kSqrt2Inv, -kSqrt2Inv},
+            {i == 0 && j == hy, 5, -kSqrt2Inv, +kSqrt2Inv},
+            {i == hx && j == 0, 6, +kSqrt2Inv, -kSqrt2Inv},
+            {i == hx && j == hy, 7, +kSqrt2Inv, +kSqrt2Inv},
+            {i == 0, 0, -1, 0},
+            {i == hx, 2, +1, 0},
+            {j == 0, 1, 0, -1},
+            {j == hy, 3, 0, +1},
+        };
+        for (const R& r : rows)
+            if (r.hit) { n_out[0] = r.nx; n_out[1] = r.ny; return r.face; }
+        return -1;
+    }
+    const bool xl = i == 0, xh = i == hx, yl = j == 0, yh = j == hy, zl = k == 0, zh = k == hz;
+    const double s3 = kSqrt3Inv, s2 = kSqrt2Inv;
+    struct R { bool hit; int face; double nx, ny, nz; };
+    const R rows[] = {
+        {xl && yl && zl, 18, -s3, -s3, -s3}, {xl && yl && zh, 19, -s3, -s3, +s3},
+        {xl && yh && zl, 20, -s3, +s3, -s3}, {xl && yh && zh, 21, -s3, +s3, +s3},
+        {xh && yl && zl, 22, +s3, -s3, -s3}, {xh && yl && zh, 23, +s3, -s3, +s3},
+        {xh && yh && zl, 24, +s3, +s3, -s3}, {xh && yh && zh, 25, +s3, +s3, +s3},
+        {yl && zl, 6, 0, -s2, -s2},  {yl && zh, 7, 0, -s2, +s2},
+        {yh && zl, 8, 0, +s2, -s2},  {yh && zh, 9, 0, +s2, +s2},
+        {zl && xl, 10, -s2, 0, -s2}, {zl && xh, 11, +s2, 0, -s2},
+        {zh && xl, 12, -s2, 0, +s2}, {zh && xh, 13, +s2, 0, +s2},
+        {xl && yl, 14, -s2, -s2, 0}, {xl && yh, 15, -s2, +s2, 0},
+        {xh && yl, 16, +s2, -s2, 0}, {xh && yh, 17, +s2, +s2, 0},
+        {xl, 0, -1, 0, 0}, {xh, 3, +1, 0, 0},
+        {yl, 1, 0, -1, 0}, {yh, 4, 0, +1, 0},
+        {zl, 2, 0, 0, -1}, {zh, 5, 0, 0, +1},
+    };
+    for (const R& r : rows)
+        if (r.hit) { n_out[0] = r.nx; n_out[1] = r.ny; n_out[2] = r.nz; return r.face; }
+    return -1;
+}
+
+// BC row (BC/Operator/Elastic.H:286-303).
+void bc_row(int dim, const int* type3, const double u[3], const double g[3][3], const double s[3][3],
+            const double nrm[3], double f[3]) {
+    for (int i = 0; i < dim; ++i) {
+        f[i] = 0.0;
+        int t = type3[i];
+        if (t == BC_DISP) f[i] = u[i];
+        else if (t == BC_TRAC) { double a = 0; for (int j = 0; j < dim; ++j) a += s[i][j] * nrm[j]; f[i] = a; }
+        else if (t == BC_NEUMANN) { double a = 0; for (int j = 0; j < dim; ++j) a += g[i][j] * nrm[j]; f[i] = a; }
+    }
+}
+
+// First derivative with the reference's one-sided stencil at the domain
+// boundary (Stencil.H:16-102): at the low wall forward, at the high wall backward.
+double d1(const G& g, const double* v, int comp, int i, int j, int k, int d) {
+    int ix[3] = {i, j, k};
+    auto at = [&](int off) {
+        int q[3] = {ix[0], ix[1], ix[2]};
+        q[d] += off;
+        return v[comp * g.nodes + g.id(q[0], q[1], q[2])];
+    };
+    if (ix[d] == 0) return (at(1) - at(0)) / g.dx[d];
+    if (ix[d] == g.n[d]) return (at(0) - at(-1)) / g.dx[d];
+    return (at(1) - at(-1)) * 0.5 / g.dx[d];
+}
+
+double d2(const G& g, const double* v, int comp, int i, int j, int k, int d) {
+    int q0[3] = {i, j, k}, qp[3] = {i, j, k}, qm[3] = {i, j, k};
+    qp[d] += 1; qm[d] -= 1;
+    double c = v[comp * g.nodes + g.id(q0[0], q0[1], q0[2])];
+    double p = v[comp * g.nodes + g.id(qp[0], qp[1], qp[2])];
+    double m = v[comp * g.nodes + g.id(qm[0], qm[1], qm[2])];
+    return (p - 2.0 * c + m) / g.dx[d] / g.dx[d];
+}
+
+double dmix(const G& g, const double* v, int comp, int i, int j, int k, int a, int b) {
+    auto at = [&](int sa, int sb) {
+        int q[3] = {i, j, k};
+        q[a] += sa; q[b] += sb;
+        return v[comp * g.nodes + g.id(q[0], q[1], q[2])];
+    };
+    return 0.25 * (at(1, 1) + at(-1, -1) - at(1, -1) - at(-1, 1)) / (g.dx[a] * g.dx[b]);
+}
+
+void load_c(const G& g, int nc, const double* C, int i, int j, int k, double* c) {
+    long id = g.id(i, j, k);
+    for (int q = 0; q < nc; ++q) c[q] = C[q * g.nodes + id];
+}
+
+bool on_boundary(const G& g, int i, int j, int k) {
+    if (i == 0 || i == g.n[0] || j == 0 || j == g.n[1]) return true;
+    return g.dim == 3 && (k == 0 || k == g.n[2]);
+}
+
+// Fapply at one node (Elastic.cpp:174-257).
+void fapply_node(const G& g, int kind, const double* C, const int* bctype, int uniform, const double* U,
+                 int i, int j, int k, double f[3]) {
+    const int dim = g.dim, nc = ncomp_of(dim, kind);
+    double u[3] = {0, 0, 0}, gu[3][3] = {{0}}, s[3][3] = {{0}}, c[21];
+    for (int p = 0; p < dim; ++p) u[p] = U[p * g.nodes + g.id(i, j, k)];
+    for (int p = 0; p < dim; ++p)
+        for (int d = 0; d < dim; ++d) gu[p][d] = d1(g, U, p, i, j, k, d);
+    load_c(g, nc, C, i, j, k, c);
+    contract2(dim, kind, c, gu, s);
+    f[0] = f[1] = f[2] = 0.0;
+    if (on_boundary(g, i, j, k)) {
+        double nrm[3];
+        int face = classify(g, i, j, k, nrm);
+        bc_row(dim, bctype + 3 * face, u, gu, s, nrm, f);
+        return;
+    }
+    double gg[3][3][3] = {{{0}}};
+    for (int p = 0; p < dim; ++p) {
+        for (int d = 0; d < dim; ++d) gg[p][d][d] = d2(g, U, p, i, j, k, d);
+        for (int a = 0; a < dim; ++a)
+            for (int b = a + 1; b < dim; ++b) {
+                double m = dmix(g, U, p, i, j, k, a, b);
+                gg[p][a][b] = m;
+                gg[p][b][a] = m;
+            }
+    }
+    contract3(dim, kind, c, gg, f);
+    if (!uniform) {
+        // sum_d (dC/dx_d : grad u) column d, with dC central ((C+ - C-)*0.5)/dx  (Elastic.cpp:247-255)
+        double add[3] = {0, 0, 0};
+        for (int d = 0; d < dim; ++d) {
+            int qp[3] = {i, j, k}, qm[3] = {i, j, k};
+            qp[d] += 1; qm[d] -= 1;
+            double cp[21], cm[21], dc[21], sd[3][3];
+            load_c(g, nc, C, qp[0], qp[1], qp[2], cp);
+            load_c(g, nc, C, qm[0], qm[1], qm[2], cm);
+            for (int q = 0; q < nc; ++q) dc[q] = ((cp[q] - cm[q]) * 0.5) / g.dx[d];
+            contract2(dim, kind, dc, gu, sd);
+            for (int p = 0; p < dim; ++p) add[p] += sd[p][d];
+        }
+        for (int p = 0; p < dim; ++p) f[p] += add[p];
+    }
+}
+
+// Diagonal at one node (Elastic.cpp:285-349).
+void diag_node(const G& g, int kind, const double* C, const int* bctype, int i, int j, int k, double dg[3]) {
+    const int dim = g.dim, nc = ncomp_of(dim, kind);
+    const bool mn[3] = {i == 0, j == 0, dim == 3 && k == 0};
+    const bool mx[3] = {i == g.n[0], j == g.n[1], dim == 3 && k == g.n[2]};
+    double c[21];
+    load_c(g, nc, C, i, j, k, c);
+    const bool bnd = on_boundary(g, i, j, k);
+    for (int p = 0; p < dim; ++p) {
+        double gu[3][3] = {{0}}, gg[3][3][3] = {{{0}}}, s[3][3];
+        for (int q = 0; q < dim; ++q)
+            for (int d = 0; d < dim; ++d) {
+                double e = (p == q) ? 1.0 : 0.0;
+                gu[q][d] = ((!mx[d] ? 0.0 : e) - (!mn[d] ? 0.0 : e)) / ((mn[d] || mx[d] ? 1.0 : 2.0) * g.dx[d]);
+                gg[q][d][d] = (p == q ? -2.0 : 0.0) / g.dx[d] / g.dx[d];
+            }
+        contract2(dim, kind, c, gu, s);
+        if (bnd) {
+            double u[3] = {0, 0, 0}, nrm[3], f[3];
+            u[p] = 1.0;
+            int face = classify(g, i, j, k, nrm);
+            bc_row(dim, bctype + 3 * face, u, gu, s, nrm, f);
+            dg[p] = f[p];
+        } else {
+            double f[3];
+            contract3(dim, kind, c, gg, f);
+            double add[3] = {0, 0, 0};
+            for (int d = 0; d < dim; ++d) {
+                int qp[3] = {i, j, k}, qm[3] = {i, j, k};
+                qp[d] += 1; qm[d] -= 1;
+                double cp[21], cm[21], dc[21], sd[3][3];
+                load_c(g, nc, C, qp[0], qp[1], qp[2], cp);
+                load_c(g, nc, C, qm[0], qm[1], qm[2], cm);
+                for (int q = 0; q < nc; ++q) dc[q] = ((cp[q] - cm[q]) * 0.5) / g.dx[d];
+                contract2(dim, kind, dc, gu, sd);
+                for (int r = 0; r < dim; ++r) add[r] += sd[r][d];
+            }
+            dg[p] = 0.0 + (f[p] + add[p]);
+        }
+    }
+}
+
+void fapply_all(const G& g, int kind, const double* C, const int* bctype, int uniform, const double* u, double* f) {
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < g.nn[2]; ++k)
+        for (int j = 0; j < g.nn[1]; ++j)
+            for (int i = 0; i < g.nn[0]; ++i) {
+                double r[3];
+                fapply_node(g, kind, C, bctype, uniform, u, i, j, k, r);
+                for (int p = 0; p < g.dim; ++p) f[p * g.nodes + g.id(i, j, k)] = r[p];
+            }
+}
+
+void diag_all(const G& g, int kind, const double* C, const int* bctype, double* dg) {
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < g.nn[2]; ++k)
+        for (int j = 0; j < g.nn[1]; ++j)
+            for (int i = 0; i < g.nn[0]; ++i) {
+                double r[3];
+                diag_node(g, kind, C, bctype, i, j, k, r);
+                for (int p = 0; p < g.dim; ++p) dg[p * g.nodes + g.id(i, j, k)] = r[p];
+            }
+}
+
+// Nodal full weighting onto the coarse grid gc from the fine grid (2x finer)
+// (Operator.cpp:486-543; the same stencil coarsens the stiffness, Elastic.cpp:759-803).
+// In 2D the reference's k index is always on its (degenerate) z boundary, so
+// the corner / X-edge / Y-edge / Z-face branches are the ones reached.
+void restrict_all(const G& gc, int ncomp, const double* fine, double* crse) {
+    or_grid fgd{gc.dim, {2 * gc.n[0], 2 * gc.n[1], gc.dim == 3 ? 2 * gc.n[2] : 0}, {gc.dx[0] / 2, gc.dx[1] / 2, gc.dx[2] / 2}};
+    G gf(&fgd);
+    const bool d3 = gc.dim == 3;
+#pragma omp parallel for schedule(static)
+    for (int K = 0; K < gc.nn[2]; ++K)
+        for (int J = 0; J < gc.nn[1]; ++J)
+            for (int I = 0; I < gc.nn[0]; ++I) {
+                const int i = 2 * I, j = 2 * J, k = 2 * K;
+                const bool bI = (I == 0 || I == gc.n[0]), bJ = (J == 0 || J == gc.n[1]);
+                const bool bK = d3 ? (K == 0 || K == gc.n[2]) : true;
+                for (int n = 0; n < ncomp; ++n) {
+                    const double* fd = fine + (long)n * gf.nodes;
+                    auto F = [&](int a, int b, int c) { return fd[gf.id(a, b, c)]; };
+                    double v;
+                    if (bI && bJ && bK) v = F(i, j, k);
+                    else if (bJ && bK) v = 0.25 * F(i - 1, j, k) + 0.5 * F(i, j, k) + 0.25 * F(i + 1, j, k);
+                    else if (bK && bI) v = 0.25 * F(i, j - 1, k) + 0.5 * F(i, j, k) + 0.25 * F(i, j + 1, k);
+                    else if (bI && bJ) v = 0.25 * F(i, j, k - 1) + 0.5 * F(i, j, k) + 0.25 * F(i, j, k + 1);
+                    else if (bI)
+                        v = (+F(i, j - 1, k - 1) + 2.0 * F(i, j, k - 1) + F(i, j + 1, k - 1) + 2.0 * F(i, j - 1, k) +
+                             4.0 * F(i, j, k) + 2.0 * F(i, j + 1, k) + F(i, j - 1, k + 1) + 2.0 * F(i, j, k + 1) +
+                             F(i, j + 1, k + 1)) / 16.0;
+                    else if (bJ)
+                        v = (+F(i - 1, j, k - 1) + 2.0 * F(i - 1, j, k) + F(i - 1, j, k + 1) + 2.0 * F(i, j, k - 1) +
+                             4.0 * F(i, j, k) + 2.0 * F(i, j, k + 1) + F(i + 1, j, k - 1) + 2.0 * F(i + 1, j, k) +
+                             F(i + 1, j, k + 1)) / 16.0;
+                    else if (bK)
+                        v = (+F(i - 1, j - 1, k) + 2.0 * F(i, j - 1, k) + F(i + 1, j - 1, k) + 2.0 * F(i - 1, j, k) +
+                             4.0 * F(i, j, k) + 2.0 * F(i + 1, j, k) + F(i - 1, j + 1, k) + 2.0 * F(i, j + 1, k) +
+                             F(i + 1, j + 1, k)) / 16.0;
+                    else {
+                        double corners = 0, edges = 0, faces = 0;
+                        for (int a = -1; a <= 1; a += 2)
+                            for (int b = -1; b <= 1; b += 2)
+                                for (int c = -1; c <= 1; c += 2) corners += F(i + a, j + b, k + c);
+                        for (int a = -1; a <= 1; a += 2)
+                            for (int b = -1; b <= 1; b += 2) {
+                                edges += F(i, j + a, k + b);
+                                edges += F(i + a, j, k + b);
+                                edges += F(i + a, j + b, k);
+                            }
+                        for (int a = -1; a <= 1; a += 2) faces += F(i + a, j, k) + F(i, j + a, k) + F(i, j, k + a);
+                        v = corners / 64.0 + edges / 32.0 + faces / 16.0 + F(i, j, k) / 8.0;
+                    }
+                    crse[(long)n * gc.nodes + gc.id(I, J, K)] = v;
+                }
+            }
+}
+
+// Trilinear nodal prolongation added into the fine field (Operator.cpp:589-618).
+void interp_add_all(const G& gf, int ncomp, const double* crse, double* fine) {
+    or_grid cgd{gf.dim, {gf.n[0] / 2, gf.n[1] / 2, gf.dim == 3 ? gf.n[2] / 2 : 0}, {2 * gf.dx[0], 2 * gf.dx[1], 2 * gf.dx[2]}};
+    G gc(&cgd);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < gf.nn[2]; ++k)
+        for (int j = 0; j < gf.nn[1]; ++j)
+            for (int i = 0; i < gf.nn[0]; ++i) {
+                const int I = i / 2, J = j / 2, K = k / 2;
+                const bool ei = i % 2 == 0, ej = j % 2 == 0, ek = k % 2 == 0;
+                for (int n = 0; n < ncomp; ++n) {
+                    const double* cd = crse + (long)n * gc.nodes;
+                    auto c = [&](int a, int b, int e) { return cd[gc.id(a, b, e)]; };
+                    double v;
+                    if (ei && ej && ek) v = c(I, J, K);
+                    else if (ej && ek) v = 0.5 * (c(I, J, K) + c(I + 1, J, K));
+                    else if (ek && ei) v = 0.5 * (c(I, J, K) + c(I, J + 1, K));
+                    else if (ei && ej) v = 0.5 * (c(I, J, K) + c(I, J, K + 1));
+                    else if (ei) v = 0.25 * (c(I, J, K) + c(I, J + 1, K) + c(I, J, K + 1) + c(I, J + 1, K + 1));
+                    else if (ej) v = 0.25 * (c(I, J, K) + c(I, J, K + 1) + c(I + 1, J, K) + c(I + 1, J, K + 1));
+                    else if (ek) v = 0.25 * (c(I, J, K) + c(I + 1, J, K) + c(I, J + 1, K) + c(I + 1, J + 1, K));
+                    else
+                        v = 0.125 * (c(I, J, K) + c(I + 1, J, K) + c(I, J + 1, K) + c(I, J, K + 1) + c(I, J + 1, K + 1) +
+                                     c(I + 1, J, K + 1) + c(I + 1, J + 1, K) + c(I + 1, J + 1, K + 1));
+                    fine[(long)n * gf.nodes + gf.id(i, j, k)] += v;
+                }
+            }
+}
+
+// Fsmooth (Operator.cpp:90-162): weighted Jacobi, omega = 2/3, two sweeps.
+void fsmooth(const G& g, int kind, const double* C, const int* bctype, const double* dg, double* x, const double* b,
+             std::vector<double>& Ax) {
+    const double omega = 2. / 3.;
+    const long N = (long)g.dim * g.nodes;
+    Ax.resize(N);
+    for (int ctr = 0; ctr < 2; ++ctr) {
+        fapply_all(g, kind, C, bctype, 0, x, Ax.data());
+#pragma omp parallel for schedule(static)
+        for (long q = 0; q < N; ++q) {
+            double Dx = x[q] * dg[q];
+            double Rx = Ax[q] - Dx;
+            x[q] = (1. - omega) * x[q] + omega * (b[q] - Rx) / dg[q];
+        }
+    }
+}
+
+double norm_inf(const double* v, long n) {
+    double m = 0.0;
+#pragma omp parallel for reduction(max : m)
+    for (long q = 0; q < n; ++q) m = std::max(m, std::fabs(v[q]));
+    return m;
+}
+
+// ---------------------------------------------------------------- multigrid
+struct Level {
+    or_grid og;
+    G g;
+    std::vector<double> C, diag, cor, res, rescor, hold, tmp;
+    explicit Level(const or_grid& o) : og(o), g(&og) {}
+};
+
+struct Dense {
+    int n = 0;
+    std::vector<double> lu;
+    std::vector<int> piv;
+    void factor(std::vector<double> a, int N) {
+        n = N;
+        piv.resize(N);
+        for (int c = 0; c < N; ++c) {
+            int p = c;
+            for (int r = c + 1; r < N; ++r)
+                if (std::fabs(a[(long)r * N + c]) > std::fabs(a[(long)p * N + c])) p = r;
+            piv[c] = p;
+            if (p != c)
+                for (int q = 0; q < N; ++q) std::swap(a[(long)c * N + q], a[(long)p * N + q]);
+            double d = a[(long)c * N + c];
+            for (int r = c + 1; r < N; ++r) {
+                double m = a[(long)r * N + c] / d;
+                a[(long)r * N + c] = m;
+                if (m != 0.0)
+                    for (int q = c + 1; q < N; ++q) a[(long)r * N + q] -= m * a[(long)c * N + q];
+            }
+        }
+        lu.swap(a);
+    }
+    void solve(double* x) const {
+        for (int c = 0; c < n; ++c) std::swap(x[c], x[piv[c]]);
+        for (int r = 0; r < n; ++r)
+            for (int q = 0; q < r; ++q) x[r] -= lu[(long)r * n + q] * x[q];
+        for (int r = n - 1; r >= 0; --r) {
+            for (int q = r + 1; q < n; ++q) x[r] -= lu[(long)r * n + q] * x[q];
+            x[r] /= lu[(long)r * n + r];
+        }
+    }
+};
+
+struct MG {
+    int kind;
+    const int* bctype;
+    std::vector<Level> lev;
+    Dense bottom;
+    int bottom_kind;  // 0 dense, 1 bicgstab
+    int nu1 = 2, nu2 = 2;
+    std::vector<double> Ax;
+
+    void A(int l, const double* x, double* y) {
+        fapply_all(lev[l].g, kind, lev[l].C.data(), bctype, 0, x, y);
+    }
+    long N(int l) const { return (long)lev[l].g.dim * lev[l].g.nodes; }
+
+    void bottom_solve(int l, double* x, const double* b) {
+        const long n = N(l);
+        if (bottom_kind == 0) {
+            std::copy(b, b + n, x);
+            bottom.solve(x);
+            return;
+        }
+        // BiCGStab from x = 0 to |r|inf <= 1e-4 |b|inf, at most 200 iterations.
+        std::vector<double> r(b, b + n), rh(b, b + n), p(n, 0.0), v(n, 0.0), s(n), t(n);
+        std::fill(x, x + n, 0.0);
+        double rho_1 = 0, alpha = 0, omega = 0;
+        const double rnorm0 = norm_inf(r.data(), n);
+        if (rnorm0 == 0.0) return;
+        for (int it = 1; it <= 200; ++it) {
+            double rho = 0;
+            for (long q = 0; q < n; ++q) rho += rh[q] * r[q];
+            if (rho == 0.0) break;
+            if (it == 1) p = r;
+            else {
+                double beta = (rho / rho_1) * (alpha / omega);
+                for (long q = 0; q < n; ++q) p[q] = r[q] + beta * (p[q] - omega * v[q]);
+            }
+            A(l, p.data(), v.data());
+            double rhv = 0;
+            for (long q = 0; q < n; ++q) rhv += rh[q] * v[q];
+            if (rhv == 0.0) break;
+            alpha = rho / rhv;
+            for (long q = 0; q < n; ++q) { x[q] += alpha * p[q]; s[q] = r[q] - alpha * v[q]; }
+            if (norm_inf(s.data(), n) <= 1e-4 * rnorm0) break;
+            A(l, s.data(), t.data());
+            double tt = 0, ts = 0;
+            for (long q = 0; q < n; ++q) { tt += t[q] * t[q]; ts += t[q] * s[q]; }
+            if (tt == 0.0) break;
+            omega = ts / tt;
+            for (long q = 0; q < n; ++q) { x[q] += omega * s[q]; r[q] = s[q] - omega * t[q]; }
+            if (norm_inf(r.data(), n) <= 1e-4 * rnorm0) break;
+            if (omega == 0.0) break;
+            rho_1 = rho;
+        }
+    }
+
+    // rescor = res - A cor
+    void res_of_cor(int l) {
+        Level& L = lev[l];
+        A(l, L.cor.data(), L.rescor.data());
+        const long n = N(l);
+        for (long q = 0; q < n; ++q) L.rescor[q] = L.res[q] - L.rescor[q];
+    }
+
+    void vcycle(int top) {
+        const int bot = (int)lev.size() - 1;
+        for (int l = top; l < bot; ++l) {
+            Level& L = lev[l];
+            std::fill(L.cor.begin(), L.cor.end(), 0.0);
+            for (int s = 0; s < nu1; ++s) fsmooth(L.g, kind, L.C.data(), bctype, L.diag.data(), L.cor.data(), L.res.data(), Ax);
+            res_of_cor(l);
+            restrict_all(lev[l + 1].g, L.g.dim, L.rescor.data(), lev[l + 1].res.data());
+        }
+        bottom_solve(bot, lev[bot].cor.data(), lev[bot].res.data());
+        for (int l = bot - 1; l >= top; --l) {
+            Level& L = lev[l];
+            interp_add_all(L.g, L.g.dim, lev[l + 1].cor.data(), L.cor.data());
+            for (int s = 0; s < nu2; ++s) fsmooth(L.g, kind, L.C.data(), bctype, L.diag.data(), L.cor.data(), L.res.data(), Ax);
+        }
+    }
+
+    void fcycle() {
+        const int bot = (int)lev.size() - 1;
+        for (int l = 1; l <= bot; ++l) restrict_all(lev[l].g, lev[l].g.dim, lev[l - 1].res.data(), lev[l].res.data());
+        bottom_solve(bot, lev[bot].cor.data(), lev[bot].res.data());
+        for (int l = bot - 1; l >= 0; --l) {
+            Level& L = lev[l];
+            std::fill(L.cor.begin(), L.cor.end(), 0.0);
+            interp_add_all(L.g, L.g.dim, lev[l + 1].cor.data(), L.cor.data());
+            res_of_cor(l);
+            L.res = L.rescor;
+            L.hold = L.cor;
+            vcycle(l);
+            for (long q = 0; q < N(l); ++q) L.cor[q] += L.hold[q];
+        }
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int or_ncomp(int dim, int kind) { return ncomp_of(dim, kind); }
+int or_mm_slot(int dim, int i, int j, int k, int l) { return mm_slot(dim, i, j, k, l); }
+
+int or_classify(const or_grid* og, int i, int j, int k, double* nrm) {
+    G g(og);
+    return classify(g, i, j, k, nrm);
+}
+
+void or_fapply(const or_grid* og, int kind, const double* C, const int* bctype, int uniform, const double* u, double* f) {
+    G g(og);
+    fapply_all(g, kind, C, bctype, uniform, u, f);
+}
+
+void or_diagonal(const or_grid* og, int kind, const double* C, const int* bctype, double* dg) {
+    G g(og);
+    diag_all(g, kind, C, bctype, dg);
+}
+
+void or_restrict(const or_grid* ogc, int ncomp, const double* fine, double* crse) {
+    G gc(ogc);
+    restrict_all(gc, ncomp, fine, crse);
+}
+
+void or_interp_add(const or_grid* ogf, int ncomp, const double* crse, double* fine) {
+    G gf(ogf);
+    interp_add_all(gf, ncomp, crse, fine);
+}
+
+void or_fsmooth(const or_grid* og, int kind, const double* C, const int* bctype, const double* dg, double* x, const double* b) {
+    G g(og);
+    std::vector<double> Ax;
+    fsmooth(g, kind, C, bctype, dg, x, b, Ax);
+}
+
+// BC::Operator::Elastic::Init (BC/Operator/Elastic.H:106-181): write prescribed
+// values into the rhs on every boundary node; homogeneous zeroes displacement rows.
+void or_bc_init(const or_grid* og, const int* bctype, const double* bcval, int homogeneous, double* rhs) {
+    G g(og);
+    for (int k = 0; k < g.nn[2]; ++k)
+        for (int j = 0; j < g.nn[1]; ++j)
+            for (int i = 0; i < g.nn[0]; ++i) {
+                double nrm[3];
+                int face = classify(g, i, j, k, nrm);
+                if (face < 0) continue;
+                for (int d = 0; d < g.dim; ++d) {
+                    double v = (homogeneous && bctype[3 * face + d] == BC_DISP) ? 0.0 : bcval[3 * face + d];
+                    rhs[d * g.nodes + g.id(i, j, k)] = v;
+                }
+            }
+}
+
+// Linear::Cubic::Define(C11,C12,C44,R) (Cubic.H:30-59): the unrotated tensor
+// Ctmp (C11 on iiii, C44 where i==k && j==l, C12 where i==j && k==l), rotated
+// as R_pi R_qj R_sk R_tl Ctmp_ijkl and written slot by slot in p,q,s,t loop
+// order, so the last (p,q,s,t) of each symmetry class sets the stored value.
+void or_cubic_define_R(int dim, double C11, double C12, double C44, const double* R9, double* out) {
+    double Ct[3][3][3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k)
+                for (int l = 0; l < 3; ++l) {
+                    if (i == j && j == k && k == l) Ct[i][j][k][l] = C11;
+                    else if (i == k && j == l) Ct[i][j][k][l] = C44;
+                    else if (i == j && k == l) Ct[i][j][k][l] = C12;
+                    else Ct[i][j][k][l] = 0.0;
+                }
+    auto R = [&](int a, int b) { return R9[3 * a + b]; };
+    const int nc = ncomp_of(dim, K_MM);
+    for (int q = 0; q < nc; ++q) out[q] = 0.0;
+    for (int p = 0; p < dim; ++p)
+        for (int q = 0; q < dim; ++q)
+            for (int s = 0; s < dim; ++s)
+                for (int t = 0; t < dim; ++t) {
+                    double acc = 0.0;
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j)
+                            for (int k = 0; k < 3; ++k)
+                                for (int l = 0; l < 3; ++l) acc += R(p, i) * R(s, k) * Ct[i][j][k][l] * R(q, j) * R(t, l);
+                    out[mm_slot(dim, p, q, s, t)] = acc;
+                }
+}
+
+// Rotation from the reference's X-Z-X angle triple (Cubic.H:24-27):
+// R = AngleAxis(phi2, X) * AngleAxis(Phi, Z) * AngleAxis(phi1, X).
+void or_cubic_rotation(double phi1, double Phi, double phi2, double* R9) {
+    auto rx = [](double a, double* m) {
+        double c = std::cos(a), s = std::sin(a);
+        double t[9] = {1, 0, 0, 0, c, -s, 0, s, c};
+        std::memcpy(m, t, sizeof t);
+    };
+    auto rz = [](double a, double* m) {
+        double c = std::cos(a), s = std::sin(a);
+        double t[9] = {c, -s, 0, s, c, 0, 0, 0, 1};
+        std::memcpy(m, t, sizeof t);
+    };
+    auto mul = [](const double* a, const double* b, double* o) {
+        double t[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double s = 0;
+                for (int k = 0; k < 3; ++k) s += a[3 * i + k] * b[3 * k + j];
+                t[3 * i + j] = s;
+            }
+        std::memcpy(o, t, sizeof t);
+    };
+    double A[9], B[9], Cm[9], T[9];
+    rx(phi2, A); rz(Phi, B); rx(phi1, Cm);
+    mul(A, B, T);
+    mul(T, Cm, R9);
+}
+
+// Assemble the operator of a small grid column by column (A e_q) into a dense
+// row-major matrix.  Used to pin Fapply/Diagonal against brute force.
+void or_assemble(const or_grid* og, int kind, const double* C, const int* bctype, double* Adense) {
+    G g(og);
+    const long n = (long)g.dim * g.nodes;
+    std::vector<double> e(n, 0.0), col(n);
+    for (long q = 0; q < n; ++q) {
+        e[q] = 1.0;
+        fapply_all(g, kind, C, bctype, 0, e.data(), col.data());
+        e[q] = 0.0;
+        for (long r = 0; r < n; ++r) Adense[r * n + q] = col[r];
+    }
+}
+
+// Multigrid solve of A u = b on one level (restated MLMG, see header).
+// levels: number of MG levels to use (<= 0: coarsen while every n is even and >= 4).
+// stats[0] iterations, [1] |r0|inf, [2] final |r|inf, [3] number of MG levels.
+// hist (optional, cap entries) receives |r|inf after each iteration.
+int or_mg_solve(const or_grid* og, int kind, const double* C, const int* bctype, const double* b, double* u,
+                double tol_rel, double tol_abs, int max_iter, int max_fmg_iter, int bottom_kind, int levels,
+                double* stats, double* hist, int cap) {
+    MG mg;
+    mg.kind = kind;
+    mg.bctype = bctype;
+    mg.bottom_kind = bottom_kind;
+    const int dim = og->dim, nc = ncomp_of(dim, kind);
+    or_grid cur = *og;
+    for (;;) {
+        mg.lev.emplace_back(cur);
+        bool can = true;
+        for (int d = 0; d < dim; ++d) can = can && (cur.n[d] % 2 == 0) && (cur.n[d] >= 4);
+        if (levels > 0 && (int)mg.lev.size() >= levels) can = false;
+        if (!can) break;
+        for (int d = 0; d < dim; ++d) { cur.n[d] /= 2; cur.dx[d] *= 2; }
+    }
+    for (auto& L : mg.lev) {
+        L.g = G(&L.og);
+        const long n = (long)dim * L.g.nodes;
+        L.cor.assign(n, 0.0); L.res.assign(n, 0.0); L.rescor.assign(n, 0.0); L.hold.assign(n, 0.0);
+        L.diag.assign(n, 0.0);
+        L.C.assign((long)nc * L.g.nodes, 0.0);
+    }
+    std::copy(C, C + (long)nc * mg.lev[0].g.nodes, mg.lev[0].C.begin());
+    for (size_t l = 1; l < mg.lev.size(); ++l) restrict_all(mg.lev[l].g, nc, mg.lev[l - 1].C.data(), mg.lev[l].C.data());
+    for (auto& L : mg.lev) diag_all(L.g, kind, L.C.data(), bctype, L.diag.data());
+    const int bot = (int)mg.lev.size() - 1;
+    if (bottom_kind == 0) {
+        const long n = mg.N(bot);
+        std::vector<double> a((size_t)n * n);
+        or_assemble(&mg.lev[bot].og, kind, mg.lev[bot].C.data(), bctype, a.data());
+        mg.bottom.factor(a, (int)n);
+    }
+    Level& L0 = mg.lev[0];
+    const long n0 = mg.N(0);
+    auto solution_residual = [&]() {
+        mg.A(0, u, L0.res.data());
+        for (long q = 0; q < n0; ++q) L0.res[q] = b[q] - L0.res[q];
+        return norm_inf(L0.res.data(), n0);
+    };
+    const double r0 = solution_residual();
+    const double target = std::max(tol_abs, std::max(tol_rel, 1e-16) * r0);
+    double rn = r0;
+    int it = 0;
+    if (r0 > target) {
+        for (it = 0; it < max_iter;) {
+            if (it < max_fmg_iter) mg.fcycle();
+            else mg.vcycle(0);
+            for (long q = 0; q < n0; ++q) u[q] += L0.cor[q];
+            rn = solution_residual();
+            if (hist && it < cap) hist[it] = rn;
+            ++it;
+            if (rn <= target) break;
+        }
+    }
+    if (stats) {
+        stats[0] = it;
+        stats[1] = r0;
+        stats[2] = rn;
+        stats[3] = (double)mg.lev.size();
+    }
+    return rn <= target ? 0 : 1;
+}
+
+}  // extern "C"
