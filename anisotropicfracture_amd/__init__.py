@@ -1,0 +1,386 @@
+"""MI355X-native nodal elasticity solve (gfx950 HIP kernels behind a C ABI).
+
+This module binds ``libela.so`` (built in-tree by ``anisotropicfracture_amd.build``)
+with ctypes and mirrors the reference's operator interface for Python callers:
+
+* :class:`Elastic` -- ``Operator::Elastic<T>`` (src/Operator/Elastic.H:20-171)
+  with the node-operator base ``Operator<Grid::Node>`` (src/Operator/Operator.H:30-128):
+  ``SetModel``, ``SetBC``, ``SetUniform``, ``prepareForSolve``, ``Fapply``,
+  ``Diagonal``, ``Fsmooth``, ``restriction``, ``interpolation``, ``residual``.
+* :class:`Linear` -- ``Solver::Nonlocal::Linear`` (src/Solver/Nonlocal/Linear.H:19-143).
+
+Every computation runs on the GPU through the library; there is no CPU
+fallback.  If the library is missing, importing works but the first use
+raises ``RuntimeError``.
+
+Host arrays are numpy float64 of shape ``(ncomp, nz+1, ny+1, nx+1)`` in 3D or
+``(ncomp, ny+1, nx+1)`` in 2D (component-major, x fastest).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libela.so")
+
+ISO, MM, DIAG = 0, 1, 2
+DISP, TRAC, PERIODIC, NEUMANN = 0, 1, 2, 3
+F64, F32 = 0, 1
+SOLVE_FP64, SOLVE_MIXED, ALLOC_BOTH = 0, 1, 2
+CYCLE_F, CYCLE_V = 0, 1
+
+ERRORS = {1: "BADARG", 2: "OOM", 3: "HIP", 4: "NONCONVERGED", 5: "NAN", 6: "STATE"}
+
+
+class ElaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("ela error %s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
+        self.code = code
+
+
+class NonConverged(ElaError):
+    pass
+
+
+class _Domain(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("n", ctypes.c_int * 3), ("dx", ctypes.c_double * 3)]
+
+
+class _Layout(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("n", ctypes.c_int * 3), ("nodes", ctypes.c_int * 3),
+                ("pitch", ctypes.c_int64), ("plane", ctypes.c_int64), ("cstride", ctypes.c_int64),
+                ("dx", ctypes.c_double * 3)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("iters", ctypes.c_int), ("converged", ctypes.c_int), ("mg_levels", ctypes.c_int),
+                ("mode", ctypes.c_int), ("res0", ctypes.c_double), ("res", ctypes.c_double),
+                ("seconds", ctypes.c_double), ("setup_seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+# exported symbol -> (argtypes, restype)
+_P = ctypes.c_void_p
+_I, _D, _SZ = ctypes.c_int, ctypes.c_double, ctypes.c_size_t
+SIGNATURES = {
+    "ela_last_error": ([], ctypes.c_char_p),
+    "ela_ctx_create": ([_I, _SZ, ctypes.POINTER(_P)], _I),
+    "ela_ctx_destroy": ([_P], _I),
+    "ela_ctx_bytes_in_use": ([_P], _SZ),
+    "ela_hier_create": ([_P, ctypes.POINTER(_Domain), _I, _I, _I, ctypes.POINTER(_P)], _I),
+    "ela_hier_destroy": ([_P], _I),
+    "ela_hier_levels": ([_P], _I),
+    "ela_level_info": ([_P, _I, ctypes.POINTER(_Layout)], _I),
+    "ela_hier_bytes": ([_P], _SZ),
+    "ela_set_model_field": ([_P, _P], _I),
+    "ela_set_model_voronoi_cubic": ([_P, _I, _P, _I, _P, _D, _D, _D], _I),
+    "ela_set_model_degraded_iso": ([_P, _D, _D, _P, _D, _D, _D], _I),
+    "ela_set_bc": ([_P, _P, _P], _I),
+    "ela_set_uniform": ([_P, _I], _I),
+    "ela_prepare": ([_P], _I),
+    "ela_field_alloc": ([_P, _I, _I, _I, ctypes.POINTER(_P)], _I),
+    "ela_field_free": ([_P, _P], _I),
+    "ela_upload": ([_P, _I, _I, _I, _P, _P], _I),
+    "ela_download": ([_P, _I, _I, _I, _P, _P], _I),
+    "ela_field_zero": ([_P, _I, _I, _I, _P], _I),
+    "ela_fapply": ([_P, _I, _I, _P, _P, _P], _I),
+    "ela_diagonal": ([_P, _I, _I, _P, _P], _I),
+    "ela_fsmooth": ([_P, _I, _I, _P, _P, _I, _D, _P], _I),
+    "ela_restriction": ([_P, _I, _I, _P, _P, _P], _I),
+    "ela_interpolation": ([_P, _I, _I, _P, _P, _P], _I),
+    "ela_residual": ([_P, _I, _I, _P, _P, _P, ctypes.POINTER(_D), _P], _I),
+    "ela_bc_init": ([_P, _P, _I], _I),
+    "ela_solve": ([_P, _P, _P, _D, _D, _I, _I, ctypes.POINTER(Stats)], _I),
+    "ela_solve_device": ([_P, _P, _P, _D, _D, _I, _I, ctypes.POINTER(Stats)], _I),
+    "ela_set_cycle": ([_P, _I, _I, _I], _I),
+    "ela_set_timing": ([_P, _I], _I),
+    "ela_kernel_times": ([_P, ctypes.POINTER(_D), ctypes.POINTER(_I), ctypes.POINTER(_D), ctypes.POINTER(_I)], _I),
+    "ela_time_fapply": ([_P, _I, _I, ctypes.POINTER(_D)], _I),
+}
+
+
+def lib():
+    """Load libela.so (raises RuntimeError if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libela.so is missing: build it with `python -m anisotropicfracture_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().ela_last_error().decode(errors="replace")
+        if rc == 4:
+            raise NonConverged(rc, msg)
+        raise ElaError(rc, msg)
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def ncomp(dim, kind):
+    return 2 if kind == ISO else ((21 if dim == 3 else 6) if kind == MM else dim * dim)
+
+
+class Context:
+    """One GPU and an HBM budget (0 = unlimited) that hierarchies are admitted against."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0):
+        p = ctypes.c_void_p()
+        _check(lib().ela_ctx_create(device, hbm_budget_bytes, ctypes.byref(p)))
+        self.ptr = p
+
+    def bytes_in_use(self):
+        return lib().ela_ctx_bytes_in_use(self.ptr)
+
+    def close(self):
+        if self.ptr:
+            lib().ela_ctx_destroy(self.ptr)
+            self.ptr = None
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class DeviceField:
+    """A device field on one MG level (owned by the hierarchy)."""
+
+    def __init__(self, op, mglev, prec, nc):
+        self.op, self.mglev, self.prec, self.nc = op, mglev, prec, nc
+        p = ctypes.c_void_p()
+        _check(lib().ela_field_alloc(op.ptr, mglev, prec, nc, ctypes.byref(p)))
+        self.ptr = p
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, dtype=np.float64)
+        _check(lib().ela_upload(self.op.ptr, self.mglev, self.prec, self.nc, _ptr(host), self.ptr))
+        return self
+
+    def download(self):
+        out = np.zeros(self.op.shape(self.mglev, self.nc))
+        _check(lib().ela_download(self.op.ptr, self.mglev, self.prec, self.nc, self.ptr, _ptr(out)))
+        return out
+
+    def zero(self):
+        _check(lib().ela_field_zero(self.op.ptr, self.mglev, self.prec, self.nc, self.ptr))
+        return self
+
+    def free(self):
+        if self.ptr:
+            lib().ela_field_free(self.op.ptr, self.ptr)
+            self.ptr = None
+
+
+class Elastic:
+    """Operator::Elastic<T> on one box: ``define`` happens in the constructor.
+
+    n: cells per direction (2 or 3 entries); L: domain lengths (default 1).
+    kind: ISO, MM or DIAG (the stiffness storage of the model T).
+    mode: SOLVE_FP64, SOLVE_MIXED or ALLOC_BOTH (which MG precisions to allocate).
+    """
+
+    def __init__(self, n, kind, L=None, mode=ALLOC_BOTH, max_mg_levels=0, ctx=None):
+        self.ctx = ctx or default_context()
+        n = tuple(int(v) for v in n)
+        self.dim = len(n)
+        L = L if L is not None else (1.0,) * self.dim
+        d = _Domain()
+        d.dim = self.dim
+        for q in range(3):
+            d.n[q] = n[q] if q < self.dim else 0
+            d.dx[q] = float(L[q]) / n[q] if q < self.dim else 1.0
+        p = ctypes.c_void_p()
+        _check(lib().ela_hier_create(self.ctx.ptr, ctypes.byref(d), kind, max_mg_levels, mode, ctypes.byref(p)))
+        self.ptr = p
+        self.kind, self.n, self.mode = kind, n, mode
+        self.nc = ncomp(self.dim, kind)
+
+    # ------------------------------------------------------------- info
+    def levels(self):
+        return lib().ela_hier_levels(self.ptr)
+
+    def layout(self, mglev=0):
+        lay = _Layout()
+        _check(lib().ela_level_info(self.ptr, mglev, ctypes.byref(lay)))
+        return lay
+
+    def nodes(self, mglev=0):
+        lay = self.layout(mglev)
+        return tuple(lay.nodes[q] for q in range(self.dim))
+
+    def shape(self, mglev=0, ncomp=None):
+        return (self.dim if ncomp is None else ncomp,) + tuple(reversed(self.nodes(mglev)))
+
+    def bytes(self):
+        return lib().ela_hier_bytes(self.ptr)
+
+    def field(self, mglev=0, prec=F64, nc=None):
+        return DeviceField(self, mglev, prec, self.dim if nc is None else nc)
+
+    # ------------------------------------------------------------- setup
+    def SetModel(self, C):
+        """SetModel(FabArray<T>) (Elastic.cpp:84-118): per-node stiffness planes, shape (nc, *nodes)."""
+        C = np.ascontiguousarray(C, dtype=np.float64)
+        if C.shape != self.shape(0, self.nc):
+            raise ValueError("stiffness field has shape %s, expected %s" % (C.shape, self.shape(0, self.nc)))
+        _check(lib().ela_set_model_field(self.ptr, _ptr(C)))
+
+    def SetModelVoronoiCubic(self, seeds_q, q, R, C11, C12, C44):
+        seeds_q = np.ascontiguousarray(seeds_q, dtype=np.int64)
+        R = np.ascontiguousarray(R, dtype=np.float64)
+        _check(lib().ela_set_model_voronoi_cubic(self.ptr, seeds_q.shape[0], _ptr(seeds_q), q, _ptr(R), C11, C12, C44))
+
+    def SetModelDegradedIsotropic(self, lambda0, mu0, eta, a, b, c):
+        eta = np.ascontiguousarray(eta, dtype=np.float64)
+        _check(lib().ela_set_model_degraded_iso(self.ptr, lambda0, mu0, _ptr(eta), a, b, c))
+
+    def SetBC(self, types, values=None):
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        v = np.zeros(t.shape) if values is None else np.ascontiguousarray(values, dtype=np.float64)
+        self._bc = (t, v)
+        _check(lib().ela_set_bc(self.ptr, _ptr(t), _ptr(v)))
+
+    def SetUniform(self, uniform):
+        _check(lib().ela_set_uniform(self.ptr, int(bool(uniform))))
+
+    def SetCycle(self, cycle=CYCLE_F, nu1=2, nu2=2):
+        _check(lib().ela_set_cycle(self.ptr, cycle, nu1, nu2))
+
+    def prepareForSolve(self):
+        _check(lib().ela_prepare(self.ptr))
+
+    def bc_init(self, rhs=None, homogeneous=False):
+        rhs = np.zeros(self.shape(0)) if rhs is None else np.ascontiguousarray(rhs, dtype=np.float64).copy()
+        _check(lib().ela_bc_init(self.ptr, _ptr(rhs), int(homogeneous)))
+        return rhs
+
+    # ------------------------------------------------------------- operator (host arrays in, host arrays out)
+    def Fapply(self, u, mglev=0, prec=F64):
+        fu = self.field(mglev, prec).upload(u)
+        ff = self.field(mglev, prec)
+        _check(lib().ela_fapply(self.ptr, mglev, prec, fu.ptr, ff.ptr, None))
+        out = ff.download()
+        fu.free()
+        ff.free()
+        return out
+
+    def Diagonal(self, mglev=0, prec=F64):
+        fd = self.field(mglev, prec)
+        _check(lib().ela_diagonal(self.ptr, mglev, prec, fd.ptr, None))
+        out = fd.download()
+        fd.free()
+        return out
+
+    def Fsmooth(self, x, b, mglev=0, prec=F64, nsweeps=2, omega=2.0 / 3.0):
+        fx = self.field(mglev, prec).upload(x)
+        fb = self.field(mglev, prec).upload(b)
+        _check(lib().ela_fsmooth(self.ptr, mglev, prec, fx.ptr, fb.ptr, nsweeps, omega, None))
+        out = fx.download()
+        fx.free()
+        fb.free()
+        return out
+
+    def restriction(self, fine, cmglev, prec=F64):
+        ff = self.field(cmglev - 1, prec).upload(fine)
+        fc = self.field(cmglev, prec)
+        _check(lib().ela_restriction(self.ptr, cmglev, prec, ff.ptr, fc.ptr, None))
+        out = fc.download()
+        ff.free()
+        fc.free()
+        return out
+
+    def interpolation(self, fine, crse, fmglev, prec=F64):
+        ff = self.field(fmglev, prec).upload(fine)
+        fc = self.field(fmglev + 1, prec).upload(crse)
+        _check(lib().ela_interpolation(self.ptr, fmglev, prec, ff.ptr, fc.ptr, None))
+        out = ff.download()
+        ff.free()
+        fc.free()
+        return out
+
+    def residual(self, x, b, mglev=0, prec=F64):
+        fx = self.field(mglev, prec).upload(x)
+        fb = self.field(mglev, prec).upload(b)
+        fr = self.field(mglev, prec)
+        nrm = ctypes.c_double()
+        _check(lib().ela_residual(self.ptr, mglev, prec, fx.ptr, fb.ptr, fr.ptr, ctypes.byref(nrm), None))
+        out = fr.download()
+        for f in (fx, fb, fr):
+            f.free()
+        return out, nrm.value
+
+    def solve_device(self, b_field, u_field, tol_rel, tol_abs=0.0, mode=SOLVE_FP64, max_iter=200):
+        """Linear::solve on device-resident level-0 FP64 fields (u holds the initial guess)."""
+        st = Stats()
+        rc = lib().ela_solve_device(self.ptr, b_field.ptr, u_field.ptr, tol_rel, tol_abs, mode, max_iter,
+                                    ctypes.byref(st))
+        return rc, st.as_dict()
+
+    def time_fapply(self, prec=F64, nrep=20):
+        ms = ctypes.c_double()
+        _check(lib().ela_time_fapply(self.ptr, prec, nrep, ctypes.byref(ms)))
+        return ms.value
+
+    def set_timing(self, on=True):
+        _check(lib().ela_set_timing(self.ptr, int(bool(on))))
+
+    def kernel_times(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        na, nb = ctypes.c_int(), ctypes.c_int()
+        _check(lib().ela_kernel_times(self.ptr, ctypes.byref(a), ctypes.byref(na), ctypes.byref(b), ctypes.byref(nb)))
+        return {"resid64_ms": a.value, "resid64_n": na.value, "smooth0_ms": b.value, "smooth0_n": nb.value}
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().ela_hier_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Linear:
+    """Solver::Nonlocal::Linear (Linear.H:19-143): holds a reference to the operator."""
+
+    def __init__(self, op):
+        self.op = op
+        self.tol_rel, self.tol_abs = 1e-8, 1e-8  # Linear.H:118
+        self.max_iter = 200
+        self.last = None
+
+    def solve(self, sol, rhs, tol_rel=None, tol_abs=None, mode=SOLVE_FP64, max_iter=None, raise_on_fail=True):
+        """Solve A sol = rhs from the initial guess in sol; returns (sol, stats dict)."""
+        tol_rel = self.tol_rel if tol_rel is None else tol_rel
+        tol_abs = self.tol_abs if tol_abs is None else tol_abs
+        max_iter = self.max_iter if max_iter is None else max_iter
+        b = np.ascontiguousarray(rhs, dtype=np.float64)
+        u = np.ascontiguousarray(sol, dtype=np.float64).copy()
+        st = Stats()
+        rc = lib().ela_solve(self.op.ptr, _ptr(b), _ptr(u), tol_rel, tol_abs, mode, max_iter, ctypes.byref(st))
+        self.last = st.as_dict()
+        if rc != 0 and (raise_on_fail or rc not in (4,)):
+            _check(rc)
+        return u, self.last

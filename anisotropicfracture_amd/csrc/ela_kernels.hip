@@ -245,13 +245,34 @@ __global__ void k_scale_from_norm(const unsigned long long* norm, double* scale)
 // ---------------------------------------------------------------- dense coarsest level
 // Dense (row-major) operator of the coarsest level: A[r][q] = (A e_q)[r],
 // unknowns numbered comp * nodes + dense node index.  One thread per (node, column q).
+// Copy column q of the coarsest operator (F = A e_q in the level layout) into
+// the dense row-major matrix M (leading dimension ld), unknowns numbered
+// comp * nodes + dense node index.
 __global__ __launch_bounds__(256) void k_assemble_col(Lay L, int nodes, const double* __restrict__ F, int q,
-                                                      int nunk, double* __restrict__ A, int dim) {
+                                                      long long ld, double* __restrict__ M, int dim) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nodes) return;
     const int i = t % L.nx, j = (t / L.nx) % L.ny, k = t / (L.nx * L.ny);
     const long long o = (long long)i + L.px * j + L.pl * k;
-    for (int p = 0; p < dim; ++p) A[(long long)(p * nodes + t) * nunk + q] = F[p * L.cs + o];
+    for (int p = 0; p < dim; ++p) M[(long long)(p * nodes + t) * ld + q] = F[p * L.cs + o];
+}
+
+// Unit vector e_q in the level layout (E zeroed beforehand except the previous unit entry).
+__global__ void k_unit(Lay L, int nodes, double* __restrict__ E, int qprev, int q) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    auto at = [&](int qq) -> double& {
+        const int p = qq / nodes, t = qq % nodes;
+        const int i = t % L.nx, j = (t / L.nx) % L.ny, k = t / (L.nx * L.ny);
+        return E[p * L.cs + i + L.px * j + L.pl * k];
+    };
+    if (qprev >= 0) at(qprev) = 0.0;
+    at(q) = 1.0;
+}
+
+// Right half of [A | I]: M[r][n + r] = 1.
+__global__ void k_identity_right(double* __restrict__ M, int n) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) M[(long long)r * (2 * n) + n + r] = 1.0;
 }
 
 // Gauss-Jordan inversion with partial pivoting, one workgroup; M is n x 2n
@@ -507,9 +528,17 @@ void launch_scale_from_norm(const unsigned long long* norm, double* scale, hipSt
     hipLaunchKernelGGL(k_scale_from_norm, dim3(1), dim3(64), 0, s, norm, scale);
 }
 
-void launch_assemble_col(const Lay& L, int nodes, const double* F, int q, int nunk, double* A, int dim,
+void launch_assemble_col(const Lay& L, int nodes, const double* F, int q, long long ld, double* M, int dim,
                          hipStream_t s) {
-    hipLaunchKernelGGL(k_assemble_col, dim3((nodes + 255) / 256), dim3(256), 0, s, L, nodes, F, q, nunk, A, dim);
+    hipLaunchKernelGGL(k_assemble_col, dim3((nodes + 255) / 256), dim3(256), 0, s, L, nodes, F, q, ld, M, dim);
+}
+
+void launch_unit(const Lay& L, int nodes, double* E, int qprev, int q, hipStream_t s) {
+    hipLaunchKernelGGL(k_unit, dim3(1), dim3(64), 0, s, L, nodes, E, qprev, q);
+}
+
+void launch_identity_right(double* M, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_identity_right, dim3((n + 255) / 256), dim3(256), 0, s, M, n);
 }
 
 void launch_gauss_jordan(double* M, int n, int* flag, hipStream_t s) {

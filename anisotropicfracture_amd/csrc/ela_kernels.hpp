@@ -1,0 +1,67 @@
+// ela_kernels.hpp -- launchers of the gfx950 kernels (ela_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ela_device.hpp"
+
+namespace ela {
+
+template <typename T>
+Inv<T> make_inv(const double h[3]);
+
+template <typename T>
+void launch_fapply(const Lay& L, int kind, const T* C, const T* u, T* f, const BCTab& bc, const double h[3],
+                   int uniform, hipStream_t s);
+
+// r = b - A x; r (T), rs = r * (*scale) (TO), max|r| into *norm; any output may be null.
+template <typename T, typename TO>
+void launch_residual(const Lay& L, int kind, const T* C, const T* x, const T* b, T* r, TO* rs, const double* scale,
+                     unsigned long long* norm, const BCTab& bc, const double h[3], int uniform, hipStream_t s);
+
+template <typename T>
+void launch_jacobi(const Lay& L, int kind, const T* C, const T* dg, const T* xin, const T* b, T* xout,
+                   const BCTab& bc, const double h[3], int uniform, double omega, hipStream_t s);
+
+template <typename T>
+void launch_jacobi0(const Lay& L, const T* dg, const T* b, T* xout, double omega, hipStream_t s);
+
+void launch_diag(const Lay& L, int kind, const double* C, double* d64, float* d32, const BCTab& bc,
+                 const double h[3], unsigned long long* flag, hipStream_t s);
+
+template <typename T>
+void launch_restrict(const Lay& Lc, const Lay& Lf, int nc, const T* fine, T* crse, hipStream_t s);
+
+template <typename T>
+void launch_prolong(const Lay& Lf, const Lay& Lc, int nc, const T* crse, T* fine, int add, hipStream_t s);
+
+// y = a*x/(*ascale) (assign) or y += a*x/(*ascale); ascale may be null (then y (+)= a*x).
+template <typename TX, typename TY>
+void launch_axpy(const Lay& L, int nc, const TX* x, TY* y, const double* ascale, double a, int assign,
+                 hipStream_t s);
+
+template <typename TX, typename TY>
+void launch_convert(long long n, const TX* x, TY* y, hipStream_t s);
+
+void launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
+void launch_scale_from_norm(const unsigned long long* norm, double* scale, hipStream_t s);
+
+void launch_assemble_col(const Lay& L, int nodes, const double* F, int q, long long ld, double* M, int dim,
+                         hipStream_t s);
+void launch_unit(const Lay& L, int nodes, double* E, int qprev, int q, hipStream_t s);
+void launch_identity_right(double* M, int n, hipStream_t s);
+void launch_gauss_jordan(double* M, int n, int* flag, hipStream_t s);
+
+template <typename T>
+void launch_bottom(const Lay& L, int nodes, int dim, const double* M, const T* b, T* x, hipStream_t s);
+
+void launch_cubic_table(int ng, const double* R, double C11, double C12, double C44, double* table, hipStream_t s);
+void launch_voronoi_cubic(const Lay& L, int nseeds, const long long* seeds, int q, const double* table, double* C,
+                          int* gid, hipStream_t s);
+void launch_degraded_iso(const Lay& L, const double* eta, double lambda0, double mu0, double a, double b, double c,
+                         double* C, hipStream_t s);
+
+// Dense host-order array (nc planes of nx*ny*nz, already on the device) <-> level layout.
+template <typename T>
+void launch_pack(const Lay& L, int nc, const double* dense, T* dev, int to_dev, hipStream_t s);
+
+}  // namespace ela

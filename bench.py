@@ -1,0 +1,182 @@
+"""Benchmark of the nodal elasticity solve on one MI355X.
+
+Workload (default): 3D single level, 256^3 cells, cubic (C11, C12, C44) =
+(1.68, 1.21, 0.75) Voronoi polycrystal of 64 grains with Haar-random
+orientations (fixed seed), uniaxial strain 0.01 along z.  A step is one full
+solve from u = 0 to a relative residual of 1e-10: set-up of the operator
+(stiffness coarsening, diagonals, coarsest inverse) plus the multigrid
+iterations, with b and u resident in HBM.  The headline value is the
+mixed-precision (FP32 smoothing / FP64 refinement) solves/s; the plain FP64
+GPU solver is measured in the same run.
+
+Prints one JSON line.  Usage: python bench.py [--steps K] [--warmup W] [--n 256]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_mode(op, b, u, mode, steps, warmup, tol, timing=False):
+    import anisotropicfracture_amd as ela
+    stats = None
+    for _ in range(warmup):
+        u.zero()
+        op.prepareForSolve()
+        rc, stats = op.solve_device(b, u, tol, 0.0, mode, 400)
+    op.set_timing(timing)
+    t0 = time.perf_counter()
+    per = []
+    for _ in range(steps):
+        ts = time.perf_counter()
+        u.zero()
+        op.prepareForSolve()
+        rc, stats = op.solve_device(b, u, tol, 0.0, mode, 400)
+        per.append(time.perf_counter() - ts)
+        if rc not in (0, 4):
+            raise RuntimeError("solve failed: rc=%d %s" % (rc, ela.lib().ela_last_error()))
+    t = time.perf_counter() - t0
+    times = op.kernel_times() if timing else None
+    op.set_timing(False)
+    return t, per, stats, rc, times
+
+
+def cpu_baseline(n, cycles, fp64_iters):
+    """The oracle (FP64 CPU restatement of the reference algorithm) on a bounded sample:
+    `cycles` F-cycles of the same workload at n^3, scaled to one 256^3 solve."""
+    from anisotropicfracture_amd import configs as K
+    from oracle import oracle as O
+    O.build()
+    cfg = K.cfg2(n)
+    gid = K.voronoi_grain_ids(cfg["seeds"], cfg["n"])
+    tab = np.stack([O.cubic_define(cfg["C11"], cfg["C12"], cfg["C44"], R) for R in cfg["R"]])
+    C = np.moveaxis(tab[gid], -1, 0).copy()
+    g = O.Grid(cfg["n"])
+    b = O.bc_init(g, cfg["bc_type"], cfg["bc_value"], g.zeros())
+    O.mg_solve(g, O.MM, C, cfg["bc_type"], b, tol_rel=1e-10, max_iter=1)  # warm caches / OpenMP pool
+    t0 = time.perf_counter()
+    _, st, _ = O.mg_solve(g, O.MM, C, cfg["bc_type"], b, tol_rel=1e-30, max_iter=cycles)
+    t = time.perf_counter() - t0
+    per_cycle = t / max(st["iters"], 1)
+    return per_cycle, t, st["iters"]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--grains", type=int, default=64)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--cpu-n", type=int, default=64)
+    ap.add_argument("--cpu-cycles", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md on sharding)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import anisotropicfracture_amd as ela
+    from anisotropicfracture_amd import configs as K
+
+    n = args.n
+    log("bench: building config 256^3-class workload n=%d grains=%d" % (n, args.grains))
+    cfg = K.cfg2(n, grains=args.grains)
+    op = ela.Elastic(cfg["n"], ela.MM, mode=ela.ALLOC_BOTH)
+    op.SetModelVoronoiCubic(cfg["seeds"], K.VORONOI_Q, cfg["R"], cfg["C11"], cfg["C12"], cfg["C44"])
+    op.SetBC(cfg["bc_type"], cfg["bc_value"])
+    op.prepareForSolve()
+    b_host = op.bc_init()
+    b = op.field(0, ela.F64).upload(b_host)
+    u = op.field(0, ela.F64)
+    nodes = (n + 1) ** 3
+    log("bench: hierarchy %d MG levels, %.2f GB on device" % (op.levels(), op.bytes() / 1e9))
+
+    # plain FP64 GPU solver
+    log("bench: fp64 solves")
+    t64, per64, st64, rc64, _ = run_mode(op, b, u, ela.SOLVE_FP64, args.steps, args.warmup, args.tol)
+    u64 = u.download()
+    log("bench: fp64 %.3f s/solve iters=%d rel=%.2e rc=%d" % (t64 / args.steps, st64["iters"], st64["res"] / st64["res0"], rc64))
+    # mixed precision (headline)
+    log("bench: mixed solves")
+    tmx, permx, stmx, rcmx, kt = run_mode(op, b, u, ela.SOLVE_MIXED, args.steps, args.warmup, args.tol, timing=True)
+    umx = u.download()
+    log("bench: mixed %.3f s/solve iters=%d rel=%.2e rc=%d" % (tmx / args.steps, stmx["iters"], stmx["res"] / stmx["res0"], rcmx))
+    disp_diff = float(np.abs(umx - u64).max() / np.abs(u64).max())
+
+    # Fapply alone (FP64 and FP32), HIP events on the library stream
+    fa64 = op.time_fapply(ela.F64, 20)
+    fa32 = op.time_fapply(ela.F32, 20)
+    nc = 21
+    bytes_fa64 = 8 * (3 + 3 + nc) * nodes
+    bytes_fa32 = 4 * (3 + 3 + nc) * nodes
+    # dominant kernel of the mixed solve: level-0 FP32 Jacobi sweep vs level-0 FP64 residual
+    bytes_jac32 = 4 * (4 * 3 + nc) * nodes          # x in, b, diag, x out, C
+    bytes_res64 = (8 * (3 + 3 + nc) + 4 * 3) * nodes  # x, b, C in FP64; scaled FP32 rhs out
+    tot_jac = kt["smooth0_ms"] * kt["smooth0_n"]
+    tot_res = kt["resid64_ms"] * kt["resid64_n"]
+    if tot_jac >= tot_res:
+        dom, dom_ms, dom_bytes = "k_jacobi<float,MM,3> (level 0)", kt["smooth0_ms"], bytes_jac32
+    else:
+        dom, dom_ms, dom_bytes = "k_residual<double,float,MM,3> (level 0)", kt["resid64_ms"], bytes_res64
+    peak = 8000.0
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        log("bench: cpu baseline (oracle, %d threads) %d^3, %d F-cycles" % (threads, args.cpu_n, args.cpu_cycles))
+        per_cycle, tcpu, cyc = cpu_baseline(args.cpu_n, args.cpu_cycles, st64["iters"])
+        scale = nodes / float((args.cpu_n + 1) ** 3)
+        t_solve = per_cycle * scale * st64["iters"]
+        cpu = {"value": 1.0 / t_solve, "unit": "solves/s", "cores": threads, "kind": "port",
+               "sample": "%d F-cycles of the FP64 oracle MG on the same workload at %d^3 (%.1f s, %.2f s/cycle), "
+                         "scaled by node count to %d^3 and by the %d cycles the FP64 solve needs"
+                         % (cyc, args.cpu_n, tcpu, per_cycle, n, st64["iters"])}
+
+    out = {
+        "metric": "solves/s at 1e-10 rel residual (mixed FP32/FP64), vs plain FP64 GPU",
+        "value": args.steps / tmx,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * tmx / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "3D single level %d^3 cells, cubic Voronoi %d grains (Haar orientations, seed %d), "
+                               "uniaxial z strain 0.01, tol_rel %.0e" % (n, args.grains, K.SEED, args.tol),
+                   "nodes": nodes, "mg_levels": op.levels()},
+        "fp64_solves_per_s": args.steps / t64,
+        "mixed_over_fp64": t64 / tmx,
+        "iters": {"fp64": st64["iters"], "mixed": stmx["iters"]},
+        "rel_residual": {"fp64": st64["res"] / st64["res0"], "mixed": stmx["res"] / stmx["res0"]},
+        "rel_disp_diff_mixed_vs_fp64": disp_diff,
+        "fapply": {"fp64_ms": fa64, "fp64_GBps": bytes_fa64 / (fa64 * 1e-3) / 1e9,
+                   "fp32_ms": fa32, "fp32_GBps": bytes_fa32 / (fa32 * 1e-3) / 1e9,
+                   "bytes_per_node_fp64": 8 * (3 + 3 + nc)},
+        "kernel_times": kt,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": peak, "unit": "GB/s", "frac": achieved / peak,
+                     "traffic": None, "kernel": dom, "kernel_ms": dom_ms, "bytes_per_launch": dom_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

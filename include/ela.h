@@ -50,7 +50,7 @@ enum {
 enum { ELA_MODEL_ISO = 0, ELA_MODEL_MM = 1, ELA_MODEL_DIAG = 2 };
 enum { ELA_BC_DISP = 0, ELA_BC_TRAC = 1, ELA_BC_PERIODIC = 2, ELA_BC_NEUMANN = 3 };
 enum { ELA_F64 = 0, ELA_F32 = 1 };
-enum { ELA_SOLVE_FP64 = 0, ELA_SOLVE_MIXED = 1 };
+enum { ELA_SOLVE_FP64 = 0, ELA_SOLVE_MIXED = 1, ELA_ALLOC_BOTH = 2 };
 enum { ELA_CYCLE_F = 0, ELA_CYCLE_V = 1 };
 
 typedef struct ela_ctx ela_ctx;
@@ -70,7 +70,7 @@ typedef struct {
     int64_t plane;   /* elements between k planes */
     int64_t cstride; /* elements between components */
     double dx[3];
-} ela_level_info;
+} ela_level_layout;
 
 typedef struct {
     int iters;          /* outer iterations (MG cycles) performed */
@@ -94,12 +94,14 @@ size_t ela_ctx_bytes_in_use(const ela_ctx* ctx);
  * (src/Operator/Elastic.cpp:30-55) and Operator<Node>::define
  * (src/Operator/Operator.cpp:215-258): the MG levels are built here.  The
  * domain is coarsened by 2 while every n is even and >= 8 (max_mg_levels
- * <= 0: no further limit).  mode selects which precisions are allocated. */
+ * <= 0: no further limit).  mode selects which MG precisions are allocated:
+ * ELA_SOLVE_FP64 (FP64 levels), ELA_SOLVE_MIXED (FP32 levels plus the FP64
+ * level-0 residual) or ELA_ALLOC_BOTH. */
 int ela_hier_create(ela_ctx* ctx, const ela_domain* dom, int model_kind, int max_mg_levels, int mode,
                     ela_hier** out);
 int ela_hier_destroy(ela_hier* h);
 int ela_hier_levels(const ela_hier* h);
-int ela_level_info(const ela_hier* h, int mglev, ela_level_info* out);
+int ela_level_info(const ela_hier* h, int mglev, ela_level_layout* out);
 size_t ela_hier_bytes(const ela_hier* h);
 
 /* SetModel(const FabArray<BaseFab<T>>&) (src/Operator/Elastic.cpp:84-118):
@@ -139,6 +141,8 @@ int ela_field_alloc(ela_hier* h, int mglev, int prec, int ncomp, void** dev);
 int ela_field_free(ela_hier* h, void* dev);
 int ela_upload(ela_hier* h, int mglev, int prec, int ncomp, const double* host_dense, void* dev);
 int ela_download(ela_hier* h, int mglev, int prec, int ncomp, const void* dev, double* host_dense);
+/* Set every element of a device field (layout incl. padding) to zero, on the library's stream. */
+int ela_field_zero(ela_hier* h, int mglev, int prec, int ncomp, void* dev);
 
 /* Fapply (src/Operator/Elastic.cpp:151-260): f = A u on MG level mglev. */
 int ela_fapply(ela_hier* h, int mglev, int prec, const void* u, void* f, void* stream);
@@ -190,6 +194,9 @@ int ela_set_cycle(ela_hier* h, int cycle, int nu1, int nu2);
  * solve's stream: average duration (ms) of the level-0 FP64 residual kernel
  * and of the level-0 smoother sweep, and their launch counts. */
 int ela_kernel_times(const ela_hier* h, double* ms_resid64, int* n_resid64, double* ms_smooth0, int* n_smooth0);
+
+/* Record HIP events around the level-0 kernels of subsequent solves (on != 0). */
+int ela_set_timing(ela_hier* h, int on);
 
 /* Time nrep back-to-back launches of Fapply on level 0 with HIP events on
  * the library's stream; returns the average ms per launch. */

@@ -445,8 +445,7 @@ void fsmooth(const G& g, int kind, const double* C, const int* bctype, const dou
 
 double norm_inf(const double* v, long n) {
     double m = 0.0;
-#pragma omp parallel for reduction(max : m)
-    for (long q = 0; q < n; ++q) m = std::max(m, std::fabs(v[q]));
+    for (long q = 0; q < n; ++q) m = (std::fabs(v[q]) > m || v[q] != v[q]) ? std::fabs(v[q]) : m;
     return m;
 }
 
@@ -720,7 +719,8 @@ void or_assemble(const or_grid* og, int kind, const double* C, const int* bctype
 }
 
 // Multigrid solve of A u = b on one level (restated MLMG, see header).
-// levels: number of MG levels to use (<= 0: coarsen while every n is even and >= 4).
+// levels: number of MG levels to use (<= 0: coarsen while every n is even and >= 8;
+// a 2-cell coarsest grid with traction faces is near-singular).
 // stats[0] iterations, [1] |r0|inf, [2] final |r|inf, [3] number of MG levels.
 // hist (optional, cap entries) receives |r|inf after each iteration.
 int or_mg_solve(const or_grid* og, int kind, const double* C, const int* bctype, const double* b, double* u,
@@ -735,7 +735,7 @@ int or_mg_solve(const or_grid* og, int kind, const double* C, const int* bctype,
     for (;;) {
         mg.lev.emplace_back(cur);
         bool can = true;
-        for (int d = 0; d < dim; ++d) can = can && (cur.n[d] % 2 == 0) && (cur.n[d] >= 4);
+        for (int d = 0; d < dim; ++d) can = can && (cur.n[d] % 2 == 0) && (cur.n[d] >= 8);
         if (levels > 0 && (int)mg.lev.size() >= levels) can = false;
         if (!can) break;
         for (int d = 0; d < dim; ++d) { cur.n[d] /= 2; cur.dx[d] *= 2; }
@@ -776,7 +776,7 @@ int or_mg_solve(const or_grid* og, int kind, const double* C, const int* bctype,
             rn = solution_residual();
             if (hist && it < cap) hist[it] = rn;
             ++it;
-            if (rn <= target) break;
+            if (rn <= target || rn != rn) break;
         }
     }
     if (stats) {
@@ -785,7 +785,7 @@ int or_mg_solve(const or_grid* og, int kind, const double* C, const int* bctype,
         stats[2] = rn;
         stats[3] = (double)mg.lev.size();
     }
-    return rn <= target ? 0 : 1;
+    return (rn <= target && rn == rn) ? 0 : 1;
 }
 
 }  // extern "C"

@@ -1,0 +1,160 @@
+"""GPU parity: libela.so (gfx950 kernels, through the C ABI) against the CPU oracle.
+
+Tolerances (FP64): operator outputs agree to 1e-12 relative to the scale of
+the stencil terms (|C|max |u|max / h^2); FP32 to 2e-5 of that scale.  Solves
+agree on the discrete solution to 1e-8 relative displacement.
+"""
+import numpy as np
+import pytest
+
+import anisotropicfracture_amd as ela
+from anisotropicfracture_amd import configs as K
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+ISO, MM, DIAG = H.ISO, H.MM, H.DIAG
+
+
+def scale_of(C, u, h):
+    return np.abs(C).max() * np.abs(u).max() / min(h) ** 2 + 1e-300
+
+
+def make(n, kind, C, t, v=None, uniform=False, mode=ela.ALLOC_BOTH):
+    op = ela.Elastic(n, kind, mode=mode)
+    op.SetModel(C)
+    op.SetBC(t, v)
+    op.SetUniform(uniform)
+    op.prepareForSolve()
+    return op
+
+
+CASES = [(2, ISO, (20, 13)), (2, MM, (17, 9)), (2, DIAG, (12, 12)),
+         (3, ISO, (9, 7, 6)), (3, MM, (11, 6, 8)), (3, DIAG, (6, 6, 6)), (3, MM, (70, 5, 4))]
+
+
+@pytest.mark.parametrize("dim,kind,n", CASES)
+@pytest.mark.parametrize("uniform", [False, True])
+def test_fapply_parity(oracle, dim, kind, n, uniform):
+    print("case", dim, kind, n, uniform, flush=True)
+    rng = np.random.default_rng(hash((dim, kind, n)) % 2 ** 32)
+    g = oracle.Grid(n)
+    C = H.random_stiffness(rng, n, kind, dim)
+    t, v = H.bc_mixed(dim)
+    u = H.random_field(rng, n, dim)
+    want = oracle.fapply(g, kind, C, t, u, uniform=uniform)
+    op = make(n, kind, C, t, v, uniform)
+    s = scale_of(C, u, g.dx)
+    got = op.Fapply(u, prec=ela.F64)
+    assert np.abs(got - want).max() <= 1e-12 * s
+    got32 = op.Fapply(u, prec=ela.F32)
+    assert np.abs(got32 - want).max() <= 2e-5 * s
+
+
+@pytest.mark.parametrize("dim,kind,n", CASES[:6])
+def test_diagonal_parity(oracle, dim, kind, n):
+    rng = np.random.default_rng(5)
+    g = oracle.Grid(n)
+    C = H.random_stiffness(rng, n, kind, dim)
+    t, v = H.bc_mixed(dim)
+    op = make(n, kind, C, t, v)
+    want = oracle.diagonal(g, kind, C, t)
+    np.testing.assert_allclose(op.Diagonal(0, ela.F64), want, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(op.Diagonal(0, ela.F32), want, rtol=2e-7, atol=1e-30)
+
+
+@pytest.mark.parametrize("dim,n", [(2, (16, 8)), (3, (16, 8, 8))])
+def test_transfer_and_coarse_operator_parity(oracle, dim, n):
+    rng = np.random.default_rng(17)
+    kind = MM
+    g = oracle.Grid(n)
+    gc = g.coarse()
+    C = H.random_stiffness(rng, n, kind, dim)
+    t, v = H.bc_mixed(dim)
+    op = make(n, kind, C, t, v)
+    assert op.levels() >= 2
+    f = H.random_field(rng, n, dim)
+    np.testing.assert_allclose(op.restriction(f, 1), oracle.restrict(gc, f), rtol=1e-14, atol=1e-15)
+    c = H.random_field(rng, gc.n, dim)
+    np.testing.assert_allclose(op.interpolation(f, c, 0), oracle.interp_add(g, c, f), rtol=1e-14, atol=1e-15)
+    # coarse operator uses the restricted stiffness (averageDownCoeffsSameAmrLevel)
+    Cc = oracle.restrict(gc, C)
+    uc = H.random_field(rng, gc.n, dim)
+    want = oracle.fapply(gc, kind, Cc, t, uc)
+    got = op.Fapply(uc, mglev=1)
+    assert np.abs(got - want).max() <= 1e-12 * scale_of(Cc, uc, gc.dx)
+
+
+def test_fsmooth_parity(oracle):
+    rng = np.random.default_rng(2)
+    n, kind = (9, 8, 7), MM
+    g = oracle.Grid(n)
+    C = H.random_stiffness(rng, n, kind, 3)
+    t, v = H.bc_mixed(3)
+    op = make(n, kind, C, t, v)
+    D = oracle.diagonal(g, kind, C, t)
+    x, b = H.random_field(rng, n, 3), H.random_field(rng, n, 3)
+    want = oracle.fsmooth(g, kind, C, t, D, x, b)
+    got = op.Fsmooth(x, b, nsweeps=2)
+    np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-12 * np.abs(want).max())
+
+
+def test_voronoi_stiffness_kernel(oracle):
+    """Grain assignment bit-exact with the integer nearest-seed checker; stiffness values to 1e-15."""
+    cfg = K.cfg2(24, grains=17, seed=9)
+    C_want, gid = H.voronoi_cubic_field(cfg, oracle.cubic_define)
+    op = ela.Elastic(cfg["n"], MM, mode=ela.ALLOC_BOTH)
+    op.SetModelVoronoiCubic(cfg["seeds"], K.VORONOI_Q, cfg["R"], cfg["C11"], cfg["C12"], cfg["C44"])
+    op.SetBC(cfg["bc_type"], cfg["bc_value"])
+    op.prepareForSolve()
+    # read back through Fapply of unit-free probes is indirect; compare the operator instead
+    rng = np.random.default_rng(1)
+    u = H.random_field(rng, cfg["n"], 3)
+    g = oracle.Grid(cfg["n"])
+    want = oracle.fapply(g, MM, C_want, cfg["bc_type"], u)
+    got = op.Fapply(u)
+    assert np.abs(got - want).max() <= 1e-12 * scale_of(C_want, u, g.dx)
+
+
+def _solve_case(oracle, cfg_n=16, grains=12):
+    cfg = K.cfg2(cfg_n, grains=grains, seed=3)
+    C, _ = H.voronoi_cubic_field(cfg, oracle.cubic_define)
+    g = oracle.Grid(cfg["n"])
+    b = oracle.bc_init(g, cfg["bc_type"], cfg["bc_value"], g.zeros())
+    return cfg, C, g, b
+
+
+@pytest.mark.parametrize("mode", [ela.SOLVE_FP64, ela.SOLVE_MIXED])
+def test_solve_matches_oracle_cubic(oracle, mode):
+    print("solve mode", mode, flush=True)
+    cfg, C, g, b = _solve_case(oracle)
+    u_or, st_or, hist = oracle.mg_solve(g, MM, C, cfg["bc_type"], b, tol_rel=1e-12, max_iter=300)
+    assert st_or["converged"]
+    op = make(cfg["n"], MM, C, cfg["bc_type"], cfg["bc_value"])
+    b_gpu = op.bc_init()
+    np.testing.assert_array_equal(b_gpu, b)
+    u, st = ela.Linear(op).solve(np.zeros_like(b), b, tol_rel=1e-10, tol_abs=0.0, mode=mode, max_iter=300)
+    assert st["converged"]
+    assert st["res"] / st["res0"] <= 1e-10
+    err = np.abs(u - u_or).max() / np.abs(u_or).max()
+    print("rel displacement error", err, "iters", st["iters"], "oracle iters", st_or["iters"])
+    assert err <= 1e-8
+    if mode == ela.SOLVE_FP64:
+        # same algorithm: the iteration counts to 1e-10 agree within one cycle
+        it_or = int(np.argmax(hist <= 1e-10 * st_or["res0"])) + 1
+        assert abs(st["iters"] - it_or) <= 1
+
+
+@pytest.mark.parametrize("mode", [ela.SOLVE_FP64, ela.SOLVE_MIXED])
+def test_uniaxial_known_answer_gpu(mode):
+    """UniaxialTest at the reference's 32^3: u_x = 0.1 x exactly (to solver tolerance)."""
+    from tests.test_oracle import uniaxial_case
+    n = 32
+    C, t, v = uniaxial_case(n)
+    op = make((n, n, n), ISO, C, t, v)
+    b = op.bc_init()
+    u, st = ela.Linear(op).solve(np.zeros_like(b), b, tol_rel=1e-10, tol_abs=0.0, mode=mode)
+    x = np.arange(n + 1) / n
+    exact = 0.1 * np.broadcast_to(x, (n + 1,) * 3)
+    assert np.abs(u[0] - exact).max() / 0.1 < 1e-8
+    assert np.abs(u[1:]).max() < 1e-9
