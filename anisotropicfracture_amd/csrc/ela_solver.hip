@@ -433,12 +433,6 @@ int ela_hier_create(ela_ctx* ctx, const ela_domain* dom, int model_kind, int max
         for (int d = 0; d < dom->dim; ++d) { n[d] /= 2; dx[d] *= 2; }
     }
     h->nunk = h->dim * h->lev.back().nodes;
-    if (h->nunk > kMaxDenseUnknowns) {
-        ela_hier_destroy(h);
-        return fail(ELA_ERR_BADARG, "coarsest MG level has " + std::to_string(h->nunk) +
-                                        " unknowns (max " + std::to_string(kMaxDenseUnknowns) +
-                                        "); use n = m * 2^k with m <= 8");
-    }
     int rc = ELA_OK;
     for (size_t l = 0; l < h->lev.size() && rc == ELA_OK; ++l) {
         Level& v = h->lev[l];
@@ -464,7 +458,7 @@ int ela_hier_create(ela_ctx* ctx, const ela_domain* dom, int model_kind, int max
     }
     if (!rc) rc = halloc_t(h, field_elems(h->lev[0], h->dim), &h->u64);
     if (!rc) rc = halloc_t(h, field_elems(h->lev[0], h->dim), &h->b64);
-    if (!rc) rc = halloc_t(h, (size_t)h->nunk * 2 * h->nunk, &h->M);
+    if (!rc && h->nunk <= kMaxDenseUnknowns) rc = halloc_t(h, (size_t)h->nunk * 2 * h->nunk, &h->M);
     if (!rc) rc = halloc_t(h, 4, &h->dnorm);
     if (!rc) rc = halloc_t(h, 4, &h->dscale);
     if (!rc) rc = halloc_t(h, 4, &h->dflag);
@@ -660,6 +654,10 @@ int ela_prepare(ela_hier* h) {
     HIPCHK(hipStreamSynchronize(h->stream));
     if (flag) return fail(ELA_ERR_NAN, "diagonal is nan or zero (Elastic.cpp:347 aborts here)");
     // dense coarsest operator: columns A e_q, then Gauss-Jordan [A | I] -> [I | A^-1]
+    if (!h->M) {  // operator-only use (Fapply, Fsmooth, ...) of a box that does not coarsen far enough
+        h->prepared = true;
+        return ELA_OK;
+    }
     Level& v = h->lev.back();
     const int n = h->nunk;
     double *E = nullptr, *F = nullptr;
@@ -853,6 +851,9 @@ int ela_bc_init(ela_hier* h, double* rhs, int homogeneous) {
 }
 
 static int solve_common(ela_hier* h, double tol_rel, double tol_abs, int mode, int max_iter, ela_stats* st) {
+    if (!h->M)
+        return fail(ELA_ERR_BADARG, "coarsest MG level has " + std::to_string(h->nunk) + " unknowns (max " +
+                                        std::to_string(kMaxDenseUnknowns) + "); use n = m * 2^k with m <= 8");
     if (mode == ELA_SOLVE_FP64 && !h->has64) return fail(ELA_ERR_STATE, "hierarchy has no FP64 levels");
     if (mode == ELA_SOLVE_MIXED && !h->has32) return fail(ELA_ERR_STATE, "hierarchy has no FP32 levels");
     if (mode != ELA_SOLVE_FP64 && mode != ELA_SOLVE_MIXED) return fail(ELA_ERR_BADARG, "unknown solve mode");
