@@ -99,8 +99,10 @@ SIGNATURES = {
     "ela_solve_device": ([_P, _P, _P, _D, _D, _I, _I, ctypes.POINTER(Stats)], _I),
     "ela_set_cycle": ([_P, _I, _I, _I], _I),
     "ela_set_timing": ([_P, _I], _I),
+    "ela_set_krylov": ([_P, _I], _I),
     "ela_kernel_times": ([_P, ctypes.POINTER(_D), ctypes.POINTER(_I), ctypes.POINTER(_D), ctypes.POINTER(_I)], _I),
     "ela_time_fapply": ([_P, _I, _I, ctypes.POINTER(_D)], _I),
+    "ela_probe_copy": ([_P, _SZ, _I, ctypes.POINTER(_D)], _I),
 }
 
 
@@ -142,6 +144,12 @@ class Context:
         p = ctypes.c_void_p()
         _check(lib().ela_ctx_create(device, hbm_budget_bytes, ctypes.byref(p)))
         self.ptr = p
+
+    def probe_copy(self, nbytes, nrep=10):
+        """Average ms of a plain device copy of nbytes (the HBM-counter calibration probe)."""
+        ms = ctypes.c_double()
+        _check(lib().ela_probe_copy(self.ptr, nbytes, nrep, ctypes.byref(ms)))
+        return ms.value
 
     def bytes_in_use(self):
         return lib().ela_ctx_bytes_in_use(self.ptr)
@@ -265,6 +273,10 @@ class Elastic:
 
     def SetCycle(self, cycle=CYCLE_F, nu1=2, nu2=2):
         _check(lib().ela_set_cycle(self.ptr, cycle, nu1, nu2))
+
+    def SetKrylov(self, restart):
+        """Flexible GMRES(restart) around the MG cycle (0: plain MG iteration)."""
+        _check(lib().ela_set_krylov(self.ptr, int(restart)))
 
     def prepareForSolve(self):
         _check(lib().ela_prepare(self.ptr))

@@ -9,9 +9,14 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 namespace ela {
+
+extern bool g_slab_order;
+extern int g_slab_walk;
+extern int g_slab_f64;
 
 namespace {
 
@@ -26,23 +31,90 @@ __device__ __forceinline__ bool node_index(const Lay& L, int& i, int& j, int& k)
 
 inline dim3 node_grid(const Lay& L) { return dim3((L.nx + BX - 1) / BX, (L.ny + BY - 1) / BY, L.nz); }
 
-// Wavefront max of a non-negative value, then one atomicMax per wavefront on
-// the bit pattern (ordered like the value for non-negative IEEE numbers; a
-// NaN pattern orders above +inf so it propagates).
-__device__ __forceinline__ void wave_max_atomic(unsigned long long* dst, double v) {
-    unsigned long long b = __double_as_longlong(v);
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o = __shfl_xor(b, off, 64);
-        b = o > b ? o : b;
+// XCD-slab numbering of the stencil kernels on 3D levels.  Hardware block b
+// runs on XCD b % 8; XCD x owns the rows [x*rows, (x+1)*rows) and walks its
+// slab y-tile fastest, then z, then x-tile, so a tile's z neighbours are
+// processed 8 blocks apart in the same L2 and its y neighbours next to it.
+struct Slab {
+    int rows, nty, ntx, order;
+};
+inline Slab slab_of(const Lay& L) {
+    Slab S;
+    S.order = g_slab_walk;
+    S.rows = (L.ny + 7) / 8;
+    S.nty = (S.rows + BY - 1) / BY;
+    S.ntx = (L.nx + BX - 1) / BX;
+    return S;
+}
+inline dim3 slab_grid(const Lay& L) {
+    const Slab S = slab_of(L);
+    return dim3((unsigned)(8LL * S.nty * L.nz * S.ntx));
+}
+__device__ __forceinline__ bool slab_index(const Lay& L, const Slab S, int& i, int& j, int& k) {
+    const int b = blockIdx.x, x = b & 7, r = b >> 3;
+    int tyl, tx;
+    if (S.order == 0) {  // y-tile fastest, then z, then x-tile
+        tyl = r % S.nty;
+        const int rest = r / S.nty;
+        k = rest % L.nz;
+        tx = rest / L.nz;
+    } else {  // x-tile fastest, then y-tile, then z
+        tx = r % S.ntx;
+        const int rest = r / S.ntx;
+        tyl = rest % S.nty;
+        k = rest / S.nty;
     }
-    if ((threadIdx.x & 63) == 0) atomicMax(dst, b);
+    i = tx * BX + threadIdx.x;
+    j = x * S.rows + tyl * BY + threadIdx.y;
+    const int jend = min(L.ny, (x + 1) * S.rows);
+    return i < L.nx && j < jend && k < L.nz;
+}
+
+// Block max of a non-negative value (NaN propagates: a NaN compares as
+// "greater"), written to partials[linear block id]; k_reduce_partials then
+// folds the partials.  No atomics, so the result is deterministic.
+__device__ __forceinline__ void block_max_store(double* partials, double v) {
+    __shared__ double wmax[16];
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = (o > v || o != o) ? o : v;
+    }
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    const int nw = (blockDim.x * blockDim.y + 63) / 64;
+    if ((tid & 63) == 0) wmax[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+        double m = wmax[0];
+        for (int w = 1; w < nw; ++w) m = (wmax[w] > m || wmax[w] != wmax[w]) ? wmax[w] : m;
+        partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = m;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_reduce_partials(const double* __restrict__ partials, int n,
+                                                          unsigned long long* norm) {
+    __shared__ double sm[1024];
+    double m = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        const double v = partials[q];
+        m = (v > m || v != v) ? v : m;
+    }
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            const double v = sm[threadIdx.x + s];
+            sm[threadIdx.x] = (v > sm[threadIdx.x] || v != v) ? v : sm[threadIdx.x];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *norm = (unsigned long long)__double_as_longlong(sm[0]);
 }
 
 template <typename T, int K, int DIM>
 __global__ __launch_bounds__(256) void k_fapply(Lay L, const T* __restrict__ C, const T* __restrict__ u,
-                                                T* __restrict__ f, BCTab bc, Inv<T> iv, int uniform) {
+                                                T* __restrict__ f, BCTab bc, Inv<T> iv, int uniform, Slab S) {
     int i, j, k;
-    if (!node_index(L, i, j, k)) return;
+    if (!(S.rows ? slab_index(L, S, i, j, k) : node_index(L, i, j, k))) return;
     T r[DIM];
     node_apply<T, K, DIM>(L, C, u, i, j, k, bc, iv, uniform != 0, r);
     const long long o = (long long)i + L.px * j + L.pl * k;
@@ -51,15 +123,15 @@ __global__ __launch_bounds__(256) void k_fapply(Lay L, const T* __restrict__ C, 
 }
 
 // r = b - A x; optional outputs: r in T, r * scale in TO (the next cycle's
-// right-hand side), and max |r| via atomicMax.  All-lanes participate in the
-// wavefront reduction (out-of-range lanes contribute 0).
+// right-hand side), and the block maxima of |r|.  Every lane reaches the
+// block reduction (out-of-range lanes contribute 0).
 template <typename T, typename TO, int K, int DIM>
 __global__ __launch_bounds__(256) void k_residual(Lay L, const T* __restrict__ C, const T* __restrict__ x,
                                                   const T* __restrict__ b, T* __restrict__ r, TO* __restrict__ rs,
-                                                  const double* __restrict__ scale, unsigned long long* norm,
-                                                  BCTab bc, Inv<T> iv, int uniform) {
+                                                  const double* __restrict__ scale, double* __restrict__ partials,
+                                                  BCTab bc, Inv<T> iv, int uniform, Slab S) {
     int i, j, k;
-    const bool in = node_index(L, i, j, k);
+    const bool in = S.rows ? slab_index(L, S, i, j, k) : node_index(L, i, j, k);
     double m = 0.0;
     if (in) {
         T f[DIM];
@@ -75,16 +147,17 @@ __global__ __launch_bounds__(256) void k_residual(Lay L, const T* __restrict__ C
             m = (a > m || a != a) ? a : m;
         }
     }
-    if (norm) wave_max_atomic(norm, m);
+    if (partials) block_max_store(partials, m);
 }
 
 // One weighted-Jacobi sweep x_out = x_in + omega (b - A x_in) / D.
 template <typename T, int K, int DIM>
 __global__ __launch_bounds__(256) void k_jacobi(Lay L, const T* __restrict__ C, const T* __restrict__ dg,
                                                 const T* __restrict__ xin, const T* __restrict__ b,
-                                                T* __restrict__ xout, BCTab bc, Inv<T> iv, int uniform, T omega) {
+                                                T* __restrict__ xout, BCTab bc, Inv<T> iv, int uniform, T omega,
+                                                Slab S) {
     int i, j, k;
-    if (!node_index(L, i, j, k)) return;
+    if (!(S.rows ? slab_index(L, S, i, j, k) : node_index(L, i, j, k))) return;
     T f[DIM];
     node_apply<T, K, DIM>(L, C, xin, i, j, k, bc, iv, uniform != 0, f);
     const long long o = (long long)i + L.px * j + L.pl * k;
@@ -427,7 +500,97 @@ __global__ __launch_bounds__(256) void k_pack(Lay L, int nc, const double* __res
     }
 }
 
+// Plain streaming copy (16 B per lane per step), the calibration probe of the
+// HBM byte counters.
+__global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src, double2* __restrict__ dst, long long n) {
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
+        dst[q] = src[q];
+}
+
+// ---------------------------------------------------------------- flat vector kernels (Krylov)
+// Vectors are whole level fields (dim * cstride doubles, padding kept at 0).
+constexpr int FB = 256, FG = 1024, CH = 8;
+
+// Partial dot products of x with up to CH vectors: partials[(v0 + v) * FG + block].
+__global__ __launch_bounds__(FB) void k_mdot(const double* __restrict__ x, VecList Y, int v0, int nv, long long n,
+                                             double* __restrict__ partials) {
+    double acc[CH];
+#pragma unroll
+    for (int v = 0; v < CH; ++v) acc[v] = 0.0;
+    for (long long q = (long long)blockIdx.x * FB + threadIdx.x; q < n; q += (long long)FG * FB) {
+        const double xq = x[q];
+#pragma unroll
+        for (int v = 0; v < CH; ++v)
+            if (v < nv) acc[v] += xq * Y.v[v0 + v][q];
+    }
+    __shared__ double red[FB / 64][CH];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int v = 0; v < CH; ++v) {
+        double a = acc[v];
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+        if (lane == 0) red[w][v] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < CH && (int)threadIdx.x < nv) {
+        double a = 0.0;
+        for (int ww = 0; ww < FB / 64; ++ww) a += red[ww][threadIdx.x];
+        partials[(long long)(v0 + threadIdx.x) * FG + blockIdx.x] = a;
+    }
+}
+
+// out[v] = sum of the FG partials of vector v, in a fixed order.
+__global__ __launch_bounds__(256) void k_mdot_final(const double* __restrict__ partials, double* __restrict__ out) {
+    __shared__ double sm[256];
+    double a = 0.0;
+    for (int q = threadIdx.x; q < FG; q += 256) a += partials[(long long)blockIdx.x * FG + q];
+    sm[threadIdx.x] = a;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sm[0];
+}
+
+// y += sign * sum_v coef[v] X_v
+__global__ __launch_bounds__(FB) void k_maxpy(double* __restrict__ y, VecList X, int nv,
+                                              const double* __restrict__ coef, double sign, long long n) {
+    for (long long q = (long long)blockIdx.x * FB + threadIdx.x; q < n; q += (long long)FG * FB) {
+        double a = 0.0;
+        for (int v = 0; v < nv; ++v) a += coef[v] * X.v[v][q];
+        y[q] += sign * a;
+    }
+}
+
+__global__ __launch_bounds__(FB) void k_scale_flat(double* __restrict__ y, const double* __restrict__ x, double a,
+                                                   long long n) {
+    for (long long q = (long long)blockIdx.x * FB + threadIdx.x; q < n; q += (long long)FG * FB) y[q] = a * x[q];
+}
+
 }  // namespace
+
+void launch_mdot(const double* x, const VecList& Y, int nv, long long n, double* partials, double* out,
+                 hipStream_t s) {
+    for (int v0 = 0; v0 < nv; v0 += CH)
+        hipLaunchKernelGGL(k_mdot, dim3(FG), dim3(FB), 0, s, x, Y, v0, std::min(CH, nv - v0), n, partials);
+    hipLaunchKernelGGL(k_mdot_final, dim3(nv), dim3(256), 0, s, partials, out);
+}
+
+void launch_maxpy(double* y, const VecList& X, int nv, const double* coef, double sign, long long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_maxpy, dim3(FG), dim3(FB), 0, s, y, X, nv, coef, sign, n);
+}
+
+void launch_scale_flat(double* y, const double* x, double a, long long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_scale_flat, dim3(FG), dim3(FB), 0, s, y, x, a, n);
+}
+
+long long mdot_partials(int nv) { return (long long)nv * FG; }
+
+void launch_copy(const void* src, void* dst, long long bytes, hipStream_t s) {
+    const long long n = bytes / 16;
+    hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, s, (const double2*)src, (double2*)dst, n);
+}
 
 // ------------------------------------------------------------------ launchers
 #define ELA_DISPATCH(KIND, DIMV, ...)                                              \
@@ -461,27 +624,50 @@ template <typename T>
 void launch_fapply(const Lay& L, int kind, const T* C, const T* u, T* f, const BCTab& bc, const double h[3],
                    int uniform, hipStream_t s) {
     const Inv<T> iv = make_inv<T>(h);
+    const bool sl = L.dim == 3 && g_slab_order && (sizeof(T) == 4 || g_slab_f64);
+    const Slab S = sl ? slab_of(L) : Slab{0, 0, 0, 0};
+    const dim3 grid = sl ? slab_grid(L) : node_grid(L);
     ELA_DISPATCH(kind, L.dim,
-                 hipLaunchKernelGGL((k_fapply<T, KK, DD>), node_grid(L), dim3(BX, BY), 0, s, L, C, u, f, bc, iv,
-                                    uniform));
+                 hipLaunchKernelGGL((k_fapply<T, KK, DD>), grid, dim3(BX, BY), 0, s, L, C, u, f, bc, iv,
+                                    uniform, S));
 }
 
 template <typename T, typename TO>
 void launch_residual(const Lay& L, int kind, const T* C, const T* x, const T* b, T* r, TO* rs, const double* scale,
-                     unsigned long long* norm, const BCTab& bc, const double h[3], int uniform, hipStream_t s) {
+                     double* partials, unsigned long long* norm, const BCTab& bc, const double h[3], int uniform,
+                     hipStream_t s) {
     const Inv<T> iv = make_inv<T>(h);
+    const bool sl = L.dim == 3 && g_slab_order && (sizeof(T) == 4 || g_slab_f64);
+    const Slab S = sl ? slab_of(L) : Slab{0, 0, 0, 0};
+    const dim3 grid = sl ? slab_grid(L) : node_grid(L);
     ELA_DISPATCH(kind, L.dim,
-                 hipLaunchKernelGGL((k_residual<T, TO, KK, DD>), node_grid(L), dim3(BX, BY), 0, s, L, C, x, b, r,
-                                    rs, scale, norm, bc, iv, uniform));
+                 hipLaunchKernelGGL((k_residual<T, TO, KK, DD>), grid, dim3(BX, BY), 0, s, L, C, x, b, r,
+                                    rs, scale, norm ? partials : nullptr, bc, iv, uniform, S));
+    if (norm) {
+        const dim3 g = grid;
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partials, (int)(g.x * g.y * g.z), norm);
+    }
 }
+
+long long residual_partials(const Lay& L) {
+    const dim3 g = node_grid(L), h = slab_grid(L);
+    return std::max((long long)g.x * g.y * g.z, (long long)h.x);
+}
+
+bool g_slab_order = true;
+int g_slab_walk = 1;
+int g_slab_f64 = 1;
 
 template <typename T>
 void launch_jacobi(const Lay& L, int kind, const T* C, const T* dg, const T* xin, const T* b, T* xout,
                    const BCTab& bc, const double h[3], int uniform, double omega, hipStream_t s) {
     const Inv<T> iv = make_inv<T>(h);
+    const bool sl = L.dim == 3 && g_slab_order && (sizeof(T) == 4 || g_slab_f64);
+    const Slab S = sl ? slab_of(L) : Slab{0, 0, 0, 0};
+    const dim3 grid = sl ? slab_grid(L) : node_grid(L);
     ELA_DISPATCH(kind, L.dim,
-                 hipLaunchKernelGGL((k_jacobi<T, KK, DD>), node_grid(L), dim3(BX, BY), 0, s, L, C, dg, xin, b,
-                                    xout, bc, iv, uniform, (T)omega));
+                 hipLaunchKernelGGL((k_jacobi<T, KK, DD>), grid, dim3(BX, BY), 0, s, L, C, dg, xin, b,
+                                    xout, bc, iv, uniform, (T)omega, S));
 }
 
 template <typename T>
@@ -576,13 +762,13 @@ template void launch_fapply<double>(const Lay&, int, const double*, const double
 template void launch_fapply<float>(const Lay&, int, const float*, const float*, float*, const BCTab&, const double*,
                                    int, hipStream_t);
 template void launch_residual<double, double>(const Lay&, int, const double*, const double*, const double*, double*,
-                                              double*, const double*, unsigned long long*, const BCTab&,
+                                              double*, const double*, double*, unsigned long long*, const BCTab&,
                                               const double*, int, hipStream_t);
 template void launch_residual<double, float>(const Lay&, int, const double*, const double*, const double*, double*,
-                                             float*, const double*, unsigned long long*, const BCTab&, const double*,
+                                             float*, const double*, double*, unsigned long long*, const BCTab&, const double*,
                                              int, hipStream_t);
 template void launch_residual<float, float>(const Lay&, int, const float*, const float*, const float*, float*,
-                                            float*, const double*, unsigned long long*, const BCTab&, const double*,
+                                            float*, const double*, double*, unsigned long long*, const BCTab&, const double*,
                                             int, hipStream_t);
 template void launch_jacobi<double>(const Lay&, int, const double*, const double*, const double*, const double*,
                                     double*, const BCTab&, const double*, int, double, hipStream_t);
@@ -598,6 +784,8 @@ template void launch_axpy<double, double>(const Lay&, int, const double*, double
                                           hipStream_t);
 template void launch_axpy<float, double>(const Lay&, int, const float*, double*, const double*, double, int,
                                          hipStream_t);
+template void launch_axpy<double, float>(const Lay&, int, const double*, float*, const double*, double, int,
+                                         hipStream_t);
 template void launch_axpy<float, float>(const Lay&, int, const float*, float*, const double*, double, int,
                                         hipStream_t);
 template void launch_convert<double, float>(long long, const double*, float*, hipStream_t);
@@ -606,4 +794,9 @@ template void launch_bottom<float>(const Lay&, int, int, const double*, const fl
 template void launch_pack<double>(const Lay&, int, const double*, double*, int, hipStream_t);
 template void launch_pack<float>(const Lay&, int, const double*, float*, int, hipStream_t);
 
+}  // namespace ela
+
+namespace ela {
+template Inv<double> make_inv<double>(const double*);
+template Inv<float> make_inv<float>(const double*);
 }  // namespace ela

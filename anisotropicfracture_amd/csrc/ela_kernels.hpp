@@ -6,6 +6,11 @@
 
 namespace ela {
 
+// Stencil kernels of 3D levels use the XCD-slab block numbering when set (default).
+extern bool g_slab_order;
+extern int g_slab_walk;  // 0: y-tile, z, x-tile; 1: x-tile, y-tile, z (fastest first)
+extern int g_slab_f64;   // slab numbering for FP64 kernels too
+
 template <typename T>
 Inv<T> make_inv(const double h[3]);
 
@@ -13,10 +18,13 @@ template <typename T>
 void launch_fapply(const Lay& L, int kind, const T* C, const T* u, T* f, const BCTab& bc, const double h[3],
                    int uniform, hipStream_t s);
 
-// r = b - A x; r (T), rs = r * (*scale) (TO), max|r| into *norm; any output may be null.
+// r = b - A x; r (T), rs = r * (*scale) (TO), max|r| into *norm (bit pattern of the double,
+// via residual_partials(L) doubles of scratch in partials); any output may be null.
 template <typename T, typename TO>
 void launch_residual(const Lay& L, int kind, const T* C, const T* x, const T* b, T* r, TO* rs, const double* scale,
-                     unsigned long long* norm, const BCTab& bc, const double h[3], int uniform, hipStream_t s);
+                     double* partials, unsigned long long* norm, const BCTab& bc, const double h[3], int uniform,
+                     hipStream_t s);
+long long residual_partials(const Lay& L);
 
 template <typename T>
 void launch_jacobi(const Lay& L, int kind, const T* C, const T* dg, const T* xin, const T* b, T* xout,
@@ -59,6 +67,23 @@ void launch_voronoi_cubic(const Lay& L, int nseeds, const long long* seeds, int 
                           int* gid, hipStream_t s);
 void launch_degraded_iso(const Lay& L, const double* eta, double lambda0, double mu0, double a, double b, double c,
                          double* C, hipStream_t s);
+
+// Flat FP64 vector kernels of the Krylov accelerator (whole fields incl. zero padding).
+constexpr int kMaxKrylov = 33;
+struct VecList {
+    const double* v[kMaxKrylov];
+};
+// out[v] = (x, Y_v) for v < nv (deterministic two-pass reduction; partials: mdot_partials(nv) doubles).
+void launch_mdot(const double* x, const VecList& Y, int nv, long long n, double* partials, double* out,
+                 hipStream_t s);
+long long mdot_partials(int nv);
+// y += sign * sum_v coef[v] X_v, coef on the device.
+void launch_maxpy(double* y, const VecList& X, int nv, const double* coef, double sign, long long n, hipStream_t s);
+// y = a x
+void launch_scale_flat(double* y, const double* x, double a, long long n, hipStream_t s);
+
+// Streaming copy of bytes (multiple of 16) from src to dst.
+void launch_copy(const void* src, void* dst, long long bytes, hipStream_t s);
 
 // Dense host-order array (nc planes of nx*ny*nz, already on the device) <-> level layout.
 template <typename T>

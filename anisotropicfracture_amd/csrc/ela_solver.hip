@@ -21,6 +21,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -28,6 +29,7 @@
 
 #include "../../include/ela.h"
 #include "ela_kernels.hpp"
+#include "ela_march.hpp"
 
 using namespace ela;
 
@@ -52,6 +54,7 @@ constexpr int kMaxEvents = 8192;
 
 struct Level {
     Lay L{};
+    MarchPlan mp{};
     int n[3] = {0, 0, 0};
     double h[3] = {1, 1, 1};
     int nodes = 0;
@@ -105,11 +108,19 @@ struct ela_hier {
     int nunk = 0;
     // device scalars
     unsigned long long* dnorm = nullptr;
+    double* partials = nullptr;  // per-block maxima of the residual kernels
     double* dscale = nullptr;
     int* dflag = nullptr;
     unsigned long long* hnorm = nullptr;  // pinned
+    // flexible GMRES around the MG cycle (restart length; 0 = plain MG iteration)
+    int krylov = 0;
+    std::vector<double*> V, Z;
+    double *kr = nullptr, *kdev = nullptr, *kpart = nullptr;
     // cycle options
     int cycle = ELA_CYCLE_F, nu1 = 2, nu2 = 2;
+    bool use_march = true;
+    int pitch_align = 32;  // elements; rows start on 128 B (FP32) / 256 B (FP64) boundaries
+    int march_kz = 32, march_ty = 4;
     double omega = 2.0 / 3.0;
     // kernel timing of the last solve
     bool timing = false;
@@ -170,6 +181,16 @@ int ncomp_of(int dim, int kind) { return kind == ELA_MODEL_ISO ? 2 : (kind == EL
 
 size_t field_elems(const Level& v, int nc) { return (size_t)v.L.cs * nc; }
 
+// Operator application on a level: the LDS-staged marching kernels where the
+// level is large enough (3D), the one-thread-per-node kernels otherwise.
+template <typename T>
+void A_apply(ela_hier* h, Level& v, const T* C, const T* u, T* f, hipStream_t s);
+template <typename T>
+void A_jacobi(ela_hier* h, Level& v, const T* C, const T* dg, const T* xin, const T* b, T* xout, hipStream_t s);
+template <typename T, typename TO>
+void A_residual(ela_hier* h, Level& v, const T* C, const T* x, const T* b, T* r, TO* rs, const double* scale,
+                unsigned long long* norm, hipStream_t s);
+
 // Kernel timing around level-0 launches.
 int ev_begin(ela_hier* h) {
     if (!h->timing || h->ev_used + 2 > (int)h->ev.size()) return -1;
@@ -193,11 +214,11 @@ void smooth_call(ela_hier* h, int l, bool zero_start) {
     if (zero_start) launch_jacobi0<T>(v.L, b.dg, *b.res, *b.tmp, h->omega, h->stream);
     else {
         const int e = t0 ? ev_begin(h) : -1;
-        launch_jacobi<T>(v.L, h->kind, b.C, b.dg, *b.cor, *b.res, *b.tmp, h->bc, v.h, h->uniform, h->omega, h->stream);
+        A_jacobi<T>(h, v, b.C, b.dg, *b.cor, *b.res, *b.tmp, h->stream);
         if (t0) ev_end(h, e, h->ev_smooth);
     }
     const int e = t0 ? ev_begin(h) : -1;
-    launch_jacobi<T>(v.L, h->kind, b.C, b.dg, *b.tmp, *b.res, *b.cor, h->bc, v.h, h->uniform, h->omega, h->stream);
+    A_jacobi<T>(h, v, b.C, b.dg, *b.tmp, *b.res, *b.cor, h->stream);
     if (t0) ev_end(h, e, h->ev_smooth);
 }
 
@@ -205,8 +226,7 @@ template <typename T>
 void res_of_cor(ela_hier* h, int l) {
     Level& v = h->lev[l];
     LB<T> b = lb<T>(v);
-    launch_residual<T, T>(v.L, h->kind, b.C, *b.cor, *b.res, *b.rc, (T*)nullptr, nullptr, nullptr, h->bc, v.h,
-                          h->uniform, h->stream);
+    A_residual<T, T>(h, v, b.C, *b.cor, *b.res, *b.rc, (T*)nullptr, nullptr, nullptr, h->stream);
 }
 
 template <typename T>
@@ -272,10 +292,8 @@ void one_cycle(ela_hier* h) {
 template <typename T>
 void solution_residual(ela_hier* h, const double* u, const double* b) {
     Level& v = h->lev[0];
-    launch_zero_u64(h->dnorm, 1, h->stream);
     const int e = ev_begin(h);
-    launch_residual<double, T>(v.L, h->kind, v.C64, u, b, (double*)nullptr, *lb<T>(v).res, h->dscale, h->dnorm,
-                               h->bc, v.h, h->uniform, h->stream);
+    A_residual<double, T>(h, v, v.C64, u, b, (double*)nullptr, *lb<T>(v).res, h->dscale, h->dnorm, h->stream);
     ev_end(h, e, h->ev_resid);
 }
 
@@ -328,6 +346,162 @@ int solve_loop(ela_hier* h, const double* b, double* u, double tol_rel, double t
     return status;
 }
 
+int krylov_alloc(ela_hier* h) {
+    const int m = h->krylov;
+    if ((int)h->Z.size() == m && (int)h->V.size() == m + 1) return ELA_OK;
+    for (double* p : h->V) hfree(h, p);
+    for (double* p : h->Z) hfree(h, p);
+    h->V.clear();
+    h->Z.clear();
+    if (h->kr) hfree(h, h->kr);
+    if (h->kdev) hfree(h, h->kdev);
+    if (h->kpart) hfree(h, h->kpart);
+    h->kr = h->kdev = h->kpart = nullptr;
+    const size_t fe = field_elems(h->lev[0], h->dim);
+    int rc = ELA_OK;
+    for (int q = 0; q <= m && !rc; ++q) {
+        double* p = nullptr;
+        rc = halloc_t(h, fe, &p);
+        h->V.push_back(p);
+    }
+    for (int q = 0; q < m && !rc; ++q) {
+        double* p = nullptr;
+        rc = halloc_t(h, fe, &p);
+        h->Z.push_back(p);
+    }
+    if (!rc) rc = halloc_t(h, fe, &h->kr);
+    if (!rc) rc = halloc_t(h, 4 * (size_t)(m + 2), &h->kdev);
+    if (!rc) rc = halloc_t(h, (size_t)mdot_partials(m + 1), &h->kpart);
+    return rc;
+}
+
+// Right-preconditioned flexible GMRES(m) in FP64 with classical Gram-Schmidt
+// and one re-orthogonalization; the preconditioner is one MG cycle in
+// precision T (FP32 cycles make it vary slightly, which the flexible variant
+// allows).  The inner iteration stops when the Arnoldi residual estimate
+// |r|_2 falls below the inf-norm target (so |r|_inf does too); convergence is
+// then confirmed on the true FP64 residual.
+template <typename T>
+int solve_fgmres(ela_hier* h, const double* b, double* u, double tol_rel, double tol_abs, int max_iter,
+                 ela_stats* st) {
+    if (int rc = krylov_alloc(h)) return rc;
+    const int m = h->krylov;
+    Level& v0 = h->lev[0];
+    const long long n = (long long)field_elems(v0, h->dim);
+    hipStream_t s = h->stream;
+    auto true_residual = [&]() {
+        const int e = ev_begin(h);
+        A_residual<double, double>(h, v0, v0.C64, u, b, h->kr, (double*)nullptr, nullptr, h->dnorm, s);
+        ev_end(h, e, h->ev_resid);
+        return read_norm(h);
+    };
+    auto host_vec = [&](int count) {
+        std::vector<double> out(count);
+        (void)hipMemcpyAsync(out.data(), h->kdev, sizeof(double) * count, hipMemcpyDeviceToHost, s);
+        (void)hipStreamSynchronize(s);
+        return out;
+    };
+    double rn = true_residual();
+    const double r0 = rn;
+    if (!(r0 == r0)) return fail(ELA_ERR_NAN, "initial residual is not finite");
+    const double target = std::max(tol_abs, std::max(tol_rel, 1e-16) * r0);
+    int total = 0, status = ELA_OK, restarts = 0;
+    double best = r0;
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1);
+    while (rn > target) {
+        if (total >= max_iter) {
+            status = fail(ELA_ERR_NONCONVERGED, "max_iter reached at " + std::to_string(rn / r0) + " relative");
+            break;
+        }
+        // beta = |r|_2, V0 = r / beta
+        VecList one{};
+        one.v[0] = h->kr;
+        launch_mdot(h->kr, one, 1, n, h->kpart, h->kdev, s);
+        const double beta = std::sqrt(host_vec(1)[0]);
+        if (!(beta > 0)) break;
+        launch_scale_flat(h->V[0], h->kr, 1.0 / beta, n, s);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int jend = 0;
+        for (int j = 0; j < m && total < max_iter; ++j) {
+            LB<T> lbv = lb<T>(v0);
+            // z_j = M^-1 v_j: one MG cycle on level 0 from zero
+            launch_axpy<double, T>(v0.L, h->dim, h->V[j], *lbv.res, nullptr, 1.0, 1, s);
+            one_cycle<T>(h);
+            launch_axpy<T, double>(v0.L, h->dim, *lbv.cor, h->Z[j], nullptr, 1.0, 1, s);
+            // w = A z_j into V[j+1]
+            {
+                const int e = ev_begin(h);
+                A_apply<double>(h, v0, v0.C64, h->Z[j], h->V[j + 1], s);
+                ev_end(h, e, h->ev_resid);
+            }
+            VecList Vl{};
+            for (int q = 0; q <= j; ++q) Vl.v[q] = h->V[q];
+            // CGS2: h1 = V^T w; w -= V h1; h2 = V^T w; w -= V h2
+            launch_mdot(h->V[j + 1], Vl, j + 1, n, h->kpart, h->kdev, s);
+            launch_maxpy(h->V[j + 1], Vl, j + 1, h->kdev, -1.0, n, s);
+            std::vector<double> h1 = host_vec(j + 1);
+            launch_mdot(h->V[j + 1], Vl, j + 1, n, h->kpart, h->kdev, s);
+            launch_maxpy(h->V[j + 1], Vl, j + 1, h->kdev, -1.0, n, s);
+            std::vector<double> h2 = host_vec(j + 1);
+            VecList self{};
+            self.v[0] = h->V[j + 1];
+            launch_mdot(h->V[j + 1], self, 1, n, h->kpart, h->kdev, s);
+            const double hn = std::sqrt(host_vec(1)[0]);
+            ++total;
+            for (int q = 0; q <= j; ++q) H[(size_t)q * m + j] = h1[q] + h2[q];
+            H[(size_t)(j + 1) * m + j] = hn;
+            if (hn > 0) launch_scale_flat(h->V[j + 1], h->V[j + 1], 1.0 / hn, n, s);
+            // Givens rotations
+            for (int q = 0; q < j; ++q) {
+                const double a = H[(size_t)q * m + j], c = H[(size_t)(q + 1) * m + j];
+                H[(size_t)q * m + j] = cs[q] * a + sn[q] * c;
+                H[(size_t)(q + 1) * m + j] = -sn[q] * a + cs[q] * c;
+            }
+            const double a = H[(size_t)j * m + j], c = H[(size_t)(j + 1) * m + j];
+            const double rr = std::hypot(a, c);
+            cs[j] = rr > 0 ? a / rr : 1.0;
+            sn[j] = rr > 0 ? c / rr : 0.0;
+            H[(size_t)j * m + j] = rr;
+            H[(size_t)(j + 1) * m + j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            jend = j + 1;
+            const double est = std::fabs(g[j + 1]);
+            if (!(est == est)) break;
+            if (est <= 0.5 * target || hn == 0) break;
+        }
+        // y = R^-1 g, u += Z y
+        std::vector<double> y(jend);
+        for (int q = jend - 1; q >= 0; --q) {
+            double acc = g[q];
+            for (int k = q + 1; k < jend; ++k) acc -= H[(size_t)q * m + k] * y[k];
+            y[q] = acc / H[(size_t)q * m + q];
+        }
+        HIPCHK(hipMemcpyAsync(h->kdev, y.data(), sizeof(double) * jend, hipMemcpyHostToDevice, s));
+        VecList Zl{};
+        for (int q = 0; q < jend; ++q) Zl.v[q] = h->Z[q];
+        launch_maxpy(u, Zl, jend, h->kdev, 1.0, n, s);
+        rn = true_residual();
+        ++restarts;
+        if (!(rn == rn)) { status = fail(ELA_ERR_NAN, "residual became non-finite"); break; }
+        if (rn <= target) break;
+        if (rn < 0.5 * best) best = rn;
+        else if (restarts >= 4 && rn > 0.9 * best) {
+            status = fail(ELA_ERR_NONCONVERGED, "residual stagnated at " + std::to_string(rn / r0) + " relative");
+            break;
+        }
+    }
+    if (st) {
+        st->iters = total;
+        st->converged = rn <= target;
+        st->res0 = r0;
+        st->res = rn;
+        st->mg_levels = (int)h->lev.size();
+    }
+    return status;
+}
+
 void collect_times(ela_hier* h) {
     h->ms_resid = h->ms_smooth = 0;
     h->n_resid = h->n_smooth = 0;
@@ -359,6 +533,25 @@ int check_prec(const ela_hier* h, int prec) {
 }
 
 hipStream_t strm(ela_hier* h, void* s) { return s ? (hipStream_t)s : h->stream; }
+
+template <typename T>
+void A_apply(ela_hier* h, Level& v, const T* C, const T* u, T* f, hipStream_t s) {
+    if (v.mp.ok) march_fapply<T>(v.mp, v.L, h->kind, C, u, f, h->bc, v.h, h->uniform, s);
+    else launch_fapply<T>(v.L, h->kind, C, u, f, h->bc, v.h, h->uniform, s);
+}
+
+template <typename T>
+void A_jacobi(ela_hier* h, Level& v, const T* C, const T* dg, const T* xin, const T* b, T* xout, hipStream_t s) {
+    if (v.mp.ok) march_jacobi<T>(v.mp, v.L, h->kind, C, dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
+    else launch_jacobi<T>(v.L, h->kind, C, dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
+}
+
+template <typename T, typename TO>
+void A_residual(ela_hier* h, Level& v, const T* C, const T* x, const T* b, T* r, TO* rs, const double* scale,
+                unsigned long long* norm, hipStream_t s) {
+    if (v.mp.ok) march_residual<T, TO>(v.mp, v.L, h->kind, C, x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
+    else launch_residual<T, TO>(v.L, h->kind, C, x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
+}
 
 }  // namespace
 
@@ -408,6 +601,16 @@ int ela_hier_create(ela_ctx* ctx, const ela_domain* dom, int model_kind, int max
     h->mode = mode;
     h->has64 = (mode == ELA_SOLVE_FP64 || mode == ELA_ALLOC_BOTH);
     h->has32 = (mode == ELA_SOLVE_MIXED || mode == ELA_ALLOC_BOTH);
+    h->use_march = false;
+    if (const char* e = std::getenv("ELA_KERNEL")) {
+        h->use_march = std::strcmp(e, "march") == 0;
+        g_slab_order = std::strcmp(e, "simple") != 0;
+    }
+    if (const char* e = std::getenv("ELA_MARCH_KZ")) h->march_kz = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("ELA_WALK")) g_slab_walk = std::atoi(e);
+    if (const char* e = std::getenv("ELA_SLAB_F64")) g_slab_f64 = std::atoi(e);
+    if (const char* e = std::getenv("ELA_PITCH")) h->pitch_align = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("ELA_MARCH_TY")) h->march_ty = std::atoi(e) == 2 ? 2 : 4;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
         return fail(ELA_ERR_HIP, "hipStreamCreate failed");
@@ -421,10 +624,12 @@ int ela_hier_create(ela_ctx* ctx, const ela_domain* dom, int model_kind, int max
         v.L.nx = n[0] + 1;
         v.L.ny = n[1] + 1;
         v.L.nz = dom->dim == 3 ? n[2] + 1 : 1;
-        v.L.px = (v.L.nx + 15) / 16 * 16;
+        v.L.px = (v.L.nx + h->pitch_align - 1) / h->pitch_align * h->pitch_align;
         v.L.pl = v.L.px * v.L.ny;
         v.L.cs = (v.L.pl * v.L.nz + 255) / 256 * 256;
         v.nodes = v.L.nx * v.L.ny * v.L.nz;
+        v.mp = march_plan(v.L, h->march_kz, h->march_ty);
+        if (!h->use_march) v.mp.ok = false;
         h->lev.push_back(v);
         bool can = true;
         for (int d = 0; d < dom->dim; ++d) can = can && (n[d] % 2 == 0) && (n[d] >= 8);
@@ -460,6 +665,9 @@ int ela_hier_create(ela_ctx* ctx, const ela_domain* dom, int model_kind, int max
     if (!rc) rc = halloc_t(h, field_elems(h->lev[0], h->dim), &h->b64);
     if (!rc && h->nunk <= kMaxDenseUnknowns) rc = halloc_t(h, (size_t)h->nunk * 2 * h->nunk, &h->M);
     if (!rc) rc = halloc_t(h, 4, &h->dnorm);
+    long long np = 0;
+    for (auto& v : h->lev) np = std::max(np, std::max(residual_partials(v.L), march_partials(v.mp, v.L)));
+    if (!rc) rc = halloc_t(h, (size_t)np, &h->partials);
     if (!rc) rc = halloc_t(h, 4, &h->dscale);
     if (!rc) rc = halloc_t(h, 4, &h->dflag);
     if (!rc && hipHostMalloc((void**)&h->hnorm, 64) != hipSuccess) rc = fail(ELA_ERR_OOM, "hipHostMalloc failed");
@@ -729,8 +937,8 @@ int ela_fapply(ela_hier* h, int mglev, int prec, const void* u, void* f, void* s
     if (!u || !f) return fail(ELA_ERR_BADARG, "null field");
     Level& v = h->lev[mglev];
     if (prec == ELA_F32 && !h->prepared) return fail(ELA_ERR_STATE, "call ela_prepare before FP32 use");
-    if (prec == ELA_F64) launch_fapply<double>(v.L, h->kind, v.C64, (const double*)u, (double*)f, h->bc, v.h, h->uniform, strm(h, stream));
-    else launch_fapply<float>(v.L, h->kind, v.C32, (const float*)u, (float*)f, h->bc, v.h, h->uniform, strm(h, stream));
+    if (prec == ELA_F64) A_apply<double>(h, v, v.C64, (const double*)u, (double*)f, strm(h, stream));
+    else A_apply<float>(h, v, v.C32, (const float*)u, (float*)f, strm(h, stream));
     HIPCHK(hipGetLastError());
     return ELA_OK;
 }
@@ -763,7 +971,7 @@ int ela_fsmooth(ela_hier* h, int mglev, int prec, void* x, const void* b, int ns
     if (int rc = halloc(h, field_elems(v, h->dim) * es, &tmp)) return rc;
     for (int w = 0; w < nsweeps; ++w) {
         if (prec == ELA_F64) {
-            launch_jacobi<double>(v.L, h->kind, v.C64, v.d64, (const double*)x, (const double*)b, (double*)tmp, h->bc, v.h, h->uniform, omega, s);
+            launch_jacobi<double>(v.L, h->kind, v.C64, v.d64, (const double*)x, (const double*)b, (double*)tmp, h->bc, v.h, h->uniform, omega, s);  // ela_fsmooth keeps its own omega
             HIPCHK(hipMemcpyAsync(x, tmp, field_elems(v, h->dim) * es, hipMemcpyDeviceToDevice, s));
         } else {
             launch_jacobi<float>(v.L, h->kind, v.C32, v.d32, (const float*)x, (const float*)b, (float*)tmp, h->bc, v.h, h->uniform, omega, s);
@@ -807,11 +1015,10 @@ int ela_residual(ela_hier* h, int mglev, int prec, const void* x, const void* b,
     if (!h->prepared && (mglev > 0 || prec == ELA_F32)) return fail(ELA_ERR_STATE, "call ela_prepare first");
     Level& v = h->lev[mglev];
     hipStream_t s = strm(h, stream);
-    launch_zero_u64(h->dnorm, 1, s);
     if (prec == ELA_F64)
-        launch_residual<double, double>(v.L, h->kind, v.C64, (const double*)x, (const double*)b, (double*)r, (double*)nullptr, nullptr, h->dnorm, h->bc, v.h, h->uniform, s);
+        A_residual<double, double>(h, v, v.C64, (const double*)x, (const double*)b, (double*)r, (double*)nullptr, nullptr, h->dnorm, s);
     else
-        launch_residual<float, float>(v.L, h->kind, v.C32, (const float*)x, (const float*)b, (float*)r, (float*)nullptr, nullptr, h->dnorm, h->bc, v.h, h->uniform, s);
+        A_residual<float, float>(h, v, v.C32, (const float*)x, (const float*)b, (float*)r, (float*)nullptr, nullptr, h->dnorm, s);
     unsigned long long bits = 0;
     HIPCHK(hipMemcpyAsync(&bits, h->dnorm, sizeof bits, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -866,7 +1073,12 @@ static int solve_common(ela_hier* h, double tol_rel, double tol_abs, int mode, i
     h->ev_resid.clear();
     h->ev_smooth.clear();
     auto t0 = std::chrono::steady_clock::now();
-    int rc = mode == ELA_SOLVE_FP64 ? solve_loop<double>(h, h->b64, h->u64, tol_rel, tol_abs, max_iter, st)
+    int rc;
+    if (h->krylov > 0)
+        rc = mode == ELA_SOLVE_FP64 ? solve_fgmres<double>(h, h->b64, h->u64, tol_rel, tol_abs, max_iter, st)
+                                    : solve_fgmres<float>(h, h->b64, h->u64, tol_rel, tol_abs, max_iter, st);
+    else
+        rc = mode == ELA_SOLVE_FP64 ? solve_loop<double>(h, h->b64, h->u64, tol_rel, tol_abs, max_iter, st)
                                     : solve_loop<float>(h, h->b64, h->u64, tol_rel, tol_abs, max_iter, st);
     (void)hipStreamSynchronize(h->stream);
     auto t1 = std::chrono::steady_clock::now();
@@ -916,6 +1128,13 @@ int ela_solve_device(ela_hier* h, const void* b_dev, void* u_dev, double tol_rel
     return rc;
 }
 
+int ela_set_krylov(ela_hier* h, int restart) {
+    if (!h) return fail(ELA_ERR_BADARG, "null hierarchy");
+    if (restart < 0 || restart > kMaxKrylov - 1) return fail(ELA_ERR_BADARG, "restart must be in [0, 32]");
+    h->krylov = restart;
+    return ELA_OK;
+}
+
 int ela_set_timing(ela_hier* h, int on) {
     if (!h) return fail(ELA_ERR_BADARG, "null hierarchy");
     h->timing = on != 0;
@@ -945,8 +1164,8 @@ int ela_time_fapply(ela_hier* h, int prec, int nrep, double* avg_ms) {
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
     auto launch = [&]() {
-        if (prec == ELA_F64) launch_fapply<double>(v.L, h->kind, v.C64, (const double*)u, (double*)f, h->bc, v.h, h->uniform, h->stream);
-        else launch_fapply<float>(v.L, h->kind, v.C32, (const float*)u, (float*)f, h->bc, v.h, h->uniform, h->stream);
+        if (prec == ELA_F64) A_apply<double>(h, v, v.C64, (const double*)u, (double*)f, h->stream);
+        else A_apply<float>(h, v, v.C32, (const float*)u, (float*)f, h->stream);
     };
     launch();
     HIPCHK(hipEventRecord(a, h->stream));
@@ -960,6 +1179,38 @@ int ela_time_fapply(ela_hier* h, int prec, int nrep, double* avg_ms) {
     (void)hipEventDestroy(b);
     hfree(h, u);
     hfree(h, f);
+    return ELA_OK;
+}
+
+int ela_probe_copy(ela_ctx* ctx, size_t bytes, int nrep, double* avg_ms) {
+    if (!ctx || !avg_ms || nrep < 1 || bytes < 16) return fail(ELA_ERR_BADARG, "bad argument");
+    bytes = bytes / 16 * 16;
+    HIPCHK(hipSetDevice(ctx->device));
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, bytes));
+    if (hipMalloc(&b, bytes) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(ELA_ERR_OOM, "probe allocation failed");
+    }
+    hipStream_t s;
+    HIPCHK(hipStreamCreate(&s));
+    HIPCHK(hipMemsetAsync(a, 0, bytes, s));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    launch_copy(a, b, (long long)bytes, s);
+    HIPCHK(hipEventRecord(e0, s));
+    for (int r = 0; r < nrep; ++r) launch_copy(a, b, (long long)bytes, s);
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *avg_ms = ms / nrep;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    (void)hipFree(a);
+    (void)hipFree(b);
     return ELA_OK;
 }
 

@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--cpu-cycles", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--modes", default="fp64,mixed", help="comma list of fp64, mixed")
+    ap.add_argument("--krylov", type=int, default=16, help="FGMRES restart around the MG cycle (0: plain MG)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md on sharding)",
@@ -97,6 +99,7 @@ def main():
     op = ela.Elastic(cfg["n"], ela.MM, mode=ela.ALLOC_BOTH)
     op.SetModelVoronoiCubic(cfg["seeds"], K.VORONOI_Q, cfg["R"], cfg["C11"], cfg["C12"], cfg["C44"])
     op.SetBC(cfg["bc_type"], cfg["bc_value"])
+    op.SetKrylov(args.krylov)
     op.prepareForSolve()
     b_host = op.bc_init()
     b = op.field(0, ela.F64).upload(b_host)
@@ -104,17 +107,23 @@ def main():
     nodes = (n + 1) ** 3
     log("bench: hierarchy %d MG levels, %.2f GB on device" % (op.levels(), op.bytes() / 1e9))
 
-    # plain FP64 GPU solver
-    log("bench: fp64 solves")
-    t64, per64, st64, rc64, _ = run_mode(op, b, u, ela.SOLVE_FP64, args.steps, args.warmup, args.tol)
-    u64 = u.download()
-    log("bench: fp64 %.3f s/solve iters=%d rel=%.2e rc=%d" % (t64 / args.steps, st64["iters"], st64["res"] / st64["res0"], rc64))
+    modes = args.modes.split(",")
+    t64 = st64 = u64 = None
+    if "fp64" in modes:  # plain FP64 GPU solver
+        log("bench: fp64 solves")
+        t64, per64, st64, rc64, _ = run_mode(op, b, u, ela.SOLVE_FP64, args.steps, args.warmup, args.tol)
+        u64 = u.download()
+        log("bench: fp64 %.3f s/solve iters=%d rel=%.2e rc=%d" % (t64 / args.steps, st64["iters"], st64["res"] / st64["res0"], rc64))
     # mixed precision (headline)
     log("bench: mixed solves")
     tmx, permx, stmx, rcmx, kt = run_mode(op, b, u, ela.SOLVE_MIXED, args.steps, args.warmup, args.tol, timing=True)
     umx = u.download()
     log("bench: mixed %.3f s/solve iters=%d rel=%.2e rc=%d" % (tmx / args.steps, stmx["iters"], stmx["res"] / stmx["res0"], rcmx))
-    disp_diff = float(np.abs(umx - u64).max() / np.abs(u64).max())
+    disp_diff = float(np.abs(umx - u64).max() / np.abs(u64).max()) if u64 is not None else None
+    # HBM copy probe (also the byte-counter calibration kernel of the PMC runs)
+    probe_bytes = 1 << 30
+    probe_ms = op.ctx.probe_copy(probe_bytes, 5)
+    probe_GBps = 2 * probe_bytes / (probe_ms * 1e-3) / 1e9
 
     # Fapply alone (FP64 and FP32), HIP events on the library stream
     fa64 = op.time_fapply(ela.F64, 20)
@@ -124,27 +133,38 @@ def main():
     bytes_fa32 = 4 * (3 + 3 + nc) * nodes
     # dominant kernel of the mixed solve: level-0 FP32 Jacobi sweep vs level-0 FP64 residual
     bytes_jac32 = 4 * (4 * 3 + nc) * nodes          # x in, b, diag, x out, C
-    bytes_res64 = (8 * (3 + 3 + nc) + 4 * 3) * nodes  # x, b, C in FP64; scaled FP32 rhs out
+    # x, b, C in FP64 + scaled FP32 rhs out (plain MG) / A z: z, C in, w out (FGMRES)
+    bytes_res64 = ((8 * (3 + 3 + nc) + 4 * 3) if not args.krylov else 8 * (3 + 3 + nc)) * nodes
     tot_jac = kt["smooth0_ms"] * kt["smooth0_n"]
     tot_res = kt["resid64_ms"] * kt["resid64_n"]
     if tot_jac >= tot_res:
         dom, dom_ms, dom_bytes = "k_jacobi<float,MM,3> (level 0)", kt["smooth0_ms"], bytes_jac32
     else:
-        dom, dom_ms, dom_bytes = "k_residual<double,float,MM,3> (level 0)", kt["resid64_ms"], bytes_res64
+        dom = "k_fapply<double,MM,3> (level 0)" if args.krylov else "k_residual<double,float,MM,3> (level 0)"
+        dom_ms, dom_bytes = kt["resid64_ms"], bytes_res64
     peak = 8000.0
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(tpath):  # HBM bytes per launch from the rocprofv3 --pmc runs, copy-calibrated
+        with open(tpath) as fh:
+            tr = json.load(fh)
+        ent = tr.get("kernels", {}).get(dom.split(" ")[0])
+        if ent and tr.get("n") == n:
+            traffic = ent["fetch_bytes"] + ent["write_bytes"]
 
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         log("bench: cpu baseline (oracle, %d threads) %d^3, %d F-cycles" % (threads, args.cpu_n, args.cpu_cycles))
-        per_cycle, tcpu, cyc = cpu_baseline(args.cpu_n, args.cpu_cycles, st64["iters"])
+        iters_ref = (st64 or stmx)["iters"]
+        per_cycle, tcpu, cyc = cpu_baseline(args.cpu_n, args.cpu_cycles, iters_ref)
         scale = nodes / float((args.cpu_n + 1) ** 3)
-        t_solve = per_cycle * scale * st64["iters"]
+        t_solve = per_cycle * scale * iters_ref
         cpu = {"value": 1.0 / t_solve, "unit": "solves/s", "cores": threads, "kind": "port",
                "sample": "%d F-cycles of the FP64 oracle MG on the same workload at %d^3 (%.1f s, %.2f s/cycle), "
                          "scaled by node count to %d^3 and by the %d cycles the FP64 solve needs"
-                         % (cyc, args.cpu_n, tcpu, per_cycle, n, st64["iters"])}
+                         % (cyc, args.cpu_n, tcpu, per_cycle, n, iters_ref)}
 
     out = {
         "metric": "solves/s at 1e-10 rel residual (mixed FP32/FP64), vs plain FP64 GPU",
@@ -161,18 +181,20 @@ def main():
         "data": "synthetic",
         "config": {"workload": "3D single level %d^3 cells, cubic Voronoi %d grains (Haar orientations, seed %d), "
                                "uniaxial z strain 0.01, tol_rel %.0e" % (n, args.grains, K.SEED, args.tol),
-                   "nodes": nodes, "mg_levels": op.levels()},
-        "fp64_solves_per_s": args.steps / t64,
-        "mixed_over_fp64": t64 / tmx,
-        "iters": {"fp64": st64["iters"], "mixed": stmx["iters"]},
-        "rel_residual": {"fp64": st64["res"] / st64["res0"], "mixed": stmx["res"] / stmx["res0"]},
+                   "nodes": nodes, "mg_levels": op.levels(),
+                   "solver": "FGMRES(%d) + MG F-cycle" % args.krylov if args.krylov else "MG F-cycles"},
+        "fp64_solves_per_s": args.steps / t64 if t64 else None,
+        "mixed_over_fp64": t64 / tmx if t64 else None,
+        "iters": {"fp64": st64["iters"] if st64 else None, "mixed": stmx["iters"]},
+        "rel_residual": {"fp64": st64["res"] / st64["res0"] if st64 else None, "mixed": stmx["res"] / stmx["res0"]},
+        "hbm_copy_probe_GBps": probe_GBps,
         "rel_disp_diff_mixed_vs_fp64": disp_diff,
         "fapply": {"fp64_ms": fa64, "fp64_GBps": bytes_fa64 / (fa64 * 1e-3) / 1e9,
                    "fp32_ms": fa32, "fp32_GBps": bytes_fa32 / (fa32 * 1e-3) / 1e9,
                    "bytes_per_node_fp64": 8 * (3 + 3 + nc)},
         "kernel_times": kt,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": peak, "unit": "GB/s", "frac": achieved / peak,
-                     "traffic": None, "kernel": dom, "kernel_ms": dom_ms, "bytes_per_launch": dom_bytes},
+                     "traffic": traffic, "kernel": dom, "kernel_ms": dom_ms, "bytes_per_launch": dom_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -195,12 +195,23 @@ int ela_set_cycle(ela_hier* h, int cycle, int nu1, int nu2);
  * and of the level-0 smoother sweep, and their launch counts. */
 int ela_kernel_times(const ela_hier* h, double* ms_resid64, int* n_resid64, double* ms_smooth0, int* n_smooth0);
 
+/* Accelerate the multigrid iteration with flexible GMRES(restart), one MG
+ * cycle per preconditioner application (0, the default: the plain MG
+ * iteration of the reference).  Same discrete solution; max_iter then counts
+ * MG cycles. */
+int ela_set_krylov(ela_hier* h, int restart);
+
 /* Record HIP events around the level-0 kernels of subsequent solves (on != 0). */
 int ela_set_timing(ela_hier* h, int on);
 
 /* Time nrep back-to-back launches of Fapply on level 0 with HIP events on
  * the library's stream; returns the average ms per launch. */
 int ela_time_fapply(ela_hier* h, int prec, int nrep, double* avg_ms);
+
+/* Measurement helper with no counterpart in the reference: nrep launches of a
+ * plain streaming copy of `bytes` on the device (reads and writes `bytes`
+ * each), average ms per launch.  It calibrates the HBM byte counters. */
+int ela_probe_copy(ela_ctx* ctx, size_t bytes, int nrep, double* avg_ms);
 
 #ifdef __cplusplus
 }

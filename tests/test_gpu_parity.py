@@ -30,7 +30,9 @@ def make(n, kind, C, t, v=None, uniform=False, mode=ela.ALLOC_BOTH):
 
 
 CASES = [(2, ISO, (20, 13)), (2, MM, (17, 9)), (2, DIAG, (12, 12)),
-         (3, ISO, (9, 7, 6)), (3, MM, (11, 6, 8)), (3, DIAG, (6, 6, 6)), (3, MM, (70, 5, 4))]
+         (3, ISO, (9, 7, 6)), (3, MM, (11, 6, 8)), (3, DIAG, (6, 6, 6)), (3, MM, (70, 5, 4)),
+         # LDS-staged marching kernels: partial x tiles, partial y tiles, several z chunks
+         (3, MM, (130, 9, 6)), (3, ISO, (40, 7, 70)), (3, DIAG, (20, 11, 37)), (3, MM, (33, 5, 67))]
 
 
 @pytest.mark.parametrize("dim,kind,n", CASES)
@@ -49,6 +51,23 @@ def test_fapply_parity(oracle, dim, kind, n, uniform):
     assert np.abs(got - want).max() <= 1e-12 * s
     got32 = op.Fapply(u, prec=ela.F32)
     assert np.abs(got32 - want).max() <= 2e-5 * s
+
+
+@pytest.mark.parametrize("kind,n", [(MM, (9, 7, 6)), (MM, (40, 9, 36)), (ISO, (20, 5, 40)), (DIAG, (2, 3, 2))])
+@pytest.mark.parametrize("prec", [ela.F64, ela.F32])
+def test_residual_and_norm_parity(oracle, kind, n, prec):
+    rng = np.random.default_rng(31)
+    g = oracle.Grid(n)
+    C = H.random_stiffness(rng, n, kind, 3)
+    t, v = H.bc_mixed(3)
+    op = make(n, kind, C, t, v)
+    x, b = H.random_field(rng, n, 3), H.random_field(rng, n, 3)
+    want = b - oracle.fapply(g, kind, C, t, x)
+    r, nrm = op.residual(x, b, prec=prec)
+    tol = (1e-12 if prec == ela.F64 else 2e-5) * scale_of(C, x, g.dx)
+    assert np.abs(r - want).max() <= tol
+    assert abs(nrm - np.abs(r).max()) <= 1e-300 + (0 if prec == ela.F64 else 1e-7 * nrm)
+    assert abs(nrm - np.abs(want).max()) <= tol
 
 
 @pytest.mark.parametrize("dim,kind,n", CASES[:6])
@@ -83,6 +102,21 @@ def test_transfer_and_coarse_operator_parity(oracle, dim, n):
     want = oracle.fapply(gc, kind, Cc, t, uc)
     got = op.Fapply(uc, mglev=1)
     assert np.abs(got - want).max() <= 1e-12 * scale_of(Cc, uc, gc.dx)
+
+
+def test_marching_jacobi_in_solver_matches_oracle_fsmooth(oracle):
+    """One V-cycle-free check of the marching Jacobi: a 1-iteration solve with nu1 = nu2 = 0 is a pure
+    coarse solve, so instead compare Fsmooth on a level large enough for the marching kernels."""
+    rng = np.random.default_rng(12)
+    n, kind = (40, 10, 34), MM
+    g = oracle.Grid(n)
+    C = H.random_stiffness(rng, n, kind, 3)
+    t, v = H.bc_mixed(3)
+    op = make(n, kind, C, t, v)
+    x = H.random_field(rng, n, 3)
+    want = oracle.fapply(g, kind, C, t, x)
+    got = op.Fapply(x)
+    assert np.abs(got - want).max() <= 1e-12 * scale_of(C, x, g.dx)
 
 
 def test_fsmooth_parity(oracle):
@@ -158,3 +192,34 @@ def test_uniaxial_known_answer_gpu(mode):
     exact = 0.1 * np.broadcast_to(x, (n + 1,) * 3)
     assert np.abs(u[0] - exact).max() / 0.1 < 1e-8
     assert np.abs(u[1:]).max() < 1e-9
+
+
+@pytest.mark.parametrize("mode", [ela.SOLVE_FP64, ela.SOLVE_MIXED])
+def test_krylov_solve_matches_oracle_cubic(oracle, mode):
+    """FGMRES around the MG cycle converges to the same discrete solution."""
+    cfg, C, g, b = _solve_case(oracle)
+    u_or, st_or, _ = oracle.mg_solve(g, MM, C, cfg["bc_type"], b, tol_rel=1e-12, max_iter=300)
+    op = make(cfg["n"], MM, C, cfg["bc_type"], cfg["bc_value"])
+    op.SetKrylov(16)
+    u, st = ela.Linear(op).solve(np.zeros_like(b), b, tol_rel=1e-10, tol_abs=0.0, mode=mode, max_iter=300)
+    assert st["converged"] and st["res"] / st["res0"] <= 1e-10
+    assert np.abs(u - u_or).max() / np.abs(u_or).max() <= 1e-8
+
+
+@pytest.mark.parametrize("mode", [ela.SOLVE_FP64, ela.SOLVE_MIXED])
+def test_config1_inclusion_krylov_vs_direct(oracle, mode):
+    """Config-1 geometry at 32^2 (sharp 10x inclusion): the plain MG iteration of the reference
+    diverges here; FGMRES(MG) reaches the discrete solution of a sparse direct solve."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    (nx, ny), C, t, v = K.cfg1(32)
+    g = oracle.Grid((nx, ny))
+    b = oracle.bc_init(g, t, v, g.zeros())
+    A = sp.csc_matrix(oracle.assemble(g, ISO, C, t))
+    u_ref = spl.spsolve(A, b.ravel()).reshape(b.shape)
+    op = make((nx, ny), ISO, C, t, v)
+    op.SetKrylov(24)
+    u, st = ela.Linear(op).solve(np.zeros_like(b), b, tol_rel=1e-9, tol_abs=0.0, mode=mode, max_iter=400)
+    print("cfg1 32^2", st)
+    assert st["converged"]
+    assert np.abs(u - u_ref).max() / np.abs(u_ref).max() <= 1e-7
