@@ -38,7 +38,20 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    build_cpp_tests()
     return LIB
+
+
+def build_cpp_tests():
+    """The C++ interface test (tests/cpp) against include/alamo_elastic.hpp and libela.so."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "test_operator_elastic.cpp")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(root, "tests", "cpp", "test_operator_elastic")
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-I", os.path.join(root, "include"), src, "-o", out,
+                           "-L", HERE, "-lela", "-Wl,-rpath," + HERE])
+    return out
 
 
 if __name__ == "__main__":

@@ -348,79 +348,79 @@ __global__ void k_identity_right(double* __restrict__ M, int n) {
     if (r < n) M[(long long)r * (2 * n) + n + r] = 1.0;
 }
 
-// Gauss-Jordan inversion with partial pivoting, one workgroup; M is n x 2n
-// row-major ([A | I] on entry, [I | A^-1] on exit).  flag != 0 on a zero pivot.
-__global__ __launch_bounds__(1024) void k_gauss_jordan(double* __restrict__ M, int n, int* flag) {
-    __shared__ double sval[1024];
-    __shared__ int sidx[1024];
-    __shared__ int piv;
-    const int tid = threadIdx.x, nt = blockDim.x, w = 2 * n;
-    for (int c = 0; c < n; ++c) {
-        double best = -1.0;
-        int bi = c;
-        for (int r = c + tid; r < n; r += nt) {
-            const double a = fabs(M[(long long)r * w + c]);
-            if (a > best) { best = a; bi = r; }
-        }
-        sval[tid] = best;
-        sidx[tid] = bi;
-        __syncthreads();
-        for (int s = nt / 2; s > 0; s >>= 1) {
-            if (tid < s) {
-                if (sval[tid + s] > sval[tid] || (sval[tid + s] == sval[tid] && sidx[tid + s] < sidx[tid])) {
-                    sval[tid] = sval[tid + s];
-                    sidx[tid] = sidx[tid + s];
-                }
-            }
-            __syncthreads();
-        }
-        if (tid == 0) {
-            piv = sidx[0];
-            if (!(sval[0] > 0.0)) *flag = 1;
-        }
-        __syncthreads();
-        const int p = piv;
-        if (p != c)
-            for (int q = tid; q < w; q += nt) {
-                const double t = M[(long long)c * w + q];
-                M[(long long)c * w + q] = M[(long long)p * w + q];
-                M[(long long)p * w + q] = t;
-            }
-        __syncthreads();
-        const double d = M[(long long)c * w + c];
-        __syncthreads();
-        for (int q = tid; q < w; q += nt) M[(long long)c * w + q] /= d;
-        __syncthreads();
-        for (long long e = tid; e < (long long)n * w; e += nt) {
-            const int r = (int)(e / w), q = (int)(e % w);
-            if (r == c || q == c) continue;
-            const double m = M[(long long)r * w + c];
-            if (m != 0.0) M[e] -= m * M[(long long)c * w + q];
-        }
-        __syncthreads();
-        for (int r = tid; r < n; r += nt)
-            if (r != c) M[(long long)r * w + c] = 0.0;
-        __syncthreads();
-    }
-}
-
 // x = A^-1 b on the coarsest level (b, x in the level layout, precision T).
+// Every block gathers b into LDS (dense unknown order); each wavefront then
+// computes whole rows of A^-1 b with coalesced row reads and a shuffle sum.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bottom(Lay L, int nodes, int dim, const double* __restrict__ M,
                                                 const T* __restrict__ b, T* __restrict__ x) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double sb[3 * 9 * 9 * 9];
     const int nunk = nodes * dim;
-    if (r >= nunk) return;
-    const double* row = M + (long long)r * (2 * nunk) + nunk;
-    double acc = 0.0;
-    for (int p = 0; p < dim; ++p)
-        for (int t = 0; t < nodes; ++t) {
+    for (int q = threadIdx.x; q < nunk; q += blockDim.x) {
+        const int p = q / nodes, t = q % nodes;
+        const int i = t % L.nx, j = (t / L.nx) % L.ny, k = t / (L.nx * L.ny);
+        sb[q] = (double)b[p * L.cs + i + L.px * j + L.pl * k];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (int r = wave; r < nunk; r += nwaves) {
+        const double* row = M + (long long)r * (2 * nunk) + nunk;
+        double acc = 0.0;
+        for (int q = lane; q < nunk; q += 64) acc += row[q] * sb[q];
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (lane == 0) {
+            const int p = r / nodes, t = r % nodes;
             const int i = t % L.nx, j = (t / L.nx) % L.ny, k = t / (L.nx * L.ny);
-            acc += row[p * nodes + t] * (double)b[p * L.cs + i + L.px * j + L.pl * k];
+            x[p * L.cs + i + L.px * j + L.pl * k] = (T)acc;
         }
-    const int p = r / nodes, t = r % nodes;
-    const int i = t % L.nx, j = (t / L.nx) % L.ny, k = t / (L.nx * L.ny);
-    x[p * L.cs + i + L.px * j + L.pl * k] = (T)acc;
+    }
+}
+
+// Gauss-Jordan step for pivot column c on M = [A | A^-1-in-progress] (n x 2n):
+// one block finds the pivot row (largest |M[r][c]|, r >= c, lowest index on
+// ties), swaps it into row c and scales row c; then k_gj_eliminate clears
+// column c in every other row.
+__global__ __launch_bounds__(1024) void k_gj_pivot(double* __restrict__ M, int n, int c, int* flag) {
+    __shared__ double sval[1024];
+    __shared__ int sidx[1024];
+    const int tid = threadIdx.x, nt = blockDim.x, w = 2 * n;
+    double best = -1.0;
+    int bi = c;
+    for (int r = c + tid; r < n; r += nt) {
+        const double a = fabs(M[(long long)r * w + c]);
+        if (a > best) { best = a; bi = r; }
+    }
+    sval[tid] = best;
+    sidx[tid] = bi;
+    __syncthreads();
+    for (int st = nt / 2; st > 0; st >>= 1) {
+        if (tid < st && (sval[tid + st] > sval[tid] || (sval[tid + st] == sval[tid] && sidx[tid + st] < sidx[tid]))) {
+            sval[tid] = sval[tid + st];
+            sidx[tid] = sidx[tid + st];
+        }
+        __syncthreads();
+    }
+    const int p = sidx[0];
+    if (tid == 0 && !(sval[0] > 0.0)) *flag = 1;
+    const double d = M[(long long)p * w + c];
+    __syncthreads();
+    for (int q = tid; q < w; q += nt) {
+        const double vp = M[(long long)p * w + q];
+        if (p != c) M[(long long)p * w + q] = M[(long long)c * w + q];
+        M[(long long)c * w + q] = vp / d;
+    }
+}
+
+// Row r != c: M[r][q] -= M[r][c] * M[c][q].  One block per row; the
+// multiplier is read before any thread of the block writes column c.
+__global__ __launch_bounds__(256) void k_gj_eliminate(double* __restrict__ M, int n, int c) {
+    const int r = blockIdx.x, w = 2 * n;
+    if (r == c) return;
+    const double m = M[(long long)r * w + c];
+    __syncthreads();
+    if (m == 0.0) return;
+    for (int q = threadIdx.x; q < w; q += blockDim.x) M[(long long)r * w + q] -= m * M[(long long)c * w + q];
 }
 
 // ---------------------------------------------------------------- stiffness evaluation
@@ -728,13 +728,17 @@ void launch_identity_right(double* M, int n, hipStream_t s) {
 }
 
 void launch_gauss_jordan(double* M, int n, int* flag, hipStream_t s) {
-    hipLaunchKernelGGL(k_gauss_jordan, dim3(1), dim3(1024), 0, s, M, n, flag);
+    for (int c = 0; c < n; ++c) {
+        hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, s, M, n, c, flag);
+        hipLaunchKernelGGL(k_gj_eliminate, dim3(n), dim3(256), 0, s, M, n, c);
+    }
 }
 
 template <typename T>
 void launch_bottom(const Lay& L, int nodes, int dim, const double* M, const T* b, T* x, hipStream_t s) {
     const int nunk = nodes * dim;
-    hipLaunchKernelGGL((k_bottom<T>), dim3((nunk + 255) / 256), dim3(256), 0, s, L, nodes, dim, M, b, x);
+    const int blocks = std::min(64, (nunk + 3) / 4);  // 4 wavefronts per block
+    hipLaunchKernelGGL((k_bottom<T>), dim3(blocks), dim3(256), 0, s, L, nodes, dim, M, b, x);
 }
 
 void launch_cubic_table(int ng, const double* R, double C11, double C12, double C44, double* table, hipStream_t s) {

@@ -437,15 +437,23 @@ int solve_fgmres(ela_hier* h, const double* b, double* u, double tol_rel, double
             }
             VecList Vl{};
             for (int q = 0; q <= j; ++q) Vl.v[q] = h->V[q];
-            // CGS2: h1 = V^T w; w -= V h1; h2 = V^T w; w -= V h2
+            // classical Gram-Schmidt, h1 = V^T w, w -= V h1, with a second pass only
+            // when w lost most of its norm (|w'| < |w| / sqrt(2): Daniel et al.)
+            VecList self{};
+            self.v[0] = h->V[j + 1];
+            launch_mdot(h->V[j + 1], self, 1, n, h->kpart, h->kdev, s);
+            const double wn0 = std::sqrt(host_vec(1)[0]);
             launch_mdot(h->V[j + 1], Vl, j + 1, n, h->kpart, h->kdev, s);
             launch_maxpy(h->V[j + 1], Vl, j + 1, h->kdev, -1.0, n, s);
             std::vector<double> h1 = host_vec(j + 1);
-            launch_mdot(h->V[j + 1], Vl, j + 1, n, h->kpart, h->kdev, s);
-            launch_maxpy(h->V[j + 1], Vl, j + 1, h->kdev, -1.0, n, s);
-            std::vector<double> h2 = host_vec(j + 1);
-            VecList self{};
-            self.v[0] = h->V[j + 1];
+            std::vector<double> h2(j + 1, 0.0);
+            double hn2 = wn0 * wn0;
+            for (int q = 0; q <= j; ++q) hn2 -= h1[q] * h1[q];
+            if (!(hn2 > 0.5 * wn0 * wn0)) {
+                launch_mdot(h->V[j + 1], Vl, j + 1, n, h->kpart, h->kdev, s);
+                launch_maxpy(h->V[j + 1], Vl, j + 1, h->kdev, -1.0, n, s);
+                h2 = host_vec(j + 1);
+            }
             launch_mdot(h->V[j + 1], self, 1, n, h->kpart, h->kdev, s);
             const double hn = std::sqrt(host_vec(1)[0]);
             ++total;

@@ -1,0 +1,308 @@
+// alamo_elastic.hpp -- C++ host interface of the nodal elasticity solve,
+// shaped like the reference's classes so a caller of
+//   Operator::Elastic<T>            (src/Operator/Elastic.H:20-171)
+//   BC::Operator::Elastic<T>        (src/BC/Operator/Elastic.H:12-407)
+//   Model::Solid::Linear::{Isotropic,Cubic,Laplacian}
+//   Model::Solid::LinearElastic::Degradable::Isotropic
+//   Solver::Nonlocal::{Linear, Newton<T>} (src/Solver/Nonlocal/*.H)
+// can switch to the gfx950 implementation.  Header-only; every computation
+// goes through the C ABI of ela.h (libela.so).  AMReX containers are replaced
+// by project types: a Geometry (cells and cell size of one box) and dense host
+// fields (std::vector<double>, component-major, x fastest).  Where the
+// reference aborts (Util::Abort), these classes throw std::runtime_error.
+#pragma once
+
+#include <array>
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ela.h"
+
+namespace Set {
+using Scalar = double;
+}
+
+namespace ela_detail {
+inline void check(int rc, const char* what) {
+    if (rc != ELA_OK) throw std::runtime_error(std::string(what) + ": " + ela_last_error());
+}
+inline ela_ctx* context() {
+    static ela_ctx* ctx = nullptr;
+    if (!ctx) check(ela_ctx_create(0, 0, &ctx), "ela_ctx_create");
+    return ctx;
+}
+}  // namespace ela_detail
+
+// ---------------------------------------------------------------- models
+namespace Model {
+namespace Solid {
+namespace Linear {
+
+// Model::Solid::Linear::Isotropic (src/Model/Solid/Linear/Isotropic.H:13-58).
+// The constructor takes (mu, lambda) as the reference's does; storage is (lambda, mu).
+class Isotropic {
+public:
+    static constexpr int kind = ELA_MODEL_ISO;
+    Isotropic() {}
+    Isotropic(Set::Scalar mu, Set::Scalar lambda) { Define(mu, lambda); }
+    void Define(Set::Scalar mu, Set::Scalar lambda) { data = {lambda, mu}; }
+    std::vector<double> data{NAN, NAN};
+};
+
+// Model::Solid::Linear::Cubic (src/Model/Solid/Linear/Cubic.H:13-114): 21 values,
+// rotated from (C11, C12, C44) exactly as the reference's Define does (its
+// unrotated tensor sets C44 only where i==k && j==l, and each storage slot
+// keeps the last (p,q,s,t) of its symmetry class).
+class Cubic {
+public:
+    static constexpr int kind = ELA_MODEL_MM;
+    Cubic() : data(21, NAN) {}
+    void Define(Set::Scalar C11, Set::Scalar C12, Set::Scalar C44, Set::Scalar phi1, Set::Scalar Phi,
+                Set::Scalar phi2) {
+        auto rx = [](double a) { return std::array<double, 9>{1, 0, 0, 0, std::cos(a), -std::sin(a), 0, std::sin(a), std::cos(a)}; };
+        auto rz = [](double a) { return std::array<double, 9>{std::cos(a), -std::sin(a), 0, std::sin(a), std::cos(a), 0, 0, 0, 1}; };
+        auto mul = [](const std::array<double, 9>& a, const std::array<double, 9>& b) {
+            std::array<double, 9> o{};
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    for (int k = 0; k < 3; ++k) o[3 * i + j] += a[3 * i + k] * b[3 * k + j];
+            return o;
+        };
+        // AngleAxis(phi2, X) * AngleAxis(Phi, Z) * AngleAxis(phi1, X)  (Cubic.H:24-27)
+        Define(C11, C12, C44, mul(mul(rx(phi2), rz(Phi)), rx(phi1)));
+    }
+    void Define(Set::Scalar C11, Set::Scalar C12, Set::Scalar C44, const std::array<double, 9>& R) {
+        double Ct[3][3][3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k)
+                    for (int l = 0; l < 3; ++l)
+                        Ct[i][j][k][l] = (i == j && j == k && k == l) ? C11
+                                         : (i == k && j == l)       ? C44
+                                         : (i == j && k == l)       ? C12
+                                                                    : 0.0;
+        auto voigt = [](int i, int j) {
+            static const int t[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+            return t[i][j];
+        };
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q)
+                for (int s = 0; s < 3; ++s)
+                    for (int t = 0; t < 3; ++t) {
+                        double acc = 0;
+                        for (int i = 0; i < 3; ++i)
+                            for (int j = 0; j < 3; ++j)
+                                for (int k = 0; k < 3; ++k)
+                                    for (int l = 0; l < 3; ++l)
+                                        acc += R[3 * p + i] * R[3 * s + k] * Ct[i][j][k][l] * R[3 * q + j] * R[3 * t + l];
+                        int I = voigt(p, q), J = voigt(s, t);
+                        if (I > J) std::swap(I, J);
+                        data[I * 6 - I * (I - 1) / 2 + (J - I)] = acc;
+                    }
+    }
+    std::vector<double> data;
+};
+
+// Model::Solid::Linear::Laplacian (src/Model/Solid/Linear/Laplacian.H:25-57): Diagonal, A = ones.
+class Laplacian {
+public:
+    static constexpr int kind = ELA_MODEL_DIAG;
+    explicit Laplacian(int dim = 3) : data(dim * dim, 1.0) {}
+    std::vector<double> data;
+};
+
+}  // namespace Linear
+
+namespace LinearElastic {
+namespace Degradable {
+// Degradable::Isotropic (src/Model/Solid/LinearElastic/Degradable/Isotropic.H:21-117).
+class Isotropic {
+public:
+    static constexpr int kind = ELA_MODEL_ISO;
+    Isotropic() {}
+    Isotropic(Set::Scalar mu, Set::Scalar lambda) : mu0(mu), lambda0(lambda), data{lambda, mu} {}
+    void DegradeModulus(Set::Scalar eta) {
+        if (eta < 0.0 || eta > 1.0) throw std::runtime_error("Invalid value of eta");
+        data = {lambda0 * (1.0 - eta), mu0 * (1.0 - eta)};
+    }
+    Set::Scalar mu0 = NAN, lambda0 = NAN;
+    std::vector<double> data{NAN, NAN};
+};
+}  // namespace Degradable
+}  // namespace LinearElastic
+}  // namespace Solid
+}  // namespace Model
+
+// ---------------------------------------------------------------- boundary conditions
+namespace BC {
+namespace Operator {
+// BC::Operator::Elastic<T> (src/BC/Operator/Elastic.H): face classes and
+// component types; every class defaults to Displacement 0 (:46-55).
+template <class T>
+class Elastic {
+public:
+    enum Type { Displacement = ELA_BC_DISP, Traction = ELA_BC_TRAC, Periodic = ELA_BC_PERIODIC, Neumann = ELA_BC_NEUMANN };
+    enum Face {
+        XLO, YLO, ZLO, XHI, YHI, ZHI,
+        YLO_ZLO, YLO_ZHI, YHI_ZLO, YHI_ZHI,
+        ZLO_XLO, ZLO_XHI, ZHI_XLO, ZHI_XHI,
+        XLO_YLO, XLO_YHI, XHI_YLO, XHI_YHI,
+        XLO_YLO_ZLO, XLO_YLO_ZHI, XLO_YHI_ZLO, XLO_YHI_ZHI,
+        XHI_YLO_ZLO, XHI_YLO_ZHI, XHI_YHI_ZLO, XHI_YHI_ZHI,
+        INT
+    };
+    enum Direction { X = 0, Y = 1, Z = 2 };
+    Elastic() : types(26 * 3, Displacement), values(26 * 3, 0.0) {}
+    void Set(Face face, Direction dir, Type type, Set::Scalar value) {
+        types[3 * face + dir] = type;
+        values[3 * face + dir] = value;
+    }
+    std::vector<int> types;
+    std::vector<double> values;
+};
+}  // namespace Operator
+}  // namespace BC
+
+// ---------------------------------------------------------------- operator
+struct Geometry {
+    int dim = 3;
+    std::array<int, 3> n{0, 0, 0};       // cells
+    std::array<double, 3> L{1, 1, 1};    // domain lengths
+};
+
+namespace Operator {
+// Operator::Elastic<T> on one box.  define() builds the MG hierarchy;
+// SetModel/SetBC/SetUniform as in the reference; Fapply/Diagonal/Fsmooth/
+// restriction/interpolation act on dense host fields of a MG level.
+template <class T>
+class Elastic {
+public:
+    Elastic() {}
+    Elastic(const Geometry& g, int mode = ELA_ALLOC_BOTH) { define(g, mode); }
+    ~Elastic() {
+        if (h) ela_hier_destroy(h);
+    }
+    Elastic(const Elastic&) = delete;
+    Elastic& operator=(const Elastic&) = delete;
+
+    void define(const Geometry& g, int mode = ELA_ALLOC_BOTH) {
+        geom = g;
+        ela_domain d{};
+        d.dim = g.dim;
+        for (int q = 0; q < 3; ++q) {
+            d.n[q] = q < g.dim ? g.n[q] : 0;
+            d.dx[q] = q < g.dim ? g.L[q] / g.n[q] : 1.0;
+        }
+        ela_detail::check(ela_hier_create(ela_detail::context(), &d, T::kind, 0, mode, &h), "define");
+    }
+    int nodes(int mglev = 0) const {
+        ela_level_layout l;
+        ela_detail::check(ela_level_info(h, mglev, &l), "ela_level_info");
+        return l.nodes[0] * l.nodes[1] * l.nodes[2];
+    }
+    void SetModel(const T& a_model) {  // Elastic.cpp:57-82: the same model on every node
+        const int nc = (int)a_model.data.size(), nn = nodes();
+        std::vector<double> C((size_t)nc * nn);
+        for (int q = 0; q < nc; ++q)
+            for (int i = 0; i < nn; ++i) C[(size_t)q * nn + i] = a_model.data[q];
+        ela_detail::check(ela_set_model_field(h, C.data()), "SetModel");
+    }
+    void SetModel(const std::vector<T>& a_model) {  // Elastic.cpp:84-118: one model per node
+        const int nn = nodes();
+        if ((int)a_model.size() != nn) throw std::runtime_error("Inconsistent model field size");
+        const int nc = (int)a_model[0].data.size();
+        std::vector<double> C((size_t)nc * nn);
+        for (int i = 0; i < nn; ++i)
+            for (int q = 0; q < nc; ++q) C[(size_t)q * nn + i] = a_model[i].data[q];
+        ela_detail::check(ela_set_model_field(h, C.data()), "SetModel");
+    }
+    void SetBC(::BC::Operator::Elastic<T>* a_bc) {
+        m_bc = a_bc;
+        ela_detail::check(ela_set_bc(h, a_bc->types.data(), a_bc->values.data()), "SetBC");
+    }
+    ::BC::Operator::Elastic<T>& GetBC() { return *m_bc; }
+    void SetUniform(bool a_uniform) { ela_detail::check(ela_set_uniform(h, a_uniform), "SetUniform"); }
+    void prepareForSolve() { ela_detail::check(ela_prepare(h), "prepareForSolve"); }
+    // BC::Operator::Elastic::Init on level 0
+    void InitRHS(std::vector<double>& rhs, bool homogeneous = false) {
+        ela_detail::check(ela_bc_init(h, rhs.data(), homogeneous), "Init");
+    }
+    std::vector<double> Fapply(int mglev, const std::vector<double>& u, int prec = ELA_F64) {
+        void *du = nullptr, *df = nullptr;
+        ela_detail::check(ela_field_alloc(h, mglev, prec, geom.dim, &du), "alloc");
+        ela_detail::check(ela_field_alloc(h, mglev, prec, geom.dim, &df), "alloc");
+        ela_detail::check(ela_upload(h, mglev, prec, geom.dim, u.data(), du), "upload");
+        ela_detail::check(ela_fapply(h, mglev, prec, du, df, nullptr), "Fapply");
+        std::vector<double> f(u.size());
+        ela_detail::check(ela_download(h, mglev, prec, geom.dim, df, f.data()), "download");
+        ela_field_free(h, du);
+        ela_field_free(h, df);
+        return f;
+    }
+    ela_hier* handle() { return h; }
+    Geometry geom;
+
+private:
+    ela_hier* h = nullptr;
+    ::BC::Operator::Elastic<T>* m_bc = nullptr;  // non-owning, as in the reference (Elastic.H:161)
+};
+}  // namespace Operator
+
+// ---------------------------------------------------------------- solvers
+namespace Solver {
+namespace Nonlocal {
+// Solver::Nonlocal::Linear (src/Solver/Nonlocal/Linear.H:19-143): holds a reference to the operator.
+template <class T>
+class Linear {
+public:
+    explicit Linear(::Operator::Elastic<T>& a_op) : linop(a_op) {}
+    void setMaxIter(int n) { max_iter = n; }
+    void setVerbose(int v) { verbose = v; }
+    void setMixedPrecision(bool on) { mode = on ? ELA_SOLVE_MIXED : ELA_SOLVE_FP64; }
+    void setKrylov(int restart) { ela_detail::check(ela_set_krylov(linop.handle(), restart), "setKrylov"); }
+    // solve(sol, rhs, tol_rel, tol_abs) -> final |r|inf (Linear.H:102-112); sol holds the initial guess.
+    Set::Scalar solve(std::vector<double>& sol, const std::vector<double>& rhs, Set::Scalar tol_rel,
+                      Set::Scalar tol_abs) {
+        const int rc = ela_solve(linop.handle(), rhs.data(), sol.data(), tol_rel, tol_abs, mode, max_iter, &stats);
+        if (rc != ELA_OK && rc != ELA_ERR_NONCONVERGED) ela_detail::check(rc, "solve");
+        if (rc == ELA_ERR_NONCONVERGED) throw std::runtime_error(std::string("MLMG failed to converge: ") + ela_last_error());
+        return stats.res;
+    }
+    ela_stats stats{};
+
+protected:
+    ::Operator::Elastic<T>& linop;
+    int max_iter = 200, verbose = 0, mode = ELA_SOLVE_FP64;
+    Set::Scalar m_tol_rel = 1E-8, m_tol_abs = 1E-8;  // Linear.H:118
+};
+
+// Solver::Nonlocal::Newton<T> (src/Solver/Nonlocal/Newton.H:15-349) for the
+// linear models: one Newton step solves A du = b - A u and updates u.  From
+// u = 0 this is exactly one Linear::solve with rhs = b, as in the reference.
+template <class T>
+class Newton : public Linear<T> {
+public:
+    explicit Newton(::Operator::Elastic<T>& a_op) : Linear<T>(a_op) {}
+    void setNRIters(int n) { nriters = n; }
+    Set::Scalar solve(std::vector<double>& u, const std::vector<double>& b, const std::vector<T>& model,
+                      Set::Scalar tol_rel, Set::Scalar tol_abs) {
+        this->linop.SetModel(model);
+        this->linop.prepareForSolve();
+        Set::Scalar res = 0;
+        for (int it = 0; it < nriters; ++it) {
+            std::vector<double> Au = this->linop.Fapply(0, u);
+            std::vector<double> rhs(b.size()), du(b.size(), 0.0);
+            for (size_t q = 0; q < b.size(); ++q) rhs[q] = b[q] - Au[q];
+            res = Linear<T>::solve(du, rhs, tol_rel, tol_abs);
+            for (size_t q = 0; q < u.size(); ++q) u[q] += du[q];
+        }
+        return res;
+    }
+
+private:
+    int nriters = 1;  // Newton.H:337
+};
+}  // namespace Nonlocal
+}  // namespace Solver
