@@ -100,6 +100,7 @@ SIGNATURES = {
     "ela_set_cycle": ([_P, _I, _I, _I], _I),
     "ela_set_timing": ([_P, _I], _I),
     "ela_set_krylov": ([_P, _I], _I),
+    "ela_set_stiffness_mode": ([_P, _I], _I),
     "ela_kernel_times": ([_P, ctypes.POINTER(_D), ctypes.POINTER(_I), ctypes.POINTER(_D), ctypes.POINTER(_I)], _I),
     "ela_time_fapply": ([_P, _I, _I, ctypes.POINTER(_D)], _I),
     "ela_probe_copy": ([_P, _SZ, _I, ctypes.POINTER(_D)], _I),
@@ -273,6 +274,10 @@ class Elastic:
 
     def SetCycle(self, cycle=CYCLE_F, nu1=2, nu2=2):
         _check(lib().ela_set_cycle(self.ptr, cycle, nu1, nu2))
+
+    def SetStiffnessMode(self, mode):
+        """0: per-node stiffness planes; 1: per-node grain id + per-grain records (Voronoi models)."""
+        _check(lib().ela_set_stiffness_mode(self.ptr, int(mode)))
 
     def SetKrylov(self, restart):
         """Flexible GMRES(restart) around the MG cycle (0: plain MG iteration)."""

@@ -44,6 +44,39 @@ struct Inv {
     T imix[3];   // 0.25 / (h_a h_b) for the pairs (x,y), (x,z), (y,z)
 };
 
+// Stiffness accessors.  PlaneAcc: ncomp planes in the level layout (the
+// reference's per-node model field).  TabAcc: a per-node grain id and one
+// record per grain (nc values, contiguous); two nodes of the same grain have
+// bitwise-identical records, so their central difference is exactly zero.
+template <typename T>
+struct PlaneAcc {
+    const T* C;
+    long long cs;
+    __device__ __forceinline__ const T* rec(long long o) const { return C + o; }
+    __device__ __forceinline__ long long stride() const { return cs; }
+    __device__ __forceinline__ bool same(long long, long long) const { return false; }
+};
+template <typename T, int NC>
+struct TabAcc {
+    const unsigned short* gid;
+    const T* tab;
+    const T* dgt;  // per-grain interior diagonal (3 values), may be null
+    __device__ __forceinline__ const T* rec(long long o) const { return tab + (long long)gid[o] * NC; }
+    __device__ __forceinline__ long long stride() const { return 1; }
+    __device__ __forceinline__ bool same(long long a, long long b) const { return gid[a] == gid[b]; }
+};
+
+// Jacobi diagonal of a node: the stored field, or the per-grain interior value.
+template <typename T>
+__device__ __forceinline__ T jdiag(const PlaneAcc<T>&, const T* dg, long long q, long long, int, bool) {
+    return dg[q];
+}
+template <typename T, int NC>
+__device__ __forceinline__ T jdiag(const TabAcc<T, NC>& A, const T* dg, long long q, long long o, int p,
+                                   bool interior) {
+    return (interior && A.dgt) ? A.dgt[(long long)A.gid[o] * 3 + p] : dg[q];
+}
+
 template <int DIM, int K>
 struct NComp;
 template <int DIM> struct NComp<DIM, K_ISO> { static constexpr int v = 2; };
@@ -202,9 +235,9 @@ __device__ __forceinline__ void bc_row(const BCTab& bc, int face, const T u[DIM]
 }
 
 // f = A u at node (i,j,k).  C points at plane 0 of the stiffness, u at comp 0.
-template <typename T, int K, int DIM>
-__device__ __forceinline__ void node_apply(const Lay& L, const T* __restrict__ C, const T* __restrict__ u, int i,
-                                           int j, int k, const BCTab& bc, const Inv<T>& iv, bool uniform, T f[DIM]) {
+template <typename T, int K, int DIM, typename Acc>
+__device__ __forceinline__ void node_apply_acc(const Lay& L, const Acc& A, const T* __restrict__ u, int i, int j,
+                                               int k, const BCTab& bc, const Inv<T>& iv, bool uniform, T f[DIM]) {
     const long long o = (long long)i + L.px * j + L.pl * k;
     const int hx = L.nx - 1, hy = L.ny - 1, hz = L.nz - 1;
     const bool bnd = (i == 0 || i == hx || j == 0 || j == hy || (DIM == 3 && (k == 0 || k == hz)));
@@ -224,7 +257,7 @@ __device__ __forceinline__ void node_apply(const Lay& L, const T* __restrict__ C
                 else g[p][d] = (up[st[d]] - up[-st[d]]) * iv.ih2[d];
             }
         }
-        stress<T, K, DIM>(C + o, L.cs, g, s);
+        stress<T, K, DIM>(A.rec(o), A.stride(), g, s);
         bc_row<T, DIM>(bc, face_of<DIM>(i, j, k, hx, hy, hz), uc, g, s, f);
         return;
     }
@@ -250,8 +283,8 @@ __device__ __forceinline__ void node_apply(const Lay& L, const T* __restrict__ C
                 gg[p][b][a] = m;
             }
     }
-    const T* c = C + o;
-    const long long cs = L.cs;
+    const T* c = A.rec(o);
+    const long long cs = A.stride();
     if (K == K_ISO) {
         const T lam = c[0], mu = c[cs];
 #pragma unroll
@@ -271,8 +304,13 @@ __device__ __forceinline__ void node_apply(const Lay& L, const T* __restrict__ C
             T dl[DIM], dm[DIM];
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
-                dl[d] = (c[st[d]] - c[-st[d]]) * iv.ih2[d];
-                dm[d] = (c[cs + st[d]] - c[cs - st[d]]) * iv.ih2[d];
+                dl[d] = dm[d] = T(0);
+                if (!A.same(o + st[d], o - st[d])) {
+                    const T* cp = A.rec(o + st[d]);
+                    const T* cm = A.rec(o - st[d]);
+                    dl[d] = (cp[0] - cm[0]) * iv.ih2[d];
+                    dm[d] = (cp[cs] - cm[cs]) * iv.ih2[d];
+                }
             }
 #pragma unroll
             for (int a = 0; a < DIM; ++a) {
@@ -305,13 +343,16 @@ __device__ __forceinline__ void node_apply(const Lay& L, const T* __restrict__ C
                 T add = 0;
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
+                    if (A.same(o + st[d], o - st[d])) continue;
+                    const T* cp = A.rec(o + st[d]);
+                    const T* cm = A.rec(o - st[d]);
                     T acc = 0;
 #pragma unroll
                     for (int kk = 0; kk < DIM; ++kk)
 #pragma unroll
                         for (int l = kk; l < DIM; ++l) {
                             const long long q = mm_slot<DIM>(a, d, kk, l) * cs;
-                            acc += (c[q + st[d]] - c[q - st[d]]) * gs[kk][l];
+                            acc += (cp[q] - cm[q]) * gs[kk][l];
                         }
                     add += acc * iv.ih2[d];
                 }
@@ -332,13 +373,20 @@ __device__ __forceinline__ void node_apply(const Lay& L, const T* __restrict__ C
                 T add = 0;
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
+                    if (A.same(o + st[d], o - st[d])) continue;
                     const long long q = (a * DIM + d) * cs;
-                    add += (c[q + st[d]] - c[q - st[d]]) * iv.ih2[d] * g[a][d];
+                    add += (A.rec(o + st[d])[q] - A.rec(o - st[d])[q]) * iv.ih2[d] * g[a][d];
                 }
                 f[a] += add;
             }
         }
     }
+}
+
+template <typename T, int K, int DIM>
+__device__ __forceinline__ void node_apply(const Lay& L, const T* __restrict__ C, const T* __restrict__ u, int i,
+                                           int j, int k, const BCTab& bc, const Inv<T>& iv, bool uniform, T f[DIM]) {
+    node_apply_acc<T, K, DIM>(L, PlaneAcc<T>{C, L.cs}, u, i, j, k, bc, iv, uniform, f);
 }
 
 // Diagonal of A at node (i,j,k) (Elastic.cpp:285-349), FP64.

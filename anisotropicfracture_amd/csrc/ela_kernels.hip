@@ -110,13 +110,13 @@ __global__ __launch_bounds__(1024) void k_reduce_partials(const double* __restri
     if (threadIdx.x == 0) *norm = (unsigned long long)__double_as_longlong(sm[0]);
 }
 
-template <typename T, int K, int DIM>
-__global__ __launch_bounds__(256) void k_fapply(Lay L, const T* __restrict__ C, const T* __restrict__ u,
-                                                T* __restrict__ f, BCTab bc, Inv<T> iv, int uniform, Slab S) {
+template <typename T, int K, int DIM, typename Acc>
+__global__ __launch_bounds__(256) void k_fapply(Lay L, Acc A, const T* __restrict__ u, T* __restrict__ f, BCTab bc,
+                                                Inv<T> iv, int uniform, Slab S) {
     int i, j, k;
     if (!(S.rows ? slab_index(L, S, i, j, k) : node_index(L, i, j, k))) return;
     T r[DIM];
-    node_apply<T, K, DIM>(L, C, u, i, j, k, bc, iv, uniform != 0, r);
+    node_apply_acc<T, K, DIM>(L, A, u, i, j, k, bc, iv, uniform != 0, r);
     const long long o = (long long)i + L.px * j + L.pl * k;
 #pragma unroll
     for (int p = 0; p < DIM; ++p) f[p * L.cs + o] = r[p];
@@ -125,8 +125,8 @@ __global__ __launch_bounds__(256) void k_fapply(Lay L, const T* __restrict__ C, 
 // r = b - A x; optional outputs: r in T, r * scale in TO (the next cycle's
 // right-hand side), and the block maxima of |r|.  Every lane reaches the
 // block reduction (out-of-range lanes contribute 0).
-template <typename T, typename TO, int K, int DIM>
-__global__ __launch_bounds__(256) void k_residual(Lay L, const T* __restrict__ C, const T* __restrict__ x,
+template <typename T, typename TO, int K, int DIM, typename Acc>
+__global__ __launch_bounds__(256) void k_residual(Lay L, Acc A, const T* __restrict__ x,
                                                   const T* __restrict__ b, T* __restrict__ r, TO* __restrict__ rs,
                                                   const double* __restrict__ scale, double* __restrict__ partials,
                                                   BCTab bc, Inv<T> iv, int uniform, Slab S) {
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void k_residual(Lay L, const T* __restrict__ C
     double m = 0.0;
     if (in) {
         T f[DIM];
-        node_apply<T, K, DIM>(L, C, x, i, j, k, bc, iv, uniform != 0, f);
+        node_apply_acc<T, K, DIM>(L, A, x, i, j, k, bc, iv, uniform != 0, f);
         const long long o = (long long)i + L.px * j + L.pl * k;
         const double s = rs ? *scale : 1.0;
 #pragma unroll
@@ -151,20 +151,20 @@ __global__ __launch_bounds__(256) void k_residual(Lay L, const T* __restrict__ C
 }
 
 // One weighted-Jacobi sweep x_out = x_in + omega (b - A x_in) / D.
-template <typename T, int K, int DIM>
-__global__ __launch_bounds__(256) void k_jacobi(Lay L, const T* __restrict__ C, const T* __restrict__ dg,
-                                                const T* __restrict__ xin, const T* __restrict__ b,
-                                                T* __restrict__ xout, BCTab bc, Inv<T> iv, int uniform, T omega,
-                                                Slab S) {
+template <typename T, int K, int DIM, typename Acc>
+__global__ __launch_bounds__(256) void k_jacobi(Lay L, Acc A, const T* __restrict__ dg, const T* __restrict__ xin,
+                                                const T* __restrict__ b, T* __restrict__ xout, BCTab bc, Inv<T> iv,
+                                                int uniform, T omega, Slab S) {
     int i, j, k;
     if (!(S.rows ? slab_index(L, S, i, j, k) : node_index(L, i, j, k))) return;
     T f[DIM];
-    node_apply<T, K, DIM>(L, C, xin, i, j, k, bc, iv, uniform != 0, f);
+    node_apply_acc<T, K, DIM>(L, A, xin, i, j, k, bc, iv, uniform != 0, f);
     const long long o = (long long)i + L.px * j + L.pl * k;
+    const bool interior = i > 0 && i < L.nx - 1 && j > 0 && j < L.ny - 1 && (DIM == 2 || (k > 0 && k < L.nz - 1));
 #pragma unroll
     for (int p = 0; p < DIM; ++p) {
         const long long q = p * L.cs + o;
-        xout[q] = xin[q] + omega * (b[q] - f[p]) / dg[q];
+        xout[q] = xin[q] + omega * (b[q] - f[p]) / jdiag(A, dg, q, o, p, interior);
     }
 }
 
@@ -458,7 +458,7 @@ __global__ void k_cubic_table(int ng, const double* __restrict__ R, double C11, 
 // index), then the grain's 21 values into the stiffness planes.
 __global__ __launch_bounds__(256) void k_voronoi_cubic(Lay L, int nseeds, const long long* __restrict__ seeds, int q,
                                                        const double* __restrict__ table, double* __restrict__ C,
-                                                       int* __restrict__ gid) {
+                                                       unsigned short* __restrict__ gid) {
     int i, j, k;
     if (!node_index(L, i, j, k)) return;
     const long long x = (long long)i * q, y = (long long)j * q, z = (long long)k * q;
@@ -471,7 +471,7 @@ __global__ __launch_bounds__(256) void k_voronoi_cubic(Lay L, int nseeds, const 
     }
     const long long o = (long long)i + L.px * j + L.pl * k;
     for (int c = 0; c < 21; ++c) C[c * L.cs + o] = table[21 * bi + c];
-    if (gid) gid[o] = bi;
+    if (gid) gid[o] = (unsigned short)bi;
 }
 
 __global__ __launch_bounds__(256) void k_degraded_iso(Lay L, const double* __restrict__ eta, double lambda0,
@@ -628,8 +628,8 @@ void launch_fapply(const Lay& L, int kind, const T* C, const T* u, T* f, const B
     const Slab S = sl ? slab_of(L) : Slab{0, 0, 0, 0};
     const dim3 grid = sl ? slab_grid(L) : node_grid(L);
     ELA_DISPATCH(kind, L.dim,
-                 hipLaunchKernelGGL((k_fapply<T, KK, DD>), grid, dim3(BX, BY), 0, s, L, C, u, f, bc, iv,
-                                    uniform, S));
+                 hipLaunchKernelGGL((k_fapply<T, KK, DD, PlaneAcc<T>>), grid, dim3(BX, BY), 0, s, L,
+                                    (PlaneAcc<T>{C, L.cs}), u, f, bc, iv, uniform, S));
 }
 
 template <typename T, typename TO>
@@ -641,8 +641,9 @@ void launch_residual(const Lay& L, int kind, const T* C, const T* x, const T* b,
     const Slab S = sl ? slab_of(L) : Slab{0, 0, 0, 0};
     const dim3 grid = sl ? slab_grid(L) : node_grid(L);
     ELA_DISPATCH(kind, L.dim,
-                 hipLaunchKernelGGL((k_residual<T, TO, KK, DD>), grid, dim3(BX, BY), 0, s, L, C, x, b, r,
-                                    rs, scale, norm ? partials : nullptr, bc, iv, uniform, S));
+                 hipLaunchKernelGGL((k_residual<T, TO, KK, DD, PlaneAcc<T>>), grid, dim3(BX, BY), 0, s, L,
+                                    (PlaneAcc<T>{C, L.cs}), x, b, r, rs, scale, norm ? partials : nullptr, bc, iv,
+                                    uniform, S));
     if (norm) {
         const dim3 g = grid;
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partials, (int)(g.x * g.y * g.z), norm);
@@ -666,8 +667,8 @@ void launch_jacobi(const Lay& L, int kind, const T* C, const T* dg, const T* xin
     const Slab S = sl ? slab_of(L) : Slab{0, 0, 0, 0};
     const dim3 grid = sl ? slab_grid(L) : node_grid(L);
     ELA_DISPATCH(kind, L.dim,
-                 hipLaunchKernelGGL((k_jacobi<T, KK, DD>), grid, dim3(BX, BY), 0, s, L, C, dg, xin, b,
-                                    xout, bc, iv, uniform, (T)omega, S));
+                 hipLaunchKernelGGL((k_jacobi<T, KK, DD, PlaneAcc<T>>), grid, dim3(BX, BY), 0, s, L,
+                                    (PlaneAcc<T>{C, L.cs}), dg, xin, b, xout, bc, iv, uniform, (T)omega, S));
 }
 
 template <typename T>
@@ -746,7 +747,7 @@ void launch_cubic_table(int ng, const double* R, double C11, double C12, double 
 }
 
 void launch_voronoi_cubic(const Lay& L, int nseeds, const long long* seeds, int q, const double* table, double* C,
-                          int* gid, hipStream_t s) {
+                          unsigned short* gid, hipStream_t s) {
     hipLaunchKernelGGL(k_voronoi_cubic, node_grid(L), dim3(BX, BY), 0, s, L, nseeds, seeds, q, table, C, gid);
 }
 
@@ -759,6 +760,82 @@ template <typename T>
 void launch_pack(const Lay& L, int nc, const double* dense, T* dev, int to_dev, hipStream_t s) {
     hipLaunchKernelGGL((k_pack<T>), node_grid(L), dim3(BX, BY), 0, s, L, nc, dense, dev, to_dev);
 }
+
+// Grain-table variants (3D major/minor levels): the stiffness of node o is tab[gid[o]].
+template <typename T>
+void launch_fapply_tab(const Lay& L, const TabDesc<T>& t, const T* u, T* f, const BCTab& bc, const double h[3],
+                       int uniform, hipStream_t s) {
+    const Inv<T> iv = make_inv<T>(h);
+    const bool sl = g_slab_order;
+    const Slab S = sl ? slab_of(L) : Slab{0, 0, 0, 0};
+    const dim3 grid = sl ? slab_grid(L) : node_grid(L);
+    using A = TabAcc<T, 21>;
+    hipLaunchKernelGGL((k_fapply<T, K_MM, 3, A>), grid, dim3(BX, BY), 0, s, L, (A{t.gid, t.tab, t.dgt}), u, f, bc, iv,
+                       uniform, S);
+}
+
+template <typename T>
+void launch_jacobi_tab(const Lay& L, const TabDesc<T>& t, const T* dg, const T* xin, const T* b, T* xout,
+                       const BCTab& bc, const double h[3], int uniform, double omega, hipStream_t s) {
+    const Inv<T> iv = make_inv<T>(h);
+    const bool sl = g_slab_order;
+    const Slab S = sl ? slab_of(L) : Slab{0, 0, 0, 0};
+    const dim3 grid = sl ? slab_grid(L) : node_grid(L);
+    using A = TabAcc<T, 21>;
+    hipLaunchKernelGGL((k_jacobi<T, K_MM, 3, A>), grid, dim3(BX, BY), 0, s, L, (A{t.gid, t.tab, t.dgt}), dg, xin, b,
+                       xout, bc, iv, uniform, (T)omega, S);
+}
+
+template <typename T, typename TO>
+void launch_residual_tab(const Lay& L, const TabDesc<T>& t, const T* x, const T* b, T* r, TO* rs,
+                         const double* scale, double* partials, unsigned long long* norm, const BCTab& bc,
+                         const double h[3], int uniform, hipStream_t s) {
+    const Inv<T> iv = make_inv<T>(h);
+    const bool sl = g_slab_order;
+    const Slab S = sl ? slab_of(L) : Slab{0, 0, 0, 0};
+    const dim3 grid = sl ? slab_grid(L) : node_grid(L);
+    using A = TabAcc<T, 21>;
+    hipLaunchKernelGGL((k_residual<T, TO, K_MM, 3, A>), grid, dim3(BX, BY), 0, s, L, (A{t.gid, t.tab, t.dgt}), x, b,
+                       r, rs, scale, norm ? partials : nullptr, bc, iv, uniform, S);
+    if (norm) hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partials, (int)(grid.x * grid.y * grid.z), norm);
+}
+
+// Per-grain interior diagonal sum_b C_{p b p b} (-2 / h_b^2) (Elastic.cpp:311-321 on an interior node).
+__global__ void k_grain_diag(int ng, const double* __restrict__ tab, double hx, double hy, double hz,
+                             double* __restrict__ d64, float* __restrict__ d32) {
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= ng) return;
+    const double h[3] = {hx, hy, hz};
+    for (int p = 0; p < 3; ++p) {
+        double acc = 0;
+        for (int b = 0; b < 3; ++b) acc += tab[21 * gi + mm_slot<3>(p, b, p, b)] * (-2.0 / h[b] / h[b]);
+        if (d64) d64[3 * gi + p] = acc;
+        if (d32) d32[3 * gi + p] = (float)acc;
+    }
+}
+
+void launch_grain_diag(int ng, const double* tab, const double h[3], double* d64, float* d32, hipStream_t s) {
+    hipLaunchKernelGGL(k_grain_diag, dim3((ng + 63) / 64), dim3(64), 0, s, ng, tab, h[0], h[1], h[2], d64, d32);
+}
+
+
+template void launch_fapply_tab<double>(const Lay&, const TabDesc<double>&, const double*, double*, const BCTab&,
+                                        const double*, int, hipStream_t);
+template void launch_fapply_tab<float>(const Lay&, const TabDesc<float>&, const float*, float*, const BCTab&,
+                                       const double*, int, hipStream_t);
+template void launch_jacobi_tab<float>(const Lay&, const TabDesc<float>&, const float*, const float*, const float*,
+                                       float*, const BCTab&, const double*, int, double, hipStream_t);
+template void launch_jacobi_tab<double>(const Lay&, const TabDesc<double>&, const double*, const double*,
+                                        const double*, double*, const BCTab&, const double*, int, double, hipStream_t);
+template void launch_residual_tab<double, float>(const Lay&, const TabDesc<double>&, const double*, const double*,
+                                                 double*, float*, const double*, double*, unsigned long long*,
+                                                 const BCTab&, const double*, int, hipStream_t);
+template void launch_residual_tab<double, double>(const Lay&, const TabDesc<double>&, const double*, const double*,
+                                                  double*, double*, const double*, double*, unsigned long long*,
+                                                  const BCTab&, const double*, int, hipStream_t);
+template void launch_residual_tab<float, float>(const Lay&, const TabDesc<float>&, const float*, const float*,
+                                                float*, float*, const double*, double*, unsigned long long*,
+                                                const BCTab&, const double*, int, hipStream_t);
 
 // explicit instantiations
 template void launch_fapply<double>(const Lay&, int, const double*, const double*, double*, const BCTab&,

@@ -64,7 +64,27 @@ void launch_bottom(const Lay& L, int nodes, int dim, const double* M, const T* b
 
 void launch_cubic_table(int ng, const double* R, double C11, double C12, double C44, double* table, hipStream_t s);
 void launch_voronoi_cubic(const Lay& L, int nseeds, const long long* seeds, int q, const double* table, double* C,
-                          int* gid, hipStream_t s);
+                          unsigned short* gid, hipStream_t s);
+
+// Grain-table stiffness (3D major/minor): per-node uint16 grain id, 21 values per grain, and the
+// per-grain interior diagonal (3 values) for the Jacobi sweep.
+template <typename T>
+struct TabDesc {
+    const unsigned short* gid;
+    const T* tab;
+    const T* dgt;
+};
+template <typename T>
+void launch_fapply_tab(const Lay& L, const TabDesc<T>& t, const T* u, T* f, const BCTab& bc, const double h[3],
+                       int uniform, hipStream_t s);
+template <typename T>
+void launch_jacobi_tab(const Lay& L, const TabDesc<T>& t, const T* dg, const T* xin, const T* b, T* xout,
+                       const BCTab& bc, const double h[3], int uniform, double omega, hipStream_t s);
+template <typename T, typename TO>
+void launch_residual_tab(const Lay& L, const TabDesc<T>& t, const T* x, const T* b, T* r, TO* rs,
+                         const double* scale, double* partials, unsigned long long* norm, const BCTab& bc,
+                         const double h[3], int uniform, hipStream_t s);
+void launch_grain_diag(int ng, const double* tab, const double h[3], double* d64, float* d32, hipStream_t s);
 void launch_degraded_iso(const Lay& L, const double* eta, double lambda0, double mu0, double a, double b, double c,
                          double* C, hipStream_t s);
 

@@ -112,6 +112,12 @@ struct ela_hier {
     double* dscale = nullptr;
     int* dflag = nullptr;
     unsigned long long* hnorm = nullptr;  // pinned
+    // grain-table stiffness of level 0 (Voronoi models): per-node grain id, per-grain records
+    unsigned short* gid = nullptr;
+    double *tab64 = nullptr, *dgt64 = nullptr;
+    float *tab32 = nullptr, *dgt32 = nullptr;
+    int ngrains = 0;
+    bool has_table = false, use_table = false;
     // flexible GMRES around the MG cycle (restart length; 0 = plain MG iteration)
     int krylov = 0;
     std::vector<double*> V, Z;
@@ -543,21 +549,32 @@ int check_prec(const ela_hier* h, int prec) {
 hipStream_t strm(ela_hier* h, void* s) { return s ? (hipStream_t)s : h->stream; }
 
 template <typename T>
+TabDesc<T> tab_of(ela_hier* h);
+template <>
+TabDesc<double> tab_of<double>(ela_hier* h) { return {h->gid, h->tab64, h->dgt64}; }
+template <>
+TabDesc<float> tab_of<float>(ela_hier* h) { return {h->gid, h->tab32, h->dgt32}; }
+bool table_on(ela_hier* h, const Level& v) { return h->use_table && h->has_table && &v == &h->lev[0]; }
+
+template <typename T>
 void A_apply(ela_hier* h, Level& v, const T* C, const T* u, T* f, hipStream_t s) {
-    if (v.mp.ok) march_fapply<T>(v.mp, v.L, h->kind, C, u, f, h->bc, v.h, h->uniform, s);
+    if (table_on(h, v)) launch_fapply_tab<T>(v.L, tab_of<T>(h), u, f, h->bc, v.h, h->uniform, s);
+    else if (v.mp.ok) march_fapply<T>(v.mp, v.L, h->kind, C, u, f, h->bc, v.h, h->uniform, s);
     else launch_fapply<T>(v.L, h->kind, C, u, f, h->bc, v.h, h->uniform, s);
 }
 
 template <typename T>
 void A_jacobi(ela_hier* h, Level& v, const T* C, const T* dg, const T* xin, const T* b, T* xout, hipStream_t s) {
-    if (v.mp.ok) march_jacobi<T>(v.mp, v.L, h->kind, C, dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
+    if (table_on(h, v)) launch_jacobi_tab<T>(v.L, tab_of<T>(h), dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
+    else if (v.mp.ok) march_jacobi<T>(v.mp, v.L, h->kind, C, dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
     else launch_jacobi<T>(v.L, h->kind, C, dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
 }
 
 template <typename T, typename TO>
 void A_residual(ela_hier* h, Level& v, const T* C, const T* x, const T* b, T* r, TO* rs, const double* scale,
                 unsigned long long* norm, hipStream_t s) {
-    if (v.mp.ok) march_residual<T, TO>(v.mp, v.L, h->kind, C, x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
+    if (table_on(h, v)) launch_residual_tab<T, TO>(v.L, tab_of<T>(h), x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
+    else if (v.mp.ok) march_residual<T, TO>(v.mp, v.L, h->kind, C, x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
     else launch_residual<T, TO>(v.L, h->kind, C, x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
 }
 
@@ -757,6 +774,7 @@ static int download_dense(ela_hier* h, const Level& v, int nc, const void* dev, 
 
 int ela_set_model_field(ela_hier* h, const double* C_host) {
     if (!h || !C_host) return fail(ELA_ERR_BADARG, "null argument");
+    h->has_table = false;
     if (int rc = upload_dense(h, h->lev[0], h->nc, C_host, h->lev[0].C64, ELA_F64)) return rc;
     h->model_set = true;
     h->prepared = false;
@@ -774,25 +792,40 @@ int ela_set_model_voronoi_cubic(ela_hier* h, int nseeds, const int64_t* seeds, i
             const long long lim = (long long)v.n[d] * q;
             if (seeds[3 * s + d] < -lim || seeds[3 * s + d] > 2 * lim) return fail(ELA_ERR_BADARG, "seed outside the domain");
         }
-    void *dseeds = nullptr, *dR = nullptr, *dtab = nullptr;
+    if (nseeds > 65535) return fail(ELA_ERR_BADARG, "at most 65535 grains");
+    // per-grain records and the per-node grain id stay resident (grain-table mode)
+    for (void* p : {(void*)h->gid, (void*)h->tab64, (void*)h->dgt64, (void*)h->tab32, (void*)h->dgt32})
+        if (p) hfree(h, p);
+    h->gid = nullptr;
+    h->tab64 = h->dgt64 = nullptr;
+    h->tab32 = h->dgt32 = nullptr;
+    h->has_table = false;
+    void *dseeds = nullptr, *dR = nullptr;
     int rc = halloc(h, sizeof(long long) * 3 * nseeds, &dseeds);
     if (!rc) rc = halloc(h, sizeof(double) * 9 * nseeds, &dR);
-    if (!rc) rc = halloc(h, sizeof(double) * 21 * nseeds, &dtab);
+    if (!rc) rc = halloc_t(h, (size_t)v.L.cs, &h->gid);
+    if (!rc) rc = halloc_t(h, (size_t)21 * nseeds, &h->tab64);
+    if (!rc) rc = halloc_t(h, (size_t)21 * nseeds, &h->tab32);
+    if (!rc) rc = halloc_t(h, (size_t)3 * nseeds, &h->dgt64);
+    if (!rc) rc = halloc_t(h, (size_t)3 * nseeds, &h->dgt32);
     hipError_t e = hipSuccess;
     if (!rc) {
         e = hipMemcpyAsync(dseeds, seeds, sizeof(long long) * 3 * nseeds, hipMemcpyHostToDevice, h->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(dR, R, sizeof(double) * 9 * nseeds, hipMemcpyHostToDevice, h->stream);
         if (e == hipSuccess) {
-            launch_cubic_table(nseeds, (const double*)dR, C11, C12, C44, (double*)dtab, h->stream);
-            launch_voronoi_cubic(v.L, nseeds, (const long long*)dseeds, q, (const double*)dtab, v.C64, nullptr, h->stream);
+            launch_cubic_table(nseeds, (const double*)dR, C11, C12, C44, h->tab64, h->stream);
+            launch_convert<double, float>(21LL * nseeds, h->tab64, h->tab32, h->stream);
+            launch_grain_diag(nseeds, h->tab64, v.h, h->dgt64, h->dgt32, h->stream);
+            launch_voronoi_cubic(v.L, nseeds, (const long long*)dseeds, q, h->tab64, v.C64, h->gid, h->stream);
             e = hipStreamSynchronize(h->stream);
         }
     }
     if (dseeds) hfree(h, dseeds);
     if (dR) hfree(h, dR);
-    if (dtab) hfree(h, dtab);
     if (rc) return rc;
     if (e != hipSuccess) return fail(ELA_ERR_HIP, std::string("voronoi stiffness: ") + hipGetErrorString(e));
+    h->ngrains = nseeds;
+    h->has_table = true;
     h->model_set = true;
     h->prepared = false;
     return ELA_OK;
@@ -801,6 +834,7 @@ int ela_set_model_voronoi_cubic(ela_hier* h, int nseeds, const int64_t* seeds, i
 int ela_set_model_degraded_iso(ela_hier* h, double lambda0, double mu0, const double* eta, double a, double b,
                                double c) {
     if (!h || !eta) return fail(ELA_ERR_BADARG, "null argument");
+    h->has_table = false;
     if (h->kind != ELA_MODEL_ISO) return fail(ELA_ERR_STATE, "degraded isotropic needs an isotropic hierarchy");
     const Level& v = h->lev[0];
     double* deta = nullptr;
@@ -1134,6 +1168,14 @@ int ela_solve_device(ela_hier* h, const void* b_dev, void* u_dev, double tol_rel
     HIPCHK(hipMemcpyAsync(u_dev, h->u64, bytes, hipMemcpyDeviceToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return rc;
+}
+
+int ela_set_stiffness_mode(ela_hier* h, int mode) {
+    if (!h) return fail(ELA_ERR_BADARG, "null hierarchy");
+    if (mode != 0 && mode != 1) return fail(ELA_ERR_BADARG, "mode must be 0 (planes) or 1 (grain table)");
+    if (mode == 1 && !h->has_table) return fail(ELA_ERR_STATE, "no grain table: set a Voronoi model first");
+    h->use_table = mode == 1;
+    return ELA_OK;
 }
 
 int ela_set_krylov(ela_hier* h, int restart) {

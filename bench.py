@@ -82,6 +82,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed", help="comma list of fp64, mixed")
     ap.add_argument("--krylov", type=int, default=16, help="FGMRES restart around the MG cycle (0: plain MG)")
+    ap.add_argument("--table", type=int, default=1, help="grain-table stiffness on level 0 for the headline mixed run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md on sharding)",
@@ -107,17 +108,29 @@ def main():
 
     modes = args.modes.split(",")
     t64 = st64 = u64 = None
-    if "fp64" in modes:  # plain FP64 GPU solver
-        log("bench: fp64 solves")
+    if "fp64" in modes:  # plain FP64 GPU solver, per-node stiffness planes (the reference's storage)
+        log("bench: fp64 solves (stiffness planes)")
         t64, per64, st64, rc64, _ = run_mode(op, b, u, ela.SOLVE_FP64, args.steps, args.warmup, args.tol)
         u64 = u.download()
         log("bench: fp64 %.3f s/solve iters=%d rel=%.2e rc=%d" % (t64 / args.steps, st64["iters"], st64["res"] / st64["res0"], rc64))
+    tmp = stmp = None
+    if "mixed" in modes and args.table:  # mixed with planes, for the precision-only ratio
+        log("bench: mixed solves (stiffness planes)")
+        tmp, _, stmp, _, _ = run_mode(op, b, u, ela.SOLVE_MIXED, args.steps, args.warmup, args.tol)
+        log("bench: mixed planes %.3f s/solve iters=%d" % (tmp / args.steps, stmp["iters"]))
+    op.SetStiffnessMode(1 if args.table else 0)
+    t64t = None
+    if "fp64" in modes and args.table:
+        log("bench: fp64 solves (grain table)")
+        t64t, _, st64t, _, _ = run_mode(op, b, u, ela.SOLVE_FP64, args.steps, args.warmup, args.tol)
+        log("bench: fp64 table %.3f s/solve iters=%d" % (t64t / args.steps, st64t["iters"]))
     # mixed precision (headline)
-    log("bench: mixed solves")
+    log("bench: mixed solves (%s)" % ("grain table" if args.table else "stiffness planes"))
     tmx, permx, stmx, rcmx, kt = run_mode(op, b, u, ela.SOLVE_MIXED, args.steps, args.warmup, args.tol, timing=True)
     umx = u.download()
     log("bench: mixed %.3f s/solve iters=%d rel=%.2e rc=%d" % (tmx / args.steps, stmx["iters"], stmx["res"] / stmx["res0"], rcmx))
     disp_diff = float(np.abs(umx - u64).max() / np.abs(u64).max()) if u64 is not None else None
+    op.SetStiffnessMode(0)
     # HBM copy probe (also the byte-counter calibration kernel of the PMC runs)
     probe_bytes = 1 << 30
     probe_ms = op.ctx.probe_copy(probe_bytes, 5)
@@ -130,15 +143,19 @@ def main():
     bytes_fa64 = 8 * (3 + 3 + nc) * nodes
     bytes_fa32 = 4 * (3 + 3 + nc) * nodes
     # dominant kernel of the mixed solve: level-0 FP32 Jacobi sweep vs level-0 FP64 residual
-    bytes_jac32 = 4 * (4 * 3 + nc) * nodes          # x in, b, diag, x out, C
+    # x in, b, diag, x out, C planes -- or x in, b, x out, 2-byte grain id (interior diagonal per grain)
+    bytes_jac32 = (4 * (4 * 3 + nc) if not args.table else 4 * 9 + 2) * nodes
     # x, b, C in FP64 + scaled FP32 rhs out (plain MG) / A z: z, C in, w out (FGMRES)
-    bytes_res64 = ((8 * (3 + 3 + nc) + 4 * 3) if not args.krylov else 8 * (3 + 3 + nc)) * nodes
+    cbytes = 8 * nc if not args.table else 2
+    bytes_res64 = ((8 * 6 + cbytes + 4 * 3) if not args.krylov else 8 * 6 + cbytes) * nodes
     tot_jac = kt["smooth0_ms"] * kt["smooth0_n"]
     tot_res = kt["resid64_ms"] * kt["resid64_n"]
     if tot_jac >= tot_res:
-        dom, dom_ms, dom_bytes = "k_jacobi<float,MM,3> (level 0)", kt["smooth0_ms"], bytes_jac32
+        dom = "k_jacobi<float,MM,3,%s> (level 0)" % ("TabAcc" if args.table else "PlaneAcc")
+        dom_ms, dom_bytes = kt["smooth0_ms"], bytes_jac32
     else:
-        dom = "k_fapply<double,MM,3> (level 0)" if args.krylov else "k_residual<double,float,MM,3> (level 0)"
+        acc = "TabAcc" if args.table else "PlaneAcc"
+        dom = ("k_fapply<double,MM,3,%s> (level 0)" if args.krylov else "k_residual<double,float,MM,3,%s> (level 0)") % acc
         dom_ms, dom_bytes = kt["resid64_ms"], bytes_res64
     peak = 8000.0
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
@@ -182,6 +199,11 @@ def main():
                    "solver": "FGMRES(%d) + MG F-cycle" % args.krylov if args.krylov else "MG F-cycles"},
         "fp64_solves_per_s": args.steps / t64 if t64 else None,
         "mixed_over_fp64": t64 / tmx if t64 else None,
+        "stiffness": "grain table on level 0 (16-bit grain id + per-grain records)" if args.table else "per-node planes",
+        "variants_solves_per_s": {"fp64_planes": args.steps / t64 if t64 else None,
+                                  "mixed_planes": args.steps / tmp if tmp else None,
+                                  "fp64_table": args.steps / t64t if t64t else None,
+                                  "mixed_table": args.steps / tmx if args.table else None},
         "iters": {"fp64": st64["iters"] if st64 else None, "mixed": stmx["iters"]},
         "rel_residual": {"fp64": st64["res"] / st64["res0"] if st64 else None, "mixed": stmx["res"] / stmx["res0"]},
         "hbm_copy_probe_GBps": probe_GBps,

@@ -117,6 +117,14 @@ int ela_set_model_field(ela_hier* h, const double* C_host);
 int ela_set_model_voronoi_cubic(ela_hier* h, int nseeds, const int64_t* seeds, int q, const double* R,
                                 double C11, double C12, double C44);
 
+/* Stiffness representation of level 0 for every kernel that follows:
+ * 0 = per-node planes (the reference's per-node model field, default);
+ * 1 = grain table: a per-node 16-bit grain id and one record per grain
+ *     (available after ela_set_model_voronoi_cubic).  The same values, so
+ *     the same operator bit for bit; coarse MG levels always use planes
+ *     (their stiffness is the full-weighting average). */
+int ela_set_stiffness_mode(ela_hier* h, int mode);
+
 /* Phase-field-degraded isotropic stiffness (Degradable::Isotropic::DegradeModulus,
  * src/Model/Solid/LinearElastic/Degradable/Isotropic.H:74-104, with a general
  * degradation g): lambda = g(eta) lambda0, mu = g(eta) mu0 with

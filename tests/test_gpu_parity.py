@@ -223,3 +223,49 @@ def test_config1_inclusion_krylov_vs_direct(oracle, mode):
     print("cfg1 32^2", st)
     assert st["converged"]
     assert np.abs(u - u_ref).max() / np.abs(u_ref).max() <= 1e-7
+
+
+@pytest.mark.parametrize("n", [(24, 24, 24), (40, 20, 33)])
+def test_grain_table_fapply_matches_planes_and_oracle(oracle, n):
+    """Grain-table stiffness (gid + per-grain records) gives the same operator as the per-node planes."""
+    cfg = K.cfg2(8, grains=13, seed=11)
+    rng = np.random.default_rng(4)
+    cfg["n"] = n
+    cfg["seeds"] = K.voronoi_seeds(rng, 13, n)
+    C_want, _ = H.voronoi_cubic_field(cfg, oracle.cubic_define)
+    op = ela.Elastic(n, MM, mode=ela.ALLOC_BOTH)
+    op.SetModelVoronoiCubic(cfg["seeds"], K.VORONOI_Q, cfg["R"], cfg["C11"], cfg["C12"], cfg["C44"])
+    t, v = H.bc_mixed(3)
+    op.SetBC(t, v)
+    op.prepareForSolve()
+    u = H.random_field(rng, n, 3)
+    g = oracle.Grid(n)
+    want = oracle.fapply(g, MM, C_want, t, u)
+    s = scale_of(C_want, u, g.dx)
+    planes64, planes32 = op.Fapply(u, prec=ela.F64), op.Fapply(u, prec=ela.F32)
+    op.SetStiffnessMode(1)
+    tab64, tab32 = op.Fapply(u, prec=ela.F64), op.Fapply(u, prec=ela.F32)
+    assert np.abs(tab64 - want).max() <= 1e-12 * s
+    assert np.abs(tab64 - planes64).max() <= 1e-13 * s
+    assert np.abs(tab32 - want).max() <= 2e-5 * s
+    r_tab, nrm_tab = op.residual(u, np.zeros_like(u), prec=ela.F64)
+    assert np.abs(r_tab + want).max() <= 1e-12 * s
+
+
+@pytest.mark.parametrize("mode", [ela.SOLVE_FP64, ela.SOLVE_MIXED])
+@pytest.mark.parametrize("krylov", [0, 16])
+def test_grain_table_solve_matches_oracle(oracle, mode, krylov):
+    cfg = K.cfg2(16, grains=12, seed=3)
+    C, _ = H.voronoi_cubic_field(cfg, oracle.cubic_define)
+    g = oracle.Grid(cfg["n"])
+    b = oracle.bc_init(g, cfg["bc_type"], cfg["bc_value"], g.zeros())
+    u_or, _, _ = oracle.mg_solve(g, MM, C, cfg["bc_type"], b, tol_rel=1e-12, max_iter=300)
+    op = ela.Elastic(cfg["n"], MM, mode=ela.ALLOC_BOTH)
+    op.SetModelVoronoiCubic(cfg["seeds"], K.VORONOI_Q, cfg["R"], cfg["C11"], cfg["C12"], cfg["C44"])
+    op.SetBC(cfg["bc_type"], cfg["bc_value"])
+    op.SetStiffnessMode(1)
+    op.SetKrylov(krylov)
+    op.prepareForSolve()
+    u, st = ela.Linear(op).solve(np.zeros_like(b), b, tol_rel=1e-10, tol_abs=0.0, mode=mode, max_iter=300)
+    assert st["converged"] and st["res"] / st["res0"] <= 1e-10
+    assert np.abs(u - u_or).max() / np.abs(u_or).max() <= 1e-8
