@@ -8,6 +8,9 @@ with ctypes and mirrors the reference's operator interface for Python callers:
   ``SetModel``, ``SetBC``, ``SetUniform``, ``prepareForSolve``, ``Fapply``,
   ``Diagonal``, ``Fsmooth``, ``restriction``, ``interpolation``, ``residual``.
 * :class:`Linear` -- ``Solver::Nonlocal::Linear`` (src/Solver/Nonlocal/Linear.H:19-143).
+* :class:`Batch` / :class:`RVE` -- many independent box problems solved
+  together (one launch per cycle kernel over all members); each member is the
+  same problem and iteration as ``Linear.solve`` on its own operator.
 
 Every computation runs on the GPU through the library; there is no CPU
 fallback.  If the library is missing, importing works but the first use
@@ -104,6 +107,32 @@ SIGNATURES = {
     "ela_kernel_times": ([_P, ctypes.POINTER(_D), ctypes.POINTER(_I), ctypes.POINTER(_D), ctypes.POINTER(_I)], _I),
     "ela_time_fapply": ([_P, _I, _I, ctypes.POINTER(_D)], _I),
     "ela_probe_copy": ([_P, _SZ, _I, ctypes.POINTER(_D)], _I),
+    "ela_hier_create_ex": ([_P, ctypes.POINTER(_Domain), _I, _I, _I, _I, ctypes.POINTER(_P)], _I),
+    # batched independent solves
+    "ela_batch_create": ([_P, _I, ctypes.POINTER(_P)], _I),
+    "ela_batch_destroy": ([_P], _I),
+    "ela_batch_bytes_in_use": ([_P], _SZ),
+    "ela_rve_bytes": ([_I, _I, _I, _I], _SZ),
+    "ela_rve_create": ([_P, _I, _I, _I, ctypes.POINTER(_P)], _I),
+    "ela_rve_destroy": ([_P], _I),
+    "ela_rve_bytes_in_use": ([_P], _SZ),
+    "ela_rve_levels": ([_P], _I),
+    "ela_rve_set_bc_types": ([_P, _P], _I),
+    "ela_rve_set_uniform": ([_P, _I], _I),
+    "ela_rve_set_voronoi_cubic": ([_P, _I, _P, _I, _P, _D, _D, _D], _I),
+    "ela_rve_set_crack": ([_P, _P, _P, _P, _D, _D, _D, _D], _I),
+    "ela_rve_set_rhs_affine": ([_P, _P], _I),
+    "ela_rve_upload": ([_P, _I, _P], _I),
+    "ela_rve_download": ([_P, _I, _P], _I),
+    "ela_rve_grain_ids": ([_P, _P], _I),
+    "ela_batch_prepare": ([_P, _I, _P], _I),
+    "ela_batch_fapply": ([_P, _I, _P, _I], _I),
+    "ela_batch_submit": ([_P, _P, _D, _D, _I], _I),
+    "ela_batch_step": ([_P, ctypes.POINTER(_I)], _I),
+    "ela_batch_solve": ([_P, _I, _P, _D, _D, _I], _I),
+    "ela_rve_status": ([_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(Stats)], _I),
+    "ela_batch_set_timing": ([_P, _I], _I),
+    "ela_batch_smoother_times": ([_P, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_I)], _I),
 }
 
 
@@ -208,7 +237,7 @@ class Elastic:
     mode: SOLVE_FP64, SOLVE_MIXED or ALLOC_BOTH (which MG precisions to allocate).
     """
 
-    def __init__(self, n, kind, L=None, mode=ALLOC_BOTH, max_mg_levels=0, ctx=None):
+    def __init__(self, n, kind, L=None, mode=ALLOC_BOTH, max_mg_levels=0, ctx=None, coarse_min=4):
         self.ctx = ctx or default_context()
         n = tuple(int(v) for v in n)
         self.dim = len(n)
@@ -219,7 +248,8 @@ class Elastic:
             d.n[q] = n[q] if q < self.dim else 0
             d.dx[q] = float(L[q]) / n[q] if q < self.dim else 1.0
         p = ctypes.c_void_p()
-        _check(lib().ela_hier_create(self.ctx.ptr, ctypes.byref(d), kind, max_mg_levels, mode, ctypes.byref(p)))
+        _check(lib().ela_hier_create_ex(self.ctx.ptr, ctypes.byref(d), kind, max_mg_levels, coarse_min, mode,
+                                        ctypes.byref(p)))
         self.ptr = p
         self.kind, self.n, self.mode = kind, n, mode
         self.nc = ncomp(self.dim, kind)
@@ -401,3 +431,145 @@ class Linear:
         if rc != 0 and (raise_on_fail or rc not in (4,)):
             _check(rc)
         return u, self.last
+
+
+# ---------------------------------------------------------------- batched independent solves
+def _ptr_array(objs):
+    arr = (ctypes.c_void_p * len(objs))(*[o.ptr for o in objs])
+    return arr
+
+
+class RVE:
+    """One member of a :class:`Batch`: a cube of n cells on [0,1]^3 (cubic
+    Voronoi polycrystal), its hierarchy resident on the device.
+
+    Dense host fields are ``(3, n+1, n+1, n+1)`` float64 arrays (x fastest)."""
+
+    U, B, R, FACTOR = 0, 1, 2, 3
+
+    def __init__(self, batch, n, max_grains=64, with_factor=False):
+        self.batch = batch
+        p = ctypes.c_void_p()
+        _check(lib().ela_rve_create(batch.ptr, int(n), int(max_grains), int(bool(with_factor)), ctypes.byref(p)))
+        self.ptr = p
+        self.n = int(n)
+        batch._members.append(self)
+
+    @property
+    def shape(self):
+        return (3, self.n + 1, self.n + 1, self.n + 1)
+
+    def levels(self):
+        return lib().ela_rve_levels(self.ptr)
+
+    def bytes(self):
+        return lib().ela_rve_bytes_in_use(self.ptr)
+
+    def SetBC(self, types):
+        t = np.ascontiguousarray(types, dtype=np.int32).reshape(26, 3)
+        _check(lib().ela_rve_set_bc_types(self.ptr, _ptr(t)))
+
+    def SetUniform(self, uniform):
+        _check(lib().ela_rve_set_uniform(self.ptr, int(bool(uniform))))
+
+    def SetModelVoronoiCubic(self, seeds_q, q, R, C11, C12, C44):
+        sd = np.ascontiguousarray(seeds_q, dtype=np.int64)
+        Rm = np.ascontiguousarray(R, dtype=np.float64)
+        _check(lib().ela_rve_set_voronoi_cubic(self.ptr, sd.shape[0], _ptr(sd), int(q), _ptr(Rm), C11, C12, C44))
+
+    def SetCrack(self, p, nrm, tv, width, a, b, c):
+        pa, na, ta = (np.ascontiguousarray(v, dtype=np.float64) for v in (p, nrm, tv))
+        _check(lib().ela_rve_set_crack(self.ptr, _ptr(pa), _ptr(na), _ptr(ta), width, a, b, c))
+
+    def SetRhsAffine(self, G):
+        g = np.ascontiguousarray(G, dtype=np.float64).reshape(9)
+        _check(lib().ela_rve_set_rhs_affine(self.ptr, _ptr(g)))
+
+    def upload(self, field, host):
+        h = np.ascontiguousarray(host, dtype=np.float64).reshape(self.shape)
+        _check(lib().ela_rve_upload(self.ptr, field, _ptr(h)))
+
+    def download(self, field):
+        shape = self.shape[1:] if field == self.FACTOR else self.shape
+        out = np.empty(shape)
+        _check(lib().ela_rve_download(self.ptr, field, _ptr(out)))
+        return out
+
+    def grain_ids(self):
+        out = np.empty(self.shape[1:], dtype=np.uint16)
+        _check(lib().ela_rve_grain_ids(self.ptr, _ptr(out)))
+        return out
+
+    def status(self):
+        st, code = ctypes.c_int(), ctypes.c_int()
+        s = Stats()
+        _check(lib().ela_rve_status(self.ptr, ctypes.byref(st), ctypes.byref(code), ctypes.byref(s)))
+        d = s.as_dict()
+        d.update({"state": st.value, "status": code.value})
+        return d
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().ela_rve_destroy(self.ptr)
+            self.ptr = None
+            if self in self.batch._members:
+                self.batch._members.remove(self)
+
+
+class Batch:
+    """Batched independent solves (mode SOLVE_MIXED or SOLVE_FP64)."""
+
+    def __init__(self, mode=SOLVE_MIXED, ctx=None):
+        self.ctx = ctx or default_context()
+        p = ctypes.c_void_p()
+        _check(lib().ela_batch_create(self.ctx.ptr, mode, ctypes.byref(p)))
+        self.ptr = p
+        self.mode = mode
+        self._members = []
+
+    @staticmethod
+    def member_bytes(n, mode=SOLVE_MIXED, max_grains=64, with_factor=False):
+        return lib().ela_rve_bytes(int(n), mode, int(max_grains), int(bool(with_factor)))
+
+    def rve(self, n, max_grains=64, with_factor=False):
+        return RVE(self, n, max_grains, with_factor)
+
+    def prepare(self, rves):
+        _check(lib().ela_batch_prepare(self.ptr, len(rves), _ptr_array(rves)))
+
+    def fapply(self, rves, prec=F64):
+        _check(lib().ela_batch_fapply(self.ptr, len(rves), _ptr_array(rves), prec))
+
+    def submit(self, rve, tol_rel=1e-10, tol_abs=0.0, max_cycles=100):
+        _check(lib().ela_batch_submit(self.ptr, rve.ptr, tol_rel, tol_abs, max_cycles))
+
+    def step(self):
+        n = ctypes.c_int()
+        _check(lib().ela_batch_step(self.ptr, ctypes.byref(n)))
+        return n.value
+
+    def solve(self, rves, tol_rel=1e-10, tol_abs=0.0, max_cycles=100, raise_on_fail=True):
+        rc = lib().ela_batch_solve(self.ptr, len(rves), _ptr_array(rves), tol_rel, tol_abs, max_cycles)
+        if rc != 0 and (raise_on_fail or rc != 4):
+            _check(rc)
+        return [r.status() for r in rves]
+
+    def set_timing(self, on=True):
+        _check(lib().ela_batch_set_timing(self.ptr, int(bool(on))))
+
+    def smoother_times(self):
+        ms, nn, nl = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_int()
+        _check(lib().ela_batch_smoother_times(self.ptr, ctypes.byref(ms), ctypes.byref(nn), ctypes.byref(nl)))
+        return {"ms": ms.value, "nodes": nn.value, "launches": nl.value}
+
+    def bytes(self):
+        return lib().ela_batch_bytes_in_use(self.ptr)
+
+    def close(self):
+        """Destroy the batch and every member still in it."""
+        if getattr(self, "ptr", None):
+            lib().ela_batch_destroy(self.ptr)
+            self.ptr = None
+            for r in self._members:
+                r.ptr = None
+            self._members = []

@@ -1,0 +1,1545 @@
+// ela_batch.hip -- many independent box problems solved together on one GPU.
+//
+// A member (ela_rve) is a cube of n cells per side on [0,1]^3: a cubic
+// Voronoi polycrystal (grain table, optionally times a per-node degradation
+// factor), its own boundary-condition types and its own multigrid hierarchy,
+// resident on the device from ela_rve_create to ela_rve_destroy.  Every
+// kernel of a cycle is ONE launch over the boxes of all members at a level:
+// a launch carries an array of box descriptors (BoxDesc) and a block -> box
+// map, and each 256-thread block works on 256 consecutive nodes of its box.
+//
+// Levels are aligned at the coarsest grid: a member of depth D occupies the
+// global levels G-D .. G-1 (G = deepest member), so every member's coarsest
+// level is global level G-1 and one F-cycle schedule serves all of them.  In
+// the schedule, the F-cycle step at global level g (a V-cycle from g) runs on
+// the members whose finest level is at or above g; with the members ordered
+// by their finest global level, those are a prefix of every level's list.
+//
+// Per member the arithmetic is that of the single-box solver (ela_solver.hip):
+// FP64 outer iteration r = b - A u, u += cycle(r) with the right-hand side
+// scaled by a power of two, F-cycles with nu1 = nu2 = 2 calls of 2 weighted
+// Jacobi sweeps (omega = 2/3), full weighting, trilinear prolongation and a
+// dense inverse on the coarsest level.  A member's results depend only on its
+// own data (no cross-member reductions), and it leaves the active set the
+// cycle it converges, so a solve gives the same bits alone and in any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ela.h"
+#include "ela_internal.hpp"
+#include "ela_kernels.hpp"
+
+namespace ela {
+namespace {
+
+constexpr int TB = 256;                // nodes per block
+constexpr int kMaxBatchDense = 1536;   // unknowns of a coarsest level (n_c <= 7)
+constexpr int kNC = 21;                // cubic stiffness: major/minor symmetric storage
+enum { F_RES = 0, F_RC, F_COR, F_TMP, F_HOLD, F_N };
+
+// One box of one launch.  Pointers of the precision that a kernel does not use may be null.
+struct BoxDesc {
+    Lay L;            // compact: px = nx, pl = nx * ny, cs = nodes
+    int tile0;        // first block of this box in launches over its list
+    int nodes;
+    int nunk;         // coarsest level: unknowns of the dense inverse
+    int uniform;      // SetUniform: drop the grad C term (callers use 0)
+    Inv<double> i64;
+    Inv<float> i32;
+    double h[3];
+    BCTab bc;
+    const unsigned short* gid;  // grain form (finest level) if non-null: s[o] * tab[gid[o]]
+    const double* tab64;
+    const float* tab32;
+    const double* sc64;  // per-node factor or null
+    const float* sc32;
+    const double* C64;  // planes (coarse levels)
+    const float* C32;
+    double* d64;  // Jacobi diagonal in the cycle precision (one of d64 / d32)
+    float* d32;
+    void* f[F_N];  // cycle fields, cycle precision
+    double* u64;   // finest level: solution, right-hand side, residual / test output
+    double* b64;
+    double* r64;
+    double* M;      // coarsest level: A^-1, nunk x nunk row-major, dense unknown order
+    double* norm;   // member scalars on the device: |r|inf of the last residual, rhs scale
+    double* scale;
+    const double* E;  // set-up: identity of size nunk (columns = unit fields)
+    double* W;        // set-up: [A | I] workspace, nunk x 2 nunk
+};
+
+template <typename T>
+struct P;
+template <>
+struct P<double> {
+    __device__ static const double* tab(const BoxDesc& d) { return d.tab64; }
+    __device__ static const double* sc(const BoxDesc& d) { return d.sc64; }
+    __device__ static const double* C(const BoxDesc& d) { return d.C64; }
+    __device__ static const double* dg(const BoxDesc& d) { return d.d64; }
+    __device__ static const Inv<double>& inv(const BoxDesc& d) { return d.i64; }
+};
+template <>
+struct P<float> {
+    __device__ static const float* tab(const BoxDesc& d) { return d.tab32; }
+    __device__ static const float* sc(const BoxDesc& d) { return d.sc32; }
+    __device__ static const float* C(const BoxDesc& d) { return d.C32; }
+    __device__ static const float* dg(const BoxDesc& d) { return d.d32; }
+    __device__ static const Inv<float>& inv(const BoxDesc& d) { return d.i32; }
+};
+
+// Node of this thread in its box (compact layout: offset = dense index); blk
+// is the block's index in the list (launches over a sub-range add an offset).
+__device__ __forceinline__ bool box_node(const BoxDesc& d, const Lay& L, int blk, int& i, int& j, int& k, long long& o) {
+    const int q = (blk - d.tile0) * TB + (int)threadIdx.x;
+    if (q >= d.nodes) return false;
+    const int nxy = L.nx * L.ny;
+    k = q / nxy;
+    const int r = q - k * nxy;
+    j = r / L.nx;
+    i = r - j * L.nx;
+    o = q;
+    return true;
+}
+
+// f = A u at (i,j,k) in precision T (cubic).
+// GF: the box's stiffness is in grain form (1) or planes (0); -1 decides at
+// run time (set-up kernels only: one stencil body per kernel keeps the
+// register count of the cycle kernels at that of a single form).
+template <typename T, int GF>
+__device__ __forceinline__ void box_apply(const BoxDesc& d, const Lay& L, const T* __restrict__ u, int i, int j, int k,
+                                          T f[3]) {
+    if (GF == 1 || (GF < 0 && d.gid))
+        node_apply_acc<T, K_MM, 3>(L, GrainAcc<T, kNC>{d.gid, P<T>::tab(d), P<T>::sc(d)}, u, i, j, k, d.bc, P<T>::inv(d),
+                                   d.uniform != 0, f);
+    else
+        node_apply_acc<T, K_MM, 3>(L, PlaneAcc<T>{P<T>::C(d), L.cs}, u, i, j, k, d.bc, P<T>::inv(d), d.uniform != 0, f);
+}
+
+// One weighted-Jacobi sweep out = in + omega (rhs - A in) / D; in < 0: out = omega rhs / D.
+template <typename T, int GF>
+__global__ __launch_bounds__(TB) void kb_smooth(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, int blk0,
+                                                int rhs, int in, int out, T omega) {
+    const int blk = blk0 + blockIdx.x;
+    const BoxDesc& d = D[bmap[blk]];
+    const Lay L = d.L;
+    int i, j, k;
+    long long o;
+    if (!box_node(d, L, blk, i, j, k, o)) return;
+    const T* b = (const T*)d.f[rhs];
+    const T* dg = P<T>::dg(d);
+    T* xo = (T*)d.f[out];
+    if (in < 0) {
+#pragma unroll
+        for (int p = 0; p < 3; ++p) xo[p * L.cs + o] = omega * b[p * L.cs + o] / dg[p * L.cs + o];
+        return;
+    }
+    const T* xi = (const T*)d.f[in];
+    T f[3];
+    box_apply<T, GF>(d, L, xi, i, j, k, f);
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const long long q = p * L.cs + o;
+        xo[q] = xi[q] + omega * (b[q] - f[p]) / dg[q];
+    }
+}
+
+// r = b - A x (cycle precision).
+template <typename T, int GF>
+__global__ __launch_bounds__(TB) void kb_resid(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, int blk0,
+                                               int x, int b, int r) {
+    const int blk = blk0 + blockIdx.x;
+    const BoxDesc& d = D[bmap[blk]];
+    const Lay L = d.L;
+    int i, j, k;
+    long long o;
+    if (!box_node(d, L, blk, i, j, k, o)) return;
+    const T* bb = (const T*)d.f[b];
+    T* rr = (T*)d.f[r];
+    T f[3];
+    box_apply<T, GF>(d, L, (const T*)d.f[x], i, j, k, f);
+#pragma unroll
+    for (int p = 0; p < 3; ++p) rr[p * L.cs + o] = bb[p * L.cs + o] - f[p];
+}
+
+// Block max of a non-negative value (NaN propagates) into partials[blockIdx.x].
+__device__ __forceinline__ void block_max(double* partials, double v) {
+    __shared__ double wmax[TB / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = (o > v || o != o) ? o : v;
+    }
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = wmax[0];
+        for (int w = 1; w < TB / 64; ++w) m = (wmax[w] > m || wmax[w] != wmax[w]) ? wmax[w] : m;
+        partials[blockIdx.x] = m;
+    }
+}
+
+// FP64 solution residual of the finest level: r = b - A u; f[res] = r * scale
+// (the next cycle's right-hand side) and the block maxima of |r|.  Every lane
+// reaches the block reduction.
+template <typename T>
+__global__ __launch_bounds__(TB) void kb_resid64(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, int res,
+                                                 double* __restrict__ partials) {
+    const BoxDesc& d = D[bmap[blockIdx.x]];
+    const Lay L = d.L;
+    int i, j, k;
+    long long o;
+    double m = 0.0;
+    if (box_node(d, L, blockIdx.x, i, j, k, o)) {
+        double f[3];
+        box_apply<double, 1>(d, L, d.u64, i, j, k, f);
+        const double s = *d.scale;
+        T* rs = (T*)d.f[res];
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const long long q = p * L.cs + o;
+            const double v = d.b64[q] - f[p];
+            rs[q] = (T)(v * s);
+            const double a = fabs(v);
+            m = (a > m || a != a) ? a : m;
+        }
+    }
+    block_max(partials, m);
+}
+
+// |r|inf of each box from its blocks' partials (fixed order): *norm and out[box].
+__global__ __launch_bounds__(64) void kb_fold(const BoxDesc* __restrict__ D, const double* __restrict__ partials,
+                                              double* __restrict__ out) {
+    const BoxDesc& d = D[blockIdx.x];
+    const int nt = (d.nodes + TB - 1) / TB;
+    double m = 0.0;
+    for (int q = threadIdx.x; q < nt; q += 64) {
+        const double v = partials[d.tile0 + q];
+        m = (v > m || v != v) ? v : m;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off, 64);
+        m = (o > m || o != o) ? o : m;
+    }
+    if (threadIdx.x == 0) {
+        *d.norm = m;
+        out[blockIdx.x] = m;
+    }
+}
+
+// scale = 2^-e with 2^e the power of two at or above |r|inf (exact in any precision).
+__global__ void kb_scale(const BoxDesc* __restrict__ D, int nbox) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    const double r = *D[b].norm;
+    int e = 0;
+    if (r > 0.0 && r < 1e300) frexp(r, &e);
+    *D[b].scale = ldexp(1.0, -e);
+}
+
+// Full weighting of f[src] (fine) into f[dst] (coarse); launched over the coarse list.
+template <typename T>
+__global__ __launch_bounds__(TB) void kb_restrict(const BoxDesc* __restrict__ Dc, const BoxDesc* __restrict__ Df,
+                                                  const int* __restrict__ bmap, int src, int dst) {
+    const int box = bmap[blockIdx.x];
+    const BoxDesc& c = Dc[box];
+    const BoxDesc& f = Df[box];
+    const Lay Lc = c.L, Lf = f.L;
+    int I, J, K;
+    long long oc;
+    if (!box_node(c, Lc, blockIdx.x, I, J, K, oc)) return;
+    const T* F = (const T*)f.f[src];
+    T* out = (T*)c.f[dst];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const T* Fp = F + p * Lf.cs;
+        out[p * Lc.cs + oc] = restrict_node<T, 3>(Lc, Lf, I, J, K, [Fp](long long o) { return Fp[o]; });
+    }
+}
+
+// f[dst] (fine) (+)= P f[src] (coarse); launched over the fine list.
+template <typename T>
+__global__ __launch_bounds__(TB) void kb_prolong(const BoxDesc* __restrict__ Df, const BoxDesc* __restrict__ Dc,
+                                                 const int* __restrict__ bmap, int src, int dst, int add) {
+    const int box = bmap[blockIdx.x];
+    const BoxDesc& f = Df[box];
+    const BoxDesc& c = Dc[box];
+    const Lay Lf = f.L, Lc = c.L;
+    int i, j, k;
+    long long o;
+    if (!box_node(f, Lf, blockIdx.x, i, j, k, o)) return;
+    const long long oc = (long long)(i >> 1) + Lc.px * (j >> 1) + Lc.pl * (k >> 1);
+    const T* C = (const T*)c.f[src];
+    T* out = (T*)f.f[dst];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const T v = prolong_node<T>(Lc, i, j, k, C + p * Lc.cs + oc);
+        T* y = out + p * Lf.cs + o;
+        *y = add ? (*y + v) : v;
+    }
+}
+
+// f[y] += f[x]
+template <typename T>
+__global__ __launch_bounds__(TB) void kb_axpy(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, int x, int y) {
+    const BoxDesc& d = D[bmap[blockIdx.x]];
+    const int q = ((int)blockIdx.x - d.tile0) * TB + (int)threadIdx.x;
+    if (q >= d.nodes) return;
+    const T* xs = (const T*)d.f[x];
+    T* ys = (T*)d.f[y];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) ys[p * d.L.cs + q] += xs[p * d.L.cs + q];
+}
+
+// u64 += f[src] / scale (the refinement update, FP64).
+template <typename T>
+__global__ __launch_bounds__(TB) void kb_update(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, int src) {
+    const BoxDesc& d = D[bmap[blockIdx.x]];
+    const int q = ((int)blockIdx.x - d.tile0) * TB + (int)threadIdx.x;
+    if (q >= d.nodes) return;
+    const T* c = (const T*)d.f[src];
+    const double s = 1.0 / *d.scale;  // a power of two: exact
+#pragma unroll
+    for (int p = 0; p < 3; ++p) d.u64[p * d.L.cs + q] += (double)c[p * d.L.cs + q] * s;
+}
+
+// Coarsest level: f[dst] = A^-1 f[src].  Grid (row blocks, boxes); each block
+// gathers the right-hand side into LDS and each wavefront computes whole rows.
+template <typename T>
+__global__ __launch_bounds__(TB) void kb_bottom(const BoxDesc* __restrict__ D, int src, int dst) {
+    __shared__ double sb[kMaxBatchDense];
+    const BoxDesc& d = D[blockIdx.y];
+    const int n = d.nunk;
+    const int rows_per_block = 4 * (TB / 64);
+    const int r0 = blockIdx.x * rows_per_block;
+    if (r0 >= n) return;
+    const T* b = (const T*)d.f[src];
+    for (int q = threadIdx.x; q < n; q += TB) sb[q] = (double)b[q];  // compact layout: dense unknown order
+    __syncthreads();
+    T* x = (T*)d.f[dst];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int r = r0 + w; r < min(n, r0 + rows_per_block); r += TB / 64) {
+        const double* row = d.M + (long long)r * n;
+        double acc = 0.0;
+        for (int q = lane; q < n; q += 64) acc += row[q] * sb[q];
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (lane == 0) x[r] = (T)acc;
+    }
+}
+
+// f[dst] = A f[src] (cycle precision), and the FP64 finest-level r64 = A u64.
+template <typename T>
+__global__ __launch_bounds__(TB) void kb_apply(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, int src,
+                                               int dst) {
+    const BoxDesc& d = D[bmap[blockIdx.x]];
+    const Lay L = d.L;
+    int i, j, k;
+    long long o;
+    if (!box_node(d, L, blockIdx.x, i, j, k, o)) return;
+    T f[3];
+    box_apply<T, 1>(d, L, (const T*)d.f[src], i, j, k, f);
+    T* out = (T*)d.f[dst];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) out[p * L.cs + o] = f[p];
+}
+
+__global__ __launch_bounds__(TB) void kb_apply64(const BoxDesc* __restrict__ D, const int* __restrict__ bmap) {
+    const BoxDesc& d = D[bmap[blockIdx.x]];
+    const Lay L = d.L;
+    int i, j, k;
+    long long o;
+    if (!box_node(d, L, blockIdx.x, i, j, k, o)) return;
+    double f[3];
+    box_apply<double, 1>(d, L, d.u64, i, j, k, f);
+#pragma unroll
+    for (int p = 0; p < 3; ++p) d.r64[p * L.cs + o] = f[p];
+}
+
+// f[dst] = (T) u64 and r64 = (double) f[src] (test conversions on the finest level).
+template <typename T>
+__global__ __launch_bounds__(TB) void kb_cvt(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, int dst,
+                                             int src) {
+    const BoxDesc& d = D[bmap[blockIdx.x]];
+    const int q = ((int)blockIdx.x - d.tile0) * TB + (int)threadIdx.x;
+    if (q >= d.nodes) return;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const long long e = p * d.L.cs + q;
+        if (dst >= 0) ((T*)d.f[dst])[e] = (T)d.u64[e];
+        if (src >= 0) d.r64[e] = (double)((const T*)d.f[src])[e];
+    }
+}
+
+// ------------------------------------------------------------------ set-up
+// Stiffness of a coarse level: the full-weighting average of the fine level's
+// (Elastic.cpp:722-809), from the grain form or from planes; launched over the coarse list.
+__global__ __launch_bounds__(TB) void kb_coef(const BoxDesc* __restrict__ Dc, const BoxDesc* __restrict__ Df,
+                                              const int* __restrict__ bmap) {
+    const int box = bmap[blockIdx.x];
+    const BoxDesc& c = Dc[box];
+    const BoxDesc& f = Df[box];
+    const Lay Lc = c.L, Lf = f.L;
+    int I, J, K;
+    long long oc;
+    if (!box_node(c, Lc, blockIdx.x, I, J, K, oc)) return;
+    double* out = (double*)c.C64;
+    if (f.gid) {
+        const GrainAcc<double, kNC> A{f.gid, f.tab64, f.sc64};
+        for (int q = 0; q < kNC; ++q)
+            out[q * Lc.cs + oc] = restrict_node<double, 3>(Lc, Lf, I, J, K, [&A, q](long long o) { return A.rec(o)[q]; });
+    } else {
+        for (int q = 0; q < kNC; ++q) {
+            const double* F = f.C64 + q * Lf.cs;
+            out[q * Lc.cs + oc] = restrict_node<double, 3>(Lc, Lf, I, J, K, [F](long long o) { return F[o]; });
+        }
+    }
+}
+
+// C32 = (float) C64 on levels stored as planes.
+__global__ __launch_bounds__(TB) void kb_planes32(const BoxDesc* __restrict__ D, const int* __restrict__ bmap) {
+    const BoxDesc& d = D[bmap[blockIdx.x]];
+    const int q = ((int)blockIdx.x - d.tile0) * TB + (int)threadIdx.x;
+    if (q >= d.nodes || d.gid || !d.C32) return;
+    for (int c = 0; c < kNC; ++c) ((float*)d.C32)[c * d.L.cs + q] = (float)d.C64[c * d.L.cs + q];
+}
+
+// Jacobi diagonal (Elastic.cpp:264-352) from the FP64 stiffness, stored in the cycle precision.
+__global__ __launch_bounds__(TB) void kb_diag(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, int* flag) {
+    const BoxDesc& d = D[bmap[blockIdx.x]];
+    const Lay L = d.L;
+    int i, j, k;
+    long long o;
+    if (!box_node(d, L, blockIdx.x, i, j, k, o)) return;
+    double dg[3];
+    if (d.gid) node_diag_acc<K_MM, 3>(L, GrainAcc<double, kNC>{d.gid, d.tab64, d.sc64}, i, j, k, d.bc, d.h, dg);
+    else node_diag_acc<K_MM, 3>(L, PlaneAcc<double>{d.C64, L.cs}, i, j, k, d.bc, d.h, dg);
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        if (d.d64) d.d64[p * L.cs + o] = dg[p];
+        if (d.d32) d.d32[p * L.cs + o] = (float)dg[p];
+        if (!(dg[p] == dg[p]) || dg[p] == 0.0) *flag = 1;
+    }
+}
+
+// Identity of size n (row-major): the unit fields of a coarsest layout.
+__global__ void kb_identity(double* E, int n) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < (long long)n * n) E[t] = (t / n == t % n) ? 1.0 : 0.0;
+}
+
+// Dense coarsest operator by probing: block (column q, node block) of box y
+// applies A to the unit field e_q and writes W[r][q] = (A e_q)_r, plus the
+// identity in W's right half.  Grid (maxnunk * nb, boxes).
+__global__ __launch_bounds__(TB) void kb_probe(const BoxDesc* __restrict__ D, int nb) {
+    const BoxDesc& d = D[blockIdx.y];
+    const int q = blockIdx.x / nb, t = (blockIdx.x % nb) * TB + threadIdx.x;
+    if (q >= d.nunk || t >= d.nodes) return;
+    const Lay L = d.L;
+    const int nxy = L.nx * L.ny;
+    const int k = t / nxy, j = (t - k * nxy) / L.nx, i = t - k * nxy - j * L.nx;
+    double f[3];
+    box_apply<double, -1>(d, L, d.E + (long long)q * d.nunk, i, j, k, f);
+    const long long w = 2LL * d.nunk;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const long long r = (long long)p * d.nodes + t;
+        d.W[r * w + q] = f[p];
+        if (r == q) d.W[r * w + d.nunk + r] = 1.0;
+    }
+}
+
+__global__ __launch_bounds__(1024) void kb_gj_pivot(const BoxDesc* __restrict__ D, int c, int* flag) {
+    __shared__ double sval[1024];
+    __shared__ int sidx[1024];
+    const BoxDesc& d = D[blockIdx.x];
+    if (c >= d.nunk) return;
+    gj_pivot_block(d.W, d.nunk, c, flag, sval, sidx);
+}
+
+__global__ __launch_bounds__(256) void kb_gj_eliminate(const BoxDesc* __restrict__ D, int c) {
+    const BoxDesc& d = D[blockIdx.y];
+    if (c >= d.nunk || (int)blockIdx.x >= d.nunk) return;
+    gj_eliminate_row(d.W, d.nunk, c, blockIdx.x);
+}
+
+// M = right half of W.  Grid (maxnunk rows, boxes).
+__global__ __launch_bounds__(256) void kb_gj_extract(const BoxDesc* __restrict__ D) {
+    const BoxDesc& d = D[blockIdx.y];
+    const int r = blockIdx.x, n = d.nunk;
+    if (r >= n) return;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) d.M[(long long)r * n + q] = d.W[(long long)r * 2 * n + n + q];
+}
+
+// Per-member model of the finest level: nearest Voronoi seed (exact int64
+// distances, ties to the lowest index) and, with a crack, the degradation
+// factor s = a eta^2 + b eta + c with eta = (1 - tanh(dist / width)) / 2, dist
+// the distance to the half-plane {x : nrm.(x-p) = 0, tv.(x-p) <= 0}.
+struct Crack {
+    double p[3], nrm[3], tv[3];
+    double width, a, b, c;
+};
+__global__ __launch_bounds__(TB) void kb_model(Lay L, double h, int nseeds, const long long* __restrict__ seeds, int qs,
+                                               unsigned short* __restrict__ gid, int with_crack, Crack cr,
+                                               double* __restrict__ sc64, float* __restrict__ sc32) {
+    const int t = blockIdx.x * TB + threadIdx.x;
+    const int nodes = L.nx * L.ny * L.nz;
+    if (t >= nodes) return;
+    const int nxy = L.nx * L.ny;
+    const int k = t / nxy, j = (t - k * nxy) / L.nx, i = t - k * nxy - j * L.nx;
+    if (gid) {
+        const long long x = (long long)i * qs, y = (long long)j * qs, z = (long long)k * qs;
+        long long best = -1;
+        int bi = 0;
+        for (int s = 0; s < nseeds; ++s) {
+            const long long dx = x - seeds[3 * s], dy = y - seeds[3 * s + 1], dz = z - seeds[3 * s + 2];
+            const long long dd = dx * dx + dy * dy + dz * dz;
+            if (best < 0 || dd < best) { best = dd; bi = s; }
+        }
+        gid[t] = (unsigned short)bi;
+    }
+    if (with_crack && sc64) {
+        const double X[3] = {i * h, j * h, k * h};
+        double dn = 0, dt = 0;
+        for (int a = 0; a < 3; ++a) {
+            dn += cr.nrm[a] * (X[a] - cr.p[a]);
+            dt += cr.tv[a] * (X[a] - cr.p[a]);
+        }
+        const double dist = sqrt(dn * dn + (dt > 0 ? dt * dt : 0.0));
+        const double eta = 0.5 * (1.0 - tanh(dist / cr.width));
+        const double s = (cr.a * eta + cr.b) * eta + cr.c;
+        sc64[t] = s;
+        if (sc32) sc32[t] = (float)s;
+    }
+}
+
+// Right-hand side of an affine displacement u = G x on the Displacement rows
+// of boundary nodes (BC::Operator::Elastic::Init with a field value), 0 on
+// every other row (interior rows carry no body force, traction rows 0).
+__global__ __launch_bounds__(TB) void kb_rhs_affine(Lay L, double h, BCTab bc, double g00, double g01, double g02,
+                                                    double g10, double g11, double g12, double g20, double g21,
+                                                    double g22, double* __restrict__ b) {
+    const int t = blockIdx.x * TB + threadIdx.x;
+    const int nodes = L.nx * L.ny * L.nz;
+    if (t >= nodes) return;
+    const int nxy = L.nx * L.ny;
+    const int k = t / nxy, j = (t - k * nxy) / L.nx, i = t - k * nxy - j * L.nx;
+    const int face = face_of<3>(i, j, k, L.nx - 1, L.ny - 1, L.nz - 1);
+    const double x = i * h, y = j * h, z = k * h;
+    const double v[3] = {g00 * x + g01 * y + g02 * z, g10 * x + g11 * y + g12 * z, g20 * x + g21 * y + g22 * z};
+    for (int p = 0; p < 3; ++p) b[p * L.cs + t] = (face >= 0 && bc.t[face][p] == BC_DISP) ? v[p] : 0.0;
+}
+
+}  // namespace
+}  // namespace ela
+
+using namespace ela;
+
+// ====================================================================== host side
+namespace {
+
+struct RLev {
+    Lay L{};
+    int n = 0;
+    int nodes = 0;
+    double h = 1;
+    double* C64 = nullptr;
+    float* C32 = nullptr;
+    double* d64 = nullptr;
+    float* d32 = nullptr;
+    void* f[F_N] = {};
+};
+
+enum { ST_IDLE = 0, ST_QUEUED = 1, ST_ACTIVE = 2, ST_DONE = 3 };
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct ela_rve {
+    ela_batch* B = nullptr;
+    int n = 0, depth = 0, max_grains = 0, ngrains = 0, nunk = 0;
+    int uniform = 0;
+    bool factor = false, model = false, prepared = false;
+    std::vector<RLev> lev;
+    BCTab bc{};
+    unsigned short* gid = nullptr;
+    double *tab64 = nullptr, *sc64 = nullptr;
+    float *tab32 = nullptr, *sc32 = nullptr;
+    long long* seeds = nullptr;  // device copy of the seeds (model evaluation)
+    double *u64 = nullptr, *b64 = nullptr, *r64 = nullptr, *M = nullptr, *scal = nullptr;
+    void* block = nullptr;
+    size_t bytes = 0;
+    // solve state
+    int state = ST_IDLE, status = ELA_OK, cycles = 0, max_cycles = 0;
+    double tol_rel = 0, tol_abs = 0, res0 = 0, res = 0, target = 0, best = 0;
+    int stall = 0;
+    long long seq = 0;
+    double t_submit = 0, t_done = 0;
+};
+
+struct ela_batch {
+    ela_ctx* ctx = nullptr;
+    int mode = ELA_SOLVE_MIXED;
+    size_t tsize = 4;  // bytes of the cycle precision
+    hipStream_t stream = nullptr;
+    long long seq = 0;
+    std::vector<ela_rve*> members;  // every live member
+    std::vector<ela_rve*> active;   // in the schedule, ordered by (finest global level, seq)
+    std::vector<ela_rve*> queued;   // submitted, join at the next step
+    bool dirty = true;
+    // launch lists of the active set
+    int G = 0;                               // global levels
+    std::vector<int> cnt;                    // cnt[g]: members whose finest global level <= g
+    std::vector<std::vector<int>> tiles;     // tiles[g][m]: first block of the m-th box of level g (size cnt+1)
+    std::vector<size_t> off_desc, off_bmap;  // device offsets of level lists (bytes)
+    size_t off_top_desc = 0, off_top_bmap = 0;
+    std::vector<int> top_tiles;
+    char* dlists = nullptr;
+    size_t dlists_cap = 0;
+    std::vector<char> hlists;
+    // scratch
+    double* partials = nullptr;
+    size_t partials_cap = 0;
+    double* dnorms = nullptr;
+    double* hnorms = nullptr;  // pinned
+    size_t norms_cap = 0;
+    int* dflag = nullptr;
+    // timing of the smoother launches (HIP events on the batch stream)
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    std::vector<std::pair<int, long long>> ev_launch;  // event pair index, nodes of the launch
+    double ms_smooth = 0;
+    long long smooth_nodes = 0;
+    int n_smooth = 0;
+};
+
+namespace {
+
+int fail(int code, const std::string& msg) { return set_error(code, msg); }
+
+// Levels of a cube of n cells: coarsened while n is even and n / 2 >= 2.
+int depth_of(int n) {
+    int d = 1;
+    while (n % 2 == 0 && n / 2 >= 2) { n /= 2; ++d; }
+    return d;
+}
+int coarsest_of(int n) {
+    while (n % 2 == 0 && n / 2 >= 2) n /= 2;
+    return n;
+}
+
+size_t up256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Byte layout of a member's single allocation; with base == nullptr only sizes.
+size_t layout_member(ela_rve* r, char* base) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> char* {
+        char* p = base ? base + off : nullptr;
+        off += up256(bytes);
+        return p;
+    };
+    const size_t ts = r->B->tsize;
+    const bool mixed = r->B->mode == ELA_SOLVE_MIXED;
+    for (int l = 0; l < r->depth; ++l) {
+        RLev& v = r->lev[l];
+        const size_t N = (size_t)v.nodes;
+        if (l == 0) {
+            r->gid = (unsigned short*)take(N * 2);
+            r->tab64 = (double*)take((size_t)r->max_grains * kNC * 8);
+            if (mixed) r->tab32 = (float*)take((size_t)r->max_grains * kNC * 4);
+            r->seeds = (long long*)take((size_t)r->max_grains * 3 * 8);
+            if (r->factor) {
+                r->sc64 = (double*)take(N * 8);
+                if (mixed) r->sc32 = (float*)take(N * 4);
+            }
+            r->u64 = (double*)take(3 * N * 8);
+            r->b64 = (double*)take(3 * N * 8);
+            r->r64 = (double*)take(3 * N * 8);
+        } else {
+            v.C64 = (double*)take(kNC * N * 8);
+            if (mixed) v.C32 = (float*)take(kNC * N * 4);
+        }
+        if (mixed) v.d32 = (float*)take(3 * N * 4);
+        else v.d64 = (double*)take(3 * N * 8);
+        for (int q = 0; q < F_N; ++q) v.f[q] = take(3 * N * ts);
+    }
+    r->M = (double*)take((size_t)r->nunk * r->nunk * 8);
+    r->scal = (double*)take(2 * 8);
+    return off;
+}
+
+BoxDesc desc_of(const ela_rve* r, int l) {
+    const RLev& v = r->lev[l];
+    BoxDesc d;
+    std::memset(&d, 0, sizeof d);
+    d.L = v.L;
+    d.nodes = v.nodes;
+    const double h3[3] = {v.h, v.h, v.h};
+    d.i64 = make_inv<double>(h3);
+    d.i32 = make_inv<float>(h3);
+    d.h[0] = d.h[1] = d.h[2] = v.h;
+    d.bc = r->bc;
+    d.uniform = r->uniform;
+    if (l == 0) {
+        d.gid = r->gid;
+        d.tab64 = r->tab64;
+        d.tab32 = r->tab32;
+        d.sc64 = r->sc64;
+        d.sc32 = r->sc32;
+        d.u64 = r->u64;
+        d.b64 = r->b64;
+        d.r64 = r->r64;
+    } else {
+        d.C64 = v.C64;
+        d.C32 = v.C32;
+    }
+    d.d64 = v.d64;
+    d.d32 = v.d32;
+    for (int q = 0; q < F_N; ++q) d.f[q] = v.f[q];
+    if (l == r->depth - 1) {
+        d.nunk = r->nunk;
+        d.M = r->M;
+    }
+    d.norm = r->scal;
+    d.scale = r->scal + 1;
+    return d;
+}
+
+int ensure_device(ela_batch* B, void** p, size_t* cap, size_t bytes) {
+    if (*cap >= bytes) return ELA_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    const size_t nb = std::max(bytes, (size_t)4096) * 3 / 2;
+    ELA_HIPCHK(hipMalloc(p, nb));
+    *cap = nb;
+    return ELA_OK;
+}
+
+// A list of boxes with the block -> box map, packed into B->hlists.
+struct ListBuild {
+    std::vector<BoxDesc> descs;
+    std::vector<int> tile;  // prefix of blocks (size descs + 1)
+};
+
+void list_push(ListBuild& lb, BoxDesc d) {
+    if (lb.tile.empty()) lb.tile.push_back(0);
+    d.tile0 = lb.tile.back();
+    lb.descs.push_back(d);
+    lb.tile.push_back(d.tile0 + (d.nodes + TB - 1) / TB);
+}
+
+void pack_list(ela_batch* B, const ListBuild& lb, size_t* off_desc, size_t* off_bmap) {
+    auto align = [&]() { B->hlists.resize(up256(B->hlists.size())); };
+    align();
+    *off_desc = B->hlists.size();
+    const size_t nd = lb.descs.size() * sizeof(BoxDesc);
+    B->hlists.resize(*off_desc + nd);
+    if (nd) std::memcpy(B->hlists.data() + *off_desc, lb.descs.data(), nd);
+    align();
+    *off_bmap = B->hlists.size();
+    const int nblk = lb.tile.empty() ? 0 : lb.tile.back();
+    B->hlists.resize(*off_bmap + (size_t)nblk * sizeof(int));
+    int* bm = (int*)(B->hlists.data() + *off_bmap);
+    for (size_t b = 0; b + 1 < lb.tile.size(); ++b)
+        for (int t = lb.tile[b]; t < lb.tile[b + 1]; ++t) bm[t] = (int)b;
+}
+
+// Rebuild the level lists of the active set (members ordered by their finest global level).
+int build_lists(ela_batch* B) {
+    std::stable_sort(B->active.begin(), B->active.end(), [](const ela_rve* a, const ela_rve* b) {
+        return a->depth != b->depth ? a->depth > b->depth : a->seq < b->seq;
+    });
+    int G = 0;
+    for (auto* r : B->active) G = std::max(G, r->depth);
+    B->G = G;
+    B->cnt.assign(G, 0);
+    B->tiles.assign(G, {});
+    B->off_desc.assign(G, 0);
+    B->off_bmap.assign(G, 0);
+    B->hlists.clear();
+    for (int g = 0; g < G; ++g) {
+        ListBuild lb;
+        for (auto* r : B->active) {
+            const int t = G - r->depth;
+            if (t <= g) list_push(lb, desc_of(r, g - t));
+        }
+        B->cnt[g] = (int)lb.descs.size();
+        B->tiles[g] = lb.tile.empty() ? std::vector<int>{0} : lb.tile;
+        pack_list(B, lb, &B->off_desc[g], &B->off_bmap[g]);
+    }
+    ListBuild top;
+    for (auto* r : B->active) list_push(top, desc_of(r, 0));
+    B->top_tiles = top.tile.empty() ? std::vector<int>{0} : top.tile;
+    pack_list(B, top, &B->off_top_desc, &B->off_top_bmap);
+    if (int rc = ensure_device(B, (void**)&B->dlists, &B->dlists_cap, B->hlists.size())) return rc;
+    if (!B->hlists.empty())
+        ELA_HIPCHK(hipMemcpyAsync(B->dlists, B->hlists.data(), B->hlists.size(), hipMemcpyHostToDevice, B->stream));
+    const size_t nblk_top = (size_t)B->top_tiles.back();
+    if (int rc = ensure_device(B, (void**)&B->partials, &B->partials_cap, nblk_top * sizeof(double))) return rc;
+    if (B->norms_cap < B->active.size()) {
+        if (B->dnorms) (void)hipFree(B->dnorms);
+        if (B->hnorms) (void)hipHostFree(B->hnorms);
+        B->dnorms = nullptr;
+        B->hnorms = nullptr;
+        B->norms_cap = 0;
+        const size_t cap = std::max<size_t>(B->active.size() * 2, 64);
+        ELA_HIPCHK(hipMalloc(&B->dnorms, cap * sizeof(double)));
+        ELA_HIPCHK(hipHostMalloc((void**)&B->hnorms, cap * sizeof(double), hipHostMallocDefault));
+        B->norms_cap = cap;
+    }
+    B->dirty = false;
+    return ELA_OK;
+}
+
+// ------------------------------------------------------------------ launch helpers
+const BoxDesc* LD(const ela_batch* B, int g) { return (const BoxDesc*)(B->dlists + B->off_desc[g]); }
+const int* LM(const ela_batch* B, int g) { return (const int*)(B->dlists + B->off_bmap[g]); }
+const BoxDesc* TD(const ela_batch* B) { return (const BoxDesc*)(B->dlists + B->off_top_desc); }
+const int* TM(const ela_batch* B) { return (const int*)(B->dlists + B->off_top_bmap); }
+int nblocks(const ela_batch* B, int g, int count) { return B->tiles[g][count]; }
+long long nodes_of(const ela_batch* B, int g, int count) {
+    const BoxDesc* d = (const BoxDesc*)(B->hlists.data() + B->off_desc[g]);
+    long long s = 0;
+    for (int m = 0; m < count; ++m) s += d[m].nodes;
+    return s;
+}
+
+// Boxes [0, count) of level g: planes [0, s), grain form [s, count), where s
+// counts the members that already exist one level up (their finest is above g).
+int split_of(const ela_batch* B, int g, int count) { return std::min(count, g > 0 ? B->cnt[g - 1] : 0); }
+
+template <typename T>
+void smooth(ela_batch* B, int g, int count, int rhs, int in, int out) {
+    const int nb = nblocks(B, g, count);
+    if (!nb) return;
+    int e = -1;
+    if (B->timing && B->ev_launch.size() * 2 + 2 <= B->ev.size()) {
+        e = (int)B->ev_launch.size() * 2;
+        (void)hipEventRecord(B->ev[e], B->stream);
+    }
+    const int s = split_of(B, g, count), b0 = nblocks(B, g, s);
+    if (b0)
+        hipLaunchKernelGGL((kb_smooth<T, 0>), dim3(b0), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), 0, rhs, in, out,
+                           (T)(2.0 / 3.0));
+    if (nb > b0)
+        hipLaunchKernelGGL((kb_smooth<T, 1>), dim3(nb - b0), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), b0, rhs, in,
+                           out, (T)(2.0 / 3.0));
+    if (e >= 0) {
+        (void)hipEventRecord(B->ev[e + 1], B->stream);
+        B->ev_launch.push_back({e, nodes_of(B, g, count)});
+    }
+}
+
+// nu calls of Fsmooth (2 sweeps each); the first call of a pre-smoothing starts from zero.
+template <typename T>
+void fsmooth(ela_batch* B, int g, int count, int rhs, bool zero_start) {
+    for (int c = 0; c < 2; ++c) {
+        smooth<T>(B, g, count, rhs, (zero_start && c == 0) ? -1 : F_COR, F_TMP);
+        smooth<T>(B, g, count, rhs, F_TMP, F_COR);
+    }
+}
+
+template <typename T>
+void resid(ela_batch* B, int g, int count, int x, int b, int r) {
+    const int nb = nblocks(B, g, count);
+    const int s = split_of(B, g, count), b0 = nblocks(B, g, s);
+    if (b0) hipLaunchKernelGGL((kb_resid<T, 0>), dim3(b0), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), 0, x, b, r);
+    if (nb > b0)
+        hipLaunchKernelGGL((kb_resid<T, 1>), dim3(nb - b0), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), b0, x, b, r);
+}
+
+template <typename T>
+void restrict_to(ela_batch* B, int g, int count, int src, int dst) {  // level g -> g + 1
+    const int nb = nblocks(B, g + 1, count);
+    if (nb)
+        hipLaunchKernelGGL((kb_restrict<T>), dim3(nb), dim3(TB), 0, B->stream, LD(B, g + 1), LD(B, g), LM(B, g + 1),
+                           src, dst);
+}
+
+template <typename T>
+void prolong_to(ela_batch* B, int g, int count, int src, int dst, int add) {  // level g + 1 -> g
+    const int nb = nblocks(B, g, count);
+    if (nb)
+        hipLaunchKernelGGL((kb_prolong<T>), dim3(nb), dim3(TB), 0, B->stream, LD(B, g), LD(B, g + 1), LM(B, g), src,
+                           dst, add);
+}
+
+template <typename T>
+void axpy(ela_batch* B, int g, int count, int x, int y) {
+    const int nb = nblocks(B, g, count);
+    if (nb) hipLaunchKernelGGL((kb_axpy<T>), dim3(nb), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), x, y);
+}
+
+template <typename T>
+void bottom(ela_batch* B, int count, int src, int dst) {
+    const int g = B->G - 1;
+    if (!count) return;
+    const BoxDesc* hd = (const BoxDesc*)(B->hlists.data() + B->off_desc[g]);
+    int maxn = 0;
+    for (int m = 0; m < count; ++m) maxn = std::max(maxn, hd[m].nunk);
+    const int rpb = 4 * (TB / 64);
+    hipLaunchKernelGGL((kb_bottom<T>), dim3((maxn + rpb - 1) / rpb, count), dim3(TB), 0, B->stream, LD(B, g), src, dst);
+}
+
+// V-cycle from global level s on the members whose finest level is <= s; the
+// right-hand side on level s is f[RC], on coarser levels f[RES].
+template <typename T>
+void vcycle(ela_batch* B, int s) {
+    const int G = B->G, n = B->cnt[s];
+    for (int g = s; g < G - 1; ++g) {
+        const int rhs = g == s ? F_RC : F_RES;
+        fsmooth<T>(B, g, n, rhs, true);
+        resid<T>(B, g, n, F_COR, rhs, F_TMP);
+        restrict_to<T>(B, g, n, F_TMP, F_RES);
+    }
+    bottom<T>(B, n, s == G - 1 ? F_RC : F_RES, F_COR);
+    for (int g = G - 2; g >= s; --g) {
+        const int rhs = g == s ? F_RC : F_RES;
+        prolong_to<T>(B, g, n, F_COR, F_COR, 1);
+        fsmooth<T>(B, g, n, rhs, false);
+    }
+}
+
+// F-cycle on the whole active set: restrict the right-hand side to every
+// level, solve the coarsest, then per level upward: hold = P cor_c,
+// rc = res - A hold, V-cycle on rc, cor += hold.
+template <typename T>
+void fcycle(ela_batch* B) {
+    const int G = B->G;
+    for (int g = 0; g + 1 < G; ++g) restrict_to<T>(B, g, B->cnt[g], F_RES, F_RES);
+    bottom<T>(B, B->cnt[G - 1], F_RES, F_COR);
+    for (int g = G - 2; g >= 0; --g) {
+        const int n = B->cnt[g];
+        if (!n) continue;
+        prolong_to<T>(B, g, n, F_COR, F_HOLD, 0);
+        resid<T>(B, g, n, F_HOLD, F_RES, F_RC);
+        vcycle<T>(B, g);
+        axpy<T>(B, g, n, F_HOLD, F_COR);
+    }
+}
+
+template <typename T>
+void resid64_top(ela_batch* B, const BoxDesc* D, const int* bmap, int nbox, int nblk) {
+    if (!nbox) return;
+    hipLaunchKernelGGL((kb_resid64<T>), dim3(nblk), dim3(TB), 0, B->stream, D, bmap, F_RES, B->partials);
+    hipLaunchKernelGGL(kb_fold, dim3(nbox), dim3(64), 0, B->stream, D, B->partials, B->dnorms);
+}
+
+int read_norms(ela_batch* B, int nbox) {
+    ELA_HIPCHK(hipMemcpyAsync(B->hnorms, B->dnorms, nbox * sizeof(double), hipMemcpyDeviceToHost, B->stream));
+    ELA_HIPCHK(hipStreamSynchronize(B->stream));
+    return ELA_OK;
+}
+
+// Admit queued members: initial residual (u as uploaded, zero by default) with scale 1.
+int admit(ela_batch* B) {
+    if (B->queued.empty()) return ELA_OK;
+    ListBuild jl;
+    for (auto* r : B->queued) list_push(jl, desc_of(r, 0));
+    std::vector<char> save;
+    save.swap(B->hlists);
+    size_t od = 0, om = 0;
+    pack_list(B, jl, &od, &om);
+    std::vector<char> jbytes;
+    jbytes.swap(B->hlists);
+    B->hlists.swap(save);
+    // The join lists go to a separate device buffer region at the end of dlists.
+    const size_t base = up256(B->hlists.size());
+    if (int rc = ensure_device(B, (void**)&B->dlists, &B->dlists_cap, base + jbytes.size())) return rc;
+    B->dirty = true;  // dlists may have been reallocated: the active lists are rebuilt anyway
+    ELA_HIPCHK(hipMemcpyAsync(B->dlists + base, jbytes.data(), jbytes.size(), hipMemcpyHostToDevice, B->stream));
+    const size_t nblk = (size_t)jl.tile.back();
+    if (int rc = ensure_device(B, (void**)&B->partials, &B->partials_cap, nblk * sizeof(double))) return rc;
+    if (B->norms_cap < B->queued.size()) {
+        if (B->dnorms) (void)hipFree(B->dnorms);
+        if (B->hnorms) (void)hipHostFree(B->hnorms);
+        const size_t cap = std::max<size_t>(B->queued.size() * 2, 64);
+        ELA_HIPCHK(hipMalloc(&B->dnorms, cap * sizeof(double)));
+        ELA_HIPCHK(hipHostMalloc((void**)&B->hnorms, cap * sizeof(double), hipHostMallocDefault));
+        B->norms_cap = cap;
+    }
+    const double one[2] = {0.0, 1.0};
+    for (auto* r : B->queued) ELA_HIPCHK(hipMemcpyAsync(r->scal, one, sizeof one, hipMemcpyHostToDevice, B->stream));
+    const BoxDesc* D = (const BoxDesc*)(B->dlists + base + od);
+    const int* bm = (const int*)(B->dlists + base + om);
+    if (B->mode == ELA_SOLVE_MIXED) resid64_top<float>(B, D, bm, (int)jl.descs.size(), (int)nblk);
+    else resid64_top<double>(B, D, bm, (int)jl.descs.size(), (int)nblk);
+    if (int rc = read_norms(B, (int)jl.descs.size())) return rc;
+    for (size_t m = 0; m < B->queued.size(); ++m) {
+        ela_rve* r = B->queued[m];
+        const double r0 = B->hnorms[m];
+        r->res0 = r->res = r->best = r0;
+        r->cycles = 0;
+        r->stall = 0;
+        r->target = std::max(r->tol_abs, std::max(r->tol_rel, 1e-16) * r0);
+        if (!(r0 == r0)) {
+            r->status = fail(ELA_ERR_NAN, "initial residual is not finite");
+            r->state = ST_DONE;
+            r->t_done = now_s();
+        } else if (r0 <= r->target) {
+            r->status = ELA_OK;
+            r->state = ST_DONE;
+            r->t_done = now_s();
+        } else {
+            r->state = ST_ACTIVE;
+            B->active.push_back(r);
+        }
+    }
+    B->queued.clear();
+    return ELA_OK;
+}
+
+// One F-cycle on every active member, then the FP64 update and residual;
+// members that converge, fail or run out of cycles leave the active set.
+template <typename T>
+int step_t(ela_batch* B) {
+    fcycle<T>(B);
+    const int nbox = (int)B->active.size();
+    const int nblk = B->top_tiles.back();
+    hipLaunchKernelGGL((kb_update<T>), dim3(nblk), dim3(TB), 0, B->stream, TD(B), TM(B), F_COR);
+    hipLaunchKernelGGL(kb_scale, dim3((nbox + 63) / 64), dim3(64), 0, B->stream, TD(B), nbox);
+    resid64_top<T>(B, TD(B), TM(B), nbox, nblk);
+    return read_norms(B, nbox);
+}
+
+int step(ela_batch* B) {
+    if (int rc = admit(B)) return rc;
+    if (B->active.empty()) return ELA_OK;
+    if (B->dirty)
+        if (int rc = build_lists(B)) return rc;
+    const int rc = B->mode == ELA_SOLVE_MIXED ? step_t<float>(B) : step_t<double>(B);
+    if (rc) return rc;
+    const double t = now_s();
+    std::vector<ela_rve*> keep;
+    for (size_t m = 0; m < B->active.size(); ++m) {
+        ela_rve* r = B->active[m];
+        const double rn = B->hnorms[m];
+        r->res = rn;
+        ++r->cycles;
+        bool done = false;
+        if (!(rn == rn)) {
+            r->status = ELA_ERR_NAN;
+            done = true;
+        } else if (rn <= r->target) {
+            r->status = ELA_OK;
+            done = true;
+        } else {
+            if (rn < 0.5 * r->best) {
+                r->best = rn;
+                r->stall = 0;
+            } else if (++r->stall >= 12) {
+                r->status = ELA_ERR_NONCONVERGED;
+                done = true;
+            }
+            if (r->cycles >= r->max_cycles) {
+                r->status = ELA_ERR_NONCONVERGED;
+                done = true;
+            }
+        }
+        if (done) {
+            r->state = ST_DONE;
+            r->t_done = t;
+            B->dirty = true;
+        } else {
+            keep.push_back(r);
+        }
+    }
+    B->active.swap(keep);
+    return ELA_OK;
+}
+
+int check_rve(const ela_rve* r) {
+    if (!r) return fail(ELA_ERR_BADARG, "null member");
+    return ELA_OK;
+}
+
+// Batched set-up of the members: coarse stiffness, diagonals, coarsest inverses.
+int prepare(ela_batch* B, int count, ela_rve* const* rves) {
+    std::vector<ela_rve*> set(rves, rves + count);
+    for (auto* r : set) {
+        if (int rc = check_rve(r)) return rc;
+        if (r->B != B) return fail(ELA_ERR_BADARG, "member belongs to another batch");
+        if (!r->model) return fail(ELA_ERR_STATE, "model not set (SetModel before prepareForSolve)");
+        if (r->state == ST_QUEUED || r->state == ST_ACTIVE) return fail(ELA_ERR_STATE, "member is being solved");
+    }
+    if (set.empty()) return ELA_OK;
+    std::stable_sort(set.begin(), set.end(), [](const ela_rve* a, const ela_rve* b) { return a->depth > b->depth; });
+    const int G = set.front()->depth;
+    // Per global level lists (the same alignment as the solve).
+    std::vector<ListBuild> L(G);
+    for (int g = 0; g < G; ++g)
+        for (auto* r : set) {
+            const int t = G - r->depth;
+            if (t <= g) list_push(L[g], desc_of(r, g - t));
+        }
+    // Set-up workspace: identities per coarsest size and [A | I] per member.
+    std::vector<int> sizes;
+    for (auto* r : set) sizes.push_back(r->nunk);
+    std::sort(sizes.begin(), sizes.end());
+    sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+    size_t ws = 0;
+    for (int s : sizes) ws += up256((size_t)s * s * 8);
+    for (auto* r : set) ws += up256((size_t)r->nunk * 2 * r->nunk * 8);
+    char* wbase = nullptr;
+    ELA_HIPCHK(hipMalloc((void**)&wbase, std::max<size_t>(ws, 256)));
+    ELA_HIPCHK(hipMemsetAsync(wbase, 0, std::max<size_t>(ws, 256), B->stream));
+    size_t woff = 0;
+    std::vector<double*> eye(sizes.size());
+    for (size_t q = 0; q < sizes.size(); ++q) {
+        eye[q] = (double*)(wbase + woff);
+        woff += up256((size_t)sizes[q] * sizes[q] * 8);
+        const long long nn = (long long)sizes[q] * sizes[q];
+        hipLaunchKernelGGL(kb_identity, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, B->stream, eye[q], sizes[q]);
+    }
+    for (auto& d : L[G - 1].descs) {
+        const int q = (int)(std::lower_bound(sizes.begin(), sizes.end(), d.nunk) - sizes.begin());
+        d.E = eye[q];
+        d.W = (double*)(wbase + woff);
+        woff += up256((size_t)d.nunk * 2 * d.nunk * 8);
+    }
+    // Pack and upload the lists.
+    std::vector<char> save;
+    save.swap(B->hlists);
+    std::vector<size_t> od(G), om(G);
+    for (int g = 0; g < G; ++g) pack_list(B, L[g], &od[g], &om[g]);
+    char* dl = nullptr;
+    int rc = ELA_OK;
+    if (hipMalloc((void**)&dl, std::max<size_t>(B->hlists.size(), 256)) != hipSuccess) rc = fail(ELA_ERR_OOM, "hipMalloc");
+    if (!rc)
+        (void)hipMemcpyAsync(dl, B->hlists.data(), B->hlists.size(), hipMemcpyHostToDevice, B->stream);
+    B->hlists.swap(save);
+    if (!rc) {
+        ELA_HIPCHK(hipMemsetAsync(B->dflag, 0, sizeof(int), B->stream));
+        auto D = [&](int g) { return (const BoxDesc*)(dl + od[g]); };
+        auto M = [&](int g) { return (const int*)(dl + om[g]); };
+        auto nb = [&](int g, int c) { return L[g].tile[c]; };
+        // coarse stiffness (level by level, each member's next level from its previous one)
+        for (int g = 0; g + 1 < G; ++g) {
+            const int c = (int)L[g].descs.size();
+            hipLaunchKernelGGL(kb_coef, dim3(nb(g + 1, c)), dim3(TB), 0, B->stream, D(g + 1), D(g), M(g + 1));
+        }
+        for (int g = 0; g < G; ++g) {
+            const int c = (int)L[g].descs.size();
+            if (B->mode == ELA_SOLVE_MIXED)
+                hipLaunchKernelGGL(kb_planes32, dim3(nb(g, c)), dim3(TB), 0, B->stream, D(g), M(g));
+            hipLaunchKernelGGL(kb_diag, dim3(nb(g, c)), dim3(TB), 0, B->stream, D(g), M(g), B->dflag);
+        }
+        // coarsest inverse: probe A, Gauss-Jordan on [A | I], keep A^-1
+        const int nbox = (int)L[G - 1].descs.size();
+        int maxn = 0, maxnodes = 0;
+        for (auto& d : L[G - 1].descs) {
+            maxn = std::max(maxn, d.nunk);
+            maxnodes = std::max(maxnodes, d.nodes);
+        }
+        const int nbk = (maxnodes + TB - 1) / TB;
+        hipLaunchKernelGGL(kb_probe, dim3(maxn * nbk, nbox), dim3(TB), 0, B->stream, D(G - 1), nbk);
+        for (int c = 0; c < maxn; ++c) {
+            hipLaunchKernelGGL(kb_gj_pivot, dim3(nbox), dim3(1024), 0, B->stream, D(G - 1), c, B->dflag);
+            hipLaunchKernelGGL(kb_gj_eliminate, dim3(maxn, nbox), dim3(256), 0, B->stream, D(G - 1), c);
+        }
+        hipLaunchKernelGGL(kb_gj_extract, dim3(maxn, nbox), dim3(256), 0, B->stream, D(G - 1));
+        int flag = 0;
+        (void)hipMemcpyAsync(&flag, B->dflag, sizeof flag, hipMemcpyDeviceToHost, B->stream);
+        const hipError_t e = hipStreamSynchronize(B->stream);
+        if (e != hipSuccess) rc = fail(ELA_ERR_HIP, std::string("prepare: ") + hipGetErrorString(e));
+        else if (flag) rc = fail(ELA_ERR_NAN, "zero or non-finite diagonal or singular coarsest operator");
+    }
+    if (dl) (void)hipFree(dl);
+    (void)hipFree(wbase);
+    if (!rc)
+        for (auto* r : set) r->prepared = true;
+    return rc;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+int ela_batch_create(ela_ctx* ctx, int mode, ela_batch** out) {
+    if (!ctx || !out) return fail(ELA_ERR_BADARG, "null argument");
+    *out = nullptr;
+    if (mode != ELA_SOLVE_FP64 && mode != ELA_SOLVE_MIXED) return fail(ELA_ERR_BADARG, "mode must be FP64 or MIXED");
+    ELA_HIPCHK(hipSetDevice(ctx->device));
+    ela_batch* B = new ela_batch();
+    B->ctx = ctx;
+    B->mode = mode;
+    B->tsize = mode == ELA_SOLVE_MIXED ? 4 : 8;
+    if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void**)&B->dflag, sizeof(int)) != hipSuccess) {
+        if (B->stream) (void)hipStreamDestroy(B->stream);
+        delete B;
+        return fail(ELA_ERR_HIP, "stream / scratch allocation failed");
+    }
+    *out = B;
+    return ELA_OK;
+}
+
+int ela_batch_destroy(ela_batch* B) {
+    if (!B) return ELA_OK;
+    (void)hipStreamSynchronize(B->stream);
+    std::vector<ela_rve*> ms = B->members;
+    for (auto* r : ms) ela_rve_destroy(r);
+    for (auto e : B->ev) (void)hipEventDestroy(e);
+    if (B->dlists) (void)hipFree(B->dlists);
+    if (B->partials) (void)hipFree(B->partials);
+    if (B->dnorms) (void)hipFree(B->dnorms);
+    if (B->hnorms) (void)hipHostFree(B->hnorms);
+    if (B->dflag) (void)hipFree(B->dflag);
+    (void)hipStreamDestroy(B->stream);
+    delete B;
+    return ELA_OK;
+}
+
+size_t ela_rve_bytes(int n, int mode, int max_grains, int with_factor) {
+    if (n < 4 || max_grains < 1) return 0;
+    ela_batch Bt;
+    Bt.mode = mode;
+    Bt.tsize = mode == ELA_SOLVE_MIXED ? 4 : 8;
+    ela_rve r;
+    r.B = &Bt;
+    r.n = n;
+    r.depth = depth_of(n);
+    r.max_grains = max_grains;
+    r.factor = with_factor != 0;
+    const int c = coarsest_of(n);
+    r.nunk = 3 * (c + 1) * (c + 1) * (c + 1);
+    r.lev.resize(r.depth);
+    int m = n;
+    for (int l = 0; l < r.depth; ++l, m /= 2) r.lev[l].nodes = (m + 1) * (m + 1) * (m + 1);
+    return layout_member(&r, nullptr);
+}
+
+int ela_rve_create(ela_batch* B, int n, int max_grains, int with_factor, ela_rve** out) {
+    if (!B || !out) return fail(ELA_ERR_BADARG, "null argument");
+    *out = nullptr;
+    if (n < 4 || n > 1024) return fail(ELA_ERR_BADARG, "member side must be 4..1024 cells");
+    if (max_grains < 1 || max_grains > 65536) return fail(ELA_ERR_BADARG, "max_grains must be 1..65536");
+    const int c = coarsest_of(n);
+    const int nunk = 3 * (c + 1) * (c + 1) * (c + 1);
+    if (nunk > kMaxBatchDense)
+        return fail(ELA_ERR_BADARG, "coarsest level of n = " + std::to_string(n) + " has " + std::to_string(c) +
+                                        " cells; members need n = c * 2^k with c <= 7");
+    ela_rve* r = new ela_rve();
+    r->B = B;
+    r->n = n;
+    r->depth = depth_of(n);
+    r->max_grains = max_grains;
+    r->factor = with_factor != 0;
+    r->nunk = nunk;
+    r->lev.resize(r->depth);
+    int m = n;
+    for (int l = 0; l < r->depth; ++l, m /= 2) {
+        RLev& v = r->lev[l];
+        v.n = m;
+        v.h = 1.0 / m;
+        v.L.dim = 3;
+        v.L.nx = v.L.ny = v.L.nz = m + 1;
+        v.L.px = m + 1;
+        v.L.pl = (long long)(m + 1) * (m + 1);
+        v.L.cs = v.L.pl * (m + 1);
+        v.nodes = (int)v.L.cs;
+    }
+    r->bytes = layout_member(r, nullptr);
+    if (int rc = budget_reserve(B->ctx, r->bytes)) {
+        delete r;
+        return rc;
+    }
+    if (hipMalloc(&r->block, r->bytes) != hipSuccess) {
+        budget_release(B->ctx, r->bytes);
+        delete r;
+        return fail(ELA_ERR_OOM, "hipMalloc of a member failed");
+    }
+    layout_member(r, (char*)r->block);
+    if (hipMemsetAsync(r->block, 0, r->bytes, B->stream) != hipSuccess) {
+        (void)hipFree(r->block);
+        budget_release(B->ctx, r->bytes);
+        delete r;
+        return fail(ELA_ERR_HIP, "hipMemsetAsync of a member failed");
+    }
+    if (r->factor) {  // factor 1 until a crack is set
+        Crack one{};
+        one.width = 1.0;
+        one.c = 1.0;
+        const RLev& v = r->lev[0];
+        hipLaunchKernelGGL(kb_model, dim3((v.nodes + TB - 1) / TB), dim3(TB), 0, B->stream, v.L, v.h, 0,
+                           (const long long*)nullptr, 1, (unsigned short*)nullptr, 1, one, r->sc64, r->sc32);
+    }
+    r->seq = B->seq++;
+    B->members.push_back(r);
+    *out = r;
+    return ELA_OK;
+}
+
+int ela_rve_destroy(ela_rve* r) {
+    if (!r) return ELA_OK;
+    ela_batch* B = r->B;
+    (void)hipStreamSynchronize(B->stream);
+    auto drop = [r](std::vector<ela_rve*>& v) { v.erase(std::remove(v.begin(), v.end(), r), v.end()); };
+    if (std::find(B->active.begin(), B->active.end(), r) != B->active.end()) B->dirty = true;
+    drop(B->active);
+    drop(B->queued);
+    drop(B->members);
+    (void)hipFree(r->block);
+    budget_release(B->ctx, r->bytes);
+    delete r;
+    return ELA_OK;
+}
+
+size_t ela_rve_bytes_in_use(const ela_rve* r) { return r ? r->bytes : 0; }
+
+int ela_rve_levels(const ela_rve* r) { return r ? r->depth : 0; }
+
+int ela_rve_set_bc_types(ela_rve* r, const int* types) {
+    if (int rc = check_rve(r)) return rc;
+    if (!types) return fail(ELA_ERR_BADARG, "null types");
+    for (int f = 0; f < 26; ++f)
+        for (int c = 0; c < 3; ++c) {
+            const int t = types[3 * f + c];
+            if (t < 0 || t > 3) return fail(ELA_ERR_BADARG, "unknown BC type");
+            r->bc.t[f][c] = (unsigned char)t;
+        }
+    r->prepared = false;
+    return ELA_OK;
+}
+
+int ela_rve_set_uniform(ela_rve* r, int uniform) {
+    if (int rc = check_rve(r)) return rc;
+    r->uniform = uniform != 0;
+    r->prepared = false;
+    return ELA_OK;
+}
+
+int ela_rve_set_voronoi_cubic(ela_rve* r, int nseeds, const int64_t* seeds, int q, const double* R, double C11,
+                              double C12, double C44) {
+    if (int rc = check_rve(r)) return rc;
+    if (nseeds < 1 || nseeds > r->max_grains || !seeds || !R || q < 1)
+        return fail(ELA_ERR_BADARG, "need 1..max_grains seeds, rotations and q >= 1");
+    ela_batch* B = r->B;
+    std::vector<double> Rh(R, R + 9 * (size_t)nseeds);
+    double* dR = nullptr;
+    ELA_HIPCHK(hipMalloc((void**)&dR, Rh.size() * sizeof(double)));
+    (void)hipMemcpyAsync(dR, Rh.data(), Rh.size() * sizeof(double), hipMemcpyHostToDevice, B->stream);
+    (void)hipMemcpyAsync(r->seeds, seeds, 3 * (size_t)nseeds * sizeof(long long), hipMemcpyHostToDevice, B->stream);
+    launch_cubic_table(nseeds, dR, C11, C12, C44, r->tab64, B->stream);
+    if (r->tab32) launch_convert<double, float>((long long)nseeds * kNC, r->tab64, r->tab32, B->stream);
+    const RLev& v = r->lev[0];
+    hipLaunchKernelGGL(kb_model, dim3((v.nodes + TB - 1) / TB), dim3(TB), 0, B->stream, v.L, v.h, nseeds,
+                       (const long long*)r->seeds, q, r->gid, 0, Crack{}, (double*)nullptr, (float*)nullptr);
+    const hipError_t e = hipStreamSynchronize(B->stream);
+    (void)hipFree(dR);
+    if (e != hipSuccess) return fail(ELA_ERR_HIP, std::string("set_voronoi_cubic: ") + hipGetErrorString(e));
+    r->ngrains = nseeds;
+    r->model = true;
+    r->prepared = false;
+    return ELA_OK;
+}
+
+int ela_rve_set_crack(ela_rve* r, const double p[3], const double nrm[3], const double tv[3], double width, double a,
+                      double b, double c) {
+    if (int rc = check_rve(r)) return rc;
+    if (!r->factor) return fail(ELA_ERR_STATE, "member was created without a degradation factor field");
+    if (!p || !nrm || !tv || !(width > 0)) return fail(ELA_ERR_BADARG, "need p, normal, in-plane direction, width > 0");
+    Crack cr;
+    for (int q = 0; q < 3; ++q) {
+        cr.p[q] = p[q];
+        cr.nrm[q] = nrm[q];
+        cr.tv[q] = tv[q];
+    }
+    cr.width = width;
+    cr.a = a;
+    cr.b = b;
+    cr.c = c;
+    const RLev& v = r->lev[0];
+    hipLaunchKernelGGL(kb_model, dim3((v.nodes + TB - 1) / TB), dim3(TB), 0, r->B->stream, v.L, v.h, 0,
+                       (const long long*)nullptr, 1, (unsigned short*)nullptr, 1, cr, r->sc64, r->sc32);
+    ELA_HIPCHK(hipStreamSynchronize(r->B->stream));
+    r->prepared = false;
+    return ELA_OK;
+}
+
+int ela_rve_set_rhs_affine(ela_rve* r, const double G[9]) {
+    if (int rc = check_rve(r)) return rc;
+    if (!G) return fail(ELA_ERR_BADARG, "null gradient");
+    const RLev& v = r->lev[0];
+    hipLaunchKernelGGL(kb_rhs_affine, dim3((v.nodes + TB - 1) / TB), dim3(TB), 0, r->B->stream, v.L, v.h, r->bc, G[0],
+                       G[1], G[2], G[3], G[4], G[5], G[6], G[7], G[8], r->b64);
+    ELA_HIPCHK(hipStreamSynchronize(r->B->stream));
+    return ELA_OK;
+}
+
+// field: 0 = u, 1 = b, 2 = r (test output / residual), 3 = degradation factor (1 comp)
+int ela_rve_upload(ela_rve* r, int field, const double* host) {
+    if (int rc = check_rve(r)) return rc;
+    if (!host || field < 0 || field > 1) return fail(ELA_ERR_BADARG, "upload: field must be 0 (u) or 1 (b)");
+    if (r->state == ST_QUEUED || r->state == ST_ACTIVE) return fail(ELA_ERR_STATE, "member is being solved");
+    double* dst = field == 0 ? r->u64 : r->b64;
+    ELA_HIPCHK(hipMemcpyAsync(dst, host, 3 * (size_t)r->lev[0].nodes * 8, hipMemcpyHostToDevice, r->B->stream));
+    ELA_HIPCHK(hipStreamSynchronize(r->B->stream));
+    return ELA_OK;
+}
+
+int ela_rve_download(const ela_rve* r, int field, double* host) {
+    if (int rc = check_rve(r)) return rc;
+    if (!host || field < 0 || field > 3) return fail(ELA_ERR_BADARG, "download: field must be 0..3");
+    const size_t N = (size_t)r->lev[0].nodes;
+    if (field == 3) {
+        if (!r->sc64) return fail(ELA_ERR_STATE, "member has no degradation factor");
+        ELA_HIPCHK(hipMemcpyAsync(host, r->sc64, N * 8, hipMemcpyDeviceToHost, r->B->stream));
+    } else {
+        const double* src = field == 0 ? r->u64 : (field == 1 ? r->b64 : r->r64);
+        ELA_HIPCHK(hipMemcpyAsync(host, src, 3 * N * 8, hipMemcpyDeviceToHost, r->B->stream));
+    }
+    ELA_HIPCHK(hipStreamSynchronize(r->B->stream));
+    return ELA_OK;
+}
+
+int ela_rve_grain_ids(const ela_rve* r, uint16_t* host) {
+    if (int rc = check_rve(r)) return rc;
+    if (!host) return fail(ELA_ERR_BADARG, "null output");
+    ELA_HIPCHK(hipMemcpyAsync(host, r->gid, (size_t)r->lev[0].nodes * 2, hipMemcpyDeviceToHost, r->B->stream));
+    ELA_HIPCHK(hipStreamSynchronize(r->B->stream));
+    return ELA_OK;
+}
+
+int ela_batch_prepare(ela_batch* B, int count, ela_rve* const* rves) {
+    if (!B || count < 0 || (count && !rves)) return fail(ELA_ERR_BADARG, "bad arguments");
+    return prepare(B, count, rves);
+}
+
+// f = A u on the finest level of each member: prec ELA_F64 from u into r;
+// ELA_F32: u rounded to FP32, the FP32 operator, result into r.
+int ela_batch_fapply(ela_batch* B, int count, ela_rve* const* rves, int prec) {
+    if (!B || count < 1 || !rves) return fail(ELA_ERR_BADARG, "bad arguments");
+    if (prec == ELA_F32 && B->mode != ELA_SOLVE_MIXED) return fail(ELA_ERR_STATE, "no FP32 fields in an FP64 batch");
+    ListBuild lb;
+    for (int m = 0; m < count; ++m) {
+        if (int rc = check_rve(rves[m])) return rc;
+        if (!rves[m]->model) return fail(ELA_ERR_STATE, "model not set");
+        list_push(lb, desc_of(rves[m], 0));
+    }
+    std::vector<char> save;
+    save.swap(B->hlists);
+    size_t od = 0, om = 0;
+    pack_list(B, lb, &od, &om);
+    char* dl = nullptr;
+    ELA_HIPCHK(hipMalloc((void**)&dl, B->hlists.size()));
+    (void)hipMemcpyAsync(dl, B->hlists.data(), B->hlists.size(), hipMemcpyHostToDevice, B->stream);
+    B->hlists.swap(save);
+    const BoxDesc* D = (const BoxDesc*)(dl + od);
+    const int* bm = (const int*)(dl + om);
+    const int nb = lb.tile.back();
+    if (prec == ELA_F64) {
+        hipLaunchKernelGGL(kb_apply64, dim3(nb), dim3(TB), 0, B->stream, D, bm);
+    } else {
+        hipLaunchKernelGGL((kb_cvt<float>), dim3(nb), dim3(TB), 0, B->stream, D, bm, F_TMP, -1);
+        hipLaunchKernelGGL((kb_apply<float>), dim3(nb), dim3(TB), 0, B->stream, D, bm, F_TMP, F_HOLD);
+        hipLaunchKernelGGL((kb_cvt<float>), dim3(nb), dim3(TB), 0, B->stream, D, bm, -1, F_HOLD);
+    }
+    const hipError_t e = hipStreamSynchronize(B->stream);
+    (void)hipFree(dl);
+    if (e != hipSuccess) return fail(ELA_ERR_HIP, std::string("batch fapply: ") + hipGetErrorString(e));
+    return ELA_OK;
+}
+
+int ela_batch_submit(ela_batch* B, ela_rve* r, double tol_rel, double tol_abs, int max_cycles) {
+    if (!B || !r || r->B != B) return fail(ELA_ERR_BADARG, "bad arguments");
+    if (!r->prepared) return fail(ELA_ERR_STATE, "member not prepared (ela_batch_prepare)");
+    if (r->state == ST_QUEUED || r->state == ST_ACTIVE) return fail(ELA_ERR_STATE, "member already submitted");
+    if (max_cycles < 1) return fail(ELA_ERR_BADARG, "max_cycles must be >= 1");
+    r->tol_rel = tol_rel;
+    r->tol_abs = tol_abs;
+    r->max_cycles = max_cycles;
+    r->state = ST_QUEUED;
+    r->status = ELA_OK;
+    r->seq = B->seq++;
+    r->t_submit = now_s();
+    B->queued.push_back(r);
+    return ELA_OK;
+}
+
+int ela_batch_step(ela_batch* B, int* n_active) {
+    if (!B) return fail(ELA_ERR_BADARG, "null batch");
+    const int rc = step(B);
+    if (n_active) *n_active = (int)(B->active.size() + B->queued.size());
+    return rc;
+}
+
+int ela_batch_solve(ela_batch* B, int count, ela_rve* const* rves, double tol_rel, double tol_abs, int max_cycles) {
+    if (!B || count < 0 || (count && !rves)) return fail(ELA_ERR_BADARG, "bad arguments");
+    for (int m = 0; m < count; ++m)
+        if (int rc = ela_batch_submit(B, rves[m], tol_rel, tol_abs, max_cycles)) return rc;
+    for (;;) {
+        if (int rc = step(B)) return rc;
+        if (B->active.empty() && B->queued.empty()) break;
+    }
+    int worst = ELA_OK;
+    for (int m = 0; m < count; ++m)
+        if (rves[m]->status != ELA_OK && worst == ELA_OK) worst = rves[m]->status;
+    if (worst == ELA_ERR_NONCONVERGED) return fail(worst, "a member did not converge");
+    if (worst == ELA_ERR_NAN) return fail(worst, "a member's residual became non-finite");
+    return worst;
+}
+
+// state: 0 idle, 1 queued, 2 active, 3 done (status: ELA_OK, ELA_ERR_NONCONVERGED or ELA_ERR_NAN).
+int ela_rve_status(const ela_rve* r, int* state, int* status, ela_stats* st) {
+    if (int rc = check_rve(r)) return rc;
+    if (state) *state = r->state;
+    if (status) *status = r->status;
+    if (st) {
+        st->iters = r->cycles;
+        st->converged = (r->state == ST_DONE && r->status == ELA_OK) ? 1 : 0;
+        st->mg_levels = r->depth;
+        st->mode = r->B->mode;
+        st->res0 = r->res0;
+        st->res = r->res;
+        st->seconds = r->state == ST_DONE ? r->t_done - r->t_submit : 0.0;
+        st->setup_seconds = 0;
+    }
+    return ELA_OK;
+}
+
+int ela_batch_set_timing(ela_batch* B, int on) {
+    if (!B) return fail(ELA_ERR_BADARG, "null batch");
+    B->timing = on != 0;
+    B->ev_launch.clear();
+    if (B->timing && B->ev.empty()) {
+        B->ev.resize(16384);
+        for (auto& e : B->ev) ELA_HIPCHK(hipEventCreate(&e));
+    }
+    return ELA_OK;
+}
+
+// Smoother launches since timing was switched on: total ms, total nodes swept, launch count.
+int ela_batch_smoother_times(ela_batch* B, double* ms, long long* nodes, int* launches) {
+    if (!B) return fail(ELA_ERR_BADARG, "null batch");
+    ELA_HIPCHK(hipStreamSynchronize(B->stream));
+    double t = 0;
+    long long nn = 0;
+    for (auto& pr : B->ev_launch) {
+        float x = 0;
+        ELA_HIPCHK(hipEventElapsedTime(&x, B->ev[pr.first], B->ev[pr.first + 1]));
+        t += x;
+        nn += pr.second;
+    }
+    if (ms) *ms = t;
+    if (nodes) *nodes = nn;
+    if (launches) *launches = (int)B->ev_launch.size();
+    return ELA_OK;
+}
+
+size_t ela_batch_bytes_in_use(const ela_batch* B) {
+    if (!B) return 0;
+    size_t s = 0;
+    for (auto* r : B->members) s += r->bytes;
+    return s;
+}
+
+}  // extern "C"

@@ -44,16 +44,36 @@ struct Inv {
     T imix[3];   // 0.25 / (h_a h_b) for the pairs (x,y), (x,z), (y,z)
 };
 
-// Stiffness accessors.  PlaneAcc: ncomp planes in the level layout (the
+// Stiffness accessors.  rec(o) is a view of node o's stiffness record, indexed
+// by storage slot.  PlaneAcc: ncomp planes in the level layout (the
 // reference's per-node model field).  TabAcc: a per-node grain id and one
 // record per grain (nc values, contiguous); two nodes of the same grain have
 // bitwise-identical records, so their central difference is exactly zero.
+// GrainAcc: a grain table times an optional per-node factor s (a degraded
+// polycrystal, s = g(eta)); the stiffness of node o is s[o] * tab[gid[o]].
+template <typename T>
+struct PlaneRec {
+    const T* p;
+    long long cs;
+    __device__ __forceinline__ T operator[](int q) const { return p[q * cs]; }
+};
+template <typename T>
+struct TabRec {
+    const T* p;
+    __device__ __forceinline__ T operator[](int q) const { return p[q]; }
+};
+template <typename T>
+struct ScaledRec {
+    const T* p;
+    T s;
+    __device__ __forceinline__ T operator[](int q) const { return s * p[q]; }
+};
+
 template <typename T>
 struct PlaneAcc {
     const T* C;
     long long cs;
-    __device__ __forceinline__ const T* rec(long long o) const { return C + o; }
-    __device__ __forceinline__ long long stride() const { return cs; }
+    __device__ __forceinline__ PlaneRec<T> rec(long long o) const { return PlaneRec<T>{C + o, cs}; }
     __device__ __forceinline__ bool same(long long, long long) const { return false; }
 };
 template <typename T, int NC>
@@ -61,14 +81,25 @@ struct TabAcc {
     const unsigned short* gid;
     const T* tab;
     const T* dgt;  // per-grain interior diagonal (3 values), may be null
-    __device__ __forceinline__ const T* rec(long long o) const { return tab + (long long)gid[o] * NC; }
-    __device__ __forceinline__ long long stride() const { return 1; }
+    __device__ __forceinline__ TabRec<T> rec(long long o) const { return TabRec<T>{tab + (long long)gid[o] * NC}; }
     __device__ __forceinline__ bool same(long long a, long long b) const { return gid[a] == gid[b]; }
+};
+template <typename T, int NC>
+struct GrainAcc {
+    const unsigned short* gid;
+    const T* tab;
+    const T* s;  // per-node factor, may be null (factor 1)
+    __device__ __forceinline__ ScaledRec<T> rec(long long o) const {
+        return ScaledRec<T>{tab + (long long)gid[o] * NC, s ? s[o] : T(1)};
+    }
+    __device__ __forceinline__ bool same(long long a, long long b) const {
+        return gid[a] == gid[b] && (!s || s[a] == s[b]);
+    }
 };
 
 // Jacobi diagonal of a node: the stored field, or the per-grain interior value.
-template <typename T>
-__device__ __forceinline__ T jdiag(const PlaneAcc<T>&, const T* dg, long long q, long long, int, bool) {
+template <typename T, typename Acc>
+__device__ __forceinline__ T jdiag(const Acc&, const T* dg, long long q, long long, int, bool) {
     return dg[q];
 }
 template <typename T, int NC>
@@ -173,11 +204,11 @@ __device__ __forceinline__ void normal_of(int face, T n[3]) {
     }
 }
 
-// sigma_ij = C_ijkl g_kl at one node, stiffness planes c[q * cs].
-template <typename T, int K, int DIM>
-__device__ __forceinline__ void stress(const T* __restrict__ c, long long cs, const T g[DIM][DIM], T s[DIM][DIM]) {
+// sigma_ij = C_ijkl g_kl at one node; c is the node's stiffness record.
+template <typename T, int K, int DIM, typename Rec>
+__device__ __forceinline__ void stress(const Rec& c, const T g[DIM][DIM], T s[DIM][DIM]) {
     if (K == K_ISO) {
-        const T lam = c[0], mu = c[cs];
+        const T lam = c[0], mu = c[1];
         T tr = 0;
 #pragma unroll
         for (int a = 0; a < DIM; ++a) tr += g[a][a];
@@ -199,7 +230,7 @@ __device__ __forceinline__ void stress(const T* __restrict__ c, long long cs, co
 #pragma unroll
                 for (int k = 0; k < DIM; ++k)
 #pragma unroll
-                    for (int l = k; l < DIM; ++l) acc += c[mm_slot<DIM>(a, b, k, l) * cs] * gs[k][l];
+                    for (int l = k; l < DIM; ++l) acc += c[mm_slot<DIM>(a, b, k, l)] * gs[k][l];
                 s[a][b] = acc;
                 s[b][a] = acc;
             }
@@ -207,7 +238,7 @@ __device__ __forceinline__ void stress(const T* __restrict__ c, long long cs, co
 #pragma unroll
         for (int a = 0; a < DIM; ++a)
 #pragma unroll
-            for (int b = 0; b < DIM; ++b) s[a][b] = c[(a * DIM + b) * cs] * g[a][b];
+            for (int b = 0; b < DIM; ++b) s[a][b] = c[a * DIM + b] * g[a][b];
     }
 }
 
@@ -257,7 +288,7 @@ __device__ __forceinline__ void node_apply_acc(const Lay& L, const Acc& A, const
                 else g[p][d] = (up[st[d]] - up[-st[d]]) * iv.ih2[d];
             }
         }
-        stress<T, K, DIM>(A.rec(o), A.stride(), g, s);
+        stress<T, K, DIM>(A.rec(o), g, s);
         bc_row<T, DIM>(bc, face_of<DIM>(i, j, k, hx, hy, hz), uc, g, s, f);
         return;
     }
@@ -283,10 +314,9 @@ __device__ __forceinline__ void node_apply_acc(const Lay& L, const Acc& A, const
                 gg[p][b][a] = m;
             }
     }
-    const T* c = A.rec(o);
-    const long long cs = A.stride();
+    const auto c = A.rec(o);
     if (K == K_ISO) {
-        const T lam = c[0], mu = c[cs];
+        const T lam = c[0], mu = c[1];
 #pragma unroll
         for (int a = 0; a < DIM; ++a) {
             T lap = 0, gd = 0;
@@ -306,10 +336,10 @@ __device__ __forceinline__ void node_apply_acc(const Lay& L, const Acc& A, const
             for (int d = 0; d < DIM; ++d) {
                 dl[d] = dm[d] = T(0);
                 if (!A.same(o + st[d], o - st[d])) {
-                    const T* cp = A.rec(o + st[d]);
-                    const T* cm = A.rec(o - st[d]);
+                    const auto cp = A.rec(o + st[d]);
+                    const auto cm = A.rec(o - st[d]);
                     dl[d] = (cp[0] - cm[0]) * iv.ih2[d];
-                    dm[d] = (cp[cs] - cm[cs]) * iv.ih2[d];
+                    dm[d] = (cp[1] - cm[1]) * iv.ih2[d];
                 }
             }
 #pragma unroll
@@ -329,7 +359,7 @@ __device__ __forceinline__ void node_apply_acc(const Lay& L, const Acc& A, const
 #pragma unroll
                 for (int kk = 0; kk < DIM; ++kk)
 #pragma unroll
-                    for (int l = 0; l < DIM; ++l) acc += c[mm_slot<DIM>(a, b, kk, l) * cs] * gg[kk][l][b];
+                    for (int l = 0; l < DIM; ++l) acc += c[mm_slot<DIM>(a, b, kk, l)] * gg[kk][l][b];
             f[a] = acc;
         }
         if (!uniform) {
@@ -344,14 +374,14 @@ __device__ __forceinline__ void node_apply_acc(const Lay& L, const Acc& A, const
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
                     if (A.same(o + st[d], o - st[d])) continue;
-                    const T* cp = A.rec(o + st[d]);
-                    const T* cm = A.rec(o - st[d]);
+                    const auto cp = A.rec(o + st[d]);
+                    const auto cm = A.rec(o - st[d]);
                     T acc = 0;
 #pragma unroll
                     for (int kk = 0; kk < DIM; ++kk)
 #pragma unroll
                         for (int l = kk; l < DIM; ++l) {
-                            const long long q = mm_slot<DIM>(a, d, kk, l) * cs;
+                            const int q = mm_slot<DIM>(a, d, kk, l);
                             acc += (cp[q] - cm[q]) * gs[kk][l];
                         }
                     add += acc * iv.ih2[d];
@@ -364,7 +394,7 @@ __device__ __forceinline__ void node_apply_acc(const Lay& L, const Acc& A, const
         for (int a = 0; a < DIM; ++a) {
             T acc = 0;
 #pragma unroll
-            for (int b = 0; b < DIM; ++b) acc += c[(a * DIM + b) * cs] * gg[a][b][b];
+            for (int b = 0; b < DIM; ++b) acc += c[a * DIM + b] * gg[a][b][b];
             f[a] = acc;
         }
         if (!uniform) {
@@ -374,7 +404,7 @@ __device__ __forceinline__ void node_apply_acc(const Lay& L, const Acc& A, const
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
                     if (A.same(o + st[d], o - st[d])) continue;
-                    const long long q = (a * DIM + d) * cs;
+                    const int q = a * DIM + d;
                     add += (A.rec(o + st[d])[q] - A.rec(o - st[d])[q]) * iv.ih2[d] * g[a][d];
                 }
                 f[a] += add;
@@ -390,16 +420,15 @@ __device__ __forceinline__ void node_apply(const Lay& L, const T* __restrict__ C
 }
 
 // Diagonal of A at node (i,j,k) (Elastic.cpp:285-349), FP64.
-template <int K, int DIM>
-__device__ __forceinline__ void node_diag(const Lay& L, const double* __restrict__ C, int i, int j, int k,
-                                          const BCTab& bc, const double h[3], double dg[DIM]) {
+template <int K, int DIM, typename Acc>
+__device__ __forceinline__ void node_diag_acc(const Lay& L, const Acc& A, int i, int j, int k, const BCTab& bc,
+                                              const double h[3], double dg[DIM]) {
     const long long o = (long long)i + L.px * j + L.pl * k;
     const int hx = L.nx - 1, hy = L.ny - 1, hz = L.nz - 1;
     const bool mn[3] = {i == 0, j == 0, DIM == 3 && k == 0};
     const bool mx[3] = {i == hx, j == hy, DIM == 3 && k == hz};
     const bool bnd = mn[0] || mx[0] || mn[1] || mx[1] || mn[2] || mx[2];
-    const double* c = C + o;
-    const long long cs = L.cs;
+    const auto c = A.rec(o);
 #pragma unroll
     for (int p = 0; p < DIM; ++p) {
         if (bnd) {
@@ -413,7 +442,7 @@ __device__ __forceinline__ void node_diag(const Lay& L, const double* __restrict
                     g[q][d] = ((!mx[d] ? 0.0 : e) - (!mn[d] ? 0.0 : e)) / ((mn[d] || mx[d] ? 1.0 : 2.0) * h[d]);
                 }
             }
-            stress<double, K, DIM>(c, cs, g, s);
+            stress<double, K, DIM>(c, g, s);
             bc_row<double, DIM>(bc, face_of<DIM>(i, j, k, hx, hy, hz), uu, g, s, f);
             dg[p] = f[p];
         } else {
@@ -422,14 +451,120 @@ __device__ __forceinline__ void node_diag(const Lay& L, const double* __restrict
 #pragma unroll
             for (int b = 0; b < DIM; ++b) {
                 double cpp;
-                if (K == K_ISO) cpp = c[cs] + (b == p ? c[cs] + c[0] : 0.0);
-                else if (K == K_MM) cpp = c[mm_slot<DIM>(p, b, p, b) * cs];
-                else cpp = c[(p * DIM + b) * cs];
+                if (K == K_ISO) cpp = c[1] + (b == p ? c[1] + c[0] : 0.0);
+                else if (K == K_MM) cpp = c[mm_slot<DIM>(p, b, p, b)];
+                else cpp = c[p * DIM + b];
                 acc += cpp * (-2.0 / h[b] / h[b]);
             }
             dg[p] = acc;
         }
     }
+}
+
+template <int K, int DIM>
+__device__ __forceinline__ void node_diag(const Lay& L, const double* __restrict__ C, int i, int j, int k,
+                                          const BCTab& bc, const double h[3], double dg[DIM]) {
+    node_diag_acc<K, DIM>(L, PlaneAcc<double>{C, L.cs}, i, j, k, bc, h, dg);
+}
+
+// Nodal full weighting onto coarse node (I,J,K) of Lc from the fine level Lf
+// (Operator.cpp:486-543; the same stencil coarsens the stiffness,
+// Elastic.cpp:759-803).  F(o) returns the fine value at offset o.  In 2D the k
+// direction counts as "on the boundary" (the reference's degenerate z box),
+// which selects the 2D branches.
+template <typename T, int DIM, typename Get>
+__device__ __forceinline__ T restrict_node(const Lay& Lc, const Lay& Lf, int I, int J, int K, const Get& F) {
+    const bool bI = I == 0 || I == Lc.nx - 1, bJ = J == 0 || J == Lc.ny - 1;
+    const bool bK = DIM == 2 ? true : (K == 0 || K == Lc.nz - 1);
+    const long long sx = 1, sy = Lf.px, sz = Lf.pl;
+    const long long o = 2LL * I + Lf.px * (2LL * J) + Lf.pl * (2LL * K);
+    if (bI && bJ && bK) return F(o);
+    if (bJ && bK) return T(0.25) * F(o - sx) + T(0.5) * F(o) + T(0.25) * F(o + sx);
+    if (bK && bI) return T(0.25) * F(o - sy) + T(0.5) * F(o) + T(0.25) * F(o + sy);
+    if (bI && bJ) return T(0.25) * F(o - sz) + T(0.5) * F(o) + T(0.25) * F(o + sz);
+    if (bI || bJ || bK) {
+        // face: (1 2 1) x (1 2 1) / 16 over the two tangential directions
+        const long long u1 = bI ? sy : sx, u2 = bK ? sy : sz;
+        return (F(o - u1 - u2) + T(2) * F(o - u2) + F(o + u1 - u2) + T(2) * F(o - u1) + T(4) * F(o) +
+                T(2) * F(o + u1) + F(o - u1 + u2) + T(2) * F(o + u2) + F(o + u1 + u2)) /
+               T(16);
+    }
+    T corners = 0, edges = 0, faces = 0;
+#pragma unroll
+    for (int a = -1; a <= 1; a += 2)
+#pragma unroll
+        for (int bb = -1; bb <= 1; bb += 2)
+#pragma unroll
+            for (int c = -1; c <= 1; c += 2) corners += F(o + a * sx + bb * sy + c * sz);
+#pragma unroll
+    for (int a = -1; a <= 1; a += 2)
+#pragma unroll
+        for (int bb = -1; bb <= 1; bb += 2)
+            edges += F(o + a * sy + bb * sz) + F(o + a * sx + bb * sz) + F(o + a * sx + bb * sy);
+#pragma unroll
+    for (int a = -1; a <= 1; a += 2) faces += F(o + a * sx) + F(o + a * sy) + F(o + a * sz);
+    return corners / T(64) + edges / T(32) + faces / T(16) + F(o) / T(8);
+}
+
+// Trilinear nodal prolongation (Operator.cpp:589-618): the coarse-grid
+// interpolant at fine node (i,j,k); c points at coarse node (i/2, j/2, k/2).
+template <typename T>
+__device__ __forceinline__ T prolong_node(const Lay& Lc, int i, int j, int k, const T* __restrict__ c) {
+    const bool ei = (i & 1) == 0, ej = (j & 1) == 0, ek = (k & 1) == 0;
+    const long long sx = 1, sy = Lc.px, sz = Lc.pl;
+    if (ei && ej && ek) return c[0];
+    if (ej && ek) return T(0.5) * (c[0] + c[sx]);
+    if (ek && ei) return T(0.5) * (c[0] + c[sy]);
+    if (ei && ej) return T(0.5) * (c[0] + c[sz]);
+    if (ei) return T(0.25) * (c[0] + c[sy] + c[sz] + c[sy + sz]);
+    if (ej) return T(0.25) * (c[0] + c[sz] + c[sx] + c[sx + sz]);
+    if (ek) return T(0.25) * (c[0] + c[sx] + c[sy] + c[sx + sy]);
+    return T(0.125) * (c[0] + c[sx] + c[sy] + c[sz] + c[sy + sz] + c[sx + sz] + c[sx + sy] + c[sx + sy + sz]);
+}
+
+// Gauss-Jordan step for pivot column c on M = [A | A^-1-in-progress] (n x 2n,
+// row-major), executed by one block: find the pivot row (largest |M[r][c]|,
+// r >= c, lowest index on ties), swap it into row c and scale row c.  sval /
+// sidx are blockDim.x entries of LDS.  Sets *flag on a zero pivot.
+__device__ __forceinline__ void gj_pivot_block(double* __restrict__ M, int n, int c, int* flag, double* sval,
+                                               int* sidx) {
+    const int tid = threadIdx.x, nt = blockDim.x, w = 2 * n;
+    double best = -1.0;
+    int bi = c;
+    for (int r = c + tid; r < n; r += nt) {
+        const double a = fabs(M[(long long)r * w + c]);
+        if (a > best) { best = a; bi = r; }
+    }
+    sval[tid] = best;
+    sidx[tid] = bi;
+    __syncthreads();
+    for (int st = nt / 2; st > 0; st >>= 1) {
+        if (tid < st && (sval[tid + st] > sval[tid] || (sval[tid + st] == sval[tid] && sidx[tid + st] < sidx[tid]))) {
+            sval[tid] = sval[tid + st];
+            sidx[tid] = sidx[tid + st];
+        }
+        __syncthreads();
+    }
+    const int p = sidx[0];
+    if (tid == 0 && !(sval[0] > 0.0)) *flag = 1;
+    const double d = M[(long long)p * w + c];
+    __syncthreads();
+    for (int q = tid; q < w; q += nt) {
+        const double vp = M[(long long)p * w + q];
+        if (p != c) M[(long long)p * w + q] = M[(long long)c * w + q];
+        M[(long long)c * w + q] = vp / d;
+    }
+}
+
+// Row r != c of the same step: M[r][q] -= M[r][c] * M[c][q], by one block.
+// The multiplier is read before any thread of the block writes column c.
+__device__ __forceinline__ void gj_eliminate_row(double* __restrict__ M, int n, int c, int r) {
+    const int w = 2 * n;
+    if (r == c) return;
+    const double m = M[(long long)r * w + c];
+    __syncthreads();
+    if (m == 0.0) return;
+    for (int q = threadIdx.x; q < w; q += blockDim.x) M[(long long)r * w + q] -= m * M[(long long)c * w + q];
 }
 
 }  // namespace ela

@@ -200,80 +200,29 @@ __global__ __launch_bounds__(256) void k_diag(Lay L, const double* __restrict__ 
     }
 }
 
-// Nodal full weighting from the fine level (Lf) onto the coarse level (Lc),
-// nc components (Operator.cpp:486-543; the same stencil coarsens the stiffness,
-// Elastic.cpp:759-803).  In 2D the k direction counts as "on the boundary"
-// (the reference's degenerate z box), which selects the 2D branches.
+// Nodal full weighting from the fine level (Lf) onto the coarse level (Lc), nc components.
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void k_restrict(Lay Lc, Lay Lf, int nc, const T* __restrict__ fine,
                                                   T* __restrict__ crse) {
     int I, J, K;
     if (!node_index(Lc, I, J, K)) return;
-    const int i = 2 * I, j = 2 * J, k = 2 * K;
-    const bool bI = I == 0 || I == Lc.nx - 1, bJ = J == 0 || J == Lc.ny - 1;
-    const bool bK = DIM == 2 ? true : (K == 0 || K == Lc.nz - 1);
-    const long long sx = 1, sy = Lf.px, sz = Lf.pl;
-    const long long of = (long long)i + Lf.px * j + Lf.pl * k;
     const long long oc = (long long)I + Lc.px * J + Lc.pl * K;
     for (int n = 0; n < nc; ++n) {
-        const T* F = fine + n * Lf.cs + of;
-        T v;
-        if (bI && bJ && bK) {
-            v = F[0];
-        } else if (bJ && bK) {
-            v = T(0.25) * F[-sx] + T(0.5) * F[0] + T(0.25) * F[sx];
-        } else if (bK && bI) {
-            v = T(0.25) * F[-sy] + T(0.5) * F[0] + T(0.25) * F[sy];
-        } else if (bI && bJ) {
-            v = T(0.25) * F[-sz] + T(0.5) * F[0] + T(0.25) * F[sz];
-        } else if (bI || bJ || bK) {
-            // face: (1 2 1) x (1 2 1) / 16 over the two tangential directions
-            const long long u1 = bI ? sy : sx, u2 = bK ? sy : sz;
-            v = (F[-u1 - u2] + T(2) * F[-u2] + F[u1 - u2] + T(2) * F[-u1] + T(4) * F[0] + T(2) * F[u1] +
-                 F[-u1 + u2] + T(2) * F[u2] + F[u1 + u2]) /
-                T(16);
-        } else {
-            T corners = 0, edges = 0, faces = 0;
-#pragma unroll
-            for (int a = -1; a <= 1; a += 2)
-#pragma unroll
-                for (int bb = -1; bb <= 1; bb += 2)
-#pragma unroll
-                    for (int c = -1; c <= 1; c += 2) corners += F[a * sx + bb * sy + c * sz];
-#pragma unroll
-            for (int a = -1; a <= 1; a += 2)
-#pragma unroll
-                for (int bb = -1; bb <= 1; bb += 2) edges += F[a * sy + bb * sz] + F[a * sx + bb * sz] + F[a * sx + bb * sy];
-#pragma unroll
-            for (int a = -1; a <= 1; a += 2) faces += F[a * sx] + F[a * sy] + F[a * sz];
-            v = corners / T(64) + edges / T(32) + faces / T(16) + F[0] / T(8);
-        }
-        crse[n * Lc.cs + oc] = v;
+        const T* F = fine + n * Lf.cs;
+        crse[n * Lc.cs + oc] = restrict_node<T, DIM>(Lc, Lf, I, J, K, [F](long long o) { return F[o]; });
     }
 }
 
-// Trilinear nodal prolongation (Operator.cpp:589-618): fine (+)= P crse.
+// Trilinear nodal prolongation: fine (+)= P crse.
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void k_prolong(Lay Lf, Lay Lc, int nc, const T* __restrict__ crse,
                                                  T* __restrict__ fine, int add) {
     int i, j, k;
     if (!node_index(Lf, i, j, k)) return;
-    const int I = i >> 1, J = j >> 1, K = k >> 1;
-    const bool ei = (i & 1) == 0, ej = (j & 1) == 0, ek = (k & 1) == 0;
-    const long long sx = 1, sy = Lc.px, sz = Lc.pl;
-    const long long oc = (long long)I + Lc.px * J + Lc.pl * K;
+    const long long oc = (long long)(i >> 1) + Lc.px * (j >> 1) + Lc.pl * (k >> 1);
     const long long of = (long long)i + Lf.px * j + Lf.pl * k;
     for (int n = 0; n < nc; ++n) {
-        const T* c = crse + n * Lc.cs + oc;
-        T v;
-        if (ei && ej && ek) v = c[0];
-        else if (ej && ek) v = T(0.5) * (c[0] + c[sx]);
-        else if (ek && ei) v = T(0.5) * (c[0] + c[sy]);
-        else if (ei && ej) v = T(0.5) * (c[0] + c[sz]);
-        else if (ei) v = T(0.25) * (c[0] + c[sy] + c[sz] + c[sy + sz]);
-        else if (ej) v = T(0.25) * (c[0] + c[sz] + c[sx] + c[sx + sz]);
-        else if (ek) v = T(0.25) * (c[0] + c[sx] + c[sy] + c[sx + sy]);
-        else v = T(0.125) * (c[0] + c[sx] + c[sy] + c[sz] + c[sy + sz] + c[sx + sz] + c[sx + sy] + c[sx + sy + sz]);
+        const T v = prolong_node<T>(Lc, i, j, k, crse + n * Lc.cs + oc);
         T* fp = fine + n * Lf.cs + of;
         *fp = add ? (*fp + v) : v;
     }
@@ -377,50 +326,16 @@ __global__ __launch_bounds__(256) void k_bottom(Lay L, int nodes, int dim, const
     }
 }
 
-// Gauss-Jordan step for pivot column c on M = [A | A^-1-in-progress] (n x 2n):
-// one block finds the pivot row (largest |M[r][c]|, r >= c, lowest index on
-// ties), swaps it into row c and scales row c; then k_gj_eliminate clears
-// column c in every other row.
+// Gauss-Jordan on the dense coarsest operator, one column per launch pair
+// (the step bodies are gj_pivot_block / gj_eliminate_row).
 __global__ __launch_bounds__(1024) void k_gj_pivot(double* __restrict__ M, int n, int c, int* flag) {
     __shared__ double sval[1024];
     __shared__ int sidx[1024];
-    const int tid = threadIdx.x, nt = blockDim.x, w = 2 * n;
-    double best = -1.0;
-    int bi = c;
-    for (int r = c + tid; r < n; r += nt) {
-        const double a = fabs(M[(long long)r * w + c]);
-        if (a > best) { best = a; bi = r; }
-    }
-    sval[tid] = best;
-    sidx[tid] = bi;
-    __syncthreads();
-    for (int st = nt / 2; st > 0; st >>= 1) {
-        if (tid < st && (sval[tid + st] > sval[tid] || (sval[tid + st] == sval[tid] && sidx[tid + st] < sidx[tid]))) {
-            sval[tid] = sval[tid + st];
-            sidx[tid] = sidx[tid + st];
-        }
-        __syncthreads();
-    }
-    const int p = sidx[0];
-    if (tid == 0 && !(sval[0] > 0.0)) *flag = 1;
-    const double d = M[(long long)p * w + c];
-    __syncthreads();
-    for (int q = tid; q < w; q += nt) {
-        const double vp = M[(long long)p * w + q];
-        if (p != c) M[(long long)p * w + q] = M[(long long)c * w + q];
-        M[(long long)c * w + q] = vp / d;
-    }
+    gj_pivot_block(M, n, c, flag, sval, sidx);
 }
 
-// Row r != c: M[r][q] -= M[r][c] * M[c][q].  One block per row; the
-// multiplier is read before any thread of the block writes column c.
 __global__ __launch_bounds__(256) void k_gj_eliminate(double* __restrict__ M, int n, int c) {
-    const int r = blockIdx.x, w = 2 * n;
-    if (r == c) return;
-    const double m = M[(long long)r * w + c];
-    __syncthreads();
-    if (m == 0.0) return;
-    for (int q = threadIdx.x; q < w; q += blockDim.x) M[(long long)r * w + q] -= m * M[(long long)c * w + q];
+    gj_eliminate_row(M, n, c, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- stiffness evaluation

@@ -28,26 +28,44 @@
 #include <vector>
 
 #include "../../include/ela.h"
+#include "ela_internal.hpp"
 #include "ela_kernels.hpp"
 #include "ela_march.hpp"
 
 using namespace ela;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const std::string& msg) {
+namespace ela {
+
+int set_error(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
 
-#define HIPCHK(expr)                                                                              \
-    do {                                                                                          \
-        hipError_t e_ = (expr);                                                                   \
-        if (e_ != hipSuccess)                                                                     \
-            return fail(ELA_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
-    } while (0)
+int budget_reserve(ela_ctx* ctx, size_t bytes) {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (ctx->budget && ctx->in_use + bytes > ctx->budget)
+        return set_error(ELA_ERR_OOM, "HBM budget exceeded: need " + std::to_string(bytes) + " more bytes, " +
+                                          std::to_string(ctx->budget - ctx->in_use) + " left");
+    ctx->in_use += bytes;
+    return ELA_OK;
+}
+
+void budget_release(ela_ctx* ctx, size_t bytes) {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->in_use -= bytes;
+}
+
+}  // namespace ela
+
+namespace {
+
+int fail(int code, const std::string& msg) { return set_error(code, msg); }
+
+#define HIPCHK(expr) ELA_HIPCHK(expr)
 
 constexpr int kMaxDenseUnknowns = 3 * 9 * 9 * 9;
 constexpr int kMaxEvents = 8192;
@@ -81,13 +99,6 @@ template <>
 LB<float> lb<float>(Level& v) { return lb32(v); }
 
 }  // namespace
-
-struct ela_ctx {
-    int device = 0;
-    size_t budget = 0;
-    size_t in_use = 0;
-    std::mutex mu;
-};
 
 struct ela_hier {
     ela_ctx* ctx = nullptr;
@@ -142,17 +153,10 @@ namespace {
 int halloc(ela_hier* h, size_t bytes, void** out) {
     *out = nullptr;
     if (bytes == 0) return ELA_OK;
-    {
-        std::lock_guard<std::mutex> g(h->ctx->mu);
-        if (h->ctx->budget && h->ctx->in_use + bytes > h->ctx->budget)
-            return fail(ELA_ERR_OOM, "HBM budget exceeded: need " + std::to_string(bytes) + " more bytes, " +
-                                         std::to_string(h->ctx->budget - h->ctx->in_use) + " left");
-        h->ctx->in_use += bytes;
-    }
+    if (int rc = budget_reserve(h->ctx, bytes)) return rc;
     hipError_t e = hipMalloc(out, bytes);
     if (e != hipSuccess) {
-        std::lock_guard<std::mutex> g(h->ctx->mu);
-        h->ctx->in_use -= bytes;
+        budget_release(h->ctx, bytes);
         *out = nullptr;
         return fail(ELA_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
     }
@@ -175,8 +179,7 @@ void hfree(ela_hier* h, void* p) {
     for (size_t q = 0; q < h->allocs.size(); ++q)
         if (h->allocs[q].first == p) {
             (void)hipFree(p);
-            std::lock_guard<std::mutex> g(h->ctx->mu);
-            h->ctx->in_use -= h->allocs[q].second;
+            budget_release(h->ctx, h->allocs[q].second);
             h->bytes -= h->allocs[q].second;
             h->allocs.erase(h->allocs.begin() + q);
             return;
@@ -607,7 +610,13 @@ size_t ela_ctx_bytes_in_use(const ela_ctx* ctx) { return ctx ? ctx->in_use : 0; 
 
 int ela_hier_create(ela_ctx* ctx, const ela_domain* dom, int model_kind, int max_mg_levels, int mode,
                     ela_hier** out) {
+    return ela_hier_create_ex(ctx, dom, model_kind, max_mg_levels, 4, mode, out);
+}
+
+int ela_hier_create_ex(ela_ctx* ctx, const ela_domain* dom, int model_kind, int max_mg_levels, int coarse_min,
+                       int mode, ela_hier** out) {
     if (!ctx || !dom || !out) return fail(ELA_ERR_BADARG, "null argument");
+    if (coarse_min < 2) return fail(ELA_ERR_BADARG, "coarse_min must be >= 2");
     *out = nullptr;
     if (dom->dim != 2 && dom->dim != 3) return fail(ELA_ERR_BADARG, "dim must be 2 or 3");
     for (int d = 0; d < dom->dim; ++d) {
@@ -657,7 +666,7 @@ int ela_hier_create(ela_ctx* ctx, const ela_domain* dom, int model_kind, int max
         if (!h->use_march) v.mp.ok = false;
         h->lev.push_back(v);
         bool can = true;
-        for (int d = 0; d < dom->dim; ++d) can = can && (n[d] % 2 == 0) && (n[d] >= 8);
+        for (int d = 0; d < dom->dim; ++d) can = can && (n[d] % 2 == 0) && (n[d] / 2 >= coarse_min);
         if (max_mg_levels > 0 && (int)h->lev.size() >= max_mg_levels) can = false;
         if (!can) break;
         for (int d = 0; d < dom->dim; ++d) { n[d] /= 2; dx[d] *= 2; }

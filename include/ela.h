@@ -99,6 +99,11 @@ size_t ela_ctx_bytes_in_use(const ela_ctx* ctx);
  * level-0 residual) or ELA_ALLOC_BOTH. */
 int ela_hier_create(ela_ctx* ctx, const ela_domain* dom, int model_kind, int max_mg_levels, int mode,
                     ela_hier** out);
+/* The same with the coarsening floor as a parameter: a level is coarsened
+ * while every n is even and n / 2 >= coarse_min (ela_hier_create uses 4).
+ * Problems with Displacement rows on every face may go down to 2 cells. */
+int ela_hier_create_ex(ela_ctx* ctx, const ela_domain* dom, int model_kind, int max_mg_levels, int coarse_min,
+                       int mode, ela_hier** out);
 int ela_hier_destroy(ela_hier* h);
 int ela_hier_levels(const ela_hier* h);
 int ela_level_info(const ela_hier* h, int mglev, ela_level_layout* out);
@@ -220,6 +225,105 @@ int ela_time_fapply(ela_hier* h, int prec, int nrep, double* avg_ms);
  * plain streaming copy of `bytes` on the device (reads and writes `bytes`
  * each), average ms per launch.  It calibrates the HBM byte counters. */
 int ela_probe_copy(ela_ctx* ctx, size_t bytes, int nrep, double* avg_ms);
+
+/* ------------------------------------------------------------------ batched independent solves
+ *
+ * The reference solves one problem per Solver::Nonlocal::Linear::solve call
+ * (src/Solver/Nonlocal/Linear.H:102-112).  A batch holds many independent
+ * members ("RVEs": cubes of n cells per side on [0,1]^3, cubic Voronoi
+ * polycrystals) with their multigrid hierarchies resident on the device, and
+ * runs each kernel of a cycle as one launch over all members.  Per member it
+ * is the same discrete problem and the same F-cycle iteration as ela_solve
+ * with coarse_min = 2; a member's iterates do not depend on the other
+ * members, so its solution is bitwise the same alone and in any mix.
+ *
+ * Members: n = c * 2^k with c <= 7 (the coarsest level is c cells, solved by
+ * a dense inverse).  Fields on the host are dense (x fastest), 3 components. */
+typedef struct ela_batch ela_batch;
+typedef struct ela_rve ela_rve;
+
+/* mode: ELA_SOLVE_MIXED (FP32 cycles inside FP64 refinement) or ELA_SOLVE_FP64. */
+int ela_batch_create(ela_ctx* ctx, int mode, ela_batch** out);
+int ela_batch_destroy(ela_batch* b);
+size_t ela_batch_bytes_in_use(const ela_batch* b);
+
+/* Device bytes a member needs (admission control against the context budget). */
+size_t ela_rve_bytes(int n, int mode, int max_grains, int with_factor);
+
+/* A member of n cells per side, its hierarchy allocated against the context's
+ * HBM budget (ELA_ERR_OOM when it does not fit).  Replaces, per member,
+ * Operator::Elastic<T>::define (src/Operator/Elastic.cpp:30-55).  with_factor
+ * reserves a per-node degradation factor (ela_rve_set_crack).  Every BC type
+ * starts as Displacement, the reference's default (BC/Operator/Elastic.H:46-55). */
+int ela_rve_create(ela_batch* b, int n, int max_grains, int with_factor, ela_rve** out);
+int ela_rve_destroy(ela_rve* r);
+size_t ela_rve_bytes_in_use(const ela_rve* r);
+int ela_rve_levels(const ela_rve* r);
+
+/* SetBC types (src/Operator/Elastic.H:58-66): types[26][3], faces as ela_set_bc. */
+int ela_rve_set_bc_types(ela_rve* r, const int* types);
+
+/* SetUniform (src/Operator/Elastic.H:111): uniform != 0 drops the grad(C) term (default 0). */
+int ela_rve_set_uniform(ela_rve* r, int uniform);
+
+/* Per-node cubic stiffness of a Voronoi polycrystal (Cubic::Define,
+ * src/Model/Solid/Linear/Cubic.H:30-59; the per-node fill of
+ * src/Integrator/PhaseFieldMicrostructure.cpp:529-543), stored as a 16-bit
+ * grain id per node and one record per grain.  Seeds as in
+ * ela_set_model_voronoi_cubic. */
+int ela_rve_set_voronoi_cubic(ela_rve* r, int nseeds, const int64_t* seeds, int q, const double* R, double C11,
+                              double C12, double C44);
+
+/* Phase-field degradation of the stiffness (the role of
+ * LinearElastic::Degradable::Isotropic::DegradeModulus,
+ * src/Model/Solid/LinearElastic/Degradable/Isotropic.H:74-104, applied to the
+ * cubic polycrystal): C(x) = g(eta(x)) C_grain(x), g(eta) = a eta^2 + b eta + c,
+ * eta = (1 - tanh(d / width)) / 2 with d the distance to the crack
+ * {x : nrm.(x - p) = 0, tv.(x - p) <= 0} (a half-plane with front through p). */
+int ela_rve_set_crack(ela_rve* r, const double p[3], const double nrm[3], const double tv[3], double width, double a,
+                      double b, double c);
+
+/* Right-hand side for a prescribed affine displacement u = G x (G row-major
+ * 3x3) on the Displacement rows of boundary nodes and 0 elsewhere
+ * (BC::Operator::Elastic::Init, src/BC/Operator/Elastic.H:106-181, with
+ * node-dependent values).  Uses the member's current BC types. */
+int ela_rve_set_rhs_affine(ela_rve* r, const double G[9]);
+
+/* Host <-> member fields of the finest level: 0 = u, 1 = b, 2 = r (output of
+ * ela_batch_fapply), 3 = degradation factor (1 component, download only). */
+int ela_rve_upload(ela_rve* r, int field, const double* host);
+int ela_rve_download(const ela_rve* r, int field, double* host);
+int ela_rve_grain_ids(const ela_rve* r, uint16_t* host);
+
+/* prepareForSolve for a set of members (src/Operator/Operator.cpp:445-452):
+ * coarse stiffness, diagonals and the coarsest inverses, batched. */
+int ela_batch_prepare(ela_batch* b, int count, ela_rve* const* rves);
+
+/* Fapply (src/Operator/Elastic.cpp:151-260) on the finest level of each
+ * member, one launch: r = A u (prec ELA_F64), or u rounded to FP32 through the
+ * FP32 operator (ELA_F32, mixed batches). */
+int ela_batch_fapply(ela_batch* b, int count, ela_rve* const* rves, int prec);
+
+/* Linear::solve per member (src/Solver/Nonlocal/Linear.H:102-112): submit
+ * queues a member (u on the device is the initial guess, zero after
+ * create); each step admits the queued members and runs one F-cycle on every
+ * active member; members leave the set when |r|inf <= max(tol_abs,
+ * tol_rel |r0|inf), on max_cycles or on a non-finite residual.  n_active
+ * counts members still queued or active.  ela_batch_solve submits count
+ * members and steps until all are done. */
+int ela_batch_submit(ela_batch* b, ela_rve* r, double tol_rel, double tol_abs, int max_cycles);
+int ela_batch_step(ela_batch* b, int* n_active);
+int ela_batch_solve(ela_batch* b, int count, ela_rve* const* rves, double tol_rel, double tol_abs, int max_cycles);
+
+/* state: 0 idle, 1 queued, 2 active, 3 done; status of a finished solve
+ * (ELA_OK, ELA_ERR_NONCONVERGED, ELA_ERR_NAN); stats as for ela_solve
+ * (seconds = submit to done). */
+int ela_rve_status(const ela_rve* r, int* state, int* status, ela_stats* stats);
+
+/* HIP-event timing of every smoother launch of later steps (on != 0), and
+ * the totals since: milliseconds, nodes swept, launches. */
+int ela_batch_set_timing(ela_batch* b, int on);
+int ela_batch_smoother_times(ela_batch* b, double* ms, long long* nodes, int* launches);
 
 #ifdef __cplusplus
 }

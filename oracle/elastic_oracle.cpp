@@ -719,23 +719,26 @@ void or_assemble(const or_grid* og, int kind, const double* C, const int* bctype
 }
 
 // Multigrid solve of A u = b on one level (restated MLMG, see header).
-// levels: number of MG levels to use (<= 0: coarsen while every n is even and >= 8;
-// a 2-cell coarsest grid with traction faces is near-singular).
+// levels: number of MG levels to use (<= 0: no limit).  A level is coarsened
+// while every n is even and n / 2 >= coarse_min (4 when coarse_min <= 0: a
+// 2-cell coarsest grid with traction faces is near-singular; problems with
+// Displacement rows on every face can go down to 2 cells).
 // stats[0] iterations, [1] |r0|inf, [2] final |r|inf, [3] number of MG levels.
 // hist (optional, cap entries) receives |r|inf after each iteration.
 int or_mg_solve(const or_grid* og, int kind, const double* C, const int* bctype, const double* b, double* u,
                 double tol_rel, double tol_abs, int max_iter, int max_fmg_iter, int bottom_kind, int levels,
-                double* stats, double* hist, int cap) {
+                int coarse_min, double* stats, double* hist, int cap) {
     MG mg;
     mg.kind = kind;
     mg.bctype = bctype;
     mg.bottom_kind = bottom_kind;
     const int dim = og->dim, nc = ncomp_of(dim, kind);
     or_grid cur = *og;
+    const int cmin = coarse_min > 0 ? coarse_min : 4;
     for (;;) {
         mg.lev.emplace_back(cur);
         bool can = true;
-        for (int d = 0; d < dim; ++d) can = can && (cur.n[d] % 2 == 0) && (cur.n[d] >= 8);
+        for (int d = 0; d < dim; ++d) can = can && (cur.n[d] % 2 == 0) && (cur.n[d] / 2 >= cmin);
         if (levels > 0 && (int)mg.lev.size() >= levels) can = false;
         if (!can) break;
         for (int d = 0; d < dim; ++d) { cur.n[d] /= 2; cur.dx[d] *= 2; }

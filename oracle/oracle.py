@@ -71,7 +71,7 @@ def lib():
         L.or_cubic_define_R.argtypes = [i, d, d, d, vp, vp]
         L.or_cubic_rotation.argtypes = [d, d, d, vp]
         L.or_assemble.argtypes = [P(_Grid), i, vp, vp, vp]
-        L.or_mg_solve.argtypes = [P(_Grid), i, vp, vp, vp, vp, d, d, i, i, i, i, vp, vp, i]
+        L.or_mg_solve.argtypes = [P(_Grid), i, vp, vp, vp, vp, d, d, i, i, i, i, i, vp, vp, i]
         L.or_mg_solve.restype = i
         _lib = L
     return _lib
@@ -211,14 +211,15 @@ def assemble(g, kind, C, bct):
 
 
 def mg_solve(g, kind, C, bct, b, u0=None, tol_rel=1e-10, tol_abs=0.0, max_iter=200, max_fmg_iter=10 ** 8,
-             bottom="direct", levels=0, hist_cap=256):
+             bottom="direct", levels=0, hist_cap=256, coarse_min=4):
     """Restated MLMG solve; returns (u, stats dict, residual history)."""
     C, bct, b = _c(C), _c(bct, np.int32), _c(b)
     u = np.zeros_like(b) if u0 is None else _c(u0).copy()
     stats = np.zeros(4)
     hist = np.zeros(hist_cap)
     rc = lib().or_mg_solve(ctypes.byref(g.c()), kind, _p(C), _p(bct), _p(b), _p(u), tol_rel, tol_abs, max_iter,
-                           max_fmg_iter, 0 if bottom == "direct" else 1, levels, _p(stats), _p(hist), hist_cap)
+                           max_fmg_iter, 0 if bottom == "direct" else 1, levels, coarse_min, _p(stats), _p(hist),
+                           hist_cap)
     it = int(stats[0])
     return u, {"converged": rc == 0, "iters": it, "res0": stats[1], "res": stats[2], "levels": int(stats[3])}, hist[:it]
 

@@ -1,0 +1,222 @@
+"""Batched independent solves (ela_batch / ela_rve) against the oracle.
+
+Each member is a cubic Voronoi polycrystal on [0,1]^3 with Displacement rows on
+every face and the affine boundary displacement u = G x (the fixed-corner
+"periodic-like" RVE loading); some members carry a crack degradation factor.
+The checker is the CPU oracle on per-node stiffness planes rebuilt from the
+member's grain ids and the closed-form cubic table.
+"""
+import numpy as np
+import pytest
+
+import anisotropicfracture_amd as E
+import anisotropicfracture_amd.configs as K
+from helpers import cubic_table_numpy
+
+pytestmark = pytest.mark.gpu
+
+def problem(seed, n, ngrains, crack=False):
+    rng = np.random.default_rng(seed)
+    seeds = K.voronoi_seeds(rng, ngrains, (n, n, n))
+    R = K.haar_rotations(rng, ngrains)
+    a = rng.standard_normal(3)
+    a /= np.linalg.norm(a)
+    G = 0.01 * np.outer(a, a)
+    cr = None
+    if crack:
+        nrm = rng.standard_normal(3)
+        nrm /= np.linalg.norm(nrm)
+        tv = np.cross(nrm, rng.standard_normal(3))
+        tv /= np.linalg.norm(tv)
+        # soft crack: g = (1 - eta)^2 + 1e-3
+        cr = (rng.random(3) * 0.5 + 0.25, nrm, tv, 2.0 / n, 1.0, -2.0, 1.0 + 1e-3)
+    return {"n": n, "seeds": seeds, "R": R, "G": G, "crack": cr}
+
+
+def make(batch, p):
+    r = batch.rve(p["n"], max_grains=len(p["seeds"]), with_factor=p["crack"] is not None)
+    r.SetModelVoronoiCubic(p["seeds"], K.VORONOI_Q, p["R"].reshape(-1, 9), K.C11, K.C12, K.C44)
+    if p["crack"] is not None:
+        r.SetCrack(*p["crack"])
+    r.SetRhsAffine(p["G"])
+    return r
+
+
+def planes(r, p):
+    """Per-node stiffness planes of the member, rebuilt on the host."""
+    gid = r.grain_ids().astype(np.int64)
+    tab = np.stack([cubic_table_numpy(K.C11, K.C12, K.C44, Ri) for Ri in p["R"]])
+    C = np.moveaxis(tab[gid], -1, 0).copy()
+    if p["crack"] is not None:
+        C *= r.download(E.RVE.FACTOR)[None]
+    return C
+
+
+def all_disp():
+    return np.zeros((26, 3), dtype=np.int32)
+
+
+def oracle_solution(oracle, r, p, C):
+    n = p["n"]
+    b = r.download(E.RVE.B)
+    u, st, _ = oracle.mg_solve(oracle.Grid((n, n, n)), oracle.MM, C, all_disp(), b, tol_rel=1e-14, coarse_min=2)
+    assert st["res"] <= 1e-12 * st["res0"], st
+    return u
+
+
+def test_member_inputs_match_closed_forms():
+    b = E.Batch(E.SOLVE_MIXED)
+    p = problem(1, 16, 12, crack=True)
+    r = make(b, p)
+    n = p["n"]
+    gid = r.grain_ids()
+    assert np.array_equal(gid.astype(np.int64), K.voronoi_grain_ids(p["seeds"], (n, n, n)))
+    # degradation factor: eta from the distance to the half-plane
+    x = np.arange(n + 1) / n
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    P = np.stack([X, Y, Z])
+    c0, nrm, tv, w, a, bb, cc = p["crack"]
+    dn = np.tensordot(nrm, P - np.asarray(c0)[:, None, None, None], 1)
+    dt = np.tensordot(tv, P - np.asarray(c0)[:, None, None, None], 1)
+    eta = 0.5 * (1 - np.tanh(np.sqrt(dn ** 2 + np.maximum(dt, 0) ** 2) / w))
+    g = (a * eta + bb) * eta + cc
+    np.testing.assert_allclose(r.download(E.RVE.FACTOR), g, rtol=1e-13, atol=1e-15)
+    # right-hand side: G x on every boundary node (all rows Displacement), 0 inside
+    rhs = r.download(E.RVE.B)
+    want = np.tensordot(p["G"], P, 1)
+    bnd = np.zeros((n + 1,) * 3, bool)
+    bnd[[0, -1]] = True
+    bnd[:, [0, -1]] = True
+    bnd[:, :, [0, -1]] = True
+    np.testing.assert_allclose(rhs[:, bnd], want[:, bnd], rtol=1e-14, atol=1e-18)
+    assert not rhs[:, ~bnd].any()
+    r.close()
+    b.close()
+
+
+@pytest.mark.parametrize("prec,tol", [(E.F64, 1e-12), (E.F32, 2e-5)])
+def test_batched_fapply_matches_oracle(oracle, prec, tol):
+    b = E.Batch(E.SOLVE_MIXED)
+    probs = [problem(11, 16, 8), problem(12, 12, 5, crack=True), problem(13, 10, 3)]
+    rs = [make(b, p) for p in probs]
+    # boundary rows of every kind: traction faces, displacement faces, a mix
+    bcts = [all_disp(), all_disp(), all_disp()]
+    bcts[0][:] = K.TRAC
+    bcts[2][[0, 1, 3, 4]] = K.TRAC
+    bcts[2][6:, 0] = K.TRAC
+    rng = np.random.default_rng(3)
+    us = []
+    for r, bct in zip(rs, bcts):
+        r.SetBC(bct)
+        u = rng.standard_normal(r.shape)
+        r.upload(E.RVE.U, u)
+        us.append(u)
+    b.fapply(rs, prec)  # one launch over the three members
+    for r, p, u, bct in zip(rs, probs, us, bcts):
+        n = p["n"]
+        f = r.download(E.RVE.R)
+        want = oracle.fapply(oracle.Grid((n, n, n)), oracle.MM, planes(r, p), bct, u)
+        scale = np.abs(want).max()
+        assert np.abs(f - want).max() <= tol * scale, (n, np.abs(f - want).max() / scale)
+    for r in rs:
+        r.close()
+    b.close()
+
+
+def test_batched_solve_matches_oracle(oracle):
+    b = E.Batch(E.SOLVE_MIXED)
+    probs = [problem(21, 16, 10), problem(22, 24, 16), problem(23, 20, 6, crack=True), problem(24, 12, 4)]
+    rs = [make(b, p) for p in probs]
+    b.prepare(rs)
+    sts = b.solve(rs, tol_rel=1e-10, max_cycles=80)
+    for r, p, st in zip(rs, probs, sts):
+        print("n=%d levels=%d cycles=%d res/res0=%.2e" % (p["n"], r.levels(), st["iters"], st["res"] / st["res0"]))
+        assert st["converged"] and st["res"] <= 1e-10 * st["res0"]
+        u = r.download(E.RVE.U)
+        ref = oracle_solution(oracle, r, p, planes(r, p))
+        err = np.abs(u - ref).max() / np.abs(ref).max()
+        assert err <= 1e-8, (p["n"], err)
+    b.close()
+
+
+def test_fp64_batch_matches_mixed():
+    sol = {}
+    for mode in (E.SOLVE_FP64, E.SOLVE_MIXED):
+        b = E.Batch(mode)
+        p = problem(31, 16, 8)
+        r = make(b, p)
+        b.prepare([r])
+        st = b.solve([r], tol_rel=1e-10, max_cycles=80)[0]
+        assert st["converged"]
+        sol[mode] = r.download(E.RVE.U)
+        b.close()
+    ref = sol[E.SOLVE_FP64]
+    assert np.abs(sol[E.SOLVE_MIXED] - ref).max() <= 1e-8 * np.abs(ref).max()
+
+
+def test_member_result_is_the_same_alone_and_in_a_mix():
+    target = problem(41, 16, 9, crack=True)
+    others = [problem(42, 32, 12), problem(43, 12, 5), problem(44, 20, 7, crack=True)]
+    b = E.Batch(E.SOLVE_MIXED)
+    r = make(b, target)
+    b.prepare([r])
+    st_alone = b.solve([r], tol_rel=1e-10, max_cycles=80)[0]
+    alone = r.download(E.RVE.U)
+    b.close()
+
+    b = E.Batch(E.SOLVE_MIXED)
+    rs = [make(b, p) for p in others[:2]] + [make(b, target)] + [make(b, others[2])]
+    b.prepare(rs)
+    sts = b.solve(rs, tol_rel=1e-10, max_cycles=80)
+    mixed = rs[2].download(E.RVE.U)
+    assert sts[2]["iters"] == st_alone["iters"]
+    assert np.array_equal(alone, mixed)
+    b.close()
+
+
+def test_continuous_admission_gives_the_same_bits():
+    pa, pb = problem(51, 16, 8), problem(52, 24, 11, crack=True)
+    alone = {}
+    for key, p in (("a", pa), ("b", pb)):
+        b = E.Batch(E.SOLVE_MIXED)
+        r = make(b, p)
+        b.prepare([r])
+        b.solve([r], tol_rel=1e-10, max_cycles=80)
+        alone[key] = r.download(E.RVE.U)
+        b.close()
+    b = E.Batch(E.SOLVE_MIXED)
+    ra, rb = make(b, pa), make(b, pb)
+    b.prepare([ra, rb])
+    b.submit(ra, 1e-10, 0.0, 80)
+    for _ in range(3):
+        b.step()
+    b.submit(rb, 1e-10, 0.0, 80)  # joins while ra is in flight
+    while b.step():
+        pass
+    assert ra.status()["converged"] and rb.status()["converged"]
+    assert np.array_equal(ra.download(E.RVE.U), alone["a"])
+    assert np.array_equal(rb.download(E.RVE.U), alone["b"])
+    b.close()
+
+
+def test_member_admission_respects_the_hbm_budget():
+    need = E.Batch.member_bytes(32, E.SOLVE_MIXED, 16)
+    ctx = E.Context(0, hbm_budget_bytes=int(need * 1.5))
+    b = E.Batch(E.SOLVE_MIXED, ctx=ctx)
+    r1 = b.rve(32, 16)
+    with pytest.raises(E.ElaError) as ei:
+        b.rve(32, 16)
+    assert ei.value.code == 2
+    r1.close()
+    r2 = b.rve(32, 16)  # fits again once the first member is gone
+    r2.close()
+    b.close()
+    ctx.close()
+
+
+def test_member_sizes_outside_the_dense_coarsest_limit_are_rejected():
+    b = E.Batch(E.SOLVE_MIXED)
+    with pytest.raises(E.ElaError) as ei:
+        b.rve(36, 4)  # 36 -> 18 -> 9: a 9-cell coarsest level
+    assert ei.value.code == 1
+    b.close()
