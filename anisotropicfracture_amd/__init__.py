@@ -65,6 +65,20 @@ class Stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Request(ctypes.Structure):
+    _fields_ = [("arrival", ctypes.c_double), ("n", ctypes.c_int), ("nseeds", ctypes.c_int),
+                ("seeds", ctypes.c_void_p), ("q", ctypes.c_int), ("R", ctypes.c_void_p),
+                ("C11", ctypes.c_double), ("C12", ctypes.c_double), ("C44", ctypes.c_double),
+                ("crack", ctypes.c_int), ("crack_p", ctypes.c_double * 3), ("crack_nrm", ctypes.c_double * 3),
+                ("crack_tv", ctypes.c_double * 3), ("crack_width", ctypes.c_double), ("crack_g", ctypes.c_double * 3),
+                ("G", ctypes.c_double * 9), ("u_out", ctypes.c_void_p)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("admitted", ctypes.c_double), ("finished", ctypes.c_double), ("cycles", ctypes.c_int),
+                ("status", ctypes.c_int), ("res0", ctypes.c_double), ("res", ctypes.c_double)]
+
+
 _lib = None
 
 # exported symbol -> (argtypes, restype)
@@ -132,6 +146,7 @@ SIGNATURES = {
     "ela_batch_solve": ([_P, _I, _P, _D, _D, _I], _I),
     "ela_rve_status": ([_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(Stats)], _I),
     "ela_batch_set_timing": ([_P, _I], _I),
+    "ela_serve": ([_P, _I, ctypes.POINTER(Request), _D, _D, _I, _I, _D, ctypes.POINTER(Result)], _I),
     "ela_batch_smoother_times": ([_P, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_I)], _I),
 }
 
@@ -556,6 +571,39 @@ class Batch:
 
     def set_timing(self, on=True):
         _check(lib().ela_batch_set_timing(self.ptr, int(bool(on))))
+
+    def serve(self, requests, tol_rel=1e-10, tol_abs=0.0, max_cycles=100, max_active=256, max_seconds=0.0,
+              keep=()):
+        """Run the request loop (ela_serve) over `requests` (dicts: arrival, n, seeds, R, G and optional
+        crack = (p, nrm, tv, width, (g2, g1, g0))), sorted by arrival.  Solutions of the request indices in
+        `keep` are returned.  Returns (results: list of dicts, solutions: {index: array})."""
+        from . import configs as K
+        reqs = (Request * len(requests))()
+        hold, sols = [], {}
+        for i, q in enumerate(requests):
+            r = reqs[i]
+            sd = np.ascontiguousarray(q["seeds"], dtype=np.int64)
+            Rm = np.ascontiguousarray(q["R"], dtype=np.float64).reshape(-1, 9)
+            hold += [sd, Rm]
+            r.arrival, r.n, r.nseeds, r.q = float(q["arrival"]), int(q["n"]), sd.shape[0], K.VORONOI_Q
+            r.seeds, r.R = sd.ctypes.data, Rm.ctypes.data
+            r.C11, r.C12, r.C44 = q.get("C11", K.C11), q.get("C12", K.C12), q.get("C44", K.C44)
+            cr = q.get("crack")
+            r.crack = 1 if cr is not None else 0
+            if cr is not None:
+                p, nrm, tv, width, g = cr
+                r.crack_p[:], r.crack_nrm[:], r.crack_tv[:] = list(p), list(nrm), list(tv)
+                r.crack_width = float(width)
+                r.crack_g[:] = list(g)
+            r.G[:] = list(np.asarray(q["G"], dtype=np.float64).reshape(9))
+            if i in keep:
+                sols[i] = np.zeros((3,) + (int(q["n"]) + 1,) * 3)
+                r.u_out = sols[i].ctypes.data
+        res = (Result * len(requests))()
+        _check(lib().ela_serve(self.ptr, len(requests), reqs, tol_rel, tol_abs, max_cycles, max_active, max_seconds,
+                               res))
+        out = [{k: getattr(x, k) for k, _ in Result._fields_} for x in res]
+        return out, sols
 
     def smoother_times(self):
         ms, nn, nl = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_int()

@@ -162,3 +162,80 @@ def cfg2(n=256, grains=64, seed=SEED):
     R = haar_rotations(rng, grains)
     t, v = uniaxial_z_bcs(0.01)
     return {"n": size, "seeds": seeds, "R": R, "C11": C11, "C12": C12, "C44": C44, "bc_type": t, "bc_value": v}
+
+
+# ------------------------------------------------------------------ batched RVEs
+RVE_SIDES = (32, 48, 64, 96)
+
+
+def fixed_corner_gradient(strain=0.01):
+    """Displacement gradient of the RVE loading u = strain * z * e_z (applied on every face)."""
+    G = np.zeros((3, 3))
+    G[2, 2] = strain
+    return G
+
+
+def cfg4(count=512, seed=SEED, sides=RVE_SIDES, grains=(8, 32)):
+    """Batched RVEs: side n drawn uniformly from `sides`, 8-32 Voronoi grains with
+    Haar orientations, cubic (1.68, 1.21, 0.75), Displacement on every face with
+    u = 0.01 z e_z there (all corners included).
+
+    Returns a list of dicts with n, quantized seeds, rotations and G.
+    """
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.choice(sides))
+        ng = int(rng.integers(grains[0], grains[1] + 1))
+        seeds = voronoi_seeds(rng, ng, (n, n, n))
+        R = haar_rotations(rng, ng)
+        out.append({"n": n, "seeds": seeds, "R": R, "G": fixed_corner_gradient(0.01)})
+    return out
+
+
+# ------------------------------------------------------------------ serving stream
+# Member sides with a coarsest level of at most 7 cells (n = c 2^k, c in {2,3,5,7}).
+SERVE_SIDES = (32, 40, 48, 56, 64, 80, 96, 112, 128, 160, 192)
+
+
+def serve_side(n):
+    """Smallest member side >= n in SERVE_SIDES (requests are solved at that resolution)."""
+    for s in SERVE_SIDES:
+        if s >= n:
+            return s
+    return SERVE_SIDES[-1]
+
+
+def random_crack(rng, n):
+    """A random half-plane crack: point p in [0.25, 0.75]^3, unit normal, in-plane direction tv
+    (the crack covers tv.(x - p) <= 0), width max(0.01, 2 h), degradation g = (1 - eta)^2 + 1e-4."""
+    p = rng.random(3) * 0.5 + 0.25
+    nrm = rng.standard_normal(3)
+    nrm /= np.linalg.norm(nrm)
+    tv = np.cross(nrm, rng.standard_normal(3))
+    tv /= np.linalg.norm(tv)
+    return (p, nrm, tv, max(0.01, 2.0 / n), (1.0, -2.0, 1.0 + 1e-4))
+
+
+def cfg5(rate=200.0, duration=60.0, seed=SEED, nmin=32, nmax=192, grains=(8, 64), p_crack=0.5, strain=0.01):
+    """Serving stream: Poisson arrivals (rate per s) over `duration` s; side log-uniform in
+    [nmin, nmax] rounded to the nearest integer and then up to SERVE_SIDES; 8-64 Voronoi grains
+    (Haar orientations, cubic constants above); a crack with probability p_crack; load
+    u = strain (a.x) a on every face for a random unit vector a.  Sorted by arrival."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration:
+            break
+        n_req = int(np.rint(np.exp(rng.uniform(np.log(nmin), np.log(nmax)))))
+        n = serve_side(n_req)
+        ng = int(rng.integers(grains[0], grains[1] + 1))
+        seeds = voronoi_seeds(rng, ng, (n, n, n))
+        R = haar_rotations(rng, ng)
+        a = rng.standard_normal(3)
+        a /= np.linalg.norm(a)
+        crack = random_crack(rng, n) if rng.random() < p_crack else None
+        out.append({"arrival": t, "n": n, "n_requested": n_req, "seeds": seeds, "R": R,
+                    "G": strain * np.outer(a, a), "crack": crack})
+    return out

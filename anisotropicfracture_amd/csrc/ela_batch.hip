@@ -28,7 +28,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/ela.h"
@@ -571,7 +573,8 @@ struct ela_rve {
     unsigned short* gid = nullptr;
     double *tab64 = nullptr, *sc64 = nullptr;
     float *tab32 = nullptr, *sc32 = nullptr;
-    long long* seeds = nullptr;  // device copy of the seeds (model evaluation)
+    long long* seeds = nullptr;  // device copies of the seeds and rotations (model evaluation)
+    double* rot = nullptr;
     double *u64 = nullptr, *b64 = nullptr, *r64 = nullptr, *M = nullptr, *scal = nullptr;
     void* block = nullptr;
     size_t bytes = 0;
@@ -654,6 +657,7 @@ size_t layout_member(ela_rve* r, char* base) {
             r->tab64 = (double*)take((size_t)r->max_grains * kNC * 8);
             if (mixed) r->tab32 = (float*)take((size_t)r->max_grains * kNC * 4);
             r->seeds = (long long*)take((size_t)r->max_grains * 3 * 8);
+            r->rot = (double*)take((size_t)r->max_grains * 9 * 8);
             if (r->factor) {
                 r->sc64 = (double*)take(N * 8);
                 if (mixed) r->sc32 = (float*)take(N * 4);
@@ -1323,18 +1327,16 @@ int ela_rve_set_voronoi_cubic(ela_rve* r, int nseeds, const int64_t* seeds, int 
     if (nseeds < 1 || nseeds > r->max_grains || !seeds || !R || q < 1)
         return fail(ELA_ERR_BADARG, "need 1..max_grains seeds, rotations and q >= 1");
     ela_batch* B = r->B;
-    std::vector<double> Rh(R, R + 9 * (size_t)nseeds);
-    double* dR = nullptr;
-    ELA_HIPCHK(hipMalloc((void**)&dR, Rh.size() * sizeof(double)));
-    (void)hipMemcpyAsync(dR, Rh.data(), Rh.size() * sizeof(double), hipMemcpyHostToDevice, B->stream);
-    (void)hipMemcpyAsync(r->seeds, seeds, 3 * (size_t)nseeds * sizeof(long long), hipMemcpyHostToDevice, B->stream);
-    launch_cubic_table(nseeds, dR, C11, C12, C44, r->tab64, B->stream);
+    ELA_HIPCHK(hipMemcpyAsync(r->rot, R, 9 * (size_t)nseeds * sizeof(double), hipMemcpyHostToDevice, B->stream));
+    ELA_HIPCHK(
+        hipMemcpyAsync(r->seeds, seeds, 3 * (size_t)nseeds * sizeof(long long), hipMemcpyHostToDevice, B->stream));
+    launch_cubic_table(nseeds, r->rot, C11, C12, C44, r->tab64, B->stream);
     if (r->tab32) launch_convert<double, float>((long long)nseeds * kNC, r->tab64, r->tab32, B->stream);
     const RLev& v = r->lev[0];
     hipLaunchKernelGGL(kb_model, dim3((v.nodes + TB - 1) / TB), dim3(TB), 0, B->stream, v.L, v.h, nseeds,
                        (const long long*)r->seeds, q, r->gid, 0, Crack{}, (double*)nullptr, (float*)nullptr);
+    // The host arrays are only borrowed for this call.
     const hipError_t e = hipStreamSynchronize(B->stream);
-    (void)hipFree(dR);
     if (e != hipSuccess) return fail(ELA_ERR_HIP, std::string("set_voronoi_cubic: ") + hipGetErrorString(e));
     r->ngrains = nseeds;
     r->model = true;
@@ -1360,7 +1362,7 @@ int ela_rve_set_crack(ela_rve* r, const double p[3], const double nrm[3], const 
     const RLev& v = r->lev[0];
     hipLaunchKernelGGL(kb_model, dim3((v.nodes + TB - 1) / TB), dim3(TB), 0, r->B->stream, v.L, v.h, 0,
                        (const long long*)nullptr, 1, (unsigned short*)nullptr, 1, cr, r->sc64, r->sc32);
-    ELA_HIPCHK(hipStreamSynchronize(r->B->stream));
+    ELA_HIPCHK(hipGetLastError());
     r->prepared = false;
     return ELA_OK;
 }
@@ -1371,7 +1373,7 @@ int ela_rve_set_rhs_affine(ela_rve* r, const double G[9]) {
     const RLev& v = r->lev[0];
     hipLaunchKernelGGL(kb_rhs_affine, dim3((v.nodes + TB - 1) / TB), dim3(TB), 0, r->B->stream, v.L, v.h, r->bc, G[0],
                        G[1], G[2], G[3], G[4], G[5], G[6], G[7], G[8], r->b64);
-    ELA_HIPCHK(hipStreamSynchronize(r->B->stream));
+    ELA_HIPCHK(hipGetLastError());
     return ELA_OK;
 }
 
@@ -1540,6 +1542,103 @@ size_t ela_batch_bytes_in_use(const ela_batch* B) {
     size_t s = 0;
     for (auto* r : B->members) s += r->bytes;
     return s;
+}
+
+// Request loop with continuous batching.  Requests (sorted by arrival time)
+// are admitted first-come first-served while the context's HBM budget and
+// max_active allow: a member is created (its hierarchy stays resident until
+// the request finishes), its model and right-hand side are evaluated on the
+// device, the admitted group is prepared together and joins the running set
+// at the next cycle.  Every step is one F-cycle over all members in flight;
+// finished members are copied out (if the request asks) and freed, which
+// lets waiting requests in.  Times are seconds from the start of the call.
+int ela_serve(ela_batch* B, int nreq, const ela_request* req, double tol_rel, double tol_abs, int max_cycles,
+              int max_active, double max_seconds, ela_result* out) {
+    if (!B || nreq < 0 || (nreq && (!req || !out)) || max_active < 1 || max_cycles < 1)
+        return fail(ELA_ERR_BADARG, "bad arguments");
+    for (int i = 1; i < nreq; ++i)
+        if (req[i].arrival < req[i - 1].arrival) return fail(ELA_ERR_BADARG, "requests must be sorted by arrival");
+    if (!B->active.empty() || !B->queued.empty()) return fail(ELA_ERR_STATE, "batch is busy");
+    for (int i = 0; i < nreq; ++i) {
+        out[i] = ela_result{};
+        out[i].status = -1;
+        out[i].admitted = out[i].finished = -1.0;
+    }
+    const double t0 = now_s();
+    std::deque<int> waiting;
+    std::vector<std::pair<ela_rve*, int>> live;
+    int next = 0, done = 0, rc = ELA_OK;
+    while (done < nreq) {
+        const double t = now_s() - t0;
+        if (max_seconds > 0 && t > max_seconds) break;
+        while (next < nreq && req[next].arrival <= t) waiting.push_back(next++);
+        // admission
+        std::vector<ela_rve*> group;
+        std::vector<int> gidx;
+        while (!waiting.empty() && (int)live.size() + (int)group.size() < max_active) {
+            const int i = waiting.front();
+            const ela_request& q = req[i];
+            ela_rve* r = nullptr;
+            const int crc = ela_rve_create(B, q.n, std::max(1, q.nseeds), q.crack ? 1 : 0, &r);
+            if (crc == ELA_ERR_OOM && (!live.empty() || !group.empty())) break;  // wait for memory
+            int mrc = crc;
+            if (!mrc) mrc = ela_rve_set_voronoi_cubic(r, q.nseeds, q.seeds, q.q, q.R, q.C11, q.C12, q.C44);
+            if (!mrc && q.crack)
+                mrc = ela_rve_set_crack(r, q.crack_p, q.crack_nrm, q.crack_tv, q.crack_width, q.crack_g[0],
+                                        q.crack_g[1], q.crack_g[2]);
+            if (!mrc) mrc = ela_rve_set_rhs_affine(r, q.G);
+            waiting.pop_front();
+            if (mrc) {  // a request that cannot be served (bad size, or larger than the whole budget)
+                if (r) ela_rve_destroy(r);
+                out[i].status = mrc;
+                out[i].admitted = out[i].finished = now_s() - t0;
+                ++done;
+                continue;
+            }
+            group.push_back(r);
+            gidx.push_back(i);
+        }
+        if (!group.empty()) {
+            if ((rc = prepare(B, (int)group.size(), group.data()))) break;
+            const double ta = now_s() - t0;
+            for (size_t m = 0; m < group.size(); ++m) {
+                if ((rc = ela_batch_submit(B, group[m], tol_rel, tol_abs, max_cycles))) break;
+                out[gidx[m]].admitted = ta;
+                live.push_back({group[m], gidx[m]});
+            }
+            if (rc) break;
+        }
+        if (!B->active.empty() || !B->queued.empty()) {
+            if ((rc = step(B))) break;
+            std::vector<std::pair<ela_rve*, int>> keep;
+            for (auto& pr : live) {
+                ela_rve* r = pr.first;
+                if (r->state != ST_DONE) {
+                    keep.push_back(pr);
+                    continue;
+                }
+                ela_result& o = out[pr.second];
+                o.finished = r->t_done - t0;
+                o.cycles = r->cycles;
+                o.status = r->status;
+                o.res0 = r->res0;
+                o.res = r->res;
+                if (req[pr.second].u_out) ela_rve_download(r, 0, req[pr.second].u_out);
+                ela_rve_destroy(r);
+                ++done;
+            }
+            live.swap(keep);
+        } else if (waiting.empty() && next < nreq) {
+            const double wait = req[next].arrival - (now_s() - t0);
+            if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.01)));
+        }
+    }
+    // leave nothing behind: requests still in flight at the time limit are dropped
+    for (auto& pr : live) ela_rve_destroy(pr.first);
+    B->active.clear();
+    B->queued.clear();
+    B->dirty = true;
+    return rc;
 }
 
 }  // extern "C"

@@ -70,6 +70,130 @@ def cpu_baseline(n):
     return t, st["iters"], st["converged"]
 
 
+def batched_rves(count, sample, tol):
+    """Config 4: `count` independent RVEs (sides 32/48/64/96, 8-32 grains, fixed-corner uniaxial z
+    strain 0.01).  Batched mixed and batched FP64 solve every RVE, one launch per cycle kernel over all
+    of them; the plain FP64 GPU solver (ela_solve on one hierarchy per size, one RVE at a time, the same
+    MG iteration) solves the first `sample` RVEs.  Timed per RVE: stiffness evaluation on the device,
+    set-up (coarse stiffness, diagonals, coarsest inverse) and the solve."""
+    import anisotropicfracture_amd as ela
+    from anisotropicfracture_amd import configs as K
+    cfg = K.cfg4(count)
+    out = {"rves": count, "nodes": int(sum((m["n"] + 1) ** 3 for m in cfg))}
+    cycles = {}
+
+    def load(b, rs, ms):
+        for r, m in zip(rs, ms):
+            r.SetModelVoronoiCubic(m["seeds"], K.VORONOI_Q, m["R"].reshape(-1, 9), K.C11, K.C12, K.C44)
+            r.SetRhsAffine(m["G"])
+
+    for mode, key in ((ela.SOLVE_MIXED, "mixed"), (ela.SOLVE_FP64, "fp64")):
+        b = ela.Batch(mode)
+        warm = [b.rve(m["n"], max_grains=32) for m in cfg[:8]]  # first launches of every kernel
+        load(b, warm, cfg[:8])
+        b.prepare(warm)
+        b.solve(warm, tol_rel=tol, max_cycles=100)
+        for r in warm:
+            r.close()
+        rs = [b.rve(m["n"], max_grains=32) for m in cfg]
+        b.set_timing(key == "mixed")
+        t0 = time.perf_counter()
+        load(b, rs, cfg)
+        b.prepare(rs)
+        sts = b.solve(rs, tol_rel=tol, max_cycles=100, raise_on_fail=False)
+        t = time.perf_counter() - t0
+        it = np.array([q["iters"] for q in sts])
+        cycles[key] = it
+        ent = {"solves_per_s": count / t, "seconds": t, "cycles_mean": float(it.mean()), "cycles_max": int(it.max()),
+               "converged": int(sum(q["converged"] for q in sts)),
+               "max_rel_residual": float(max(q["res"] / q["res0"] for q in sts)), "device_GB": b.bytes() / 1e9}
+        if key == "mixed":
+            sm = b.smoother_times()
+            ent["smoother"] = {"ms": sm["ms"], "launches": sm["launches"], "node_sweeps": sm["nodes"],
+                               "Gnodes_per_s": sm["nodes"] / (sm["ms"] * 1e-3) / 1e9}
+        out["batched_" + key] = ent
+        log("bench: batched %s %.1f solves/s (%.2f s, cycles %.1f)" % (key, count / t, t, it.mean()))
+        b.close()
+    # plain FP64 GPU solver, one RVE at a time (hierarchies reused per size)
+    ops, tot, its = {}, 0.0, []
+    for m in cfg[:sample]:
+        n = m["n"]
+        if n not in ops:
+            op = ela.Elastic((n, n, n), ela.MM, mode=ela.SOLVE_FP64, coarse_min=2)
+            op.SetBC(np.zeros((26, 3), np.int32), np.zeros((26, 3)))
+            ops[n] = (op, op.field(0, ela.F64, 3), op.field(0, ela.F64, 3))
+        op, bf, uf = ops[n]
+        x = np.arange(n + 1) / n
+        z = np.broadcast_to(x[:, None, None], (n + 1,) * 3)
+        bh = np.zeros((3,) + (n + 1,) * 3)
+        bnd = np.ones((n + 1,) * 3, bool)
+        bnd[1:-1, 1:-1, 1:-1] = False
+        bh[2][bnd] = m["G"][2, 2] * z[bnd]
+        bf.upload(bh)
+        uf.zero()
+        t0 = time.perf_counter()
+        op.SetModelVoronoiCubic(m["seeds"], K.VORONOI_Q, m["R"].reshape(-1, 9), K.C11, K.C12, K.C44)
+        op.SetStiffnessMode(1)
+        op.prepareForSolve()
+        rc, st = op.solve_device(bf, uf, tol, 0.0, ela.SOLVE_FP64, 200)
+        tot += time.perf_counter() - t0
+        its.append(st["iters"])
+    for op, _, _ in ops.values():
+        op.close()
+    out["plain_fp64"] = {"solves_per_s": sample / tot, "sample": sample, "cycles_mean": float(np.mean(its)),
+                         "cycles_same_as_batched_fp64": bool(np.array_equal(its, cycles["fp64"][:sample]))}
+    out["batched_mixed_over_plain_fp64"] = out["batched_mixed"]["solves_per_s"] / out["plain_fp64"]["solves_per_s"]
+    out["batched_fp64_over_plain_fp64"] = out["batched_fp64"]["solves_per_s"] / out["plain_fp64"]["solves_per_s"]
+    log("bench: plain fp64 %.1f solves/s (sample %d); batched mixed / plain fp64 = %.2f"
+        % (sample / tot, sample, out["batched_mixed_over_plain_fp64"]))
+    return out
+
+
+def serving(seconds, max_wall, tol, budget_gb=200.0):
+    """Config 5: the Poisson(200/s) request stream, arrivals in the first `seconds` s, served by the
+    continuous-batching loop (ela_serve) under a `budget_gb` HBM budget, stopped after `max_wall` s.
+    Two requests are re-solved alone afterwards and compared bit for bit."""
+    import anisotropicfracture_amd as ela
+    from anisotropicfracture_amd import configs as K
+    reqs = [q for q in K.cfg5() if q["arrival"] < seconds]
+    ctx = ela.Context(0, int(budget_gb * 1e9))
+    b = ela.Batch(ela.SOLVE_MIXED, ctx=ctx)
+    keep = (0, min(40, len(reqs) - 1))
+    res, sols = b.serve(reqs, tol_rel=tol, max_cycles=100, max_active=512, max_seconds=max_wall, keep=keep)
+    done = [(r, q) for r, q in zip(res, reqs) if r["status"] == 0 and r["finished"] >= 0]
+    lat = np.array([r["finished"] - q["arrival"] for r, q in done])
+    wall = max(r["finished"] for r, _ in done) if done else float("nan")
+    same = True
+    for i in keep:
+        if res[i]["status"] != 0:
+            same = None
+            continue
+        q = reqs[i]
+        r = b.rve(q["n"], max_grains=len(q["seeds"]), with_factor=q["crack"] is not None)
+        r.SetModelVoronoiCubic(q["seeds"], K.VORONOI_Q, q["R"].reshape(-1, 9), K.C11, K.C12, K.C44)
+        if q["crack"] is not None:
+            p, nrm, tv, w, g = q["crack"]
+            r.SetCrack(p, nrm, tv, w, *g)
+        r.SetRhsAffine(q["G"])
+        b.prepare([r])
+        b.solve([r], tol_rel=tol, max_cycles=100)
+        same = same and bool(np.array_equal(r.download(ela.RVE.U), sols[i]))
+        r.close()
+    b.close()
+    ctx.close()
+    nodes = np.array([(q["n"] + 1) ** 3 for _, q in done], dtype=np.float64)
+    out = {"offered_rate_per_s": 200.0, "arrival_window_s": seconds, "requests": len(reqs), "served": len(done),
+           "wall_s": wall, "solves_per_s": len(done) / wall if done else 0.0,
+           "p50_ms": float(np.percentile(lat, 50) * 1e3) if done else None,
+           "p99_ms": float(np.percentile(lat, 99) * 1e3) if done else None,
+           "served_Mnodes_per_s": float(nodes.sum() / wall / 1e6) if done else None,
+           "budget_GB": budget_gb, "alone_vs_mix_bitwise": same,
+           "not_served_by_limit": int(sum(1 for r in res if r["status"] == -1))}
+    log("bench: serving %d/%d requests in %.1f s: %.1f solves/s p50 %.0f ms p99 %.0f ms"
+        % (len(done), len(reqs), wall, out["solves_per_s"], out["p50_ms"] or -1, out["p99_ms"] or -1))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +207,11 @@ def main():
     ap.add_argument("--modes", default="fp64,mixed", help="comma list of fp64, mixed")
     ap.add_argument("--krylov", type=int, default=16, help="FGMRES restart around the MG cycle (0: plain MG)")
     ap.add_argument("--table", type=int, default=1, help="grain-table stiffness on level 0 for the headline mixed run")
+    ap.add_argument("--rves", type=int, default=512, help="batched RVE count (0: skip the batched workload)")
+    ap.add_argument("--plain-sample", type=int, default=24, help="RVEs solved one at a time by the plain FP64 solver")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="arrival window of the serving stream (0: skip serving)")
+    ap.add_argument("--serve-wall", type=float, default=40.0, help="wall-time limit of the serving run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md on sharding)",
@@ -168,6 +297,11 @@ def main():
         if ent and tr.get("n") == n:
             traffic = ent["fetch_bytes"] + ent["write_bytes"]
 
+    levels = op.levels()
+    op.close()
+    batched = batched_rves(args.rves, args.plain_sample, args.tol) if args.rves > 0 else None
+    serve = serving(args.serve_seconds, args.serve_wall, args.tol) if args.serve_seconds > 0 else None
+
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -195,7 +329,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "3D single level %d^3 cells, cubic Voronoi %d grains (Haar orientations, seed %d), "
                                "uniaxial z strain 0.01, tol_rel %.0e" % (n, args.grains, K.SEED, args.tol),
-                   "nodes": nodes, "mg_levels": op.levels(),
+                   "nodes": nodes, "mg_levels": levels,
                    "solver": "FGMRES(%d) + MG F-cycle" % args.krylov if args.krylov else "MG F-cycles"},
         "fp64_solves_per_s": args.steps / t64 if t64 else None,
         "mixed_over_fp64": t64 / tmx if t64 else None,
@@ -215,6 +349,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": peak, "unit": "GB/s", "frac": achieved / peak,
                      "traffic": traffic, "kernel": dom, "kernel_ms": dom_ms, "bytes_per_launch": dom_bytes},
         "cpu_baseline": cpu,
+        "batched_rves": batched,
+        "serving": serve,
     }
     print(json.dumps(out), flush=True)
 

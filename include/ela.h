@@ -320,6 +320,43 @@ int ela_batch_solve(ela_batch* b, int count, ela_rve* const* rves, double tol_re
  * (seconds = submit to done). */
 int ela_rve_status(const ela_rve* r, int* state, int* status, ela_stats* stats);
 
+/* One request of the serving loop: a Voronoi cubic polycrystal member of n
+ * cells per side (n = c * 2^k, c <= 7), optionally cracked (ela_rve_set_crack
+ * with g(eta) = crack_g[0] eta^2 + crack_g[1] eta + crack_g[2]), loaded by the
+ * affine displacement u = G x on every face.  u_out (3 (n+1)^3 doubles) or
+ * NULL receives the solution. */
+typedef struct {
+    double arrival; /* seconds from the start of ela_serve */
+    int n;
+    int nseeds;
+    const int64_t* seeds;
+    int q;
+    const double* R;
+    double C11, C12, C44;
+    int crack;
+    double crack_p[3], crack_nrm[3], crack_tv[3];
+    double crack_width;
+    double crack_g[3];
+    double G[9];
+    double* u_out;
+} ela_request;
+
+typedef struct {
+    double admitted; /* seconds from the start of ela_serve; -1 if never admitted */
+    double finished; /* -1 if not finished within max_seconds */
+    int cycles;
+    int status;      /* ELA_OK, ELA_ERR_NONCONVERGED, ELA_ERR_NAN, an admission error, or -1 (not served) */
+    double res0, res;
+} ela_result;
+
+/* Serving loop with continuous batching (no counterpart in the reference):
+ * requests sorted by arrival are admitted first-come first-served while the
+ * context's HBM budget and max_active allow; each step is one F-cycle over
+ * every member in flight; members leave as they converge and free their
+ * memory.  Stops when all requests are done or after max_seconds (> 0). */
+int ela_serve(ela_batch* b, int nreq, const ela_request* req, double tol_rel, double tol_abs, int max_cycles,
+              int max_active, double max_seconds, ela_result* out);
+
 /* HIP-event timing of every smoother launch of later steps (on != 0), and
  * the totals since: milliseconds, nodes swept, launches. */
 int ela_batch_set_timing(ela_batch* b, int on);

@@ -29,7 +29,7 @@ def problem(seed, n, ngrains, crack=False):
         tv = np.cross(nrm, rng.standard_normal(3))
         tv /= np.linalg.norm(tv)
         # soft crack: g = (1 - eta)^2 + 1e-3
-        cr = (rng.random(3) * 0.5 + 0.25, nrm, tv, 2.0 / n, 1.0, -2.0, 1.0 + 1e-3)
+        cr = (rng.random(3) * 0.5 + 0.25, nrm, tv, 2.0 / n, (1.0, -2.0, 1.0 + 1e-3))
     return {"n": n, "seeds": seeds, "R": R, "G": G, "crack": cr}
 
 
@@ -37,7 +37,8 @@ def make(batch, p):
     r = batch.rve(p["n"], max_grains=len(p["seeds"]), with_factor=p["crack"] is not None)
     r.SetModelVoronoiCubic(p["seeds"], K.VORONOI_Q, p["R"].reshape(-1, 9), K.C11, K.C12, K.C44)
     if p["crack"] is not None:
-        r.SetCrack(*p["crack"])
+        c0, nrm, tv, width, g = p["crack"]
+        r.SetCrack(c0, nrm, tv, width, *g)
     r.SetRhsAffine(p["G"])
     return r
 
@@ -75,7 +76,7 @@ def test_member_inputs_match_closed_forms():
     x = np.arange(n + 1) / n
     Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
     P = np.stack([X, Y, Z])
-    c0, nrm, tv, w, a, bb, cc = p["crack"]
+    c0, nrm, tv, w, (a, bb, cc) = p["crack"]
     dn = np.tensordot(nrm, P - np.asarray(c0)[:, None, None, None], 1)
     dt = np.tensordot(tv, P - np.asarray(c0)[:, None, None, None], 1)
     eta = 0.5 * (1 - np.tanh(np.sqrt(dn ** 2 + np.maximum(dt, 0) ** 2) / w))
@@ -219,4 +220,26 @@ def test_member_sizes_outside_the_dense_coarsest_limit_are_rejected():
     with pytest.raises(E.ElaError) as ei:
         b.rve(36, 4)  # 36 -> 18 -> 9: a 9-cell coarsest level
     assert ei.value.code == 1
+    b.close()
+
+
+def test_serving_loop_results_match_alone_solves():
+    """Requests served in the continuous-batching loop give bitwise the solution of the same
+    request solved alone."""
+    reqs = K.cfg5(rate=400.0, duration=0.08, seed=7, nmin=24, nmax=48)
+    assert len(reqs) >= 10
+    keep = (0, len(reqs) // 2, len(reqs) - 1)
+    b = E.Batch(E.SOLVE_MIXED)
+    res, sols = b.serve(reqs, tol_rel=1e-10, max_cycles=80, max_active=16, keep=keep)
+    for i, r in enumerate(res):
+        assert r["status"] == 0 and r["finished"] >= r["admitted"] >= 0.0, (i, r)
+        assert r["res"] <= 1e-10 * r["res0"]
+    for i in keep:
+        q = reqs[i]
+        r = make(b, q)
+        b.prepare([r])
+        st = b.solve([r], tol_rel=1e-10, max_cycles=80)[0]
+        assert st["iters"] == res[i]["cycles"]
+        assert np.array_equal(r.download(E.RVE.U), sols[i]), i
+        r.close()
     b.close()
