@@ -146,6 +146,8 @@ SIGNATURES = {
     "ela_batch_solve": ([_P, _I, _P, _D, _D, _I], _I),
     "ela_rve_status": ([_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(Stats)], _I),
     "ela_batch_set_timing": ([_P, _I], _I),
+    "ela_reflux": ([ctypes.POINTER(_Layout), _P, ctypes.POINTER(_Layout), _P, _P, _I, _I, _P], _I),
+    "ela_average_down": ([ctypes.POINTER(_Layout), _P, ctypes.POINTER(_Layout), _P, _P, _I, _I, _P], _I),
     "ela_serve": ([_P, _I, ctypes.POINTER(Request), _D, _D, _I, _I, _D, ctypes.POINTER(Result)], _I),
     "ela_batch_smoother_times": ([_P, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_I)], _I),
 }
@@ -446,6 +448,34 @@ class Linear:
         if rc != 0 and (raise_on_fail or rc not in (4,)):
             _check(rc)
         return u, self.last
+
+
+# ---------------------------------------------------------------- AMR level transfers
+def _pair_args(crse, fine, fine_origin):
+    if crse.prec != fine.prec or crse.nc != fine.nc:
+        raise ValueError("coarse and fine fields differ in precision or components")
+    lc, lf = crse.op.layout(crse.mglev), fine.op.layout(fine.mglev)
+    o = np.zeros(3, dtype=np.int32)
+    o[:len(fine_origin)] = fine_origin
+    return lc, lf, o
+
+
+def reflux(crse_res, fine_res, fine_origin):
+    """Operator<Node>::reflux (Operator.cpp:728-839) between two device fields: the coarse residual
+    (a field of the coarse level's hierarchy) takes the full-weighting restriction of the fine
+    residual (a field on the fine box's hierarchy) on the coarsened fine box.  fine_origin: fine-level
+    node index (x, y[, z]) of the fine box's first node."""
+    lc, lf, o = _pair_args(crse_res, fine_res, fine_origin)
+    _check(lib().ela_reflux(ctypes.byref(lc), crse_res.ptr, ctypes.byref(lf), fine_res.ptr, _ptr(o),
+                            crse_res.nc, crse_res.prec, None))
+
+
+def average_down(crse_sol, fine_sol, fine_origin):
+    """averageDownSolutionRHS (Operator.cpp:626-641): coarse nodes of the coarsened fine box take the
+    coincident fine solution value."""
+    lc, lf, o = _pair_args(crse_sol, fine_sol, fine_origin)
+    _check(lib().ela_average_down(ctypes.byref(lc), crse_sol.ptr, ctypes.byref(lf), fine_sol.ptr, _ptr(o),
+                                  crse_sol.nc, crse_sol.prec, None))
 
 
 # ---------------------------------------------------------------- batched independent solves

@@ -254,12 +254,14 @@ __global__ __launch_bounds__(TB) void kb_restrict(const BoxDesc* __restrict__ Dc
     int I, J, K;
     long long oc;
     if (!box_node(c, Lc, blockIdx.x, I, J, K, oc)) return;
-    const T* F = (const T*)f.f[src];
+    const long long px = Lf.px, pl = Lf.pl, of = 2LL * I + px * (2LL * J) + pl * (2LL * K);
+    const T* F = (const T*)f.f[src] + of;
     T* out = (T*)c.f[dst];
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
         const T* Fp = F + p * Lf.cs;
-        out[p * Lc.cs + oc] = restrict_node<T, 3>(Lc, Lf, I, J, K, [Fp](long long o) { return Fp[o]; });
+        out[p * Lc.cs + oc] =
+            restrict_node<T, 3>(Lc, I, J, K, [Fp, px, pl](int a, int b, int c) { return Fp[a + px * b + pl * c]; });
     }
 }
 
@@ -389,14 +391,17 @@ __global__ __launch_bounds__(TB) void kb_coef(const BoxDesc* __restrict__ Dc, co
     long long oc;
     if (!box_node(c, Lc, blockIdx.x, I, J, K, oc)) return;
     double* out = (double*)c.C64;
+    const long long px = Lf.px, pl = Lf.pl, of = 2LL * I + px * (2LL * J) + pl * (2LL * K);
     if (f.gid) {
         const GrainAcc<double, kNC> A{f.gid, f.tab64, f.sc64};
         for (int q = 0; q < kNC; ++q)
-            out[q * Lc.cs + oc] = restrict_node<double, 3>(Lc, Lf, I, J, K, [&A, q](long long o) { return A.rec(o)[q]; });
+            out[q * Lc.cs + oc] = restrict_node<double, 3>(
+                Lc, I, J, K, [&A, q, of, px, pl](int a, int b, int c) { return A.rec(of + a + px * b + pl * c)[q]; });
     } else {
         for (int q = 0; q < kNC; ++q) {
-            const double* F = f.C64 + q * Lf.cs;
-            out[q * Lc.cs + oc] = restrict_node<double, 3>(Lc, Lf, I, J, K, [F](long long o) { return F[o]; });
+            const double* F = f.C64 + q * Lf.cs + of;
+            out[q * Lc.cs + oc] =
+                restrict_node<double, 3>(Lc, I, J, K, [F, px, pl](int a, int b, int c) { return F[a + px * b + pl * c]; });
         }
     }
 }

@@ -467,26 +467,28 @@ __device__ __forceinline__ void node_diag(const Lay& L, const double* __restrict
     node_diag_acc<K, DIM>(L, PlaneAcc<double>{C, L.cs}, i, j, k, bc, h, dg);
 }
 
-// Nodal full weighting onto coarse node (I,J,K) of Lc from the fine level Lf
+// Nodal full weighting onto coarse node (I,J,K) of the coarse level Lc
 // (Operator.cpp:486-543; the same stencil coarsens the stiffness,
-// Elastic.cpp:759-803).  F(o) returns the fine value at offset o.  In 2D the k
-// direction counts as "on the boundary" (the reference's degenerate z box),
-// which selects the 2D branches.
+// Elastic.cpp:759-803, and refluxes the fine residual, Operator.cpp:779-822).
+// F(a, b, c) returns the fine value at fine node (2I+a, 2J+b, 2K+c).  The
+// branches follow the coarse level's domain boundary; in 2D the k direction
+// counts as "on the boundary" (the reference's degenerate z box), which
+// selects the 2D branches.
 template <typename T, int DIM, typename Get>
-__device__ __forceinline__ T restrict_node(const Lay& Lc, const Lay& Lf, int I, int J, int K, const Get& F) {
+__device__ __forceinline__ T restrict_node(const Lay& Lc, int I, int J, int K, const Get& F) {
     const bool bI = I == 0 || I == Lc.nx - 1, bJ = J == 0 || J == Lc.ny - 1;
     const bool bK = DIM == 2 ? true : (K == 0 || K == Lc.nz - 1);
-    const long long sx = 1, sy = Lf.px, sz = Lf.pl;
-    const long long o = 2LL * I + Lf.px * (2LL * J) + Lf.pl * (2LL * K);
-    if (bI && bJ && bK) return F(o);
-    if (bJ && bK) return T(0.25) * F(o - sx) + T(0.5) * F(o) + T(0.25) * F(o + sx);
-    if (bK && bI) return T(0.25) * F(o - sy) + T(0.5) * F(o) + T(0.25) * F(o + sy);
-    if (bI && bJ) return T(0.25) * F(o - sz) + T(0.5) * F(o) + T(0.25) * F(o + sz);
+    if (bI && bJ && bK) return F(0, 0, 0);
+    if (bJ && bK) return T(0.25) * F(-1, 0, 0) + T(0.5) * F(0, 0, 0) + T(0.25) * F(1, 0, 0);
+    if (bK && bI) return T(0.25) * F(0, -1, 0) + T(0.5) * F(0, 0, 0) + T(0.25) * F(0, 1, 0);
+    if (bI && bJ) return T(0.25) * F(0, 0, -1) + T(0.5) * F(0, 0, 0) + T(0.25) * F(0, 0, 1);
     if (bI || bJ || bK) {
-        // face: (1 2 1) x (1 2 1) / 16 over the two tangential directions
-        const long long u1 = bI ? sy : sx, u2 = bK ? sy : sz;
-        return (F(o - u1 - u2) + T(2) * F(o - u2) + F(o + u1 - u2) + T(2) * F(o - u1) + T(4) * F(o) +
-                T(2) * F(o + u1) + F(o - u1 + u2) + T(2) * F(o + u2) + F(o + u1 + u2)) /
+        // face: (1 2 1) x (1 2 1) / 16 over the two tangential directions u, w
+        const int ux = bI ? 0 : 1, uy = bI ? 1 : 0;              // u: y on an x face, else x
+        const int wy = bK ? 1 : 0, wz = bK ? 0 : 1;              // w: y on a z face, else z
+        auto G = [&](int a, int c) { return F(a * ux, a * uy + c * wy, c * wz); };
+        return (G(-1, -1) + T(2) * G(0, -1) + G(1, -1) + T(2) * G(-1, 0) + T(4) * G(0, 0) + T(2) * G(1, 0) +
+                G(-1, 1) + T(2) * G(0, 1) + G(1, 1)) /
                T(16);
     }
     T corners = 0, edges = 0, faces = 0;
@@ -495,15 +497,14 @@ __device__ __forceinline__ T restrict_node(const Lay& Lc, const Lay& Lf, int I, 
 #pragma unroll
         for (int bb = -1; bb <= 1; bb += 2)
 #pragma unroll
-            for (int c = -1; c <= 1; c += 2) corners += F(o + a * sx + bb * sy + c * sz);
+            for (int c = -1; c <= 1; c += 2) corners += F(a, bb, c);
 #pragma unroll
     for (int a = -1; a <= 1; a += 2)
 #pragma unroll
-        for (int bb = -1; bb <= 1; bb += 2)
-            edges += F(o + a * sy + bb * sz) + F(o + a * sx + bb * sz) + F(o + a * sx + bb * sy);
+        for (int bb = -1; bb <= 1; bb += 2) edges += F(0, a, bb) + F(a, 0, bb) + F(a, bb, 0);
 #pragma unroll
-    for (int a = -1; a <= 1; a += 2) faces += F(o + a * sx) + F(o + a * sy) + F(o + a * sz);
-    return corners / T(64) + edges / T(32) + faces / T(16) + F(o) / T(8);
+    for (int a = -1; a <= 1; a += 2) faces += F(a, 0, 0) + F(0, a, 0) + F(0, 0, a);
+    return corners / T(64) + edges / T(32) + faces / T(16) + F(0, 0, 0) / T(8);
 }
 
 // Trilinear nodal prolongation (Operator.cpp:589-618): the coarse-grid

@@ -207,9 +207,11 @@ __global__ __launch_bounds__(256) void k_restrict(Lay Lc, Lay Lf, int nc, const 
     int I, J, K;
     if (!node_index(Lc, I, J, K)) return;
     const long long oc = (long long)I + Lc.px * J + Lc.pl * K;
+    const long long px = Lf.px, pl = Lf.pl;
     for (int n = 0; n < nc; ++n) {
-        const T* F = fine + n * Lf.cs;
-        crse[n * Lc.cs + oc] = restrict_node<T, DIM>(Lc, Lf, I, J, K, [F](long long o) { return F[o]; });
+        const T* F = fine + n * Lf.cs + 2LL * I + px * (2LL * J) + pl * (2LL * K);
+        crse[n * Lc.cs + oc] =
+            restrict_node<T, DIM>(Lc, I, J, K, [F, px, pl](int a, int b, int c) { return F[a + px * b + pl * c]; });
     }
 }
 

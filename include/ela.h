@@ -226,6 +226,28 @@ int ela_time_fapply(ela_hier* h, int prec, int nrep, double* avg_ms);
  * each), average ms per launch.  It calibrates the HBM byte counters. */
 int ela_probe_copy(ela_ctx* ctx, size_t bytes, int nrep, double* avg_ms);
 
+/* ------------------------------------------------------------------ AMR level transfers
+ *
+ * An AMR level's refined region is one box of nodes of the finer level, with
+ * its own layout (`fine`, e.g. the level 0 of a hierarchy built on the box);
+ * fine_origin is the fine-level node index of its first node (even).  The
+ * coarse level is described by `crse` (its node counts are the coarse
+ * domain's).  Fields on the device in those layouts, ncomp components.
+ * stream NULL: the call completes before returning. */
+
+/* reflux (src/Operator/Operator.cpp:728-839): every coarse node of the
+ * coarsened fine box takes the full-weighting restriction of the fine
+ * residual (the corner / edge / face / interior branches of the coarse
+ * domain); fine values outside the box are ghost nodes, zero. */
+int ela_reflux(const ela_level_layout* crse, void* crse_res, const ela_level_layout* fine, const void* fine_res,
+               const int fine_origin[3], int ncomp, int prec, void* stream);
+
+/* averageDownSolutionRHS (src/Operator/Operator.cpp:626-641): coarse nodes of
+ * the coarsened fine box take the coincident fine solution value (nodal
+ * amrex::average_down). */
+int ela_average_down(const ela_level_layout* crse, void* crse_sol, const ela_level_layout* fine,
+                     const void* fine_sol, const int fine_origin[3], int ncomp, int prec, void* stream);
+
 /* ------------------------------------------------------------------ batched independent solves
  *
  * The reference solves one problem per Solver::Nonlocal::Linear::solve call

@@ -343,58 +343,60 @@ void diag_all(const G& g, int kind, const double* C, const int* bctype, double* 
             }
 }
 
-// Nodal full weighting onto the coarse grid gc from the fine grid (2x finer)
-// (Operator.cpp:486-543; the same stencil coarsens the stiffness, Elastic.cpp:759-803).
-// In 2D the reference's k index is always on its (degenerate) z boundary, so
-// the corner / X-edge / Y-edge / Z-face branches are the ones reached.
+// Nodal full weighting at coarse node (I,J,K) of gc (Operator.cpp:486-543; the
+// same stencil coarsens the stiffness, Elastic.cpp:759-803, and refluxes the
+// fine residual, Operator.cpp:779-822).  F(i,j,k) reads the fine level at fine
+// node indices.  In 2D the reference's k index is always on its (degenerate)
+// z boundary, so the corner / X-edge / Y-edge / Z-face branches are the ones
+// reached.
+template <typename Get>
+double restrict_value(const G& gc, int I, int J, int K, const Get& F) {
+    const int i = 2 * I, j = 2 * J, k = 2 * K;
+    const bool bI = (I == 0 || I == gc.n[0]), bJ = (J == 0 || J == gc.n[1]);
+    const bool bK = gc.dim == 3 ? (K == 0 || K == gc.n[2]) : true;
+    if (bI && bJ && bK) return F(i, j, k);
+    if (bJ && bK) return 0.25 * F(i - 1, j, k) + 0.5 * F(i, j, k) + 0.25 * F(i + 1, j, k);
+    if (bK && bI) return 0.25 * F(i, j - 1, k) + 0.5 * F(i, j, k) + 0.25 * F(i, j + 1, k);
+    if (bI && bJ) return 0.25 * F(i, j, k - 1) + 0.5 * F(i, j, k) + 0.25 * F(i, j, k + 1);
+    if (bI)
+        return (+F(i, j - 1, k - 1) + 2.0 * F(i, j, k - 1) + F(i, j + 1, k - 1) + 2.0 * F(i, j - 1, k) +
+                4.0 * F(i, j, k) + 2.0 * F(i, j + 1, k) + F(i, j - 1, k + 1) + 2.0 * F(i, j, k + 1) +
+                F(i, j + 1, k + 1)) / 16.0;
+    if (bJ)
+        return (+F(i - 1, j, k - 1) + 2.0 * F(i - 1, j, k) + F(i - 1, j, k + 1) + 2.0 * F(i, j, k - 1) +
+                4.0 * F(i, j, k) + 2.0 * F(i, j, k + 1) + F(i + 1, j, k - 1) + 2.0 * F(i + 1, j, k) +
+                F(i + 1, j, k + 1)) / 16.0;
+    if (bK)
+        return (+F(i - 1, j - 1, k) + 2.0 * F(i, j - 1, k) + F(i + 1, j - 1, k) + 2.0 * F(i - 1, j, k) +
+                4.0 * F(i, j, k) + 2.0 * F(i + 1, j, k) + F(i - 1, j + 1, k) + 2.0 * F(i, j + 1, k) +
+                F(i + 1, j + 1, k)) / 16.0;
+    double corners = 0, edges = 0, faces = 0;
+    for (int a = -1; a <= 1; a += 2)
+        for (int b = -1; b <= 1; b += 2)
+            for (int c = -1; c <= 1; c += 2) corners += F(i + a, j + b, k + c);
+    for (int a = -1; a <= 1; a += 2)
+        for (int b = -1; b <= 1; b += 2) {
+            edges += F(i, j + a, k + b);
+            edges += F(i + a, j, k + b);
+            edges += F(i + a, j + b, k);
+        }
+    for (int a = -1; a <= 1; a += 2) faces += F(i + a, j, k) + F(i, j + a, k) + F(i, j, k + a);
+    return corners / 64.0 + edges / 32.0 + faces / 16.0 + F(i, j, k) / 8.0;
+}
+
+// Full weighting of a whole fine level onto the coarse grid gc.
 void restrict_all(const G& gc, int ncomp, const double* fine, double* crse) {
     or_grid fgd{gc.dim, {2 * gc.n[0], 2 * gc.n[1], gc.dim == 3 ? 2 * gc.n[2] : 0}, {gc.dx[0] / 2, gc.dx[1] / 2, gc.dx[2] / 2}};
     G gf(&fgd);
-    const bool d3 = gc.dim == 3;
 #pragma omp parallel for schedule(static)
     for (int K = 0; K < gc.nn[2]; ++K)
         for (int J = 0; J < gc.nn[1]; ++J)
-            for (int I = 0; I < gc.nn[0]; ++I) {
-                const int i = 2 * I, j = 2 * J, k = 2 * K;
-                const bool bI = (I == 0 || I == gc.n[0]), bJ = (J == 0 || J == gc.n[1]);
-                const bool bK = d3 ? (K == 0 || K == gc.n[2]) : true;
+            for (int I = 0; I < gc.nn[0]; ++I)
                 for (int n = 0; n < ncomp; ++n) {
                     const double* fd = fine + (long)n * gf.nodes;
-                    auto F = [&](int a, int b, int c) { return fd[gf.id(a, b, c)]; };
-                    double v;
-                    if (bI && bJ && bK) v = F(i, j, k);
-                    else if (bJ && bK) v = 0.25 * F(i - 1, j, k) + 0.5 * F(i, j, k) + 0.25 * F(i + 1, j, k);
-                    else if (bK && bI) v = 0.25 * F(i, j - 1, k) + 0.5 * F(i, j, k) + 0.25 * F(i, j + 1, k);
-                    else if (bI && bJ) v = 0.25 * F(i, j, k - 1) + 0.5 * F(i, j, k) + 0.25 * F(i, j, k + 1);
-                    else if (bI)
-                        v = (+F(i, j - 1, k - 1) + 2.0 * F(i, j, k - 1) + F(i, j + 1, k - 1) + 2.0 * F(i, j - 1, k) +
-                             4.0 * F(i, j, k) + 2.0 * F(i, j + 1, k) + F(i, j - 1, k + 1) + 2.0 * F(i, j, k + 1) +
-                             F(i, j + 1, k + 1)) / 16.0;
-                    else if (bJ)
-                        v = (+F(i - 1, j, k - 1) + 2.0 * F(i - 1, j, k) + F(i - 1, j, k + 1) + 2.0 * F(i, j, k - 1) +
-                             4.0 * F(i, j, k) + 2.0 * F(i, j, k + 1) + F(i + 1, j, k - 1) + 2.0 * F(i + 1, j, k) +
-                             F(i + 1, j, k + 1)) / 16.0;
-                    else if (bK)
-                        v = (+F(i - 1, j - 1, k) + 2.0 * F(i, j - 1, k) + F(i + 1, j - 1, k) + 2.0 * F(i - 1, j, k) +
-                             4.0 * F(i, j, k) + 2.0 * F(i + 1, j, k) + F(i - 1, j + 1, k) + 2.0 * F(i, j + 1, k) +
-                             F(i + 1, j + 1, k)) / 16.0;
-                    else {
-                        double corners = 0, edges = 0, faces = 0;
-                        for (int a = -1; a <= 1; a += 2)
-                            for (int b = -1; b <= 1; b += 2)
-                                for (int c = -1; c <= 1; c += 2) corners += F(i + a, j + b, k + c);
-                        for (int a = -1; a <= 1; a += 2)
-                            for (int b = -1; b <= 1; b += 2) {
-                                edges += F(i, j + a, k + b);
-                                edges += F(i + a, j, k + b);
-                                edges += F(i + a, j + b, k);
-                            }
-                        for (int a = -1; a <= 1; a += 2) faces += F(i + a, j, k) + F(i, j + a, k) + F(i, j, k + a);
-                        v = corners / 64.0 + edges / 32.0 + faces / 16.0 + F(i, j, k) / 8.0;
-                    }
-                    crse[(long)n * gc.nodes + gc.id(I, J, K)] = v;
+                    crse[(long)n * gc.nodes + gc.id(I, J, K)] =
+                        restrict_value(gc, I, J, K, [&](int a, int b, int c) { return fd[gf.id(a, b, c)]; });
                 }
-            }
 }
 
 // Trilinear nodal prolongation added into the fine field (Operator.cpp:589-618).
@@ -716,6 +718,35 @@ void or_assemble(const or_grid* og, int kind, const double* C, const int* bctype
         e[q] = 0.0;
         for (long r = 0; r < n; ++r) Adense[r * n + q] = col[r];
     }
+}
+
+// reflux (Operator.cpp:728-839): every coarse node of the coarsened fine box
+// takes the full-weighting restriction of the fine residual (the branches at
+// the coarse DOMAIN boundary).  The fine box covers fine-level nodes
+// [o, o + nb - 1] per direction (o even, nb odd) and its residual is a dense
+// array of those nodes; fine values outside the box are the residual's ghost
+// nodes, taken as zero (the state of the residuals in the reference's
+// RefluxTest, src/Test/Operator/Elastic_RefluxTest.cpp:80-82).
+void or_reflux(const or_grid* crse, int ncomp, double* crse_res, const int* fine_origin, const int* fine_nodes,
+               const double* fine_res) {
+    G gc(crse);
+    const int o[3] = {fine_origin[0], fine_origin[1], gc.dim == 3 ? fine_origin[2] : 0};
+    const int nb[3] = {fine_nodes[0], fine_nodes[1], gc.dim == 3 ? fine_nodes[2] : 1};
+    const long fnodes = (long)nb[0] * nb[1] * nb[2];
+    const int lo[3] = {o[0] / 2, o[1] / 2, o[2] / 2};
+    const int hi[3] = {(o[0] + nb[0] - 1) / 2, (o[1] + nb[1] - 1) / 2, (o[2] + nb[2] - 1) / 2};
+    for (int K = lo[2]; K <= hi[2]; ++K)
+        for (int J = lo[1]; J <= hi[1]; ++J)
+            for (int I = lo[0]; I <= hi[0]; ++I)
+                for (int n = 0; n < ncomp; ++n) {
+                    const double* fd = fine_res + (long)n * fnodes;
+                    auto F = [&](int a, int b, int c) {
+                        const int x = a - o[0], y = b - o[1], z = c - o[2];
+                        if (x < 0 || y < 0 || z < 0 || x >= nb[0] || y >= nb[1] || z >= nb[2]) return 0.0;
+                        return fd[x + (long)nb[0] * (y + (long)nb[1] * z)];
+                    };
+                    crse_res[(long)n * gc.nodes + gc.id(I, J, K)] = restrict_value(gc, I, J, K, F);
+                }
 }
 
 // Multigrid solve of A u = b on one level (restated MLMG, see header).

@@ -73,6 +73,7 @@ def lib():
         L.or_assemble.argtypes = [P(_Grid), i, vp, vp, vp]
         L.or_mg_solve.argtypes = [P(_Grid), i, vp, vp, vp, vp, d, d, i, i, i, i, i, vp, vp, i]
         L.or_mg_solve.restype = i
+        L.or_reflux.argtypes = [P(_Grid), i, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -166,6 +167,23 @@ def restrict(gc, fine):
     crse = gc.zeros(nc)
     lib().or_restrict(ctypes.byref(gc.c()), nc, _p(fine), _p(crse))
     return crse
+
+
+def reflux(gc, crse_res, fine_origin, fine_res):
+    """reflux (Operator.cpp:728-839): coarse residual nodes of the coarsened fine box take the
+    full-weighting restriction of the fine residual (zero ghost values outside the box).
+
+    crse_res: (ncomp, *coarse nodes reversed); fine_res: (ncomp, *box nodes reversed);
+    fine_origin: fine-level node index (x, y[, z]) of the box's first node (even)."""
+    out = _c(crse_res).copy()
+    f = _c(fine_res)
+    nc = f.shape[0]
+    fn = list(reversed(f.shape[1:])) + [1]
+    org = list(fine_origin) + [0]
+    o = np.array(org[:3], dtype=np.int32)
+    nb = np.array(fn[:3], dtype=np.int32)
+    lib().or_reflux(ctypes.byref(gc.c()), nc, _p(out), _p(o), _p(nb), _p(f))
+    return out
 
 
 def interp_add(gf, crse, fine):

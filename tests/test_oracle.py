@@ -350,3 +350,67 @@ def test_trig_known_answer(oracle):
     assert st["converged"]
     exact = -trig / (3 * np.pi ** 2)
     assert np.abs(u[0] - exact).max() / np.abs(exact).max() < 0.02
+
+
+# ------------------------------------------------------------------ reflux (Operator.cpp:728-839)
+def _reflux_setup(oracle, nc_cells=8, origin=(4, 4, 4), box_cells=(8, 8, 8)):
+    gc = oracle.Grid((nc_cells,) * 3)
+    fine_nodes = tuple(c + 1 for c in box_cells)
+    return gc, origin, fine_nodes
+
+
+def test_reflux_known_answer_reference_reflux_test(oracle):
+    """The reference's RefluxTest (src/Test/Operator/Elastic_RefluxTest.cpp:14-108): two AMR levels
+    (coarse 16^3, fine box the refinement of the middle half), isotropic model (mu, lambda) = (2.6, 6.0),
+    affine-halfspace solutions on both levels; with both residuals zero, reflux leaves the coarse
+    residual zero (< 1e-12 relative to |rhs|)."""
+    n = 16
+    gc = oracle.Grid((n, n, n))
+    C = np.zeros((2, n + 1, n + 1, n + 1))
+    C[0], C[1] = 6.0, 2.6
+    bct = np.zeros((26, 3), np.int32)
+    x, y, z = gc.coords()
+    u = gc.zeros()
+    u[1] = np.where(x - 0.25 > 0, (x - 0.25) ** 2, 0.0)  # IC::Affine, halfspace, m = 2, comp 1
+    rhs0 = oracle.fapply(gc, oracle.ISO, C, bct, u)
+    box = (2 * n // 2 + 1,) * 3  # fine box = refinement of coarse cells [n/4, 3n/4)
+    res0 = gc.zeros()
+    res1 = np.zeros((3,) + box)
+    out = oracle.reflux(gc, res0, (n // 2,) * 3, res1)
+    assert np.abs(out).max() / np.abs(rhs0).max() < 1e-12
+
+
+def test_reflux_of_a_constant_is_a_partition_of_unity(oracle):
+    """Every branch's weights sum to 1: a constant fine residual over the whole fine domain refluxes
+    to the same constant on every coarse node (corners, edges, faces, interior)."""
+    gc = oracle.Grid((6, 6, 6))
+    fine = np.full((3, 13, 13, 13), 2.5)
+    out = oracle.reflux(gc, gc.zeros(), (0, 0, 0), fine)
+    assert np.allclose(out, 2.5, rtol=0, atol=1e-15)
+
+
+def test_reflux_reproduces_affine_fields_and_ghost_zeros(oracle):
+    """Full weighting reproduces affine fields on coarse nodes whose stencil lies inside the fine box;
+    on the box's faces / edges / corners the zero ghost nodes remove a 1/4 slab per direction, so a
+    constant refluxes to 3/4, 9/16, 27/64 there (separable (1/4, 1/2, 1/4) weights)."""
+    gc = oracle.Grid((12, 12, 12))
+    o, cells = (4, 6, 8), (10, 8, 12)
+    nb = tuple(c + 1 for c in cells)
+    h = 1.0 / 24
+    zz, yy, xx = np.meshgrid(*[(o[d] + np.arange(nb[d])) * h for d in (2, 1, 0)], indexing="ij")
+    aff = np.stack([1.0 + 2 * xx - 3 * yy + 0.5 * zz, -xx + yy, 0.25 + 4 * zz])
+    out = oracle.reflux(gc, gc.zeros(), o, aff)
+    X, Y, Z = gc.coords()
+    want = np.stack([1.0 + 2 * X - 3 * Y + 0.5 * Z, -X + Y, 0.25 + 4 * Z])
+    lo = [o[d] // 2 for d in range(3)]
+    hi = [(o[d] + nb[d] - 1) // 2 for d in range(3)]
+    inner = (slice(lo[2] + 1, hi[2]), slice(lo[1] + 1, hi[1]), slice(lo[0] + 1, hi[0]))
+    assert np.abs(out[(slice(None),) + inner] - want[(slice(None),) + inner]).max() < 1e-14
+    ones = oracle.reflux(gc, gc.zeros(), o, np.ones((3,) + tuple(reversed(nb))))
+    k, j, i = lo[2] + 2, lo[1] + 2, lo[0]
+    assert ones[0, k, j, i] == pytest.approx(0.75, abs=1e-15)  # x face of the box
+    assert ones[0, k, lo[1], i] == pytest.approx(0.5625, abs=1e-15)  # box edge
+    assert ones[0, lo[2], lo[1], i] == pytest.approx(27 / 64, abs=1e-15)  # box corner
+    outside = np.ones_like(ones[0], bool)
+    outside[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = False
+    assert not ones[:, outside].any()  # nodes outside the coarsened box are untouched
