@@ -146,6 +146,8 @@ SIGNATURES = {
     "ela_batch_solve": ([_P, _I, _P, _D, _D, _I], _I),
     "ela_rve_status": ([_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(Stats)], _I),
     "ela_batch_set_timing": ([_P, _I], _I),
+    "ela_batch_set_option": ([_P, _I, _I], _I),
+    "ela_normalize": ([_P, _I, _I, _P, _P], _I),
     "ela_reflux": ([ctypes.POINTER(_Layout), _P, ctypes.POINTER(_Layout), _P, _P, _I, _I, _P], _I),
     "ela_average_down": ([ctypes.POINTER(_Layout), _P, ctypes.POINTER(_Layout), _P, _P, _I, _I, _P], _I),
     "ela_serve": ([_P, _I, ctypes.POINTER(Request), _D, _D, _I, _I, _D, ctypes.POINTER(Result)], _I),
@@ -353,6 +355,15 @@ class Elastic:
         _check(lib().ela_diagonal(self.ptr, mglev, prec, fd.ptr, None))
         out = fd.download()
         fd.free()
+        return out
+
+    def normalize(self, x, mglev=0, prec=F64):
+        """Operator<Node>::normalize (Operator.cpp:164-195) as the reference computes it: component 0
+        of x divided by diagonal component 0 once per component; other components unchanged."""
+        f = self.field(mglev, prec).upload(x)
+        _check(lib().ela_normalize(self.ptr, mglev, prec, f.ptr, None))
+        out = f.download()
+        f.free()
         return out
 
     def Fsmooth(self, x, b, mglev=0, prec=F64, nsweeps=2, omega=2.0 / 3.0):
@@ -601,6 +612,13 @@ class Batch:
 
     def set_timing(self, on=True):
         _check(lib().ela_batch_set_timing(self.ptr, int(bool(on))))
+
+    MARCH, KZ = 0, 1
+
+    def set_option(self, option, value):
+        """ela_batch_set_option: Batch.MARCH (LDS z-marching kernels on the finest levels, default on) or
+        Batch.KZ (planes per marching tile, default 16)."""
+        _check(lib().ela_batch_set_option(self.ptr, option, int(value)))
 
     def serve(self, requests, tol_rel=1e-10, tol_abs=0.0, max_cycles=100, max_active=256, max_seconds=0.0,
               keep=()):

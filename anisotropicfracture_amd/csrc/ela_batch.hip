@@ -52,6 +52,10 @@ struct BoxDesc {
     int nodes;
     int nunk;         // coarsest level: unknowns of the dense inverse
     int uniform;      // SetUniform: drop the grad C term (callers use 0)
+    // Marching launches over grain-form boxes: interior tiles of MTX x MTY
+    // nodes marching kz planes in z, plus the boundary shell in blocks of TB.
+    int mt0, ntx, nty, ntz;  // first marching block of the box, tiles per direction
+    int kz, st0, nsh, ngr;   // planes per tile, first shell block, shell blocks, grains
     Inv<double> i64;
     Inv<float> i32;
     double h[3];
@@ -378,6 +382,241 @@ __global__ __launch_bounds__(TB) void kb_cvt(const BoxDesc* __restrict__ D, cons
     }
 }
 
+// ------------------------------------------------------------------ marching kernels (grain form)
+// The interior of a grain-form box is cut into MTX x MTY tiles of (i, j) that
+// march up kz planes in k.  A ring of three planes of u (tile + one-node
+// halo), of the grain ids and of the degradation factor lives in LDS, with the
+// member's grain table; each plane of u is read from memory once per tile
+// column instead of once per neighbouring row of every wavefront.  The
+// stencil is interior_apply on an LDS context, so every node gets the same
+// bits as from the global-memory kernels.  Boundary nodes (the box's shell)
+// are done by a separate kernel with the global-memory boundary rows.
+constexpr int MTX = 32, MTY = 8, MRW = MTY + 2, MCW = MTX + 2, MPL = MRW * MCW;
+constexpr int kMarchMaxGrains = 256;
+
+template <typename T>
+struct LdsCtx {
+    const T* su;                // [3 slots][3 comps][MRW][MCW]
+    const unsigned short* sg;   // [3 slots][MRW][MCW]
+    const T* ss;                // [3 slots][MRW][MCW] or null
+    const T* stab;              // grain table (kNC per grain)
+    int sl[3];                  // ring slots of planes k-1, k, k+1
+    int r, c;                   // this node's row / column in the tile arrays
+    __device__ __forceinline__ int at(int dx, int dy, int dz) const {
+        return (sl[dz + 1] * MRW + r + dy) * MCW + c + dx;
+    }
+    __device__ __forceinline__ T u(int p, int dx, int dy, int dz) const {
+        return su[((sl[dz + 1] * 3 + p) * MRW + r + dy) * MCW + c + dx];
+    }
+    __device__ __forceinline__ ScaledRec<T> rec(int dx, int dy, int dz) const {
+        const int q = at(dx, dy, dz);
+        return ScaledRec<T>{stab + (int)sg[q] * kNC, ss ? ss[q] : T(1)};
+    }
+    __device__ __forceinline__ bool same(int d) const {
+        const int qp = at(d == 0, d == 1, d == 2), qm = at(-(d == 0), -(d == 1), -(d == 2));
+        return sg[qp] == sg[qm] && (!ss || ss[qp] == ss[qm]);
+    }
+};
+
+// Operands of a marching / shell launch.  MODE 0: Jacobi sweep f[out] = f[in]
+// + omega (f[rhs] - A f[in]) / D; MODE 1: f[out] = f[rhs] - A f[in]; MODE 2:
+// FP64 solution residual r = b64 - A u64 into f[out] (precision TO) scaled by
+// the member's scale, with block maxima of |r| in partials.
+struct MOps {
+    int rhs, in, out;
+    double omega;
+    double* partials;
+    int mtot;  // marching blocks of the launch (shell partials follow them)
+};
+
+template <typename T, int MODE, typename TO>
+__device__ __forceinline__ double m_epilogue(const BoxDesc& d, const MOps& op, long long o, const T xc[3],
+                                             const T f[3]) {
+    const long long cs = d.L.cs;
+    double m = 0.0;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const long long q = p * cs + o;
+        if (MODE == 0) {
+            const T* b = (const T*)d.f[op.rhs];
+            ((T*)d.f[op.out])[q] = xc[p] + (T)op.omega * (b[q] - f[p]) / P<T>::dg(d)[q];
+        } else if (MODE == 1) {
+            ((T*)d.f[op.out])[q] = ((const T*)d.f[op.rhs])[q] - f[p];
+        } else {
+            const double v = d.b64[q] - (double)f[p];
+            ((TO*)d.f[op.out])[q] = (TO)(v * *d.scale);
+            const double a = fabs(v);
+            m = (a > m || a != a) ? a : m;
+        }
+    }
+    return m;
+}
+
+template <typename T, int MODE, typename TO>
+__global__ __launch_bounds__(MTX* MTY) void km_grain(const BoxDesc* __restrict__ D, const int* __restrict__ bmap,
+                                                     MOps op) {
+    static_assert(MODE != 2 || sizeof(T) == 8, "the solution residual is FP64");
+    __shared__ T su[3 * 3 * MPL];
+    __shared__ unsigned short sg[3 * MPL];
+    __shared__ T ss[3 * MPL];
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    T* stab = (T*)dyn_lds;
+    const BoxDesc& d = D[bmap[blockIdx.x]];
+    const Lay L = d.L;
+    const int tid = threadIdx.x;
+    const int t = (int)blockIdx.x - d.mt0;
+    const int bx = t % d.ntx, by = (t / d.ntx) % d.nty, bz = t / (d.ntx * d.nty);
+    const int i0 = 1 + bx * MTX, j0 = 1 + by * MTY;  // first interior node of the tile
+    const int tx = tid % MTX, ty = tid / MTX;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool active = i <= L.nx - 2 && j <= L.ny - 2;
+    const int kb = 1 + bz * d.kz, ke = min(kb + d.kz, L.nz - 1);
+    const T* u = MODE == 2 ? (const T*)d.u64 : (const T*)d.f[op.in];
+    const T* tab = P<T>::tab(d);
+    const T* sc = P<T>::sc(d);
+    const unsigned short* gid = d.gid;
+    for (int q = tid; q < d.ngr * kNC; q += MTX * MTY) stab[q] = tab[q];
+
+    // plane loads: u (3 comps), grain ids, factors of the tile + halo at plane kk
+    constexpr int PU = (3 * MPL + MTX * MTY - 1) / (MTX * MTY), PG = (MPL + MTX * MTY - 1) / (MTX * MTY);
+    T ru[PU], rs[PG];
+    unsigned short rg[PG];
+    auto load = [&](int kk) {
+#pragma unroll
+        for (int e = 0; e < PU; ++e) {
+            const int q = tid + e * MTX * MTY;
+            T v = 0;
+            if (q < 3 * MPL) {
+                const int p = q / MPL, rem = q - p * MPL, rr = rem / MCW, cc = rem - rr * MCW;
+                const int ii = i0 - 1 + cc, jj = j0 - 1 + rr;
+                if (ii < L.nx && jj < L.ny) v = u[p * L.cs + ii + L.px * jj + L.pl * kk];
+            }
+            ru[e] = v;
+        }
+#pragma unroll
+        for (int e = 0; e < PG; ++e) {
+            const int q = tid + e * MTX * MTY;
+            unsigned short gv = 0;
+            T sv = 1;
+            if (q < MPL) {
+                const int rr = q / MCW, cc = q - rr * MCW;
+                const int ii = i0 - 1 + cc, jj = j0 - 1 + rr;
+                if (ii < L.nx && jj < L.ny) {
+                    const long long o = ii + L.px * jj + L.pl * kk;
+                    gv = gid[o];
+                    if (sc) sv = sc[o];
+                }
+            }
+            rg[e] = gv;
+            rs[e] = sv;
+        }
+    };
+    auto store = [&](int slot) {
+#pragma unroll
+        for (int e = 0; e < PU; ++e) {
+            const int q = tid + e * MTX * MTY;
+            if (q < 3 * MPL) su[slot * 3 * MPL + q] = ru[e];
+        }
+#pragma unroll
+        for (int e = 0; e < PG; ++e) {
+            const int q = tid + e * MTX * MTY;
+            if (q < MPL) {
+                sg[slot * MPL + q] = rg[e];
+                ss[slot * MPL + q] = rs[e];
+            }
+        }
+    };
+    load(kb - 1);
+    store((kb - 1) % 3);
+    load(kb);
+    store(kb % 3);
+    load(kb + 1);
+    store((kb + 1) % 3);
+    __syncthreads();
+    double m = 0.0;
+    for (int k = kb; k < ke; ++k) {
+        const bool more = k + 1 < ke;
+        if (more) load(k + 2);
+        if (active) {
+            LdsCtx<T> S{su, sg, sc ? ss : nullptr, stab, {(k - 1) % 3, k % 3, (k + 1) % 3}, ty + 1, tx + 1};
+            T f[3], xc[3];
+            interior_apply<T, K_MM, 3>(S, P<T>::inv(d), d.uniform != 0, f);
+#pragma unroll
+            for (int p = 0; p < 3; ++p) xc[p] = S.u(p, 0, 0, 0);
+            const double mm = m_epilogue<T, MODE, TO>(d, op, (long long)i + L.px * j + L.pl * k, xc, f);
+            m = (mm > m || mm != mm) ? mm : m;
+        }
+        __syncthreads();  // every read of plane k-1's slot is done
+        if (more) store((k + 2) % 3);
+        __syncthreads();
+    }
+    if (MODE == 2) block_max(op.partials, m);
+}
+
+// Boundary shell of a grain-form box: both z faces, then the y faces and the
+// x faces between them; boundary rows through the global-memory path.
+template <typename T, int MODE, typename TO>
+__global__ __launch_bounds__(TB) void ks_grain(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, MOps op) {
+    const BoxDesc& d = D[bmap[blockIdx.x]];
+    const Lay L = d.L;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    int q = ((int)blockIdx.x - d.st0) * TB + (int)threadIdx.x;
+    const int A = nx * ny, Bf = nx * (nz - 2), Cf = (ny - 2) * (nz - 2);
+    bool in = q < 2 * A + 2 * Bf + 2 * Cf;
+    int i = 0, j = 0, k = 0;
+    if (in) {
+        if (q < 2 * A) {
+            k = q < A ? 0 : nz - 1;
+            const int r = q % A;
+            j = r / nx;
+            i = r - j * nx;
+        } else if ((q -= 2 * A) < 2 * Bf) {
+            j = q < Bf ? 0 : ny - 1;
+            const int r = q % Bf;
+            k = 1 + r / nx;
+            i = r - (k - 1) * nx;
+        } else {
+            q -= 2 * Bf;
+            i = q < Cf ? 0 : nx - 1;
+            const int r = q % Cf;
+            k = 1 + r / (ny - 2);
+            j = 1 + r - (k - 1) * (ny - 2);
+        }
+    }
+    double m = 0.0;
+    if (in) {
+        const T* u = MODE == 2 ? (const T*)d.u64 : (const T*)d.f[op.in];
+        T f[3], xc[3];
+        node_apply_acc<T, K_MM, 3>(L, GrainAcc<T, kNC>{d.gid, P<T>::tab(d), P<T>::sc(d)}, u, i, j, k, d.bc,
+                                   P<T>::inv(d), d.uniform != 0, f);
+        const long long o = (long long)i + L.px * j + L.pl * k;
+#pragma unroll
+        for (int p = 0; p < 3; ++p) xc[p] = u[p * L.cs + o];
+        m = m_epilogue<T, MODE, TO>(d, op, o, xc, f);
+    }
+    if (MODE == 2) block_max(op.partials + op.mtot, m);
+}
+
+// |r|inf of each box from the partials of its marching and shell blocks.
+__global__ __launch_bounds__(64) void kb_fold_m(const BoxDesc* __restrict__ D, const double* __restrict__ partials,
+                                                int mtot, double* __restrict__ out) {
+    const BoxDesc& d = D[blockIdx.x];
+    const int nm = d.ntx * d.nty * d.ntz;
+    double m = 0.0;
+    for (int q = threadIdx.x; q < nm + d.nsh; q += 64) {
+        const double v = q < nm ? partials[d.mt0 + q] : partials[mtot + d.st0 + q - nm];
+        m = (v > m || v != v) ? v : m;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off, 64);
+        m = (o > m || o != o) ? o : m;
+    }
+    if (threadIdx.x == 0) {
+        *d.norm = m;
+        out[blockIdx.x] = m;
+    }
+}
+
 // ------------------------------------------------------------------ set-up
 // Stiffness of a coarse level: the full-weighting average of the fine level's
 // (Elastic.cpp:722-809), from the grain form or from planes; launched over the coarse list.
@@ -608,6 +847,15 @@ struct ela_batch {
     std::vector<size_t> off_desc, off_bmap;  // device offsets of level lists (bytes)
     size_t off_top_desc = 0, off_top_bmap = 0;
     std::vector<int> top_tiles;
+    // marching launches over the grain-form boxes of each level and of the top list
+    struct March {
+        size_t off_bm = 0, off_bs = 0;  // device offsets of the marching / shell block maps
+        int nm = 0, ns = 0, maxng = 0;  // marching blocks, shell blocks, largest grain count
+    };
+    std::vector<March> mlev;
+    March mtop;
+    bool march = true;
+    int kz = 16;
     char* dlists = nullptr;
     size_t dlists_cap = 0;
     std::vector<char> hlists;
@@ -696,6 +944,7 @@ BoxDesc desc_of(const ela_rve* r, int l) {
     d.bc = r->bc;
     d.uniform = r->uniform;
     if (l == 0) {
+        d.ngr = r->ngrains;
         d.gid = r->gid;
         d.tab64 = r->tab64;
         d.tab32 = r->tab32;
@@ -760,6 +1009,40 @@ void pack_list(ela_batch* B, const ListBuild& lb, size_t* off_desc, size_t* off_
         for (int t = lb.tile[b]; t < lb.tile[b + 1]; ++t) bm[t] = (int)b;
 }
 
+// Marching numbering of the grain-form boxes [a, b) of a list: interior tiles
+// and shell blocks per box, and the block -> box maps of both launches.
+void march_number(ListBuild& lb, int a, int b, int kz, std::vector<int>& bm, std::vector<int>& bs,
+                  ela_batch::March& mi) {
+    mi = ela_batch::March{};
+    bm.clear();
+    bs.clear();
+    for (int x = a; x < b; ++x) {
+        BoxDesc& d = lb.descs[x];
+        const Lay& L = d.L;
+        d.kz = kz;
+        d.ntx = (L.nx - 2 + MTX - 1) / MTX;
+        d.nty = (L.ny - 2 + MTY - 1) / MTY;
+        d.ntz = (L.nz - 2 + kz - 1) / kz;
+        d.mt0 = mi.nm;
+        mi.nm += d.ntx * d.nty * d.ntz;
+        const long long shell = (long long)d.nodes - (long long)(L.nx - 2) * (L.ny - 2) * (L.nz - 2);
+        d.nsh = (int)((shell + TB - 1) / TB);
+        d.st0 = mi.ns;
+        mi.ns += d.nsh;
+        mi.maxng = std::max(mi.maxng, d.ngr);
+        bm.insert(bm.end(), (size_t)(d.ntx * d.ntz * d.nty), x);
+        bs.insert(bs.end(), (size_t)d.nsh, x);
+    }
+}
+
+size_t pack_ints(ela_batch* B, const std::vector<int>& v) {
+    B->hlists.resize(up256(B->hlists.size()));
+    const size_t off = B->hlists.size();
+    B->hlists.resize(off + v.size() * sizeof(int));
+    if (!v.empty()) std::memcpy(B->hlists.data() + off, v.data(), v.size() * sizeof(int));
+    return off;
+}
+
 // Rebuild the level lists of the active set (members ordered by their finest global level).
 int build_lists(ela_batch* B) {
     std::stable_sort(B->active.begin(), B->active.end(), [](const ela_rve* a, const ela_rve* b) {
@@ -773,6 +1056,8 @@ int build_lists(ela_batch* B) {
     B->off_desc.assign(G, 0);
     B->off_bmap.assign(G, 0);
     B->hlists.clear();
+    B->mlev.assign(G, ela_batch::March{});
+    std::vector<int> bm, bs;
     for (int g = 0; g < G; ++g) {
         ListBuild lb;
         for (auto* r : B->active) {
@@ -781,16 +1066,22 @@ int build_lists(ela_batch* B) {
         }
         B->cnt[g] = (int)lb.descs.size();
         B->tiles[g] = lb.tile.empty() ? std::vector<int>{0} : lb.tile;
+        march_number(lb, g > 0 ? B->cnt[g - 1] : 0, B->cnt[g], B->kz, bm, bs, B->mlev[g]);
         pack_list(B, lb, &B->off_desc[g], &B->off_bmap[g]);
+        B->mlev[g].off_bm = pack_ints(B, bm);
+        B->mlev[g].off_bs = pack_ints(B, bs);
     }
     ListBuild top;
     for (auto* r : B->active) list_push(top, desc_of(r, 0));
     B->top_tiles = top.tile.empty() ? std::vector<int>{0} : top.tile;
+    march_number(top, 0, (int)top.descs.size(), B->kz, bm, bs, B->mtop);
     pack_list(B, top, &B->off_top_desc, &B->off_top_bmap);
+    B->mtop.off_bm = pack_ints(B, bm);
+    B->mtop.off_bs = pack_ints(B, bs);
     if (int rc = ensure_device(B, (void**)&B->dlists, &B->dlists_cap, B->hlists.size())) return rc;
     if (!B->hlists.empty())
         ELA_HIPCHK(hipMemcpyAsync(B->dlists, B->hlists.data(), B->hlists.size(), hipMemcpyHostToDevice, B->stream));
-    const size_t nblk_top = (size_t)B->top_tiles.back();
+    const size_t nblk_top = std::max((size_t)B->top_tiles.back(), (size_t)(B->mtop.nm + B->mtop.ns));
     if (int rc = ensure_device(B, (void**)&B->partials, &B->partials_cap, nblk_top * sizeof(double))) return rc;
     if (B->norms_cap < B->active.size()) {
         if (B->dnorms) (void)hipFree(B->dnorms);
@@ -824,6 +1115,19 @@ long long nodes_of(const ela_batch* B, int g, int count) {
 // counts the members that already exist one level up (their finest is above g).
 int split_of(const ela_batch* B, int g, int count) { return std::min(count, g > 0 ? B->cnt[g - 1] : 0); }
 
+bool use_march(const ela_batch* B, const ela_batch::March& mi) {
+    return B->march && mi.nm > 0 && mi.maxng <= kMarchMaxGrains;
+}
+
+template <typename T, int MODE, typename TO>
+void march_launch(ela_batch* B, const BoxDesc* D, const ela_batch::March& mi, const MOps& op) {
+    const size_t lds = (size_t)mi.maxng * kNC * sizeof(T);
+    hipLaunchKernelGGL((km_grain<T, MODE, TO>), dim3(mi.nm), dim3(MTX * MTY), lds, B->stream, D,
+                       (const int*)(B->dlists + mi.off_bm), op);
+    hipLaunchKernelGGL((ks_grain<T, MODE, TO>), dim3(mi.ns), dim3(TB), 0, B->stream, D,
+                       (const int*)(B->dlists + mi.off_bs), op);
+}
+
 template <typename T>
 void smooth(ela_batch* B, int g, int count, int rhs, int in, int out) {
     const int nb = nblocks(B, g, count);
@@ -837,9 +1141,13 @@ void smooth(ela_batch* B, int g, int count, int rhs, int in, int out) {
     if (b0)
         hipLaunchKernelGGL((kb_smooth<T, 0>), dim3(b0), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), 0, rhs, in, out,
                            (T)(2.0 / 3.0));
-    if (nb > b0)
-        hipLaunchKernelGGL((kb_smooth<T, 1>), dim3(nb - b0), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), b0, rhs, in,
-                           out, (T)(2.0 / 3.0));
+    if (nb > b0) {
+        if (in >= 0 && count == B->cnt[g] && use_march(B, B->mlev[g]))
+            march_launch<T, 0, T>(B, LD(B, g), B->mlev[g], MOps{rhs, in, out, 2.0 / 3.0, nullptr, 0});
+        else
+            hipLaunchKernelGGL((kb_smooth<T, 1>), dim3(nb - b0), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), b0, rhs,
+                               in, out, (T)(2.0 / 3.0));
+    }
     if (e >= 0) {
         (void)hipEventRecord(B->ev[e + 1], B->stream);
         B->ev_launch.push_back({e, nodes_of(B, g, count)});
@@ -860,8 +1168,13 @@ void resid(ela_batch* B, int g, int count, int x, int b, int r) {
     const int nb = nblocks(B, g, count);
     const int s = split_of(B, g, count), b0 = nblocks(B, g, s);
     if (b0) hipLaunchKernelGGL((kb_resid<T, 0>), dim3(b0), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), 0, x, b, r);
-    if (nb > b0)
-        hipLaunchKernelGGL((kb_resid<T, 1>), dim3(nb - b0), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), b0, x, b, r);
+    if (nb > b0) {
+        if (count == B->cnt[g] && use_march(B, B->mlev[g]))
+            march_launch<T, 1, T>(B, LD(B, g), B->mlev[g], MOps{b, x, r, 0.0, nullptr, 0});
+        else
+            hipLaunchKernelGGL((kb_resid<T, 1>), dim3(nb - b0), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), b0, x, b,
+                               r);
+    }
 }
 
 template <typename T>
@@ -1014,7 +1327,12 @@ int step_t(ela_batch* B) {
     const int nblk = B->top_tiles.back();
     hipLaunchKernelGGL((kb_update<T>), dim3(nblk), dim3(TB), 0, B->stream, TD(B), TM(B), F_COR);
     hipLaunchKernelGGL(kb_scale, dim3((nbox + 63) / 64), dim3(64), 0, B->stream, TD(B), nbox);
-    resid64_top<T>(B, TD(B), TM(B), nbox, nblk);
+    if (use_march(B, B->mtop)) {
+        march_launch<double, 2, T>(B, TD(B), B->mtop, MOps{0, 0, F_RES, 0.0, B->partials, B->mtop.nm});
+        hipLaunchKernelGGL(kb_fold_m, dim3(nbox), dim3(64), 0, B->stream, TD(B), B->partials, B->mtop.nm, B->dnorms);
+    } else {
+        resid64_top<T>(B, TD(B), TM(B), nbox, nblk);
+    }
     return read_norms(B, nbox);
 }
 
@@ -1510,6 +1828,21 @@ int ela_rve_status(const ela_rve* r, int* state, int* status, ela_stats* st) {
         st->seconds = r->state == ST_DONE ? r->t_done - r->t_submit : 0.0;
         st->setup_seconds = 0;
     }
+    return ELA_OK;
+}
+
+int ela_batch_set_option(ela_batch* B, int option, int value) {
+    if (!B) return fail(ELA_ERR_BADARG, "null batch");
+    if (!B->active.empty() || !B->queued.empty()) return fail(ELA_ERR_STATE, "batch is busy");
+    if (option == ELA_BATCH_MARCH) {
+        B->march = value != 0;
+    } else if (option == ELA_BATCH_KZ) {
+        if (value < 1 || value > 1024) return fail(ELA_ERR_BADARG, "planes per marching tile must be 1..1024");
+        B->kz = value;
+    } else {
+        return fail(ELA_ERR_BADARG, "unknown option");
+    }
+    B->dirty = true;
     return ELA_OK;
 }
 

@@ -265,41 +265,23 @@ __device__ __forceinline__ void bc_row(const BCTab& bc, int face, const T u[DIM]
     }
 }
 
-// f = A u at node (i,j,k).  C points at plane 0 of the stiffness, u at comp 0.
-template <typename T, int K, int DIM, typename Acc>
-__device__ __forceinline__ void node_apply_acc(const Lay& L, const Acc& A, const T* __restrict__ u, int i, int j,
-                                               int k, const BCTab& bc, const Inv<T>& iv, bool uniform, T f[DIM]) {
-    const long long o = (long long)i + L.px * j + L.pl * k;
-    const int hx = L.nx - 1, hy = L.ny - 1, hz = L.nz - 1;
-    const bool bnd = (i == 0 || i == hx || j == 0 || j == hy || (DIM == 3 && (k == 0 || k == hz)));
-    const long long st[3] = {1, L.px, L.pl};
-    const int idx[3] = {i, j, k};
-    const int top[3] = {hx, hy, hz};
-    if (bnd) {
-        T uc[DIM], g[DIM][DIM], s[DIM][DIM];
-#pragma unroll
-        for (int p = 0; p < DIM; ++p) {
-            const T* up = u + p * L.cs + o;
-            uc[p] = up[0];
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                if (idx[d] == 0) g[p][d] = (up[st[d]] - up[0]) * iv.ih[d];
-                else if (idx[d] == top[d]) g[p][d] = (up[0] - up[-st[d]]) * iv.ih[d];
-                else g[p][d] = (up[st[d]] - up[-st[d]]) * iv.ih2[d];
-            }
-        }
-        stress<T, K, DIM>(A.rec(o), g, s);
-        bc_row<T, DIM>(bc, face_of<DIM>(i, j, k, hx, hy, hz), uc, g, s, f);
-        return;
-    }
+// Interior row of A (Elastic.cpp:151-260): f = C : grad grad u + sum_d (d_d C :
+// grad u) e_d with central differences, on a stencil context S that provides
+//   S.u(p, dx, dy, dz)  component p of u at the neighbour (dx, dy, dz),
+//   S.rec(dx, dy, dz)   the stiffness record there (centre and +-e_d only),
+//   S.same(d)           whether the records at +e_d and -e_d are bitwise equal
+// (their central difference is then exactly zero and is skipped).  Every
+// context evaluates the same expressions in the same order, so a node gets
+// the same bits whether u comes from global memory or from an LDS tile.
+template <typename T, int K, int DIM, typename Ctx>
+__device__ __forceinline__ void interior_apply(const Ctx& S, const Inv<T>& iv, bool uniform, T f[DIM]) {
     T g[DIM][DIM], gg[DIM][DIM][DIM];  // gg[p][a][b] = u_{p,ab}
 #pragma unroll
     for (int p = 0; p < DIM; ++p) {
-        const T* up = u + p * L.cs + o;
-        const T c0 = up[0];
+        const T c0 = S.u(p, 0, 0, 0);
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
-            const T vp = up[st[d]], vm = up[-st[d]];
+            const T vp = S.u(p, d == 0, d == 1, d == 2), vm = S.u(p, -(d == 0), -(d == 1), -(d == 2));
             g[p][d] = (vp - vm) * iv.ih2[d];
             gg[p][d][d] = ((vp - c0) + (vm - c0)) * iv.ihh[d];
         }
@@ -307,14 +289,15 @@ __device__ __forceinline__ void node_apply_acc(const Lay& L, const Acc& A, const
         for (int a = 0; a < DIM; ++a)
 #pragma unroll
             for (int b = a + 1; b < DIM; ++b) {
-                const long long sa = st[a], sb = st[b];
-                const T pp = up[sa + sb], pm = up[sa - sb], mp = up[-sa + sb], mm = up[-sa - sb];
+                const int ax = a == 0, ay = a == 1, az = a == 2, bx = b == 0, by = b == 1, bz = b == 2;
+                const T pp = S.u(p, ax + bx, ay + by, az + bz), pm = S.u(p, ax - bx, ay - by, az - bz);
+                const T mp = S.u(p, -ax + bx, -ay + by, -az + bz), mm = S.u(p, -ax - bx, -ay - by, -az - bz);
                 const T m = ((pp - pm) - (mp - mm)) * iv.imix[a + b - 1];
                 gg[p][a][b] = m;
                 gg[p][b][a] = m;
             }
     }
-    const auto c = A.rec(o);
+    const auto c = S.rec(0, 0, 0);
     if (K == K_ISO) {
         const T lam = c[0], mu = c[1];
 #pragma unroll
@@ -335,9 +318,9 @@ __device__ __forceinline__ void node_apply_acc(const Lay& L, const Acc& A, const
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
                 dl[d] = dm[d] = T(0);
-                if (!A.same(o + st[d], o - st[d])) {
-                    const auto cp = A.rec(o + st[d]);
-                    const auto cm = A.rec(o - st[d]);
+                if (!S.same(d)) {
+                    const auto cp = S.rec(d == 0, d == 1, d == 2);
+                    const auto cm = S.rec(-(d == 0), -(d == 1), -(d == 2));
                     dl[d] = (cp[0] - cm[0]) * iv.ih2[d];
                     dm[d] = (cp[1] - cm[1]) * iv.ih2[d];
                 }
@@ -373,9 +356,9 @@ __device__ __forceinline__ void node_apply_acc(const Lay& L, const Acc& A, const
                 T add = 0;
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
-                    if (A.same(o + st[d], o - st[d])) continue;
-                    const auto cp = A.rec(o + st[d]);
-                    const auto cm = A.rec(o - st[d]);
+                    if (S.same(d)) continue;
+                    const auto cp = S.rec(d == 0, d == 1, d == 2);
+                    const auto cm = S.rec(-(d == 0), -(d == 1), -(d == 2));
                     T acc = 0;
 #pragma unroll
                     for (int kk = 0; kk < DIM; ++kk)
@@ -403,14 +386,60 @@ __device__ __forceinline__ void node_apply_acc(const Lay& L, const Acc& A, const
                 T add = 0;
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
-                    if (A.same(o + st[d], o - st[d])) continue;
+                    if (S.same(d)) continue;
                     const int q = a * DIM + d;
-                    add += (A.rec(o + st[d])[q] - A.rec(o - st[d])[q]) * iv.ih2[d] * g[a][d];
+                    add += (S.rec(d == 0, d == 1, d == 2)[q] - S.rec(-(d == 0), -(d == 1), -(d == 2))[q]) * iv.ih2[d] *
+                           g[a][d];
                 }
                 f[a] += add;
             }
         }
     }
+}
+
+// Stencil context on global memory: u in the level layout, stiffness through an accessor.
+template <typename T, typename Acc>
+struct GlobalCtx {
+    const T* __restrict__ up;
+    long long cs, o, sx, sy, sz;
+    const Acc& A;
+    __device__ __forceinline__ long long at(int dx, int dy, int dz) const { return o + dx * sx + dy * sy + dz * sz; }
+    __device__ __forceinline__ T u(int p, int dx, int dy, int dz) const { return up[p * cs + at(dx, dy, dz)]; }
+    __device__ __forceinline__ auto rec(int dx, int dy, int dz) const { return A.rec(at(dx, dy, dz)); }
+    __device__ __forceinline__ bool same(int d) const {
+        return A.same(at(d == 0, d == 1, d == 2), at(-(d == 0), -(d == 1), -(d == 2)));
+    }
+};
+
+// f = A u at node (i,j,k).  C points at plane 0 of the stiffness, u at comp 0.
+template <typename T, int K, int DIM, typename Acc>
+__device__ __forceinline__ void node_apply_acc(const Lay& L, const Acc& A, const T* __restrict__ u, int i, int j,
+                                               int k, const BCTab& bc, const Inv<T>& iv, bool uniform, T f[DIM]) {
+    const long long o = (long long)i + L.px * j + L.pl * k;
+    const int hx = L.nx - 1, hy = L.ny - 1, hz = L.nz - 1;
+    const bool bnd = (i == 0 || i == hx || j == 0 || j == hy || (DIM == 3 && (k == 0 || k == hz)));
+    const long long st[3] = {1, L.px, L.pl};
+    const int idx[3] = {i, j, k};
+    const int top[3] = {hx, hy, hz};
+    if (bnd) {
+        T uc[DIM], g[DIM][DIM], s[DIM][DIM];
+#pragma unroll
+        for (int p = 0; p < DIM; ++p) {
+            const T* up = u + p * L.cs + o;
+            uc[p] = up[0];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                if (idx[d] == 0) g[p][d] = (up[st[d]] - up[0]) * iv.ih[d];
+                else if (idx[d] == top[d]) g[p][d] = (up[0] - up[-st[d]]) * iv.ih[d];
+                else g[p][d] = (up[st[d]] - up[-st[d]]) * iv.ih2[d];
+            }
+        }
+        stress<T, K, DIM>(A.rec(o), g, s);
+        bc_row<T, DIM>(bc, face_of<DIM>(i, j, k, hx, hy, hz), uc, g, s, f);
+        return;
+    }
+    const GlobalCtx<T, Acc> S{u, L.cs, o, st[0], st[1], st[2], A};
+    interior_apply<T, K, DIM>(S, iv, uniform, f);
 }
 
 template <typename T, int K, int DIM>

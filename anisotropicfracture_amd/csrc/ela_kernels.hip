@@ -245,6 +245,19 @@ __global__ __launch_bounds__(256) void k_axpy(Lay L, int nc, const TX* __restric
     }
 }
 
+// Operator<Node>::normalize (Operator.cpp:164-195): the reference's lambda
+// divides x(i,j,k) -- component 0 -- by diag(i,j,k) -- component 0 -- once
+// per component of the operator; the other components are left as they are.
+template <typename T>
+__global__ __launch_bounds__(256) void k_normalize(Lay L, int ncomp, const T* __restrict__ dg, T* __restrict__ x) {
+    int i, j, k;
+    if (!node_index(L, i, j, k)) return;
+    const long long o = (long long)i + L.px * j + L.pl * k;
+    T v = x[o];
+    for (int n = 0; n < ncomp; ++n) v /= dg[o];
+    x[o] = v;
+}
+
 template <typename TX, typename TY>
 __global__ __launch_bounds__(256) void k_convert(long long n, const TX* __restrict__ x, TY* __restrict__ y) {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -617,6 +630,13 @@ template <typename TX, typename TY>
 void launch_axpy(const Lay& L, int nc, const TX* x, TY* y, const double* ascale, double a, int assign, hipStream_t s) {
     hipLaunchKernelGGL((k_axpy<TX, TY>), node_grid(L), dim3(BX, BY), 0, s, L, nc, x, y, ascale, a, assign);
 }
+
+template <typename T>
+void launch_normalize(const Lay& L, int ncomp, const T* dg, T* x, hipStream_t s) {
+    hipLaunchKernelGGL((k_normalize<T>), node_grid(L), dim3(BX, BY), 0, s, L, ncomp, dg, x);
+}
+template void launch_normalize<double>(const Lay&, int, const double*, double*, hipStream_t);
+template void launch_normalize<float>(const Lay&, int, const float*, float*, hipStream_t);
 
 template <typename TX, typename TY>
 void launch_convert(long long n, const TX* x, TY* y, hipStream_t s) {

@@ -994,6 +994,23 @@ int ela_fapply(ela_hier* h, int mglev, int prec, const void* u, void* f, void* s
     return ELA_OK;
 }
 
+int ela_normalize(ela_hier* h, int mglev, int prec, void* x, void* stream) {
+    if (int rc = check_level(h, mglev)) return rc;
+    if (int rc = check_prec(h, prec)) return rc;
+    if (!h->prepared) return fail(ELA_ERR_STATE, "Operator::Diagonal() must be called before using normalize");
+    if (!x) return fail(ELA_ERR_BADARG, "null field");
+    Level& v = h->lev[mglev];
+    if (prec == ELA_F64) {
+        if (!v.d64) return fail(ELA_ERR_STATE, "no FP64 diagonal on this hierarchy");
+        launch_normalize<double>(v.L, h->dim, v.d64, (double*)x, strm(h, stream));
+    } else {
+        if (!v.d32) return fail(ELA_ERR_STATE, "no FP32 diagonal on this hierarchy");
+        launch_normalize<float>(v.L, h->dim, v.d32, (float*)x, strm(h, stream));
+    }
+    HIPCHK(hipGetLastError());
+    return ELA_OK;
+}
+
 int ela_diagonal(ela_hier* h, int mglev, int prec, void* diag, void* stream) {
     if (int rc = check_level(h, mglev)) return rc;
     if (!h->prepared) return fail(ELA_ERR_STATE, "call ela_prepare first");

@@ -163,6 +163,13 @@ int ela_fapply(ela_hier* h, int mglev, int prec, const void* u, void* f, void* s
 /* Diagonal (src/Operator/Elastic.cpp:264-352) into a caller buffer. */
 int ela_diagonal(ela_hier* h, int mglev, int prec, void* diag, void* stream);
 
+/* normalize (src/Operator/Operator.cpp:164-195), as the reference computes it:
+ * its per-component lambda indexes neither x nor diag by component, so
+ * component 0 of x is divided by component 0 of the diagonal once per
+ * operator component (dim times) and the other components are unchanged.
+ * Requires ela_prepare (the reference aborts unless Diagonal was computed). */
+int ela_normalize(ela_hier* h, int mglev, int prec, void* x, void* stream);
+
 /* Fsmooth (src/Operator/Operator.cpp:90-162): nsweeps weighted-Jacobi sweeps
  * x <- x + omega (b - A x) / diag (the reference uses omega = 2/3, 2 sweeps). */
 int ela_fsmooth(ela_hier* h, int mglev, int prec, void* x, const void* b, int nsweeps, double omega, void* stream);
@@ -378,6 +385,14 @@ typedef struct {
  * memory.  Stops when all requests are done or after max_seconds (> 0). */
 int ela_serve(ela_batch* b, int nreq, const ela_request* req, double tol_rel, double tol_abs, int max_cycles,
               int max_active, double max_seconds, ela_result* out);
+
+/* Kernel options of a batch (set while no member is queued or active):
+ * ELA_BATCH_MARCH  1 (default): the finest level of each member (grain form)
+ *                  runs the LDS-tiled z-marching stencil kernels; 0: the
+ *                  one-thread-per-node kernels (the same arithmetic).
+ * ELA_BATCH_KZ     planes marched per tile (default 16). */
+enum { ELA_BATCH_MARCH = 0, ELA_BATCH_KZ = 1 };
+int ela_batch_set_option(ela_batch* b, int option, int value);
 
 /* HIP-event timing of every smoother launch of later steps (on != 0), and
  * the totals since: milliseconds, nodes swept, launches. */

@@ -720,6 +720,14 @@ void or_assemble(const or_grid* og, int kind, const double* C, const int* bctype
     }
 }
 
+// normalize (Operator.cpp:164-195): the ParallelFor lambda indexes neither x
+// nor diag by component, so x(i,j,k,0) /= diag(i,j,k,0) once per component.
+void or_normalize(const or_grid* og, const double* diag, double* x) {
+    G g(og);
+    for (long q = 0; q < g.nodes; ++q)
+        for (int n = 0; n < g.dim; ++n) x[q] /= diag[q];
+}
+
 // reflux (Operator.cpp:728-839): every coarse node of the coarsened fine box
 // takes the full-weighting restriction of the fine residual (the branches at
 // the coarse DOMAIN boundary).  The fine box covers fine-level nodes

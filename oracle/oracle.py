@@ -74,6 +74,7 @@ def lib():
         L.or_mg_solve.argtypes = [P(_Grid), i, vp, vp, vp, vp, d, d, i, i, i, i, i, vp, vp, i]
         L.or_mg_solve.restype = i
         L.or_reflux.argtypes = [P(_Grid), i, vp, vp, vp, vp]
+        L.or_normalize.argtypes = [P(_Grid), vp, vp]
         _lib = L
     return _lib
 
@@ -167,6 +168,14 @@ def restrict(gc, fine):
     crse = gc.zeros(nc)
     lib().or_restrict(ctypes.byref(gc.c()), nc, _p(fine), _p(crse))
     return crse
+
+
+def normalize(g, diag, x):
+    """Operator<Node>::normalize as the reference computes it (component 0 divided by diag
+    component 0, once per component; Operator.cpp:164-195)."""
+    out = _c(x).copy()
+    lib().or_normalize(ctypes.byref(g.c()), _p(_c(diag)), _p(out))
+    return out
 
 
 def reflux(gc, crse_res, fine_origin, fine_res):

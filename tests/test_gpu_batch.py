@@ -243,3 +243,25 @@ def test_serving_loop_results_match_alone_solves():
         assert np.array_equal(r.download(E.RVE.U), sols[i]), i
         r.close()
     b.close()
+
+
+@pytest.mark.parametrize("mode", [E.SOLVE_MIXED, E.SOLVE_FP64])
+def test_marching_kernels_match_the_node_kernels(mode):
+    """The LDS z-marching kernels of the finest levels and the one-thread-per-node kernels evaluate the
+    same expressions; solves agree to rounding (member sizes with partial tiles in x, y and z)."""
+    probs = [problem(61, 40, 12), problem(62, 24, 9, crack=True), problem(63, 32, 20)]
+    sols, cyc = {}, {}
+    for march in (0, 1):
+        b = E.Batch(mode)
+        b.set_option(E.Batch.MARCH, march)
+        b.set_option(E.Batch.KZ, 16)
+        rs = [make(b, p) for p in probs]
+        b.prepare(rs)
+        sts = b.solve(rs, tol_rel=1e-10, max_cycles=80)
+        sols[march] = [r.download(E.RVE.U) for r in rs]
+        cyc[march] = [s["iters"] for s in sts]
+        b.close()
+    assert cyc[0] == cyc[1]
+    for a, c in zip(sols[0], sols[1]):
+        print("bitwise identical:", np.array_equal(a, c))
+        assert np.abs(a - c).max() <= 1e-13 * np.abs(a).max()

@@ -414,3 +414,15 @@ def test_reflux_reproduces_affine_fields_and_ghost_zeros(oracle):
     outside = np.ones_like(ones[0], bool)
     outside[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = False
     assert not ones[:, outside].any()  # nodes outside the coarsened box are untouched
+
+
+def test_normalize_restates_the_reference_indexing(oracle):
+    """normalize (Operator.cpp:164-195) divides component 0 by diagonal component 0 once per component
+    (its lambda indexes neither field by component); a per-component division would differ here."""
+    g = oracle.Grid((4, 5, 3))
+    rng = np.random.default_rng(2)
+    d = 1.0 + rng.random(g.shape(3))
+    x = rng.standard_normal(g.shape(3))
+    out = oracle.normalize(g, d, x)
+    assert np.allclose(out[0], x[0] / d[0] ** 3, rtol=1e-15, atol=0)
+    assert np.array_equal(out[1:], x[1:])
