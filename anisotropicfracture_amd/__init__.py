@@ -32,6 +32,7 @@ DISP, TRAC, PERIODIC, NEUMANN = 0, 1, 2, 3
 F64, F32 = 0, 1
 SOLVE_FP64, SOLVE_MIXED, ALLOC_BOTH = 0, 1, 2
 CYCLE_F, CYCLE_V = 0, 1
+OPT_MARCH, OPT_KZ = 0, 1
 
 ERRORS = {1: "BADARG", 2: "OOM", 3: "HIP", 4: "NONCONVERGED", 5: "NAN", 6: "STATE"}
 
@@ -118,6 +119,7 @@ SIGNATURES = {
     "ela_set_timing": ([_P, _I], _I),
     "ela_set_krylov": ([_P, _I], _I),
     "ela_set_stiffness_mode": ([_P, _I], _I),
+    "ela_set_option": ([_P, _I, _I], _I),
     "ela_kernel_times": ([_P, ctypes.POINTER(_D), ctypes.POINTER(_I), ctypes.POINTER(_D), ctypes.POINTER(_I)], _I),
     "ela_time_fapply": ([_P, _I, _I, ctypes.POINTER(_D)], _I),
     "ela_probe_copy": ([_P, _SZ, _I, ctypes.POINTER(_D)], _I),
@@ -327,6 +329,11 @@ class Elastic:
     def SetStiffnessMode(self, mode):
         """0: per-node stiffness planes; 1: per-node grain id + per-grain records (Voronoi models)."""
         _check(lib().ela_set_stiffness_mode(self.ptr, int(mode)))
+
+    def set_option(self, option, value):
+        """ela_set_option: OPT_MARCH (LDS z-marching kernels for grain-form stiffness, default on) or
+        OPT_KZ (planes per marching tile, default 16)."""
+        _check(lib().ela_set_option(self.ptr, option, int(value)))
 
     def SetKrylov(self, restart):
         """Flexible GMRES(restart) around the MG cycle (0: plain MG iteration)."""
@@ -616,8 +623,8 @@ class Batch:
     MARCH, KZ = 0, 1
 
     def set_option(self, option, value):
-        """ela_batch_set_option: Batch.MARCH (LDS z-marching kernels on the finest levels, default on) or
-        Batch.KZ (planes per marching tile, default 16)."""
+        """ela_batch_set_option: OPT_MARCH (LDS z-marching kernels on the grain-form levels, default on)
+        or OPT_KZ (planes per marching tile, default 16)."""
         _check(lib().ela_batch_set_option(self.ptr, option, int(value)))
 
     def serve(self, requests, tol_rel=1e-10, tol_abs=0.0, max_cycles=100, max_active=256, max_seconds=0.0,

@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "../../include/ela.h"
+#include "ela_grain.hpp"
 #include "ela_internal.hpp"
 #include "ela_kernels.hpp"
 
@@ -42,7 +43,7 @@ namespace {
 
 constexpr int TB = 256;                // nodes per block
 constexpr int kMaxBatchDense = 1536;   // unknowns of a coarsest level (n_c <= 7)
-constexpr int kNC = 21;                // cubic stiffness: major/minor symmetric storage
+constexpr int kNC = kGrainNC;          // cubic stiffness: major/minor symmetric storage
 enum { F_RES = 0, F_RC, F_COR, F_TMP, F_HOLD, F_N };
 
 // One box of one launch.  Pointers of the precision that a kernel does not use may be null.
@@ -383,217 +384,55 @@ __global__ __launch_bounds__(TB) void kb_cvt(const BoxDesc* __restrict__ D, cons
 }
 
 // ------------------------------------------------------------------ marching kernels (grain form)
-// The interior of a grain-form box is cut into MTX x MTY tiles of (i, j) that
-// march up kz planes in k.  A ring of three planes of u (tile + one-node
-// halo), of the grain ids and of the degradation factor lives in LDS, with the
-// member's grain table; each plane of u is read from memory once per tile
-// column instead of once per neighbouring row of every wavefront.  The
-// stencil is interior_apply on an LDS context, so every node gets the same
-// bits as from the global-memory kernels.  Boundary nodes (the box's shell)
-// are done by a separate kernel with the global-memory boundary rows.
-constexpr int MTX = 32, MTY = 8, MRW = MTY + 2, MCW = MTX + 2, MPL = MRW * MCW;
-constexpr int kMarchMaxGrains = 256;
-
-template <typename T>
-struct LdsCtx {
-    const T* su;                // [3 slots][3 comps][MRW][MCW]
-    const unsigned short* sg;   // [3 slots][MRW][MCW]
-    const T* ss;                // [3 slots][MRW][MCW] or null
-    const T* stab;              // grain table (kNC per grain)
-    int sl[3];                  // ring slots of planes k-1, k, k+1
-    int r, c;                   // this node's row / column in the tile arrays
-    __device__ __forceinline__ int at(int dx, int dy, int dz) const {
-        return (sl[dz + 1] * MRW + r + dy) * MCW + c + dx;
-    }
-    __device__ __forceinline__ T u(int p, int dx, int dy, int dz) const {
-        return su[((sl[dz + 1] * 3 + p) * MRW + r + dy) * MCW + c + dx];
-    }
-    __device__ __forceinline__ ScaledRec<T> rec(int dx, int dy, int dz) const {
-        const int q = at(dx, dy, dz);
-        return ScaledRec<T>{stab + (int)sg[q] * kNC, ss ? ss[q] : T(1)};
-    }
-    __device__ __forceinline__ bool same(int d) const {
-        const int qp = at(d == 0, d == 1, d == 2), qm = at(-(d == 0), -(d == 1), -(d == 2));
-        return sg[qp] == sg[qm] && (!ss || ss[qp] == ss[qm]);
-    }
-};
-
-// Operands of a marching / shell launch.  MODE 0: Jacobi sweep f[out] = f[in]
-// + omega (f[rhs] - A f[in]) / D; MODE 1: f[out] = f[rhs] - A f[in]; MODE 2:
-// FP64 solution residual r = b64 - A u64 into f[out] (precision TO) scaled by
-// the member's scale, with block maxima of |r| in partials.
+// Operands of a marching / shell launch over a list: field indices (ela_grain.hpp
+// MODE 0/1), or the member's u64 / b64 / f[out] / scale (MODE 2), and the
+// partials of MODE 2 (marching blocks first, then the shell blocks).
 struct MOps {
     int rhs, in, out;
     double omega;
     double* partials;
-    int mtot;  // marching blocks of the launch (shell partials follow them)
+    int mtot;  // marching blocks of the launch
 };
 
-template <typename T, int MODE, typename TO>
-__device__ __forceinline__ double m_epilogue(const BoxDesc& d, const MOps& op, long long o, const T xc[3],
-                                             const T f[3]) {
-    const long long cs = d.L.cs;
-    double m = 0.0;
-#pragma unroll
-    for (int p = 0; p < 3; ++p) {
-        const long long q = p * cs + o;
-        if (MODE == 0) {
-            const T* b = (const T*)d.f[op.rhs];
-            ((T*)d.f[op.out])[q] = xc[p] + (T)op.omega * (b[q] - f[p]) / P<T>::dg(d)[q];
-        } else if (MODE == 1) {
-            ((T*)d.f[op.out])[q] = ((const T*)d.f[op.rhs])[q] - f[p];
-        } else {
-            const double v = d.b64[q] - (double)f[p];
-            ((TO*)d.f[op.out])[q] = (TO)(v * *d.scale);
-            const double a = fabs(v);
-            m = (a > m || a != a) ? a : m;
-        }
-    }
-    return m;
+template <typename T, int MODE>
+__device__ __forceinline__ GrainBox<T> grain_of(const BoxDesc& d, const MOps& op) {
+    GrainBox<T> g;
+    g.L = d.L;
+    g.iv = P<T>::inv(d);
+    g.bc = &d.bc;
+    g.uniform = d.uniform;
+    g.gid = d.gid;
+    g.tab = P<T>::tab(d);
+    g.sc = P<T>::sc(d);
+    g.ngr = d.ngr;
+    g.ntx = d.ntx;
+    g.nty = d.nty;
+    g.kz = d.kz;
+    g.in = MODE == 2 ? (const T*)d.u64 : (const T*)d.f[op.in];
+    g.rhs = MODE == 2 ? (const T*)d.b64 : (const T*)d.f[op.rhs];
+    g.out = d.f[op.out];
+    g.out2 = nullptr;
+    g.dg = P<T>::dg(d);
+    g.scale = d.scale;
+    g.omega = (T)op.omega;
+    return g;
 }
 
 template <typename T, int MODE, typename TO>
 __global__ __launch_bounds__(MTX* MTY) void km_grain(const BoxDesc* __restrict__ D, const int* __restrict__ bmap,
                                                      MOps op) {
-    static_assert(MODE != 2 || sizeof(T) == 8, "the solution residual is FP64");
-    __shared__ T su[3 * 3 * MPL];
-    __shared__ unsigned short sg[3 * MPL];
-    __shared__ T ss[3 * MPL];
+    __shared__ MarchLds<T> sh;
     extern __shared__ __align__(16) unsigned char dyn_lds[];
-    T* stab = (T*)dyn_lds;
     const BoxDesc& d = D[bmap[blockIdx.x]];
-    const Lay L = d.L;
-    const int tid = threadIdx.x;
-    const int t = (int)blockIdx.x - d.mt0;
-    const int bx = t % d.ntx, by = (t / d.ntx) % d.nty, bz = t / (d.ntx * d.nty);
-    const int i0 = 1 + bx * MTX, j0 = 1 + by * MTY;  // first interior node of the tile
-    const int tx = tid % MTX, ty = tid / MTX;
-    const int i = i0 + tx, j = j0 + ty;
-    const bool active = i <= L.nx - 2 && j <= L.ny - 2;
-    const int kb = 1 + bz * d.kz, ke = min(kb + d.kz, L.nz - 1);
-    const T* u = MODE == 2 ? (const T*)d.u64 : (const T*)d.f[op.in];
-    const T* tab = P<T>::tab(d);
-    const T* sc = P<T>::sc(d);
-    const unsigned short* gid = d.gid;
-    for (int q = tid; q < d.ngr * kNC; q += MTX * MTY) stab[q] = tab[q];
-
-    // plane loads: u (3 comps), grain ids, factors of the tile + halo at plane kk
-    constexpr int PU = (3 * MPL + MTX * MTY - 1) / (MTX * MTY), PG = (MPL + MTX * MTY - 1) / (MTX * MTY);
-    T ru[PU], rs[PG];
-    unsigned short rg[PG];
-    auto load = [&](int kk) {
-#pragma unroll
-        for (int e = 0; e < PU; ++e) {
-            const int q = tid + e * MTX * MTY;
-            T v = 0;
-            if (q < 3 * MPL) {
-                const int p = q / MPL, rem = q - p * MPL, rr = rem / MCW, cc = rem - rr * MCW;
-                const int ii = i0 - 1 + cc, jj = j0 - 1 + rr;
-                if (ii < L.nx && jj < L.ny) v = u[p * L.cs + ii + L.px * jj + L.pl * kk];
-            }
-            ru[e] = v;
-        }
-#pragma unroll
-        for (int e = 0; e < PG; ++e) {
-            const int q = tid + e * MTX * MTY;
-            unsigned short gv = 0;
-            T sv = 1;
-            if (q < MPL) {
-                const int rr = q / MCW, cc = q - rr * MCW;
-                const int ii = i0 - 1 + cc, jj = j0 - 1 + rr;
-                if (ii < L.nx && jj < L.ny) {
-                    const long long o = ii + L.px * jj + L.pl * kk;
-                    gv = gid[o];
-                    if (sc) sv = sc[o];
-                }
-            }
-            rg[e] = gv;
-            rs[e] = sv;
-        }
-    };
-    auto store = [&](int slot) {
-#pragma unroll
-        for (int e = 0; e < PU; ++e) {
-            const int q = tid + e * MTX * MTY;
-            if (q < 3 * MPL) su[slot * 3 * MPL + q] = ru[e];
-        }
-#pragma unroll
-        for (int e = 0; e < PG; ++e) {
-            const int q = tid + e * MTX * MTY;
-            if (q < MPL) {
-                sg[slot * MPL + q] = rg[e];
-                ss[slot * MPL + q] = rs[e];
-            }
-        }
-    };
-    load(kb - 1);
-    store((kb - 1) % 3);
-    load(kb);
-    store(kb % 3);
-    load(kb + 1);
-    store((kb + 1) % 3);
-    __syncthreads();
-    double m = 0.0;
-    for (int k = kb; k < ke; ++k) {
-        const bool more = k + 1 < ke;
-        if (more) load(k + 2);
-        if (active) {
-            LdsCtx<T> S{su, sg, sc ? ss : nullptr, stab, {(k - 1) % 3, k % 3, (k + 1) % 3}, ty + 1, tx + 1};
-            T f[3], xc[3];
-            interior_apply<T, K_MM, 3>(S, P<T>::inv(d), d.uniform != 0, f);
-#pragma unroll
-            for (int p = 0; p < 3; ++p) xc[p] = S.u(p, 0, 0, 0);
-            const double mm = m_epilogue<T, MODE, TO>(d, op, (long long)i + L.px * j + L.pl * k, xc, f);
-            m = (mm > m || mm != mm) ? mm : m;
-        }
-        __syncthreads();  // every read of plane k-1's slot is done
-        if (more) store((k + 2) % 3);
-        __syncthreads();
-    }
+    const double m = march_tile<T, MODE, TO>(grain_of<T, MODE>(d, op), (int)blockIdx.x - d.mt0, sh, (T*)dyn_lds);
     if (MODE == 2) block_max(op.partials, m);
 }
 
-// Boundary shell of a grain-form box: both z faces, then the y faces and the
-// x faces between them; boundary rows through the global-memory path.
 template <typename T, int MODE, typename TO>
 __global__ __launch_bounds__(TB) void ks_grain(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, MOps op) {
     const BoxDesc& d = D[bmap[blockIdx.x]];
-    const Lay L = d.L;
-    const int nx = L.nx, ny = L.ny, nz = L.nz;
-    int q = ((int)blockIdx.x - d.st0) * TB + (int)threadIdx.x;
-    const int A = nx * ny, Bf = nx * (nz - 2), Cf = (ny - 2) * (nz - 2);
-    bool in = q < 2 * A + 2 * Bf + 2 * Cf;
-    int i = 0, j = 0, k = 0;
-    if (in) {
-        if (q < 2 * A) {
-            k = q < A ? 0 : nz - 1;
-            const int r = q % A;
-            j = r / nx;
-            i = r - j * nx;
-        } else if ((q -= 2 * A) < 2 * Bf) {
-            j = q < Bf ? 0 : ny - 1;
-            const int r = q % Bf;
-            k = 1 + r / nx;
-            i = r - (k - 1) * nx;
-        } else {
-            q -= 2 * Bf;
-            i = q < Cf ? 0 : nx - 1;
-            const int r = q % Cf;
-            k = 1 + r / (ny - 2);
-            j = 1 + r - (k - 1) * (ny - 2);
-        }
-    }
-    double m = 0.0;
-    if (in) {
-        const T* u = MODE == 2 ? (const T*)d.u64 : (const T*)d.f[op.in];
-        T f[3], xc[3];
-        node_apply_acc<T, K_MM, 3>(L, GrainAcc<T, kNC>{d.gid, P<T>::tab(d), P<T>::sc(d)}, u, i, j, k, d.bc,
-                                   P<T>::inv(d), d.uniform != 0, f);
-        const long long o = (long long)i + L.px * j + L.pl * k;
-#pragma unroll
-        for (int p = 0; p < 3; ++p) xc[p] = u[p * L.cs + o];
-        m = m_epilogue<T, MODE, TO>(d, op, o, xc, f);
-    }
+    const double m =
+        shell_node<T, MODE, TO>(grain_of<T, MODE>(d, op), ((int)blockIdx.x - d.st0) * TB + (int)threadIdx.x);
     if (MODE == 2) block_max(op.partials + op.mtot, m);
 }
 
@@ -1020,13 +859,9 @@ void march_number(ListBuild& lb, int a, int b, int kz, std::vector<int>& bm, std
         BoxDesc& d = lb.descs[x];
         const Lay& L = d.L;
         d.kz = kz;
-        d.ntx = (L.nx - 2 + MTX - 1) / MTX;
-        d.nty = (L.ny - 2 + MTY - 1) / MTY;
-        d.ntz = (L.nz - 2 + kz - 1) / kz;
+        grain_tiles(L, kz, d.ntx, d.nty, d.ntz, d.nsh);
         d.mt0 = mi.nm;
         mi.nm += d.ntx * d.nty * d.ntz;
-        const long long shell = (long long)d.nodes - (long long)(L.nx - 2) * (L.ny - 2) * (L.nz - 2);
-        d.nsh = (int)((shell + TB - 1) / TB);
         d.st0 = mi.ns;
         mi.ns += d.nsh;
         mi.maxng = std::max(mi.maxng, d.ngr);
@@ -1834,9 +1669,9 @@ int ela_rve_status(const ela_rve* r, int* state, int* status, ela_stats* st) {
 int ela_batch_set_option(ela_batch* B, int option, int value) {
     if (!B) return fail(ELA_ERR_BADARG, "null batch");
     if (!B->active.empty() || !B->queued.empty()) return fail(ELA_ERR_STATE, "batch is busy");
-    if (option == ELA_BATCH_MARCH) {
+    if (option == ELA_OPT_MARCH) {
         B->march = value != 0;
-    } else if (option == ELA_BATCH_KZ) {
+    } else if (option == ELA_OPT_KZ) {
         if (value < 1 || value > 1024) return fail(ELA_ERR_BADARG, "planes per marching tile must be 1..1024");
         B->kz = value;
     } else {

@@ -9,6 +9,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "ela_grain.hpp"
+
 #include <algorithm>
 #include <cmath>
 
@@ -736,6 +738,51 @@ void launch_residual_tab(const Lay& L, const TabDesc<T>& t, const T* x, const T*
                        r, rs, scale, norm ? partials : nullptr, bc, iv, uniform, S);
     if (norm) hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partials, (int)(grid.x * grid.y * grid.z), norm);
 }
+
+// Grain-form z-marching stencil on one box (ela_grain.hpp): interior tiles, then the shell.
+namespace {
+template <typename T, int MODE, typename TO>
+__global__ __launch_bounds__(MTX* MTY) void k_grain_march(GrainBox<T> g, double* partials) {
+    __shared__ MarchLds<T> sh;
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    const double m = march_tile<T, MODE, TO>(g, (int)blockIdx.x, sh, (T*)dyn_lds);
+    if (MODE == 2 && partials) block_max_store(partials, m);
+}
+
+template <typename T, int MODE, typename TO>
+__global__ __launch_bounds__(256) void k_grain_shell(GrainBox<T> g, double* partials) {
+    const double m = shell_node<T, MODE, TO>(g, (int)blockIdx.x * 256 + (int)threadIdx.x);
+    if (MODE == 2 && partials) block_max_store(partials, m);
+}
+}  // namespace
+
+long long grain_partials(const Lay& L, int kz) {
+    int ntx, nty, ntz, nsh;
+    grain_tiles(L, kz, ntx, nty, ntz, nsh);
+    return (long long)ntx * nty * ntz + nsh;
+}
+
+template <typename T, int MODE, typename TO>
+void launch_grain(GrainBox<T> g, int kz, double* partials, unsigned long long* norm, hipStream_t s) {
+    int ntx, nty, ntz, nsh;
+    grain_tiles(g.L, kz, ntx, nty, ntz, nsh);
+    g.ntx = ntx;
+    g.nty = nty;
+    g.kz = kz;
+    const int nm = ntx * nty * ntz;
+    const size_t lds = (size_t)g.ngr * kGrainNC * sizeof(T);
+    hipLaunchKernelGGL((k_grain_march<T, MODE, TO>), dim3(nm), dim3(MTX * MTY), lds, s, g, partials);
+    hipLaunchKernelGGL((k_grain_shell<T, MODE, TO>), dim3(nsh), dim3(256), 0, s, g, partials ? partials + nm : nullptr);
+    if (MODE == 2 && norm) hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partials, nm + nsh, norm);
+}
+template void launch_grain<float, 0, float>(GrainBox<float>, int, double*, unsigned long long*, hipStream_t);
+template void launch_grain<double, 0, double>(GrainBox<double>, int, double*, unsigned long long*, hipStream_t);
+template void launch_grain<float, 1, float>(GrainBox<float>, int, double*, unsigned long long*, hipStream_t);
+template void launch_grain<double, 1, double>(GrainBox<double>, int, double*, unsigned long long*, hipStream_t);
+template void launch_grain<double, 2, float>(GrainBox<double>, int, double*, unsigned long long*, hipStream_t);
+template void launch_grain<double, 2, double>(GrainBox<double>, int, double*, unsigned long long*, hipStream_t);
+template void launch_grain<float, 3, float>(GrainBox<float>, int, double*, unsigned long long*, hipStream_t);
+template void launch_grain<double, 3, double>(GrainBox<double>, int, double*, unsigned long long*, hipStream_t);
 
 // Per-grain interior diagonal sum_b C_{p b p b} (-2 / h_b^2) (Elastic.cpp:311-321 on an interior node).
 __global__ void k_grain_diag(int ng, const double* __restrict__ tab, double hx, double hy, double hz,

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "ela_device.hpp"
+#include "ela_grain.hpp"
 
 namespace ela {
 
@@ -89,6 +90,13 @@ void launch_residual_tab(const Lay& L, const TabDesc<T>& t, const T* x, const T*
                          const double* scale, double* partials, unsigned long long* norm, const BCTab& bc,
                          const double h[3], int uniform, hipStream_t s);
 void launch_grain_diag(int ng, const double* tab, const double h[3], double* d64, float* d32, hipStream_t s);
+
+// Grain-form z-marching stencil on one box (modes of GrainBox, ela_grain.hpp);
+// MODE 2 writes nm + nsh block maxima into partials (grain_partials(L, kz)
+// doubles) and folds them into *norm when norm is non-null.
+template <typename T, int MODE, typename TO>
+void launch_grain(GrainBox<T> g, int kz, double* partials, unsigned long long* norm, hipStream_t s);
+long long grain_partials(const Lay& L, int kz);
 void launch_degraded_iso(const Lay& L, const double* eta, double lambda0, double mu0, double a, double b, double c,
                          double* C, hipStream_t s);
 

@@ -129,6 +129,11 @@ struct ela_hier {
     float *tab32 = nullptr, *dgt32 = nullptr;
     int ngrains = 0;
     bool has_table = false, use_table = false;
+    // z-marching kernels for the grain-form level 0 (planes per tile), and the
+    // BC table on the device for them
+    bool grain_march = true;
+    int kz = 16;
+    BCTab* dbc = nullptr;
     // flexible GMRES around the MG cycle (restart length; 0 = plain MG iteration)
     int krylov = 0;
     std::vector<double*> V, Z;
@@ -559,16 +564,60 @@ template <>
 TabDesc<float> tab_of<float>(ela_hier* h) { return {h->gid, h->tab32, h->dgt32}; }
 bool table_on(ela_hier* h, const Level& v) { return h->use_table && h->has_table && &v == &h->lev[0]; }
 
+// MODE 2 (FP64 solution residual) exists for FP64 operands only.
+template <typename T, typename TO>
+void grain_resid64(const GrainBox<T>&, int, double*, unsigned long long*, hipStream_t) {}
+template <>
+void grain_resid64<double, double>(const GrainBox<double>& g, int kz, double* p, unsigned long long* n, hipStream_t s) {
+    launch_grain<double, 2, double>(g, kz, p, n, s);
+}
+template <>
+void grain_resid64<double, float>(const GrainBox<double>& g, int kz, double* p, unsigned long long* n, hipStream_t s) {
+    launch_grain<double, 2, float>(g, kz, p, n, s);
+}
+
+// The grain-form level 0 through the z-marching kernels.
+bool grain_on(ela_hier* h, const Level& v) {
+    return table_on(h, v) && h->grain_march && h->ngrains <= kMarchMaxGrains && h->dbc != nullptr;
+}
+
+template <typename T>
+GrainBox<T> grain_box(ela_hier* h, const Level& v) {
+    GrainBox<T> g{};
+    g.L = v.L;
+    g.iv = make_inv<T>(v.h);
+    g.bc = h->dbc;
+    g.uniform = h->uniform;
+    g.gid = h->gid;
+    g.tab = tab_of<T>(h).tab;
+    g.sc = nullptr;
+    g.ngr = h->ngrains;
+    return g;
+}
+
 template <typename T>
 void A_apply(ela_hier* h, Level& v, const T* C, const T* u, T* f, hipStream_t s) {
-    if (table_on(h, v)) launch_fapply_tab<T>(v.L, tab_of<T>(h), u, f, h->bc, v.h, h->uniform, s);
+    if (grain_on(h, v)) {
+        GrainBox<T> g = grain_box<T>(h, v);
+        g.in = u;
+        g.out = f;
+        launch_grain<T, 3, T>(g, h->kz, nullptr, nullptr, s);
+    } else if (table_on(h, v)) launch_fapply_tab<T>(v.L, tab_of<T>(h), u, f, h->bc, v.h, h->uniform, s);
     else if (v.mp.ok) march_fapply<T>(v.mp, v.L, h->kind, C, u, f, h->bc, v.h, h->uniform, s);
     else launch_fapply<T>(v.L, h->kind, C, u, f, h->bc, v.h, h->uniform, s);
 }
 
 template <typename T>
 void A_jacobi(ela_hier* h, Level& v, const T* C, const T* dg, const T* xin, const T* b, T* xout, hipStream_t s) {
-    if (table_on(h, v)) launch_jacobi_tab<T>(v.L, tab_of<T>(h), dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
+    if (grain_on(h, v)) {
+        GrainBox<T> g = grain_box<T>(h, v);
+        g.in = xin;
+        g.rhs = b;
+        g.out = xout;
+        g.dg = dg;
+        g.omega = (T)h->omega;
+        launch_grain<T, 0, T>(g, h->kz, nullptr, nullptr, s);
+    } else if (table_on(h, v)) launch_jacobi_tab<T>(v.L, tab_of<T>(h), dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
     else if (v.mp.ok) march_jacobi<T>(v.mp, v.L, h->kind, C, dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
     else launch_jacobi<T>(v.L, h->kind, C, dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
 }
@@ -576,7 +625,21 @@ void A_jacobi(ela_hier* h, Level& v, const T* C, const T* dg, const T* xin, cons
 template <typename T, typename TO>
 void A_residual(ela_hier* h, Level& v, const T* C, const T* x, const T* b, T* r, TO* rs, const double* scale,
                 unsigned long long* norm, hipStream_t s) {
-    if (table_on(h, v)) launch_residual_tab<T, TO>(v.L, tab_of<T>(h), x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
+    const bool plain = !rs && !norm;
+    if (grain_on(h, v) && (plain || sizeof(T) == 8)) {
+        GrainBox<T> g = grain_box<T>(h, v);
+        g.in = x;
+        g.rhs = b;
+        if (plain) {
+            g.out = r;
+            launch_grain<T, 1, T>(g, h->kz, nullptr, nullptr, s);
+        } else {
+            g.out = rs;
+            g.out2 = (double*)r;
+            g.scale = scale;
+            grain_resid64<T, TO>(g, h->kz, h->partials, norm, s);
+        }
+    } else if (table_on(h, v)) launch_residual_tab<T, TO>(v.L, tab_of<T>(h), x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
     else if (v.mp.ok) march_residual<T, TO>(v.mp, v.L, h->kind, C, x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
     else launch_residual<T, TO>(v.L, h->kind, C, x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
 }
@@ -701,6 +764,8 @@ int ela_hier_create_ex(ela_ctx* ctx, const ela_domain* dom, int model_kind, int 
     if (!rc) rc = halloc_t(h, 4, &h->dnorm);
     long long np = 0;
     for (auto& v : h->lev) np = std::max(np, std::max(residual_partials(v.L), march_partials(v.mp, v.L)));
+    if (h->dim == 3) np = std::max(np, grain_partials(h->lev[0].L, 1));  // every z-marching tile height
+    if (!rc) rc = halloc_t(h, 1, &h->dbc);
     if (!rc) rc = halloc_t(h, (size_t)np, &h->partials);
     if (!rc) rc = halloc_t(h, 4, &h->dscale);
     if (!rc) rc = halloc_t(h, 4, &h->dflag);
@@ -875,6 +940,8 @@ int ela_set_bc(ela_hier* h, const int* types, const double* values) {
             h->bc.t[f][c] = (unsigned char)(c < h->dim ? types[3 * f + c] : 0);
             h->bcval[f][c] = values ? values[3 * f + c] : 0.0;
         }
+    if (h->dbc) HIPCHK(hipMemcpyAsync(h->dbc, &h->bc, sizeof h->bc, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
     h->prepared = false;
     return ELA_OK;
 }
@@ -1201,6 +1268,19 @@ int ela_set_stiffness_mode(ela_hier* h, int mode) {
     if (mode != 0 && mode != 1) return fail(ELA_ERR_BADARG, "mode must be 0 (planes) or 1 (grain table)");
     if (mode == 1 && !h->has_table) return fail(ELA_ERR_STATE, "no grain table: set a Voronoi model first");
     h->use_table = mode == 1;
+    return ELA_OK;
+}
+
+int ela_set_option(ela_hier* h, int option, int value) {
+    if (!h) return fail(ELA_ERR_BADARG, "null hierarchy");
+    if (option == ELA_OPT_MARCH) {
+        h->grain_march = value != 0;
+    } else if (option == ELA_OPT_KZ) {
+        if (value < 1 || value > 1024) return fail(ELA_ERR_BADARG, "planes per marching tile must be 1..1024");
+        h->kz = value;
+    } else {
+        return fail(ELA_ERR_BADARG, "unknown option");
+    }
     return ELA_OK;
 }
 

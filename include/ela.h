@@ -221,6 +221,15 @@ int ela_kernel_times(const ela_hier* h, double* ms_resid64, int* n_resid64, doub
  * MG cycles. */
 int ela_set_krylov(ela_hier* h, int restart);
 
+/* Kernel options (no counterpart in the reference):
+ * ELA_OPT_MARCH  1 (default): grain-form stiffness (a finest level in grain
+ *                form, up to 256 grains) runs the LDS-tiled z-marching stencil
+ *                kernels; 0: the one-thread-per-node kernels.  The arithmetic
+ *                is the same, so are the results, bit for bit.
+ * ELA_OPT_KZ     planes marched per tile (default 16). */
+enum { ELA_OPT_MARCH = 0, ELA_OPT_KZ = 1 };
+int ela_set_option(ela_hier* h, int option, int value);
+
 /* Record HIP events around the level-0 kernels of subsequent solves (on != 0). */
 int ela_set_timing(ela_hier* h, int on);
 
@@ -386,12 +395,8 @@ typedef struct {
 int ela_serve(ela_batch* b, int nreq, const ela_request* req, double tol_rel, double tol_abs, int max_cycles,
               int max_active, double max_seconds, ela_result* out);
 
-/* Kernel options of a batch (set while no member is queued or active):
- * ELA_BATCH_MARCH  1 (default): the finest level of each member (grain form)
- *                  runs the LDS-tiled z-marching stencil kernels; 0: the
- *                  one-thread-per-node kernels (the same arithmetic).
- * ELA_BATCH_KZ     planes marched per tile (default 16). */
-enum { ELA_BATCH_MARCH = 0, ELA_BATCH_KZ = 1 };
+/* Kernel options of a batch (set while no member is queued or active); see
+ * ela_set_option. */
 int ela_batch_set_option(ela_batch* b, int option, int value);
 
 /* HIP-event timing of every smoother launch of later steps (on != 0), and

@@ -269,3 +269,28 @@ def test_grain_table_solve_matches_oracle(oracle, mode, krylov):
     u, st = ela.Linear(op).solve(np.zeros_like(b), b, tol_rel=1e-10, tol_abs=0.0, mode=mode, max_iter=300)
     assert st["converged"] and st["res"] / st["res0"] <= 1e-10
     assert np.abs(u - u_or).max() / np.abs(u_or).max() <= 1e-8
+
+
+@pytest.mark.parametrize("mode", [ela.SOLVE_FP64, ela.SOLVE_MIXED])
+@pytest.mark.parametrize("krylov", [0, 16])
+def test_grain_marching_kernels_give_the_node_kernel_bits(mode, krylov):
+    """Level 0 in grain form through the LDS z-marching kernels and through the one-thread-per-node
+    kernels: the same arithmetic, so the same solve bit for bit (a 40^3 box has partial tiles in x, y, z)."""
+    cfg = K.cfg2(40, grains=14, seed=5)
+    sols, iters = [], []
+    for march in (0, 1):
+        op = ela.Elastic(cfg["n"], MM, mode=ela.ALLOC_BOTH)
+        op.SetModelVoronoiCubic(cfg["seeds"], K.VORONOI_Q, cfg["R"], cfg["C11"], cfg["C12"], cfg["C44"])
+        op.SetBC(cfg["bc_type"], cfg["bc_value"])
+        op.SetStiffnessMode(1)
+        op.SetKrylov(krylov)
+        op.set_option(ela.OPT_MARCH, march)
+        op.prepareForSolve()
+        b = op.bc_init()
+        u, st = ela.Linear(op).solve(np.zeros_like(b), b, tol_rel=1e-10, tol_abs=0.0, mode=mode, max_iter=300)
+        assert st["converged"]
+        sols.append(u)
+        iters.append(st["iters"])
+        op.close()
+    assert iters[0] == iters[1]
+    assert np.array_equal(sols[0], sols[1])
