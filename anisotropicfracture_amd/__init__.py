@@ -150,6 +150,7 @@ SIGNATURES = {
     "ela_batch_set_timing": ([_P, _I], _I),
     "ela_batch_set_option": ([_P, _I, _I], _I),
     "ela_normalize": ([_P, _I, _I, _P, _P], _I),
+    "ela_postprocess": ([_P, _I, _I, _P, _P, _P], _I),
     "ela_reflux": ([ctypes.POINTER(_Layout), _P, ctypes.POINTER(_Layout), _P, _P, _I, _I, _P], _I),
     "ela_average_down": ([ctypes.POINTER(_Layout), _P, ctypes.POINTER(_Layout), _P, _P, _I, _I, _P], _I),
     "ela_serve": ([_P, _I, ctypes.POINTER(Request), _D, _D, _I, _I, _D, ctypes.POINTER(Result)], _I),
@@ -363,6 +364,28 @@ class Elastic:
         out = fd.download()
         fd.free()
         return out
+
+    def _post(self, what, u, voigt):
+        nc = 1 if what == 2 else ((6 if self.dim == 3 else 3) if voigt else self.dim * self.dim)
+        fu = self.field(0, F64, self.dim).upload(u)
+        fo = self.field(0, F64, nc)
+        _check(lib().ela_postprocess(self.ptr, what, int(bool(voigt)), fu.ptr, fo.ptr, None))
+        out = fo.download()
+        fu.free()
+        fo.free()
+        return out
+
+    def Strain(self, u, voigt=False):
+        """Elastic<T>::Strain (Elastic.cpp:399-447): sym(grad u) per node."""
+        return self._post(0, u, voigt)
+
+    def Stress(self, u, voigt=False):
+        """Elastic<T>::Stress (Elastic.cpp:451-499): C : grad u per node."""
+        return self._post(1, u, voigt)
+
+    def Energy(self, u):
+        """Elastic<T>::Energy (Elastic.cpp:502-549): W(grad u) + sigma:eps/2 per node (the reference's sum)."""
+        return self._post(2, u, False)[0]
 
     def normalize(self, x, mglev=0, prec=F64):
         """Operator<Node>::normalize (Operator.cpp:164-195) as the reference computes it: component 0

@@ -411,6 +411,28 @@ struct GlobalCtx {
     }
 };
 
+// grad u at node (i,j,k), g[p][d] = du_p/dx_d: central differences inside,
+// one-sided on domain walls (Numeric::GetStencil with the domain box,
+// src/Numeric/Stencil.H:16-102).
+template <typename T, int DIM>
+__device__ __forceinline__ void node_grad(const Lay& L, const T* __restrict__ u, int i, int j, int k,
+                                          const Inv<T>& iv, T g[DIM][DIM]) {
+    const long long o = (long long)i + L.px * j + L.pl * k;
+    const long long st[3] = {1, L.px, L.pl};
+    const int idx[3] = {i, j, k};
+    const int top[3] = {L.nx - 1, L.ny - 1, L.nz - 1};
+#pragma unroll
+    for (int p = 0; p < DIM; ++p) {
+        const T* up = u + p * L.cs + o;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            if (idx[d] == 0) g[p][d] = (up[st[d]] - up[0]) * iv.ih[d];
+            else if (idx[d] == top[d]) g[p][d] = (up[0] - up[-st[d]]) * iv.ih[d];
+            else g[p][d] = (up[st[d]] - up[-st[d]]) * iv.ih2[d];
+        }
+    }
+}
+
 // f = A u at node (i,j,k).  C points at plane 0 of the stiffness, u at comp 0.
 template <typename T, int K, int DIM, typename Acc>
 __device__ __forceinline__ void node_apply_acc(const Lay& L, const Acc& A, const T* __restrict__ u, int i, int j,
@@ -423,17 +445,9 @@ __device__ __forceinline__ void node_apply_acc(const Lay& L, const Acc& A, const
     const int top[3] = {hx, hy, hz};
     if (bnd) {
         T uc[DIM], g[DIM][DIM], s[DIM][DIM];
+        node_grad<T, DIM>(L, u, i, j, k, iv, g);
 #pragma unroll
-        for (int p = 0; p < DIM; ++p) {
-            const T* up = u + p * L.cs + o;
-            uc[p] = up[0];
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                if (idx[d] == 0) g[p][d] = (up[st[d]] - up[0]) * iv.ih[d];
-                else if (idx[d] == top[d]) g[p][d] = (up[0] - up[-st[d]]) * iv.ih[d];
-                else g[p][d] = (up[st[d]] - up[-st[d]]) * iv.ih2[d];
-            }
-        }
+        for (int p = 0; p < DIM; ++p) uc[p] = u[p * L.cs + o];
         stress<T, K, DIM>(A.rec(o), g, s);
         bc_row<T, DIM>(bc, face_of<DIM>(i, j, k, hx, hy, hz), uc, g, s, f);
         return;

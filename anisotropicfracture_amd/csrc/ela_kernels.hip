@@ -739,6 +739,71 @@ void launch_residual_tab(const Lay& L, const TabDesc<T>& t, const T* x, const T*
     if (norm) hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partials, (int)(grid.x * grid.y * grid.z), norm);
 }
 
+// Post-processing (Elastic.cpp:399-616): Strain = sym(grad u), Stress =
+// C : grad u (DW of the linear models), Energy = W(grad u) + sigma : eps / 2
+// with W = tr(grad u^T sigma) / 2, as the reference adds them (which is
+// sigma : eps for the symmetric sigma of these models).  what: 0 strain,
+// 1 stress, 2 energy; voigt: (00,11,22,12,20,01) in 3D, (00,11,01) in 2D,
+// otherwise all DIM*DIM entries row-major.
+namespace {
+template <int K, int DIM>
+__global__ __launch_bounds__(256) void k_post(Lay L, const double* __restrict__ C, const double* __restrict__ u,
+                                              double* __restrict__ out, Inv<double> iv, int what, int voigt) {
+    int i, j, k;
+    if (!node_index(L, i, j, k)) return;
+    const long long o = (long long)i + L.px * j + L.pl * k;
+    double g[DIM][DIM], e[DIM][DIM], s[DIM][DIM] = {};
+    node_grad<double, DIM>(L, u, i, j, k, iv, g);
+#pragma unroll
+    for (int a = 0; a < DIM; ++a)
+#pragma unroll
+        for (int b = 0; b < DIM; ++b) e[a][b] = 0.5 * (g[a][b] + g[b][a]);
+    if (what != 0) stress<double, K, DIM>(PlaneRec<double>{C + o, L.cs}, g, s);
+    if (what == 2) {
+        double w = 0.0, se = 0.0;
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+#pragma unroll
+            for (int b = 0; b < DIM; ++b) w += g[a][b] * s[a][b];
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+#pragma unroll
+            for (int b = 0; b < DIM; ++b) se += 0.5 * s[a][b] * e[a][b];
+        out[o] = 0.5 * w + se;
+        return;
+    }
+    double m[DIM][DIM];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a)
+#pragma unroll
+        for (int b = 0; b < DIM; ++b) m[a][b] = what == 0 ? e[a][b] : s[a][b];
+    if (voigt) {
+        if (DIM == 3) {
+            const int vi[6] = {0, 1, 2, 1, 2, 0}, vj[6] = {0, 1, 2, 2, 0, 1};
+#pragma unroll
+            for (int c = 0; c < 6; ++c) out[c * L.cs + o] = m[vi[c]][vj[c]];
+        } else {
+            const int vi[3] = {0, 1, 0}, vj[3] = {0, 1, 1};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) out[c * L.cs + o] = m[vi[c]][vj[c]];
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+#pragma unroll
+            for (int b = 0; b < DIM; ++b) out[(a * DIM + b) * L.cs + o] = m[a][b];
+    }
+}
+}  // namespace
+
+void launch_post(const Lay& L, int kind, const double* C, const double* u, double* out, const double h[3], int what,
+                 int voigt, hipStream_t s) {
+    const Inv<double> iv = make_inv<double>(h);
+    ELA_DISPATCH(kind, L.dim,
+                 hipLaunchKernelGGL((k_post<KK, DD>), node_grid(L), dim3(BX, BY), 0, s, L, C, u, out, iv, what,
+                                    voigt));
+}
+
 // Grain-form z-marching stencil on one box (ela_grain.hpp): interior tiles, then the shell.
 namespace {
 template <typename T, int MODE, typename TO>

@@ -91,6 +91,10 @@ void launch_residual_tab(const Lay& L, const TabDesc<T>& t, const T* x, const T*
                          const double h[3], int uniform, hipStream_t s);
 void launch_grain_diag(int ng, const double* tab, const double h[3], double* d64, float* d32, hipStream_t s);
 
+// Strain / Stress / Energy of u (what 0 / 1 / 2) into out (6 or 9 / 6 or 9 / 1 planes in 3D).
+void launch_post(const Lay& L, int kind, const double* C, const double* u, double* out, const double h[3], int what,
+                 int voigt, hipStream_t s);
+
 // Grain-form z-marching stencil on one box (modes of GrainBox, ela_grain.hpp);
 // MODE 2 writes nm + nsh block maxima into partials (grain_partials(L, kz)
 // doubles) and folds them into *norm when norm is non-null.

@@ -1061,6 +1061,18 @@ int ela_fapply(ela_hier* h, int mglev, int prec, const void* u, void* f, void* s
     return ELA_OK;
 }
 
+int ela_postprocess(ela_hier* h, int what, int voigt, const void* u, void* out, void* stream) {
+    if (!h) return fail(ELA_ERR_BADARG, "null hierarchy");
+    if (!h->model_set) return fail(ELA_ERR_STATE, "Attempting to use operator before calling SetModel");
+    if (what < 0 || what > 2) return fail(ELA_ERR_BADARG, "what: 0 strain, 1 stress, 2 energy");
+    if (!u || !out) return fail(ELA_ERR_BADARG, "null field");
+    Level& v = h->lev[0];
+    if (!v.C64) return fail(ELA_ERR_STATE, "no FP64 stiffness on level 0");
+    launch_post(v.L, h->kind, v.C64, (const double*)u, (double*)out, v.h, what, voigt != 0, strm(h, stream));
+    HIPCHK(hipGetLastError());
+    return ELA_OK;
+}
+
 int ela_normalize(ela_hier* h, int mglev, int prec, void* x, void* stream) {
     if (int rc = check_level(h, mglev)) return rc;
     if (int rc = check_prec(h, prec)) return rc;

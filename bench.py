@@ -271,20 +271,22 @@ def main():
     nc = 21
     bytes_fa64 = 8 * (3 + 3 + nc) * nodes
     bytes_fa32 = 4 * (3 + 3 + nc) * nodes
-    # dominant kernel of the mixed solve: level-0 FP32 Jacobi sweep vs level-0 FP64 residual
-    # x in, b, diag, x out, C planes -- or x in, b, x out, 2-byte grain id (interior diagonal per grain)
-    bytes_jac32 = (4 * (4 * 3 + nc) if not args.table else 4 * 9 + 2) * nodes
+    # dominant kernel of the mixed solve: level-0 FP32 Jacobi sweep vs level-0 FP64 residual.
+    # Jacobi bytes per node: x in, b, diagonal, x out (3 comps each) + the stiffness: 21 planes, or a
+    # 2-byte grain id (the grain records sit in LDS / L1, read once per block)
+    bytes_jac32 = (4 * (4 * 3 + nc) if not args.table else 4 * 12 + 2) * nodes
     # x, b, C in FP64 + scaled FP32 rhs out (plain MG) / A z: z, C in, w out (FGMRES)
     cbytes = 8 * nc if not args.table else 2
     bytes_res64 = ((8 * 6 + cbytes + 4 * 3) if not args.krylov else 8 * 6 + cbytes) * nodes
     tot_jac = kt["smooth0_ms"] * kt["smooth0_n"]
     tot_res = kt["resid64_ms"] * kt["resid64_n"]
     if tot_jac >= tot_res:
-        dom = "k_jacobi<float,MM,3,%s> (level 0)" % ("TabAcc" if args.table else "PlaneAcc")
+        dom = ("k_grain_march<float,0> + k_grain_shell<float,0> (level-0 Jacobi sweep, grain form)" if args.table
+               else "k_jacobi<float,MM,3,PlaneAcc> (level-0 Jacobi sweep)")
         dom_ms, dom_bytes = kt["smooth0_ms"], bytes_jac32
     else:
-        acc = "TabAcc" if args.table else "PlaneAcc"
-        dom = ("k_fapply<double,MM,3,%s> (level 0)" if args.krylov else "k_residual<double,float,MM,3,%s> (level 0)") % acc
+        dom = ("k_grain_march<double,%d> + k_grain_shell<double,%d> (level 0, grain form)" % ((3, 3) if args.krylov else (2, 2))
+               if args.table else "k_residual<double,float,MM,3,PlaneAcc> (level 0)")
         dom_ms, dom_bytes = kt["resid64_ms"], bytes_res64
     peak = 8000.0
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
@@ -293,7 +295,7 @@ def main():
     if os.path.exists(tpath):  # HBM bytes per launch from the rocprofv3 --pmc runs, copy-calibrated
         with open(tpath) as fh:
             tr = json.load(fh)
-        ent = tr.get("kernels", {}).get(dom.split(" ")[0])
+        ent = tr.get("kernels", {}).get(dom)
         if ent and tr.get("n") == n:
             traffic = ent["fetch_bytes"] + ent["write_bytes"]
 

@@ -163,6 +163,17 @@ int ela_fapply(ela_hier* h, int mglev, int prec, const void* u, void* f, void* s
 /* Diagonal (src/Operator/Elastic.cpp:264-352) into a caller buffer. */
 int ela_diagonal(ela_hier* h, int mglev, int prec, void* diag, void* stream);
 
+/* Post-processing of a level-0 displacement field u (FP64, level layout):
+ * what = 0 Strain (src/Operator/Elastic.cpp:399-447): sym(grad u);
+ * what = 1 Stress (Elastic.cpp:451-499): C : grad u;
+ * what = 2 Energy (Elastic.cpp:502-549): W(grad u) + sigma : eps / 2 with
+ *          W = tr(grad u^T sigma) / 2, the sum the reference forms (for the
+ *          symmetric sigma of these models it equals sigma : eps).
+ * grad u uses central differences inside and one-sided ones on domain walls.
+ * Output (FP64, level layout): voigt != 0 gives 6 components (00,11,22,12,20,01)
+ * in 3D, 3 (00,11,01) in 2D; otherwise dim*dim row-major; energy has 1. */
+int ela_postprocess(ela_hier* h, int what, int voigt, const void* u, void* out, void* stream);
+
 /* normalize (src/Operator/Operator.cpp:164-195), as the reference computes it:
  * its per-component lambda indexes neither x nor diag by component, so
  * component 0 of x is divided by component 0 of the diagonal once per

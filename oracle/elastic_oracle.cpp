@@ -720,6 +720,52 @@ void or_assemble(const or_grid* og, int kind, const double* C, const int* bctype
     }
 }
 
+// Strain / Stress / Energy (Elastic.cpp:399-616).  grad u with the domain
+// stencils (d1: central inside, one-sided on walls); Strain = (grad u +
+// grad u^T)/2; Stress = C(i,j,k)(grad u) = ddw * grad u; Energy =
+// C.W(grad u) + sum_mn 0.5 sig_mn eps_mn with W = tr(0.5 grad u^T (ddw grad u))
+// (Linear/Cubic.H:60-63, Linear/Isotropic.H:31-34).  Output layout as the
+// reference's: voigt (00,11,22,12,20,01) / (00,11,01), else all dim*dim
+// row-major; energy one plane.
+void or_post(const or_grid* og, int kind, const double* C, const double* u, int what, int voigt, double* out) {
+    G g(og);
+    const int nc = ncomp_of(g.dim, kind), dim = g.dim;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < g.nn[2]; ++k)
+        for (int j = 0; j < g.nn[1]; ++j)
+            for (int i = 0; i < g.nn[0]; ++i) {
+                double gu[3][3] = {}, eps[3][3] = {}, sig[3][3] = {}, c[21];
+                for (int p = 0; p < dim; ++p)
+                    for (int d = 0; d < dim; ++d) gu[p][d] = d1(g, u, p, i, j, k, d);
+                for (int a = 0; a < dim; ++a)
+                    for (int b = 0; b < dim; ++b) eps[a][b] = 0.5 * (gu[a][b] + gu[b][a]);
+                load_c(g, nc, C, i, j, k, c);
+                contract2(dim, kind, c, gu, sig);
+                const long q = g.id(i, j, k);
+                if (what == 2) {
+                    double w = 0.0;  // tr(grad u^T sig)
+                    for (int a = 0; a < dim; ++a)
+                        for (int b = 0; b < dim; ++b) w += gu[b][a] * sig[b][a];
+                    double e = 0.5 * w;
+                    for (int m = 0; m < dim; ++m)
+                        for (int n = 0; n < dim; ++n) e += 0.5 * sig[m][n] * eps[m][n];
+                    out[q] = e;
+                    continue;
+                }
+                const double(*M)[3] = what == 0 ? eps : sig;
+                if (voigt && dim == 3) {
+                    const int vi[6] = {0, 1, 2, 1, 2, 0}, vj[6] = {0, 1, 2, 2, 0, 1};
+                    for (int cc = 0; cc < 6; ++cc) out[cc * g.nodes + q] = M[vi[cc]][vj[cc]];
+                } else if (voigt) {
+                    const int vi[3] = {0, 1, 0}, vj[3] = {0, 1, 1};
+                    for (int cc = 0; cc < 3; ++cc) out[cc * g.nodes + q] = M[vi[cc]][vj[cc]];
+                } else {
+                    for (int a = 0; a < dim; ++a)
+                        for (int b = 0; b < dim; ++b) out[(a * dim + b) * g.nodes + q] = M[a][b];
+                }
+            }
+}
+
 // normalize (Operator.cpp:164-195): the ParallelFor lambda indexes neither x
 // nor diag by component, so x(i,j,k,0) /= diag(i,j,k,0) once per component.
 void or_normalize(const or_grid* og, const double* diag, double* x) {

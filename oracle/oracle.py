@@ -75,6 +75,7 @@ def lib():
         L.or_mg_solve.restype = i
         L.or_reflux.argtypes = [P(_Grid), i, vp, vp, vp, vp]
         L.or_normalize.argtypes = [P(_Grid), vp, vp]
+        L.or_post.argtypes = [P(_Grid), i, vp, vp, i, i, vp]
         _lib = L
     return _lib
 
@@ -168,6 +169,14 @@ def restrict(gc, fine):
     crse = gc.zeros(nc)
     lib().or_restrict(ctypes.byref(gc.c()), nc, _p(fine), _p(crse))
     return crse
+
+
+def post(g, kind, C, u, what, voigt=False):
+    """Strain (what 0), Stress (1) or Energy (2) of u (Elastic.cpp:399-616)."""
+    nc = 1 if what == 2 else ((6 if g.dim == 3 else 3) if voigt else g.dim * g.dim)
+    out = np.zeros(g.shape(nc))
+    lib().or_post(ctypes.byref(g.c()), kind, _p(_c(C)), _p(_c(u)), what, int(bool(voigt)), _p(out))
+    return out[0] if what == 2 else out
 
 
 def normalize(g, diag, x):

@@ -426,3 +426,27 @@ def test_normalize_restates_the_reference_indexing(oracle):
     out = oracle.normalize(g, d, x)
     assert np.allclose(out[0], x[0] / d[0] ** 3, rtol=1e-15, atol=0)
     assert np.array_equal(out[1:], x[1:])
+
+
+# ------------------------------------------------------------------ Strain / Stress / Energy (Elastic.cpp:399-616)
+@pytest.mark.parametrize("voigt", [False, True])
+def test_post_processing_closed_forms(oracle, voigt):
+    """Affine u = G x: the wall stencils are exact, so every node has strain sym(G), isotropic stress
+    lambda tr(eps) I + 2 mu eps (textbook), and the reference's energy sum W + sig:eps/2 = sig:eps."""
+    g = oracle.Grid((5, 4, 6))
+    lam, mu = 1.7, 0.9
+    C = np.zeros(g.shape(2))
+    C[0], C[1] = lam, mu
+    G = np.array([[0.3, -0.2, 0.1], [0.05, -0.4, 0.2], [0.7, 0.0, 0.25]])
+    X = g.coords()
+    u = np.stack([sum(G[p, d] * X[d] for d in range(3)) for p in range(3)])
+    eps = 0.5 * (G + G.T)
+    sig = lam * np.trace(eps) * np.eye(3) + 2 * mu * eps
+    e_out = oracle.post(g, oracle.ISO, C, u, 0, voigt)
+    s_out = oracle.post(g, oracle.ISO, C, u, 1, voigt)
+    idx = [(0, 0), (1, 1), (2, 2), (1, 2), (2, 0), (0, 1)] if voigt else [(a, b) for a in range(3) for b in range(3)]
+    for c, (a, b) in enumerate(idx):
+        assert np.allclose(e_out[c], eps[a, b], rtol=0, atol=1e-14)
+        assert np.allclose(s_out[c], sig[a, b], rtol=0, atol=1e-13)
+    en = oracle.post(g, oracle.ISO, C, u, 2)
+    assert np.allclose(en, np.sum(sig * eps), rtol=1e-14, atol=0)  # twice the strain energy density
