@@ -32,7 +32,7 @@ DISP, TRAC, PERIODIC, NEUMANN = 0, 1, 2, 3
 F64, F32 = 0, 1
 SOLVE_FP64, SOLVE_MIXED, ALLOC_BOTH = 0, 1, 2
 CYCLE_F, CYCLE_V = 0, 1
-OPT_MARCH, OPT_KZ = 0, 1
+OPT_MARCH, OPT_KZ, OPT_XCD = 0, 1, 2
 
 ERRORS = {1: "BADARG", 2: "OOM", 3: "HIP", 4: "NONCONVERGED", 5: "NAN", 6: "STATE"}
 
@@ -73,6 +73,13 @@ class Request(ctypes.Structure):
                 ("crack", ctypes.c_int), ("crack_p", ctypes.c_double * 3), ("crack_nrm", ctypes.c_double * 3),
                 ("crack_tv", ctypes.c_double * 3), ("crack_width", ctypes.c_double), ("crack_g", ctypes.c_double * 3),
                 ("G", ctypes.c_double * 9), ("u_out", ctypes.c_void_p)]
+
+
+class ServeSummary(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int), ("groups", ctypes.c_int), ("admitted", ctypes.c_int),
+                ("peak_active", ctypes.c_int), ("peak_bytes", ctypes.c_size_t), ("t_admit", ctypes.c_double),
+                ("t_prepare", ctypes.c_double), ("t_step", ctypes.c_double), ("t_retire", ctypes.c_double),
+                ("t_idle", ctypes.c_double)]
 
 
 class Result(ctypes.Structure):
@@ -153,7 +160,8 @@ SIGNATURES = {
     "ela_postprocess": ([_P, _I, _I, _P, _P, _P], _I),
     "ela_reflux": ([ctypes.POINTER(_Layout), _P, ctypes.POINTER(_Layout), _P, _P, _I, _I, _P], _I),
     "ela_average_down": ([ctypes.POINTER(_Layout), _P, ctypes.POINTER(_Layout), _P, _P, _I, _I, _P], _I),
-    "ela_serve": ([_P, _I, ctypes.POINTER(Request), _D, _D, _I, _I, _D, ctypes.POINTER(Result)], _I),
+    "ela_serve": ([_P, _I, ctypes.POINTER(Request), _D, _D, _I, _I, _D, ctypes.POINTER(Result),
+                   ctypes.POINTER(ServeSummary)], _I),
     "ela_batch_smoother_times": ([_P, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_I)], _I),
 }
 
@@ -678,9 +686,11 @@ class Batch:
                 sols[i] = np.zeros((3,) + (int(q["n"]) + 1,) * 3)
                 r.u_out = sols[i].ctypes.data
         res = (Result * len(requests))()
+        summ = ServeSummary()
         _check(lib().ela_serve(self.ptr, len(requests), reqs, tol_rel, tol_abs, max_cycles, max_active, max_seconds,
-                               res))
+                               res, ctypes.byref(summ)))
         out = [{k: getattr(x, k) for k, _ in Result._fields_} for x in res]
+        self.last_serve = {k: getattr(summ, k) for k, _ in ServeSummary._fields_}
         return out, sols
 
     def smoother_times(self):

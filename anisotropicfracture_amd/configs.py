@@ -194,8 +194,12 @@ def cfg4(count=512, seed=SEED, sides=RVE_SIDES, grains=(8, 32)):
 
 
 # ------------------------------------------------------------------ serving stream
-# Member sides with a coarsest level of at most 7 cells (n = c 2^k, c in {2,3,5,7}).
-SERVE_SIDES = (32, 40, 48, 56, 64, 80, 96, 112, 128, 160, 192)
+# Member sides of the serving stream (n = c 2^k).  A coarsest level of c cells
+# costs a dense inverse of 3 (c+1)^3 unknowns at admission: for c = 7 that set-up
+# (~23 ms of elimination traffic per member) exceeds the whole solve of a 112^3
+# member, and for c = 5 it exceeds the solve of a 40^3 member, so those sizes
+# are served at the next side up: c in {2, 3} plus 80 and 160 (c = 5).
+SERVE_SIDES = (32, 48, 64, 80, 96, 128, 160, 192)
 
 
 def serve_side(n):
@@ -219,7 +223,8 @@ def random_crack(rng, n):
 
 def cfg5(rate=200.0, duration=60.0, seed=SEED, nmin=32, nmax=192, grains=(8, 64), p_crack=0.5, strain=0.01):
     """Serving stream: Poisson arrivals (rate per s) over `duration` s; side log-uniform in
-    [nmin, nmax] rounded to the nearest integer and then up to SERVE_SIDES; 8-64 Voronoi grains
+    [nmin, nmax] rounded to the nearest integer and then up to SERVE_SIDES (the requested side is
+    kept as n_requested); 8-64 Voronoi grains
     (Haar orientations, cubic constants above); a crack with probability p_crack; load
     u = strain (a.x) a on every face for a random unit vector a.  Sorted by arrival."""
     rng = np.random.default_rng(seed)

@@ -174,6 +174,22 @@ __global__ __launch_bounds__(TB) void kb_resid(const BoxDesc* __restrict__ D, co
     for (int p = 0; p < 3; ++p) rr[p * L.cs + o] = bb[p * L.cs + o] - f[p];
 }
 
+// Block max of a non-negative value (NaN propagates) into partials[slot].
+__device__ __forceinline__ void block_max_at(double* partials, int slot, double v) {
+    __shared__ double wmax[TB / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = (o > v || o != o) ? o : v;
+    }
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = wmax[0];
+        for (int w = 1; w < TB / 64; ++w) m = (wmax[w] > m || wmax[w] != wmax[w]) ? wmax[w] : m;
+        partials[slot] = m;
+    }
+}
+
 // Block max of a non-negative value (NaN propagates) into partials[blockIdx.x].
 __device__ __forceinline__ void block_max(double* partials, double v) {
     __shared__ double wmax[TB / 64];
@@ -392,6 +408,7 @@ struct MOps {
     double omega;
     double* partials;
     int mtot;  // marching blocks of the launch
+    int xcd;   // XCD-contiguous tile order (else hardware order)
 };
 
 template <typename T, int MODE>
@@ -408,6 +425,7 @@ __device__ __forceinline__ GrainBox<T> grain_of(const BoxDesc& d, const MOps& op
     g.ntx = d.ntx;
     g.nty = d.nty;
     g.kz = d.kz;
+    g.xcd = op.xcd;
     g.in = MODE == 2 ? (const T*)d.u64 : (const T*)d.f[op.in];
     g.rhs = MODE == 2 ? (const T*)d.b64 : (const T*)d.f[op.rhs];
     g.out = d.f[op.out];
@@ -423,16 +441,65 @@ __global__ __launch_bounds__(MTX* MTY) void km_grain(const BoxDesc* __restrict__
                                                      MOps op) {
     __shared__ MarchLds<T> sh;
     extern __shared__ __align__(16) unsigned char dyn_lds[];
-    const BoxDesc& d = D[bmap[blockIdx.x]];
-    const double m = march_tile<T, MODE, TO>(grain_of<T, MODE>(d, op), (int)blockIdx.x - d.mt0, sh, (T*)dyn_lds);
-    if (MODE == 2) block_max(op.partials, m);
+    const int lb = op.xcd ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+    const BoxDesc& d = D[bmap[lb]];
+    const double m = march_tile<T, MODE, TO>(grain_of<T, MODE>(d, op), lb - d.mt0, sh, (T*)dyn_lds);
+    if (MODE == 2) block_max_at(op.partials, lb, m);
 }
 
+// Shell of a grain-form box, reading the descriptor in place (a local copy of
+// the operands would be indexed per lane and spill to scratch).
 template <typename T, int MODE, typename TO>
 __global__ __launch_bounds__(TB) void ks_grain(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, MOps op) {
     const BoxDesc& d = D[bmap[blockIdx.x]];
-    const double m =
-        shell_node<T, MODE, TO>(grain_of<T, MODE>(d, op), ((int)blockIdx.x - d.st0) * TB + (int)threadIdx.x);
+    const Lay L = d.L;
+    int q = ((int)blockIdx.x - d.st0) * TB + (int)threadIdx.x;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int A = nx * ny, Bf = nx * (nz - 2), Cf = (ny - 2) * (nz - 2);
+    const bool in = q < 2 * A + 2 * Bf + 2 * Cf;
+    int i = 0, j = 0, k = 0;
+    if (in) {
+        if (q < 2 * A) {
+            k = q < A ? 0 : nz - 1;
+            const int r = q % A;
+            j = r / nx;
+            i = r - j * nx;
+        } else if ((q -= 2 * A) < 2 * Bf) {
+            j = q < Bf ? 0 : ny - 1;
+            const int r = q % Bf;
+            k = 1 + r / nx;
+            i = r - (k - 1) * nx;
+        } else {
+            q -= 2 * Bf;
+            i = q < Cf ? 0 : nx - 1;
+            const int r = q % Cf;
+            k = 1 + r / (ny - 2);
+            j = 1 + r - (k - 1) * (ny - 2);
+        }
+    }
+    double m = 0.0;
+    if (in) {
+        const T* u = MODE == 2 ? (const T*)d.u64 : (const T*)d.f[op.in];
+        const T* rhs = MODE == 2 ? (const T*)d.b64 : (const T*)d.f[op.rhs];
+        T f[3];
+        node_apply_acc<T, K_MM, 3>(L, GrainAcc<T, kNC>{d.gid, P<T>::tab(d), P<T>::sc(d)}, u, i, j, k, d.bc,
+                                   P<T>::inv(d), d.uniform != 0, f);
+        const long long o = (long long)i + L.px * j + L.pl * k;
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const long long e = p * L.cs + o;
+            if (MODE == 0) {
+                ((T*)d.f[op.out])[e] = u[e] + (T)op.omega * (rhs[e] - f[p]) / P<T>::dg(d)[e];
+            } else if (MODE == 1) {
+                ((T*)d.f[op.out])[e] = rhs[e] - f[p];
+            } else {
+                const double v = (double)rhs[e] - (double)f[p];
+                ((TO*)d.f[op.out])[e] = (TO)(v * *d.scale);
+                const double a = fabs(v);
+                m = (a > m || a != a) ? a : m;
+            }
+        }
+    }
     if (MODE == 2) block_max(op.partials + op.mtot, m);
 }
 
@@ -695,6 +762,7 @@ struct ela_batch {
     March mtop;
     bool march = true;
     int kz = 16;
+    bool xcd = false;  // XCD-contiguous tile order of the marching launches
     char* dlists = nullptr;
     size_t dlists_cap = 0;
     std::vector<char> hlists;
@@ -978,7 +1046,7 @@ void smooth(ela_batch* B, int g, int count, int rhs, int in, int out) {
                            (T)(2.0 / 3.0));
     if (nb > b0) {
         if (in >= 0 && count == B->cnt[g] && use_march(B, B->mlev[g]))
-            march_launch<T, 0, T>(B, LD(B, g), B->mlev[g], MOps{rhs, in, out, 2.0 / 3.0, nullptr, 0});
+            march_launch<T, 0, T>(B, LD(B, g), B->mlev[g], MOps{rhs, in, out, 2.0 / 3.0, nullptr, 0, B->xcd});
         else
             hipLaunchKernelGGL((kb_smooth<T, 1>), dim3(nb - b0), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), b0, rhs,
                                in, out, (T)(2.0 / 3.0));
@@ -1005,7 +1073,7 @@ void resid(ela_batch* B, int g, int count, int x, int b, int r) {
     if (b0) hipLaunchKernelGGL((kb_resid<T, 0>), dim3(b0), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), 0, x, b, r);
     if (nb > b0) {
         if (count == B->cnt[g] && use_march(B, B->mlev[g]))
-            march_launch<T, 1, T>(B, LD(B, g), B->mlev[g], MOps{b, x, r, 0.0, nullptr, 0});
+            march_launch<T, 1, T>(B, LD(B, g), B->mlev[g], MOps{b, x, r, 0.0, nullptr, 0, B->xcd});
         else
             hipLaunchKernelGGL((kb_resid<T, 1>), dim3(nb - b0), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), b0, x, b,
                                r);
@@ -1163,7 +1231,7 @@ int step_t(ela_batch* B) {
     hipLaunchKernelGGL((kb_update<T>), dim3(nblk), dim3(TB), 0, B->stream, TD(B), TM(B), F_COR);
     hipLaunchKernelGGL(kb_scale, dim3((nbox + 63) / 64), dim3(64), 0, B->stream, TD(B), nbox);
     if (use_march(B, B->mtop)) {
-        march_launch<double, 2, T>(B, TD(B), B->mtop, MOps{0, 0, F_RES, 0.0, B->partials, B->mtop.nm});
+        march_launch<double, 2, T>(B, TD(B), B->mtop, MOps{0, 0, F_RES, 0.0, B->partials, B->mtop.nm, B->xcd});
         hipLaunchKernelGGL(kb_fold_m, dim3(nbox), dim3(64), 0, B->stream, TD(B), B->partials, B->mtop.nm, B->dnorms);
     } else {
         resid64_top<T>(B, TD(B), TM(B), nbox, nblk);
@@ -1674,6 +1742,8 @@ int ela_batch_set_option(ela_batch* B, int option, int value) {
     } else if (option == ELA_OPT_KZ) {
         if (value < 1 || value > 1024) return fail(ELA_ERR_BADARG, "planes per marching tile must be 1..1024");
         B->kz = value;
+    } else if (option == ELA_OPT_XCD) {
+        B->xcd = value != 0;
     } else {
         return fail(ELA_ERR_BADARG, "unknown option");
     }
@@ -1726,7 +1796,7 @@ size_t ela_batch_bytes_in_use(const ela_batch* B) {
 // finished members are copied out (if the request asks) and freed, which
 // lets waiting requests in.  Times are seconds from the start of the call.
 int ela_serve(ela_batch* B, int nreq, const ela_request* req, double tol_rel, double tol_abs, int max_cycles,
-              int max_active, double max_seconds, ela_result* out) {
+              int max_active, double max_seconds, ela_result* out, ela_serve_summary* summary) {
     if (!B || nreq < 0 || (nreq && (!req || !out)) || max_active < 1 || max_cycles < 1)
         return fail(ELA_ERR_BADARG, "bad arguments");
     for (int i = 1; i < nreq; ++i)
@@ -1741,11 +1811,14 @@ int ela_serve(ela_batch* B, int nreq, const ela_request* req, double tol_rel, do
     std::deque<int> waiting;
     std::vector<std::pair<ela_rve*, int>> live;
     int next = 0, done = 0, rc = ELA_OK;
+    ela_serve_summary sm{};
+    size_t live_bytes = 0;
     while (done < nreq) {
         const double t = now_s() - t0;
         if (max_seconds > 0 && t > max_seconds) break;
         while (next < nreq && req[next].arrival <= t) waiting.push_back(next++);
         // admission
+        const double ta0 = now_s();
         std::vector<ela_rve*> group;
         std::vector<int> gidx;
         while (!waiting.empty() && (int)live.size() + (int)group.size() < max_active) {
@@ -1770,9 +1843,15 @@ int ela_serve(ela_batch* B, int nreq, const ela_request* req, double tol_rel, do
             }
             group.push_back(r);
             gidx.push_back(i);
+            live_bytes += r->bytes;
         }
+        sm.t_admit += now_s() - ta0;
         if (!group.empty()) {
+            const double tp0 = now_s();
             if ((rc = prepare(B, (int)group.size(), group.data()))) break;
+            sm.t_prepare += now_s() - tp0;
+            ++sm.groups;
+            sm.admitted += (int)group.size();
             const double ta = now_s() - t0;
             for (size_t m = 0; m < group.size(); ++m) {
                 if ((rc = ela_batch_submit(B, group[m], tol_rel, tol_abs, max_cycles))) break;
@@ -1781,8 +1860,14 @@ int ela_serve(ela_batch* B, int nreq, const ela_request* req, double tol_rel, do
             }
             if (rc) break;
         }
+        sm.peak_active = std::max(sm.peak_active, (int)live.size());
+        sm.peak_bytes = std::max(sm.peak_bytes, live_bytes);
         if (!B->active.empty() || !B->queued.empty()) {
+            const double ts0 = now_s();
             if ((rc = step(B))) break;
+            sm.t_step += now_s() - ts0;
+            ++sm.steps;
+            const double tr0 = now_s();
             std::vector<std::pair<ela_rve*, int>> keep;
             for (auto& pr : live) {
                 ela_rve* r = pr.first;
@@ -1797,15 +1882,20 @@ int ela_serve(ela_batch* B, int nreq, const ela_request* req, double tol_rel, do
                 o.res0 = r->res0;
                 o.res = r->res;
                 if (req[pr.second].u_out) ela_rve_download(r, 0, req[pr.second].u_out);
+                live_bytes -= r->bytes;
                 ela_rve_destroy(r);
                 ++done;
             }
             live.swap(keep);
+            sm.t_retire += now_s() - tr0;
         } else if (waiting.empty() && next < nreq) {
+            const double ti0 = now_s();
             const double wait = req[next].arrival - (now_s() - t0);
             if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.01)));
+            sm.t_idle += now_s() - ti0;
         }
     }
+    if (summary) *summary = sm;
     // leave nothing behind: requests still in flight at the time limit are dropped
     for (auto& pr : live) ela_rve_destroy(pr.first);
     B->active.clear();

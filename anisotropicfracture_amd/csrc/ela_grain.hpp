@@ -58,6 +58,7 @@ struct GrainBox {
     const T* sc;
     int ngr;
     int ntx, nty, kz;
+    int xcd;  // XCD-contiguous tile order
     const T* in;
     const T* rhs;
     void* out;
@@ -222,6 +223,17 @@ __device__ __forceinline__ double shell_node(const GrainBox<T>& g, int q) {
 #pragma unroll
     for (int p = 0; p < 3; ++p) xc[p] = g.in[p * L.cs + o];
     return grain_epilogue<T, MODE, TO>(g, o, xc, f);
+}
+
+// XCD-contiguous block numbering: hardware block b of a launch of n blocks
+// runs on XCD b % 8; give each XCD a contiguous range of logical blocks, so
+// neighbouring tiles (which share halo lines) run under the same L2.
+__device__ __forceinline__ int xcd_block(int b, int n) {
+    const int per = (n + 7) / 8;
+    const int x = b % 8, r = b / 8;
+    const int full = n % 8 == 0 ? 8 : n % 8;  // XCDs that receive `per` blocks
+    if (x < full) return x * per + r;
+    return full * per + (x - full) * (per - 1) + r;
 }
 
 // Tiles and shell blocks of a box (host and device).

@@ -810,8 +810,9 @@ template <typename T, int MODE, typename TO>
 __global__ __launch_bounds__(MTX* MTY) void k_grain_march(GrainBox<T> g, double* partials) {
     __shared__ MarchLds<T> sh;
     extern __shared__ __align__(16) unsigned char dyn_lds[];
-    const double m = march_tile<T, MODE, TO>(g, (int)blockIdx.x, sh, (T*)dyn_lds);
-    if (MODE == 2 && partials) block_max_store(partials, m);
+    const int t = g.xcd ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+    const double m = march_tile<T, MODE, TO>(g, t, sh, (T*)dyn_lds);
+    if (MODE == 2 && partials) block_max_store(partials, m);  // a max: the slot order does not matter
 }
 
 template <typename T, int MODE, typename TO>

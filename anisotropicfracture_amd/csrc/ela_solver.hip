@@ -133,6 +133,7 @@ struct ela_hier {
     // BC table on the device for them
     bool grain_march = true;
     int kz = 16;
+    bool xcd = true;  // XCD-contiguous tile order of the marching kernels
     BCTab* dbc = nullptr;
     // flexible GMRES around the MG cycle (restart length; 0 = plain MG iteration)
     int krylov = 0;
@@ -592,6 +593,7 @@ GrainBox<T> grain_box(ela_hier* h, const Level& v) {
     g.tab = tab_of<T>(h).tab;
     g.sc = nullptr;
     g.ngr = h->ngrains;
+    g.xcd = h->xcd;
     return g;
 }
 
@@ -1290,6 +1292,8 @@ int ela_set_option(ela_hier* h, int option, int value) {
     } else if (option == ELA_OPT_KZ) {
         if (value < 1 || value > 1024) return fail(ELA_ERR_BADARG, "planes per marching tile must be 1..1024");
         h->kz = value;
+    } else if (option == ELA_OPT_XCD) {
+        h->xcd = value != 0;
     } else {
         return fail(ELA_ERR_BADARG, "unknown option");
     }

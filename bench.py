@@ -149,7 +149,7 @@ def batched_rves(count, sample, tol):
     return out
 
 
-def serving(seconds, max_wall, tol, budget_gb=200.0):
+def serving(seconds, max_wall, tol, budget_gb=200.0, max_active=256):
     """Config 5: the Poisson(200/s) request stream, arrivals in the first `seconds` s, served by the
     continuous-batching loop (ela_serve) under a `budget_gb` HBM budget, stopped after `max_wall` s.
     Two requests are re-solved alone afterwards and compared bit for bit."""
@@ -159,7 +159,8 @@ def serving(seconds, max_wall, tol, budget_gb=200.0):
     ctx = ela.Context(0, int(budget_gb * 1e9))
     b = ela.Batch(ela.SOLVE_MIXED, ctx=ctx)
     keep = (0, min(40, len(reqs) - 1))
-    res, sols = b.serve(reqs, tol_rel=tol, max_cycles=100, max_active=512, max_seconds=max_wall, keep=keep)
+    res, sols = b.serve(reqs, tol_rel=tol, max_cycles=100, max_active=max_active, max_seconds=max_wall, keep=keep)
+    summary = dict(b.last_serve)
     done = [(r, q) for r, q in zip(res, reqs) if r["status"] == 0 and r["finished"] >= 0]
     lat = np.array([r["finished"] - q["arrival"] for r, q in done])
     wall = max(r["finished"] for r, _ in done) if done else float("nan")
@@ -187,8 +188,10 @@ def serving(seconds, max_wall, tol, budget_gb=200.0):
            "p50_ms": float(np.percentile(lat, 50) * 1e3) if done else None,
            "p99_ms": float(np.percentile(lat, 99) * 1e3) if done else None,
            "served_Mnodes_per_s": float(nodes.sum() / wall / 1e6) if done else None,
-           "budget_GB": budget_gb, "alone_vs_mix_bitwise": same,
-           "not_served_by_limit": int(sum(1 for r in res if r["status"] == -1))}
+           "budget_GB": budget_gb, "max_active": max_active, "alone_vs_mix_bitwise": same,
+           "requested_Mnodes": float(sum((q["n_requested"] + 1) ** 3 for _, q in done) / 1e6) if done else None,
+           "not_served_by_limit": int(sum(1 for r in res if r["status"] == -1)),
+           "time_breakdown_s": summary}
     log("bench: serving %d/%d requests in %.1f s: %.1f solves/s p50 %.0f ms p99 %.0f ms"
         % (len(done), len(reqs), wall, out["solves_per_s"], out["p50_ms"] or -1, out["p99_ms"] or -1))
     return out
@@ -205,7 +208,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed", help="comma list of fp64, mixed")
-    ap.add_argument("--krylov", type=int, default=16, help="FGMRES restart around the MG cycle (0: plain MG)")
+    ap.add_argument("--krylov", type=int, default=8, help="FGMRES restart around the MG cycle (0: plain MG)")
     ap.add_argument("--table", type=int, default=1, help="grain-table stiffness on level 0 for the headline mixed run")
     ap.add_argument("--rves", type=int, default=512, help="batched RVE count (0: skip the batched workload)")
     ap.add_argument("--plain-sample", type=int, default=24, help="RVEs solved one at a time by the plain FP64 solver")

@@ -237,8 +237,11 @@ int ela_set_krylov(ela_hier* h, int restart);
  *                form, up to 256 grains) runs the LDS-tiled z-marching stencil
  *                kernels; 0: the one-thread-per-node kernels.  The arithmetic
  *                is the same, so are the results, bit for bit.
- * ELA_OPT_KZ     planes marched per tile (default 16). */
-enum { ELA_OPT_MARCH = 0, ELA_OPT_KZ = 1 };
+ * ELA_OPT_KZ     planes marched per tile (default 16).
+ * ELA_OPT_XCD    1: consecutive tiles of a marching launch run on one XCD
+ *                (shared halo lines stay in its L2); 0: hardware order.
+ *                Default 1 for a hierarchy, 0 for a batch. */
+enum { ELA_OPT_MARCH = 0, ELA_OPT_KZ = 1, ELA_OPT_XCD = 2 };
 int ela_set_option(ela_hier* h, int option, int value);
 
 /* Record HIP events around the level-0 kernels of subsequent solves (on != 0). */
@@ -398,13 +401,28 @@ typedef struct {
     double res0, res;
 } ela_result;
 
+/* Where a serving run spent its time (seconds of host wall time). */
+typedef struct {
+    int steps;         /* cycles run over the members in flight */
+    int groups;        /* admission groups (one set-up each) */
+    int admitted;
+    int peak_active;   /* most members in flight at once */
+    size_t peak_bytes; /* most member bytes resident at once */
+    double t_admit;    /* member creation, model and right-hand side evaluation */
+    double t_prepare;  /* batched set-up of admitted groups */
+    double t_step;     /* cycles (including the FP64 residual and its norm read-back) */
+    double t_retire;   /* copy-out and freeing of finished members */
+    double t_idle;     /* waiting for arrivals with nothing in flight */
+} ela_serve_summary;
+
 /* Serving loop with continuous batching (no counterpart in the reference):
  * requests sorted by arrival are admitted first-come first-served while the
  * context's HBM budget and max_active allow; each step is one F-cycle over
  * every member in flight; members leave as they converge and free their
- * memory.  Stops when all requests are done or after max_seconds (> 0). */
+ * memory.  Stops when all requests are done or after max_seconds (> 0).
+ * summary (may be NULL) receives the time breakdown. */
 int ela_serve(ela_batch* b, int nreq, const ela_request* req, double tol_rel, double tol_abs, int max_cycles,
-              int max_active, double max_seconds, ela_result* out);
+              int max_active, double max_seconds, ela_result* out, ela_serve_summary* summary);
 
 /* Kernel options of a batch (set while no member is queued or active); see
  * ela_set_option. */
