@@ -127,6 +127,8 @@ SIGNATURES = {
     "ela_set_krylov": ([_P, _I], _I),
     "ela_set_stiffness_mode": ([_P, _I], _I),
     "ela_set_option": ([_P, _I, _I], _I),
+    "ela_autotune": ([_P, _I, _P], _I),
+    "ela_batch_autotune": ([_P, _I, _P, _I, _P], _I),
     "ela_kernel_times": ([_P, ctypes.POINTER(_D), ctypes.POINTER(_I), ctypes.POINTER(_D), ctypes.POINTER(_I)], _I),
     "ela_time_fapply": ([_P, _I, _I, ctypes.POINTER(_D)], _I),
     "ela_probe_copy": ([_P, _SZ, _I, ctypes.POINTER(_D)], _I),
@@ -343,6 +345,13 @@ class Elastic:
         """ela_set_option: OPT_MARCH (LDS z-marching kernels for grain-form stiffness, default on) or
         OPT_KZ (planes per marching tile, default 16)."""
         _check(lib().ela_set_option(self.ptr, option, int(value)))
+
+    def autotune(self, nrep=10):
+        """Pick the marching tile height / order by timing level-0 sweeps (ela_autotune); returns
+        {(kz, xcd): ms}."""
+        ms = np.zeros(6)
+        _check(lib().ela_autotune(self.ptr, int(nrep), _ptr(ms)))
+        return {(kz, x): ms[2 * a + x] for a, kz in enumerate((8, 16, 32)) for x in (0, 1)}
 
     def SetKrylov(self, restart):
         """Flexible GMRES(restart) around the MG cycle (0: plain MG iteration)."""
@@ -657,6 +666,12 @@ class Batch:
         """ela_batch_set_option: OPT_MARCH (LDS z-marching kernels on the grain-form levels, default on)
         or OPT_KZ (planes per marching tile, default 16)."""
         _check(lib().ela_batch_set_option(self.ptr, option, int(value)))
+
+    def autotune(self, rves, nrep=5):
+        """ela_batch_autotune over the finest levels of `rves`; returns {(kz, xcd): ms}."""
+        ms = np.zeros(6)
+        _check(lib().ela_batch_autotune(self.ptr, len(rves), _ptr_array(rves), int(nrep), _ptr(ms)))
+        return {(kz, x): ms[2 * a + x] for a, kz in enumerate((8, 16, 32)) for x in (0, 1)}
 
     def serve(self, requests, tol_rel=1e-10, tol_abs=0.0, max_cycles=100, max_active=256, max_seconds=0.0,
               keep=()):

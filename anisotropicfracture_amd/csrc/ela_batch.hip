@@ -1751,6 +1751,81 @@ int ela_batch_set_option(ela_batch* B, int option, int value) {
     return ELA_OK;
 }
 
+int ela_batch_autotune(ela_batch* B, int count, ela_rve* const* rves, int nrep, double* ms) {
+    if (!B || count < 1 || !rves || nrep < 1) return fail(ELA_ERR_BADARG, "bad arguments");
+    if (!B->active.empty() || !B->queued.empty()) return fail(ELA_ERR_STATE, "batch is busy");
+    for (int m = 0; m < count; ++m) {
+        if (int rc = check_rve(rves[m])) return rc;
+        if (!rves[m]->prepared) return fail(ELA_ERR_STATE, "member not prepared");
+        if (rves[m]->ngrains > kMarchMaxGrains) return fail(ELA_ERR_STATE, "too many grains for the marching kernels");
+    }
+    const int kzs[3] = {8, 16, 32};
+    double best = 1e300;
+    int bkz = B->kz, bx = B->xcd ? 1 : 0;
+    hipEvent_t e0, e1;
+    ELA_HIPCHK(hipEventCreate(&e0));
+    ELA_HIPCHK(hipEventCreate(&e1));
+    char* dl = nullptr;
+    size_t dl_cap = 0;
+    int rc = ELA_OK;
+    for (int a = 0; a < 3 && !rc; ++a) {
+        // the finest levels of the members, numbered for this tile height
+        ListBuild lb;
+        for (int m = 0; m < count; ++m) list_push(lb, desc_of(rves[m], 0));
+        std::vector<int> bm, bs;
+        ela_batch::March mi;
+        march_number(lb, 0, count, kzs[a], bm, bs, mi);
+        std::vector<char> save;
+        save.swap(B->hlists);
+        size_t od = 0, om = 0;
+        pack_list(B, lb, &od, &om);
+        mi.off_bm = pack_ints(B, bm);
+        mi.off_bs = pack_ints(B, bs);
+        std::vector<char> bytes;
+        bytes.swap(B->hlists);
+        B->hlists.swap(save);
+        if ((rc = ensure_device(B, (void**)&dl, &dl_cap, bytes.size()))) break;
+        if (hipMemcpyAsync(dl, bytes.data(), bytes.size(), hipMemcpyHostToDevice, B->stream) != hipSuccess) {
+            rc = fail(ELA_ERR_HIP, "autotune upload");
+            break;
+        }
+        char* keep = B->dlists;
+        B->dlists = dl;  // march_launch reads the block maps through B->dlists
+        const BoxDesc* D = (const BoxDesc*)(dl + od);
+        for (int x = 0; x < 2; ++x) {
+            const MOps op{F_RES, F_COR, F_TMP, 2.0 / 3.0, nullptr, 0, x};
+            auto sweep = [&]() {
+                if (B->mode == ELA_SOLVE_MIXED) march_launch<float, 0, float>(B, D, mi, op);
+                else march_launch<double, 0, double>(B, D, mi, op);
+            };
+            sweep();
+            (void)hipEventRecord(e0, B->stream);
+            for (int r = 0; r < nrep; ++r) sweep();
+            (void)hipEventRecord(e1, B->stream);
+            (void)hipEventSynchronize(e1);
+            float t = 0;
+            (void)hipEventElapsedTime(&t, e0, e1);
+            const double avg = (double)t / nrep;
+            if (ms) ms[2 * a + x] = avg;
+            if (avg < best) {
+                best = avg;
+                bkz = kzs[a];
+                bx = x;
+            }
+        }
+        B->dlists = keep;
+        (void)hipStreamSynchronize(B->stream);
+    }
+    if (dl) (void)hipFree(dl);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    B->kz = bkz;
+    B->xcd = bx != 0;
+    B->dirty = true;
+    return ELA_OK;
+}
+
 int ela_batch_set_timing(ela_batch* B, int on) {
     if (!B) return fail(ELA_ERR_BADARG, "null batch");
     B->timing = on != 0;

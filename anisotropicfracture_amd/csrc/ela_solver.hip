@@ -1322,6 +1322,55 @@ int ela_kernel_times(const ela_hier* h, double* ms_resid64, int* n_resid64, doub
     return ELA_OK;
 }
 
+}  // extern "C"
+
+namespace {
+template <typename T>
+int autotune_t(ela_hier* h, int nrep, double* ms) {
+    Level& v = h->lev[0];
+    LB<T> b = lb<T>(v);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const int kzs[3] = {8, 16, 32};
+    double best = 1e300;
+    int bkz = h->kz, bx = h->xcd ? 1 : 0;
+    for (int a = 0; a < 3; ++a)
+        for (int x = 0; x < 2; ++x) {
+            h->kz = kzs[a];
+            h->xcd = x != 0;
+            A_jacobi<T>(h, v, b.C, b.dg, *b.cor, *b.res, *b.tmp, h->stream);  // first launch of the shape
+            HIPCHK(hipEventRecord(e0, h->stream));
+            for (int r = 0; r < nrep; ++r) A_jacobi<T>(h, v, b.C, b.dg, *b.cor, *b.res, *b.tmp, h->stream);
+            HIPCHK(hipEventRecord(e1, h->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, e0, e1));
+            const double avg = (double)t / nrep;
+            if (ms) ms[2 * a + x] = avg;
+            if (avg < best) {
+                best = avg;
+                bkz = kzs[a];
+                bx = x;
+            }
+        }
+    h->kz = bkz;
+    h->xcd = bx != 0;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ELA_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int ela_autotune(ela_hier* h, int nrep, double* ms) {
+    if (!h || nrep < 1) return fail(ELA_ERR_BADARG, "bad argument");
+    if (!h->prepared) return fail(ELA_ERR_STATE, "call ela_prepare first");
+    if (!grain_on(h, h->lev[0])) return fail(ELA_ERR_STATE, "level 0 does not run the marching kernels");
+    return h->has32 ? autotune_t<float>(h, nrep, ms) : autotune_t<double>(h, nrep, ms);
+}
+
 int ela_time_fapply(ela_hier* h, int prec, int nrep, double* avg_ms) {
     if (!h || !avg_ms || nrep < 1) return fail(ELA_ERR_BADARG, "bad argument");
     if (int rc = check_prec(h, prec)) return rc;

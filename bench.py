@@ -96,6 +96,12 @@ def batched_rves(count, sample, tol):
         for r in warm:
             r.close()
         rs = [b.rve(m["n"], max_grains=32) for m in cfg]
+        probe = [b.rve(n, max_grains=32) for n in sorted({m["n"] for m in cfg})]  # one member per size
+        load(b, probe, [next(m for m in cfg if m["n"] == r.n) for r in probe])
+        b.prepare(probe)
+        tuned = {"%d/%d" % k: v for k, v in b.autotune(probe, nrep=3).items()}
+        for r in probe:
+            r.close()
         b.set_timing(key == "mixed")
         t0 = time.perf_counter()
         load(b, rs, cfg)
@@ -106,7 +112,8 @@ def batched_rves(count, sample, tol):
         cycles[key] = it
         ent = {"solves_per_s": count / t, "seconds": t, "cycles_mean": float(it.mean()), "cycles_max": int(it.max()),
                "converged": int(sum(q["converged"] for q in sts)),
-               "max_rel_residual": float(max(q["res"] / q["res0"] for q in sts)), "device_GB": b.bytes() / 1e9}
+               "max_rel_residual": float(max(q["res"] / q["res0"] for q in sts)), "device_GB": b.bytes() / 1e9,
+               "autotune_ms_per_sweep": tuned}
         if key == "mixed":
             sm = b.smoother_times()
             ent["smoother"] = {"ms": sm["ms"], "launches": sm["launches"], "node_sweeps": sm["nodes"],
@@ -232,6 +239,11 @@ def main():
     op.SetBC(cfg["bc_type"], cfg["bc_value"])
     op.SetKrylov(args.krylov)
     op.prepareForSolve()
+    tuned = None
+    if args.table:  # marching tile shape for this hierarchy, picked before the timed region
+        op.SetStiffnessMode(1)
+        tuned = {"%d/%d" % k: v for k, v in op.autotune(nrep=10).items()}
+        op.SetStiffnessMode(0)
     b_host = op.bc_init()
     b = op.field(0, ela.F64).upload(b_host)
     u = op.field(0, ela.F64)
@@ -339,6 +351,7 @@ def main():
         "fp64_solves_per_s": args.steps / t64 if t64 else None,
         "mixed_over_fp64": t64 / tmx if t64 else None,
         "stiffness": "grain table on level 0 (16-bit grain id + per-grain records)" if args.table else "per-node planes",
+        "autotune_ms_per_sweep": tuned,
         "variants_solves_per_s": {"fp64_planes": args.steps / t64 if t64 else None,
                                   "mixed_planes": args.steps / tmp if tmp else None,
                                   "fp64_table": args.steps / t64t if t64t else None,

@@ -244,6 +244,13 @@ int ela_set_krylov(ela_hier* h, int restart);
 enum { ELA_OPT_MARCH = 0, ELA_OPT_KZ = 1, ELA_OPT_XCD = 2 };
 int ela_set_option(ela_hier* h, int option, int value);
 
+/* Autotuner of the marching kernels (no reference counterpart): times nrep
+ * level-0 Jacobi sweeps on this hierarchy's data for every tile height in
+ * {8, 16, 32} and both tile orders, and keeps the fastest (ELA_OPT_KZ,
+ * ELA_OPT_XCD).  ms (6 doubles, may be NULL) receives the average sweep
+ * times, kz 8/16/32 x order 0/1.  Requires ela_prepare and grain form. */
+int ela_autotune(ela_hier* h, int nrep, double* ms);
+
 /* Record HIP events around the level-0 kernels of subsequent solves (on != 0). */
 int ela_set_timing(ela_hier* h, int on);
 
@@ -427,6 +434,11 @@ int ela_serve(ela_batch* b, int nreq, const ela_request* req, double tol_rel, do
 /* Kernel options of a batch (set while no member is queued or active); see
  * ela_set_option. */
 int ela_batch_set_option(ela_batch* b, int option, int value);
+
+/* The autotuner of ela_autotune over the finest levels of the given prepared
+ * members (a representative set of the batch's sizes); keeps the fastest
+ * tile height and order for the batch. */
+int ela_batch_autotune(ela_batch* b, int count, ela_rve* const* rves, int nrep, double* ms);
 
 /* HIP-event timing of every smoother launch of later steps (on != 0), and
  * the totals since: milliseconds, nodes swept, launches. */

@@ -265,3 +265,20 @@ def test_marching_kernels_match_the_node_kernels(mode):
     for a, c in zip(sols[0], sols[1]):
         print("bitwise identical:", np.array_equal(a, c))
         assert np.abs(a - c).max() <= 1e-13 * np.abs(a).max()
+
+
+def test_batch_autotune_keeps_the_bits():
+    probs = [problem(71, 32, 10), problem(72, 24, 7, crack=True)]
+    sols = []
+    for tune in (False, True):
+        b = E.Batch(E.SOLVE_MIXED)
+        rs = [make(b, p) for p in probs]
+        b.prepare(rs)
+        if tune:
+            times = b.autotune(rs, nrep=2)
+            assert len(times) == 6 and all(t > 0 for t in times.values())
+        b.solve(rs, tol_rel=1e-10, max_cycles=80)
+        sols.append([r.download(E.RVE.U) for r in rs])
+        b.close()
+    for a, c in zip(*sols):
+        assert np.array_equal(a, c)

@@ -294,3 +294,23 @@ def test_grain_marching_kernels_give_the_node_kernel_bits(mode, krylov):
         op.close()
     assert iters[0] == iters[1]
     assert np.array_equal(sols[0], sols[1])
+
+
+def test_autotune_picks_a_tile_shape_without_changing_results():
+    cfg = K.cfg2(40, grains=10, seed=6)
+    sols = []
+    for tune in (False, True):
+        op = ela.Elastic(cfg["n"], MM, mode=ela.ALLOC_BOTH)
+        op.SetModelVoronoiCubic(cfg["seeds"], K.VORONOI_Q, cfg["R"], cfg["C11"], cfg["C12"], cfg["C44"])
+        op.SetBC(cfg["bc_type"], cfg["bc_value"])
+        op.SetStiffnessMode(1)
+        op.prepareForSolve()
+        if tune:
+            times = op.autotune(nrep=3)
+            assert len(times) == 6 and all(t > 0 for t in times.values())
+        b = op.bc_init()
+        u, st = ela.Linear(op).solve(np.zeros_like(b), b, tol_rel=1e-10, tol_abs=0.0, mode=ela.SOLVE_MIXED, max_iter=300)
+        assert st["converged"]
+        sols.append(u)
+        op.close()
+    assert np.array_equal(sols[0], sols[1])
