@@ -436,14 +436,14 @@ __device__ __forceinline__ GrainBox<T> grain_of(const BoxDesc& d, const MOps& op
     return g;
 }
 
-template <typename T, int MODE, typename TO>
+template <typename T, int MODE, typename TO, bool SC>
 __global__ __launch_bounds__(MTX* MTY) void km_grain(const BoxDesc* __restrict__ D, const int* __restrict__ bmap,
                                                      MOps op) {
     __shared__ MarchLds<T> sh;
     extern __shared__ __align__(16) unsigned char dyn_lds[];
     const int lb = op.xcd ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
     const BoxDesc& d = D[bmap[lb]];
-    const double m = march_tile<T, MODE, TO>(grain_of<T, MODE>(d, op), lb - d.mt0, sh, (T*)dyn_lds);
+    const double m = march_tile<T, MODE, TO, SC>(grain_of<T, MODE>(d, op), lb - d.mt0, sh, (T*)dyn_lds);
     if (MODE == 2) block_max_at(op.partials, lb, m);
 }
 
@@ -757,6 +757,7 @@ struct ela_batch {
     struct March {
         size_t off_bm = 0, off_bs = 0;  // device offsets of the marching / shell block maps
         int nm = 0, ns = 0, maxng = 0;  // marching blocks, shell blocks, largest grain count
+        bool scaled = false;            // some box of the launch carries a per-node factor
     };
     std::vector<March> mlev;
     March mtop;
@@ -933,6 +934,7 @@ void march_number(ListBuild& lb, int a, int b, int kz, std::vector<int>& bm, std
         d.st0 = mi.ns;
         mi.ns += d.nsh;
         mi.maxng = std::max(mi.maxng, d.ngr);
+        mi.scaled = mi.scaled || d.sc64 != nullptr;
         bm.insert(bm.end(), (size_t)(d.ntx * d.ntz * d.nty), x);
         bs.insert(bs.end(), (size_t)d.nsh, x);
     }
@@ -1025,8 +1027,12 @@ bool use_march(const ela_batch* B, const ela_batch::March& mi) {
 template <typename T, int MODE, typename TO>
 void march_launch(ela_batch* B, const BoxDesc* D, const ela_batch::March& mi, const MOps& op) {
     const size_t lds = (size_t)mi.maxng * kNC * sizeof(T);
-    hipLaunchKernelGGL((km_grain<T, MODE, TO>), dim3(mi.nm), dim3(MTX * MTY), lds, B->stream, D,
-                       (const int*)(B->dlists + mi.off_bm), op);
+    if (mi.scaled)
+        hipLaunchKernelGGL((km_grain<T, MODE, TO, true>), dim3(mi.nm), dim3(MTX * MTY), lds, B->stream, D,
+                           (const int*)(B->dlists + mi.off_bm), op);
+    else
+        hipLaunchKernelGGL((km_grain<T, MODE, TO, false>), dim3(mi.nm), dim3(MTX * MTY), lds, B->stream, D,
+                           (const int*)(B->dlists + mi.off_bm), op);
     hipLaunchKernelGGL((ks_grain<T, MODE, TO>), dim3(mi.ns), dim3(TB), 0, B->stream, D,
                        (const int*)(B->dlists + mi.off_bs), op);
 }

@@ -19,11 +19,14 @@ constexpr int MTX = 32, MTY = 8, MRW = MTY + 2, MCW = MTX + 2, MPL = MRW * MCW;
 constexpr int kGrainNC = 21;
 constexpr int kMarchMaxGrains = 256;  // the table is staged in LDS up to this many grains
 
-template <typename T>
+// SC: the launch carries per-node factors (LDS ss holds them, 1 for a box
+// without); else the records are read as they are.  The values are the same,
+// since 1 x = x exactly.
+template <typename T, bool SC>
 struct LdsCtx {
     const T* su;               // [3 slots][3 comps][MRW][MCW]
     const unsigned short* sg;  // [3 slots][MRW][MCW]
-    const T* ss;               // [3 slots][MRW][MCW] or null
+    const T* ss;               // [3 slots][MRW][MCW] (SC only)
     const T* stab;             // grain table
     int sl[3];                 // ring slots of planes k-1, k, k+1
     int r, c;                  // this node's row / column in the tile arrays
@@ -33,13 +36,15 @@ struct LdsCtx {
     __device__ __forceinline__ T u(int p, int dx, int dy, int dz) const {
         return su[((sl[dz + 1] * 3 + p) * MRW + r + dy) * MCW + c + dx];
     }
-    __device__ __forceinline__ ScaledRec<T> rec(int dx, int dy, int dz) const {
+    __device__ __forceinline__ auto rec(int dx, int dy, int dz) const {
         const int q = at(dx, dy, dz);
-        return ScaledRec<T>{stab + (int)sg[q] * kGrainNC, ss ? ss[q] : T(1)};
+        if constexpr (SC) return ScaledRec<T>{stab + (int)sg[q] * kGrainNC, ss[q]};
+        else return TabRec<T>{stab + (int)sg[q] * kGrainNC};
     }
     __device__ __forceinline__ bool same(int d) const {
         const int qp = at(d == 0, d == 1, d == 2), qm = at(-(d == 0), -(d == 1), -(d == 2));
-        return sg[qp] == sg[qm] && (!ss || ss[qp] == ss[qm]);
+        if constexpr (SC) return sg[qp] == sg[qm] && ss[qp] == ss[qm];
+        else return sg[qp] == sg[qm];
     }
 };
 
@@ -102,7 +107,7 @@ struct MarchLds {
 
 // Tile t of box g (256 threads); returns the thread's max |v| (MODE 2).
 // Every thread of the block calls this (it synchronizes).
-template <typename T, int MODE, typename TO>
+template <typename T, int MODE, typename TO, bool SC>
 __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchLds<T>& sh, T* stab) {
     static_assert(MODE != 2 || sizeof(T) == 8, "the solution residual is FP64");
     const Lay& L = g.L;
@@ -141,7 +146,9 @@ __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchL
                 if (ii < L.nx && jj < L.ny) {
                     const long long o = ii + L.px * jj + L.pl * kk;
                     gv = g.gid[o];
-                    if (g.sc) sv = g.sc[o];
+                    // a batched launch takes SC when any of its boxes has a factor;
+                    // the boxes without one read s = 1
+                    if (SC && g.sc) sv = g.sc[o];
                 }
             }
             rg[e] = gv;
@@ -159,7 +166,7 @@ __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchL
             const int q = tid + e * MTX * MTY;
             if (q < MPL) {
                 sh.sg[slot * MPL + q] = rg[e];
-                sh.ss[slot * MPL + q] = rs[e];
+                if (SC) sh.ss[slot * MPL + q] = rs[e];
             }
         }
     };
@@ -175,7 +182,7 @@ __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchL
         const bool more = k + 1 < ke;
         if (more) load(k + 2);
         if (active) {
-            LdsCtx<T> S{sh.su, sh.sg, g.sc ? sh.ss : nullptr, stab, {(k - 1) % 3, k % 3, (k + 1) % 3}, ty + 1, tx + 1};
+            LdsCtx<T, SC> S{sh.su, sh.sg, sh.ss, stab, {(k - 1) % 3, k % 3, (k + 1) % 3}, ty + 1, tx + 1};
             T f[3], xc[3];
             interior_apply<T, K_MM, 3>(S, g.iv, g.uniform != 0, f);
 #pragma unroll

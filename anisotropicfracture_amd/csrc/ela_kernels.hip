@@ -806,12 +806,12 @@ void launch_post(const Lay& L, int kind, const double* C, const double* u, doubl
 
 // Grain-form z-marching stencil on one box (ela_grain.hpp): interior tiles, then the shell.
 namespace {
-template <typename T, int MODE, typename TO>
+template <typename T, int MODE, typename TO, bool SC>
 __global__ __launch_bounds__(MTX* MTY) void k_grain_march(GrainBox<T> g, double* partials) {
     __shared__ MarchLds<T> sh;
     extern __shared__ __align__(16) unsigned char dyn_lds[];
     const int t = g.xcd ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
-    const double m = march_tile<T, MODE, TO>(g, t, sh, (T*)dyn_lds);
+    const double m = march_tile<T, MODE, TO, SC>(g, t, sh, (T*)dyn_lds);
     if (MODE == 2 && partials) block_max_store(partials, m);  // a max: the slot order does not matter
 }
 
@@ -837,7 +837,8 @@ void launch_grain(GrainBox<T> g, int kz, double* partials, unsigned long long* n
     g.kz = kz;
     const int nm = ntx * nty * ntz;
     const size_t lds = (size_t)g.ngr * kGrainNC * sizeof(T);
-    hipLaunchKernelGGL((k_grain_march<T, MODE, TO>), dim3(nm), dim3(MTX * MTY), lds, s, g, partials);
+    if (g.sc) hipLaunchKernelGGL((k_grain_march<T, MODE, TO, true>), dim3(nm), dim3(MTX * MTY), lds, s, g, partials);
+    else hipLaunchKernelGGL((k_grain_march<T, MODE, TO, false>), dim3(nm), dim3(MTX * MTY), lds, s, g, partials);
     hipLaunchKernelGGL((k_grain_shell<T, MODE, TO>), dim3(nsh), dim3(256), 0, s, g, partials ? partials + nm : nullptr);
     if (MODE == 2 && norm) hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partials, nm + nsh, norm);
 }
