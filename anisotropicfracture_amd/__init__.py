@@ -165,6 +165,22 @@ SIGNATURES = {
     "ela_serve": ([_P, _I, ctypes.POINTER(Request), _D, _D, _I, _I, _D, ctypes.POINTER(Result),
                    ctypes.POINTER(ServeSummary)], _I),
     "ela_batch_smoother_times": ([_P, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_I)], _I),
+    # composite (AMR) solve
+    "ela_amr_create": ([_P, ctypes.POINTER(_Domain), _I, _P, _I, _I, _I, ctypes.POINTER(_P)], _I),
+    "ela_amr_destroy": ([_P], _I),
+    "ela_amr_bytes": ([_P], _SZ),
+    "ela_amr_level_info": ([_P, _I, ctypes.POINTER(_Layout), _P], _I),
+    "ela_amr_set_model_field": ([_P, _I, _P], _I),
+    "ela_amr_set_bc": ([_P, _P, _P], _I),
+    "ela_amr_set_uniform": ([_P, _I], _I),
+    "ela_amr_bc_init": ([_P, _I, _P, _I], _I),
+    "ela_amr_set_cycle": ([_P, _I, _I, _I], _I),
+    "ela_amr_set_krylov": ([_P, _I], _I),
+    "ela_amr_prepare": ([_P], _I),
+    "ela_amr_upload": ([_P, _I, _I, _P], _I),
+    "ela_amr_download": ([_P, _I, _I, _P], _I),
+    "ela_amr_residual": ([_P, ctypes.POINTER(_D)], _I),
+    "ela_amr_solve": ([_P, _D, _D, _I, ctypes.POINTER(Stats)], _I),
 }
 
 
@@ -534,6 +550,140 @@ def average_down(crse_sol, fine_sol, fine_origin):
     lc, lf, o = _pair_args(crse_sol, fine_sol, fine_origin)
     _check(lib().ela_average_down(ctypes.byref(lc), crse_sol.ptr, ctypes.byref(lf), fine_sol.ptr, _ptr(o),
                                   crse_sol.nc, crse_sol.prec, None))
+
+
+# ---------------------------------------------------------------- composite (AMR) solve
+class ElasticAMR:
+    """Operator::Elastic<T> defined on several AMR levels (Elastic.cpp:30-55) with the multi-level
+    Solver::Nonlocal::Linear::solve (Linear.H:102-112).
+
+    n0: level-0 cells per direction; boxes: [(lo, hi), ...] for levels 1.. (node indices of the
+    level, ratio 2, even corners).  Every per-level host array covers the level's box grown by 2
+    nodes and clipped to the domain (``shape(lev)``; ``origin(lev)`` is the global node index of
+    its first node); the outer nodes hold the interpolated coarser solution.
+    """
+
+    U, B, R = 0, 1, 2
+
+    def __init__(self, n0, boxes, kind, L=None, mode=SOLVE_MIXED, coarse_min=4, ctx=None):
+        self.ctx = ctx or default_context()
+        n0 = tuple(int(v) for v in n0)
+        self.dim = len(n0)
+        L = L if L is not None else (1.0,) * self.dim
+        d = _Domain()
+        d.dim = self.dim
+        for q in range(3):
+            d.n[q] = n0[q] if q < self.dim else 0
+            d.dx[q] = float(L[q]) / n0[q] if q < self.dim else 1.0
+        self.nlev = len(boxes) + 1
+        bx = np.zeros(6 * max(1, len(boxes)), dtype=np.int32)
+        for l, (lo, hi) in enumerate(boxes):
+            bx[6 * l:6 * l + self.dim] = lo
+            bx[6 * l + 3:6 * l + 3 + self.dim] = hi
+        p = ctypes.c_void_p()
+        _check(lib().ela_amr_create(self.ctx.ptr, ctypes.byref(d), self.nlev, _ptr(bx), kind, coarse_min, mode,
+                                    ctypes.byref(p)))
+        self.ptr = p
+        self.kind, self.mode = kind, mode
+        self.nc = ncomp(self.dim, kind)
+
+    def _info(self, lev):
+        lay = _Layout()
+        o = np.zeros(3, dtype=np.int32)
+        _check(lib().ela_amr_level_info(self.ptr, lev, ctypes.byref(lay), _ptr(o)))
+        return lay, o
+
+    def shape(self, lev, ncomp=None):
+        lay, _ = self._info(lev)
+        nodes = tuple(lay.nodes[q] for q in range(self.dim))
+        return ((self.dim if ncomp is None else ncomp),) + tuple(reversed(nodes))
+
+    def origin(self, lev):
+        return tuple(int(v) for v in self._info(lev)[1][:self.dim])
+
+    def bytes(self):
+        return lib().ela_amr_bytes(self.ptr)
+
+    def SetModel(self, lev, C):
+        C = np.ascontiguousarray(C, dtype=np.float64)
+        if C.shape != self.shape(lev, self.nc):
+            raise ValueError("model shape %s, expected %s" % (C.shape, self.shape(lev, self.nc)))
+        _check(lib().ela_amr_set_model_field(self.ptr, lev, _ptr(C)))
+
+    def SetBC(self, types, values=None):
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        v = None if values is None else np.ascontiguousarray(values, dtype=np.float64)
+        _check(lib().ela_amr_set_bc(self.ptr, _ptr(t), None if v is None else _ptr(v)))
+
+    def SetUniform(self, uniform):
+        _check(lib().ela_amr_set_uniform(self.ptr, int(bool(uniform))))
+
+    def bc_init(self, lev, rhs=None, homogeneous=False):
+        """BC::Operator::Elastic::Init on level lev's array (domain-boundary nodes take their values)."""
+        out = np.zeros(self.shape(lev)) if rhs is None else np.array(rhs, dtype=np.float64, copy=True)
+        _check(lib().ela_amr_bc_init(self.ptr, lev, _ptr(out), int(bool(homogeneous))))
+        return out
+
+    def coords(self, lev):
+        """Nodal coordinates of level lev's array, one array per direction."""
+        lay, o = self._info(lev)
+        axes = [(o[d] + np.arange(lay.nodes[d])) * lay.dx[d] for d in range(self.dim)]
+        return list(reversed(np.meshgrid(*reversed(axes), indexing="ij")))
+
+    def SetCycle(self, cycle=CYCLE_F, nu1=2, nu2=2):
+        _check(lib().ela_amr_set_cycle(self.ptr, cycle, nu1, nu2))
+
+    def SetKrylov(self, restart):
+        _check(lib().ela_amr_set_krylov(self.ptr, restart))
+
+    def prepareForSolve(self):
+        _check(lib().ela_amr_prepare(self.ptr))
+
+    def upload(self, field, lev, host):
+        a = np.ascontiguousarray(host, dtype=np.float64)
+        if a.shape != self.shape(lev):
+            raise ValueError("field shape %s, expected %s" % (a.shape, self.shape(lev)))
+        _check(lib().ela_amr_upload(self.ptr, lev, field, _ptr(a)))
+
+    def download(self, field, lev):
+        out = np.zeros(self.shape(lev))
+        _check(lib().ela_amr_download(self.ptr, lev, field, _ptr(out)))
+        return out
+
+    def residual(self, u, b):
+        """Composite residual (solutionResidual + reflux) of per-level u, b; (r per level, |r|inf)."""
+        for l in range(self.nlev):
+            self.upload(self.U, l, u[l])
+            self.upload(self.B, l, b[l])
+        nrm = ctypes.c_double()
+        _check(lib().ela_amr_residual(self.ptr, ctypes.byref(nrm)))
+        return [self.download(self.R, l) for l in range(self.nlev)], nrm.value
+
+    def solve_resident(self, tol_rel=1e-10, tol_abs=0.0, max_iter=200, raise_on_fail=True):
+        st = Stats()
+        rc = lib().ela_amr_solve(self.ptr, tol_rel, tol_abs, max_iter, ctypes.byref(st))
+        if rc != 0 and (raise_on_fail or rc != 4):
+            _check(rc)
+        return st.as_dict()
+
+    def solve(self, sol, rhs, tol_rel=1e-10, tol_abs=0.0, max_iter=200, raise_on_fail=True):
+        """Solve from the initial guess sol (per-level arrays); returns (sol per level, stats dict)."""
+        for l in range(self.nlev):
+            self.upload(self.U, l, sol[l])
+            self.upload(self.B, l, rhs[l])
+        st = self.solve_resident(tol_rel, tol_abs, max_iter, raise_on_fail)
+        return [self.download(self.U, l) for l in range(self.nlev)], st
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().ela_amr_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------- batched independent solves

@@ -164,6 +164,68 @@ def cfg2(n=256, grains=64, seed=SEED):
     return {"n": size, "seeds": seeds, "R": R, "C11": C11, "C12": C12, "C44": C44, "bc_type": t, "bc_value": v}
 
 
+# ------------------------------------------------------------------ AMR crack (composite solve)
+def cfg3_boxes(n=128):
+    """The refined boxes of the mode-I crack on an n^3 base (ratio 2, node indices of each level).
+
+    Level 1 covers |z - 0.5| <= 0.125, full x and y.  Level 2 covers |z - 0.5| <= 0.0625 and the
+    crack front |x - 0.4| <= 0.125, its x range rounded out to [0.25, 0.5625] so that the box spans a
+    whole number of 32-cell blocks at n = 128 (160 cells); full y.
+    """
+    l1 = ((0, 0, 3 * n // 4), (2 * n, 2 * n, 5 * n // 4))
+    l2 = ((n, 0, 7 * n // 4), (9 * n // 4, 4 * n, 9 * n // 4))
+    return [l1, l2]
+
+
+def crack_eta(X, Y, Z, width=0.01, z0=0.5, x_tip=0.4):
+    """eta = 0.5 (1 - tanh(d / width)), d the distance to the half-plane {z = z0, x < x_tip}."""
+    dx = np.maximum(X - x_tip, 0.0)
+    d = np.sqrt(dx * dx + (Z - z0) ** 2)
+    return 0.5 * (1.0 - np.tanh(d / width))
+
+
+def mode1_bcs(load=0.005):
+    """ZHI u_z = +load, ZLO u_z = -load (ZLO also pins u_x = u_y = 0 against rigid motion), the
+    lateral faces traction free; edges and corners follow the z face."""
+    zlo = {0: (DISP, 0.0), 1: (DISP, 0.0), 2: (DISP, -load)}
+    zhi = {0: (TRAC, 0.0), 1: (TRAC, 0.0), 2: (DISP, load)}
+    free = {0: (TRAC, 0.0), 1: (TRAC, 0.0), 2: (TRAC, 0.0)}
+    spec = {}
+    for f in FACES3:
+        parts = f.split("_")
+        spec[f] = zlo if "ZLO" in parts else (zhi if "ZHI" in parts else free)
+    return bc_tables(3, spec)
+
+
+G_LITERAL = (1.0, 0.0, 1e-4)          # eta^2 + 1e-4: soft bulk, stiff crack plane (contrast ~2.5e3)
+G_DAMAGE = (1.0, -2.0, 1.0 + 1e-4)    # (1 - eta)^2 + 1e-4: eta as damage (Degradable/Isotropic.H:78-79)
+
+
+def cfg3(n=128, g=G_DAMAGE, E=1.0, nu=0.3, width=0.01, load=0.005):
+    """3D AMR mode-I crack: base n^3 cells on [0,1]^3, 3 levels (ratio 2).
+
+    Stiffness g(eta) C_iso(E, nu) per node with g(eta) = a eta^2 + b eta + c.  The default is the
+    reference's convention, eta as damage: g = (1 - eta)^2 + 1e-4, the crack plane soft (g = 0.25 on
+    it) in a stiff bulk, a contrast of 4.  The formula eta^2 + 1e-4 read literally (G_LITERAL) makes
+    the bulk soft and the crack plane stiff, a contrast of about 2.5e3 across a layer thinner than a
+    cell; with the strong-form operator's grad(C) terms the Jacobi-smoothed multigrid then diverges
+    (already at a contrast of 26), in the oracle as on the device, so it is not a solvable workload.
+    """
+    lam, mu = lame(E, nu)
+    t, v = mode1_bcs(load)
+    return {"n": (n, n, n), "boxes": cfg3_boxes(n), "lame": (lam, mu), "g": tuple(g), "width": width,
+            "bc_type": t, "bc_value": v}
+
+
+def cfg3_model(cfg, X, Y, Z):
+    """(lambda, mu) planes of the degraded isotropic stiffness at the nodes (X, Y, Z)."""
+    a, b, c = cfg["g"]
+    eta = crack_eta(X, Y, Z, cfg["width"])
+    gg = a * eta * eta + b * eta + c
+    lam, mu = cfg["lame"]
+    return np.stack([lam * gg, mu * gg])
+
+
 # ------------------------------------------------------------------ batched RVEs
 RVE_SIDES = (32, 48, 64, 96)
 

@@ -1,0 +1,927 @@
+// ela_composite.hip -- composite (AMR) solve of the nodal elasticity problem
+// over a nest of boxes: Operator::Elastic<T> defined on several AMR levels
+// (src/Operator/Elastic.cpp:30-55) and solved by Solver::Nonlocal::Linear
+// (src/Solver/Nonlocal/Linear.H:102-112) with the reference's AMR pieces:
+// reflux (src/Operator/Operator.cpp:728-839), averageDownSolutionRHS
+// (:626-641), trilinear interpolation (:556-624), and fine ghost nodes that
+// hold the interpolated coarse solution (the ghost-node coarse/fine strategy
+// selected at Linear.H:23-29).
+//
+// Levels.  Level 0 is the domain.  Level l >= 1 is one box B_l of level-l
+// nodes [lo, hi] (even corners, 2 coarse cells inside its parent unless on the
+// domain boundary), stored on G_l = B_l grown by 2 nodes, clipped to the
+// domain.  Node classes of level l:
+//   slave     on a face of B_l away from the domain boundary;
+//   ghost     in G_l outside B_l;
+//   covered   strictly inside B_{l+1} / 2 (away from its coarse/fine faces);
+//   interface on a coarse/fine face of B_{l+1} / 2;
+//   unknown   every other node of B_l (interface nodes included).
+// Slaves and ghosts take the trilinear interpolation of level l-1; covered
+// nodes the coincident level-(l+1) value.  The equations are b - A_l u = 0 at
+// unknown nodes (A_l the single-box operator on G_l with the domain's boundary
+// rows on domain faces) and, at interface nodes, the reflux of level l+1's
+// residual = 0 -- the problem restated by the oracle (oracle/amr.py).
+//
+// Solver.  Flexible GMRES(m) in FP64 on the composite unknowns, right-
+// preconditioned by one multiplicative FAC cycle: from the finest level down,
+// one multigrid cycle of the level's box hierarchy (homogeneous Dirichlet on
+// its coarse/fine faces) on the level's residual, whose correction's residual
+// is restricted into the coarser level under the box; the level-0 hierarchy's
+// cycle; then upwards, the coarse correction interpolated onto the finer level
+// (ghosts and slaves included) and one more box cycle on the residual left.
+// Box cycles run in FP32 (mixed) or FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ela.h"
+#include "ela_hooks.hpp"
+#include "ela_internal.hpp"
+#include "ela_kernels.hpp"
+
+namespace ela {
+namespace {
+
+constexpr int CX = 64, CY = 4;
+enum { NODE_UNKNOWN = 0, NODE_FILL = 1, NODE_COVERED = 2 };
+
+// Geometry of one level (global node indices of the level).
+struct Geo {
+    int dim;
+    int glo[3];               // first node of G
+    int lo[3], hi[3];         // B
+    int N[3];                 // domain cells of the level
+    int cflo[3], cfhi[3];     // 1: that face of B is a coarse/fine face
+    int has_fine;
+    int flo[3], fhi[3];       // B_{l+1} / 2
+    int fflo[3], ffhi[3];     // 1: that face of B_{l+1} / 2 is a coarse/fine face
+};
+
+__device__ __forceinline__ int node_class(const Geo& g, const int p[3], bool* iface) {
+    *iface = false;
+    bool slave = false;
+    for (int d = 0; d < g.dim; ++d) {
+        if (p[d] < g.lo[d] || p[d] > g.hi[d]) return NODE_FILL;
+        slave = slave || (g.cflo[d] && p[d] == g.lo[d]) || (g.cfhi[d] && p[d] == g.hi[d]);
+    }
+    if (slave) return NODE_FILL;
+    if (g.has_fine) {
+        bool under = true, on = false;
+        for (int d = 0; d < g.dim; ++d) {
+            under = under && p[d] >= g.flo[d] && p[d] <= g.fhi[d];
+            on = on || (g.fflo[d] && p[d] == g.flo[d]) || (g.ffhi[d] && p[d] == g.fhi[d]);
+        }
+        if (under && !on) return NODE_COVERED;
+        *iface = under && on;
+    }
+    return NODE_UNKNOWN;
+}
+
+__device__ __forceinline__ long long at(const Lay& L, int i, int j, int k) { return i + L.px * j + L.pl * k; }
+
+// Fine level (layout Lf, geometry gf) from the coarser level (Lc, first node gclo):
+// mode 0 assigns the interpolation at slave and ghost nodes, mode 1 adds it at every node.
+template <typename T>
+__global__ __launch_bounds__(CX* CY) void k_fill(Lay Lf, Geo gf, Lay Lc, int gclo0, int gclo1, int gclo2,
+                                                 const T* __restrict__ crse, T* __restrict__ fine, int nc, int mode) {
+    const int i = blockIdx.x * CX + threadIdx.x, j = blockIdx.y * CY + threadIdx.y, k = blockIdx.z;
+    if (i >= Lf.nx || j >= Lf.ny) return;
+    const int p[3] = {gf.glo[0] + i, gf.glo[1] + j, gf.glo[2] + k};
+    if (mode == 0) {
+        bool iface;
+        if (node_class(gf, p, &iface) != NODE_FILL) return;
+    }
+    const long long oc = at(Lc, (p[0] >> 1) - gclo0, (p[1] >> 1) - gclo1, (p[2] >> 1) - gclo2);
+    const long long of = at(Lf, i, j, k);
+    for (int n = 0; n < nc; ++n) {
+        const T v = prolong_node<T>(Lc, p[0], p[1], p[2], crse + n * Lc.cs + oc);
+        if (mode == 0) fine[n * Lf.cs + of] = v;
+        else fine[n * Lf.cs + of] += v;
+    }
+}
+
+// Coarse nodes under B_{l+1} / 2 (coarse geometry gc) take the coincident fine value.
+template <typename T>
+__global__ __launch_bounds__(CX* CY) void k_inject(Lay Lc, Geo gc, Lay Lf, int gflo0, int gflo1, int gflo2,
+                                                   const T* __restrict__ fine, T* __restrict__ crse, int nc) {
+    const int a = blockIdx.x * CX + threadIdx.x, b = blockIdx.y * CY + threadIdx.y, c = blockIdx.z;
+    const int I = gc.flo[0] + a, J = gc.flo[1] + b, K = gc.flo[2] + c;
+    if (I > gc.fhi[0] || J > gc.fhi[1]) return;
+    const long long oc = at(Lc, I - gc.glo[0], J - gc.glo[1], K - gc.glo[2]);
+    const long long of = at(Lf, 2 * I - gflo0, 2 * J - gflo1, 2 * K - gflo2);
+    for (int n = 0; n < nc; ++n) crse[n * Lc.cs + oc] = fine[n * Lf.cs + of];
+}
+
+// Reflux into the coarse nodes under B_{l+1} / 2: the full weighting of the fine
+// field v (read on G_{l+1}), branches by the coarse domain (Operator.cpp:779-822).
+// mode 0: crse = R v.  mode 1 (a FAC down-sweep): crse = (interface ? crse : R v) - R w.
+template <typename T, int DIM>
+__global__ __launch_bounds__(CX* CY) void k_reflux_c(Lay Lc, Geo gc, Lay Lf, int gflo0, int gflo1, int gflo2,
+                                                     const T* __restrict__ v, const T* __restrict__ w,
+                                                     T* __restrict__ crse, int nc, int mode) {
+    const int a = blockIdx.x * CX + threadIdx.x, b = blockIdx.y * CY + threadIdx.y, c = blockIdx.z;
+    const int I = gc.flo[0] + a, J = gc.flo[1] + b, K = gc.flo[2] + c;
+    if (I > gc.fhi[0] || J > gc.fhi[1]) return;
+    Lay Ld = Lc;  // the branches test the coarse domain boundary
+    Ld.nx = gc.N[0] + 1;
+    Ld.ny = gc.N[1] + 1;
+    Ld.nz = DIM == 3 ? gc.N[2] + 1 : 1;
+    const int p[3] = {I, J, K};
+    bool iface = false;
+    (void)node_class(gc, p, &iface);
+    const long long oc = at(Lc, I - gc.glo[0], J - gc.glo[1], K - gc.glo[2]);
+    const int fx = 2 * I - gflo0, fy = 2 * J - gflo1, fz = 2 * K - gflo2;
+    for (int n = 0; n < nc; ++n) {
+        auto getw = [&](int di, int dj, int dk) -> T { return w[n * Lf.cs + at(Lf, fx + di, fy + dj, fz + dk)]; };
+        auto getv = [&](int di, int dj, int dk) -> T { return v[n * Lf.cs + at(Lf, fx + di, fy + dj, fz + dk)]; };
+        T out;
+        if (mode == 0) out = restrict_node<T, DIM>(Ld, I, J, K, getv);
+        else {
+            const T base = iface ? crse[n * Lc.cs + oc] : restrict_node<T, DIM>(Ld, I, J, K, getv);
+            out = base - restrict_node<T, DIM>(Ld, I, J, K, getw);
+        }
+        crse[n * Lc.cs + oc] = out;
+    }
+}
+
+// Zero every component at the nodes that are not composite unknowns.
+template <typename T>
+__global__ __launch_bounds__(CX* CY) void k_mask(Lay L, Geo g, T* __restrict__ x, int nc) {
+    const int i = blockIdx.x * CX + threadIdx.x, j = blockIdx.y * CY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    const int p[3] = {g.glo[0] + i, g.glo[1] + j, g.glo[2] + k};
+    bool iface;
+    if (node_class(g, p, &iface) == NODE_UNKNOWN) return;
+    for (int n = 0; n < nc; ++n) x[n * L.cs + at(L, i, j, k)] = T(0);
+}
+
+// G_l field -> right-hand side of the box hierarchy (zero on slaves: their rows
+// are homogeneous Dirichlet rows of the box problem).
+template <typename TO>
+__global__ __launch_bounds__(CX* CY) void k_to_box(Lay Lg, Geo g, const double* __restrict__ src, Lay Lb,
+                                                   TO* __restrict__ dst, int nc) {
+    const int i = blockIdx.x * CX + threadIdx.x, j = blockIdx.y * CY + threadIdx.y, k = blockIdx.z;
+    if (i >= Lb.nx || j >= Lb.ny) return;
+    const int p[3] = {g.lo[0] + i, g.lo[1] + j, g.lo[2] + k};
+    bool iface;
+    const bool slave = node_class(g, p, &iface) == NODE_FILL;
+    const long long os = at(Lg, p[0] - g.glo[0], p[1] - g.glo[1], p[2] - g.glo[2]);
+    for (int n = 0; n < nc; ++n) dst[n * Lb.cs + at(Lb, i, j, k)] = slave ? TO(0) : (TO)src[n * Lg.cs + os];
+}
+
+// Box correction -> G_l field on B_l (assign or add).
+template <typename TI>
+__global__ __launch_bounds__(CX* CY) void k_from_box(Lay Lb, const TI* __restrict__ src, Lay Lg, Geo g,
+                                                     double* __restrict__ dst, int nc, int add) {
+    const int i = blockIdx.x * CX + threadIdx.x, j = blockIdx.y * CY + threadIdx.y, k = blockIdx.z;
+    if (i >= Lb.nx || j >= Lb.ny) return;
+    const long long od = at(Lg, g.lo[0] + i - g.glo[0], g.lo[1] + j - g.glo[1], g.lo[2] + k - g.glo[2]);
+    for (int n = 0; n < nc; ++n) {
+        const double v = (double)src[n * Lb.cs + at(Lb, i, j, k)];
+        if (add) dst[n * Lg.cs + od] += v;
+        else dst[n * Lg.cs + od] = v;
+    }
+}
+
+// A box of G (stiffness planes) into the box hierarchy's level 0.
+__global__ __launch_bounds__(CX* CY) void k_subbox(Lay Ls, int ox, int oy, int oz, const double* __restrict__ src,
+                                                   Lay Ld, double* __restrict__ dst, int nc) {
+    const int i = blockIdx.x * CX + threadIdx.x, j = blockIdx.y * CY + threadIdx.y, k = blockIdx.z;
+    if (i >= Ld.nx || j >= Ld.ny) return;
+    for (int n = 0; n < nc; ++n) dst[n * Ld.cs + at(Ld, i, j, k)] = src[n * Ls.cs + at(Ls, i + ox, j + oy, k + oz)];
+}
+
+// max |x| over a flat array into *out (bit pattern of a non-negative double).
+__global__ __launch_bounds__(256) void k_maxabs(const double* __restrict__ x, long long n,
+                                                 unsigned long long* __restrict__ out) {
+    __shared__ double s[256];
+    double m = 0;
+    for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += (long long)gridDim.x * 256) {
+        const double a = fabs(x[q]);
+        m = (a > m || a != a) ? a : m;
+    }
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            const double o = s[threadIdx.x + st];
+            if (o > s[threadIdx.x] || o != o) s[threadIdx.x] = o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double r = s[0];
+        if (r != r) r = __builtin_huge_val();
+        atomicMax(out, (unsigned long long)__double_as_longlong(r));
+    }
+}
+
+// y += x over a flat array
+__global__ __launch_bounds__(256) void k_add_flat(double* __restrict__ y, const double* __restrict__ x, long long n) {
+    for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += (long long)gridDim.x * 256) y[q] += x[q];
+}
+
+dim3 grid_of(const Lay& L) { return dim3((L.nx + CX - 1) / CX, (L.ny + CY - 1) / CY, L.nz); }
+
+// Faces of BC class c (numbering of BC::Operator::Elastic::Face): side[d] = 0 (low),
+// 1 (high) or -1 (not on a face in direction d).
+void class_sides(int dim, int c, int side[3]) {
+    side[0] = side[1] = side[2] = -1;
+    if (dim == 2) {
+        static const int tab[8][2] = {{0, -1}, {-1, 0}, {1, -1}, {-1, 1}, {0, 0}, {0, 1}, {1, 0}, {1, 1}};
+        side[0] = tab[c][0];
+        side[1] = tab[c][1];
+        return;
+    }
+    if (c < 6) {
+        side[c % 3] = c < 3 ? 0 : 1;
+    } else if (c < 10) {
+        side[1] = ((c - 6) >> 1) & 1;
+        side[2] = (c - 6) & 1;
+    } else if (c < 14) {
+        side[2] = ((c - 10) >> 1) & 1;
+        side[0] = (c - 10) & 1;
+    } else if (c < 18) {
+        side[0] = ((c - 14) >> 1) & 1;
+        side[1] = (c - 14) & 1;
+    } else {
+        side[0] = ((c - 18) >> 2) & 1;
+        side[1] = ((c - 18) >> 1) & 1;
+        side[2] = (c - 18) & 1;
+    }
+}
+
+struct CLev {
+    Geo geo{};
+    Lay L{};
+    double h[3] = {1, 1, 1};
+    size_t off = 0;     // element offset of the level's segment in composite vectors
+    size_t elems = 0;   // dim * L.cs
+    BCTab bcG{};        // boundary rows of G (domain classes; Displacement on artificial faces)
+    double* C = nullptr;
+    bool model_set = false;
+    ela_hier* H = nullptr;
+    Lay Lb{};           // the box hierarchy's level 0
+};
+
+}  // namespace
+}  // namespace ela
+
+using namespace ela;
+
+struct ela_amr {
+    ela_ctx* ctx = nullptr;
+    int dim = 3, kind = 0, nc = 2, mode = 0, nlev = 1, cprec = ELA_F64;
+    int uniform = 0;
+    std::vector<CLev> lev;
+    BCTab bc{};
+    int bctypes[26][3] = {};
+    double bcval[26][3] = {};
+    hipStream_t stream = nullptr;
+    size_t ntot = 0;
+    double *u = nullptr, *b = nullptr, *r = nullptr, *z = nullptr, *w = nullptr, *t = nullptr;
+    int krylov = 8;
+    std::vector<double*> V, Z;
+    double *kdev = nullptr, *kpart = nullptr;
+    unsigned long long* dnorm = nullptr;
+    unsigned long long* hnorm = nullptr;
+    std::vector<std::pair<void*, size_t>> allocs;
+    size_t bytes = 0;
+    bool prepared = false;
+};
+
+namespace {
+
+int afail(int code, const std::string& m) { return set_error(code, m); }
+
+int aalloc(ela_amr* a, size_t bytes, void** out) {
+    *out = nullptr;
+    if (int rc = budget_reserve(a->ctx, bytes)) return rc;
+    if (hipMalloc(out, bytes) != hipSuccess) {
+        budget_release(a->ctx, bytes);
+        *out = nullptr;
+        return afail(ELA_ERR_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    if (hipMemsetAsync(*out, 0, bytes, a->stream) != hipSuccess) return afail(ELA_ERR_HIP, "hipMemsetAsync failed");
+    a->allocs.push_back({*out, bytes});
+    a->bytes += bytes;
+    return ELA_OK;
+}
+
+template <typename T>
+int aalloc_t(ela_amr* a, size_t count, T** out) {
+    void* p = nullptr;
+    int rc = aalloc(a, count * sizeof(T), &p);
+    *out = (T*)p;
+    return rc;
+}
+
+void afree_all(ela_amr* a) {
+    for (auto& pr : a->allocs) {
+        (void)hipFree(pr.first);
+        budget_release(a->ctx, pr.second);
+    }
+    a->allocs.clear();
+    a->bytes = 0;
+}
+
+double* seg(double* base, const CLev& v) { return base + v.off; }
+
+// slaves and ghosts of every finer level from the coarser one, then covered nodes
+// from the finer one
+void fill(ela_amr* a, double* x) {
+    for (int l = 1; l < a->nlev; ++l) {
+        CLev &f = a->lev[l], &c = a->lev[l - 1];
+        hipLaunchKernelGGL(k_fill<double>, grid_of(f.L), dim3(CX, CY), 0, a->stream, f.L, f.geo, c.L, c.geo.glo[0],
+                           c.geo.glo[1], c.geo.glo[2], (const double*)seg(x, c), seg(x, f), a->dim, 0);
+    }
+    for (int l = a->nlev - 1; l >= 1; --l) {
+        CLev &f = a->lev[l], &c = a->lev[l - 1];
+        const dim3 g((c.geo.fhi[0] - c.geo.flo[0] + CX) / CX, (c.geo.fhi[1] - c.geo.flo[1] + CY) / CY,
+                     a->dim == 3 ? c.geo.fhi[2] - c.geo.flo[2] + 1 : 1);
+        hipLaunchKernelGGL(k_inject<double>, g, dim3(CX, CY), 0, a->stream, c.L, c.geo, f.L, f.geo.glo[0],
+                           f.geo.glo[1], f.geo.glo[2], (const double*)seg(x, f), seg(x, c), a->dim);
+    }
+}
+
+void reflux(ela_amr* a, int l, const double* v, const double* w, double* crse, int mode) {
+    CLev &f = a->lev[l], &c = a->lev[l - 1];
+    const dim3 g((c.geo.fhi[0] - c.geo.flo[0] + CX) / CX, (c.geo.fhi[1] - c.geo.flo[1] + CY) / CY,
+                 a->dim == 3 ? c.geo.fhi[2] - c.geo.flo[2] + 1 : 1);
+    if (a->dim == 3)
+        hipLaunchKernelGGL((k_reflux_c<double, 3>), g, dim3(CX, CY), 0, a->stream, c.L, c.geo, f.L, f.geo.glo[0],
+                           f.geo.glo[1], f.geo.glo[2], v, w, crse, a->dim, mode);
+    else
+        hipLaunchKernelGGL((k_reflux_c<double, 2>), g, dim3(CX, CY), 0, a->stream, c.L, c.geo, f.L, f.geo.glo[0],
+                           f.geo.glo[1], f.geo.glo[2], v, w, crse, a->dim, mode);
+}
+
+void mask(ela_amr* a, double* x) {
+    for (auto& v : a->lev)
+        hipLaunchKernelGGL(k_mask<double>, grid_of(v.L), dim3(CX, CY), 0, a->stream, v.L, v.geo, seg(x, v), a->dim);
+}
+
+void maxabs(ela_amr* a, const double* x) {
+    launch_zero_u64(a->dnorm, 1, a->stream);
+    const long long n = (long long)a->ntot;
+    const int blocks = (int)std::min<long long>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(k_maxabs, dim3(blocks), dim3(256), 0, a->stream, x, n, a->dnorm);
+}
+
+double read_norm(ela_amr* a) {
+    (void)hipMemcpyAsync(a->hnorm, a->dnorm, sizeof(unsigned long long), hipMemcpyDeviceToHost, a->stream);
+    (void)hipStreamSynchronize(a->stream);
+    double v;
+    std::memcpy(&v, a->hnorm, sizeof v);
+    return v;
+}
+
+// r = composite residual of x (masked), |r|inf into dnorm
+void residual(ela_amr* a, double* x, const double* bb, double* r) {
+    fill(a, x);
+    for (auto& v : a->lev)
+        launch_residual<double, double>(v.L, a->kind, v.C, seg(x, v), bb + v.off, seg(r, v), (double*)nullptr,
+                                        nullptr, nullptr, nullptr, v.bcG, v.h, a->uniform, a->stream);
+    for (int l = a->nlev - 1; l >= 1; --l) reflux(a, l, seg(r, a->lev[l]), nullptr, seg(r, a->lev[l - 1]), 0);
+    mask(a, r);
+    maxabs(a, r);
+}
+
+// y = A_c x (masked)
+void apply(ela_amr* a, double* x, double* y) {
+    fill(a, x);
+    for (auto& v : a->lev)
+        launch_fapply<double>(v.L, a->kind, v.C, seg(x, v), seg(y, v), v.bcG, v.h, a->uniform, a->stream);
+    for (int l = a->nlev - 1; l >= 1; --l) reflux(a, l, seg(y, a->lev[l]), nullptr, seg(y, a->lev[l - 1]), 0);
+    mask(a, y);
+}
+
+// dst (+)= one box cycle on the level's residual src
+int box_correct(ela_amr* a, int l, const double* src, double* dst, int add) {
+    CLev& v = a->lev[l];
+    const dim3 gb = grid_of(v.Lb);
+    void* res = hier_res0(v.H, a->cprec);
+    if (a->cprec == ELA_F32)
+        hipLaunchKernelGGL(k_to_box<float>, gb, dim3(CX, CY), 0, a->stream, v.L, v.geo, src, v.Lb, (float*)res, a->dim);
+    else
+        hipLaunchKernelGGL(k_to_box<double>, gb, dim3(CX, CY), 0, a->stream, v.L, v.geo, src, v.Lb, (double*)res,
+                           a->dim);
+    if (int rc = hier_cycle(v.H, a->cprec)) return rc;
+    const void* cor = hier_cor0(v.H, a->cprec);
+    if (a->cprec == ELA_F32)
+        hipLaunchKernelGGL(k_from_box<float>, gb, dim3(CX, CY), 0, a->stream, v.Lb, (const float*)cor, v.L, v.geo,
+                           dst, a->dim, add);
+    else
+        hipLaunchKernelGGL(k_from_box<double>, gb, dim3(CX, CY), 0, a->stream, v.Lb, (const double*)cor, v.L, v.geo,
+                           dst, a->dim, add);
+    return ELA_OK;
+}
+
+// z = FAC(v): one multiplicative FAC cycle (t, w: work vectors)
+int fac(ela_amr* a, const double* v, double* z) {
+    hipStream_t s = a->stream;
+    const int L = a->nlev - 1;
+    if (hipMemsetAsync(z, 0, a->ntot * sizeof(double), s) != hipSuccess) return afail(ELA_ERR_HIP, "memset");
+    if (hipMemcpyAsync(a->t, v, a->ntot * sizeof(double), hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return afail(ELA_ERR_HIP, "copy");
+    for (int l = L; l >= 1; --l) {
+        CLev& f = a->lev[l];
+        if (int rc = box_correct(a, l, seg(a->t, f), seg(z, f), 0)) return rc;
+        launch_fapply<double>(f.L, a->kind, f.C, seg(z, f), seg(a->w, f), f.bcG, f.h, a->uniform, s);
+        reflux(a, l, seg(a->t, f), seg(a->w, f), seg(a->t, a->lev[l - 1]), 1);
+    }
+    if (int rc = box_correct(a, 0, seg(a->t, a->lev[0]), seg(z, a->lev[0]), 0)) return rc;
+    for (int l = 1; l <= L; ++l) {
+        CLev &f = a->lev[l], &c = a->lev[l - 1];
+        hipLaunchKernelGGL(k_fill<double>, grid_of(f.L), dim3(CX, CY), 0, s, f.L, f.geo, c.L, c.geo.glo[0],
+                           c.geo.glo[1], c.geo.glo[2], (const double*)seg(z, c), seg(z, f), a->dim, 1);
+        launch_residual<double, double>(f.L, a->kind, f.C, seg(z, f), seg(a->t, f), seg(a->w, f), (double*)nullptr,
+                                        nullptr, nullptr, nullptr, f.bcG, f.h, a->uniform, s);
+        if (int rc = box_correct(a, l, seg(a->w, f), seg(z, f), 1)) return rc;
+    }
+    return ELA_OK;
+}
+
+int krylov_alloc(ela_amr* a) {
+    const int m = a->krylov;
+    if ((int)a->Z.size() == m && (int)a->V.size() == m + 1) return ELA_OK;
+    if (m > kMaxKrylov - 1) return afail(ELA_ERR_BADARG, "restart too long");
+    a->V.clear();
+    a->Z.clear();
+    int rc = ELA_OK;
+    for (int q = 0; q <= m && !rc; ++q) {
+        double* p = nullptr;
+        rc = aalloc_t(a, a->ntot, &p);
+        a->V.push_back(p);
+    }
+    for (int q = 0; q < m && !rc; ++q) {
+        double* p = nullptr;
+        rc = aalloc_t(a, a->ntot, &p);
+        a->Z.push_back(p);
+    }
+    if (!rc && !a->kdev) rc = aalloc_t(a, 4 * (size_t)(kMaxKrylov + 2), &a->kdev);
+    if (!rc && !a->kpart) rc = aalloc_t(a, (size_t)mdot_partials(kMaxKrylov), &a->kpart);
+    return rc;
+}
+
+std::vector<double> host_vec(ela_amr* a, int count) {
+    std::vector<double> out(count);
+    (void)hipMemcpyAsync(out.data(), a->kdev, sizeof(double) * count, hipMemcpyDeviceToHost, a->stream);
+    (void)hipStreamSynchronize(a->stream);
+    return out;
+}
+
+// Right-preconditioned flexible GMRES(m) on the composite unknowns (classical
+// Gram-Schmidt with a second pass when the new vector lost most of its norm);
+// m = 0 runs the plain FAC iteration u += FAC(r).
+int solve(ela_amr* a, double tol_rel, double tol_abs, int max_iter, ela_stats* st) {
+    hipStream_t s = a->stream;
+    const long long n = (long long)a->ntot;
+    residual(a, a->u, a->b, a->r);
+    double rn = read_norm(a);
+    const double r0 = rn;
+    if (!(r0 == r0)) return afail(ELA_ERR_NAN, "initial residual is not finite");
+    const double target = std::max(tol_abs, std::max(tol_rel, 1e-16) * r0);
+    int total = 0, status = ELA_OK, restarts = 0;
+    double best = r0;
+    const int m = a->krylov;
+    if (m > 0)
+        if (int rc = krylov_alloc(a)) return rc;
+    std::vector<double> H((size_t)(m + 1) * std::max(m, 1)), cs(m), sn(m), g(m + 1);
+    while (rn > target) {
+        if (total >= max_iter) {
+            status = afail(ELA_ERR_NONCONVERGED, "max_iter reached at " + std::to_string(rn / r0) + " relative");
+            break;
+        }
+        if (m == 0) {
+            if (int rc = fac(a, a->r, a->z)) return rc;
+            hipLaunchKernelGGL(k_add_flat, dim3((int)std::min<long long>(4096, (n + 255) / 256)), dim3(256), 0, s,
+                               a->u, (const double*)a->z, n);
+            ++total;
+        } else {
+            VecList one{};
+            one.v[0] = a->r;
+            launch_mdot(a->r, one, 1, n, a->kpart, a->kdev, s);
+            const double beta = std::sqrt(host_vec(a, 1)[0]);
+            if (!(beta > 0)) break;
+            launch_scale_flat(a->V[0], a->r, 1.0 / beta, n, s);
+            std::fill(g.begin(), g.end(), 0.0);
+            g[0] = beta;
+            int jend = 0;
+            for (int j = 0; j < m && total < max_iter; ++j) {
+                if (int rc = fac(a, a->V[j], a->Z[j])) return rc;
+                apply(a, a->Z[j], a->V[j + 1]);
+                VecList Vl{}, self{};
+                for (int q = 0; q <= j; ++q) Vl.v[q] = a->V[q];
+                self.v[0] = a->V[j + 1];
+                launch_mdot(a->V[j + 1], self, 1, n, a->kpart, a->kdev, s);
+                const double wn0 = std::sqrt(host_vec(a, 1)[0]);
+                launch_mdot(a->V[j + 1], Vl, j + 1, n, a->kpart, a->kdev, s);
+                launch_maxpy(a->V[j + 1], Vl, j + 1, a->kdev, -1.0, n, s);
+                std::vector<double> h1 = host_vec(a, j + 1), h2(j + 1, 0.0);
+                double hn2 = wn0 * wn0;
+                for (int q = 0; q <= j; ++q) hn2 -= h1[q] * h1[q];
+                if (!(hn2 > 0.5 * wn0 * wn0)) {
+                    launch_mdot(a->V[j + 1], Vl, j + 1, n, a->kpart, a->kdev, s);
+                    launch_maxpy(a->V[j + 1], Vl, j + 1, a->kdev, -1.0, n, s);
+                    h2 = host_vec(a, j + 1);
+                }
+                launch_mdot(a->V[j + 1], self, 1, n, a->kpart, a->kdev, s);
+                const double hn = std::sqrt(host_vec(a, 1)[0]);
+                ++total;
+                for (int q = 0; q <= j; ++q) H[(size_t)q * m + j] = h1[q] + h2[q];
+                H[(size_t)(j + 1) * m + j] = hn;
+                if (hn > 0) launch_scale_flat(a->V[j + 1], a->V[j + 1], 1.0 / hn, n, s);
+                for (int q = 0; q < j; ++q) {
+                    const double x = H[(size_t)q * m + j], y = H[(size_t)(q + 1) * m + j];
+                    H[(size_t)q * m + j] = cs[q] * x + sn[q] * y;
+                    H[(size_t)(q + 1) * m + j] = -sn[q] * x + cs[q] * y;
+                }
+                const double x = H[(size_t)j * m + j], y = H[(size_t)(j + 1) * m + j];
+                const double rr = std::hypot(x, y);
+                cs[j] = rr > 0 ? x / rr : 1.0;
+                sn[j] = rr > 0 ? y / rr : 0.0;
+                H[(size_t)j * m + j] = rr;
+                H[(size_t)(j + 1) * m + j] = 0.0;
+                g[j + 1] = -sn[j] * g[j];
+                g[j] = cs[j] * g[j];
+                jend = j + 1;
+                const double est = std::fabs(g[j + 1]);
+                if (!(est == est)) break;
+                if (est <= 0.5 * target || hn == 0) break;
+            }
+            std::vector<double> y(jend);
+            for (int q = jend - 1; q >= 0; --q) {
+                double acc = g[q];
+                for (int k = q + 1; k < jend; ++k) acc -= H[(size_t)q * m + k] * y[k];
+                y[q] = acc / H[(size_t)q * m + q];
+            }
+            if (jend > 0) {
+                ELA_HIPCHK(hipMemcpyAsync(a->kdev, y.data(), sizeof(double) * jend, hipMemcpyHostToDevice, s));
+                VecList Zl{};
+                for (int q = 0; q < jend; ++q) Zl.v[q] = a->Z[q];
+                launch_maxpy(a->u, Zl, jend, a->kdev, 1.0, n, s);
+            }
+        }
+        residual(a, a->u, a->b, a->r);
+        rn = read_norm(a);
+        ++restarts;
+        if (!(rn == rn)) { status = afail(ELA_ERR_NAN, "residual became non-finite"); break; }
+        if (rn <= target) break;
+        if (rn < 0.5 * best) best = rn;
+        else if (restarts >= (m == 0 ? 12 : 4) && rn > 0.9 * best) {
+            status = afail(ELA_ERR_NONCONVERGED, "residual stagnated at " + std::to_string(rn / r0) + " relative");
+            break;
+        }
+    }
+    fill(a, a->u);
+    if (hipStreamSynchronize(s) != hipSuccess) return afail(ELA_ERR_HIP, "composite solve failed");
+    if (st) {
+        st->iters = total;
+        st->converged = rn <= target;
+        st->res0 = r0;
+        st->res = rn;
+        st->mg_levels = a->nlev;
+        st->mode = a->mode;
+    }
+    return status;
+}
+
+int check_lev(const ela_amr* a, int l) {
+    if (!a) return afail(ELA_ERR_BADARG, "null composite");
+    if (l < 0 || l >= a->nlev) return afail(ELA_ERR_BADARG, "level out of range");
+    return ELA_OK;
+}
+
+// dense host array (nc planes over G_l) <-> the level's layout
+int xfer(ela_amr* a, const CLev& v, int nc, const double* host, double* dev, double* host_out, const double* dev_in) {
+    const size_t nn = (size_t)v.L.nx * v.L.ny * v.L.nz;
+    void* stage = nullptr;
+    if (int rc = aalloc(a, nn * nc * sizeof(double), &stage)) return rc;
+    hipError_t e = hipSuccess;
+    if (host) {
+        e = hipMemcpyAsync(stage, host, nn * nc * sizeof(double), hipMemcpyHostToDevice, a->stream);
+        if (e == hipSuccess) launch_pack<double>(v.L, nc, (const double*)stage, dev, 1, a->stream);
+    } else {
+        launch_pack<double>(v.L, nc, (const double*)stage, (double*)dev_in, 0, a->stream);
+        e = hipMemcpyAsync(host_out, stage, nn * nc * sizeof(double), hipMemcpyDeviceToHost, a->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(a->stream);
+    (void)hipFree(stage);
+    budget_release(a->ctx, nn * nc * sizeof(double));
+    a->allocs.pop_back();
+    a->bytes -= nn * nc * sizeof(double);
+    if (e != hipSuccess) return afail(ELA_ERR_HIP, std::string("transfer: ") + hipGetErrorString(e));
+    return ELA_OK;
+}
+
+double* field_of(ela_amr* a, int which) {
+    switch (which) {
+        case ELA_AMR_U: return a->u;
+        case ELA_AMR_B: return a->b;
+        case ELA_AMR_R: return a->r;
+        default: return nullptr;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ela_amr_create(ela_ctx* ctx, const ela_domain* dom, int nlev, const int* boxes, int model_kind, int coarse_min,
+                   int mode, ela_amr** out) {
+    if (!ctx || !dom || !out || (nlev > 1 && !boxes)) return afail(ELA_ERR_BADARG, "null argument");
+    *out = nullptr;
+    if (dom->dim != 2 && dom->dim != 3) return afail(ELA_ERR_BADARG, "dim must be 2 or 3");
+    if (nlev < 1 || nlev > 8) return afail(ELA_ERR_BADARG, "1 to 8 levels");
+    if (mode != ELA_SOLVE_FP64 && mode != ELA_SOLVE_MIXED) return afail(ELA_ERR_BADARG, "mode must be FP64 or MIXED");
+    if (model_kind < 0 || model_kind > 2) return afail(ELA_ERR_BADARG, "unknown model kind");
+    const int dim = dom->dim;
+    // geometry and nesting
+    std::vector<Geo> geo(nlev);
+    for (int l = 0; l < nlev; ++l) {
+        Geo& g = geo[l];
+        std::memset(&g, 0, sizeof g);
+        g.dim = dim;
+        for (int d = 0; d < 3; ++d) {
+            g.N[d] = d < dim ? dom->n[d] << l : 0;
+            if (l == 0) {
+                g.lo[d] = 0;
+                g.hi[d] = g.N[d];
+            } else {
+                g.lo[d] = d < dim ? boxes[6 * (l - 1) + d] : 0;
+                g.hi[d] = d < dim ? boxes[6 * (l - 1) + 3 + d] : 0;
+            }
+        }
+        for (int d = 0; d < dim; ++d) {
+            if (dom->n[d] < 2 || !(dom->dx[d] > 0)) return afail(ELA_ERR_BADARG, "bad level-0 domain");
+            if (l == 0) continue;
+            const int lo = g.lo[d], hi = g.hi[d], N = g.N[d];
+            const int clo = geo[l - 1].lo[d], chi = geo[l - 1].hi[d], cN = geo[l - 1].N[d];
+            if (lo % 2 || hi % 2 || lo < 0 || hi <= lo || hi > N)
+                return afail(ELA_ERR_BADARG, "level " + std::to_string(l) + ": box corners must be even and inside the domain");
+            if ((lo > 0 && lo / 2 < clo + 2) || (hi < N && hi / 2 > chi - 2))
+                return afail(ELA_ERR_BADARG, "level " + std::to_string(l) + ": box must sit 2 coarse cells inside its parent");
+            if ((lo == 0 && clo != 0) || (hi == N && chi != cN))
+                return afail(ELA_ERR_BADARG, "level " + std::to_string(l) + ": box on the domain boundary needs a parent there");
+            g.cflo[d] = lo > 0;
+            g.cfhi[d] = hi < N;
+        }
+        for (int d = 0; d < 3; ++d) g.glo[d] = (l == 0 || d >= dim) ? 0 : std::max(g.lo[d] - 2, 0);
+    }
+    for (int l = 0; l + 1 < nlev; ++l) {
+        Geo &g = geo[l], &f = geo[l + 1];
+        g.has_fine = 1;
+        for (int d = 0; d < dim; ++d) {
+            g.flo[d] = f.lo[d] / 2;
+            g.fhi[d] = f.hi[d] / 2;
+            g.fflo[d] = f.cflo[d];
+            g.ffhi[d] = f.cfhi[d];
+        }
+    }
+    ela_amr* a = new ela_amr();
+    a->ctx = ctx;
+    a->dim = dim;
+    a->kind = model_kind;
+    a->nc = model_kind == ELA_MODEL_ISO ? 2 : (model_kind == ELA_MODEL_MM ? (dim == 3 ? 21 : 6) : dim * dim);
+    a->mode = mode;
+    a->cprec = mode == ELA_SOLVE_MIXED ? ELA_F32 : ELA_F64;
+    a->nlev = nlev;
+    a->lev.resize(nlev);
+    int rc = ELA_OK;
+    if (hipSetDevice(ctx->device) != hipSuccess) rc = afail(ELA_ERR_HIP, "hipSetDevice failed");
+    size_t off = 0;
+    for (int l = 0; l < nlev && !rc; ++l) {
+        CLev& v = a->lev[l];
+        v.geo = geo[l];
+        for (int d = 0; d < 3; ++d) v.h[d] = d < dim ? dom->dx[d] / (double)(1 << l) : 1.0;
+        const int gh[3] = {l == 0 ? v.geo.N[0] : std::min(v.geo.hi[0] + 2, v.geo.N[0]),
+                           l == 0 ? v.geo.N[1] : std::min(v.geo.hi[1] + 2, v.geo.N[1]),
+                           dim == 3 ? (l == 0 ? v.geo.N[2] : std::min(v.geo.hi[2] + 2, v.geo.N[2])) : 0};
+        v.L.dim = dim;
+        v.L.nx = gh[0] - v.geo.glo[0] + 1;
+        v.L.ny = gh[1] - v.geo.glo[1] + 1;
+        v.L.nz = dim == 3 ? gh[2] - v.geo.glo[2] + 1 : 1;
+        v.L.px = (v.L.nx + 31) / 32 * 32;
+        v.L.pl = v.L.px * v.L.ny;
+        v.L.cs = (v.L.pl * v.L.nz + 255) / 256 * 256;
+        v.elems = (size_t)dim * v.L.cs;
+        v.off = off;
+        off += v.elems;
+        // the level's box hierarchy (its own stream until the composite's is shared)
+        ela_domain bd{};
+        bd.dim = dim;
+        for (int d = 0; d < 3; ++d) {
+            bd.n[d] = d < dim ? v.geo.hi[d] - v.geo.lo[d] : 0;
+            bd.dx[d] = v.h[d];
+        }
+        rc = ela_hier_create_ex(ctx, &bd, model_kind, 0, l == 0 ? std::max(coarse_min, 2) : 2, mode, &v.H);
+        if (!rc) rc = hier_level0(v.H, &v.Lb);
+    }
+    if (!rc) {
+        a->stream = hier_stream(a->lev[0].H);
+        for (int l = 1; l < nlev && !rc; ++l) rc = hier_share_stream(a->lev[l].H, a->stream);
+    }
+    a->ntot = off;
+    for (int l = 0; l < nlev && !rc; ++l) rc = aalloc_t(a, (size_t)a->nc * a->lev[l].L.cs, &a->lev[l].C);
+    for (double** p : {&a->u, &a->b, &a->r, &a->z, &a->w, &a->t})
+        if (!rc) rc = aalloc_t(a, a->ntot, p);
+    if (!rc) rc = aalloc_t(a, 4, &a->dnorm);
+    if (!rc && hipHostMalloc((void**)&a->hnorm, 64) != hipSuccess) rc = afail(ELA_ERR_OOM, "hipHostMalloc failed");
+    if (!rc && a->stream && hipStreamSynchronize(a->stream) != hipSuccess) rc = afail(ELA_ERR_HIP, "zeroing failed");
+    if (rc) {
+        std::string keep = ela_last_error();
+        ela_amr_destroy(a);
+        set_error(rc, keep);
+        return rc;
+    }
+    // default BCs: Displacement everywhere (BC/Operator/Elastic.H:46-55)
+    int types[26 * 3] = {};
+    ela_amr_set_bc(a, types, nullptr);
+    *out = a;
+    return ELA_OK;
+}
+
+int ela_amr_destroy(ela_amr* a) {
+    if (!a) return ELA_OK;
+    if (a->stream) (void)hipStreamSynchronize(a->stream);
+    afree_all(a);
+    if (a->hnorm) (void)hipHostFree(a->hnorm);
+    for (int l = a->nlev - 1; l >= 0; --l)  // level 0 owns the shared stream
+        if (l < (int)a->lev.size() && a->lev[l].H) ela_hier_destroy(a->lev[l].H);
+    delete a;
+    return ELA_OK;
+}
+
+size_t ela_amr_bytes(const ela_amr* a) {
+    if (!a) return 0;
+    size_t s = a->bytes;
+    for (auto& v : a->lev) s += ela_hier_bytes(v.H);
+    return s;
+}
+
+int ela_amr_level_info(const ela_amr* a, int lev, ela_level_layout* out, int origin[3]) {
+    if (int rc = check_lev(a, lev)) return rc;
+    if (!out) return afail(ELA_ERR_BADARG, "null output");
+    const CLev& v = a->lev[lev];
+    out->dim = a->dim;
+    out->nodes[0] = v.L.nx;
+    out->nodes[1] = v.L.ny;
+    out->nodes[2] = v.L.nz;
+    for (int d = 0; d < 3; ++d) {
+        out->n[d] = d < a->dim ? out->nodes[d] - 1 : 0;
+        out->dx[d] = v.h[d];
+        if (origin) origin[d] = v.geo.glo[d];
+    }
+    out->pitch = v.L.px;
+    out->plane = v.L.pl;
+    out->cstride = v.L.cs;
+    return ELA_OK;
+}
+
+int ela_amr_set_model_field(ela_amr* a, int lev, const double* C_host) {
+    if (int rc = check_lev(a, lev)) return rc;
+    if (!C_host) return afail(ELA_ERR_BADARG, "null model");
+    CLev& v = a->lev[lev];
+    if (int rc = xfer(a, v, a->nc, C_host, v.C, nullptr, nullptr)) return rc;
+    v.model_set = true;
+    a->prepared = false;
+    return ELA_OK;
+}
+
+int ela_amr_set_bc(ela_amr* a, const int* types, const double* values) {
+    if (!a || !types) return afail(ELA_ERR_BADARG, "null argument");
+    const int nf = a->dim == 3 ? 26 : 8;
+    for (int f = 0; f < nf; ++f)
+        for (int c = 0; c < a->dim; ++c)
+            if (types[3 * f + c] < 0 || types[3 * f + c] > 3) return afail(ELA_ERR_BADARG, "unknown boundary type");
+    std::memset(&a->bc, 0, sizeof a->bc);
+    for (int f = 0; f < nf; ++f)
+        for (int c = 0; c < 3; ++c) {
+            a->bctypes[f][c] = c < a->dim ? types[3 * f + c] : 0;
+            a->bcval[f][c] = (values && c < a->dim) ? values[3 * f + c] : 0.0;
+            a->bc.t[f][c] = (unsigned char)a->bctypes[f][c];
+        }
+    // per level: the table of G (artificial faces -> Displacement) and of the box
+    // hierarchy (coarse/fine faces -> homogeneous Displacement)
+    for (int l = 0; l < a->nlev; ++l) {
+        CLev& v = a->lev[l];
+        std::vector<int> bt(26 * 3, 0);
+        v.bcG = a->bc;
+        for (int f = 0; f < nf; ++f) {
+            int side[3];
+            class_sides(a->dim, f, side);
+            bool art = false, cf = false;
+            for (int d = 0; d < a->dim; ++d) {
+                if (side[d] < 0) continue;
+                const int gend = side[d] ? v.geo.glo[d] + v.L.nx * (d == 0) + v.L.ny * (d == 1) + v.L.nz * (d == 2) - 1
+                                         : v.geo.glo[d];
+                art = art || (side[d] ? gend < v.geo.N[d] : gend > 0);
+                cf = cf || (side[d] ? v.geo.cfhi[d] : v.geo.cflo[d]);
+            }
+            for (int c = 0; c < 3; ++c) {
+                if (art) v.bcG.t[f][c] = ELA_BC_DISP;
+                bt[3 * f + c] = cf ? ELA_BC_DISP : a->bctypes[f][c];
+            }
+        }
+        if (int rc = ela_set_bc(v.H, bt.data(), nullptr)) return rc;
+    }
+    a->prepared = false;
+    return ELA_OK;
+}
+
+int ela_amr_bc_init(ela_amr* a, int lev, double* rhs, int homogeneous) {
+    if (int rc = check_lev(a, lev)) return rc;
+    if (!rhs) return afail(ELA_ERR_BADARG, "null rhs");
+    const CLev& v = a->lev[lev];
+    const Geo& g = v.geo;
+    const long long nn = (long long)v.L.nx * v.L.ny * v.L.nz;
+    for (int k = 0; k < v.L.nz; ++k)
+        for (int j = 0; j < v.L.ny; ++j)
+            for (int i = 0; i < v.L.nx; ++i) {
+                const int px = g.glo[0] + i, py = g.glo[1] + j, pz = g.glo[2] + k;
+                const int face = a->dim == 3 ? face_of<3>(px, py, pz, g.N[0], g.N[1], g.N[2])
+                                             : face_of<2>(px, py, 0, g.N[0], g.N[1], 0);
+                if (face < 0) continue;
+                const long long id = i + (long long)v.L.nx * (j + (long long)v.L.ny * k);
+                for (int d = 0; d < a->dim; ++d)
+                    rhs[d * nn + id] = (homogeneous && a->bctypes[face][d] == ELA_BC_DISP) ? 0.0 : a->bcval[face][d];
+            }
+    return ELA_OK;
+}
+
+int ela_amr_set_uniform(ela_amr* a, int uniform) {
+    if (!a) return afail(ELA_ERR_BADARG, "null composite");
+    a->uniform = uniform ? 1 : 0;
+    for (auto& v : a->lev) ela_set_uniform(v.H, a->uniform);
+    a->prepared = false;
+    return ELA_OK;
+}
+
+int ela_amr_set_krylov(ela_amr* a, int restart) {
+    if (!a) return afail(ELA_ERR_BADARG, "null composite");
+    if (restart < 0 || restart > kMaxKrylov - 1) return afail(ELA_ERR_BADARG, "restart out of range");
+    a->krylov = restart;
+    return ELA_OK;
+}
+
+int ela_amr_set_cycle(ela_amr* a, int cycle, int nu1, int nu2) {
+    if (!a) return afail(ELA_ERR_BADARG, "null composite");
+    for (auto& v : a->lev)
+        if (int rc = ela_set_cycle(v.H, cycle, nu1, nu2)) return rc;
+    return ELA_OK;
+}
+
+int ela_amr_prepare(ela_amr* a) {
+    if (!a) return afail(ELA_ERR_BADARG, "null composite");
+    for (int l = 0; l < a->nlev; ++l)
+        if (!a->lev[l].model_set)
+            return afail(ELA_ERR_STATE, "Attempting to use operator before calling SetModel (level " + std::to_string(l) + ")");
+    for (auto& v : a->lev) {
+        hipLaunchKernelGGL(k_subbox, grid_of(v.Lb), dim3(CX, CY), 0, a->stream, v.L, v.geo.lo[0] - v.geo.glo[0],
+                           v.geo.lo[1] - v.geo.glo[1], v.geo.lo[2] - v.geo.glo[2], (const double*)v.C, v.Lb,
+                           hier_C0(v.H), a->nc);
+        hier_model_set(v.H);
+        if (int rc = ela_prepare(v.H)) return rc;
+    }
+    ELA_HIPCHK(hipStreamSynchronize(a->stream));
+    a->prepared = true;
+    return ELA_OK;
+}
+
+int ela_amr_upload(ela_amr* a, int lev, int field, const double* host) {
+    if (int rc = check_lev(a, lev)) return rc;
+    double* f = field_of(a, field);
+    if (!f || !host) return afail(ELA_ERR_BADARG, "bad field or null host array");
+    return xfer(a, a->lev[lev], a->dim, host, seg(f, a->lev[lev]), nullptr, nullptr);
+}
+
+int ela_amr_download(ela_amr* a, int lev, int field, double* host) {
+    if (int rc = check_lev(a, lev)) return rc;
+    double* f = field_of(a, field);
+    if (!f || !host) return afail(ELA_ERR_BADARG, "bad field or null host array");
+    return xfer(a, a->lev[lev], a->dim, nullptr, nullptr, host, seg(f, a->lev[lev]));
+}
+
+int ela_amr_residual(ela_amr* a, double* norm_inf) {
+    if (!a) return afail(ELA_ERR_BADARG, "null composite");
+    if (!a->prepared) return afail(ELA_ERR_STATE, "call ela_amr_prepare first");
+    residual(a, a->u, a->b, a->r);
+    const double v = read_norm(a);
+    if (norm_inf) *norm_inf = v;
+    return ELA_OK;
+}
+
+int ela_amr_solve(ela_amr* a, double tol_rel, double tol_abs, int max_iter, ela_stats* st) {
+    if (!a) return afail(ELA_ERR_BADARG, "null composite");
+    if (!a->prepared) return afail(ELA_ERR_STATE, "call ela_amr_prepare first");
+    if (max_iter < 1) return afail(ELA_ERR_BADARG, "max_iter must be >= 1");
+    if (hipSetDevice(a->ctx->device) != hipSuccess) return afail(ELA_ERR_HIP, "hipSetDevice failed");
+    return solve(a, tol_rel, tol_abs, max_iter, st);
+}
+
+}  // extern "C"

@@ -135,7 +135,7 @@ __host__ __device__ constexpr int mm_slot(int i, int j, int k, int l) {
 // ZX and XY edges, then faces; BC/Operator/Elastic.H:239-278).  Returns -1 for
 // interior nodes.
 template <int DIM>
-__device__ __forceinline__ int face_of(int i, int j, int k, int hx, int hy, int hz) {
+__host__ __device__ __forceinline__ int face_of(int i, int j, int k, int hx, int hy, int hz) {
     const bool xl = i == 0, xh = i == hx, yl = j == 0, yh = j == hy;
     if (DIM == 2) {
         if (xl && yl) return 4;

@@ -30,6 +30,7 @@
 #include "../../include/ela.h"
 #include "ela_internal.hpp"
 #include "ela_kernels.hpp"
+#include "ela_hooks.hpp"
 #include "ela_march.hpp"
 
 using namespace ela;
@@ -110,6 +111,7 @@ struct ela_hier {
     int uniform = 0;
     bool model_set = false, prepared = false;
     hipStream_t stream = nullptr;
+    bool own_stream = true;  // false once a composite solve shares its stream with the hierarchy
     std::vector<std::pair<void*, size_t>> allocs;
     size_t bytes = 0;
     // level-0 solve fields (FP64)
@@ -648,6 +650,61 @@ void A_residual(ela_hier* h, Level& v, const T* C, const T* x, const T* b, T* r,
 
 }  // namespace
 
+// ====================================================================== composite-solve hooks
+namespace ela {
+
+int hier_level0(ela_hier* h, Lay* L) {
+    if (!h || !L) return fail(ELA_ERR_BADARG, "null argument");
+    *L = h->lev[0].L;
+    return ELA_OK;
+}
+
+void* hier_res0(ela_hier* h, int prec) {
+    Level& v = h->lev[0];
+    return prec == ELA_F32 ? (void*)v.res32 : (void*)v.res64;
+}
+
+void* hier_cor0(ela_hier* h, int prec) {
+    Level& v = h->lev[0];
+    return prec == ELA_F32 ? (void*)v.cor32 : (void*)v.cor64;
+}
+
+int hier_cycle(ela_hier* h, int prec) {
+    if (!h) return fail(ELA_ERR_BADARG, "null hierarchy");
+    if (!h->prepared) return fail(ELA_ERR_STATE, "call ela_prepare first");
+    if (!h->M) return fail(ELA_ERR_STATE, "the box does not coarsen to a dense-solvable level");
+    if (prec == ELA_F32) {
+        if (!h->has32) return fail(ELA_ERR_STATE, "no FP32 levels");
+        one_cycle<float>(h);
+    } else {
+        if (!h->has64) return fail(ELA_ERR_STATE, "no FP64 levels");
+        one_cycle<double>(h);
+    }
+    return ELA_OK;
+}
+
+int hier_share_stream(ela_hier* h, hipStream_t s) {
+    if (!h || !s) return fail(ELA_ERR_BADARG, "null argument");
+    if (h->stream == s) return ELA_OK;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (h->own_stream) HIPCHK(hipStreamDestroy(h->stream));
+    h->stream = s;
+    h->own_stream = false;
+    return ELA_OK;
+}
+
+hipStream_t hier_stream(const ela_hier* h) { return h->stream; }
+
+double* hier_C0(ela_hier* h) { return h->lev[0].C64; }
+
+void hier_model_set(ela_hier* h) {
+    h->has_table = false;
+    h->model_set = true;
+    h->prepared = false;
+}
+
+}  // namespace ela
+
 // ====================================================================== C ABI
 extern "C" {
 
@@ -794,7 +851,7 @@ int ela_hier_destroy(ela_hier* h) {
     for (auto& e : h->ev) (void)hipEventDestroy(e);
     while (!h->allocs.empty()) hfree(h, h->allocs.back().first);
     if (h->hnorm) (void)hipHostFree(h->hnorm);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return ELA_OK;
 }

@@ -154,6 +154,51 @@ def batched_rves(count, sample, tol):
     return out
 
 
+def amr_crack(n, steps, tol, krylov):
+    """Config 3: the mode-I crack on a 3-level AMR nest (composite solve), mixed and FP64.
+
+    A step is prepareForSolve + one composite solve from u = 0 with b and the model resident."""
+    import anisotropicfracture_amd as ela
+    from anisotropicfracture_amd import configs as K
+    cfg = K.cfg3(n)
+    out = {"base": n, "boxes": [[list(lo), list(hi)] for lo, hi in cfg["boxes"]], "g": list(cfg["g"]),
+           "width": cfg["width"]}
+    sols = {}
+    for key, mode in (("fp64", ela.SOLVE_FP64), ("mixed", ela.SOLVE_MIXED)):
+        a = ela.ElasticAMR(cfg["n"], cfg["boxes"], ela.ISO, mode=mode, coarse_min=4)
+        a.SetBC(cfg["bc_type"], cfg["bc_value"])
+        a.SetKrylov(krylov)
+        nodes = []
+        for l in range(a.nlev):
+            X = a.coords(l)
+            a.SetModel(l, K.cfg3_model(cfg, X[0], X[1], X[2]))
+            a.upload(a.B, l, a.bc_init(l))
+            nodes.append(int(np.prod(a.shape(l)[1:])))
+        zero = [np.zeros(a.shape(l)) for l in range(a.nlev)]
+        times, st = [], None
+        for it in range(steps + 1):  # the first solve is untimed (first launches)
+            for l in range(a.nlev):
+                a.upload(a.U, l, zero[l])
+            t0 = time.perf_counter()
+            a.prepareForSolve()
+            st = a.solve_resident(tol_rel=tol, max_iter=400, raise_on_fail=False)
+            t1 = time.perf_counter()
+            if it:
+                times.append(t1 - t0)
+        sols[key] = [a.download(a.U, l) for l in range(a.nlev)]
+        out[key] = {"solves_per_s": len(times) / sum(times), "s_per_solve": sum(times) / len(times),
+                    "iters": st["iters"], "rel_residual": st["res"] / st["res0"], "converged": bool(st["converged"]),
+                    "device_GB": a.bytes() / 1e9}
+        out["nodes_per_level"] = nodes
+        log("bench: amr crack %s %.3f s/solve iters=%d rel=%.2e" % (key, sum(times) / len(times), st["iters"],
+                                                                     st["res"] / st["res0"]))
+        a.close()
+    out["mixed_over_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    scale = max(np.abs(x).max() for x in sols["fp64"])
+    out["rel_disp_diff_mixed_vs_fp64"] = max(np.abs(x - y).max() for x, y in zip(sols["mixed"], sols["fp64"])) / scale
+    return out
+
+
 def serving(seconds, max_wall, tol, budget_gb=200.0, max_active=256):
     """Config 5: the Poisson(200/s) request stream, arrivals in the first `seconds` s, served by the
     continuous-batching loop (ela_serve) under a `budget_gb` HBM budget, stopped after `max_wall` s.
@@ -220,6 +265,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="arrival window of the serving stream (0: skip serving)")
     ap.add_argument("--serve-wall", type=float, default=40.0, help="wall-time limit of the serving run")
+    ap.add_argument("--amr-n", type=int, default=128, help="base cells of the AMR crack nest (0: skip it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md on sharding)",
@@ -316,6 +362,7 @@ def main():
     op.close()
     batched = batched_rves(args.rves, args.plain_sample, args.tol) if args.rves > 0 else None
     serve = serving(args.serve_seconds, args.serve_wall, args.tol) if args.serve_seconds > 0 else None
+    amr = amr_crack(args.amr_n, args.steps, args.tol, args.krylov) if args.amr_n > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -367,6 +414,7 @@ def main():
         "cpu_baseline": cpu,
         "batched_rves": batched,
         "serving": serve,
+        "amr_crack": amr,
     }
     print(json.dumps(out), flush=True)
 

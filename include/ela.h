@@ -285,6 +285,58 @@ int ela_reflux(const ela_level_layout* crse, void* crse_res, const ela_level_lay
 int ela_average_down(const ela_level_layout* crse, void* crse_sol, const ela_level_layout* fine,
                      const void* fine_sol, const int fine_origin[3], int ncomp, int prec, void* stream);
 
+/* ------------------------------------------------------------------ composite (AMR) solve
+ *
+ * Operator::Elastic<T>::define on several AMR levels (src/Operator/Elastic.cpp:30-55,
+ * src/Operator/Operator.cpp:215-258) and the multi-level
+ * Solver::Nonlocal::Linear::solve (src/Solver/Nonlocal/Linear.H:102-112).
+ * Level 0 is the domain; level l >= 1 (ratio 2) is one box of level-l nodes
+ * [lo, hi] (even corners, at least 2 coarse cells inside its parent unless on
+ * the domain boundary).  Fields of level l live on that box grown by 2 nodes
+ * and clipped to the domain (ela_amr_level_info: layout and the global index of
+ * its first node); the outer nodes hold the interpolated coarser solution, as
+ * the reference's fine ghost nodes do (Operator.cpp:556-624, Linear.H:23-29).
+ * Coarse nodes under a finer box take its values (averageDownSolutionRHS,
+ * Operator.cpp:626-641); at coarse nodes on the coarse/fine interface the
+ * equation is the reflux of the fine residual (Operator.cpp:728-839).  The
+ * solve is flexible GMRES(restart) in FP64 preconditioned by one FAC cycle
+ * over the levels' multigrid hierarchies (FP32 or FP64 cycles by mode). */
+typedef struct ela_amr ela_amr;
+enum { ELA_AMR_U = 0, ELA_AMR_B = 1, ELA_AMR_R = 2 };
+
+/* boxes: (nlev - 1) records of lo[3], hi[3] (level-l node indices); the level-0
+ * box hierarchy coarsens down to coarse_min cells, the finer boxes to 2.
+ * mode: ELA_SOLVE_FP64 or ELA_SOLVE_MIXED. */
+int ela_amr_create(ela_ctx* ctx, const ela_domain* dom, int nlev, const int* boxes, int model_kind, int coarse_min,
+                   int mode, ela_amr** out);
+int ela_amr_destroy(ela_amr* a);
+size_t ela_amr_bytes(const ela_amr* a);
+int ela_amr_level_info(const ela_amr* a, int lev, ela_level_layout* out, int origin[3]);
+/* SetModel(amrlev, FabArray) (src/Operator/Elastic.H:49-54): dense host SoA
+ * over the level's grown box. */
+int ela_amr_set_model_field(ela_amr* a, int lev, const double* C_host);
+/* SetBC / SetUniform / cycle / Krylov options as for a single box. */
+int ela_amr_set_bc(ela_amr* a, const int* types, const double* values);
+/* BC::Operator::Elastic::Init (src/BC/Operator/Elastic.H:106-181) on level lev:
+ * nodes of the grown box on the domain boundary take their class's value. */
+int ela_amr_bc_init(ela_amr* a, int lev, double* rhs_host_dense, int homogeneous);
+int ela_amr_set_uniform(ela_amr* a, int uniform);
+int ela_amr_set_cycle(ela_amr* a, int cycle, int nu1, int nu2);
+/* restart 0: the plain FAC iteration u += FAC(r). */
+int ela_amr_set_krylov(ela_amr* a, int restart);
+/* prepareForSolve on every level (Operator.cpp:445-452). */
+int ela_amr_prepare(ela_amr* a);
+/* Resident fields (ELA_AMR_U solution, ELA_AMR_B rhs, ELA_AMR_R composite
+ * residual), dense host arrays over a level's grown box. */
+int ela_amr_upload(ela_amr* a, int lev, int field, const double* host);
+int ela_amr_download(ela_amr* a, int lev, int field, double* host);
+/* Composite residual of the resident u and b into ELA_AMR_R (zero off the
+ * unknowns), with its max norm (solutionResidual + reflux, Operator.cpp:728-863). */
+int ela_amr_residual(ela_amr* a, double* norm_inf);
+/* Solve from the resident u to |r|inf <= max(tol_abs, tol_rel |r0|inf);
+ * max_iter counts FAC cycles. */
+int ela_amr_solve(ela_amr* a, double tol_rel, double tol_abs, int max_iter, ela_stats* stats);
+
 /* ------------------------------------------------------------------ batched independent solves
  *
  * The reference solves one problem per Solver::Nonlocal::Linear::solve call
