@@ -172,6 +172,12 @@ struct Geometry {
     std::array<double, 3> L{1, 1, 1};    // domain lengths
 };
 
+// The refined region of an AMR level (ratio 2): node indices of the level,
+// even corners (the reference's BoxArray of that level, as one box).
+struct Box {
+    std::array<int, 3> lo{0, 0, 0}, hi{0, 0, 0};
+};
+
 namespace Operator {
 // Operator::Elastic<T> on one box.  define() builds the MG hierarchy;
 // SetModel/SetBC/SetUniform as in the reference; Fapply/Diagonal/Fsmooth/
@@ -181,8 +187,11 @@ class Elastic {
 public:
     Elastic() {}
     Elastic(const Geometry& g, int mode = ELA_ALLOC_BOTH) { define(g, mode); }
+    // several AMR levels: level 0 on g, level l >= 1 on fine[l - 1]
+    Elastic(const Geometry& g, const std::vector<Box>& fine, int mode = ELA_SOLVE_MIXED) { define(g, fine, mode); }
     ~Elastic() {
         if (h) ela_hier_destroy(h);
+        if (a) ela_amr_destroy(a);
     }
     Elastic(const Elastic&) = delete;
     Elastic& operator=(const Elastic&) = delete;
@@ -196,6 +205,64 @@ public:
             d.dx[q] = q < g.dim ? g.L[q] / g.n[q] : 1.0;
         }
         ela_detail::check(ela_hier_create(ela_detail::context(), &d, T::kind, 0, mode, &h), "define");
+    }
+    // Elastic<T>::define(geom[], grids[], dmap[], info) with several AMR levels (Elastic.cpp:30-55)
+    void define(const Geometry& g, const std::vector<Box>& fine, int mode = ELA_SOLVE_MIXED) {
+        geom = g;
+        ela_domain d{};
+        d.dim = g.dim;
+        for (int q = 0; q < 3; ++q) {
+            d.n[q] = q < g.dim ? g.n[q] : 0;
+            d.dx[q] = q < g.dim ? g.L[q] / g.n[q] : 1.0;
+        }
+        std::vector<int> boxes;
+        for (const Box& b : fine) {
+            boxes.insert(boxes.end(), b.lo.begin(), b.lo.end());
+            boxes.insert(boxes.end(), b.hi.begin(), b.hi.end());
+        }
+        ela_detail::check(ela_amr_create(ela_detail::context(), &d, (int)fine.size() + 1, boxes.data(), T::kind, 4,
+                                         mode, &a),
+                          "define");
+        nlev = (int)fine.size() + 1;
+    }
+    int NumAMRLevels() const { return nlev; }
+    // node counts and first global node index of an AMR level's arrays (the level's box grown by 2 nodes)
+    std::array<int, 3> levelNodes(int amrlev) const {
+        ela_level_layout l;
+        ela_detail::check(ela_amr_level_info(a, amrlev, &l, nullptr), "ela_amr_level_info");
+        return {l.nodes[0], l.nodes[1], l.nodes[2]};
+    }
+    std::array<int, 3> levelOrigin(int amrlev) const {
+        ela_level_layout l;
+        int o[3];
+        ela_detail::check(ela_amr_level_info(a, amrlev, &l, o), "ela_amr_level_info");
+        return {o[0], o[1], o[2]};
+    }
+    int levelSize(int amrlev) const {
+        const auto n = levelNodes(amrlev);
+        return n[0] * n[1] * n[2];
+    }
+    // SetModel(amrlev, FabArray) (Elastic.H:49-54): one model per node of the level's arrays
+    void SetModel(int amrlev, const std::vector<T>& a_model) {
+        if (!a) {
+            if (amrlev != 0) throw std::runtime_error("SetModel: one AMR level");
+            SetModel(a_model);
+            return;
+        }
+        const int nn = levelSize(amrlev);
+        if ((int)a_model.size() != nn) throw std::runtime_error("Inconsistent model field size");
+        const int nc = (int)a_model[0].data.size();
+        std::vector<double> C((size_t)nc * nn);
+        for (int i = 0; i < nn; ++i)
+            for (int q = 0; q < nc; ++q) C[(size_t)q * nn + i] = a_model[i].data[q];
+        ela_detail::check(ela_amr_set_model_field(a, amrlev, C.data()), "SetModel");
+    }
+    void InitRHS(int amrlev, std::vector<double>& rhs, bool homogeneous = false) {
+        if (!a) {
+            InitRHS(rhs, homogeneous);
+            return;
+        }
+        ela_detail::check(ela_amr_bc_init(a, amrlev, rhs.data(), homogeneous), "Init");
     }
     int nodes(int mglev = 0) const {
         ela_level_layout l;
@@ -220,11 +287,18 @@ public:
     }
     void SetBC(::BC::Operator::Elastic<T>* a_bc) {
         m_bc = a_bc;
-        ela_detail::check(ela_set_bc(h, a_bc->types.data(), a_bc->values.data()), "SetBC");
+        if (a) ela_detail::check(ela_amr_set_bc(a, a_bc->types.data(), a_bc->values.data()), "SetBC");
+        else ela_detail::check(ela_set_bc(h, a_bc->types.data(), a_bc->values.data()), "SetBC");
     }
     ::BC::Operator::Elastic<T>& GetBC() { return *m_bc; }
-    void SetUniform(bool a_uniform) { ela_detail::check(ela_set_uniform(h, a_uniform), "SetUniform"); }
-    void prepareForSolve() { ela_detail::check(ela_prepare(h), "prepareForSolve"); }
+    void SetUniform(bool a_uniform) {
+        if (a) ela_detail::check(ela_amr_set_uniform(a, a_uniform), "SetUniform");
+        else ela_detail::check(ela_set_uniform(h, a_uniform), "SetUniform");
+    }
+    void prepareForSolve() {
+        if (a) ela_detail::check(ela_amr_prepare(a), "prepareForSolve");
+        else ela_detail::check(ela_prepare(h), "prepareForSolve");
+    }
     // BC::Operator::Elastic::Init on level 0
     void InitRHS(std::vector<double>& rhs, bool homogeneous = false) {
         ela_detail::check(ela_bc_init(h, rhs.data(), homogeneous), "Init");
@@ -242,10 +316,13 @@ public:
         return f;
     }
     ela_hier* handle() { return h; }
+    ela_amr* amr_handle() { return a; }
     Geometry geom;
 
 private:
     ela_hier* h = nullptr;
+    ela_amr* a = nullptr;  // several AMR levels
+    int nlev = 1;
     ::BC::Operator::Elastic<T>* m_bc = nullptr;  // non-owning, as in the reference (Elastic.H:161)
 };
 }  // namespace Operator
@@ -268,6 +345,23 @@ public:
         const int rc = ela_solve(linop.handle(), rhs.data(), sol.data(), tol_rel, tol_abs, mode, max_iter, &stats);
         if (rc != ELA_OK && rc != ELA_ERR_NONCONVERGED) ela_detail::check(rc, "solve");
         if (rc == ELA_ERR_NONCONVERGED) throw std::runtime_error(std::string("MLMG failed to converge: ") + ela_last_error());
+        return stats.res;
+    }
+    // solve(Vector<MultiFab*> sol, rhs, tol_rel, tol_abs) over several AMR levels (Linear.H:102-112):
+    // one host vector per level (the level's grown box); sol holds the initial guess.
+    Set::Scalar solve(std::vector<std::vector<double>>& sol, const std::vector<std::vector<double>>& rhs,
+                      Set::Scalar tol_rel, Set::Scalar tol_abs) {
+        ela_amr* a = linop.amr_handle();
+        if (!a) throw std::runtime_error("solve: the operator has one AMR level");
+        for (int l = 0; l < (int)sol.size(); ++l) {
+            ela_detail::check(ela_amr_upload(a, l, ELA_AMR_U, sol[l].data()), "upload");
+            ela_detail::check(ela_amr_upload(a, l, ELA_AMR_B, rhs[l].data()), "upload");
+        }
+        const int rc = ela_amr_solve(a, tol_rel, tol_abs, max_iter, &stats);
+        if (rc != ELA_OK && rc != ELA_ERR_NONCONVERGED) ela_detail::check(rc, "solve");
+        if (rc == ELA_ERR_NONCONVERGED) throw std::runtime_error(std::string("MLMG failed to converge: ") + ela_last_error());
+        for (int l = 0; l < (int)sol.size(); ++l)
+            ela_detail::check(ela_amr_download(a, l, ELA_AMR_U, sol[l].data()), "download");
         return stats.res;
     }
     ela_stats stats{};
@@ -299,6 +393,14 @@ public:
             for (size_t q = 0; q < u.size(); ++q) u[q] += du[q];
         }
         return res;
+    }
+
+    // several AMR levels, linear models: the Newton step from u solves the composite system from u
+    Set::Scalar solve(std::vector<std::vector<double>>& u, const std::vector<std::vector<double>>& b,
+                      const std::vector<std::vector<T>>& model, Set::Scalar tol_rel, Set::Scalar tol_abs) {
+        for (int l = 0; l < (int)model.size(); ++l) this->linop.SetModel(l, model[l]);
+        this->linop.prepareForSolve();
+        return Linear<T>::solve(u, b, tol_rel, tol_abs);
     }
 
 private:

@@ -1,13 +1,151 @@
 // C++ restatement of the reference's operator tests, driven through the
 // C++ interface of include/alamo_elastic.hpp (which calls libela.so on the GPU):
 //   UniaxialTest (src/Test/Operator/Elastic_UniaxialTest.cpp:13-206), component 0, 32^3, 1 level;
-//   TrigTest     (src/Test/Operator/Elastic_TrigTest.cpp:17-140), Laplacian model, 32^3, 1 level.
+//   TrigTest     (src/Test/Operator/Elastic_TrigTest.cpp:17-140), Laplacian model, 32^3, 1 level;
+//   and, as src/test.cc:128-150 runs them, UniaxialTest with 2 levels, 3 levels and "2 non-centered
+//   levels" and TrigTest with 2 and 3 levels (the levels of Test/Operator/Elastic.cpp:56-89: each the
+//   refinement of the middle half of the one below; Grid::YZ keeps the full x range).
 // Prints one line per test and returns the number of failures, like src/test.cc.
 #include <cmath>
 #include <cstdio>
 #include <vector>
 
 #include "alamo_elastic.hpp"
+
+template <class M>
+static void uniaxial_bcs(BC::Operator::Elastic<M>& bc) {
+    using B = BC::Operator::Elastic<M>;
+    bc.Set(B::XHI, B::X, B::Displacement, 0.1);
+    bc.Set(B::XHI, B::Y, B::Neumann, 0.0);
+    bc.Set(B::YLO, B::X, B::Neumann, 0.0);
+    bc.Set(B::YHI, B::X, B::Neumann, 0.0);
+    bc.Set(B::XHI_YHI, B::X, B::Displacement, 0.1);
+    bc.Set(B::XHI_YLO, B::X, B::Displacement, 0.1);
+    bc.Set(B::XHI, B::Z, B::Neumann, 0.0);
+    bc.Set(B::ZLO, B::X, B::Neumann, 0.0);
+    bc.Set(B::ZHI, B::X, B::Neumann, 0.0);
+    bc.Set(B::ZHI_XHI, B::X, B::Displacement, 0.1);
+    bc.Set(B::ZLO_XHI, B::X, B::Displacement, 0.1);
+    bc.Set(B::XHI_YLO_ZLO, B::X, B::Displacement, 0.1);
+    bc.Set(B::XHI_YLO_ZHI, B::X, B::Displacement, 0.1);
+    bc.Set(B::XHI_YHI_ZLO, B::X, B::Displacement, 0.1);
+    bc.Set(B::XHI_YHI_ZHI, B::X, B::Displacement, 0.1);
+    for (auto f : {B::YLO_ZLO, B::YLO_ZHI, B::YHI_ZLO, B::YHI_ZHI}) {
+        bc.Set(f, B::X, B::Neumann, 0.0);
+        bc.Set(f, B::Y, B::Neumann, 0.0);
+    }
+}
+
+// The levels of the reference's test geometry: nlevels - 1 boxes, each the refinement of the middle
+// half (or, for Grid::YZ, the middle half in y and z with the full x range) of the level below.
+static std::vector<Box> test_boxes(int n, int nlevels, bool yz) {
+    std::vector<Box> out;
+    int lo[3] = {0, 0, 0}, hi[3] = {n, n, n};
+    for (int l = 1; l < nlevels; ++l) {
+        Box b;
+        for (int d = 0; d < 3; ++d) {
+            const int w = (hi[d] - lo[d]);
+            const bool grow = !(yz && d == 0);
+            const int clo = grow ? lo[d] + w / 4 : lo[d], chi = grow ? hi[d] - w / 4 : hi[d];
+            b.lo[d] = 2 * clo;
+            b.hi[d] = 2 * chi;
+            lo[d] = b.lo[d];
+            hi[d] = b.hi[d];
+        }
+        out.push_back(b);
+    }
+    return out;
+}
+
+// Global coordinates of node `id` of AMR level l's arrays.
+template <class Op>
+static void coords(Op& op, int l, int id, int n0, double x[3]) {
+    const auto nn = op.levelNodes(l);
+    const auto o = op.levelOrigin(l);
+    const int i = id % nn[0], j = (id / nn[0]) % nn[1], k = id / (nn[0] * nn[1]);
+    const double h = 1.0 / (n0 << l);
+    x[0] = (o[0] + i) * h;
+    x[1] = (o[1] + j) * h;
+    x[2] = (o[2] + k) * h;
+}
+
+static int uniaxial_amr(int n, int nlevels, bool yz) {
+    using model_type = Model::Solid::Linear::Isotropic;
+    model_type model(2.6, 6.0);
+    Geometry g;
+    g.n = {n, n, n};
+    Operator::Elastic<model_type> elastic(g, test_boxes(n, nlevels, yz));
+    elastic.SetUniform(false);
+    BC::Operator::Elastic<model_type> bc;
+    uniaxial_bcs(bc);
+    elastic.SetBC(&bc);
+    std::vector<std::vector<double>> rhs(nlevels), u(nlevels);
+    std::vector<std::vector<model_type>> modelfab(nlevels);
+    for (int l = 0; l < nlevels; ++l) {
+        const int nn = elastic.levelSize(l);
+        modelfab[l].assign(nn, model);
+        rhs[l].assign(3 * (size_t)nn, 0.0);
+        u[l].assign(3 * (size_t)nn, 0.0);
+        elastic.InitRHS(l, rhs[l]);
+    }
+    Solver::Nonlocal::Newton<model_type> mlmg(elastic);
+    mlmg.solve(u, rhs, modelfab, 1e-10, 0.0);
+    double err = 0;
+    for (int l = 0; l < nlevels; ++l) {
+        const int nn = elastic.levelSize(l);
+        for (int id = 0; id < nn; ++id) {
+            double x[3];
+            coords(elastic, l, id, n, x);
+            err = std::fmax(err, std::fabs(u[l][id] - 0.1 * x[0]));
+            err = std::fmax(err, std::fabs(u[l][nn + id]));
+            err = std::fmax(err, std::fabs(u[l][2 * (size_t)nn + id]));
+        }
+    }
+    const double rel = err / 0.1;
+    const int failed = rel > 1e-7;
+    std::printf("UniaxialTest %d levels%s %d^3: relative error = %.3e %% %s\n", nlevels, yz ? " (non-centered)" : "",
+                n, 100 * rel, failed ? "FAIL" : "PASS");
+    return failed;
+}
+
+static int trig_amr(int n, int nlevels) {
+    using model_type = Model::Solid::Linear::Laplacian;
+    Geometry g;
+    g.n = {n, n, n};
+    Operator::Elastic<model_type> elastic(g, test_boxes(n, nlevels, false));
+    elastic.SetUniform(false);
+    BC::Operator::Elastic<model_type> bc;
+    elastic.SetBC(&bc);
+    const double pi = 3.14159265358979323846;
+    std::vector<std::vector<double>> rhs(nlevels), u(nlevels);
+    std::vector<std::vector<model_type>> modelfab(nlevels);
+    for (int l = 0; l < nlevels; ++l) {
+        const int nn = elastic.levelSize(l);
+        modelfab[l].assign(nn, model_type(3));
+        rhs[l].assign(3 * (size_t)nn, 0.0);
+        u[l].assign(3 * (size_t)nn, 0.0);
+        for (int id = 0; id < nn; ++id) {
+            double x[3];
+            coords(elastic, l, id, n, x);
+            rhs[l][id] = std::sin(pi * x[0]) * std::sin(pi * x[1]) * std::sin(pi * x[2]);
+        }
+        elastic.InitRHS(l, rhs[l]);
+    }
+    Solver::Nonlocal::Newton<model_type> mlmg(elastic);
+    mlmg.solve(u, rhs, modelfab, 1e-8, 0.0);
+    double e = 0, m = 0;  // level 0, as the reference's TrigTest measures it
+    const int nn = elastic.levelSize(0);
+    for (int id = 0; id < nn; ++id) {
+        double x[3];
+        coords(elastic, 0, id, n, x);
+        const double ex = -std::sin(pi * x[0]) * std::sin(pi * x[1]) * std::sin(pi * x[2]) / (3 * pi * pi);
+        e = std::fmax(e, std::fabs(u[0][id] - ex));
+        m = std::fmax(m, std::fabs(ex));
+    }
+    const int failed = e / m > 0.02;
+    std::printf("TrigTest %d levels %d^3: relative error = %.3e %% %s\n", nlevels, n, 100 * e / m, failed ? "FAIL" : "PASS");
+    return failed;
+}
 
 static int uniaxial(int n, bool mixed) {
     using model_type = Model::Solid::Linear::Isotropic;
@@ -103,6 +241,11 @@ int main() {
     failed += uniaxial(32, false);
     failed += uniaxial(32, true);
     failed += trig(32);
+    failed += uniaxial_amr(32, 2, false);
+    failed += uniaxial_amr(32, 3, false);
+    failed += uniaxial_amr(32, 2, true);
+    failed += trig_amr(32, 2);
+    failed += trig_amr(32, 3);
     std::printf("%d tests failed\n", failed);
     return failed;
 }
