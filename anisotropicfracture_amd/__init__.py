@@ -160,6 +160,7 @@ SIGNATURES = {
     "ela_batch_set_option": ([_P, _I, _I], _I),
     "ela_normalize": ([_P, _I, _I, _P, _P], _I),
     "ela_postprocess": ([_P, _I, _I, _P, _P, _P], _I),
+    "ela_newton_rhs": ([_P, _P, _P, _P, _P, _P], _I),
     "ela_reflux": ([ctypes.POINTER(_Layout), _P, ctypes.POINTER(_Layout), _P, _P, _I, _I, _P], _I),
     "ela_average_down": ([ctypes.POINTER(_Layout), _P, ctypes.POINTER(_Layout), _P, _P, _I, _I, _P], _I),
     "ela_serve": ([_P, _I, ctypes.POINTER(Request), _D, _D, _I, _I, _D, ctypes.POINTER(Result),
@@ -391,6 +392,20 @@ class Elastic:
         ff.free()
         return out
 
+    def newton_rhs(self, u, b, F0=None):
+        """Newton<T>::prepareForSolve's right-hand side (Newton.H:30-140): b - div(DDW : (grad u - F0)) inside,
+        b - BC row(u, grad u, dw) on the boundary.  F0: (dim*dim, *nodes) per-node eigenstrain or None."""
+        fu = self.field(0, F64).upload(u)
+        fb = self.field(0, F64).upload(b)
+        fr = self.field(0, F64)
+        ff = self.field(0, F64, self.dim * self.dim).upload(F0) if F0 is not None else None
+        _check(lib().ela_newton_rhs(self.ptr, fu.ptr, fb.ptr, ff.ptr if ff else None, fr.ptr, None))
+        out = fr.download()
+        for f in (fu, fb, fr, ff):
+            if f:
+                f.free()
+        return out
+
     def Diagonal(self, mglev=0, prec=F64):
         fd = self.field(mglev, prec)
         _check(lib().ela_diagonal(self.ptr, mglev, prec, fd.ptr, None))
@@ -522,6 +537,26 @@ class Linear:
         if rc != 0 and (raise_on_fail or rc not in (4,)):
             _check(rc)
         return u, self.last
+
+
+class Newton(Linear):
+    """Solver::Nonlocal::Newton<T> (Newton.H:15-349) for the models whose kinematic variable is grad u:
+    each of nriters steps builds rhs = b - div(DDW : (grad u - F0)) (prepareForSolve), solves A du = rhs
+    from du = 0 and adds du to u.  F0 (dim*dim planes per node, or None) makes the models affine
+    (Affine::Isotropic / Affine::Cubic, Affine/*.H)."""
+
+    def __init__(self, op, nriters=1):
+        super().__init__(op)
+        self.nriters = nriters  # Newton.H:337
+
+    def solve(self, u, b, F0=None, tol_rel=None, tol_abs=None, mode=SOLVE_FP64, max_iter=None, raise_on_fail=True):
+        u = np.array(u, dtype=np.float64, copy=True)
+        st = None
+        for _ in range(self.nriters):
+            rhs = self.op.newton_rhs(u, b, F0)
+            du, st = Linear.solve(self, np.zeros_like(u), rhs, tol_rel, tol_abs, mode, max_iter, raise_on_fail)
+            u += du
+        return u, st
 
 
 # ---------------------------------------------------------------- AMR level transfers

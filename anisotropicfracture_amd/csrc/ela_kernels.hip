@@ -804,6 +804,83 @@ void launch_post(const Lay& L, int kind, const double* C, const double* u, doubl
                                     voigt));
 }
 
+// Newton<T>::prepareForSolve (src/Solver/Nonlocal/Newton.H:30-140), models with
+// kinematic variable grad u: dw = DDW : (grad u - F0) per node (one-sided
+// gradient on the walls), stored as dim*dim planes (row-major).
+template <int K, int DIM>
+__global__ __launch_bounds__(256) void k_newton_dw(Lay L, const double* __restrict__ C, const double* __restrict__ F0,
+                                                  const double* __restrict__ u, double* __restrict__ dw,
+                                                  Inv<double> iv) {
+    int i, j, k;
+    if (!node_index(L, i, j, k)) return;
+    const long long o = (long long)i + L.px * j + L.pl * k;
+    double g[DIM][DIM], s[DIM][DIM] = {};
+    node_grad<double, DIM>(L, u, i, j, k, iv, g);
+    if (F0) {
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+#pragma unroll
+            for (int b = 0; b < DIM; ++b) g[a][b] -= F0[(a * DIM + b) * L.cs + o];
+    }
+    stress<double, K, DIM>(PlaneRec<double>{C + o, L.cs}, g, s);
+#pragma unroll
+    for (int a = 0; a < DIM; ++a)
+#pragma unroll
+        for (int b = 0; b < DIM; ++b) dw[(a * DIM + b) * L.cs + o] = s[a][b];
+}
+
+// rhs = b - BC row(u, grad u, dw) on the boundary, b - div dw (central
+// differences of dw, in the reference's order of operations) inside.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_newton_rhs(Lay L, const double* __restrict__ u, const double* __restrict__ b,
+                                                   const double* __restrict__ dw, double* __restrict__ rhs, BCTab bc,
+                                                   Inv<double> iv, double hx, double hy, double hz) {
+    int i, j, k;
+    if (!node_index(L, i, j, k)) return;
+    const long long o = (long long)i + L.px * j + L.pl * k;
+    const int ex = L.nx - 1, ey = L.ny - 1, ez = L.nz - 1;
+    const bool wall = i == 0 || i == ex || j == 0 || j == ey || (DIM == 3 && (k == 0 || k == ez));
+    if (wall) {
+        double g[DIM][DIM], s[DIM][DIM], uc[DIM], f[DIM];
+        node_grad<double, DIM>(L, u, i, j, k, iv, g);
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) {
+            uc[a] = u[a * L.cs + o];
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) s[a][c] = dw[(a * DIM + c) * L.cs + o];
+        }
+        bc_row<double, DIM>(bc, face_of<DIM>(i, j, k, ex, ey, ez), uc, g, s, f);
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) rhs[a * L.cs + o] = b[a * L.cs + o] - f[a];
+        return;
+    }
+    const double h[3] = {hx, hy, hz};
+    const long long st[3] = {1, L.px, L.pl};
+#pragma unroll
+    for (int p = 0; p < DIM; ++p) {
+        double div = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const double* plane = dw + (p * DIM + d) * L.cs + o;
+            div += (plane[st[d]] - plane[-st[d]]) / 2. / h[d];
+        }
+        rhs[p * L.cs + o] = b[p * L.cs + o] - div;
+    }
+}
+
+void launch_newton_rhs(const Lay& L, int kind, const double* C, const double* F0, const double* u, const double* b,
+                       double* dw, double* rhs, const BCTab& bc, const double h[3], hipStream_t s) {
+    const Inv<double> iv = make_inv<double>(h);
+    ELA_DISPATCH(kind, L.dim,
+                 hipLaunchKernelGGL((k_newton_dw<KK, DD>), node_grid(L), dim3(BX, BY), 0, s, L, C, F0, u, dw, iv));
+    if (L.dim == 3)
+        hipLaunchKernelGGL((k_newton_rhs<3>), node_grid(L), dim3(BX, BY), 0, s, L, u, b, dw, rhs, bc, iv, h[0], h[1],
+                           h[2]);
+    else
+        hipLaunchKernelGGL((k_newton_rhs<2>), node_grid(L), dim3(BX, BY), 0, s, L, u, b, dw, rhs, bc, iv, h[0], h[1],
+                           1.0);
+}
+
 // Grain-form z-marching stencil on one box (ela_grain.hpp): interior tiles, then the shell.
 namespace {
 template <typename T, int MODE, typename TO, bool SC>

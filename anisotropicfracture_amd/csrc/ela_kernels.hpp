@@ -91,6 +91,11 @@ void launch_residual_tab(const Lay& L, const TabDesc<T>& t, const T* x, const T*
                          const double h[3], int uniform, hipStream_t s);
 void launch_grain_diag(int ng, const double* tab, const double h[3], double* d64, float* d32, hipStream_t s);
 
+// Newton<T>::prepareForSolve: dw = DDW : (grad u - F0) into dw (dim*dim planes; F0 the same, or null),
+// then rhs = b - div dw inside, b - BC row(u, grad u, dw) on the boundary.  FP64.
+void launch_newton_rhs(const Lay& L, int kind, const double* C, const double* F0, const double* u, const double* b,
+                       double* dw, double* rhs, const BCTab& bc, const double h[3], hipStream_t s);
+
 // Strain / Stress / Energy of u (what 0 / 1 / 2) into out (6 or 9 / 6 or 9 / 1 planes in 3D).
 void launch_post(const Lay& L, int kind, const double* C, const double* u, double* out, const double h[3], int what,
                  int voigt, hipStream_t s);

@@ -137,6 +137,8 @@ struct ela_hier {
     int kz = 16;
     bool xcd = true;  // XCD-contiguous tile order of the marching kernels
     BCTab* dbc = nullptr;
+    // dw = DDW : (grad u - F0) of Newton's right-hand side (dim*dim FP64 planes, allocated on first use)
+    double* newton_dw = nullptr;
     // flexible GMRES around the MG cycle (restart length; 0 = plain MG iteration)
     int krylov = 0;
     std::vector<double*> V, Z;
@@ -1117,6 +1119,20 @@ int ela_fapply(ela_hier* h, int mglev, int prec, const void* u, void* f, void* s
     if (prec == ELA_F64) A_apply<double>(h, v, v.C64, (const double*)u, (double*)f, strm(h, stream));
     else A_apply<float>(h, v, v.C32, (const float*)u, (float*)f, strm(h, stream));
     HIPCHK(hipGetLastError());
+    return ELA_OK;
+}
+
+int ela_newton_rhs(ela_hier* h, const void* u, const void* b, const void* F0, void* rhs, void* stream) {
+    if (!h || !u || !b || !rhs) return fail(ELA_ERR_BADARG, "null argument");
+    if (!h->model_set) return fail(ELA_ERR_STATE, "Attempting to use operator before calling SetModel");
+    Level& v = h->lev[0];
+    if (!h->newton_dw)
+        if (int rc = halloc_t(h, field_elems(v, h->dim * h->dim), &h->newton_dw)) return rc;
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    launch_newton_rhs(v.L, h->kind, v.C64, (const double*)F0, (const double*)u, (const double*)b, h->newton_dw,
+                      (double*)rhs, h->bc, v.h, s);
+    HIPCHK(hipGetLastError());
+    if (!stream) HIPCHK(hipStreamSynchronize(s));
     return ELA_OK;
 }
 

@@ -3,6 +3,7 @@
 //   Operator::Elastic<T>            (src/Operator/Elastic.H:20-171)
 //   BC::Operator::Elastic<T>        (src/BC/Operator/Elastic.H:12-407)
 //   Model::Solid::Linear::{Isotropic,Cubic,Laplacian}
+//   Model::Solid::Affine::{Isotropic,Cubic}
 //   Model::Solid::LinearElastic::Degradable::Isotropic
 //   Solver::Nonlocal::{Linear, Newton<T>} (src/Solver/Nonlocal/*.H)
 // can switch to the gfx950 implementation.  Header-only; every computation
@@ -114,6 +115,31 @@ public:
 };
 
 }  // namespace Linear
+
+namespace Affine {
+// Model::Solid::Affine::Isotropic (src/Model/Solid/Affine/Isotropic.H:12-84): the linear stiffness
+// plus an eigenstrain F0 (row-major 3x3), DW(gradu) = DDW : (gradu - F0).
+class Isotropic : public Linear::Isotropic {
+public:
+    static constexpr bool affine = true;
+    Isotropic() {}
+    Isotropic(Set::Scalar mu, Set::Scalar lambda, const std::array<double, 9>& a_F0 = {})
+        : Linear::Isotropic(mu, lambda), F0(a_F0) {}
+    std::array<double, 9> F0{};
+};
+// Model::Solid::Affine::Cubic (src/Model/Solid/Affine/Cubic.H:13-97).
+class Cubic : public Linear::Cubic {
+public:
+    static constexpr bool affine = true;
+    Cubic() {}
+    void Define(Set::Scalar C11, Set::Scalar C12, Set::Scalar C44, const std::array<double, 9>& R,
+                const std::array<double, 9>& a_F0 = {}) {
+        Linear::Cubic::Define(C11, C12, C44, R);
+        F0 = a_F0;
+    }
+    std::array<double, 9> F0{};
+};
+}  // namespace Affine
 
 namespace LinearElastic {
 namespace Degradable {
@@ -330,6 +356,14 @@ private:
 // ---------------------------------------------------------------- solvers
 namespace Solver {
 namespace Nonlocal {
+template <class M, class = void>
+struct has_F0 {
+    static constexpr bool value = false;
+};
+template <class M>
+struct has_F0<M, decltype((void)M::affine)> {
+    static constexpr bool value = M::affine;
+};
 // Solver::Nonlocal::Linear (src/Solver/Nonlocal/Linear.H:19-143): holds a reference to the operator.
 template <class T>
 class Linear {
@@ -373,8 +407,11 @@ protected:
 };
 
 // Solver::Nonlocal::Newton<T> (src/Solver/Nonlocal/Newton.H:15-349) for the
-// linear models: one Newton step solves A du = b - A u and updates u.  From
-// u = 0 this is exactly one Linear::solve with rhs = b, as in the reference.
+// models whose kinematic variable is grad u: each step builds
+// rhs = b - div(DDW : (grad u - F0)) (prepareForSolve, on the GPU: ela_newton_rhs),
+// solves A du = rhs from du = 0 and adds du.  For the linear models from
+// u = 0 the rhs is b, so this is exactly one Linear::solve, as in the
+// reference; Affine models bring their eigenstrain F0.
 template <class T>
 class Newton : public Linear<T> {
 public:
@@ -384,17 +421,38 @@ public:
                       Set::Scalar tol_rel, Set::Scalar tol_abs) {
         this->linop.SetModel(model);
         this->linop.prepareForSolve();
-        Set::Scalar res = 0;
-        for (int it = 0; it < nriters; ++it) {
-            std::vector<double> Au = this->linop.Fapply(0, u);
-            std::vector<double> rhs(b.size()), du(b.size(), 0.0);
-            for (size_t q = 0; q < b.size(); ++q) rhs[q] = b[q] - Au[q];
-            res = Linear<T>::solve(du, rhs, tol_rel, tol_abs);
-            for (size_t q = 0; q < u.size(); ++q) u[q] += du[q];
+        ela_hier* h = this->linop.handle();
+        const int dim = this->linop.geom.dim, nn = this->linop.nodes();
+        std::vector<double> F0;
+        if constexpr (has_F0<T>::value) {
+            F0.assign((size_t)dim * dim * nn, 0.0);
+            for (int i = 0; i < nn; ++i)
+                for (int a = 0; a < dim; ++a)
+                    for (int c = 0; c < dim; ++c) F0[(size_t)(a * dim + c) * nn + i] = model[i].F0[3 * a + c];
         }
+        void *du = nullptr, *db = nullptr, *dr = nullptr, *df = nullptr;
+        ela_detail::check(ela_field_alloc(h, 0, ELA_F64, dim, &du), "alloc");
+        ela_detail::check(ela_field_alloc(h, 0, ELA_F64, dim, &db), "alloc");
+        ela_detail::check(ela_field_alloc(h, 0, ELA_F64, dim, &dr), "alloc");
+        if (!F0.empty()) {
+            ela_detail::check(ela_field_alloc(h, 0, ELA_F64, dim * dim, &df), "alloc");
+            ela_detail::check(ela_upload(h, 0, ELA_F64, dim * dim, F0.data(), df), "upload");
+        }
+        ela_detail::check(ela_upload(h, 0, ELA_F64, dim, b.data(), db), "upload");
+        Set::Scalar res = 0;
+        std::vector<double> rhs(b.size());
+        for (int it = 0; it < nriters; ++it) {
+            ela_detail::check(ela_upload(h, 0, ELA_F64, dim, u.data(), du), "upload");
+            ela_detail::check(ela_newton_rhs(h, du, db, df, dr, nullptr), "prepareForSolve");
+            ela_detail::check(ela_download(h, 0, ELA_F64, dim, dr, rhs.data()), "download");
+            std::vector<double> d(b.size(), 0.0);
+            res = Linear<T>::solve(d, rhs, tol_rel, tol_abs);
+            for (size_t q = 0; q < u.size(); ++q) u[q] += d[q];
+        }
+        for (void* p : {du, db, dr, df})
+            if (p) ela_field_free(h, p);
         return res;
     }
-
     // several AMR levels, linear models: the Newton step from u solves the composite system from u
     Set::Scalar solve(std::vector<std::vector<double>>& u, const std::vector<std::vector<double>>& b,
                       const std::vector<std::vector<T>>& model, Set::Scalar tol_rel, Set::Scalar tol_abs) {

@@ -174,6 +174,15 @@ int ela_diagonal(ela_hier* h, int mglev, int prec, void* diag, void* stream);
  * in 3D, 3 (00,11,01) in 2D; otherwise dim*dim row-major; energy has 1. */
 int ela_postprocess(ela_hier* h, int what, int voigt, const void* u, void* out, void* stream);
 
+/* Newton<T>::prepareForSolve (src/Solver/Nonlocal/Newton.H:30-140) for the
+ * models whose kinematic variable is grad u (Linear and Affine Isotropic /
+ * Cubic, Laplacian): with dw = DDW : (grad u - F0) per node (one-sided
+ * gradient on the walls), rhs = b - BC row(u, grad u, dw) on the boundary and
+ * rhs = b - div dw (central differences of dw) inside.  u, b, rhs: FP64
+ * level-0 fields (dim components); F0: dim*dim FP64 planes (row-major) in the
+ * level-0 layout, or NULL for F0 = 0.  Newton's step then solves A du = rhs. */
+int ela_newton_rhs(ela_hier* h, const void* u, const void* b, const void* F0, void* rhs, void* stream);
+
 /* normalize (src/Operator/Operator.cpp:164-195), as the reference computes it:
  * its per-component lambda indexes neither x nor diag by component, so
  * component 0 of x is divided by component 0 of the diagonal once per

@@ -766,6 +766,65 @@ void or_post(const or_grid* og, int kind, const double* C, const double* u, int 
             }
 }
 
+// Newton<T>::prepareForSolve (src/Solver/Nonlocal/Newton.H:30-140) for the
+// models whose kinematic variable is grad u (Linear and Affine Isotropic /
+// Cubic, Laplacian): dw = DDW : (grad u - F0) at every node with the
+// GetStencil gradient (one-sided on the walls, Newton.H:55-58); then
+// rhs = b - BC row(u, grad u, dw) on the boundary (Newton.H:114-124) and
+// rhs = b - sum_d (dw(+e_d)(p,d) - dw(-e_d)(p,d)) / 2 / dx_d inside (:126-131).
+// F0: dim*dim planes (row-major), or null for F0 = 0.
+void or_newton_rhs(const or_grid* og, int kind, const double* C, const double* F0, const int* bctype,
+                   const double* u, const double* b, double* rhs) {
+    G g(og);
+    const int nc = ncomp_of(g.dim, kind), dim = g.dim;
+    std::vector<double> dw((size_t)dim * dim * g.nodes);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < g.nn[2]; ++k)
+        for (int j = 0; j < g.nn[1]; ++j)
+            for (int i = 0; i < g.nn[0]; ++i) {
+                double gu[3][3] = {}, s[3][3] = {}, c[21];
+                const long q = g.id(i, j, k);
+                for (int p = 0; p < dim; ++p)
+                    for (int d = 0; d < dim; ++d)
+                        gu[p][d] = d1(g, u, p, i, j, k, d) - (F0 ? F0[(p * dim + d) * g.nodes + q] : 0.0);
+                load_c(g, nc, C, i, j, k, c);
+                contract2(dim, kind, c, gu, s);
+                for (int a = 0; a < dim; ++a)
+                    for (int bb = 0; bb < dim; ++bb) dw[(size_t)(a * dim + bb) * g.nodes + q] = s[a][bb];
+            }
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < g.nn[2]; ++k)
+        for (int j = 0; j < g.nn[1]; ++j)
+            for (int i = 0; i < g.nn[0]; ++i) {
+                const long q = g.id(i, j, k);
+                if (on_boundary(g, i, j, k)) {
+                    double uu[3] = {0, 0, 0}, gu[3][3] = {}, s[3][3] = {}, nrm[3], f[3];
+                    for (int p = 0; p < dim; ++p) {
+                        uu[p] = u[p * g.nodes + q];
+                        for (int d = 0; d < dim; ++d) {
+                            gu[p][d] = d1(g, u, p, i, j, k, d);
+                            s[p][d] = dw[(size_t)(p * dim + d) * g.nodes + q];
+                        }
+                    }
+                    const int face = classify(g, i, j, k, nrm);
+                    bc_row(dim, bctype + 3 * face, uu, gu, s, nrm, f);
+                    for (int p = 0; p < dim; ++p) rhs[p * g.nodes + q] = b[p * g.nodes + q] - f[p];
+                    continue;
+                }
+                for (int p = 0; p < dim; ++p) {
+                    double div = 0.0;
+                    for (int d = 0; d < dim; ++d) {
+                        int qp[3] = {i, j, k}, qm[3] = {i, j, k};
+                        qp[d] += 1;
+                        qm[d] -= 1;
+                        const double* plane = dw.data() + (size_t)(p * dim + d) * g.nodes;
+                        div += (plane[g.id(qp[0], qp[1], qp[2])] - plane[g.id(qm[0], qm[1], qm[2])]) / 2. / g.dx[d];
+                    }
+                    rhs[p * g.nodes + q] = b[p * g.nodes + q] - div;
+                }
+            }
+}
+
 // normalize (Operator.cpp:164-195): the ParallelFor lambda indexes neither x
 // nor diag by component, so x(i,j,k,0) /= diag(i,j,k,0) once per component.
 void or_normalize(const or_grid* og, const double* diag, double* x) {

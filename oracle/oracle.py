@@ -76,6 +76,7 @@ def lib():
         L.or_reflux.argtypes = [P(_Grid), i, vp, vp, vp, vp]
         L.or_normalize.argtypes = [P(_Grid), vp, vp]
         L.or_post.argtypes = [P(_Grid), i, vp, vp, i, i, vp]
+        L.or_newton_rhs.argtypes = [P(_Grid), i, vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -177,6 +178,16 @@ def post(g, kind, C, u, what, voigt=False):
     out = np.zeros(g.shape(nc))
     lib().or_post(ctypes.byref(g.c()), kind, _p(_c(C)), _p(_c(u)), what, int(bool(voigt)), _p(out))
     return out[0] if what == 2 else out
+
+
+def newton_rhs(g, kind, C, bct, u, b, F0=None):
+    """Newton<T>::prepareForSolve's right-hand side (Newton.H:30-140): b - div(DDW : (grad u - F0)) inside
+    (central differences of dw), b - BC row(u, grad u, dw) on the boundary.  F0: (dim*dim, *nodes) or None."""
+    out = np.zeros_like(_c(b))
+    f0 = None if F0 is None else _c(F0)
+    lib().or_newton_rhs(ctypes.byref(g.c()), kind, _p(_c(C)), None if f0 is None else _p(f0),
+                        _p(_c(bct, np.int32)), _p(_c(u)), _p(_c(b)), _p(out))
+    return out
 
 
 def normalize(g, diag, x):

@@ -4,7 +4,10 @@
 //   TrigTest     (src/Test/Operator/Elastic_TrigTest.cpp:17-140), Laplacian model, 32^3, 1 level;
 //   and, as src/test.cc:128-150 runs them, UniaxialTest with 2 levels, 3 levels and "2 non-centered
 //   levels" and TrigTest with 2 and 3 levels (the levels of Test/Operator/Elastic.cpp:56-89: each the
-//   refinement of the middle half of the one below; Grid::YZ keeps the full x range).
+//   refinement of the middle half of the one below; Grid::YZ keeps the full x range);
+//   and the affine Newton path (Newton.H:30-140 with Affine::Isotropic, Affine/Isotropic.H): a free
+//   thermal expansion, uniform F0 = e0 I with traction-free faces and ZLO held at e0 (x, y, 0),
+//   whose exact discrete solution is u = e0 x (the stress C:(grad u - F0) vanishes everywhere).
 // Prints one line per test and returns the number of failures, like src/test.cc.
 #include <cmath>
 #include <cstdio>
@@ -147,6 +150,49 @@ static int trig_amr(int n, int nlevels) {
     return failed;
 }
 
+static int affine_free_expansion(int n) {
+    using model_type = Model::Solid::Affine::Isotropic;
+    const double e0 = 0.01;
+    model_type model(2.6, 6.0, {e0, 0, 0, 0, e0, 0, 0, 0, e0});
+    Geometry g;
+    g.n = {n, n, n};
+    Operator::Elastic<model_type> elastic(g);
+    elastic.SetUniform(false);
+    BC::Operator::Elastic<model_type> bc;
+    using B = BC::Operator::Elastic<model_type>;
+    // every class Traction 0, except the classes on ZLO: Displacement in all components
+    for (int f = 0; f < 26; ++f) {
+        const bool zlo = f == B::ZLO || f == B::YLO_ZLO || f == B::YHI_ZLO || f == B::ZLO_XLO || f == B::ZLO_XHI ||
+                         f == B::XLO_YLO_ZLO || f == B::XLO_YHI_ZLO || f == B::XHI_YLO_ZLO || f == B::XHI_YHI_ZLO;
+        for (auto d : {B::X, B::Y, B::Z}) bc.Set((typename B::Face)f, d, zlo ? B::Displacement : B::Traction, 0.0);
+    }
+    elastic.SetBC(&bc);
+    const int nn = (n + 1) * (n + 1) * (n + 1);
+    std::vector<model_type> modelfab(nn, model);
+    std::vector<double> rhs(3 * (size_t)nn, 0.0), u(3 * (size_t)nn, 0.0);
+    elastic.InitRHS(rhs);
+    for (int j = 0; j <= n; ++j)  // the prescribed displacement on ZLO: e0 (x, y, 0)
+        for (int i = 0; i <= n; ++i) {
+            const size_t id = i + (size_t)(n + 1) * j;
+            rhs[id] = e0 * i / n;
+            rhs[(size_t)nn + id] = e0 * j / n;
+        }
+    Solver::Nonlocal::Newton<model_type> mlmg(elastic);
+    mlmg.solve(u, rhs, modelfab, 1e-11, 0.0);
+    double err = 0;
+    for (int k = 0; k <= n; ++k)
+        for (int j = 0; j <= n; ++j)
+            for (int i = 0; i <= n; ++i) {
+                const size_t id = i + (size_t)(n + 1) * (j + (size_t)(n + 1) * k);
+                const double ex[3] = {e0 * i / n, e0 * j / n, e0 * k / n};
+                for (int p = 0; p < 3; ++p) err = std::fmax(err, std::fabs(u[p * (size_t)nn + id] - ex[p]));
+            }
+    const double rel = err / e0;
+    const int failed = rel > 1e-7;
+    std::printf("Affine free expansion %d^3: relative error = %.3e %% %s\n", n, 100 * rel, failed ? "FAIL" : "PASS");
+    return failed;
+}
+
 static int uniaxial(int n, bool mixed) {
     using model_type = Model::Solid::Linear::Isotropic;
     model_type model(2.6, 6.0);  // (mu, lambda) as the reference passes (lame, shear)
@@ -246,6 +292,7 @@ int main() {
     failed += uniaxial_amr(32, 2, true);
     failed += trig_amr(32, 2);
     failed += trig_amr(32, 3);
+    failed += affine_free_expansion(16);
     std::printf("%d tests failed\n", failed);
     return failed;
 }
