@@ -13,6 +13,7 @@
 // reference aborts (Util::Abort), these classes throw std::runtime_error.
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <stdexcept>
@@ -136,6 +137,21 @@ public:
                 const std::array<double, 9>& a_F0 = {}) {
         Linear::Cubic::Define(C11, C12, C44, R);
         F0 = a_F0;
+    }
+    // Combine (Affine/Cubic.H:70-94): the eta-weighted mean of the stiffness records and of F0,
+    // weights eta[n] / sum(eta), accumulated in model order.
+    static Cubic Combine(const std::vector<Cubic>& models, const std::vector<Set::Scalar>& eta) {
+        Cubic ret;
+        std::fill(ret.data.begin(), ret.data.end(), 0.0);
+        ret.F0.fill(0.0);
+        Set::Scalar etasum = 0.;
+        for (size_t n = 0; n < models.size(); ++n) etasum += eta[n];
+        for (size_t n = 0; n < models.size(); ++n) {
+            const Set::Scalar w = eta[n] / etasum;
+            for (size_t q = 0; q < ret.data.size(); ++q) ret.data[q] += models[n].data[q] * w;
+            for (int q = 0; q < 9; ++q) ret.F0[q] += models[n].F0[q] * w;
+        }
+        return ret;
     }
     std::array<double, 9> F0{};
 };

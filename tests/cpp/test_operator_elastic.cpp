@@ -193,6 +193,22 @@ static int affine_free_expansion(int n) {
     return failed;
 }
 
+// Affine::Cubic::Combine (Affine/Cubic.H:70-94): eta-weighted mean of the records and of F0.
+static int combine_check() {
+    using M = Model::Solid::Affine::Cubic;
+    M a, b;
+    a.Define(1.68, 1.21, 0.75, {1, 0, 0, 0, 1, 0, 0, 0, 1}, {0.01, 0, 0, 0, 0, 0, 0, 0, 0});
+    const double c = std::cos(0.3), s = std::sin(0.3);
+    b.Define(2.0, 1.0, 0.5, {c, -s, 0, s, c, 0, 0, 0, 1}, {0, 0, 0, 0, 0.02, 0, 0, 0, -0.01});
+    const M m = M::Combine({a, b}, {1.0, 3.0});
+    double err = 0;
+    for (size_t q = 0; q < m.data.size(); ++q) err = std::fmax(err, std::fabs(m.data[q] - (0.25 * a.data[q] + 0.75 * b.data[q])));
+    for (int q = 0; q < 9; ++q) err = std::fmax(err, std::fabs(m.F0[q] - (0.25 * a.F0[q] + 0.75 * b.F0[q])));
+    const int failed = err > 1e-15;
+    std::printf("Affine::Cubic::Combine: max deviation = %.3e %s\n", err, failed ? "FAIL" : "PASS");
+    return failed;
+}
+
 static int uniaxial(int n, bool mixed) {
     using model_type = Model::Solid::Linear::Isotropic;
     model_type model(2.6, 6.0);  // (mu, lambda) as the reference passes (lame, shear)
@@ -293,6 +309,7 @@ int main() {
     failed += trig_amr(32, 2);
     failed += trig_amr(32, 3);
     failed += affine_free_expansion(16);
+    failed += combine_check();
     std::printf("%d tests failed\n", failed);
     return failed;
 }

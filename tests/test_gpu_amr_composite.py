@@ -179,6 +179,42 @@ def test_cfg3_crack_small_solution_is_a_composite_solution(oracle, amr):
         assert np.abs(sols[E.SOLVE_MIXED][l] - sols[E.SOLVE_FP64][l]).max() <= 1e-8 * scale
 
 
+@pytest.mark.parametrize("mode", [E.SOLVE_FP64, E.SOLVE_MIXED])
+def test_composite_2d_matches_oracle(oracle, amr, mode):
+    """2D nest of 3 levels (16^2 base): composite residual and solution against the oracle."""
+    nest = amr.Nest((16, 16), [((8, 8), (24, 24)), ((24, 24), (40, 40))])
+    t, v = oracle.bc_arrays(2, {"YLO": {0: (0, 0.0), 1: (0, 0.0)}, "YHI": {0: (1, 0.0), 1: (0, 0.01)},
+                                "XLO_YLO": {0: (0, 0), 1: (0, 0)}, "XHI_YLO": {0: (0, 0), 1: (0, 0)}},
+                            default=(1, 0.0))
+    C = []
+    for l in range(nest.nlev):
+        X = nest.coords(l)
+        c = np.zeros(nest.shape(l, 2))
+        c[0] = 1.0 + 0.5 * np.sin(3 * X[0])
+        c[1] = 0.7 + 0.2 * np.cos(2 * X[1])
+        C.append(c)
+    rng = np.random.default_rng(4)
+    a = make(nest, ISO, C, t, mode, values=v)
+    uu = [rng.standard_normal(nest.shape(l, 2)) for l in range(nest.nlev)]
+    bb = [rng.standard_normal(nest.shape(l, 2)) for l in range(nest.nlev)]
+    rg, _ = a.residual(uu, bb)
+    ro, _ = nest.residual(ISO, C, t, bb, uu)
+    scale = max(np.abs(x).max() for x in ro)
+    for l in range(nest.nlev):
+        assert np.abs(rg[l] - ro[l]).max() <= 1e-12 * scale
+    b = [nest.bc_init(l, t, v, np.zeros(nest.shape(l, 2))) for l in range(nest.nlev)]
+    for l in range(nest.nlev):
+        assert np.array_equal(a.bc_init(l), b[l])
+    ref = nest.solve_direct(ISO, C, t, b)
+    u, st = a.solve([np.zeros_like(x) for x in b], b, tol_rel=1e-10, max_iter=200)
+    assert st["converged"]
+    scale = max(np.abs(x).max() for x in ref)
+    for l in range(nest.nlev):
+        m = nest.classify(l)["inB"]
+        assert np.abs((u[l] - ref[l])[:, m]).max() <= 1e-8 * scale
+    a.close()
+
+
 def test_bad_nesting_is_rejected():
     with pytest.raises(E.ElaError):
         E.ElasticAMR((8, 8, 8), [((2, 2, 2), (12, 12, 12))], ISO)  # 1 coarse cell inside its parent
