@@ -176,6 +176,7 @@ SIGNATURES = {
     "ela_amr_set_uniform": ([_P, _I], _I),
     "ela_amr_bc_init": ([_P, _I, _P, _I], _I),
     "ela_amr_set_cycle": ([_P, _I, _I, _I], _I),
+    "ela_amr_set_level_cycle": ([_P, _I, _I, _I, _I], _I),
     "ela_amr_set_krylov": ([_P, _I], _I),
     "ela_amr_prepare": ([_P], _I),
     "ela_amr_upload": ([_P, _I, _I, _P], _I),
@@ -665,8 +666,11 @@ class ElasticAMR:
         axes = [(o[d] + np.arange(lay.nodes[d])) * lay.dx[d] for d in range(self.dim)]
         return list(reversed(np.meshgrid(*reversed(axes), indexing="ij")))
 
-    def SetCycle(self, cycle=CYCLE_F, nu1=2, nu2=2):
-        _check(lib().ela_amr_set_cycle(self.ptr, cycle, nu1, nu2))
+    def SetCycle(self, cycle=CYCLE_F, nu1=2, nu2=2, lev=None):
+        if lev is None:
+            _check(lib().ela_amr_set_cycle(self.ptr, cycle, nu1, nu2))
+        else:
+            _check(lib().ela_amr_set_level_cycle(self.ptr, lev, cycle, nu1, nu2))
 
     def SetKrylov(self, restart):
         _check(lib().ela_amr_set_krylov(self.ptr, restart))

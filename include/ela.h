@@ -331,6 +331,10 @@ int ela_amr_set_bc(ela_amr* a, const int* types, const double* values);
 int ela_amr_bc_init(ela_amr* a, int lev, double* rhs_host_dense, int homogeneous);
 int ela_amr_set_uniform(ela_amr* a, int uniform);
 int ela_amr_set_cycle(ela_amr* a, int cycle, int nu1, int nu2);
+/* The box cycle of one level only (ela_amr_set_cycle sets every level).
+ * Defaults: F-cycle with nu1 = nu2 = 2 on level 0, V-cycle with nu1 = nu2 = 1
+ * on the refined boxes. */
+int ela_amr_set_level_cycle(ela_amr* a, int lev, int cycle, int nu1, int nu2);
 /* restart 0: the plain FAC iteration u += FAC(r). */
 int ela_amr_set_krylov(ela_amr* a, int restart);
 /* prepareForSolve on every level (Operator.cpp:445-452). */
