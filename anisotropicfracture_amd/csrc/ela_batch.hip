@@ -436,24 +436,14 @@ __device__ __forceinline__ GrainBox<T> grain_of(const BoxDesc& d, const MOps& op
     return g;
 }
 
-template <typename T, int MODE, typename TO, bool SC>
-__global__ __launch_bounds__(MTX* MTY) void km_grain(const BoxDesc* __restrict__ D, const int* __restrict__ bmap,
-                                                     MOps op) {
-    __shared__ MarchLds<T> sh;
-    extern __shared__ __align__(16) unsigned char dyn_lds[];
-    const int lb = op.xcd ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
-    const BoxDesc& d = D[bmap[lb]];
-    const double m = march_tile<T, MODE, TO, SC>(grain_of<T, MODE>(d, op), lb - d.mt0, sh, (T*)dyn_lds);
-    if (MODE == 2) block_max_at(op.partials, lb, m);
-}
-
-// Shell of a grain-form box, reading the descriptor in place (a local copy of
-// the operands would be indexed per lane and spill to scratch).
+// Shell block b of a grain-form launch, reading the descriptor in place (a
+// local copy of the operands would be indexed per lane and spill to scratch).
 template <typename T, int MODE, typename TO>
-__global__ __launch_bounds__(TB) void ks_grain(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, MOps op) {
-    const BoxDesc& d = D[bmap[blockIdx.x]];
+__device__ __forceinline__ void shell_block(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, int b,
+                                            const MOps& op) {
+    const BoxDesc& d = D[bmap[b]];
     const Lay L = d.L;
-    int q = ((int)blockIdx.x - d.st0) * TB + (int)threadIdx.x;
+    int q = (b - d.st0) * TB + (int)threadIdx.x;
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const int A = nx * ny, Bf = nx * (nz - 2), Cf = (ny - 2) * (nz - 2);
     const bool in = q < 2 * A + 2 * Bf + 2 * Cf;
@@ -501,6 +491,27 @@ __global__ __launch_bounds__(TB) void ks_grain(const BoxDesc* __restrict__ D, co
         }
     }
     if (MODE == 2) block_max(op.partials + op.mtot, m);
+}
+
+// One launch per grain-form level: blocks [0, nsp) compute the shells (the
+// last nsp - ns are padding), the others march the tiles.  The shells run
+// beside the tiles instead of after them; nsp is a multiple of 8, so a tile's
+// hardware block lands on the XCD that xcd_block assigns it.
+template <typename T, int MODE, typename TO, bool SC>
+__global__ __launch_bounds__(MTX* MTY) void kf_grain(const BoxDesc* __restrict__ D, const int* __restrict__ bmap_m,
+                                                     const int* __restrict__ bmap_s, int ns, int nsp, MOps op) {
+    const int b = (int)blockIdx.x;
+    if (b < nsp) {
+        if (b < ns) shell_block<T, MODE, TO>(D, bmap_s, b, op);
+        return;
+    }
+    __shared__ MarchLds<T> sh;
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    const int nm = (int)gridDim.x - nsp, bm = b - nsp;
+    const int lb = op.xcd ? xcd_block(bm, nm) : bm;
+    const BoxDesc& d = D[bmap_m[lb]];
+    const double m = march_tile<T, MODE, TO, SC>(grain_of<T, MODE>(d, op), lb - d.mt0, sh, (T*)dyn_lds);
+    if (MODE == 2) block_max_at(op.partials, lb, m);
 }
 
 // |r|inf of each box from the partials of its marching and shell blocks.
@@ -1027,14 +1038,15 @@ bool use_march(const ela_batch* B, const ela_batch::March& mi) {
 template <typename T, int MODE, typename TO>
 void march_launch(ela_batch* B, const BoxDesc* D, const ela_batch::March& mi, const MOps& op) {
     const size_t lds = (size_t)mi.maxng * kNC * sizeof(T);
+    const int nsp = (mi.ns + 7) / 8 * 8;
+    const int* bm = (const int*)(B->dlists + mi.off_bm);
+    const int* bs = (const int*)(B->dlists + mi.off_bs);
     if (mi.scaled)
-        hipLaunchKernelGGL((km_grain<T, MODE, TO, true>), dim3(mi.nm), dim3(MTX * MTY), lds, B->stream, D,
-                           (const int*)(B->dlists + mi.off_bm), op);
+        hipLaunchKernelGGL((kf_grain<T, MODE, TO, true>), dim3(nsp + mi.nm), dim3(MTX * MTY), lds, B->stream, D, bm,
+                           bs, mi.ns, nsp, op);
     else
-        hipLaunchKernelGGL((km_grain<T, MODE, TO, false>), dim3(mi.nm), dim3(MTX * MTY), lds, B->stream, D,
-                           (const int*)(B->dlists + mi.off_bm), op);
-    hipLaunchKernelGGL((ks_grain<T, MODE, TO>), dim3(mi.ns), dim3(TB), 0, B->stream, D,
-                       (const int*)(B->dlists + mi.off_bs), op);
+        hipLaunchKernelGGL((kf_grain<T, MODE, TO, false>), dim3(nsp + mi.nm), dim3(MTX * MTY), lds, B->stream, D, bm,
+                           bs, mi.ns, nsp, op);
 }
 
 template <typename T>

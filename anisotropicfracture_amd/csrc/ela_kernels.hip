@@ -881,28 +881,33 @@ void launch_newton_rhs(const Lay& L, int kind, const double* C, const double* F0
                            1.0);
 }
 
-// Grain-form z-marching stencil on one box (ela_grain.hpp): interior tiles, then the shell.
+// Grain-form z-marching stencil on one box (ela_grain.hpp), one launch: the
+// first nshp blocks compute the shell (boundary rows, latency-bound), the
+// others march the interior tiles, so the shell overlaps the tiles instead of
+// running after them.  nshp is a multiple of 8, so a tile's hardware block
+// lands on the XCD that xcd_block assigns it.
 namespace {
 template <typename T, int MODE, typename TO, bool SC>
-__global__ __launch_bounds__(MTX* MTY) void k_grain_march(GrainBox<T> g, double* partials) {
+__global__ __launch_bounds__(MTX* MTY) void k_grain_sweep(GrainBox<T> g, int nshp, double* partials) {
     __shared__ MarchLds<T> sh;
     extern __shared__ __align__(16) unsigned char dyn_lds[];
-    const int t = g.xcd ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
-    const double m = march_tile<T, MODE, TO, SC>(g, t, sh, (T*)dyn_lds);
+    const int b = (int)blockIdx.x;
+    double m;
+    if (b < nshp) {
+        m = shell_node<T, MODE, TO>(g, b * (MTX * MTY) + (int)threadIdx.x);
+    } else {
+        const int nm = (int)gridDim.x - nshp, bm = b - nshp;
+        const int t = g.xcd ? xcd_block(bm, nm) : bm;
+        m = march_tile<T, MODE, TO, SC>(g, t, sh, (T*)dyn_lds);
+    }
     if (MODE == 2 && partials) block_max_store(partials, m);  // a max: the slot order does not matter
-}
-
-template <typename T, int MODE, typename TO>
-__global__ __launch_bounds__(256) void k_grain_shell(GrainBox<T> g, double* partials) {
-    const double m = shell_node<T, MODE, TO>(g, (int)blockIdx.x * 256 + (int)threadIdx.x);
-    if (MODE == 2 && partials) block_max_store(partials, m);
 }
 }  // namespace
 
 long long grain_partials(const Lay& L, int kz) {
     int ntx, nty, ntz, nsh;
     grain_tiles(L, kz, ntx, nty, ntz, nsh);
-    return (long long)ntx * nty * ntz + nsh;
+    return (long long)ntx * nty * ntz + (nsh + 7) / 8 * 8;
 }
 
 template <typename T, int MODE, typename TO>
@@ -912,12 +917,15 @@ void launch_grain(GrainBox<T> g, int kz, double* partials, unsigned long long* n
     g.ntx = ntx;
     g.nty = nty;
     g.kz = kz;
-    const int nm = ntx * nty * ntz;
+    const int nm = ntx * nty * ntz, nshp = (nsh + 7) / 8 * 8;
     const size_t lds = (size_t)g.ngr * kGrainNC * sizeof(T);
-    if (g.sc) hipLaunchKernelGGL((k_grain_march<T, MODE, TO, true>), dim3(nm), dim3(MTX * MTY), lds, s, g, partials);
-    else hipLaunchKernelGGL((k_grain_march<T, MODE, TO, false>), dim3(nm), dim3(MTX * MTY), lds, s, g, partials);
-    hipLaunchKernelGGL((k_grain_shell<T, MODE, TO>), dim3(nsh), dim3(256), 0, s, g, partials ? partials + nm : nullptr);
-    if (MODE == 2 && norm) hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partials, nm + nsh, norm);
+    if (g.sc)
+        hipLaunchKernelGGL((k_grain_sweep<T, MODE, TO, true>), dim3(nshp + nm), dim3(MTX * MTY), lds, s, g, nshp,
+                           partials);
+    else
+        hipLaunchKernelGGL((k_grain_sweep<T, MODE, TO, false>), dim3(nshp + nm), dim3(MTX * MTY), lds, s, g, nshp,
+                           partials);
+    if (MODE == 2 && norm) hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partials, nm + nshp, norm);
 }
 template void launch_grain<float, 0, float>(GrainBox<float>, int, double*, unsigned long long*, hipStream_t);
 template void launch_grain<double, 0, double>(GrainBox<double>, int, double*, unsigned long long*, hipStream_t);

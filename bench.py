@@ -342,11 +342,11 @@ def main():
     tot_jac = kt["smooth0_ms"] * kt["smooth0_n"]
     tot_res = kt["resid64_ms"] * kt["resid64_n"]
     if tot_jac >= tot_res:
-        dom = ("k_grain_march<float,0> + k_grain_shell<float,0> (level-0 Jacobi sweep, grain form)" if args.table
+        dom = ("k_grain_sweep<float,0> (level-0 Jacobi sweep, grain form: shell + marching tiles)" if args.table
                else "k_jacobi<float,MM,3,PlaneAcc> (level-0 Jacobi sweep)")
         dom_ms, dom_bytes = kt["smooth0_ms"], bytes_jac32
     else:
-        dom = ("k_grain_march<double,%d> + k_grain_shell<double,%d> (level 0, grain form)" % ((3, 3) if args.krylov else (2, 2))
+        dom = ("k_grain_sweep<double,%d> (level 0, grain form)" % (3 if args.krylov else 2)
                if args.table else "k_residual<double,float,MM,3,PlaneAcc> (level 0)")
         dom_ms, dom_bytes = kt["resid64_ms"], bytes_res64
     peak = 8000.0
