@@ -268,6 +268,8 @@ def main():
     ap.add_argument("--serve-light", type=float, default=20.0,
                     help="arrival rate of a second, unsaturated serving run (0: skip it)")
     ap.add_argument("--amr-n", type=int, default=128, help="base cells of the AMR crack nest (0: skip it)")
+    ap.add_argument("--kz", type=int, default=0,
+                    help="marching tile height of the config-2 hierarchy (0: autotune it before the timed region)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md on sharding)",
@@ -286,7 +288,10 @@ def main():
     op.SetKrylov(args.krylov)
     op.prepareForSolve()
     tuned = None
-    if args.table:  # marching tile shape for this hierarchy, picked before the timed region
+    if args.table and args.kz:  # a fixed marching tile height (XCD-contiguous tile order)
+        op.set_option(ela.OPT_KZ, args.kz)
+        op.set_option(ela.OPT_XCD, 1)
+    elif args.table:  # marching tile shape for this hierarchy, picked before the timed region
         op.SetStiffnessMode(1)
         tuned = {"%d/%d" % k: v for k, v in op.autotune(nrep=10).items()}
         op.SetStiffnessMode(0)
