@@ -334,6 +334,12 @@ def main():
     # Fapply alone (FP64 and FP32), HIP events on the library stream
     fa64 = op.time_fapply(ela.F64, 20)
     fa32 = op.time_fapply(ela.F32, 20)
+    ga64 = ga32 = None
+    if args.table:  # the same operator with the grain-form stiffness (2-byte grain id per node)
+        op.SetStiffnessMode(1)
+        ga64 = op.time_fapply(ela.F64, 20)
+        ga32 = op.time_fapply(ela.F32, 20)
+        op.SetStiffnessMode(0)
     nc = 21
     bytes_fa64 = 8 * (3 + 3 + nc) * nodes
     bytes_fa32 = 4 * (3 + 3 + nc) * nodes
@@ -418,7 +424,12 @@ def main():
         "rel_disp_diff_mixed_vs_fp64": disp_diff,
         "fapply": {"fp64_ms": fa64, "fp64_GBps": bytes_fa64 / (fa64 * 1e-3) / 1e9,
                    "fp32_ms": fa32, "fp32_GBps": bytes_fa32 / (fa32 * 1e-3) / 1e9,
-                   "bytes_per_node_fp64": 8 * (3 + 3 + nc)},
+                   "bytes_per_node_fp64": 8 * (3 + 3 + nc),
+                   "grain_form": None if ga64 is None else {
+                       "fp64_ms": ga64, "fp64_GBps": (8 * 6 + 2) * nodes / (ga64 * 1e-3) / 1e9,
+                       "fp32_ms": ga32, "fp32_GBps": (4 * 6 + 2) * nodes / (ga32 * 1e-3) / 1e9,
+                       "bytes_per_node_fp64": 8 * 6 + 2, "bytes_per_node_fp32": 4 * 6 + 2,
+                       "fp64_speedup_over_planes": fa64 / ga64}},
         "kernel_times": kt,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": peak, "unit": "GB/s", "frac": achieved / peak,
                      "traffic": traffic, "kernel": dom, "kernel_ms": dom_ms, "bytes_per_launch": dom_bytes},
