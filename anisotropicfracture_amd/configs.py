@@ -64,8 +64,8 @@ def node_coords(n):
     return list(reversed(mesh))
 
 
-# ------------------------------------------------------------------ config 1
-def cfg1(n=64):
+# ------------------------------------------------------------------ 2D inclusion
+def inclusion_2d(n=64):
     """2D, n x n cells on [0,1]^2, isotropic E=210 nu=0.3 with a centred r=0.2 inclusion of 10x stiffness.
 
     BCs: YLO displacement (0,0) (its corners too, the reference default);
@@ -151,7 +151,7 @@ def uniaxial_z_bcs(strain=0.01):
     return bc_tables(3, spec)
 
 
-def cfg2(n=256, grains=64, seed=SEED):
+def polycrystal(n=256, grains=64, seed=SEED):
     """3D, n^3 cells, cubic (1.68,1.21,0.75) Voronoi polycrystal with Haar orientations, uniaxial z strain 0.01.
 
     Returns a dict with the grid size, quantized seeds, rotations and BC tables.
@@ -165,7 +165,7 @@ def cfg2(n=256, grains=64, seed=SEED):
 
 
 # ------------------------------------------------------------------ AMR crack (composite solve)
-def cfg3_boxes(n=128):
+def crack_nest_boxes(n=128):
     """The refined boxes of the mode-I crack on an n^3 base (ratio 2, node indices of each level).
 
     Level 1 covers |z - 0.5| <= 0.125, full x and y.  Level 2 covers |z - 0.5| <= 0.0625 and the
@@ -201,7 +201,7 @@ G_LITERAL = (1.0, 0.0, 1e-4)          # eta^2 + 1e-4: soft bulk, stiff crack pla
 G_DAMAGE = (1.0, -2.0, 1.0 + 1e-4)    # (1 - eta)^2 + 1e-4: eta as damage (Degradable/Isotropic.H:78-79)
 
 
-def cfg3(n=128, g=G_DAMAGE, E=1.0, nu=0.3, width=0.01, load=0.005):
+def crack_nest(n=128, g=G_DAMAGE, E=1.0, nu=0.3, width=0.01, load=0.005):
     """3D AMR mode-I crack: base n^3 cells on [0,1]^3, 3 levels (ratio 2).
 
     Stiffness g(eta) C_iso(E, nu) per node with g(eta) = a eta^2 + b eta + c.  The default is the
@@ -213,11 +213,11 @@ def cfg3(n=128, g=G_DAMAGE, E=1.0, nu=0.3, width=0.01, load=0.005):
     """
     lam, mu = lame(E, nu)
     t, v = mode1_bcs(load)
-    return {"n": (n, n, n), "boxes": cfg3_boxes(n), "lame": (lam, mu), "g": tuple(g), "width": width,
+    return {"n": (n, n, n), "boxes": crack_nest_boxes(n), "lame": (lam, mu), "g": tuple(g), "width": width,
             "bc_type": t, "bc_value": v}
 
 
-def cfg3_model(cfg, X, Y, Z):
+def crack_nest_model(cfg, X, Y, Z):
     """(lambda, mu) planes of the degraded isotropic stiffness at the nodes (X, Y, Z)."""
     a, b, c = cfg["g"]
     eta = crack_eta(X, Y, Z, cfg["width"])
@@ -237,7 +237,7 @@ def fixed_corner_gradient(strain=0.01):
     return G
 
 
-def cfg4(count=512, seed=SEED, sides=RVE_SIDES, grains=(8, 32)):
+def rve_batch(count=512, seed=SEED, sides=RVE_SIDES, grains=(8, 32)):
     """Batched RVEs: side n drawn uniformly from `sides`, 8-32 Voronoi grains with
     Haar orientations, cubic (1.68, 1.21, 0.75), Displacement on every face with
     u = 0.01 z e_z there (all corners included).
@@ -283,7 +283,7 @@ def random_crack(rng, n):
     return (p, nrm, tv, max(0.01, 2.0 / n), (1.0, -2.0, 1.0 + 1e-4))
 
 
-def cfg5(rate=200.0, duration=60.0, seed=SEED, nmin=32, nmax=192, grains=(8, 64), p_crack=0.5, strain=0.01):
+def request_stream(rate=200.0, duration=60.0, seed=SEED, nmin=32, nmax=192, grains=(8, 64), p_crack=0.5, strain=0.01):
     """Serving stream: Poisson arrivals (rate per s) over `duration` s; side log-uniform in
     [nmin, nmax] rounded to the nearest integer and then up to SERVE_SIDES (the requested side is
     kept as n_requested); 8-64 Voronoi grains

@@ -58,7 +58,7 @@ def cpu_baseline(n):
     from anisotropicfracture_amd import configs as K
     from oracle import oracle as O
     O.build()
-    cfg = K.cfg2(n)
+    cfg = K.polycrystal(n)
     gid = K.voronoi_grain_ids(cfg["seeds"], cfg["n"])
     tab = np.stack([O.cubic_define(cfg["C11"], cfg["C12"], cfg["C44"], R) for R in cfg["R"]])
     C = np.moveaxis(tab[gid], -1, 0).copy()
@@ -71,14 +71,14 @@ def cpu_baseline(n):
 
 
 def batched_rves(count, sample, tol):
-    """Config 4: `count` independent RVEs (sides 32/48/64/96, 8-32 grains, fixed-corner uniaxial z
+    """Batched RVEs: `count` independent RVEs (sides 32/48/64/96, 8-32 grains, fixed-corner uniaxial z
     strain 0.01).  Batched mixed and batched FP64 solve every RVE, one launch per cycle kernel over all
     of them; the plain FP64 GPU solver (ela_solve on one hierarchy per size, one RVE at a time, the same
     MG iteration) solves the first `sample` RVEs.  Timed per RVE: stiffness evaluation on the device,
     set-up (coarse stiffness, diagonals, coarsest inverse) and the solve."""
     import anisotropicfracture_amd as ela
     from anisotropicfracture_amd import configs as K
-    cfg = K.cfg4(count)
+    cfg = K.rve_batch(count)
     out = {"rves": count, "nodes": int(sum((m["n"] + 1) ** 3 for m in cfg))}
     cycles = {}
 
@@ -155,12 +155,12 @@ def batched_rves(count, sample, tol):
 
 
 def amr_crack(n, steps, tol, krylov):
-    """Config 3: the mode-I crack on a 3-level AMR nest (composite solve), mixed and FP64.
+    """The mode-I crack on a 3-level AMR nest (128^3 base) (composite solve), mixed and FP64.
 
     A step is prepareForSolve + one composite solve from u = 0 with b and the model resident."""
     import anisotropicfracture_amd as ela
     from anisotropicfracture_amd import configs as K
-    cfg = K.cfg3(n)
+    cfg = K.crack_nest(n)
     out = {"base": n, "boxes": [[list(lo), list(hi)] for lo, hi in cfg["boxes"]], "g": list(cfg["g"]),
            "width": cfg["width"]}
     sols = {}
@@ -171,7 +171,7 @@ def amr_crack(n, steps, tol, krylov):
         nodes = []
         for l in range(a.nlev):
             X = a.coords(l)
-            a.SetModel(l, K.cfg3_model(cfg, X[0], X[1], X[2]))
+            a.SetModel(l, K.crack_nest_model(cfg, X[0], X[1], X[2]))
             a.upload(a.B, l, a.bc_init(l))
             nodes.append(int(np.prod(a.shape(l)[1:])))
         zero = [np.zeros(a.shape(l)) for l in range(a.nlev)]
@@ -200,12 +200,12 @@ def amr_crack(n, steps, tol, krylov):
 
 
 def serving(seconds, max_wall, tol, budget_gb=200.0, max_active=256, rate=200.0, check=True):
-    """Config 5: the Poisson(rate) request stream (200/s), arrivals in the first `seconds` s, served by
+    """The serving stream: Poisson(rate) requests (200/s), arrivals in the first `seconds` s, served by
     the continuous-batching loop (ela_serve) under a `budget_gb` HBM budget, stopped after `max_wall` s.
     With check, two requests are re-solved alone afterwards and compared bit for bit."""
     import anisotropicfracture_amd as ela
     from anisotropicfracture_amd import configs as K
-    reqs = [q for q in K.cfg5(rate=rate) if q["arrival"] < seconds]
+    reqs = [q for q in K.request_stream(rate=rate) if q["arrival"] < seconds]
     ctx = ela.Context(0, int(budget_gb * 1e9))
     b = ela.Batch(ela.SOLVE_MIXED, ctx=ctx)
     keep = (0, min(40, len(reqs) - 1)) if check else ()
@@ -269,10 +269,10 @@ def main():
                     help="arrival rate of a second, unsaturated serving run (0: skip it)")
     ap.add_argument("--amr-n", type=int, default=128, help="base cells of the AMR crack nest (0: skip it)")
     ap.add_argument("--kz", type=int, default=0,
-                    help="marching tile height of the config-2 hierarchy (0: autotune it before the timed region)")
+                    help="marching tile height of the 256^3 polycrystal hierarchy (0: autotune it before the timed region)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md on sharding)",
+        print("bench.py measures one GPU: the solver has no multi-GPU path, so --gpus must be 1",
               file=sys.stderr)
         sys.exit(2)
 
@@ -280,8 +280,8 @@ def main():
     from anisotropicfracture_amd import configs as K
 
     n = args.n
-    log("bench: building config 256^3-class workload n=%d grains=%d" % (n, args.grains))
-    cfg = K.cfg2(n, grains=args.grains)
+    log("bench: building the polycrystal workload n=%d grains=%d" % (n, args.grains))
+    cfg = K.polycrystal(n, grains=args.grains)
     op = ela.Elastic(cfg["n"], ela.MM, mode=ela.ALLOC_BOTH)
     op.SetModelVoronoiCubic(cfg["seeds"], K.VORONOI_Q, cfg["R"], cfg["C11"], cfg["C12"], cfg["C44"])
     op.SetBC(cfg["bc_type"], cfg["bc_value"])

@@ -148,17 +148,17 @@ def test_plain_fac_iteration_converges(oracle, amr):
     a.close()
 
 
-def test_cfg3_crack_small_solution_is_a_composite_solution(oracle, amr):
+def test_crack_nest_small_solution_is_a_composite_solution(oracle, amr):
     """The mode-I crack nest (3 levels, base 16^3): the GPU solution's composite residual, evaluated
     by the oracle, is at the solve's tolerance, in both precisions; mixed and FP64 agree."""
     from anisotropicfracture_amd import configs as K
-    cfg = K.cfg3(16)
+    cfg = K.crack_nest(16)
     nest = amr.Nest(cfg["n"], cfg["boxes"])
     t, v = cfg["bc_type"], cfg["bc_value"]
     C = []
     for l in range(nest.nlev):
         X = nest.coords(l)
-        C.append(K.cfg3_model(cfg, X[0], X[1], X[2]))
+        C.append(K.crack_nest_model(cfg, X[0], X[1], X[2]))
     b = [nest.bc_init(l, t, v, np.zeros(nest.shape(l, 3))) for l in range(nest.nlev)]
     r0, _ = nest.residual(ISO, C, t, b, [np.zeros_like(x) for x in b])
     r0 = max(np.abs(x).max() for x in r0)
@@ -168,7 +168,7 @@ def test_cfg3_crack_small_solution_is_a_composite_solution(oracle, amr):
         for l in range(nest.nlev):
             assert np.array_equal(a.bc_init(l), b[l])  # BC::Init on the grown boxes
         u, st = a.solve([np.zeros_like(x) for x in b], b, tol_rel=1e-10, max_iter=300)
-        print("cfg3 n=16 mode", mode, "iters", st["iters"], "res/res0", st["res"] / st["res0"])
+        print("crack nest n=16 mode", mode, "iters", st["iters"], "res/res0", st["res"] / st["res0"])
         assert st["converged"]
         r, _ = nest.residual(ISO, C, t, b, u)
         assert max(np.abs(x).max() for x in r) <= 2e-10 * r0

@@ -226,7 +226,7 @@ def test_member_sizes_outside_the_dense_coarsest_limit_are_rejected():
 def test_serving_loop_results_match_alone_solves():
     """Requests served in the continuous-batching loop give bitwise the solution of the same
     request solved alone."""
-    reqs = K.cfg5(rate=400.0, duration=0.08, seed=7, nmin=24, nmax=48)
+    reqs = K.request_stream(rate=400.0, duration=0.08, seed=7, nmin=24, nmax=48)
     assert len(reqs) >= 10
     keep = (0, len(reqs) // 2, len(reqs) - 1)
     b = E.Batch(E.SOLVE_MIXED)

@@ -135,7 +135,7 @@ def test_fsmooth_parity(oracle):
 
 def test_voronoi_stiffness_kernel(oracle):
     """Grain assignment bit-exact with the integer nearest-seed checker; stiffness values to 1e-15."""
-    cfg = K.cfg2(24, grains=17, seed=9)
+    cfg = K.polycrystal(24, grains=17, seed=9)
     C_want, gid = H.voronoi_cubic_field(cfg, oracle.cubic_define)
     op = ela.Elastic(cfg["n"], MM, mode=ela.ALLOC_BOTH)
     op.SetModelVoronoiCubic(cfg["seeds"], K.VORONOI_Q, cfg["R"], cfg["C11"], cfg["C12"], cfg["C44"])
@@ -151,7 +151,7 @@ def test_voronoi_stiffness_kernel(oracle):
 
 
 def _solve_case(oracle, cfg_n=16, grains=12):
-    cfg = K.cfg2(cfg_n, grains=grains, seed=3)
+    cfg = K.polycrystal(cfg_n, grains=grains, seed=3)
     C, _ = H.voronoi_cubic_field(cfg, oracle.cubic_define)
     g = oracle.Grid(cfg["n"])
     b = oracle.bc_init(g, cfg["bc_type"], cfg["bc_value"], g.zeros())
@@ -207,12 +207,12 @@ def test_krylov_solve_matches_oracle_cubic(oracle, mode):
 
 
 @pytest.mark.parametrize("mode", [ela.SOLVE_FP64, ela.SOLVE_MIXED])
-def test_config1_inclusion_krylov_vs_direct(oracle, mode):
-    """Config-1 geometry at 32^2 (sharp 10x inclusion): the plain MG iteration of the reference
+def test_inclusion_2d_krylov_vs_direct(oracle, mode):
+    """The 2D inclusion at 32^2 (sharp 10x inclusion): the plain MG iteration of the reference
     diverges here; FGMRES(MG) reaches the discrete solution of a sparse direct solve."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spl
-    (nx, ny), C, t, v = K.cfg1(32)
+    (nx, ny), C, t, v = K.inclusion_2d(32)
     g = oracle.Grid((nx, ny))
     b = oracle.bc_init(g, t, v, g.zeros())
     A = sp.csc_matrix(oracle.assemble(g, ISO, C, t))
@@ -220,7 +220,7 @@ def test_config1_inclusion_krylov_vs_direct(oracle, mode):
     op = make((nx, ny), ISO, C, t, v)
     op.SetKrylov(24)
     u, st = ela.Linear(op).solve(np.zeros_like(b), b, tol_rel=1e-9, tol_abs=0.0, mode=mode, max_iter=400)
-    print("cfg1 32^2", st)
+    print("2D inclusion 32^2", st)
     assert st["converged"]
     assert np.abs(u - u_ref).max() / np.abs(u_ref).max() <= 1e-7
 
@@ -228,7 +228,7 @@ def test_config1_inclusion_krylov_vs_direct(oracle, mode):
 @pytest.mark.parametrize("n", [(24, 24, 24), (40, 20, 33)])
 def test_grain_table_fapply_matches_planes_and_oracle(oracle, n):
     """Grain-table stiffness (gid + per-grain records) gives the same operator as the per-node planes."""
-    cfg = K.cfg2(8, grains=13, seed=11)
+    cfg = K.polycrystal(8, grains=13, seed=11)
     rng = np.random.default_rng(4)
     cfg["n"] = n
     cfg["seeds"] = K.voronoi_seeds(rng, 13, n)
@@ -255,7 +255,7 @@ def test_grain_table_fapply_matches_planes_and_oracle(oracle, n):
 @pytest.mark.parametrize("mode", [ela.SOLVE_FP64, ela.SOLVE_MIXED])
 @pytest.mark.parametrize("krylov", [0, 16])
 def test_grain_table_solve_matches_oracle(oracle, mode, krylov):
-    cfg = K.cfg2(16, grains=12, seed=3)
+    cfg = K.polycrystal(16, grains=12, seed=3)
     C, _ = H.voronoi_cubic_field(cfg, oracle.cubic_define)
     g = oracle.Grid(cfg["n"])
     b = oracle.bc_init(g, cfg["bc_type"], cfg["bc_value"], g.zeros())
@@ -276,7 +276,7 @@ def test_grain_table_solve_matches_oracle(oracle, mode, krylov):
 def test_grain_marching_kernels_give_the_node_kernel_bits(mode, krylov):
     """Level 0 in grain form through the LDS z-marching kernels and through the one-thread-per-node
     kernels: the same arithmetic, so the same solve bit for bit (a 40^3 box has partial tiles in x, y, z)."""
-    cfg = K.cfg2(40, grains=14, seed=5)
+    cfg = K.polycrystal(40, grains=14, seed=5)
     sols, iters = [], []
     for march in (0, 1):
         op = ela.Elastic(cfg["n"], MM, mode=ela.ALLOC_BOTH)
@@ -297,7 +297,7 @@ def test_grain_marching_kernels_give_the_node_kernel_bits(mode, krylov):
 
 
 def test_autotune_picks_a_tile_shape_without_changing_results():
-    cfg = K.cfg2(40, grains=10, seed=6)
+    cfg = K.polycrystal(40, grains=10, seed=6)
     sols = []
     for tune in (False, True):
         op = ela.Elastic(cfg["n"], MM, mode=ela.ALLOC_BOTH)

@@ -269,8 +269,8 @@ def _direct(oracle, g, kind, C, t, b):
 
 
 def test_mg_matches_direct_solve_cubic_voronoi(oracle):
-    """The restated MG converges to the same discrete solution as a sparse direct solve (8^3 config-2 case)."""
-    cfg = K.cfg2(8, grains=6, seed=4)
+    """The restated MG converges to the same discrete solution as a sparse direct solve (the polycrystal workload at 8^3)."""
+    cfg = K.polycrystal(8, grains=6, seed=4)
     C, _ = H.voronoi_cubic_field(cfg, oracle.cubic_define)
     g = oracle.Grid(cfg["n"])
     t, v = cfg["bc_type"], cfg["bc_value"]
@@ -289,7 +289,7 @@ def test_mg_bicgstab_bottom_same_solution(oracle):
     lam, mu = K.lame(210.0, 0.3)
     C = np.zeros(g.shape(2))
     C[0], C[1] = lam, mu
-    _, _, t, v = K.cfg1(16)
+    _, _, t, v = K.inclusion_2d(16)
     b = oracle.bc_init(g, t, v, g.zeros())
     u1, s1, _ = oracle.mg_solve(g, ISO, C, t, b, tol_rel=1e-10, bottom="direct")
     u2, s2, _ = oracle.mg_solve(g, ISO, C, t, b, tol_rel=1e-10, bottom="bicgstab")
