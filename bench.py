@@ -258,7 +258,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed", help="comma list of fp64, mixed")
-    ap.add_argument("--krylov", type=int, default=8, help="FGMRES restart around the MG cycle (0: plain MG)")
+    ap.add_argument("--krylov", type=int, default=6, help="FGMRES restart around the MG cycle (0: plain MG)")
     ap.add_argument("--table", type=int, default=1, help="grain-table stiffness on level 0 for the headline mixed run")
     ap.add_argument("--rves", type=int, default=512, help="batched RVE count (0: skip the batched workload)")
     ap.add_argument("--plain-sample", type=int, default=24, help="RVEs solved one at a time by the plain FP64 solver")
