@@ -69,7 +69,7 @@ struct BoxDesc {
     const double* C64;  // planes (coarse levels)
     const float* C32;
     double* d64;  // Jacobi diagonal in the cycle precision (one of d64 / d32)
-    float* d32;
+    float* d32;   // FP32: its inverse
     void* f[F_N];  // cycle fields, cycle precision
     double* u64;   // finest level: solution, right-hand side, residual / test output
     double* b64;
@@ -143,7 +143,7 @@ __global__ __launch_bounds__(TB) void kb_smooth(const BoxDesc* __restrict__ D, c
     T* xo = (T*)d.f[out];
     if (in < 0) {
 #pragma unroll
-        for (int p = 0; p < 3; ++p) xo[p * L.cs + o] = omega * b[p * L.cs + o] / dg[p * L.cs + o];
+        for (int p = 0; p < 3; ++p) xo[p * L.cs + o] = jacobi_first(omega, b[p * L.cs + o], dg[p * L.cs + o]);
         return;
     }
     const T* xi = (const T*)d.f[in];
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(TB) void kb_smooth(const BoxDesc* __restrict__ D, c
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
         const long long q = p * L.cs + o;
-        xo[q] = xi[q] + omega * (b[q] - f[p]) / dg[q];
+        xo[q] = jacobi_update(xi[q], omega, b[q] - f[p], dg[q]);
     }
 }
 
@@ -505,7 +505,7 @@ __global__ __launch_bounds__(MTX* MTY) void kf_grain(const BoxDesc* __restrict__
         if (b < ns) shell_block<T, MODE, TO>(D, bmap_s, b, op);
         return;
     }
-    __shared__ MarchLds<T> sh;
+    __shared__ MarchLds<T, SC> sh;
     extern __shared__ __align__(16) unsigned char dyn_lds[];
     const int nm = (int)gridDim.x - nsp, bm = b - nsp;
     const int lb = op.xcd ? xcd_block(bm, nm) : bm;
@@ -583,7 +583,7 @@ __global__ __launch_bounds__(TB) void kb_diag(const BoxDesc* __restrict__ D, con
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
         if (d.d64) d.d64[p * L.cs + o] = dg[p];
-        if (d.d32) d.d32[p * L.cs + o] = (float)dg[p];
+        if (d.d32) store_jacobi_diag(d.d32 + p * L.cs + o, dg[p]);  // the inverse
         if (!(dg[p] == dg[p]) || dg[p] == 0.0) *flag = 1;
     }
 }
@@ -769,6 +769,7 @@ struct ela_batch {
         size_t off_bm = 0, off_bs = 0;  // device offsets of the marching / shell block maps
         int nm = 0, ns = 0, maxng = 0;  // marching blocks, shell blocks, largest grain count
         bool scaled = false;            // some box of the launch carries a per-node factor
+        bool fits = true;               // every box's fields within 32-bit offsets (grain_offsets_fit)
     };
     std::vector<March> mlev;
     March mtop;
@@ -946,6 +947,7 @@ void march_number(ListBuild& lb, int a, int b, int kz, std::vector<int>& bm, std
         mi.ns += d.nsh;
         mi.maxng = std::max(mi.maxng, d.ngr);
         mi.scaled = mi.scaled || d.sc64 != nullptr;
+        mi.fits = mi.fits && grain_offsets_fit(L);
         bm.insert(bm.end(), (size_t)(d.ntx * d.ntz * d.nty), x);
         bs.insert(bs.end(), (size_t)d.nsh, x);
     }
@@ -1032,7 +1034,7 @@ long long nodes_of(const ela_batch* B, int g, int count) {
 int split_of(const ela_batch* B, int g, int count) { return std::min(count, g > 0 ? B->cnt[g - 1] : 0); }
 
 bool use_march(const ela_batch* B, const ela_batch::March& mi) {
-    return B->march && mi.nm > 0 && mi.maxng <= kMarchMaxGrains;
+    return B->march && mi.nm > 0 && mi.maxng <= kMarchMaxGrains && mi.fits;
 }
 
 template <typename T, int MODE, typename TO>

@@ -80,7 +80,7 @@ template <typename T, int NC>
 struct TabAcc {
     const unsigned short* gid;
     const T* tab;
-    const T* dgt;  // per-grain interior diagonal (3 values), may be null
+    const T* dgt;  // per-grain interior diagonal (3 values; FP32: the inverse), may be null
     __device__ __forceinline__ TabRec<T> rec(long long o) const { return TabRec<T>{tab + (long long)gid[o] * NC}; }
     __device__ __forceinline__ bool same(long long a, long long b) const { return gid[a] == gid[b]; }
 };
@@ -97,7 +97,25 @@ struct GrainAcc {
     }
 };
 
-// Jacobi diagonal of a node: the stored field, or the per-grain interior value.
+// Weighted Jacobi update x + omega r / D (Operator.cpp:119-157).  FP64 divides
+// by the diagonal, as the reference does; the FP32 smoothers hold the inverse
+// diagonal, 1/D rounded once from the FP64 value, and multiply (a division is
+// a dozen instructions per component in the smoother's inner loop).
+__device__ __forceinline__ double jacobi_update(double x, double omega, double r, double d) {
+    return x + omega * r / d;
+}
+__device__ __forceinline__ float jacobi_update(float x, float omega, float r, float dinv) {
+    return x + omega * r * dinv;
+}
+// The first sweep from x = 0: omega b / D.
+__device__ __forceinline__ double jacobi_first(double omega, double b, double d) { return omega * b / d; }
+__device__ __forceinline__ float jacobi_first(float omega, float b, float dinv) { return omega * b * dinv; }
+// The value the smoothers of precision T store for diagonal d.
+__device__ __forceinline__ void store_jacobi_diag(double* p, double d) { *p = d; }
+__device__ __forceinline__ void store_jacobi_diag(float* p, double d) { *p = (float)(1.0 / d); }
+
+// Jacobi diagonal of a node as the smoothers hold it (the inverse in FP32):
+// the stored field, or the per-grain interior value.
 template <typename T, typename Acc>
 __device__ __forceinline__ T jdiag(const Acc&, const T* dg, long long q, long long, int, bool) {
     return dg[q];

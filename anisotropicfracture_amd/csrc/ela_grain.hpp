@@ -16,6 +16,8 @@
 namespace ela {
 
 constexpr int MTX = 32, MTY = 8, MRW = MTY + 2, MCW = MTX + 2, MPL = MRW * MCW;
+constexpr int MHALO = 2 * MCW + 2 * MTY;  // nodes of a tile plane's one-node halo ring
+static_assert(3 * MHALO <= MTX * MTY, "one halo node per thread and component");
 constexpr int kGrainNC = 21;
 constexpr int kMarchMaxGrains = 256;  // the table is staged in LDS up to this many grains
 
@@ -28,13 +30,13 @@ struct LdsCtx {
     const unsigned short* sg;  // [3 slots][MRW][MCW]
     const T* ss;               // [3 slots][MRW][MCW] (SC only)
     const T* stab;             // grain table
-    int sl[3];                 // ring slots of planes k-1, k, k+1
-    int r, c;                  // this node's row / column in the tile arrays
-    __device__ __forceinline__ int at(int dx, int dy, int dz) const {
-        return (sl[dz + 1] * MRW + r + dy) * MCW + c + dx;
-    }
+    int pb[3];                 // slot * MPL of planes k-1, k, k+1 (uniform over the block)
+    int tb;                    // this node's row * MCW + column in the tile arrays
+    // Index = uniform plane base + per-thread base + a compile-time constant,
+    // so each access is one LDS read with an immediate offset.
+    __device__ __forceinline__ int at(int dx, int dy, int dz) const { return pb[dz + 1] + tb + (dy * MCW + dx); }
     __device__ __forceinline__ T u(int p, int dx, int dy, int dz) const {
-        return su[((sl[dz + 1] * 3 + p) * MRW + r + dy) * MCW + c + dx];
+        return su[3 * pb[dz + 1] + tb + (p * MPL + dy * MCW + dx)];
     }
     __device__ __forceinline__ auto rec(int dx, int dy, int dz) const {
         const int q = at(dx, dy, dz);
@@ -73,19 +75,33 @@ struct GrainBox {
     T omega;
 };
 
+// The epilogue's operands of node o: rhs (MODES 0-2) and the diagonal (MODE 0).
+template <typename T, int MODE>
+__device__ __forceinline__ void epilogue_load(const GrainBox<T>& g, long long o, T rb[3], T db[3]) {
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const long long q = p * g.L.cs + o;
+        rb[p] = MODE != 3 ? g.rhs[q] : T(0);
+        db[p] = MODE == 0 ? g.dg[q] : T(1);
+    }
+}
+
+// Output of node o from its stencil value f, its current value xc and the
+// preloaded operands rb, db of epilogue_load.
 template <typename T, int MODE, typename TO>
-__device__ __forceinline__ double grain_epilogue(const GrainBox<T>& g, long long o, const T xc[3], const T f[3]) {
+__device__ __forceinline__ double grain_epilogue(const GrainBox<T>& g, long long o, const T xc[3], const T f[3],
+                                                 const T rb[3], const T db[3]) {
     const long long cs = g.L.cs;
     double m = 0.0;
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
         const long long q = p * cs + o;
         if (MODE == 0) {
-            ((T*)g.out)[q] = xc[p] + g.omega * (g.rhs[q] - f[p]) / g.dg[q];
+            ((T*)g.out)[q] = jacobi_update(xc[p], g.omega, rb[p] - f[p], db[p]);
         } else if (MODE == 1) {
-            ((T*)g.out)[q] = g.rhs[q] - f[p];
+            ((T*)g.out)[q] = rb[p] - f[p];
         } else if (MODE == 2) {
-            const double v = (double)g.rhs[q] - (double)f[p];
+            const double v = (double)rb[p] - (double)f[p];
             if (g.out) ((TO*)g.out)[q] = (TO)(v * *g.scale);
             if (g.out2) g.out2[q] = v;
             const double a = fabs(v);
@@ -97,18 +113,21 @@ __device__ __forceinline__ double grain_epilogue(const GrainBox<T>& g, long long
     return m;
 }
 
-// LDS arrays of one marching block.
-template <typename T>
+// LDS arrays of one marching block: a ring of MRING planes (three are read
+// while the fourth is written, so a plane needs one barrier, not two).  The
+// factor planes exist only in launches that carry factors (SC).
+constexpr int MRING = 4;
+template <typename T, bool SC>
 struct MarchLds {
-    T su[3 * 3 * MPL];
-    unsigned short sg[3 * MPL];
-    T ss[3 * MPL];
+    T su[MRING * 3 * MPL];
+    unsigned short sg[MRING * MPL];
+    T ss[SC ? MRING * MPL : 1];
 };
 
 // Tile t of box g (256 threads); returns the thread's max |v| (MODE 2).
 // Every thread of the block calls this (it synchronizes).
 template <typename T, int MODE, typename TO, bool SC>
-__device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchLds<T>& sh, T* stab) {
+__device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchLds<T, SC>& sh, T* stab) {
     static_assert(MODE != 2 || sizeof(T) == 8, "the solution residual is FP64");
     const Lay& L = g.L;
     const int tid = threadIdx.x;
@@ -120,79 +139,90 @@ __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchL
     const int kb = 1 + bz * g.kz, ke = min(kb + g.kz, L.nz - 1);
     for (int q = tid; q < g.ngr * kGrainNC; q += MTX * MTY) stab[q] = g.tab[q];
 
-    constexpr int PU = (3 * MPL + MTX * MTY - 1) / (MTX * MTY), PG = (MPL + MTX * MTY - 1) / (MTX * MTY);
-    T ru[PU], rs[PG];
-    unsigned short rg[PG];
+    // Plane loads.  Every thread reads its own node (i, j) of the 32 x 8 tile
+    // for the three components, at one 32-bit offset from the plane's base;
+    // the threads tid < 3 * MHALO read one node of the one-node halo ring each
+    // (component tid / MHALO), and the first MHALO of them, whose halo node is
+    // of component 0, also read its grain id (and factor).  Offsets fit 32
+    // bits: grain_offsets_fit holds for every box marched.
+    const int tb = (ty + 1) * MCW + tx + 1;  // this node's position in a tile plane
+    const bool iok = i < L.nx && j < L.ny;
+    const unsigned ie = (unsigned)(i + L.px * j);
+    const int hp = tid / MHALO, hh = tid - hp * MHALO;
+    const int hr = hh < MCW ? 0 : hh < 2 * MCW ? MRW - 1 : 1 + ((hh - 2 * MCW) >> 1);
+    const int hc = hh < MCW ? hh : hh < 2 * MCW ? hh - MCW : (((hh - 2 * MCW) & 1) ? MCW - 1 : 0);
+    const bool hok = tid < 3 * MHALO && i0 - 1 + hc < L.nx && j0 - 1 + hr < L.ny;
+    const bool gok = hok && tid < MHALO;
+    const unsigned he = (unsigned)(hp * L.cs + (i0 - 1 + hc) + L.px * (j0 - 1 + hr));
+    const int hpos = hp * MPL + hr * MCW + hc;
+    const bool scl = SC && g.sc != nullptr;  // a batched launch takes SC when any of its boxes has
+                                             // a factor; the boxes without one read s = 1
+    T ru[3], rh, rs = 1, rsh = 1;
+    unsigned short rg = 0, rgh = 0;
+    auto at = [](const void* base, unsigned bytes) { return (const char*)base + bytes; };
     auto load = [&](int kk) {
+        const T* up = g.in + L.pl * kk;
 #pragma unroll
-        for (int e = 0; e < PU; ++e) {
-            const int q = tid + e * MTX * MTY;
-            T v = 0;
-            if (q < 3 * MPL) {
-                const int p = q / MPL, rem = q - p * MPL, rr = rem / MCW, cc = rem - rr * MCW;
-                const int ii = i0 - 1 + cc, jj = j0 - 1 + rr;
-                if (ii < L.nx && jj < L.ny) v = g.in[p * L.cs + ii + L.px * jj + L.pl * kk];
-            }
-            ru[e] = v;
-        }
-#pragma unroll
-        for (int e = 0; e < PG; ++e) {
-            const int q = tid + e * MTX * MTY;
-            unsigned short gv = 0;
-            T sv = 1;
-            if (q < MPL) {
-                const int rr = q / MCW, cc = q - rr * MCW;
-                const int ii = i0 - 1 + cc, jj = j0 - 1 + rr;
-                if (ii < L.nx && jj < L.ny) {
-                    const long long o = ii + L.px * jj + L.pl * kk;
-                    gv = g.gid[o];
-                    // a batched launch takes SC when any of its boxes has a factor;
-                    // the boxes without one read s = 1
-                    if (SC && g.sc) sv = g.sc[o];
-                }
-            }
-            rg[e] = gv;
-            rs[e] = sv;
+        for (int p = 0; p < 3; ++p) ru[p] = iok ? *(const T*)at(up + p * L.cs, ie * (unsigned)sizeof(T)) : T(0);
+        rh = hok ? *(const T*)at(up, he * (unsigned)sizeof(T)) : T(0);
+        const unsigned short* gp = g.gid + L.pl * kk;
+        rg = iok ? *(const unsigned short*)at(gp, ie * 2u) : (unsigned short)0;
+        rgh = gok ? *(const unsigned short*)at(gp, he * 2u) : (unsigned short)0;
+        if constexpr (SC) {
+            const T* sp = g.sc + L.pl * kk;
+            rs = iok && scl ? *(const T*)at(sp, ie * (unsigned)sizeof(T)) : T(1);
+            rsh = gok && scl ? *(const T*)at(sp, he * (unsigned)sizeof(T)) : T(1);
         }
     };
     auto store = [&](int slot) {
+        T* su = sh.su + slot * 3 * MPL;
 #pragma unroll
-        for (int e = 0; e < PU; ++e) {
-            const int q = tid + e * MTX * MTY;
-            if (q < 3 * MPL) sh.su[slot * 3 * MPL + q] = ru[e];
-        }
-#pragma unroll
-        for (int e = 0; e < PG; ++e) {
-            const int q = tid + e * MTX * MTY;
-            if (q < MPL) {
-                sh.sg[slot * MPL + q] = rg[e];
-                if (SC) sh.ss[slot * MPL + q] = rs[e];
-            }
+        for (int p = 0; p < 3; ++p) su[p * MPL + tb] = ru[p];
+        if (tid < 3 * MHALO) su[hpos] = rh;
+        sh.sg[slot * MPL + tb] = rg;
+        if (tid < MHALO) sh.sg[slot * MPL + hpos] = rgh;
+        if constexpr (SC) {
+            sh.ss[slot * MPL + tb] = rs;
+            if (tid < MHALO) sh.ss[slot * MPL + hpos] = rsh;
         }
     };
+    // ring slots of planes k-1, k, k+1 and the free slot s3, which receives
+    // plane k+2 while plane k is computed
+    int s0 = 0, s1 = 1, s2 = 2, s3 = 3;
     load(kb - 1);
-    store((kb - 1) % 3);
+    store(s0);
     load(kb);
-    store(kb % 3);
+    store(s1);
     load(kb + 1);
-    store((kb + 1) % 3);
+    store(s2);
     __syncthreads();
     double m = 0.0;
     for (int k = kb; k < ke; ++k) {
         const bool more = k + 1 < ke;
+        const long long o = (long long)i + L.px * j + L.pl * k;
+        // the epilogue's loads are issued before the stencil, so their latency
+        // overlaps it, as the next plane's loads do
+        T rb[3], db[3];
+        if (active) epilogue_load<T, MODE>(g, o, rb, db);
         if (more) load(k + 2);
         if (active) {
-            LdsCtx<T, SC> S{sh.su, sh.sg, sh.ss, stab, {(k - 1) % 3, k % 3, (k + 1) % 3}, ty + 1, tx + 1};
+            LdsCtx<T, SC> S{sh.su, sh.sg, sh.ss, stab, {s0 * MPL, s1 * MPL, s2 * MPL}, tb};
             T f[3], xc[3];
             interior_apply<T, K_MM, 3>(S, g.iv, g.uniform != 0, f);
 #pragma unroll
             for (int p = 0; p < 3; ++p) xc[p] = S.u(p, 0, 0, 0);
-            const double mm = grain_epilogue<T, MODE, TO>(g, (long long)i + L.px * j + L.pl * k, xc, f);
+            const double mm = grain_epilogue<T, MODE, TO>(g, o, xc, f, rb, db);
             m = (mm > m || mm != mm) ? mm : m;
         }
-        __syncthreads();  // every read of plane k-1's slot is done
-        if (more) store((k + 2) % 3);
-        __syncthreads();
+        // s3 held plane k-2, whose last reads were in the previous plane,
+        // before the previous barrier
+        if (more) store(s3);
+        __syncthreads();  // plane k+2 is written; every read of plane k-1 is done
+        const int t0 = s0;
+        s0 = s1;
+        s1 = s2;
+        s2 = s3;
+        s3 = t0;
     }
     return m;
 }
@@ -229,7 +259,9 @@ __device__ __forceinline__ double shell_node(const GrainBox<T>& g, int q) {
     const long long o = (long long)i + L.px * j + L.pl * k;
 #pragma unroll
     for (int p = 0; p < 3; ++p) xc[p] = g.in[p * L.cs + o];
-    return grain_epilogue<T, MODE, TO>(g, o, xc, f);
+    T rb[3], db[3];
+    epilogue_load<T, MODE>(g, o, rb, db);
+    return grain_epilogue<T, MODE, TO>(g, o, xc, f, rb, db);
 }
 
 // XCD-contiguous block numbering: hardware block b of a launch of n blocks
@@ -242,6 +274,10 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
     if (x < full) return x * per + r;
     return full * per + (x - full) * (per - 1) + r;
 }
+
+// The marching kernels address a box's fields with 32-bit byte offsets from a
+// plane's base (three components of FP64 at most).
+__host__ __device__ inline bool grain_offsets_fit(const Lay& L) { return 3LL * L.cs * 8 < (1LL << 32); }
 
 // Tiles and shell blocks of a box (host and device).
 __host__ __device__ inline void grain_tiles(const Lay& L, int kz, int& ntx, int& nty, int& ntz, int& nsh) {

@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void k_jacobi(Lay L, Acc A, const T* __restric
 #pragma unroll
     for (int p = 0; p < DIM; ++p) {
         const long long q = p * L.cs + o;
-        xout[q] = xin[q] + omega * (b[q] - f[p]) / jdiag(A, dg, q, o, p, interior);
+        xout[q] = jacobi_update(xin[q], omega, b[q] - f[p], jdiag(A, dg, q, o, p, interior));
     }
 }
 
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(256) void k_jacobi0(Lay L, const T* __restrict__ dg
 #pragma unroll
     for (int p = 0; p < DIM; ++p) {
         const long long q = p * L.cs + o;
-        xout[q] = omega * b[q] / dg[q];
+        xout[q] = jacobi_first(omega, b[q], dg[q]);
     }
 }
 
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(256) void k_diag(Lay L, const double* __restrict__ 
 #pragma unroll
     for (int p = 0; p < DIM; ++p) {
         if (d64) d64[p * L.cs + o] = dg[p];
-        if (d32) d32[p * L.cs + o] = (float)dg[p];
+        if (d32) store_jacobi_diag(d32 + p * L.cs + o, dg[p]);  // the inverse
         if (!(dg[p] == dg[p]) || dg[p] == 0.0) atomicMax(nan_flag, 1ull);
     }
 }
@@ -250,13 +250,15 @@ __global__ __launch_bounds__(256) void k_axpy(Lay L, int nc, const TX* __restric
 // Operator<Node>::normalize (Operator.cpp:164-195): the reference's lambda
 // divides x(i,j,k) -- component 0 -- by diag(i,j,k) -- component 0 -- once
 // per component of the operator; the other components are left as they are.
+// The diagonal is the FP64 one, rounded to the precision of x.
 template <typename T>
-__global__ __launch_bounds__(256) void k_normalize(Lay L, int ncomp, const T* __restrict__ dg, T* __restrict__ x) {
+__global__ __launch_bounds__(256) void k_normalize(Lay L, int ncomp, const double* __restrict__ dg, T* __restrict__ x) {
     int i, j, k;
     if (!node_index(L, i, j, k)) return;
     const long long o = (long long)i + L.px * j + L.pl * k;
+    const T d = (T)dg[o];
     T v = x[o];
-    for (int n = 0; n < ncomp; ++n) v /= dg[o];
+    for (int n = 0; n < ncomp; ++n) v /= d;
     x[o] = v;
 }
 
@@ -634,11 +636,11 @@ void launch_axpy(const Lay& L, int nc, const TX* x, TY* y, const double* ascale,
 }
 
 template <typename T>
-void launch_normalize(const Lay& L, int ncomp, const T* dg, T* x, hipStream_t s) {
+void launch_normalize(const Lay& L, int ncomp, const double* dg, T* x, hipStream_t s) {
     hipLaunchKernelGGL((k_normalize<T>), node_grid(L), dim3(BX, BY), 0, s, L, ncomp, dg, x);
 }
 template void launch_normalize<double>(const Lay&, int, const double*, double*, hipStream_t);
-template void launch_normalize<float>(const Lay&, int, const float*, float*, hipStream_t);
+template void launch_normalize<float>(const Lay&, int, const double*, float*, hipStream_t);
 
 template <typename TX, typename TY>
 void launch_convert(long long n, const TX* x, TY* y, hipStream_t s) {
@@ -888,8 +890,8 @@ void launch_newton_rhs(const Lay& L, int kind, const double* C, const double* F0
 // lands on the XCD that xcd_block assigns it.
 namespace {
 template <typename T, int MODE, typename TO, bool SC>
-__global__ __launch_bounds__(MTX* MTY) void k_grain_sweep(GrainBox<T> g, int nshp, double* partials) {
-    __shared__ MarchLds<T> sh;
+__global__ __launch_bounds__(MTX* MTY) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1))) void k_grain_sweep(GrainBox<T> g, int nshp, double* partials) {
+    __shared__ MarchLds<T, SC> sh;
     extern __shared__ __align__(16) unsigned char dyn_lds[];
     const int b = (int)blockIdx.x;
     double m;
@@ -946,7 +948,7 @@ __global__ void k_grain_diag(int ng, const double* __restrict__ tab, double hx, 
         double acc = 0;
         for (int b = 0; b < 3; ++b) acc += tab[21 * gi + mm_slot<3>(p, b, p, b)] * (-2.0 / h[b] / h[b]);
         if (d64) d64[3 * gi + p] = acc;
-        if (d32) d32[3 * gi + p] = (float)acc;
+        if (d32) store_jacobi_diag(d32 + 3 * gi + p, acc);  // the inverse
     }
 }
 

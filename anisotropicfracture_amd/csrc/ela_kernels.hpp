@@ -53,7 +53,7 @@ void launch_convert(long long n, const TX* x, TY* y, hipStream_t s);
 
 // x0 /= diag0, ncomp times (component 0 only, as the reference's normalize does).
 template <typename T>
-void launch_normalize(const Lay& L, int ncomp, const T* dg, T* x, hipStream_t s);
+void launch_normalize(const Lay& L, int ncomp, const double* dg, T* x, hipStream_t s);
 
 void launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
 void launch_scale_from_norm(const unsigned long long* norm, double* scale, hipStream_t s);

@@ -48,7 +48,7 @@ __device__ __forceinline__ void epilogue(const OpArgs<T, TO>& a, long long cs, l
         if (MODE == 0) {
             a.out[q] = f[p];
         } else if (MODE == 1) {
-            a.out[q] = xc[p] + a.omega * (a.b[q] - f[p]) / a.dg[q];
+            a.out[q] = jacobi_update(xc[p], a.omega, a.b[q] - f[p], a.dg[q]);
         } else {
             const T v = a.b[q] - f[p];
             if (a.out) a.out[q] = v;

@@ -80,7 +80,7 @@ struct Level {
     double* C64 = nullptr;
     float* C32 = nullptr;
     double* d64 = nullptr;
-    float* d32 = nullptr;
+    float* d32 = nullptr;  // inverse diagonal of the FP32 smoothers
     double *cor64 = nullptr, *tmp64 = nullptr, *res64 = nullptr, *rc64 = nullptr, *hold64 = nullptr;
     float *cor32 = nullptr, *tmp32 = nullptr, *res32 = nullptr, *rc32 = nullptr, *hold32 = nullptr;
 };
@@ -128,7 +128,7 @@ struct ela_hier {
     // grain-table stiffness of level 0 (Voronoi models): per-node grain id, per-grain records
     unsigned short* gid = nullptr;
     double *tab64 = nullptr, *dgt64 = nullptr;
-    float *tab32 = nullptr, *dgt32 = nullptr;
+    float *tab32 = nullptr, *dgt32 = nullptr;  // dgt32: inverse of the per-grain interior diagonal
     int ngrains = 0;
     bool has_table = false, use_table = false;
     // z-marching kernels for the grain-form level 0 (planes per tile), and the
@@ -583,7 +583,8 @@ void grain_resid64<double, float>(const GrainBox<double>& g, int kz, double* p, 
 
 // The grain-form level 0 through the z-marching kernels.
 bool grain_on(ela_hier* h, const Level& v) {
-    return table_on(h, v) && h->grain_march && h->ngrains <= kMarchMaxGrains && h->dbc != nullptr;
+    return table_on(h, v) && h->grain_march && h->ngrains <= kMarchMaxGrains && h->dbc != nullptr &&
+           grain_offsets_fit(v.L);
 }
 
 template <typename T>
@@ -1154,14 +1155,23 @@ int ela_normalize(ela_hier* h, int mglev, int prec, void* x, void* stream) {
     if (!h->prepared) return fail(ELA_ERR_STATE, "Operator::Diagonal() must be called before using normalize");
     if (!x) return fail(ELA_ERR_BADARG, "null field");
     Level& v = h->lev[mglev];
-    if (prec == ELA_F64) {
-        if (!v.d64) return fail(ELA_ERR_STATE, "no FP64 diagonal on this hierarchy");
-        launch_normalize<double>(v.L, h->dim, v.d64, (double*)x, strm(h, stream));
-    } else {
-        if (!v.d32) return fail(ELA_ERR_STATE, "no FP32 diagonal on this hierarchy");
-        launch_normalize<float>(v.L, h->dim, v.d32, (float*)x, strm(h, stream));
+    // the FP64 diagonal (the FP32 smoothers keep only its inverse): the
+    // level's, or evaluated into a temporary
+    const double* dg = v.d64;
+    void* tmp = nullptr;
+    hipStream_t s = strm(h, stream);
+    if (!dg) {
+        if (int rc = halloc(h, field_elems(v, h->dim) * 8, &tmp)) return rc;
+        launch_diag(v.L, h->kind, v.C64, (double*)tmp, nullptr, h->bc, v.h, (unsigned long long*)h->dflag, s);
+        dg = (const double*)tmp;
     }
+    if (prec == ELA_F64) launch_normalize<double>(v.L, h->dim, dg, (double*)x, s);
+    else launch_normalize<float>(v.L, h->dim, dg, (float*)x, s);
     HIPCHK(hipGetLastError());
+    if (tmp) {
+        HIPCHK(hipStreamSynchronize(s));
+        hfree(h, tmp);
+    }
     return ELA_OK;
 }
 
@@ -1175,8 +1185,22 @@ int ela_diagonal(ela_hier* h, int mglev, int prec, void* diag, void* stream) {
         if (v.d64) HIPCHK(hipMemcpyAsync(diag, v.d64, fe * 8, hipMemcpyDeviceToDevice, strm(h, stream)));
         else launch_diag(v.L, h->kind, v.C64, (double*)diag, nullptr, h->bc, v.h, (unsigned long long*)h->dflag, strm(h, stream));
     } else {
-        if (v.d32) HIPCHK(hipMemcpyAsync(diag, v.d32, fe * 4, hipMemcpyDeviceToDevice, strm(h, stream)));
-        else launch_diag(v.L, h->kind, v.C64, nullptr, (float*)diag, h->bc, v.h, (unsigned long long*)h->dflag, strm(h, stream));
+        // the FP64 diagonal rounded to FP32 (the level's FP32 array holds the
+        // smoothers' inverse)
+        hipStream_t s = strm(h, stream);
+        void* tmp = nullptr;
+        const double* dg = v.d64;
+        if (!dg) {
+            if (int rc = halloc(h, fe * 8, &tmp)) return rc;
+            launch_diag(v.L, h->kind, v.C64, (double*)tmp, nullptr, h->bc, v.h, (unsigned long long*)h->dflag, s);
+            dg = (const double*)tmp;
+        }
+        launch_convert<double, float>((long long)fe, dg, (float*)diag, s);
+        HIPCHK(hipGetLastError());
+        if (tmp) {
+            HIPCHK(hipStreamSynchronize(s));
+            hfree(h, tmp);
+        }
     }
     return ELA_OK;
 }
