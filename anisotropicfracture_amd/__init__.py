@@ -32,7 +32,7 @@ DISP, TRAC, PERIODIC, NEUMANN = 0, 1, 2, 3
 F64, F32 = 0, 1
 SOLVE_FP64, SOLVE_MIXED, ALLOC_BOTH = 0, 1, 2
 CYCLE_F, CYCLE_V = 0, 1
-OPT_MARCH, OPT_KZ, OPT_XCD = 0, 1, 2
+OPT_MARCH, OPT_KZ, OPT_XCD, OPT_MASK = 0, 1, 2, 3
 
 ERRORS = {1: "BADARG", 2: "OOM", 3: "HIP", 4: "NONCONVERGED", 5: "NAN", 6: "STATE"}
 
@@ -360,8 +360,9 @@ class Elastic:
         _check(lib().ela_set_stiffness_mode(self.ptr, int(mode)))
 
     def set_option(self, option, value):
-        """ela_set_option: OPT_MARCH (LDS z-marching kernels for grain-form stiffness, default on) or
-        OPT_KZ (planes per marching tile, default 16)."""
+        """ela_set_option: OPT_MARCH (LDS z-marching kernels for grain-form stiffness, default on),
+        OPT_KZ (planes per marching tile, default 16), OPT_XCD (tile order) or OPT_MASK (plane levels
+        skip the derivative of C where it is exactly zero: 0 never, 1 where it pays, 2 always)."""
         _check(lib().ela_set_option(self.ptr, option, int(value)))
 
     def autotune(self, nrep=10):

@@ -69,12 +69,25 @@ struct ScaledRec {
     __device__ __forceinline__ T operator[](int q) const { return s * p[q]; }
 };
 
+// same_dir(o, d, a, b): the stiffness records of nodes a = o + e_d and
+// b = o - e_d are bitwise equal, so the d-derivative of C at o is exactly 0
+// and its term may be skipped.  PlaneAcc never skips; MaskedPlaneAcc reads a
+// per-node mask (bit d) built from the level's FP64 planes (equal FP64
+// records stay equal in FP32).
 template <typename T>
 struct PlaneAcc {
     const T* C;
     long long cs;
     __device__ __forceinline__ PlaneRec<T> rec(long long o) const { return PlaneRec<T>{C + o, cs}; }
-    __device__ __forceinline__ bool same(long long, long long) const { return false; }
+    __device__ __forceinline__ bool same_dir(long long, int, long long, long long) const { return false; }
+};
+template <typename T>
+struct MaskedPlaneAcc {
+    const T* C;
+    long long cs;
+    const unsigned char* sm;
+    __device__ __forceinline__ PlaneRec<T> rec(long long o) const { return PlaneRec<T>{C + o, cs}; }
+    __device__ __forceinline__ bool same_dir(long long o, int d, long long, long long) const { return (sm[o] >> d) & 1; }
 };
 template <typename T, int NC>
 struct TabAcc {
@@ -82,7 +95,7 @@ struct TabAcc {
     const T* tab;
     const T* dgt;  // per-grain interior diagonal (3 values; FP32: the inverse), may be null
     __device__ __forceinline__ TabRec<T> rec(long long o) const { return TabRec<T>{tab + (long long)gid[o] * NC}; }
-    __device__ __forceinline__ bool same(long long a, long long b) const { return gid[a] == gid[b]; }
+    __device__ __forceinline__ bool same_dir(long long, int, long long a, long long b) const { return gid[a] == gid[b]; }
 };
 template <typename T, int NC>
 struct GrainAcc {
@@ -92,7 +105,7 @@ struct GrainAcc {
     __device__ __forceinline__ ScaledRec<T> rec(long long o) const {
         return ScaledRec<T>{tab + (long long)gid[o] * NC, s ? s[o] : T(1)};
     }
-    __device__ __forceinline__ bool same(long long a, long long b) const {
+    __device__ __forceinline__ bool same_dir(long long, int, long long a, long long b) const {
         return gid[a] == gid[b] && (!s || s[a] == s[b]);
     }
 };
@@ -425,7 +438,7 @@ struct GlobalCtx {
     __device__ __forceinline__ T u(int p, int dx, int dy, int dz) const { return up[p * cs + at(dx, dy, dz)]; }
     __device__ __forceinline__ auto rec(int dx, int dy, int dz) const { return A.rec(at(dx, dy, dz)); }
     __device__ __forceinline__ bool same(int d) const {
-        return A.same(at(d == 0, d == 1, d == 2), at(-(d == 0), -(d == 1), -(d == 2)));
+        return A.same_dir(o, d, at(d == 0, d == 1, d == 2), at(-(d == 0), -(d == 1), -(d == 2)));
     }
 };
 

@@ -556,28 +556,39 @@ Inv<T> make_inv(const double h[3]) {
 
 template <typename T>
 void launch_fapply(const Lay& L, int kind, const T* C, const T* u, T* f, const BCTab& bc, const double h[3],
-                   int uniform, hipStream_t s) {
+                   int uniform, hipStream_t s, const unsigned char* sm) {
     const Inv<T> iv = make_inv<T>(h);
     const bool sl = L.dim == 3 && g_slab_order && (sizeof(T) == 4 || g_slab_f64);
     const Slab S = sl ? slab_of(L) : Slab{0, 0, 0, 0};
     const dim3 grid = sl ? slab_grid(L) : node_grid(L);
-    ELA_DISPATCH(kind, L.dim,
-                 hipLaunchKernelGGL((k_fapply<T, KK, DD, PlaneAcc<T>>), grid, dim3(BX, BY), 0, s, L,
-                                    (PlaneAcc<T>{C, L.cs}), u, f, bc, iv, uniform, S));
+    if (sm)
+        ELA_DISPATCH(kind, L.dim,
+                     hipLaunchKernelGGL((k_fapply<T, KK, DD, MaskedPlaneAcc<T>>), grid, dim3(BX, BY), 0, s, L,
+                                        (MaskedPlaneAcc<T>{C, L.cs, sm}), u, f, bc, iv, uniform, S));
+    else
+        ELA_DISPATCH(kind, L.dim,
+                     hipLaunchKernelGGL((k_fapply<T, KK, DD, PlaneAcc<T>>), grid, dim3(BX, BY), 0, s, L,
+                                        (PlaneAcc<T>{C, L.cs}), u, f, bc, iv, uniform, S));
 }
 
 template <typename T, typename TO>
 void launch_residual(const Lay& L, int kind, const T* C, const T* x, const T* b, T* r, TO* rs, const double* scale,
                      double* partials, unsigned long long* norm, const BCTab& bc, const double h[3], int uniform,
-                     hipStream_t s) {
+                     hipStream_t s, const unsigned char* sm) {
     const Inv<T> iv = make_inv<T>(h);
     const bool sl = L.dim == 3 && g_slab_order && (sizeof(T) == 4 || g_slab_f64);
     const Slab S = sl ? slab_of(L) : Slab{0, 0, 0, 0};
     const dim3 grid = sl ? slab_grid(L) : node_grid(L);
-    ELA_DISPATCH(kind, L.dim,
-                 hipLaunchKernelGGL((k_residual<T, TO, KK, DD, PlaneAcc<T>>), grid, dim3(BX, BY), 0, s, L,
-                                    (PlaneAcc<T>{C, L.cs}), x, b, r, rs, scale, norm ? partials : nullptr, bc, iv,
-                                    uniform, S));
+    if (sm)
+        ELA_DISPATCH(kind, L.dim,
+                     hipLaunchKernelGGL((k_residual<T, TO, KK, DD, MaskedPlaneAcc<T>>), grid, dim3(BX, BY), 0, s, L,
+                                        (MaskedPlaneAcc<T>{C, L.cs, sm}), x, b, r, rs, scale,
+                                        norm ? partials : nullptr, bc, iv, uniform, S));
+    else
+        ELA_DISPATCH(kind, L.dim,
+                     hipLaunchKernelGGL((k_residual<T, TO, KK, DD, PlaneAcc<T>>), grid, dim3(BX, BY), 0, s, L,
+                                        (PlaneAcc<T>{C, L.cs}), x, b, r, rs, scale, norm ? partials : nullptr, bc,
+                                        iv, uniform, S));
     if (norm) {
         const dim3 g = grid;
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partials, (int)(g.x * g.y * g.z), norm);
@@ -595,14 +606,61 @@ int g_slab_f64 = 1;
 
 template <typename T>
 void launch_jacobi(const Lay& L, int kind, const T* C, const T* dg, const T* xin, const T* b, T* xout,
-                   const BCTab& bc, const double h[3], int uniform, double omega, hipStream_t s) {
+                   const BCTab& bc, const double h[3], int uniform, double omega, hipStream_t s,
+                   const unsigned char* sm) {
     const Inv<T> iv = make_inv<T>(h);
     const bool sl = L.dim == 3 && g_slab_order && (sizeof(T) == 4 || g_slab_f64);
     const Slab S = sl ? slab_of(L) : Slab{0, 0, 0, 0};
     const dim3 grid = sl ? slab_grid(L) : node_grid(L);
-    ELA_DISPATCH(kind, L.dim,
-                 hipLaunchKernelGGL((k_jacobi<T, KK, DD, PlaneAcc<T>>), grid, dim3(BX, BY), 0, s, L,
-                                    (PlaneAcc<T>{C, L.cs}), dg, xin, b, xout, bc, iv, uniform, (T)omega, S));
+    if (sm)
+        ELA_DISPATCH(kind, L.dim,
+                     hipLaunchKernelGGL((k_jacobi<T, KK, DD, MaskedPlaneAcc<T>>), grid, dim3(BX, BY), 0, s, L,
+                                        (MaskedPlaneAcc<T>{C, L.cs, sm}), dg, xin, b, xout, bc, iv, uniform,
+                                        (T)omega, S));
+    else
+        ELA_DISPATCH(kind, L.dim,
+                     hipLaunchKernelGGL((k_jacobi<T, KK, DD, PlaneAcc<T>>), grid, dim3(BX, BY), 0, s, L,
+                                        (PlaneAcc<T>{C, L.cs}), dg, xin, b, xout, bc, iv, uniform, (T)omega, S));
+}
+
+// Same-direction mask of a plane-form level (PlaneAcc::same_dir): bit d of an
+// interior node is set when all nc stiffness planes hold bitwise-equal values
+// at its two d-neighbours; boundary nodes get 0.  A wavefront (64 nodes along
+// x) skips the d-derivative of C only when all its interior nodes have bit d,
+// so stats[0] counts the wavefronts with an interior node and stats[1] the
+// (wavefront, d) pairs that skip.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_same_mask(Lay L, int nc, const double* __restrict__ C,
+                                                   unsigned char* __restrict__ sm, unsigned long long* stats) {
+    int i, j, k;
+    const bool valid = node_index(L, i, j, k);
+    const bool in = valid && i > 0 && i < L.nx - 1 && j > 0 && j < L.ny - 1 && (DIM == 2 || (k > 0 && k < L.nz - 1));
+    const long long o = (long long)i + L.px * j + L.pl * k;
+    unsigned m = 0;
+    if (in) {
+        const long long st[3] = {1, L.px, L.pl};
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            bool eq = true;
+            for (int q = 0; q < nc && eq; ++q) eq = C[q * L.cs + o + st[d]] == C[q * L.cs + o - st[d]];
+            m |= (eq ? 1u : 0u) << d;
+        }
+    }
+    if (valid) sm[o] = (unsigned char)m;
+    const unsigned long long wi = __ballot(in);
+    unsigned long long skip = 0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) skip += (wi != 0 && __ballot(in && ((m >> d) & 1)) == wi) ? 1 : 0;
+    if (threadIdx.x == 0 && wi != 0) {  // one lane per wavefront (a wavefront is one 64-wide row)
+        atomicAdd(stats, 1ull);
+        atomicAdd(stats + 1, skip);
+    }
+}
+
+void launch_same_mask(const Lay& L, int nc, const double* C, unsigned char* sm, unsigned long long* stats,
+                      hipStream_t s) {
+    if (L.dim == 3) hipLaunchKernelGGL(k_same_mask<3>, node_grid(L), dim3(BX, BY), 0, s, L, nc, C, sm, stats);
+    else hipLaunchKernelGGL(k_same_mask<2>, node_grid(L), dim3(BX, BY), 0, s, L, nc, C, sm, stats);
 }
 
 template <typename T>
@@ -977,22 +1035,22 @@ template void launch_residual_tab<float, float>(const Lay&, const TabDesc<float>
 
 // explicit instantiations
 template void launch_fapply<double>(const Lay&, int, const double*, const double*, double*, const BCTab&,
-                                    const double*, int, hipStream_t);
+                                    const double*, int, hipStream_t, const unsigned char*);
 template void launch_fapply<float>(const Lay&, int, const float*, const float*, float*, const BCTab&, const double*,
-                                   int, hipStream_t);
+                                   int, hipStream_t, const unsigned char*);
 template void launch_residual<double, double>(const Lay&, int, const double*, const double*, const double*, double*,
                                               double*, const double*, double*, unsigned long long*, const BCTab&,
-                                              const double*, int, hipStream_t);
+                                              const double*, int, hipStream_t, const unsigned char*);
 template void launch_residual<double, float>(const Lay&, int, const double*, const double*, const double*, double*,
                                              float*, const double*, double*, unsigned long long*, const BCTab&, const double*,
-                                             int, hipStream_t);
+                                             int, hipStream_t, const unsigned char*);
 template void launch_residual<float, float>(const Lay&, int, const float*, const float*, const float*, float*,
                                             float*, const double*, double*, unsigned long long*, const BCTab&, const double*,
-                                            int, hipStream_t);
+                                            int, hipStream_t, const unsigned char*);
 template void launch_jacobi<double>(const Lay&, int, const double*, const double*, const double*, const double*,
-                                    double*, const BCTab&, const double*, int, double, hipStream_t);
+                                    double*, const BCTab&, const double*, int, double, hipStream_t, const unsigned char*);
 template void launch_jacobi<float>(const Lay&, int, const float*, const float*, const float*, const float*, float*,
-                                   const BCTab&, const double*, int, double, hipStream_t);
+                                   const BCTab&, const double*, int, double, hipStream_t, const unsigned char*);
 template void launch_jacobi0<double>(const Lay&, const double*, const double*, double*, double, hipStream_t);
 template void launch_jacobi0<float>(const Lay&, const float*, const float*, float*, double, hipStream_t);
 template void launch_restrict<double>(const Lay&, const Lay&, int, const double*, double*, hipStream_t);

@@ -17,19 +17,28 @@ Inv<T> make_inv(const double h[3]);
 
 template <typename T>
 void launch_fapply(const Lay& L, int kind, const T* C, const T* u, T* f, const BCTab& bc, const double h[3],
-                   int uniform, hipStream_t s);
+                   int uniform, hipStream_t s, const unsigned char* sm = nullptr);
 
 // r = b - A x; r (T), rs = r * (*scale) (TO), max|r| into *norm (bit pattern of the double,
 // via residual_partials(L) doubles of scratch in partials); any output may be null.
 template <typename T, typename TO>
 void launch_residual(const Lay& L, int kind, const T* C, const T* x, const T* b, T* r, TO* rs, const double* scale,
                      double* partials, unsigned long long* norm, const BCTab& bc, const double h[3], int uniform,
-                     hipStream_t s);
+                     hipStream_t s, const unsigned char* sm = nullptr);
 long long residual_partials(const Lay& L);
 
 template <typename T>
 void launch_jacobi(const Lay& L, int kind, const T* C, const T* dg, const T* xin, const T* b, T* xout,
-                   const BCTab& bc, const double h[3], int uniform, double omega, hipStream_t s);
+                   const BCTab& bc, const double h[3], int uniform, double omega, hipStream_t s,
+                   const unsigned char* sm = nullptr);
+
+// sm (one byte per node): bit d set where the stiffness records at the two
+// d-neighbours of an interior node are bitwise equal (the plane launchers
+// above then skip that d-derivative of C, whose terms are exactly zero).
+// stats[0] += wavefronts with an interior node, stats[1] += (wavefront,
+// direction) pairs whose interior nodes all have the bit (the work skipped).
+void launch_same_mask(const Lay& L, int nc, const double* C, unsigned char* sm, unsigned long long* stats,
+                      hipStream_t s);
 
 template <typename T>
 void launch_jacobi0(const Lay& L, const T* dg, const T* b, T* xout, double omega, hipStream_t s);

@@ -249,8 +249,14 @@ int ela_set_krylov(ela_hier* h, int restart);
  * ELA_OPT_KZ     planes marched per tile (default 16).
  * ELA_OPT_XCD    1: consecutive tiles of a marching launch run on one XCD
  *                (shared halo lines stay in its L2); 0: hardware order.
- *                Default 1 for a hierarchy, 0 for a batch. */
-enum { ELA_OPT_MARCH = 0, ELA_OPT_KZ = 1, ELA_OPT_XCD = 2 };
+ *                Default 1 for a hierarchy, 0 for a batch.
+ * ELA_OPT_MASK   per-node-plane levels: skip the derivative of C where the
+ *                records at a node's two neighbours are bitwise equal (its
+ *                terms are exactly zero, so results keep their bits up to
+ *                the sign of a zero).  0 never, 1 on the levels where whole
+ *                wavefronts skip often enough to pay (default), 2 always.
+ *                Hierarchies only. */
+enum { ELA_OPT_MARCH = 0, ELA_OPT_KZ = 1, ELA_OPT_XCD = 2, ELA_OPT_MASK = 3 };
 int ela_set_option(ela_hier* h, int option, int value);
 
 /* Autotuner of the marching kernels (no reference counterpart): times nrep

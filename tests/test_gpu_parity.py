@@ -296,6 +296,33 @@ def test_grain_marching_kernels_give_the_node_kernel_bits(mode, krylov):
     assert np.array_equal(sols[0], sols[1])
 
 
+@pytest.mark.parametrize("mode", [ela.SOLVE_FP64, ela.SOLVE_MIXED])
+def test_plane_mask_keeps_the_bits(mode):
+    """Per-node-plane stiffness: skipping the derivative of C where the records at a node's two
+    neighbours are bitwise equal (OPT_MASK 2, every level) against never skipping (OPT_MASK 0).  The
+    skipped terms are exact zeros, so Fapply, the residual and whole solves agree bit for bit (numpy
+    equality, which also takes -0.0 == 0.0).  Few large grains, so whole wavefronts do skip."""
+    cfg = K.polycrystal(48, grains=5, seed=9)
+    out = {}
+    for m in (0, 2):
+        op = ela.Elastic(cfg["n"], MM, mode=ela.ALLOC_BOTH)
+        op.SetModelVoronoiCubic(cfg["seeds"], K.VORONOI_Q, cfg["R"], cfg["C11"], cfg["C12"], cfg["C44"])
+        op.SetBC(cfg["bc_type"], cfg["bc_value"])
+        op.SetStiffnessMode(0)
+        op.set_option(ela.OPT_MASK, m)
+        op.prepareForSolve()
+        u = H.random_field(np.random.default_rng(3), cfg["n"], 3)
+        b = op.bc_init()
+        f64, f32 = op.Fapply(u, prec=ela.F64), op.Fapply(u, prec=ela.F32)
+        r, nrm = op.residual(u, b, prec=ela.F64)
+        x, st = ela.Linear(op).solve(np.zeros_like(b), b, tol_rel=1e-10, tol_abs=0.0, mode=mode, max_iter=300)
+        assert st["converged"]
+        out[m] = (f64, f32, r, nrm, x, st["iters"])
+        op.close()
+    for a, c in zip(out[0], out[2]):
+        assert np.array_equal(a, c)
+
+
 def test_autotune_picks_a_tile_shape_without_changing_results():
     cfg = K.polycrystal(40, grains=10, seed=6)
     sols = []
