@@ -82,6 +82,7 @@ struct Level {
     double* d64 = nullptr;
     float* d32 = nullptr;  // inverse diagonal of the FP32 smoothers
     unsigned char* sm = nullptr;  // same-direction mask of the stiffness planes (launch_same_mask)
+    bool sm_ok = false;           // sm was built by the last ela_prepare
     bool use_sm = false;          // enough wavefronts skip a derivative of C for the mask to pay
     double *cor64 = nullptr, *tmp64 = nullptr, *res64 = nullptr, *rc64 = nullptr, *hold64 = nullptr;
     float *cor32 = nullptr, *tmp32 = nullptr, *res32 = nullptr, *rc32 = nullptr, *hold32 = nullptr;
@@ -594,7 +595,7 @@ constexpr double kMaskMinSkip = 0.2;
 // The same-direction mask of a level's planes: valid from ela_prepare until
 // the model changes (every model setter clears `prepared`).
 const unsigned char* plane_mask(const ela_hier* h, const Level& v) {
-    const bool on = h->mask_mode == 2 || (h->mask_mode == 1 && v.use_sm);
+    const bool on = v.sm_ok && (h->mask_mode == 2 || (h->mask_mode == 1 && v.use_sm));
     return h->prepared && on ? v.sm : nullptr;
 }
 
@@ -1055,10 +1056,15 @@ int ela_prepare(ela_hier* h) {
         for (auto& v : h->lev) launch_convert<double, float>((long long)field_elems(v, h->nc), v.C64, v.C32, h->stream);
     // where C is constant across a node, the plane-form stencils may skip its
     // derivative: worth it on a level where whole wavefronts skip often
+    // (not on a level that runs in grain form, where the grain ids say it)
     const size_t nl = h->lev.size();
+    std::vector<char> masked(nl, 0);
     HIPCHK(hipMemsetAsync(h->mstats, 0, 2 * nl * sizeof(unsigned long long), h->stream));
-    for (size_t l = 0; l < nl; ++l)
+    for (size_t l = 0; l < nl; ++l) {
+        if (table_on(h, h->lev[l])) continue;
         launch_same_mask(h->lev[l].L, h->nc, h->lev[l].C64, h->lev[l].sm, h->mstats + 2 * l, h->stream);
+        masked[l] = 1;
+    }
     // diagonals
     HIPCHK(hipMemsetAsync(h->dflag, 0, sizeof(unsigned long long), h->stream));
     for (auto& v : h->lev)
@@ -1069,8 +1075,11 @@ int ela_prepare(ela_hier* h) {
                           h->stream));
     HIPCHK(hipMemcpyAsync(&flag, h->dflag, sizeof flag, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    for (size_t l = 0; l < nl; ++l)
-        h->lev[l].use_sm = (double)ms[2 * l + 1] >= kMaskMinSkip * (double)h->dim * (double)ms[2 * l];
+    for (size_t l = 0; l < nl; ++l) {
+        h->lev[l].sm_ok = masked[l] != 0;
+        h->lev[l].use_sm = masked[l] && ms[2 * l] > 0 &&
+                           (double)ms[2 * l + 1] >= kMaskMinSkip * (double)h->dim * (double)ms[2 * l];
+    }
     if (flag) return fail(ELA_ERR_NAN, "diagonal is nan or zero (Elastic.cpp:347 aborts here)");
     // dense coarsest operator: columns A e_q, then Gauss-Jordan [A | I] -> [I | A^-1]
     if (!h->M) {  // operator-only use (Fapply, Fsmooth, ...) of a box that does not coarsen far enough
