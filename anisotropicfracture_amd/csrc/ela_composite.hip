@@ -515,23 +515,30 @@ int solve(ela_amr* a, double tol_rel, double tol_abs, int max_iter, ela_stats* s
             for (int j = 0; j < m && total < max_iter; ++j) {
                 if (int rc = fac(a, a->V[j], a->Z[j])) return rc;
                 apply(a, a->Z[j], a->V[j + 1]);
-                VecList Vl{}, self{};
+                VecList Vl{};
                 for (int q = 0; q <= j; ++q) Vl.v[q] = a->V[q];
-                self.v[0] = a->V[j + 1];
-                launch_mdot(a->V[j + 1], self, 1, n, a->kpart, a->kdev, s);
-                const double wn0 = std::sqrt(host_vec(a, 1)[0]);
-                launch_mdot(a->V[j + 1], Vl, j + 1, n, a->kpart, a->kdev, s);
-                launch_maxpy(a->V[j + 1], Vl, j + 1, a->kdev, -1.0, n, s);
-                std::vector<double> h1 = host_vec(a, j + 1), h2(j + 1, 0.0);
+                // h1 and |w|^2 in one pass (w last in the list); |w'|^2 with the update
+                VecList Vw = Vl;
+                Vw.v[j + 1] = a->V[j + 1];
+                launch_mdot(a->V[j + 1], Vw, j + 2, n, a->kpart, a->kdev, s);
+                std::vector<double> h1 = host_vec(a, j + 2), h2(j + 1, 0.0);
+                const double wn0 = std::sqrt(h1[j + 1]);
+                h1.resize(j + 1);
                 double hn2 = wn0 * wn0;
                 for (int q = 0; q <= j; ++q) hn2 -= h1[q] * h1[q];
+                double* const norm2 = a->kdev + 2 * (kMaxKrylov + 2);  // beyond the coefficients
                 if (!(hn2 > 0.5 * wn0 * wn0)) {
-                    launch_mdot(a->V[j + 1], Vl, j + 1, n, a->kpart, a->kdev, s);
                     launch_maxpy(a->V[j + 1], Vl, j + 1, a->kdev, -1.0, n, s);
+                    launch_mdot(a->V[j + 1], Vl, j + 1, n, a->kpart, a->kdev, s);
+                    launch_maxpy_norm(a->V[j + 1], Vl, j + 1, a->kdev, -1.0, n, a->kpart, norm2, s);
                     h2 = host_vec(a, j + 1);
+                } else {
+                    launch_maxpy_norm(a->V[j + 1], Vl, j + 1, a->kdev, -1.0, n, a->kpart, norm2, s);
                 }
-                launch_mdot(a->V[j + 1], self, 1, n, a->kpart, a->kdev, s);
-                const double hn = std::sqrt(host_vec(a, 1)[0]);
+                double hn2_dev = 0.0;
+                ELA_HIPCHK(hipMemcpyAsync(&hn2_dev, norm2, sizeof(double), hipMemcpyDeviceToHost, s));
+                ELA_HIPCHK(hipStreamSynchronize(s));
+                const double hn = std::sqrt(hn2_dev);
                 ++total;
                 for (int q = 0; q <= j; ++q) H[(size_t)q * m + j] = h1[q] + h2[q];
                 H[(size_t)(j + 1) * m + j] = hn;

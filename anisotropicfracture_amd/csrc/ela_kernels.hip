@@ -497,6 +497,31 @@ __global__ __launch_bounds__(FB) void k_maxpy(double* __restrict__ y, VecList X,
     }
 }
 
+// y += sign * sum_v coef[v] X_v, and the block partials of |y|^2 (the new y),
+// reduced as k_mdot reduces (x, x): partials[block].
+__global__ __launch_bounds__(FB) void k_maxpy_norm(double* __restrict__ y, VecList X, int nv,
+                                                   const double* __restrict__ coef, double sign, long long n,
+                                                   double* __restrict__ partials) {
+    double acc = 0.0;
+    for (long long q = (long long)blockIdx.x * FB + threadIdx.x; q < n; q += (long long)FG * FB) {
+        double a = 0.0;
+        for (int v = 0; v < nv; ++v) a += coef[v] * X.v[v][q];
+        const double yq = y[q] + sign * a;
+        y[q] = yq;
+        acc += yq * yq;
+    }
+    __shared__ double red[FB / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) red[w] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (int ww = 0; ww < FB / 64; ++ww) a += red[ww];
+        partials[blockIdx.x] = a;
+    }
+}
+
 __global__ __launch_bounds__(FB) void k_scale_flat(double* __restrict__ y, const double* __restrict__ x, double a,
                                                    long long n) {
     for (long long q = (long long)blockIdx.x * FB + threadIdx.x; q < n; q += (long long)FG * FB) y[q] = a * x[q];
@@ -513,6 +538,12 @@ void launch_mdot(const double* x, const VecList& Y, int nv, long long n, double*
 
 void launch_maxpy(double* y, const VecList& X, int nv, const double* coef, double sign, long long n, hipStream_t s) {
     hipLaunchKernelGGL(k_maxpy, dim3(FG), dim3(FB), 0, s, y, X, nv, coef, sign, n);
+}
+
+void launch_maxpy_norm(double* y, const VecList& X, int nv, const double* coef, double sign, long long n,
+                       double* partials, double* norm2, hipStream_t s) {
+    hipLaunchKernelGGL(k_maxpy_norm, dim3(FG), dim3(FB), 0, s, y, X, nv, coef, sign, n, partials);
+    hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(256), 0, s, partials, norm2);
 }
 
 void launch_scale_flat(double* y, const double* x, double a, long long n, hipStream_t s) {

@@ -129,6 +129,10 @@ void launch_mdot(const double* x, const VecList& Y, int nv, long long n, double*
 long long mdot_partials(int nv);
 // y += sign * sum_v coef[v] X_v, coef on the device.
 void launch_maxpy(double* y, const VecList& X, int nv, const double* coef, double sign, long long n, hipStream_t s);
+// launch_maxpy, and |y|^2 of the result into *norm2 (device), summed in the
+// order launch_mdot sums (y, y) (partials: mdot_partials(1) doubles).
+void launch_maxpy_norm(double* y, const VecList& X, int nv, const double* coef, double sign, long long n,
+                       double* partials, double* norm2, hipStream_t s);
 // y = a x
 void launch_scale_flat(double* y, const double* x, double a, long long n, hipStream_t s);
 

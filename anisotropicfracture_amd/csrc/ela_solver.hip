@@ -462,24 +462,31 @@ int solve_fgmres(ela_hier* h, const double* b, double* u, double tol_rel, double
             VecList Vl{};
             for (int q = 0; q <= j; ++q) Vl.v[q] = h->V[q];
             // classical Gram-Schmidt, h1 = V^T w, w -= V h1, with a second pass only
-            // when w lost most of its norm (|w'| < |w| / sqrt(2): Daniel et al.)
-            VecList self{};
-            self.v[0] = h->V[j + 1];
-            launch_mdot(h->V[j + 1], self, 1, n, h->kpart, h->kdev, s);
-            const double wn0 = std::sqrt(host_vec(1)[0]);
-            launch_mdot(h->V[j + 1], Vl, j + 1, n, h->kpart, h->kdev, s);
-            launch_maxpy(h->V[j + 1], Vl, j + 1, h->kdev, -1.0, n, s);
-            std::vector<double> h1 = host_vec(j + 1);
+            // when w lost most of its norm (|w'| < |w| / sqrt(2): Daniel et al.).
+            // One pass over w gives h1 and |w|^2 (w is the last vector of the
+            // list); the update w -= V h1 also sums |w'|^2.
+            VecList Vw = Vl;
+            Vw.v[j + 1] = h->V[j + 1];
+            launch_mdot(h->V[j + 1], Vw, j + 2, n, h->kpart, h->kdev, s);
+            std::vector<double> h1 = host_vec(j + 2);
+            const double wn0 = std::sqrt(h1[j + 1]);
+            h1.resize(j + 1);
             std::vector<double> h2(j + 1, 0.0);
             double hn2 = wn0 * wn0;
             for (int q = 0; q <= j; ++q) hn2 -= h1[q] * h1[q];
+            double* const norm2 = h->kdev + 2 * (m + 2);  // beyond the coefficients the update reads
             if (!(hn2 > 0.5 * wn0 * wn0)) {
-                launch_mdot(h->V[j + 1], Vl, j + 1, n, h->kpart, h->kdev, s);
                 launch_maxpy(h->V[j + 1], Vl, j + 1, h->kdev, -1.0, n, s);
+                launch_mdot(h->V[j + 1], Vl, j + 1, n, h->kpart, h->kdev, s);
+                launch_maxpy_norm(h->V[j + 1], Vl, j + 1, h->kdev, -1.0, n, h->kpart, norm2, s);
                 h2 = host_vec(j + 1);
+            } else {
+                launch_maxpy_norm(h->V[j + 1], Vl, j + 1, h->kdev, -1.0, n, h->kpart, norm2, s);
             }
-            launch_mdot(h->V[j + 1], self, 1, n, h->kpart, h->kdev, s);
-            const double hn = std::sqrt(host_vec(1)[0]);
+            double hn2_dev = 0.0;
+            HIPCHK(hipMemcpyAsync(&hn2_dev, norm2, sizeof(double), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            const double hn = std::sqrt(hn2_dev);
             ++total;
             for (int q = 0; q <= j; ++q) H[(size_t)q * m + j] = h1[q] + h2[q];
             H[(size_t)(j + 1) * m + j] = hn;
