@@ -362,7 +362,8 @@ class Elastic:
     def set_option(self, option, value):
         """ela_set_option: OPT_MARCH (LDS z-marching kernels for grain-form stiffness, default on),
         OPT_KZ (planes per marching tile, default 16), OPT_XCD (tile order) or OPT_MASK (plane levels
-        skip the derivative of C where it is exactly zero: 0 never, 1 where it pays, 2 always)."""
+        skip the derivative of C where it is exactly zero, constant-C nodes launched first: 0 never,
+        1 where it pays, 2 always)."""
         _check(lib().ela_set_option(self.ptr, option, int(value)))
 
     def autotune(self, nrep=10):

@@ -39,6 +39,9 @@ inline dim3 node_grid(const Lay& L) { return dim3((L.nx + BX - 1) / BX, (L.ny + 
 // processed 8 blocks apart in the same L2 and its y neighbours next to it.
 struct Slab {
     int rows, nty, ntx, order;
+    // node-list order instead (NodeList): thread q of the launch takes node list[q]
+    const unsigned long long* list = nullptr;
+    long long nlist = 0;
 };
 inline Slab slab_of(const Lay& L) {
     Slab S;
@@ -70,6 +73,20 @@ __device__ __forceinline__ bool slab_index(const Lay& L, const Slab S, int& i, i
     j = x * S.rows + tyl * BY + threadIdx.y;
     const int jend = min(L.ny, (x + 1) * S.rows);
     return i < L.nx && j < jend && k < L.nz;
+}
+
+// Node of thread q in a node-list launch (1D grid of BX x BY blocks).
+__device__ __forceinline__ bool list_index(const Slab& S, int& i, int& j, int& k) {
+    const long long q = (long long)blockIdx.x * (BX * BY) + threadIdx.x + BX * threadIdx.y;
+    if (q >= S.nlist) return false;
+    const unsigned long long v = S.list[q];
+    i = (int)(v & 0x1FFFFFu);
+    j = (int)((v >> 21) & 0x1FFFFFu);
+    k = (int)(v >> 42);
+    return true;
+}
+__device__ __forceinline__ bool pick_index(const Lay& L, const Slab& S, int& i, int& j, int& k) {
+    return S.list ? list_index(S, i, j, k) : S.rows ? slab_index(L, S, i, j, k) : node_index(L, i, j, k);
 }
 
 // Block max of a non-negative value (NaN propagates: a NaN compares as
@@ -116,7 +133,7 @@ template <typename T, int K, int DIM, typename Acc>
 __global__ __launch_bounds__(256) void k_fapply(Lay L, Acc A, const T* __restrict__ u, T* __restrict__ f, BCTab bc,
                                                 Inv<T> iv, int uniform, Slab S) {
     int i, j, k;
-    if (!(S.rows ? slab_index(L, S, i, j, k) : node_index(L, i, j, k))) return;
+    if (!pick_index(L, S, i, j, k)) return;
     T r[DIM];
     node_apply_acc<T, K, DIM>(L, A, u, i, j, k, bc, iv, uniform != 0, r);
     const long long o = (long long)i + L.px * j + L.pl * k;
@@ -133,7 +150,7 @@ __global__ __launch_bounds__(256) void k_residual(Lay L, Acc A, const T* __restr
                                                   const double* __restrict__ scale, double* __restrict__ partials,
                                                   BCTab bc, Inv<T> iv, int uniform, Slab S) {
     int i, j, k;
-    const bool in = S.rows ? slab_index(L, S, i, j, k) : node_index(L, i, j, k);
+    const bool in = pick_index(L, S, i, j, k);
     double m = 0.0;
     if (in) {
         T f[DIM];
@@ -158,7 +175,7 @@ __global__ __launch_bounds__(256) void k_jacobi(Lay L, Acc A, const T* __restric
                                                 const T* __restrict__ b, T* __restrict__ xout, BCTab bc, Inv<T> iv,
                                                 int uniform, T omega, Slab S) {
     int i, j, k;
-    if (!(S.rows ? slab_index(L, S, i, j, k) : node_index(L, i, j, k))) return;
+    if (!pick_index(L, S, i, j, k)) return;
     T f[DIM];
     node_apply_acc<T, K, DIM>(L, A, xin, i, j, k, bc, iv, uniform != 0, f);
     const long long o = (long long)i + L.px * j + L.pl * k;
@@ -585,13 +602,31 @@ Inv<T> make_inv(const double h[3]) {
     return iv;
 }
 
+// Launch order of the one-thread-per-node plane kernels: a node list (every
+// node of the level once, in the list's order), the XCD-slab order (3D), or
+// the plain block grid.
+template <typename T>
+void plane_launch(const Lay& L, const PlaneOpt& po, Slab& S, dim3& grid) {
+    if (po.list) {
+        S = Slab{0, 0, 0, 0};
+        S.list = po.list;
+        S.nlist = po.nlist;
+        grid = dim3((unsigned)((po.nlist + BX * BY - 1) / (BX * BY)));
+        return;
+    }
+    const bool sl = L.dim == 3 && g_slab_order && (sizeof(T) == 4 || g_slab_f64);
+    S = sl ? slab_of(L) : Slab{0, 0, 0, 0};
+    grid = sl ? slab_grid(L) : node_grid(L);
+}
+
 template <typename T>
 void launch_fapply(const Lay& L, int kind, const T* C, const T* u, T* f, const BCTab& bc, const double h[3],
-                   int uniform, hipStream_t s, const unsigned char* sm) {
+                   int uniform, hipStream_t s, const PlaneOpt& po) {
     const Inv<T> iv = make_inv<T>(h);
-    const bool sl = L.dim == 3 && g_slab_order && (sizeof(T) == 4 || g_slab_f64);
-    const Slab S = sl ? slab_of(L) : Slab{0, 0, 0, 0};
-    const dim3 grid = sl ? slab_grid(L) : node_grid(L);
+    const unsigned char* sm = po.sm;
+    Slab S;
+    dim3 grid;
+    plane_launch<T>(L, po, S, grid);
     if (sm)
         ELA_DISPATCH(kind, L.dim,
                      hipLaunchKernelGGL((k_fapply<T, KK, DD, MaskedPlaneAcc<T>>), grid, dim3(BX, BY), 0, s, L,
@@ -605,11 +640,12 @@ void launch_fapply(const Lay& L, int kind, const T* C, const T* u, T* f, const B
 template <typename T, typename TO>
 void launch_residual(const Lay& L, int kind, const T* C, const T* x, const T* b, T* r, TO* rs, const double* scale,
                      double* partials, unsigned long long* norm, const BCTab& bc, const double h[3], int uniform,
-                     hipStream_t s, const unsigned char* sm) {
+                     hipStream_t s, const PlaneOpt& po) {
     const Inv<T> iv = make_inv<T>(h);
-    const bool sl = L.dim == 3 && g_slab_order && (sizeof(T) == 4 || g_slab_f64);
-    const Slab S = sl ? slab_of(L) : Slab{0, 0, 0, 0};
-    const dim3 grid = sl ? slab_grid(L) : node_grid(L);
+    const unsigned char* sm = po.sm;
+    Slab S;
+    dim3 grid;
+    plane_launch<T>(L, po, S, grid);
     if (sm)
         ELA_DISPATCH(kind, L.dim,
                      hipLaunchKernelGGL((k_residual<T, TO, KK, DD, MaskedPlaneAcc<T>>), grid, dim3(BX, BY), 0, s, L,
@@ -638,11 +674,12 @@ int g_slab_f64 = 1;
 template <typename T>
 void launch_jacobi(const Lay& L, int kind, const T* C, const T* dg, const T* xin, const T* b, T* xout,
                    const BCTab& bc, const double h[3], int uniform, double omega, hipStream_t s,
-                   const unsigned char* sm) {
+                   const PlaneOpt& po) {
     const Inv<T> iv = make_inv<T>(h);
-    const bool sl = L.dim == 3 && g_slab_order && (sizeof(T) == 4 || g_slab_f64);
-    const Slab S = sl ? slab_of(L) : Slab{0, 0, 0, 0};
-    const dim3 grid = sl ? slab_grid(L) : node_grid(L);
+    const unsigned char* sm = po.sm;
+    Slab S;
+    dim3 grid;
+    plane_launch<T>(L, po, S, grid);
     if (sm)
         ELA_DISPATCH(kind, L.dim,
                      hipLaunchKernelGGL((k_jacobi<T, KK, DD, MaskedPlaneAcc<T>>), grid, dim3(BX, BY), 0, s, L,
@@ -692,6 +729,132 @@ void launch_same_mask(const Lay& L, int nc, const double* C, unsigned char* sm, 
                       hipStream_t s) {
     if (L.dim == 3) hipLaunchKernelGGL(k_same_mask<3>, node_grid(L), dim3(BX, BY), 0, s, L, nc, C, sm, stats);
     else hipLaunchKernelGGL(k_same_mask<2>, node_grid(L), dim3(BX, BY), 0, s, L, nc, C, sm, stats);
+}
+
+// Node list of a plane level (PlaneOpt::list): the interior nodes with every
+// direction bit of the mask (C constant across them: no derivative of C to
+// evaluate) first, then every other node, each part in block order (64 x 4
+// patches of node_grid, row-major inside), so the wavefronts of the first part
+// all skip.  Entries pack (i, j, k) in 21 bits each.
+template <int DIM>
+__device__ __forceinline__ bool node_pure(const Lay& L, const unsigned char* sm, int i, int j, int k) {
+    const bool in = i > 0 && i < L.nx - 1 && j > 0 && j < L.ny - 1 && (DIM == 2 || (k > 0 && k < L.nz - 1));
+    const unsigned full = (1u << DIM) - 1;
+    return in && (sm[(long long)i + L.px * j + L.pl * k] & full) == full;
+}
+
+__device__ __forceinline__ long long grid_block() {
+    return blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+}
+
+// cnt[2 b], cnt[2 b + 1]: the block's nodes of the first and of the second part.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_list_count(Lay L, const unsigned char* __restrict__ sm,
+                                                    int* __restrict__ cnt) {
+    int i, j, k;
+    const bool valid = node_index(L, i, j, k);
+    const bool pure = valid && node_pure<DIM>(L, sm, i, j, k);
+    const unsigned long long bp = __ballot(pure), bv = __ballot(valid);
+    __shared__ int wp[BY], wv[BY];
+    if (threadIdx.x == 0) {  // a wavefront is one 64-wide row of the block
+        wp[threadIdx.y] = __popcll(bp);
+        wv[threadIdx.y] = __popcll(bv);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        int np = 0, nv = 0;
+        for (int w = 0; w < BY; ++w) {
+            np += wp[w];
+            nv += wv[w];
+        }
+        const long long b = grid_block();
+        cnt[2 * b] = np;
+        cnt[2 * b + 1] = nv - np;
+    }
+}
+
+// Exclusive offsets of every block in both parts (one block of 1024 threads,
+// each scanning a contiguous run of blocks); totals = {first part, second part}.
+__global__ __launch_bounds__(1024) void k_list_scan(int nb, const int* __restrict__ cnt, int* __restrict__ off,
+                                                    long long* __restrict__ totals) {
+    __shared__ long long sp[1024], sr[1024];
+    const int t = threadIdx.x;
+    const int per = (nb + 1023) / 1024, b0 = min(nb, t * per), b1 = min(nb, b0 + per);
+    long long p = 0, r = 0;
+    for (int b = b0; b < b1; ++b) {
+        p += cnt[2 * b];
+        r += cnt[2 * b + 1];
+    }
+    sp[t] = p;
+    sr[t] = r;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {  // inclusive scan of the runs' sums
+        const long long ap = t >= d ? sp[t - d] : 0, ar = t >= d ? sr[t - d] : 0;
+        __syncthreads();
+        sp[t] += ap;
+        sr[t] += ar;
+        __syncthreads();
+    }
+    long long ep = sp[t] - p, er = sr[t] - r;
+    for (int b = b0; b < b1; ++b) {
+        off[2 * b] = (int)ep;
+        off[2 * b + 1] = (int)er;
+        ep += cnt[2 * b];
+        er += cnt[2 * b + 1];
+    }
+    if (t == 1023) {
+        totals[0] = sp[1023];
+        totals[1] = sr[1023];
+    }
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_list_scatter(Lay L, const unsigned char* __restrict__ sm,
+                                                      const int* __restrict__ off, const long long* __restrict__ totals,
+                                                      unsigned long long* __restrict__ list) {
+    int i, j, k;
+    const bool valid = node_index(L, i, j, k);
+    const bool pure = valid && node_pure<DIM>(L, sm, i, j, k);
+    const unsigned long long bp = __ballot(pure), br = __ballot(valid && !pure);
+    const unsigned long long below = (1ull << threadIdx.x) - 1;  // lanes of this row before this one
+    __shared__ int wp[BY], wr[BY];
+    if (threadIdx.x == 0) {
+        wp[threadIdx.y] = __popcll(bp);
+        wr[threadIdx.y] = __popcll(br);
+    }
+    __syncthreads();
+    if (!valid) return;
+    int op = 0, orr = 0;
+    for (int w = 0; w < (int)threadIdx.y; ++w) {
+        op += wp[w];
+        orr += wr[w];
+    }
+    const long long b = grid_block();
+    const unsigned long long v =
+        (unsigned long long)i | ((unsigned long long)j << 21) | ((unsigned long long)k << 42);
+    if (pure) list[off[2 * b] + op + __popcll(bp & below)] = v;
+    else list[totals[0] + off[2 * b + 1] + orr + __popcll(br & below)] = v;
+}
+
+long long node_list_blocks(const Lay& L) {
+    const dim3 g = node_grid(L);
+    return (long long)g.x * g.y * g.z;
+}
+
+void launch_node_list(const Lay& L, const unsigned char* sm, int* cnt, long long* totals, unsigned long long* list,
+                      hipStream_t s) {
+    const dim3 g = node_grid(L);
+    const int nb = (int)node_list_blocks(L);
+    int* off = cnt + 2LL * nb;
+    if (L.dim == 3) {
+        hipLaunchKernelGGL(k_list_count<3>, g, dim3(BX, BY), 0, s, L, sm, cnt);
+        hipLaunchKernelGGL(k_list_scan, dim3(1), dim3(1024), 0, s, nb, cnt, off, totals);
+        hipLaunchKernelGGL(k_list_scatter<3>, g, dim3(BX, BY), 0, s, L, sm, off, totals, list);
+    } else {
+        hipLaunchKernelGGL(k_list_count<2>, g, dim3(BX, BY), 0, s, L, sm, cnt);
+        hipLaunchKernelGGL(k_list_scan, dim3(1), dim3(1024), 0, s, nb, cnt, off, totals);
+        hipLaunchKernelGGL(k_list_scatter<2>, g, dim3(BX, BY), 0, s, L, sm, off, totals, list);
+    }
 }
 
 template <typename T>
@@ -1066,22 +1229,22 @@ template void launch_residual_tab<float, float>(const Lay&, const TabDesc<float>
 
 // explicit instantiations
 template void launch_fapply<double>(const Lay&, int, const double*, const double*, double*, const BCTab&,
-                                    const double*, int, hipStream_t, const unsigned char*);
+                                    const double*, int, hipStream_t, const PlaneOpt&);
 template void launch_fapply<float>(const Lay&, int, const float*, const float*, float*, const BCTab&, const double*,
-                                   int, hipStream_t, const unsigned char*);
+                                   int, hipStream_t, const PlaneOpt&);
 template void launch_residual<double, double>(const Lay&, int, const double*, const double*, const double*, double*,
                                               double*, const double*, double*, unsigned long long*, const BCTab&,
-                                              const double*, int, hipStream_t, const unsigned char*);
+                                              const double*, int, hipStream_t, const PlaneOpt&);
 template void launch_residual<double, float>(const Lay&, int, const double*, const double*, const double*, double*,
                                              float*, const double*, double*, unsigned long long*, const BCTab&, const double*,
-                                             int, hipStream_t, const unsigned char*);
+                                             int, hipStream_t, const PlaneOpt&);
 template void launch_residual<float, float>(const Lay&, int, const float*, const float*, const float*, float*,
                                             float*, const double*, double*, unsigned long long*, const BCTab&, const double*,
-                                            int, hipStream_t, const unsigned char*);
+                                            int, hipStream_t, const PlaneOpt&);
 template void launch_jacobi<double>(const Lay&, int, const double*, const double*, const double*, const double*,
-                                    double*, const BCTab&, const double*, int, double, hipStream_t, const unsigned char*);
+                                    double*, const BCTab&, const double*, int, double, hipStream_t, const PlaneOpt&);
 template void launch_jacobi<float>(const Lay&, int, const float*, const float*, const float*, const float*, float*,
-                                   const BCTab&, const double*, int, double, hipStream_t, const unsigned char*);
+                                   const BCTab&, const double*, int, double, hipStream_t, const PlaneOpt&);
 template void launch_jacobi0<double>(const Lay&, const double*, const double*, double*, double, hipStream_t);
 template void launch_jacobi0<float>(const Lay&, const float*, const float*, float*, double, hipStream_t);
 template void launch_restrict<double>(const Lay&, const Lay&, int, const double*, double*, hipStream_t);

@@ -15,22 +15,31 @@ extern int g_slab_f64;   // slab numbering for FP64 kernels too
 template <typename T>
 Inv<T> make_inv(const double h[3]);
 
+// Options of the per-node-plane stencil launchers: sm, the same-direction mask
+// (launch_same_mask) or null; list, the level's nodes in the order to launch
+// them (launch_node_list; nlist entries, every node once) or null.
+struct PlaneOpt {
+    const unsigned char* sm = nullptr;
+    const unsigned long long* list = nullptr;
+    long long nlist = 0;
+};
+
 template <typename T>
 void launch_fapply(const Lay& L, int kind, const T* C, const T* u, T* f, const BCTab& bc, const double h[3],
-                   int uniform, hipStream_t s, const unsigned char* sm = nullptr);
+                   int uniform, hipStream_t s, const PlaneOpt& po = PlaneOpt{});
 
 // r = b - A x; r (T), rs = r * (*scale) (TO), max|r| into *norm (bit pattern of the double,
 // via residual_partials(L) doubles of scratch in partials); any output may be null.
 template <typename T, typename TO>
 void launch_residual(const Lay& L, int kind, const T* C, const T* x, const T* b, T* r, TO* rs, const double* scale,
                      double* partials, unsigned long long* norm, const BCTab& bc, const double h[3], int uniform,
-                     hipStream_t s, const unsigned char* sm = nullptr);
+                     hipStream_t s, const PlaneOpt& po = PlaneOpt{});
 long long residual_partials(const Lay& L);
 
 template <typename T>
 void launch_jacobi(const Lay& L, int kind, const T* C, const T* dg, const T* xin, const T* b, T* xout,
                    const BCTab& bc, const double h[3], int uniform, double omega, hipStream_t s,
-                   const unsigned char* sm = nullptr);
+                   const PlaneOpt& po = PlaneOpt{});
 
 // sm (one byte per node): bit d set where the stiffness records at the two
 // d-neighbours of an interior node are bitwise equal (the plane launchers
@@ -38,6 +47,13 @@ void launch_jacobi(const Lay& L, int kind, const T* C, const T* dg, const T* xin
 // stats[0] += wavefronts with an interior node, stats[1] += (wavefront,
 // direction) pairs whose interior nodes all have the bit (the work skipped).
 void launch_same_mask(const Lay& L, int nc, const double* C, unsigned char* sm, unsigned long long* stats,
+                      hipStream_t s);
+// Node list of a plane level from its mask (PlaneOpt::list): the interior
+// nodes with every direction bit first, then the others; list holds one
+// entry per node, totals = {first part, second part} (device), cnt
+// 4 * node_list_blocks(L) ints of scratch.
+long long node_list_blocks(const Lay& L);
+void launch_node_list(const Lay& L, const unsigned char* sm, int* cnt, long long* totals, unsigned long long* list,
                       hipStream_t s);
 
 template <typename T>

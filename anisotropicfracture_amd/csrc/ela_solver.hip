@@ -84,6 +84,10 @@ struct Level {
     unsigned char* sm = nullptr;  // same-direction mask of the stiffness planes (launch_same_mask)
     bool sm_ok = false;           // sm was built by the last ela_prepare
     bool use_sm = false;          // enough wavefronts skip a derivative of C for the mask to pay
+    unsigned long long* nl = nullptr;  // node list (launch_node_list): constant-C interior nodes first
+    int* nlcnt = nullptr;              // its scratch
+    long long* nltot = nullptr;        // its part sizes (device)
+    bool use_list = false;             // enough constant-C nodes for the list order to pay
     double *cor64 = nullptr, *tmp64 = nullptr, *res64 = nullptr, *rc64 = nullptr, *hold64 = nullptr;
     float *cor32 = nullptr, *tmp32 = nullptr, *res32 = nullptr, *rc32 = nullptr, *hold32 = nullptr;
 };
@@ -598,12 +602,29 @@ void grain_resid64<double, float>(const GrainBox<double>& g, int kz, double* p, 
 // 257^3 Voronoi level (28 % of the pairs skip) and slows down its coarser
 // levels (under 1 %), where the branch and the mask load buy nothing.
 constexpr double kMaskMinSkip = 0.2;
+// Share of a level's nodes with constant C around them for the level to run
+// its FP64 plane stencils in node-list order (those nodes first, so whole
+// wavefronts skip the derivative of C; the list order gives up the XCD-slab
+// order's L2 reuse).  Measured on MI355X with a 64-grain Voronoi field: its
+// 257^3 level (96 %) gains, its 129^3 level (79 %) loses.
+constexpr double kListMinPure = 0.9;
 
-// The same-direction mask of a level's planes: valid from ela_prepare until
-// the model changes (every model setter clears `prepared`).
-const unsigned char* plane_mask(const ela_hier* h, const Level& v) {
-    const bool on = v.sm_ok && (h->mask_mode == 2 || (h->mask_mode == 1 && v.use_sm));
-    return h->prepared && on ? v.sm : nullptr;
+// Mask and node list of a level's planes: valid from ela_prepare until the
+// model changes (every model setter clears `prepared`).  ELA_OPT_MASK 2 uses
+// both on every plane level, 1 where they pay, 0 neither.
+template <typename T>
+PlaneOpt plane_opt(const ela_hier* h, const Level& v) {
+    PlaneOpt po;
+    if (!h->prepared || !v.sm_ok || h->mask_mode == 0) return po;
+    // the list order pays in FP64 only (measured: the FP32 stencils lose more
+    // L2 reuse than they gain)
+    const bool list = h->mask_mode == 2 || (v.use_list && sizeof(T) == 8);
+    if (list || v.use_sm) po.sm = v.sm;
+    if (list) {
+        po.list = v.nl;
+        po.nlist = v.nodes;
+    }
+    return po;
 }
 
 // The grain-form level 0 through the z-marching kernels.
@@ -636,7 +657,7 @@ void A_apply(ela_hier* h, Level& v, const T* C, const T* u, T* f, hipStream_t s)
         launch_grain<T, 3, T>(g, h->kz, nullptr, nullptr, s);
     } else if (table_on(h, v)) launch_fapply_tab<T>(v.L, tab_of<T>(h), u, f, h->bc, v.h, h->uniform, s);
     else if (v.mp.ok) march_fapply<T>(v.mp, v.L, h->kind, C, u, f, h->bc, v.h, h->uniform, s);
-    else launch_fapply<T>(v.L, h->kind, C, u, f, h->bc, v.h, h->uniform, s, plane_mask(h, v));
+    else launch_fapply<T>(v.L, h->kind, C, u, f, h->bc, v.h, h->uniform, s, plane_opt<T>(h, v));
 }
 
 template <typename T>
@@ -651,7 +672,7 @@ void A_jacobi(ela_hier* h, Level& v, const T* C, const T* dg, const T* xin, cons
         launch_grain<T, 0, T>(g, h->kz, nullptr, nullptr, s);
     } else if (table_on(h, v)) launch_jacobi_tab<T>(v.L, tab_of<T>(h), dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
     else if (v.mp.ok) march_jacobi<T>(v.mp, v.L, h->kind, C, dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
-    else launch_jacobi<T>(v.L, h->kind, C, dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s, plane_mask(h, v));
+    else launch_jacobi<T>(v.L, h->kind, C, dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s, plane_opt<T>(h, v));
 }
 
 template <typename T, typename TO>
@@ -674,7 +695,7 @@ void A_residual(ela_hier* h, Level& v, const T* C, const T* x, const T* b, T* r,
     } else if (table_on(h, v)) launch_residual_tab<T, TO>(v.L, tab_of<T>(h), x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
     else if (v.mp.ok) march_residual<T, TO>(v.mp, v.L, h->kind, C, x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
     else launch_residual<T, TO>(v.L, h->kind, C, x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s,
-                                plane_mask(h, v));
+                                plane_opt<T>(h, v));
 }
 
 }  // namespace
@@ -829,6 +850,9 @@ int ela_hier_create_ex(ela_ctx* ctx, const ela_domain* dom, int model_kind, int 
         const size_t fe = field_elems(v, h->dim);
         rc = halloc_t(h, field_elems(v, h->nc), &v.C64);
         if (rc == ELA_OK) rc = halloc_t(h, field_elems(v, 1), &v.sm);
+        if (rc == ELA_OK) rc = halloc_t(h, (size_t)v.nodes, &v.nl);
+        if (rc == ELA_OK) rc = halloc_t(h, 4 * (size_t)node_list_blocks(v.L), &v.nlcnt);
+        if (rc == ELA_OK) rc = halloc_t(h, 2, &v.nltot);
         if (rc == ELA_OK && h->has64) {
             rc = halloc_t(h, fe, &v.d64);
             if (!rc) rc = halloc_t(h, fe, &v.cor64);
@@ -1068,8 +1092,10 @@ int ela_prepare(ela_hier* h) {
     std::vector<char> masked(nl, 0);
     HIPCHK(hipMemsetAsync(h->mstats, 0, 2 * nl * sizeof(unsigned long long), h->stream));
     for (size_t l = 0; l < nl; ++l) {
-        if (table_on(h, h->lev[l])) continue;
-        launch_same_mask(h->lev[l].L, h->nc, h->lev[l].C64, h->lev[l].sm, h->mstats + 2 * l, h->stream);
+        Level& v = h->lev[l];
+        if (table_on(h, v)) continue;
+        launch_same_mask(v.L, h->nc, v.C64, v.sm, h->mstats + 2 * l, h->stream);
+        launch_node_list(v.L, v.sm, v.nlcnt, v.nltot, v.nl, h->stream);
         masked[l] = 1;
     }
     // diagonals
@@ -1080,12 +1106,18 @@ int ela_prepare(ela_hier* h) {
     std::vector<unsigned long long> ms(2 * nl);
     HIPCHK(hipMemcpyAsync(ms.data(), h->mstats, ms.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                           h->stream));
+    std::vector<long long> npure(nl, 0);
+    for (size_t l = 0; l < nl; ++l)
+        if (masked[l])
+            HIPCHK(hipMemcpyAsync(&npure[l], h->lev[l].nltot, sizeof(long long), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipMemcpyAsync(&flag, h->dflag, sizeof flag, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     for (size_t l = 0; l < nl; ++l) {
-        h->lev[l].sm_ok = masked[l] != 0;
-        h->lev[l].use_sm = masked[l] && ms[2 * l] > 0 &&
-                           (double)ms[2 * l + 1] >= kMaskMinSkip * (double)h->dim * (double)ms[2 * l];
+        Level& v = h->lev[l];
+        v.sm_ok = masked[l] != 0;
+        v.use_sm = masked[l] && ms[2 * l] > 0 &&
+                   (double)ms[2 * l + 1] >= kMaskMinSkip * (double)h->dim * (double)ms[2 * l];
+        v.use_list = masked[l] && (double)npure[l] >= kListMinPure * (double)v.nodes;
     }
     if (flag) return fail(ELA_ERR_NAN, "diagonal is nan or zero (Elastic.cpp:347 aborts here)");
     // dense coarsest operator: columns A e_q, then Gauss-Jordan [A | I] -> [I | A^-1]
@@ -1264,10 +1296,10 @@ int ela_fsmooth(ela_hier* h, int mglev, int prec, void* x, const void* b, int ns
     if (int rc = halloc(h, field_elems(v, h->dim) * es, &tmp)) return rc;
     for (int w = 0; w < nsweeps; ++w) {
         if (prec == ELA_F64) {
-            launch_jacobi<double>(v.L, h->kind, v.C64, v.d64, (const double*)x, (const double*)b, (double*)tmp, h->bc, v.h, h->uniform, omega, s, plane_mask(h, v));  // ela_fsmooth keeps its own omega
+            launch_jacobi<double>(v.L, h->kind, v.C64, v.d64, (const double*)x, (const double*)b, (double*)tmp, h->bc, v.h, h->uniform, omega, s, plane_opt<double>(h, v));  // ela_fsmooth keeps its own omega
             HIPCHK(hipMemcpyAsync(x, tmp, field_elems(v, h->dim) * es, hipMemcpyDeviceToDevice, s));
         } else {
-            launch_jacobi<float>(v.L, h->kind, v.C32, v.d32, (const float*)x, (const float*)b, (float*)tmp, h->bc, v.h, h->uniform, omega, s, plane_mask(h, v));
+            launch_jacobi<float>(v.L, h->kind, v.C32, v.d32, (const float*)x, (const float*)b, (float*)tmp, h->bc, v.h, h->uniform, omega, s, plane_opt<float>(h, v));
             HIPCHK(hipMemcpyAsync(x, tmp, field_elems(v, h->dim) * es, hipMemcpyDeviceToDevice, s));
         }
     }

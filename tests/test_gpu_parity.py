@@ -299,9 +299,10 @@ def test_grain_marching_kernels_give_the_node_kernel_bits(mode, krylov):
 @pytest.mark.parametrize("mode", [ela.SOLVE_FP64, ela.SOLVE_MIXED])
 def test_plane_mask_keeps_the_bits(mode):
     """Per-node-plane stiffness: skipping the derivative of C where the records at a node's two
-    neighbours are bitwise equal (OPT_MASK 2, every level) against never skipping (OPT_MASK 0).  The
-    skipped terms are exact zeros, so Fapply, the residual and whole solves agree bit for bit (numpy
-    equality, which also takes -0.0 == 0.0).  Few large grains, so whole wavefronts do skip."""
+    neighbours are bitwise equal, with the constant-C nodes launched first (OPT_MASK 2, every level),
+    against never skipping (OPT_MASK 0).  The skipped terms are exact zeros and every node is computed
+    once in either order, so Fapply, the residual and whole solves agree bit for bit (numpy equality,
+    which also takes -0.0 == 0.0).  Few large grains, so whole wavefronts do skip."""
     cfg = K.polycrystal(48, grains=5, seed=9)
     out = {}
     for m in (0, 2):
