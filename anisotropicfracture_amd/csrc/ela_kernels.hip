@@ -327,6 +327,33 @@ __global__ void k_unit(Lay L, int nodes, double* __restrict__ E, int qprev, int 
     at(q) = 1.0;
 }
 
+// Every column of the coarsest operator in one launch: E holds nunk unit
+// fields (E + q * fstride is e_q in the level layout, set by k_unit_all) and
+// thread (node t, column q = blockIdx.y) writes A[p * nodes + t][q] = (A e_q)
+// at node t, component p, into M (leading dimension ld) -- the per-node
+// operator of the plane kernels, so the same bits as probing column by column.
+template <int K, int DIM>
+__global__ __launch_bounds__(256) void k_probe_all(Lay L, int nodes, const double* __restrict__ C,
+                                                   const double* __restrict__ E, long long fstride, BCTab bc,
+                                                   Inv<double> iv, int uniform, double* __restrict__ M, long long ld) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, q = blockIdx.y;
+    if (t >= nodes) return;
+    const int i = t % L.nx, j = (t / L.nx) % L.ny, k = t / (L.nx * L.ny);
+    double f[DIM];
+    node_apply<double, K, DIM>(L, C, E + (long long)q * fstride, i, j, k, bc, iv, uniform != 0, f);
+#pragma unroll
+    for (int p = 0; p < DIM; ++p) M[(long long)(p * nodes + t) * ld + q] = f[p];
+}
+
+// E + q * fstride = e_q for every unknown q (E zeroed beforehand).
+__global__ void k_unit_all(Lay L, int nodes, int dim, double* __restrict__ E, long long fstride) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nodes * dim) return;
+    const int p = q / nodes, t = q % nodes;
+    const int i = t % L.nx, j = (t / L.nx) % L.ny, k = t / (L.nx * L.ny);
+    E[(long long)q * fstride + p * L.cs + i + L.px * j + L.pl * k] = 1.0;
+}
+
 // Right half of [A | I]: M[r][n + r] = 1.
 __global__ void k_identity_right(double* __restrict__ M, int n) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -911,6 +938,17 @@ void launch_scale_from_norm(const unsigned long long* norm, double* scale, hipSt
 void launch_assemble_col(const Lay& L, int nodes, const double* F, int q, long long ld, double* M, int dim,
                          hipStream_t s) {
     hipLaunchKernelGGL(k_assemble_col, dim3((nodes + 255) / 256), dim3(256), 0, s, L, nodes, F, q, ld, M, dim);
+}
+
+void launch_probe_all(const Lay& L, int kind, int nodes, const double* C, double* E, long long fstride,
+                      const BCTab& bc, const double h[3], int uniform, double* M, long long ld, hipStream_t s) {
+    const int n = nodes * L.dim;
+    hipLaunchKernelGGL(k_unit_all, dim3((n + 255) / 256), dim3(256), 0, s, L, nodes, L.dim, E, fstride);
+    const Inv<double> iv = make_inv<double>(h);
+    const dim3 grid((nodes + 255) / 256, n);
+    ELA_DISPATCH(kind, L.dim,
+                 hipLaunchKernelGGL((k_probe_all<KK, DD>), grid, dim3(256), 0, s, L, nodes, C, E, fstride, bc, iv,
+                                    uniform, M, ld));
 }
 
 void launch_unit(const Lay& L, int nodes, double* E, int qprev, int q, hipStream_t s) {

@@ -86,6 +86,11 @@ void launch_scale_from_norm(const unsigned long long* norm, double* scale, hipSt
 void launch_assemble_col(const Lay& L, int nodes, const double* F, int q, long long ld, double* M, int dim,
                          hipStream_t s);
 void launch_unit(const Lay& L, int nodes, double* E, int qprev, int q, hipStream_t s);
+// All columns of the coarsest operator into M (rows p * nodes + t, leading
+// dimension ld) in two launches; E: nodes * dim zeroed fields of fstride
+// doubles (scratch).
+void launch_probe_all(const Lay& L, int kind, int nodes, const double* C, double* E, long long fstride,
+                      const BCTab& bc, const double h[3], int uniform, double* M, long long ld, hipStream_t s);
 void launch_identity_right(double* M, int n, hipStream_t s);
 void launch_gauss_jordan(double* M, int n, int* flag, hipStream_t s);
 

@@ -1127,18 +1127,26 @@ int ela_prepare(ela_hier* h) {
     }
     Level& v = h->lev.back();
     const int n = h->nunk;
+    const size_t fe = field_elems(v, h->dim);
+    // all columns in one launch when n unit fields fit in 256 MB of scratch
+    const bool all = (size_t)n * fe <= ((size_t)1 << 25);
     double *E = nullptr, *F = nullptr;
-    int rc = halloc_t(h, field_elems(v, h->dim), &E);
-    if (!rc) rc = halloc_t(h, field_elems(v, h->dim), &F);
+    int rc = halloc_t(h, all ? (size_t)n * fe : fe, &E);
+    if (!rc && !all) rc = halloc_t(h, fe, &F);
     if (rc) {
         if (E) hfree(h, E);
         return rc;
     }
     HIPCHK(hipMemsetAsync(h->M, 0, sizeof(double) * (size_t)n * 2 * n, h->stream));
-    for (int q = 0; q < n; ++q) {
-        launch_unit(v.L, v.nodes, E, q - 1, q, h->stream);
-        launch_fapply<double>(v.L, h->kind, v.C64, E, F, h->bc, v.h, h->uniform, h->stream);
-        launch_assemble_col(v.L, v.nodes, F, q, 2LL * n, h->M, h->dim, h->stream);
+    if (all) {  // E comes zeroed from halloc
+        launch_probe_all(v.L, h->kind, v.nodes, v.C64, E, (long long)fe, h->bc, v.h, h->uniform, h->M, 2LL * n,
+                         h->stream);
+    } else {
+        for (int q = 0; q < n; ++q) {
+            launch_unit(v.L, v.nodes, E, q - 1, q, h->stream);
+            launch_fapply<double>(v.L, h->kind, v.C64, E, F, h->bc, v.h, h->uniform, h->stream);
+            launch_assemble_col(v.L, v.nodes, F, q, 2LL * n, h->M, h->dim, h->stream);
+        }
     }
     launch_identity_right(h->M, n, h->stream);
     HIPCHK(hipMemsetAsync(h->dflag, 0, sizeof(int), h->stream));
@@ -1147,7 +1155,7 @@ int ela_prepare(ela_hier* h) {
     HIPCHK(hipMemcpyAsync(&gj, h->dflag, sizeof gj, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     hfree(h, E);
-    hfree(h, F);
+    if (F) hfree(h, F);
     if (gj) return fail(ELA_ERR_NAN, "coarsest-level operator is singular");
     h->prepared = true;
     return ELA_OK;
