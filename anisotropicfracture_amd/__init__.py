@@ -363,7 +363,7 @@ class Elastic:
         """ela_set_option: OPT_MARCH (LDS z-marching kernels for grain-form stiffness, default on),
         OPT_KZ (planes per marching tile, default 16), OPT_XCD (tile order) or OPT_MASK (plane levels
         skip the derivative of C where it is exactly zero, constant-C nodes launched first: 0 never,
-        1 where it pays, 2 always)."""
+        1 where it pays, 2 always with node lists, 3 always with in-block regrouping)."""
         _check(lib().ela_set_option(self.ptr, option, int(value)))
 
     def autotune(self, nrep=10):

@@ -42,6 +42,8 @@ struct Slab {
     // node-list order instead (NodeList): thread q of the launch takes node list[q]
     const unsigned long long* list = nullptr;
     long long nlist = 0;
+    // 3D: regroup each block's nodes, those with every bit of the mask gsm first
+    const unsigned char* gsm = nullptr;
 };
 inline Slab slab_of(const Lay& L) {
     Slab S;
@@ -85,8 +87,51 @@ __device__ __forceinline__ bool list_index(const Slab& S, int& i, int& j, int& k
     k = (int)(v >> 42);
     return true;
 }
+// The block's nodes (slab or grid order) regrouped: the interior nodes with
+// all three mask bits (constant C around them) take the first threads, the
+// others follow, each group in row order.  Whole wavefronts then skip the
+// derivative of C while the block keeps its place in the launch order.
+// Every thread of the block must call it (it synchronizes).
+__device__ __forceinline__ bool group_index(const Lay& L, const Slab& S, int& i, int& j, int& k) {
+    const bool valid = S.rows ? slab_index(L, S, i, j, k) : node_index(L, i, j, k);
+    const bool in = valid && i > 0 && i < L.nx - 1 && j > 0 && j < L.ny - 1 && k > 0 && k < L.nz - 1;
+    const bool pure = in && (S.gsm[(long long)i + L.px * j + L.pl * k] & 7u) == 7u;
+    const unsigned long long bp = __ballot(pure), bo = __ballot(valid && !pure);
+    __shared__ int wp[BY], wo[BY];
+    __shared__ unsigned long long slot[BX * BY];
+    const int lane = threadIdx.x, w = threadIdx.y;  // a wavefront is one 64-wide row of the block
+    if (lane == 0) {
+        wp[w] = __popcll(bp);
+        wo[w] = __popcll(bo);
+    }
+    __syncthreads();
+    int np = 0, no = 0, pp = 0, po = 0;
+    for (int ww = 0; ww < BY; ++ww) {
+        if (ww < w) {
+            pp += wp[ww];
+            po += wo[ww];
+        }
+        np += wp[ww];
+        no += wo[ww];
+    }
+    const unsigned long long below = (1ull << lane) - 1;
+    const unsigned long long v = (unsigned long long)i | ((unsigned long long)j << 21) | ((unsigned long long)k << 42);
+    if (pure) slot[pp + __popcll(bp & below)] = v;
+    else if (valid) slot[np + po + __popcll(bo & below)] = v;
+    __syncthreads();
+    const int tid = lane + BX * w;
+    if (tid >= np + no) return false;
+    const unsigned long long u = slot[tid];
+    i = (int)(u & 0x1FFFFFu);
+    j = (int)((u >> 21) & 0x1FFFFFu);
+    k = (int)(u >> 42);
+    return true;
+}
+
 __device__ __forceinline__ bool pick_index(const Lay& L, const Slab& S, int& i, int& j, int& k) {
-    return S.list ? list_index(S, i, j, k) : S.rows ? slab_index(L, S, i, j, k) : node_index(L, i, j, k);
+    if (S.list) return list_index(S, i, j, k);
+    if (S.gsm) return group_index(L, S, i, j, k);
+    return S.rows ? slab_index(L, S, i, j, k) : node_index(L, i, j, k);
 }
 
 // Block max of a non-negative value (NaN propagates: a NaN compares as
@@ -644,6 +689,7 @@ void plane_launch(const Lay& L, const PlaneOpt& po, Slab& S, dim3& grid) {
     const bool sl = L.dim == 3 && g_slab_order && (sizeof(T) == 4 || g_slab_f64);
     S = sl ? slab_of(L) : Slab{0, 0, 0, 0};
     grid = sl ? slab_grid(L) : node_grid(L);
+    if (po.group && po.sm && L.dim == 3) S.gsm = po.sm;
 }
 
 template <typename T>

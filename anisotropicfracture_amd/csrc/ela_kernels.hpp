@@ -22,6 +22,7 @@ struct PlaneOpt {
     const unsigned char* sm = nullptr;
     const unsigned long long* list = nullptr;
     long long nlist = 0;
+    bool group = false;  // 3D, with sm: constant-C nodes take each block's first wavefronts
 };
 
 template <typename T>

@@ -88,6 +88,7 @@ struct Level {
     int* nlcnt = nullptr;              // its scratch
     long long* nltot = nullptr;        // its part sizes (device)
     bool use_list = false;             // enough constant-C nodes for the list order to pay
+    bool use_group = false;            // enough for regrouping them inside the blocks to pay
     double *cor64 = nullptr, *tmp64 = nullptr, *res64 = nullptr, *rc64 = nullptr, *hold64 = nullptr;
     float *cor32 = nullptr, *tmp32 = nullptr, *res32 = nullptr, *rc32 = nullptr, *hold32 = nullptr;
 };
@@ -132,7 +133,7 @@ struct ela_hier {
     double* dscale = nullptr;
     int* dflag = nullptr;
     unsigned long long* mstats = nullptr;  // per level: wavefronts, skipping (wavefront, direction) pairs
-    int mask_mode = 1;                     // ELA_OPT_MASK: 0 never, 1 where it pays, 2 every level
+    int mask_mode = 1;                     // ELA_OPT_MASK: 0 never, 1 where it pays, 2 / 3 every level
     unsigned long long* hnorm = nullptr;  // pinned
     // grain-table stiffness of level 0 (Voronoi models): per-node grain id, per-grain records
     unsigned short* gid = nullptr;
@@ -608,22 +609,32 @@ constexpr double kMaskMinSkip = 0.2;
 // order's L2 reuse).  Measured on MI355X with a 64-grain Voronoi field: its
 // 257^3 level (96 %) gains, its 129^3 level (79 %) loses.
 constexpr double kListMinPure = 0.9;
+// Share for regrouping them inside each block instead (the launch order and
+// its L2 reuse stay): a block of 256 nodes then fills whole wavefronts with them.
+constexpr double kGroupMinPure = 0.4;
 
 // Mask and node list of a level's planes: valid from ela_prepare until the
 // model changes (every model setter clears `prepared`).  ELA_OPT_MASK 2 uses
-// both on every plane level, 1 where they pay, 0 neither.
+// the mask and node lists on every plane level, 3 the mask and in-block
+// regrouping, 1 each where it pays, 0 neither.
 template <typename T>
 PlaneOpt plane_opt(const ela_hier* h, const Level& v) {
     PlaneOpt po;
     if (!h->prepared || !v.sm_ok || h->mask_mode == 0) return po;
-    // the list order pays in FP64 only (measured: the FP32 stencils lose more
-    // L2 reuse than they gain)
-    const bool list = h->mask_mode == 2 || (v.use_list && sizeof(T) == 8);
-    if (list || v.use_sm) po.sm = v.sm;
+    // Both orders pay in FP64 only.  Measured on MI355X (256^3 polycrystal):
+    // FP64 solves gain 1.07 -> 0.98 s (planes) and 0.62 -> 0.56 s (grain form,
+    // FP64 coarse levels); the FP32 stencils do not gain from skipping the
+    // derivative of C (their 129^3 sweep: 137 us plain, 154 us regrouped,
+    // 221 us in list order).
+    const bool fp64 = sizeof(T) == 8;
+    const bool list = h->mask_mode == 2 || (h->mask_mode == 1 && v.use_list && fp64);
+    const bool group = !list && (h->mask_mode == 3 || (h->mask_mode == 1 && v.use_group && fp64));
+    if (list || group || v.use_sm) po.sm = v.sm;
     if (list) {
         po.list = v.nl;
         po.nlist = v.nodes;
     }
+    po.group = group;
     return po;
 }
 
@@ -1118,6 +1129,7 @@ int ela_prepare(ela_hier* h) {
         v.use_sm = masked[l] && ms[2 * l] > 0 &&
                    (double)ms[2 * l + 1] >= kMaskMinSkip * (double)h->dim * (double)ms[2 * l];
         v.use_list = masked[l] && (double)npure[l] >= kListMinPure * (double)v.nodes;
+        v.use_group = masked[l] && h->dim == 3 && (double)npure[l] >= kGroupMinPure * (double)v.nodes;
     }
     if (flag) return fail(ELA_ERR_NAN, "diagonal is nan or zero (Elastic.cpp:347 aborts here)");
     // dense coarsest operator: columns A e_q, then Gauss-Jordan [A | I] -> [I | A^-1]
@@ -1479,7 +1491,7 @@ int ela_set_option(ela_hier* h, int option, int value) {
     } else if (option == ELA_OPT_XCD) {
         h->xcd = value != 0;
     } else if (option == ELA_OPT_MASK) {
-        if (value < 0 || value > 2) return fail(ELA_ERR_BADARG, "mask mode must be 0, 1 or 2");
+        if (value < 0 || value > 3) return fail(ELA_ERR_BADARG, "mask mode must be 0, 1, 2 or 3");
         h->mask_mode = value;
     } else {
         return fail(ELA_ERR_BADARG, "unknown option");

@@ -254,9 +254,11 @@ int ela_set_krylov(ela_hier* h, int restart);
  *                records at a node's two neighbours are bitwise equal (its
  *                terms are exactly zero, so results keep their bits up to
  *                the sign of a zero), and launch the nodes with constant C
- *                around them first, so that whole wavefronts skip.  0 never,
- *                1 on the levels where it pays (default), 2 on every plane
- *                level.  Hierarchies only. */
+ *                around them first (a level-wide node list, or regrouped
+ *                inside each block), so that whole wavefronts skip.  0 never,
+ *                1 on the levels where it pays (default), 2 every plane level
+ *                with node lists, 3 every plane level with in-block
+ *                regrouping.  Hierarchies only. */
 enum { ELA_OPT_MARCH = 0, ELA_OPT_KZ = 1, ELA_OPT_XCD = 2, ELA_OPT_MASK = 3 };
 int ela_set_option(ela_hier* h, int option, int value);
 

@@ -299,13 +299,13 @@ def test_grain_marching_kernels_give_the_node_kernel_bits(mode, krylov):
 @pytest.mark.parametrize("mode", [ela.SOLVE_FP64, ela.SOLVE_MIXED])
 def test_plane_mask_keeps_the_bits(mode):
     """Per-node-plane stiffness: skipping the derivative of C where the records at a node's two
-    neighbours are bitwise equal, with the constant-C nodes launched first (OPT_MASK 2, every level),
-    against never skipping (OPT_MASK 0).  The skipped terms are exact zeros and every node is computed
+    neighbours are bitwise equal, with the constant-C nodes launched first (OPT_MASK 2: node lists,
+    OPT_MASK 3: regrouped inside each block; every level), against never skipping (OPT_MASK 0).  The skipped terms are exact zeros and every node is computed
     once in either order, so Fapply, the residual and whole solves agree bit for bit (numpy equality,
     which also takes -0.0 == 0.0).  Few large grains, so whole wavefronts do skip."""
     cfg = K.polycrystal(48, grains=5, seed=9)
     out = {}
-    for m in (0, 2):
+    for m in (0, 2, 3):
         op = ela.Elastic(cfg["n"], MM, mode=ela.ALLOC_BOTH)
         op.SetModelVoronoiCubic(cfg["seeds"], K.VORONOI_Q, cfg["R"], cfg["C11"], cfg["C12"], cfg["C44"])
         op.SetBC(cfg["bc_type"], cfg["bc_value"])
@@ -320,8 +320,9 @@ def test_plane_mask_keeps_the_bits(mode):
         assert st["converged"]
         out[m] = (f64, f32, r, nrm, x, st["iters"])
         op.close()
-    for a, c in zip(out[0], out[2]):
-        assert np.array_equal(a, c)
+    for m in (2, 3):
+        for a, c in zip(out[0], out[m]):
+            assert np.array_equal(a, c)
 
 
 def test_autotune_picks_a_tile_shape_without_changing_results():
