@@ -34,9 +34,13 @@ __device__ __forceinline__ bool node_index(const Lay& L, int& i, int& j, int& k)
 inline dim3 node_grid(const Lay& L) { return dim3((L.nx + BX - 1) / BX, (L.ny + BY - 1) / BY, L.nz); }
 
 // XCD-slab numbering of the stencil kernels on 3D levels.  Hardware block b
-// runs on XCD b % 8; XCD x owns the rows [x*rows, (x+1)*rows) and walks its
-// slab y-tile fastest, then z, then x-tile, so a tile's z neighbours are
-// processed 8 blocks apart in the same L2 and its y neighbours next to it.
+// runs on XCD b % 8; XCD x owns the rows [x*rows, (x+1)*rows).  Walk 2 (the
+// default) cuts each plane of the slab, nx * rows nodes in row order, into
+// blocks of 256 consecutive nodes and walks them plane by plane: no thread
+// idles on a 64-wide x tile past the row end (levels have 2^l + 1 nodes a
+// row, so 64-wide tiles leave a tile column holding one node).  Walks 0 and 1
+// use 64 x 4 tiles (y-tile fastest, then z, then x-tile; or x-tile fastest,
+// then y-tile, then z).
 struct Slab {
     int rows, nty, ntx, order;
     // node-list order instead (NodeList): thread q of the launch takes node list[q]
@@ -49,8 +53,13 @@ inline Slab slab_of(const Lay& L) {
     Slab S;
     S.order = g_slab_walk;
     S.rows = (L.ny + 7) / 8;
-    S.nty = (S.rows + BY - 1) / BY;
-    S.ntx = (L.nx + BX - 1) / BX;
+    if (S.order == 2) {
+        S.nty = 1;
+        S.ntx = (int)(((long long)L.nx * S.rows + BX * BY - 1) / (BX * BY));  // blocks per slab plane
+    } else {
+        S.nty = (S.rows + BY - 1) / BY;
+        S.ntx = (L.nx + BX - 1) / BX;
+    }
     return S;
 }
 inline dim3 slab_grid(const Lay& L) {
@@ -59,6 +68,15 @@ inline dim3 slab_grid(const Lay& L) {
 }
 __device__ __forceinline__ bool slab_index(const Lay& L, const Slab S, int& i, int& j, int& k) {
     const int b = blockIdx.x, x = b & 7, r = b >> 3;
+    if (S.order == 2) {  // 256 consecutive nodes of a slab plane, planes in turn
+        const int bl = r % S.ntx;
+        k = r / S.ntx;
+        const unsigned t = (unsigned)bl * (BX * BY) + threadIdx.x + BX * threadIdx.y;
+        const unsigned jl = t / (unsigned)L.nx;
+        i = (int)(t - jl * (unsigned)L.nx);
+        j = x * S.rows + (int)jl;
+        return (int)jl < S.rows && j < L.ny && k < L.nz;
+    }
     int tyl, tx;
     if (S.order == 0) {  // y-tile fastest, then z, then x-tile
         tyl = r % S.nty;
@@ -741,7 +759,7 @@ long long residual_partials(const Lay& L) {
 }
 
 bool g_slab_order = true;
-int g_slab_walk = 1;
+int g_slab_walk = 2;
 int g_slab_f64 = 1;
 
 template <typename T>
