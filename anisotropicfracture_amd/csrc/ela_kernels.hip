@@ -327,6 +327,21 @@ __global__ __launch_bounds__(256) void k_axpy(Lay L, int nc, const TX* __restric
     }
 }
 
+// out = (double) (T) (cor + hold) over nc components on valid nodes: the last
+// correction of an F-cycle (k_axpy's cor += hold) written straight into the
+// FP64 preconditioned vector, with k_axpy's rounding.
+template <typename T>
+__global__ __launch_bounds__(256) void k_add_to64(Lay L, int nc, const T* __restrict__ hold,
+                                                  const T* __restrict__ cor, double* __restrict__ out) {
+    int i, j, k;
+    if (!node_index(L, i, j, k)) return;
+    const long long o = (long long)i + L.px * j + L.pl * k;
+    for (int n = 0; n < nc; ++n) {
+        const long long q = n * L.cs + o;
+        out[q] = (double)(T)((double)cor[q] + (double)hold[q]);
+    }
+}
+
 // Operator<Node>::normalize (Operator.cpp:164-195): the reference's lambda
 // divides x(i,j,k) -- component 0 -- by diag(i,j,k) -- component 0 -- once
 // per component of the operator; the other components are left as they are.
@@ -634,6 +649,17 @@ __global__ __launch_bounds__(FB) void k_scale_flat(double* __restrict__ y, const
     for (long long q = (long long)blockIdx.x * FB + threadIdx.x; q < n; q += (long long)FG * FB) y[q] = a * x[q];
 }
 
+// y = a x, and yo = (TO) y (the next MG cycle's right-hand side), flat.
+template <typename TO>
+__global__ __launch_bounds__(FB) void k_scale_flat_to(double* y, const double* x, double a, TO* __restrict__ yo,
+                                                      long long n) {  // y may be x
+    for (long long q = (long long)blockIdx.x * FB + threadIdx.x; q < n; q += (long long)FG * FB) {
+        const double v = a * x[q];
+        y[q] = v;
+        yo[q] = (TO)v;
+    }
+}
+
 }  // namespace
 
 void launch_mdot(const double* x, const VecList& Y, int nv, long long n, double* partials, double* out,
@@ -656,6 +682,13 @@ void launch_maxpy_norm(double* y, const VecList& X, int nv, const double* coef, 
 void launch_scale_flat(double* y, const double* x, double a, long long n, hipStream_t s) {
     hipLaunchKernelGGL(k_scale_flat, dim3(FG), dim3(FB), 0, s, y, x, a, n);
 }
+
+template <typename TO>
+void launch_scale_flat_to(double* y, const double* x, double a, TO* yo, long long n, hipStream_t s) {
+    hipLaunchKernelGGL((k_scale_flat_to<TO>), dim3(FG), dim3(FB), 0, s, y, x, a, yo, n);
+}
+template void launch_scale_flat_to<float>(double*, const double*, double, float*, long long, hipStream_t);
+template void launch_scale_flat_to<double>(double*, const double*, double, double*, long long, hipStream_t);
 
 long long mdot_partials(int nv) { return (long long)nv * FG; }
 
@@ -977,6 +1010,13 @@ template <typename TX, typename TY>
 void launch_axpy(const Lay& L, int nc, const TX* x, TY* y, const double* ascale, double a, int assign, hipStream_t s) {
     hipLaunchKernelGGL((k_axpy<TX, TY>), node_grid(L), dim3(BX, BY), 0, s, L, nc, x, y, ascale, a, assign);
 }
+
+template <typename T>
+void launch_add_to64(const Lay& L, int nc, const T* hold, const T* cor, double* out, hipStream_t s) {
+    hipLaunchKernelGGL((k_add_to64<T>), node_grid(L), dim3(BX, BY), 0, s, L, nc, hold, cor, out);
+}
+template void launch_add_to64<float>(const Lay&, int, const float*, const float*, double*, hipStream_t);
+template void launch_add_to64<double>(const Lay&, int, const double*, const double*, double*, hipStream_t);
 
 template <typename T>
 void launch_normalize(const Lay& L, int ncomp, const double* dg, T* x, hipStream_t s) {

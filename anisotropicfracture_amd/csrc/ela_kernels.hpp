@@ -157,6 +157,12 @@ void launch_maxpy_norm(double* y, const VecList& X, int nv, const double* coef, 
                        double* partials, double* norm2, hipStream_t s);
 // y = a x
 void launch_scale_flat(double* y, const double* x, double a, long long n, hipStream_t s);
+// y = a x and yo = (TO) y, flat over n elements (yo: a field of the same layout)
+template <typename TO>
+void launch_scale_flat_to(double* y, const double* x, double a, TO* yo, long long n, hipStream_t s);
+// out = (double) (T) (cor + hold) on the valid nodes of a level (nc components)
+template <typename T>
+void launch_add_to64(const Lay& L, int nc, const T* hold, const T* cor, double* out, hipStream_t s);
 
 // Streaming copy of bytes (multiple of 16) from src to dst.
 void launch_copy(const void* src, void* dst, long long bytes, hipStream_t s);

@@ -290,8 +290,10 @@ void vcycle(ela_hier* h, int top) {
     }
 }
 
+// out64 (may be null): level 0's final correction cor + hold is written there
+// in FP64 (the preconditioned vector of a Krylov step) instead of into cor.
 template <typename T>
-void fcycle(ela_hier* h) {
+void fcycle(ela_hier* h, double* out64 = nullptr) {
     const int bot = (int)h->lev.size() - 1;
     for (int l = 1; l <= bot; ++l)
         launch_restrict<T>(h->lev[l].L, h->lev[l - 1].L, h->dim, *lb<T>(h->lev[l - 1]).res, *lb<T>(h->lev[l]).res,
@@ -306,14 +308,21 @@ void fcycle(ela_hier* h) {
         std::swap(*b.res, *b.rc);
         std::swap(*b.cor, *b.hold);
         vcycle<T>(h, l);
-        launch_axpy<T, T>(v.L, h->dim, *b.hold, *b.cor, nullptr, 1.0, 0, h->stream);
+        if (l == 0 && out64) launch_add_to64<T>(v.L, h->dim, *b.hold, *b.cor, out64, h->stream);
+        else launch_axpy<T, T>(v.L, h->dim, *b.hold, *b.cor, nullptr, 1.0, 0, h->stream);
     }
 }
 
+// One MG cycle on level 0's res into cor; with out64, an F-cycle leaves its
+// correction in out64 (FP64) instead, and one_cycle returns true.
 template <typename T>
-void one_cycle(ela_hier* h) {
-    if (h->cycle == ELA_CYCLE_F) fcycle<T>(h);
-    else vcycle<T>(h, 0);
+bool one_cycle(ela_hier* h, double* out64 = nullptr) {
+    if (h->cycle == ELA_CYCLE_F) {
+        fcycle<T>(h, out64);
+        return out64 != nullptr;
+    }
+    vcycle<T>(h, 0);
+    return false;
 }
 
 // FP64 solution residual of level 0 into the T right-hand side of the next
@@ -448,16 +457,16 @@ int solve_fgmres(ela_hier* h, const double* b, double* u, double tol_rel, double
         launch_mdot(h->kr, one, 1, n, h->kpart, h->kdev, s);
         const double beta = std::sqrt(host_vec(1)[0]);
         if (!(beta > 0)) break;
-        launch_scale_flat(h->V[0], h->kr, 1.0 / beta, n, s);
+        // V0 and, in the same pass, the first cycle's right-hand side (T) V0
+        launch_scale_flat_to<T>(h->V[0], h->kr, 1.0 / beta, *lb<T>(v0).res, n, s);
         std::fill(g.begin(), g.end(), 0.0);
         g[0] = beta;
         int jend = 0;
         for (int j = 0; j < m && total < max_iter; ++j) {
             LB<T> lbv = lb<T>(v0);
-            // z_j = M^-1 v_j: one MG cycle on level 0 from zero
-            launch_axpy<double, T>(v0.L, h->dim, h->V[j], *lbv.res, nullptr, 1.0, 1, s);
-            one_cycle<T>(h);
-            launch_axpy<T, double>(v0.L, h->dim, *lbv.cor, h->Z[j], nullptr, 1.0, 1, s);
+            // z_j = M^-1 v_j: one MG cycle on level 0 from zero; res already holds
+            // (T) v_j, written with v_j; an F-cycle writes z_j itself
+            if (!one_cycle<T>(h, h->Z[j])) launch_axpy<T, double>(v0.L, h->dim, *lbv.cor, h->Z[j], nullptr, 1.0, 1, s);
             // w = A z_j into V[j+1]
             {
                 const int e = ev_begin(h);
@@ -495,7 +504,8 @@ int solve_fgmres(ela_hier* h, const double* b, double* u, double tol_rel, double
             ++total;
             for (int q = 0; q <= j; ++q) H[(size_t)q * m + j] = h1[q] + h2[q];
             H[(size_t)(j + 1) * m + j] = hn;
-            if (hn > 0) launch_scale_flat(h->V[j + 1], h->V[j + 1], 1.0 / hn, n, s);
+            // v_{j+1}, and the next cycle's right-hand side (T) v_{j+1} in the same pass
+            if (hn > 0) launch_scale_flat_to<T>(h->V[j + 1], h->V[j + 1], 1.0 / hn, *lb<T>(v0).res, n, s);
             // Givens rotations
             for (int q = 0; q < j; ++q) {
                 const double a = H[(size_t)q * m + j], c = H[(size_t)(q + 1) * m + j];
