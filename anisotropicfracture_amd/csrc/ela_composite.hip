@@ -527,7 +527,7 @@ int solve(ela_amr* a, double tol_rel, double tol_abs, int max_iter, ela_stats* s
                 double hn2 = wn0 * wn0;
                 for (int q = 0; q <= j; ++q) hn2 -= h1[q] * h1[q];
                 double* const norm2 = a->kdev + 2 * (kMaxKrylov + 2);  // beyond the coefficients
-                if (!(hn2 > 0.5 * wn0 * wn0)) {
+                if (!(hn2 > kReorth2 * wn0 * wn0)) {
                     launch_maxpy(a->V[j + 1], Vl, j + 1, a->kdev, -1.0, n, s);
                     launch_mdot(a->V[j + 1], Vl, j + 1, n, a->kpart, a->kdev, s);
                     launch_maxpy_norm(a->V[j + 1], Vl, j + 1, a->kdev, -1.0, n, a->kpart, norm2, s);

@@ -476,7 +476,8 @@ int solve_fgmres(ela_hier* h, const double* b, double* u, double tol_rel, double
             VecList Vl{};
             for (int q = 0; q <= j; ++q) Vl.v[q] = h->V[q];
             // classical Gram-Schmidt, h1 = V^T w, w -= V h1, with a second pass only
-            // when w lost most of its norm (|w'| < |w| / sqrt(2): Daniel et al.).
+            // when w lost most of its norm (|w'| < 0.1 |w|, kReorth2: the test of
+            // Daniel et al. with a smaller bound).
             // One pass over w gives h1 and |w|^2 (w is the last vector of the
             // list); the update w -= V h1 also sums |w'|^2.
             VecList Vw = Vl;
@@ -489,7 +490,7 @@ int solve_fgmres(ela_hier* h, const double* b, double* u, double tol_rel, double
             double hn2 = wn0 * wn0;
             for (int q = 0; q <= j; ++q) hn2 -= h1[q] * h1[q];
             double* const norm2 = h->kdev + 2 * (m + 2);  // beyond the coefficients the update reads
-            if (!(hn2 > 0.5 * wn0 * wn0)) {
+            if (!(hn2 > kReorth2 * wn0 * wn0)) {
                 launch_maxpy(h->V[j + 1], Vl, j + 1, h->kdev, -1.0, n, s);
                 launch_mdot(h->V[j + 1], Vl, j + 1, n, h->kpart, h->kdev, s);
                 launch_maxpy_norm(h->V[j + 1], Vl, j + 1, h->kdev, -1.0, n, h->kpart, norm2, s);
