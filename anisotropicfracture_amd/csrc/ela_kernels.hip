@@ -994,6 +994,26 @@ void launch_diag(const Lay& L, int kind, const double* C, double* d64, float* d3
                                     h[1], h[2], flag));
 }
 
+// Injection from the finer level: crse(I, J, K) = fine(2I, 2J, 2K), nc
+// components (coarse grain ids and stiffness of a grain-form hierarchy).
+template <typename T>
+__global__ __launch_bounds__(256) void k_inject_level(Lay Lc, Lay Lf, int nc, const T* __restrict__ fine,
+                                                      T* __restrict__ crse) {
+    int I, J, K;
+    if (!node_index(Lc, I, J, K)) return;
+    const long long oc = (long long)I + Lc.px * J + Lc.pl * K;
+    const long long of = 2LL * I + Lf.px * (2LL * J) + Lf.pl * (Lc.dim == 3 ? 2LL * K : 0LL);
+    for (int n = 0; n < nc; ++n) crse[n * Lc.cs + oc] = fine[n * Lf.cs + of];
+}
+
+template <typename T>
+void launch_inject_level(const Lay& Lc, const Lay& Lf, int nc, const T* fine, T* crse, hipStream_t s) {
+    hipLaunchKernelGGL((k_inject_level<T>), node_grid(Lc), dim3(BX, BY), 0, s, Lc, Lf, nc, fine, crse);
+}
+template void launch_inject_level<double>(const Lay&, const Lay&, int, const double*, double*, hipStream_t);
+template void launch_inject_level<unsigned short>(const Lay&, const Lay&, int, const unsigned short*,
+                                                  unsigned short*, hipStream_t);
+
 template <typename T>
 void launch_restrict(const Lay& Lc, const Lay& Lf, int nc, const T* fine, T* crse, hipStream_t s) {
     if (Lc.dim == 3) hipLaunchKernelGGL((k_restrict<T, 3>), node_grid(Lc), dim3(BX, BY), 0, s, Lc, Lf, nc, fine, crse);

@@ -65,6 +65,9 @@ void launch_diag(const Lay& L, int kind, const double* C, double* d64, float* d3
 
 template <typename T>
 void launch_restrict(const Lay& Lc, const Lay& Lf, int nc, const T* fine, T* crse, hipStream_t s);
+// crse(I, J, K) = fine(2I, 2J, 2K) for nc components.
+template <typename T>
+void launch_inject_level(const Lay& Lc, const Lay& Lf, int nc, const T* fine, T* crse, hipStream_t s);
 
 template <typename T>
 void launch_prolong(const Lay& Lf, const Lay& Lc, int nc, const T* crse, T* fine, int add, hipStream_t s);

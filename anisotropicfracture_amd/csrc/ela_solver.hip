@@ -82,6 +82,7 @@ struct Level {
     double* d64 = nullptr;
     float* d32 = nullptr;  // inverse diagonal of the FP32 smoothers
     unsigned char* sm = nullptr;  // same-direction mask of the stiffness planes (launch_same_mask)
+    unsigned short* gid = nullptr;  // coarse levels of a grain-form hierarchy: grain ids by injection
     bool sm_ok = false;           // sm was built by the last ela_prepare
     bool use_sm = false;          // enough wavefronts skip a derivative of C for the mask to pay
     unsigned long long* nl = nullptr;  // node list (launch_node_list): constant-C interior nodes first
@@ -596,6 +597,11 @@ TabDesc<double> tab_of<double>(ela_hier* h) { return {h->gid, h->tab64, h->dgt64
 template <>
 TabDesc<float> tab_of<float>(ela_hier* h) { return {h->gid, h->tab32, h->dgt32}; }
 bool table_on(ela_hier* h, const Level& v) { return h->use_table && h->has_table && &v == &h->lev[0]; }
+// The grain form holds on every level of a grain-form hierarchy: the coarse
+// levels take grain ids and stiffness by injection (so a coarse node's record
+// is its grain's), not the full weighting of the per-node planes.
+bool grain_levels(const ela_hier* h) { return h->use_table && h->has_table; }
+const unsigned short* level_gid(const ela_hier* h, const Level& v) { return &v == &h->lev[0] ? h->gid : v.gid; }
 
 // MODE 2 (FP64 solution residual) exists for FP64 operands only.
 template <typename T, typename TO>
@@ -649,10 +655,15 @@ PlaneOpt plane_opt(const ela_hier* h, const Level& v) {
     return po;
 }
 
-// The grain-form level 0 through the z-marching kernels.
+// The grain-form levels through the z-marching kernels: level 0, and coarse
+// levels of at least kMarchMinNodes nodes a side (on smaller ones the shell
+// and the launch dominate and the plane kernels run).
+constexpr int kMarchMinNodes = 65;
 bool grain_on(ela_hier* h, const Level& v) {
-    return table_on(h, v) && h->grain_march && h->ngrains <= kMarchMaxGrains && h->dbc != nullptr &&
-           grain_offsets_fit(v.L);
+    if (!grain_levels(h) || !h->grain_march || h->ngrains > kMarchMaxGrains || !h->dbc || !grain_offsets_fit(v.L))
+        return false;
+    if (&v == &h->lev[0]) return true;
+    return v.gid && h->dim == 3 && v.L.nx >= kMarchMinNodes && v.L.ny >= kMarchMinNodes && v.L.nz >= kMarchMinNodes;
 }
 
 template <typename T>
@@ -662,7 +673,7 @@ GrainBox<T> grain_box(ela_hier* h, const Level& v) {
     g.iv = make_inv<T>(v.h);
     g.bc = h->dbc;
     g.uniform = h->uniform;
-    g.gid = h->gid;
+    g.gid = level_gid(h, v);
     g.tab = tab_of<T>(h).tab;
     g.sc = nullptr;
     g.ngr = h->ngrains;
@@ -1007,6 +1018,10 @@ int ela_set_model_voronoi_cubic(ela_hier* h, int nseeds, const int64_t* seeds, i
     for (void* p : {(void*)h->gid, (void*)h->tab64, (void*)h->dgt64, (void*)h->tab32, (void*)h->dgt32})
         if (p) hfree(h, p);
     h->gid = nullptr;
+    for (size_t l = 1; l < h->lev.size(); ++l) {
+        if (h->lev[l].gid) hfree(h, h->lev[l].gid);
+        h->lev[l].gid = nullptr;
+    }
     h->tab64 = h->dgt64 = nullptr;
     h->tab32 = h->dgt32 = nullptr;
     h->has_table = false;
@@ -1014,6 +1029,7 @@ int ela_set_model_voronoi_cubic(ela_hier* h, int nseeds, const int64_t* seeds, i
     int rc = halloc(h, sizeof(long long) * 3 * nseeds, &dseeds);
     if (!rc) rc = halloc(h, sizeof(double) * 9 * nseeds, &dR);
     if (!rc) rc = halloc_t(h, (size_t)v.L.cs, &h->gid);
+    for (size_t l = 1; l < h->lev.size() && !rc; ++l) rc = halloc_t(h, (size_t)h->lev[l].L.cs, &h->lev[l].gid);
     if (!rc) rc = halloc_t(h, (size_t)21 * nseeds, &h->tab64);
     if (!rc) rc = halloc_t(h, (size_t)21 * nseeds, &h->tab32);
     if (!rc) rc = halloc_t(h, (size_t)3 * nseeds, &h->dgt64);
@@ -1027,6 +1043,9 @@ int ela_set_model_voronoi_cubic(ela_hier* h, int nseeds, const int64_t* seeds, i
             launch_convert<double, float>(21LL * nseeds, h->tab64, h->tab32, h->stream);
             launch_grain_diag(nseeds, h->tab64, v.h, h->dgt64, h->dgt32, h->stream);
             launch_voronoi_cubic(v.L, nseeds, (const long long*)dseeds, q, h->tab64, v.C64, h->gid, h->stream);
+            for (size_t l = 1; l < h->lev.size(); ++l)
+                launch_inject_level<unsigned short>(h->lev[l].L, h->lev[l - 1].L, 1, level_gid(h, h->lev[l - 1]),
+                                                    h->lev[l].gid, h->stream);
             e = hipStreamSynchronize(h->stream);
         }
     }
@@ -1103,8 +1122,14 @@ int ela_prepare(ela_hier* h) {
     if (!h) return fail(ELA_ERR_BADARG, "null hierarchy");
     if (!h->model_set) return fail(ELA_ERR_STATE, "Attempting to use operator before calling SetModel");
     // stiffness on every MG level (FP64), then FP32 copies
-    for (size_t l = 1; l < h->lev.size(); ++l)
-        launch_restrict<double>(h->lev[l].L, h->lev[l - 1].L, h->nc, h->lev[l - 1].C64, h->lev[l].C64, h->stream);
+    for (size_t l = 1; l < h->lev.size(); ++l) {
+        if (grain_levels(h))  // the records of the coarse grain ids (Elastic.cpp:759-803 averages instead)
+            launch_inject_level<double>(h->lev[l].L, h->lev[l - 1].L, h->nc, h->lev[l - 1].C64, h->lev[l].C64,
+                                        h->stream);
+        else
+            launch_restrict<double>(h->lev[l].L, h->lev[l - 1].L, h->nc, h->lev[l - 1].C64, h->lev[l].C64,
+                                    h->stream);
+    }
     if (h->has32)
         for (auto& v : h->lev) launch_convert<double, float>((long long)field_elems(v, h->nc), v.C64, v.C32, h->stream);
     // where C is constant across a node, the plane-form stencils may skip its
@@ -1115,7 +1140,7 @@ int ela_prepare(ela_hier* h) {
     HIPCHK(hipMemsetAsync(h->mstats, 0, 2 * nl * sizeof(unsigned long long), h->stream));
     for (size_t l = 0; l < nl; ++l) {
         Level& v = h->lev[l];
-        if (table_on(h, v)) continue;
+        if (table_on(h, v) || grain_on(h, v)) continue;
         launch_same_mask(v.L, h->nc, v.C64, v.sm, h->mstats + 2 * l, h->stream);
         launch_node_list(v.L, v.sm, v.nlcnt, v.nltot, v.nl, h->stream);
         masked[l] = 1;
