@@ -656,9 +656,10 @@ PlaneOpt plane_opt(const ela_hier* h, const Level& v) {
 }
 
 // The grain-form levels through the z-marching kernels: level 0, and coarse
-// levels of at least kMarchMinNodes nodes a side (on smaller ones the shell
-// and the launch dominate and the plane kernels run).
-constexpr int kMarchMinNodes = 65;
+// levels of at least kMarchMinNodes nodes a side.  On smaller ones the shell
+// and too few tiles dominate and the plane kernels run (measured on MI355X:
+// a 129^3 level's FP32 sweep 137 -> 59 us marching, a 65^3 one 28 -> 43 us).
+constexpr int kMarchMinNodes = 129;
 bool grain_on(ela_hier* h, const Level& v) {
     if (!grain_levels(h) || !h->grain_march || h->ngrains > kMarchMaxGrains || !h->dbc || !grain_offsets_fit(v.L))
         return false;

@@ -325,6 +325,57 @@ def test_plane_mask_keeps_the_bits(mode):
             assert np.array_equal(a, c)
 
 
+def test_grain_form_coarse_operator_is_the_injected_one(oracle):
+    """A grain-form hierarchy coarsens by injection: level 1's operator is the oracle's operator with the
+    stiffness of the fine nodes (2I, 2J, 2K), i.e. every coarse node keeps its grain's record."""
+    n = (40, 36, 34)
+    cfg = K.polycrystal(8, grains=9, seed=21)
+    cfg["n"] = n
+    cfg["seeds"] = K.voronoi_seeds(np.random.default_rng(21), 9, n)
+    C, _ = H.voronoi_cubic_field(cfg, oracle.cubic_define)
+    op = ela.Elastic(n, MM, mode=ela.ALLOC_BOTH)
+    op.SetModelVoronoiCubic(cfg["seeds"], K.VORONOI_Q, cfg["R"], cfg["C11"], cfg["C12"], cfg["C44"])
+    t, v = H.bc_mixed(3)
+    op.SetBC(t, v)
+    op.SetStiffnessMode(1)
+    op.prepareForSolve()
+    gc = oracle.Grid(n).coarse()
+    Cc = np.ascontiguousarray(C[:, ::2, ::2, ::2])
+    uc = H.random_field(np.random.default_rng(5), gc.n, 3)
+    want = oracle.fapply(gc, MM, Cc, t, uc)
+    s = scale_of(Cc, uc, gc.dx)
+    assert np.abs(op.Fapply(uc, mglev=1, prec=ela.F64) - want).max() <= 1e-12 * s
+    assert np.abs(op.Fapply(uc, mglev=1, prec=ela.F32) - want).max() <= 2e-5 * s
+    op.close()
+
+
+def test_coarse_marching_levels_give_the_plane_kernel_bits():
+    """On a 256^3 grain-form hierarchy, level 1 (129^3) runs the LDS marching kernels; with OPT_MARCH 0 it
+    runs the per-node plane kernels on the same injected records.  Both evaluate the same expressions, so
+    level 1's operator and the whole mixed solve agree bit for bit (numpy equality)."""
+    cfg = K.polycrystal(256, grains=16, seed=4)
+    u1 = H.random_field(np.random.default_rng(8), (128, 128, 128), 3)
+    out = []
+    for march in (0, 1):
+        op = ela.Elastic(cfg["n"], MM, mode=ela.ALLOC_BOTH)
+        op.SetModelVoronoiCubic(cfg["seeds"], K.VORONOI_Q, cfg["R"], cfg["C11"], cfg["C12"], cfg["C44"])
+        op.SetBC(cfg["bc_type"], cfg["bc_value"])
+        op.SetStiffnessMode(1)
+        op.SetKrylov(6)
+        op.set_option(ela.OPT_MARCH, march)
+        op.prepareForSolve()
+        f64, f32 = op.Fapply(u1, mglev=1, prec=ela.F64), op.Fapply(u1, mglev=1, prec=ela.F32)
+        b = op.field(0, ela.F64).upload(op.bc_init())
+        u = op.field(0, ela.F64)
+        u.zero()
+        rc, st = op.solve_device(b, u, 1e-10, 0.0, ela.SOLVE_MIXED, 100)
+        assert rc == 0 and st["converged"]
+        out.append((f64, f32, u.download(), st["iters"]))
+        op.close()
+    for a, c in zip(out[0], out[1]):
+        assert np.array_equal(a, c)
+
+
 def test_autotune_picks_a_tile_shape_without_changing_results():
     cfg = K.polycrystal(40, grains=10, seed=6)
     sols = []
