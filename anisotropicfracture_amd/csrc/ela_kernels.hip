@@ -568,16 +568,20 @@ __global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src, d
 constexpr int FB = 256, FG = 1024, CH = 8;
 
 // Partial dot products of x with up to CH vectors: partials[(v0 + v) * FG + block].
-__global__ __launch_bounds__(FB) void k_mdot(const double* __restrict__ x, VecList Y, int v0, int nv, long long n,
-                                             double* __restrict__ partials) {
-    double acc[CH];
+// TB: the list's precision (products and sums in FP64).  self >= 0: also
+// (x, x) into partials[self * FG + block], summed as a list entry equal to x is.
+template <typename TB>
+__global__ __launch_bounds__(FB) void k_mdot(const double* __restrict__ x, VecListT<TB> Y, int v0, int nv, long long n,
+                                             double* __restrict__ partials, int self) {
+    double acc[CH], accs = 0.0;
 #pragma unroll
     for (int v = 0; v < CH; ++v) acc[v] = 0.0;
     for (long long q = (long long)blockIdx.x * FB + threadIdx.x; q < n; q += (long long)FG * FB) {
         const double xq = x[q];
 #pragma unroll
         for (int v = 0; v < CH; ++v)
-            if (v < nv) acc[v] += xq * Y.v[v0 + v][q];
+            if (v < nv) acc[v] += xq * (double)Y.v[v0 + v][q];
+        if (self >= 0) accs += xq * xq;
     }
     __shared__ double red[FB / 64][CH];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -587,11 +591,22 @@ __global__ __launch_bounds__(FB) void k_mdot(const double* __restrict__ x, VecLi
         for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
         if (lane == 0) red[w][v] = a;
     }
+    __shared__ double reds[FB / 64];
+    if (self >= 0) {
+        double a = accs;
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+        if (lane == 0) reds[w] = a;
+    }
     __syncthreads();
     if (threadIdx.x < CH && (int)threadIdx.x < nv) {
         double a = 0.0;
         for (int ww = 0; ww < FB / 64; ++ww) a += red[ww][threadIdx.x];
         partials[(long long)(v0 + threadIdx.x) * FG + blockIdx.x] = a;
+    }
+    if (self >= 0 && threadIdx.x == CH) {
+        double a = 0.0;
+        for (int ww = 0; ww < FB / 64; ++ww) a += reds[ww];
+        partials[(long long)self * FG + blockIdx.x] = a;
     }
 }
 
@@ -610,24 +625,26 @@ __global__ __launch_bounds__(256) void k_mdot_final(const double* __restrict__ p
 }
 
 // y += sign * sum_v coef[v] X_v
-__global__ __launch_bounds__(FB) void k_maxpy(double* __restrict__ y, VecList X, int nv,
+template <typename TB>
+__global__ __launch_bounds__(FB) void k_maxpy(double* __restrict__ y, VecListT<TB> X, int nv,
                                               const double* __restrict__ coef, double sign, long long n) {
     for (long long q = (long long)blockIdx.x * FB + threadIdx.x; q < n; q += (long long)FG * FB) {
         double a = 0.0;
-        for (int v = 0; v < nv; ++v) a += coef[v] * X.v[v][q];
+        for (int v = 0; v < nv; ++v) a += coef[v] * (double)X.v[v][q];
         y[q] += sign * a;
     }
 }
 
 // y += sign * sum_v coef[v] X_v, and the block partials of |y|^2 (the new y),
 // reduced as k_mdot reduces (x, x): partials[block].
-__global__ __launch_bounds__(FB) void k_maxpy_norm(double* __restrict__ y, VecList X, int nv,
+template <typename TB>
+__global__ __launch_bounds__(FB) void k_maxpy_norm(double* __restrict__ y, VecListT<TB> X, int nv,
                                                    const double* __restrict__ coef, double sign, long long n,
                                                    double* __restrict__ partials) {
     double acc = 0.0;
     for (long long q = (long long)blockIdx.x * FB + threadIdx.x; q < n; q += (long long)FG * FB) {
         double a = 0.0;
-        for (int v = 0; v < nv; ++v) a += coef[v] * X.v[v][q];
+        for (int v = 0; v < nv; ++v) a += coef[v] * (double)X.v[v][q];
         const double yq = y[q] + sign * a;
         y[q] = yq;
         acc += yq * yq;
@@ -665,22 +682,56 @@ __global__ __launch_bounds__(FB) void k_scale_flat_to(double* y, const double* x
 void launch_mdot(const double* x, const VecList& Y, int nv, long long n, double* partials, double* out,
                  hipStream_t s) {
     for (int v0 = 0; v0 < nv; v0 += CH)
-        hipLaunchKernelGGL(k_mdot, dim3(FG), dim3(FB), 0, s, x, Y, v0, std::min(CH, nv - v0), n, partials);
+        hipLaunchKernelGGL(k_mdot<double>, dim3(FG), dim3(FB), 0, s, x, Y, v0, std::min(CH, nv - v0), n, partials,
+                           -1);
     hipLaunchKernelGGL(k_mdot_final, dim3(nv), dim3(256), 0, s, partials, out);
 }
 
+void launch_mdot_self(const double* x, const VecList32& Y, int nv, long long n, double* partials, double* out,
+                      hipStream_t s) {
+    for (int v0 = 0; v0 < nv || v0 == 0; v0 += CH)
+        hipLaunchKernelGGL(k_mdot<float>, dim3(FG), dim3(FB), 0, s, x, Y, v0, std::max(0, std::min(CH, nv - v0)), n,
+                           partials, v0 == 0 ? nv : -1);
+    hipLaunchKernelGGL(k_mdot_final, dim3(nv + 1), dim3(256), 0, s, partials, out);
+}
+
 void launch_maxpy(double* y, const VecList& X, int nv, const double* coef, double sign, long long n, hipStream_t s) {
-    hipLaunchKernelGGL(k_maxpy, dim3(FG), dim3(FB), 0, s, y, X, nv, coef, sign, n);
+    hipLaunchKernelGGL(k_maxpy<double>, dim3(FG), dim3(FB), 0, s, y, X, nv, coef, sign, n);
 }
 
 void launch_maxpy_norm(double* y, const VecList& X, int nv, const double* coef, double sign, long long n,
                        double* partials, double* norm2, hipStream_t s) {
-    hipLaunchKernelGGL(k_maxpy_norm, dim3(FG), dim3(FB), 0, s, y, X, nv, coef, sign, n, partials);
+    hipLaunchKernelGGL(k_maxpy_norm<double>, dim3(FG), dim3(FB), 0, s, y, X, nv, coef, sign, n, partials);
+    hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(256), 0, s, partials, norm2);
+}
+
+void launch_maxpy32(double* y, const VecList32& X, int nv, const double* coef, double sign, long long n,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(k_maxpy<float>, dim3(FG), dim3(FB), 0, s, y, X, nv, coef, sign, n);
+}
+
+void launch_maxpy_norm32(double* y, const VecList32& X, int nv, const double* coef, double sign, long long n,
+                         double* partials, double* norm2, hipStream_t s) {
+    hipLaunchKernelGGL(k_maxpy_norm<float>, dim3(FG), dim3(FB), 0, s, y, X, nv, coef, sign, n, partials);
     hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(256), 0, s, partials, norm2);
 }
 
 void launch_scale_flat(double* y, const double* x, double a, long long n, hipStream_t s) {
     hipLaunchKernelGGL(k_scale_flat, dim3(FG), dim3(FB), 0, s, y, x, a, n);
+}
+
+// v = (float) (a x) and r = v, flat (an FP32 Krylov vector and the next MG
+// cycle's right-hand side).
+__global__ __launch_bounds__(FB) void k_scale_to2f(const double* __restrict__ x, double a, float* __restrict__ v,
+                                                   float* __restrict__ r, long long n) {
+    for (long long q = (long long)blockIdx.x * FB + threadIdx.x; q < n; q += (long long)FG * FB) {
+        const float f = (float)(a * x[q]);
+        v[q] = f;
+        r[q] = f;
+    }
+}
+void launch_scale_to2f(const double* x, double a, float* v, float* r, long long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_scale_to2f, dim3(FG), dim3(FB), 0, s, x, a, v, r, n);
 }
 
 template <typename TO>

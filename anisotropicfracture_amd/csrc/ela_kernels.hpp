@@ -150,12 +150,19 @@ constexpr int kMaxKrylov = 33;
 // FGMRES(6): the bound 1/2 re-orthogonalized nearly every step at the same
 // iteration count and final residual, 10 % slower.
 constexpr double kReorth2 = 0.01;
-struct VecList {
-    const double* v[kMaxKrylov];
+template <typename TB>
+struct VecListT {
+    const TB* v[kMaxKrylov];
 };
+using VecList = VecListT<double>;
+using VecList32 = VecListT<float>;
 // out[v] = (x, Y_v) for v < nv (deterministic two-pass reduction; partials: mdot_partials(nv) doubles).
 void launch_mdot(const double* x, const VecList& Y, int nv, long long n, double* partials, double* out,
                  hipStream_t s);
+// out[v] = (x, Y_v) for v < nv with an FP32 list (FP64 sums), and out[nv] = (x, x)
+// (partials: mdot_partials(nv + 1) doubles).
+void launch_mdot_self(const double* x, const VecList32& Y, int nv, long long n, double* partials, double* out,
+                      hipStream_t s);
 long long mdot_partials(int nv);
 // y += sign * sum_v coef[v] X_v, coef on the device.
 void launch_maxpy(double* y, const VecList& X, int nv, const double* coef, double sign, long long n, hipStream_t s);
@@ -163,11 +170,18 @@ void launch_maxpy(double* y, const VecList& X, int nv, const double* coef, doubl
 // order launch_mdot sums (y, y) (partials: mdot_partials(1) doubles).
 void launch_maxpy_norm(double* y, const VecList& X, int nv, const double* coef, double sign, long long n,
                        double* partials, double* norm2, hipStream_t s);
+// The same with an FP32 list (FP64 arithmetic).
+void launch_maxpy32(double* y, const VecList32& X, int nv, const double* coef, double sign, long long n,
+                    hipStream_t s);
+void launch_maxpy_norm32(double* y, const VecList32& X, int nv, const double* coef, double sign, long long n,
+                         double* partials, double* norm2, hipStream_t s);
 // y = a x
 void launch_scale_flat(double* y, const double* x, double a, long long n, hipStream_t s);
 // y = a x and yo = (TO) y, flat over n elements (yo: a field of the same layout)
 template <typename TO>
 void launch_scale_flat_to(double* y, const double* x, double a, TO* yo, long long n, hipStream_t s);
+// v = (float) (a x), r = v, flat over n elements
+void launch_scale_to2f(const double* x, double a, float* v, float* r, long long n, hipStream_t s);
 // out = (double) (T) (cor + hold) on the valid nodes of a level (nc components)
 template <typename T>
 void launch_add_to64(const Lay& L, int nc, const T* hold, const T* cor, double* out, hipStream_t s);

@@ -153,6 +153,8 @@ struct ela_hier {
     // flexible GMRES around the MG cycle (restart length; 0 = plain MG iteration)
     int krylov = 0;
     std::vector<double*> V, Z;
+    std::vector<float*> V32;  // FP32 Krylov basis of the mixed solve
+    double* kw = nullptr;     // its FP64 work vector w = A z
     double *kr = nullptr, *kdev = nullptr, *kpart = nullptr;
     // cycle options
     int cycle = ELA_CYCLE_F, nu1 = 2, nu2 = 2;
@@ -385,45 +387,65 @@ int solve_loop(ela_hier* h, const double* b, double* u, double tol_rel, double t
     return status;
 }
 
-int krylov_alloc(ela_hier* h) {
+// Krylov storage: m FP64 preconditioned vectors z, the residual, the device
+// scalars, and the basis: m + 1 FP64 vectors, or (b32) m + 1 FP32 vectors and
+// one FP64 work vector.
+int krylov_alloc(ela_hier* h, bool b32) {
     const int m = h->krylov;
-    if ((int)h->Z.size() == m && (int)h->V.size() == m + 1) return ELA_OK;
+    const bool basis_ok = b32 ? ((int)h->V32.size() == m + 1 && h->kw && h->V.empty())
+                              : ((int)h->V.size() == m + 1 && h->V32.empty());
+    if ((int)h->Z.size() == m && basis_ok) return ELA_OK;
     for (double* p : h->V) hfree(h, p);
+    for (float* p : h->V32) hfree(h, p);
     for (double* p : h->Z) hfree(h, p);
     h->V.clear();
+    h->V32.clear();
     h->Z.clear();
-    if (h->kr) hfree(h, h->kr);
-    if (h->kdev) hfree(h, h->kdev);
-    if (h->kpart) hfree(h, h->kpart);
-    h->kr = h->kdev = h->kpart = nullptr;
+    for (double** p : {&h->kr, &h->kdev, &h->kpart, &h->kw}) {
+        if (*p) hfree(h, *p);
+        *p = nullptr;
+    }
     const size_t fe = field_elems(h->lev[0], h->dim);
     int rc = ELA_OK;
     for (int q = 0; q <= m && !rc; ++q) {
-        double* p = nullptr;
-        rc = halloc_t(h, fe, &p);
-        h->V.push_back(p);
+        if (b32) {
+            float* p = nullptr;
+            rc = halloc_t(h, fe, &p);
+            h->V32.push_back(p);
+        } else {
+            double* p = nullptr;
+            rc = halloc_t(h, fe, &p);
+            h->V.push_back(p);
+        }
     }
     for (int q = 0; q < m && !rc; ++q) {
         double* p = nullptr;
         rc = halloc_t(h, fe, &p);
         h->Z.push_back(p);
     }
+    if (!rc && b32) rc = halloc_t(h, fe, &h->kw);
     if (!rc) rc = halloc_t(h, fe, &h->kr);
     if (!rc) rc = halloc_t(h, 4 * (size_t)(m + 2), &h->kdev);
     if (!rc) rc = halloc_t(h, (size_t)mdot_partials(m + 1), &h->kpart);
     return rc;
 }
 
-// Right-preconditioned flexible GMRES(m) in FP64 with classical Gram-Schmidt
-// and one re-orthogonalization; the preconditioner is one MG cycle in
+// Right-preconditioned flexible GMRES(m) with classical Gram-Schmidt and a
+// selective re-orthogonalization; the preconditioner is one MG cycle in
 // precision T (FP32 cycles make it vary slightly, which the flexible variant
 // allows).  The inner iteration stops when the Arnoldi residual estimate
 // |r|_2 falls below the inf-norm target (so |r|_inf does too); convergence is
-// then confirmed on the true FP64 residual.
+// then confirmed on the true FP64 residual, from which every restart begins.
+// FP64 solves keep the basis in FP64.  Mixed solves keep it in FP32 (w = A z,
+// the dot products, the updates and the solution stay FP64): a basis vector
+// rounded to FP32 perturbs the Arnoldi relation by ~1e-7 of the restart's
+// residual, well below the reduction one restart asks for, and the vectors
+// are the MG cycle's FP32 right-hand sides as they stand.
 template <typename T>
 int solve_fgmres(ela_hier* h, const double* b, double* u, double tol_rel, double tol_abs, int max_iter,
                  ela_stats* st) {
-    if (int rc = krylov_alloc(h)) return rc;
+    constexpr bool B32 = sizeof(T) == 4;
+    if (int rc = krylov_alloc(h, B32)) return rc;
     const int m = h->krylov;
     Level& v0 = h->lev[0];
     const long long n = (long long)field_elems(v0, h->dim);
@@ -459,7 +481,8 @@ int solve_fgmres(ela_hier* h, const double* b, double* u, double tol_rel, double
         const double beta = std::sqrt(host_vec(1)[0]);
         if (!(beta > 0)) break;
         // V0 and, in the same pass, the first cycle's right-hand side (T) V0
-        launch_scale_flat_to<T>(h->V[0], h->kr, 1.0 / beta, *lb<T>(v0).res, n, s);
+        if constexpr (B32) launch_scale_to2f(h->kr, 1.0 / beta, h->V32[0], *lb<T>(v0).res, n, s);
+        else launch_scale_flat_to<T>(h->V[0], h->kr, 1.0 / beta, *lb<T>(v0).res, n, s);
         std::fill(g.begin(), g.end(), 0.0);
         g[0] = beta;
         int jend = 0;
@@ -468,22 +491,31 @@ int solve_fgmres(ela_hier* h, const double* b, double* u, double tol_rel, double
             // z_j = M^-1 v_j: one MG cycle on level 0 from zero; res already holds
             // (T) v_j, written with v_j; an F-cycle writes z_j itself
             if (!one_cycle<T>(h, h->Z[j])) launch_axpy<T, double>(v0.L, h->dim, *lbv.cor, h->Z[j], nullptr, 1.0, 1, s);
-            // w = A z_j into V[j+1]
+            // w = A z_j (into V[j+1] with an FP64 basis)
+            double* const w = B32 ? h->kw : h->V[j + 1];
             {
                 const int e = ev_begin(h);
-                A_apply<double>(h, v0, v0.C64, h->Z[j], h->V[j + 1], s);
+                A_apply<double>(h, v0, v0.C64, h->Z[j], w, s);
                 ev_end(h, e, h->ev_resid);
             }
             VecList Vl{};
-            for (int q = 0; q <= j; ++q) Vl.v[q] = h->V[q];
+            VecList32 Vl32{};
+            for (int q = 0; q <= j; ++q) {
+                if constexpr (B32) Vl32.v[q] = h->V32[q];
+                else Vl.v[q] = h->V[q];
+            }
             // classical Gram-Schmidt, h1 = V^T w, w -= V h1, with a second pass only
             // when w lost most of its norm (|w'| < 0.1 |w|, kReorth2: the test of
             // Daniel et al. with a smaller bound).
             // One pass over w gives h1 and |w|^2 (w is the last vector of the
             // list); the update w -= V h1 also sums |w'|^2.
-            VecList Vw = Vl;
-            Vw.v[j + 1] = h->V[j + 1];
-            launch_mdot(h->V[j + 1], Vw, j + 2, n, h->kpart, h->kdev, s);
+            if constexpr (B32) {
+                launch_mdot_self(w, Vl32, j + 1, n, h->kpart, h->kdev, s);
+            } else {
+                VecList Vw = Vl;
+                Vw.v[j + 1] = w;
+                launch_mdot(w, Vw, j + 2, n, h->kpart, h->kdev, s);
+            }
             std::vector<double> h1 = host_vec(j + 2);
             const double wn0 = std::sqrt(h1[j + 1]);
             h1.resize(j + 1);
@@ -492,12 +524,19 @@ int solve_fgmres(ela_hier* h, const double* b, double* u, double tol_rel, double
             for (int q = 0; q <= j; ++q) hn2 -= h1[q] * h1[q];
             double* const norm2 = h->kdev + 2 * (m + 2);  // beyond the coefficients the update reads
             if (!(hn2 > kReorth2 * wn0 * wn0)) {
-                launch_maxpy(h->V[j + 1], Vl, j + 1, h->kdev, -1.0, n, s);
-                launch_mdot(h->V[j + 1], Vl, j + 1, n, h->kpart, h->kdev, s);
-                launch_maxpy_norm(h->V[j + 1], Vl, j + 1, h->kdev, -1.0, n, h->kpart, norm2, s);
+                if constexpr (B32) {
+                    launch_maxpy32(w, Vl32, j + 1, h->kdev, -1.0, n, s);
+                    launch_mdot_self(w, Vl32, j + 1, n, h->kpart, h->kdev, s);
+                    launch_maxpy_norm32(w, Vl32, j + 1, h->kdev, -1.0, n, h->kpart, norm2, s);
+                } else {
+                    launch_maxpy(w, Vl, j + 1, h->kdev, -1.0, n, s);
+                    launch_mdot(w, Vl, j + 1, n, h->kpart, h->kdev, s);
+                    launch_maxpy_norm(w, Vl, j + 1, h->kdev, -1.0, n, h->kpart, norm2, s);
+                }
                 h2 = host_vec(j + 1);
             } else {
-                launch_maxpy_norm(h->V[j + 1], Vl, j + 1, h->kdev, -1.0, n, h->kpart, norm2, s);
+                if constexpr (B32) launch_maxpy_norm32(w, Vl32, j + 1, h->kdev, -1.0, n, h->kpart, norm2, s);
+                else launch_maxpy_norm(w, Vl, j + 1, h->kdev, -1.0, n, h->kpart, norm2, s);
             }
             double hn2_dev = 0.0;
             HIPCHK(hipMemcpyAsync(&hn2_dev, norm2, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -507,7 +546,10 @@ int solve_fgmres(ela_hier* h, const double* b, double* u, double tol_rel, double
             for (int q = 0; q <= j; ++q) H[(size_t)q * m + j] = h1[q] + h2[q];
             H[(size_t)(j + 1) * m + j] = hn;
             // v_{j+1}, and the next cycle's right-hand side (T) v_{j+1} in the same pass
-            if (hn > 0) launch_scale_flat_to<T>(h->V[j + 1], h->V[j + 1], 1.0 / hn, *lb<T>(v0).res, n, s);
+            if (hn > 0) {
+                if constexpr (B32) launch_scale_to2f(w, 1.0 / hn, h->V32[j + 1], *lb<T>(v0).res, n, s);
+                else launch_scale_flat_to<T>(h->V[j + 1], w, 1.0 / hn, *lb<T>(v0).res, n, s);
+            }
             // Givens rotations
             for (int q = 0; q < j; ++q) {
                 const double a = H[(size_t)q * m + j], c = H[(size_t)(q + 1) * m + j];
