@@ -21,6 +21,12 @@ static_assert(3 * MHALO <= MTX * MTY, "one halo node per thread and component");
 constexpr int kGrainNC = 21;
 constexpr int kMarchMaxGrains = 256;  // the table is staged in LDS up to this many grains
 
+// First i of a box's tile columns.  Where rows start on a multiple of MTX
+// elements (the single-box layout), tiles start at i = 0, so a tile row of the
+// interior streams (rhs, diagonal, output) is whole cache lines; node 0 of the
+// first tile is a shell node and is skipped.  Compact layouts start at i = 1.
+__host__ __device__ inline int grain_x0(const Lay& L) { return L.px % MTX == 0 ? 0 : 1; }
+
 // SC: the launch carries per-node factors (LDS ss holds them, 1 for a box
 // without); else the records are read as they are.  The values are the same,
 // since 1 x = x exactly.
@@ -132,10 +138,10 @@ __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchL
     const Lay& L = g.L;
     const int tid = threadIdx.x;
     const int bx = t % g.ntx, by = (t / g.ntx) % g.nty, bz = t / (g.ntx * g.nty);
-    const int i0 = 1 + bx * MTX, j0 = 1 + by * MTY;  // first interior node of the tile
+    const int i0 = grain_x0(L) + bx * MTX, j0 = 1 + by * MTY;  // first node of the tile
     const int tx = tid % MTX, ty = tid / MTX;
     const int i = i0 + tx, j = j0 + ty;
-    const bool active = i <= L.nx - 2 && j <= L.ny - 2;
+    const bool active = i >= 1 && i <= L.nx - 2 && j <= L.ny - 2;
     const int kb = 1 + bz * g.kz, ke = min(kb + g.kz, L.nz - 1);
     for (int q = tid; q < g.ngr * kGrainNC; q += MTX * MTY) stab[q] = g.tab[q];
 
@@ -151,7 +157,7 @@ __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchL
     const int hp = tid / MHALO, hh = tid - hp * MHALO;
     const int hr = hh < MCW ? 0 : hh < 2 * MCW ? MRW - 1 : 1 + ((hh - 2 * MCW) >> 1);
     const int hc = hh < MCW ? hh : hh < 2 * MCW ? hh - MCW : (((hh - 2 * MCW) & 1) ? MCW - 1 : 0);
-    const bool hok = tid < 3 * MHALO && i0 - 1 + hc < L.nx && j0 - 1 + hr < L.ny;
+    const bool hok = tid < 3 * MHALO && i0 - 1 + hc >= 0 && i0 - 1 + hc < L.nx && j0 - 1 + hr < L.ny;
     const bool gok = hok && tid < MHALO;
     const unsigned he = (unsigned)(hp * L.cs + (i0 - 1 + hc) + L.px * (j0 - 1 + hr));
     const int hpos = hp * MPL + hr * MCW + hc;
@@ -281,7 +287,7 @@ __host__ __device__ inline bool grain_offsets_fit(const Lay& L) { return 3LL * L
 
 // Tiles and shell blocks of a box (host and device).
 __host__ __device__ inline void grain_tiles(const Lay& L, int kz, int& ntx, int& nty, int& ntz, int& nsh) {
-    ntx = (L.nx - 2 + MTX - 1) / MTX;
+    ntx = (L.nx - 1 - grain_x0(L) + MTX - 1) / MTX;
     nty = (L.ny - 2 + MTY - 1) / MTY;
     ntz = (L.nz - 2 + kz - 1) / kz;
     const long long shell = (long long)L.nx * L.ny * L.nz - (long long)(L.nx - 2) * (L.ny - 2) * (L.nz - 2);

@@ -83,6 +83,7 @@ struct Level {
     float* d32 = nullptr;  // inverse diagonal of the FP32 smoothers
     unsigned char* sm = nullptr;  // same-direction mask of the stiffness planes (launch_same_mask)
     unsigned short* gid = nullptr;  // coarse levels of a grain-form hierarchy: grain ids by injection
+    int kz = 0;                     // coarse marching level: tile height picked by ela_autotune (0: the hierarchy's)
     bool sm_ok = false;           // sm was built by the last ela_prepare
     bool use_sm = false;          // enough wavefronts skip a derivative of C for the mask to pay
     unsigned long long* nl = nullptr;  // node list (launch_node_list): constant-C interior nodes first
@@ -709,6 +710,10 @@ bool grain_on(ela_hier* h, const Level& v) {
     return v.gid && h->dim == 3 && v.L.nx >= kMarchMinNodes && v.L.ny >= kMarchMinNodes && v.L.nz >= kMarchMinNodes;
 }
 
+// Marching tile height of a level: level 0 and untuned levels use the
+// hierarchy's (ELA_OPT_KZ or the autotuner's level-0 choice).
+int level_kz(const ela_hier* h, const Level& v) { return &v != &h->lev[0] && v.kz > 0 ? v.kz : h->kz; }
+
 template <typename T>
 GrainBox<T> grain_box(ela_hier* h, const Level& v) {
     GrainBox<T> g{};
@@ -730,7 +735,7 @@ void A_apply(ela_hier* h, Level& v, const T* C, const T* u, T* f, hipStream_t s)
         GrainBox<T> g = grain_box<T>(h, v);
         g.in = u;
         g.out = f;
-        launch_grain<T, 3, T>(g, h->kz, nullptr, nullptr, s);
+        launch_grain<T, 3, T>(g, level_kz(h, v), nullptr, nullptr, s);
     } else if (table_on(h, v)) launch_fapply_tab<T>(v.L, tab_of<T>(h), u, f, h->bc, v.h, h->uniform, s);
     else if (v.mp.ok) march_fapply<T>(v.mp, v.L, h->kind, C, u, f, h->bc, v.h, h->uniform, s);
     else launch_fapply<T>(v.L, h->kind, C, u, f, h->bc, v.h, h->uniform, s, plane_opt<T>(h, v));
@@ -745,7 +750,7 @@ void A_jacobi(ela_hier* h, Level& v, const T* C, const T* dg, const T* xin, cons
         g.out = xout;
         g.dg = dg;
         g.omega = (T)h->omega;
-        launch_grain<T, 0, T>(g, h->kz, nullptr, nullptr, s);
+        launch_grain<T, 0, T>(g, level_kz(h, v), nullptr, nullptr, s);
     } else if (table_on(h, v)) launch_jacobi_tab<T>(v.L, tab_of<T>(h), dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
     else if (v.mp.ok) march_jacobi<T>(v.mp, v.L, h->kind, C, dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
     else launch_jacobi<T>(v.L, h->kind, C, dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s, plane_opt<T>(h, v));
@@ -761,12 +766,12 @@ void A_residual(ela_hier* h, Level& v, const T* C, const T* x, const T* b, T* r,
         g.rhs = b;
         if (plain) {
             g.out = r;
-            launch_grain<T, 1, T>(g, h->kz, nullptr, nullptr, s);
+            launch_grain<T, 1, T>(g, level_kz(h, v), nullptr, nullptr, s);
         } else {
             g.out = rs;
             g.out2 = (double*)r;
             g.scale = scale;
-            grain_resid64<T, TO>(g, h->kz, h->partials, norm, s);
+            grain_resid64<T, TO>(g, level_kz(h, v), h->partials, norm, s);
         }
     } else if (table_on(h, v)) launch_residual_tab<T, TO>(v.L, tab_of<T>(h), x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
     else if (v.mp.ok) march_residual<T, TO>(v.mp, v.L, h->kind, C, x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
@@ -1567,6 +1572,7 @@ int ela_set_option(ela_hier* h, int option, int value) {
     } else if (option == ELA_OPT_KZ) {
         if (value < 1 || value > 1024) return fail(ELA_ERR_BADARG, "planes per marching tile must be 1..1024");
         h->kz = value;
+        for (Level& v : h->lev) v.kz = 0;
     } else if (option == ELA_OPT_XCD) {
         h->xcd = value != 0;
     } else if (option == ELA_OPT_MASK) {
@@ -1634,6 +1640,31 @@ int autotune_t(ela_hier* h, int nrep, double* ms) {
         }
     h->kz = bkz;
     h->xcd = bx != 0;
+    // the coarse marching levels: their own tile height (fewer tiles per
+    // level, so a lower height can fill the chip where level 0's would not)
+    const int ckz[4] = {4, 8, 16, 32};
+    for (size_t l = 1; l < h->lev.size(); ++l) {
+        Level& c = h->lev[l];
+        if (!grain_on(h, c)) continue;
+        LB<T> cb = lb<T>(c);
+        double cbest = 1e300;
+        int ck = 0;
+        for (int a = 0; a < 4; ++a) {
+            c.kz = ckz[a];
+            A_jacobi<T>(h, c, cb.C, cb.dg, *cb.cor, *cb.res, *cb.tmp, h->stream);
+            HIPCHK(hipEventRecord(e0, h->stream));
+            for (int r = 0; r < nrep; ++r) A_jacobi<T>(h, c, cb.C, cb.dg, *cb.cor, *cb.res, *cb.tmp, h->stream);
+            HIPCHK(hipEventRecord(e1, h->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, e0, e1));
+            if (t < cbest) {
+                cbest = t;
+                ck = ckz[a];
+            }
+        }
+        c.kz = ck;
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return ELA_OK;

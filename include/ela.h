@@ -265,7 +265,9 @@ int ela_set_option(ela_hier* h, int option, int value);
 /* Autotuner of the marching kernels (no reference counterpart): times nrep
  * level-0 Jacobi sweeps on this hierarchy's data for every tile height in
  * {8, 16, 32} and both tile orders, and keeps the fastest (ELA_OPT_KZ,
- * ELA_OPT_XCD).  ms (6 doubles, may be NULL) receives the average sweep
+ * ELA_OPT_XCD); then gives every coarse marching level the fastest of the
+ * heights {4, 8, 16, 32} for its own sweeps (ELA_OPT_KZ resets those to the
+ * hierarchy's).  ms (6 doubles, may be NULL) receives level 0's average sweep
  * times, kz 8/16/32 x order 0/1.  Requires ela_prepare and grain form. */
 int ela_autotune(ela_hier* h, int nrep, double* ms);
 

@@ -82,6 +82,24 @@ def test_diagonal_parity(oracle, dim, kind, n):
     np.testing.assert_allclose(op.Diagonal(0, ela.F32), want, rtol=2e-7, atol=1e-30)
 
 
+@pytest.mark.parametrize("n", [(128, 6, 4), (256, 4, 4)])
+def test_prolongation_of_wide_levels(oracle, n):
+    """Rows of 129+ nodes take the x-pair prolongation kernel (the last pair of
+    a row holds one node): FP64 against the oracle, FP32 against the oracle on
+    the FP32-rounded inputs."""
+    rng = np.random.default_rng(29)
+    g = oracle.Grid(n)
+    gc = g.coarse()
+    op = ela.Elastic(n, MM, coarse_min=2)
+    assert op.levels() >= 2
+    f = H.random_field(rng, n, 3)
+    c = H.random_field(rng, gc.n, 3)
+    np.testing.assert_allclose(op.interpolation(f, c, 0), oracle.interp_add(g, c, f), rtol=1e-14, atol=1e-15)
+    got = op.interpolation(f, c, 0, prec=ela.F32)
+    want = oracle.interp_add(g, c.astype(np.float32).astype(np.float64), f.astype(np.float32).astype(np.float64))
+    np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-6)
+
+
 @pytest.mark.parametrize("dim,n", [(2, (16, 8)), (3, (16, 8, 8))])
 def test_transfer_and_coarse_operator_parity(oracle, dim, n):
     rng = np.random.default_rng(17)
