@@ -640,6 +640,25 @@ TabDesc<double> tab_of<double>(ela_hier* h) { return {h->gid, h->tab64, h->dgt64
 template <>
 TabDesc<float> tab_of<float>(ela_hier* h) { return {h->gid, h->tab32, h->dgt32}; }
 bool table_on(ela_hier* h, const Level& v) { return h->use_table && h->has_table && &v == &h->lev[0]; }
+// Coarse levels of a grain-form hierarchy below the marching size, from
+// kTabMinNodes nodes a side: the node kernels read the injected grain ids and
+// the table (50 B a node in FP32) instead of the 21 injected stiffness planes
+// (132 B); the records are the planes' values, so the bits are the plane
+// kernels' (up to the sign of a zero).  Measured on MI355X (256^3
+// polycrystal, FP32): the 65^3 sweep 24.1 -> 18.7 us; at 33^3 and below the
+// grain-id -> record gather lengthens the latency-bound sweeps (7.7 -> 8.8 us).
+constexpr int kTabMinNodes = 65;
+bool coarse_tab_on(ela_hier* h, const Level& v) {
+    return h->use_table && h->has_table && &v != &h->lev[0] && v.gid && h->dim == 3 && h->kind == K_MM &&
+           v.L.nx >= kTabMinNodes && v.L.ny >= kTabMinNodes && v.L.nz >= kTabMinNodes;
+}
+template <typename T>
+TabDesc<T> coarse_tab(ela_hier* h, const Level& v) {
+    TabDesc<T> t = tab_of<T>(h);
+    t.gid = v.gid;
+    t.dgt = nullptr;  // the per-grain interior diagonal is level 0's
+    return t;
+}
 // The grain form holds on every level of a grain-form hierarchy: the coarse
 // levels take grain ids and stiffness by injection (so a coarse node's record
 // is its grain's), not the full weighting of the per-node planes.
@@ -737,6 +756,7 @@ void A_apply(ela_hier* h, Level& v, const T* C, const T* u, T* f, hipStream_t s)
         g.out = f;
         launch_grain<T, 3, T>(g, level_kz(h, v), nullptr, nullptr, s);
     } else if (table_on(h, v)) launch_fapply_tab<T>(v.L, tab_of<T>(h), u, f, h->bc, v.h, h->uniform, s);
+    else if (coarse_tab_on(h, v)) launch_fapply_tab<T>(v.L, coarse_tab<T>(h, v), u, f, h->bc, v.h, h->uniform, s);
     else if (v.mp.ok) march_fapply<T>(v.mp, v.L, h->kind, C, u, f, h->bc, v.h, h->uniform, s);
     else launch_fapply<T>(v.L, h->kind, C, u, f, h->bc, v.h, h->uniform, s, plane_opt<T>(h, v));
 }
@@ -752,6 +772,7 @@ void A_jacobi(ela_hier* h, Level& v, const T* C, const T* dg, const T* xin, cons
         g.omega = (T)h->omega;
         launch_grain<T, 0, T>(g, level_kz(h, v), nullptr, nullptr, s);
     } else if (table_on(h, v)) launch_jacobi_tab<T>(v.L, tab_of<T>(h), dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
+    else if (coarse_tab_on(h, v)) launch_jacobi_tab<T>(v.L, coarse_tab<T>(h, v), dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
     else if (v.mp.ok) march_jacobi<T>(v.mp, v.L, h->kind, C, dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
     else launch_jacobi<T>(v.L, h->kind, C, dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s, plane_opt<T>(h, v));
 }
@@ -774,6 +795,7 @@ void A_residual(ela_hier* h, Level& v, const T* C, const T* x, const T* b, T* r,
             grain_resid64<T, TO>(g, level_kz(h, v), h->partials, norm, s);
         }
     } else if (table_on(h, v)) launch_residual_tab<T, TO>(v.L, tab_of<T>(h), x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
+    else if (coarse_tab_on(h, v)) launch_residual_tab<T, TO>(v.L, coarse_tab<T>(h, v), x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
     else if (v.mp.ok) march_residual<T, TO>(v.mp, v.L, h->kind, C, x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
     else launch_residual<T, TO>(v.L, h->kind, C, x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s,
                                 plane_opt<T>(h, v));

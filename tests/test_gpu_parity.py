@@ -343,10 +343,12 @@ def test_plane_mask_keeps_the_bits(mode):
             assert np.array_equal(a, c)
 
 
-def test_grain_form_coarse_operator_is_the_injected_one(oracle):
+@pytest.mark.parametrize("n", [(40, 36, 34), (128, 128, 128)])
+def test_grain_form_coarse_operator_is_the_injected_one(oracle, n):
     """A grain-form hierarchy coarsens by injection: level 1's operator is the oracle's operator with the
-    stiffness of the fine nodes (2I, 2J, 2K), i.e. every coarse node keeps its grain's record."""
-    n = (40, 36, 34)
+    stiffness of the fine nodes (2I, 2J, 2K), i.e. every coarse node keeps its grain's record.  A level
+    of 65+ nodes a side below the marching size runs the grain-table node kernels, a smaller one the
+    per-node planes."""
     cfg = K.polycrystal(8, grains=9, seed=21)
     cfg["n"] = n
     cfg["seeds"] = K.voronoi_seeds(np.random.default_rng(21), 9, n)
