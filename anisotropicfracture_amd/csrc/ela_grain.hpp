@@ -72,6 +72,7 @@ struct GrainBox {
     int ngr;
     int ntx, nty, kz;
     int xcd;  // XCD-contiguous tile order
+    bool nt;  // rhs, diagonal and Jacobi output non-temporal (see epilogue_load)
     const T* in;
     const T* rhs;
     void* out;
@@ -82,17 +83,25 @@ struct GrainBox {
 };
 
 // The epilogue's operands of node o: rhs (MODES 0-2) and the diagonal (MODE 0).
-// Nothing else reads them near in time, so they are loaded non-temporally:
-// they then leave more of the L2 to the lines of u and of the grain ids that
-// neighbouring tiles re-read as halos (measured: 0.299 -> 0.291 ms a 257^3
-// FP32 sweep).  The Jacobi output is stored the same way.
+// With g.nt (levels far larger than the caches) nothing reads them again
+// before they are evicted, so they are loaded non-temporally: they then leave
+// more of the L2 to the lines of u and of the grain ids that neighbouring
+// tiles re-read as halos (measured: 0.299 -> 0.291 ms a 257^3 FP32 sweep).
+// The Jacobi output is stored the same way.  A 129^3 level's fields stay in
+// the caches between its sweeps, and the hint made its sweep slower (59 -> 67
+// us), so smaller levels load normally.
 template <typename T, int MODE>
 __device__ __forceinline__ void epilogue_load(const GrainBox<T>& g, long long o, T rb[3], T db[3]) {
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
         const long long q = p * g.L.cs + o;
-        rb[p] = MODE != 3 ? __builtin_nontemporal_load(g.rhs + q) : T(0);
-        db[p] = MODE == 0 ? __builtin_nontemporal_load(g.dg + q) : T(1);
+        if (g.nt) {
+            rb[p] = MODE != 3 ? __builtin_nontemporal_load(g.rhs + q) : T(0);
+            db[p] = MODE == 0 ? __builtin_nontemporal_load(g.dg + q) : T(1);
+        } else {
+            rb[p] = MODE != 3 ? g.rhs[q] : T(0);
+            db[p] = MODE == 0 ? g.dg[q] : T(1);
+        }
     }
 }
 
@@ -107,7 +116,9 @@ __device__ __forceinline__ double grain_epilogue(const GrainBox<T>& g, long long
     for (int p = 0; p < 3; ++p) {
         const long long q = p * cs + o;
         if (MODE == 0) {
-            __builtin_nontemporal_store(jacobi_update(xc[p], g.omega, rb[p] - f[p], db[p]), (T*)g.out + q);
+            const T x = jacobi_update(xc[p], g.omega, rb[p] - f[p], db[p]);
+            if (g.nt) __builtin_nontemporal_store(x, (T*)g.out + q);
+            else ((T*)g.out)[q] = x;
         } else if (MODE == 1) {
             ((T*)g.out)[q] = rb[p] - f[p];
         } else if (MODE == 2) {

@@ -733,6 +733,9 @@ bool grain_on(ela_hier* h, const Level& v) {
 // hierarchy's (ELA_OPT_KZ or the autotuner's level-0 choice).
 int level_kz(const ela_hier* h, const Level& v) { return &v != &h->lev[0] && v.kz > 0 ? v.kz : h->kz; }
 
+// Levels from this many nodes (a 257^3 level, not a 129^3 one) take the
+// non-temporal epilogue accesses of the marching kernels (GrainBox::nt).
+constexpr long long kNtMinNodes = 8LL << 20;
 template <typename T>
 GrainBox<T> grain_box(ela_hier* h, const Level& v) {
     GrainBox<T> g{};
@@ -745,6 +748,7 @@ GrainBox<T> grain_box(ela_hier* h, const Level& v) {
     g.sc = nullptr;
     g.ngr = h->ngrains;
     g.xcd = h->xcd;
+    g.nt = (long long)v.nodes >= kNtMinNodes;
     return g;
 }
 
