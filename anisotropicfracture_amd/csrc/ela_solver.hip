@@ -136,6 +136,18 @@ struct ela_hier {
     int* dflag = nullptr;
     unsigned long long* mstats = nullptr;  // per level: wavefronts, skipping (wavefront, direction) pairs
     int mask_mode = 1;                     // ELA_OPT_MASK: 0 never, 1 where it pays, 2 / 3 every level
+    // ELA_OPT_GRAPH: the launches of levels >= 1 replayed from captured HIP
+    // graphs, one per launch configuration (the key holds everything those
+    // launches read on the host)
+    bool graph_on = false;
+    long long epoch = 0;  // bumped by every ela_prepare
+    struct CycleGraph {
+        std::vector<unsigned long long> key;
+        hipGraphExec_t exec = nullptr;
+        long long used = 0;
+    };
+    std::vector<CycleGraph> graphs;
+    long long graph_clock = 0;
     unsigned long long* hnorm = nullptr;  // pinned
     // grain-table stiffness of level 0 (Voronoi models): per-node grain id, per-grain records
     unsigned short* gid = nullptr;
@@ -269,41 +281,46 @@ void bottom(ela_hier* h) {
     launch_bottom<T>(v.L, v.nodes, h->dim, h->M, *b.res, *b.cor, h->stream);
 }
 
+// The V-cycle's way down on level l (pre-smoothing, correction residual,
+// restriction) and its way up (prolongation, post-smoothing).
+template <typename T>
+void v_down(ela_hier* h, int l) {
+    Level& v = h->lev[l];
+    LB<T> b = lb<T>(v);
+    if (h->nu1 == 0) (void)hipMemsetAsync(*b.cor, 0, field_elems(v, h->dim) * sizeof(T), h->stream);
+    for (int s = 0; s < h->nu1; ++s) smooth_call<T>(h, l, s == 0);
+    res_of_cor<T>(h, l);
+    Level& c = h->lev[l + 1];
+    launch_restrict<T>(c.L, v.L, h->dim, *b.rc, *lb<T>(c).res, h->stream);
+}
+template <typename T>
+void v_up(ela_hier* h, int l) {
+    Level& v = h->lev[l];
+    Level& c = h->lev[l + 1];
+    launch_prolong<T>(v.L, c.L, h->dim, *lb<T>(c).cor, *lb<T>(v).cor, 1, h->stream);
+    for (int s = 0; s < h->nu2; ++s) smooth_call<T>(h, l, false);
+}
+
 template <typename T>
 void vcycle(ela_hier* h, int top) {
     const int bot = (int)h->lev.size() - 1;
-    if (top == bot) {
-        bottom<T>(h);
-        return;
-    }
-    for (int l = top; l < bot; ++l) {
-        Level& v = h->lev[l];
-        LB<T> b = lb<T>(v);
-        if (h->nu1 == 0) (void)hipMemsetAsync(*b.cor, 0, field_elems(v, h->dim) * sizeof(T), h->stream);
-        for (int s = 0; s < h->nu1; ++s) smooth_call<T>(h, l, s == 0);
-        res_of_cor<T>(h, l);
-        Level& c = h->lev[l + 1];
-        launch_restrict<T>(c.L, v.L, h->dim, *b.rc, *lb<T>(c).res, h->stream);
-    }
+    for (int l = top; l < bot; ++l) v_down<T>(h, l);
     bottom<T>(h);
-    for (int l = bot - 1; l >= top; --l) {
-        Level& v = h->lev[l];
-        Level& c = h->lev[l + 1];
-        launch_prolong<T>(v.L, c.L, h->dim, *lb<T>(c).cor, *lb<T>(v).cor, 1, h->stream);
-        for (int s = 0; s < h->nu2; ++s) smooth_call<T>(h, l, false);
-    }
+    for (int l = bot - 1; l >= top; --l) v_up<T>(h, l);
 }
 
-// out64 (may be null): level 0's final correction cor + hold is written there
-// in FP64 (the preconditioned vector of a Krylov step) instead of into cor.
+// An F-cycle's part on levels 1..bot: the restrictions of level 0's rhs down
+// to the coarsest, the coarsest solve, and the V-cycles from levels bot-1 ..
+// 1 with their initial guesses and corrections.  Each level l swaps res with
+// rc and cor with hold on the host (fcycle_swaps repeats that for a replay).
 template <typename T>
-void fcycle(ela_hier* h, double* out64 = nullptr) {
+void fcycle_upper(ela_hier* h) {
     const int bot = (int)h->lev.size() - 1;
     for (int l = 1; l <= bot; ++l)
         launch_restrict<T>(h->lev[l].L, h->lev[l - 1].L, h->dim, *lb<T>(h->lev[l - 1]).res, *lb<T>(h->lev[l]).res,
                            h->stream);
     bottom<T>(h);
-    for (int l = bot - 1; l >= 0; --l) {
+    for (int l = bot - 1; l >= 1; --l) {
         Level& v = h->lev[l];
         Level& c = h->lev[l + 1];
         LB<T> b = lb<T>(v);
@@ -312,21 +329,158 @@ void fcycle(ela_hier* h, double* out64 = nullptr) {
         std::swap(*b.res, *b.rc);
         std::swap(*b.cor, *b.hold);
         vcycle<T>(h, l);
-        if (l == 0 && out64) launch_add_to64<T>(v.L, h->dim, *b.hold, *b.cor, out64, h->stream);
-        else launch_axpy<T, T>(v.L, h->dim, *b.hold, *b.cor, nullptr, 1.0, 0, h->stream);
+        launch_axpy<T, T>(v.L, h->dim, *b.hold, *b.cor, nullptr, 1.0, 0, h->stream);
+    }
+}
+template <typename T>
+void fcycle_swaps(ela_hier* h) {
+    for (int l = (int)h->lev.size() - 2; l >= 1; --l) {
+        LB<T> b = lb<T>(h->lev[l]);
+        std::swap(*b.res, *b.rc);
+        std::swap(*b.cor, *b.hold);
     }
 }
 
+// Everything the launches of levels >= 1 read on the host: buffers,
+// stiffness, per-level options, the set-up epoch (masks, node lists, kernel
+// choices), the boundary table.
+template <typename T>
+std::vector<unsigned long long> cycle_key(ela_hier* h, int kind) {
+    std::vector<unsigned long long> k;
+    auto put = [&](const void* p) { k.push_back((unsigned long long)(uintptr_t)p); };
+    k.push_back((unsigned long long)kind);
+    k.push_back(sizeof(T));
+    k.push_back((unsigned long long)h->epoch);
+    k.push_back(((unsigned long long)h->kz << 32) | ((unsigned long long)h->xcd << 16) |
+                ((unsigned long long)h->mask_mode << 8) | (unsigned long long)h->grain_march);
+    k.push_back(((unsigned long long)h->nu1 << 32) | (unsigned long long)h->nu2);
+    unsigned long long om;
+    std::memcpy(&om, &h->omega, sizeof om);
+    k.push_back(om);
+    k.push_back(((unsigned long long)h->uniform << 8) | (unsigned long long)h->use_table);
+    put(h->stream);
+    put(h->M);
+    put(h->dbc);
+    put(h->gid);
+    put(h->tab64);
+    put(h->tab32);
+    put(h->dgt64);
+    put(h->dgt32);
+    for (int f = 0; f < 26; ++f)
+        k.push_back((unsigned long long)h->bc.t[f][0] | ((unsigned long long)h->bc.t[f][1] << 8) |
+                    ((unsigned long long)h->bc.t[f][2] << 16));
+    for (Level& v : h->lev) {
+        LB<T> b = lb<T>(v);
+        put(b.C);
+        put(b.dg);
+        put(*b.res);
+        put(*b.rc);
+        put(*b.cor);
+        put(*b.hold);
+        put(*b.tmp);
+        put(v.gid);
+        k.push_back((unsigned long long)v.kz);
+    }
+    return k;
+}
+
+// The launches of levels >= 1 (kind 0: fcycle_upper, kind 1: the V-cycle
+// from level 1) replayed from a HIP graph captured on first use of a key;
+// false when graphs are off or the capture failed (the caller then launches
+// them).  Level 0's launches, the long ones, stay outside with their timing
+// events.  Graphs hold kernel nodes only: a cycle without pre-smoothing
+// clears cor with hipMemsetAsync and is never captured.
+constexpr int kMaxCycleGraphs = 8;
+template <typename T>
+bool graph_run(ela_hier* h, int kind) {
+    if (!h->graph_on || h->nu1 == 0 || h->lev.size() < 2) return false;
+    auto body = [&] {
+        if (kind == 0) fcycle_upper<T>(h);
+        else vcycle<T>(h, 1);
+    };
+    auto swaps = [&] {
+        if (kind == 0) fcycle_swaps<T>(h);
+    };
+    std::vector<unsigned long long> key = cycle_key<T>(h, kind);
+    for (auto& g : h->graphs)
+        if (g.key == key) {
+            if (hipGraphLaunch(g.exec, h->stream) != hipSuccess) return false;
+            g.used = ++h->graph_clock;
+            swaps();
+            return true;
+        }
+    if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        h->graph_on = false;
+        return false;
+    }
+    body();  // records the launches and makes the host swaps
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const bool ok = hipStreamEndCapture(h->stream, &graph) == hipSuccess && graph &&
+                    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (graph) (void)hipGraphDestroy(graph);
+    if (!ok) {
+        (void)hipGetLastError();
+        swaps();  // nothing ran: back to the state before the capture
+        h->graph_on = false;
+        return false;
+    }
+    if ((int)h->graphs.size() >= kMaxCycleGraphs) {
+        auto lru = std::min_element(h->graphs.begin(), h->graphs.end(),
+                                    [](const auto& x, const auto& y) { return x.used < y.used; });
+        (void)hipGraphExecDestroy(lru->exec);
+        h->graphs.erase(lru);
+    }
+    h->graphs.push_back({key, exec, ++h->graph_clock});
+    // the recorded launches use the state before the swaps
+    if (hipGraphLaunch(exec, h->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        swaps();
+        h->graph_on = false;
+        return false;
+    }
+    return true;
+}
+
+void drop_graphs(ela_hier* h) {
+    for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
+    h->graphs.clear();
+}
+
+// The V-cycle from level 0, its part on levels >= 1 from a graph when on.
+template <typename T>
+void vcycle0(ela_hier* h) {
+    if (h->lev.size() < 2) {
+        bottom<T>(h);
+        return;
+    }
+    v_down<T>(h, 0);
+    if (!graph_run<T>(h, 1)) vcycle<T>(h, 1);
+    v_up<T>(h, 0);
+}
+
 // One MG cycle on level 0's res into cor; with out64, an F-cycle leaves its
-// correction in out64 (FP64) instead, and one_cycle returns true.
+// correction cor + hold in out64 (FP64, the preconditioned vector of a
+// Krylov step) instead of in cor, and one_cycle returns true.
 template <typename T>
 bool one_cycle(ela_hier* h, double* out64 = nullptr) {
-    if (h->cycle == ELA_CYCLE_F) {
-        fcycle<T>(h, out64);
-        return out64 != nullptr;
+    if (h->cycle != ELA_CYCLE_F || h->lev.size() < 2) {
+        vcycle0<T>(h);
+        return false;
     }
-    vcycle<T>(h, 0);
-    return false;
+    if (!graph_run<T>(h, 0)) fcycle_upper<T>(h);
+    Level& v = h->lev[0];
+    Level& c = h->lev[1];
+    LB<T> b = lb<T>(v);
+    launch_prolong<T>(v.L, c.L, h->dim, *lb<T>(c).cor, *b.cor, 0, h->stream);
+    res_of_cor<T>(h, 0);
+    std::swap(*b.res, *b.rc);
+    std::swap(*b.cor, *b.hold);
+    vcycle0<T>(h);
+    if (out64) launch_add_to64<T>(v.L, h->dim, *b.hold, *b.cor, out64, h->stream);
+    else launch_axpy<T, T>(v.L, h->dim, *b.hold, *b.cor, nullptr, 1.0, 0, h->stream);
+    return out64 != nullptr;
 }
 
 // FP64 solution residual of level 0 into the T right-hand side of the next
@@ -1010,6 +1164,7 @@ int ela_hier_create_ex(ela_ctx* ctx, const ela_domain* dom, int model_kind, int 
 int ela_hier_destroy(ela_hier* h) {
     if (!h) return ELA_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    drop_graphs(h);
     for (auto& e : h->ev) (void)hipEventDestroy(e);
     while (!h->allocs.empty()) hfree(h, h->allocs.back().first);
     if (h->hnorm) (void)hipHostFree(h->hnorm);
@@ -1195,6 +1350,7 @@ int ela_set_cycle(ela_hier* h, int cycle, int nu1, int nu2) {
 int ela_prepare(ela_hier* h) {
     if (!h) return fail(ELA_ERR_BADARG, "null hierarchy");
     if (!h->model_set) return fail(ELA_ERR_STATE, "Attempting to use operator before calling SetModel");
+    ++h->epoch;  // captured cycles of the previous set-up no longer apply
     // stiffness on every MG level (FP64), then FP32 copies
     for (size_t l = 1; l < h->lev.size(); ++l) {
         if (grain_levels(h))  // the records of the coarse grain ids (Elastic.cpp:759-803 averages instead)
@@ -1604,6 +1760,8 @@ int ela_set_option(ela_hier* h, int option, int value) {
     } else if (option == ELA_OPT_MASK) {
         if (value < 0 || value > 3) return fail(ELA_ERR_BADARG, "mask mode must be 0, 1, 2 or 3");
         h->mask_mode = value;
+    } else if (option == ELA_OPT_GRAPH) {
+        h->graph_on = value != 0;
     } else {
         return fail(ELA_ERR_BADARG, "unknown option");
     }

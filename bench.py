@@ -330,6 +330,13 @@ def main():
     res_norms = {"inf": rinf_mx / float(np.abs(b_host).max()),
                  "l2": float(np.linalg.norm(r_mx) / np.linalg.norm(b_host))}
     del r_mx
+    # the same mixed solves with the cycles' launches on levels >= 1 replayed from HIP graphs (opt-in)
+    tgr = None
+    if args.table:
+        op.set_option(ela.OPT_GRAPH, 1)
+        tgr, _, stgr, _, _ = run_mode(op, b, u, ela.SOLVE_MIXED, args.steps, args.warmup, args.tol)
+        op.set_option(ela.OPT_GRAPH, 0)
+        log("bench: mixed, graph-replayed levels >= 1: %.3f s/solve iters=%d" % (tgr / args.steps, stgr["iters"]))
     op.SetStiffnessMode(0)
     # HBM copy probe (also the byte-counter calibration kernel of the PMC runs)
     probe_bytes = 1 << 30
@@ -422,7 +429,8 @@ def main():
         "variants_solves_per_s": {"fp64_planes": args.steps / t64 if t64 else None,
                                   "mixed_planes": args.steps / tmp if tmp else None,
                                   "fp64_table": args.steps / t64t if t64t else None,
-                                  "mixed_table": args.steps / tmx if args.table else None},
+                                  "mixed_table": args.steps / tmx if args.table else None,
+                                  "mixed_table_graph": args.steps / tgr if tgr else None},
         "iters": {"fp64": st64["iters"] if st64 else None, "mixed": stmx["iters"]},
         "rel_residual": {"fp64": st64["res"] / st64["res0"] if st64 else None, "mixed": stmx["res"] / stmx["res0"]},
         "hbm_copy_probe_GBps": probe_GBps,

@@ -258,8 +258,15 @@ int ela_set_krylov(ela_hier* h, int restart);
  *                inside each block), so that whole wavefronts skip.  0 never,
  *                1 on the levels where it pays (default), 2 every plane level
  *                with node lists, 3 every plane level with in-block
- *                regrouping.  Hierarchies only. */
-enum { ELA_OPT_MARCH = 0, ELA_OPT_KZ = 1, ELA_OPT_XCD = 2, ELA_OPT_MASK = 3 };
+ *                regrouping.  Hierarchies only.
+ * ELA_OPT_GRAPH  1: the cycles' launches on levels >= 1 (about 255 of an
+ *                F-cycle's 270) are captured into HIP graphs, one per launch
+ *                configuration and buffer state, and replayed; level 0's
+ *                launches stay on the stream.  Same kernels in the same
+ *                order, so the same results.  0 (default): stream launches
+ *                (on MI355X the replay measured no faster: the small levels'
+ *                cost is each kernel's own latency).  Hierarchies only. */
+enum { ELA_OPT_MARCH = 0, ELA_OPT_KZ = 1, ELA_OPT_XCD = 2, ELA_OPT_MASK = 3, ELA_OPT_GRAPH = 4 };
 int ela_set_option(ela_hier* h, int option, int value);
 
 /* Autotuner of the marching kernels (no reference counterpart): times nrep

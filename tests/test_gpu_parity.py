@@ -414,3 +414,45 @@ def test_autotune_picks_a_tile_shape_without_changing_results():
         sols.append(u)
         op.close()
     assert np.array_equal(sols[0], sols[1])
+
+
+@pytest.mark.parametrize("case", ["polycrystal", "iso_vcycle"])
+def test_graph_replayed_cycles_keep_the_bits(case):
+    """OPT_GRAPH 1 replays the cycles' launches on levels >= 1 from captured HIP graphs (one per buffer
+    state: the F-cycle's exchanges alternate two).  Several solves in a row (each replays the graphs many
+    times between eager level-0 work), a changed option and a new set-up in between (new captures) and both
+    precisions must give the stream launches' bits."""
+    out = []
+    for graph in (0, 1):
+        if case == "polycrystal":
+            cfg = K.polycrystal(48, grains=7, seed=12)
+            op = ela.Elastic(cfg["n"], MM, mode=ela.ALLOC_BOTH)
+            op.SetModelVoronoiCubic(cfg["seeds"], K.VORONOI_Q, cfg["R"], cfg["C11"], cfg["C12"], cfg["C44"])
+            op.SetBC(cfg["bc_type"], cfg["bc_value"])
+            op.SetStiffnessMode(1)
+            b = None
+        else:
+            n = (16, 16, 32)
+            op = make(n, ISO, H.random_stiffness(np.random.default_rng(3), n, ISO, 3), *H.bc_mixed(3))
+            op.SetCycle(ela.CYCLE_V, 2, 2)
+            b = H.random_field(np.random.default_rng(4), n, 3)
+        op.SetKrylov(6)
+        op.set_option(ela.OPT_GRAPH, graph)
+        op.prepareForSolve()
+        if b is None:
+            b = op.bc_init()
+        res = []
+        for rep in range(3):
+            if rep == 2:
+                op.set_option(ela.OPT_KZ, 8)
+                op.prepareForSolve()
+            for mode in (ela.SOLVE_MIXED, ela.SOLVE_FP64):
+                u, st = ela.Linear(op).solve(np.zeros_like(b), b, tol_rel=1e-10, tol_abs=0.0, mode=mode,
+                                             max_iter=300)
+                assert st["converged"]
+                res.append((u, st["iters"]))
+        out.append(res)
+        op.close()
+    for (ua, ia), (ub, ib) in zip(out[0], out[1]):
+        assert ia == ib
+        assert np.array_equal(ua, ub)
