@@ -370,9 +370,10 @@ def test_grain_form_coarse_operator_is_the_injected_one(oracle, n):
 
 
 def test_coarse_marching_levels_give_the_plane_kernel_bits():
-    """On a 256^3 grain-form hierarchy, level 1 (129^3) runs the LDS marching kernels; with OPT_MARCH 0 it
-    runs the per-node plane kernels on the same injected records.  Both evaluate the same expressions, so
-    level 1's operator and the whole mixed solve agree bit for bit (numpy equality)."""
+    """On a 256^3 grain-form hierarchy, level 1 (129^3) runs the LDS marching kernels (with the tile heights
+    the autotuner picks for levels 0 and 1); with OPT_MARCH 0 it runs the per-node kernels on the same
+    injected records.  Both evaluate the same expressions, so level 1's operator and the whole mixed solve
+    agree bit for bit (numpy equality)."""
     cfg = K.polycrystal(256, grains=16, seed=4)
     u1 = H.random_field(np.random.default_rng(8), (128, 128, 128), 3)
     out = []
@@ -384,6 +385,8 @@ def test_coarse_marching_levels_give_the_plane_kernel_bits():
         op.SetKrylov(6)
         op.set_option(ela.OPT_MARCH, march)
         op.prepareForSolve()
+        if march:
+            op.autotune(nrep=1)
         f64, f32 = op.Fapply(u1, mglev=1, prec=ela.F64), op.Fapply(u1, mglev=1, prec=ela.F32)
         b = op.field(0, ela.F64).upload(op.bc_init())
         u = op.field(0, ela.F64)
