@@ -315,41 +315,47 @@ __global__ __launch_bounds__(256) void k_prolong(Lay Lf, Lay Lc, int nc, const T
     }
 }
 
-// The same in 3D with one thread per fine x-pair (2I, 2I + 1) of a row:
-// both nodes read the coarse column I (the odd one also I + 1), so a lane
-// loads each coarse value once, a wavefront's lanes share the row's (j, k)
-// parities (no divergence inside a row) and the fine pair moves in one
-// access.  Each node's value is prolong_node's expression, so the bits are
-// the per-node kernel's.
+// The same in 3D with one thread per coarse node (I, J, K): the up to 8 fine nodes
+// (2I + a, 2J + b, 2K + c), a, b, c in {0, 1}, read only the coarse nodes
+// (I..I+1, J..J+1, K..K+1), so each coarse value is loaded once per thread
+// for 8 fine nodes, and each fine row pair moves in one access.  Each fine
+// node is prolong_node's expression (its loads repeat the thread's and are
+// merged), so the bits are the per-node kernel's.
 template <typename T>
-__global__ __launch_bounds__(256) void k_prolong_pairs(Lay Lf, Lay Lc, int nc, const T* __restrict__ crse,
-                                                       T* __restrict__ fine, int add) {
+__global__ __launch_bounds__(256) void k_prolong_oct(Lay Lf, Lay Lc, int nc, const T* __restrict__ crse,
+                                                     T* __restrict__ fine, int add) {
     using T2 = std::conditional_t<sizeof(T) == 8, double2, float2>;
-    const unsigned np = (unsigned)(Lf.nx + 1) >> 1;
-    const unsigned t = blockIdx.x * 256u + threadIdx.x;  // pair of plane k = blockIdx.y
-    if (t >= np * (unsigned)Lf.ny) return;
-    const int I = (int)(t % np), j = (int)(t / np), k = (int)blockIdx.y, i = 2 * I;
+    const unsigned t = blockIdx.x * 256u + threadIdx.x;  // coarse node of plane K = blockIdx.y
+    if (t >= (unsigned)Lc.nx * (unsigned)Lc.ny) return;
+    const int I = (int)(t % (unsigned)Lc.nx), J = (int)(t / (unsigned)Lc.nx), K = (int)blockIdx.y;
+    const int i = 2 * I, j0 = 2 * J, k0 = 2 * K;
     const bool two = i + 1 < Lf.nx;
-    const long long oc = (long long)I + Lc.px * (j >> 1) + Lc.pl * (k >> 1);
-    const long long of = (long long)i + Lf.px * j + Lf.pl * k;
+    const long long oc = (long long)I + Lc.px * J + Lc.pl * K;
     for (int n = 0; n < nc; ++n) {
         const T* c = crse + n * Lc.cs + oc;
-        T* fp = fine + n * Lf.cs + of;
-        T v0 = prolong_node<T>(Lc, i, j, k, c);
-        if (two) {
-            T v1 = prolong_node<T>(Lc, i + 1, j, k, c);
-            if (add) {
-                const T2 f = *(const T2*)fp;
-                v0 = f.x + v0;
-                v1 = f.y + v1;
+#pragma unroll
+        for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+            for (int dj = 0; dj < 2; ++dj) {
+                const int j = j0 + dj, k = k0 + dk;
+                if (j >= Lf.ny || k >= Lf.nz) continue;
+                T* fp = fine + n * Lf.cs + (long long)i + Lf.px * j + Lf.pl * k;
+                T v0 = prolong_node<T>(Lc, i, j, k, c);
+                if (two) {
+                    T v1 = prolong_node<T>(Lc, i + 1, j, k, c);
+                    if (add) {
+                        const T2 f = *(const T2*)fp;
+                        v0 = f.x + v0;
+                        v1 = f.y + v1;
+                    }
+                    T2 o;
+                    o.x = v0;
+                    o.y = v1;
+                    *(T2*)fp = o;
+                } else {
+                    *fp = add ? (*fp + v0) : v0;
+                }
             }
-            T2 o;
-            o.x = v0;
-            o.y = v1;
-            *(T2*)fp = o;
-        } else {
-            *fp = add ? (*fp + v0) : v0;
-        }
     }
 }
 
@@ -1258,15 +1264,17 @@ void launch_restrict(const Lay& Lc, const Lay& Lf, int nc, const T* fine, T* crs
 
 template <typename T>
 void launch_prolong(const Lay& Lf, const Lay& Lc, int nc, const T* crse, T* fine, int add, hipStream_t s) {
-    // The pair kernel: fine levels of 129+ nodes a row (measured: 129^3 FP32
-    // 26 -> 23 us, 257^3 193 -> 159 us; at 65^3 and below the per-node kernel's
-    // shorter dependency chain is as fast or faster), pair-aligned fine
-    // accesses (even pitches, a 2-element-aligned base).
+    // The octant kernel: fine levels of 129+ nodes a row (measured, FP32: a
+    // 257^3 level takes 193 us with a thread per fine node, 159 us with a
+    // thread per fine x-pair, 106 us with a thread per coarse node; 129^3:
+    // 26, 23, 17 us; at 65^3 and below the per-node kernel's shorter
+    // dependency chain is as fast or faster), pair-aligned fine accesses (even
+    // pitches, a 2-element-aligned base).
     const bool pairs = Lf.dim == 3 && Lf.nx >= 129 && Lf.px % 2 == 0 && Lf.pl % 2 == 0 && Lf.cs % 2 == 0 &&
                        (uintptr_t)fine % (2 * sizeof(T)) == 0;
     if (pairs) {
-        const unsigned nt = (unsigned)((Lf.nx + 1) / 2) * (unsigned)Lf.ny;
-        hipLaunchKernelGGL((k_prolong_pairs<T>), dim3((nt + 255) / 256, Lf.nz), dim3(256), 0, s, Lf, Lc, nc, crse,
+        const unsigned nt = (unsigned)Lc.nx * (unsigned)Lc.ny;
+        hipLaunchKernelGGL((k_prolong_oct<T>), dim3((nt + 255) / 256, Lc.nz), dim3(256), 0, s, Lf, Lc, nc, crse,
                            fine, add);
     } else if (Lf.dim == 3) {
         hipLaunchKernelGGL((k_prolong<T, 3>), node_grid(Lf), dim3(BX, BY), 0, s, Lf, Lc, nc, crse, fine, add);

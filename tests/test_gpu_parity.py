@@ -84,8 +84,9 @@ def test_diagonal_parity(oracle, dim, kind, n):
 
 @pytest.mark.parametrize("n", [(128, 6, 4), (256, 4, 4)])
 def test_prolongation_of_wide_levels(oracle, n):
-    """Rows of 129+ nodes take the x-pair prolongation kernel (the last pair of
-    a row holds one node): FP64 against the oracle, FP32 against the oracle on
+    """Fine levels of 129+ nodes a row take the prolongation kernel with a thread
+    per coarse node (the last coarse node of a row, column or plane has one fine
+    node in that direction): FP64 against the oracle, FP32 against the oracle on
     the FP32-rounded inputs."""
     rng = np.random.default_rng(29)
     g = oracle.Grid(n)
