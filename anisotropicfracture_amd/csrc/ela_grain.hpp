@@ -59,7 +59,10 @@ struct LdsCtx {
 // Operands of one grain-form box.  MODE 0: Jacobi sweep out = in + omega (rhs -
 // A in) / dg; MODE 1: out = rhs - A in; MODE 2: v = rhs - A in in FP64, out
 // (precision TO, may be null) = v * (*scale), out2 (FP64, may be null) = v,
-// block maxima of |v|; MODE 3: out = A in.
+// block maxima of |v|; MODE 3: out = A in; MODE 4: the Jacobi sweep of MODE 0
+// followed by k_add_to64's step: out2 (FP64) = (T) (x + hold) with x the
+// sweep's value, which is not stored (a cycle's last sweep, whose correction
+// goes straight into an FP64 Krylov vector).
 template <typename T>
 struct GrainBox {
     Lay L;
@@ -78,6 +81,7 @@ struct GrainBox {
     void* out;
     double* out2;
     const T* dg;
+    const T* hold;  // MODE 4
     const double* scale;
     T omega;
 };
@@ -97,10 +101,10 @@ __device__ __forceinline__ void epilogue_load(const GrainBox<T>& g, long long o,
         const long long q = p * g.L.cs + o;
         if (g.nt) {
             rb[p] = MODE != 3 ? __builtin_nontemporal_load(g.rhs + q) : T(0);
-            db[p] = MODE == 0 ? __builtin_nontemporal_load(g.dg + q) : T(1);
+            db[p] = (MODE == 0 || MODE == 4) ? __builtin_nontemporal_load(g.dg + q) : T(1);
         } else {
             rb[p] = MODE != 3 ? g.rhs[q] : T(0);
-            db[p] = MODE == 0 ? g.dg[q] : T(1);
+            db[p] = (MODE == 0 || MODE == 4) ? g.dg[q] : T(1);
         }
     }
 }
@@ -119,6 +123,11 @@ __device__ __forceinline__ double grain_epilogue(const GrainBox<T>& g, long long
             const T x = jacobi_update(xc[p], g.omega, rb[p] - f[p], db[p]);
             if (g.nt) __builtin_nontemporal_store(x, (T*)g.out + q);
             else ((T*)g.out)[q] = x;
+        } else if (MODE == 4) {
+            const T x = jacobi_update(xc[p], g.omega, rb[p] - f[p], db[p]);
+            const double v = (double)(T)((double)x + (double)g.hold[q]);
+            if (g.nt) __builtin_nontemporal_store(v, g.out2 + q);
+            else g.out2[q] = v;
         } else if (MODE == 1) {
             ((T*)g.out)[q] = rb[p] - f[p];
         } else if (MODE == 2) {

@@ -1607,6 +1607,8 @@ template void launch_grain<double, 1, double>(GrainBox<double>, int, double*, un
 template void launch_grain<double, 2, float>(GrainBox<double>, int, double*, unsigned long long*, hipStream_t);
 template void launch_grain<double, 2, double>(GrainBox<double>, int, double*, unsigned long long*, hipStream_t);
 template void launch_grain<float, 3, float>(GrainBox<float>, int, double*, unsigned long long*, hipStream_t);
+template void launch_grain<float, 4, float>(GrainBox<float>, int, double*, unsigned long long*, hipStream_t);
+template void launch_grain<double, 4, double>(GrainBox<double>, int, double*, unsigned long long*, hipStream_t);
 template void launch_grain<double, 3, double>(GrainBox<double>, int, double*, unsigned long long*, hipStream_t);
 
 // Per-grain interior diagonal sum_b C_{p b p b} (-2 / h_b^2) (Elastic.cpp:311-321 on an interior node).

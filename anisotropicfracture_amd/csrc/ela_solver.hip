@@ -232,6 +232,10 @@ template <typename T>
 void A_apply(ela_hier* h, Level& v, const T* C, const T* u, T* f, hipStream_t s);
 template <typename T>
 void A_jacobi(ela_hier* h, Level& v, const T* C, const T* dg, const T* xin, const T* b, T* xout, hipStream_t s);
+template <typename T>
+void grain_jacobi_to64(ela_hier* h, Level& v, const T* dg, const T* xin, const T* b, const T* hold, double* out64,
+                       hipStream_t s);
+bool grain_on(ela_hier* h, const Level& v);
 template <typename T, typename TO>
 void A_residual(ela_hier* h, Level& v, const T* C, const T* x, const T* b, T* r, TO* rs, const double* scale,
                 unsigned long long* norm, hipStream_t s);
@@ -251,8 +255,12 @@ void ev_end(ela_hier* h, int a, std::vector<std::pair<int, int>>& into) {
 }
 
 // ------------------------------------------------------------------ cycle pieces
+// One Fsmooth call (2 sweeps cor -> tmp -> cor).  With out64 (a grain-form
+// level 0 only), the second sweep writes (T) (its value + hold) into out64 in
+// FP64 instead of into cor (MODE 4: k_add_to64 folded into the sweep); that
+// sweep is left out of the level-0 sweep timing, whose bytes it does not match.
 template <typename T>
-void smooth_call(ela_hier* h, int l, bool zero_start) {
+void smooth_call(ela_hier* h, int l, bool zero_start, double* out64 = nullptr) {
     Level& v = h->lev[l];
     LB<T> b = lb<T>(v);
     const bool t0 = (l == 0);
@@ -261,6 +269,10 @@ void smooth_call(ela_hier* h, int l, bool zero_start) {
         const int e = t0 ? ev_begin(h) : -1;
         A_jacobi<T>(h, v, b.C, b.dg, *b.cor, *b.res, *b.tmp, h->stream);
         if (t0) ev_end(h, e, h->ev_smooth);
+    }
+    if (out64) {
+        grain_jacobi_to64<T>(h, v, b.dg, *b.tmp, *b.res, *b.hold, out64, h->stream);
+        return;
     }
     const int e = t0 ? ev_begin(h) : -1;
     A_jacobi<T>(h, v, b.C, b.dg, *b.tmp, *b.res, *b.cor, h->stream);
@@ -293,12 +305,13 @@ void v_down(ela_hier* h, int l) {
     Level& c = h->lev[l + 1];
     launch_restrict<T>(c.L, v.L, h->dim, *b.rc, *lb<T>(c).res, h->stream);
 }
+// out64: the last sweep writes cor + hold into out64 instead (smooth_call).
 template <typename T>
-void v_up(ela_hier* h, int l) {
+void v_up(ela_hier* h, int l, double* out64 = nullptr) {
     Level& v = h->lev[l];
     Level& c = h->lev[l + 1];
     launch_prolong<T>(v.L, c.L, h->dim, *lb<T>(c).cor, *lb<T>(v).cor, 1, h->stream);
-    for (int s = 0; s < h->nu2; ++s) smooth_call<T>(h, l, false);
+    for (int s = 0; s < h->nu2; ++s) smooth_call<T>(h, l, false, s == h->nu2 - 1 ? out64 : nullptr);
 }
 
 template <typename T>
@@ -450,14 +463,14 @@ void drop_graphs(ela_hier* h) {
 
 // The V-cycle from level 0, its part on levels >= 1 from a graph when on.
 template <typename T>
-void vcycle0(ela_hier* h) {
+void vcycle0(ela_hier* h, double* out64 = nullptr) {
     if (h->lev.size() < 2) {
         bottom<T>(h);
         return;
     }
     v_down<T>(h, 0);
     if (!graph_run<T>(h, 1)) vcycle<T>(h, 1);
-    v_up<T>(h, 0);
+    v_up<T>(h, 0, out64);
 }
 
 // One MG cycle on level 0's res into cor; with out64, an F-cycle leaves its
@@ -477,7 +490,10 @@ bool one_cycle(ela_hier* h, double* out64 = nullptr) {
     res_of_cor<T>(h, 0);
     std::swap(*b.res, *b.rc);
     std::swap(*b.cor, *b.hold);
-    vcycle0<T>(h);
+    // a grain-form level 0 folds the final cor + hold into its last sweep
+    const bool fold = out64 && h->nu2 > 0 && grain_on(h, v);
+    vcycle0<T>(h, fold ? out64 : nullptr);
+    if (fold) return true;
     if (out64) launch_add_to64<T>(v.L, h->dim, *b.hold, *b.cor, out64, h->stream);
     else launch_axpy<T, T>(v.L, h->dim, *b.hold, *b.cor, nullptr, 1.0, 0, h->stream);
     return out64 != nullptr;
@@ -933,6 +949,19 @@ void A_jacobi(ela_hier* h, Level& v, const T* C, const T* dg, const T* xin, cons
     else if (coarse_tab_on(h, v)) launch_jacobi_tab<T>(v.L, coarse_tab<T>(h, v), dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
     else if (v.mp.ok) march_jacobi<T>(v.mp, v.L, h->kind, C, dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
     else launch_jacobi<T>(v.L, h->kind, C, dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s, plane_opt<T>(h, v));
+}
+
+template <typename T>
+void grain_jacobi_to64(ela_hier* h, Level& v, const T* dg, const T* xin, const T* b, const T* hold, double* out64,
+                       hipStream_t s) {
+    GrainBox<T> g = grain_box<T>(h, v);
+    g.in = xin;
+    g.rhs = b;
+    g.dg = dg;
+    g.hold = hold;
+    g.out2 = out64;
+    g.omega = (T)h->omega;
+    launch_grain<T, 4, T>(g, level_kz(h, v), nullptr, nullptr, s);
 }
 
 template <typename T, typename TO>
