@@ -427,6 +427,7 @@ __device__ __forceinline__ GrainBox<T> grain_of(const BoxDesc& d, const MOps& op
     g.kz = d.kz;
     g.xcd = op.xcd;
     g.nt = false;
+    g.xface = false;  // the batched shells compute the x faces
     g.in = MODE == 2 ? (const T*)d.u64 : (const T*)d.f[op.in];
     g.rhs = MODE == 2 ? (const T*)d.b64 : (const T*)d.f[op.rhs];
     g.out = d.f[op.out];

@@ -75,7 +75,8 @@ struct GrainBox {
     int ngr;
     int ntx, nty, kz;
     int xcd;  // XCD-contiguous tile order
-    bool nt;  // rhs, diagonal and Jacobi output non-temporal (see epilogue_load)
+    bool nt;     // rhs, diagonal and Jacobi output non-temporal (see epilogue_load)
+    bool xface;  // x faces in face blocks (face_patch); the shell then skips them
     const T* in;
     const T* rhs;
     void* out;
@@ -264,7 +265,7 @@ __device__ __forceinline__ double shell_node(const GrainBox<T>& g, int q) {
     const Lay& L = g.L;
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const int A = nx * ny, Bf = nx * (nz - 2), Cf = (ny - 2) * (nz - 2);
-    if (q >= 2 * A + 2 * Bf + 2 * Cf) return 0.0;
+    if (q >= 2 * A + 2 * Bf + (g.xface ? 0 : 2 * Cf)) return 0.0;
     int i, j, k;
     if (q < 2 * A) {
         k = q < A ? 0 : nz - 1;
@@ -294,6 +295,58 @@ __device__ __forceinline__ double shell_node(const GrainBox<T>& g, int q) {
     return grain_epilogue<T, MODE, TO>(g, o, xc, f, rb, db);
 }
 
+// Face blocks (GrainBox::xface): block p takes a FJ x FK patch of an x face
+// (i = 0 for p < half the patches, else nx - 1) over j, k in [1, n - 2].  The
+// face's nodes lie a row apart, so every load of a wavefront there touches 64
+// lines; the block first stages the patch's two columns (the face and its
+// neighbour) with their one-node (j, k) ring in LDS, each line once, then
+// computes node_apply_acc's boundary row from LDS with node_grad's
+// expressions (one-sided x, central y and z differences), so the bits are the
+// shell's.  scratch: 3 * (FJ + 2) * (FK + 2) * 2 values of T.
+constexpr int FJ = 16, FK = 16;
+__host__ __device__ inline int grain_face_patches(const Lay& L) {
+    return 2 * ((L.ny - 2 + FJ - 1) / FJ) * ((L.nz - 2 + FK - 1) / FK);
+}
+template <typename T, int MODE, typename TO>
+__device__ __forceinline__ double face_patch(const GrainBox<T>& g, int p, T* su) {
+    const Lay& L = g.L;
+    const int pj = (L.ny - 2 + FJ - 1) / FJ, pk = (L.nz - 2 + FK - 1) / FK;
+    const int side = p / (pj * pk), r = p - side * (pj * pk);
+    const int i = side ? L.nx - 1 : 0, sx = side ? -1 : 1;
+    const int j0 = 1 + (r % pj) * FJ, k0 = 1 + (r / pj) * FK;
+    constexpr int RJ = FJ + 2, RK = FK + 2, NS = RJ * RK * 2;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < 3 * RJ * RK; e += MTX * MTY) {
+        const int c = e / (RJ * RK), q = e - c * (RJ * RK), kk = q / RJ, jj = q - kk * RJ;
+        const int j = j0 - 1 + jj, k = k0 - 1 + kk;
+        if (j < L.ny && k < L.nz) {
+            const T* up = g.in + c * L.cs + (long long)i + L.px * j + L.pl * k;
+            su[c * NS + q * 2] = up[0];
+            su[c * NS + q * 2 + 1] = up[sx];
+        }
+    }
+    __syncthreads();
+    const int jj = tid % FJ, kk = tid / FJ;
+    const int j = j0 + jj, k = k0 + kk;
+    if (kk >= FK || j > L.ny - 2 || k > L.nz - 2) return 0.0;
+    const long long o = (long long)i + L.px * j + L.pl * k;
+    T rb[3], db[3];
+    epilogue_load<T, MODE>(g, o, rb, db);
+    auto U = [&](int c, int dj, int dk, int col) { return su[c * NS + ((kk + 1 + dk) * RJ + (jj + 1 + dj)) * 2 + col]; };
+    T gr[3][3], s[3][3], uc[3], f[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const T u0 = U(c, 0, 0, 0);
+        uc[c] = u0;
+        gr[c][0] = side ? (u0 - U(c, 0, 0, 1)) * g.iv.ih[0] : (U(c, 0, 0, 1) - u0) * g.iv.ih[0];
+        gr[c][1] = (U(c, 1, 0, 0) - U(c, -1, 0, 0)) * g.iv.ih2[1];
+        gr[c][2] = (U(c, 0, 1, 0) - U(c, 0, -1, 0)) * g.iv.ih2[2];
+    }
+    stress<T, K_MM, 3>(GrainAcc<T, kGrainNC>{g.gid, g.tab, g.sc}.rec(o), gr, s);
+    bc_row<T, 3>(*g.bc, face_of<3>(i, j, k, L.nx - 1, L.ny - 1, L.nz - 1), uc, gr, s, f);
+    return grain_epilogue<T, MODE, TO>(g, o, uc, f, rb, db);
+}
+
 // XCD-contiguous block numbering: hardware block b of a launch of n blocks
 // runs on XCD b % 8; give each XCD a contiguous range of logical blocks, so
 // neighbouring tiles (which share halo lines) run under the same L2.
@@ -310,11 +363,13 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
 __host__ __device__ inline bool grain_offsets_fit(const Lay& L) { return 3LL * L.cs * 8 < (1LL << 32); }
 
 // Tiles and shell blocks of a box (host and device).
-__host__ __device__ inline void grain_tiles(const Lay& L, int kz, int& ntx, int& nty, int& ntz, int& nsh) {
+__host__ __device__ inline void grain_tiles(const Lay& L, int kz, int& ntx, int& nty, int& ntz, int& nsh,
+                                            bool xface = false) {
     ntx = (L.nx - 1 - grain_x0(L) + MTX - 1) / MTX;
     nty = (L.ny - 2 + MTY - 1) / MTY;
     ntz = (L.nz - 2 + kz - 1) / kz;
-    const long long shell = (long long)L.nx * L.ny * L.nz - (long long)(L.nx - 2) * (L.ny - 2) * (L.nz - 2);
+    const long long shell = (long long)L.nx * L.ny * L.nz - (long long)(L.nx - 2) * (L.ny - 2) * (L.nz - 2) -
+                            (xface ? 2LL * (L.ny - 2) * (L.nz - 2) : 0LL);
     nsh = (int)((shell + 255) / 256);
 }
 

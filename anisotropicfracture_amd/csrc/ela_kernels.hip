@@ -1561,15 +1561,17 @@ void launch_newton_rhs(const Lay& L, int kind, const double* C, const double* F0
 // lands on the XCD that xcd_block assigns it.
 namespace {
 template <typename T, int MODE, typename TO, bool SC>
-__global__ __launch_bounds__(MTX* MTY) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1))) void k_grain_sweep(GrainBox<T> g, int nshp, double* partials) {
+__global__ __launch_bounds__(MTX* MTY) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1))) void k_grain_sweep(GrainBox<T> g, int nshp, int nfp, double* partials) {
     __shared__ MarchLds<T, SC> sh;
     extern __shared__ __align__(16) unsigned char dyn_lds[];
     const int b = (int)blockIdx.x;
     double m;
     if (b < nshp) {
         m = shell_node<T, MODE, TO>(g, b * (MTX * MTY) + (int)threadIdx.x);
+    } else if (b < nshp + nfp) {
+        m = b - nshp < grain_face_patches(g.L) ? face_patch<T, MODE, TO>(g, b - nshp, sh.su) : 0.0;
     } else {
-        const int nm = (int)gridDim.x - nshp, bm = b - nshp;
+        const int nm = (int)gridDim.x - nshp - nfp, bm = b - nshp - nfp;
         const int t = g.xcd ? xcd_block(bm, nm) : bm;
         m = march_tile<T, MODE, TO, SC>(g, t, sh, (T*)dyn_lds);
     }
@@ -1577,28 +1579,33 @@ __global__ __launch_bounds__(MTX* MTY) __attribute__((amdgpu_waves_per_eu(sizeof
 }
 }  // namespace
 
+static_assert(3 * (FJ + 2) * (FK + 2) * 2 <= MRING * 3 * MPL, "a face patch fits the marching LDS ring");
+static_assert(FJ * FK <= MTX * MTY, "one face node per thread");
+
 long long grain_partials(const Lay& L, int kz) {
     int ntx, nty, ntz, nsh;
     grain_tiles(L, kz, ntx, nty, ntz, nsh);
-    return (long long)ntx * nty * ntz + (nsh + 7) / 8 * 8;
+    return (long long)ntx * nty * ntz + (nsh + 7) / 8 * 8 + (grain_face_patches(L) + 7) / 8 * 8;
 }
 
 template <typename T, int MODE, typename TO>
 void launch_grain(GrainBox<T> g, int kz, double* partials, unsigned long long* norm, hipStream_t s) {
     int ntx, nty, ntz, nsh;
-    grain_tiles(g.L, kz, ntx, nty, ntz, nsh);
+    grain_tiles(g.L, kz, ntx, nty, ntz, nsh, g.xface);
     g.ntx = ntx;
     g.nty = nty;
     g.kz = kz;
     const int nm = ntx * nty * ntz, nshp = (nsh + 7) / 8 * 8;
+    const int nfp = g.xface ? (grain_face_patches(g.L) + 7) / 8 * 8 : 0;
     const size_t lds = (size_t)g.ngr * kGrainNC * sizeof(T);
     if (g.sc)
-        hipLaunchKernelGGL((k_grain_sweep<T, MODE, TO, true>), dim3(nshp + nm), dim3(MTX * MTY), lds, s, g, nshp,
-                           partials);
+        hipLaunchKernelGGL((k_grain_sweep<T, MODE, TO, true>), dim3(nshp + nfp + nm), dim3(MTX * MTY), lds, s, g,
+                           nshp, nfp, partials);
     else
-        hipLaunchKernelGGL((k_grain_sweep<T, MODE, TO, false>), dim3(nshp + nm), dim3(MTX * MTY), lds, s, g, nshp,
-                           partials);
-    if (MODE == 2 && norm) hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partials, nm + nshp, norm);
+        hipLaunchKernelGGL((k_grain_sweep<T, MODE, TO, false>), dim3(nshp + nfp + nm), dim3(MTX * MTY), lds, s, g,
+                           nshp, nfp, partials);
+    if (MODE == 2 && norm)
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partials, nm + nshp + nfp, norm);
 }
 template void launch_grain<float, 0, float>(GrainBox<float>, int, double*, unsigned long long*, hipStream_t);
 template void launch_grain<double, 0, double>(GrainBox<double>, int, double*, unsigned long long*, hipStream_t);

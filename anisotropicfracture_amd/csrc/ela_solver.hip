@@ -919,6 +919,7 @@ GrainBox<T> grain_box(ela_hier* h, const Level& v) {
     g.ngr = h->ngrains;
     g.xcd = h->xcd;
     g.nt = (long long)v.nodes >= kNtMinNodes;
+    g.xface = true;
     return g;
 }
 
