@@ -481,7 +481,8 @@ __device__ __forceinline__ void shell_block(const BoxDesc* __restrict__ D, const
         for (int p = 0; p < 3; ++p) {
             const long long e = p * L.cs + o;
             if (MODE == 0) {
-                ((T*)d.f[op.out])[e] = u[e] + (T)op.omega * (rhs[e] - f[p]) / P<T>::dg(d)[e];
+                // P<float>::dg holds the inverse diagonal: jacobi_update multiplies in FP32 and divides in FP64
+                ((T*)d.f[op.out])[e] = jacobi_update(u[e], (T)op.omega, rhs[e] - f[p], P<T>::dg(d)[e]);
             } else if (MODE == 1) {
                 ((T*)d.f[op.out])[e] = rhs[e] - f[p];
             } else {

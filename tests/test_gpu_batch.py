@@ -282,3 +282,26 @@ def test_batch_autotune_keeps_the_bits():
         b.close()
     for a, c in zip(*sols):
         assert np.array_equal(a, c)
+
+
+def test_mixed_batch_smooths_traction_boundaries_like_fp64():
+    """The FP32 levels hold the inverse Jacobi diagonal: the marching launches' shell rows must multiply by it
+    (a division there scaled the update of every traction boundary node by D^2).  With traction faces the
+    mixed batched solve then needs the FP64 solve's cycles, to the same solution."""
+    out = {}
+    for mode in (E.SOLVE_FP64, E.SOLVE_MIXED):
+        b = E.Batch(mode)
+        p = problem(51, 16, 8)
+        r = make(b, p)
+        bct = all_disp()
+        bct[[0, 1, 3, 4]] = K.TRAC
+        bct[6:, 0] = K.TRAC
+        r.SetBC(bct)
+        b.prepare([r])
+        st = b.solve([r], tol_rel=1e-10, max_cycles=100)[0]
+        assert st["converged"]
+        out[mode] = (r.download(E.RVE.U), st["iters"])
+        b.close()
+    (u64, i64), (umx, imx) = out[E.SOLVE_FP64], out[E.SOLVE_MIXED]
+    assert imx <= i64 + 2, (imx, i64)
+    assert np.abs(umx - u64).max() <= 1e-8 * np.abs(u64).max()
