@@ -32,7 +32,7 @@ DISP, TRAC, PERIODIC, NEUMANN = 0, 1, 2, 3
 F64, F32 = 0, 1
 SOLVE_FP64, SOLVE_MIXED, ALLOC_BOTH = 0, 1, 2
 CYCLE_F, CYCLE_V = 0, 1
-OPT_MARCH, OPT_KZ, OPT_XCD, OPT_MASK, OPT_GRAPH = 0, 1, 2, 3, 4
+OPT_MARCH, OPT_KZ, OPT_XCD, OPT_MASK, OPT_GRAPH, OPT_FOLD = 0, 1, 2, 3, 4, 5
 
 ERRORS = {1: "BADARG", 2: "OOM", 3: "HIP", 4: "NONCONVERGED", 5: "NAN", 6: "STATE"}
 
@@ -364,7 +364,8 @@ class Elastic:
         OPT_KZ (planes per marching tile, default 16), OPT_XCD (tile order) or OPT_MASK (plane levels
         skip the derivative of C where it is exactly zero, constant-C nodes launched first: 0 never,
         1 where it pays, 2 always with node lists, 3 always with in-block regrouping) or OPT_GRAPH (the cycles'
-        launches on levels >= 1 replayed from captured HIP graphs, default off)."""
+        launches on levels >= 1 replayed from captured HIP graphs, default off) or OPT_FOLD (the last sweep of a
+        Krylov step writes the FP64 vector itself, default on; the same bits either way)."""
         _check(lib().ela_set_option(self.ptr, option, int(value)))
 
     def autotune(self, nrep=10):

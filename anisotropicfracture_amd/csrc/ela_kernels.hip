@@ -19,10 +19,6 @@
 
 namespace ela {
 
-extern bool g_slab_order;
-extern int g_slab_walk;
-extern int g_slab_f64;
-
 namespace {
 
 constexpr int BX = 64, BY = 4;
@@ -37,15 +33,14 @@ __device__ __forceinline__ bool node_index(const Lay& L, int& i, int& j, int& k)
 inline dim3 node_grid(const Lay& L) { return dim3((L.nx + BX - 1) / BX, (L.ny + BY - 1) / BY, L.nz); }
 
 // XCD-slab numbering of the stencil kernels on 3D levels.  Hardware block b
-// runs on XCD b % 8; XCD x owns the rows [x*rows, (x+1)*rows).  Walk 2 (the
-// default) cuts each plane of the slab, nx * rows nodes in row order, into
-// blocks of 256 consecutive nodes and walks them plane by plane: no thread
-// idles on a 64-wide x tile past the row end (levels have 2^l + 1 nodes a
-// row, so 64-wide tiles leave a tile column holding one node).  Walks 0 and 1
-// use 64 x 4 tiles (y-tile fastest, then z, then x-tile; or x-tile fastest,
-// then y-tile, then z).
+// runs on XCD b % 8; XCD x owns the rows [x*rows, (x+1)*rows).  Each plane of
+// the slab, nx * rows nodes in row order, is cut into blocks of 256
+// consecutive nodes, walked plane by plane: no thread idles on a 64-wide x
+// tile past the row end (levels have 2^l + 1 nodes a row, so 64-wide tiles
+// would leave a tile column holding one node).  rows == 0 marks the plain
+// block grid (node_index).
 struct Slab {
-    int rows, nty, ntx, order;
+    int rows, ntx;
     // node-list order instead (NodeList): thread q of the launch takes node list[q]
     const unsigned long long* list = nullptr;
     long long nlist = 0;
@@ -54,48 +49,23 @@ struct Slab {
 };
 inline Slab slab_of(const Lay& L) {
     Slab S;
-    S.order = g_slab_walk;
     S.rows = (L.ny + 7) / 8;
-    if (S.order == 2) {
-        S.nty = 1;
-        S.ntx = (int)(((long long)L.nx * S.rows + BX * BY - 1) / (BX * BY));  // blocks per slab plane
-    } else {
-        S.nty = (S.rows + BY - 1) / BY;
-        S.ntx = (L.nx + BX - 1) / BX;
-    }
+    S.ntx = (int)(((long long)L.nx * S.rows + BX * BY - 1) / (BX * BY));  // blocks per slab plane
     return S;
 }
 inline dim3 slab_grid(const Lay& L) {
     const Slab S = slab_of(L);
-    return dim3((unsigned)(8LL * S.nty * L.nz * S.ntx));
+    return dim3((unsigned)(8LL * L.nz * S.ntx));
 }
 __device__ __forceinline__ bool slab_index(const Lay& L, const Slab S, int& i, int& j, int& k) {
     const int b = blockIdx.x, x = b & 7, r = b >> 3;
-    if (S.order == 2) {  // 256 consecutive nodes of a slab plane, planes in turn
-        const int bl = r % S.ntx;
-        k = r / S.ntx;
-        const unsigned t = (unsigned)bl * (BX * BY) + threadIdx.x + BX * threadIdx.y;
-        const unsigned jl = t / (unsigned)L.nx;
-        i = (int)(t - jl * (unsigned)L.nx);
-        j = x * S.rows + (int)jl;
-        return (int)jl < S.rows && j < L.ny && k < L.nz;
-    }
-    int tyl, tx;
-    if (S.order == 0) {  // y-tile fastest, then z, then x-tile
-        tyl = r % S.nty;
-        const int rest = r / S.nty;
-        k = rest % L.nz;
-        tx = rest / L.nz;
-    } else {  // x-tile fastest, then y-tile, then z
-        tx = r % S.ntx;
-        const int rest = r / S.ntx;
-        tyl = rest % S.nty;
-        k = rest / S.nty;
-    }
-    i = tx * BX + threadIdx.x;
-    j = x * S.rows + tyl * BY + threadIdx.y;
-    const int jend = min(L.ny, (x + 1) * S.rows);
-    return i < L.nx && j < jend && k < L.nz;
+    const int bl = r % S.ntx;  // 256 consecutive nodes of a slab plane, planes in turn
+    k = r / S.ntx;
+    const unsigned t = (unsigned)bl * (BX * BY) + threadIdx.x + BX * threadIdx.y;
+    const unsigned jl = t / (unsigned)L.nx;
+    i = (int)(t - jl * (unsigned)L.nx);
+    j = x * S.rows + (int)jl;
+    return (int)jl < S.rows && j < L.ny && k < L.nz;
 }
 
 // Node of thread q in a node-list launch (1D grid of BX x BY blocks).
@@ -973,14 +943,14 @@ Inv<T> make_inv(const double h[3]) {
 template <typename T>
 void plane_launch(const Lay& L, const PlaneOpt& po, Slab& S, dim3& grid) {
     if (po.list) {
-        S = Slab{0, 0, 0, 0};
+        S = Slab{0, 0};
         S.list = po.list;
         S.nlist = po.nlist;
         grid = dim3((unsigned)((po.nlist + BX * BY - 1) / (BX * BY)));
         return;
     }
-    const bool sl = L.dim == 3 && g_slab_order && (sizeof(T) == 4 || g_slab_f64);
-    S = sl ? slab_of(L) : Slab{0, 0, 0, 0};
+    const bool sl = L.dim == 3;
+    S = sl ? slab_of(L) : Slab{0, 0};
     grid = sl ? slab_grid(L) : node_grid(L);
     if (po.group && po.sm && L.dim == 3) S.gsm = po.sm;
 }
@@ -1033,9 +1003,6 @@ long long residual_partials(const Lay& L) {
     return std::max((long long)g.x * g.y * g.z, (long long)h.x);
 }
 
-bool g_slab_order = true;
-int g_slab_walk = 2;
-int g_slab_f64 = 1;
 
 template <typename T>
 void launch_jacobi(const Lay& L, int kind, const T* C, const T* dg, const T* xin, const T* b, T* xout,
@@ -1378,9 +1345,8 @@ template <typename T>
 void launch_fapply_tab(const Lay& L, const TabDesc<T>& t, const T* u, T* f, const BCTab& bc, const double h[3],
                        int uniform, hipStream_t s) {
     const Inv<T> iv = make_inv<T>(h);
-    const bool sl = g_slab_order;
-    const Slab S = sl ? slab_of(L) : Slab{0, 0, 0, 0};
-    const dim3 grid = sl ? slab_grid(L) : node_grid(L);
+    const Slab S = slab_of(L);
+    const dim3 grid = slab_grid(L);
     using A = TabAcc<T, 21>;
     hipLaunchKernelGGL((k_fapply<T, K_MM, 3, A>), grid, dim3(BX, BY), 0, s, L, (A{t.gid, t.tab, t.dgt}), u, f, bc, iv,
                        uniform, S);
@@ -1390,9 +1356,8 @@ template <typename T>
 void launch_jacobi_tab(const Lay& L, const TabDesc<T>& t, const T* dg, const T* xin, const T* b, T* xout,
                        const BCTab& bc, const double h[3], int uniform, double omega, hipStream_t s) {
     const Inv<T> iv = make_inv<T>(h);
-    const bool sl = g_slab_order;
-    const Slab S = sl ? slab_of(L) : Slab{0, 0, 0, 0};
-    const dim3 grid = sl ? slab_grid(L) : node_grid(L);
+    const Slab S = slab_of(L);
+    const dim3 grid = slab_grid(L);
     using A = TabAcc<T, 21>;
     hipLaunchKernelGGL((k_jacobi<T, K_MM, 3, A>), grid, dim3(BX, BY), 0, s, L, (A{t.gid, t.tab, t.dgt}), dg, xin, b,
                        xout, bc, iv, uniform, (T)omega, S);
@@ -1403,9 +1368,8 @@ void launch_residual_tab(const Lay& L, const TabDesc<T>& t, const T* x, const T*
                          const double* scale, double* partials, unsigned long long* norm, const BCTab& bc,
                          const double h[3], int uniform, hipStream_t s) {
     const Inv<T> iv = make_inv<T>(h);
-    const bool sl = g_slab_order;
-    const Slab S = sl ? slab_of(L) : Slab{0, 0, 0, 0};
-    const dim3 grid = sl ? slab_grid(L) : node_grid(L);
+    const Slab S = slab_of(L);
+    const dim3 grid = slab_grid(L);
     using A = TabAcc<T, 21>;
     hipLaunchKernelGGL((k_residual<T, TO, K_MM, 3, A>), grid, dim3(BX, BY), 0, s, L, (A{t.gid, t.tab, t.dgt}), x, b,
                        r, rs, scale, norm ? partials : nullptr, bc, iv, uniform, S);

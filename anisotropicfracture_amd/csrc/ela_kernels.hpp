@@ -7,11 +7,6 @@
 
 namespace ela {
 
-// Stencil kernels of 3D levels use the XCD-slab block numbering when set (default).
-extern bool g_slab_order;
-extern int g_slab_walk;  // 0: y-tile, z, x-tile; 1: x-tile, y-tile, z (fastest first)
-extern int g_slab_f64;   // slab numbering for FP64 kernels too
-
 template <typename T>
 Inv<T> make_inv(const double h[3]);
 

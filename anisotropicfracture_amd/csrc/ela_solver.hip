@@ -31,7 +31,6 @@
 #include "ela_internal.hpp"
 #include "ela_kernels.hpp"
 #include "ela_hooks.hpp"
-#include "ela_march.hpp"
 
 using namespace ela;
 
@@ -73,7 +72,6 @@ constexpr int kMaxEvents = 8192;
 
 struct Level {
     Lay L{};
-    MarchPlan mp{};
     int n[3] = {0, 0, 0};
     double h[3] = {1, 1, 1};
     int nodes = 0;
@@ -140,6 +138,7 @@ struct ela_hier {
     // graphs, one per launch configuration (the key holds everything those
     // launches read on the host)
     bool graph_on = false;
+    bool fold = true;  // ELA_OPT_FOLD: the F-cycle's last level-0 sweep writes the Krylov vector itself
     long long epoch = 0;  // bumped by every ela_prepare
     struct CycleGraph {
         std::vector<unsigned long long> key;
@@ -171,9 +170,7 @@ struct ela_hier {
     double *kr = nullptr, *kdev = nullptr, *kpart = nullptr;
     // cycle options
     int cycle = ELA_CYCLE_F, nu1 = 2, nu2 = 2;
-    bool use_march = true;
     int pitch_align = 32;  // elements; rows start on 128 B (FP32) / 256 B (FP64) boundaries
-    int march_kz = 32, march_ty = 4;
     double omega = 2.0 / 3.0;
     // kernel timing of the last solve
     bool timing = false;
@@ -373,6 +370,9 @@ std::vector<unsigned long long> cycle_key(ela_hier* h, int kind) {
     k.push_back(((unsigned long long)h->uniform << 8) | (unsigned long long)h->use_table);
     put(h->stream);
     put(h->M);
+    put(h->partials);
+    put(h->dscale);
+    put(h->dnorm);
     put(h->dbc);
     put(h->gid);
     put(h->tab64);
@@ -442,6 +442,10 @@ bool graph_run(ela_hier* h, int kind) {
     if ((int)h->graphs.size() >= kMaxCycleGraphs) {
         auto lru = std::min_element(h->graphs.begin(), h->graphs.end(),
                                     [](const auto& x, const auto& y) { return x.used < y.used; });
+        // A launch of the evicted graph may still be queued on the stream:
+        // its executable (kernel nodes and their argument buffers) must
+        // outlive it, so drain the stream before destroying it.
+        (void)hipStreamSynchronize(h->stream);
         (void)hipGraphExecDestroy(lru->exec);
         h->graphs.erase(lru);
     }
@@ -457,6 +461,7 @@ bool graph_run(ela_hier* h, int kind) {
 }
 
 void drop_graphs(ela_hier* h) {
+    if (!h->graphs.empty() && h->stream) (void)hipStreamSynchronize(h->stream);  // no launch of them in flight
     for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
     h->graphs.clear();
 }
@@ -475,7 +480,10 @@ void vcycle0(ela_hier* h, double* out64 = nullptr) {
 
 // One MG cycle on level 0's res into cor; with out64, an F-cycle leaves its
 // correction cor + hold in out64 (FP64, the preconditioned vector of a
-// Krylov step) instead of in cor, and one_cycle returns true.
+// Krylov step) instead of in cor, and one_cycle returns true.  When the
+// last sweep is folded (grain-form level 0, ELA_OPT_FOLD on) that sweep
+// writes out64 directly and never stores its own value: level 0's cor is
+// then undefined after the call, and only out64 holds the correction.
 template <typename T>
 bool one_cycle(ela_hier* h, double* out64 = nullptr) {
     if (h->cycle != ELA_CYCLE_F || h->lev.size() < 2) {
@@ -491,7 +499,7 @@ bool one_cycle(ela_hier* h, double* out64 = nullptr) {
     std::swap(*b.res, *b.rc);
     std::swap(*b.cor, *b.hold);
     // a grain-form level 0 folds the final cor + hold into its last sweep
-    const bool fold = out64 && h->nu2 > 0 && grain_on(h, v);
+    const bool fold = out64 && h->fold && h->nu2 > 0 && grain_on(h, v);
     vcycle0<T>(h, fold ? out64 : nullptr);
     if (fold) return true;
     if (out64) launch_add_to64<T>(v.L, h->dim, *b.hold, *b.cor, out64, h->stream);
@@ -932,7 +940,6 @@ void A_apply(ela_hier* h, Level& v, const T* C, const T* u, T* f, hipStream_t s)
         launch_grain<T, 3, T>(g, level_kz(h, v), nullptr, nullptr, s);
     } else if (table_on(h, v)) launch_fapply_tab<T>(v.L, tab_of<T>(h), u, f, h->bc, v.h, h->uniform, s);
     else if (coarse_tab_on(h, v)) launch_fapply_tab<T>(v.L, coarse_tab<T>(h, v), u, f, h->bc, v.h, h->uniform, s);
-    else if (v.mp.ok) march_fapply<T>(v.mp, v.L, h->kind, C, u, f, h->bc, v.h, h->uniform, s);
     else launch_fapply<T>(v.L, h->kind, C, u, f, h->bc, v.h, h->uniform, s, plane_opt<T>(h, v));
 }
 
@@ -948,7 +955,6 @@ void A_jacobi(ela_hier* h, Level& v, const T* C, const T* dg, const T* xin, cons
         launch_grain<T, 0, T>(g, level_kz(h, v), nullptr, nullptr, s);
     } else if (table_on(h, v)) launch_jacobi_tab<T>(v.L, tab_of<T>(h), dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
     else if (coarse_tab_on(h, v)) launch_jacobi_tab<T>(v.L, coarse_tab<T>(h, v), dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
-    else if (v.mp.ok) march_jacobi<T>(v.mp, v.L, h->kind, C, dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s);
     else launch_jacobi<T>(v.L, h->kind, C, dg, xin, b, xout, h->bc, v.h, h->uniform, h->omega, s, plane_opt<T>(h, v));
 }
 
@@ -984,7 +990,6 @@ void A_residual(ela_hier* h, Level& v, const T* C, const T* x, const T* b, T* r,
         }
     } else if (table_on(h, v)) launch_residual_tab<T, TO>(v.L, tab_of<T>(h), x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
     else if (coarse_tab_on(h, v)) launch_residual_tab<T, TO>(v.L, coarse_tab<T>(h, v), x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
-    else if (v.mp.ok) march_residual<T, TO>(v.mp, v.L, h->kind, C, x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
     else launch_residual<T, TO>(v.L, h->kind, C, x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s,
                                 plane_opt<T>(h, v));
 }
@@ -1098,16 +1103,6 @@ int ela_hier_create_ex(ela_ctx* ctx, const ela_domain* dom, int model_kind, int 
     h->mode = mode;
     h->has64 = (mode == ELA_SOLVE_FP64 || mode == ELA_ALLOC_BOTH);
     h->has32 = (mode == ELA_SOLVE_MIXED || mode == ELA_ALLOC_BOTH);
-    h->use_march = false;
-    if (const char* e = std::getenv("ELA_KERNEL")) {
-        h->use_march = std::strcmp(e, "march") == 0;
-        g_slab_order = std::strcmp(e, "simple") != 0;
-    }
-    if (const char* e = std::getenv("ELA_MARCH_KZ")) h->march_kz = std::max(1, std::atoi(e));
-    if (const char* e = std::getenv("ELA_WALK")) g_slab_walk = std::atoi(e);
-    if (const char* e = std::getenv("ELA_SLAB_F64")) g_slab_f64 = std::atoi(e);
-    if (const char* e = std::getenv("ELA_PITCH")) h->pitch_align = std::max(1, std::atoi(e));
-    if (const char* e = std::getenv("ELA_MARCH_TY")) h->march_ty = std::atoi(e) == 2 ? 2 : 4;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
         return fail(ELA_ERR_HIP, "hipStreamCreate failed");
@@ -1125,8 +1120,6 @@ int ela_hier_create_ex(ela_ctx* ctx, const ela_domain* dom, int model_kind, int 
         v.L.pl = v.L.px * v.L.ny;
         v.L.cs = (v.L.pl * v.L.nz + 255) / 256 * 256;
         v.nodes = v.L.nx * v.L.ny * v.L.nz;
-        v.mp = march_plan(v.L, h->march_kz, h->march_ty);
-        if (!h->use_march) v.mp.ok = false;
         h->lev.push_back(v);
         bool can = true;
         for (int d = 0; d < dom->dim; ++d) can = can && (n[d] % 2 == 0) && (n[d] / 2 >= coarse_min);
@@ -1167,7 +1160,7 @@ int ela_hier_create_ex(ela_ctx* ctx, const ela_domain* dom, int model_kind, int 
     if (!rc && h->nunk <= kMaxDenseUnknowns) rc = halloc_t(h, (size_t)h->nunk * 2 * h->nunk, &h->M);
     if (!rc) rc = halloc_t(h, 4, &h->dnorm);
     long long np = 0;
-    for (auto& v : h->lev) np = std::max(np, std::max(residual_partials(v.L), march_partials(v.mp, v.L)));
+    for (auto& v : h->lev) np = std::max(np, residual_partials(v.L));
     if (h->dim == 3) np = std::max(np, grain_partials(h->lev[0].L, 1));  // every z-marching tile height
     if (!rc) rc = halloc_t(h, 1, &h->dbc);
     if (!rc) rc = halloc_t(h, (size_t)np, &h->partials);
@@ -1792,6 +1785,8 @@ int ela_set_option(ela_hier* h, int option, int value) {
         h->mask_mode = value;
     } else if (option == ELA_OPT_GRAPH) {
         h->graph_on = value != 0;
+    } else if (option == ELA_OPT_FOLD) {
+        h->fold = value != 0;
     } else {
         return fail(ELA_ERR_BADARG, "unknown option");
     }

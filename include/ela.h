@@ -122,12 +122,17 @@ int ela_set_model_field(ela_hier* h, const double* C_host);
 int ela_set_model_voronoi_cubic(ela_hier* h, int nseeds, const int64_t* seeds, int q, const double* R,
                                 double C11, double C12, double C44);
 
-/* Stiffness representation of level 0 for every kernel that follows:
+/* Stiffness representation for every kernel that follows:
  * 0 = per-node planes (the reference's per-node model field, default);
  * 1 = grain table: a per-node 16-bit grain id and one record per grain
- *     (available after ela_set_model_voronoi_cubic).  The same values, so
- *     the same operator bit for bit; coarse MG levels always use planes
- *     (their stiffness is the full-weighting average). */
+ *     (available after ela_set_model_voronoi_cubic).
+ * Level 0 holds the same values in both, so its operator is the same bit for
+ * bit.  The coarse MG levels differ: plane form coarsens the stiffness by
+ * full weighting (averageDownCoeffsSameAmrLevel, Elastic.cpp:722-809), grain
+ * form by injection (a coarse node keeps the grain id and record of the fine
+ * node it coincides with).  The preconditioner, and so the iteration count,
+ * can therefore differ between the two; the converged solution is that of
+ * the same level-0 operator. */
 int ela_set_stiffness_mode(ela_hier* h, int mode);
 
 /* Phase-field-degraded isotropic stiffness (Degradable::Isotropic::DegradeModulus,
@@ -265,8 +270,16 @@ int ela_set_krylov(ela_hier* h, int restart);
  *                launches stay on the stream.  Same kernels in the same
  *                order, so the same results.  0 (default): stream launches
  *                (on MI355X the replay measured no faster: the small levels'
- *                cost is each kernel's own latency).  Hierarchies only. */
-enum { ELA_OPT_MARCH = 0, ELA_OPT_KZ = 1, ELA_OPT_XCD = 2, ELA_OPT_MASK = 3, ELA_OPT_GRAPH = 4 };
+ *                cost is each kernel's own latency).  Hierarchies only.
+ *                At most 8 graphs are kept per hierarchy; an evicted one is
+ *                destroyed only after the stream has drained.
+ * ELA_OPT_FOLD   1 (default): in a Krylov step on a grain-form level 0 the
+ *                F-cycle's last smoothing sweep writes the FP64 vector
+ *                (its value + the held correction, rounded as the separate
+ *                addition rounds) instead of storing its own value; 0: the
+ *                sweep stores its value and a separate kernel adds.  Same
+ *                bits either way.  Hierarchies only. */
+enum { ELA_OPT_MARCH = 0, ELA_OPT_KZ = 1, ELA_OPT_XCD = 2, ELA_OPT_MASK = 3, ELA_OPT_GRAPH = 4, ELA_OPT_FOLD = 5 };
 int ela_set_option(ela_hier* h, int option, int value);
 
 /* Autotuner of the marching kernels (no reference counterpart): times nrep

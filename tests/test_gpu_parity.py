@@ -460,3 +460,58 @@ def test_graph_replayed_cycles_keep_the_bits(case):
     for (ua, ia), (ub, ib) in zip(out[0], out[1]):
         assert ia == ib
         assert np.array_equal(ua, ub)
+
+
+def test_graph_eviction_keeps_the_bits():
+    """More graphs than the hierarchy keeps (8): every tile height in turn gives each solve's launches new
+    keys (4 a precision: two kinds times two buffer states), so captured graphs are evicted and destroyed
+    while later ones replay, and a height seen before is captured again after its eviction.  An evicted
+    graph is destroyed only after the stream has drained.  Every solve must keep the stream launches' bits."""
+    cfg = K.polycrystal(48, grains=7, seed=12)
+    out = []
+    for graph in (0, 1):
+        op = ela.Elastic(cfg["n"], MM, mode=ela.ALLOC_BOTH)
+        op.SetModelVoronoiCubic(cfg["seeds"], K.VORONOI_Q, cfg["R"], cfg["C11"], cfg["C12"], cfg["C44"])
+        op.SetBC(cfg["bc_type"], cfg["bc_value"])
+        op.SetStiffnessMode(1)
+        op.SetKrylov(6)
+        op.set_option(ela.OPT_GRAPH, graph)
+        op.prepareForSolve()
+        b = op.bc_init()
+        res = []
+        for kz in (16, 8, 4, 32, 16, 8):
+            op.set_option(ela.OPT_KZ, kz)
+            for mode in (ela.SOLVE_MIXED, ela.SOLVE_FP64):
+                u, st = ela.Linear(op).solve(np.zeros_like(b), b, tol_rel=1e-10, tol_abs=0.0, mode=mode,
+                                             max_iter=300)
+                assert st["converged"]
+                res.append((u, st["iters"]))
+        out.append(res)
+        op.close()
+    for (ua, ia), (ub, ib) in zip(out[0], out[1]):
+        assert ia == ib
+        assert np.array_equal(ua, ub)
+
+
+@pytest.mark.parametrize("mode", [ela.SOLVE_FP64, ela.SOLVE_MIXED])
+def test_folded_last_sweep_gives_the_separate_addition_bits(mode):
+    """OPT_FOLD 1 (default): the F-cycle's last level-0 sweep of a Krylov step writes the FP64 vector
+    (its value + the held correction) itself; OPT_FOLD 0 stores the sweep's value and adds in a separate
+    kernel.  The FGMRES iterates, hence the iteration count and solution, must be the same bits."""
+    cfg = K.polycrystal(40, grains=6, seed=21)
+    out = []
+    for fold in (0, 1):
+        op = ela.Elastic(cfg["n"], MM, mode=ela.ALLOC_BOTH)
+        op.SetModelVoronoiCubic(cfg["seeds"], K.VORONOI_Q, cfg["R"], cfg["C11"], cfg["C12"], cfg["C44"])
+        op.SetBC(cfg["bc_type"], cfg["bc_value"])
+        op.SetStiffnessMode(1)
+        op.SetKrylov(6)
+        op.set_option(ela.OPT_FOLD, fold)
+        op.prepareForSolve()
+        b = op.bc_init()
+        u, st = ela.Linear(op).solve(np.zeros_like(b), b, tol_rel=1e-10, tol_abs=0.0, mode=mode, max_iter=300)
+        assert st["converged"]
+        out.append((u, st["iters"]))
+        op.close()
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][0], out[1][0])
