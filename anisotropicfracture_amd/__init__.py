@@ -104,7 +104,13 @@ SIGNATURES = {
     "ela_hier_bytes": ([_P], _SZ),
     "ela_set_model_field": ([_P, _P], _I),
     "ela_set_model_voronoi_cubic": ([_P, _I, _P, _I, _P, _D, _D, _D], _I),
-    "ela_set_model_degraded_iso": ([_P, _D, _D, _P, _D, _D, _D], _I),
+    "ela_set_model_degraded_iso": ([_P, _D, _D, _I, _P], _I),
+    "ela_degrade_material": ([_P, _D, _D, _I, _P, _P], _I),
+    "ela_combine_cubic": ([_P, _I, _P, _P, _P, _P, _P], _I),
+    "ela_newton_w_dw": ([_P, _P, _P, _D, _P, _P, _P], _I),
+    "ela_dev_alloc": ([_P, _SZ, ctypes.POINTER(_P)], _I),
+    "ela_dev_free": ([_P, _P], _I),
+    "ela_memcpy": ([_P, _P, _SZ], _I),
     "ela_set_bc": ([_P, _P, _P], _I),
     "ela_set_uniform": ([_P, _I], _I),
     "ela_prepare": ([_P], _I),
@@ -163,6 +169,10 @@ SIGNATURES = {
     "ela_newton_rhs": ([_P, _P, _P, _P, _P, _P], _I),
     "ela_reflux": ([ctypes.POINTER(_Layout), _P, ctypes.POINTER(_Layout), _P, _P, _I, _I, _P], _I),
     "ela_average_down": ([ctypes.POINTER(_Layout), _P, ctypes.POINTER(_Layout), _P, _P, _I, _I, _P], _I),
+    "ela_reflux_window": ([ctypes.POINTER(_Layout), ctypes.POINTER(_Layout), _P, _P, ctypes.POINTER(_Layout), _P, _P, _I,
+                           _I, _P], _I),
+    "ela_average_down_window": ([ctypes.POINTER(_Layout), ctypes.POINTER(_Layout), _P, _P, ctypes.POINTER(_Layout), _P,
+                                 _P, _I, _I, _P], _I),
     "ela_serve": ([_P, _I, ctypes.POINTER(Request), _D, _D, _I, _I, _D, ctypes.POINTER(Result),
                    ctypes.POINTER(ServeSummary)], _I),
     "ela_batch_smoother_times": ([_P, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_I)], _I),
@@ -248,6 +258,32 @@ def default_context():
     if _default_ctx is None:
         _default_ctx = Context(0)
     return _default_ctx
+
+
+class DeviceBuffer:
+    """Raw device memory counted against a context's budget (ela_dev_alloc)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx, self.nbytes = ctx, int(nbytes)
+        p = ctypes.c_void_p()
+        _check(lib().ela_dev_alloc(ctx.ptr, self.nbytes, ctypes.byref(p)))
+        self.ptr = p
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host)
+        assert host.nbytes <= self.nbytes
+        _check(lib().ela_memcpy(self.ptr, _ptr(host), host.nbytes))
+        return self
+
+    def download(self, shape, dtype=np.float64):
+        out = np.zeros(shape, dtype=dtype)
+        _check(lib().ela_memcpy(_ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().ela_dev_free(self.ctx.ptr, self.ptr)
+            self.ptr = None
 
 
 class DeviceField:
@@ -339,9 +375,50 @@ class Elastic:
         R = np.ascontiguousarray(R, dtype=np.float64)
         _check(lib().ela_set_model_voronoi_cubic(self.ptr, seeds_q.shape[0], _ptr(seeds_q), q, _ptr(R), C11, C12, C44))
 
-    def SetModelDegradedIsotropic(self, lambda0, mu0, eta, a, b, c):
+    def SetModelDegradedIsotropic(self, lambda0, mu0, eta):
+        """Degradable::Isotropic::DegradeModulus (Degradable/Isotropic.H:74-104) per node: eta of shape
+        (*nodes) or (1, *nodes) degrades lambda and mu by (1 - eta); (2, *nodes) degrades lambda by
+        (1 - eta[0]) and mu by (1 - eta[1]).  Raises ElaError (NAN / BADARG) where the reference aborts."""
         eta = np.ascontiguousarray(eta, dtype=np.float64)
-        _check(lib().ela_set_model_degraded_iso(self.ptr, lambda0, mu0, _ptr(eta), a, b, c))
+        neta = 1 if eta.shape == self.shape(0, 1)[1:] else eta.shape[0]
+        if eta.size != neta * int(np.prod(self.shape(0, 1))):
+            raise ValueError("eta has shape %s" % (eta.shape,))
+        _check(lib().ela_set_model_degraded_iso(self.ptr, lambda0, mu0, neta, _ptr(eta)))
+
+    def _cells_on_device(self, eta_cells):
+        cells = np.ascontiguousarray(eta_cells, dtype=np.float64)
+        want = tuple(v + 2 for v in reversed(self.n))
+        if cells.shape[1:] != want:
+            raise ValueError("cell field has shape %s, expected (ncomp,) + %s" % (cells.shape, want))
+        buf = DeviceBuffer(self.ctx, cells.nbytes)
+        buf.upload(cells)
+        return buf, cells.shape[0]
+
+    def DegradeMaterial(self, lambda0, mu0, eta_cells):
+        """PolymerDegradation::DegradeMaterial (PolymerDegradation.cpp:793-833) on the device: cell damage of
+        shape (number_of_eta, *(n + 2)) with one ghost layer, averaged onto the nodes."""
+        buf, neta = self._cells_on_device(eta_cells)
+        try:
+            _check(lib().ela_degrade_material(self.ptr, lambda0, mu0, neta, buf.ptr, None))
+        finally:
+            buf.free()
+
+    def CombineCubic(self, ddw, eta_cells, F0=None, want_F0=False):
+        """PhaseFieldMicrostructure's fill with Affine::Cubic::Combine (PFM.cpp:529-543, Affine/Cubic.H:70-94):
+        ddw (nph, 21) model records, F0 (nph, 3, 3) or None, order parameters (nph, *(n + 2)) in cells.
+        Returns the per-node F0 planes (dim*dim, *nodes) when want_F0."""
+        ddw = np.ascontiguousarray(ddw, dtype=np.float64)
+        F0a = None if F0 is None else np.ascontiguousarray(F0, dtype=np.float64)
+        buf, nph = self._cells_on_device(eta_cells)
+        fo = self.field(0, F64, self.dim * self.dim) if want_F0 else None
+        try:
+            _check(lib().ela_combine_cubic(self.ptr, nph, _ptr(ddw), _ptr(F0a) if F0a is not None else None, buf.ptr,
+                                           fo.ptr if fo else None, None))
+            return fo.download() if fo else None
+        finally:
+            buf.free()
+            if fo:
+                fo.free()
 
     def SetBC(self, types, values=None):
         t = np.ascontiguousarray(types, dtype=np.int32)
@@ -427,6 +504,20 @@ class Elastic:
         fu.free()
         fo.free()
         return out
+
+    def W_DW(self, u, F0=None, wfac=0.5):
+        """Newton<T>::W and DW (Newton.H:233-333): energy density (*nodes) and dw = DDW : (grad u - F0)
+        (dim*dim, *nodes).  wfac 0.5 for the Linear models and Affine::Cubic, 1 for Affine::Isotropic."""
+        fu = self.field(0, F64).upload(u)
+        fw = self.field(0, F64, 1)
+        fd = self.field(0, F64, self.dim * self.dim)
+        ff = self.field(0, F64, self.dim * self.dim).upload(F0) if F0 is not None else None
+        _check(lib().ela_newton_w_dw(self.ptr, fu.ptr, ff.ptr if ff else None, wfac, fw.ptr, fd.ptr, None))
+        w, dw = fw.download()[0], fd.download()
+        for f in (fu, fw, fd, ff):
+            if f:
+                f.free()
+        return w, dw
 
     def Strain(self, u, voigt=False):
         """Elastic<T>::Strain (Elastic.cpp:399-447): sym(grad u) per node."""

@@ -22,9 +22,15 @@ constexpr int RX = 64, RY = 4;
 // Coarse node (I,J,K) = (ox/2 + a, oy/2 + b, oz/2 + c) of the coarsened fine
 // box takes the full weighting of the fine residual; fine values outside the
 // box are its ghost nodes, zero.
+// The coarse field may be a window of the coarse level: its element of
+// coarse node (I,J,K) is at (I - cw.x) + px (J - cw.y) + pl (K - cw.z), while
+// Lc's node counts (the coarse domain's) decide the boundary branches.
+struct Win {
+    int x, y, z;
+};
 template <typename T, int DIM>
 __global__ __launch_bounds__(RX* RY) void k_reflux(Lay Lc, Lay Lf, int ox, int oy, int oz, int nc,
-                                                   const T* __restrict__ fine, T* __restrict__ crse) {
+                                                   const T* __restrict__ fine, T* __restrict__ crse, Win cw) {
     const int cnx = (Lf.nx - 1) / 2 + 1, cny = (Lf.ny - 1) / 2 + 1, cnz = DIM == 3 ? (Lf.nz - 1) / 2 + 1 : 1;
     const int a = blockIdx.x * RX + threadIdx.x, b = blockIdx.y * RY + threadIdx.y, c = blockIdx.z;
     if (a >= cnx || b >= cny || c >= cnz) return;
@@ -38,7 +44,7 @@ __global__ __launch_bounds__(RX* RY) void k_reflux(Lay Lc, Lay Lf, int ox, int o
             if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) return T(0);
             return F[x + px * y + pl * z];
         };
-        crse[n * Lc.cs + I + Lc.px * J + Lc.pl * K] = restrict_node<T, DIM>(Lc, I, J, K, get);
+        crse[n * Lc.cs + (I - cw.x) + Lc.px * (J - cw.y) + Lc.pl * (K - cw.z)] = restrict_node<T, DIM>(Lc, I, J, K, get);
     }
 }
 
@@ -46,13 +52,14 @@ __global__ __launch_bounds__(RX* RY) void k_reflux(Lay Lc, Lay Lf, int ox, int o
 // take the coincident fine value (amrex::average_down on nodal data).
 template <typename T, int DIM>
 __global__ __launch_bounds__(RX* RY) void k_average_down(Lay Lc, Lay Lf, int ox, int oy, int oz, int nc,
-                                                         const T* __restrict__ fine, T* __restrict__ crse) {
+                                                         const T* __restrict__ fine, T* __restrict__ crse, Win cw) {
     const int cnx = (Lf.nx - 1) / 2 + 1, cny = (Lf.ny - 1) / 2 + 1, cnz = DIM == 3 ? (Lf.nz - 1) / 2 + 1 : 1;
     const int a = blockIdx.x * RX + threadIdx.x, b = blockIdx.y * RY + threadIdx.y, c = blockIdx.z;
     if (a >= cnx || b >= cny || c >= cnz) return;
     const int I = ox / 2 + a, J = oy / 2 + b, K = oz / 2 + c;
     for (int n = 0; n < nc; ++n)
-        crse[n * Lc.cs + I + Lc.px * J + Lc.pl * K] = fine[n * Lf.cs + 2 * a + Lf.px * (2 * b) + Lf.pl * (2 * c)];
+        crse[n * Lc.cs + (I - cw.x) + Lc.px * (J - cw.y) + Lc.pl * (K - cw.z)] =
+            fine[n * Lf.cs + 2 * a + Lf.px * (2 * b) + Lf.pl * (2 * c)];
 }
 
 Lay lay_of(const ela_level_layout& l) {
@@ -82,12 +89,39 @@ int check_pair(const ela_level_layout* c, const ela_level_layout* f, const int* 
     return ELA_OK;
 }
 
+// A window of the coarse level (win: its layout, wo: the coarse node index of
+// its first node) must hold every coarse node under the fine box.
+int check_window(const ela_level_layout* c, const ela_level_layout* win, const int* wo, const ela_level_layout* f,
+                 const int* o) {
+    if (!win || !wo) return set_error(ELA_ERR_BADARG, "null window");
+    if (win->dim != c->dim) return set_error(ELA_ERR_BADARG, "dimension mismatch");
+    for (int d = 0; d < c->dim; ++d) {
+        const int lo = o[d] / 2, hi = (o[d] + f->nodes[d] - 1) / 2;
+        if (wo[d] < 0 || lo < wo[d] || hi > wo[d] + win->nodes[d] - 1)
+            return set_error(ELA_ERR_BADARG, "the coarse window does not cover the coarsened fine box");
+    }
+    if (win->pitch < win->nodes[0] || (win->dim == 3 && win->plane < win->pitch * win->nodes[1]))
+        return set_error(ELA_ERR_BADARG, "bad window pitch");
+    return ELA_OK;
+}
+
 template <template <typename, int> class Kern>
 int launch_pair(const ela_level_layout* crse, void* crse_f, const ela_level_layout* fine, const void* fine_f,
-                const int* o, int ncomp, int prec, void* stream) {
+                const int* o, int ncomp, int prec, void* stream, const ela_level_layout* win = nullptr,
+                const int* wo = nullptr) {
     if (int rc = check_pair(crse, fine, o, ncomp, prec)) return rc;
+    if (win)
+        if (int rc = check_window(crse, win, wo, fine, o)) return rc;
     if (!crse_f || !fine_f) return set_error(ELA_ERR_BADARG, "null field");
-    const Lay Lc = lay_of(*crse), Lf = lay_of(*fine);
+    Lay Lc = lay_of(*crse);
+    const Lay Lf = lay_of(*fine);
+    Win cw{0, 0, 0};
+    if (win) {  // node counts of the domain, strides of the window
+        Lc.px = win->pitch;
+        Lc.pl = win->plane;
+        Lc.cs = win->cstride;
+        cw = Win{wo[0], wo[1], crse->dim == 3 ? wo[2] : 0};
+    }
     const int d3 = crse->dim == 3;
     const int cnx = (Lf.nx - 1) / 2 + 1, cny = (Lf.ny - 1) / 2 + 1, cnz = d3 ? (Lf.nz - 1) / 2 + 1 : 1;
     const dim3 grid((cnx + RX - 1) / RX, (cny + RY - 1) / RY, cnz), block(RX, RY);
@@ -95,14 +129,14 @@ int launch_pair(const ela_level_layout* crse, void* crse_f, const ela_level_layo
     const int oz = d3 ? o[2] : 0;
     if (prec == ELA_F64) {
         if (d3) hipLaunchKernelGGL((Kern<double, 3>::k), grid, block, 0, s, Lc, Lf, o[0], o[1], oz, ncomp,
-                                   (const double*)fine_f, (double*)crse_f);
+                                   (const double*)fine_f, (double*)crse_f, cw);
         else hipLaunchKernelGGL((Kern<double, 2>::k), grid, block, 0, s, Lc, Lf, o[0], o[1], oz, ncomp,
-                                (const double*)fine_f, (double*)crse_f);
+                                (const double*)fine_f, (double*)crse_f, cw);
     } else {
         if (d3) hipLaunchKernelGGL((Kern<float, 3>::k), grid, block, 0, s, Lc, Lf, o[0], o[1], oz, ncomp,
-                                   (const float*)fine_f, (float*)crse_f);
+                                   (const float*)fine_f, (float*)crse_f, cw);
         else hipLaunchKernelGGL((Kern<float, 2>::k), grid, block, 0, s, Lc, Lf, o[0], o[1], oz, ncomp,
-                                (const float*)fine_f, (float*)crse_f);
+                                (const float*)fine_f, (float*)crse_f, cw);
     }
     ELA_HIPCHK(hipGetLastError());
     if (!stream) ELA_HIPCHK(hipStreamSynchronize(nullptr));
@@ -128,6 +162,19 @@ extern "C" {
 int ela_reflux(const ela_level_layout* crse, void* crse_res, const ela_level_layout* fine, const void* fine_res,
                const int fine_origin[3], int ncomp, int prec, void* stream) {
     return launch_pair<Reflux>(crse, crse_res, fine, fine_res, fine_origin, ncomp, prec, stream);
+}
+
+int ela_reflux_window(const ela_level_layout* crse, const ela_level_layout* crse_win, const int crse_origin[3],
+                      void* crse_res, const ela_level_layout* fine, const void* fine_res, const int fine_origin[3],
+                      int ncomp, int prec, void* stream) {
+    return launch_pair<Reflux>(crse, crse_res, fine, fine_res, fine_origin, ncomp, prec, stream, crse_win, crse_origin);
+}
+
+int ela_average_down_window(const ela_level_layout* crse, const ela_level_layout* crse_win, const int crse_origin[3],
+                            void* crse_sol, const ela_level_layout* fine, const void* fine_sol,
+                            const int fine_origin[3], int ncomp, int prec, void* stream) {
+    return launch_pair<AverageDown>(crse, crse_sol, fine, fine_sol, fine_origin, ncomp, prec, stream, crse_win,
+                                    crse_origin);
 }
 
 int ela_average_down(const ela_level_layout* crse, void* crse_sol, const ela_level_layout* fine,

@@ -7,6 +7,7 @@
 #include <cstddef>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 
 #include "../../include/ela.h"
 
@@ -15,6 +16,7 @@ struct ela_ctx {
     size_t budget = 0;
     size_t in_use = 0;
     std::mutex mu;
+    std::unordered_map<void*, size_t> raw;  // ela_dev_alloc'd buffers and their sizes
 };
 
 namespace ela {

@@ -547,16 +547,100 @@ __global__ __launch_bounds__(256) void k_voronoi_cubic(Lay L, int nseeds, const 
     if (gid) gid[o] = (unsigned short)bi;
 }
 
-__global__ __launch_bounds__(256) void k_degraded_iso(Lay L, const double* __restrict__ eta, double lambda0,
-                                                      double mu0, double a, double b, double c,
-                                                      double* __restrict__ C) {
+// ---------------------------------------------------------------- model-field assembly
+// Node value of an order-parameter field eta (component c) at node (i, j, k)
+// of level L, from one of the sources of EtaSrc: node values, or cell values
+// averaged onto the node the way the reference's callers do it.
+template <int DIM>
+__device__ __forceinline__ double eta_at(const Lay& L, const EtaSrc& e, int c, int i, int j, int k) {
+    if (e.avg == ETA_NODES) return e.p[c * e.cs + (long long)i + (long long)L.nx * (j + (long long)L.ny * k)];
+    // cells -1..n per direction (one ghost layer), x fastest
+    auto cv = [&](int ci, int cj, int ck) {
+        const long long kk = DIM == 3 ? (long long)(ck + 1) : 0;
+        return e.p[c * e.cs + (long long)(ci + 1) + (long long)e.gx * ((cj + 1) + (long long)e.gy * kk)];
+    };
+    if (e.avg == ETA_XY4)  // PhaseFieldMicrostructure.cpp:537: the 4 cells of the x-y plane, also in 3D
+        return 0.25 * (cv(i, j, k) + cv(i, j - 1, k) + cv(i - 1, j, k) + cv(i - 1, j - 1, k));
+    // ETA_CELLS: every cell around the node, times 1/(2+2) or 1/(2+2+4)
+    // (PolymerDegradation.cpp:817-827, in its order of summation)
+    const double mul = 1.0 / (DIM == 3 ? 8.0 : 4.0);
+    double t = cv(i, j, k) + cv(i - 1, j, k);
+    t = t + cv(i, j - 1, k) + cv(i - 1, j - 1, k);
+    if (DIM == 3) t = t + cv(i, j, k - 1) + cv(i - 1, j, k - 1) + cv(i, j - 1, k - 1) + cv(i - 1, j - 1, k - 1);
+    return mul * t;
+}
+
+// Checks of DegradeModulus (Degradable/Isotropic.H:74-104) on the node
+// values of components [c0, c1): bit 0 a NaN, bit 1 an infinite eta in
+// component 0 (the reference's "Infs" test looks at eta[0] only), bit 2 a
+// value outside [0, 1] (an infinite component 1 lands here, as in the reference).
+template <int DIM>
+__global__ __launch_bounds__(256) void k_eta_check(Lay L, EtaSrc e, int c0, int c1, int* __restrict__ flags) {
+    int i, j, k;
+    if (!node_index(L, i, j, k)) return;
+    int f = 0;
+    for (int c = c0; c < c1; ++c) {
+        const double v = eta_at<DIM>(L, e, c, i, j, k);
+        if (v != v) f |= 1;
+        else if (c == 0 && (v == INFINITY || v == -INFINITY)) f |= 2;
+        else if (v < 0.0 || v > 1.0) f |= 4;
+    }
+    if (f) atomicOr(flags, f);
+}
+
+// lambda = lambda0 (1 - eta_a), mu = mu0 (1 - eta_b) (DegradeModulus; a == b
+// for its one-eta forms).
+template <int DIM>
+__global__ __launch_bounds__(256) void k_degrade_iso(Lay L, EtaSrc e, int ca, int cb, double lambda0, double mu0,
+                                                     double* __restrict__ C) {
     int i, j, k;
     if (!node_index(L, i, j, k)) return;
     const long long o = (long long)i + L.px * j + L.pl * k;
-    const double e = eta[o];
-    const double g = (a * e + b) * e + c;
-    C[o] = lambda0 * g;
-    C[L.cs + o] = mu0 * g;
+    const double ea = eta_at<DIM>(L, e, ca, i, j, k);
+    const double eb = cb == ca ? ea : eta_at<DIM>(L, e, cb, i, j, k);
+    C[o] = lambda0 * (1.0 - ea);
+    C[L.cs + o] = mu0 * (1.0 - eb);
+}
+
+// Affine::Cubic::Combine (Affine/Cubic.H:70-94) at every node: the weights
+// eta_n / sum(eta) of the node-averaged order parameters, each record and F0
+// accumulated in model order from zero.  NC stiffness values, DIM x DIM F0.
+template <int DIM, int NC>
+__global__ __launch_bounds__(256) void k_combine(Lay L, EtaSrc e, int nph, const double* __restrict__ ddw,
+                                                 const double* __restrict__ F0s, double* __restrict__ C,
+                                                 double* __restrict__ F0out, int* __restrict__ flags) {
+    int i, j, k;
+    if (!node_index(L, i, j, k)) return;
+    const long long o = (long long)i + L.px * j + L.pl * k;
+    double etasum = 0.;
+    for (int n = 0; n < nph; ++n) etasum += eta_at<DIM>(L, e, n, i, j, k);
+    double acc[NC], facc[DIM * DIM];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) acc[q] = 0.0;
+#pragma unroll
+    for (int q = 0; q < DIM * DIM; ++q) facc[q] = 0.0;
+    for (int n = 0; n < nph; ++n) {
+        const double w = eta_at<DIM>(L, e, n, i, j, k) / etasum;
+#pragma unroll
+        for (int q = 0; q < NC; ++q) acc[q] += ddw[n * NC + q] * w;
+        if (F0s) {
+#pragma unroll
+            for (int a = 0; a < DIM; ++a)
+#pragma unroll
+                for (int b = 0; b < DIM; ++b) facc[a * DIM + b] += F0s[n * 9 + 3 * a + b] * w;
+        }
+    }
+    bool bad = false;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        C[q * L.cs + o] = acc[q];
+        bad |= !(acc[q] == acc[q]);
+    }
+    if (F0out) {
+#pragma unroll
+        for (int q = 0; q < DIM * DIM; ++q) F0out[q * L.cs + o] = facc[q];
+    }
+    if (bad) atomicOr(flags, 1);
 }
 
 template <typename T>
@@ -1330,9 +1414,22 @@ void launch_voronoi_cubic(const Lay& L, int nseeds, const long long* seeds, int 
     hipLaunchKernelGGL(k_voronoi_cubic, node_grid(L), dim3(BX, BY), 0, s, L, nseeds, seeds, q, table, C, gid);
 }
 
-void launch_degraded_iso(const Lay& L, const double* eta, double lambda0, double mu0, double a, double b, double c,
-                         double* C, hipStream_t s) {
-    hipLaunchKernelGGL(k_degraded_iso, node_grid(L), dim3(BX, BY), 0, s, L, eta, lambda0, mu0, a, b, c, C);
+void launch_eta_check(const Lay& L, const EtaSrc& e, int c0, int c1, int* flags, hipStream_t s) {
+    if (L.dim == 3) hipLaunchKernelGGL(k_eta_check<3>, node_grid(L), dim3(BX, BY), 0, s, L, e, c0, c1, flags);
+    else hipLaunchKernelGGL(k_eta_check<2>, node_grid(L), dim3(BX, BY), 0, s, L, e, c0, c1, flags);
+}
+void launch_degrade_iso(const Lay& L, const EtaSrc& e, int ca, int cb, double lambda0, double mu0, double* C,
+                        hipStream_t s) {
+    if (L.dim == 3)
+        hipLaunchKernelGGL(k_degrade_iso<3>, node_grid(L), dim3(BX, BY), 0, s, L, e, ca, cb, lambda0, mu0, C);
+    else hipLaunchKernelGGL(k_degrade_iso<2>, node_grid(L), dim3(BX, BY), 0, s, L, e, ca, cb, lambda0, mu0, C);
+}
+void launch_combine(const Lay& L, const EtaSrc& e, int nph, const double* ddw, const double* F0s, double* C,
+                    double* F0out, int* flags, hipStream_t s) {
+    if (L.dim == 3)
+        hipLaunchKernelGGL((k_combine<3, 21>), node_grid(L), dim3(BX, BY), 0, s, L, e, nph, ddw, F0s, C, F0out, flags);
+    else
+        hipLaunchKernelGGL((k_combine<2, 6>), node_grid(L), dim3(BX, BY), 0, s, L, e, nph, ddw, F0s, C, F0out, flags);
 }
 
 template <typename T>
@@ -1447,7 +1544,7 @@ void launch_post(const Lay& L, int kind, const double* C, const double* u, doubl
 template <int K, int DIM>
 __global__ __launch_bounds__(256) void k_newton_dw(Lay L, const double* __restrict__ C, const double* __restrict__ F0,
                                                   const double* __restrict__ u, double* __restrict__ dw,
-                                                  Inv<double> iv) {
+                                                  Inv<double> iv, double* __restrict__ w, double wfac) {
     int i, j, k;
     if (!node_index(L, i, j, k)) return;
     const long long o = (long long)i + L.px * j + L.pl * k;
@@ -1460,10 +1557,25 @@ __global__ __launch_bounds__(256) void k_newton_dw(Lay L, const double* __restri
             for (int b = 0; b < DIM; ++b) g[a][b] -= F0[(a * DIM + b) * L.cs + o];
     }
     stress<double, K, DIM>(PlaneRec<double>{C + o, L.cs}, g, s);
+    if (dw) {
 #pragma unroll
-    for (int a = 0; a < DIM; ++a)
+        for (int a = 0; a < DIM; ++a)
 #pragma unroll
-        for (int b = 0; b < DIM; ++b) dw[(a * DIM + b) * L.cs + o] = s[a][b];
+            for (int b = 0; b < DIM; ++b) dw[(a * DIM + b) * L.cs + o] = s[a][b];
+    }
+    if (w) {
+        // W = trace((wfac g)^T (DDW g)) (Linear/Isotropic.H:31-34, Affine/Isotropic.H:31-34):
+        // column sums of the product, then the trace, in index order
+        double tr = 0.0;
+#pragma unroll
+        for (int b = 0; b < DIM; ++b) {
+            double col = 0.0;
+#pragma unroll
+            for (int a = 0; a < DIM; ++a) col += (wfac * g[a][b]) * s[a][b];
+            tr += col;
+        }
+        w[o] = tr;
+    }
 }
 
 // rhs = b - BC row(u, grad u, dw) on the boundary, b - div dw (central
@@ -1505,11 +1617,20 @@ __global__ __launch_bounds__(256) void k_newton_rhs(Lay L, const double* __restr
     }
 }
 
+void launch_newton_w_dw(const Lay& L, int kind, const double* C, const double* F0, const double* u, double wfac,
+                        double* w, double* dw, const double h[3], hipStream_t s) {
+    const Inv<double> iv = make_inv<double>(h);
+    ELA_DISPATCH(kind, L.dim,
+                 hipLaunchKernelGGL((k_newton_dw<KK, DD>), node_grid(L), dim3(BX, BY), 0, s, L, C, F0, u, dw, iv, w,
+                                    wfac));
+}
+
 void launch_newton_rhs(const Lay& L, int kind, const double* C, const double* F0, const double* u, const double* b,
                        double* dw, double* rhs, const BCTab& bc, const double h[3], hipStream_t s) {
     const Inv<double> iv = make_inv<double>(h);
     ELA_DISPATCH(kind, L.dim,
-                 hipLaunchKernelGGL((k_newton_dw<KK, DD>), node_grid(L), dim3(BX, BY), 0, s, L, C, F0, u, dw, iv));
+                 hipLaunchKernelGGL((k_newton_dw<KK, DD>), node_grid(L), dim3(BX, BY), 0, s, L, C, F0, u, dw, iv,
+                                    (double*)nullptr, 0.0));
     if (L.dim == 3)
         hipLaunchKernelGGL((k_newton_rhs<3>), node_grid(L), dim3(BX, BY), 0, s, L, u, b, dw, rhs, bc, iv, h[0], h[1],
                            h[2]);

@@ -135,8 +135,27 @@ void launch_post(const Lay& L, int kind, const double* C, const double* u, doubl
 template <typename T, int MODE, typename TO>
 void launch_grain(GrainBox<T> g, int kz, double* partials, unsigned long long* norm, hipStream_t s);
 long long grain_partials(const Lay& L, int kz);
-void launch_degraded_iso(const Lay& L, const double* eta, double lambda0, double mu0, double a, double b, double c,
-                         double* C, hipStream_t s);
+// Order-parameter fields for the model assembly: node values (dense over the
+// level's nodes), or cell values with one ghost layer ((n+2) cells per
+// direction, x fastest) averaged onto the nodes over every cell around a node
+// (ETA_CELLS) or over the 4 cells of its x-y plane (ETA_XY4).  cs: elements
+// between components.
+enum { ETA_NODES = 0, ETA_CELLS = 1, ETA_XY4 = 2 };
+struct EtaSrc {
+    const double* p;
+    int avg;
+    int gx, gy;  // cells per row / rows per plane, ghosts included (cell sources)
+    long long cs;
+};
+void launch_eta_check(const Lay& L, const EtaSrc& e, int c0, int c1, int* flags, hipStream_t s);
+void launch_degrade_iso(const Lay& L, const EtaSrc& e, int ca, int cb, double lambda0, double mu0, double* C,
+                        hipStream_t s);
+void launch_combine(const Lay& L, const EtaSrc& e, int nph, const double* ddw, const double* F0s, double* C,
+                    double* F0out, int* flags, hipStream_t s);
+// Newton::W / DW (Newton.H:233-333): dw = DDW : (grad u - F0) (dim*dim planes),
+// W = trace((wfac (grad u - F0))^T dw); either output may be null.
+void launch_newton_w_dw(const Lay& L, int kind, const double* C, const double* F0, const double* u, double wfac,
+                        double* w, double* dw, const double h[3], hipStream_t s);
 
 // Flat FP64 vector kernels of the Krylov accelerator (whole fields incl. zero padding).
 constexpr int kMaxKrylov = 33;

@@ -1076,6 +1076,50 @@ int ela_ctx_destroy(ela_ctx* ctx) {
 
 size_t ela_ctx_bytes_in_use(const ela_ctx* ctx) { return ctx ? ctx->in_use : 0; }
 
+int ela_dev_alloc(ela_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return fail(ELA_ERR_BADARG, "null argument");
+    *out = nullptr;
+    if (bytes == 0) return ELA_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    if (int rc = budget_reserve(ctx, bytes)) return rc;
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        budget_release(ctx, bytes);
+        return fail(ELA_ERR_OOM, "hipMalloc failed");
+    }
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        ctx->raw[p] = bytes;
+    }
+    *out = p;
+    return ELA_OK;
+}
+
+int ela_dev_free(ela_ctx* ctx, void* p) {
+    if (!ctx) return fail(ELA_ERR_BADARG, "null context");
+    if (!p) return ELA_OK;
+    size_t bytes = 0;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        auto it = ctx->raw.find(p);
+        if (it == ctx->raw.end()) return fail(ELA_ERR_BADARG, "not a buffer of ela_dev_alloc on this context");
+        bytes = it->second;
+        ctx->raw.erase(it);
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipFree(p));
+    budget_release(ctx, bytes);
+    return ELA_OK;
+}
+
+int ela_memcpy(void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return ELA_OK;
+    if (!dst || !src) return fail(ELA_ERR_BADARG, "null pointer");
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
+    return ELA_OK;
+}
+
 int ela_hier_create(ela_ctx* ctx, const ela_domain* dom, int model_kind, int max_mg_levels, int mode,
                     ela_hier** out) {
     return ela_hier_create_ex(ctx, dom, model_kind, max_mg_levels, 4, mode, out);
@@ -1312,24 +1356,97 @@ int ela_set_model_voronoi_cubic(ela_hier* h, int nseeds, const int64_t* seeds, i
     return ELA_OK;
 }
 
-int ela_set_model_degraded_iso(ela_hier* h, double lambda0, double mu0, const double* eta, double a, double b,
-                               double c) {
+// Cell-valued order parameter with one ghost layer on level 0.
+static EtaSrc cell_src(const Level& v, int dim, const double* p, int avg) {
+    EtaSrc e{p, avg, v.n[0] + 2, v.n[1] + 2, 0};
+    e.cs = (long long)e.gx * e.gy * (dim == 3 ? v.n[2] + 2 : 1);
+    return e;
+}
+
+// Run the eta checks of DegradeModulus on the device and turn them into a status.
+static int eta_status(ela_hier* h, const Level& v, const EtaSrc& e, int c0, int c1, hipStream_t s) {
+    HIPCHK(hipMemsetAsync(h->dflag, 0, sizeof(int), s));
+    launch_eta_check(v.L, e, c0, c1, h->dflag, s);
+    int f = 0;
+    HIPCHK(hipMemcpyAsync(&f, h->dflag, sizeof f, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (f & 1) return fail(ELA_ERR_NAN, "Nans in eta");
+    if (f & 2) return fail(ELA_ERR_NAN, "Infs in eta");
+    if (f & 4) return fail(ELA_ERR_BADARG, "Invalid value of eta (outside [0, 1])");
+    return ELA_OK;
+}
+
+int ela_set_model_degraded_iso(ela_hier* h, double lambda0, double mu0, int neta, const double* eta) {
     if (!h || !eta) return fail(ELA_ERR_BADARG, "null argument");
-    h->has_table = false;
     if (h->kind != ELA_MODEL_ISO) return fail(ELA_ERR_STATE, "degraded isotropic needs an isotropic hierarchy");
+    if (neta != 1 && neta != 2) return fail(ELA_ERR_BADARG, "one or two eta fields");
     const Level& v = h->lev[0];
     double* deta = nullptr;
-    int rc = halloc_t(h, field_elems(v, 1), &deta);
+    int rc = halloc_t(h, (size_t)v.nodes * neta, &deta);
     if (rc) return rc;
-    rc = upload_dense(h, v, 1, eta, deta, ELA_F64);
+    HIPCHK(hipMemcpyAsync(deta, eta, sizeof(double) * v.nodes * neta, hipMemcpyHostToDevice, h->stream));
+    const EtaSrc e{deta, ETA_NODES, 0, 0, (long long)v.nodes};
+    rc = eta_status(h, v, e, 0, neta, h->stream);
     if (!rc) {
-        launch_degraded_iso(v.L, deta, lambda0, mu0, a, b, c, v.C64, h->stream);
+        launch_degrade_iso(v.L, e, 0, neta - 1, lambda0, mu0, v.C64, h->stream);
         if (hipStreamSynchronize(h->stream) != hipSuccess) rc = fail(ELA_ERR_HIP, "degraded stiffness kernel failed");
     }
     hfree(h, deta);
     if (rc) return rc;
+    h->has_table = false;
     h->model_set = true;
     h->prepared = false;
+    return ELA_OK;
+}
+
+int ela_degrade_material(ela_hier* h, double lambda0, double mu0, int neta, const double* eta_cells, void* stream) {
+    if (!h || !eta_cells) return fail(ELA_ERR_BADARG, "null argument");
+    if (h->kind != ELA_MODEL_ISO) return fail(ELA_ERR_STATE, "DegradeMaterial needs an isotropic hierarchy");
+    if (neta < 1) return fail(ELA_ERR_BADARG, "number_of_eta must be positive");
+    const Level& v = h->lev[0];
+    hipStream_t s = strm(h, stream);
+    const EtaSrc e = cell_src(v, h->dim, eta_cells, ETA_CELLS);
+    if (int rc = eta_status(h, v, e, 0, neta, s)) return rc;
+    // each component calls DegradeModulus(temp) from (lambda0, mu0): the last one stays
+    launch_degrade_iso(v.L, e, neta - 1, neta - 1, lambda0, mu0, v.C64, s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    h->has_table = false;
+    h->model_set = true;
+    h->prepared = false;
+    return ELA_OK;
+}
+
+int ela_combine_cubic(ela_hier* h, int nph, const double* ddw, const double* F0, const double* eta_cells,
+                      void* F0_out, void* stream) {
+    if (!h || !ddw || !eta_cells) return fail(ELA_ERR_BADARG, "null argument");
+    if (h->kind != ELA_MODEL_MM) return fail(ELA_ERR_STATE, "Combine needs a major/minor hierarchy");
+    if (nph < 1 || nph > 4096) return fail(ELA_ERR_BADARG, "1 to 4096 models");
+    const Level& v = h->lev[0];
+    hipStream_t s = strm(h, stream);
+    const int nc = h->nc;
+    double* dtab = nullptr;
+    int rc = halloc_t(h, (size_t)nph * (nc + 9), &dtab);
+    if (rc) return rc;
+    std::vector<double> tab((size_t)nph * (nc + 9), 0.0);
+    std::copy(ddw, ddw + (size_t)nph * nc, tab.begin());
+    if (F0) std::copy(F0, F0 + (size_t)nph * 9, tab.begin() + (size_t)nph * nc);
+    hipError_t e = hipMemcpyAsync(dtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemsetAsync(h->dflag, 0, sizeof(int), s);
+    if (e == hipSuccess) {
+        launch_combine(v.L, cell_src(v, h->dim, eta_cells, ETA_XY4), nph, dtab, F0 ? dtab + (size_t)nph * nc : nullptr,
+                       v.C64, (double*)F0_out, h->dflag, s);
+        e = hipGetLastError();
+    }
+    int f = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&f, h->dflag, sizeof f, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hfree(h, dtab);
+    if (e != hipSuccess) return fail(ELA_ERR_HIP, std::string("Combine: ") + hipGetErrorString(e));
+    h->has_table = false;
+    h->model_set = true;
+    h->prepared = false;
+    if (f) return fail(ELA_ERR_NAN, "Combine: non-finite stiffness (the eta weights sum to zero at a node)");
     return ELA_OK;
 }
 
@@ -1522,6 +1639,19 @@ int ela_newton_rhs(ela_hier* h, const void* u, const void* b, const void* F0, vo
                       (double*)rhs, h->bc, v.h, s);
     HIPCHK(hipGetLastError());
     if (!stream) HIPCHK(hipStreamSynchronize(s));
+    return ELA_OK;
+}
+
+int ela_newton_w_dw(ela_hier* h, const void* u, const void* F0, double wfac, void* w, void* dw, void* stream) {
+    if (!h || !u) return fail(ELA_ERR_BADARG, "null argument");
+    if (!h->model_set) return fail(ELA_ERR_STATE, "Attempting to use operator before calling SetModel");
+    Level& v = h->lev[0];
+    if (!v.C64) return fail(ELA_ERR_STATE, "no FP64 stiffness on level 0");
+    if (!w && !dw) return ELA_OK;
+    launch_newton_w_dw(v.L, h->kind, v.C64, (const double*)F0, (const double*)u, wfac, (double*)w, (double*)dw, v.h,
+                       strm(h, stream));
+    HIPCHK(hipGetLastError());
+    if (!stream) HIPCHK(hipStreamSynchronize(h->stream));
     return ELA_OK;
 }
 

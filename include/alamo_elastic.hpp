@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -123,6 +124,7 @@ namespace Affine {
 class Isotropic : public Linear::Isotropic {
 public:
     static constexpr bool affine = true;
+    static constexpr double w_factor = 1.0;  // W = (F-F0):C:(F-F0), no 1/2 (Affine/Isotropic.H:31-34)
     Isotropic() {}
     Isotropic(Set::Scalar mu, Set::Scalar lambda, const std::array<double, 9>& a_F0 = {})
         : Linear::Isotropic(mu, lambda), F0(a_F0) {}
@@ -165,9 +167,25 @@ public:
     static constexpr int kind = ELA_MODEL_ISO;
     Isotropic() {}
     Isotropic(Set::Scalar mu, Set::Scalar lambda) : mu0(mu), lambda0(lambda), data{lambda, mu} {}
-    void DegradeModulus(Set::Scalar eta) {
+    void DegradeModulus(Set::Scalar eta) {  // Isotropic.H:74-81
         if (eta < 0.0 || eta > 1.0) throw std::runtime_error("Invalid value of eta");
         data = {lambda0 * (1.0 - eta), mu0 * (1.0 - eta)};
+    }
+    // Isotropic.H:83-104: one eta degrades both moduli, two degrade lambda and mu separately;
+    // any other count leaves the moduli unchanged.  The checks are the reference's, in its order.
+    void DegradeModulus(const std::vector<Set::Scalar>& eta) {
+        if (eta.size() == 1) {
+            if (std::isnan(eta[0])) throw std::runtime_error("Nans in eta");
+            if (std::isinf(eta[0])) throw std::runtime_error("Infs in eta");
+            if (eta[0] < 0.0 || eta[0] > 1.0) throw std::runtime_error("Invalid value of eta");
+            data = {lambda0 * (1.0 - eta[0]), mu0 * (1.0 - eta[0])};
+        } else if (eta.size() == 2) {
+            if (std::isnan(eta[0]) || std::isnan(eta[1])) throw std::runtime_error("Nans in eta");
+            if (std::isinf(eta[0])) throw std::runtime_error("Infs in eta");
+            if (eta[0] < 0.0 || eta[0] > 1.0 || eta[1] < 0.0 || eta[1] > 1.0)
+                throw std::runtime_error("Invalid value of eta");
+            data = {lambda0 * (1.0 - eta[0]), mu0 * (1.0 - eta[1])};
+        }
     }
     Set::Scalar mu0 = NAN, lambda0 = NAN;
     std::vector<double> data{NAN, NAN};
@@ -220,10 +238,45 @@ struct Box {
     std::array<int, 3> lo{0, 0, 0}, hi{0, 0, 0};
 };
 
+namespace ela_detail {
+// A device copy of a dense host array, freed with the object.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    explicit DevBuf(size_t n) : bytes(n) { check(ela_dev_alloc(context(), n, &p), "ela_dev_alloc"); }
+    DevBuf(const std::vector<double>& host) : DevBuf(host.size() * sizeof(double)) {
+        check(ela_memcpy(p, host.data(), bytes), "ela_memcpy");
+    }
+    void to(std::vector<double>& host) const { check(ela_memcpy(host.data(), p, bytes), "ela_memcpy"); }
+    ~DevBuf() {
+        if (p) ela_dev_free(context(), p);
+    }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+};
+inline ela_level_layout dense_layout(int dim, const std::array<int, 3>& nodes) {
+    ela_level_layout l{};
+    l.dim = dim;
+    for (int d = 0; d < 3; ++d) {
+        l.nodes[d] = d < dim ? nodes[d] : 1;
+        l.n[d] = l.nodes[d] - 1;
+    }
+    l.pitch = l.nodes[0];
+    l.plane = (int64_t)l.nodes[0] * l.nodes[1];
+    l.cstride = l.plane * l.nodes[2];
+    return l;
+}
+}  // namespace ela_detail
+
 namespace Operator {
-// Operator::Elastic<T> on one box.  define() builds the MG hierarchy;
-// SetModel/SetBC/SetUniform as in the reference; Fapply/Diagonal/Fsmooth/
-// restriction/interpolation act on dense host fields of a MG level.
+// Operator::Elastic<T> (src/Operator/Elastic.H:20-171) with the virtuals of
+// Operator<Grid::Node> (src/Operator/Operator.H:79-100).  define() builds the
+// MG hierarchy; SetModel/SetBC/SetUniform/SetHomogeneous as in the reference.
+// Fields are dense host vectors of the MG level they belong to (dim
+// components, x fastest); the per-level operations (Fapply, Diagonal,
+// Fsmooth, normalize, restriction, interpolation, the residuals, Strain /
+// Stress / Energy) run on one AMR level's hierarchy (amrlev 0 of a single-box
+// operator); reflux runs between two levels of a several-level operator.
 template <class T>
 class Elastic {
 public:
@@ -266,6 +319,7 @@ public:
                                          mode, &a),
                           "define");
         nlev = (int)fine.size() + 1;
+        fine_boxes = fine;
     }
     int NumAMRLevels() const { return nlev; }
     // node counts and first global node index of an AMR level's arrays (the level's box grown by 2 nodes)
@@ -345,6 +399,142 @@ public:
     void InitRHS(std::vector<double>& rhs, bool homogeneous = false) {
         ela_detail::check(ela_bc_init(h, rhs.data(), homogeneous), "Init");
     }
+    // ------------------------------------------------ per-level operations (Elastic.H:116-136, Operator.H:79-100)
+    int getNComp() const { return geom.dim; }         // Elastic.H:127
+    bool isCrossStencil() const { return false; }     // Elastic.H:128
+    int getNGrow() const { return 2; }                // Operator.H:95
+    int NMGLevels(int amrlev = 0) const {
+        single(amrlev);
+        return ela_hier_levels(h);
+    }
+    // Node counts of MG level mglev (a field there has getNComp() * nodes(mglev) values).
+    std::array<int, 3> levelShape(int mglev) const {
+        ela_level_layout l;
+        ela_detail::check(ela_level_info(h, mglev, &l), "ela_level_info");
+        return {l.nodes[0], l.nodes[1], l.nodes[2]};
+    }
+    // SetHomogeneous (Elastic.H:48): stored as in the reference.  The models here ignore it in
+    // their operator() (Solid.H:24-34), so it changes no operator here.
+    void SetHomogeneous(bool a_homogeneous) { m_homogeneous = a_homogeneous; }
+    bool GetHomogeneous() const { return m_homogeneous; }
+    // Fapply(amrlev, mglev, out, in) (Elastic.cpp:151-260)
+    void Fapply(int amrlev, int mglev, std::vector<double>& out, const std::vector<double>& in, int prec = ELA_F64) const {
+        single(amrlev);
+        Field du(h, mglev, prec, geom.dim), df(h, mglev, prec, geom.dim);
+        du.up(in);
+        ela_detail::check(ela_fapply(h, mglev, prec, du.p, df.p, nullptr), "Fapply");
+        out.resize(in.size());
+        df.down(out);
+    }
+    // Diagonal(amrlev, mglev, diag) (Elastic.cpp:264-352)
+    void Diagonal(int amrlev, int mglev, std::vector<double>& diag) const {
+        single(amrlev);
+        Field dd(h, mglev, ELA_F64, geom.dim);
+        ela_detail::check(ela_diagonal(h, mglev, ELA_F64, dd.p, nullptr), "Diagonal");
+        diag.resize((size_t)geom.dim * nodes(mglev));
+        dd.down(diag);
+    }
+    // Fsmooth(amrlev, mglev, x, b) (Operator.cpp:90-162): 2 weighted-Jacobi sweeps, omega = 2/3
+    void Fsmooth(int amrlev, int mglev, std::vector<double>& x, const std::vector<double>& b, int prec = ELA_F64) const {
+        single(amrlev);
+        Field dx(h, mglev, prec, geom.dim), db(h, mglev, prec, geom.dim);
+        dx.up(x);
+        db.up(b);
+        ela_detail::check(ela_fsmooth(h, mglev, prec, dx.p, db.p, 2, 2.0 / 3.0, nullptr), "Fsmooth");
+        dx.down(x);
+    }
+    // normalize(amrlev, mglev, mf) (Operator.cpp:164-195), with the reference's component indexing
+    void normalize(int amrlev, int mglev, std::vector<double>& mf) const {
+        single(amrlev);
+        Field d(h, mglev, ELA_F64, geom.dim);
+        d.up(mf);
+        ela_detail::check(ela_normalize(h, mglev, ELA_F64, d.p, nullptr), "normalize");
+        d.down(mf);
+    }
+    // restriction(amrlev, cmglev, crse, fine) (Operator.cpp:454-554)
+    void restriction(int amrlev, int cmglev, std::vector<double>& crse, std::vector<double>& fine) const {
+        single(amrlev);
+        Field df(h, cmglev - 1, ELA_F64, geom.dim), dc(h, cmglev, ELA_F64, geom.dim);
+        df.up(fine);
+        ela_detail::check(ela_restriction(h, cmglev, ELA_F64, df.p, dc.p, nullptr), "restriction");
+        crse.resize((size_t)geom.dim * nodes(cmglev));
+        dc.down(crse);
+    }
+    // interpolation(amrlev, fmglev, fine, crse) (Operator.cpp:556-624): fine += P crse
+    void interpolation(int amrlev, int fmglev, std::vector<double>& fine, const std::vector<double>& crse) const {
+        single(amrlev);
+        Field df(h, fmglev, ELA_F64, geom.dim), dc(h, fmglev + 1, ELA_F64, geom.dim);
+        df.up(fine);
+        dc.up(crse);
+        ela_detail::check(ela_interpolation(h, fmglev, ELA_F64, df.p, dc.p, nullptr), "interpolation");
+        df.down(fine);
+    }
+    // solutionResidual(amrlev, resid, x, b) (Operator.cpp:841-850): resid = b - A x; returns |resid|inf
+    double solutionResidual(int amrlev, std::vector<double>& resid, const std::vector<double>& x,
+                            const std::vector<double>& b) const {
+        return correctionResidual(amrlev, 0, resid, x, b);
+    }
+    // correctionResidual(amrlev, mglev, resid, x, b, ...) (Operator.cpp:852-863)
+    double correctionResidual(int amrlev, int mglev, std::vector<double>& resid, const std::vector<double>& x,
+                              const std::vector<double>& b) const {
+        single(amrlev);
+        Field dx(h, mglev, ELA_F64, geom.dim), db(h, mglev, ELA_F64, geom.dim), dr(h, mglev, ELA_F64, geom.dim);
+        dx.up(x);
+        db.up(b);
+        double nrm = 0;
+        ela_detail::check(ela_residual(h, mglev, ELA_F64, dx.p, db.p, dr.p, &nrm, nullptr), "residual");
+        resid.resize(x.size());
+        dr.down(resid);
+        return nrm;
+    }
+    // Strain / Stress / Energy(amrlev, out, u, voigt) (Elastic.H:76-92, Elastic.cpp:399-549) on level 0
+    void Strain(int amrlev, std::vector<double>& out, const std::vector<double>& u, bool voigt = false) const {
+        post(amrlev, 0, out, u, voigt);
+    }
+    void Stress(int amrlev, std::vector<double>& out, const std::vector<double>& u, bool voigt = false) const {
+        post(amrlev, 1, out, u, voigt);
+    }
+    void Energy(int amrlev, std::vector<double>& out, const std::vector<double>& u) const { post(amrlev, 2, out, u, false); }
+    // Reflux(crse_amrlev, res, crse_sol, crse_rhs, fine_res, fine_sol, fine_rhs) (Elastic.H:104-107,
+    // Operator.cpp:728-839) of a several-level operator: every coarse node under the fine box takes the
+    // full-weighting restriction of the fine residual (fine values outside the box: zero ghosts).  Fields
+    // are the levels' arrays over their grown boxes (levelNodes); only res and fine_res are read, as in
+    // the reference, whose solution and rhs arguments are unused.
+    void reflux(int crse_amrlev, std::vector<double>& res, const std::vector<double>& /*crse_sol*/,
+                const std::vector<double>& /*crse_rhs*/, std::vector<double>& fine_res,
+                const std::vector<double>& /*fine_sol*/, const std::vector<double>& /*fine_rhs*/) const {
+        if (!a) throw std::runtime_error("reflux needs a several-level operator");
+        if (crse_amrlev < 0 || crse_amrlev + 1 >= nlev) throw std::runtime_error("reflux: no finer level");
+        const int dim = geom.dim, f = crse_amrlev + 1;
+        const auto cn = levelNodes(crse_amrlev), co = levelOrigin(crse_amrlev);
+        const auto fn = levelNodes(f), fo = levelOrigin(f);
+        const Box& B = fine_boxes[f - 1];
+        std::array<int, 3> dom{1, 1, 1}, bn{1, 1, 1};
+        for (int d = 0; d < dim; ++d) {
+            dom[d] = (geom.n[d] << crse_amrlev) + 1;
+            bn[d] = B.hi[d] - B.lo[d] + 1;
+        }
+        const ela_level_layout Lc = ela_detail::dense_layout(dim, dom), Lw = ela_detail::dense_layout(dim, cn);
+        ela_level_layout Lf = ela_detail::dense_layout(dim, bn);
+        const ela_level_layout Lg = ela_detail::dense_layout(dim, fn);
+        Lf.pitch = Lg.pitch;  // the fine box inside the fine level's grown-box array
+        Lf.plane = Lg.plane;
+        Lf.cstride = Lg.cstride;
+        ela_detail::DevBuf dc(res), dfb(fine_res);
+        long long off = 0;
+        for (int d = 0; d < dim; ++d) off += (long long)(B.lo[d] - fo[d]) * (d == 0 ? 1 : d == 1 ? Lg.pitch : Lg.plane);
+        int forig[3] = {B.lo[0], B.lo[1], dim == 3 ? B.lo[2] : 0}, corig[3] = {co[0], co[1], dim == 3 ? co[2] : 0};
+        ela_detail::check(ela_reflux_window(&Lc, &Lw, corig, dc.p, &Lf, (const double*)dfb.p + off, forig, dim, ELA_F64,
+                                            nullptr),
+                          "reflux");
+        dc.to(res);
+    }
+    void Reflux(int crse_amrlev, std::vector<double>& res, const std::vector<double>& crse_sol,
+                const std::vector<double>& crse_rhs, std::vector<double>& fine_res, const std::vector<double>& fine_sol,
+                const std::vector<double>& fine_rhs) const {
+        reflux(crse_amrlev, res, crse_sol, crse_rhs, fine_res, fine_sol, fine_rhs);
+    }
+
     std::vector<double> Fapply(int mglev, const std::vector<double>& u, int prec = ELA_F64) {
         void *du = nullptr, *df = nullptr;
         ela_detail::check(ela_field_alloc(h, mglev, prec, geom.dim, &du), "alloc");
@@ -362,6 +552,37 @@ public:
     Geometry geom;
 
 private:
+    // A device field of one MG level, freed with the object.
+    struct Field {
+        ela_hier* h;
+        int mglev, prec, nc;
+        void* p = nullptr;
+        Field(ela_hier* a_h, int a_mglev, int a_prec, int a_nc) : h(a_h), mglev(a_mglev), prec(a_prec), nc(a_nc) {
+            ela_detail::check(ela_field_alloc(h, mglev, prec, nc, &p), "ela_field_alloc");
+        }
+        ~Field() {
+            if (p) ela_field_free(h, p);
+        }
+        void up(const std::vector<double>& host) { ela_detail::check(ela_upload(h, mglev, prec, nc, host.data(), p), "upload"); }
+        void down(std::vector<double>& host) { ela_detail::check(ela_download(h, mglev, prec, nc, p, host.data()), "download"); }
+        Field(const Field&) = delete;
+        Field& operator=(const Field&) = delete;
+    };
+    void single(int amrlev) const {
+        if (!h || amrlev != 0) throw std::runtime_error("per-level operations run on a single-box operator's amrlev 0");
+    }
+    void post(int amrlev, int what, std::vector<double>& out, const std::vector<double>& u, bool voigt) const {
+        single(amrlev);
+        const int dim = geom.dim;
+        const int nout = what == 2 ? 1 : (voigt ? (dim == 3 ? 6 : 3) : dim * dim);
+        Field du(h, 0, ELA_F64, dim), dout(h, 0, ELA_F64, nout);
+        du.up(u);
+        ela_detail::check(ela_postprocess(h, what, voigt, du.p, dout.p, nullptr), "postprocess");
+        out.resize((size_t)nout * nodes(0));
+        dout.down(out);
+    }
+    bool m_homogeneous = false;
+    std::vector<Box> fine_boxes;
     ela_hier* h = nullptr;
     ela_amr* a = nullptr;  // several AMR levels
     int nlev = 1;
@@ -372,6 +593,15 @@ private:
 // ---------------------------------------------------------------- solvers
 namespace Solver {
 namespace Nonlocal {
+// The factor of W's trace: 1/2 for the Linear models and Affine::Cubic, the model's own otherwise.
+template <class M, class = void>
+struct w_factor {
+    static constexpr double value = 0.5;
+};
+template <class M>
+struct w_factor<M, decltype((void)M::w_factor)> {
+    static constexpr double value = M::w_factor;
+};
 template <class M, class = void>
 struct has_F0 {
     static constexpr bool value = false;
@@ -477,7 +707,73 @@ public:
         return Linear<T>::solve(u, b, tol_rel, tol_abs);
     }
 
+    // compResidual(res, u, b, model) (Newton.H:211-229): res = b - div DW(grad u) inside and
+    // b - BC row(u, grad u, dw) on the boundary, the right-hand side of a Newton step from u.
+    void compResidual(std::vector<double>& res, const std::vector<double>& u, const std::vector<double>& b,
+                      const std::vector<T>& model) {
+        this->linop.SetModel(model);
+        Fields f(this->linop, model);
+        f.u.up(u);
+        f.b.up(b);
+        ela_detail::check(ela_newton_rhs(f.h, f.u.p, f.b.p, f.F0p(), f.r.p, nullptr), "compResidual");
+        res.resize(u.size());
+        f.r.down(res);
+    }
+    // W(w, u, model) (Newton.H:231-275): the strain energy density W(grad u) per node.
+    void W(std::vector<double>& w, const std::vector<double>& u, const std::vector<T>& model) {
+        this->linop.SetModel(model);
+        Fields f(this->linop, model);
+        f.u.up(u);
+        Buf wb(f.h, 1);
+        ela_detail::check(ela_newton_w_dw(f.h, f.u.p, f.F0p(), w_factor<T>::value, wb.p, nullptr, nullptr), "W");
+        w.resize((size_t)this->linop.nodes());
+        wb.down(w);
+    }
+    // DW(dw, u, model) (Newton.H:277-333): dw = DW(grad u) per node, dim*dim components (row-major).
+    void DW(std::vector<double>& dw, const std::vector<double>& u, const std::vector<T>& model) {
+        this->linop.SetModel(model);
+        Fields f(this->linop, model);
+        f.u.up(u);
+        const int dim = this->linop.geom.dim;
+        Buf db(f.h, dim * dim);
+        ela_detail::check(ela_newton_w_dw(f.h, f.u.p, f.F0p(), 0.0, nullptr, db.p, nullptr), "DW");
+        dw.resize((size_t)dim * dim * this->linop.nodes());
+        db.down(dw);
+    }
+
 private:
+    struct Buf {
+        ela_hier* h;
+        int nc;
+        void* p = nullptr;
+        Buf(ela_hier* a_h, int a_nc) : h(a_h), nc(a_nc) { ela_detail::check(ela_field_alloc(h, 0, ELA_F64, nc, &p), "alloc"); }
+        ~Buf() {
+            if (p) ela_field_free(h, p);
+        }
+        void up(const std::vector<double>& v) { ela_detail::check(ela_upload(h, 0, ELA_F64, nc, v.data(), p), "upload"); }
+        void down(std::vector<double>& v) { ela_detail::check(ela_download(h, 0, ELA_F64, nc, p, v.data()), "download"); }
+        Buf(const Buf&) = delete;
+        Buf& operator=(const Buf&) = delete;
+    };
+    // u, b, rhs and (affine models) the per-node F0 planes on level 0
+    struct Fields {
+        ela_hier* h;
+        Buf u, b, r;
+        std::unique_ptr<Buf> F0;
+        Fields(::Operator::Elastic<T>& op, const std::vector<T>& model)
+            : h(op.handle()), u(h, op.geom.dim), b(h, op.geom.dim), r(h, op.geom.dim) {
+            if constexpr (has_F0<T>::value) {
+                const int dim = op.geom.dim, nn = op.nodes();
+                std::vector<double> f((size_t)dim * dim * nn);
+                for (int i = 0; i < nn; ++i)
+                    for (int a = 0; a < dim; ++a)
+                        for (int c = 0; c < dim; ++c) f[(size_t)(a * dim + c) * nn + i] = model[i].F0[3 * a + c];
+                F0.reset(new Buf(h, dim * dim));
+                F0->up(f);
+            }
+        }
+        void* F0p() const { return F0 ? F0->p : nullptr; }
+    };
     int nriters = 1;  // Newton.H:337
 };
 }  // namespace Nonlocal

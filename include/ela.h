@@ -135,12 +135,43 @@ int ela_set_model_voronoi_cubic(ela_hier* h, int nseeds, const int64_t* seeds, i
  * the same level-0 operator. */
 int ela_set_stiffness_mode(ela_hier* h, int mode);
 
-/* Phase-field-degraded isotropic stiffness (Degradable::Isotropic::DegradeModulus,
- * src/Model/Solid/LinearElastic/Degradable/Isotropic.H:74-104, with a general
- * degradation g): lambda = g(eta) lambda0, mu = g(eta) mu0 with
- * g(eta) = a*eta^2 + b*eta + c, eta a dense host array of node values. */
-int ela_set_model_degraded_iso(ela_hier* h, double lambda0, double mu0, const double* eta, double a, double b,
-                               double c);
+/* Degradable::Isotropic::DegradeModulus (src/Model/Solid/LinearElastic/
+ * Degradable/Isotropic.H:74-104) at every level-0 node of an isotropic
+ * hierarchy with undegraded moduli (lambda0, mu0): neta = 1 gives
+ * lambda = lambda0 (1 - eta), mu = mu0 (1 - eta); neta = 2 gives
+ * lambda = lambda0 (1 - eta0), mu = mu0 (1 - eta1).  eta: dense host node
+ * values, neta components.  Where the reference aborts, this returns, with
+ * the model left as it was: ELA_ERR_NAN for a NaN eta or an infinite eta0
+ * ("Nans in eta" / "Infs in eta"), ELA_ERR_BADARG for a value outside [0, 1]
+ * (an infinite eta1 included: the reference's Inf test reads eta0 only). */
+int ela_set_model_degraded_iso(ela_hier* h, double lambda0, double mu0, int neta, const double* eta);
+
+/* PolymerDegradation::DegradeMaterial (src/Integrator/PolymerDegradation.cpp:
+ * 793-833) on the device.  eta_cells: device array of cell-centred damage,
+ * neta components of (n0+2)(n1+2)[(n2+2)] cells each (cells -1..n per
+ * direction, one ghost layer, x fastest).  Each node takes the mean of the
+ * 2^dim cells around it; the reference calls the one-eta DegradeModulus once
+ * per component, each from (lambda0, mu0), so the last component sets the
+ * moduli, and every component's node mean must lie in [0, 1]
+ * (ELA_ERR_BADARG; a NaN gives ELA_ERR_NAN, which the reference would meet as
+ * its NaN-diagonal abort, Elastic.cpp:347).  stream NULL: the library's. */
+int ela_degrade_material(ela_hier* h, double lambda0, double mu0, int neta, const double* eta_cells, void* stream);
+
+/* The model fill of PhaseFieldMicrostructure (src/Integrator/
+ * PhaseFieldMicrostructure.cpp:529-543) with Model::Solid::Affine::Cubic::
+ * Combine (src/Model/Solid/Affine/Cubic.H:70-94), on the device, into a
+ * major/minor hierarchy's level 0.  nph models: ddw (host, nph records of 21
+ * values in 3D / 6 in 2D, the ELA_MODEL_MM order), F0 (host, nph row-major
+ * 3x3, or NULL for zero).  eta_cells: device cell array of nph order
+ * parameters laid out as for ela_degrade_material; a node's eta_n is the mean
+ * of the 4 cells (i, j), (i, j-1), (i-1, j), (i-1, j-1) of its x-y plane at
+ * cell layer k (the reference's 2D average, also in 3D).  Each node's record
+ * and F0 are sum_n ddw_n eta_n / sum(eta), accumulated in model order.
+ * F0_out: device, dim*dim FP64 planes (row-major) in the level-0 layout, the
+ * F0 input of ela_newton_rhs / ela_newton_w_dw, or NULL.  A node whose
+ * weights sum to zero makes a NaN record: ELA_ERR_NAN (the model is written). */
+int ela_combine_cubic(ela_hier* h, int nph, const double* ddw, const double* F0, const double* eta_cells,
+                      void* F0_out, void* stream);
 
 /* SetBC (src/Operator/Elastic.H:58-66); values are used by ela_bc_init. */
 int ela_set_bc(ela_hier* h, const int* types, const double* values);
@@ -153,6 +184,13 @@ int ela_set_uniform(ela_hier* h, int uniform);
  * Elastic.cpp:722-809), compute the diagonals (Diagonal, Elastic.cpp:264-352)
  * and factor the coarsest level. */
 int ela_prepare(ela_hier* h);
+
+/* Raw device buffers, counted against the context's HBM budget (for inputs
+ * that are not level fields, e.g. the cell arrays of ela_degrade_material),
+ * and a copy between any host or device pointers (hipMemcpyDefault). */
+int ela_dev_alloc(ela_ctx* ctx, size_t bytes, void** out);
+int ela_dev_free(ela_ctx* ctx, void* p);
+int ela_memcpy(void* dst, const void* src, size_t bytes);
 
 /* Device field helpers on a level's layout. */
 int ela_field_alloc(ela_hier* h, int mglev, int prec, int ncomp, void** dev);
@@ -187,6 +225,17 @@ int ela_postprocess(ela_hier* h, int what, int voigt, const void* u, void* out, 
  * level-0 fields (dim components); F0: dim*dim FP64 planes (row-major) in the
  * level-0 layout, or NULL for F0 = 0.  Newton's step then solves A du = rhs. */
 int ela_newton_rhs(ela_hier* h, const void* u, const void* b, const void* F0, void* rhs, void* stream);
+
+/* Newton<T>::W and Newton<T>::DW (src/Solver/Nonlocal/Newton.H:233-333) for
+ * the models whose kinematic variable is grad u, at every level-0 node with
+ * the domain stencil (one-sided on the walls): dw = DDW : (grad u - F0)
+ * (dim*dim FP64 planes, row-major) and
+ * W = trace((wfac (grad u - F0))^T dw), where wfac = 0.5 for the Linear
+ * models and Affine::Cubic and wfac = 1 for Affine::Isotropic (its W has no
+ * 1/2, src/Model/Solid/Affine/Isotropic.H:31-34).  u: FP64 level-0 field; F0:
+ * dim*dim planes or NULL; w (1 plane) or dw may be NULL.  stream NULL: the
+ * call completes before returning. */
+int ela_newton_w_dw(ela_hier* h, const void* u, const void* F0, double wfac, void* w, void* dw, void* stream);
 
 /* normalize (src/Operator/Operator.cpp:164-195), as the reference computes it:
  * its per-component lambda indexes neither x nor diag by component, so
@@ -318,6 +367,19 @@ int ela_probe_copy(ela_ctx* ctx, size_t bytes, int nrep, double* avg_ms);
  * domain); fine values outside the box are ghost nodes, zero. */
 int ela_reflux(const ela_level_layout* crse, void* crse_res, const ela_level_layout* fine, const void* fine_res,
                const int fine_origin[3], int ncomp, int prec, void* stream);
+
+/* The same two transfers into a window of the coarse level (e.g. an AMR
+ * level's own arrays, which cover only its grown box): crse describes the
+ * coarse domain (node counts; they decide the domain-boundary branches),
+ * crse_win the window's array (node counts and strides) and crse_origin the
+ * coarse node index of its first node; the window must hold every coarse
+ * node under the fine box. */
+int ela_reflux_window(const ela_level_layout* crse, const ela_level_layout* crse_win, const int crse_origin[3],
+                      void* crse_res, const ela_level_layout* fine, const void* fine_res, const int fine_origin[3],
+                      int ncomp, int prec, void* stream);
+int ela_average_down_window(const ela_level_layout* crse, const ela_level_layout* crse_win, const int crse_origin[3],
+                            void* crse_sol, const ela_level_layout* fine, const void* fine_sol,
+                            const int fine_origin[3], int ncomp, int prec, void* stream);
 
 /* averageDownSolutionRHS (src/Operator/Operator.cpp:626-641): coarse nodes of
  * the coarsened fine box take the coincident fine solution value (nodal

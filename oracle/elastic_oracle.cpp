@@ -825,6 +825,39 @@ void or_newton_rhs(const or_grid* og, int kind, const double* C, const double* F
             }
 }
 
+// Newton<T>::W and DW (src/Solver/Nonlocal/Newton.H:233-333) for the models
+// whose kinematic variable is grad u: at every node with the GetStencil
+// gradient (one-sided on the walls), sig = DDW : (grad u - F0) into dw (dim*dim
+// planes, row-major) and W = trace((wfac (grad u - F0))^T sig) into w
+// (Linear/Isotropic.H:31-34 and Linear/Cubic.H:60-63 with wfac = 1/2,
+// Affine/Isotropic.H:31-34 with wfac = 1).  F0 null: zero; w / dw may be null.
+void or_newton_w_dw(const or_grid* og, int kind, const double* C, const double* F0, const double* u, double wfac,
+                    double* w, double* dw) {
+    G g(og);
+    const int nc = ncomp_of(g.dim, kind), dim = g.dim;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < g.nn[2]; ++k)
+        for (int j = 0; j < g.nn[1]; ++j)
+            for (int i = 0; i < g.nn[0]; ++i) {
+                double gu[3][3] = {}, s[3][3] = {}, c[21];
+                const long q = g.id(i, j, k);
+                for (int p = 0; p < dim; ++p)
+                    for (int d = 0; d < dim; ++d)
+                        gu[p][d] = d1(g, u, p, i, j, k, d) - (F0 ? F0[(p * dim + d) * g.nodes + q] : 0.0);
+                load_c(g, nc, C, i, j, k, c);
+                contract2(dim, kind, c, gu, s);
+                if (dw)
+                    for (int a = 0; a < dim; ++a)
+                        for (int b = 0; b < dim; ++b) dw[(size_t)(a * dim + b) * g.nodes + q] = s[a][b];
+                if (w) {
+                    double tr = 0.0;
+                    for (int jj = 0; jj < dim; ++jj)  // (A^T B)_jj = sum_i A_ij B_ij, A = wfac grad u
+                        for (int ii = 0; ii < dim; ++ii) tr += wfac * gu[ii][jj] * s[ii][jj];
+                    w[q] = tr;
+                }
+            }
+}
+
 // normalize (Operator.cpp:164-195): the ParallelFor lambda indexes neither x
 // nor diag by component, so x(i,j,k,0) /= diag(i,j,k,0) once per component.
 void or_normalize(const or_grid* og, const double* diag, double* x) {

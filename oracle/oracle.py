@@ -77,6 +77,7 @@ def lib():
         L.or_normalize.argtypes = [P(_Grid), vp, vp]
         L.or_post.argtypes = [P(_Grid), i, vp, vp, i, i, vp]
         L.or_newton_rhs.argtypes = [P(_Grid), i, vp, vp, vp, vp, vp, vp]
+        L.or_newton_w_dw.argtypes = [P(_Grid), i, vp, vp, vp, d, vp, vp]
         _lib = L
     return _lib
 
@@ -188,6 +189,76 @@ def newton_rhs(g, kind, C, bct, u, b, F0=None):
     lib().or_newton_rhs(ctypes.byref(g.c()), kind, _p(_c(C)), None if f0 is None else _p(f0),
                         _p(_c(bct, np.int32)), _p(_c(u)), _p(_c(b)), _p(out))
     return out
+
+
+def newton_w_dw(g, kind, C, u, F0=None, wfac=0.5):
+    """Newton<T>::W and DW (Newton.H:233-333): (W per node, dw (dim*dim, *nodes))."""
+    w = np.zeros(g.shape(1))[0]
+    dw = np.zeros(g.shape(g.dim * g.dim))
+    f0 = None if F0 is None else _c(F0)
+    lib().or_newton_w_dw(ctypes.byref(g.c()), kind, _p(_c(C)), None if f0 is None else _p(f0), _p(_c(u)),
+                         ctypes.c_double(wfac), _p(w), _p(dw))
+    return w, dw
+
+
+def _cell(eta, dim, i, j, k):
+    """Cell values at cell index (i, j, k) of cells -1..n (one ghost layer): eta[c, k+1, j+1, i+1]."""
+    if dim == 3:
+        return eta[:, k + 1, j + 1, i + 1]
+    return eta[:, j + 1, i + 1]
+
+
+def degrade_material_nodes(g, eta_cells):
+    """The node value PolymerDegradation::DegradeMaterial hands to DegradeModulus
+    (PolymerDegradation.cpp:817-828): mul * (sum of the 2^dim cells around the node) in the reference's
+    order, mul = 1/(2+2) in 2D and 1/(2+2+4) in 3D.  eta_cells: (ncomp, *(n + 2)) with one ghost layer.
+    Returns (ncomp, *nodes).  Pure numpy (small grids)."""
+    dim = g.dim
+    I = np.arange(g.n[0] + 1)
+    J = np.arange(g.n[1] + 1)
+    if dim == 3:
+        K = np.arange(g.n[2] + 1)
+        e = lambda di, dj, dk: eta_cells[:, (K + dk + 1)[:, None, None], (J + dj + 1)[None, :, None],
+                                         (I + di + 1)[None, None, :]]
+        t = e(0, 0, 0) + e(-1, 0, 0)
+        t = t + e(0, -1, 0) + e(-1, -1, 0)
+        t = t + e(0, 0, -1) + e(-1, 0, -1) + e(0, -1, -1) + e(-1, -1, -1)
+        return (1.0 / 8.0) * t
+    e = lambda di, dj: eta_cells[:, (J + dj + 1)[:, None], (I + di + 1)[None, :]]
+    t = e(0, 0) + e(-1, 0)
+    t = t + e(0, -1) + e(-1, -1)
+    return (1.0 / 4.0) * t
+
+
+def pfm_eta_nodes(g, eta_cells):
+    """PhaseFieldMicrostructure's node average (PFM.cpp:537): 0.25 (eta(i,j,k) + eta(i,j-1,k) + eta(i-1,j,k)
+    + eta(i-1,j-1,k)), the x-y cells at cell layer k, also in 3D."""
+    I = np.arange(g.n[0] + 1)
+    J = np.arange(g.n[1] + 1)
+    if g.dim == 3:
+        K = np.arange(g.n[2] + 1)
+        e = lambda di, dj: eta_cells[:, (K + 1)[:, None, None], (J + dj + 1)[None, :, None], (I + di + 1)[None, None, :]]
+    else:
+        e = lambda di, dj: eta_cells[:, (J + dj + 1)[:, None], (I + di + 1)[None, :]]
+    return 0.25 * (e(0, 0) + e(0, -1) + e(-1, 0) + e(-1, -1))
+
+
+def combine(ddw, F0, etas):
+    """Affine::Cubic::Combine (Affine/Cubic.H:70-94) per node: ddw (nph, nc), F0 (nph, d, d) or None,
+    etas (nph, *nodes).  Returns (C (nc, *nodes), F0 (d*d, *nodes) or None)."""
+    etasum = np.zeros(etas.shape[1:])
+    for n in range(etas.shape[0]):
+        etasum = etasum + etas[n]
+    C = np.zeros((ddw.shape[1],) + etas.shape[1:])
+    F = None if F0 is None else np.zeros((F0.shape[1] * F0.shape[2],) + etas.shape[1:])
+    for n in range(etas.shape[0]):
+        w = etas[n] / etasum
+        for q in range(ddw.shape[1]):
+            C[q] = C[q] + ddw[n, q] * w
+        if F is not None:
+            for q in range(F.shape[0]):
+                F[q] = F[q] + F0[n].reshape(-1)[q] * w
+    return C, F
 
 
 def normalize(g, diag, x):

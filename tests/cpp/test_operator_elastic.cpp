@@ -209,6 +209,166 @@ static int combine_check() {
     return failed;
 }
 
+// The rest of the operator interface (Elastic.H:48-136, Operator.H:79-100, Newton.H:211-333) on known
+// answers.  Uniform isotropic (lambda, mu), default BCs (Displacement 0 on every class), u = G x.
+static int interface_known_answers() {
+    using M = Model::Solid::Linear::Isotropic;
+    const int n = 8, nn = (n + 1) * (n + 1) * (n + 1);
+    const double mu = 1.3, lambda = 2.1, h = 1.0 / n;
+    const double G[3][3] = {{0.01, 0.002, -0.003}, {0.004, -0.02, 0.001}, {0.0, 0.005, 0.015}};
+    Geometry g;
+    g.n = {n, n, n};
+    Operator::Elastic<M> op(g);
+    BC::Operator::Elastic<M> bc;
+    op.SetModel(M(mu, lambda));
+    op.SetBC(&bc);
+    op.SetUniform(false);
+    op.prepareForSolve();
+    int failed = 0;
+    auto report = [&](const char* what, double err, double tol) {
+        const int f = !(err <= tol);
+        failed += f;
+        std::printf("  %-44s max deviation = %.3e %s\n", what, err, f ? "FAIL" : "PASS");
+    };
+    auto id = [&](int i, int j, int k, int m) { return i + (size_t)(m + 1) * (j + (size_t)(m + 1) * k); };
+    std::vector<double> u(3 * (size_t)nn);
+    for (int k = 0; k <= n; ++k)
+        for (int j = 0; j <= n; ++j)
+            for (int i = 0; i <= n; ++i) {
+                const double x[3] = {i * h, j * h, k * h};
+                for (int p = 0; p < 3; ++p) u[p * (size_t)nn + id(i, j, k, n)] = G[p][0] * x[0] + G[p][1] * x[1] + G[p][2] * x[2];
+            }
+    auto wall = [&](int i, int j, int k, int m) { return i == 0 || j == 0 || k == 0 || i == m || j == m || k == m; };
+    std::printf("Operator interface known answers (%d^3, uniform isotropic, u = G x):\n", n);
+    report("getNComp / isCrossStencil / getNGrow", std::fabs(op.getNComp() - 3.0) + op.isCrossStencil() +
+                                                     std::fabs(op.getNGrow() - 2.0), 0.0);
+    op.SetHomogeneous(true);
+    report("SetHomogeneous stores the flag", op.GetHomogeneous() ? 0.0 : 1.0, 0.0);
+    // Fapply: interior rows of an affine field vanish, Displacement rows return u
+    std::vector<double> f;
+    op.Fapply(0, 0, f, u);
+    double e = 0;
+    for (int k = 0; k <= n; ++k)
+        for (int j = 0; j <= n; ++j)
+            for (int i = 0; i <= n; ++i)
+                for (int p = 0; p < 3; ++p) {
+                    const size_t q = p * (size_t)nn + id(i, j, k, n);
+                    e = std::fmax(e, std::fabs(f[q] - (wall(i, j, k, n) ? u[q] : 0.0)));
+                }
+    report("Fapply(amrlev, mglev, out, in)", e, 1e-12);
+    // residuals of the exact solution
+    std::vector<double> r;
+    const double rn = op.solutionResidual(0, r, u, f);
+    e = rn;
+    for (double v : r) e = std::fmax(e, std::fabs(v));
+    report("solutionResidual of the exact solution", e, 1e-12);
+    // Diagonal: 1 on Displacement rows, -2/h^2 (C_pppp + sum_{d != p} C_pdpd) inside
+    std::vector<double> dg;
+    op.Diagonal(0, 0, dg);
+    e = 0;
+    for (int p = 0; p < 3; ++p) {
+        e = std::fmax(e, std::fabs(dg[p * (size_t)nn + id(0, 3, 5, n)] - 1.0));
+        const double want = -2.0 / (h * h) * ((lambda + 2 * mu) + 2 * mu);
+        e = std::fmax(e, std::fabs(dg[p * (size_t)nn + id(4, 4, 4, n)] - want) / std::fabs(want));
+    }
+    report("Diagonal(amrlev, mglev, diag)", e, 1e-14);
+    // Fsmooth: the exact solution is a fixed point of the Jacobi sweeps
+    std::vector<double> x = u;
+    op.Fsmooth(0, 0, x, f);
+    e = 0;
+    for (size_t q = 0; q < x.size(); ++q) e = std::fmax(e, std::fabs(x[q] - u[q]));
+    report("Fsmooth keeps the exact solution", e, 1e-15);
+    // normalize: component 0 divided by diagonal component 0 once per component (3 times)
+    std::vector<double> y = u;
+    op.normalize(0, 0, y);
+    e = 0;
+    for (int q = 0; q < nn; ++q) {
+        const double d0 = dg[q];
+        e = std::fmax(e, std::fabs(y[q] - u[q] / d0 / d0 / d0) / (std::fabs(u[q] / d0 / d0 / d0) + 1e-300));
+        e = std::fmax(e, std::fabs(y[nn + q] - u[nn + q]) + std::fabs(y[2 * nn + q] - u[2 * nn + q]));
+    }
+    report("normalize (the reference's component indexing)", e, 1e-15);
+    // restriction / interpolation reproduce affine fields exactly
+    const int nc = n / 2, nnc = (nc + 1) * (nc + 1) * (nc + 1);
+    std::vector<double> crse;
+    std::vector<double> fine = u;
+    op.restriction(0, 1, crse, fine);
+    e = 0;
+    for (int k = 0; k <= nc; ++k)
+        for (int j = 0; j <= nc; ++j)
+            for (int i = 0; i <= nc; ++i)
+                for (int p = 0; p < 3; ++p)
+                    e = std::fmax(e, std::fabs(crse[p * (size_t)nnc + id(i, j, k, nc)] - u[p * (size_t)nn + id(2 * i, 2 * j, 2 * k, n)]));
+    report("restriction of an affine field", e, 1e-15);
+    std::vector<double> back(u.size(), 0.0);
+    op.interpolation(0, 0, back, crse);
+    e = 0;
+    for (size_t q = 0; q < u.size(); ++q) e = std::fmax(e, std::fabs(back[q] - u[q]));
+    report("interpolation of an affine field", e, 1e-15);
+    // Strain / Stress / Energy
+    double sig[3][3], eps[3][3], W = 0;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            eps[a][b] = 0.5 * (G[a][b] + G[b][a]);
+            sig[a][b] = mu * (G[a][b] + G[b][a]) + (a == b ? lambda * (G[0][0] + G[1][1] + G[2][2]) : 0.0);
+        }
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) W += 0.5 * G[a][b] * sig[a][b];
+    std::vector<double> st, ss, en;
+    op.Strain(0, st, u);
+    op.Stress(0, ss, u);
+    op.Energy(0, en, u);
+    e = 0;
+    for (int q = 0; q < nn; q += 37)
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                e = std::fmax(e, std::fabs(st[(a * 3 + b) * (size_t)nn + q] - eps[a][b]));
+                e = std::fmax(e, std::fabs(ss[(a * 3 + b) * (size_t)nn + q] - sig[a][b]) / 0.1);
+            }
+    for (int q = 0; q < nn; q += 37) e = std::fmax(e, std::fabs(en[q] - 2 * W) / (2 * W));
+    report("Strain / Stress / Energy", e, 1e-12);
+    // Newton W / DW / compResidual
+    std::vector<M> model(nn, M(mu, lambda));
+    Solver::Nonlocal::Newton<M> newton(op);
+    std::vector<double> w, dw, res, zero(u.size(), 0.0);
+    newton.W(w, u, model);
+    newton.DW(dw, u, model);
+    newton.compResidual(res, u, zero, model);
+    e = 0;
+    for (int q = 0; q < nn; ++q) {
+        e = std::fmax(e, std::fabs(w[q] - W) / W);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) e = std::fmax(e, std::fabs(dw[(a * 3 + b) * (size_t)nn + q] - sig[a][b]) / 0.1);
+    }
+    for (int k = 0; k <= n; ++k)
+        for (int j = 0; j <= n; ++j)
+            for (int i = 0; i <= n; ++i)
+                for (int p = 0; p < 3; ++p) {
+                    const size_t q = p * (size_t)nn + id(i, j, k, n);
+                    e = std::fmax(e, std::fabs(res[q] - (wall(i, j, k, n) ? -u[q] : 0.0)));
+                }
+    report("Newton W / DW / compResidual", e, 1e-12);
+    // reflux over two AMR levels: a residual of ones on the fine box restricts to 1 inside and to the
+    // zero-ghost weights on the coarsened box's faces, edges and corners
+    {
+        const int n0 = 16;
+        Geometry g2;
+        g2.n = {n0, n0, n0};
+        const std::vector<Box> fb = test_boxes(n0, 2, false);
+        Operator::Elastic<M> amr(g2, fb);
+        std::vector<double> cres(3 * (size_t)amr.levelSize(0), -7.0), fres(3 * (size_t)amr.levelSize(1), 1.0), none;
+        amr.Reflux(0, cres, none, none, fres, none, none);
+        const auto cn = amr.levelNodes(0);
+        auto at = [&](int i, int j, int k) { return cres[i + (size_t)cn[0] * (j + (size_t)cn[1] * k)]; };
+        const int lo = fb[0].lo[0] / 2, hi = fb[0].hi[0] / 2, mid = (lo + hi) / 2;
+        e = std::fabs(at(mid, mid, mid) - 1.0) + std::fabs(at(lo, mid, mid) - 0.75) +
+            std::fabs(at(lo, lo, mid) - 0.5625) + std::fabs(at(lo, lo, lo) - 27.0 / 64.0) +
+            std::fabs(at(hi, hi, hi) - 27.0 / 64.0) + std::fabs(at(lo - 1, mid, mid) + 7.0);
+        report("Reflux (partition of unity, zero ghosts)", e, 1e-15);
+    }
+    return failed;
+}
+
 static int uniaxial(int n, bool mixed) {
     using model_type = Model::Solid::Linear::Isotropic;
     model_type model(2.6, 6.0);  // (mu, lambda) as the reference passes (lame, shear)
@@ -310,6 +470,7 @@ int main() {
     failed += trig_amr(32, 3);
     failed += affine_free_expansion(16);
     failed += combine_check();
+    failed += interface_known_answers();
     std::printf("%d tests failed\n", failed);
     return failed;
 }
