@@ -125,6 +125,7 @@ SIGNATURES = {
     "ela_restriction": ([_P, _I, _I, _P, _P, _P], _I),
     "ela_interpolation": ([_P, _I, _I, _P, _P, _P], _I),
     "ela_residual": ([_P, _I, _I, _P, _P, _P, ctypes.POINTER(_D), _P], _I),
+    "ela_residual_noalloc": ([_P, _I, _I, _P, _P, _P, ctypes.POINTER(_D), _P], _I),
     "ela_bc_init": ([_P, _P, _I], _I),
     "ela_solve": ([_P, _P, _P, _D, _D, _I, _I, ctypes.POINTER(Stats)], _I),
     "ela_solve_device": ([_P, _P, _P, _D, _D, _I, _I, ctypes.POINTER(Stats)], _I),

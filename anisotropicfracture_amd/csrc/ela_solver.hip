@@ -34,6 +34,32 @@
 
 using namespace ela;
 
+// ---------------------------------------------------------------- allocation-free residual
+// Scratch of ela_residual_noalloc: kScratchPartials per-block maxima followed
+// by the norm word, one copy per GPU, held until the process ends.  Host code
+// never passes the array's name to a kernel or a copy: it is a host-side
+// stand-in there.  g_scratch[device] is the device address from
+// hipGetSymbolAddress, taken once per device with that device current.
+constexpr long long kScratchPartials = 1LL << 17;  // 1 MiB of doubles
+static __device__ __align__(256) double scratch[kScratchPartials + 32];
+namespace {
+constexpr int kMaxDevices = 64;
+double* g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return ela::set_error(ELA_ERR_BADARG, "device index too large");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch[device]) return ELA_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return ela::set_error(ELA_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = (double*)p;
+    return ELA_OK;
+}
+}  // namespace
+
 namespace {
 thread_local std::string g_err;
 }  // namespace
@@ -233,9 +259,10 @@ template <typename T>
 void grain_jacobi_to64(ela_hier* h, Level& v, const T* dg, const T* xin, const T* b, const T* hold, double* out64,
                        hipStream_t s);
 bool grain_on(ela_hier* h, const Level& v);
+// part: scratch for the per-block maxima of the norm (null: the hierarchy's own).
 template <typename T, typename TO>
 void A_residual(ela_hier* h, Level& v, const T* C, const T* x, const T* b, T* r, TO* rs, const double* scale,
-                unsigned long long* norm, hipStream_t s);
+                unsigned long long* norm, hipStream_t s, double* part = nullptr);
 
 // Kernel timing around level-0 launches.
 int ev_begin(ela_hier* h) {
@@ -973,7 +1000,8 @@ void grain_jacobi_to64(ela_hier* h, Level& v, const T* dg, const T* xin, const T
 
 template <typename T, typename TO>
 void A_residual(ela_hier* h, Level& v, const T* C, const T* x, const T* b, T* r, TO* rs, const double* scale,
-                unsigned long long* norm, hipStream_t s) {
+                unsigned long long* norm, hipStream_t s, double* part) {
+    double* const P = part ? part : h->partials;
     const bool plain = !rs && !norm;
     if (grain_on(h, v) && (plain || sizeof(T) == 8)) {
         GrainBox<T> g = grain_box<T>(h, v);
@@ -986,11 +1014,11 @@ void A_residual(ela_hier* h, Level& v, const T* C, const T* x, const T* b, T* r,
             g.out = rs;
             g.out2 = (double*)r;
             g.scale = scale;
-            grain_resid64<T, TO>(g, level_kz(h, v), h->partials, norm, s);
+            grain_resid64<T, TO>(g, level_kz(h, v), P, norm, s);
         }
-    } else if (table_on(h, v)) launch_residual_tab<T, TO>(v.L, tab_of<T>(h), x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
-    else if (coarse_tab_on(h, v)) launch_residual_tab<T, TO>(v.L, coarse_tab<T>(h, v), x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s);
-    else launch_residual<T, TO>(v.L, h->kind, C, x, b, r, rs, scale, h->partials, norm, h->bc, v.h, h->uniform, s,
+    } else if (table_on(h, v)) launch_residual_tab<T, TO>(v.L, tab_of<T>(h), x, b, r, rs, scale, P, norm, h->bc, v.h, h->uniform, s);
+    else if (coarse_tab_on(h, v)) launch_residual_tab<T, TO>(v.L, coarse_tab<T>(h, v), x, b, r, rs, scale, P, norm, h->bc, v.h, h->uniform, s);
+    else launch_residual<T, TO>(v.L, h->kind, C, x, b, r, rs, scale, P, norm, h->bc, v.h, h->uniform, s,
                                 plane_opt<T>(h, v));
 }
 
@@ -1062,6 +1090,7 @@ int ela_ctx_create(int device, size_t hbm_budget_bytes, ela_ctx** out) {
     HIPCHK(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(ELA_ERR_BADARG, "no such device");
     HIPCHK(hipSetDevice(device));
+    if (int rc = scratch_init(device)) return rc;
     ela_ctx* c = new ela_ctx();
     c->device = device;
     c->budget = hbm_budget_bytes;
@@ -1785,6 +1814,38 @@ int ela_residual(ela_hier* h, int mglev, int prec, const void* x, const void* b,
         A_residual<float, float>(h, v, v.C32, (const float*)x, (const float*)b, (float*)r, (float*)nullptr, nullptr, h->dnorm, s);
     unsigned long long bits = 0;
     HIPCHK(hipMemcpyAsync(&bits, h->dnorm, sizeof bits, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (norm_inf) std::memcpy(norm_inf, &bits, sizeof(double));
+    return ELA_OK;
+}
+
+int ela_residual_noalloc(ela_hier* h, int mglev, int prec, const void* x, const void* b, void* r, double* norm_inf,
+                         void* stream) {
+    if (int rc = check_level(h, mglev)) return rc;
+    if (int rc = check_prec(h, prec)) return rc;
+    if (!x || !b) return fail(ELA_ERR_BADARG, "null field");
+    if (!h->prepared && (mglev > 0 || prec == ELA_F32)) return fail(ELA_ERR_STATE, "call ela_prepare first");
+    Level& v = h->lev[mglev];
+    long long need = residual_partials(v.L);
+    if (v.L.dim == 3) need = std::max(need, grain_partials(v.L, level_kz(h, v)));
+    if (need > kScratchPartials)
+        return fail(ELA_ERR_BADARG, "ela_residual_noalloc: the level needs " + std::to_string(need) +
+                                        " block maxima, more than the " + std::to_string(kScratchPartials) +
+                                        " of the static scratch (use ela_residual)");
+    const int dev = h->ctx->device;
+    double* part = (dev >= 0 && dev < kMaxDevices) ? g_scratch[dev] : nullptr;
+    if (!part) return fail(ELA_ERR_STATE, "scratch not set up for this device (ela_ctx_create does it)");
+    unsigned long long* nw = (unsigned long long*)(part + kScratchPartials);
+    hipStream_t s = strm(h, stream);
+    if (prec == ELA_F64)
+        A_residual<double, double>(h, v, v.C64, (const double*)x, (const double*)b, (double*)r, (double*)nullptr,
+                                   nullptr, nw, s, part);
+    else
+        A_residual<float, float>(h, v, v.C32, (const float*)x, (const float*)b, (float*)r, (float*)nullptr, nullptr,
+                                 nw, s, part);
+    HIPCHK(hipGetLastError());
+    unsigned long long bits = 0;
+    HIPCHK(hipMemcpyAsync(&bits, nw, sizeof bits, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (norm_inf) std::memcpy(norm_inf, &bits, sizeof(double));
     return ELA_OK;

@@ -261,6 +261,21 @@ int ela_interpolation(ela_hier* h, int fmglev, int prec, void* fine, const void*
 int ela_residual(ela_hier* h, int mglev, int prec, const void* x, const void* b, void* r, double* norm_inf,
                  void* stream);
 
+/* ela_residual without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels, arguments and results as ela_residual, bit for bit; the per-block
+ * maxima of the norm and the norm word live in a file-scope device array
+ * (1 MiB + 256 B per GPU, set up by ela_ctx_create and held until the process
+ * ends) instead of the hierarchy's buffers.  A level needing more than 131072
+ * block maxima (more nodes than a 257^3 level) returns ELA_ERR_BADARG.  Every
+ * call on a GPU shares that GPU's array: calls that overlap on one GPU (other
+ * host threads, other streams) race, and must be serialized by the caller.
+ * Nothing in the array needs to start at zero: every call writes each block
+ * maximum it reads. */
+int ela_residual_noalloc(ela_hier* h, int mglev, int prec, const void* x, const void* b, void* r, double* norm_inf,
+                         void* stream);
+
 /* BC::Operator::Elastic::Init (src/BC/Operator/Elastic.H:106-181): write the
  * prescribed boundary values into a dense host rhs of level 0. */
 int ela_bc_init(ela_hier* h, double* rhs_host_dense, int homogeneous);
