@@ -178,6 +178,7 @@ SIGNATURES = {
                    ctypes.POINTER(ServeSummary)], _I),
     "ela_batch_smoother_times": ([_P, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_I)], _I),
     # composite (AMR) solve
+    "ela_amr_create_boxes": ([_P, ctypes.POINTER(_Domain), _I, _P, _P, _I, _I, _I, ctypes.POINTER(_P)], _I),
     "ela_amr_create": ([_P, ctypes.POINTER(_Domain), _I, _P, _I, _I, _I, ctypes.POINTER(_P)], _I),
     "ela_amr_destroy": ([_P], _I),
     "ela_amr_bytes": ([_P], _SZ),
@@ -689,10 +690,12 @@ class ElasticAMR:
     """Operator::Elastic<T> defined on several AMR levels (Elastic.cpp:30-55) with the multi-level
     Solver::Nonlocal::Linear::solve (Linear.H:102-112).
 
-    n0: level-0 cells per direction; boxes: [(lo, hi), ...] for levels 1.. (node indices of the
-    level, ratio 2, even corners).  Every per-level host array covers the level's box grown by 2
-    nodes and clipped to the domain (``shape(lev)``; ``origin(lev)`` is the global node index of
-    its first node); the outer nodes hold the interpolated coarser solution.
+    n0: level-0 cells per direction; boxes: one entry per level 1.., either a box (lo, hi) or a list of
+    boxes [(lo, hi), ...] whose union is the level (node indices of the level, ratio 2, even corners;
+    e.g. a region chopped into boxes of at most max_grid_size cells, or an L shape).  Every per-level
+    host array covers the bounding box of the level's union grown by 2 nodes and clipped to the domain
+    (``shape(lev)``; ``origin(lev)`` is the global node index of its first node); nodes outside the
+    union and on its coarse/fine boundary hold the interpolated coarser solution.
     """
 
     U, B, R = 0, 1, 2
@@ -708,13 +711,17 @@ class ElasticAMR:
             d.n[q] = n0[q] if q < self.dim else 0
             d.dx[q] = float(L[q]) / n0[q] if q < self.dim else 1.0
         self.nlev = len(boxes) + 1
-        bx = np.zeros(6 * max(1, len(boxes)), dtype=np.int32)
-        for l, (lo, hi) in enumerate(boxes):
-            bx[6 * l:6 * l + self.dim] = lo
-            bx[6 * l + 3:6 * l + 3 + self.dim] = hi
+        lists = [[b] if np.ndim(b[0]) == 1 else list(b) for b in boxes]
+        counts = np.array([len(b) for b in lists] or [0], dtype=np.int32)
+        flat = [bb for b in lists for bb in b]
+        bx = np.zeros(6 * max(1, len(flat)), dtype=np.int32)
+        for q, (lo, hi) in enumerate(flat):
+            bx[6 * q:6 * q + self.dim] = lo
+            bx[6 * q + 3:6 * q + 3 + self.dim] = hi
+        self.boxes = lists
         p = ctypes.c_void_p()
-        _check(lib().ela_amr_create(self.ctx.ptr, ctypes.byref(d), self.nlev, _ptr(bx), kind, coarse_min, mode,
-                                    ctypes.byref(p)))
+        _check(lib().ela_amr_create_boxes(self.ctx.ptr, ctypes.byref(d), self.nlev, _ptr(counts), _ptr(bx), kind,
+                                          coarse_min, mode, ctypes.byref(p)))
         self.ptr = p
         self.kind, self.mode = kind, mode
         self.nc = ncomp(self.dim, kind)

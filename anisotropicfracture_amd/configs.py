@@ -165,16 +165,39 @@ def polycrystal(n=256, grains=64, seed=SEED):
 
 
 # ------------------------------------------------------------------ AMR crack (composite solve)
-def crack_nest_boxes(n=128):
-    """The refined boxes of the mode-I crack on an n^3 base (ratio 2, node indices of each level).
+def chop_box(lo, hi, max_grid_size):
+    """A box of nodes [lo, hi] cut into boxes of at most max_grid_size cells per direction, the pieces of a
+    direction as equal as even corners allow (the role of AMReX's BoxArray::maxSize): e.g. 130 cells at
+    32 give 5 pieces of 26."""
+    cuts = []
+    for a, b in zip(lo, hi):
+        cells = b - a
+        npiece = -(-cells // max_grid_size)
+        pairs = cells // 2
+        edges = [a + 2 * ((q * pairs) // npiece) for q in range(npiece)] + [b]
+        cuts.append(edges)
+    out = []
+    for idx in np.ndindex(*[len(c) - 1 for c in cuts]):
+        out.append((tuple(cuts[d][idx[d]] for d in range(len(lo))), tuple(cuts[d][idx[d] + 1] for d in range(len(lo)))))
+    return out
 
-    Level 1 covers |z - 0.5| <= 0.125, full x and y.  Level 2 covers |z - 0.5| <= 0.0625 and the
-    crack front |x - 0.4| <= 0.125, its x range rounded out to [0.25, 0.5625] so that the box spans a
-    whole number of 32-cell blocks at n = 128 (160 cells); full y.
+
+def crack_nest_boxes(n=128, max_grid_size=None):
+    """The refined regions of the mode-I crack on an n^3 base (ratio 2, node indices of each level).
+
+    Level 1 covers |z - 0.5| <= 0.125, full x and y.  Level 2 covers |z - 0.5| <= 0.0625 and the crack
+    front |x - 0.4| <= 0.125 (x rounded out to even nodes), full y.  With max_grid_size each region is a
+    list of boxes of at most that many cells per direction (uneven where a region is not a multiple of
+    it); without, one box per level.
     """
+    n2 = 4 * n
+    x0 = int(np.floor(0.275 * n2 / 2)) * 2
+    x1 = int(np.ceil(0.525 * n2 / 2)) * 2
     l1 = ((0, 0, 3 * n // 4), (2 * n, 2 * n, 5 * n // 4))
-    l2 = ((n, 0, 7 * n // 4), (9 * n // 4, 4 * n, 9 * n // 4))
-    return [l1, l2]
+    l2 = ((x0, 0, 7 * n // 4), (x1, 4 * n, 9 * n // 4))
+    if max_grid_size is None:
+        return [l1, l2]
+    return [chop_box(l1[0], l1[1], max_grid_size), chop_box(l2[0], l2[1], max_grid_size)]
 
 
 def crack_eta(X, Y, Z, width=0.01, z0=0.5, x_tip=0.4):
@@ -201,7 +224,7 @@ G_LITERAL = (1.0, 0.0, 1e-4)          # eta^2 + 1e-4: soft bulk, stiff crack pla
 G_DAMAGE = (1.0, -2.0, 1.0 + 1e-4)    # (1 - eta)^2 + 1e-4: eta as damage (Degradable/Isotropic.H:78-79)
 
 
-def crack_nest(n=128, g=G_DAMAGE, E=1.0, nu=0.3, width=0.01, load=0.005):
+def crack_nest(n=128, g=G_DAMAGE, E=1.0, nu=0.3, width=0.01, load=0.005, max_grid_size=32):
     """3D AMR mode-I crack: base n^3 cells on [0,1]^3, 3 levels (ratio 2).
 
     Stiffness g(eta) C_iso(E, nu) per node with g(eta) = a eta^2 + b eta + c.  The default is the
@@ -213,7 +236,8 @@ def crack_nest(n=128, g=G_DAMAGE, E=1.0, nu=0.3, width=0.01, load=0.005):
     """
     lam, mu = lame(E, nu)
     t, v = mode1_bcs(load)
-    return {"n": (n, n, n), "boxes": crack_nest_boxes(n), "lame": (lam, mu), "g": tuple(g), "width": width,
+    return {"n": (n, n, n), "boxes": crack_nest_boxes(n, max_grid_size), "regions": crack_nest_boxes(n),
+            "max_grid_size": max_grid_size, "lame": (lam, mu), "g": tuple(g), "width": width,
             "bc_type": t, "bc_value": v}
 
 

@@ -7,29 +7,40 @@
 // hold the interpolated coarse solution (the ghost-node coarse/fine strategy
 // selected at Linear.H:23-29).
 //
-// Levels.  Level 0 is the domain.  Level l >= 1 is one box B_l of level-l
-// nodes [lo, hi] (even corners, 2 coarse cells inside its parent unless on the
-// domain boundary), stored on G_l = B_l grown by 2 nodes, clipped to the
-// domain.  Node classes of level l:
-//   slave     on a face of B_l away from the domain boundary;
-//   ghost     in G_l outside B_l;
-//   covered   strictly inside B_{l+1} / 2 (away from its coarse/fine faces);
-//   interface on a coarse/fine face of B_{l+1} / 2;
-//   unknown   every other node of B_l (interface nodes included).
-// Slaves and ghosts take the trilinear interpolation of level l-1; covered
-// nodes the coincident level-(l+1) value.  The equations are b - A_l u = 0 at
-// unknown nodes (A_l the single-box operator on G_l with the domain's boundary
-// rows on domain faces) and, at interface nodes, the reflux of level l+1's
-// residual = 0 -- the problem restated by the oracle (oracle/amr.py).
+// Levels.  Level 0 is the domain.  Level l >= 1 is the union of a list of
+// boxes of level-l nodes [lo, hi] (even corners; the boxes may share faces and
+// their union need not be a box, e.g. an L shape, or one region chopped into
+// boxes of at most max_grid_size cells).  On one GPU the boxes of a level share
+// one array over G_l, the bounding box of the union grown by 2 nodes and clipped
+// to the domain: a node shared by two boxes is stored once, so AMReX's
+// box-to-box halo fills and nodalSync are identities, and every kernel of a
+// level is one launch over G_l.  Node classes of level l, built on the host
+// from the boxes and kept as one byte per node of G_l:
+//   fill      not in the union, or on the union's boundary away from the domain
+//             boundary (a slave: some cell touching it is in the domain but not
+//             in the union);
+//   covered   every in-domain cell touching it lies under level l+1;
+//   interface some but not all of those cells lie under level l+1;
+//   unknown   every other node of the union (interface nodes included).
+// Fill nodes take the trilinear interpolation of level l-1; covered nodes the
+// coincident level-(l+1) value.  The equations are b - A_l u = 0 at unknown
+// nodes (A_l the operator on G_l with the domain's boundary rows on domain
+// faces) and, at interface nodes, the reflux of level l+1's residual = 0 --
+// the problem restated by the oracle (oracle/amr.py).
 //
 // Solver.  Flexible GMRES(m) in FP64 on the composite unknowns, right-
-// preconditioned by one multiplicative FAC cycle: from the finest level down,
-// one multigrid cycle of the level's box hierarchy (homogeneous Dirichlet on
-// its coarse/fine faces) on the level's residual, whose correction's residual
-// is restricted into the coarser level under the box; the level-0 hierarchy's
-// cycle; then upwards, the coarse correction interpolated onto the finer level
-// (ghosts and slaves included) and one more box cycle on the residual left.
-// Box cycles run in FP32 (mixed) or FP64.
+// preconditioned by one multiplicative FAC cycle over the AMR levels: from the
+// finest level down, a correction of the level's residual equation from zero,
+// whose residual is restricted into the coarser level under the union; the
+// level-0 multigrid cycle; then upwards, the coarse correction interpolated
+// onto the finer level (every node) and one more correction.  A level whose
+// union is one box (level 0, or a region chopped into max_grid_size boxes)
+// corrects with a V(1,1) cycle of a multigrid hierarchy on that box
+// (homogeneous Dirichlet rows on its coarse/fine faces); any other union
+// smooths, as AMReX's MLMG does on fine AMR levels: nu1 / nu2 Fsmooth calls (2
+// weighted-Jacobi sweeps each, omega = 2/3, on the level's non-fill nodes, fill
+// nodes held).  Hierarchy cycles run in FP32 (mixed) or FP64, the smoothing and
+// everything else in FP64.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,35 +60,30 @@ namespace {
 constexpr int CX = 64, CY = 4;
 enum { NODE_UNKNOWN = 0, NODE_FILL = 1, NODE_COVERED = 2 };
 
+// Node-class bits of the per-level mask (one byte per node of G).
+enum { CLS_FILL = 1, CLS_COVERED = 2, CLS_IFACE = 4, CLS_UNDER = 8 };
+
 // Geometry of one level (global node indices of the level).
 struct Geo {
     int dim;
-    int glo[3];               // first node of G
-    int lo[3], hi[3];         // B
-    int N[3];                 // domain cells of the level
-    int cflo[3], cfhi[3];     // 1: that face of B is a coarse/fine face
+    int glo[3];                 // first node of G
+    int gn[3];                  // nodes of G per direction
+    int N[3];                   // domain cells of the level
     int has_fine;
-    int flo[3], fhi[3];       // B_{l+1} / 2
-    int fflo[3], ffhi[3];     // 1: that face of B_{l+1} / 2 is a coarse/fine face
+    int ulo[3], uhi[3];         // bounding box of the nodes under level l+1 (has_fine)
+    const unsigned char* cls;   // class bits per node of G (dense, x fastest)
 };
 
+__device__ __forceinline__ unsigned char cls_at(const Geo& g, const int p[3]) {
+    const int i = p[0] - g.glo[0], j = p[1] - g.glo[1], k = g.dim == 3 ? p[2] - g.glo[2] : 0;
+    return g.cls[i + (long long)g.gn[0] * (j + (long long)g.gn[1] * k)];
+}
+
 __device__ __forceinline__ int node_class(const Geo& g, const int p[3], bool* iface) {
-    *iface = false;
-    bool slave = false;
-    for (int d = 0; d < g.dim; ++d) {
-        if (p[d] < g.lo[d] || p[d] > g.hi[d]) return NODE_FILL;
-        slave = slave || (g.cflo[d] && p[d] == g.lo[d]) || (g.cfhi[d] && p[d] == g.hi[d]);
-    }
-    if (slave) return NODE_FILL;
-    if (g.has_fine) {
-        bool under = true, on = false;
-        for (int d = 0; d < g.dim; ++d) {
-            under = under && p[d] >= g.flo[d] && p[d] <= g.fhi[d];
-            on = on || (g.fflo[d] && p[d] == g.flo[d]) || (g.ffhi[d] && p[d] == g.fhi[d]);
-        }
-        if (under && !on) return NODE_COVERED;
-        *iface = under && on;
-    }
+    const unsigned char m = cls_at(g, p);
+    *iface = (m & CLS_IFACE) != 0;
+    if (m & CLS_FILL) return NODE_FILL;
+    if (m & CLS_COVERED) return NODE_COVERED;
     return NODE_UNKNOWN;
 }
 
@@ -104,19 +110,21 @@ __global__ __launch_bounds__(CX* CY) void k_fill(Lay Lf, Geo gf, Lay Lc, int gcl
     }
 }
 
-// Coarse nodes under B_{l+1} / 2 (coarse geometry gc) take the coincident fine value.
+// Coarse nodes under level l+1 (coarse geometry gc) take the coincident fine value.
 template <typename T>
 __global__ __launch_bounds__(CX* CY) void k_inject(Lay Lc, Geo gc, Lay Lf, int gflo0, int gflo1, int gflo2,
                                                    const T* __restrict__ fine, T* __restrict__ crse, int nc) {
     const int a = blockIdx.x * CX + threadIdx.x, b = blockIdx.y * CY + threadIdx.y, c = blockIdx.z;
-    const int I = gc.flo[0] + a, J = gc.flo[1] + b, K = gc.flo[2] + c;
-    if (I > gc.fhi[0] || J > gc.fhi[1]) return;
+    const int I = gc.ulo[0] + a, J = gc.ulo[1] + b, K = gc.ulo[2] + c;
+    if (I > gc.uhi[0] || J > gc.uhi[1]) return;
+    const int p[3] = {I, J, K};
+    if (!(cls_at(gc, p) & CLS_UNDER)) return;
     const long long oc = at(Lc, I - gc.glo[0], J - gc.glo[1], K - gc.glo[2]);
     const long long of = at(Lf, 2 * I - gflo0, 2 * J - gflo1, 2 * K - gflo2);
     for (int n = 0; n < nc; ++n) crse[n * Lc.cs + oc] = fine[n * Lf.cs + of];
 }
 
-// Reflux into the coarse nodes under B_{l+1} / 2: the full weighting of the fine
+// Reflux into the coarse nodes under level l+1: the full weighting of the fine
 // field v (read on G_{l+1}), branches by the coarse domain (Operator.cpp:779-822).
 // mode 0: crse = R v.  mode 1 (a FAC down-sweep): crse = (interface ? crse : R v) - R w.
 template <typename T, int DIM>
@@ -124,8 +132,12 @@ __global__ __launch_bounds__(CX* CY) void k_reflux_c(Lay Lc, Geo gc, Lay Lf, int
                                                      const T* __restrict__ v, const T* __restrict__ w,
                                                      T* __restrict__ crse, int nc, int mode) {
     const int a = blockIdx.x * CX + threadIdx.x, b = blockIdx.y * CY + threadIdx.y, c = blockIdx.z;
-    const int I = gc.flo[0] + a, J = gc.flo[1] + b, K = gc.flo[2] + c;
-    if (I > gc.fhi[0] || J > gc.fhi[1]) return;
+    const int I = gc.ulo[0] + a, J = gc.ulo[1] + b, K = gc.ulo[2] + c;
+    if (I > gc.uhi[0] || J > gc.uhi[1]) return;
+    {
+        const int q[3] = {I, J, K};
+        if (!(cls_at(gc, q) & CLS_UNDER)) return;
+    }
     Lay Ld = Lc;  // the branches test the coarse domain boundary
     Ld.nx = gc.N[0] + 1;
     Ld.ny = gc.N[1] + 1;
@@ -159,27 +171,28 @@ __global__ __launch_bounds__(CX* CY) void k_mask(Lay L, Geo g, T* __restrict__ x
     for (int n = 0; n < nc; ++n) x[n * L.cs + at(L, i, j, k)] = T(0);
 }
 
-// G_l field -> right-hand side of the box hierarchy (zero on slaves: their rows
-// are homogeneous Dirichlet rows of the box problem).
+// G_l field -> right-hand side of the level's multigrid hierarchy on its union
+// box (first node `lo`, global), converted to TO; zero at fill nodes (slaves:
+// their rows are homogeneous Dirichlet rows of the box problem).
 template <typename TO>
-__global__ __launch_bounds__(CX* CY) void k_to_box(Lay Lg, Geo g, const double* __restrict__ src, Lay Lb,
-                                                   TO* __restrict__ dst, int nc) {
+__global__ __launch_bounds__(CX* CY) void k_to_box(Lay Lg, Geo g, int lo0, int lo1, int lo2,
+                                                   const double* __restrict__ src, Lay Lb, TO* __restrict__ dst,
+                                                   int nc) {
     const int i = blockIdx.x * CX + threadIdx.x, j = blockIdx.y * CY + threadIdx.y, k = blockIdx.z;
     if (i >= Lb.nx || j >= Lb.ny) return;
-    const int p[3] = {g.lo[0] + i, g.lo[1] + j, g.lo[2] + k};
-    bool iface;
-    const bool slave = node_class(g, p, &iface) == NODE_FILL;
+    const int p[3] = {lo0 + i, lo1 + j, lo2 + k};
+    const bool fill = (cls_at(g, p) & CLS_FILL) != 0;
     const long long os = at(Lg, p[0] - g.glo[0], p[1] - g.glo[1], p[2] - g.glo[2]);
-    for (int n = 0; n < nc; ++n) dst[n * Lb.cs + at(Lb, i, j, k)] = slave ? TO(0) : (TO)src[n * Lg.cs + os];
+    for (int n = 0; n < nc; ++n) dst[n * Lb.cs + at(Lb, i, j, k)] = fill ? TO(0) : (TO)src[n * Lg.cs + os];
 }
 
-// Box correction -> G_l field on B_l (assign or add).
+// The hierarchy's correction -> G_l on the union box (assign or add).
 template <typename TI>
-__global__ __launch_bounds__(CX* CY) void k_from_box(Lay Lb, const TI* __restrict__ src, Lay Lg, Geo g,
-                                                     double* __restrict__ dst, int nc, int add) {
+__global__ __launch_bounds__(CX* CY) void k_from_box(Lay Lb, const TI* __restrict__ src, Lay Lg, int o0, int o1,
+                                                     int o2, double* __restrict__ dst, int nc, int add) {
     const int i = blockIdx.x * CX + threadIdx.x, j = blockIdx.y * CY + threadIdx.y, k = blockIdx.z;
     if (i >= Lb.nx || j >= Lb.ny) return;
-    const long long od = at(Lg, g.lo[0] + i - g.glo[0], g.lo[1] + j - g.glo[1], g.lo[2] + k - g.glo[2]);
+    const long long od = at(Lg, i + o0, j + o1, k + o2);
     for (int n = 0; n < nc; ++n) {
         const double v = (double)src[n * Lb.cs + at(Lb, i, j, k)];
         if (add) dst[n * Lg.cs + od] += v;
@@ -187,12 +200,24 @@ __global__ __launch_bounds__(CX* CY) void k_from_box(Lay Lb, const TI* __restric
     }
 }
 
-// A box of G (stiffness planes) into the box hierarchy's level 0.
-__global__ __launch_bounds__(CX* CY) void k_subbox(Lay Ls, int ox, int oy, int oz, const double* __restrict__ src,
+// The union box of G's stiffness planes (offset o) into the hierarchy's level 0.
+__global__ __launch_bounds__(CX* CY) void k_subbox(Lay Ls, int o0, int o1, int o2, const double* __restrict__ src,
                                                    Lay Ld, double* __restrict__ dst, int nc) {
     const int i = blockIdx.x * CX + threadIdx.x, j = blockIdx.y * CY + threadIdx.y, k = blockIdx.z;
     if (i >= Ld.nx || j >= Ld.ny) return;
-    for (int n = 0; n < nc; ++n) dst[n * Ld.cs + at(Ld, i, j, k)] = src[n * Ls.cs + at(Ls, i + ox, j + oy, k + oz)];
+    for (int n = 0; n < nc; ++n) dst[n * Ld.cs + at(Ld, i, j, k)] = src[n * Ls.cs + at(Ls, i + o0, j + o1, k + o2)];
+}
+
+// A Jacobi sweep's output keeps the input at fill nodes (they are the sweep's
+// Dirichlet data: zero in a down-sweep, the interpolated coarse correction in an up-sweep).
+__global__ __launch_bounds__(CX* CY) void k_hold_fill(Lay L, Geo g, const double* __restrict__ xin,
+                                                      double* __restrict__ xout, int nc) {
+    const int i = blockIdx.x * CX + threadIdx.x, j = blockIdx.y * CY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    const int p[3] = {g.glo[0] + i, g.glo[1] + j, g.glo[2] + k};
+    if (!(cls_at(g, p) & CLS_FILL)) return;
+    const long long o = at(L, i, j, k);
+    for (int n = 0; n < nc; ++n) xout[n * L.cs + o] = xin[n * L.cs + o];
 }
 
 // max |x| over a flat array into *out (bit pattern of a non-negative double).
@@ -264,8 +289,16 @@ struct CLev {
     BCTab bcG{};        // boundary rows of G (domain classes; Displacement on artificial faces)
     double* C = nullptr;
     bool model_set = false;
+    std::vector<int> boxes;          // lo[3], hi[3] per box (level l >= 1)
+    unsigned char* cls = nullptr;    // node classes of G on the device
+    double* dg = nullptr;            // diagonal of A_l on G (levels >= 1, FP64)
+    int nu1 = 2, nu2 = 2;            // Fsmooth calls of the FAC cycle (smoothed levels)
+    // a level whose union is one box (level 0: the domain) carries a multigrid
+    // hierarchy on that box; its FAC step is a cycle of it instead of smoothing
     ela_hier* H = nullptr;
-    Lay Lb{};           // the box hierarchy's level 0
+    Lay Lb{};                        // the hierarchy's level 0
+    int ulo[3] = {0, 0, 0};          // the union's bounding box (global nodes)
+    int uhi[3] = {0, 0, 0};
 };
 
 }  // namespace
@@ -283,7 +316,7 @@ struct ela_amr {
     double bcval[26][3] = {};
     hipStream_t stream = nullptr;
     size_t ntot = 0;
-    double *u = nullptr, *b = nullptr, *r = nullptr, *z = nullptr, *w = nullptr, *t = nullptr;
+    double *u = nullptr, *b = nullptr, *r = nullptr, *z = nullptr, *w = nullptr, *t = nullptr, *q = nullptr;
     int krylov = 8;
     std::vector<double*> V, Z;
     double *kdev = nullptr, *kpart = nullptr;
@@ -331,6 +364,12 @@ void afree_all(ela_amr* a) {
 
 double* seg(double* base, const CLev& v) { return base + v.off; }
 
+// Launch grid over the bounding box of a level's nodes under the next finer level.
+dim3 under_grid(const ela_amr* a, const CLev& c) {
+    return dim3((c.geo.uhi[0] - c.geo.ulo[0] + CX) / CX, (c.geo.uhi[1] - c.geo.ulo[1] + CY) / CY,
+                a->dim == 3 ? c.geo.uhi[2] - c.geo.ulo[2] + 1 : 1);
+}
+
 // slaves and ghosts of every finer level from the coarser one, then covered nodes
 // from the finer one
 void fill(ela_amr* a, double* x) {
@@ -341,17 +380,14 @@ void fill(ela_amr* a, double* x) {
     }
     for (int l = a->nlev - 1; l >= 1; --l) {
         CLev &f = a->lev[l], &c = a->lev[l - 1];
-        const dim3 g((c.geo.fhi[0] - c.geo.flo[0] + CX) / CX, (c.geo.fhi[1] - c.geo.flo[1] + CY) / CY,
-                     a->dim == 3 ? c.geo.fhi[2] - c.geo.flo[2] + 1 : 1);
-        hipLaunchKernelGGL(k_inject<double>, g, dim3(CX, CY), 0, a->stream, c.L, c.geo, f.L, f.geo.glo[0],
-                           f.geo.glo[1], f.geo.glo[2], (const double*)seg(x, f), seg(x, c), a->dim);
+        hipLaunchKernelGGL(k_inject<double>, under_grid(a, c), dim3(CX, CY), 0, a->stream, c.L, c.geo, f.L,
+                           f.geo.glo[0], f.geo.glo[1], f.geo.glo[2], (const double*)seg(x, f), seg(x, c), a->dim);
     }
 }
 
 void reflux(ela_amr* a, int l, const double* v, const double* w, double* crse, int mode) {
     CLev &f = a->lev[l], &c = a->lev[l - 1];
-    const dim3 g((c.geo.fhi[0] - c.geo.flo[0] + CX) / CX, (c.geo.fhi[1] - c.geo.flo[1] + CY) / CY,
-                 a->dim == 3 ? c.geo.fhi[2] - c.geo.flo[2] + 1 : 1);
+    const dim3 g = under_grid(a, c);
     if (a->dim == 3)
         hipLaunchKernelGGL((k_reflux_c<double, 3>), g, dim3(CX, CY), 0, a->stream, c.L, c.geo, f.L, f.geo.glo[0],
                            f.geo.glo[1], f.geo.glo[2], v, w, crse, a->dim, mode);
@@ -400,28 +436,44 @@ void apply(ela_amr* a, double* x, double* y) {
     mask(a, y);
 }
 
-// dst (+)= one box cycle on the level's residual src
+// dst (+)= one multigrid cycle of the level's box hierarchy on the level's residual src
 int box_correct(ela_amr* a, int l, const double* src, double* dst, int add) {
     CLev& v = a->lev[l];
     const dim3 gb = grid_of(v.Lb);
     void* res = hier_res0(v.H, a->cprec);
     if (a->cprec == ELA_F32)
-        hipLaunchKernelGGL(k_to_box<float>, gb, dim3(CX, CY), 0, a->stream, v.L, v.geo, src, v.Lb, (float*)res, a->dim);
+        hipLaunchKernelGGL(k_to_box<float>, gb, dim3(CX, CY), 0, a->stream, v.L, v.geo, v.ulo[0], v.ulo[1], v.ulo[2],
+                           src, v.Lb, (float*)res, a->dim);
     else
-        hipLaunchKernelGGL(k_to_box<double>, gb, dim3(CX, CY), 0, a->stream, v.L, v.geo, src, v.Lb, (double*)res,
-                           a->dim);
+        hipLaunchKernelGGL(k_to_box<double>, gb, dim3(CX, CY), 0, a->stream, v.L, v.geo, v.ulo[0], v.ulo[1],
+                           v.ulo[2], src, v.Lb, (double*)res, a->dim);
     if (int rc = hier_cycle(v.H, a->cprec)) return rc;
     const void* cor = hier_cor0(v.H, a->cprec);
+    const int o0 = v.ulo[0] - v.geo.glo[0], o1 = v.ulo[1] - v.geo.glo[1], o2 = v.ulo[2] - v.geo.glo[2];
     if (a->cprec == ELA_F32)
-        hipLaunchKernelGGL(k_from_box<float>, gb, dim3(CX, CY), 0, a->stream, v.Lb, (const float*)cor, v.L, v.geo,
-                           dst, a->dim, add);
+        hipLaunchKernelGGL(k_from_box<float>, gb, dim3(CX, CY), 0, a->stream, v.Lb, (const float*)cor, v.L, o0, o1,
+                           o2, dst, a->dim, add);
     else
-        hipLaunchKernelGGL(k_from_box<double>, gb, dim3(CX, CY), 0, a->stream, v.Lb, (const double*)cor, v.L, v.geo,
-                           dst, a->dim, add);
+        hipLaunchKernelGGL(k_from_box<double>, gb, dim3(CX, CY), 0, a->stream, v.Lb, (const double*)cor, v.L, o0, o1,
+                           o2, dst, a->dim, add);
     return ELA_OK;
 }
 
-// z = FAC(v): one multiplicative FAC cycle (t, w: work vectors)
+// ncalls Fsmooth calls (2 weighted-Jacobi sweeps each, Operator.cpp:90-162) of
+// A_l x = rhs on level l >= 1: the non-fill nodes are updated, the fill nodes
+// keep their values.  tmp: work segment of the level.
+void level_smooth(ela_amr* a, int l, double* x, const double* rhs, double* tmp, int ncalls) {
+    CLev& f = a->lev[l];
+    const dim3 g = grid_of(f.L);
+    for (int c = 0; c < ncalls; ++c) {
+        launch_jacobi<double>(f.L, a->kind, f.C, f.dg, x, rhs, tmp, f.bcG, f.h, a->uniform, 2.0 / 3.0, a->stream);
+        hipLaunchKernelGGL(k_hold_fill, g, dim3(CX, CY), 0, a->stream, f.L, f.geo, (const double*)x, tmp, a->dim);
+        launch_jacobi<double>(f.L, a->kind, f.C, f.dg, tmp, rhs, x, f.bcG, f.h, a->uniform, 2.0 / 3.0, a->stream);
+        hipLaunchKernelGGL(k_hold_fill, g, dim3(CX, CY), 0, a->stream, f.L, f.geo, (const double*)tmp, x, a->dim);
+    }
+}
+
+// z = FAC(v): one multiplicative FAC cycle over the AMR levels (t, w, q: work vectors)
 int fac(ela_amr* a, const double* v, double* z) {
     hipStream_t s = a->stream;
     const int L = a->nlev - 1;
@@ -430,7 +482,11 @@ int fac(ela_amr* a, const double* v, double* z) {
         return afail(ELA_ERR_HIP, "copy");
     for (int l = L; l >= 1; --l) {
         CLev& f = a->lev[l];
-        if (int rc = box_correct(a, l, seg(a->t, f), seg(z, f), 0)) return rc;
+        if (f.H) {
+            if (int rc = box_correct(a, l, seg(a->t, f), seg(z, f), 0)) return rc;
+        } else {
+            level_smooth(a, l, seg(z, f), seg(a->t, f), seg(a->q, f), f.nu1);
+        }
         launch_fapply<double>(f.L, a->kind, f.C, seg(z, f), seg(a->w, f), f.bcG, f.h, a->uniform, s);
         reflux(a, l, seg(a->t, f), seg(a->w, f), seg(a->t, a->lev[l - 1]), 1);
     }
@@ -439,9 +495,13 @@ int fac(ela_amr* a, const double* v, double* z) {
         CLev &f = a->lev[l], &c = a->lev[l - 1];
         hipLaunchKernelGGL(k_fill<double>, grid_of(f.L), dim3(CX, CY), 0, s, f.L, f.geo, c.L, c.geo.glo[0],
                            c.geo.glo[1], c.geo.glo[2], (const double*)seg(z, c), seg(z, f), a->dim, 1);
-        launch_residual<double, double>(f.L, a->kind, f.C, seg(z, f), seg(a->t, f), seg(a->w, f), (double*)nullptr,
-                                        nullptr, nullptr, nullptr, f.bcG, f.h, a->uniform, s);
-        if (int rc = box_correct(a, l, seg(a->w, f), seg(z, f), 1)) return rc;
+        if (f.H) {
+            launch_residual<double, double>(f.L, a->kind, f.C, seg(z, f), seg(a->t, f), seg(a->w, f),
+                                            (double*)nullptr, nullptr, nullptr, nullptr, f.bcG, f.h, a->uniform, s);
+            if (int rc = box_correct(a, l, seg(a->w, f), seg(z, f), 1)) return rc;
+        } else {
+            level_smooth(a, l, seg(z, f), seg(a->t, f), seg(a->q, f), f.nu2);
+        }
     }
     return ELA_OK;
 }
@@ -635,60 +695,209 @@ double* field_of(ela_amr* a, int which) {
     }
 }
 
+// ---------------------------------------------------------------- level geometry (host)
+// The boxes of one level and its array box G (the union's bounding box grown
+// by 2 nodes, clipped to the domain).
+struct LevelBoxes {
+    int dim = 3;
+    int N[3] = {0, 0, 0};      // domain cells of the level
+    std::vector<int> boxes;    // lo[3], hi[3] per box
+    int glo[3] = {0, 0, 0};    // first node of G
+    int gn[3] = {1, 1, 1};     // nodes of G
+    int gc(int d) const { return d < dim ? gn[d] - 1 : 1; }  // cells of G
+};
+
+int level_extent(LevelBoxes& B, int l) {
+    for (int d = 0; d < 3; ++d) {
+        if (d >= B.dim) {
+            B.glo[d] = 0;
+            B.gn[d] = 1;
+            continue;
+        }
+        int lo = B.N[d], hi = 0;
+        for (size_t q = 0; q < B.boxes.size(); q += 6) {
+            lo = std::min(lo, B.boxes[q + d]);
+            hi = std::max(hi, B.boxes[q + 3 + d]);
+        }
+        const int glo = l == 0 ? 0 : std::max(lo - 2, 0), ghi = l == 0 ? B.N[d] : std::min(hi + 2, B.N[d]);
+        B.glo[d] = glo;
+        B.gn[d] = ghi - glo + 1;
+    }
+    return ELA_OK;
+}
+
+// Cells of G covered by the level's boxes (cell c spans nodes c..c+1), dense, x fastest.
+void union_cells(const LevelBoxes& B, std::vector<unsigned char>& cells) {
+    const long long cx = B.gc(0), cy = B.gc(1), cz = B.gc(2);
+    cells.assign((size_t)(cx * cy * cz), 0);
+    for (size_t q = 0; q < B.boxes.size(); q += 6) {
+        int lo[3], hi[3];
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = d < B.dim ? B.boxes[q + d] - B.glo[d] : 0;
+            hi[d] = d < B.dim ? B.boxes[q + 3 + d] - B.glo[d] : 1;
+        }
+        for (int k = lo[2]; k < hi[2]; ++k)
+            for (int j = lo[1]; j < hi[1]; ++j)
+                for (int i = lo[0]; i < hi[0]; ++i) cells[(size_t)(i + cx * (j + cy * k))] = 1;
+    }
+}
+
+// Global cell (x, y, z) of the level in `cells` (false outside G).
+bool cell_in(const LevelBoxes& B, const std::vector<unsigned char>& cells, const int c[3]) {
+    long long idx = 0, stride = 1;
+    for (int d = 0; d < 3; ++d) {
+        const int loc = d < B.dim ? c[d] - B.glo[d] : 0;
+        if (loc < 0 || loc >= B.gc(d)) return false;
+        idx += loc * stride;
+        stride *= B.gc(d);
+    }
+    return cells[(size_t)idx] != 0;
+}
+
+// Proper nesting: every fine box, coarsened and grown by 2 coarse cells (clipped
+// to the domain), lies in the coarse level's union.
+bool nested(const LevelBoxes& C, const std::vector<unsigned char>& ccells, const LevelBoxes& F) {
+    for (size_t q = 0; q < F.boxes.size(); q += 6) {
+        int lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+        for (int d = 0; d < F.dim; ++d) {
+            lo[d] = std::max(F.boxes[q + d] / 2 - 2, 0);
+            hi[d] = std::min(F.boxes[q + 3 + d] / 2 + 2, C.N[d]);
+        }
+        for (int k = lo[2]; k < hi[2]; ++k)
+            for (int j = lo[1]; j < hi[1]; ++j)
+                for (int i = lo[0]; i < hi[0]; ++i) {
+                    const int c[3] = {i, j, F.dim == 3 ? k : 0};
+                    if (!cell_in(C, ccells, c)) return false;
+                }
+    }
+    return true;
+}
+
+// Class bits of every node of G (see the top of the file).  A node's touching
+// cells are the 2^dim cells it is a corner of; cells outside the domain do not
+// count.  F / fcells: the next finer level (null on the finest).  ulo / uhi
+// receive the bounding box of the nodes under F (global node indices).
+void node_classes(const LevelBoxes& B, const std::vector<unsigned char>& cells, const LevelBoxes* F,
+                  const std::vector<unsigned char>* fcells, bool refined, std::vector<unsigned char>& mask,
+                  int ulo[3], int uhi[3]) {
+    const int dim = B.dim;
+    const long long nx = B.gn[0], ny = B.gn[1], nz = B.gn[2];
+    mask.assign((size_t)(nx * ny * nz), 0);
+    for (int d = 0; d < 3; ++d) {
+        ulo[d] = d < dim ? B.N[d] : 0;
+        uhi[d] = d < dim ? -1 : 0;
+    }
+    const int nk = dim == 3 ? 2 : 1;
+    for (long long k = 0; k < nz; ++k)
+        for (long long j = 0; j < ny; ++j)
+            for (long long i = 0; i < nx; ++i) {
+                const int p[3] = {(int)(B.glo[0] + i), (int)(B.glo[1] + j), (int)(B.glo[2] + k)};
+                bool any_in = false, all_in = true, any_u = false, all_u = true;
+                for (int dk = 0; dk < nk; ++dk)
+                    for (int dj = 0; dj < 2; ++dj)
+                        for (int di = 0; di < 2; ++di) {
+                            const int c[3] = {p[0] - 1 + di, p[1] - 1 + dj, dim == 3 ? p[2] - 1 + dk : 0};
+                            bool indom = true;
+                            for (int d = 0; d < dim; ++d) indom = indom && c[d] >= 0 && c[d] < B.N[d];
+                            if (!indom) continue;
+                            const bool in = cell_in(B, cells, c);
+                            any_in = any_in || in;
+                            all_in = all_in && in;
+                            if (F) {
+                                const int fc[3] = {2 * c[0], 2 * c[1], dim == 3 ? 2 * c[2] : 0};
+                                const bool u = cell_in(*F, *fcells, fc);
+                                any_u = any_u || u;
+                                all_u = all_u && u;
+                            }
+                        }
+                unsigned char m = 0;
+                if (!any_in || (refined && !all_in)) m |= CLS_FILL;
+                if (F && any_u) {
+                    m |= CLS_UNDER;
+                    if (all_u && any_in) m |= CLS_COVERED;
+                    else m |= CLS_IFACE;
+                    for (int d = 0; d < dim; ++d) {
+                        ulo[d] = std::min(ulo[d], p[d]);
+                        uhi[d] = std::max(uhi[d], p[d]);
+                    }
+                }
+                mask[(size_t)(i + nx * (j + ny * k))] = m;
+            }
+    if (!F)
+        for (int d = 0; d < 3; ++d) ulo[d] = uhi[d] = 0;
+}
+
 }  // namespace
 
 extern "C" {
 
 int ela_amr_create(ela_ctx* ctx, const ela_domain* dom, int nlev, const int* boxes, int model_kind, int coarse_min,
                    int mode, ela_amr** out) {
-    if (!ctx || !dom || !out || (nlev > 1 && !boxes)) return afail(ELA_ERR_BADARG, "null argument");
+    std::vector<int> counts(nlev > 1 ? nlev - 1 : 0, 1);
+    return ela_amr_create_boxes(ctx, dom, nlev, counts.data(), boxes, model_kind, coarse_min, mode, out);
+}
+
+int ela_amr_create_boxes(ela_ctx* ctx, const ela_domain* dom, int nlev, const int* nboxes, const int* boxes,
+                         int model_kind, int coarse_min, int mode, ela_amr** out) {
+    if (!ctx || !dom || !out || (nlev > 1 && (!boxes || !nboxes))) return afail(ELA_ERR_BADARG, "null argument");
     *out = nullptr;
     if (dom->dim != 2 && dom->dim != 3) return afail(ELA_ERR_BADARG, "dim must be 2 or 3");
     if (nlev < 1 || nlev > 8) return afail(ELA_ERR_BADARG, "1 to 8 levels");
     if (mode != ELA_SOLVE_FP64 && mode != ELA_SOLVE_MIXED) return afail(ELA_ERR_BADARG, "mode must be FP64 or MIXED");
     if (model_kind < 0 || model_kind > 2) return afail(ELA_ERR_BADARG, "unknown model kind");
     const int dim = dom->dim;
-    // geometry and nesting
+    for (int d = 0; d < dim; ++d)
+        if (dom->n[d] < 2 || !(dom->dx[d] > 0)) return afail(ELA_ERR_BADARG, "bad level-0 domain");
+    // boxes per level, their bounding boxes and the arrays G_l
+    std::vector<std::vector<int>> bx(nlev);
+    std::vector<LevelBoxes> lb(nlev);
+    size_t pos = 0;
+    for (int l = 0; l < nlev; ++l) {
+        LevelBoxes& B = lb[l];
+        B.dim = dim;
+        for (int d = 0; d < 3; ++d) B.N[d] = d < dim ? dom->n[d] << l : 0;
+        if (l == 0) {
+            bx[0] = {0, 0, 0, B.N[0], B.N[1], B.N[2]};
+        } else {
+            const int nb = nboxes[l - 1];
+            if (nb < 1 || nb > 1 << 20) return afail(ELA_ERR_BADARG, "level " + std::to_string(l) + ": no boxes");
+            bx[l].assign(boxes + 6 * pos, boxes + 6 * (pos + nb));
+            pos += nb;
+            for (int q = 0; q < nb; ++q)
+                for (int d = 0; d < 3; ++d) {
+                    int& lo = bx[l][6 * q + d];
+                    int& hi = bx[l][6 * q + 3 + d];
+                    if (d >= dim) {
+                        lo = hi = 0;
+                        continue;
+                    }
+                    if (lo % 2 || hi % 2 || lo < 0 || hi <= lo || hi > B.N[d])
+                        return afail(ELA_ERR_BADARG, "level " + std::to_string(l) +
+                                                         ": box corners must be even and inside the domain");
+                }
+        }
+        B.boxes = bx[l];
+        if (int rc = level_extent(B, l)) return rc;
+    }
+    std::vector<std::vector<unsigned char>> cells(nlev), masks(nlev);
+    for (int l = 0; l < nlev; ++l) union_cells(lb[l], cells[l]);
+    for (int l = 1; l < nlev; ++l)
+        if (!nested(lb[l - 1], cells[l - 1], lb[l]))
+            return afail(ELA_ERR_BADARG, "level " + std::to_string(l) + ": every box must sit 2 coarse cells inside "
+                                         "its parent's union (or reach the domain boundary with it)");
     std::vector<Geo> geo(nlev);
     for (int l = 0; l < nlev; ++l) {
         Geo& g = geo[l];
         std::memset(&g, 0, sizeof g);
         g.dim = dim;
         for (int d = 0; d < 3; ++d) {
-            g.N[d] = d < dim ? dom->n[d] << l : 0;
-            if (l == 0) {
-                g.lo[d] = 0;
-                g.hi[d] = g.N[d];
-            } else {
-                g.lo[d] = d < dim ? boxes[6 * (l - 1) + d] : 0;
-                g.hi[d] = d < dim ? boxes[6 * (l - 1) + 3 + d] : 0;
-            }
+            g.glo[d] = lb[l].glo[d];
+            g.gn[d] = lb[l].gn[d];
+            g.N[d] = lb[l].N[d];
         }
-        for (int d = 0; d < dim; ++d) {
-            if (dom->n[d] < 2 || !(dom->dx[d] > 0)) return afail(ELA_ERR_BADARG, "bad level-0 domain");
-            if (l == 0) continue;
-            const int lo = g.lo[d], hi = g.hi[d], N = g.N[d];
-            const int clo = geo[l - 1].lo[d], chi = geo[l - 1].hi[d], cN = geo[l - 1].N[d];
-            if (lo % 2 || hi % 2 || lo < 0 || hi <= lo || hi > N)
-                return afail(ELA_ERR_BADARG, "level " + std::to_string(l) + ": box corners must be even and inside the domain");
-            if ((lo > 0 && lo / 2 < clo + 2) || (hi < N && hi / 2 > chi - 2))
-                return afail(ELA_ERR_BADARG, "level " + std::to_string(l) + ": box must sit 2 coarse cells inside its parent");
-            if ((lo == 0 && clo != 0) || (hi == N && chi != cN))
-                return afail(ELA_ERR_BADARG, "level " + std::to_string(l) + ": box on the domain boundary needs a parent there");
-            g.cflo[d] = lo > 0;
-            g.cfhi[d] = hi < N;
-        }
-        for (int d = 0; d < 3; ++d) g.glo[d] = (l == 0 || d >= dim) ? 0 : std::max(g.lo[d] - 2, 0);
-    }
-    for (int l = 0; l + 1 < nlev; ++l) {
-        Geo &g = geo[l], &f = geo[l + 1];
-        g.has_fine = 1;
-        for (int d = 0; d < dim; ++d) {
-            g.flo[d] = f.lo[d] / 2;
-            g.fhi[d] = f.hi[d] / 2;
-            g.fflo[d] = f.cflo[d];
-            g.ffhi[d] = f.cfhi[d];
-        }
+        node_classes(lb[l], cells[l], l + 1 < nlev ? &lb[l + 1] : nullptr, l + 1 < nlev ? &cells[l + 1] : nullptr,
+                     l > 0, masks[l], g.ulo, g.uhi);
+        g.has_fine = l + 1 < nlev;
     }
     ela_amr* a = new ela_amr();
     a->ctx = ctx;
@@ -705,41 +914,58 @@ int ela_amr_create(ela_ctx* ctx, const ela_domain* dom, int nlev, const int* box
     for (int l = 0; l < nlev && !rc; ++l) {
         CLev& v = a->lev[l];
         v.geo = geo[l];
+        v.boxes = bx[l];
         for (int d = 0; d < 3; ++d) v.h[d] = d < dim ? dom->dx[d] / (double)(1 << l) : 1.0;
-        const int gh[3] = {l == 0 ? v.geo.N[0] : std::min(v.geo.hi[0] + 2, v.geo.N[0]),
-                           l == 0 ? v.geo.N[1] : std::min(v.geo.hi[1] + 2, v.geo.N[1]),
-                           dim == 3 ? (l == 0 ? v.geo.N[2] : std::min(v.geo.hi[2] + 2, v.geo.N[2])) : 0};
         v.L.dim = dim;
-        v.L.nx = gh[0] - v.geo.glo[0] + 1;
-        v.L.ny = gh[1] - v.geo.glo[1] + 1;
-        v.L.nz = dim == 3 ? gh[2] - v.geo.glo[2] + 1 : 1;
+        v.L.nx = v.geo.gn[0];
+        v.L.ny = v.geo.gn[1];
+        v.L.nz = dim == 3 ? v.geo.gn[2] : 1;
         v.L.px = (v.L.nx + 31) / 32 * 32;
         v.L.pl = v.L.px * v.L.ny;
         v.L.cs = (v.L.pl * v.L.nz + 255) / 256 * 256;
         v.elems = (size_t)dim * v.L.cs;
         v.off = off;
         off += v.elems;
-        // the level's box hierarchy (its own stream until the composite's is shared)
-        ela_domain bd{};
-        bd.dim = dim;
+        // the union's bounding box; a union that fills it is one box and gets a multigrid hierarchy
+        size_t bbox_cells = 1, ucells = 0;
         for (int d = 0; d < 3; ++d) {
-            bd.n[d] = d < dim ? v.geo.hi[d] - v.geo.lo[d] : 0;
-            bd.dx[d] = v.h[d];
+            v.ulo[d] = d < dim ? v.geo.N[d] : 0;
+            v.uhi[d] = 0;
+            for (size_t q = 0; q < bx[l].size(); q += 6) {
+                v.ulo[d] = std::min(v.ulo[d], bx[l][q + d]);
+                v.uhi[d] = std::max(v.uhi[d], bx[l][q + 3 + d]);
+            }
+            if (d < dim) bbox_cells *= (size_t)(v.uhi[d] - v.ulo[d]);
         }
-        rc = ela_hier_create_ex(ctx, &bd, model_kind, 0, l == 0 ? std::max(coarse_min, 2) : 2, mode, &v.H);
-        if (!rc) rc = hier_level0(v.H, &v.Lb);
-        // level 0 keeps the F-cycle; a refined box (Displacement rows on its coarse/fine faces, a
-        // well-conditioned local problem) takes a V(1,1) cycle: the 3-level crack nest on a 128^3 base converges in 31 instead of 30
-        // iterations at 0.65x the time per solve
-        if (!rc && l > 0) rc = ela_set_cycle(v.H, ELA_CYCLE_V, 1, 1);
-    }
-    if (!rc) {
-        a->stream = hier_stream(a->lev[0].H);
-        for (int l = 1; l < nlev && !rc; ++l) rc = hier_share_stream(a->lev[l].H, a->stream);
+        for (unsigned char c : cells[l]) ucells += c;
+        if (l == 0 || ucells == bbox_cells) {  // its own stream until the composite's is shared
+            ela_domain bd{};
+            bd.dim = dim;
+            for (int d = 0; d < 3; ++d) {
+                bd.n[d] = d < dim ? v.uhi[d] - v.ulo[d] : 0;
+                bd.dx[d] = v.h[d];
+            }
+            rc = ela_hier_create_ex(ctx, &bd, model_kind, 0, l == 0 ? std::max(coarse_min, 2) : 2, mode, &v.H);
+            if (!rc) rc = hier_level0(v.H, &v.Lb);
+            if (!rc && l == 0) a->stream = hier_stream(v.H);
+            if (!rc && l > 0) rc = hier_share_stream(v.H, a->stream);
+            // a refined box (Displacement rows on its coarse/fine faces, a well-conditioned local problem)
+            // takes a V(1,1) cycle: the 3-level crack nest on a 128^3 base converges in 31 instead of 30
+            // iterations at 0.65x the time per solve
+            if (!rc && l > 0) rc = ela_set_cycle(v.H, ELA_CYCLE_V, 1, 1);
+        }
     }
     a->ntot = off;
-    for (int l = 0; l < nlev && !rc; ++l) rc = aalloc_t(a, (size_t)a->nc * a->lev[l].L.cs, &a->lev[l].C);
-    for (double** p : {&a->u, &a->b, &a->r, &a->z, &a->w, &a->t})
+    for (int l = 0; l < nlev && !rc; ++l) {
+        CLev& v = a->lev[l];
+        rc = aalloc_t(a, (size_t)a->nc * v.L.cs, &v.C);
+        if (!rc) rc = aalloc_t(a, masks[l].size(), &v.cls);
+        if (!rc && hipMemcpyAsync(v.cls, masks[l].data(), masks[l].size(), hipMemcpyHostToDevice, a->stream) != hipSuccess)
+            rc = afail(ELA_ERR_HIP, "mask upload failed");
+        v.geo.cls = v.cls;
+        if (!rc && !v.H) rc = aalloc_t(a, (size_t)dim * v.L.cs, &v.dg);
+    }
+    for (double** p : {&a->u, &a->b, &a->r, &a->z, &a->w, &a->t, &a->q})
         if (!rc) rc = aalloc_t(a, a->ntot, p);
     if (!rc) rc = aalloc_t(a, 4, &a->dnorm);
     if (!rc && hipHostMalloc((void**)&a->hnorm, 64) != hipSuccess) rc = afail(ELA_ERR_OOM, "hipHostMalloc failed");
@@ -762,8 +988,8 @@ int ela_amr_destroy(ela_amr* a) {
     if (a->stream) (void)hipStreamSynchronize(a->stream);
     afree_all(a);
     if (a->hnorm) (void)hipHostFree(a->hnorm);
-    for (int l = a->nlev - 1; l >= 0; --l)  // level 0 owns the shared stream
-        if (l < (int)a->lev.size() && a->lev[l].H) ela_hier_destroy(a->lev[l].H);
+    for (int l = (int)a->lev.size() - 1; l >= 0; --l)  // level 0 owns the shared stream
+        if (a->lev[l].H) ela_hier_destroy(a->lev[l].H);
     delete a;
     return ELA_OK;
 }
@@ -771,7 +997,7 @@ int ela_amr_destroy(ela_amr* a) {
 size_t ela_amr_bytes(const ela_amr* a) {
     if (!a) return 0;
     size_t s = a->bytes;
-    for (auto& v : a->lev) s += ela_hier_bytes(v.H);
+    for (auto& v : a->lev) s += v.H ? ela_hier_bytes(v.H) : 0;
     return s;
 }
 
@@ -818,7 +1044,7 @@ int ela_amr_set_bc(ela_amr* a, const int* types, const double* values) {
             a->bc.t[f][c] = (unsigned char)a->bctypes[f][c];
         }
     // per level: the table of G (artificial faces -> Displacement) and of the box
-    // hierarchy (coarse/fine faces -> homogeneous Displacement)
+    // hierarchy, if any (coarse/fine faces of the union box -> homogeneous Displacement)
     for (int l = 0; l < a->nlev; ++l) {
         CLev& v = a->lev[l];
         std::vector<int> bt(26 * 3, 0);
@@ -829,17 +1055,17 @@ int ela_amr_set_bc(ela_amr* a, const int* types, const double* values) {
             bool art = false, cf = false;
             for (int d = 0; d < a->dim; ++d) {
                 if (side[d] < 0) continue;
-                const int gend = side[d] ? v.geo.glo[d] + v.L.nx * (d == 0) + v.L.ny * (d == 1) + v.L.nz * (d == 2) - 1
-                                         : v.geo.glo[d];
+                const int gend = side[d] ? v.geo.glo[d] + v.geo.gn[d] - 1 : v.geo.glo[d];
                 art = art || (side[d] ? gend < v.geo.N[d] : gend > 0);
-                cf = cf || (side[d] ? v.geo.cfhi[d] : v.geo.cflo[d]);
+                cf = cf || (side[d] ? v.uhi[d] < v.geo.N[d] : v.ulo[d] > 0);
             }
             for (int c = 0; c < 3; ++c) {
                 if (art) v.bcG.t[f][c] = ELA_BC_DISP;
                 bt[3 * f + c] = cf ? ELA_BC_DISP : a->bctypes[f][c];
             }
         }
-        if (int rc = ela_set_bc(v.H, bt.data(), nullptr)) return rc;
+        if (v.H)
+            if (int rc = ela_set_bc(v.H, bt.data(), nullptr)) return rc;
     }
     a->prepared = false;
     return ELA_OK;
@@ -868,7 +1094,8 @@ int ela_amr_bc_init(ela_amr* a, int lev, double* rhs, int homogeneous) {
 int ela_amr_set_uniform(ela_amr* a, int uniform) {
     if (!a) return afail(ELA_ERR_BADARG, "null composite");
     a->uniform = uniform ? 1 : 0;
-    for (auto& v : a->lev) ela_set_uniform(v.H, a->uniform);
+    for (auto& v : a->lev)
+        if (v.H) ela_set_uniform(v.H, a->uniform);
     a->prepared = false;
     return ELA_OK;
 }
@@ -882,14 +1109,19 @@ int ela_amr_set_krylov(ela_amr* a, int restart) {
 
 int ela_amr_set_cycle(ela_amr* a, int cycle, int nu1, int nu2) {
     if (!a) return afail(ELA_ERR_BADARG, "null composite");
-    for (auto& v : a->lev)
-        if (int rc = ela_set_cycle(v.H, cycle, nu1, nu2)) return rc;
+    for (int l = 0; l < a->nlev; ++l)
+        if (int rc = ela_amr_set_level_cycle(a, l, cycle, nu1, nu2)) return rc;
     return ELA_OK;
 }
 
 int ela_amr_set_level_cycle(ela_amr* a, int lev, int cycle, int nu1, int nu2) {
     if (int rc = check_lev(a, lev)) return rc;
-    return ela_set_cycle(a->lev[lev].H, cycle, nu1, nu2);
+    if (a->lev[lev].H) return ela_set_cycle(a->lev[lev].H, cycle, nu1, nu2);
+    if (nu1 < 0 || nu2 < 0 || nu1 + nu2 < 1 || nu1 > 64 || nu2 > 64)
+        return afail(ELA_ERR_BADARG, "smoothing calls must be in [0, 64], not both zero");
+    a->lev[lev].nu1 = nu1;
+    a->lev[lev].nu2 = nu2;
+    return ELA_OK;
 }
 
 int ela_amr_prepare(ela_amr* a) {
@@ -898,11 +1130,15 @@ int ela_amr_prepare(ela_amr* a) {
         if (!a->lev[l].model_set)
             return afail(ELA_ERR_STATE, "Attempting to use operator before calling SetModel (level " + std::to_string(l) + ")");
     for (auto& v : a->lev) {
-        hipLaunchKernelGGL(k_subbox, grid_of(v.Lb), dim3(CX, CY), 0, a->stream, v.L, v.geo.lo[0] - v.geo.glo[0],
-                           v.geo.lo[1] - v.geo.glo[1], v.geo.lo[2] - v.geo.glo[2], (const double*)v.C, v.Lb,
-                           hier_C0(v.H), a->nc);
-        hier_model_set(v.H);
-        if (int rc = ela_prepare(v.H)) return rc;
+        if (v.H) {  // the union box's stiffness into its hierarchy
+            hipLaunchKernelGGL(k_subbox, grid_of(v.Lb), dim3(CX, CY), 0, a->stream, v.L, v.ulo[0] - v.geo.glo[0],
+                               v.ulo[1] - v.geo.glo[1], v.ulo[2] - v.geo.glo[2], (const double*)v.C, v.Lb,
+                               hier_C0(v.H), a->nc);
+            hier_model_set(v.H);
+            if (int rc = ela_prepare(v.H)) return rc;
+        } else {  // the diagonal of A_l on G_l for the Jacobi sweeps (its values at fill nodes are never used)
+            launch_diag(v.L, a->kind, v.C, v.dg, nullptr, v.bcG, v.h, a->dnorm, a->stream);
+        }
     }
     ELA_HIPCHK(hipStreamSynchronize(a->stream));
     a->prepared = true;

@@ -407,23 +407,35 @@ int ela_average_down(const ela_level_layout* crse, void* crse_sol, const ela_lev
  * Operator::Elastic<T>::define on several AMR levels (src/Operator/Elastic.cpp:30-55,
  * src/Operator/Operator.cpp:215-258) and the multi-level
  * Solver::Nonlocal::Linear::solve (src/Solver/Nonlocal/Linear.H:102-112).
- * Level 0 is the domain; level l >= 1 (ratio 2) is one box of level-l nodes
- * [lo, hi] (even corners, at least 2 coarse cells inside its parent unless on
- * the domain boundary).  Fields of level l live on that box grown by 2 nodes
- * and clipped to the domain (ela_amr_level_info: layout and the global index of
- * its first node); the outer nodes hold the interpolated coarser solution, as
- * the reference's fine ghost nodes do (Operator.cpp:556-624, Linear.H:23-29).
- * Coarse nodes under a finer box take its values (averageDownSolutionRHS,
- * Operator.cpp:626-641); at coarse nodes on the coarse/fine interface the
- * equation is the reflux of the fine residual (Operator.cpp:728-839).  The
- * solve is flexible GMRES(restart) in FP64 preconditioned by one FAC cycle
- * over the levels' multigrid hierarchies (FP32 or FP64 cycles by mode). */
+ * Level 0 is the domain; level l >= 1 (ratio 2) is the union of a list of boxes
+ * of level-l nodes [lo, hi] (even corners; boxes may share faces, and the union
+ * need not be a box -- the reference's BoxArray of the level, e.g. a region
+ * chopped into boxes of at most max_grid_size cells).  Every box, coarsened and
+ * grown by 2 coarse cells (clipped to the domain), must lie in the coarser
+ * level's union.  Fields of level l live on the bounding box of its union grown
+ * by 2 nodes and clipped to the domain (ela_amr_level_info: layout and the
+ * global index of its first node); the boxes share that array, so their shared
+ * nodes are stored once (AMReX's halo fills and nodalSync are identities).
+ * Nodes outside the union and on its coarse/fine boundary hold the
+ * interpolated coarser solution, as the reference's fine ghost nodes do
+ * (Operator.cpp:556-624, Linear.H:23-29).  Coarse nodes under a finer union
+ * take its values (averageDownSolutionRHS, Operator.cpp:626-641); at coarse
+ * nodes on the coarse/fine interface the equation is the reflux of the fine
+ * residual (Operator.cpp:728-839).  The solve is flexible GMRES(restart) in
+ * FP64 preconditioned by one FAC cycle: a level whose union is one box (level
+ * 0, or a box chopped into max_grid_size boxes) corrects with a cycle of a
+ * multigrid hierarchy on that box (FP32 or FP64 by mode), any other union
+ * with weighted-Jacobi smoothing (FP64), as AMReX's MLMG smooths fine AMR
+ * levels. */
 typedef struct ela_amr ela_amr;
 enum { ELA_AMR_U = 0, ELA_AMR_B = 1, ELA_AMR_R = 2 };
 
-/* boxes: (nlev - 1) records of lo[3], hi[3] (level-l node indices); the level-0
- * box hierarchy coarsens down to coarse_min cells, the finer boxes to 2.
- * mode: ELA_SOLVE_FP64 or ELA_SOLVE_MIXED. */
+/* nboxes: (nlev - 1) box counts; boxes: the boxes of level 1, then of level 2,
+ * ..., each lo[3], hi[3] (level-l node indices).  Level 0's multigrid hierarchy
+ * coarsens down to coarse_min cells.  mode: ELA_SOLVE_FP64 or ELA_SOLVE_MIXED. */
+int ela_amr_create_boxes(ela_ctx* ctx, const ela_domain* dom, int nlev, const int* nboxes, const int* boxes,
+                         int model_kind, int coarse_min, int mode, ela_amr** out);
+/* The same with one box per refined level: boxes holds (nlev - 1) records. */
 int ela_amr_create(ela_ctx* ctx, const ela_domain* dom, int nlev, const int* boxes, int model_kind, int coarse_min,
                    int mode, ela_amr** out);
 int ela_amr_destroy(ela_amr* a);
@@ -439,9 +451,11 @@ int ela_amr_set_bc(ela_amr* a, const int* types, const double* values);
 int ela_amr_bc_init(ela_amr* a, int lev, double* rhs_host_dense, int homogeneous);
 int ela_amr_set_uniform(ela_amr* a, int uniform);
 int ela_amr_set_cycle(ela_amr* a, int cycle, int nu1, int nu2);
-/* The box cycle of one level only (ela_amr_set_cycle sets every level).
- * Defaults: F-cycle with nu1 = nu2 = 2 on level 0, V-cycle with nu1 = nu2 = 1
- * on the refined boxes. */
+/* The cycle of one level only (ela_amr_set_cycle sets every level).  A level
+ * with a hierarchy (its union is one box): that hierarchy's cycle (defaults:
+ * F-cycle with nu1 = nu2 = 2 on level 0, V-cycle with nu1 = nu2 = 1 on refined
+ * levels).  Any other refined level: nu1 / nu2 Fsmooth calls (2 Jacobi sweeps
+ * each) before / after the coarse correction (default 2 / 2; cycle unused). */
 int ela_amr_set_level_cycle(ela_amr* a, int lev, int cycle, int nu1, int nu2);
 /* restart 0: the plain FAC iteration u += FAC(r). */
 int ela_amr_set_krylov(ela_amr* a, int restart);

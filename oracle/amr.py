@@ -51,8 +51,11 @@ def _face_sets(dim):
 
 
 class Nest:
-    """Levels 0..L.  n0: cells of level 0 per direction; boxes: [(lo, hi)] for levels 1..L,
-    node indices in the level's own index space (even); extent: domain lengths."""
+    """Levels 0..L.  n0: cells of level 0 per direction; boxes: for levels 1..L, either one box (lo, hi)
+    or a list of boxes [(lo, hi), ...] whose union is the level's refined region (node indices in the
+    level's own index space, even corners; the boxes of a level may share faces); extent: domain
+    lengths.  A level's arrays cover G_l, the bounding box of its union grown by 2 nodes, clipped to
+    the domain."""
 
     def __init__(self, n0, boxes, extent=None):
         self.dim = len(n0)
@@ -61,8 +64,12 @@ class Nest:
         ext = tuple(extent) if extent is not None else (1.0,) * dim
         self.N = [tuple(int(c) << l for c in n0) for l in range(self.nlev)]
         self.dx = [tuple(ext[d] / self.N[l][d] for d in range(dim)) for l in range(self.nlev)]
-        self.lo = [(0,) * dim] + [tuple(int(v) for v in b[0]) for b in boxes]
-        self.hi = [self.N[0]] + [tuple(int(v) for v in b[1]) for b in boxes]
+        self.boxes = [[((0,) * dim, self.N[0])]]
+        for b in boxes:
+            lst = [b] if np.ndim(b[0]) == 1 else list(b)
+            self.boxes.append([(tuple(int(v) for v in lo), tuple(int(v) for v in hi)) for lo, hi in lst])
+        self.lo = [tuple(min(bx[0][d] for bx in self.boxes[l]) for d in range(dim)) for l in range(self.nlev)]
+        self.hi = [tuple(max(bx[1][d] for bx in self.boxes[l]) for d in range(dim)) for l in range(self.nlev)]
         self.glo, self.ghi = [], []
         for l in range(self.nlev):
             if l == 0:
@@ -71,15 +78,50 @@ class Nest:
             else:
                 self.glo.append(tuple(max(self.lo[l][d] - 2, 0) for d in range(dim)))
                 self.ghi.append(tuple(min(self.hi[l][d] + 2, self.N[l][d]) for d in range(dim)))
+        self.cells = [self._union_cells(l) for l in range(self.nlev)]
         for l in range(1, self.nlev):
-            for d in range(dim):
-                lo, hi, N = self.lo[l][d], self.hi[l][d], self.N[l][d]
-                clo, chi = self.lo[l - 1][d], self.hi[l - 1][d]
-                assert lo % 2 == 0 and hi % 2 == 0 and 0 <= lo < hi <= N, "box corners must be even"
-                assert lo == 0 or lo // 2 >= clo + 2, "box must sit 2 coarse cells inside its parent"
-                assert hi == N or hi // 2 <= chi - 2, "box must sit 2 coarse cells inside its parent"
-                assert lo > 0 or clo == 0, "a box on the domain boundary needs a parent there"
-                assert hi < N or chi == self.N[l - 1][d], "a box on the domain boundary needs a parent there"
+            for lo, hi in self.boxes[l]:
+                for d in range(dim):
+                    assert lo[d] % 2 == 0 and hi[d] % 2 == 0 and 0 <= lo[d] < hi[d] <= self.N[l][d], \
+                        "box corners must be even"
+                # proper nesting: the coarsened box grown by 2 coarse cells (clipped to the domain) lies in
+                # the coarser level's union
+                rng = [(max(lo[d] // 2 - 2, 0), min(hi[d] // 2 + 2, self.N[l - 1][d])) for d in range(dim)]
+                sl = tuple(reversed([slice(a - self.glo[l - 1][d], b - self.glo[l - 1][d])
+                                     for d, (a, b) in enumerate(rng)]))
+                assert self.cells[l - 1][sl].all(), "box must sit 2 coarse cells inside its parent"
+
+    def _union_cells(self, l):
+        """Cells of G_l covered by the level's boxes (cell c spans nodes c..c+1)."""
+        shp = tuple(reversed([self.ghi[l][d] - self.glo[l][d] for d in range(self.dim)]))
+        m = np.zeros(shp, bool)
+        for lo, hi in self.boxes[l]:
+            sl = tuple(reversed([slice(max(lo[d], self.glo[l][d]) - self.glo[l][d],
+                                       min(hi[d], self.ghi[l][d]) - self.glo[l][d]) for d in range(self.dim)]))
+            m[sl] = True
+        return m
+
+    def _touch(self, l, cells):
+        """Per node of G_l: (some touching cell in `cells`, every touching in-domain cell in `cells`);
+        cells outside G_l count as not in `cells`, cells outside the domain do not count."""
+        dim = self.dim
+        pad = np.pad(cells, 1, constant_values=False)
+        dom = np.zeros(pad.shape, bool)
+        sl = []
+        for d in reversed(range(dim)):
+            # padded index j is global cell glo + j - 1, in the domain when 0 <= glo + j - 1 < N
+            a = max(0, 1 - self.glo[l][d])
+            b = min(pad.shape[dim - 1 - d], self.N[l][d] - self.glo[l][d] + 1)
+            sl.append(slice(a, b))
+        dom[tuple(sl)] = True
+        nshape = tuple(v + 1 for v in cells.shape)
+        anyc = np.zeros(nshape, bool)
+        allc = np.ones(nshape, bool)
+        for off in np.ndindex(*(2,) * dim):
+            w = tuple(slice(o, o + n) for o, n in zip(off, nshape))
+            anyc |= pad[w]
+            allc &= pad[w] | ~dom[w]
+        return anyc, allc
 
     # -------------------------------------------------------------- geometry
     def grid(self, l):
@@ -98,36 +140,39 @@ class Nest:
     def coords(self, l):
         return [g * self.dx[l][d] for d, g in enumerate(self.gidx(l))]
 
+    def _under_cells(self, l):
+        """Cells of G_l under level l+1's union (coarse cell c is under it when fine cell 2c is)."""
+        fc = self.cells[l + 1]
+        shp = self.cells[l].shape
+        P = [np.arange(shp[self.dim - 1 - d]) + self.glo[l][d] for d in range(self.dim)]  # global coarse cells
+        idx = []
+        for d in range(self.dim):
+            f = 2 * P[d] - self.glo[l + 1][d]
+            idx.append(f)
+        grids = np.meshgrid(*reversed(idx), indexing="ij")
+        valid = np.ones(shp, bool)
+        for gdim, gg in enumerate(grids):
+            valid &= (gg >= 0) & (gg < fc.shape[gdim])
+        clipped = [np.clip(gg, 0, fc.shape[gdim] - 1) for gdim, gg in enumerate(grids)]
+        out = valid & fc[tuple(clipped)]
+        return out
+
+    def under(self, l):
+        """Nodes of G_l under level l+1's union (touching at least one cell under it)."""
+        return self._touch(l, self._under_cells(l))[0]
+
     def classify(self, l):
-        """Boolean masks over G_l: inB, slave, covered (strictly, by l+1), interface, unknown."""
-        P = self.gidx(l)
-        dim = self.dim
-        inB = np.ones(P[0].shape, bool)
-        slave = np.zeros(P[0].shape, bool)
-        for d in range(dim):
-            inB &= (P[d] >= self.lo[l][d]) & (P[d] <= self.hi[l][d])
-        if l > 0:
-            for d in range(dim):
-                if self.lo[l][d] > 0:
-                    slave |= P[d] == self.lo[l][d]
-                if self.hi[l][d] < self.N[l][d]:
-                    slave |= P[d] == self.hi[l][d]
-            slave &= inB
-        covered = np.zeros(P[0].shape, bool)
-        iface = np.zeros(P[0].shape, bool)
+        """Boolean masks over G_l: inB (in the union), slave (on the union's boundary away from the
+        domain boundary), covered (every touching in-domain cell under level l+1), interface (under
+        level l+1 but not covered), unknown."""
+        inB, allin = self._touch(l, self.cells[l])
+        slave = inB & ~allin if l > 0 else np.zeros(inB.shape, bool)
+        covered = np.zeros(inB.shape, bool)
+        iface = np.zeros(inB.shape, bool)
         if l + 1 < self.nlev:
-            clo = [v // 2 for v in self.lo[l + 1]]
-            chi = [v // 2 for v in self.hi[l + 1]]
-            under = np.ones(P[0].shape, bool)
-            on = np.zeros(P[0].shape, bool)
-            for d in range(dim):
-                under &= (P[d] >= clo[d]) & (P[d] <= chi[d])
-                if self.lo[l + 1][d] > 0:
-                    on |= P[d] == clo[d]
-                if self.hi[l + 1][d] < self.N[l + 1][d]:
-                    on |= P[d] == chi[d]
-            iface = under & on
-            covered = under & ~on
+            anyu, allu = self._touch(l, self._under_cells(l))
+            covered = allu & inB
+            iface = anyu & ~allu
         unknown = inB & ~slave & ~covered
         return dict(inB=inB, slave=slave, covered=covered, iface=iface, unknown=unknown)
 
@@ -185,14 +230,17 @@ class Nest:
             cnt += use
         return out / cnt
 
+    def _under_nodes(self, l):
+        """Global indices (one array per direction) of the level-(l-1) nodes under level l's union."""
+        m = self.under(l - 1)
+        nz = np.nonzero(m)
+        return [nz[self.dim - 1 - d] + self.glo[l - 1][d] for d in range(self.dim)]
+
     def reflux(self, l, crse_res, fine_res):
-        """Coarse nodes under B_l / 2 take the restriction of the fine residual (read on G_l)."""
+        """Coarse nodes under level l's union take the restriction of the fine residual (read on G_l)."""
         dim = self.dim
         out = np.array(crse_res, dtype=np.float64).copy()
-        clo = [v // 2 for v in self.lo[l]]
-        chi = [v // 2 for v in self.hi[l]]
-        axes = [np.arange(clo[d], chi[d] + 1) for d in range(dim)]
-        C = list(reversed(np.meshgrid(*reversed(axes), indexing="ij")))
+        C = self._under_nodes(l)
         acc = np.zeros((fine_res.shape[0],) + C[0].shape)
         for off in np.ndindex(*(3,) * dim):
             w = np.ones(C[0].shape)
@@ -214,15 +262,12 @@ class Nest:
         return out
 
     def inject(self, l, crse, fine):
-        """averageDownSolutionRHS: coarse nodes under B_l / 2 take the coincident fine value."""
+        """averageDownSolutionRHS: coarse nodes under level l's union take the coincident fine value."""
         dim = self.dim
         out = np.array(crse, dtype=np.float64).copy()
-        clo = [v // 2 for v in self.lo[l]]
-        chi = [v // 2 for v in self.hi[l]]
-        dst = tuple(reversed([slice(clo[d] - self.glo[l - 1][d], chi[d] - self.glo[l - 1][d] + 1)
-                              for d in range(dim)]))
-        src = tuple(reversed([slice(self.lo[l][d] - self.glo[l][d], self.hi[l][d] - self.glo[l][d] + 1, 2)
-                              for d in range(dim)]))
+        C = self._under_nodes(l)
+        dst = tuple(reversed([C[d] - self.glo[l - 1][d] for d in range(dim)]))
+        src = tuple(reversed([2 * C[d] - self.glo[l][d] for d in range(dim)]))
         for c in range(out.shape[0]):
             out[c][dst] = fine[c][src]
         return out
@@ -391,14 +436,11 @@ class Nest:
         return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(nf, nc))
 
     def _inject_matrix(self, l):
-        """(coarse G_{l-1} nodes) x (fine G_l nodes): coincident values under B_l / 2."""
+        """(coarse G_{l-1} nodes) x (fine G_l nodes): coincident values under level l's union."""
         dim = self.dim
         cshape = self.shape(l - 1, 1)[1:]
         fshape = self.shape(l, 1)[1:]
-        clo = [v // 2 for v in self.lo[l]]
-        chi = [v // 2 for v in self.hi[l]]
-        axes = [np.arange(clo[d], chi[d] + 1) for d in range(dim)]
-        C = [c.ravel() for c in reversed(np.meshgrid(*reversed(axes), indexing="ij"))]
+        C = self._under_nodes(l)
         rl = np.zeros(C[0].size, dtype=np.int64)
         fl = np.zeros(C[0].size, dtype=np.int64)
         rs = fs = 1
@@ -413,10 +455,7 @@ class Nest:
         dim = self.dim
         cshape = self.shape(l - 1, 1)[1:]
         fshape = self.shape(l, 1)[1:]
-        clo = [v // 2 for v in self.lo[l]]
-        chi = [v // 2 for v in self.hi[l]]
-        axes = [np.arange(clo[d], chi[d] + 1) for d in range(dim)]
-        C = [c.ravel() for c in reversed(np.meshgrid(*reversed(axes), indexing="ij"))]
+        C = self._under_nodes(l)
         rl = np.zeros(C[0].size, dtype=np.int64)
         rs = 1
         for d in range(dim):

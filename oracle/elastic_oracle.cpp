@@ -32,7 +32,10 @@
 //   * boundary conditions: type[face][comp] (0 displacement, 1 traction,
 //     2 periodic, 3 neumann) and value[face][comp], faces numbered as the
 //     reference's BC::Operator::Elastic::Face enum (BC/Operator/Elastic.H:24-42).
+#include <omp.h>
+
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -900,11 +903,14 @@ void or_reflux(const or_grid* crse, int ncomp, double* crse_res, const int* fine
 // while every n is even and n / 2 >= coarse_min (4 when coarse_min <= 0: a
 // 2-cell coarsest grid with traction faces is near-singular; problems with
 // Displacement rows on every face can go down to 2 cells).
-// stats[0] iterations, [1] |r0|inf, [2] final |r|inf, [3] number of MG levels.
+// stats[0] iterations, [1] |r0|inf, [2] final |r|inf, [3] number of MG levels,
+// [4] seconds of set-up (coefficient coarsening, diagonals, bottom factor),
+// [5] seconds of the iterations.
 // hist (optional, cap entries) receives |r|inf after each iteration.
 int or_mg_solve(const or_grid* og, int kind, const double* C, const int* bctype, const double* b, double* u,
                 double tol_rel, double tol_abs, int max_iter, int max_fmg_iter, int bottom_kind, int levels,
                 int coarse_min, double* stats, double* hist, int cap) {
+    const auto t_start = std::chrono::steady_clock::now();
     MG mg;
     mg.kind = kind;
     mg.bctype = bctype;
@@ -937,6 +943,7 @@ int or_mg_solve(const or_grid* og, int kind, const double* C, const int* bctype,
         or_assemble(&mg.lev[bot].og, kind, mg.lev[bot].C.data(), bctype, a.data());
         mg.bottom.factor(a, (int)n);
     }
+    const auto t_setup = std::chrono::steady_clock::now();
     Level& L0 = mg.lev[0];
     const long n0 = mg.N(0);
     auto solution_residual = [&]() {
@@ -964,8 +971,15 @@ int or_mg_solve(const or_grid* og, int kind, const double* C, const int* bctype,
         stats[1] = r0;
         stats[2] = rn;
         stats[3] = (double)mg.lev.size();
+        const auto t_end = std::chrono::steady_clock::now();
+        stats[4] = std::chrono::duration<double>(t_setup - t_start).count();
+        stats[5] = std::chrono::duration<double>(t_end - t_setup).count();
     }
     return (rn <= target && rn == rn) ? 0 : 1;
 }
+
+// OpenMP threads of the oracle's loops (the CPU baseline's single-thread timing).
+void or_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int or_get_threads(void) { return omp_get_max_threads(); }
 
 }  // extern "C"

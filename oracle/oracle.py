@@ -78,6 +78,8 @@ def lib():
         L.or_post.argtypes = [P(_Grid), i, vp, vp, i, i, vp]
         L.or_newton_rhs.argtypes = [P(_Grid), i, vp, vp, vp, vp, vp, vp]
         L.or_newton_w_dw.argtypes = [P(_Grid), i, vp, vp, vp, d, vp, vp]
+        L.or_set_threads.argtypes = [i]
+        L.or_get_threads.restype = i
         _lib = L
     return _lib
 
@@ -333,13 +335,49 @@ def mg_solve(g, kind, C, bct, b, u0=None, tol_rel=1e-10, tol_abs=0.0, max_iter=2
     """Restated MLMG solve; returns (u, stats dict, residual history)."""
     C, bct, b = _c(C), _c(bct, np.int32), _c(b)
     u = np.zeros_like(b) if u0 is None else _c(u0).copy()
-    stats = np.zeros(4)
+    stats = np.zeros(6)
     hist = np.zeros(hist_cap)
     rc = lib().or_mg_solve(ctypes.byref(g.c()), kind, _p(C), _p(bct), _p(b), _p(u), tol_rel, tol_abs, max_iter,
                            max_fmg_iter, 0 if bottom == "direct" else 1, levels, coarse_min, _p(stats), _p(hist),
                            hist_cap)
     it = int(stats[0])
-    return u, {"converged": rc == 0, "iters": it, "res0": stats[1], "res": stats[2], "levels": int(stats[3])}, hist[:it]
+    return u, {"converged": rc == 0, "iters": it, "res0": stats[1], "res": stats[2], "levels": int(stats[3]),
+               "setup_s": stats[4], "iter_s": stats[5]}, hist[:it]
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's loops."""
+    lib().or_set_threads(int(n))
+
+
+def get_threads():
+    return int(lib().or_get_threads())
+
+
+def voronoi_ids(seeds_q, n, q):
+    """Nearest-seed id per node of an (n[0], n[1], n[2])-cell grid, node i at i*q in seed units, ties to the
+    lowest seed index; shape (nz+1, ny+1, nx+1).  Squared distances of these integer coordinates are below
+    2^53, so the float64 distances of the k-d tree are exact and a tie is an exact equality; the 4 nearest
+    candidates are compared and the lowest index among the equal nearest wins."""
+    from scipy.spatial import cKDTree
+    S = np.asarray(seeds_q, dtype=np.float64)
+    tree = cKDTree(S)
+    nx, ny, nz = (v + 1 for v in n)
+    out = np.empty((nz, ny, nx), dtype=np.int32)
+    J, I = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    kk = min(4, len(S))
+    for k in range(nz):
+        P = np.stack([I.ravel() * float(q), J.ravel() * float(q), np.full(I.size, k * float(q))], axis=1)
+        d, idx = tree.query(P, k=kk, workers=-1)
+        if kk == 1:
+            out[k] = idx.reshape(ny, nx)
+            continue
+        best = idx[:, 0].copy()
+        for c in range(1, kk):
+            tie = (d[:, c] == d[:, 0]) & (idx[:, c] < best)
+            best[tie] = idx[tie, c]
+        out[k] = best.reshape(ny, nx)
+    return out
 
 
 def lame(E, nu):

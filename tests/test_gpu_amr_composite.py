@@ -10,7 +10,7 @@ import pytest
 
 import anisotropicfracture_amd as E
 from tests.test_oracle import _uniaxial_bcs
-from tests.test_oracle_amr import central, trig_inputs, uniaxial_inputs
+from tests.test_oracle_amr import L_SHAPE, central, chop, trig_inputs, uniaxial_inputs
 
 pytestmark = pytest.mark.gpu
 
@@ -24,7 +24,7 @@ def amr(oracle):
 
 
 def make(nest, kind, C, t, mode, coarse_min=4, uniform=False, values=None):
-    a = E.ElasticAMR(nest.N[0], [(nest.lo[l], nest.hi[l]) for l in range(1, nest.nlev)], kind, mode=mode,
+    a = E.ElasticAMR(nest.N[0], [nest.boxes[l] for l in range(1, nest.nlev)], kind, mode=mode,
                      coarse_min=coarse_min)
     for l in range(nest.nlev):
         assert a.shape(l) == nest.shape(l, nest.dim)
@@ -53,14 +53,16 @@ def mixed_bcs(oracle):
                                 "ZHI": {0: (1, 0.0), 1: (3, 0.0), 2: (0, 0.01)}}, default=(1, 0.0))
 
 
-@pytest.mark.parametrize("boxes", [
-    [((4,) * 3, (12,) * 3)],
-    [((4,) * 3, (12,) * 3), ((12,) * 3, (20,) * 3)],
-    [((0, 4, 4), (16, 12, 12))],
+@pytest.mark.parametrize("n0,boxes", [
+    (8, [((4,) * 3, (12,) * 3)]),
+    (8, [((4,) * 3, (12,) * 3), ((12,) * 3, (20,) * 3)]),
+    (8, [((0, 4, 4), (16, 12, 12))]),
+    (12, L_SHAPE),
+    (12, [chop((4, 4, 4), (20, 12, 12), 4), chop((12, 12, 12), (20, 20, 20), 4)]),
 ])
-def test_composite_residual_matches_oracle(oracle, amr, boxes):
+def test_composite_residual_matches_oracle(oracle, amr, n0, boxes):
     rng = np.random.default_rng(11)
-    nest = amr.Nest((8, 8, 8), boxes)
+    nest = amr.Nest((n0, n0, n0), boxes)
     C = random_iso(nest, rng)
     t, v = mixed_bcs(oracle)
     u = [rng.standard_normal(nest.shape(l, 3)) for l in range(nest.nlev)]
@@ -216,7 +218,60 @@ def test_composite_2d_matches_oracle(oracle, amr, mode):
 
 
 def test_bad_nesting_is_rejected():
+    # 1 coarse cell inside its parent box (away from the domain boundary): the coarsened box grown by 2
+    # coarse cells leaves the parent's union
     with pytest.raises(E.ElaError):
-        E.ElasticAMR((8, 8, 8), [((2, 2, 2), (12, 12, 12))], ISO)  # 1 coarse cell inside its parent
+        E.ElasticAMR((8, 8, 8), [((4, 4, 4), (12, 12, 12)), ((10, 10, 10), (20, 20, 20))], ISO)
+    # an L-shaped parent whose re-entrant corner is too close to the fine box
+    with pytest.raises(E.ElaError):
+        E.ElasticAMR((12, 12, 12), L_SHAPE[0:1] + [[((18, 18, 12), (22, 22, 16))]], ISO)
     with pytest.raises(E.ElaError):
         E.ElasticAMR((8, 8, 8), [((3, 4, 4), (12, 12, 12))], ISO)  # odd corner
+
+
+@pytest.mark.parametrize("mode", [E.SOLVE_FP64, E.SOLVE_MIXED])
+def test_l_shaped_level_solve_matches_oracle_direct(oracle, amr, mode):
+    """A refined level that is the union of three boxes in an L shape (its re-entrant corner gives
+    coarse/fine faces of both orientations): the composite solution equals the oracle's sparse direct
+    solve of the same composite system to 1e-8."""
+    rng = np.random.default_rng(5)
+    nest = amr.Nest((12, 12, 12), L_SHAPE)
+    C = random_iso(nest, rng)
+    t, v = mixed_bcs(oracle)
+    b = [nest.bc_init(l, t, v, np.zeros(nest.shape(l, 3))) for l in range(nest.nlev)]
+    ref = nest.solve_direct(ISO, C, t, b)
+    a = make(nest, ISO, C, t, mode)
+    u, st = a.solve([np.zeros_like(x) for x in b], b, tol_rel=1e-10, max_iter=300)
+    print("L shape mode", mode, "iters", st["iters"], "res/res0", st["res"] / st["res0"])
+    assert st["converged"]
+    scale = max(np.abs(x).max() for x in ref)
+    for l in range(nest.nlev):
+        m = nest.classify(l)["inB"]
+        assert np.abs((u[l] - ref[l])[:, m]).max() <= 1e-8 * scale
+    a.close()
+
+
+def test_chopped_levels_solve_bit_for_bit_like_one_box(oracle, amr):
+    """max_grid_size chopping changes the box list, not the union: the composite solve of a 3-level nest
+    whose levels are chopped into boxes of at most 8 cells gives the bits of the unchopped nest."""
+    from anisotropicfracture_amd import configs as K
+    cfg = K.crack_nest(32)
+    outs = []
+    for size in (0, 8):
+        boxes = [[b] if size == 0 else chop(b[0], b[1], size) for b in cfg["regions"]]
+        nest = amr.Nest(cfg["n"], boxes)
+        C = []
+        for l in range(nest.nlev):
+            X = nest.coords(l)
+            C.append(K.crack_nest_model(cfg, X[0], X[1], X[2]))
+        t, v = cfg["bc_type"], cfg["bc_value"]
+        b = [nest.bc_init(l, t, v, np.zeros(nest.shape(l, 3))) for l in range(nest.nlev)]
+        a = make(nest, ISO, C, t, E.SOLVE_MIXED, values=v)
+        u, st = a.solve([np.zeros_like(x) for x in b], b, tol_rel=1e-10, max_iter=300)
+        assert st["converged"]
+        outs.append((u, st["iters"], sum(len(x) for x in boxes)))
+        a.close()
+    assert outs[1][2] > 3 * outs[0][2]
+    assert outs[0][1] == outs[1][1]
+    for x, y in zip(outs[0][0], outs[1][0]):
+        assert np.array_equal(x, y)

@@ -450,3 +450,19 @@ def test_post_processing_closed_forms(oracle, voigt):
         assert np.allclose(s_out[c], sig[a, b], rtol=0, atol=1e-13)
     en = oracle.post(g, oracle.ISO, C, u, 2)
     assert np.allclose(en, np.sum(sig * eps), rtol=1e-14, atol=0)  # twice the strain energy density
+
+
+def test_voronoi_ids_with_kdtree_match_brute_force(oracle):
+    """The k-d tree grain ids (used on large grids) equal the brute-force exact-integer nearest seed with
+    ties to the lowest index, including planted ties (mirror-image seeds)."""
+    rng = np.random.default_rng(8)
+    n = (12, 10, 14)
+    seeds = K.voronoi_seeds(rng, 9, n)
+    # exact ties: seed 9 mirrors seed 2 through the plane x = 6 cells, seed 10 duplicates seed 4
+    m = seeds[2].copy()
+    m[0] = 2 * 6 * K.VORONOI_Q - m[0]
+    seeds = np.vstack([seeds, m, seeds[4]])
+    want = K.voronoi_grain_ids(seeds, n)
+    got = oracle.voronoi_ids(seeds, n, K.VORONOI_Q)
+    assert np.array_equal(got, want)
+    assert not np.any(got == 10)  # the duplicate never wins against its lower index

@@ -155,3 +155,88 @@ def test_direct_solution_has_zero_composite_residual(oracle, amr):
     r, _ = nest.residual(ISO, C, t, b, u)
     r0, _ = nest.residual(ISO, C, t, b, [np.zeros_like(x) for x in b])
     assert max(np.abs(x).max() for x in r) < 1e-10 * max(np.abs(x).max() for x in r0)
+
+
+# ---------------------------------------------------------------- levels made of several boxes
+L_SHAPE = [[((4, 4, 4), (12, 12, 12)), ((12, 4, 4), (20, 12, 12)), ((4, 12, 4), (12, 20, 12))]]
+
+
+def chop(lo, hi, size):
+    """A box cut into boxes of at most `size` cells per direction (AMReX's max_grid_size chopping)."""
+    axes = [list(range(lo[d], hi[d], size)) + [hi[d]] for d in range(len(lo))]
+    out = []
+    for corner in np.ndindex(*[len(a) - 1 for a in axes]):
+        out.append((tuple(axes[d][corner[d]] for d in range(len(lo))),
+                    tuple(axes[d][corner[d] + 1] for d in range(len(lo)))))
+    return out
+
+
+def classes_by_points(nest, l):
+    """Node classes restated with real coordinates and closed/open box containment: a node is in the
+    union when it lies in a closed box; a touching cell (its centre at the node +- h/2) counts when its
+    centre is inside the domain; it is in the union (under level l+1) when the centre lies in a box (a
+    level-(l+1) box scaled by 1/2)."""
+    dim = nest.dim
+    X = [x.ravel() for x in nest.gidx(l)]
+    N = nest.N[l]
+    inbox = lambda p, boxes, s: any(all(lo[d] * s <= p[d] <= hi[d] * s for d in range(dim)) for lo, hi in boxes)
+    out = {k: np.zeros(X[0].size, bool) for k in ("inB", "slave", "covered", "iface")}
+    for q in range(X[0].size):
+        p = [float(X[d][q]) for d in range(dim)]
+        out["inB"][q] = inbox(p, nest.boxes[l], 1.0)
+        cells = [[p[d] + o[d] - 0.5 for d in range(dim)] for o in np.ndindex(*(2,) * dim)]
+        cells = [c for c in cells if all(0 < c[d] < N[d] for d in range(dim))]
+        inu = [inbox(c, nest.boxes[l], 1.0) for c in cells]
+        out["slave"][q] = l > 0 and out["inB"][q] and not all(inu)
+        if l + 1 < nest.nlev:
+            und = [inbox(c, nest.boxes[l + 1], 0.5) for c in cells]
+            out["covered"][q] = all(und) and out["inB"][q]
+            out["iface"][q] = any(und) and not all(und)
+    shp = nest.shape(l, 1)[1:]
+    return {k: v.reshape(shp) for k, v in out.items()}
+
+
+@pytest.mark.parametrize("boxes", [L_SHAPE, [chop((4, 4, 4), (20, 12, 12), 4)],
+                                   [[((4, 4, 4), (12, 12, 12)), ((12, 4, 4), (20, 12, 12))], [((12, 12, 12), (20, 20, 20))]]])
+def test_union_classes_match_a_point_containment_restatement(amr, boxes):
+    nest = amr.Nest((12, 12, 12), boxes)
+    for l in range(nest.nlev):
+        want = classes_by_points(nest, l)
+        got = nest.classify(l)
+        for k in want:
+            assert np.array_equal(got[k], want[k]), (l, k)
+
+
+def test_chopped_levels_give_the_same_composite_problem(oracle, amr):
+    """A region chopped into boxes of at most 4 cells (max_grid_size) is the same union: the same
+    classes, the same composite residual and the same direct solution as the region as one box."""
+    one = amr.Nest((8, 8, 8), [((4, 4, 4), (12, 12, 12))])
+    many = amr.Nest((8, 8, 8), [chop((4, 4, 4), (12, 12, 12), 4)])
+    assert len(many.boxes[1]) == 8
+    for l in range(2):
+        a, b = one.classify(l), many.classify(l)
+        for k in a:
+            assert np.array_equal(a[k], b[k])
+    C, t, bb = uniaxial_inputs(one, oracle)
+    rng = np.random.default_rng(4)
+    u = [rng.standard_normal(one.shape(l, 3)) for l in range(2)]
+    r1, _ = one.residual(ISO, C, t, bb, u)
+    r2, _ = many.residual(ISO, C, t, bb, u)
+    for x, y in zip(r1, r2):
+        assert np.array_equal(x, y)
+
+
+def test_uniaxial_composite_exact_on_an_l_shaped_level(oracle, amr):
+    """UniaxialTest (Elastic_UniaxialTest.cpp:13-206) with a level that is the union of three boxes in an
+    L shape (its re-entrant corner makes coarse/fine faces of both orientations): u_x = 0.1 x is still
+    the composite solution, and the direct solve's composite residual is at round-off."""
+    nest = amr.Nest((12, 12, 12), L_SHAPE)
+    C, t, b = uniaxial_inputs(nest, oracle)
+    u = nest.solve_direct(ISO, C, t, b)
+    for l in range(nest.nlev):
+        X = nest.coords(l)
+        m = nest.classify(l)["inB"]
+        assert np.abs((u[l][0] - 0.1 * X[0])[m]).max() < 1e-10
+        assert np.abs(u[l][1:][:, m]).max() < 1e-10
+    r, _ = nest.residual(ISO, C, t, b, u)
+    assert max(np.abs(x).max() for x in r) < 1e-10
