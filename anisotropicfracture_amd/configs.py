@@ -182,7 +182,7 @@ def chop_box(lo, hi, max_grid_size):
     return out
 
 
-def crack_nest_boxes(n=128, max_grid_size=None):
+def crack_nest_boxes(n=128, max_grid_size=None, front="physical"):
     """The refined regions of the mode-I crack on an n^3 base (ratio 2, node indices of each level).
 
     Level 1 covers |z - 0.5| <= 0.125, full x and y.  Level 2 covers |z - 0.5| <= 0.0625 and the crack
@@ -193,6 +193,8 @@ def crack_nest_boxes(n=128, max_grid_size=None):
     n2 = 4 * n
     x0 = int(np.floor(0.275 * n2 / 2)) * 2
     x1 = int(np.ceil(0.525 * n2 / 2)) * 2
+    if front == "blocks":  # rounded out to whole 32-cell blocks at n = 128: x in [0.25, 0.5625]
+        x0, x1 = n, 9 * n // 4
     l1 = ((0, 0, 3 * n // 4), (2 * n, 2 * n, 5 * n // 4))
     l2 = ((x0, 0, 7 * n // 4), (x1, 4 * n, 9 * n // 4))
     if max_grid_size is None:
@@ -224,7 +226,7 @@ G_LITERAL = (1.0, 0.0, 1e-4)          # eta^2 + 1e-4: soft bulk, stiff crack pla
 G_DAMAGE = (1.0, -2.0, 1.0 + 1e-4)    # (1 - eta)^2 + 1e-4: eta as damage (Degradable/Isotropic.H:78-79)
 
 
-def crack_nest(n=128, g=G_DAMAGE, E=1.0, nu=0.3, width=0.01, load=0.005, max_grid_size=32):
+def crack_nest(n=128, g=G_DAMAGE, E=1.0, nu=0.3, width=0.01, load=0.005, max_grid_size=32, front="physical"):
     """3D AMR mode-I crack: base n^3 cells on [0,1]^3, 3 levels (ratio 2).
 
     Stiffness g(eta) C_iso(E, nu) per node with g(eta) = a eta^2 + b eta + c.  The default is the
@@ -236,7 +238,7 @@ def crack_nest(n=128, g=G_DAMAGE, E=1.0, nu=0.3, width=0.01, load=0.005, max_gri
     """
     lam, mu = lame(E, nu)
     t, v = mode1_bcs(load)
-    return {"n": (n, n, n), "boxes": crack_nest_boxes(n, max_grid_size), "regions": crack_nest_boxes(n),
+    return {"n": (n, n, n), "boxes": crack_nest_boxes(n, max_grid_size, front), "regions": crack_nest_boxes(n, None, front),
             "max_grid_size": max_grid_size, "lame": (lam, mu), "g": tuple(g), "width": width,
             "bc_type": t, "bc_value": v}
 

@@ -36,7 +36,8 @@
 // onto the finer level (every node) and one more correction.  A level whose
 // union is one box (level 0, or a region chopped into max_grid_size boxes)
 // corrects with a V(1,1) cycle of a multigrid hierarchy on that box
-// (homogeneous Dirichlet rows on its coarse/fine faces); any other union
+// (homogeneous Dirichlet rows on its coarse/fine faces) when the box coarsens
+// to a dense-solvable level; any other level
 // smooths, as AMReX's MLMG does on fine AMR levels: nu1 / nu2 Fsmooth calls (2
 // weighted-Jacobi sweeps each, omega = 2/3, on the level's non-fill nodes, fill
 // nodes held).  Hierarchy cycles run in FP32 (mixed) or FP64, the smoothing and
@@ -726,6 +727,23 @@ int level_extent(LevelBoxes& B, int l) {
     return ELA_OK;
 }
 
+// Whether a refined level's union box [lo, hi] coarsens (by 2 while every side
+// is even and its half >= 2 cells) to a level whose dense inverse the
+// hierarchy builds (at most 3 * 9^3 unknowns).  Otherwise the level smooths.
+bool dense_coarsest(const int lo[3], const int hi[3], int dim) {
+    int n[3] = {1, 1, 1};
+    for (int d = 0; d < dim; ++d) n[d] = hi[d] - lo[d];
+    for (;;) {
+        bool can = true;
+        for (int d = 0; d < dim; ++d) can = can && n[d] % 2 == 0 && n[d] / 2 >= 2;
+        if (!can) break;
+        for (int d = 0; d < dim; ++d) n[d] /= 2;
+    }
+    long long unk = dim;
+    for (int d = 0; d < dim; ++d) unk *= n[d] + 1;
+    return unk <= 3 * 9 * 9 * 9;
+}
+
 // Cells of G covered by the level's boxes (cell c spans nodes c..c+1), dense, x fastest.
 void union_cells(const LevelBoxes& B, std::vector<unsigned char>& cells) {
     const long long cx = B.gc(0), cy = B.gc(1), cz = B.gc(2);
@@ -938,7 +956,8 @@ int ela_amr_create_boxes(ela_ctx* ctx, const ela_domain* dom, int nlev, const in
             if (d < dim) bbox_cells *= (size_t)(v.uhi[d] - v.ulo[d]);
         }
         for (unsigned char c : cells[l]) ucells += c;
-        if (l == 0 || ucells == bbox_cells) {  // its own stream until the composite's is shared
+        if (l == 0 || (ucells == bbox_cells && dense_coarsest(v.ulo, v.uhi, dim))) {
+            // its own stream until the composite's is shared
             ela_domain bd{};
             bd.dim = dim;
             for (int d = 0; d < 3; ++d) {

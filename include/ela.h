@@ -422,9 +422,10 @@ int ela_average_down(const ela_level_layout* crse, void* crse_sol, const ela_lev
  * take its values (averageDownSolutionRHS, Operator.cpp:626-641); at coarse
  * nodes on the coarse/fine interface the equation is the reflux of the fine
  * residual (Operator.cpp:728-839).  The solve is flexible GMRES(restart) in
- * FP64 preconditioned by one FAC cycle: a level whose union is one box (level
- * 0, or a box chopped into max_grid_size boxes) corrects with a cycle of a
- * multigrid hierarchy on that box (FP32 or FP64 by mode), any other union
+ * FP64 preconditioned by one FAC cycle: level 0, and a refined level whose
+ * union is one box (e.g. a box chopped into max_grid_size boxes) that
+ * coarsens by 2 to a dense-solvable level, correct with a cycle of a
+ * multigrid hierarchy on that box (FP32 or FP64 by mode); any other level
  * with weighted-Jacobi smoothing (FP64), as AMReX's MLMG smooths fine AMR
  * levels. */
 typedef struct ela_amr ela_amr;
