@@ -72,7 +72,7 @@ class Request(ctypes.Structure):
                 ("C11", ctypes.c_double), ("C12", ctypes.c_double), ("C44", ctypes.c_double),
                 ("crack", ctypes.c_int), ("crack_p", ctypes.c_double * 3), ("crack_nrm", ctypes.c_double * 3),
                 ("crack_tv", ctypes.c_double * 3), ("crack_width", ctypes.c_double), ("crack_g", ctypes.c_double * 3),
-                ("G", ctypes.c_double * 9), ("u_out", ctypes.c_void_p)]
+                ("G", ctypes.c_double * 9), ("u_out", ctypes.c_void_p), ("max_cycles", ctypes.c_int)]
 
 
 class ServeSummary(ctypes.Structure):
@@ -174,6 +174,10 @@ SIGNATURES = {
                            _I, _P], _I),
     "ela_average_down_window": ([ctypes.POINTER(_Layout), ctypes.POINTER(_Layout), _P, _P, ctypes.POINTER(_Layout), _P,
                                  _P, _I, _I, _P], _I),
+    "ela_server_start": ([_P, _D, _D, _I, _I, ctypes.POINTER(_P)], _I),
+    "ela_server_submit": ([_P, _P, ctypes.POINTER(ctypes.c_int64)], _I),
+    "ela_server_poll": ([_P, ctypes.c_int64, ctypes.POINTER(_I), _P], _I),
+    "ela_server_stop": ([_P, _I], _I),
     "ela_serve": ([_P, _I, ctypes.POINTER(Request), _D, _D, _I, _I, _D, ctypes.POINTER(Result),
                    ctypes.POINTER(ServeSummary)], _I),
     "ela_batch_smoother_times": ([_P, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_I)], _I),
@@ -911,6 +915,61 @@ class RVE:
                 self.batch._members.remove(self)
 
 
+def _fill_request(r, q):
+    """An ela_request from a request dict (arrival, n, seeds, R, G; optional crack, C11/C12/C44,
+    max_cycles); returns the arrays it points into (keep them alive while the request is in use)."""
+    from . import configs as K
+    sd = np.ascontiguousarray(q["seeds"], dtype=np.int64)
+    Rm = np.ascontiguousarray(q["R"], dtype=np.float64).reshape(-1, 9)
+    r.arrival, r.n, r.nseeds, r.q = float(q.get("arrival", 0.0)), int(q["n"]), sd.shape[0], K.VORONOI_Q
+    r.seeds, r.R = sd.ctypes.data, Rm.ctypes.data
+    r.C11, r.C12, r.C44 = q.get("C11", K.C11), q.get("C12", K.C12), q.get("C44", K.C44)
+    cr = q.get("crack")
+    r.crack = 1 if cr is not None else 0
+    if cr is not None:
+        p, nrm, tv, width, g = cr
+        r.crack_p[:], r.crack_nrm[:], r.crack_tv[:] = list(p), list(nrm), list(tv)
+        r.crack_width = float(width)
+        r.crack_g[:] = list(g)
+    r.G[:] = list(np.asarray(q["G"], dtype=np.float64).reshape(9))
+    r.max_cycles = int(q.get("max_cycles", 0))
+    return [sd, Rm]
+
+
+class Server:
+    """Online serving on a batch (ela_server_*): a host thread admits submitted requests into the running
+    continuous-batching loop.  submit() returns a ticket; poll() returns None until the result is ready."""
+
+    def __init__(self, batch, tol_rel=1e-10, tol_abs=0.0, max_cycles=100, max_active=256):
+        p = ctypes.c_void_p()
+        _check(lib().ela_server_start(batch.ptr, tol_rel, tol_abs, max_cycles, max_active, ctypes.byref(p)))
+        self.ptr, self.batch, self._hold = p, batch, {}
+
+    def submit(self, q, u_out=None):
+        r = Request()
+        arrays = _fill_request(r, q)
+        if u_out is not None:
+            r.u_out = u_out.ctypes.data
+            arrays.append(u_out)
+        t = ctypes.c_int64()
+        _check(lib().ela_server_submit(self.ptr, ctypes.byref(r), ctypes.byref(t)))
+        self._hold[t.value] = arrays
+        return t.value
+
+    def poll(self, ticket):
+        done, res = ctypes.c_int(), Result()
+        _check(lib().ela_server_poll(self.ptr, ctypes.c_int64(ticket), ctypes.byref(done), ctypes.byref(res)))
+        if not done.value:
+            return None
+        self._hold.pop(ticket, None)
+        return {k: getattr(res, k) for k, _ in Result._fields_}
+
+    def stop(self, drain=True):
+        if self.ptr:
+            p, self.ptr = self.ptr, None
+            _check(lib().ela_server_stop(p, int(bool(drain))))
+
+
 class Batch:
     """Batched independent solves (mode SOLVE_MIXED or SOLVE_FP64)."""
 
@@ -970,28 +1029,13 @@ class Batch:
         """Run the request loop (ela_serve) over `requests` (dicts: arrival, n, seeds, R, G and optional
         crack = (p, nrm, tv, width, (g2, g1, g0))), sorted by arrival.  Solutions of the request indices in
         `keep` are returned.  Returns (results: list of dicts, solutions: {index: array})."""
-        from . import configs as K
         reqs = (Request * len(requests))()
         hold, sols = [], {}
         for i, q in enumerate(requests):
-            r = reqs[i]
-            sd = np.ascontiguousarray(q["seeds"], dtype=np.int64)
-            Rm = np.ascontiguousarray(q["R"], dtype=np.float64).reshape(-1, 9)
-            hold += [sd, Rm]
-            r.arrival, r.n, r.nseeds, r.q = float(q["arrival"]), int(q["n"]), sd.shape[0], K.VORONOI_Q
-            r.seeds, r.R = sd.ctypes.data, Rm.ctypes.data
-            r.C11, r.C12, r.C44 = q.get("C11", K.C11), q.get("C12", K.C12), q.get("C44", K.C44)
-            cr = q.get("crack")
-            r.crack = 1 if cr is not None else 0
-            if cr is not None:
-                p, nrm, tv, width, g = cr
-                r.crack_p[:], r.crack_nrm[:], r.crack_tv[:] = list(p), list(nrm), list(tv)
-                r.crack_width = float(width)
-                r.crack_g[:] = list(g)
-            r.G[:] = list(np.asarray(q["G"], dtype=np.float64).reshape(9))
+            hold += _fill_request(reqs[i], q)
             if i in keep:
                 sols[i] = np.zeros((3,) + (int(q["n"]) + 1,) * 3)
-                r.u_out = sols[i].ctypes.data
+                reqs[i].u_out = sols[i].ctypes.data
         res = (Result * len(requests))()
         summ = ServeSummary()
         _check(lib().ela_serve(self.ptr, len(requests), reqs, tol_rel, tol_abs, max_cycles, max_active, max_seconds,

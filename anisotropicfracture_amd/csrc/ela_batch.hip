@@ -27,10 +27,14 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/ela.h"
@@ -750,8 +754,10 @@ struct ela_rve {
     double t_submit = 0, t_done = 0;
 };
 
+struct ela_server;
 struct ela_batch {
     ela_ctx* ctx = nullptr;
+    ela_server* server = nullptr;  // an online server running on this batch
     int mode = ELA_SOLVE_MIXED;
     size_t tsize = 4;  // bytes of the cycle precision
     hipStream_t stream = nullptr;
@@ -1411,6 +1417,263 @@ int prepare(ela_batch* B, int count, ela_rve* const* rves) {
     return rc;
 }
 
+// ------------------------------------------------------------------ serving core
+// One request with its arrays owned by the loop (an online submit copies them).
+struct OwnedRequest {
+    ela_request q{};
+    std::vector<int64_t> seeds;
+    std::vector<double> R;
+    long long ticket = 0;
+    double t_submit = 0;
+};
+
+// A member for one request: created against the HBM budget, its model and right-hand side evaluated.
+int make_member(ela_batch* B, const ela_request& q, ela_rve** out) {
+    *out = nullptr;
+    ela_rve* r = nullptr;
+    int rc = ela_rve_create(B, q.n, std::max(1, q.nseeds), q.crack ? 1 : 0, &r);
+    if (rc) return rc;
+    rc = ela_rve_set_voronoi_cubic(r, q.nseeds, q.seeds, q.q, q.R, q.C11, q.C12, q.C44);
+    if (!rc && q.crack)
+        rc = ela_rve_set_crack(r, q.crack_p, q.crack_nrm, q.crack_tv, q.crack_width, q.crack_g[0], q.crack_g[1],
+                               q.crack_g[2]);
+    if (!rc) rc = ela_rve_set_rhs_affine(r, q.G);
+    if (rc) {
+        ela_rve_destroy(r);
+        return rc;
+    }
+    *out = r;
+    return ELA_OK;
+}
+
+// Set-up of an admitted group.  A member whose set-up fails (a non-finite or zero
+// diagonal, a singular coarsest operator) fails alone: the group is then set up
+// member by member, which gives each member the bits it gets in any group.
+// status[m] receives each member's result; the return value is a fatal error only.
+int prepare_group(ela_batch* B, const std::vector<ela_rve*>& group, std::vector<int>& status) {
+    status.assign(group.size(), ELA_OK);
+    const int rc = prepare(B, (int)group.size(), group.data());
+    if (rc == ELA_OK) return ELA_OK;
+    if (rc != ELA_ERR_NAN) return rc;
+    for (size_t m = 0; m < group.size(); ++m) {
+        ela_rve* one = group[m];
+        const int r1 = prepare(B, 1, &one);
+        if (r1 != ELA_OK && r1 != ELA_ERR_NAN) return r1;
+        status[m] = r1;
+    }
+    return ELA_OK;
+}
+
+// The continuous-batching loop of ela_serve and of the online server: requests
+// waiting for admission, members in flight, and the time breakdown.
+struct ServeCore {
+    ela_batch* B;
+    double tol_rel, tol_abs;
+    int max_cycles, max_active;
+    double t0;
+    std::deque<std::pair<ela_request, long long>> waiting;  // request, caller id
+    std::vector<std::pair<ela_rve*, long long>> live;
+    std::vector<std::pair<long long, double*>> uout;          // caller id -> u_out
+    size_t live_bytes = 0;
+    long long done = 0;
+    ela_serve_summary sm{};
+    ServeCore(ela_batch* b, double tr, double ta, int mc, int ma)
+        : B(b), tol_rel(tr), tol_abs(ta), max_cycles(mc), max_active(ma), t0(now_s()) {}
+    bool busy() const { return !B->active.empty() || !B->queued.empty(); }
+    static ela_result ended(int status, double t) {
+        ela_result o{};
+        o.status = status;
+        o.admitted = o.finished = t;
+        return o;
+    }
+    // Admit waiting requests first come, first served while memory and max_active allow.
+    template <class Sink>
+    int admit(Sink&& sink) {
+        const double ta0 = now_s();
+        std::vector<ela_rve*> group;
+        std::vector<std::pair<ela_request, long long>> gq;
+        while (!waiting.empty() && (int)live.size() + (int)group.size() < max_active) {
+            const auto& w = waiting.front();
+            ela_rve* r = nullptr;
+            const int crc = make_member(B, w.first, &r);
+            if (crc == ELA_ERR_OOM && (!live.empty() || !group.empty())) break;  // wait for memory
+            if (crc) {  // a request that cannot be served (bad size or model, or larger than the whole budget)
+                sink(w.second, ended(crc, now_s() - t0));
+                ++done;
+                waiting.pop_front();
+                continue;
+            }
+            group.push_back(r);
+            gq.push_back(w);
+            live_bytes += r->bytes;
+            waiting.pop_front();
+        }
+        sm.t_admit += now_s() - ta0;
+        if (group.empty()) return ELA_OK;
+        const double tp0 = now_s();
+        std::vector<int> st;
+        if (int rc = prepare_group(B, group, st)) return rc;
+        sm.t_prepare += now_s() - tp0;
+        ++sm.groups;
+        sm.admitted += (int)group.size();
+        const double ta = now_s() - t0;
+        for (size_t m = 0; m < group.size(); ++m) {
+            const ela_request& q = gq[m].first;
+            if (st[m] != ELA_OK) {
+                live_bytes -= group[m]->bytes;
+                ela_rve_destroy(group[m]);
+                ela_result o = ended(st[m], ta);
+                sink(gq[m].second, o);
+                ++done;
+                continue;
+            }
+            if (int rc = ela_batch_submit(B, group[m], tol_rel, tol_abs, q.max_cycles > 0 ? q.max_cycles : max_cycles))
+                return rc;
+            live.push_back({group[m], gq[m].second});
+            if (q.u_out) uout.push_back({gq[m].second, q.u_out});
+            admitted_at.push_back({gq[m].second, ta});
+        }
+        sm.peak_active = std::max(sm.peak_active, (int)live.size());
+        sm.peak_bytes = std::max(sm.peak_bytes, live_bytes);
+        return ELA_OK;
+    }
+    std::vector<std::pair<long long, double>> admitted_at;
+    template <class V>
+    static typename V::value_type::second_type take(V& v, long long id, typename V::value_type::second_type dflt) {
+        for (size_t q = 0; q < v.size(); ++q)
+            if (v[q].first == id) {
+                auto x = v[q].second;
+                v[q] = v.back();
+                v.pop_back();
+                return x;
+            }
+        return dflt;
+    }
+    // One F-cycle over every member in flight; finished members are reported, copied out and freed.
+    template <class Sink>
+    int step_retire(Sink&& sink) {
+        const double ts0 = now_s();
+        if (int rc = step(B)) return rc;
+        sm.t_step += now_s() - ts0;
+        ++sm.steps;
+        const double tr0 = now_s();
+        std::vector<std::pair<ela_rve*, long long>> keep;
+        for (auto& pr : live) {
+            ela_rve* r = pr.first;
+            if (r->state != ST_DONE) {
+                keep.push_back(pr);
+                continue;
+            }
+            ela_result o{};
+            o.admitted = take(admitted_at, pr.second, -1.0);
+            o.finished = r->t_done - t0;
+            o.cycles = r->cycles;
+            o.status = r->status;
+            o.res0 = r->res0;
+            o.res = r->res;
+            if (double* u = take(uout, pr.second, (double*)nullptr)) ela_rve_download(r, 0, u);
+            live_bytes -= r->bytes;
+            ela_rve_destroy(r);
+            ++done;
+            sink(pr.second, o);
+        }
+        live.swap(keep);
+        sm.t_retire += now_s() - tr0;
+        return ELA_OK;
+    }
+    // Members still in flight are dropped (reported by the caller as not served).
+    void drop() {
+        for (auto& pr : live) ela_rve_destroy(pr.first);
+        live.clear();
+        B->active.clear();
+        B->queued.clear();
+        B->dirty = true;
+    }
+};
+
+}  // namespace
+
+// The online server: a host thread runs ServeCore over the requests submitted so far.
+struct ela_server {
+    ela_batch* B = nullptr;
+    std::unique_ptr<ServeCore> core;
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv;
+    bool stopping = false, drain = true;
+    std::deque<OwnedRequest> inbox;                  // submitted, not yet taken by the loop
+    std::deque<OwnedRequest> held;                   // taken: their arrays stay alive until done
+    std::unordered_map<long long, ela_result> results;
+    long long next_ticket = 1;
+    int fatal = ELA_OK;
+    std::string fatal_msg;
+};
+
+namespace {
+
+void server_loop(ela_server* sv) {
+    if (hipSetDevice(sv->B->ctx->device) != hipSuccess) {
+        std::lock_guard<std::mutex> g(sv->mu);
+        sv->fatal = ELA_ERR_HIP;
+        sv->fatal_msg = "hipSetDevice failed in the serving thread";
+        return;
+    }
+    ServeCore& core = *sv->core;
+    auto sink = [sv](long long ticket, const ela_result& r) {
+        std::lock_guard<std::mutex> g(sv->mu);
+        sv->results[ticket] = r;
+        for (auto it = sv->held.begin(); it != sv->held.end(); ++it)
+            if (it->ticket == ticket) {
+                sv->held.erase(it);
+                break;
+            }
+    };
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> g(sv->mu);
+            if (core.waiting.empty() && !core.busy() && sv->inbox.empty() && !sv->stopping)
+                sv->cv.wait(g, [sv] { return !sv->inbox.empty() || sv->stopping; });
+            while (!sv->inbox.empty()) {
+                sv->held.push_back(std::move(sv->inbox.front()));
+                sv->inbox.pop_front();
+                OwnedRequest& o = sv->held.back();
+                o.q.seeds = o.seeds.data();
+                o.q.R = o.R.data();
+                core.waiting.push_back({o.q, o.ticket});
+            }
+            if (sv->stopping && (!sv->drain || (core.waiting.empty() && !core.busy()))) break;
+        }
+        int rc = core.admit(sink);
+        if (!rc && core.busy()) rc = core.step_retire(sink);
+        if (rc) {
+            std::lock_guard<std::mutex> g(sv->mu);
+            sv->fatal = rc;
+            sv->fatal_msg = ela_last_error();
+            break;
+        }
+    }
+    // requests still waiting or in flight are not served
+    std::vector<long long> left;
+    for (auto& w : core.waiting) left.push_back(w.second);
+    for (auto& pr : core.live) left.push_back(pr.second);
+    core.drop();
+    std::lock_guard<std::mutex> g(sv->mu);
+    for (long long t : left) {
+        ela_result o{};
+        o.status = -1;
+        o.admitted = o.finished = -1.0;
+        sv->results[t] = o;
+    }
+    for (auto& o : sv->inbox) {
+        ela_result r{};
+        r.status = -1;
+        r.admitted = r.finished = -1.0;
+        sv->results[o.ticket] = r;
+    }
+    sv->inbox.clear();
+    sv->held.clear();
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -1899,107 +2162,98 @@ int ela_serve(ela_batch* B, int nreq, const ela_request* req, double tol_rel, do
         return fail(ELA_ERR_BADARG, "bad arguments");
     for (int i = 1; i < nreq; ++i)
         if (req[i].arrival < req[i - 1].arrival) return fail(ELA_ERR_BADARG, "requests must be sorted by arrival");
-    if (!B->active.empty() || !B->queued.empty()) return fail(ELA_ERR_STATE, "batch is busy");
+    if (!B->active.empty() || !B->queued.empty() || B->server) return fail(ELA_ERR_STATE, "batch is busy");
     for (int i = 0; i < nreq; ++i) {
         out[i] = ela_result{};
         out[i].status = -1;
         out[i].admitted = out[i].finished = -1.0;
     }
-    const double t0 = now_s();
-    std::deque<int> waiting;
-    std::vector<std::pair<ela_rve*, int>> live;
-    int next = 0, done = 0, rc = ELA_OK;
-    ela_serve_summary sm{};
-    size_t live_bytes = 0;
-    while (done < nreq) {
-        const double t = now_s() - t0;
+    ServeCore core(B, tol_rel, tol_abs, max_cycles, max_active);
+    int next = 0, rc = ELA_OK;
+    auto sink = [&](long long i, const ela_result& r) { out[i] = r; };
+    while (core.done < nreq) {
+        const double t = now_s() - core.t0;
         if (max_seconds > 0 && t > max_seconds) break;
-        while (next < nreq && req[next].arrival <= t) waiting.push_back(next++);
-        // admission
-        const double ta0 = now_s();
-        std::vector<ela_rve*> group;
-        std::vector<int> gidx;
-        while (!waiting.empty() && (int)live.size() + (int)group.size() < max_active) {
-            const int i = waiting.front();
-            const ela_request& q = req[i];
-            ela_rve* r = nullptr;
-            const int crc = ela_rve_create(B, q.n, std::max(1, q.nseeds), q.crack ? 1 : 0, &r);
-            if (crc == ELA_ERR_OOM && (!live.empty() || !group.empty())) break;  // wait for memory
-            int mrc = crc;
-            if (!mrc) mrc = ela_rve_set_voronoi_cubic(r, q.nseeds, q.seeds, q.q, q.R, q.C11, q.C12, q.C44);
-            if (!mrc && q.crack)
-                mrc = ela_rve_set_crack(r, q.crack_p, q.crack_nrm, q.crack_tv, q.crack_width, q.crack_g[0],
-                                        q.crack_g[1], q.crack_g[2]);
-            if (!mrc) mrc = ela_rve_set_rhs_affine(r, q.G);
-            waiting.pop_front();
-            if (mrc) {  // a request that cannot be served (bad size, or larger than the whole budget)
-                if (r) ela_rve_destroy(r);
-                out[i].status = mrc;
-                out[i].admitted = out[i].finished = now_s() - t0;
-                ++done;
-                continue;
-            }
-            group.push_back(r);
-            gidx.push_back(i);
-            live_bytes += r->bytes;
+        while (next < nreq && req[next].arrival <= t) {
+            core.waiting.push_back({req[next], next});
+            ++next;
         }
-        sm.t_admit += now_s() - ta0;
-        if (!group.empty()) {
-            const double tp0 = now_s();
-            if ((rc = prepare(B, (int)group.size(), group.data()))) break;
-            sm.t_prepare += now_s() - tp0;
-            ++sm.groups;
-            sm.admitted += (int)group.size();
-            const double ta = now_s() - t0;
-            for (size_t m = 0; m < group.size(); ++m) {
-                if ((rc = ela_batch_submit(B, group[m], tol_rel, tol_abs, max_cycles))) break;
-                out[gidx[m]].admitted = ta;
-                live.push_back({group[m], gidx[m]});
-            }
-            if (rc) break;
-        }
-        sm.peak_active = std::max(sm.peak_active, (int)live.size());
-        sm.peak_bytes = std::max(sm.peak_bytes, live_bytes);
-        if (!B->active.empty() || !B->queued.empty()) {
-            const double ts0 = now_s();
-            if ((rc = step(B))) break;
-            sm.t_step += now_s() - ts0;
-            ++sm.steps;
-            const double tr0 = now_s();
-            std::vector<std::pair<ela_rve*, int>> keep;
-            for (auto& pr : live) {
-                ela_rve* r = pr.first;
-                if (r->state != ST_DONE) {
-                    keep.push_back(pr);
-                    continue;
-                }
-                ela_result& o = out[pr.second];
-                o.finished = r->t_done - t0;
-                o.cycles = r->cycles;
-                o.status = r->status;
-                o.res0 = r->res0;
-                o.res = r->res;
-                if (req[pr.second].u_out) ela_rve_download(r, 0, req[pr.second].u_out);
-                live_bytes -= r->bytes;
-                ela_rve_destroy(r);
-                ++done;
-            }
-            live.swap(keep);
-            sm.t_retire += now_s() - tr0;
-        } else if (waiting.empty() && next < nreq) {
+        if ((rc = core.admit(sink))) break;
+        if (core.busy()) {
+            if ((rc = core.step_retire(sink))) break;
+        } else if (core.waiting.empty() && next < nreq) {
             const double ti0 = now_s();
-            const double wait = req[next].arrival - (now_s() - t0);
+            const double wait = req[next].arrival - (now_s() - core.t0);
             if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.01)));
-            sm.t_idle += now_s() - ti0;
+            core.sm.t_idle += now_s() - ti0;
         }
     }
-    if (summary) *summary = sm;
-    // leave nothing behind: requests still in flight at the time limit are dropped
-    for (auto& pr : live) ela_rve_destroy(pr.first);
-    B->active.clear();
-    B->queued.clear();
-    B->dirty = true;
+    if (summary) *summary = core.sm;
+    core.drop();  // leave nothing behind: requests still in flight at the time limit are dropped
     return rc;
+}
+
+// ------------------------------------------------------------------ online serving
+int ela_server_start(ela_batch* B, double tol_rel, double tol_abs, int max_cycles, int max_active, ela_server** out) {
+    if (!B || !out || max_cycles < 1 || max_active < 1) return fail(ELA_ERR_BADARG, "bad arguments");
+    *out = nullptr;
+    if (!B->active.empty() || !B->queued.empty() || B->server) return fail(ELA_ERR_STATE, "batch is busy");
+    ela_server* sv = new ela_server();
+    sv->B = B;
+    sv->core.reset(new ServeCore(B, tol_rel, tol_abs, max_cycles, max_active));
+    B->server = sv;
+    sv->th = std::thread([sv] { server_loop(sv); });
+    *out = sv;
+    return ELA_OK;
+}
+
+int ela_server_submit(ela_server* sv, const ela_request* req, int64_t* ticket) {
+    if (!sv || !req || !ticket) return fail(ELA_ERR_BADARG, "null argument");
+    if (req->nseeds < 1 || !req->seeds || !req->R) return fail(ELA_ERR_BADARG, "a request needs seeds and rotations");
+    std::lock_guard<std::mutex> g(sv->mu);
+    if (sv->stopping) return fail(ELA_ERR_STATE, "server is stopping");
+    OwnedRequest o;
+    o.q = *req;
+    o.seeds.assign(req->seeds, req->seeds + 3 * (size_t)req->nseeds);
+    o.R.assign(req->R, req->R + 9 * (size_t)req->nseeds);
+    o.ticket = sv->next_ticket++;
+    o.t_submit = now_s() - sv->core->t0;
+    *ticket = o.ticket;
+    sv->inbox.push_back(std::move(o));
+    sv->cv.notify_all();
+    return ELA_OK;
+}
+
+int ela_server_poll(ela_server* sv, int64_t ticket, int* done, ela_result* out) {
+    if (!sv || !done) return fail(ELA_ERR_BADARG, "null argument");
+    std::lock_guard<std::mutex> g(sv->mu);
+    *done = 0;
+    auto it = sv->results.find(ticket);
+    if (it == sv->results.end()) {
+        if (ticket < 1 || ticket >= sv->next_ticket) return fail(ELA_ERR_BADARG, "unknown ticket");
+        if (sv->fatal) return fail(sv->fatal, "the serving loop stopped: " + sv->fatal_msg);
+        return ELA_OK;
+    }
+    *done = 1;
+    if (out) *out = it->second;
+    sv->results.erase(it);
+    return ELA_OK;
+}
+
+int ela_server_stop(ela_server* sv, int drain) {
+    if (!sv) return ELA_OK;
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        sv->stopping = true;
+        sv->drain = drain != 0;
+        sv->cv.notify_all();
+    }
+    if (sv->th.joinable()) sv->th.join();
+    const int rc = sv->fatal;
+    const std::string msg = sv->fatal_msg;
+    sv->B->server = nullptr;
+    delete sv;
+    return rc ? fail(rc, "the serving loop stopped: " + msg) : ELA_OK;
 }
 
 }  // extern "C"

@@ -586,6 +586,7 @@ typedef struct {
     double crack_g[3];
     double G[9];
     double* u_out;
+    int max_cycles; /* > 0: this request's cycle limit; 0: the loop's */
 } ela_request;
 
 typedef struct {
@@ -614,10 +615,33 @@ typedef struct {
  * requests sorted by arrival are admitted first-come first-served while the
  * context's HBM budget and max_active allow; each step is one F-cycle over
  * every member in flight; members leave as they converge and free their
- * memory.  Stops when all requests are done or after max_seconds (> 0).
- * summary (may be NULL) receives the time breakdown. */
+ * memory.  A request that fails (see ela_server_start) gets its own status
+ * while the others continue.  Stops when all requests are done or after
+ * max_seconds (> 0).  summary (may be NULL) receives the time breakdown. */
 int ela_serve(ela_batch* b, int nreq, const ela_request* req, double tol_rel, double tol_abs, int max_cycles,
               int max_active, double max_seconds, ela_result* out, ela_serve_summary* summary);
+
+/* Online serving (no counterpart in the reference): a host thread runs the
+ * continuous-batching loop of ela_serve on batch b.  Requests submitted at any
+ * time are admitted first come, first served as the context's HBM budget and
+ * max_active allow, join the members in flight at the next cycle and leave as
+ * they finish.  A request whose member cannot be made (bad size, a model or
+ * set-up that is not finite, larger than the whole budget) or that fails in
+ * flight (ELA_ERR_NAN, ELA_ERR_NONCONVERGED on its cycle limit) gets its own
+ * status; the others are unaffected and keep the bits they have alone.
+ * submit copies the request and its seed and rotation arrays (u_out, if set,
+ * must stay valid until the request is done) and returns a ticket (> 0);
+ * poll sets done = 1 and the result once the ticket is finished (result times
+ * are seconds from ela_server_start; a result is reported once).  stop with
+ * drain != 0 finishes the requests submitted so far, otherwise the waiting and
+ * in-flight ones end unserved (status -1); it joins the thread and frees the
+ * server, returning the loop's fatal error if one stopped it.  One server per
+ * batch; the batch takes no other calls while it runs. */
+typedef struct ela_server ela_server;
+int ela_server_start(ela_batch* b, double tol_rel, double tol_abs, int max_cycles, int max_active, ela_server** out);
+int ela_server_submit(ela_server* s, const ela_request* req, int64_t* ticket);
+int ela_server_poll(ela_server* s, int64_t ticket, int* done, ela_result* out);
+int ela_server_stop(ela_server* s, int drain);
 
 /* Kernel options of a batch (set while no member is queued or active); see
  * ela_set_option. */

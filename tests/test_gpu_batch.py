@@ -305,3 +305,80 @@ def test_mixed_batch_smooths_traction_boundaries_like_fp64():
     (u64, i64), (umx, imx) = out[E.SOLVE_FP64], out[E.SOLVE_MIXED]
     assert imx <= i64 + 2, (imx, i64)
     assert np.abs(umx - u64).max() <= 1e-8 * np.abs(u64).max()
+
+
+def failing_mix():
+    """A small request stream with two failing requests: one with a non-finite stiffness (NaN C11) and
+    one with a cycle limit of 1."""
+    reqs = K.request_stream(rate=400.0, duration=0.05, seed=11, nmin=24, nmax=40)
+    assert len(reqs) >= 8
+    reqs[2] = dict(reqs[2], C11=float("nan"))
+    reqs[5] = dict(reqs[5], max_cycles=1)
+    return reqs
+
+
+def alone(b, q):
+    r = make(b, q)
+    b.prepare([r])
+    st = b.solve([r], tol_rel=1e-10, max_cycles=80)[0]
+    u = r.download(E.RVE.U)
+    r.close()
+    return st, u
+
+
+def test_serving_loop_isolates_failing_requests():
+    """ela_serve: the NaN request ends with ELA_ERR_NAN (its set-up fails alone: the admitted group is set
+    up member by member), the one with a cycle limit of 1 with ELA_ERR_NONCONVERGED; every other request
+    converges with the cycles and bits it has alone."""
+    reqs = failing_mix()
+    keep = tuple(i for i in range(len(reqs)) if i not in (2, 5))
+    b = E.Batch(E.SOLVE_MIXED)
+    res, sols = b.serve(reqs, tol_rel=1e-10, max_cycles=80, max_active=16, keep=keep)
+    assert res[2]["status"] == 5, res[2]
+    assert res[5]["status"] == 4 and res[5]["cycles"] == 1, res[5]
+    for i in keep:
+        assert res[i]["status"] == 0, (i, res[i])
+        st, u = alone(b, reqs[i])
+        assert st["iters"] == res[i]["cycles"]
+        assert np.array_equal(u, sols[i]), i
+    b.close()
+
+
+def test_online_server_admits_requests_while_others_run():
+    """ela_server_*: requests submitted in three waves while earlier ones are in flight are admitted into
+    the running loop; the failing ones get their own status and the others the bits they have alone."""
+    import time
+    reqs = failing_mix()
+    b = E.Batch(E.SOLVE_MIXED)
+    sv = E.Server(b, tol_rel=1e-10, max_cycles=80, max_active=16)
+    sols, tickets = {}, {}
+    waves = [range(0, 3), range(3, 6), range(6, len(reqs))]
+    for w in waves:
+        for i in w:
+            sols[i] = np.zeros((3,) + (reqs[i]["n"] + 1,) * 3)
+            tickets[i] = sv.submit(reqs[i], u_out=sols[i])
+        time.sleep(0.02)
+    res, t0 = {}, time.time()
+    while len(res) < len(reqs) and time.time() - t0 < 60:
+        for i, t in tickets.items():
+            if i not in res:
+                r = sv.poll(t)
+                if r is not None:
+                    res[i] = r
+        time.sleep(0.005)
+    sv.stop()
+    assert len(res) == len(reqs)
+    assert res[2]["status"] == 5 and res[5]["status"] == 4
+    overlap = False
+    for i in range(len(reqs)):
+        if i in (2, 5):
+            continue
+        assert res[i]["status"] == 0, (i, res[i])
+        st, u = alone(b, reqs[i])
+        assert st["iters"] == res[i]["cycles"]
+        assert np.array_equal(u, sols[i]), i
+    # some request was admitted while another was still in flight (continuous batching)
+    fin = [(res[i]["admitted"], res[i]["finished"]) for i in res if res[i]["status"] in (0, 4)]
+    overlap = any(a2 > a1 and a2 < f1 for a1, f1 in fin for a2, _ in fin)
+    assert overlap
+    b.close()
