@@ -1414,6 +1414,120 @@ void launch_voronoi_cubic(const Lay& L, int nseeds, const long long* seeds, int 
     hipLaunchKernelGGL(k_voronoi_cubic, node_grid(L), dim3(BX, BY), 0, s, L, nseeds, seeds, q, table, C, gid);
 }
 
+// ---------------------------------------------------------------- BiCGStab bottom solve
+// Flat reductions over n elements of one precision: out[0] = (a, b), out[1] =
+// (c, d), out[2] = max |e| (each term skipped when its inputs are null), in FP64,
+// deterministic (block partials, then one block folds them in block order).
+template <typename T>
+__global__ __launch_bounds__(256) void k_bicg_reduce(long long n, const T* __restrict__ a, const T* __restrict__ b,
+                                                     const T* __restrict__ c, const T* __restrict__ d,
+                                                     const T* __restrict__ e, double* __restrict__ partials) {
+    __shared__ double s0[256], s1[256], s2[256];
+    double x0 = 0, x1 = 0, x2 = 0;
+    for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += (long long)gridDim.x * 256) {
+        if (a) x0 += (double)a[q] * (double)b[q];
+        if (c) x1 += (double)c[q] * (double)d[q];
+        if (e) {
+            const double v = fabs((double)e[q]);
+            x2 = (v > x2 || v != v) ? v : x2;
+        }
+    }
+    s0[threadIdx.x] = x0;
+    s1[threadIdx.x] = x1;
+    s2[threadIdx.x] = x2;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            s0[threadIdx.x] += s0[threadIdx.x + st];
+            s1[threadIdx.x] += s1[threadIdx.x + st];
+            const double o = s2[threadIdx.x + st];
+            if (o > s2[threadIdx.x] || o != o) s2[threadIdx.x] = o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[3 * blockIdx.x] = s0[0];
+        partials[3 * blockIdx.x + 1] = s1[0];
+        partials[3 * blockIdx.x + 2] = s2[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bicg_fold(const double* __restrict__ partials, int nb, double* __restrict__ out) {
+    __shared__ double s0[256], s1[256], s2[256];
+    double x0 = 0, x1 = 0, x2 = 0;
+    for (int q = threadIdx.x; q < nb; q += 256) {
+        x0 += partials[3 * q];
+        x1 += partials[3 * q + 1];
+        const double v = partials[3 * q + 2];
+        x2 = (v > x2 || v != v) ? v : x2;
+    }
+    s0[threadIdx.x] = x0;
+    s1[threadIdx.x] = x1;
+    s2[threadIdx.x] = x2;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            s0[threadIdx.x] += s0[threadIdx.x + st];
+            s1[threadIdx.x] += s1[threadIdx.x + st];
+            const double o = s2[threadIdx.x + st];
+            if (o > s2[threadIdx.x] || o != o) s2[threadIdx.x] = o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s0[0];
+        out[1] = s1[0];
+        out[2] = s2[0];
+    }
+}
+
+// The vector updates of one BiCGStab step (coefficients from the host):
+// op 0: p = r + beta (p - omega v);  op 1: s = r - alpha v;
+// op 2: x += alpha p + omega s, r = s - omega t;  op 3: x += alpha p.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bicg_update(long long n, int op, double alpha, double beta, double omega,
+                                                     T* __restrict__ x, T* __restrict__ r, T* __restrict__ p,
+                                                     const T* __restrict__ v, T* __restrict__ sv,
+                                                     const T* __restrict__ t) {
+    for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += (long long)gridDim.x * 256) {
+        if (op == 0) p[q] = (T)((double)r[q] + beta * ((double)p[q] - omega * (double)v[q]));
+        else if (op == 1) sv[q] = (T)((double)r[q] - alpha * (double)v[q]);
+        else if (op == 2) {
+            x[q] = (T)((double)x[q] + alpha * (double)p[q] + omega * (double)sv[q]);
+            r[q] = (T)((double)sv[q] - omega * (double)t[q]);
+        } else {
+            x[q] = (T)((double)x[q] + alpha * (double)p[q]);
+        }
+    }
+}
+
+constexpr int kBicgBlocks = 512;
+long long bicg_partials() { return 3LL * kBicgBlocks; }
+
+template <typename T>
+void launch_bicg_reduce(long long n, const T* a, const T* b, const T* c, const T* d, const T* e, double* partials,
+                        double* out, hipStream_t s) {
+    const int nb = (int)std::min<long long>(kBicgBlocks, (n + 255) / 256);
+    hipLaunchKernelGGL(k_bicg_reduce<T>, dim3(nb), dim3(256), 0, s, n, a, b, c, d, e, partials);
+    hipLaunchKernelGGL(k_bicg_fold, dim3(1), dim3(256), 0, s, (const double*)partials, nb, out);
+}
+
+template <typename T>
+void launch_bicg_update(long long n, int op, double alpha, double beta, double omega, T* x, T* r, T* p, const T* v,
+                        T* sv, const T* t, hipStream_t s) {
+    const int nb = (int)std::min<long long>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(k_bicg_update<T>, dim3(nb), dim3(256), 0, s, n, op, alpha, beta, omega, x, r, p, v, sv, t);
+}
+
+template void launch_bicg_reduce<double>(long long, const double*, const double*, const double*, const double*,
+                                         const double*, double*, double*, hipStream_t);
+template void launch_bicg_reduce<float>(long long, const float*, const float*, const float*, const float*,
+                                        const float*, double*, double*, hipStream_t);
+template void launch_bicg_update<double>(long long, int, double, double, double, double*, double*, double*,
+                                         const double*, double*, const double*, hipStream_t);
+template void launch_bicg_update<float>(long long, int, double, double, double, float*, float*, float*,
+                                        const float*, float*, const float*, hipStream_t);
+
 void launch_eta_check(const Lay& L, const EtaSrc& e, int c0, int c1, int* flags, hipStream_t s) {
     if (L.dim == 3) hipLaunchKernelGGL(k_eta_check<3>, node_grid(L), dim3(BX, BY), 0, s, L, e, c0, c1, flags);
     else hipLaunchKernelGGL(k_eta_check<2>, node_grid(L), dim3(BX, BY), 0, s, L, e, c0, c1, flags);

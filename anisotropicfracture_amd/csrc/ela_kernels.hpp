@@ -157,6 +157,19 @@ void launch_combine(const Lay& L, const EtaSrc& e, int nph, const double* ddw, c
 void launch_newton_w_dw(const Lay& L, int kind, const double* C, const double* F0, const double* u, double wfac,
                         double* w, double* dw, const double h[3], hipStream_t s);
 
+// BiCGStab bottom solve (flat over whole fields, zero padding included):
+// out[0] = (a, b), out[1] = (c, d), out[2] = max |e| in FP64 (null inputs skip a
+// term; partials: bicg_partials() doubles), and the step updates
+// (op 0: p = r + beta (p - omega v); 1: s = r - alpha v;
+//  2: x += alpha p + omega s, r = s - omega t; 3: x += alpha p).
+template <typename T>
+void launch_bicg_reduce(long long n, const T* a, const T* b, const T* c, const T* d, const T* e, double* partials,
+                        double* out, hipStream_t s);
+template <typename T>
+void launch_bicg_update(long long n, int op, double alpha, double beta, double omega, T* x, T* r, T* p, const T* v,
+                        T* sv, const T* t, hipStream_t s);
+long long bicg_partials();
+
 // Flat FP64 vector kernels of the Krylov accelerator (whole fields incl. zero padding).
 constexpr int kMaxKrylov = 33;
 // FGMRES re-orthogonalizes w when |w'|^2 <= kReorth2 |w|^2 after a classical

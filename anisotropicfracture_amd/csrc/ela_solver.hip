@@ -174,6 +174,12 @@ struct ela_hier {
     std::vector<CycleGraph> graphs;
     long long graph_clock = 0;
     unsigned long long* hnorm = nullptr;  // pinned
+    // BiCGStab bottom solve of a coarsest level too large for the dense inverse:
+    // r, r-hat, p, v, s, t on the coarsest layout per precision, reduction scratch
+    double* bv64[6] = {};
+    float* bv32[6] = {};
+    double *bpart = nullptr, *bout = nullptr, *bhost = nullptr;  // bhost: pinned
+    int bottom_iters = 0;                                           // BiCGStab steps of the last solve
     // grain-table stiffness of level 0 (Voronoi models): per-node grain id, per-grain records
     unsigned short* gid = nullptr;
     double *tab64 = nullptr, *dgt64 = nullptr;
@@ -310,11 +316,90 @@ void res_of_cor(ela_hier* h, int l) {
     A_residual<T, T>(h, v, b.C, *b.cor, *b.res, *b.rc, (T*)nullptr, nullptr, nullptr, h->stream);
 }
 
+// BiCGStab of a coarsest level without a dense inverse, from a zero correction
+// to |r|inf <= kBottomTol |res|inf or kBottomMaxIter steps: AMReX's bottom
+// solver (MLCGSolver's BiCGStab, the bottom Linear selects, Linear.H:23-29,
+// with MLMG's bottom tolerance 1e-4).  Vectors in the cycle precision, dot
+// products and norms in FP64; the coefficients go through the host.
+constexpr double kBottomTol = 1e-4;
+constexpr int kBottomMaxIter = 200;
+template <typename T>
+T* const* bicg_vecs(ela_hier* h);
+template <>
+double* const* bicg_vecs<double>(ela_hier* h) { return h->bv64; }
+template <>
+float* const* bicg_vecs<float>(ela_hier* h) { return h->bv32; }
+
+template <typename T>
+void bicg_bottom(ela_hier* h) {
+    Level& v = h->lev.back();
+    LB<T> lbv = lb<T>(v);
+    T* x = *lbv.cor;
+    const T* b = *lbv.res;
+    T* const* W = bicg_vecs<T>(h);
+    T *r = W[0], *rh = W[1], *p = W[2], *vv = W[3], *sv = W[4], *t = W[5];
+    const long long n = (long long)field_elems(v, h->dim);
+    hipStream_t st = h->stream;
+    auto reduce = [&](const T* a, const T* bb, const T* c, const T* d, const T* e, double q[3]) {
+        launch_bicg_reduce<T>(n, a, bb, c, d, e, h->bpart, h->bout, st);
+        (void)hipMemcpyAsync(h->bhost, h->bout, 3 * sizeof(double), hipMemcpyDeviceToHost, st);
+        (void)hipStreamSynchronize(st);
+        for (int k = 0; k < 3; ++k) q[k] = h->bhost[k];
+    };
+    (void)hipMemsetAsync(x, 0, n * sizeof(T), st);
+    (void)hipMemsetAsync(p, 0, n * sizeof(T), st);
+    (void)hipMemsetAsync(vv, 0, n * sizeof(T), st);
+    (void)hipMemcpyAsync(r, b, n * sizeof(T), hipMemcpyDeviceToDevice, st);
+    (void)hipMemcpyAsync(rh, b, n * sizeof(T), hipMemcpyDeviceToDevice, st);
+    double q[3];
+    reduce(rh, r, nullptr, nullptr, r, q);
+    double rho = q[0];
+    const double rnorm0 = q[2];
+    if (!(rnorm0 > 0)) return;  // zero right-hand side: zero correction
+    const double tol = kBottomTol * rnorm0;
+    double rho1 = 1.0, alpha = 1.0, omega = 1.0;
+    int it = 0;
+    for (; it < kBottomMaxIter; ++it) {
+        if (!(rho != 0.0) || !std::isfinite(rho)) break;
+        const double beta = it == 0 ? 0.0 : (rho / rho1) * (alpha / omega);
+        launch_bicg_update<T>(n, 0, 0.0, beta, omega, x, r, p, vv, sv, t, st);
+        A_apply<T>(h, v, lbv.C, p, vv, st);
+        reduce(rh, vv, nullptr, nullptr, nullptr, q);
+        if (!(q[0] != 0.0) || !std::isfinite(q[0])) break;
+        alpha = rho / q[0];
+        launch_bicg_update<T>(n, 1, alpha, 0.0, 0.0, x, r, p, vv, sv, t, st);
+        reduce(nullptr, nullptr, nullptr, nullptr, sv, q);
+        if (q[2] <= tol) {
+            launch_bicg_update<T>(n, 3, alpha, 0.0, 0.0, x, r, p, vv, sv, t, st);
+            ++it;
+            break;
+        }
+        A_apply<T>(h, v, lbv.C, sv, t, st);
+        reduce(t, t, t, sv, nullptr, q);
+        omega = q[0] > 0.0 ? q[1] / q[0] : 0.0;
+        if (!(omega != 0.0) || !std::isfinite(omega)) {
+            launch_bicg_update<T>(n, 3, alpha, 0.0, 0.0, x, r, p, vv, sv, t, st);
+            ++it;
+            break;
+        }
+        launch_bicg_update<T>(n, 2, alpha, 0.0, omega, x, r, p, vv, sv, t, st);
+        reduce(rh, r, nullptr, nullptr, r, q);
+        rho1 = rho;
+        rho = q[0];
+        if (q[2] <= tol || !std::isfinite(q[2])) {
+            ++it;
+            break;
+        }
+    }
+    h->bottom_iters += it;
+}
+
 template <typename T>
 void bottom(ela_hier* h) {
     Level& v = h->lev.back();
     LB<T> b = lb<T>(v);
-    launch_bottom<T>(v.L, v.nodes, h->dim, h->M, *b.res, *b.cor, h->stream);
+    if (h->M) launch_bottom<T>(v.L, v.nodes, h->dim, h->M, *b.res, *b.cor, h->stream);
+    else bicg_bottom<T>(h);
 }
 
 // The V-cycle's way down on level l (pre-smoothing, correction residual,
@@ -433,7 +518,8 @@ std::vector<unsigned long long> cycle_key(ela_hier* h, int kind) {
 constexpr int kMaxCycleGraphs = 8;
 template <typename T>
 bool graph_run(ela_hier* h, int kind) {
-    if (!h->graph_on || h->nu1 == 0 || h->lev.size() < 2) return false;
+    // no capture around a BiCGStab bottom: its coefficients go through the host
+    if (!h->graph_on || h->nu1 == 0 || h->lev.size() < 2 || !h->M) return false;
     auto body = [&] {
         if (kind == 0) fcycle_upper<T>(h);
         else vcycle<T>(h, 1);
@@ -1046,7 +1132,6 @@ void* hier_cor0(ela_hier* h, int prec) {
 int hier_cycle(ela_hier* h, int prec) {
     if (!h) return fail(ELA_ERR_BADARG, "null hierarchy");
     if (!h->prepared) return fail(ELA_ERR_STATE, "call ela_prepare first");
-    if (!h->M) return fail(ELA_ERR_STATE, "the box does not coarsen to a dense-solvable level");
     if (prec == ELA_F32) {
         if (!h->has32) return fail(ELA_ERR_STATE, "no FP32 levels");
         one_cycle<float>(h);
@@ -1231,6 +1316,16 @@ int ela_hier_create_ex(ela_ctx* ctx, const ela_domain* dom, int model_kind, int 
     if (!rc) rc = halloc_t(h, field_elems(h->lev[0], h->dim), &h->u64);
     if (!rc) rc = halloc_t(h, field_elems(h->lev[0], h->dim), &h->b64);
     if (!rc && h->nunk <= kMaxDenseUnknowns) rc = halloc_t(h, (size_t)h->nunk * 2 * h->nunk, &h->M);
+    if (!rc && !h->M) {  // the coarsest level is solved by BiCGStab
+        const size_t fe = field_elems(h->lev.back(), h->dim);
+        for (int k = 0; k < 6 && !rc; ++k) {
+            if (h->has64) rc = halloc_t(h, fe, &h->bv64[k]);
+            if (!rc && h->has32) rc = halloc_t(h, fe, &h->bv32[k]);
+        }
+        if (!rc) rc = halloc_t(h, (size_t)bicg_partials(), &h->bpart);
+        if (!rc) rc = halloc_t(h, 4, &h->bout);
+        if (!rc && hipHostMalloc((void**)&h->bhost, 64) != hipSuccess) rc = fail(ELA_ERR_OOM, "hipHostMalloc failed");
+    }
     if (!rc) rc = halloc_t(h, 4, &h->dnorm);
     long long np = 0;
     for (auto& v : h->lev) np = std::max(np, residual_partials(v.L));
@@ -1264,6 +1359,7 @@ int ela_hier_destroy(ela_hier* h) {
     for (auto& e : h->ev) (void)hipEventDestroy(e);
     while (!h->allocs.empty()) hfree(h, h->allocs.back().first);
     if (h->hnorm) (void)hipHostFree(h->hnorm);
+    if (h->bhost) (void)hipHostFree(h->bhost);
     if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return ELA_OK;
@@ -1568,7 +1664,7 @@ int ela_prepare(ela_hier* h) {
     }
     if (flag) return fail(ELA_ERR_NAN, "diagonal is nan or zero (Elastic.cpp:347 aborts here)");
     // dense coarsest operator: columns A e_q, then Gauss-Jordan [A | I] -> [I | A^-1]
-    if (!h->M) {  // operator-only use (Fapply, Fsmooth, ...) of a box that does not coarsen far enough
+    if (!h->M) {  // the coarsest level is solved by BiCGStab: nothing to factor
         h->prepared = true;
         return ELA_OK;
     }
@@ -1883,9 +1979,6 @@ int ela_bc_init(ela_hier* h, double* rhs, int homogeneous) {
 }
 
 static int solve_common(ela_hier* h, double tol_rel, double tol_abs, int mode, int max_iter, ela_stats* st) {
-    if (!h->M)
-        return fail(ELA_ERR_BADARG, "coarsest MG level has " + std::to_string(h->nunk) + " unknowns (max " +
-                                        std::to_string(kMaxDenseUnknowns) + "); use n = m * 2^k with m <= 8");
     if (mode == ELA_SOLVE_FP64 && !h->has64) return fail(ELA_ERR_STATE, "hierarchy has no FP64 levels");
     if (mode == ELA_SOLVE_MIXED && !h->has32) return fail(ELA_ERR_STATE, "hierarchy has no FP32 levels");
     if (mode != ELA_SOLVE_FP64 && mode != ELA_SOLVE_MIXED) return fail(ELA_ERR_BADARG, "unknown solve mode");

@@ -94,7 +94,10 @@ size_t ela_ctx_bytes_in_use(const ela_ctx* ctx);
  * (src/Operator/Elastic.cpp:30-55) and Operator<Node>::define
  * (src/Operator/Operator.cpp:215-258): the MG levels are built here.  The
  * domain is coarsened by 2 while every n is even and >= 8 (max_mg_levels
- * <= 0: no further limit).  mode selects which MG precisions are allocated:
+ * <= 0: no further limit).  The coarsest level is solved exactly by a dense
+ * inverse when it has at most 3 * 9^3 unknowns, otherwise by BiCGStab to a
+ * relative 1e-4 (AMReX's bottom solver and MLMG's bottom tolerance; at most
+ * 200 steps), so any n is accepted.  mode selects which MG precisions are allocated:
  * ELA_SOLVE_FP64 (FP64 levels), ELA_SOLVE_MIXED (FP32 levels plus the FP64
  * level-0 residual) or ELA_ALLOC_BOTH. */
 int ela_hier_create(ela_ctx* ctx, const ela_domain* dom, int model_kind, int max_mg_levels, int mode,

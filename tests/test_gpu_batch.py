@@ -245,10 +245,35 @@ def test_serving_loop_results_match_alone_solves():
     b.close()
 
 
+def mixed_member_bcs():
+    """ZLO Displacement, ZHI Displacement in z and Traction in x, y; the x faces Traction and the y faces
+    Neumann in every component; edges and corners follow their z face, else Traction."""
+    t = np.zeros((26, 3), np.int32)
+    for f, name in enumerate(O_FACES3):
+        parts = name.split("_")
+        if "ZLO" in parts:
+            continue
+        if "ZHI" in parts:
+            t[f] = (1, 1, 0)
+        elif parts == ["YLO"] or parts == ["YHI"]:
+            t[f] = (3, 3, 3)
+        else:
+            t[f] = (1, 1, 1)
+    return t
+
+
+O_FACES3 = ["XLO", "YLO", "ZLO", "XHI", "YHI", "ZHI", "YLO_ZLO", "YLO_ZHI", "YHI_ZLO", "YHI_ZHI", "ZLO_XLO",
+            "ZLO_XHI", "ZHI_XLO", "ZHI_XHI", "XLO_YLO", "XLO_YHI", "XHI_YLO", "XHI_YHI", "XLO_YLO_ZLO",
+            "XLO_YLO_ZHI", "XLO_YHI_ZLO", "XLO_YHI_ZHI", "XHI_YLO_ZLO", "XHI_YLO_ZHI", "XHI_YHI_ZLO", "XHI_YHI_ZHI"]
+
+
 @pytest.mark.parametrize("mode", [E.SOLVE_MIXED, E.SOLVE_FP64])
-def test_marching_kernels_match_the_node_kernels(mode):
+@pytest.mark.parametrize("bcs", ["displacement", "mixed"])
+def test_marching_kernels_match_the_node_kernels(mode, bcs):
     """The LDS z-marching kernels of the finest levels and the one-thread-per-node kernels evaluate the
-    same expressions; solves agree to rounding (member sizes with partial tiles in x, y and z)."""
+    same expressions; solves agree to rounding (member sizes with partial tiles in x, y and z), also with
+    traction and Neumann faces, whose boundary rows divide by (FP64) or multiply by the inverse of (FP32)
+    a diagonal that is not 1."""
     probs = [problem(61, 40, 12), problem(62, 24, 9, crack=True), problem(63, 32, 20)]
     sols, cyc = {}, {}
     for march in (0, 1):
@@ -256,6 +281,10 @@ def test_marching_kernels_match_the_node_kernels(mode):
         b.set_option(E.Batch.MARCH, march)
         b.set_option(E.Batch.KZ, 16)
         rs = [make(b, p) for p in probs]
+        if bcs == "mixed":
+            for r, p in zip(rs, probs):
+                r.SetBC(mixed_member_bcs())
+                r.SetRhsAffine(p["G"])
         b.prepare(rs)
         sts = b.solve(rs, tol_rel=1e-10, max_cycles=80)
         sols[march] = [r.download(E.RVE.U) for r in rs]

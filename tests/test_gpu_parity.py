@@ -515,3 +515,28 @@ def test_folded_last_sweep_gives_the_separate_addition_bits(mode):
         op.close()
     assert out[0][1] == out[1][1]
     assert np.array_equal(out[0][0], out[1][0])
+
+
+@pytest.mark.parametrize("n,mode", [((36, 36, 28), ela.SOLVE_MIXED)])
+def test_sizes_without_a_dense_coarsest_level_solve_with_a_bicgstab_bottom(oracle, n, mode):
+    """A size whose coarsest level is too large for the dense inverse (36 x 36 x 28 -> 9 x 9 x 7 cells,
+    2400 unknowns) solves with the BiCGStab bottom (AMReX's MLCGSolver with MLMG's bottom tolerance
+    1e-4), plain MG and FGMRES, to the solution of the oracle's MG with its BiCGStab bottom, to 1e-8
+    relative (the oracle's solve takes about 20 s of CPU, so one case)."""
+    rng = np.random.default_rng(sum(n))
+    dim = len(n)
+    kind = MM if dim == 3 else ISO
+    C = H.random_stiffness(rng, n, kind, dim)
+    t, v = K.uniaxial_z_bcs(0.01) if dim == 3 else K.inclusion_2d(8)[2:]
+    op = make(n, kind, C, t, v)
+    g = oracle.Grid(n)
+    b = oracle.bc_init(g, t, v, g.zeros())
+    u_or, st_or, _ = oracle.mg_solve(g, kind, C, t, b, tol_rel=1e-11, max_iter=120, bottom="bicgstab")
+    assert st_or["res"] <= 5e-11 * st_or["res0"], st_or
+    for krylov in (0, 8):
+        op.SetKrylov(krylov)
+        u, st = ela.Linear(op).solve(np.zeros_like(b), b, tol_rel=1e-10, tol_abs=0.0, mode=mode, max_iter=400)
+        print(n, "krylov", krylov, "iters", st["iters"], "levels", op.levels())
+        assert st["converged"]
+        assert np.abs(u - u_or).max() <= 1e-8 * np.abs(u_or).max()
+    op.close()
