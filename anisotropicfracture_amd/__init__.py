@@ -82,6 +82,11 @@ class ServeSummary(ctypes.Structure):
                 ("t_idle", ctypes.c_double)]
 
 
+class PlotLevel(ctypes.Structure):
+    _fields_ = [("domain", ctypes.c_int * 3), ("origin", ctypes.c_int * 3), ("nodes", ctypes.c_int * 3),
+                ("data", ctypes.c_void_p), ("nboxes", ctypes.c_int), ("boxes", ctypes.c_void_p)]
+
+
 class Result(ctypes.Structure):
     _fields_ = [("admitted", ctypes.c_double), ("finished", ctypes.c_double), ("cycles", ctypes.c_int),
                 ("status", ctypes.c_int), ("res0", ctypes.c_double), ("res", ctypes.c_double)]
@@ -174,6 +179,7 @@ SIGNATURES = {
                            _I, _P], _I),
     "ela_average_down_window": ([ctypes.POINTER(_Layout), ctypes.POINTER(_Layout), _P, _P, ctypes.POINTER(_Layout), _P,
                                  _P, _I, _I, _P], _I),
+    "ela_write_plotfile": ([ctypes.c_char_p, _I, _I, _P, _I, _P, _D, _I, _P, _P], _I),
     "ela_server_start": ([_P, _D, _D, _I, _I, ctypes.POINTER(_P)], _I),
     "ela_server_submit": ([_P, _P, ctypes.POINTER(ctypes.c_int64)], _I),
     "ela_server_poll": ([_P, ctypes.c_int64, ctypes.POINTER(_I), _P], _I),
@@ -723,12 +729,31 @@ class ElasticAMR:
             bx[6 * q:6 * q + self.dim] = lo
             bx[6 * q + 3:6 * q + 3 + self.dim] = hi
         self.boxes = lists
+        self.n0 = n0
         p = ctypes.c_void_p()
         _check(lib().ela_amr_create_boxes(self.ctx.ptr, ctypes.byref(d), self.nlev, _ptr(counts), _ptr(bx), kind,
                                           coarse_min, mode, ctypes.byref(p)))
         self.ptr = p
         self.kind, self.mode = kind, mode
         self.nc = ncomp(self.dim, kind)
+
+    def WritePlotFile(self, path, fields, time=0.0, step=0):
+        """The levels' node fields as a plot file (Integrator::WritePlotFile, Integrator.cpp:582-687):
+        fields maps a name to per-level arrays (ncomp, *shape(lev)); a field of several components is
+        written as name001, name002, ... as the reference names them."""
+        names, per = [], [[] for _ in range(self.nlev)]
+        for name, arrs in fields.items():
+            nc = arrs[0].shape[0]
+            names += [name] if nc == 1 else ["%s%03d" % (name, j + 1) for j in range(nc)]
+            for l in range(self.nlev):
+                per[l].append(np.asarray(arrs[l]).reshape((nc,) + tuple(arrs[l].shape[1:])))
+        levels = []
+        for l in range(self.nlev):
+            data = np.concatenate(per[l], axis=0)
+            dom = [n << l for n in self.n0]
+            boxes = [((0,) * self.dim, tuple(dom))] if l == 0 else self.boxes[l - 1]
+            levels.append({"domain": dom, "origin": self.origin(l), "data": data, "boxes": boxes})
+        write_plotfile(path, levels, names, time, step)
 
     def _info(self, lev):
         lay = _Layout()
@@ -913,6 +938,36 @@ class RVE:
             self.ptr = None
             if self in self.batch._members:
                 self.batch._members.remove(self)
+
+
+def write_plotfile(path, levels, names, time=0.0, step=0, prob_lo=None, prob_hi=None):
+    """ela_write_plotfile: a nodal plot file in AMReX's layout (Integrator::WritePlotFile,
+    Integrator.cpp:582-687).  levels: dicts with domain (cells per direction), origin (node index of the
+    array's first node), data (ncomp, [nz,] ny, nx) float64 and boxes [(lo, hi), ...] of nodes."""
+    dim = len(levels[0]["domain"])
+    arr = (PlotLevel * len(levels))()
+    hold = []
+    for l, lv in enumerate(levels):
+        data = np.ascontiguousarray(lv["data"], dtype=np.float64)
+        if data.shape[0] != len(names):
+            raise ValueError("level %d has %d components for %d names" % (l, data.shape[0], len(names)))
+        bx = np.zeros(6 * len(lv["boxes"]), dtype=np.int32)
+        for q, (lo, hi) in enumerate(lv["boxes"]):
+            bx[6 * q:6 * q + dim] = lo
+            bx[6 * q + 3:6 * q + 3 + dim] = hi
+        hold += [data, bx]
+        a = arr[l]
+        nodes = list(reversed(data.shape[1:])) + [1] * (3 - dim)
+        for d in range(3):
+            a.domain[d] = int(lv["domain"][d]) if d < dim else 0
+            a.origin[d] = int(lv.get("origin", (0,) * dim)[d]) if d < dim else 0
+            a.nodes[d] = int(nodes[d])
+        a.data, a.nboxes, a.boxes = data.ctypes.data, len(lv["boxes"]), bx.ctypes.data
+    cn = (ctypes.c_char_p * len(names))(*[n.encode() for n in names])
+    lo = np.ascontiguousarray(prob_lo if prob_lo is not None else np.zeros(dim), dtype=np.float64)
+    hi = np.ascontiguousarray(prob_hi if prob_hi is not None else np.ones(dim), dtype=np.float64)
+    _check(lib().ela_write_plotfile(str(path).encode(), dim, len(levels), arr, len(names), cn, float(time),
+                                    int(step), _ptr(lo), _ptr(hi)))
 
 
 def _fill_request(r, q):

@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libela.so")
-SOURCES = ["ela_kernels.hip", "ela_solver.hip", "ela_batch.hip", "ela_amr.hip", "ela_composite.hip"]
+SOURCES = ["ela_kernels.hip", "ela_solver.hip", "ela_batch.hip", "ela_amr.hip", "ela_composite.hip", "ela_plotfile.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
          # no packed FP32 instructions (v_pk_{add,mul,fma}_f32) anywhere in device code
@@ -23,7 +23,7 @@ def build(verbose=False):
     objs = []
     procs = []
     for src in SOURCES:
-        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
         objs.append(obj)

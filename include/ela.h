@@ -660,6 +660,25 @@ int ela_batch_autotune(ela_batch* b, int count, ela_rve* const* rves, int nrep, 
 int ela_batch_set_timing(ela_batch* b, int on);
 int ela_batch_smoother_times(ela_batch* b, double* ms, long long* nodes, int* launches);
 
+/* ------------------------------------------------------------------ plot files
+ *
+ * Nodal plot file in AMReX's plotfile layout (HyperCLaw-V1.1 Header,
+ * Level_<l>/Cell_H, Level_<l>/Cell_D_00000), the output of the reference's
+ * Integrator::WritePlotFile (src/Integrator/Integrator.cpp:582-687) for its
+ * node fields, so results can be compared with a reference run (AMReX's
+ * fcompare).  Host arrays; refinement ratio 2 between levels. */
+typedef struct {
+    int domain[3];      /* cells of the level's domain per direction */
+    int origin[3];      /* node index of the first node of the data array */
+    int nodes[3];       /* node counts of the data array (1 in the unused direction in 2D) */
+    const double* data; /* ncomp components of nodes[0]*nodes[1]*nodes[2] values, x fastest */
+    int nboxes;
+    const int* boxes;   /* nboxes records of node lo[3], hi[3], inside the data array */
+} ela_plot_level;
+int ela_write_plotfile(const char* dir, int dim, int nlev, const ela_plot_level* levels, int ncomp,
+                       const char* const* names, double time, int step, const double* prob_lo,
+                       const double* prob_hi);
+
 #ifdef __cplusplus
 }
 #endif

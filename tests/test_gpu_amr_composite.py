@@ -275,3 +275,23 @@ def test_chopped_levels_solve_bit_for_bit_like_one_box(oracle, amr):
     assert outs[0][1] == outs[1][1]
     for x, y in zip(outs[0][0], outs[1][0]):
         assert np.array_equal(x, y)
+
+
+def test_composite_solution_plot_file(oracle, amr, tmp_path):
+    """The composite solution written as a nodal plot file (Integrator::WritePlotFile) reads back with the
+    levels' boxes and the solution values of every box."""
+    from tests.test_plotfile import read_plotfile
+    nest = amr.Nest((12, 12, 12), L_SHAPE)
+    C, t, b = uniaxial_inputs(nest, oracle)
+    a = make(nest, ISO, C, t, E.SOLVE_MIXED)
+    u, st = a.solve([np.zeros_like(x) for x in b], b, tol_rel=1e-10, max_iter=200)
+    a.WritePlotFile(tmp_path / "plt00000node", {"disp": u}, time=0.0, step=0)
+    pf = read_plotfile(str(tmp_path / "plt00000node"))
+    assert pf["names"] == ["disp001", "disp002", "disp003"] and pf["finest"] == 1
+    for l in range(nest.nlev):
+        o = nest.glo[l]
+        for (lo, hi, _), fab in zip(pf["levels"][l]["boxes"], pf["levels"][l]["fabs"]):
+            want = u[l][:, lo[2] - o[2]:hi[2] - o[2] + 1, lo[1] - o[1]:hi[1] - o[1] + 1, lo[0] - o[0]:hi[0] - o[0] + 1]
+            assert np.array_equal(fab, want)
+    assert len(pf["levels"][1]["boxes"]) == 3
+    a.close()
