@@ -144,6 +144,7 @@ SIGNATURES = {
     "ela_kernel_times": ([_P, ctypes.POINTER(_D), ctypes.POINTER(_I), ctypes.POINTER(_D), ctypes.POINTER(_I)], _I),
     "ela_time_fapply": ([_P, _I, _I, ctypes.POINTER(_D)], _I),
     "ela_probe_copy": ([_P, _SZ, _I, ctypes.POINTER(_D)], _I),
+    "ela_probe_copy_ex": ([_P, _SZ, _I, _I, ctypes.POINTER(_D)], _I),
     "ela_hier_create_ex": ([_P, ctypes.POINTER(_Domain), _I, _I, _I, _I, ctypes.POINTER(_P)], _I),
     # batched independent solves
     "ela_batch_create": ([_P, _I, ctypes.POINTER(_P)], _I),
@@ -247,10 +248,11 @@ class Context:
         _check(lib().ela_ctx_create(device, hbm_budget_bytes, ctypes.byref(p)))
         self.ptr = p
 
-    def probe_copy(self, nbytes, nrep=10):
-        """Average ms of a plain device copy of nbytes (the HBM-counter calibration probe)."""
+    def probe_copy(self, nbytes, nrep=10, variant=0):
+        """Average ms of a device copy of nbytes (the HBM-counter calibration probe); variant 0-6
+        (ela_probe_copy_ex: more bytes in flight per lane, non-temporal stores, grid sizes)."""
         ms = ctypes.c_double()
-        _check(lib().ela_probe_copy(self.ptr, nbytes, nrep, ctypes.byref(ms)))
+        _check(lib().ela_probe_copy_ex(self.ptr, nbytes, nrep, int(variant), ctypes.byref(ms)))
         return ms.value
 
     def bytes_in_use(self):

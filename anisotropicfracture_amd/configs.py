@@ -282,20 +282,12 @@ def rve_batch(count=512, seed=SEED, sides=RVE_SIDES, grains=(8, 32)):
 
 
 # ------------------------------------------------------------------ serving stream
-# Member sides of the serving stream (n = c 2^k).  A coarsest level of c cells
-# costs a dense inverse of 3 (c+1)^3 unknowns at admission: for c = 7 that set-up
-# (~23 ms of elimination traffic per member) exceeds the whole solve of a 112^3
-# member, and for c = 5 it exceeds the solve of a 40^3 member, so those sizes
-# are served at the next side up: c in {2, 3} plus 80 and 160 (c = 5).
-SERVE_SIDES = (32, 48, 64, 80, 96, 128, 160, 192)
-
-
-def serve_side(n):
-    """Smallest member side >= n in SERVE_SIDES (requests are solved at that resolution)."""
-    for s in SERVE_SIDES:
-        if s >= n:
-            return s
-    return SERVE_SIDES[-1]
+def dense_coarsest_unknowns(n):
+    """Unknowns of the coarsest level of a member of side n (n halved while even and n / 2 >= 2):
+    3 (c+1)^3; above 1536 the batch solves that level by BiCGStab instead of a dense inverse."""
+    while n % 2 == 0 and n // 2 >= 2:
+        n //= 2
+    return 3 * (n + 1) ** 3
 
 
 def random_crack(rng, n):
@@ -309,20 +301,23 @@ def random_crack(rng, n):
     return (p, nrm, tv, max(0.01, 2.0 / n), (1.0, -2.0, 1.0 + 1e-4))
 
 
-def request_stream(rate=200.0, duration=60.0, seed=SEED, nmin=32, nmax=192, grains=(8, 64), p_crack=0.5, strain=0.01):
+def request_stream(rate=200.0, duration=60.0, seed=SEED, nmin=32, nmax=192, grains=(8, 64), p_crack=0.5, strain=0.01,
+                   blocking=4):
     """Serving stream: Poisson arrivals (rate per s) over `duration` s; side log-uniform in
-    [nmin, nmax] rounded to the nearest integer and then up to SERVE_SIDES (the requested side is
-    kept as n_requested); 8-64 Voronoi grains
-    (Haar orientations, cubic constants above); a crack with probability p_crack; load
-    u = strain (a.x) a on every face for a random unit vector a.  Sorted by arrival."""
+    [nmin, nmax] (kept as n_requested), solved at the nearest multiple of `blocking` cells -- AMReX
+    requires the domain to be a multiple of amr.blocking_factor, which the reference's elastic inputs
+    set to 4 (tests/UniaxialTension/input:13, tests/Eshelby/input:13); blocking=1 solves every request at
+    its integer side; 8-64 Voronoi grains (Haar orientations, cubic constants above); a crack with
+    probability p_crack; load u = strain (a.x) a on every face for a random unit vector a.  Sorted by
+    arrival."""
     rng = np.random.default_rng(seed)
     t, out = 0.0, []
     while True:
         t += rng.exponential(1.0 / rate)
         if t >= duration:
             break
-        n_req = int(np.rint(np.exp(rng.uniform(np.log(nmin), np.log(nmax)))))
-        n = serve_side(n_req)
+        n_req = float(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+        n = max(blocking, blocking * int(np.rint(n_req / blocking)))
         ng = int(rng.integers(grains[0], grains[1] + 1))
         seeds = voronoi_seeds(rng, ng, (n, n, n))
         R = haar_rotations(rng, ng)

@@ -18,10 +18,14 @@
 // Per member the arithmetic is that of the single-box solver (ela_solver.hip):
 // FP64 outer iteration r = b - A u, u += cycle(r) with the right-hand side
 // scaled by a power of two, F-cycles with nu1 = nu2 = 2 calls of 2 weighted
-// Jacobi sweeps (omega = 2/3), full weighting, trilinear prolongation and a
-// dense inverse on the coarsest level.  A member's results depend only on its
-// own data (no cross-member reductions), and it leaves the active set the
-// cycle it converges, so a solve gives the same bits alone and in any mix.
+// Jacobi sweeps (omega = 2/3), full weighting, trilinear prolongation and, on
+// the coarsest level, a dense inverse (up to kMaxBatchDense unknowns) or
+// BiCGStab to 1e-4 (larger coarsest levels: sides with a large odd factor).
+// A member's results depend only on its own data (no cross-member reductions;
+// a member's BiCGStab scalars come from its own blocks, and a member that
+// stops iterating stays frozen while others go on), and it leaves the active
+// set the cycle it converges, so a solve gives the same bits alone and in any
+// mix.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -46,7 +50,13 @@ namespace ela {
 namespace {
 
 constexpr int TB = 256;                // nodes per block
-constexpr int kMaxBatchDense = 1536;   // unknowns of a coarsest level (n_c <= 7)
+constexpr int kMaxBatchDense = 1536;   // unknowns of a dense coarsest level (n_c <= 7)
+constexpr double kBottomTol = 1e-4;    // BiCGStab coarsest solve: |r|inf <= 1e-4 |b|inf
+constexpr int kBottomMaxIter = 200;    //   or this many iterations
+constexpr int kBottomCheck = 4;        // iterations between the host's looks at the members' states
+enum { BV_R = 0, BV_RH, BV_P, BV_V, BV_S, BV_T, BV_N };
+// Per-member BiCGStab scalars (FP64, on the device).
+enum { BS_RHO = 0, BS_RHO1, BS_ALPHA, BS_OMEGA, BS_BETA, BS_TOL, BS_IT, BS_STATE, BS_N = 8 };
 constexpr int kNC = kGrainNC;          // cubic stiffness: major/minor symmetric storage
 enum { F_RES = 0, F_RC, F_COR, F_TMP, F_HOLD, F_N };
 
@@ -83,6 +93,8 @@ struct BoxDesc {
     double* scale;
     const double* E;  // set-up: identity of size nunk (columns = unit fields)
     double* W;        // set-up: [A | I] workspace, nunk x 2 nunk
+    double* bs;       // coarsest level solved by BiCGStab: its scalars (BS_*) and vectors (BV_*)
+    void* bv[BV_N];
 };
 
 template <typename T>
@@ -358,6 +370,260 @@ __global__ __launch_bounds__(TB) void kb_bottom(const BoxDesc* __restrict__ D, i
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
         if (lane == 0) x[r] = (T)acc;
     }
+}
+
+// ---------------------------------------------------------- BiCGStab coarsest solve
+// The coarsest levels beyond the dense inverse solve A x = b by BiCGStab (the
+// single-box solver's bottom, the reference's bottom solver: Linear.H:23-29
+// with MLMG's bottom tolerance 1e-4): vectors in the cycle precision, dot
+// products and norms in FP64.  Launches go over the list of such boxes; each
+// block writes three partials (two sums and a maximum), and one 64-lane block
+// per member folds its own blocks' partials in a fixed order and runs the
+// scalar steps on its lane 0.  A member's state (BS_STATE) is 1 while it
+// iterates, 2 when it ends after the x += alpha p update, 0 when done; the
+// vector kernels skip members whose state says so, so a member stops where
+// the single-box loop breaks and stays frozen while others go on.
+
+// Sums of a and b and maximum of c over the block into partials[3 blk + 0..2].
+__device__ __forceinline__ void block_sum2_max(double* partials, int blk, double a, double b, double c) {
+    __shared__ double wred[3][TB / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+        const double o = __shfl_xor(c, off, 64);
+        c = (o > c || o != o) ? o : c;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        wred[0][threadIdx.x >> 6] = a;
+        wred[1][threadIdx.x >> 6] = b;
+        wred[2][threadIdx.x >> 6] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = wred[0][0], sb = wred[1][0], m = wred[2][0];
+        for (int w = 1; w < TB / 64; ++w) {
+            sa += wred[0][w];
+            sb += wred[1][w];
+            m = (wred[2][w] > m || wred[2][w] != wred[2][w]) ? wred[2][w] : m;
+        }
+        partials[3 * blk] = sa;
+        partials[3 * blk + 1] = sb;
+        partials[3 * blk + 2] = m;
+    }
+}
+
+__device__ __forceinline__ double amax(double m, double v) {
+    const double a = fabs(v);
+    return (a > m || a != a) ? a : m;
+}
+
+// x = f[dst] = 0, p = v = 0, r = rh = f[src]; partials rh.r and |r|inf.
+template <typename T>
+__global__ __launch_bounds__(TB) void kbb_init(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, int src,
+                                               int dst, double* __restrict__ partials) {
+    const BoxDesc& d = D[bmap[blockIdx.x]];
+    const int q = ((int)blockIdx.x - d.tile0) * TB + (int)threadIdx.x;
+    double s0 = 0.0, m = 0.0;
+    if (q < d.nodes) {
+        const T* b = (const T*)d.f[src];
+        T* x = (T*)d.f[dst];
+        T *r = (T*)d.bv[BV_R], *rh = (T*)d.bv[BV_RH], *p = (T*)d.bv[BV_P], *v = (T*)d.bv[BV_V];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const long long e = c * d.L.cs + q;
+            const T bv = b[e];
+            x[e] = p[e] = v[e] = (T)0;
+            r[e] = rh[e] = bv;
+            s0 += (double)bv * (double)bv;
+            m = amax(m, (double)bv);
+        }
+    }
+    block_sum2_max(partials, blockIdx.x, s0, 0.0, m);
+}
+
+// p = r + beta (p - omega v)
+template <typename T>
+__global__ __launch_bounds__(TB) void kbb_p(const BoxDesc* __restrict__ D, const int* __restrict__ bmap) {
+    const BoxDesc& d = D[bmap[blockIdx.x]];
+    if (d.bs[BS_STATE] != 1.0) return;
+    const int q = ((int)blockIdx.x - d.tile0) * TB + (int)threadIdx.x;
+    if (q >= d.nodes) return;
+    const T beta = (T)d.bs[BS_BETA], omega = (T)d.bs[BS_OMEGA];
+    const T *r = (const T*)d.bv[BV_R], *v = (const T*)d.bv[BV_V];
+    T* p = (T*)d.bv[BV_P];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const long long e = c * d.L.cs + q;
+        p[e] = r[e] + beta * (p[e] - omega * v[e]);
+    }
+}
+
+// which 0: v = A p, partial rh.v; which 1: t = A s, partials t.t and t.s.
+template <typename T>
+__global__ __launch_bounds__(TB) void kbb_apply(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, int which,
+                                                double* __restrict__ partials) {
+    const BoxDesc& d = D[bmap[blockIdx.x]];
+    if (d.bs[BS_STATE] != 1.0) return;  // uniform over the block
+    const Lay L = d.L;
+    int i, j, k;
+    long long o;
+    double s0 = 0.0, s1 = 0.0;
+    if (box_node(d, L, blockIdx.x, i, j, k, o)) {
+        const T* in = (const T*)d.bv[which ? BV_S : BV_P];
+        T* out = (T*)d.bv[which ? BV_T : BV_V];
+        const T* w = (const T*)d.bv[which ? BV_T : BV_RH];
+        T f[3];
+        box_apply<T, -1>(d, L, in, i, j, k, f);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const long long e = c * L.cs + o;
+            out[e] = f[c];
+            if (which) {
+                s0 += (double)f[c] * (double)f[c];
+                s1 += (double)f[c] * (double)in[e];
+            } else {
+                s0 += (double)w[e] * (double)f[c];
+            }
+        }
+    }
+    block_sum2_max(partials, blockIdx.x, s0, s1, 0.0);
+}
+
+// s = r - alpha v; partial |s|inf.
+template <typename T>
+__global__ __launch_bounds__(TB) void kbb_s(const BoxDesc* __restrict__ D, const int* __restrict__ bmap,
+                                            double* __restrict__ partials) {
+    const BoxDesc& d = D[bmap[blockIdx.x]];
+    if (d.bs[BS_STATE] != 1.0) return;
+    const int q = ((int)blockIdx.x - d.tile0) * TB + (int)threadIdx.x;
+    double m = 0.0;
+    if (q < d.nodes) {
+        const T alpha = (T)d.bs[BS_ALPHA];
+        const T *r = (const T*)d.bv[BV_R], *v = (const T*)d.bv[BV_V];
+        T* sv = (T*)d.bv[BV_S];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const long long e = c * d.L.cs + q;
+            const T x = r[e] - alpha * v[e];
+            sv[e] = x;
+            m = amax(m, (double)x);
+        }
+    }
+    block_sum2_max(partials, blockIdx.x, 0.0, 0.0, m);
+}
+
+// State 1: x += alpha p + omega s, r = s - omega t, partials rh.r and |r|inf.
+// State 2: x += alpha p.
+template <typename T>
+__global__ __launch_bounds__(TB) void kbb_x(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, int dst,
+                                            double* __restrict__ partials) {
+    const BoxDesc& d = D[bmap[blockIdx.x]];
+    const double st = d.bs[BS_STATE];
+    if (st != 1.0 && st != 2.0) return;
+    const int q = ((int)blockIdx.x - d.tile0) * TB + (int)threadIdx.x;
+    const T alpha = (T)d.bs[BS_ALPHA], omega = (T)d.bs[BS_OMEGA];
+    T* x = (T*)d.f[dst];
+    const T* p = (const T*)d.bv[BV_P];
+    if (st == 2.0) {
+        if (q < d.nodes)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const long long e = c * d.L.cs + q;
+                x[e] += alpha * p[e];
+            }
+        return;
+    }
+    double s0 = 0.0, m = 0.0;
+    if (q < d.nodes) {
+        const T *sv = (const T*)d.bv[BV_S], *t = (const T*)d.bv[BV_T], *rh = (const T*)d.bv[BV_RH];
+        T* r = (T*)d.bv[BV_R];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const long long e = c * d.L.cs + q;
+            x[e] += alpha * p[e] + omega * sv[e];
+            const T rr = sv[e] - omega * t[e];
+            r[e] = rr;
+            s0 += (double)rh[e] * (double)rr;
+            m = amax(m, (double)rr);
+        }
+    }
+    block_sum2_max(partials, blockIdx.x, s0, 0.0, m);
+}
+
+enum { BP_INIT = 0, BP_V, BP_S, BP_T, BP_R };
+
+// One block of 64 lanes per member: fold its partials (fixed order), then the
+// scalar step of phase `phase` on lane 0; state[box] = the member's state.
+__global__ __launch_bounds__(64) void kbb_fold(const BoxDesc* __restrict__ D, const double* __restrict__ partials,
+                                               int phase, int* __restrict__ state) {
+    const BoxDesc& d = D[blockIdx.x];
+    double* S = d.bs;
+    const double st = S[BS_STATE];
+    const bool skip = phase != BP_INIT && st != 1.0 && !(phase == BP_R && st == 2.0);
+    double q0 = 0.0, q1 = 0.0, q2 = 0.0;
+    if (!skip) {
+        const int nt = (d.nodes + TB - 1) / TB;
+        for (int t = threadIdx.x; t < nt; t += 64) {
+            const double* pp = partials + 3LL * (d.tile0 + t);
+            q0 += pp[0];
+            q1 += pp[1];
+            q2 = (pp[2] > q2 || pp[2] != pp[2]) ? pp[2] : q2;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            q0 += __shfl_xor(q0, off, 64);
+            q1 += __shfl_xor(q1, off, 64);
+            const double o = __shfl_xor(q2, off, 64);
+            q2 = (o > q2 || o != o) ? o : q2;
+        }
+    }
+    if (threadIdx.x != 0) return;
+    double ns = skip ? st : 1.0;
+    if (!skip) switch (phase) {
+            case BP_INIT:  // rho = rh.r, tol from |b|inf; a zero right-hand side gives x = 0
+                S[BS_RHO] = q0;
+                S[BS_RHO1] = S[BS_ALPHA] = S[BS_OMEGA] = 1.0;
+                S[BS_BETA] = 0.0;
+                S[BS_IT] = 0.0;
+                S[BS_TOL] = kBottomTol * q2;
+                ns = (q2 > 0.0 && q0 != 0.0 && isfinite(q0)) ? 1.0 : 0.0;
+                break;
+            case BP_V:  // alpha = rho / rh.v
+                if (!(q0 != 0.0) || !isfinite(q0)) ns = 0.0;
+                else S[BS_ALPHA] = S[BS_RHO] / q0;
+                break;
+            case BP_S:  // |s| small: x += alpha p and stop
+                if (q2 <= S[BS_TOL]) {
+                    ns = 2.0;
+                    S[BS_IT] += 1.0;
+                }
+                break;
+            case BP_T: {  // omega = t.s / t.t
+                const double om = q0 > 0.0 ? q1 / q0 : 0.0;
+                if (!(om != 0.0) || !isfinite(om)) {
+                    ns = 2.0;
+                    S[BS_IT] += 1.0;
+                } else {
+                    S[BS_OMEGA] = om;
+                }
+                break;
+            }
+            case BP_R:  // rho = rh.r; stop on |r| small, the iteration limit or a breakdown of rho
+                if (st == 2.0) {
+                    ns = 0.0;
+                    break;
+                }
+                S[BS_RHO1] = S[BS_RHO];
+                S[BS_RHO] = q0;
+                S[BS_IT] += 1.0;
+                if (q2 <= S[BS_TOL] || !isfinite(q2) || S[BS_IT] >= (double)kBottomMaxIter ||
+                    !(q0 != 0.0) || !isfinite(q0))
+                    ns = 0.0;
+                else
+                    S[BS_BETA] = (S[BS_RHO] / S[BS_RHO1]) * (S[BS_ALPHA] / S[BS_OMEGA]);
+                break;
+        }
+    S[BS_STATE] = ns;
+    state[blockIdx.x] = (int)ns;
 }
 
 // f[dst] = A f[src] (cycle precision), and the FP64 finest-level r64 = A u64.
@@ -744,6 +1010,9 @@ struct ela_rve {
     long long* seeds = nullptr;  // device copies of the seeds and rotations (model evaluation)
     double* rot = nullptr;
     double *u64 = nullptr, *b64 = nullptr, *r64 = nullptr, *M = nullptr, *scal = nullptr;
+    bool bicg = false;  // coarsest level too large for the dense inverse: BiCGStab
+    double* bs = nullptr;
+    void* bv[BV_N] = {};
     void* block = nullptr;
     size_t bytes = 0;
     // solve state
@@ -782,6 +1051,13 @@ struct ela_batch {
     };
     std::vector<March> mlev;
     March mtop;
+    // coarsest boxes solved by BiCGStab (in the order of the coarsest list)
+    size_t off_bdesc = 0, off_bbmap = 0;
+    std::vector<int> btiles;  // first block of each such box (size nbicg + 1)
+    std::vector<int> bpre;    // bpre[c]: such boxes among the first c boxes of the coarsest list
+    int* dbstate = nullptr;   // their states after a fold (device, and pinned host)
+    int* hbstate = nullptr;
+    size_t bstate_cap = 0;
     bool march = true;
     int kz = 16;
     bool xcd = false;  // XCD-contiguous tile order of the marching launches
@@ -857,6 +1133,11 @@ size_t layout_member(ela_rve* r, char* base) {
     }
     r->M = (double*)take((size_t)r->nunk * r->nunk * 8);
     r->scal = (double*)take(2 * 8);
+    if (r->bicg) {
+        const size_t N = (size_t)r->lev[r->depth - 1].nodes;
+        for (int q = 0; q < BV_N; ++q) r->bv[q] = take(3 * N * ts);
+        r->bs = (double*)take(BS_N * 8);
+    }
     return off;
 }
 
@@ -892,6 +1173,8 @@ BoxDesc desc_of(const ela_rve* r, int l) {
     if (l == r->depth - 1) {
         d.nunk = r->nunk;
         d.M = r->M;
+        d.bs = r->bs;
+        for (int q = 0; q < BV_N; ++q) d.bv[q] = r->bv[q];
     }
     d.norm = r->scal;
     d.scale = r->scal + 1;
@@ -998,6 +1281,17 @@ int build_lists(ela_batch* B) {
         B->mlev[g].off_bm = pack_ints(B, bm);
         B->mlev[g].off_bs = pack_ints(B, bs);
     }
+    ListBuild bl;
+    B->bpre.assign(1, 0);
+    if (G > 0) {
+        const BoxDesc* cd = (const BoxDesc*)(B->hlists.data() + B->off_desc[G - 1]);
+        for (int m = 0; m < B->cnt[G - 1]; ++m) {
+            if (cd[m].bs) list_push(bl, cd[m]);
+            B->bpre.push_back((int)bl.descs.size());
+        }
+    }
+    B->btiles = bl.tile.empty() ? std::vector<int>{0} : bl.tile;
+    pack_list(B, bl, &B->off_bdesc, &B->off_bbmap);
     ListBuild top;
     for (auto* r : B->active) list_push(top, desc_of(r, 0));
     B->top_tiles = top.tile.empty() ? std::vector<int>{0} : top.tile;
@@ -1008,8 +1302,21 @@ int build_lists(ela_batch* B) {
     if (int rc = ensure_device(B, (void**)&B->dlists, &B->dlists_cap, B->hlists.size())) return rc;
     if (!B->hlists.empty())
         ELA_HIPCHK(hipMemcpyAsync(B->dlists, B->hlists.data(), B->hlists.size(), hipMemcpyHostToDevice, B->stream));
-    const size_t nblk_top = std::max((size_t)B->top_tiles.back(), (size_t)(B->mtop.nm + B->mtop.ns));
+    const size_t nblk_top = std::max({(size_t)B->top_tiles.back(), (size_t)(B->mtop.nm + B->mtop.ns),
+                                      3 * (size_t)B->btiles.back()});
     if (int rc = ensure_device(B, (void**)&B->partials, &B->partials_cap, nblk_top * sizeof(double))) return rc;
+    const size_t nbc = B->btiles.size() - 1;
+    if (nbc && B->bstate_cap < nbc) {
+        if (B->dbstate) (void)hipFree(B->dbstate);
+        if (B->hbstate) (void)hipHostFree(B->hbstate);
+        B->dbstate = nullptr;
+        B->hbstate = nullptr;
+        B->bstate_cap = 0;
+        const size_t cap = std::max<size_t>(nbc * 2, 64);
+        ELA_HIPCHK(hipMalloc((void**)&B->dbstate, cap * sizeof(int)));
+        ELA_HIPCHK(hipHostMalloc((void**)&B->hbstate, cap * sizeof(int), hipHostMallocDefault));
+        B->bstate_cap = cap;
+    }
     if (B->norms_cap < B->active.size()) {
         if (B->dnorms) (void)hipFree(B->dnorms);
         if (B->hnorms) (void)hipHostFree(B->hnorms);
@@ -1131,6 +1438,43 @@ void axpy(ela_batch* B, int g, int count, int x, int y) {
     if (nb) hipLaunchKernelGGL((kb_axpy<T>), dim3(nb), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), x, y);
 }
 
+const BoxDesc* BD(const ela_batch* B) { return (const BoxDesc*)(B->dlists + B->off_bdesc); }
+const int* BM(const ela_batch* B) { return (const int*)(B->dlists + B->off_bbmap); }
+
+// BiCGStab on the first nbc boxes of the BiCGStab list: f[dst] = A^-1 f[src]
+// to 1e-4.  Every kBottomCheck iterations the host reads the states and stops
+// once no member iterates; launches past a member's end leave it unchanged.
+template <typename T>
+void bicg_bottom(ela_batch* B, int nbc, int src, int dst) {
+    const int nb = B->btiles[nbc];
+    const BoxDesc* D = BD(B);
+    const int* bm = BM(B);
+    hipStream_t st = B->stream;
+    auto fold = [&](int phase) {
+        hipLaunchKernelGGL(kbb_fold, dim3(nbc), dim3(64), 0, st, D, B->partials, phase, B->dbstate);
+    };
+    hipLaunchKernelGGL((kbb_init<T>), dim3(nb), dim3(TB), 0, st, D, bm, src, dst, B->partials);
+    fold(BP_INIT);
+    for (int it = 0; it < kBottomMaxIter; ++it) {
+        if (it % kBottomCheck == 0) {
+            (void)hipMemcpyAsync(B->hbstate, B->dbstate, nbc * sizeof(int), hipMemcpyDeviceToHost, st);
+            (void)hipStreamSynchronize(st);
+            bool any = false;
+            for (int m = 0; m < nbc && !any; ++m) any = B->hbstate[m] != 0;
+            if (!any) break;
+        }
+        hipLaunchKernelGGL((kbb_p<T>), dim3(nb), dim3(TB), 0, st, D, bm);
+        hipLaunchKernelGGL((kbb_apply<T>), dim3(nb), dim3(TB), 0, st, D, bm, 0, B->partials);
+        fold(BP_V);
+        hipLaunchKernelGGL((kbb_s<T>), dim3(nb), dim3(TB), 0, st, D, bm, B->partials);
+        fold(BP_S);
+        hipLaunchKernelGGL((kbb_apply<T>), dim3(nb), dim3(TB), 0, st, D, bm, 1, B->partials);
+        fold(BP_T);
+        hipLaunchKernelGGL((kbb_x<T>), dim3(nb), dim3(TB), 0, st, D, bm, dst, B->partials);
+        fold(BP_R);
+    }
+}
+
 template <typename T>
 void bottom(ela_batch* B, int count, int src, int dst) {
     const int g = B->G - 1;
@@ -1139,7 +1483,11 @@ void bottom(ela_batch* B, int count, int src, int dst) {
     int maxn = 0;
     for (int m = 0; m < count; ++m) maxn = std::max(maxn, hd[m].nunk);
     const int rpb = 4 * (TB / 64);
-    hipLaunchKernelGGL((kb_bottom<T>), dim3((maxn + rpb - 1) / rpb, count), dim3(TB), 0, B->stream, LD(B, g), src, dst);
+    if (maxn > 0)
+        hipLaunchKernelGGL((kb_bottom<T>), dim3((maxn + rpb - 1) / rpb, count), dim3(TB), 0, B->stream, LD(B, g), src,
+                           dst);
+    const int nbc = B->bpre[count];
+    if (nbc > 0) bicg_bottom<T>(B, nbc, src, dst);
 }
 
 // V-cycle from global level s on the members whose finest level is <= s; the
@@ -1340,7 +1688,8 @@ int prepare(ela_batch* B, int count, ela_rve* const* rves) {
         }
     // Set-up workspace: identities per coarsest size and [A | I] per member.
     std::vector<int> sizes;
-    for (auto* r : set) sizes.push_back(r->nunk);
+    for (auto* r : set)
+        if (r->nunk > 0) sizes.push_back(r->nunk);
     std::sort(sizes.begin(), sizes.end());
     sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
     size_t ws = 0;
@@ -1358,6 +1707,7 @@ int prepare(ela_batch* B, int count, ela_rve* const* rves) {
         hipLaunchKernelGGL(kb_identity, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, B->stream, eye[q], sizes[q]);
     }
     for (auto& d : L[G - 1].descs) {
+        if (d.nunk == 0) continue;
         const int q = (int)(std::lower_bound(sizes.begin(), sizes.end(), d.nunk) - sizes.begin());
         d.E = eye[q];
         d.W = (double*)(wbase + woff);
@@ -1398,12 +1748,14 @@ int prepare(ela_batch* B, int count, ela_rve* const* rves) {
             maxnodes = std::max(maxnodes, d.nodes);
         }
         const int nbk = (maxnodes + TB - 1) / TB;
-        hipLaunchKernelGGL(kb_probe, dim3(maxn * nbk, nbox), dim3(TB), 0, B->stream, D(G - 1), nbk);
-        for (int c = 0; c < maxn; ++c) {
-            hipLaunchKernelGGL(kb_gj_pivot, dim3(nbox), dim3(1024), 0, B->stream, D(G - 1), c, B->dflag);
-            hipLaunchKernelGGL(kb_gj_eliminate, dim3(maxn, nbox), dim3(256), 0, B->stream, D(G - 1), c);
+        if (maxn > 0) {  // (members with a BiCGStab coarsest level have nunk = 0 and are skipped)
+            hipLaunchKernelGGL(kb_probe, dim3(maxn * nbk, nbox), dim3(TB), 0, B->stream, D(G - 1), nbk);
+            for (int c = 0; c < maxn; ++c) {
+                hipLaunchKernelGGL(kb_gj_pivot, dim3(nbox), dim3(1024), 0, B->stream, D(G - 1), c, B->dflag);
+                hipLaunchKernelGGL(kb_gj_eliminate, dim3(maxn, nbox), dim3(256), 0, B->stream, D(G - 1), c);
+            }
+            hipLaunchKernelGGL(kb_gj_extract, dim3(maxn, nbox), dim3(256), 0, B->stream, D(G - 1));
         }
-        hipLaunchKernelGGL(kb_gj_extract, dim3(maxn, nbox), dim3(256), 0, B->stream, D(G - 1));
         int flag = 0;
         (void)hipMemcpyAsync(&flag, B->dflag, sizeof flag, hipMemcpyDeviceToHost, B->stream);
         const hipError_t e = hipStreamSynchronize(B->stream);
@@ -1709,6 +2061,8 @@ int ela_batch_destroy(ela_batch* B) {
     if (B->dnorms) (void)hipFree(B->dnorms);
     if (B->hnorms) (void)hipHostFree(B->hnorms);
     if (B->dflag) (void)hipFree(B->dflag);
+    if (B->dbstate) (void)hipFree(B->dbstate);
+    if (B->hbstate) (void)hipHostFree(B->hbstate);
     (void)hipStreamDestroy(B->stream);
     delete B;
     return ELA_OK;
@@ -1726,7 +2080,9 @@ size_t ela_rve_bytes(int n, int mode, int max_grains, int with_factor) {
     r.max_grains = max_grains;
     r.factor = with_factor != 0;
     const int c = coarsest_of(n);
-    r.nunk = 3 * (c + 1) * (c + 1) * (c + 1);
+    const int nunk = 3 * (c + 1) * (c + 1) * (c + 1);
+    r.bicg = nunk > kMaxBatchDense;
+    r.nunk = r.bicg ? 0 : nunk;
     r.lev.resize(r.depth);
     int m = n;
     for (int l = 0; l < r.depth; ++l, m /= 2) r.lev[l].nodes = (m + 1) * (m + 1) * (m + 1);
@@ -1740,16 +2096,14 @@ int ela_rve_create(ela_batch* B, int n, int max_grains, int with_factor, ela_rve
     if (max_grains < 1 || max_grains > 65536) return fail(ELA_ERR_BADARG, "max_grains must be 1..65536");
     const int c = coarsest_of(n);
     const int nunk = 3 * (c + 1) * (c + 1) * (c + 1);
-    if (nunk > kMaxBatchDense)
-        return fail(ELA_ERR_BADARG, "coarsest level of n = " + std::to_string(n) + " has " + std::to_string(c) +
-                                        " cells; members need n = c * 2^k with c <= 7");
     ela_rve* r = new ela_rve();
     r->B = B;
     r->n = n;
     r->depth = depth_of(n);
     r->max_grains = max_grains;
     r->factor = with_factor != 0;
-    r->nunk = nunk;
+    r->bicg = nunk > kMaxBatchDense;
+    r->nunk = r->bicg ? 0 : nunk;
     r->lev.resize(r->depth);
     int m = n;
     for (int l = 0; l < r->depth; ++l, m /= 2) {

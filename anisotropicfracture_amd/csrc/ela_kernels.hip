@@ -664,6 +664,31 @@ __global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src, d
         dst[q] = src[q];
 }
 
+// The copy with U 16-B loads in flight per lane (each of a wavefront's U
+// accesses is one coalesced 1 KiB run), all issued before the stores, and
+// optionally non-temporal stores (nothing reads the destination again).
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+template <int U, bool NT>
+__global__ __launch_bounds__(256) void k_copy_u(const dvec2* __restrict__ src, dvec2* __restrict__ dst, long long n) {
+    const long long stride = (long long)gridDim.x * 256 * U;
+    for (long long base = (long long)blockIdx.x * 256 * U + threadIdx.x; base < n; base += stride) {
+        dvec2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long q = base + (long long)u * 256;
+            if (q < n) v[u] = src[q];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long q = base + (long long)u * 256;
+            if (q < n) {
+                if (NT) __builtin_nontemporal_store(v[u], dst + q);
+                else dst[q] = v[u];
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- flat vector kernels (Krylov)
 // Vectors are whole level fields (dim * cstride doubles, padding kept at 0).
 // FG blocks of FB threads: 16 waves per CU.  Each lane moves V consecutive
@@ -991,6 +1016,25 @@ long long mdot_partials(int nv) { return (long long)nv * FG; }
 void launch_copy(const void* src, void* dst, long long bytes, hipStream_t s) {
     const long long n = bytes / 16;
     hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, s, (const double2*)src, (double2*)dst, n);
+}
+
+// Copy variants of the bandwidth probe: 0 is launch_copy (8192 blocks, one
+// 16-B access per lane and step); 1..6 use U = 4 or 8 accesses in flight per
+// lane, non-temporal stores, and 2048, 8192 or 32768 blocks.
+constexpr int kCopyVariants = 7;
+int copy_variants() { return kCopyVariants; }
+void launch_copy_variant(int var, const void* src, void* dst, long long bytes, hipStream_t s) {
+    const long long n = bytes / 16;
+    if (var <= 0) {
+        launch_copy(src, dst, bytes, s);
+        return;
+    }
+    const int blocks[3] = {2048, 8192, 32768};
+    const int b = blocks[(var - 1) % 3];
+    if (var <= 3)
+        hipLaunchKernelGGL((k_copy_u<4, true>), dim3(b), dim3(256), 0, s, (const dvec2*)src, (dvec2*)dst, n);
+    else
+        hipLaunchKernelGGL((k_copy_u<8, true>), dim3(b), dim3(256), 0, s, (const dvec2*)src, (dvec2*)dst, n);
 }
 
 // ------------------------------------------------------------------ launchers

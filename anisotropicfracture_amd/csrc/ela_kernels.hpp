@@ -215,6 +215,9 @@ void launch_add_to64(const Lay& L, int nc, const T* hold, const T* cor, double* 
 
 // Streaming copy of bytes (multiple of 16) from src to dst.
 void launch_copy(const void* src, void* dst, long long bytes, hipStream_t s);
+// Variant var in [0, copy_variants()) of the copy (0: launch_copy; others more bytes in flight).
+int copy_variants();
+void launch_copy_variant(int var, const void* src, void* dst, long long bytes, hipStream_t s);
 
 // Dense host-order array (nc planes of nx*ny*nz, already on the device) <-> level layout.
 template <typename T>

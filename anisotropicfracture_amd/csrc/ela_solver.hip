@@ -2206,7 +2206,12 @@ int ela_time_fapply(ela_hier* h, int prec, int nrep, double* avg_ms) {
 }
 
 int ela_probe_copy(ela_ctx* ctx, size_t bytes, int nrep, double* avg_ms) {
+    return ela_probe_copy_ex(ctx, bytes, nrep, 0, avg_ms);
+}
+
+int ela_probe_copy_ex(ela_ctx* ctx, size_t bytes, int nrep, int variant, double* avg_ms) {
     if (!ctx || !avg_ms || nrep < 1 || bytes < 16) return fail(ELA_ERR_BADARG, "bad argument");
+    if (variant < 0 || variant >= copy_variants()) return fail(ELA_ERR_BADARG, "unknown copy variant");
     bytes = bytes / 16 * 16;
     HIPCHK(hipSetDevice(ctx->device));
     void *a = nullptr, *b = nullptr;
@@ -2221,9 +2226,9 @@ int ela_probe_copy(ela_ctx* ctx, size_t bytes, int nrep, double* avg_ms) {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    launch_copy(a, b, (long long)bytes, s);
+    launch_copy_variant(variant, a, b, (long long)bytes, s);
     HIPCHK(hipEventRecord(e0, s));
-    for (int r = 0; r < nrep; ++r) launch_copy(a, b, (long long)bytes, s);
+    for (int r = 0; r < nrep; ++r) launch_copy_variant(variant, a, b, (long long)bytes, s);
     HIPCHK(hipEventRecord(e1, s));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;

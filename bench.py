@@ -426,7 +426,9 @@ def serving(seconds, max_wall, tol, budget_gb=200.0, max_active=256, rate=200.0,
            "p99_ms": float(np.percentile(lat, 99) * 1e3) if done else None,
            "served_Mnodes_per_s": float(nodes.sum() / wall / 1e6) if done else None,
            "budget_GB": budget_gb, "max_active": max_active, "alone_vs_mix_bitwise": same,
-           "requested_Mnodes": float(sum((q["n_requested"] + 1) ** 3 for _, q in done) / 1e6) if done else None,
+           "sides": "requested side rounded to the nearest multiple of 4 (the reference's amr.blocking_factor)",
+           "max_side_deviation": float(max(abs(q["n"] - q["n_requested"]) / q["n_requested"] for q in reqs)),
+           "bicgstab_bottom_requests": int(sum(1 for q in reqs if K.dense_coarsest_unknowns(q["n"]) > 1536)),
            "not_served_by_limit": int(sum(1 for r in res if r["status"] == -1)),
            "time_breakdown_s": summary}
     log("bench: serving at %.0f/s: %d/%d requests in %.1f s: %.1f solves/s p50 %.0f ms p99 %.0f ms"
@@ -474,7 +476,7 @@ def main():
     nodes = (n + 1) ** 3
     poly = n > 0 and "poly" in args.only.split(",")
     tmx = t64 = t64t = tmp = tgr = st64 = stmx = kt = None
-    umx = tuned = res_norms = disp_diff = probe_GBps = fa64 = fa32 = ga64 = ga32 = None
+    umx = tuned = res_norms = disp_diff = probe_GBps = probe_var = fa64 = fa32 = ga64 = ga32 = None
     dom = dom_ms = dom_bytes = achieved = traffic = levels = plain_cycles = None
     peak, nc = 8000.0, 21
     if poly:
@@ -549,8 +551,8 @@ def main():
             log("bench: plain MG FP64 (planes) %d cycles, rel %.2e" % (plain_cycles, stp["res"] / stp["res0"]))
         # HBM copy probe (also the byte-counter calibration kernel of the PMC runs)
         probe_bytes = 1 << 30
-        probe_ms = op.ctx.probe_copy(probe_bytes, 5)
-        probe_GBps = 2 * probe_bytes / (probe_ms * 1e-3) / 1e9
+        probe_var = {v: 2 * probe_bytes / (op.ctx.probe_copy(probe_bytes, 5, v) * 1e-3) / 1e9 for v in range(7)}
+        probe_GBps = max(probe_var.values())
 
         # Fapply alone (FP64 and FP32), HIP events on the library stream
         fa64 = op.time_fapply(ela.F64, 20)
@@ -664,6 +666,7 @@ def main():
         "iters": {"fp64": st64["iters"] if st64 else None, "mixed": stmx["iters"]},
         "rel_residual": {"fp64": st64["res"] / st64["res0"] if st64 else None, "mixed": stmx["res"] / stmx["res0"]},
         "hbm_copy_probe_GBps": probe_GBps,
+        "hbm_copy_probe_variants_GBps": probe_var,
         "rel_disp_diff_mixed_vs_fp64": disp_diff,
         "mixed_rel_residual_norms": res_norms,
         "fapply": {"fp64_ms": fa64, "fp64_GBps": bytes_fa64 / (fa64 * 1e-3) / 1e9,

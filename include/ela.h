@@ -369,6 +369,11 @@ int ela_time_fapply(ela_hier* h, int prec, int nrep, double* avg_ms);
  * plain streaming copy of `bytes` on the device (reads and writes `bytes`
  * each), average ms per launch.  It calibrates the HBM byte counters. */
 int ela_probe_copy(ela_ctx* ctx, size_t bytes, int nrep, double* avg_ms);
+/* The same with a copy variant: 0 the kernel above (8192 blocks, one 16-B
+ * access per lane and step); 1-3 four 16-B loads in flight per lane and
+ * non-temporal stores on 2048 / 8192 / 32768 blocks; 4-6 eight loads in
+ * flight, same grids.  The bandwidth probe reports the fastest. */
+int ela_probe_copy_ex(ela_ctx* ctx, size_t bytes, int nrep, int variant, double* avg_ms);
 
 /* ------------------------------------------------------------------ AMR level transfers
  *
@@ -487,8 +492,12 @@ int ela_amr_solve(ela_amr* a, double tol_rel, double tol_abs, int max_iter, ela_
  * with coarse_min = 2; a member's iterates do not depend on the other
  * members, so its solution is bitwise the same alone and in any mix.
  *
- * Members: n = c * 2^k with c <= 7 (the coarsest level is c cells, solved by
- * a dense inverse).  Fields on the host are dense (x fastest), 3 components. */
+ * Members: any n in 4..1024.  A member's levels halve n while it is even and
+ * n / 2 >= 2; its coarsest level of c cells is solved by a dense inverse when
+ * 3 (c+1)^3 <= 1536 (c <= 7), else by BiCGStab to 1e-4 (|r|inf relative, at
+ * most 200 iterations; the reference's bottom solver, Linear.H:23-29), with
+ * each member's scalars from its own partial sums.  Fields on the host are
+ * dense (x fastest), 3 components. */
 typedef struct ela_batch ela_batch;
 typedef struct ela_rve ela_rve;
 
@@ -546,7 +555,7 @@ int ela_rve_download(const ela_rve* r, int field, double* host);
 int ela_rve_grain_ids(const ela_rve* r, uint16_t* host);
 
 /* prepareForSolve for a set of members (src/Operator/Operator.cpp:445-452):
- * coarse stiffness, diagonals and the coarsest inverses, batched. */
+ * coarse stiffness, diagonals and the dense coarsest inverses, batched. */
 int ela_batch_prepare(ela_batch* b, int count, ela_rve* const* rves);
 
 /* Fapply (src/Operator/Elastic.cpp:151-260) on the finest level of each
@@ -571,7 +580,7 @@ int ela_batch_solve(ela_batch* b, int count, ela_rve* const* rves, double tol_re
 int ela_rve_status(const ela_rve* r, int* state, int* status, ela_stats* stats);
 
 /* One request of the serving loop: a Voronoi cubic polycrystal member of n
- * cells per side (n = c * 2^k, c <= 7), optionally cracked (ela_rve_set_crack
+ * cells per side (any n in 4..1024), optionally cracked (ela_rve_set_crack
  * with g(eta) = crack_g[0] eta^2 + crack_g[1] eta + crack_g[2]), loaded by the
  * affine displacement u = G x on every face.  u_out (3 (n+1)^3 doubles) or
  * NULL receives the solution. */

@@ -215,11 +215,45 @@ def test_member_admission_respects_the_hbm_budget():
     ctx.close()
 
 
-def test_member_sizes_outside_the_dense_coarsest_limit_are_rejected():
+def test_members_of_any_side_solve_with_a_bicgstab_coarsest_level(oracle):
+    """Sides whose coarsest level is beyond the dense inverse (36 -> 18 -> 9: 3000 unknowns; 27, odd: one
+    level, solved by BiCGStab alone) solve in one batch with a dense-coarsest member (20 -> 10 -> 5) and
+    match the oracle's MG with its BiCGStab bottom (the reference's bottom solver) to 1e-8."""
+    probs = [problem(61, 36, 9), problem(62, 27, 6, crack=True), problem(63, 20, 5)]
     b = E.Batch(E.SOLVE_MIXED)
-    with pytest.raises(E.ElaError) as ei:
-        b.rve(36, 4)  # 36 -> 18 -> 9: a 9-cell coarsest level
-    assert ei.value.code == 1
+    rs = [make(b, p) for p in probs]
+    b.prepare(rs)
+    sts = b.solve(rs, tol_rel=1e-10, max_cycles=80)
+    for r, p, st in zip(rs, probs, sts):
+        n = p["n"]
+        print("n=%d levels=%d cycles=%d res/res0=%.2e" % (n, r.levels(), st["iters"], st["res"] / st["res0"]))
+        assert st["converged"] and st["res"] <= 1e-10 * st["res0"]
+        bb = r.download(E.RVE.B)
+        ref, st_or, _ = oracle.mg_solve(oracle.Grid((n, n, n)), oracle.MM, planes(r, p), all_disp(), bb,
+                                        tol_rel=1e-11, coarse_min=2, bottom="bicgstab")
+        assert st_or["res"] <= 1e-11 * st_or["res0"], st_or
+        err = np.abs(r.download(E.RVE.U) - ref).max() / np.abs(ref).max()
+        assert err <= 1e-8, (n, err)
+    assert [r.levels() for r in rs] == [3, 1, 3]
+    b.close()
+
+
+def test_bicgstab_coarsest_member_is_the_same_alone_and_in_a_mix():
+    target = problem(71, 44, 8, crack=True)  # 44 -> 22 -> 11: BiCGStab on 3 * 12^3 unknowns
+    others = [problem(72, 27, 5), problem(73, 16, 7), problem(74, 36, 6)]
+    b = E.Batch(E.SOLVE_MIXED)
+    r = make(b, target)
+    b.prepare([r])
+    st_alone = b.solve([r], tol_rel=1e-10, max_cycles=80)[0]
+    alone = r.download(E.RVE.U)
+    b.close()
+    b = E.Batch(E.SOLVE_MIXED)
+    rs = [make(b, others[0]), make(b, target)] + [make(b, p) for p in others[1:]]
+    b.prepare(rs)
+    sts = b.solve(rs, tol_rel=1e-10, max_cycles=80)
+    assert all(st["converged"] for st in sts)
+    assert sts[1]["iters"] == st_alone["iters"]
+    assert np.array_equal(rs[1].download(E.RVE.U), alone)
     b.close()
 
 
