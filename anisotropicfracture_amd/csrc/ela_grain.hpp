@@ -15,9 +15,17 @@
 
 namespace ela {
 
-constexpr int MTX = 32, MTY = 8, MRW = MTY + 2, MCW = MTX + 2, MPL = MRW * MCW;
-constexpr int MHALO = 2 * MCW + 2 * MTY;  // nodes of a tile plane's one-node halo ring
-static_assert(3 * MHALO <= MTX * MTY, "one halo node per thread and component");
+constexpr int MTX = 32, MTY = 8, MCW = MTX + 2;
+// A tile is MTX x (MTY RY) nodes: each of the MTX x MTY threads takes RY
+// adjacent rows (j), whose stencils share two thirds of their LDS reads.
+template <int RY>
+struct MTile {
+    static constexpr int TY = MTY * RY, RW = TY + 2, PL = RW * MCW;
+    static constexpr int HALO = 2 * MCW + 2 * TY;                       // nodes of a plane's one-node halo ring
+    static constexpr int NH = (3 * HALO + MTX * MTY - 1) / (MTX * MTY);  // halo loads per thread and plane
+};
+constexpr int MRW = MTile<1>::RW, MPL = MTile<1>::PL, MHALO = MTile<1>::HALO;
+static_assert(MTile<1>::NH == 1 && MTile<2>::NH == 2, "halo loads per thread");
 constexpr int kGrainNC = 21;
 constexpr int kMarchMaxGrains = 256;  // the table is staged in LDS up to this many grains
 
@@ -30,11 +38,11 @@ __host__ __device__ inline int grain_x0(const Lay& L) { return L.px % MTX == 0 ?
 // SC: the launch carries per-node factors (LDS ss holds them, 1 for a box
 // without); else the records are read as they are.  The values are the same,
 // since 1 x = x exactly.
-template <typename T, bool SC>
+template <typename T, bool SC, int PL = MPL>
 struct LdsCtx {
-    const T* su;               // [3 slots][3 comps][MRW][MCW]
-    const unsigned short* sg;  // [3 slots][MRW][MCW]
-    const T* ss;               // [3 slots][MRW][MCW] (SC only)
+    const T* su;               // [slots][3 comps][rows][MCW] (PL = rows * MCW per component plane)
+    const unsigned short* sg;  // [slots][rows][MCW]
+    const T* ss;               // [slots][rows][MCW] (SC only)
     const T* stab;             // grain table
     int pb[3];                 // slot * MPL of planes k-1, k, k+1 (uniform over the block)
     int tb;                    // this node's row * MCW + column in the tile arrays
@@ -42,7 +50,7 @@ struct LdsCtx {
     // so each access is one LDS read with an immediate offset.
     __device__ __forceinline__ int at(int dx, int dy, int dz) const { return pb[dz + 1] + tb + (dy * MCW + dx); }
     __device__ __forceinline__ T u(int p, int dx, int dy, int dz) const {
-        return su[3 * pb[dz + 1] + tb + (p * MPL + dy * MCW + dx)];
+        return su[3 * pb[dz + 1] + tb + (p * PL + dy * MCW + dx)];
     }
     __device__ __forceinline__ auto rec(int dx, int dy, int dz) const {
         const int q = at(dx, dy, dz);
@@ -74,6 +82,7 @@ struct GrainBox {
     const T* sc;
     int ngr;
     int ntx, nty, kz;
+    int ry;   // rows per thread of the marching tiles (1 or 2)
     int xcd;  // XCD-contiguous tile order
     bool nt;     // rhs, diagonal and Jacobi output non-temporal (see epilogue_load)
     bool xface;  // x faces in face blocks (face_patch); the shell then skips them
@@ -148,73 +157,115 @@ __device__ __forceinline__ double grain_epilogue(const GrainBox<T>& g, long long
 // while the fourth is written, so a plane needs one barrier, not two).  The
 // factor planes exist only in launches that carry factors (SC).
 constexpr int MRING = 4;
-template <typename T, bool SC>
+template <typename T, bool SC, int RY = 1>
 struct MarchLds {
-    T su[MRING * 3 * MPL];
-    unsigned short sg[MRING * MPL];
-    T ss[SC ? MRING * MPL : 1];
+    static constexpr int PL = MTile<RY>::PL;
+    T su[MRING * 3 * PL];
+    unsigned short sg[MRING * PL];
+    T ss[SC ? MRING * PL : 1];
 };
 
 // Tile t of box g (256 threads); returns the thread's max |v| (MODE 2).
 // Every thread of the block calls this (it synchronizes).
-template <typename T, int MODE, typename TO, bool SC>
-__device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchLds<T, SC>& sh, T* stab) {
+template <typename T, int MODE, typename TO, bool SC, int RY = 1>
+__device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchLds<T, SC, RY>& sh, T* stab) {
     static_assert(MODE != 2 || sizeof(T) == 8, "the solution residual is FP64");
+    using TL = MTile<RY>;
+    constexpr int PL = TL::PL, HALO = TL::HALO, NH = TL::NH, RW = TL::RW;
     const Lay& L = g.L;
     const int tid = threadIdx.x;
     const int bx = t % g.ntx, by = (t / g.ntx) % g.nty, bz = t / (g.ntx * g.nty);
-    const int i0 = grain_x0(L) + bx * MTX, j0 = 1 + by * MTY;  // first node of the tile
+    const int i0 = grain_x0(L) + bx * MTX, j0 = 1 + by * TL::TY;  // first node of the tile
     const int tx = tid % MTX, ty = tid / MTX;
-    const int i = i0 + tx, j = j0 + ty;
-    const bool active = i >= 1 && i <= L.nx - 2 && j <= L.ny - 2;
+    const int i = i0 + tx;
     const int kb = 1 + bz * g.kz, ke = min(kb + g.kz, L.nz - 1);
     for (int q = tid; q < g.ngr * kGrainNC; q += MTX * MTY) stab[q] = g.tab[q];
 
-    // Plane loads.  Every thread reads its own node (i, j) of the 32 x 8 tile
-    // for the three components, at one 32-bit offset from the plane's base;
-    // the threads tid < 3 * MHALO read one node of the one-node halo ring each
-    // (component tid / MHALO), and the first MHALO of them, whose halo node is
-    // of component 0, also read its grain id (and factor).  Offsets fit 32
-    // bits: grain_offsets_fit holds for every box marched.
-    const int tb = (ty + 1) * MCW + tx + 1;  // this node's position in a tile plane
-    const bool iok = i < L.nx && j < L.ny;
-    const unsigned ie = (unsigned)(i + L.px * j);
-    const int hp = tid / MHALO, hh = tid - hp * MHALO;
-    const int hr = hh < MCW ? 0 : hh < 2 * MCW ? MRW - 1 : 1 + ((hh - 2 * MCW) >> 1);
-    const int hc = hh < MCW ? hh : hh < 2 * MCW ? hh - MCW : (((hh - 2 * MCW) & 1) ? MCW - 1 : 0);
-    const bool hok = tid < 3 * MHALO && i0 - 1 + hc >= 0 && i0 - 1 + hc < L.nx && j0 - 1 + hr < L.ny;
-    const bool gok = hok && tid < MHALO;
-    const unsigned he = (unsigned)(hp * L.cs + (i0 - 1 + hc) + L.px * (j0 - 1 + hr));
-    const int hpos = hp * MPL + hr * MCW + hc;
+    // Plane loads.  Every thread reads its own RY nodes (i, j0 + RY ty + y) of
+    // the tile for the three components, at 32-bit offsets from the plane's
+    // base; the halo ring's 3 HALO values (node, component) are spread over the
+    // threads, NH per thread, and the loads of component 0 also read the node's
+    // grain id (and factor).  Offsets fit 32 bits: grain_offsets_fit holds for
+    // every box marched.
+    int tb[RY];
+    bool act[RY], iok[RY];
+    unsigned ie[RY];
+#pragma unroll
+    for (int y = 0; y < RY; ++y) {
+        const int r = ty * RY + y, j = j0 + r;
+        tb[y] = (r + 1) * MCW + tx + 1;  // this node's position in a tile plane
+        act[y] = i >= 1 && i <= L.nx - 2 && j <= L.ny - 2;
+        iok[y] = i < L.nx && j < L.ny;
+        ie[y] = (unsigned)(i + L.px * j);
+    }
+    unsigned he[NH];
+    int hpos[NH];
+    bool hok[NH], gok[NH];
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+        const int e = tid + h * (MTX * MTY);
+        const int hp = e / HALO, hh = e - hp * HALO;
+        const int hr = hh < MCW ? 0 : hh < 2 * MCW ? RW - 1 : 1 + ((hh - 2 * MCW) >> 1);
+        const int hc = hh < MCW ? hh : hh < 2 * MCW ? hh - MCW : (((hh - 2 * MCW) & 1) ? MCW - 1 : 0);
+        hok[h] = e < 3 * HALO && i0 - 1 + hc >= 0 && i0 - 1 + hc < L.nx && j0 - 1 + hr < L.ny;
+        gok[h] = hok[h] && e < HALO;
+        he[h] = (unsigned)(hp * L.cs + (i0 - 1 + hc) + L.px * (j0 - 1 + hr));
+        hpos[h] = hp * PL + hr * MCW + hc;
+    }
     const bool scl = SC && g.sc != nullptr;  // a batched launch takes SC when any of its boxes has
                                              // a factor; the boxes without one read s = 1
-    T ru[3], rh, rs = 1, rsh = 1;
-    unsigned short rg = 0, rgh = 0;
+    T ru[RY][3], rh[NH], rs[RY], rsh[NH];
+    unsigned short rg[RY], rgh[NH];
+#pragma unroll
+    for (int y = 0; y < RY; ++y) {
+        rs[y] = 1;
+        rg[y] = 0;
+    }
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+        rsh[h] = 1;
+        rgh[h] = 0;
+    }
     auto at = [](const void* base, unsigned bytes) { return (const char*)base + bytes; };
     auto load = [&](int kk) {
         const T* up = g.in + L.pl * kk;
-#pragma unroll
-        for (int p = 0; p < 3; ++p) ru[p] = iok ? *(const T*)at(up + p * L.cs, ie * (unsigned)sizeof(T)) : T(0);
-        rh = hok ? *(const T*)at(up, he * (unsigned)sizeof(T)) : T(0);
         const unsigned short* gp = g.gid + L.pl * kk;
-        rg = iok ? *(const unsigned short*)at(gp, ie * 2u) : (unsigned short)0;
-        rgh = gok ? *(const unsigned short*)at(gp, he * 2u) : (unsigned short)0;
+#pragma unroll
+        for (int y = 0; y < RY; ++y)
+#pragma unroll
+            for (int p = 0; p < 3; ++p)
+                ru[y][p] = iok[y] ? *(const T*)at(up + p * L.cs, ie[y] * (unsigned)sizeof(T)) : T(0);
+#pragma unroll
+        for (int h = 0; h < NH; ++h) rh[h] = hok[h] ? *(const T*)at(up, he[h] * (unsigned)sizeof(T)) : T(0);
+#pragma unroll
+        for (int y = 0; y < RY; ++y) rg[y] = iok[y] ? *(const unsigned short*)at(gp, ie[y] * 2u) : (unsigned short)0;
+#pragma unroll
+        for (int h = 0; h < NH; ++h) rgh[h] = gok[h] ? *(const unsigned short*)at(gp, he[h] * 2u) : (unsigned short)0;
         if constexpr (SC) {
             const T* sp = g.sc + L.pl * kk;
-            rs = iok && scl ? *(const T*)at(sp, ie * (unsigned)sizeof(T)) : T(1);
-            rsh = gok && scl ? *(const T*)at(sp, he * (unsigned)sizeof(T)) : T(1);
+#pragma unroll
+            for (int y = 0; y < RY; ++y) rs[y] = iok[y] && scl ? *(const T*)at(sp, ie[y] * (unsigned)sizeof(T)) : T(1);
+#pragma unroll
+            for (int h = 0; h < NH; ++h) rsh[h] = gok[h] && scl ? *(const T*)at(sp, he[h] * (unsigned)sizeof(T)) : T(1);
         }
     };
     auto store = [&](int slot) {
-        T* su = sh.su + slot * 3 * MPL;
+        T* su = sh.su + slot * 3 * PL;
 #pragma unroll
-        for (int p = 0; p < 3; ++p) su[p * MPL + tb] = ru[p];
-        if (tid < 3 * MHALO) su[hpos] = rh;
-        sh.sg[slot * MPL + tb] = rg;
-        if (tid < MHALO) sh.sg[slot * MPL + hpos] = rgh;
-        if constexpr (SC) {
-            sh.ss[slot * MPL + tb] = rs;
-            if (tid < MHALO) sh.ss[slot * MPL + hpos] = rsh;
+        for (int y = 0; y < RY; ++y) {
+#pragma unroll
+            for (int p = 0; p < 3; ++p) su[p * PL + tb[y]] = ru[y][p];
+            sh.sg[slot * PL + tb[y]] = rg[y];
+            if constexpr (SC) sh.ss[slot * PL + tb[y]] = rs[y];
+        }
+#pragma unroll
+        for (int h = 0; h < NH; ++h) {
+            const int e = tid + h * (MTX * MTY);
+            if (e < 3 * HALO) su[hpos[h]] = rh[h];
+            if (e < HALO) {  // component 0: the node's grain id (and factor)
+                sh.sg[slot * PL + hpos[h]] = rgh[h];
+                if constexpr (SC) sh.ss[slot * PL + hpos[h]] = rsh[h];
+            }
         }
     };
     // ring slots of planes k-1, k, k+1 and the free slot s3, which receives
@@ -230,21 +281,27 @@ __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchL
     double m = 0.0;
     for (int k = kb; k < ke; ++k) {
         const bool more = k + 1 < ke;
-        const long long o = (long long)i + L.px * j + L.pl * k;
-        // the epilogue's loads are issued before the stencil, so their latency
-        // overlaps it, as the next plane's loads do
-        T rb[3], db[3];
-        if (active) epilogue_load<T, MODE>(g, o, rb, db);
+        // the epilogue's loads are issued before the stencils, so their latency
+        // overlaps them, as the next plane's loads do (separate scalars per
+        // row: arrays indexed by the row cost the one-row kernel registers)
+        const long long o0 = (long long)i + L.px * (j0 + ty * RY) + L.pl * k, o1 = o0 + L.px;
+        T rb0[3], db0[3], rb1[3], db1[3];
+        if (act[0]) epilogue_load<T, MODE>(g, o0, rb0, db0);
+        if constexpr (RY == 2)
+            if (act[RY - 1]) epilogue_load<T, MODE>(g, o1, rb1, db1);
         if (more) load(k + 2);
-        if (active) {
-            LdsCtx<T, SC> S{sh.su, sh.sg, sh.ss, stab, {s0 * MPL, s1 * MPL, s2 * MPL}, tb};
-            T f[3], xc[3];
+        auto node = [&](int y, long long o, const T* rb, const T* db) {
+            LdsCtx<T, SC, PL> S{sh.su, sh.sg, sh.ss, stab, {s0 * PL, s1 * PL, s2 * PL}, tb[y]};
+            T f[3], xc[3], r3[3] = {rb[0], rb[1], rb[2]}, d3[3] = {db[0], db[1], db[2]};
             interior_apply<T, K_MM, 3>(S, g.iv, g.uniform != 0, f);
 #pragma unroll
             for (int p = 0; p < 3; ++p) xc[p] = S.u(p, 0, 0, 0);
-            const double mm = grain_epilogue<T, MODE, TO>(g, o, xc, f, rb, db);
+            const double mm = grain_epilogue<T, MODE, TO>(g, o, xc, f, r3, d3);
             m = (mm > m || mm != mm) ? mm : m;
-        }
+        };
+        if (act[0]) node(0, o0, rb0, db0);
+        if constexpr (RY == 2)
+            if (act[RY - 1]) node(RY - 1, o1, rb1, db1);
         // s3 held plane k-2, whose last reads were in the previous plane,
         // before the previous barrier
         if (more) store(s3);
@@ -362,11 +419,11 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
 // plane's base (three components of FP64 at most).
 __host__ __device__ inline bool grain_offsets_fit(const Lay& L) { return 3LL * L.cs * 8 < (1LL << 32); }
 
-// Tiles and shell blocks of a box (host and device).
+// Tiles and shell blocks of a box (host and device); ry rows per thread.
 __host__ __device__ inline void grain_tiles(const Lay& L, int kz, int& ntx, int& nty, int& ntz, int& nsh,
-                                            bool xface = false) {
+                                            bool xface = false, int ry = 1) {
     ntx = (L.nx - 1 - grain_x0(L) + MTX - 1) / MTX;
-    nty = (L.ny - 2 + MTY - 1) / MTY;
+    nty = (L.ny - 2 + MTY * ry - 1) / (MTY * ry);
     ntz = (L.nz - 2 + kz - 1) / kz;
     const long long shell = (long long)L.nx * L.ny * L.nz - (long long)(L.nx - 2) * (L.ny - 2) * (L.nz - 2) -
                             (xface ? 2LL * (L.ny - 2) * (L.nz - 2) : 0LL);

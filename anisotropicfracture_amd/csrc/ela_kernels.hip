@@ -1803,9 +1803,14 @@ void launch_newton_rhs(const Lay& L, int kind, const double* C, const double* F0
 // running after them.  nshp is a multiple of 8, so a tile's hardware block
 // lands on the XCD that xcd_block assigns it.
 namespace {
-template <typename T, int MODE, typename TO, bool SC>
-__global__ __launch_bounds__(MTX* MTY) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1))) void k_grain_sweep(GrainBox<T> g, int nshp, int nfp, double* partials) {
-    __shared__ MarchLds<T, SC> sh;
+// Occupancy floor of the sweep: FP32 with one row per thread keeps 5 waves per
+// SIMD (93 VGPRs); two rows per thread need about 155 VGPRs (two stencils in
+// flight), so 3 waves (at 4 the compiler spills).
+template <typename T, int RY>
+constexpr int grain_waves() { return sizeof(T) == 4 ? (RY == 1 ? 5 : 3) : 1; }
+template <typename T, int MODE, typename TO, bool SC, int RY>
+__global__ __launch_bounds__(MTX* MTY) __attribute__((amdgpu_waves_per_eu(grain_waves<T, RY>()))) void k_grain_sweep(GrainBox<T> g, int nshp, int nfp, double* partials) {
+    __shared__ MarchLds<T, SC, RY> sh;
     extern __shared__ __align__(16) unsigned char dyn_lds[];
     const int b = (int)blockIdx.x;
     double m;
@@ -1816,7 +1821,7 @@ __global__ __launch_bounds__(MTX* MTY) __attribute__((amdgpu_waves_per_eu(sizeof
     } else {
         const int nm = (int)gridDim.x - nshp - nfp, bm = b - nshp - nfp;
         const int t = g.xcd ? xcd_block(bm, nm) : bm;
-        m = march_tile<T, MODE, TO, SC>(g, t, sh, (T*)dyn_lds);
+        m = march_tile<T, MODE, TO, SC, RY>(g, t, sh, (T*)dyn_lds);
     }
     if (MODE == 2 && partials) block_max_store(partials, m);  // a max: the slot order does not matter
 }
@@ -1834,19 +1839,24 @@ long long grain_partials(const Lay& L, int kz) {
 template <typename T, int MODE, typename TO>
 void launch_grain(GrainBox<T> g, int kz, double* partials, unsigned long long* norm, hipStream_t s) {
     int ntx, nty, ntz, nsh;
-    grain_tiles(g.L, kz, ntx, nty, ntz, nsh, g.xface);
+    if (g.ry != 2 || sizeof(T) != 4) g.ry = 1;  // two rows per thread: FP32 (the FP64 ring would need 64 KB)
+    grain_tiles(g.L, kz, ntx, nty, ntz, nsh, g.xface, g.ry);
     g.ntx = ntx;
     g.nty = nty;
     g.kz = kz;
     const int nm = ntx * nty * ntz, nshp = (nsh + 7) / 8 * 8;
     const int nfp = g.xface ? (grain_face_patches(g.L) + 7) / 8 * 8 : 0;
     const size_t lds = (size_t)g.ngr * kGrainNC * sizeof(T);
-    if (g.sc)
-        hipLaunchKernelGGL((k_grain_sweep<T, MODE, TO, true>), dim3(nshp + nfp + nm), dim3(MTX * MTY), lds, s, g,
-                           nshp, nfp, partials);
+    const dim3 grid(nshp + nfp + nm), block(MTX * MTY);
+    constexpr int R2 = sizeof(T) == 4 ? 2 : 1;
+    if (g.sc && g.ry == 2)
+        hipLaunchKernelGGL((k_grain_sweep<T, MODE, TO, true, R2>), grid, block, lds, s, g, nshp, nfp, partials);
+    else if (g.sc)
+        hipLaunchKernelGGL((k_grain_sweep<T, MODE, TO, true, 1>), grid, block, lds, s, g, nshp, nfp, partials);
+    else if (g.ry == 2)
+        hipLaunchKernelGGL((k_grain_sweep<T, MODE, TO, false, R2>), grid, block, lds, s, g, nshp, nfp, partials);
     else
-        hipLaunchKernelGGL((k_grain_sweep<T, MODE, TO, false>), dim3(nshp + nfp + nm), dim3(MTX * MTY), lds, s, g,
-                           nshp, nfp, partials);
+        hipLaunchKernelGGL((k_grain_sweep<T, MODE, TO, false, 1>), grid, block, lds, s, g, nshp, nfp, partials);
     if (MODE == 2 && norm)
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partials, nm + nshp + nfp, norm);
 }

@@ -493,7 +493,7 @@ def main():
             op.set_option(ela.OPT_XCD, 1)
         elif args.table:  # marching tile shape for this hierarchy, picked before the timed region
             op.SetStiffnessMode(1)
-            tuned = {"%d/%d" % k: v for k, v in op.autotune(nrep=10).items()}
+            tuned = {"%d/%d/%d" % k: v for k, v in op.autotune(nrep=10).items()}
             op.SetStiffnessMode(0)
         b_host = op.bc_init()
         b = op.field(0, ela.F64).upload(b_host)

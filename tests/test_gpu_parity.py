@@ -293,26 +293,32 @@ def test_grain_table_solve_matches_oracle(oracle, mode, krylov):
 @pytest.mark.parametrize("mode", [ela.SOLVE_FP64, ela.SOLVE_MIXED])
 @pytest.mark.parametrize("krylov", [0, 16])
 def test_grain_marching_kernels_give_the_node_kernel_bits(mode, krylov):
-    """Level 0 in grain form through the LDS z-marching kernels and through the one-thread-per-node
-    kernels: the same arithmetic, so the same solve bit for bit (a 40^3 box has partial tiles in x, y, z)."""
+    """Level 0 in grain form through the LDS z-marching kernels (one and two rows per thread) and through
+    the one-thread-per-node kernels: the same arithmetic, so the same solve and the same Fapply bit for bit
+    (a 40^3 box has partial tiles in x, y, z)."""
     cfg = K.polycrystal(40, grains=14, seed=5)
-    sols, iters = [], []
-    for march in (0, 1):
+    u1 = H.random_field(np.random.default_rng(8), cfg["n"], 3)
+    sols, iters, fa = [], [], []
+    for march, rows in ((0, 1), (1, 1), (1, 2)):
         op = ela.Elastic(cfg["n"], MM, mode=ela.ALLOC_BOTH)
         op.SetModelVoronoiCubic(cfg["seeds"], K.VORONOI_Q, cfg["R"], cfg["C11"], cfg["C12"], cfg["C44"])
         op.SetBC(cfg["bc_type"], cfg["bc_value"])
         op.SetStiffnessMode(1)
         op.SetKrylov(krylov)
         op.set_option(ela.OPT_MARCH, march)
+        op.set_option(ela.OPT_ROWS, rows)
         op.prepareForSolve()
+        fa.append((op.Fapply(u1, prec=ela.F64), op.Fapply(u1, prec=ela.F32)))
         b = op.bc_init()
         u, st = ela.Linear(op).solve(np.zeros_like(b), b, tol_rel=1e-10, tol_abs=0.0, mode=mode, max_iter=300)
         assert st["converged"]
         sols.append(u)
         iters.append(st["iters"])
         op.close()
-    assert iters[0] == iters[1]
-    assert np.array_equal(sols[0], sols[1])
+    for v in (1, 2):
+        assert iters[0] == iters[v]
+        assert np.array_equal(sols[0], sols[v])
+        assert np.array_equal(fa[0][0], fa[v][0]) and np.array_equal(fa[0][1], fa[v][1])
 
 
 @pytest.mark.parametrize("mode", [ela.SOLVE_FP64, ela.SOLVE_MIXED])
@@ -411,7 +417,7 @@ def test_autotune_picks_a_tile_shape_without_changing_results():
         op.prepareForSolve()
         if tune:
             times = op.autotune(nrep=3)
-            assert len(times) == 6 and all(t > 0 for t in times.values())
+            assert len(times) == 12 and all(t > 0 for t in times.values())
         b = op.bc_init()
         u, st = ela.Linear(op).solve(np.zeros_like(b), b, tol_rel=1e-10, tol_abs=0.0, mode=ela.SOLVE_MIXED, max_iter=300)
         assert st["converged"]
