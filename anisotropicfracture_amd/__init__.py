@@ -141,6 +141,7 @@ SIGNATURES = {
     "ela_set_option": ([_P, _I, _I], _I),
     "ela_autotune": ([_P, _I, _P], _I),
     "ela_batch_autotune": ([_P, _I, _P, _I, _P], _I),
+    "ela_batch_graph_stats": ([_P, ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
     "ela_kernel_times": ([_P, ctypes.POINTER(_D), ctypes.POINTER(_I), ctypes.POINTER(_D), ctypes.POINTER(_I)], _I),
     "ela_time_fapply": ([_P, _I, _I, ctypes.POINTER(_D)], _I),
     "ela_probe_copy": ([_P, _SZ, _I, ctypes.POINTER(_D)], _I),
@@ -1073,9 +1074,16 @@ class Batch:
     MARCH, KZ = 0, 1
 
     def set_option(self, option, value):
-        """ela_batch_set_option: OPT_MARCH (LDS z-marching kernels on the grain-form levels, default on)
-        or OPT_KZ (planes per marching tile, default 16)."""
+        """ela_batch_set_option: OPT_MARCH (LDS z-marching kernels on the grain-form levels, default on),
+        OPT_KZ (planes per marching tile, default 16), OPT_XCD (tile order) or OPT_GRAPH (each step's F-cycle
+        replayed from a captured HIP graph while the active set is unchanged, default off)."""
         _check(lib().ela_batch_set_option(self.ptr, option, int(value)))
+
+    def graph_stats(self):
+        """(captures, replays) of the F-cycle graph (OPT_GRAPH)."""
+        c, r = ctypes.c_int(), ctypes.c_int()
+        _check(lib().ela_batch_graph_stats(self.ptr, ctypes.byref(c), ctypes.byref(r)))
+        return c.value, r.value
 
     def autotune(self, rves, nrep=5):
         """ela_batch_autotune over the finest levels of `rves`; returns {(kz, xcd): ms}."""

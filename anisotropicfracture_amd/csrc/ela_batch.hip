@@ -1061,6 +1061,14 @@ struct ela_batch {
     bool march = true;
     int kz = 16;
     bool xcd = false;  // XCD-contiguous tile order of the marching launches
+    // ELA_OPT_GRAPH: the F-cycle's launches captured into a HIP graph, re-captured
+    // when the launch lists change (list_epoch), replayed otherwise
+    bool graph_on = false;
+    long long list_epoch = 0;
+    hipGraphExec_t gexec = nullptr;
+    long long gexec_epoch = -1;
+    size_t gexec_tsize = 0;
+    int graph_captures = 0, graph_replays = 0;
     char* dlists = nullptr;
     size_t dlists_cap = 0;
     std::vector<char> hlists;
@@ -1329,6 +1337,7 @@ int build_lists(ela_batch* B) {
         B->norms_cap = cap;
     }
     B->dirty = false;
+    ++B->list_epoch;
     return ELA_OK;
 }
 
@@ -1598,11 +1607,49 @@ int admit(ela_batch* B) {
     return ELA_OK;
 }
 
+// The F-cycle as a HIP graph (ELA_OPT_GRAPH): captured from the stream when
+// the launch lists changed since the last capture, else replayed.  Kernel
+// nodes only (the cycle has no memsets or copies).  Cycles with a BiCGStab
+// coarsest level read the members' states on the host, and timed cycles
+// record events, so those run on the stream.
+template <typename T>
+int fcycle_graph(ela_batch* B) {
+    if (!B->gexec || B->gexec_epoch != B->list_epoch || B->gexec_tsize != sizeof(T)) {
+        if (B->gexec) {
+            ELA_HIPCHK(hipStreamSynchronize(B->stream));  // its last replay has finished
+            (void)hipGraphExecDestroy(B->gexec);
+            B->gexec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        ELA_HIPCHK(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+        fcycle<T>(B);
+        const hipError_t e = hipStreamEndCapture(B->stream, &g);
+        if (e != hipSuccess) return fail(ELA_ERR_HIP, std::string("batch cycle capture: ") + hipGetErrorString(e));
+        const hipError_t ei = hipGraphInstantiate(&B->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            B->gexec = nullptr;
+            return fail(ELA_ERR_HIP, std::string("batch cycle graph: ") + hipGetErrorString(ei));
+        }
+        B->gexec_epoch = B->list_epoch;
+        B->gexec_tsize = sizeof(T);
+        ++B->graph_captures;
+    } else {
+        ++B->graph_replays;
+    }
+    ELA_HIPCHK(hipGraphLaunch(B->gexec, B->stream));
+    return ELA_OK;
+}
+
 // One F-cycle on every active member, then the FP64 update and residual;
 // members that converge, fail or run out of cycles leave the active set.
 template <typename T>
 int step_t(ela_batch* B) {
-    fcycle<T>(B);
+    if (B->graph_on && !B->timing && B->bpre.back() == 0) {
+        if (int rc = fcycle_graph<T>(B)) return rc;
+    } else {
+        fcycle<T>(B);
+    }
     const int nbox = (int)B->active.size();
     const int nblk = B->top_tiles.back();
     hipLaunchKernelGGL((kb_update<T>), dim3(nblk), dim3(TB), 0, B->stream, TD(B), TM(B), F_COR);
@@ -2063,6 +2110,7 @@ int ela_batch_destroy(ela_batch* B) {
     if (B->dflag) (void)hipFree(B->dflag);
     if (B->dbstate) (void)hipFree(B->dbstate);
     if (B->hbstate) (void)hipHostFree(B->hbstate);
+    if (B->gexec) (void)hipGraphExecDestroy(B->gexec);  // the stream was synchronized above
     (void)hipStreamDestroy(B->stream);
     delete B;
     return ELA_OK;
@@ -2384,10 +2432,19 @@ int ela_batch_set_option(ela_batch* B, int option, int value) {
         B->kz = value;
     } else if (option == ELA_OPT_XCD) {
         B->xcd = value != 0;
+    } else if (option == ELA_OPT_GRAPH) {
+        B->graph_on = value != 0;
     } else {
         return fail(ELA_ERR_BADARG, "unknown option");
     }
     B->dirty = true;
+    return ELA_OK;
+}
+
+int ela_batch_graph_stats(const ela_batch* B, int* captures, int* replays) {
+    if (!B) return fail(ELA_ERR_BADARG, "null batch");
+    if (captures) *captures = B->graph_captures;
+    if (replays) *replays = B->graph_replays;
     return ELA_OK;
 }
 

@@ -671,8 +671,16 @@ int ela_server_poll(ela_server* s, int64_t ticket, int* done, ela_result* out);
 int ela_server_stop(ela_server* s, int drain);
 
 /* Kernel options of a batch (set while no member is queued or active); see
- * ela_set_option. */
+ * ela_set_option.  ELA_OPT_MARCH, ELA_OPT_KZ, ELA_OPT_XCD and ELA_OPT_GRAPH:
+ * 1 captures a step's F-cycle (every launch on every level) into a HIP graph
+ * and replays it while the active set is unchanged; a change of the set (a
+ * member admitted or retired) re-captures.  Cycles with a BiCGStab coarsest
+ * level (their host reads the members' states) and timed cycles run on the
+ * stream.  Same kernels in the same order: the same bits.  Default 0. */
 int ela_batch_set_option(ela_batch* b, int option, int value);
+/* Captures and replays of the batch's F-cycle graph since its creation (no
+ * reference counterpart). */
+int ela_batch_graph_stats(const ela_batch* b, int* captures, int* replays);
 
 /* The autotuner of ela_autotune over the finest levels of the given prepared
  * members (a representative set of the batch's sizes); keeps the fastest

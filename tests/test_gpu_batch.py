@@ -330,6 +330,43 @@ def test_marching_kernels_match_the_node_kernels(mode, bcs):
         assert np.abs(a - c).max() <= 1e-13 * np.abs(a).max()
 
 
+def test_graph_replayed_batch_cycles_keep_the_bits():
+    """OPT_GRAPH captures each step's F-cycle into a HIP graph and replays it while the active set stays
+    the same; members retiring re-capture.  Two solves in a row (eager level-0 updates and residuals and a
+    host synchronization between replays, captures and replays both exercised) give the stream launches'
+    bits in both precisions; a batch with a BiCGStab coarsest member runs its cycles on the stream."""
+    probs = [problem(81, 16, 8), problem(82, 24, 9, crack=True), problem(83, 32, 11), problem(84, 20, 5)]
+    for mode in (E.SOLVE_MIXED, E.SOLVE_FP64):
+        out = {}
+        for graph in (0, 1):
+            b = E.Batch(mode)
+            b.set_option(E.OPT_GRAPH, graph)
+            rs = [make(b, p) for p in probs]
+            b.prepare(rs)
+            sols = []
+            for rep in range(2):
+                for r in rs:
+                    r.upload(E.RVE.U, np.zeros((3,) + (r.n + 1,) * 3))
+                sts = b.solve(rs, tol_rel=1e-10, max_cycles=80)
+                assert all(st["converged"] for st in sts)
+                sols.append([r.download(E.RVE.U) for r in rs] + [[st["iters"] for st in sts]])
+            out[graph] = sols
+            if graph:
+                caps, reps = b.graph_stats()
+                assert caps >= 2 and reps > caps, (caps, reps)
+            b.close()
+        for a, c in zip(out[0], out[1]):
+            for x, y in zip(a, c):
+                assert np.array_equal(np.asarray(x), np.asarray(y))
+    b = E.Batch(E.SOLVE_MIXED)
+    b.set_option(E.OPT_GRAPH, 1)
+    rs = [make(b, problem(85, 36, 6)), make(b, problem(86, 16, 5))]
+    b.prepare(rs)
+    assert all(st["converged"] for st in b.solve(rs, tol_rel=1e-10, max_cycles=80))
+    assert b.graph_stats() == (0, 0)
+    b.close()
+
+
 def test_batch_autotune_keeps_the_bits():
     probs = [problem(71, 32, 10), problem(72, 24, 7, crack=True)]
     sols = []
