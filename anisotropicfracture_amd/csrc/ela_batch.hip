@@ -376,13 +376,23 @@ __global__ __launch_bounds__(TB) void kb_bottom(const BoxDesc* __restrict__ D, i
 // The coarsest levels beyond the dense inverse solve A x = b by BiCGStab (the
 // single-box solver's bottom, the reference's bottom solver: Linear.H:23-29
 // with MLMG's bottom tolerance 1e-4): vectors in the cycle precision, dot
-// products and norms in FP64.  Launches go over the list of such boxes; each
-// block writes three partials (two sums and a maximum), and one 64-lane block
-// per member folds its own blocks' partials in a fixed order and runs the
-// scalar steps on its lane 0.  A member's state (BS_STATE) is 1 while it
-// iterates, 2 when it ends after the x += alpha p update, 0 when done; the
-// vector kernels skip members whose state says so, so a member stops where
-// the single-box loop breaks and stays frozen while others go on.
+// products and norms in FP64.  Launches go over the list of such boxes, five
+// per iteration.  Each block writes three partials (two sums and a maximum);
+// the next launch that needs the scalar they define folds them again in
+// every block of the member (its own blocks' partials, in a fixed order, so
+// every block gets the same bits) and applies the scalar step.  The scalars
+// (BS_*) are double-buffered: a launch reads set ph and the member's first
+// block writes set ph ^ 1, so no block reads what another block of the same
+// launch writes; the two partial buffers alternate the same way.  A member's
+// state is 1 while it iterates, 2 when it ends after x += alpha p, 0 when
+// done; the vector work skips members whose state says so, so a member stops
+// where the single-box loop breaks and stays frozen while others go on.
+
+__device__ __forceinline__ double amax(double m, double v) {
+    const double a = fabs(v);
+    return (a > m || a != a) ? a : m;
+}
+__device__ __forceinline__ double nmax(double a, double b) { return (b > a || b != b) ? b : a; }
 
 // Sums of a and b and maximum of c over the block into partials[3 blk + 0..2].
 __device__ __forceinline__ void block_sum2_max(double* partials, int blk, double a, double b, double c) {
@@ -390,8 +400,7 @@ __device__ __forceinline__ void block_sum2_max(double* partials, int blk, double
     for (int off = 32; off > 0; off >>= 1) {
         a += __shfl_xor(a, off, 64);
         b += __shfl_xor(b, off, 64);
-        const double o = __shfl_xor(c, off, 64);
-        c = (o > c || o != o) ? o : c;
+        c = nmax(c, __shfl_xor(c, off, 64));
     }
     if ((threadIdx.x & 63) == 0) {
         wred[0][threadIdx.x >> 6] = a;
@@ -404,7 +413,7 @@ __device__ __forceinline__ void block_sum2_max(double* partials, int blk, double
         for (int w = 1; w < TB / 64; ++w) {
             sa += wred[0][w];
             sb += wred[1][w];
-            m = (wred[2][w] > m || wred[2][w] != wred[2][w]) ? wred[2][w] : m;
+            m = nmax(m, wred[2][w]);
         }
         partials[3 * blk] = sa;
         partials[3 * blk + 1] = sb;
@@ -412,15 +421,62 @@ __device__ __forceinline__ void block_sum2_max(double* partials, int blk, double
     }
 }
 
-__device__ __forceinline__ double amax(double m, double v) {
-    const double a = fabs(v);
-    return (a > m || a != a) ? a : m;
+// Fold of a member's partials (sums of slots 0, 1, max of slot 2) into q on
+// every thread of the block; the same order in every block.
+__device__ __forceinline__ void fold_member(const BoxDesc& d, const double* __restrict__ P, double q[3]) {
+    __shared__ double fr[3][TB / 64];
+    const int nt = (d.nodes + TB - 1) / TB;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int t = threadIdx.x; t < nt; t += TB) {
+        const double* pp = P + 3LL * (d.tile0 + t);
+        a += pp[0];
+        b += pp[1];
+        c = nmax(c, pp[2]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+        c = nmax(c, __shfl_xor(c, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        fr[0][threadIdx.x >> 6] = a;
+        fr[1][threadIdx.x >> 6] = b;
+        fr[2][threadIdx.x >> 6] = c;
+    }
+    __syncthreads();
+    q[0] = fr[0][0];
+    q[1] = fr[1][0];
+    q[2] = fr[2][0];
+    for (int w = 1; w < TB / 64; ++w) {
+        q[0] += fr[0][w];
+        q[1] += fr[1][w];
+        q[2] = nmax(q[2], fr[2][w]);
+    }
+    __syncthreads();  // fr is reused by a later fold of the same kernel
 }
 
-// x = f[dst] = 0, p = v = 0, r = rh = f[src]; partials rh.r and |r|inf.
+// The member's scalars of set ph (every thread holds a copy), and the write of
+// set ph ^ 1 by the member's first block (state also to state[box]).
+struct BScal {
+    double v[BS_N];
+};
+__device__ __forceinline__ BScal scal_load(const BoxDesc& d, int ph) {
+    BScal S;
+#pragma unroll
+    for (int k = 0; k < BS_N; ++k) S.v[k] = d.bs[ph * BS_N + k];
+    return S;
+}
+__device__ __forceinline__ void scal_store(const BoxDesc& d, int blk, int ph, const BScal& S, int* state, int box) {
+    if (blk != d.tile0 || threadIdx.x != 0) return;
+#pragma unroll
+    for (int k = 0; k < BS_N; ++k) d.bs[(ph ^ 1) * BS_N + k] = S.v[k];
+    if (state) state[box] = (int)S.v[BS_STATE];
+}
+
+// x = f[dst] = 0, p = v = 0, r = rh = f[src]; partials rh.r and |r|inf into P.
 template <typename T>
 __global__ __launch_bounds__(TB) void kbb_init(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, int src,
-                                               int dst, double* __restrict__ partials) {
+                                               int dst, double* __restrict__ P) {
     const BoxDesc& d = D[bmap[blockIdx.x]];
     const int q = ((int)blockIdx.x - d.tile0) * TB + (int)threadIdx.x;
     double s0 = 0.0, m = 0.0;
@@ -438,108 +494,208 @@ __global__ __launch_bounds__(TB) void kbb_init(const BoxDesc* __restrict__ D, co
             m = amax(m, (double)bv);
         }
     }
-    block_sum2_max(partials, blockIdx.x, s0, 0.0, m);
+    block_sum2_max(P, blockIdx.x, s0, 0.0, m);
 }
 
-// p = r + beta (p - omega v)
+// Iteration start.  first: rho = rh.r and the tolerance from |b|inf (a zero
+// right-hand side gives x = 0); else rho = rh.r of the last update, stop on
+// |r| small, the iteration limit or a breakdown of rho, beta.  Then
+// p = r + beta (p - omega v).
 template <typename T>
-__global__ __launch_bounds__(TB) void kbb_p(const BoxDesc* __restrict__ D, const int* __restrict__ bmap) {
-    const BoxDesc& d = D[bmap[blockIdx.x]];
-    if (d.bs[BS_STATE] != 1.0) return;
-    const int q = ((int)blockIdx.x - d.tile0) * TB + (int)threadIdx.x;
-    if (q >= d.nodes) return;
-    const T beta = (T)d.bs[BS_BETA], omega = (T)d.bs[BS_OMEGA];
+__global__ __launch_bounds__(TB) void kbb_p(const BoxDesc* __restrict__ D, const int* __restrict__ bmap,
+                                            const double* __restrict__ P, int ph, int first, int* state) {
+    const int box = bmap[blockIdx.x];
+    const BoxDesc& d = D[box];
+    BScal S = scal_load(d, ph);
+    if (!first && S.v[BS_STATE] != 1.0) {
+        scal_store(d, blockIdx.x, ph, S, state, box);
+        return;
+    }
+    double q[3];
+    fold_member(d, P, q);
+    if (first) {
+        S.v[BS_RHO] = q[0];
+        S.v[BS_RHO1] = S.v[BS_ALPHA] = S.v[BS_OMEGA] = 1.0;
+        S.v[BS_BETA] = 0.0;
+        S.v[BS_IT] = 0.0;
+        S.v[BS_TOL] = kBottomTol * q[2];
+        S.v[BS_STATE] = (q[2] > 0.0 && q[0] != 0.0 && isfinite(q[0])) ? 1.0 : 0.0;
+    } else {
+        S.v[BS_RHO1] = S.v[BS_RHO];
+        S.v[BS_RHO] = q[0];
+        S.v[BS_IT] += 1.0;
+        if (q[2] <= S.v[BS_TOL] || !isfinite(q[2]) || S.v[BS_IT] >= (double)kBottomMaxIter || !(q[0] != 0.0) ||
+            !isfinite(q[0]))
+            S.v[BS_STATE] = 0.0;
+        else
+            S.v[BS_BETA] = (S.v[BS_RHO] / S.v[BS_RHO1]) * (S.v[BS_ALPHA] / S.v[BS_OMEGA]);
+    }
+    scal_store(d, blockIdx.x, ph, S, state, box);
+    if (S.v[BS_STATE] != 1.0) return;
+    const int qn = ((int)blockIdx.x - d.tile0) * TB + (int)threadIdx.x;
+    if (qn >= d.nodes) return;
+    const T beta = (T)S.v[BS_BETA], omega = (T)S.v[BS_OMEGA];
     const T *r = (const T*)d.bv[BV_R], *v = (const T*)d.bv[BV_V];
     T* p = (T*)d.bv[BV_P];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const long long e = c * d.L.cs + q;
+        const long long e = c * d.L.cs + qn;
         p[e] = r[e] + beta * (p[e] - omega * v[e]);
     }
 }
 
-// which 0: v = A p, partial rh.v; which 1: t = A s, partials t.t and t.s.
+// v = A p, partial rh.v into Pout.
 template <typename T>
-__global__ __launch_bounds__(TB) void kbb_apply(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, int which,
-                                                double* __restrict__ partials) {
-    const BoxDesc& d = D[bmap[blockIdx.x]];
-    if (d.bs[BS_STATE] != 1.0) return;  // uniform over the block
+__global__ __launch_bounds__(TB) void kbb_av(const BoxDesc* __restrict__ D, const int* __restrict__ bmap,
+                                             double* __restrict__ Pout, int ph) {
+    const int box = bmap[blockIdx.x];
+    const BoxDesc& d = D[box];
+    const BScal S = scal_load(d, ph);
+    scal_store(d, blockIdx.x, ph, S, nullptr, box);
+    if (S.v[BS_STATE] != 1.0) return;  // uniform over the block
     const Lay L = d.L;
     int i, j, k;
     long long o;
-    double s0 = 0.0, s1 = 0.0;
+    double s0 = 0.0;
     if (box_node(d, L, blockIdx.x, i, j, k, o)) {
-        const T* in = (const T*)d.bv[which ? BV_S : BV_P];
-        T* out = (T*)d.bv[which ? BV_T : BV_V];
-        const T* w = (const T*)d.bv[which ? BV_T : BV_RH];
+        const T* in = (const T*)d.bv[BV_P];
+        const T* rh = (const T*)d.bv[BV_RH];
+        T* out = (T*)d.bv[BV_V];
         T f[3];
         box_apply<T, -1>(d, L, in, i, j, k, f);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const long long e = c * L.cs + o;
             out[e] = f[c];
-            if (which) {
-                s0 += (double)f[c] * (double)f[c];
-                s1 += (double)f[c] * (double)in[e];
-            } else {
-                s0 += (double)w[e] * (double)f[c];
-            }
+            s0 += (double)rh[e] * (double)f[c];
         }
     }
-    block_sum2_max(partials, blockIdx.x, s0, s1, 0.0);
+    block_sum2_max(Pout, blockIdx.x, s0, 0.0, 0.0);
 }
 
-// s = r - alpha v; partial |s|inf.
+// alpha = rho / rh.v (a breakdown ends the member without an update);
+// s = r - alpha v, partial |s|inf into Pout.
 template <typename T>
 __global__ __launch_bounds__(TB) void kbb_s(const BoxDesc* __restrict__ D, const int* __restrict__ bmap,
-                                            double* __restrict__ partials) {
-    const BoxDesc& d = D[bmap[blockIdx.x]];
-    if (d.bs[BS_STATE] != 1.0) return;
-    const int q = ((int)blockIdx.x - d.tile0) * TB + (int)threadIdx.x;
+                                            const double* __restrict__ Pin, double* __restrict__ Pout, int ph) {
+    const int box = bmap[blockIdx.x];
+    const BoxDesc& d = D[box];
+    BScal S = scal_load(d, ph);
+    if (S.v[BS_STATE] != 1.0) {
+        scal_store(d, blockIdx.x, ph, S, nullptr, box);
+        return;
+    }
+    double q[3];
+    fold_member(d, Pin, q);
+    if (!(q[0] != 0.0) || !isfinite(q[0])) S.v[BS_STATE] = 0.0;
+    else S.v[BS_ALPHA] = S.v[BS_RHO] / q[0];
+    scal_store(d, blockIdx.x, ph, S, nullptr, box);
+    if (S.v[BS_STATE] != 1.0) return;
+    const int qn = ((int)blockIdx.x - d.tile0) * TB + (int)threadIdx.x;
     double m = 0.0;
-    if (q < d.nodes) {
-        const T alpha = (T)d.bs[BS_ALPHA];
+    if (qn < d.nodes) {
+        const T alpha = (T)S.v[BS_ALPHA];
         const T *r = (const T*)d.bv[BV_R], *v = (const T*)d.bv[BV_V];
         T* sv = (T*)d.bv[BV_S];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const long long e = c * d.L.cs + q;
+            const long long e = c * d.L.cs + qn;
             const T x = r[e] - alpha * v[e];
             sv[e] = x;
             m = amax(m, (double)x);
         }
     }
-    block_sum2_max(partials, blockIdx.x, 0.0, 0.0, m);
+    block_sum2_max(Pout, blockIdx.x, 0.0, 0.0, m);
 }
 
-// State 1: x += alpha p + omega s, r = s - omega t, partials rh.r and |r|inf.
-// State 2: x += alpha p.
+// |s| small: the member ends after x += alpha p (state 2); else t = A s,
+// partials t.t and t.s into Pout.
+template <typename T>
+__global__ __launch_bounds__(TB) void kbb_at(const BoxDesc* __restrict__ D, const int* __restrict__ bmap,
+                                             const double* __restrict__ Pin, double* __restrict__ Pout, int ph) {
+    const int box = bmap[blockIdx.x];
+    const BoxDesc& d = D[box];
+    BScal S = scal_load(d, ph);
+    if (S.v[BS_STATE] != 1.0) {
+        scal_store(d, blockIdx.x, ph, S, nullptr, box);
+        return;
+    }
+    double q[3];
+    fold_member(d, Pin, q);
+    if (q[2] <= S.v[BS_TOL]) {
+        S.v[BS_STATE] = 2.0;
+        S.v[BS_IT] += 1.0;
+    }
+    scal_store(d, blockIdx.x, ph, S, nullptr, box);
+    if (S.v[BS_STATE] != 1.0) return;
+    const Lay L = d.L;
+    int i, j, k;
+    long long o;
+    double s0 = 0.0, s1 = 0.0;
+    if (box_node(d, L, blockIdx.x, i, j, k, o)) {
+        const T* in = (const T*)d.bv[BV_S];
+        T* out = (T*)d.bv[BV_T];
+        T f[3];
+        box_apply<T, -1>(d, L, in, i, j, k, f);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const long long e = c * L.cs + o;
+            out[e] = f[c];
+            s0 += (double)f[c] * (double)f[c];
+            s1 += (double)f[c] * (double)in[e];
+        }
+    }
+    block_sum2_max(Pout, blockIdx.x, s0, s1, 0.0);
+}
+
+// State 2: x += alpha p, done.  State 1: omega = t.s / t.t (a breakdown: x +=
+// alpha p, done), x += alpha p + omega s, r = s - omega t, partials rh.r and
+// |r|inf into Pout.
 template <typename T>
 __global__ __launch_bounds__(TB) void kbb_x(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, int dst,
-                                            double* __restrict__ partials) {
-    const BoxDesc& d = D[bmap[blockIdx.x]];
-    const double st = d.bs[BS_STATE];
-    if (st != 1.0 && st != 2.0) return;
-    const int q = ((int)blockIdx.x - d.tile0) * TB + (int)threadIdx.x;
-    const T alpha = (T)d.bs[BS_ALPHA], omega = (T)d.bs[BS_OMEGA];
+                                            const double* __restrict__ Pin, double* __restrict__ Pout, int ph) {
+    const int box = bmap[blockIdx.x];
+    const BoxDesc& d = D[box];
+    BScal S = scal_load(d, ph);
+    const double st = S.v[BS_STATE];
+    if (st != 1.0 && st != 2.0) {
+        scal_store(d, blockIdx.x, ph, S, nullptr, box);
+        return;
+    }
+    bool only_p = st == 2.0;
+    if (!only_p) {
+        double q[3];
+        fold_member(d, Pin, q);
+        const double om = q[0] > 0.0 ? q[1] / q[0] : 0.0;
+        if (!(om != 0.0) || !isfinite(om)) {
+            only_p = true;
+            S.v[BS_IT] += 1.0;
+        } else {
+            S.v[BS_OMEGA] = om;
+        }
+    }
+    if (only_p) S.v[BS_STATE] = 0.0;
+    scal_store(d, blockIdx.x, ph, S, nullptr, box);
+    const int qn = ((int)blockIdx.x - d.tile0) * TB + (int)threadIdx.x;
+    const T alpha = (T)S.v[BS_ALPHA], omega = (T)S.v[BS_OMEGA];
     T* x = (T*)d.f[dst];
     const T* p = (const T*)d.bv[BV_P];
-    if (st == 2.0) {
-        if (q < d.nodes)
+    if (only_p) {
+        if (qn < d.nodes)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const long long e = c * d.L.cs + q;
+                const long long e = c * d.L.cs + qn;
                 x[e] += alpha * p[e];
             }
         return;
     }
     double s0 = 0.0, m = 0.0;
-    if (q < d.nodes) {
+    if (qn < d.nodes) {
         const T *sv = (const T*)d.bv[BV_S], *t = (const T*)d.bv[BV_T], *rh = (const T*)d.bv[BV_RH];
         T* r = (T*)d.bv[BV_R];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const long long e = c * d.L.cs + q;
+            const long long e = c * d.L.cs + qn;
             x[e] += alpha * p[e] + omega * sv[e];
             const T rr = sv[e] - omega * t[e];
             r[e] = rr;
@@ -547,83 +703,7 @@ __global__ __launch_bounds__(TB) void kbb_x(const BoxDesc* __restrict__ D, const
             m = amax(m, (double)rr);
         }
     }
-    block_sum2_max(partials, blockIdx.x, s0, 0.0, m);
-}
-
-enum { BP_INIT = 0, BP_V, BP_S, BP_T, BP_R };
-
-// One block of 64 lanes per member: fold its partials (fixed order), then the
-// scalar step of phase `phase` on lane 0; state[box] = the member's state.
-__global__ __launch_bounds__(64) void kbb_fold(const BoxDesc* __restrict__ D, const double* __restrict__ partials,
-                                               int phase, int* __restrict__ state) {
-    const BoxDesc& d = D[blockIdx.x];
-    double* S = d.bs;
-    const double st = S[BS_STATE];
-    const bool skip = phase != BP_INIT && st != 1.0 && !(phase == BP_R && st == 2.0);
-    double q0 = 0.0, q1 = 0.0, q2 = 0.0;
-    if (!skip) {
-        const int nt = (d.nodes + TB - 1) / TB;
-        for (int t = threadIdx.x; t < nt; t += 64) {
-            const double* pp = partials + 3LL * (d.tile0 + t);
-            q0 += pp[0];
-            q1 += pp[1];
-            q2 = (pp[2] > q2 || pp[2] != pp[2]) ? pp[2] : q2;
-        }
-        for (int off = 32; off > 0; off >>= 1) {
-            q0 += __shfl_xor(q0, off, 64);
-            q1 += __shfl_xor(q1, off, 64);
-            const double o = __shfl_xor(q2, off, 64);
-            q2 = (o > q2 || o != o) ? o : q2;
-        }
-    }
-    if (threadIdx.x != 0) return;
-    double ns = skip ? st : 1.0;
-    if (!skip) switch (phase) {
-            case BP_INIT:  // rho = rh.r, tol from |b|inf; a zero right-hand side gives x = 0
-                S[BS_RHO] = q0;
-                S[BS_RHO1] = S[BS_ALPHA] = S[BS_OMEGA] = 1.0;
-                S[BS_BETA] = 0.0;
-                S[BS_IT] = 0.0;
-                S[BS_TOL] = kBottomTol * q2;
-                ns = (q2 > 0.0 && q0 != 0.0 && isfinite(q0)) ? 1.0 : 0.0;
-                break;
-            case BP_V:  // alpha = rho / rh.v
-                if (!(q0 != 0.0) || !isfinite(q0)) ns = 0.0;
-                else S[BS_ALPHA] = S[BS_RHO] / q0;
-                break;
-            case BP_S:  // |s| small: x += alpha p and stop
-                if (q2 <= S[BS_TOL]) {
-                    ns = 2.0;
-                    S[BS_IT] += 1.0;
-                }
-                break;
-            case BP_T: {  // omega = t.s / t.t
-                const double om = q0 > 0.0 ? q1 / q0 : 0.0;
-                if (!(om != 0.0) || !isfinite(om)) {
-                    ns = 2.0;
-                    S[BS_IT] += 1.0;
-                } else {
-                    S[BS_OMEGA] = om;
-                }
-                break;
-            }
-            case BP_R:  // rho = rh.r; stop on |r| small, the iteration limit or a breakdown of rho
-                if (st == 2.0) {
-                    ns = 0.0;
-                    break;
-                }
-                S[BS_RHO1] = S[BS_RHO];
-                S[BS_RHO] = q0;
-                S[BS_IT] += 1.0;
-                if (q2 <= S[BS_TOL] || !isfinite(q2) || S[BS_IT] >= (double)kBottomMaxIter ||
-                    !(q0 != 0.0) || !isfinite(q0))
-                    ns = 0.0;
-                else
-                    S[BS_BETA] = (S[BS_RHO] / S[BS_RHO1]) * (S[BS_ALPHA] / S[BS_OMEGA]);
-                break;
-        }
-    S[BS_STATE] = ns;
-    state[blockIdx.x] = (int)ns;
+    block_sum2_max(Pout, blockIdx.x, s0, 0.0, m);
 }
 
 // f[dst] = A f[src] (cycle precision), and the FP64 finest-level r64 = A u64.
@@ -1144,7 +1224,7 @@ size_t layout_member(ela_rve* r, char* base) {
     if (r->bicg) {
         const size_t N = (size_t)r->lev[r->depth - 1].nodes;
         for (int q = 0; q < BV_N; ++q) r->bv[q] = take(3 * N * ts);
-        r->bs = (double*)take(BS_N * 8);
+        r->bs = (double*)take(2 * BS_N * 8);  // two sets (double-buffered)
     }
     return off;
 }
@@ -1311,7 +1391,7 @@ int build_lists(ela_batch* B) {
     if (!B->hlists.empty())
         ELA_HIPCHK(hipMemcpyAsync(B->dlists, B->hlists.data(), B->hlists.size(), hipMemcpyHostToDevice, B->stream));
     const size_t nblk_top = std::max({(size_t)B->top_tiles.back(), (size_t)(B->mtop.nm + B->mtop.ns),
-                                      3 * (size_t)B->btiles.back()});
+                                      6 * (size_t)B->btiles.back()});
     if (int rc = ensure_device(B, (void**)&B->partials, &B->partials_cap, nblk_top * sizeof(double))) return rc;
     const size_t nbc = B->btiles.size() - 1;
     if (nbc && B->bstate_cap < nbc) {
@@ -1451,20 +1531,23 @@ const BoxDesc* BD(const ela_batch* B) { return (const BoxDesc*)(B->dlists + B->o
 const int* BM(const ela_batch* B) { return (const int*)(B->dlists + B->off_bbmap); }
 
 // BiCGStab on the first nbc boxes of the BiCGStab list: f[dst] = A^-1 f[src]
-// to 1e-4.  Every kBottomCheck iterations the host reads the states and stops
-// once no member iterates; launches past a member's end leave it unchanged.
+// to 1e-4, five launches an iteration.  Every kBottomCheck iterations the host
+// reads the states the iteration's first launch wrote and stops once no member
+// iterates; launches past a member's end leave it unchanged.
 template <typename T>
 void bicg_bottom(ela_batch* B, int nbc, int src, int dst) {
     const int nb = B->btiles[nbc];
     const BoxDesc* D = BD(B);
     const int* bm = BM(B);
     hipStream_t st = B->stream;
-    auto fold = [&](int phase) {
-        hipLaunchKernelGGL(kbb_fold, dim3(nbc), dim3(64), 0, st, D, B->partials, phase, B->dbstate);
-    };
-    hipLaunchKernelGGL((kbb_init<T>), dim3(nb), dim3(TB), 0, st, D, bm, src, dst, B->partials);
-    fold(BP_INIT);
+    double* Pa = B->partials;
+    double* Pb = B->partials + 3LL * B->btiles.back();
+    int ph = 0;
+    hipLaunchKernelGGL((kbb_init<T>), dim3(nb), dim3(TB), 0, st, D, bm, src, dst, Pa);
     for (int it = 0; it < kBottomMaxIter; ++it) {
+        hipLaunchKernelGGL((kbb_p<T>), dim3(nb), dim3(TB), 0, st, D, bm, (const double*)Pa, ph, it == 0 ? 1 : 0,
+                           B->dbstate);
+        ph ^= 1;
         if (it % kBottomCheck == 0) {
             (void)hipMemcpyAsync(B->hbstate, B->dbstate, nbc * sizeof(int), hipMemcpyDeviceToHost, st);
             (void)hipStreamSynchronize(st);
@@ -1472,15 +1555,14 @@ void bicg_bottom(ela_batch* B, int nbc, int src, int dst) {
             for (int m = 0; m < nbc && !any; ++m) any = B->hbstate[m] != 0;
             if (!any) break;
         }
-        hipLaunchKernelGGL((kbb_p<T>), dim3(nb), dim3(TB), 0, st, D, bm);
-        hipLaunchKernelGGL((kbb_apply<T>), dim3(nb), dim3(TB), 0, st, D, bm, 0, B->partials);
-        fold(BP_V);
-        hipLaunchKernelGGL((kbb_s<T>), dim3(nb), dim3(TB), 0, st, D, bm, B->partials);
-        fold(BP_S);
-        hipLaunchKernelGGL((kbb_apply<T>), dim3(nb), dim3(TB), 0, st, D, bm, 1, B->partials);
-        fold(BP_T);
-        hipLaunchKernelGGL((kbb_x<T>), dim3(nb), dim3(TB), 0, st, D, bm, dst, B->partials);
-        fold(BP_R);
+        hipLaunchKernelGGL((kbb_av<T>), dim3(nb), dim3(TB), 0, st, D, bm, Pb, ph);
+        ph ^= 1;
+        hipLaunchKernelGGL((kbb_s<T>), dim3(nb), dim3(TB), 0, st, D, bm, (const double*)Pb, Pa, ph);
+        ph ^= 1;
+        hipLaunchKernelGGL((kbb_at<T>), dim3(nb), dim3(TB), 0, st, D, bm, (const double*)Pa, Pb, ph);
+        ph ^= 1;
+        hipLaunchKernelGGL((kbb_x<T>), dim3(nb), dim3(TB), 0, st, D, bm, dst, (const double*)Pb, Pa, ph);
+        ph ^= 1;
     }
 }
 

@@ -192,9 +192,12 @@ def cpu_and_checks(n_full, n_small, plain_cycles, cases):
 def batched_rves(count, sample, tol):
     """Batched RVEs: `count` independent RVEs (sides 32/48/64/96, 8-32 grains, fixed-corner uniaxial z
     strain 0.01).  Batched mixed and batched FP64 solve every RVE, one launch per cycle kernel over all
-    of them; the plain FP64 GPU solver (ela_solve on one hierarchy per size, one RVE at a time, the same
-    MG iteration) solves the first `sample` RVEs.  Timed per RVE: stiffness evaluation on the device,
-    set-up (coarse stiffness, diagonals, coarsest inverse) and the solve."""
+    of them (mixed also with each step's F-cycle replayed from a captured HIP graph); the plain FP64 GPU
+    solver (ela_solve on one hierarchy per size, per-node stiffness planes, one RVE at a time: the batch's
+    MG iteration, same coarse operators) solves `sample` RVEs, the same number of each side, and its
+    rate for the whole set is the per-side mean time weighted by the set's side counts.  Timed per RVE:
+    stiffness evaluation on the device, set-up (coarse stiffness, diagonals, coarsest inverse) and the
+    solve."""
     import anisotropicfracture_amd as ela
     from anisotropicfracture_amd import configs as K
     cfg = K.rve_batch(count)
@@ -206,15 +209,17 @@ def batched_rves(count, sample, tol):
             r.SetModelVoronoiCubic(m["seeds"], K.VORONOI_Q, m["R"].reshape(-1, 9), K.C11, K.C12, K.C44)
             r.SetRhsAffine(m["G"])
 
-    for mode, key in ((ela.SOLVE_MIXED, "mixed"), (ela.SOLVE_FP64, "fp64")):
+    for mode, key, graph in ((ela.SOLVE_MIXED, "mixed", 0), (ela.SOLVE_MIXED, "mixed_graph", 1),
+                             (ela.SOLVE_FP64, "fp64", 0)):
         b = ela.Batch(mode)
+        b.set_option(ela.OPT_GRAPH, graph)
         # untimed warm-up batch: first launches of every kernel, and the marching tile shape picked on
         # a launch large enough to fill the GPU
         nw = min(64, count)
         warm = [b.rve(m["n"], max_grains=32) for m in cfg[:nw]]
         load(b, warm, cfg[:nw])
         b.prepare(warm)
-        tuned = {"%d/%d" % k: v for k, v in b.autotune(warm, nrep=3).items()}
+        tuned = {"%d/%d" % k: v for k, v in b.autotune(warm, nrep=3).items()} if not graph else None
         b.solve(warm, tol_rel=tol, max_cycles=100)
         for r in warm:
             r.close()
@@ -231,6 +236,8 @@ def batched_rves(count, sample, tol):
                "converged": int(sum(q["converged"] for q in sts)),
                "max_rel_residual": float(max(q["res"] / q["res0"] for q in sts)), "device_GB": b.bytes() / 1e9,
                "autotune_ms_per_sweep": tuned}
+        if graph:
+            ent["graph_captures_replays"] = b.graph_stats()
         if key == "mixed":
             sm = b.smoother_times()
             ent["smoother"] = {"ms": sm["ms"], "launches": sm["launches"], "node_sweeps": sm["nodes"],
@@ -238,9 +245,13 @@ def batched_rves(count, sample, tol):
         out["batched_" + key] = ent
         log("bench: batched %s %.1f solves/s (%.2f s, cycles %.1f)" % (key, count / t, t, it.mean()))
         b.close()
-    # plain FP64 GPU solver, one RVE at a time (hierarchies reused per size)
-    ops, tot, its = {}, 0.0, []
-    for m in cfg[:sample]:
+    # plain FP64 GPU solver, one RVE at a time (hierarchies reused per size), the same number of each side
+    sides = sorted(set(m["n"] for m in cfg))
+    per = max(1, sample // len(sides))
+    pick = [i for s_ in sides for i in [j for j, m in enumerate(cfg) if m["n"] == s_][:per]]
+    ops, tot, its, tside = {}, 0.0, [], {s_: [] for s_ in sides}
+    for i in pick:
+        m = cfg[i]
         n = m["n"]
         if n not in ops:
             op = ela.Elastic((n, n, n), ela.MM, mode=ela.SOLVE_FP64, coarse_min=2)
@@ -257,19 +268,27 @@ def batched_rves(count, sample, tol):
         uf.zero()
         t0 = time.perf_counter()
         op.SetModelVoronoiCubic(m["seeds"], K.VORONOI_Q, m["R"].reshape(-1, 9), K.C11, K.C12, K.C44)
-        op.SetStiffnessMode(1)
+        op.SetStiffnessMode(0)
         op.prepareForSolve()
         rc, st = op.solve_device(bf, uf, tol, 0.0, ela.SOLVE_FP64, 200)
-        tot += time.perf_counter() - t0
+        dt = time.perf_counter() - t0
+        tot += dt
+        tside[n].append(dt)
         its.append(st["iters"])
     for op, _, _ in ops.values():
         op.close()
-    out["plain_fp64"] = {"solves_per_s": sample / tot, "sample": sample, "cycles_mean": float(np.mean(its)),
-                         "cycles_same_as_batched_fp64": bool(np.array_equal(its, cycles["fp64"][:sample]))}
+    est = sum(sum(1 for m in cfg if m["n"] == s_) * float(np.mean(tside[s_])) for s_ in sides)
+    same = [a == cycles["fp64"][i] for a, i in zip(its, pick)]
+    out["plain_fp64"] = {"solves_per_s": count / est, "sample": len(pick), "per_side": per,
+                         "s_per_solve_by_side": {str(s_): float(np.mean(tside[s_])) for s_ in sides},
+                         "estimate": "sum over sides of (RVEs of that side in the set) x (mean sampled time)",
+                         "cycles_mean": float(np.mean(its)), "stiffness": "per-node planes (full-weighting coarse levels)",
+                         "cycles_same_as_batched_fp64": bool(all(same)),
+                         "cycles_equal_fraction": float(np.mean(same))}
     out["batched_mixed_over_plain_fp64"] = out["batched_mixed"]["solves_per_s"] / out["plain_fp64"]["solves_per_s"]
     out["batched_fp64_over_plain_fp64"] = out["batched_fp64"]["solves_per_s"] / out["plain_fp64"]["solves_per_s"]
     log("bench: plain fp64 %.1f solves/s (sample %d); batched mixed / plain fp64 = %.2f"
-        % (sample / tot, sample, out["batched_mixed_over_plain_fp64"]))
+        % (out["plain_fp64"]["solves_per_s"], len(pick), out["batched_mixed_over_plain_fp64"]))
     return out
 
 
