@@ -32,7 +32,7 @@ DISP, TRAC, PERIODIC, NEUMANN = 0, 1, 2, 3
 F64, F32 = 0, 1
 SOLVE_FP64, SOLVE_MIXED, ALLOC_BOTH = 0, 1, 2
 CYCLE_F, CYCLE_V = 0, 1
-OPT_MARCH, OPT_KZ, OPT_XCD, OPT_MASK, OPT_GRAPH, OPT_FOLD, OPT_ROWS = 0, 1, 2, 3, 4, 5, 6
+OPT_MARCH, OPT_KZ, OPT_XCD, OPT_MASK, OPT_GRAPH, OPT_FOLD, OPT_ROWS, OPT_PREFETCH = 0, 1, 2, 3, 4, 5, 6, 7
 
 ERRORS = {1: "BADARG", 2: "OOM", 3: "HIP", 4: "NONCONVERGED", 5: "NAN", 6: "STATE"}
 
@@ -458,16 +458,18 @@ class Elastic:
         1 where it pays, 2 always with node lists, 3 always with in-block regrouping) or OPT_GRAPH (the cycles'
         launches on levels >= 1 replayed from captured HIP graphs, default off) or OPT_FOLD (the last sweep of a
         Krylov step writes the FP64 vector itself, default on; the same bits either way) or OPT_ROWS (rows per
-        thread of level 0's FP32 marching tiles, 1 or 2; the same bits either way)."""
+        thread of level 0's FP32 marching tiles, 1 or 2; the same bits either way) or OPT_PREFETCH (planes a marching
+        tile loads ahead, 1 or 2; the same bits either way)."""
         _check(lib().ela_set_option(self.ptr, option, int(value)))
 
     def autotune(self, nrep=10):
-        """Pick the marching tile height / order / rows per thread by timing level-0 sweeps
-        (ela_autotune); returns {(kz, xcd, rows): ms} (rows 2 only for FP32 sweeps)."""
-        ms = np.zeros(12)
+        """Pick the marching tile height / order / rows per thread / planes loaded ahead by timing level-0
+        sweeps (ela_autotune); returns {(kz, xcd, rows, prefetch): ms} (rows 2 only for FP32 sweeps)."""
+        ms = np.zeros(24)
         _check(lib().ela_autotune(self.ptr, int(nrep), _ptr(ms)))
-        return {(kz, x, ry): ms[6 * (ry - 1) + 2 * a + x] for ry in (1, 2) for a, kz in enumerate((8, 16, 32))
-                for x in (0, 1) if ms[6 * (ry - 1) + 2 * a + x] > 0}
+        idx = lambda pf, ry, a, x: 12 * (pf - 1) + 6 * (ry - 1) + 2 * a + x
+        return {(kz, x, ry, pf): ms[idx(pf, ry, a, x)] for pf in (1, 2) for ry in (1, 2)
+                for a, kz in enumerate((8, 16, 32)) for x in (0, 1) if ms[idx(pf, ry, a, x)] > 0}
 
     def SetKrylov(self, restart):
         """Flexible GMRES(restart) around the MG cycle (0: plain MG iteration)."""

@@ -776,6 +776,8 @@ __device__ __forceinline__ GrainBox<T> grain_of(const BoxDesc& d, const MOps& op
     g.nty = d.nty;
     g.kz = d.kz;
     g.xcd = op.xcd;
+    g.ry = 1;
+    g.pf = 1;
     g.nt = false;
     g.xface = false;  // the batched shells compute the x faces
     g.in = MODE == 2 ? (const T*)d.u64 : (const T*)d.f[op.in];

@@ -83,6 +83,7 @@ struct GrainBox {
     int ngr;
     int ntx, nty, kz;
     int ry;   // rows per thread of the marching tiles (1 or 2)
+    int pf;   // planes loaded ahead by the marching tiles (1 or 2; march_tile's PF)
     int xcd;  // XCD-contiguous tile order
     bool nt;     // rhs, diagonal and Jacobi output non-temporal (see epilogue_load)
     bool xface;  // x faces in face blocks (face_patch); the shell then skips them
@@ -167,7 +168,7 @@ struct MarchLds {
 
 // Tile t of box g (256 threads); returns the thread's max |v| (MODE 2).
 // Every thread of the block calls this (it synchronizes).
-template <typename T, int MODE, typename TO, bool SC, int RY = 1>
+template <typename T, int MODE, typename TO, bool SC, int RY = 1, int PF = 1>
 __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchLds<T, SC, RY>& sh, T* stab) {
     static_assert(MODE != 2 || sizeof(T) == 8, "the solution residual is FP64");
     using TL = MTile<RY>;
@@ -214,82 +215,104 @@ __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchL
     }
     const bool scl = SC && g.sc != nullptr;  // a batched launch takes SC when any of its boxes has
                                              // a factor; the boxes without one read s = 1
-    T ru[RY][3], rh[NH], rs[RY], rsh[NH];
-    unsigned short rg[RY], rgh[NH];
-#pragma unroll
-    for (int y = 0; y < RY; ++y) {
-        rs[y] = 1;
-        rg[y] = 0;
-    }
-#pragma unroll
-    for (int h = 0; h < NH; ++h) {
-        rsh[h] = 1;
-        rgh[h] = 0;
-    }
+    // One plane's loads in registers: the tile's nodes (ru, grain ids rg,
+    // factors rs) and this thread's halo values (rh, rgh, rsh).
+    struct Regs {
+        T ru[RY][3], rh[NH], rs[RY], rsh[NH];
+        unsigned short rg[RY], rgh[NH];
+    };
     auto at = [](const void* base, unsigned bytes) { return (const char*)base + bytes; };
-    auto load = [&](int kk) {
+    auto load = [&](int kk, Regs& R) {
         const T* up = g.in + L.pl * kk;
         const unsigned short* gp = g.gid + L.pl * kk;
 #pragma unroll
         for (int y = 0; y < RY; ++y)
 #pragma unroll
             for (int p = 0; p < 3; ++p)
-                ru[y][p] = iok[y] ? *(const T*)at(up + p * L.cs, ie[y] * (unsigned)sizeof(T)) : T(0);
+                R.ru[y][p] = iok[y] ? *(const T*)at(up + p * L.cs, ie[y] * (unsigned)sizeof(T)) : T(0);
 #pragma unroll
-        for (int h = 0; h < NH; ++h) rh[h] = hok[h] ? *(const T*)at(up, he[h] * (unsigned)sizeof(T)) : T(0);
+        for (int h = 0; h < NH; ++h) R.rh[h] = hok[h] ? *(const T*)at(up, he[h] * (unsigned)sizeof(T)) : T(0);
 #pragma unroll
-        for (int y = 0; y < RY; ++y) rg[y] = iok[y] ? *(const unsigned short*)at(gp, ie[y] * 2u) : (unsigned short)0;
+        for (int y = 0; y < RY; ++y) R.rg[y] = iok[y] ? *(const unsigned short*)at(gp, ie[y] * 2u) : (unsigned short)0;
 #pragma unroll
-        for (int h = 0; h < NH; ++h) rgh[h] = gok[h] ? *(const unsigned short*)at(gp, he[h] * 2u) : (unsigned short)0;
+        for (int h = 0; h < NH; ++h) R.rgh[h] = gok[h] ? *(const unsigned short*)at(gp, he[h] * 2u) : (unsigned short)0;
         if constexpr (SC) {
             const T* sp = g.sc + L.pl * kk;
 #pragma unroll
-            for (int y = 0; y < RY; ++y) rs[y] = iok[y] && scl ? *(const T*)at(sp, ie[y] * (unsigned)sizeof(T)) : T(1);
+            for (int y = 0; y < RY; ++y) R.rs[y] = iok[y] && scl ? *(const T*)at(sp, ie[y] * (unsigned)sizeof(T)) : T(1);
 #pragma unroll
-            for (int h = 0; h < NH; ++h) rsh[h] = gok[h] && scl ? *(const T*)at(sp, he[h] * (unsigned)sizeof(T)) : T(1);
+            for (int h = 0; h < NH; ++h) R.rsh[h] = gok[h] && scl ? *(const T*)at(sp, he[h] * (unsigned)sizeof(T)) : T(1);
+        } else {
+#pragma unroll
+            for (int y = 0; y < RY; ++y) R.rs[y] = 1;
+#pragma unroll
+            for (int h = 0; h < NH; ++h) R.rsh[h] = 1;
         }
     };
-    auto store = [&](int slot) {
+    auto store = [&](int slot, const Regs& R) {
         T* su = sh.su + slot * 3 * PL;
 #pragma unroll
         for (int y = 0; y < RY; ++y) {
 #pragma unroll
-            for (int p = 0; p < 3; ++p) su[p * PL + tb[y]] = ru[y][p];
-            sh.sg[slot * PL + tb[y]] = rg[y];
-            if constexpr (SC) sh.ss[slot * PL + tb[y]] = rs[y];
+            for (int p = 0; p < 3; ++p) su[p * PL + tb[y]] = R.ru[y][p];
+            sh.sg[slot * PL + tb[y]] = R.rg[y];
+            if constexpr (SC) sh.ss[slot * PL + tb[y]] = R.rs[y];
         }
 #pragma unroll
         for (int h = 0; h < NH; ++h) {
             const int e = tid + h * (MTX * MTY);
-            if (e < 3 * HALO) su[hpos[h]] = rh[h];
+            if (e < 3 * HALO) su[hpos[h]] = R.rh[h];
             if (e < HALO) {  // component 0: the node's grain id (and factor)
-                sh.sg[slot * PL + hpos[h]] = rgh[h];
-                if constexpr (SC) sh.ss[slot * PL + hpos[h]] = rsh[h];
+                sh.sg[slot * PL + hpos[h]] = R.rgh[h];
+                if constexpr (SC) sh.ss[slot * PL + hpos[h]] = R.rsh[h];
             }
         }
     };
     // ring slots of planes k-1, k, k+1 and the free slot s3, which receives
-    // plane k+2 while plane k is computed
+    // plane k+2 while plane k is computed.  PF 1: plane k+2 is loaded into
+    // registers (A) at the start of iteration k and the epilogue's operands
+    // of plane k there too.  PF 2: plane k+3 is loaded (B) and the epilogue's
+    // operands of plane k+1, so each load has a whole plane's computation
+    // more to arrive; A then holds plane k+2 from the previous iteration.
     int s0 = 0, s1 = 1, s2 = 2, s3 = 3;
-    load(kb - 1);
-    store(s0);
-    load(kb);
-    store(s1);
-    load(kb + 1);
-    store(s2);
+    Regs A, B;
+    load(kb - 1, A);
+    store(s0, A);
+    load(kb, A);
+    store(s1, A);
+    load(kb + 1, A);
+    store(s2, A);
+    // the epilogue's operands of the current plane (separate scalars per row:
+    // arrays indexed by the row cost the one-row kernel registers), and of the
+    // next plane (PF 2)
+    T rb0[3], db0[3], rb1[3], db1[3], nb0[3], nd0[3], nb1[3], nd1[3];
+    const long long ob0 = (long long)i + L.px * (j0 + ty * RY), ob1 = ob0 + L.px;
+    if constexpr (PF == 2) {
+        if (kb + 1 < ke) load(kb + 2, A);
+        if (act[0]) epilogue_load<T, MODE>(g, ob0 + L.pl * kb, rb0, db0);
+        if constexpr (RY == 2)
+            if (act[RY - 1]) epilogue_load<T, MODE>(g, ob1 + L.pl * kb, rb1, db1);
+    }
     __syncthreads();
     double m = 0.0;
     for (int k = kb; k < ke; ++k) {
         const bool more = k + 1 < ke;
-        // the epilogue's loads are issued before the stencils, so their latency
-        // overlaps them, as the next plane's loads do (separate scalars per
-        // row: arrays indexed by the row cost the one-row kernel registers)
-        const long long o0 = (long long)i + L.px * (j0 + ty * RY) + L.pl * k, o1 = o0 + L.px;
-        T rb0[3], db0[3], rb1[3], db1[3];
-        if (act[0]) epilogue_load<T, MODE>(g, o0, rb0, db0);
-        if constexpr (RY == 2)
-            if (act[RY - 1]) epilogue_load<T, MODE>(g, o1, rb1, db1);
-        if (more) load(k + 2);
+        const long long o0 = ob0 + L.pl * k, o1 = ob1 + L.pl * k;
+        if constexpr (PF == 1) {
+            // the epilogue's loads are issued before the stencils, so their
+            // latency overlaps them, as the next plane's loads do
+            if (act[0]) epilogue_load<T, MODE>(g, o0, rb0, db0);
+            if constexpr (RY == 2)
+                if (act[RY - 1]) epilogue_load<T, MODE>(g, o1, rb1, db1);
+            if (more) load(k + 2, A);
+        } else {
+            if (k + 2 < ke) load(k + 3, B);
+            if (more) {
+                if (act[0]) epilogue_load<T, MODE>(g, o0 + L.pl, nb0, nd0);
+                if constexpr (RY == 2)
+                    if (act[RY - 1]) epilogue_load<T, MODE>(g, o1 + L.pl, nb1, nd1);
+            }
+        }
         auto node = [&](int y, long long o, const T* rb, const T* db) {
             LdsCtx<T, SC, PL> S{sh.su, sh.sg, sh.ss, stab, {s0 * PL, s1 * PL, s2 * PL}, tb[y]};
             T f[3], xc[3], r3[3] = {rb[0], rb[1], rb[2]}, d3[3] = {db[0], db[1], db[2]};
@@ -304,13 +327,25 @@ __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchL
             if (act[RY - 1]) node(RY - 1, o1, rb1, db1);
         // s3 held plane k-2, whose last reads were in the previous plane,
         // before the previous barrier
-        if (more) store(s3);
+        if (more) store(s3, A);
         __syncthreads();  // plane k+2 is written; every read of plane k-1 is done
         const int t0 = s0;
         s0 = s1;
         s1 = s2;
         s2 = s3;
         s3 = t0;
+        if constexpr (PF == 2) {
+            A = B;
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+                rb0[p] = nb0[p];
+                db0[p] = nd0[p];
+                if constexpr (RY == 2) {
+                    rb1[p] = nb1[p];
+                    db1[p] = nd1[p];
+                }
+            }
+        }
     }
     return m;
 }

@@ -1806,10 +1806,10 @@ namespace {
 // Occupancy floor of the sweep: FP32 with one row per thread keeps 5 waves per
 // SIMD (93 VGPRs); two rows per thread need about 155 VGPRs (two stencils in
 // flight), so 3 waves (at 4 the compiler spills).
-template <typename T, int RY>
-constexpr int grain_waves() { return sizeof(T) == 4 ? (RY == 1 ? 5 : 3) : 1; }
-template <typename T, int MODE, typename TO, bool SC, int RY>
-__global__ __launch_bounds__(MTX* MTY) __attribute__((amdgpu_waves_per_eu(grain_waves<T, RY>()))) void k_grain_sweep(GrainBox<T> g, int nshp, int nfp, double* partials) {
+template <typename T, int RY, int PF>
+constexpr int grain_waves() { return sizeof(T) == 4 ? (RY == 1 ? (PF == 1 ? 5 : 4) : 3) : 1; }
+template <typename T, int MODE, typename TO, bool SC, int RY, int PF>
+__global__ __launch_bounds__(MTX* MTY) __attribute__((amdgpu_waves_per_eu(grain_waves<T, RY, PF>()))) void k_grain_sweep(GrainBox<T> g, int nshp, int nfp, double* partials) {
     __shared__ MarchLds<T, SC, RY> sh;
     extern __shared__ __align__(16) unsigned char dyn_lds[];
     const int b = (int)blockIdx.x;
@@ -1821,7 +1821,7 @@ __global__ __launch_bounds__(MTX* MTY) __attribute__((amdgpu_waves_per_eu(grain_
     } else {
         const int nm = (int)gridDim.x - nshp - nfp, bm = b - nshp - nfp;
         const int t = g.xcd ? xcd_block(bm, nm) : bm;
-        m = march_tile<T, MODE, TO, SC, RY>(g, t, sh, (T*)dyn_lds);
+        m = march_tile<T, MODE, TO, SC, RY, PF>(g, t, sh, (T*)dyn_lds);
     }
     if (MODE == 2 && partials) block_max_store(partials, m);  // a max: the slot order does not matter
 }
@@ -1834,6 +1834,12 @@ long long grain_partials(const Lay& L, int kz) {
     int ntx, nty, ntz, nsh;
     grain_tiles(L, kz, ntx, nty, ntz, nsh);
     return (long long)ntx * nty * ntz + (nsh + 7) / 8 * 8 + (grain_face_patches(L) + 7) / 8 * 8;
+}
+
+// MODE 2: the residual's max norm from the launch's block maxima.
+template <int MODE>
+void launch_grain_norm(int nblocks, double* partials, unsigned long long* norm, hipStream_t s) {
+    if (MODE == 2 && norm) hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partials, nblocks, norm);
 }
 
 template <typename T, int MODE, typename TO>
@@ -1849,16 +1855,29 @@ void launch_grain(GrainBox<T> g, int kz, double* partials, unsigned long long* n
     const size_t lds = (size_t)g.ngr * kGrainNC * sizeof(T);
     const dim3 grid(nshp + nfp + nm), block(MTX * MTY);
     constexpr int R2 = sizeof(T) == 4 ? 2 : 1;
-    if (g.sc && g.ry == 2)
-        hipLaunchKernelGGL((k_grain_sweep<T, MODE, TO, true, R2>), grid, block, lds, s, g, nshp, nfp, partials);
-    else if (g.sc)
-        hipLaunchKernelGGL((k_grain_sweep<T, MODE, TO, true, 1>), grid, block, lds, s, g, nshp, nfp, partials);
-    else if (g.ry == 2)
-        hipLaunchKernelGGL((k_grain_sweep<T, MODE, TO, false, R2>), grid, block, lds, s, g, nshp, nfp, partials);
-    else
-        hipLaunchKernelGGL((k_grain_sweep<T, MODE, TO, false, 1>), grid, block, lds, s, g, nshp, nfp, partials);
-    if (MODE == 2 && norm)
-        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partials, nm + nshp + nfp, norm);
+    auto go = [&](auto sc, auto ry, auto pf) {
+        hipLaunchKernelGGL((k_grain_sweep<T, MODE, TO, decltype(sc)::value, decltype(ry)::value, decltype(pf)::value>),
+                           grid, block, lds, s, g, nshp, nfp, partials);
+    };
+    using B0 = std::false_type;
+    using B1 = std::true_type;
+    using P1 = std::integral_constant<int, 1>;
+    using P2 = std::integral_constant<int, 2>;
+    using RR = std::integral_constant<int, R2>;
+    // two planes ahead: FP32 tiles of one row per thread (4 waves a SIMD, 104-124
+    // VGPRs); FP64 (about 200 VGPRs) and two-row tiles (spills) keep one
+    if constexpr (sizeof(T) == 4) {
+        if (g.pf == 2 && g.ry != 2) {
+            if (g.sc) go(B1{}, P1{}, P2{});
+            else go(B0{}, P1{}, P2{});
+            return launch_grain_norm<MODE>(nm + nshp + nfp, partials, norm, s);
+        }
+    }
+    if (g.sc && g.ry == 2) go(B1{}, RR{}, P1{});
+    else if (g.sc) go(B1{}, P1{}, P1{});
+    else if (g.ry == 2) go(B0{}, RR{}, P1{});
+    else go(B0{}, P1{}, P1{});
+    launch_grain_norm<MODE>(nm + nshp + nfp, partials, norm, s);
 }
 template void launch_grain<float, 0, float>(GrainBox<float>, int, double*, unsigned long long*, hipStream_t);
 template void launch_grain<double, 0, double>(GrainBox<double>, int, double*, unsigned long long*, hipStream_t);

@@ -192,6 +192,7 @@ struct ela_hier {
     int kz = 16;
     bool xcd = true;  // XCD-contiguous tile order of the marching kernels
     int ry = 1;       // rows per thread of level 0's marching tiles (FP32 sweeps)
+    int pf = 1;       // planes loaded ahead by the marching tiles (ELA_OPT_PREFETCH)
     BCTab* dbc = nullptr;
     // dw = DDW : (grad u - F0) of Newton's right-hand side (dim*dim FP64 planes, allocated on first use)
     double* newton_dw = nullptr;
@@ -474,7 +475,7 @@ std::vector<unsigned long long> cycle_key(ela_hier* h, int kind) {
     k.push_back((unsigned long long)kind);
     k.push_back(sizeof(T));
     k.push_back((unsigned long long)h->epoch);
-    k.push_back(((unsigned long long)h->ry << 40) | ((unsigned long long)h->kz << 32) |
+    k.push_back(((unsigned long long)h->pf << 48) | ((unsigned long long)h->ry << 40) | ((unsigned long long)h->kz << 32) |
                 ((unsigned long long)h->xcd << 16) | ((unsigned long long)h->mask_mode << 8) |
                 (unsigned long long)h->grain_march);
     k.push_back(((unsigned long long)h->nu1 << 32) | (unsigned long long)h->nu2);
@@ -1042,6 +1043,7 @@ GrainBox<T> grain_box(ela_hier* h, const Level& v) {
     g.ngr = h->ngrains;
     g.xcd = h->xcd;
     g.ry = &v == &h->lev[0] ? h->ry : 1;
+    g.pf = h->pf;
     g.nt = (long long)v.nodes >= kNtMinNodes;
     g.xface = true;
     return g;
@@ -2077,6 +2079,9 @@ int ela_set_option(ela_hier* h, int option, int value) {
     } else if (option == ELA_OPT_ROWS) {
         if (value != 1 && value != 2) return fail(ELA_ERR_BADARG, "rows per thread must be 1 or 2");
         h->ry = value;
+    } else if (option == ELA_OPT_PREFETCH) {
+        if (value != 1 && value != 2) return fail(ELA_ERR_BADARG, "planes loaded ahead must be 1 or 2");
+        h->pf = value;
     } else {
         return fail(ELA_ERR_BADARG, "unknown option");
     }
@@ -2117,14 +2122,19 @@ int autotune_t(ela_hier* h, int nrep, double* ms) {
     HIPCHK(hipEventCreate(&e1));
     const int kzs[3] = {8, 16, 32};
     double best = 1e300;
-    int bkz = h->kz, bx = h->xcd ? 1 : 0, bry = h->ry;
+    int bkz = h->kz, bx = h->xcd ? 1 : 0, bry = h->ry, bpf = h->pf;
     const int nry = sizeof(T) == 4 ? 2 : 1;  // two rows per thread exist for FP32 sweeps
-    for (int ry = 1; ry <= nry; ++ry)
+    if (ms)
+        for (int q = 0; q < 24; ++q) ms[q] = 0.0;
+    const int npf = sizeof(T) == 4 ? 2 : 1;  // two planes ahead: FP32 tiles of one row per thread
+    for (int pf = 1; pf <= npf; ++pf)
+    for (int ry = 1; ry <= (pf == 2 ? 1 : nry); ++ry)
     for (int a = 0; a < 3; ++a)
         for (int x = 0; x < 2; ++x) {
             h->kz = kzs[a];
             h->xcd = x != 0;
             h->ry = ry;
+            h->pf = pf;
             A_jacobi<T>(h, v, b.C, b.dg, *b.cor, *b.res, *b.tmp, h->stream);  // first launch of the shape
             HIPCHK(hipEventRecord(e0, h->stream));
             for (int r = 0; r < nrep; ++r) A_jacobi<T>(h, v, b.C, b.dg, *b.cor, *b.res, *b.tmp, h->stream);
@@ -2133,19 +2143,19 @@ int autotune_t(ela_hier* h, int nrep, double* ms) {
             float t = 0;
             HIPCHK(hipEventElapsedTime(&t, e0, e1));
             const double avg = (double)t / nrep;
-            if (ms) ms[6 * (ry - 1) + 2 * a + x] = avg;
+            if (ms) ms[12 * (pf - 1) + 6 * (ry - 1) + 2 * a + x] = avg;
             if (avg < best) {
                 best = avg;
                 bkz = kzs[a];
                 bx = x;
                 bry = ry;
+                bpf = pf;
             }
         }
-    if (ms && nry == 1)
-        for (int q = 6; q < 12; ++q) ms[q] = 0.0;
     h->kz = bkz;
     h->xcd = bx != 0;
     h->ry = bry;
+    h->pf = bpf;
     // the coarse marching levels: their own tile height (fewer tiles per
     // level, so a lower height can fill the chip where level 0's would not)
     const int ckz[4] = {4, 8, 16, 32};

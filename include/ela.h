@@ -350,7 +350,12 @@ int ela_set_krylov(ela_hier* h, int restart);
  *                (default; 32 x 8 nodes per tile) or 2 (32 x 16: the halo
  *                is a smaller share of the tile, and a thread's two stencils
  *                share LDS reads).  Same results, bit for bit.  Hierarchies
- *                only. */
+ *                only.
+ * ELA_OPT_PREFETCH planes of u and grain ids a marching tile loads ahead of
+ *                the plane it computes: 1 (default) or 2 (the loads of plane
+ *                k+3 and the rhs/diagonal of plane k+1 are issued while plane
+ *                k is computed, so more bytes are in flight per block).  Same
+ *                results, bit for bit.  Hierarchies only. */
 enum {
     ELA_OPT_MARCH = 0,
     ELA_OPT_KZ = 1,
@@ -358,19 +363,21 @@ enum {
     ELA_OPT_MASK = 3,
     ELA_OPT_GRAPH = 4,
     ELA_OPT_FOLD = 5,
-    ELA_OPT_ROWS = 6
+    ELA_OPT_ROWS = 6,
+    ELA_OPT_PREFETCH = 7
 };
 int ela_set_option(ela_hier* h, int option, int value);
 
 /* Autotuner of the marching kernels (no reference counterpart): times nrep
  * level-0 Jacobi sweeps on this hierarchy's data for every tile height in
- * {8, 16, 32}, both tile orders and (FP32 sweeps) 1 or 2 rows per thread,
- * and keeps the fastest (ELA_OPT_KZ, ELA_OPT_XCD, ELA_OPT_ROWS); then gives
- * every coarse marching level the fastest of the heights {4, 8, 16, 32} for
- * its own sweeps (ELA_OPT_KZ resets those to the hierarchy's).  ms (12
- * doubles, may be NULL) receives level 0's average sweep times at index
- * 6 (rows - 1) + 2 (kz 8/16/32 -> 0/1/2) + order; 0 for rows 2 of an FP64
- * hierarchy.  Requires ela_prepare and grain form. */
+ * {8, 16, 32}, both tile orders, (FP32 sweeps) 1 or 2 rows per thread and
+ * 1 or 2 planes loaded ahead, and keeps the fastest (ELA_OPT_KZ, ELA_OPT_XCD,
+ * ELA_OPT_ROWS, ELA_OPT_PREFETCH); then gives every coarse marching level the
+ * fastest of the heights {4, 8, 16, 32} for its own sweeps (ELA_OPT_KZ resets
+ * those to the hierarchy's).  ms (24 doubles, may be NULL) receives level 0's
+ * average sweep times at index 12 (prefetch - 1) + 6 (rows - 1) + 2 (kz
+ * 8/16/32 -> 0/1/2) + order; 0 for rows 2 of an FP64 hierarchy.  Requires
+ * ela_prepare and grain form. */
 int ela_autotune(ela_hier* h, int nrep, double* ms);
 
 /* Record HIP events around the level-0 kernels of subsequent solves (on != 0). */
