@@ -243,10 +243,13 @@ __device__ __forceinline__ void stress(const Rec& c, const T g[DIM][DIM], T s[DI
         T tr = 0;
 #pragma unroll
         for (int a = 0; a < DIM; ++a) tr += g[a][a];
+        // explicit fused products: every kernel that inlines this rounds the
+        // same way (a free a*b + c*d may be contracted either way)
 #pragma unroll
         for (int a = 0; a < DIM; ++a)
 #pragma unroll
-            for (int b = 0; b < DIM; ++b) s[a][b] = mu * (g[a][b] + g[b][a]) + (a == b ? lam * tr : T(0));
+            for (int b = 0; b < DIM; ++b)
+                s[a][b] = a == b ? fma(mu, g[a][b] + g[b][a], lam * tr) : mu * (g[a][b] + g[b][a]);
     } else if (K == K_MM) {
         T gs[DIM][DIM];
 #pragma unroll
@@ -299,9 +302,11 @@ __device__ __forceinline__ void bc_row(const BCTab& bc, int face, const T u[DIM]
 // Interior row of A (Elastic.cpp:151-260): f = C : grad grad u + sum_d (d_d C :
 // grad u) e_d with central differences, on a stencil context S that provides
 //   S.u(p, dx, dy, dz)  component p of u at the neighbour (dx, dy, dz),
-//   S.rec(dx, dy, dz)   the stiffness record there (centre and +-e_d only),
+//   S.rec(dx, dy, dz)   the stiffness record at +-e_d,
+//   S.center()          the node's own record,
 //   S.same(d)           whether the records at +e_d and -e_d are bitwise equal
-// (their central difference is then exactly zero and is skipped).  Every
+//                       (their central difference is then exactly zero and is
+//                       skipped).  Every
 // context evaluates the same expressions in the same order, so a node gets
 // the same bits whether u comes from global memory or from an LDS tile.
 template <typename T, int K, int DIM, typename Ctx>
@@ -328,7 +333,7 @@ __device__ __forceinline__ void interior_apply(const Ctx& S, const Inv<T>& iv, b
                 gg[p][b][a] = m;
             }
     }
-    const auto c = S.rec(0, 0, 0);
+    const auto c = S.center();
     if (K == K_ISO) {
         const T lam = c[0], mu = c[1];
 #pragma unroll
@@ -339,7 +344,7 @@ __device__ __forceinline__ void interior_apply(const Ctx& S, const Inv<T>& iv, b
                 lap += gg[a][b][b];
                 gd += gg[b][a][b];
             }
-            f[a] = mu * lap + (mu + lam) * gd;
+            f[a] = fma(mu, lap, (mu + lam) * gd);  // fused explicitly, as in stress
         }
         if (!uniform) {
             T tr = 0;
@@ -360,7 +365,7 @@ __device__ __forceinline__ void interior_apply(const Ctx& S, const Inv<T>& iv, b
             for (int a = 0; a < DIM; ++a) {
                 T add = dl[a] * tr;
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) add += dm[d] * (g[a][d] + g[d][a]);
+                for (int d = 0; d < DIM; ++d) add = fma(dm[d], g[a][d] + g[d][a], add);
                 f[a] += add;
             }
         }
@@ -437,6 +442,7 @@ struct GlobalCtx {
     __device__ __forceinline__ long long at(int dx, int dy, int dz) const { return o + dx * sx + dy * sy + dz * sz; }
     __device__ __forceinline__ T u(int p, int dx, int dy, int dz) const { return up[p * cs + at(dx, dy, dz)]; }
     __device__ __forceinline__ auto rec(int dx, int dy, int dz) const { return A.rec(at(dx, dy, dz)); }
+    __device__ __forceinline__ auto center() const { return rec(0, 0, 0); }
     __device__ __forceinline__ bool same(int d) const {
         return A.same_dir(o, d, at(d == 0, d == 1, d == 2), at(-(d == 0), -(d == 1), -(d == 2)));
     }

@@ -1,6 +1,7 @@
 // ela_grain.hpp -- LDS z-marching stencil for boxes whose stiffness is in
 // grain form (a 16-bit grain id per node, one record of 21 values per grain,
-// optionally times a per-node factor).
+// optionally times a per-node factor), or in per-node isotropic planes (the
+// reference's per-node model field of (lambda, mu), read once per node).
 //
 // The interior of a box is cut into MTX x MTY tiles of (i, j) that march kz
 // planes in k.  A ring of three planes of u (tile + one-node halo), of the
@@ -57,12 +58,56 @@ struct LdsCtx {
         if constexpr (SC) return ScaledRec<T>{stab + (int)sg[q] * kGrainNC, ss[q]};
         else return TabRec<T>{stab + (int)sg[q] * kGrainNC};
     }
+    __device__ __forceinline__ auto center() const { return rec(0, 0, 0); }
     __device__ __forceinline__ bool same(int d) const {
         const int qp = at(d == 0, d == 1, d == 2), qm = at(-(d == 0), -(d == 1), -(d == 2));
         if constexpr (SC) return sg[qp] == sg[qm] && ss[qp] == ss[qm];
         else return sg[qp] == sg[qm];
     }
 };
+
+// A node's record in per-node planes, addressed by 32-bit byte offsets from
+// the planes' base (uniform over the block): each slot is one load at the
+// node's offset from a uniform slot base, so the record costs no 64-bit
+// address registers (plane_offsets_fit holds for every box marched).
+template <typename T>
+struct PlaneRecB {
+    const char* base;
+    unsigned off, csb;  // the node's byte offset; bytes between two slots' planes
+    __device__ __forceinline__ T operator[](int q) const {
+        return *(const T*)((base + (size_t)q * csb) + off);
+    }
+};
+
+// The same on a box whose stiffness is per-node planes: u from the LDS ring,
+// the node's own record from registers (cr, loaded ahead of the stencil), a
+// neighbour's from the planes at its offset (a cache hit after the
+// neighbour's own read), and the mask byte m of the node where the level has
+// a mask (else m = 0: every derivative of C is computed).
+template <typename T, int PL = MPL>
+struct PlaneLdsCtx {
+    const T* su;
+    int pb[3];
+    int tb;
+    const char* base;
+    unsigned off, csb;
+    int px, pl;
+    unsigned m;
+    const T* cr;
+    __device__ __forceinline__ TabRec<T> center() const { return TabRec<T>{cr}; }
+    __device__ __forceinline__ T u(int p, int dx, int dy, int dz) const {
+        return su[3 * pb[dz + 1] + tb + (p * PL + dy * MCW + dx)];
+    }
+    __device__ __forceinline__ PlaneRecB<T> rec(int dx, int dy, int dz) const {
+        return PlaneRecB<T>{base, off + (unsigned)((dx + dy * px + dz * pl) * (int)sizeof(T)), csb};
+    }
+    __device__ __forceinline__ bool same(int d) const { return (m >> d) & 1u; }
+};
+
+// Stiffness source of a marching box: grain form (ids, table, factors) or
+// per-node planes, with the model kind K of the planes (grain records are
+// always major/minor).
+constexpr int kSrcGrain = 0, kSrcPlanes = 1;
 
 // Operands of one grain-form box.  MODE 0: Jacobi sweep out = in + omega (rhs -
 // A in) / dg; MODE 1: out = rhs - A in; MODE 2: v = rhs - A in in FP64, out
@@ -81,6 +126,8 @@ struct GrainBox {
     const T* tab;
     const T* sc;
     int ngr;
+    const T* C;                // plane form: stiffness planes (component stride L.cs)
+    const unsigned char* sm;   // plane form: same-direction mask (bit d: skip the d-derivative of C) or null
     int ntx, nty, kz;
     int ry;   // rows per thread of the marching tiles (1 or 2)
     int pf;   // planes loaded ahead by the marching tiles (1 or 2; march_tile's PF)
@@ -96,6 +143,12 @@ struct GrainBox {
     const double* scale;
     T omega;
 };
+
+template <typename T, int SRC>
+__device__ __forceinline__ auto box_acc(const GrainBox<T>& g) {
+    if constexpr (SRC == kSrcGrain) return GrainAcc<T, kGrainNC>{g.gid, g.tab, g.sc};
+    else return PlaneAcc<T>{g.C, g.L.cs};
+}
 
 // The epilogue's operands of node o: rhs (MODES 0-2) and the diagonal (MODE 0).
 // With g.nt (levels far larger than the caches) nothing reads them again
@@ -167,10 +220,15 @@ struct MarchLds {
 };
 
 // Tile t of box g (256 threads); returns the thread's max |v| (MODE 2).
-// Every thread of the block calls this (it synchronizes).
-template <typename T, int MODE, typename TO, bool SC, int RY = 1, int PF = 1>
+// Every thread of the block calls this (it synchronizes).  SRC: the box's
+// stiffness is in grain form (the ids, and factors with SC, ride in the LDS
+// ring, the table sits in stab) or in per-node planes of model kind K (no
+// ring for them: the stencil reads each node's record from its planes).
+template <typename T, int MODE, typename TO, bool SC, int RY = 1, int PF = 1, int SRC = kSrcGrain, int K = K_MM>
 __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchLds<T, SC, RY>& sh, T* stab) {
     static_assert(MODE != 2 || sizeof(T) == 8, "the solution residual is FP64");
+    static_assert(SRC == kSrcPlanes || K == K_MM, "grain records are major/minor");
+    constexpr bool GR = SRC == kSrcGrain;
     using TL = MTile<RY>;
     constexpr int PL = TL::PL, HALO = TL::HALO, NH = TL::NH, RW = TL::RW;
     const Lay& L = g.L;
@@ -180,7 +238,8 @@ __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchL
     const int tx = tid % MTX, ty = tid / MTX;
     const int i = i0 + tx;
     const int kb = 1 + bz * g.kz, ke = min(kb + g.kz, L.nz - 1);
-    for (int q = tid; q < g.ngr * kGrainNC; q += MTX * MTY) stab[q] = g.tab[q];
+    if constexpr (GR)
+        for (int q = tid; q < g.ngr * kGrainNC; q += MTX * MTY) stab[q] = g.tab[q];
 
     // Plane loads.  Every thread reads its own RY nodes (i, j0 + RY ty + y) of
     // the tile for the three components, at 32-bit offsets from the plane's
@@ -224,7 +283,7 @@ __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchL
     auto at = [](const void* base, unsigned bytes) { return (const char*)base + bytes; };
     auto load = [&](int kk, Regs& R) {
         const T* up = g.in + L.pl * kk;
-        const unsigned short* gp = g.gid + L.pl * kk;
+        const unsigned short* gp = GR ? g.gid + L.pl * kk : nullptr;
 #pragma unroll
         for (int y = 0; y < RY; ++y)
 #pragma unroll
@@ -232,10 +291,12 @@ __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchL
                 R.ru[y][p] = iok[y] ? *(const T*)at(up + p * L.cs, ie[y] * (unsigned)sizeof(T)) : T(0);
 #pragma unroll
         for (int h = 0; h < NH; ++h) R.rh[h] = hok[h] ? *(const T*)at(up, he[h] * (unsigned)sizeof(T)) : T(0);
+        if constexpr (GR) {
 #pragma unroll
-        for (int y = 0; y < RY; ++y) R.rg[y] = iok[y] ? *(const unsigned short*)at(gp, ie[y] * 2u) : (unsigned short)0;
+            for (int y = 0; y < RY; ++y) R.rg[y] = iok[y] ? *(const unsigned short*)at(gp, ie[y] * 2u) : (unsigned short)0;
 #pragma unroll
-        for (int h = 0; h < NH; ++h) R.rgh[h] = gok[h] ? *(const unsigned short*)at(gp, he[h] * 2u) : (unsigned short)0;
+            for (int h = 0; h < NH; ++h) R.rgh[h] = gok[h] ? *(const unsigned short*)at(gp, he[h] * 2u) : (unsigned short)0;
+        }
         if constexpr (SC) {
             const T* sp = g.sc + L.pl * kk;
 #pragma unroll
@@ -255,14 +316,14 @@ __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchL
         for (int y = 0; y < RY; ++y) {
 #pragma unroll
             for (int p = 0; p < 3; ++p) su[p * PL + tb[y]] = R.ru[y][p];
-            sh.sg[slot * PL + tb[y]] = R.rg[y];
+            if constexpr (GR) sh.sg[slot * PL + tb[y]] = R.rg[y];
             if constexpr (SC) sh.ss[slot * PL + tb[y]] = R.rs[y];
         }
 #pragma unroll
         for (int h = 0; h < NH; ++h) {
             const int e = tid + h * (MTX * MTY);
             if (e < 3 * HALO) su[hpos[h]] = R.rh[h];
-            if (e < HALO) {  // component 0: the node's grain id (and factor)
+            if (GR && e < HALO) {  // component 0: the node's grain id (and factor)
                 sh.sg[slot * PL + hpos[h]] = R.rgh[h];
                 if constexpr (SC) sh.ss[slot * PL + hpos[h]] = R.rsh[h];
             }
@@ -287,9 +348,27 @@ __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchL
     // next plane (PF 2)
     T rb0[3], db0[3], rb1[3], db1[3], nb0[3], nd0[3], nb1[3], nd1[3];
     const long long ob0 = (long long)i + L.px * (j0 + ty * RY), ob1 = ob0 + L.px;
+    // plane form: the node's own record (and mask byte), loaded with the
+    // epilogue's operands so their latency overlaps the stencil
+    static_assert(GR || RY == 1, "plane-form tiles take one row per thread");
+    constexpr int NCR = GR ? 1 : NComp<3, K>::v;
+    T cr[NCR], ncr[NCR];
+    unsigned mk = 0, nmk = 0;
+    const unsigned csb = (unsigned)L.cs * (unsigned)sizeof(T);
+    auto rec_load = [&](long long o, T* r, unsigned& mm) {
+        if constexpr (!GR) {
+            const unsigned off = (unsigned)o * (unsigned)sizeof(T);
+#pragma unroll
+            for (int q = 0; q < NCR; ++q) r[q] = *(const T*)(((const char*)g.C + (size_t)q * csb) + off);
+            mm = g.sm ? (unsigned)g.sm[o] : 0u;
+        }
+    };
     if constexpr (PF == 2) {
         if (kb + 1 < ke) load(kb + 2, A);
-        if (act[0]) epilogue_load<T, MODE>(g, ob0 + L.pl * kb, rb0, db0);
+        if (act[0]) {
+            epilogue_load<T, MODE>(g, ob0 + L.pl * kb, rb0, db0);
+            rec_load(ob0 + L.pl * kb, cr, mk);
+        }
         if constexpr (RY == 2)
             if (act[RY - 1]) epilogue_load<T, MODE>(g, ob1 + L.pl * kb, rb1, db1);
     }
@@ -301,24 +380,36 @@ __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchL
         if constexpr (PF == 1) {
             // the epilogue's loads are issued before the stencils, so their
             // latency overlaps them, as the next plane's loads do
-            if (act[0]) epilogue_load<T, MODE>(g, o0, rb0, db0);
+            if (act[0]) {
+                epilogue_load<T, MODE>(g, o0, rb0, db0);
+                rec_load(o0, cr, mk);
+            }
             if constexpr (RY == 2)
                 if (act[RY - 1]) epilogue_load<T, MODE>(g, o1, rb1, db1);
             if (more) load(k + 2, A);
         } else {
             if (k + 2 < ke) load(k + 3, B);
             if (more) {
-                if (act[0]) epilogue_load<T, MODE>(g, o0 + L.pl, nb0, nd0);
+                if (act[0]) {
+                    epilogue_load<T, MODE>(g, o0 + L.pl, nb0, nd0);
+                    rec_load(o0 + L.pl, ncr, nmk);
+                }
                 if constexpr (RY == 2)
                     if (act[RY - 1]) epilogue_load<T, MODE>(g, o1 + L.pl, nb1, nd1);
             }
         }
         auto node = [&](int y, long long o, const T* rb, const T* db) {
-            LdsCtx<T, SC, PL> S{sh.su, sh.sg, sh.ss, stab, {s0 * PL, s1 * PL, s2 * PL}, tb[y]};
             T f[3], xc[3], r3[3] = {rb[0], rb[1], rb[2]}, d3[3] = {db[0], db[1], db[2]};
-            interior_apply<T, K_MM, 3>(S, g.iv, g.uniform != 0, f);
+            if constexpr (GR) {
+                LdsCtx<T, SC, PL> S{sh.su, sh.sg, sh.ss, stab, {s0 * PL, s1 * PL, s2 * PL}, tb[y]};
+                interior_apply<T, K_MM, 3>(S, g.iv, g.uniform != 0, f);
+            } else {
+                PlaneLdsCtx<T, PL> S{sh.su, {s0 * PL, s1 * PL, s2 * PL}, tb[y], (const char*)g.C,
+                                     (unsigned)o * (unsigned)sizeof(T), csb, (int)L.px, (int)L.pl, mk, cr};
+                interior_apply<T, K, 3>(S, g.iv, g.uniform != 0, f);
+            }
 #pragma unroll
-            for (int p = 0; p < 3; ++p) xc[p] = S.u(p, 0, 0, 0);
+            for (int p = 0; p < 3; ++p) xc[p] = sh.su[3 * (s1 * PL) + tb[y] + p * PL];
             const double mm = grain_epilogue<T, MODE, TO>(g, o, xc, f, r3, d3);
             m = (mm > m || mm != mm) ? mm : m;
         };
@@ -336,6 +427,11 @@ __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchL
         s3 = t0;
         if constexpr (PF == 2) {
             A = B;
+            if constexpr (!GR) {
+#pragma unroll
+                for (int q = 0; q < NCR; ++q) cr[q] = ncr[q];
+                mk = nmk;
+            }
 #pragma unroll
             for (int p = 0; p < 3; ++p) {
                 rb0[p] = nb0[p];
@@ -352,7 +448,7 @@ __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchL
 
 // Shell node q of box g (both z faces, then the y faces and the x faces
 // between them); returns false past the shell.  Returns the node's |v| max in m.
-template <typename T, int MODE, typename TO>
+template <typename T, int MODE, typename TO, int SRC = kSrcGrain, int K = K_MM>
 __device__ __forceinline__ double shell_node(const GrainBox<T>& g, int q) {
     const Lay& L = g.L;
     const int nx = L.nx, ny = L.ny, nz = L.nz;
@@ -377,7 +473,7 @@ __device__ __forceinline__ double shell_node(const GrainBox<T>& g, int q) {
         j = 1 + r - (k - 1) * (ny - 2);
     }
     T f[3], xc[3];
-    node_apply_acc<T, K_MM, 3>(L, GrainAcc<T, kGrainNC>{g.gid, g.tab, g.sc}, g.in, i, j, k, *g.bc, g.iv,
+    node_apply_acc<T, K, 3>(L, box_acc<T, SRC>(g), g.in, i, j, k, *g.bc, g.iv,
                                g.uniform != 0, f);
     const long long o = (long long)i + L.px * j + L.pl * k;
 #pragma unroll
@@ -399,7 +495,7 @@ constexpr int FJ = 16, FK = 16;
 __host__ __device__ inline int grain_face_patches(const Lay& L) {
     return 2 * ((L.ny - 2 + FJ - 1) / FJ) * ((L.nz - 2 + FK - 1) / FK);
 }
-template <typename T, int MODE, typename TO>
+template <typename T, int MODE, typename TO, int SRC = kSrcGrain, int K = K_MM>
 __device__ __forceinline__ double face_patch(const GrainBox<T>& g, int p, T* su) {
     const Lay& L = g.L;
     const int pj = (L.ny - 2 + FJ - 1) / FJ, pk = (L.nz - 2 + FK - 1) / FK;
@@ -434,7 +530,7 @@ __device__ __forceinline__ double face_patch(const GrainBox<T>& g, int p, T* su)
         gr[c][1] = (U(c, 1, 0, 0) - U(c, -1, 0, 0)) * g.iv.ih2[1];
         gr[c][2] = (U(c, 0, 1, 0) - U(c, 0, -1, 0)) * g.iv.ih2[2];
     }
-    stress<T, K_MM, 3>(GrainAcc<T, kGrainNC>{g.gid, g.tab, g.sc}.rec(o), gr, s);
+    stress<T, K, 3>(box_acc<T, SRC>(g).rec(o), gr, s);
     bc_row<T, 3>(*g.bc, face_of<3>(i, j, k, L.nx - 1, L.ny - 1, L.nz - 1), uc, gr, s, f);
     return grain_epilogue<T, MODE, TO>(g, o, uc, f, rb, db);
 }
@@ -451,8 +547,12 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
 }
 
 // The marching kernels address a box's fields with 32-bit byte offsets from a
-// plane's base (three components of FP64 at most).
+// plane's base (three components of FP64 at most), and per-node stiffness
+// planes (nc planes of es bytes) from the planes' base.
 __host__ __device__ inline bool grain_offsets_fit(const Lay& L) { return 3LL * L.cs * 8 < (1LL << 32); }
+__host__ __device__ inline bool plane_offsets_fit(const Lay& L, int nc, int es) {
+    return (long long)nc * L.cs * es < (1LL << 32);
+}
 
 // Tiles and shell blocks of a box (host and device); ry rows per thread.
 __host__ __device__ inline void grain_tiles(const Lay& L, int kz, int& ntx, int& nty, int& ntz, int& nsh,

@@ -135,6 +135,11 @@ void launch_post(const Lay& L, int kind, const double* C, const double* u, doubl
 template <typename T, int MODE, typename TO>
 void launch_grain(GrainBox<T> g, int kz, double* partials, unsigned long long* norm, hipStream_t s);
 long long grain_partials(const Lay& L, int kz);
+// The same z-marching on a box whose stiffness is per-node isotropic planes
+// (g.C: lambda, mu; optional same-direction mask g.sm): modes 0-3 of
+// GrainBox, tiles of one row per thread and g.pf planes loaded ahead.
+template <typename T, int MODE, typename TO>
+void launch_plane(GrainBox<T> g, int kz, double* partials, unsigned long long* norm, hipStream_t s);
 // Order-parameter fields for the model assembly: node values (dense over the
 // level's nodes), or cell values with one ghost layer ((n+2) cells per
 // direction, x fastest) averaged onto the nodes over every cell around a node

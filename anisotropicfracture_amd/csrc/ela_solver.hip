@@ -971,6 +971,17 @@ void grain_resid64<double, float>(const GrainBox<double>& g, int kz, double* p, 
     launch_grain<double, 2, float>(g, kz, p, n, s);
 }
 
+template <typename T, typename TO>
+void plane_resid64(const GrainBox<T>&, int, double*, unsigned long long*, hipStream_t) {}
+template <>
+void plane_resid64<double, double>(const GrainBox<double>& g, int kz, double* p, unsigned long long* n, hipStream_t s) {
+    launch_plane<double, 2, double>(g, kz, p, n, s);
+}
+template <>
+void plane_resid64<double, float>(const GrainBox<double>& g, int kz, double* p, unsigned long long* n, hipStream_t s) {
+    launch_plane<double, 2, float>(g, kz, p, n, s);
+}
+
 // Share of (wavefront, direction) pairs that must skip the derivative of C
 // for a level to read the mask: measured on MI355X, the mask speeds up a
 // 257^3 Voronoi level (28 % of the pairs skip) and slows down its coarser
@@ -1049,9 +1060,45 @@ GrainBox<T> grain_box(ela_hier* h, const Level& v) {
     return g;
 }
 
+// Isotropic per-node-plane levels through the z-marching kernels (3D): level
+// 0, and levels of at least kMarchMinNodes nodes a side, as in grain form.
+// The stencil is the node kernels' expression, so the bits are theirs.
+// Major/minor planes keep the node kernels (launch_plane).
+bool plane_march_on(ela_hier* h, const Level& v) {
+    if (h->dim != 3 || !h->grain_march || !h->dbc || h->kind != K_ISO) return false;
+    if (grain_on(h, v) || table_on(h, v) || coarse_tab_on(h, v)) return false;
+    if (!grain_offsets_fit(v.L) || !plane_offsets_fit(v.L, 2, 8)) return false;
+    if (&v == &h->lev[0]) return true;
+    return v.L.nx >= kMarchMinNodes && v.L.ny >= kMarchMinNodes && v.L.nz >= kMarchMinNodes;
+}
+
+// The marching operands of a plane level: its stiffness planes C and, where
+// the last ela_prepare built one, its same-direction mask.
+template <typename T>
+GrainBox<T> plane_box(ela_hier* h, const Level& v, const T* C) {
+    GrainBox<T> g{};
+    g.L = v.L;
+    g.iv = make_inv<T>(v.h);
+    g.bc = h->dbc;
+    g.uniform = h->uniform;
+    g.C = C;
+    g.sm = h->prepared && v.sm_ok && h->mask_mode != 0 ? v.sm : nullptr;
+    g.xcd = h->xcd;
+    g.ry = 1;
+    g.pf = h->pf;
+    g.nt = (long long)v.nodes >= kNtMinNodes;
+    g.xface = true;
+    return g;
+}
+
 template <typename T>
 void A_apply(ela_hier* h, Level& v, const T* C, const T* u, T* f, hipStream_t s) {
-    if (grain_on(h, v)) {
+    if (plane_march_on(h, v)) {
+        GrainBox<T> g = plane_box<T>(h, v, C);
+        g.in = u;
+        g.out = f;
+        launch_plane<T, 3, T>(g, level_kz(h, v), nullptr, nullptr, s);
+    } else if (grain_on(h, v)) {
         GrainBox<T> g = grain_box<T>(h, v);
         g.in = u;
         g.out = f;
@@ -1063,7 +1110,15 @@ void A_apply(ela_hier* h, Level& v, const T* C, const T* u, T* f, hipStream_t s)
 
 template <typename T>
 void A_jacobi(ela_hier* h, Level& v, const T* C, const T* dg, const T* xin, const T* b, T* xout, hipStream_t s) {
-    if (grain_on(h, v)) {
+    if (plane_march_on(h, v)) {
+        GrainBox<T> g = plane_box<T>(h, v, C);
+        g.in = xin;
+        g.rhs = b;
+        g.out = xout;
+        g.dg = dg;
+        g.omega = (T)h->omega;
+        launch_plane<T, 0, T>(g, level_kz(h, v), nullptr, nullptr, s);
+    } else if (grain_on(h, v)) {
         GrainBox<T> g = grain_box<T>(h, v);
         g.in = xin;
         g.rhs = b;
@@ -1094,7 +1149,20 @@ void A_residual(ela_hier* h, Level& v, const T* C, const T* x, const T* b, T* r,
                 unsigned long long* norm, hipStream_t s, double* part) {
     double* const P = part ? part : h->partials;
     const bool plain = !rs && !norm;
-    if (grain_on(h, v) && (plain || sizeof(T) == 8)) {
+    if (plane_march_on(h, v) && (plain || sizeof(T) == 8)) {
+        GrainBox<T> g = plane_box<T>(h, v, C);
+        g.in = x;
+        g.rhs = b;
+        if (plain) {
+            g.out = r;
+            launch_plane<T, 1, T>(g, level_kz(h, v), nullptr, nullptr, s);
+        } else {
+            g.out = rs;
+            g.out2 = (double*)r;
+            g.scale = scale;
+            plane_resid64<T, TO>(g, level_kz(h, v), P, norm, s);
+        }
+    } else if (grain_on(h, v) && (plain || sizeof(T) == 8)) {
         GrainBox<T> g = grain_box<T>(h, v);
         g.in = x;
         g.rhs = b;

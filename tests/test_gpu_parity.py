@@ -359,6 +359,73 @@ def test_plane_mask_keeps_the_bits(mode):
             assert np.array_equal(a, c)
 
 
+@pytest.mark.parametrize("n", [(37, 20, 45), (40, 17, 33)])
+@pytest.mark.parametrize("mask", [0, 1])
+def test_plane_marching_kernels_give_the_node_kernel_bits(oracle, n, mask):
+    """Isotropic per-node planes (every node its own record: the derivative of C everywhere, except in a
+    constant half) with the mixed boundary table: level 0 through the LDS z-marching kernels (one and two
+    planes loaded ahead) and through the one-thread-per-node kernels (OPT_MARCH 0) give the same Fapply
+    (FP64, FP32), residual and norm bit for bit; Fapply agrees with the oracle to 1e-12 of the stencil
+    scale.  With the mask (OPT_MASK 2) the marching tiles read each node's same-direction byte."""
+    rng = np.random.default_rng(41)
+    g = oracle.Grid(n)
+    C = H.random_stiffness(rng, n, ISO, 3)
+    C[:, : (n[0] + 1) // 2] = C[:, :1, :1, :1]  # a constant half: its nodes skip the derivative with the mask
+    t, v = H.bc_mixed(3)
+    u, b = H.random_field(rng, n, 3), H.random_field(rng, n, 3)
+    out = []
+    for march, pf in ((0, 1), (1, 1), (1, 2)):
+        op = ela.Elastic(n, ISO, mode=ela.ALLOC_BOTH)
+        op.SetModel(C)
+        op.SetBC(t, v)
+        op.set_option(ela.OPT_MARCH, march)
+        op.set_option(ela.OPT_PREFETCH, pf)
+        op.set_option(ela.OPT_MASK, 2 * mask)
+        op.prepareForSolve()
+        res = [op.Fapply(u, prec=ela.F64), op.Fapply(u, prec=ela.F32)]
+        for prec in (ela.F64, ela.F32):
+            r, nrm = op.residual(u, b, prec=prec)
+            res += [r, np.array([nrm])]
+        op.close()
+        out.append(res)
+    want = oracle.fapply(g, ISO, C, t, u)
+    assert np.abs(out[1][0] - want).max() <= 1e-12 * scale_of(C, u, g.dx)
+    names = ("Fapply F64", "Fapply F32", "r F64", "|r| F64", "r F32", "|r| F32")
+    for v_ in (1, 2):
+        for name, a, c in zip(names, out[0], out[v_]):
+            bad = np.argwhere(a != c)
+            assert len(bad) == 0, (v_, name, len(bad), bad[:4].tolist(), float(np.abs(a - c).max()))
+
+
+@pytest.mark.parametrize("mode", [ela.SOLVE_FP64, ela.SOLVE_MIXED])
+def test_plane_marching_solve_gives_the_node_kernel_bits(mode):
+    """A smoothly varying isotropic stiffness under uniaxial z loading, FGMRES(6): the solve through the
+    marching kernels on level 0 is the solve through the node kernels, bit for bit."""
+    n = (40, 36, 44)
+    x = [np.linspace(0.0, 1.0, m + 1) for m in n]
+    X, Y, Z = np.meshgrid(*x, indexing="ij")
+    C = np.empty((2,) + X.shape)
+    C[0] = 1.2 * (1.0 + 0.5 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y))
+    C[1] = 0.8 * (1.0 + 0.4 * np.cos(2 * np.pi * Z) * np.sin(np.pi * X))
+    C = np.moveaxis(C, (1, 2, 3), (3, 2, 1)).copy()  # (comp, z, y, x)
+    cfg = K.polycrystal(8)  # its uniaxial z boundary table
+    sols = []
+    for march in (0, 1):
+        op = ela.Elastic(n, ISO, mode=ela.ALLOC_BOTH)
+        op.SetModel(C)
+        op.SetBC(cfg["bc_type"], cfg["bc_value"])
+        op.SetKrylov(6)
+        op.set_option(ela.OPT_MARCH, march)
+        op.prepareForSolve()
+        b = op.bc_init()
+        u, st = ela.Linear(op).solve(np.zeros_like(b), b, tol_rel=1e-10, tol_abs=0.0, mode=mode, max_iter=300)
+        assert st["converged"]
+        sols.append((u, st["iters"]))
+        op.close()
+    assert sols[0][1] == sols[1][1]
+    assert np.array_equal(sols[0][0], sols[1][0])
+
+
 @pytest.mark.parametrize("n", [(40, 36, 34), (128, 128, 128)])
 def test_grain_form_coarse_operator_is_the_injected_one(oracle, n):
     """A grain-form hierarchy coarsens by injection: level 1's operator is the oracle's operator with the
