@@ -146,6 +146,7 @@ SIGNATURES = {
     "ela_time_fapply": ([_P, _I, _I, ctypes.POINTER(_D)], _I),
     "ela_probe_copy": ([_P, _SZ, _I, ctypes.POINTER(_D)], _I),
     "ela_probe_copy_ex": ([_P, _SZ, _I, _I, ctypes.POINTER(_D)], _I),
+    "ela_probe_barrier": ([_P, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_D)], _I),
     "ela_hier_create_ex": ([_P, ctypes.POINTER(_Domain), _I, _I, _I, _I, ctypes.POINTER(_P)], _I),
     # batched independent solves
     "ela_batch_create": ([_P, _I, ctypes.POINTER(_P)], _I),
@@ -255,6 +256,13 @@ class Context:
         ms = ctypes.c_double()
         _check(lib().ela_probe_copy_ex(self.ptr, nbytes, nrep, int(variant), ctypes.byref(ms)))
         return ms.value
+
+    def probe_barrier(self, blocks, rounds=2000):
+        """(us per device-wide barrier among `blocks` blocks, us per back-to-back empty launch of that grid)
+        (ela_probe_barrier)."""
+        b, l = ctypes.c_double(), ctypes.c_double()
+        _check(lib().ela_probe_barrier(self.ptr, int(blocks), int(rounds), ctypes.byref(b), ctypes.byref(l)))
+        return b.value, l.value
 
     def bytes_in_use(self):
         return lib().ela_ctx_bytes_in_use(self.ptr)

@@ -1037,6 +1037,31 @@ void launch_copy_variant(int var, const void* src, void* dst, long long bytes, h
         hipLaunchKernelGGL((k_copy_u<8, true>), dim3(b), dim3(256), 0, s, (const dvec2*)src, (dvec2*)dst, n);
 }
 
+// Device-wide barrier probe: each of the launch's blocks takes part in
+// `rounds` barriers through one device-scope atomic counter; thread 0 spins
+// until every block has arrived, at most spin_max polls a barrier (then it
+// raises *err and every block leaves at its next poll or round), so every
+// wave ends whatever the blocks' residency.  rounds 0: an empty launch.
+__global__ __launch_bounds__(256) void k_barrier_probe(unsigned* cnt, int* err, int rounds, long long spin_max) {
+    const unsigned G = gridDim.x;
+    for (int r = 1; r <= rounds; ++r) {
+        if (threadIdx.x == 0) {
+            __threadfence();
+            atomicAdd(cnt, 1u);
+            long long s = 0;
+            while (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < (unsigned)r * G) {
+                if (++s > spin_max) atomicExch(err, 1);
+                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+            }
+        }
+        __syncthreads();
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    }
+}
+void launch_barrier_probe(int blocks, unsigned* cnt, int* err, int rounds, hipStream_t s) {
+    hipLaunchKernelGGL(k_barrier_probe, dim3(blocks), dim3(256), 0, s, cnt, err, rounds, 1LL << 22);
+}
+
 // ------------------------------------------------------------------ launchers
 #define ELA_DISPATCH(KIND, DIMV, ...)                                              \
     do {                                                                           \

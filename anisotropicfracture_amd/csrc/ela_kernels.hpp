@@ -52,6 +52,10 @@ long long node_list_blocks(const Lay& L);
 void launch_node_list(const Lay& L, const unsigned char* sm, int* cnt, long long* totals, unsigned long long* list,
                       hipStream_t s);
 
+// Barrier probe (ela_probe_barrier): `rounds` device-wide barriers among
+// `blocks` blocks of 256 threads, bounded spin; *err set when they time out.
+void launch_barrier_probe(int blocks, unsigned* cnt, int* err, int rounds, hipStream_t s);
+
 template <typename T>
 void launch_jacobi0(const Lay& L, const T* dg, const T* b, T* xout, double omega, hipStream_t s);
 

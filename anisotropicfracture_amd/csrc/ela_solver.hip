@@ -2333,4 +2333,43 @@ int ela_probe_copy_ex(ela_ctx* ctx, size_t bytes, int nrep, int variant, double*
     return ELA_OK;
 }
 
+int ela_probe_barrier(ela_ctx* ctx, int blocks, int rounds, double* us_per_barrier, double* us_per_launch) {
+    if (!ctx || !us_per_barrier || !us_per_launch || blocks < 1 || blocks > 1024 || rounds < 1)
+        return fail(ELA_ERR_BADARG, "bad argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    unsigned* cnt = nullptr;
+    int* err = nullptr;
+    HIPCHK(hipMalloc(&cnt, 256));
+    err = (int*)((char*)cnt + 128);
+    hipStream_t s;
+    HIPCHK(hipStreamCreate(&s));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    auto timed = [&](int r, int nlaunch, float& ms) -> int {
+        HIPCHK(hipMemsetAsync(cnt, 0, 256, s));
+        launch_barrier_probe(blocks, cnt, err, 0, s);  // first launch of the kernel
+        HIPCHK(hipEventRecord(e0, s));
+        for (int q = 0; q < nlaunch; ++q) launch_barrier_probe(blocks, cnt, err, r, s);
+        HIPCHK(hipEventRecord(e1, s));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        return ELA_OK;
+    };
+    float mb = 0, ml = 0;
+    int rc = timed(rounds, 1, mb);
+    int bad = 0;
+    if (!rc) rc = hipMemcpy(&bad, err, sizeof bad, hipMemcpyDeviceToHost) == hipSuccess ? ELA_OK : fail(ELA_ERR_HIP, "hipMemcpy failed");
+    if (!rc) rc = timed(0, 1000, ml);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    (void)hipFree(cnt);
+    if (rc) return rc;
+    if (bad) return fail(ELA_ERR_STATE, "barrier probe timed out: the blocks were not all resident");
+    *us_per_barrier = 1e3 * mb / rounds;
+    *us_per_launch = 1e3 * ml / 1000;
+    return ELA_OK;
+}
+
 }  // extern "C"

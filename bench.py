@@ -463,7 +463,7 @@ def serving(seconds, max_wall, tol, budget_gb=200.0, max_active=256, rate=200.0,
     return out
 
 
-def pmc_traffic(fetch_csv, write_csv, n, out_path):
+def pmc_traffic(fetch_csv, write_csv, n, tile, out_path):
     """HBM bytes per launch of the level-0 kernels from two `rocprofv3 --pmc` runs of this bench (one with
     FETCH_SIZE, one with WRITE_SIZE; their counter_collection.csv files), each calibrated by what the same
     counter reads for the copy probe of known size (1 GiB read + 1 GiB written a launch) in the same run.
@@ -476,8 +476,8 @@ def pmc_traffic(fetch_csv, write_csv, n, out_path):
     kinds = [(r"k_grain_sweep<float, 0,", "k_grain_sweep<float,0> (level-0 Jacobi sweep, grain form: shell + marching tiles)", 4 * 12 + 2),
              (r"k_grain_sweep<double, 3,", "k_grain_sweep<double,3> (level-0 Fapply FP64, grain form)", 8 * 6 + 2),
              (r"k_grain_sweep<float, 3,", "k_grain_sweep<float,3> (level-0 Fapply FP32, grain form)", 4 * 6 + 2),
-             (r"k_fapply<double, 1, 3, (Masked)?PlaneAcc", "k_fapply<double,MM,3,planes> (level-0 Fapply FP64, per-node planes)", 8 * 27),
-             (r"k_fapply<float, 1, 3, (Masked)?PlaneAcc", "k_fapply<float,MM,3,planes> (level-0 Fapply FP32, per-node planes)", 4 * 27)]
+             (r"k_fapply<double, 1, 3, ela::(Masked)?PlaneAcc", "k_fapply<double,MM,3,planes> (level-0 Fapply FP64, per-node planes)", 8 * 27),
+             (r"k_fapply<float, 1, 3, ela::(Masked)?PlaneAcc", "k_fapply<float,MM,3,planes> (level-0 Fapply FP32, per-node planes)", 4 * 27)]
 
     def rows(path, counter):
         with open(path) as fh:
@@ -511,9 +511,9 @@ def pmc_traffic(fetch_csv, write_csv, n, out_path):
         fb, wb = float(np.mean(fv)) * ff, float(np.mean(wv)) * wf
         table[desc] = {"grid_threads": grid, "fetch_bytes": fb, "write_bytes": wb, "algorithmic_bytes": per_node * nodes,
                        "traffic_over_algorithmic": (fb + wb) / (per_node * nodes), "dispatches": len(fv)}
-    out = {"n": n, "nodes": nodes,
+    out = {"n": n, "nodes": nodes, "tile": tile,
            "source": "rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of: python3 bench.py --only poly "
-                     "--no-cpu --graphs 0 --steps 1 --warmup 1 --n %d" % n,
+                     "--no-cpu --graphs 0 --steps 1 --warmup 1 --n %d --tile %s" % (n, tile),
            "calibration": {"kernel": "k_copy (ela_probe_copy variant 0, 1 GiB read + 1 GiB written per launch)",
                            "fetch_factor": ff, "write_factor": wf},
            "kernels": table}
@@ -524,8 +524,8 @@ def pmc_traffic(fetch_csv, write_csv, n, out_path):
 
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--pmc-traffic":
-        # python bench.py --pmc-traffic FETCH.csv WRITE.csv N OUT.json
-        pmc_traffic(sys.argv[2], sys.argv[3], int(sys.argv[4]), sys.argv[5])
+        # python bench.py --pmc-traffic FETCH.csv WRITE.csv N TILE OUT.json
+        pmc_traffic(sys.argv[2], sys.argv[3], int(sys.argv[4]), sys.argv[5], sys.argv[6])
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -554,6 +554,12 @@ def main():
     ap.add_argument("--graphs", type=int, default=1,
                     help="also time the graph-replayed variants (0: skip them, e.g. under rocprofv3 --kernel-trace, "
                          "where a HIP graph launch has crashed the profiled process)")
+    ap.add_argument("--planes", type=int, default=1,
+                    help="also solve with per-node stiffness planes (0: only the grain-table solves, e.g. to profile "
+                         "the headline)")
+    ap.add_argument("--tile", default="",
+                    help="fixed level-0 marching shape KZ/XCD/ROWS/PREFETCH (as the autotune keys), no autotuning "
+                         "(the counter runs, whose serialized launches would mislead the autotuner)")
     ap.add_argument("--kz", type=int, default=0,
                     help="marching tile height of the 256^3 polycrystal hierarchy (0: autotune it before the timed region)")
     args = ap.parse_args()
@@ -569,8 +575,8 @@ def main():
     nodes = (n + 1) ** 3
     poly = n > 0 and "poly" in args.only.split(",")
     tmx = t64 = t64t = tmp = tgr = st64 = stmx = kt = None
-    umx = tuned = res_norms = disp_diff = probe_GBps = probe_var = fa64 = fa32 = ga64 = ga32 = None
-    dom = dom_ms = dom_bytes = achieved = traffic = levels = plain_cycles = None
+    umx = tuned = res_norms = disp_diff = probe_GBps = probe_var = fa64 = fa32 = ga64 = ga32 = latency = None
+    dom = dom_ms = dom_bytes = achieved = traffic = traffic_tile = timed_tile = levels = plain_cycles = None
     peak, nc = 8000.0, 21
     if poly:
         log("bench: building the polycrystal workload n=%d grains=%d" % (n, args.grains))
@@ -581,7 +587,11 @@ def main():
         op.SetKrylov(args.krylov)
         op.prepareForSolve()
         tuned = None
-        if args.table and args.kz:  # a fixed marching tile height (XCD-contiguous tile order)
+        if args.table and args.tile:  # a fixed marching shape
+            kz_, xcd_, ry_, pf_ = (int(x) for x in args.tile.split("/"))
+            for opt, val in ((ela.OPT_KZ, kz_), (ela.OPT_XCD, xcd_), (ela.OPT_ROWS, ry_), (ela.OPT_PREFETCH, pf_)):
+                op.set_option(opt, val)
+        elif args.table and args.kz:  # a fixed marching tile height (XCD-contiguous tile order)
             op.set_option(ela.OPT_KZ, args.kz)
             op.set_option(ela.OPT_XCD, 1)
         elif args.table:  # marching tile shape for this hierarchy, picked before the timed region
@@ -596,13 +606,13 @@ def main():
 
         modes = args.modes.split(",")
         t64 = st64 = u64 = None
-        if "fp64" in modes:  # plain FP64 GPU solver, per-node stiffness planes (the reference's storage)
+        if "fp64" in modes and args.planes:  # plain FP64 GPU solver, per-node stiffness planes (the reference's storage)
             log("bench: fp64 solves (stiffness planes)")
             t64, per64, st64, rc64, _ = run_mode(op, b, u, ela.SOLVE_FP64, args.steps, args.warmup, args.tol)
             u64 = u.download()
             log("bench: fp64 %.3f s/solve iters=%d rel=%.2e rc=%d" % (t64 / args.steps, st64["iters"], st64["res"] / st64["res0"], rc64))
         tmp = stmp = None
-        if "mixed" in modes and args.table:  # mixed with planes, for the precision-only ratio
+        if "mixed" in modes and args.table and args.planes:  # mixed with planes, for the precision-only ratio
             log("bench: mixed solves (stiffness planes)")
             tmp, _, stmp, _, _ = run_mode(op, b, u, ela.SOLVE_MIXED, args.steps, args.warmup, args.tol)
             log("bench: mixed planes %.3f s/solve iters=%d" % (tmp / args.steps, stmp["iters"]))
@@ -646,6 +656,11 @@ def main():
         probe_bytes = 1 << 30
         probe_var = {v: 2 * probe_bytes / (op.ctx.probe_copy(probe_bytes, 5, v) * 1e-3) / 1e9 for v in range(7)}
         probe_GBps = max(probe_var.values())
+        # what a device-wide barrier and a bare launch cost (the floor under the small levels' kernels)
+        latency = {}
+        for nb in (8, 64, 256):
+            bar, launch = op.ctx.probe_barrier(nb, 2000)
+            latency[str(nb)] = {"barrier_us": bar, "empty_launch_us": launch}
 
         # Fapply alone (FP64 and FP32), HIP events on the library stream
         fa64 = op.time_fapply(ela.F64, 20)
@@ -678,7 +693,7 @@ def main():
             dom_ms, dom_bytes = kt["resid64_ms"], bytes_res64
         peak = 8000.0
         achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-        traffic = None
+        traffic = traffic_tile = None
         tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
         if os.path.exists(tpath):  # HBM bytes per launch from the rocprofv3 --pmc runs, copy-calibrated
             with open(tpath) as fh:
@@ -686,6 +701,8 @@ def main():
             ent = tr.get("kernels", {}).get(dom)
             if ent and tr.get("n") == n:
                 traffic = ent["fetch_bytes"] + ent["write_bytes"]
+                traffic_tile = tr.get("tile")
+        timed_tile = args.tile or (min(tuned, key=tuned.get) if tuned else None)
 
         levels = op.levels()
         op.close()
@@ -761,6 +778,7 @@ def main():
         "rel_residual": {"fp64": st64["res"] / st64["res0"] if st64 else None, "mixed": stmx["res"] / stmx["res0"]},
         "hbm_copy_probe_GBps": probe_GBps,
         "hbm_copy_probe_variants_GBps": probe_var,
+        "launch_latency_by_blocks": latency,
         "rel_disp_diff_mixed_vs_fp64": disp_diff,
         "mixed_rel_residual_norms": res_norms,
         "fapply": {"fp64_ms": fa64, "fp64_GBps": bytes_fa64 / (fa64 * 1e-3) / 1e9,
@@ -773,6 +791,7 @@ def main():
                        "fp64_speedup_over_planes": fa64 / ga64}},
         "kernel_times": kt,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": peak, "unit": "GB/s", "frac": achieved / peak,
+                     "tile": timed_tile, "traffic_tile": traffic_tile,
                      "traffic": traffic, "kernel": dom, "kernel_ms": dom_ms, "bytes_per_launch": dom_bytes},
         "cpu_baseline": cpu,
         "per_config_error": checks,

@@ -396,6 +396,13 @@ int ela_probe_copy(ela_ctx* ctx, size_t bytes, int nrep, double* avg_ms);
  * non-temporal stores on 2048 / 8192 / 32768 blocks; 4-6 eight loads in
  * flight, same grids.  The bandwidth probe reports the fastest. */
 int ela_probe_copy_ex(ela_ctx* ctx, size_t bytes, int nrep, int variant, double* avg_ms);
+/* Measurement helper with no counterpart in the reference: the cost of a
+ * device-wide barrier among `blocks` (1..1024) co-resident blocks of 256
+ * threads (one device-scope atomic counter, bounded spin; `rounds` barriers in
+ * one launch, microseconds each), and of a back-to-back empty launch of the
+ * same grid (microseconds each, over 1000 launches).  ELA_ERR_STATE if the
+ * barrier timed out. */
+int ela_probe_barrier(ela_ctx* ctx, int blocks, int rounds, double* us_per_barrier, double* us_per_launch);
 
 /* ------------------------------------------------------------------ AMR level transfers
  *
