@@ -288,6 +288,7 @@ struct CLev {
     size_t off = 0;     // element offset of the level's segment in composite vectors
     size_t elems = 0;   // dim * L.cs
     BCTab bcG{};        // boundary rows of G (domain classes; Displacement on artificial faces)
+    BCTab* dbcG = nullptr;  // its device copy (the z-marching kernels read the table from memory)
     double* C = nullptr;
     bool model_set = false;
     std::vector<int> boxes;          // lo[3], hi[3] per box (level l >= 1)
@@ -365,6 +366,64 @@ void afree_all(ela_amr* a) {
 
 double* seg(double* base, const CLev& v) { return base + v.off; }
 
+// A level's operator on G: 3D isotropic planes run the z-marching tiles
+// (k_plane_sweep: the node kernels' expressions, so the same bits), anything
+// else the node kernels.
+bool level_march(const ela_amr* a, const CLev& v) {
+    return a->dim == 3 && a->kind == K_ISO && v.dbcG && grain_offsets_fit(v.L) && plane_offsets_fit(v.L, 2, 8);
+}
+constexpr int kLevelKz = 16;  // planes per marching tile on the levels' arrays
+GrainBox<double> level_box(const ela_amr* a, const CLev& v) {
+    GrainBox<double> g{};
+    g.L = v.L;
+    g.iv = make_inv<double>(v.h);
+    g.bc = v.dbcG;
+    g.uniform = a->uniform;
+    g.C = v.C;
+    g.sm = nullptr;
+    g.xcd = 1;
+    g.ry = 1;
+    g.pf = 1;
+    g.nt = false;
+    g.xface = true;
+    return g;
+}
+void lev_apply(const ela_amr* a, const CLev& v, const double* x, double* y) {
+    if (level_march(a, v)) {
+        GrainBox<double> g = level_box(a, v);
+        g.in = x;
+        g.out = y;
+        launch_plane<double, 3, double>(g, kLevelKz, nullptr, nullptr, a->stream);
+    } else {
+        launch_fapply<double>(v.L, a->kind, v.C, x, y, v.bcG, v.h, a->uniform, a->stream);
+    }
+}
+void lev_residual(const ela_amr* a, const CLev& v, const double* x, const double* b, double* r) {
+    if (level_march(a, v)) {
+        GrainBox<double> g = level_box(a, v);
+        g.in = x;
+        g.rhs = b;
+        g.out = r;
+        launch_plane<double, 1, double>(g, kLevelKz, nullptr, nullptr, a->stream);
+    } else {
+        launch_residual<double, double>(v.L, a->kind, v.C, x, b, r, (double*)nullptr, nullptr, nullptr, nullptr,
+                                        v.bcG, v.h, a->uniform, a->stream);
+    }
+}
+void lev_jacobi(const ela_amr* a, const CLev& v, const double* x, const double* b, double* out) {
+    if (level_march(a, v)) {
+        GrainBox<double> g = level_box(a, v);
+        g.in = x;
+        g.rhs = b;
+        g.out = out;
+        g.dg = v.dg;
+        g.omega = 2.0 / 3.0;
+        launch_plane<double, 0, double>(g, kLevelKz, nullptr, nullptr, a->stream);
+    } else {
+        launch_jacobi<double>(v.L, a->kind, v.C, v.dg, x, b, out, v.bcG, v.h, a->uniform, 2.0 / 3.0, a->stream);
+    }
+}
+
 // Launch grid over the bounding box of a level's nodes under the next finer level.
 dim3 under_grid(const ela_amr* a, const CLev& c) {
     return dim3((c.geo.uhi[0] - c.geo.ulo[0] + CX) / CX, (c.geo.uhi[1] - c.geo.ulo[1] + CY) / CY,
@@ -420,9 +479,7 @@ double read_norm(ela_amr* a) {
 // r = composite residual of x (masked), |r|inf into dnorm
 void residual(ela_amr* a, double* x, const double* bb, double* r) {
     fill(a, x);
-    for (auto& v : a->lev)
-        launch_residual<double, double>(v.L, a->kind, v.C, seg(x, v), bb + v.off, seg(r, v), (double*)nullptr,
-                                        nullptr, nullptr, nullptr, v.bcG, v.h, a->uniform, a->stream);
+    for (auto& v : a->lev) lev_residual(a, v, seg(x, v), bb + v.off, seg(r, v));
     for (int l = a->nlev - 1; l >= 1; --l) reflux(a, l, seg(r, a->lev[l]), nullptr, seg(r, a->lev[l - 1]), 0);
     mask(a, r);
     maxabs(a, r);
@@ -431,8 +488,7 @@ void residual(ela_amr* a, double* x, const double* bb, double* r) {
 // y = A_c x (masked)
 void apply(ela_amr* a, double* x, double* y) {
     fill(a, x);
-    for (auto& v : a->lev)
-        launch_fapply<double>(v.L, a->kind, v.C, seg(x, v), seg(y, v), v.bcG, v.h, a->uniform, a->stream);
+    for (auto& v : a->lev) lev_apply(a, v, seg(x, v), seg(y, v));
     for (int l = a->nlev - 1; l >= 1; --l) reflux(a, l, seg(y, a->lev[l]), nullptr, seg(y, a->lev[l - 1]), 0);
     mask(a, y);
 }
@@ -467,9 +523,9 @@ void level_smooth(ela_amr* a, int l, double* x, const double* rhs, double* tmp, 
     CLev& f = a->lev[l];
     const dim3 g = grid_of(f.L);
     for (int c = 0; c < ncalls; ++c) {
-        launch_jacobi<double>(f.L, a->kind, f.C, f.dg, x, rhs, tmp, f.bcG, f.h, a->uniform, 2.0 / 3.0, a->stream);
+        lev_jacobi(a, f, x, rhs, tmp);
         hipLaunchKernelGGL(k_hold_fill, g, dim3(CX, CY), 0, a->stream, f.L, f.geo, (const double*)x, tmp, a->dim);
-        launch_jacobi<double>(f.L, a->kind, f.C, f.dg, tmp, rhs, x, f.bcG, f.h, a->uniform, 2.0 / 3.0, a->stream);
+        lev_jacobi(a, f, tmp, rhs, x);
         hipLaunchKernelGGL(k_hold_fill, g, dim3(CX, CY), 0, a->stream, f.L, f.geo, (const double*)tmp, x, a->dim);
     }
 }
@@ -488,7 +544,7 @@ int fac(ela_amr* a, const double* v, double* z) {
         } else {
             level_smooth(a, l, seg(z, f), seg(a->t, f), seg(a->q, f), f.nu1);
         }
-        launch_fapply<double>(f.L, a->kind, f.C, seg(z, f), seg(a->w, f), f.bcG, f.h, a->uniform, s);
+        lev_apply(a, f, seg(z, f), seg(a->w, f));
         reflux(a, l, seg(a->t, f), seg(a->w, f), seg(a->t, a->lev[l - 1]), 1);
     }
     if (int rc = box_correct(a, 0, seg(a->t, a->lev[0]), seg(z, a->lev[0]), 0)) return rc;
@@ -497,8 +553,7 @@ int fac(ela_amr* a, const double* v, double* z) {
         hipLaunchKernelGGL(k_fill<double>, grid_of(f.L), dim3(CX, CY), 0, s, f.L, f.geo, c.L, c.geo.glo[0],
                            c.geo.glo[1], c.geo.glo[2], (const double*)seg(z, c), seg(z, f), a->dim, 1);
         if (f.H) {
-            launch_residual<double, double>(f.L, a->kind, f.C, seg(z, f), seg(a->t, f), seg(a->w, f),
-                                            (double*)nullptr, nullptr, nullptr, nullptr, f.bcG, f.h, a->uniform, s);
+            lev_residual(a, f, seg(z, f), seg(a->t, f), seg(a->w, f));
             if (int rc = box_correct(a, l, seg(a->w, f), seg(z, f), 1)) return rc;
         } else {
             level_smooth(a, l, seg(z, f), seg(a->t, f), seg(a->q, f), f.nu2);
@@ -978,6 +1033,7 @@ int ela_amr_create_boxes(ela_ctx* ctx, const ela_domain* dom, int nlev, const in
     for (int l = 0; l < nlev && !rc; ++l) {
         CLev& v = a->lev[l];
         rc = aalloc_t(a, (size_t)a->nc * v.L.cs, &v.C);
+        if (!rc) rc = aalloc_t(a, 1, &v.dbcG);
         if (!rc) rc = aalloc_t(a, masks[l].size(), &v.cls);
         if (!rc && hipMemcpyAsync(v.cls, masks[l].data(), masks[l].size(), hipMemcpyHostToDevice, a->stream) != hipSuccess)
             rc = afail(ELA_ERR_HIP, "mask upload failed");
@@ -1083,6 +1139,8 @@ int ela_amr_set_bc(ela_amr* a, const int* types, const double* values) {
                 bt[3 * f + c] = cf ? ELA_BC_DISP : a->bctypes[f][c];
             }
         }
+        if (v.dbcG && hipMemcpyAsync(v.dbcG, &v.bcG, sizeof v.bcG, hipMemcpyHostToDevice, a->stream) != hipSuccess)
+            return afail(ELA_ERR_HIP, "hipMemcpyAsync failed");
         if (v.H)
             if (int rc = ela_set_bc(v.H, bt.data(), nullptr)) return rc;
     }

@@ -522,7 +522,34 @@ def pmc_traffic(fetch_csv, write_csv, n, tile, out_path):
     print(json.dumps(out, indent=1))
 
 
+def trace_summary(trace_csv, top=40):
+    """Markdown table of a `rocprofv3 --kernel-trace` run's kernel_trace.csv grouped by kernel and grid size
+    (a level's launches are told apart by their grid): calls, total ms, average us, share."""
+    import csv
+    acc = {}
+    with open(trace_csv) as fh:
+        for r in csv.DictReader(fh):
+            name = r["Kernel_Name"].replace("void ", "").replace("ela::(anonymous namespace)::", "").replace("ela::", "")
+            if not name.startswith("__amd"):
+                name = name.split("(")[0]
+            grid = r.get("Grid_Size_X") or r.get("Grid_Size") or "?"
+            d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
+            e = acc.setdefault((name, grid), [0, 0.0])
+            e[0] += 1
+            e[1] += d
+    total = sum(v[1] for v in acc.values())
+    print("Total kernel time %.1f ms over %d kernel/grid pairs.\n" % (total * 1e-3, len(acc)))
+    print("| kernel | grid threads | calls | total ms | avg us | % |")
+    print("|---|---|---|---|---|---|")
+    for (name, grid), (n, t) in sorted(acc.items(), key=lambda kv: -kv[1][1])[:top]:
+        print("| `%s` | %s | %d | %.2f | %.1f | %.1f |" % (name, grid, n, t * 1e-3, t / n, 100.0 * t / total))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--trace-summary":
+        # python bench.py --trace-summary KERNEL_TRACE.csv [TOP]
+        trace_summary(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 40)
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "--pmc-traffic":
         # python bench.py --pmc-traffic FETCH.csv WRITE.csv N TILE OUT.json
         pmc_traffic(sys.argv[2], sys.argv[3], int(sys.argv[4]), sys.argv[5], sys.argv[6])
