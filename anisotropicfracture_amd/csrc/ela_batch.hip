@@ -71,6 +71,7 @@ struct BoxDesc {
     // nodes marching kz planes in z, plus the boundary shell in blocks of TB.
     int mt0, ntx, nty, ntz;  // first marching block of the box, tiles per direction
     int kz, st0, nsh, ngr;   // planes per tile, first shell block, shell blocks, grains
+    int fp0, nfp;            // first x-face patch block of the box, its patches (face_patch)
     Inv<double> i64;
     Inv<float> i32;
     double h[3];
@@ -779,7 +780,7 @@ __device__ __forceinline__ GrainBox<T> grain_of(const BoxDesc& d, const MOps& op
     g.ry = 1;
     g.pf = 1;
     g.nt = false;
-    g.xface = false;  // the batched shells compute the x faces
+    g.xface = true;  // the x faces run in face patches (kf_grain)
     g.in = MODE == 2 ? (const T*)d.u64 : (const T*)d.f[op.in];
     g.rhs = MODE == 2 ? (const T*)d.b64 : (const T*)d.f[op.rhs];
     g.out = d.f[op.out];
@@ -800,7 +801,7 @@ __device__ __forceinline__ void shell_block(const BoxDesc* __restrict__ D, const
     int q = (b - d.st0) * TB + (int)threadIdx.x;
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const int A = nx * ny, Bf = nx * (nz - 2), Cf = (ny - 2) * (nz - 2);
-    const bool in = q < 2 * A + 2 * Bf + 2 * Cf;
+    const bool in = q < 2 * A + 2 * Bf;  // the x faces (2 Cf nodes) run in face patches
     int i = 0, j = 0, k = 0;
     if (in) {
         if (q < 2 * A) {
@@ -848,35 +849,53 @@ __device__ __forceinline__ void shell_block(const BoxDesc* __restrict__ D, const
     if (MODE == 2) block_max(op.partials + op.mtot, m);
 }
 
-// One launch per grain-form level: blocks [0, nsp) compute the shells (the
-// last nsp - ns are padding), the others march the tiles.  The shells run
-// beside the tiles instead of after them; nsp is a multiple of 8, so a tile's
-// hardware block lands on the XCD that xcd_block assigns it.
+// One launch per grain-form level: blocks [0, nsp) compute the shells' z and
+// y faces (the last nsp - ns are padding), blocks [nsp, nsp + nfp) the x
+// faces in LDS patches (face_patch: a face's nodes lie a row apart, so the
+// patch stages the face and its neighbour column once; the last nfp - nf are
+// padding), the others march the tiles.  The shells run beside the tiles
+// instead of after them; nsp and nfp are multiples of 8, so a tile's hardware
+// block lands on the XCD that xcd_block assigns it.
 template <typename T, int MODE, typename TO, bool SC>
 __global__ __launch_bounds__(MTX* MTY) void kf_grain(const BoxDesc* __restrict__ D, const int* __restrict__ bmap_m,
-                                                     const int* __restrict__ bmap_s, int ns, int nsp, MOps op) {
+                                                     const int* __restrict__ bmap_s, const int* __restrict__ bmap_f,
+                                                     int ns, int nsp, int nf, int nfp, MOps op) {
     const int b = (int)blockIdx.x;
     if (b < nsp) {
         if (b < ns) shell_block<T, MODE, TO>(D, bmap_s, b, op);
         return;
     }
     __shared__ MarchLds<T, SC> sh;
+    if (b < nsp + nfp) {
+        double m = 0.0;
+        const int pf = b - nsp;
+        if (pf < nf) {
+            const BoxDesc& d = D[bmap_f[pf]];
+            m = face_patch<T, MODE, TO>(grain_of<T, MODE>(d, op), pf - d.fp0, sh.su);
+        }
+        if (MODE == 2) block_max(op.partials + op.mtot, m);
+        return;
+    }
     extern __shared__ __align__(16) unsigned char dyn_lds[];
-    const int nm = (int)gridDim.x - nsp, bm = b - nsp;
+    const int nm = (int)gridDim.x - nsp - nfp, bm = b - nsp - nfp;
     const int lb = op.xcd ? xcd_block(bm, nm) : bm;
     const BoxDesc& d = D[bmap_m[lb]];
     const double m = march_tile<T, MODE, TO, SC>(grain_of<T, MODE>(d, op), lb - d.mt0, sh, (T*)dyn_lds);
     if (MODE == 2) block_max_at(op.partials, lb, m);
 }
 
-// |r|inf of each box from the partials of its marching and shell blocks.
+// |r|inf of each box from the partials of its marching, shell and x-face
+// patch blocks (the shells' slots follow the tiles', the patches' the padded
+// shells').
 __global__ __launch_bounds__(64) void kb_fold_m(const BoxDesc* __restrict__ D, const double* __restrict__ partials,
-                                                int mtot, double* __restrict__ out) {
+                                                int mtot, int nsp, double* __restrict__ out) {
     const BoxDesc& d = D[blockIdx.x];
     const int nm = d.ntx * d.nty * d.ntz;
     double m = 0.0;
-    for (int q = threadIdx.x; q < nm + d.nsh; q += 64) {
-        const double v = q < nm ? partials[d.mt0 + q] : partials[mtot + d.st0 + q - nm];
+    for (int q = threadIdx.x; q < nm + d.nsh + d.nfp; q += 64) {
+        const double v = q < nm ? partials[d.mt0 + q]
+                         : q < nm + d.nsh ? partials[mtot + d.st0 + q - nm]
+                                          : partials[mtot + nsp + d.fp0 + q - nm - d.nsh];
         m = (v > m || v != v) ? v : m;
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -1126,8 +1145,8 @@ struct ela_batch {
     std::vector<int> top_tiles;
     // marching launches over the grain-form boxes of each level and of the top list
     struct March {
-        size_t off_bm = 0, off_bs = 0;  // device offsets of the marching / shell block maps
-        int nm = 0, ns = 0, maxng = 0;  // marching blocks, shell blocks, largest grain count
+        size_t off_bm = 0, off_bs = 0, off_bf = 0;  // device offsets of the marching / shell / face block maps
+        int nm = 0, ns = 0, nf = 0, maxng = 0;      // marching, shell and x-face patch blocks, largest grain count
         bool scaled = false;            // some box of the launch carries a per-node factor
         bool fits = true;               // every box's fields within 32-bit offsets (grain_offsets_fit)
     };
@@ -1311,22 +1330,28 @@ void pack_list(ela_batch* B, const ListBuild& lb, size_t* off_desc, size_t* off_
         for (int t = lb.tile[b]; t < lb.tile[b + 1]; ++t) bm[t] = (int)b;
 }
 
-// Marching numbering of the grain-form boxes [a, b) of a list: interior tiles
-// and shell blocks per box, and the block -> box maps of both launches.
+// Marching numbering of the grain-form boxes [a, b) of a list: interior tiles,
+// shell blocks (z and y faces) and x-face patches per box, and the block ->
+// box maps of the three parts of the launch.
 void march_number(ListBuild& lb, int a, int b, int kz, std::vector<int>& bm, std::vector<int>& bs,
-                  ela_batch::March& mi) {
+                  std::vector<int>& bf, ela_batch::March& mi) {
     mi = ela_batch::March{};
     bm.clear();
     bs.clear();
+    bf.clear();
     for (int x = a; x < b; ++x) {
         BoxDesc& d = lb.descs[x];
         const Lay& L = d.L;
         d.kz = kz;
-        grain_tiles(L, kz, d.ntx, d.nty, d.ntz, d.nsh);
+        grain_tiles(L, kz, d.ntx, d.nty, d.ntz, d.nsh, /*xface=*/true);
         d.mt0 = mi.nm;
         mi.nm += d.ntx * d.nty * d.ntz;
         d.st0 = mi.ns;
         mi.ns += d.nsh;
+        d.nfp = grain_face_patches(L);
+        d.fp0 = mi.nf;
+        mi.nf += d.nfp;
+        bf.insert(bf.end(), (size_t)d.nfp, x);
         mi.maxng = std::max(mi.maxng, d.ngr);
         mi.scaled = mi.scaled || d.sc64 != nullptr;
         mi.fits = mi.fits && grain_offsets_fit(L);
@@ -1357,7 +1382,7 @@ int build_lists(ela_batch* B) {
     B->off_bmap.assign(G, 0);
     B->hlists.clear();
     B->mlev.assign(G, ela_batch::March{});
-    std::vector<int> bm, bs;
+    std::vector<int> bm, bs, bf;
     for (int g = 0; g < G; ++g) {
         ListBuild lb;
         for (auto* r : B->active) {
@@ -1366,10 +1391,11 @@ int build_lists(ela_batch* B) {
         }
         B->cnt[g] = (int)lb.descs.size();
         B->tiles[g] = lb.tile.empty() ? std::vector<int>{0} : lb.tile;
-        march_number(lb, g > 0 ? B->cnt[g - 1] : 0, B->cnt[g], B->kz, bm, bs, B->mlev[g]);
+        march_number(lb, g > 0 ? B->cnt[g - 1] : 0, B->cnt[g], B->kz, bm, bs, bf, B->mlev[g]);
         pack_list(B, lb, &B->off_desc[g], &B->off_bmap[g]);
         B->mlev[g].off_bm = pack_ints(B, bm);
         B->mlev[g].off_bs = pack_ints(B, bs);
+        B->mlev[g].off_bf = pack_ints(B, bf);
     }
     ListBuild bl;
     B->bpre.assign(1, 0);
@@ -1385,14 +1411,16 @@ int build_lists(ela_batch* B) {
     ListBuild top;
     for (auto* r : B->active) list_push(top, desc_of(r, 0));
     B->top_tiles = top.tile.empty() ? std::vector<int>{0} : top.tile;
-    march_number(top, 0, (int)top.descs.size(), B->kz, bm, bs, B->mtop);
+    march_number(top, 0, (int)top.descs.size(), B->kz, bm, bs, bf, B->mtop);
     pack_list(B, top, &B->off_top_desc, &B->off_top_bmap);
     B->mtop.off_bm = pack_ints(B, bm);
     B->mtop.off_bs = pack_ints(B, bs);
+    B->mtop.off_bf = pack_ints(B, bf);
     if (int rc = ensure_device(B, (void**)&B->dlists, &B->dlists_cap, B->hlists.size())) return rc;
     if (!B->hlists.empty())
         ELA_HIPCHK(hipMemcpyAsync(B->dlists, B->hlists.data(), B->hlists.size(), hipMemcpyHostToDevice, B->stream));
-    const size_t nblk_top = std::max({(size_t)B->top_tiles.back(), (size_t)(B->mtop.nm + B->mtop.ns),
+    const size_t nblk_top = std::max({(size_t)B->top_tiles.back(),
+                                      (size_t)B->mtop.nm + (B->mtop.ns + 7) / 8 * 8 + (B->mtop.nf + 7) / 8 * 8,
                                       6 * (size_t)B->btiles.back()});
     if (int rc = ensure_device(B, (void**)&B->partials, &B->partials_cap, nblk_top * sizeof(double))) return rc;
     const size_t nbc = B->btiles.size() - 1;
@@ -1447,15 +1475,16 @@ bool use_march(const ela_batch* B, const ela_batch::March& mi) {
 template <typename T, int MODE, typename TO>
 void march_launch(ela_batch* B, const BoxDesc* D, const ela_batch::March& mi, const MOps& op) {
     const size_t lds = (size_t)mi.maxng * kNC * sizeof(T);
-    const int nsp = (mi.ns + 7) / 8 * 8;
+    const int nsp = (mi.ns + 7) / 8 * 8, nfp = (mi.nf + 7) / 8 * 8;
     const int* bm = (const int*)(B->dlists + mi.off_bm);
     const int* bs = (const int*)(B->dlists + mi.off_bs);
+    const int* bf = (const int*)(B->dlists + mi.off_bf);
     if (mi.scaled)
-        hipLaunchKernelGGL((kf_grain<T, MODE, TO, true>), dim3(nsp + mi.nm), dim3(MTX * MTY), lds, B->stream, D, bm,
-                           bs, mi.ns, nsp, op);
+        hipLaunchKernelGGL((kf_grain<T, MODE, TO, true>), dim3(nsp + nfp + mi.nm), dim3(MTX * MTY), lds, B->stream, D,
+                           bm, bs, bf, mi.ns, nsp, mi.nf, nfp, op);
     else
-        hipLaunchKernelGGL((kf_grain<T, MODE, TO, false>), dim3(nsp + mi.nm), dim3(MTX * MTY), lds, B->stream, D, bm,
-                           bs, mi.ns, nsp, op);
+        hipLaunchKernelGGL((kf_grain<T, MODE, TO, false>), dim3(nsp + nfp + mi.nm), dim3(MTX * MTY), lds, B->stream,
+                           D, bm, bs, bf, mi.ns, nsp, mi.nf, nfp, op);
 }
 
 template <typename T>
@@ -1740,7 +1769,8 @@ int step_t(ela_batch* B) {
     hipLaunchKernelGGL(kb_scale, dim3((nbox + 63) / 64), dim3(64), 0, B->stream, TD(B), nbox);
     if (use_march(B, B->mtop)) {
         march_launch<double, 2, T>(B, TD(B), B->mtop, MOps{0, 0, F_RES, 0.0, B->partials, B->mtop.nm, B->xcd});
-        hipLaunchKernelGGL(kb_fold_m, dim3(nbox), dim3(64), 0, B->stream, TD(B), B->partials, B->mtop.nm, B->dnorms);
+        hipLaunchKernelGGL(kb_fold_m, dim3(nbox), dim3(64), 0, B->stream, TD(B), B->partials, B->mtop.nm,
+                           (B->mtop.ns + 7) / 8 * 8, B->dnorms);
     } else {
         resid64_top<T>(B, TD(B), TM(B), nbox, nblk);
     }
@@ -2553,15 +2583,16 @@ int ela_batch_autotune(ela_batch* B, int count, ela_rve* const* rves, int nrep, 
         // the finest levels of the members, numbered for this tile height
         ListBuild lb;
         for (int m = 0; m < count; ++m) list_push(lb, desc_of(rves[m], 0));
-        std::vector<int> bm, bs;
+        std::vector<int> bm, bs, bf;
         ela_batch::March mi;
-        march_number(lb, 0, count, kzs[a], bm, bs, mi);
+        march_number(lb, 0, count, kzs[a], bm, bs, bf, mi);
         std::vector<char> save;
         save.swap(B->hlists);
         size_t od = 0, om = 0;
         pack_list(B, lb, &od, &om);
         mi.off_bm = pack_ints(B, bm);
         mi.off_bs = pack_ints(B, bs);
+        mi.off_bf = pack_ints(B, bf);
         std::vector<char> bytes;
         bytes.swap(B->hlists);
         B->hlists.swap(save);
