@@ -54,7 +54,7 @@ constexpr int kMaxBatchDense = 1536;   // unknowns of a dense coarsest level (n_
 constexpr double kBottomTol = 1e-4;    // BiCGStab coarsest solve: |r|inf <= 1e-4 |b|inf
 constexpr int kBottomMaxIter = 200;    //   or this many iterations
 constexpr int kBottomCheck = 4;        // iterations between the host's looks at the members' states
-enum { BV_R = 0, BV_RH, BV_P, BV_V, BV_S, BV_T, BV_N };
+enum { BV_R = 0, BV_RH, BV_P, BV_V, BV_S, BV_T, BV_PH, BV_SH, BV_N };  // BV_PH, BV_SH: D^-1 p, D^-1 s
 // Per-member BiCGStab scalars (FP64, on the device).
 enum { BS_RHO = 0, BS_RHO1, BS_ALPHA, BS_OMEGA, BS_BETA, BS_TOL, BS_IT, BS_STATE, BS_N = 8 };
 constexpr int kNC = kGrainNC;          // cubic stiffness: major/minor symmetric storage
@@ -376,8 +376,11 @@ __global__ __launch_bounds__(TB) void kb_bottom(const BoxDesc* __restrict__ D, i
 // ---------------------------------------------------------- BiCGStab coarsest solve
 // The coarsest levels beyond the dense inverse solve A x = b by BiCGStab (the
 // single-box solver's bottom, the reference's bottom solver: Linear.H:23-29
-// with MLMG's bottom tolerance 1e-4): vectors in the cycle precision, dot
-// products and norms in FP64.  Launches go over the list of such boxes, five
+// with MLMG's bottom tolerance 1e-4), right-preconditioned by the level's
+// Jacobi diagonal D: A is applied to D^-1 p and D^-1 s and x gathers those, as
+// the reference's operators scale them through their normalize hook
+// (Operator.cpp:164-195).  Vectors in the cycle precision, dot products and
+// norms in FP64.  Launches go over the list of such boxes, five
 // per iteration.  Each block writes three partials (two sums and a maximum);
 // the next launch that needs the scalar they define folds them again in
 // every block of the member (its own blocks' partials, in a fixed order, so
@@ -474,6 +477,13 @@ __device__ __forceinline__ void scal_store(const BoxDesc& d, int blk, int ph, co
     if (state) state[box] = (int)S.v[BS_STATE];
 }
 
+// z = D^-1 y at element e of the level's layout (FP32 levels hold the inverse diagonal).
+template <typename T>
+__device__ __forceinline__ T dinv_apply(const BoxDesc& d, long long e, T y) {
+    if constexpr (sizeof(T) == 4) return y * P<T>::dg(d)[e];
+    else return y / P<T>::dg(d)[e];
+}
+
 // x = f[dst] = 0, p = v = 0, r = rh = f[src]; partials rh.r and |r|inf into P.
 template <typename T>
 __global__ __launch_bounds__(TB) void kbb_init(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, int src,
@@ -501,7 +511,7 @@ __global__ __launch_bounds__(TB) void kbb_init(const BoxDesc* __restrict__ D, co
 // Iteration start.  first: rho = rh.r and the tolerance from |b|inf (a zero
 // right-hand side gives x = 0); else rho = rh.r of the last update, stop on
 // |r| small, the iteration limit or a breakdown of rho, beta.  Then
-// p = r + beta (p - omega v).
+// p = r + beta (p - omega v) and p^ = D^-1 p.
 template <typename T>
 __global__ __launch_bounds__(TB) void kbb_p(const BoxDesc* __restrict__ D, const int* __restrict__ bmap,
                                             const double* __restrict__ P, int ph, int first, int* state) {
@@ -538,14 +548,17 @@ __global__ __launch_bounds__(TB) void kbb_p(const BoxDesc* __restrict__ D, const
     const T beta = (T)S.v[BS_BETA], omega = (T)S.v[BS_OMEGA];
     const T *r = (const T*)d.bv[BV_R], *v = (const T*)d.bv[BV_V];
     T* p = (T*)d.bv[BV_P];
+    T* ph_ = (T*)d.bv[BV_PH];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const long long e = c * d.L.cs + qn;
-        p[e] = r[e] + beta * (p[e] - omega * v[e]);
+        const T pe = r[e] + beta * (p[e] - omega * v[e]);
+        p[e] = pe;
+        ph_[e] = dinv_apply<T>(d, e, pe);
     }
 }
 
-// v = A p, partial rh.v into Pout.
+// v = A p^, partial rh.v into Pout.
 template <typename T>
 __global__ __launch_bounds__(TB) void kbb_av(const BoxDesc* __restrict__ D, const int* __restrict__ bmap,
                                              double* __restrict__ Pout, int ph) {
@@ -559,7 +572,7 @@ __global__ __launch_bounds__(TB) void kbb_av(const BoxDesc* __restrict__ D, cons
     long long o;
     double s0 = 0.0;
     if (box_node(d, L, blockIdx.x, i, j, k, o)) {
-        const T* in = (const T*)d.bv[BV_P];
+        const T* in = (const T*)d.bv[BV_PH];
         const T* rh = (const T*)d.bv[BV_RH];
         T* out = (T*)d.bv[BV_V];
         T f[3];
@@ -575,7 +588,7 @@ __global__ __launch_bounds__(TB) void kbb_av(const BoxDesc* __restrict__ D, cons
 }
 
 // alpha = rho / rh.v (a breakdown ends the member without an update);
-// s = r - alpha v, partial |s|inf into Pout.
+// s = r - alpha v and s^ = D^-1 s, partial |s|inf into Pout.
 template <typename T>
 __global__ __launch_bounds__(TB) void kbb_s(const BoxDesc* __restrict__ D, const int* __restrict__ bmap,
                                             const double* __restrict__ Pin, double* __restrict__ Pout, int ph) {
@@ -598,18 +611,20 @@ __global__ __launch_bounds__(TB) void kbb_s(const BoxDesc* __restrict__ D, const
         const T alpha = (T)S.v[BS_ALPHA];
         const T *r = (const T*)d.bv[BV_R], *v = (const T*)d.bv[BV_V];
         T* sv = (T*)d.bv[BV_S];
+        T* sh_ = (T*)d.bv[BV_SH];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const long long e = c * d.L.cs + qn;
             const T x = r[e] - alpha * v[e];
             sv[e] = x;
+            sh_[e] = dinv_apply<T>(d, e, x);
             m = amax(m, (double)x);
         }
     }
     block_sum2_max(Pout, blockIdx.x, 0.0, 0.0, m);
 }
 
-// |s| small: the member ends after x += alpha p (state 2); else t = A s,
+// |s| small: the member ends after x += alpha p^ (state 2); else t = A s^,
 // partials t.t and t.s into Pout.
 template <typename T>
 __global__ __launch_bounds__(TB) void kbb_at(const BoxDesc* __restrict__ D, const int* __restrict__ bmap,
@@ -634,7 +649,8 @@ __global__ __launch_bounds__(TB) void kbb_at(const BoxDesc* __restrict__ D, cons
     long long o;
     double s0 = 0.0, s1 = 0.0;
     if (box_node(d, L, blockIdx.x, i, j, k, o)) {
-        const T* in = (const T*)d.bv[BV_S];
+        const T* in = (const T*)d.bv[BV_SH];
+        const T* sv = (const T*)d.bv[BV_S];
         T* out = (T*)d.bv[BV_T];
         T f[3];
         box_apply<T, -1>(d, L, in, i, j, k, f);
@@ -643,15 +659,15 @@ __global__ __launch_bounds__(TB) void kbb_at(const BoxDesc* __restrict__ D, cons
             const long long e = c * L.cs + o;
             out[e] = f[c];
             s0 += (double)f[c] * (double)f[c];
-            s1 += (double)f[c] * (double)in[e];
+            s1 += (double)f[c] * (double)sv[e];
         }
     }
     block_sum2_max(Pout, blockIdx.x, s0, s1, 0.0);
 }
 
-// State 2: x += alpha p, done.  State 1: omega = t.s / t.t (a breakdown: x +=
-// alpha p, done), x += alpha p + omega s, r = s - omega t, partials rh.r and
-// |r|inf into Pout.
+// State 2: x += alpha p^, done.  State 1: omega = t.s / t.t (a breakdown: x +=
+// alpha p^, done), x += alpha p^ + omega s^, r = s - omega t, partials rh.r
+// and |r|inf into Pout.
 template <typename T>
 __global__ __launch_bounds__(TB) void kbb_x(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, int dst,
                                             const double* __restrict__ Pin, double* __restrict__ Pout, int ph) {
@@ -680,7 +696,7 @@ __global__ __launch_bounds__(TB) void kbb_x(const BoxDesc* __restrict__ D, const
     const int qn = ((int)blockIdx.x - d.tile0) * TB + (int)threadIdx.x;
     const T alpha = (T)S.v[BS_ALPHA], omega = (T)S.v[BS_OMEGA];
     T* x = (T*)d.f[dst];
-    const T* p = (const T*)d.bv[BV_P];
+    const T* p = (const T*)d.bv[BV_PH];
     if (only_p) {
         if (qn < d.nodes)
 #pragma unroll
@@ -693,11 +709,12 @@ __global__ __launch_bounds__(TB) void kbb_x(const BoxDesc* __restrict__ D, const
     double s0 = 0.0, m = 0.0;
     if (qn < d.nodes) {
         const T *sv = (const T*)d.bv[BV_S], *t = (const T*)d.bv[BV_T], *rh = (const T*)d.bv[BV_RH];
+        const T* sh_ = (const T*)d.bv[BV_SH];
         T* r = (T*)d.bv[BV_R];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const long long e = c * d.L.cs + qn;
-            x[e] += alpha * p[e] + omega * sv[e];
+            x[e] += alpha * p[e] + omega * sh_[e];
             const T rr = sv[e] - omega * t[e];
             r[e] = rr;
             s0 += (double)rh[e] * (double)rr;

@@ -1553,19 +1553,38 @@ __global__ __launch_bounds__(256) void k_bicg_fold(const double* __restrict__ pa
 // The vector updates of one BiCGStab step (coefficients from the host):
 // op 0: p = r + beta (p - omega v);  op 1: s = r - alpha v;
 // op 2: x += alpha p + omega s, r = s - omega t;  op 3: x += alpha p.
+// BiCGStab vector steps, right-preconditioned by the Jacobi diagonal dg of the
+// level (inv: dg holds the inverse, FP32 levels): op 0 p = r + beta (p -
+// omega v) and p^ = D^-1 p; op 1 s = r - alpha v and s^ = D^-1 s; op 2 x +=
+// alpha p^ + omega s^, r = s - omega t; op 3 x += alpha p^.  Elements of the
+// layout's padding get p^ = s^ = 0.
+template <typename T>
+__device__ __forceinline__ T bicg_precond(const Lay& L, long long q, const T* __restrict__ dg, int inv, T y) {
+    const long long w = q % L.cs;
+    const long long k = w / L.pl, rr = w - k * L.pl, j = rr / L.px, i = rr - j * L.px;
+    if (i >= L.nx || j >= L.ny || k >= L.nz) return T(0);
+    return inv ? (T)(y * dg[q]) : (T)(y / dg[q]);
+}
 template <typename T>
 __global__ __launch_bounds__(256) void k_bicg_update(long long n, int op, double alpha, double beta, double omega,
                                                      T* __restrict__ x, T* __restrict__ r, T* __restrict__ p,
                                                      const T* __restrict__ v, T* __restrict__ sv,
-                                                     const T* __restrict__ t) {
+                                                     const T* __restrict__ t, const T* __restrict__ dg, int inv,
+                                                     T* __restrict__ ph, T* __restrict__ sh, Lay L) {
     for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += (long long)gridDim.x * 256) {
-        if (op == 0) p[q] = (T)((double)r[q] + beta * ((double)p[q] - omega * (double)v[q]));
-        else if (op == 1) sv[q] = (T)((double)r[q] - alpha * (double)v[q]);
-        else if (op == 2) {
-            x[q] = (T)((double)x[q] + alpha * (double)p[q] + omega * (double)sv[q]);
+        if (op == 0) {
+            const T pq = (T)((double)r[q] + beta * ((double)p[q] - omega * (double)v[q]));
+            p[q] = pq;
+            ph[q] = bicg_precond(L, q, dg, inv, pq);
+        } else if (op == 1) {
+            const T sq = (T)((double)r[q] - alpha * (double)v[q]);
+            sv[q] = sq;
+            sh[q] = bicg_precond(L, q, dg, inv, sq);
+        } else if (op == 2) {
+            x[q] = (T)((double)x[q] + alpha * (double)ph[q] + omega * (double)sh[q]);
             r[q] = (T)((double)sv[q] - omega * (double)t[q]);
         } else {
-            x[q] = (T)((double)x[q] + alpha * (double)p[q]);
+            x[q] = (T)((double)x[q] + alpha * (double)ph[q]);
         }
     }
 }
@@ -1583,9 +1602,10 @@ void launch_bicg_reduce(long long n, const T* a, const T* b, const T* c, const T
 
 template <typename T>
 void launch_bicg_update(long long n, int op, double alpha, double beta, double omega, T* x, T* r, T* p, const T* v,
-                        T* sv, const T* t, hipStream_t s) {
+                        T* sv, const T* t, const T* dg, T* ph, T* sh, const Lay& L, hipStream_t s) {
     const int nb = (int)std::min<long long>(4096, (n + 255) / 256);
-    hipLaunchKernelGGL(k_bicg_update<T>, dim3(nb), dim3(256), 0, s, n, op, alpha, beta, omega, x, r, p, v, sv, t);
+    hipLaunchKernelGGL(k_bicg_update<T>, dim3(nb), dim3(256), 0, s, n, op, alpha, beta, omega, x, r, p, v, sv, t, dg,
+                       sizeof(T) == 4 ? 1 : 0, ph, sh, L);
 }
 
 template void launch_bicg_reduce<double>(long long, const double*, const double*, const double*, const double*,
@@ -1593,9 +1613,11 @@ template void launch_bicg_reduce<double>(long long, const double*, const double*
 template void launch_bicg_reduce<float>(long long, const float*, const float*, const float*, const float*,
                                         const float*, double*, double*, hipStream_t);
 template void launch_bicg_update<double>(long long, int, double, double, double, double*, double*, double*,
-                                         const double*, double*, const double*, hipStream_t);
+                                         const double*, double*, const double*, const double*, double*, double*,
+                                         const Lay&, hipStream_t);
 template void launch_bicg_update<float>(long long, int, double, double, double, float*, float*, float*,
-                                        const float*, float*, const float*, hipStream_t);
+                                        const float*, float*, const float*, const float*, float*, float*, const Lay&,
+                                        hipStream_t);
 
 void launch_eta_check(const Lay& L, const EtaSrc& e, int c0, int c1, int* flags, hipStream_t s) {
     if (L.dim == 3) hipLaunchKernelGGL(k_eta_check<3>, node_grid(L), dim3(BX, BY), 0, s, L, e, c0, c1, flags);

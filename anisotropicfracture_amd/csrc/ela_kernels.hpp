@@ -176,7 +176,7 @@ void launch_bicg_reduce(long long n, const T* a, const T* b, const T* c, const T
                         double* out, hipStream_t s);
 template <typename T>
 void launch_bicg_update(long long n, int op, double alpha, double beta, double omega, T* x, T* r, T* p, const T* v,
-                        T* sv, const T* t, hipStream_t s);
+                        T* sv, const T* t, const T* dg, T* ph, T* sh, const Lay& L, hipStream_t s);
 long long bicg_partials();
 
 // Flat FP64 vector kernels of the Krylov accelerator (whole fields incl. zero padding).

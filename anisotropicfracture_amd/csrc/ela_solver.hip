@@ -176,8 +176,8 @@ struct ela_hier {
     unsigned long long* hnorm = nullptr;  // pinned
     // BiCGStab bottom solve of a coarsest level too large for the dense inverse:
     // r, r-hat, p, v, s, t on the coarsest layout per precision, reduction scratch
-    double* bv64[6] = {};
-    float* bv32[6] = {};
+    double* bv64[8] = {};  // coarsest BiCGStab: r, rh, p, v, s, t, D^-1 p, D^-1 s
+    float* bv32[8] = {};
     double *bpart = nullptr, *bout = nullptr, *bhost = nullptr;  // bhost: pinned
     int bottom_iters = 0;                                           // BiCGStab steps of the last solve
     // grain-table stiffness of level 0 (Voronoi models): per-node grain id, per-grain records
@@ -321,8 +321,11 @@ void res_of_cor(ela_hier* h, int l) {
 // BiCGStab of a coarsest level without a dense inverse, from a zero correction
 // to |r|inf <= kBottomTol |res|inf or kBottomMaxIter steps: AMReX's bottom
 // solver (MLCGSolver's BiCGStab, the bottom Linear selects, Linear.H:23-29,
-// with MLMG's bottom tolerance 1e-4).  Vectors in the cycle precision, dot
-// products and norms in FP64; the coefficients go through the host.
+// with MLMG's bottom tolerance 1e-4), right-preconditioned by the level's
+// Jacobi diagonal (A applied to D^-1 p and D^-1 s, as the reference's
+// operators scale them through normalize, Operator.cpp:164-195).  Vectors in
+// the cycle precision, dot products and norms in FP64; the coefficients go
+// through the host.
 constexpr double kBottomTol = 1e-4;
 constexpr int kBottomMaxIter = 200;
 template <typename T>
@@ -339,7 +342,8 @@ void bicg_bottom(ela_hier* h) {
     T* x = *lbv.cor;
     const T* b = *lbv.res;
     T* const* W = bicg_vecs<T>(h);
-    T *r = W[0], *rh = W[1], *p = W[2], *vv = W[3], *sv = W[4], *t = W[5];
+    T *r = W[0], *rh = W[1], *p = W[2], *vv = W[3], *sv = W[4], *t = W[5], *ph = W[6], *sh = W[7];
+    const T* dg = lbv.dg;  // right preconditioner: the level's Jacobi diagonal (FP32: its inverse)
     const long long n = (long long)field_elems(v, h->dim);
     hipStream_t st = h->stream;
     auto reduce = [&](const T* a, const T* bb, const T* c, const T* d, const T* e, double q[3]) {
@@ -364,27 +368,27 @@ void bicg_bottom(ela_hier* h) {
     for (; it < kBottomMaxIter; ++it) {
         if (!(rho != 0.0) || !std::isfinite(rho)) break;
         const double beta = it == 0 ? 0.0 : (rho / rho1) * (alpha / omega);
-        launch_bicg_update<T>(n, 0, 0.0, beta, omega, x, r, p, vv, sv, t, st);
-        A_apply<T>(h, v, lbv.C, p, vv, st);
+        launch_bicg_update<T>(n, 0, 0.0, beta, omega, x, r, p, vv, sv, t, dg, ph, sh, v.L, st);
+        A_apply<T>(h, v, lbv.C, ph, vv, st);
         reduce(rh, vv, nullptr, nullptr, nullptr, q);
         if (!(q[0] != 0.0) || !std::isfinite(q[0])) break;
         alpha = rho / q[0];
-        launch_bicg_update<T>(n, 1, alpha, 0.0, 0.0, x, r, p, vv, sv, t, st);
+        launch_bicg_update<T>(n, 1, alpha, 0.0, 0.0, x, r, p, vv, sv, t, dg, ph, sh, v.L, st);
         reduce(nullptr, nullptr, nullptr, nullptr, sv, q);
         if (q[2] <= tol) {
-            launch_bicg_update<T>(n, 3, alpha, 0.0, 0.0, x, r, p, vv, sv, t, st);
+            launch_bicg_update<T>(n, 3, alpha, 0.0, 0.0, x, r, p, vv, sv, t, dg, ph, sh, v.L, st);
             ++it;
             break;
         }
-        A_apply<T>(h, v, lbv.C, sv, t, st);
+        A_apply<T>(h, v, lbv.C, sh, t, st);
         reduce(t, t, t, sv, nullptr, q);
         omega = q[0] > 0.0 ? q[1] / q[0] : 0.0;
         if (!(omega != 0.0) || !std::isfinite(omega)) {
-            launch_bicg_update<T>(n, 3, alpha, 0.0, 0.0, x, r, p, vv, sv, t, st);
+            launch_bicg_update<T>(n, 3, alpha, 0.0, 0.0, x, r, p, vv, sv, t, dg, ph, sh, v.L, st);
             ++it;
             break;
         }
-        launch_bicg_update<T>(n, 2, alpha, 0.0, omega, x, r, p, vv, sv, t, st);
+        launch_bicg_update<T>(n, 2, alpha, 0.0, omega, x, r, p, vv, sv, t, dg, ph, sh, v.L, st);
         reduce(rh, r, nullptr, nullptr, r, q);
         rho1 = rho;
         rho = q[0];
@@ -1391,7 +1395,7 @@ int ela_hier_create_ex(ela_ctx* ctx, const ela_domain* dom, int model_kind, int 
     if (!rc && h->nunk <= kMaxDenseUnknowns) rc = halloc_t(h, (size_t)h->nunk * 2 * h->nunk, &h->M);
     if (!rc && !h->M) {  // the coarsest level is solved by BiCGStab
         const size_t fe = field_elems(h->lev.back(), h->dim);
-        for (int k = 0; k < 6 && !rc; ++k) {
+        for (int k = 0; k < 8 && !rc; ++k) {
             if (h->has64) rc = halloc_t(h, fe, &h->bv64[k]);
             if (!rc && h->has32) rc = halloc_t(h, fe, &h->bv32[k]);
         }

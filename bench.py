@@ -522,9 +522,9 @@ def pmc_traffic(fetch_csv, write_csv, n, tile, out_path):
     print(json.dumps(out, indent=1))
 
 
-def trace_summary(trace_csv, top=40):
+def trace_summary(trace_csv, top=40, by_grid=True):
     """Markdown table of a `rocprofv3 --kernel-trace` run's kernel_trace.csv grouped by kernel and grid size
-    (a level's launches are told apart by their grid): calls, total ms, average us, share."""
+    (a level's launches are told apart by their grid), or by kernel only: calls, total ms, average us, share."""
     import csv
     acc = {}
     with open(trace_csv) as fh:
@@ -532,7 +532,7 @@ def trace_summary(trace_csv, top=40):
             name = r["Kernel_Name"].replace("void ", "").replace("ela::(anonymous namespace)::", "").replace("ela::", "")
             if not name.startswith("__amd"):
                 name = name.split("(")[0]
-            grid = r.get("Grid_Size_X") or r.get("Grid_Size") or "?"
+            grid = (r.get("Grid_Size_X") or r.get("Grid_Size") or "?") if by_grid else "all"
             d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
             e = acc.setdefault((name, grid), [0, 0.0])
             e[0] += 1
@@ -547,8 +547,9 @@ def trace_summary(trace_csv, top=40):
 
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--trace-summary":
-        # python bench.py --trace-summary KERNEL_TRACE.csv [TOP]
-        trace_summary(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 40)
+        # python bench.py --trace-summary KERNEL_TRACE.csv [TOP [name]]
+        trace_summary(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 40,
+                      by_grid=not (len(sys.argv) > 4 and sys.argv[4] == "name"))
         return
     if len(sys.argv) > 1 and sys.argv[1] == "--pmc-traffic":
         # python bench.py --pmc-traffic FETCH.csv WRITE.csv N TILE OUT.json
