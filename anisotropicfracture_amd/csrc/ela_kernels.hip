@@ -472,7 +472,7 @@ __global__ __launch_bounds__(256) void k_bottom(Lay L, int nodes, int dim, const
     const int lane = threadIdx.x & 63, wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int nwaves = (gridDim.x * blockDim.x) >> 6;
     for (int r = wave; r < nunk; r += nwaves) {
-        const double* row = M + (long long)r * (2 * nunk) + nunk;
+        const double* row = M + (long long)r * (2 * nunk);  // A^-1 in the left half
         double acc = 0.0;
         for (int q = lane; q < nunk; q += 64) acc += row[q] * sb[q];
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
@@ -495,6 +495,92 @@ __global__ __launch_bounds__(1024) void k_gj_pivot(double* __restrict__ M, int n
 __global__ __launch_bounds__(256) void k_gj_eliminate(double* __restrict__ M, int n, int c) {
     gj_eliminate_row(M, n, c, blockIdx.x);
 }
+
+// Gauss-Jordan of [A | I] (n rows of 2n), one launch a column with the pivot
+// search in every block: the pivots of the pivot/eliminate launch pairs (the
+// largest |M[r][c]| over the rows not yet pivots, ties to the first in the
+// swapped order) and the same operations on every element they keep, so the
+// same bits in half the launches.  The row swaps are a permutation (perm_in ->
+// perm_out, written by block 0; perm_in null: the identity); the pivot row is
+// divided by its pivot where it is read and, in the matrix, by its owner one
+// column later (dpiv[c] keeps each pivot); the left half is updated right of
+// the pivot column only (the columns left of it are never read again).  So no
+// element that one block writes is read by another in the same launch.  Block
+// r updates matrix row r.
+__global__ __launch_bounds__(256) void k_gj_col(double* __restrict__ M, int n, int c, const int* __restrict__ perm_in,
+                                                int* __restrict__ perm_out, double* __restrict__ dpiv, int* flag) {
+    __shared__ double sval[256];
+    __shared__ int sidx[256];
+    const int tid = threadIdx.x, r = blockIdx.x;
+    const long long w = 2LL * n;
+    auto pin = [&](int i) { return perm_in ? perm_in[i] : i; };
+    double best = -1.0;
+    int bi = c;
+    for (int i = c + tid; i < n; i += 256) {
+        const double a = fabs(M[(long long)pin(i) * w + c]);
+        if (a > best) { best = a; bi = i; }
+    }
+    sval[tid] = best;
+    sidx[tid] = bi;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (tid < st && (sval[tid + st] > sval[tid] || (sval[tid + st] == sval[tid] && sidx[tid + st] < sidx[tid]))) {
+            sval[tid] = sval[tid + st];
+            sidx[tid] = sidx[tid + st];
+        }
+        __syncthreads();
+    }
+    const int pi = sidx[0];
+    const int prow = pin(pi);
+    const double d = M[(long long)prow * w + c];
+    if (r == 0) {
+        for (int i = tid; i < n; i += 256) perm_out[i] = i == c ? prow : (i == pi ? pin(c) : pin(i));
+        if (tid == 0) {
+            dpiv[c] = d;
+            if (!(sval[0] > 0.0)) *flag = 1;
+        }
+    }
+    if (r == prow) return;
+    double* Mr = M + (long long)r * w;
+    const double* P = M + (long long)prow * w;
+    const int nl = n - 1 - c, nq = nl + n;  // columns kept: c+1 .. n-1 and n .. 2n-1
+    if (c > 0 && r == pin(c - 1)) {  // last column's pivot row: its division, then this column's elimination
+        const double dp = dpiv[c - 1];
+        const double m = Mr[c] / dp;
+        for (int t = tid; t < nq; t += 256) {
+            const int q = t < nl ? c + 1 + t : n + (t - nl);
+            const double s = Mr[q] / dp;
+            Mr[q] = m == 0.0 ? s : s - m * (P[q] / d);
+        }
+    } else {
+        const double m = Mr[c];
+        if (m == 0.0) return;
+        for (int t = tid; t < nq; t += 256) {
+            const int q = t < nl ? c + 1 + t : n + (t - nl);
+            Mr[q] -= m * (P[q] / d);
+        }
+    }
+}
+
+// After k_gj_col's last column: row i of A^-1 is the right half of matrix row
+// perm[i] (the last pivot row over its pivot); block i writes it into the left
+// half of row i (no block reads a left half).
+__global__ __launch_bounds__(256) void k_gj_final(double* __restrict__ M, int n, const int* __restrict__ perm,
+                                                  const double* __restrict__ dpiv) {
+    const long long w = 2LL * n;
+    const int i = blockIdx.x;
+    const double* src = M + (long long)perm[i] * w + n;
+    double* dst = M + (long long)i * w;
+    const double dl = dpiv[n - 1];
+    for (int q = threadIdx.x; q < n; q += 256) dst[q] = i == n - 1 ? src[q] / dl : src[q];
+}
+
+// After the per-pivot launches: row r of A^-1 (right half) into the left half.
+__global__ __launch_bounds__(256) void k_gj_left(double* __restrict__ M, int n) {
+    const long long w = 2LL * n;
+    for (int q = threadIdx.x; q < n; q += 256) M[blockIdx.x * w + q] = M[blockIdx.x * w + n + q];
+}
+
 
 // ---------------------------------------------------------------- stiffness evaluation
 // Per-grain cubic stiffness (Linear::Cubic::Define, Cubic.H:30-59): the
@@ -1465,6 +1551,14 @@ void launch_gauss_jordan(double* M, int n, int* flag, hipStream_t s) {
         hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, s, M, n, c, flag);
         hipLaunchKernelGGL(k_gj_eliminate, dim3(n), dim3(256), 0, s, M, n, c);
     }
+    hipLaunchKernelGGL(k_gj_left, dim3(n), dim3(256), 0, s, M, n);
+}
+
+void launch_gauss_jordan_col(double* M, int n, int* flag, int* perm, double* dpiv, hipStream_t s) {
+    for (int c = 0; c < n; ++c)
+        hipLaunchKernelGGL(k_gj_col, dim3(n), dim3(256), 0, s, M, n, c, c ? perm + (size_t)((c - 1) & 1) * n : nullptr,
+                           perm + (size_t)(c & 1) * n, dpiv, flag);
+    hipLaunchKernelGGL(k_gj_final, dim3(n), dim3(256), 0, s, M, n, perm + (size_t)((n - 1) & 1) * n, dpiv);
 }
 
 template <typename T>

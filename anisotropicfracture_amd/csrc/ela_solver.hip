@@ -193,6 +193,7 @@ struct ela_hier {
     bool xcd = true;  // XCD-contiguous tile order of the marching kernels
     int ry = 1;       // rows per thread of level 0's marching tiles (FP32 sweeps)
     int pf = 1;       // planes loaded ahead by the marching tiles (ELA_OPT_PREFETCH)
+    bool gj_one = true;  // Gauss-Jordan of the dense coarsest operator in one launch a column (ELA_OPT_GJ)
     BCTab* dbc = nullptr;
     // dw = DDW : (grad u - F0) of Newton's right-hand side (dim*dim FP64 planes, allocated on first use)
     double* newton_dw = nullptr;
@@ -1740,7 +1741,7 @@ int ela_prepare(ela_hier* h) {
         v.use_group = masked[l] && h->dim == 3 && (double)npure[l] >= kGroupMinPure * (double)v.nodes;
     }
     if (flag) return fail(ELA_ERR_NAN, "diagonal is nan or zero (Elastic.cpp:347 aborts here)");
-    // dense coarsest operator: columns A e_q, then Gauss-Jordan [A | I] -> [I | A^-1]
+    // dense coarsest operator: columns A e_q, then Gauss-Jordan [A | I] -> A^-1 (left half)
     if (!h->M) {  // the coarsest level is solved by BiCGStab: nothing to factor
         h->prepared = true;
         return ELA_OK;
@@ -1769,13 +1770,18 @@ int ela_prepare(ela_hier* h) {
         }
     }
     launch_identity_right(h->M, n, h->stream);
-    HIPCHK(hipMemsetAsync(h->dflag, 0, sizeof(int), h->stream));
-    launch_gauss_jordan(h->M, n, h->dflag, h->stream);
     int gj = 0;
+    HIPCHK(hipMemsetAsync(h->dflag, 0, sizeof(int), h->stream));
+    double* G = nullptr;  // one-launch-a-column scratch: pivots (n doubles), two permutations (2n ints)
+    if (h->gj_one && halloc_t(h, 2 * (size_t)n, &G) == ELA_OK)
+        launch_gauss_jordan_col(h->M, n, h->dflag, (int*)(G + n), G, h->stream);
+    else
+        launch_gauss_jordan(h->M, n, h->dflag, h->stream);
     HIPCHK(hipMemcpyAsync(&gj, h->dflag, sizeof gj, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     hfree(h, E);
     if (F) hfree(h, F);
+    if (G) hfree(h, G);
     if (gj) return fail(ELA_ERR_NAN, "coarsest-level operator is singular");
     h->prepared = true;
     return ELA_OK;
@@ -2154,6 +2160,9 @@ int ela_set_option(ela_hier* h, int option, int value) {
     } else if (option == ELA_OPT_PREFETCH) {
         if (value != 1 && value != 2) return fail(ELA_ERR_BADARG, "planes loaded ahead must be 1 or 2");
         h->pf = value;
+    } else if (option == ELA_OPT_GJ) {
+        h->gj_one = value != 0;
+        h->prepared = false;
     } else {
         return fail(ELA_ERR_BADARG, "unknown option");
     }

@@ -355,7 +355,12 @@ int ela_set_krylov(ela_hier* h, int restart);
  *                the plane it computes: 1 (default) or 2 (the loads of plane
  *                k+3 and the rhs/diagonal of plane k+1 are issued while plane
  *                k is computed, so more bytes are in flight per block).  Same
- *                results, bit for bit.  Hierarchies only. */
+ *                results, bit for bit.  Hierarchies only.
+ * ELA_OPT_GJ     1 (default): ela_prepare inverts the dense coarsest operator
+ *                by Gauss-Jordan in one launch a column (every block finds
+ *                the pivot itself; the row swaps are kept as a permutation);
+ *                0: a pivot launch and an elimination launch a column.  Same
+ *                pivots and bits either way.  Hierarchies only. */
 enum {
     ELA_OPT_MARCH = 0,
     ELA_OPT_KZ = 1,
@@ -364,7 +369,8 @@ enum {
     ELA_OPT_GRAPH = 4,
     ELA_OPT_FOLD = 5,
     ELA_OPT_ROWS = 6,
-    ELA_OPT_PREFETCH = 7
+    ELA_OPT_PREFETCH = 7,
+    ELA_OPT_GJ = 8
 };
 int ela_set_option(ela_hier* h, int option, int value);
 

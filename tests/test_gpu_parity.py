@@ -599,6 +599,33 @@ def test_folded_last_sweep_gives_the_separate_addition_bits(mode):
     assert np.array_equal(out[0][0], out[1][0])
 
 
+@pytest.mark.parametrize("n,kind", [((32, 32, 32), MM), ((56, 64, 72), ISO), ((48, 40), MM)])
+def test_gauss_jordan_one_launch_a_column_gives_the_launch_pair_bits(n, kind):
+    """OPT_GJ 1 (default): ela_prepare inverts the dense coarsest operator by Gauss-Jordan in one launch a
+    column (the pivot search in every block, the row swaps as a permutation, the pivot row divided where
+    it is read); OPT_GJ 0 launches the pivot search and the elimination separately.  Same pivots and
+    operations, so FP64 and mixed solves give the same iterations and solution bits.  Coarsest levels of
+    375 (5^3 nodes), 2160 (8 x 9 x 10 nodes) and 2D unknowns, with traction and Neumann faces."""
+    dim = len(n)
+    C = H.random_stiffness(np.random.default_rng(sum(n)), n, kind, dim)
+    t, v = H.bc_mixed(dim)
+    for mode in (ela.SOLVE_FP64, ela.SOLVE_MIXED):
+        out = []
+        for gj in (0, 1):
+            op = make(n, kind, C, t, v)
+            op.set_option(ela.OPT_GJ, gj)
+            op.SetKrylov(6)
+            op.prepareForSolve()
+            b = op.bc_init()
+            u, st = ela.Linear(op).solve(np.zeros_like(b), b, tol_rel=1e-10, tol_abs=0.0, mode=mode, max_iter=300)
+            assert st["converged"], (n, gj, st)
+            out.append((u, st["iters"]))
+            op.close()
+        print(n, "mode", mode, "iters", out[0][1], out[1][1])
+        assert out[0][1] == out[1][1]
+        assert np.array_equal(out[0][0], out[1][0])
+
+
 @pytest.mark.parametrize("n,mode", [((36, 36, 28), ela.SOLVE_MIXED)])
 def test_sizes_without_a_dense_coarsest_level_solve_with_a_bicgstab_bottom(oracle, n, mode):
     """A size whose coarsest level is too large for the dense inverse (36 x 36 x 28 -> 9 x 9 x 7 cells,
