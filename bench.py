@@ -522,21 +522,39 @@ def pmc_traffic(fetch_csv, write_csv, n, tile, out_path):
     print(json.dumps(out, indent=1))
 
 
-def trace_summary(trace_csv, top=40, by_grid=True):
+def trace_summary(trace_csv, top=40, by_grid=True, segment=None, gap_ms=0.5):
     """Markdown table of a `rocprofv3 --kernel-trace` run's kernel_trace.csv grouped by kernel and grid size
-    (a level's launches are told apart by their grid), or by kernel only: calls, total ms, average us, share."""
+    (a level's launches are told apart by their grid), or by kernel only: calls, total ms, average us, share.
+    segment K: only the K-th run of dispatches separated from the next by more than gap_ms of idle GPU (a
+    solve's launches follow each other closely; the host work between two solves leaves a gap), with its
+    span and the idle time between its kernels."""
     import csv
-    acc = {}
     with open(trace_csv) as fh:
-        for r in csv.DictReader(fh):
-            name = r["Kernel_Name"].replace("void ", "").replace("ela::(anonymous namespace)::", "").replace("ela::", "")
-            if not name.startswith("__amd"):
-                name = name.split("(")[0]
-            grid = (r.get("Grid_Size_X") or r.get("Grid_Size") or "?") if by_grid else "all"
-            d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
-            e = acc.setdefault((name, grid), [0, 0.0])
-            e[0] += 1
-            e[1] += d
+        rows = sorted(csv.DictReader(fh), key=lambda r: int(r["Start_Timestamp"]))
+    if segment is not None:
+        segs = [[rows[0]]]
+        for a, b in zip(rows, rows[1:]):
+            if (int(b["Start_Timestamp"]) - int(a["End_Timestamp"])) * 1e-6 > gap_ms:
+                segs.append([])
+            segs[-1].append(b)
+        for i, sg in enumerate(segs):
+            busy = sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in sg) * 1e-6
+            span = (int(sg[-1]["End_Timestamp"]) - int(sg[0]["Start_Timestamp"])) * 1e-6
+            if len(sg) > 100:
+                print("segment %d: %d dispatches, span %.2f ms, kernel time %.2f ms" % (i, len(sg), span, busy))
+        rows = segs[segment]
+        span = (int(rows[-1]["End_Timestamp"]) - int(rows[0]["Start_Timestamp"])) * 1e-6
+        print("\nSegment %d: %d dispatches over %.2f ms.\n" % (segment, len(rows), span))
+    acc = {}
+    for r in rows:
+        name = r["Kernel_Name"].replace("void ", "").replace("ela::(anonymous namespace)::", "").replace("ela::", "")
+        if not name.startswith("__amd"):
+            name = name.split("(")[0]
+        grid = (r.get("Grid_Size_X") or r.get("Grid_Size") or "?") if by_grid else "all"
+        d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
+        e = acc.setdefault((name, grid), [0, 0.0])
+        e[0] += 1
+        e[1] += d
     total = sum(v[1] for v in acc.values())
     print("Total kernel time %.1f ms over %d kernel/grid pairs.\n" % (total * 1e-3, len(acc)))
     print("| kernel | grid threads | calls | total ms | avg us | % |")
@@ -547,9 +565,11 @@ def trace_summary(trace_csv, top=40, by_grid=True):
 
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--trace-summary":
-        # python bench.py --trace-summary KERNEL_TRACE.csv [TOP [name]]
+        # python bench.py --trace-summary KERNEL_TRACE.csv [TOP [name|grid [SEGMENT [GAP_MS]]]]
         trace_summary(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 40,
-                      by_grid=not (len(sys.argv) > 4 and sys.argv[4] == "name"))
+                      by_grid=not (len(sys.argv) > 4 and sys.argv[4] == "name"),
+                      segment=int(sys.argv[5]) if len(sys.argv) > 5 else None,
+                      gap_ms=float(sys.argv[6]) if len(sys.argv) > 6 else 0.5)
         return
     if len(sys.argv) > 1 and sys.argv[1] == "--pmc-traffic":
         # python bench.py --pmc-traffic FETCH.csv WRITE.csv N TILE OUT.json
