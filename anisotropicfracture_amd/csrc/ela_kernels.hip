@@ -967,50 +967,6 @@ __global__ __launch_bounds__(FB) void k_maxpy(double* __restrict__ y, VecListT<T
     }
 }
 
-// The Arnoldi step's update and normalization in one pass: y = w - sum_v
-// coef[v] X_v (k_maxpy's operations with sign -1), then v = (TV) (a y) and
-// r = (TR) (a y) (the next basis vector and the next MG cycle's right-hand
-// side, as k_scale_to2f / k_scale_flat_to write them).  w itself is not
-// written; v may be w.
-template <typename TB, typename TV, typename TR, int V>
-__global__ __launch_bounds__(FB) void k_maxpy_scale(const double* w, VecListT<TB> X, int nv,
-                                                    const double* __restrict__ coef, double a, TV* v,
-                                                    TR* __restrict__ r, long long n) {
-    auto step = [&](long long q, auto vw) {
-        constexpr int W = decltype(vw)::value;
-        double yq[W], s[W];
-        ldv<W>(w, q, yq);
-#pragma unroll
-        for (int e = 0; e < W; ++e) s[e] = 0.0;
-        for (int v0 = 0; v0 < nv; v0 += CH) {
-            TB xq[CH][W];
-#pragma unroll
-            for (int u = 0; u < CH; ++u)
-                if (v0 + u < nv) ldv<W>(X.v[v0 + u], q, xq[u]);
-#pragma unroll
-            for (int u = 0; u < CH; ++u)
-                if (v0 + u < nv) {
-                    const double c = coef[v0 + u];
-#pragma unroll
-                    for (int e = 0; e < W; ++e) s[e] += c * (double)xq[u][e];
-                }
-        }
-        TV ov[W];
-        TR orr[W];
-#pragma unroll
-        for (int e = 0; e < W; ++e) {
-            const double y = yq[e] + -1.0 * s[e];
-            const double z = a * y;
-            ov[e] = (TV)z;
-            orr[e] = (TR)z;
-        }
-        stv<W>(v, q, ov);
-        stv<W>(r, q, orr);
-    };
-    ELA_FLAT_LOOP(V, n, q) step(q, std::integral_constant<int, V>{});
-    ELA_FLAT_TAIL(V, n, q) step(q, std::integral_constant<int, 1>{});
-}
-
 // y = a x, and yo = (TO) y when yo is non-null (the next MG cycle's
 // right-hand side), flat.  y may be x.
 template <typename TO, int V>
@@ -1123,21 +1079,6 @@ void launch_maxpy_norm32(double* y, const VecList32& X, int nv, const double* co
                          double* partials, double* norm2, hipStream_t s) {
     maxpy_any<float, true>(y, X, nv, coef, sign, n, partials, norm2, s);
 }
-
-template <typename TB, typename TV, typename TR>
-void launch_maxpy_scale(const double* w, const VecListT<TB>& X, int nv, const double* coef, double a, TV* v, TR* r,
-                        long long n, hipStream_t s) {
-    if (vec_ok(w, X, nv) && vec_ok({v, r}))
-        hipLaunchKernelGGL((k_maxpy_scale<TB, TV, TR, 2>), dim3(FG), dim3(FB), 0, s, w, X, nv, coef, a, v, r, n);
-    else
-        hipLaunchKernelGGL((k_maxpy_scale<TB, TV, TR, 1>), dim3(FG), dim3(FB), 0, s, w, X, nv, coef, a, v, r, n);
-}
-template void launch_maxpy_scale<float, float, float>(const double*, const VecList32&, int, const double*, double,
-                                                      float*, float*, long long, hipStream_t);
-template void launch_maxpy_scale<double, double, float>(const double*, const VecList&, int, const double*, double,
-                                                        double*, float*, long long, hipStream_t);
-template void launch_maxpy_scale<double, double, double>(const double*, const VecList&, int, const double*, double,
-                                                         double*, double*, long long, hipStream_t);
 
 void launch_scale_flat(double* y, const double* x, double a, long long n, hipStream_t s) {
     launch_scale_flat_to<double>(y, x, a, nullptr, n, s);

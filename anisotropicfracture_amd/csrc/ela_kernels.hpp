@@ -222,12 +222,6 @@ void launch_scale_flat(double* y, const double* x, double a, long long n, hipStr
 template <typename TO>
 void launch_scale_flat_to(double* y, const double* x, double a, TO* yo, long long n, hipStream_t s);
 // v = (float) (a x), r = v, flat over n elements
-// v = (TV) (a (w - sum_v coef[v] X_v)) and r = (TR) of the same value, one
-// pass (k_maxpy's and the scale kernels' operations); w is not written, v may
-// be w.
-template <typename TB, typename TV, typename TR>
-void launch_maxpy_scale(const double* w, const VecListT<TB>& X, int nv, const double* coef, double a, TV* v, TR* r,
-                        long long n, hipStream_t s);
 void launch_scale_to2f(const double* x, double a, float* v, float* r, long long n, hipStream_t s);
 // out = (double) (T) (cor + hold) on the valid nodes of a level (nc components)
 template <typename T>

@@ -823,7 +823,6 @@ int solve_fgmres(ela_hier* h, const double* b, double* u, double tol_rel, double
             double hn2 = wn0 * wn0;
             for (int q = 0; q <= j; ++q) hn2 -= h1[q] * h1[q];
             double* const norm2 = h->kdev + 2 * (m + 2);  // beyond the coefficients the update reads
-            double hn = 0.0;
             if (!(hn2 > kReorth2 * wn0 * wn0)) {
                 if constexpr (B32) {
                     launch_maxpy32(w, Vl32, j + 1, h->kdev, -1.0, n, s);
@@ -835,26 +834,22 @@ int solve_fgmres(ela_hier* h, const double* b, double* u, double tol_rel, double
                     launch_maxpy_norm(w, Vl, j + 1, h->kdev, -1.0, n, h->kpart, norm2, s);
                 }
                 h2 = host_vec(j + 1);
-                double hn2_dev = 0.0;
-                HIPCHK(hipMemcpyAsync(&hn2_dev, norm2, sizeof(double), hipMemcpyDeviceToHost, s));
-                HIPCHK(hipStreamSynchronize(s));
-                hn = std::sqrt(hn2_dev);
-                // v_{j+1}, and the next cycle's right-hand side (T) v_{j+1} in the same pass
-                if (hn > 0) {
-                    if constexpr (B32) launch_scale_to2f(w, 1.0 / hn, h->V32[j + 1], *lb<T>(v0).res, n, s);
-                    else launch_scale_flat_to<T>(h->V[j + 1], w, 1.0 / hn, *lb<T>(v0).res, n, s);
-                }
             } else {
-                // |w - V h1| from |w|^2 - |h1|^2: with at most a 100-fold cancellation (the test above)
-                // it holds ~1e-14 relative, so the update, v_{j+1} = (w - V h1) / |w'| and the next
-                // cycle's right-hand side take one pass and w' is never stored
-                hn = std::sqrt(hn2);
-                if constexpr (B32) launch_maxpy_scale<float, float, float>(w, Vl32, j + 1, h->kdev, 1.0 / hn, h->V32[j + 1], *lb<T>(v0).res, n, s);
-                else launch_maxpy_scale<double, double, T>(w, Vl, j + 1, h->kdev, 1.0 / hn, h->V[j + 1], *lb<T>(v0).res, n, s);
+                if constexpr (B32) launch_maxpy_norm32(w, Vl32, j + 1, h->kdev, -1.0, n, h->kpart, norm2, s);
+                else launch_maxpy_norm(w, Vl, j + 1, h->kdev, -1.0, n, h->kpart, norm2, s);
             }
+            double hn2_dev = 0.0;
+            HIPCHK(hipMemcpyAsync(&hn2_dev, norm2, sizeof(double), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            const double hn = std::sqrt(hn2_dev);
             ++total;
             for (int q = 0; q <= j; ++q) H[(size_t)q * m + j] = h1[q] + h2[q];
             H[(size_t)(j + 1) * m + j] = hn;
+            // v_{j+1}, and the next cycle's right-hand side (T) v_{j+1} in the same pass
+            if (hn > 0) {
+                if constexpr (B32) launch_scale_to2f(w, 1.0 / hn, h->V32[j + 1], *lb<T>(v0).res, n, s);
+                else launch_scale_flat_to<T>(h->V[j + 1], w, 1.0 / hn, *lb<T>(v0).res, n, s);
+            }
             // Givens rotations
             for (int q = 0; q < j; ++q) {
                 const double a = H[(size_t)q * m + j], c = H[(size_t)(q + 1) * m + j];
