@@ -233,7 +233,9 @@ def test_inclusion_2d_krylov_vs_direct(oracle, mode, nn):
     solve.  The reference solution is the LU solve plus one FP64 refinement step: the LU solve alone stops
     at a relative residual of 2e-8 at 64^2 with an error of the same size, the refined one at the operator's
     FP64 floor (2.2e-9), where a second step changes it by 1e-15.  The GPU solves run to just above that
-    floor."""
+    floor.  Near the floor the iteration count is sensitive to the Krylov step's accuracy (a normalization
+    taken from |w|^2 - |h|^2 instead of the stored vector raised it from 24 to 35 in FP64), so it is bounded
+    here: measured 21 / 33 (FP64 / mixed) at 32^2 and 24 / 37 at 64^2."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spl
     (nx, ny), C, t, v = K.inclusion_2d(nn)
@@ -250,6 +252,8 @@ def test_inclusion_2d_krylov_vs_direct(oracle, mode, nn):
     print("2D inclusion %d^2" % nn, st)
     assert st["converged"]
     assert np.abs(u - u_ref).max() / np.abs(u_ref).max() <= 1e-8
+    bound = {(32, ela.SOLVE_FP64): 25, (32, ela.SOLVE_MIXED): 39, (64, ela.SOLVE_FP64): 28, (64, ela.SOLVE_MIXED): 43}
+    assert st["iters"] <= bound[(nn, mode)], st
 
 
 @pytest.mark.parametrize("n", [(24, 24, 24), (40, 20, 33)])
