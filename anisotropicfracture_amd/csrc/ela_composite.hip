@@ -40,8 +40,10 @@
 // to a dense-solvable level; any other level
 // smooths, as AMReX's MLMG does on fine AMR levels: nu1 / nu2 Fsmooth calls (2
 // weighted-Jacobi sweeps each, omega = 2/3, on the level's non-fill nodes, fill
-// nodes held).  Hierarchy cycles run in FP32 (mixed) or FP64, the smoothing and
-// everything else in FP64.
+// nodes held).  In mixed precision the hierarchy cycles and the smoothing run
+// in FP32 (FP32 stiffness and inverse diagonal of the level, the right-hand
+// side and the iterate rounded in, the result widened out); everything else
+// in FP64.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -211,8 +213,9 @@ __global__ __launch_bounds__(CX* CY) void k_subbox(Lay Ls, int o0, int o1, int o
 
 // A Jacobi sweep's output keeps the input at fill nodes (they are the sweep's
 // Dirichlet data: zero in a down-sweep, the interpolated coarse correction in an up-sweep).
-__global__ __launch_bounds__(CX* CY) void k_hold_fill(Lay L, Geo g, const double* __restrict__ xin,
-                                                      double* __restrict__ xout, int nc) {
+template <typename T>
+__global__ __launch_bounds__(CX* CY) void k_hold_fill(Lay L, Geo g, const T* __restrict__ xin, T* __restrict__ xout,
+                                                      int nc) {
     const int i = blockIdx.x * CX + threadIdx.x, j = blockIdx.y * CY + threadIdx.y, k = blockIdx.z;
     if (i >= L.nx || j >= L.ny) return;
     const int p[3] = {g.glo[0] + i, g.glo[1] + j, g.glo[2] + k};
@@ -294,6 +297,9 @@ struct CLev {
     std::vector<int> boxes;          // lo[3], hi[3] per box (level l >= 1)
     unsigned char* cls = nullptr;    // node classes of G on the device
     double* dg = nullptr;            // diagonal of A_l on G (levels >= 1, FP64)
+    // mixed precision, a level that smooths: its stiffness and inverse diagonal in FP32, and FP32 work
+    // arrays for the iterate, the sweep output and the right-hand side
+    float *C32 = nullptr, *dg32 = nullptr, *x32 = nullptr, *t32 = nullptr, *r32 = nullptr;
     int nu1 = 2, nu2 = 2;            // Fsmooth calls of the FAC cycle (smoothed levels)
     // a level whose union is one box (level 0: the domain) carries a multigrid
     // hierarchy on that box; its FAC step is a cycle of it instead of smoothing
@@ -373,13 +379,15 @@ bool level_march(const ela_amr* a, const CLev& v) {
     return a->dim == 3 && a->kind == K_ISO && v.dbcG && grain_offsets_fit(v.L) && plane_offsets_fit(v.L, 2, 8);
 }
 constexpr int kLevelKz = 16;  // planes per marching tile on the levels' arrays
-GrainBox<double> level_box(const ela_amr* a, const CLev& v) {
-    GrainBox<double> g{};
+template <typename T = double>
+GrainBox<T> level_box(const ela_amr* a, const CLev& v) {
+    GrainBox<T> g{};
     g.L = v.L;
-    g.iv = make_inv<double>(v.h);
+    g.iv = make_inv<T>(v.h);
     g.bc = v.dbcG;
     g.uniform = a->uniform;
-    g.C = v.C;
+    if constexpr (sizeof(T) == 8) g.C = v.C;
+    else g.C = v.C32;
     g.sm = nullptr;
     g.xcd = 1;
     g.ry = 1;
@@ -421,6 +429,19 @@ void lev_jacobi(const ela_amr* a, const CLev& v, const double* x, const double* 
         launch_plane<double, 0, double>(g, kLevelKz, nullptr, nullptr, a->stream);
     } else {
         launch_jacobi<double>(v.L, a->kind, v.C, v.dg, x, b, out, v.bcG, v.h, a->uniform, 2.0 / 3.0, a->stream);
+    }
+}
+void lev_jacobi32(const ela_amr* a, const CLev& v, const float* x, const float* b, float* out) {
+    if (level_march(a, v)) {
+        GrainBox<float> g = level_box<float>(a, v);
+        g.in = x;
+        g.rhs = b;
+        g.out = out;
+        g.dg = v.dg32;
+        g.omega = 2.0f / 3.0f;
+        launch_plane<float, 0, float>(g, kLevelKz, nullptr, nullptr, a->stream);
+    } else {
+        launch_jacobi<float>(v.L, a->kind, v.C32, v.dg32, x, b, out, v.bcG, v.h, a->uniform, 2.0 / 3.0, a->stream);
     }
 }
 
@@ -522,11 +543,28 @@ int box_correct(ela_amr* a, int l, const double* src, double* dst, int add) {
 void level_smooth(ela_amr* a, int l, double* x, const double* rhs, double* tmp, int ncalls) {
     CLev& f = a->lev[l];
     const dim3 g = grid_of(f.L);
+    if (f.x32) {  // mixed: the sweeps in FP32 (fill nodes held at their rounded values, widened back out)
+        const long long n = (long long)f.elems;
+        launch_convert<double, float>(n, rhs, f.r32, a->stream);
+        launch_convert<double, float>(n, x, f.x32, a->stream);
+        for (int c = 0; c < ncalls; ++c) {
+            lev_jacobi32(a, f, f.x32, f.r32, f.t32);
+            hipLaunchKernelGGL(k_hold_fill<float>, g, dim3(CX, CY), 0, a->stream, f.L, f.geo, (const float*)f.x32,
+                               f.t32, a->dim);
+            lev_jacobi32(a, f, f.t32, f.r32, f.x32);
+            hipLaunchKernelGGL(k_hold_fill<float>, g, dim3(CX, CY), 0, a->stream, f.L, f.geo, (const float*)f.t32,
+                               f.x32, a->dim);
+        }
+        launch_convert<float, double>(n, f.x32, x, a->stream);
+        return;
+    }
     for (int c = 0; c < ncalls; ++c) {
         lev_jacobi(a, f, x, rhs, tmp);
-        hipLaunchKernelGGL(k_hold_fill, g, dim3(CX, CY), 0, a->stream, f.L, f.geo, (const double*)x, tmp, a->dim);
+        hipLaunchKernelGGL(k_hold_fill<double>, g, dim3(CX, CY), 0, a->stream, f.L, f.geo, (const double*)x, tmp,
+                           a->dim);
         lev_jacobi(a, f, tmp, rhs, x);
-        hipLaunchKernelGGL(k_hold_fill, g, dim3(CX, CY), 0, a->stream, f.L, f.geo, (const double*)tmp, x, a->dim);
+        hipLaunchKernelGGL(k_hold_fill<double>, g, dim3(CX, CY), 0, a->stream, f.L, f.geo, (const double*)tmp, x,
+                           a->dim);
     }
 }
 
@@ -1039,6 +1077,11 @@ int ela_amr_create_boxes(ela_ctx* ctx, const ela_domain* dom, int nlev, const in
             rc = afail(ELA_ERR_HIP, "mask upload failed");
         v.geo.cls = v.cls;
         if (!rc && !v.H) rc = aalloc_t(a, (size_t)dim * v.L.cs, &v.dg);
+        if (!rc && !v.H && a->cprec == ELA_F32) {
+            rc = aalloc_t(a, (size_t)a->nc * v.L.cs, &v.C32);
+            for (float** p : {&v.dg32, &v.x32, &v.t32, &v.r32})
+                if (!rc) rc = aalloc_t(a, (size_t)dim * v.L.cs, p);
+        }
     }
     for (double** p : {&a->u, &a->b, &a->r, &a->z, &a->w, &a->t, &a->q})
         if (!rc) rc = aalloc_t(a, a->ntot, p);
@@ -1214,7 +1257,8 @@ int ela_amr_prepare(ela_amr* a) {
             hier_model_set(v.H);
             if (int rc = ela_prepare(v.H)) return rc;
         } else {  // the diagonal of A_l on G_l for the Jacobi sweeps (its values at fill nodes are never used)
-            launch_diag(v.L, a->kind, v.C, v.dg, nullptr, v.bcG, v.h, a->dnorm, a->stream);
+            launch_diag(v.L, a->kind, v.C, v.dg, v.dg32, v.bcG, v.h, a->dnorm, a->stream);
+            if (v.C32) launch_convert<double, float>((long long)a->nc * v.L.cs, v.C, v.C32, a->stream);
         }
     }
     ELA_HIPCHK(hipStreamSynchronize(a->stream));

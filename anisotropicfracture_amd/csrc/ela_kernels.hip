@@ -2162,6 +2162,7 @@ template void launch_axpy<double, float>(const Lay&, int, const double*, float*,
 template void launch_axpy<float, float>(const Lay&, int, const float*, float*, const double*, double, int,
                                         hipStream_t);
 template void launch_convert<double, float>(long long, const double*, float*, hipStream_t);
+template void launch_convert<float, double>(long long, const float*, double*, hipStream_t);
 template void launch_bottom<double>(const Lay&, int, int, const double*, const double*, double*, hipStream_t);
 template void launch_bottom<float>(const Lay&, int, int, const double*, const float*, float*, hipStream_t);
 template void launch_pack<double>(const Lay&, int, const double*, double*, int, hipStream_t);
