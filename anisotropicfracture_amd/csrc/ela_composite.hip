@@ -327,6 +327,8 @@ struct ela_amr {
     double *u = nullptr, *b = nullptr, *r = nullptr, *z = nullptr, *w = nullptr, *t = nullptr, *q = nullptr;
     int krylov = 8;
     std::vector<double*> V, Z;
+    std::vector<float*> V32;  // mixed precision: the Krylov basis in FP32
+    double* kw = nullptr;     // and w = A z in FP64
     double *kdev = nullptr, *kpart = nullptr;
     unsigned long long* dnorm = nullptr;
     unsigned long long* hnorm = nullptr;
@@ -569,11 +571,12 @@ void level_smooth(ela_amr* a, int l, double* x, const double* rhs, double* tmp, 
 }
 
 // z = FAC(v): one multiplicative FAC cycle over the AMR levels (t, w, q: work vectors)
-int fac(ela_amr* a, const double* v, double* z) {
+int fac(ela_amr* a, const double* v, double* z, const float* v32 = nullptr) {
     hipStream_t s = a->stream;
     const int L = a->nlev - 1;
     if (hipMemsetAsync(z, 0, a->ntot * sizeof(double), s) != hipSuccess) return afail(ELA_ERR_HIP, "memset");
-    if (hipMemcpyAsync(a->t, v, a->ntot * sizeof(double), hipMemcpyDeviceToDevice, s) != hipSuccess)
+    if (v32) launch_convert<float, double>((long long)a->ntot, v32, a->t, s);
+    else if (hipMemcpyAsync(a->t, v, a->ntot * sizeof(double), hipMemcpyDeviceToDevice, s) != hipSuccess)
         return afail(ELA_ERR_HIP, "copy");
     for (int l = L; l >= 1; --l) {
         CLev& f = a->lev[l];
@@ -602,16 +605,25 @@ int fac(ela_amr* a, const double* v, double* z) {
 
 int krylov_alloc(ela_amr* a) {
     const int m = a->krylov;
-    if ((int)a->Z.size() == m && (int)a->V.size() == m + 1) return ELA_OK;
+    const bool b32 = a->cprec == ELA_F32;
+    if ((int)a->Z.size() == m && (b32 ? (int)a->V32.size() : (int)a->V.size()) == m + 1) return ELA_OK;
     if (m > kMaxKrylov - 1) return afail(ELA_ERR_BADARG, "restart too long");
     a->V.clear();
+    a->V32.clear();
     a->Z.clear();
     int rc = ELA_OK;
     for (int q = 0; q <= m && !rc; ++q) {
-        double* p = nullptr;
-        rc = aalloc_t(a, a->ntot, &p);
-        a->V.push_back(p);
+        if (b32) {
+            float* p = nullptr;
+            rc = aalloc_t(a, a->ntot, &p);
+            a->V32.push_back(p);
+        } else {
+            double* p = nullptr;
+            rc = aalloc_t(a, a->ntot, &p);
+            a->V.push_back(p);
+        }
     }
+    if (!rc && b32 && !a->kw) rc = aalloc_t(a, a->ntot, &a->kw);
     for (int q = 0; q < m && !rc; ++q) {
         double* p = nullptr;
         rc = aalloc_t(a, a->ntot, &p);
@@ -631,7 +643,8 @@ std::vector<double> host_vec(ela_amr* a, int count) {
 
 // Right-preconditioned flexible GMRES(m) on the composite unknowns (classical
 // Gram-Schmidt with a second pass when the new vector lost most of its norm);
-// m = 0 runs the plain FAC iteration u += FAC(r).
+// m = 0 runs the plain FAC iteration u += FAC(r).  Mixed precision keeps the
+// basis vectors in FP32.
 int solve(ela_amr* a, double tol_rel, double tol_abs, int max_iter, ela_stats* st) {
     hipStream_t s = a->stream;
     const long long n = (long long)a->ntot;
@@ -657,24 +670,36 @@ int solve(ela_amr* a, double tol_rel, double tol_abs, int max_iter, ela_stats* s
                                a->u, (const double*)a->z, n);
             ++total;
         } else {
+            // mixed precision keeps the basis in FP32 (w = A z, the dots, the updates and the solution in FP64)
+            const bool b32 = a->cprec == ELA_F32;
             VecList one{};
             one.v[0] = a->r;
             launch_mdot(a->r, one, 1, n, a->kpart, a->kdev, s);
             const double beta = std::sqrt(host_vec(a, 1)[0]);
             if (!(beta > 0)) break;
-            launch_scale_flat(a->V[0], a->r, 1.0 / beta, n, s);
+            if (b32) launch_scale_to2f(a->r, 1.0 / beta, a->V32[0], nullptr, n, s);
+            else launch_scale_flat(a->V[0], a->r, 1.0 / beta, n, s);
             std::fill(g.begin(), g.end(), 0.0);
             g[0] = beta;
             int jend = 0;
             for (int j = 0; j < m && total < max_iter; ++j) {
-                if (int rc = fac(a, a->V[j], a->Z[j])) return rc;
-                apply(a, a->Z[j], a->V[j + 1]);
+                if (int rc = b32 ? fac(a, nullptr, a->Z[j], a->V32[j]) : fac(a, a->V[j], a->Z[j])) return rc;
+                double* const w = b32 ? a->kw : a->V[j + 1];
+                apply(a, a->Z[j], w);
                 VecList Vl{};
-                for (int q = 0; q <= j; ++q) Vl.v[q] = a->V[q];
+                VecList32 Vl32{};
+                for (int q = 0; q <= j; ++q) {
+                    if (b32) Vl32.v[q] = a->V32[q];
+                    else Vl.v[q] = a->V[q];
+                }
                 // h1 and |w|^2 in one pass (w last in the list); |w'|^2 with the update
-                VecList Vw = Vl;
-                Vw.v[j + 1] = a->V[j + 1];
-                launch_mdot(a->V[j + 1], Vw, j + 2, n, a->kpart, a->kdev, s);
+                if (b32) {
+                    launch_mdot_self(w, Vl32, j + 1, n, a->kpart, a->kdev, s);
+                } else {
+                    VecList Vw = Vl;
+                    Vw.v[j + 1] = w;
+                    launch_mdot(w, Vw, j + 2, n, a->kpart, a->kdev, s);
+                }
                 std::vector<double> h1 = host_vec(a, j + 2), h2(j + 1, 0.0);
                 const double wn0 = std::sqrt(h1[j + 1]);
                 h1.resize(j + 1);
@@ -682,12 +707,19 @@ int solve(ela_amr* a, double tol_rel, double tol_abs, int max_iter, ela_stats* s
                 for (int q = 0; q <= j; ++q) hn2 -= h1[q] * h1[q];
                 double* const norm2 = a->kdev + 2 * (kMaxKrylov + 2);  // beyond the coefficients
                 if (!(hn2 > kReorth2 * wn0 * wn0)) {
-                    launch_maxpy(a->V[j + 1], Vl, j + 1, a->kdev, -1.0, n, s);
-                    launch_mdot(a->V[j + 1], Vl, j + 1, n, a->kpart, a->kdev, s);
-                    launch_maxpy_norm(a->V[j + 1], Vl, j + 1, a->kdev, -1.0, n, a->kpart, norm2, s);
+                    if (b32) {
+                        launch_maxpy32(w, Vl32, j + 1, a->kdev, -1.0, n, s);
+                        launch_mdot_self(w, Vl32, j + 1, n, a->kpart, a->kdev, s);
+                        launch_maxpy_norm32(w, Vl32, j + 1, a->kdev, -1.0, n, a->kpart, norm2, s);
+                    } else {
+                        launch_maxpy(w, Vl, j + 1, a->kdev, -1.0, n, s);
+                        launch_mdot(w, Vl, j + 1, n, a->kpart, a->kdev, s);
+                        launch_maxpy_norm(w, Vl, j + 1, a->kdev, -1.0, n, a->kpart, norm2, s);
+                    }
                     h2 = host_vec(a, j + 1);
                 } else {
-                    launch_maxpy_norm(a->V[j + 1], Vl, j + 1, a->kdev, -1.0, n, a->kpart, norm2, s);
+                    if (b32) launch_maxpy_norm32(w, Vl32, j + 1, a->kdev, -1.0, n, a->kpart, norm2, s);
+                    else launch_maxpy_norm(w, Vl, j + 1, a->kdev, -1.0, n, a->kpart, norm2, s);
                 }
                 double hn2_dev = 0.0;
                 ELA_HIPCHK(hipMemcpyAsync(&hn2_dev, norm2, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -696,7 +728,10 @@ int solve(ela_amr* a, double tol_rel, double tol_abs, int max_iter, ela_stats* s
                 ++total;
                 for (int q = 0; q <= j; ++q) H[(size_t)q * m + j] = h1[q] + h2[q];
                 H[(size_t)(j + 1) * m + j] = hn;
-                if (hn > 0) launch_scale_flat(a->V[j + 1], a->V[j + 1], 1.0 / hn, n, s);
+                if (hn > 0) {
+                    if (b32) launch_scale_to2f(w, 1.0 / hn, a->V32[j + 1], nullptr, n, s);
+                    else launch_scale_flat(a->V[j + 1], a->V[j + 1], 1.0 / hn, n, s);
+                }
                 for (int q = 0; q < j; ++q) {
                     const double x = H[(size_t)q * m + j], y = H[(size_t)(q + 1) * m + j];
                     H[(size_t)q * m + j] = cs[q] * x + sn[q] * y;

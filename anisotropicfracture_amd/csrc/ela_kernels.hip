@@ -989,8 +989,8 @@ __global__ __launch_bounds__(FB) void k_scale_flat_to(double* y, const double* x
     ELA_FLAT_TAIL(V, n, q) step(q, std::integral_constant<int, 1>{});
 }
 
-// v = (float) (a x) and r = v, flat (an FP32 Krylov vector and the next MG
-// cycle's right-hand side).
+// v = (float) (a x) and r = v (r may be null), flat (an FP32 Krylov vector and
+// the next MG cycle's right-hand side).
 template <int V>
 __global__ __launch_bounds__(FB) void k_scale_to2f(const double* __restrict__ x, double a, float* __restrict__ v,
                                                    float* __restrict__ r, long long n) {
@@ -1002,7 +1002,7 @@ __global__ __launch_bounds__(FB) void k_scale_to2f(const double* __restrict__ x,
 #pragma unroll
         for (int e = 0; e < W; ++e) f[e] = (float)(a * xq[e]);
         stv<W>(v, q, f);
-        stv<W>(r, q, f);
+        if (r) stv<W>(r, q, f);
     };
     ELA_FLAT_LOOP(V, n, q) step(q, std::integral_constant<int, V>{});
     ELA_FLAT_TAIL(V, n, q) step(q, std::integral_constant<int, 1>{});
