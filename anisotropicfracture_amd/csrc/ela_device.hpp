@@ -370,15 +370,31 @@ __device__ __forceinline__ void interior_apply(const Ctx& S, const Inv<T>& iv, b
             }
         }
     } else if (K == K_MM) {
+        // sum over b, k, l of C_abkl u_k,lb.  C_abkl = C_ablk (minor symmetry), so the pair (k, l), k < l,
+        // takes one product with u_k,lb + u_l,kb: per (a, b) DIM + DIM (DIM - 1) / 2 products instead of DIM^2
+        constexpr int NP = DIM * (DIM - 1) / 2;
+        T sp[NP][DIM];
+#pragma unroll
+        for (int b = 0; b < DIM; ++b) {
+            int q = 0;
+#pragma unroll
+            for (int kk = 0; kk < DIM; ++kk)
+#pragma unroll
+                for (int l = kk + 1; l < DIM; ++l) sp[q++][b] = gg[kk][l][b] + gg[l][kk][b];
+        }
 #pragma unroll
         for (int a = 0; a < DIM; ++a) {
             T acc = 0;
 #pragma unroll
-            for (int b = 0; b < DIM; ++b)
+            for (int b = 0; b < DIM; ++b) {
+#pragma unroll
+                for (int kk = 0; kk < DIM; ++kk) acc += c[mm_slot<DIM>(a, b, kk, kk)] * gg[kk][kk][b];
+                int q = 0;
 #pragma unroll
                 for (int kk = 0; kk < DIM; ++kk)
 #pragma unroll
-                    for (int l = 0; l < DIM; ++l) acc += c[mm_slot<DIM>(a, b, kk, l)] * gg[kk][l][b];
+                    for (int l = kk + 1; l < DIM; ++l) acc += c[mm_slot<DIM>(a, b, kk, l)] * sp[q++][b];
+            }
             f[a] = acc;
         }
         if (!uniform) {
