@@ -18,9 +18,13 @@
 // Per member the arithmetic is that of the single-box solver (ela_solver.hip):
 // FP64 outer iteration r = b - A u, u += cycle(r) with the right-hand side
 // scaled by a power of two, F-cycles with nu1 = nu2 = 2 calls of 2 weighted
-// Jacobi sweeps (omega = 2/3), full weighting, trilinear prolongation and, on
-// the coarsest level, a dense inverse (up to kMaxBatchDense unknowns) or
-// BiCGStab to 1e-4 (larger coarsest levels: sides with a large odd factor).
+// Jacobi sweeps (omega = 2/3), full weighting of residuals, trilinear
+// prolongation, grain-form stiffness on every level (the coarse levels' grain
+// ids and factors injected, as a grain-form single box coarsens, so every
+// level runs the grain kernels and, when the whole level is in flight, the
+// marching tiles) and, on the coarsest level, a dense inverse (up to
+// kMaxBatchDense unknowns) or BiCGStab to 1e-4 (larger coarsest levels: sides
+// with a large odd factor).
 // A member's results depend only on its own data (no cross-member reductions;
 // a member's BiCGStab scalars come from its own blocks, and a member that
 // stops iterating stays frozen while others go on), and it leaves the active
@@ -76,13 +80,13 @@ struct BoxDesc {
     Inv<float> i32;
     double h[3];
     BCTab bc;
-    const unsigned short* gid;  // grain form (finest level) if non-null: s[o] * tab[gid[o]]
+    // stiffness in grain form on every level: s[o] * tab[gid[o]] (the coarse levels' ids and factors are
+    // injected from the next finer level's, as a grain-form single box coarsens)
+    const unsigned short* gid;
     const double* tab64;
     const float* tab32;
     const double* sc64;  // per-node factor or null
     const float* sc32;
-    const double* C64;  // planes (coarse levels)
-    const float* C32;
     double* d64;  // Jacobi diagonal in the cycle precision (one of d64 / d32)
     float* d32;   // FP32: its inverse
     void* f[F_N];  // cycle fields, cycle precision
@@ -104,7 +108,6 @@ template <>
 struct P<double> {
     __device__ static const double* tab(const BoxDesc& d) { return d.tab64; }
     __device__ static const double* sc(const BoxDesc& d) { return d.sc64; }
-    __device__ static const double* C(const BoxDesc& d) { return d.C64; }
     __device__ static const double* dg(const BoxDesc& d) { return d.d64; }
     __device__ static const Inv<double>& inv(const BoxDesc& d) { return d.i64; }
 };
@@ -112,7 +115,6 @@ template <>
 struct P<float> {
     __device__ static const float* tab(const BoxDesc& d) { return d.tab32; }
     __device__ static const float* sc(const BoxDesc& d) { return d.sc32; }
-    __device__ static const float* C(const BoxDesc& d) { return d.C32; }
     __device__ static const float* dg(const BoxDesc& d) { return d.d32; }
     __device__ static const Inv<float>& inv(const BoxDesc& d) { return d.i32; }
 };
@@ -131,22 +133,16 @@ __device__ __forceinline__ bool box_node(const BoxDesc& d, const Lay& L, int blk
     return true;
 }
 
-// f = A u at (i,j,k) in precision T (cubic).
-// GF: the box's stiffness is in grain form (1) or planes (0); -1 decides at
-// run time (set-up kernels only: one stencil body per kernel keeps the
-// register count of the cycle kernels at that of a single form).
-template <typename T, int GF>
+// f = A u at (i,j,k) in precision T (cubic, grain form).
+template <typename T>
 __device__ __forceinline__ void box_apply(const BoxDesc& d, const Lay& L, const T* __restrict__ u, int i, int j, int k,
                                           T f[3]) {
-    if (GF == 1 || (GF < 0 && d.gid))
-        node_apply_acc<T, K_MM, 3>(L, GrainAcc<T, kNC>{d.gid, P<T>::tab(d), P<T>::sc(d)}, u, i, j, k, d.bc, P<T>::inv(d),
-                                   d.uniform != 0, f);
-    else
-        node_apply_acc<T, K_MM, 3>(L, PlaneAcc<T>{P<T>::C(d), L.cs}, u, i, j, k, d.bc, P<T>::inv(d), d.uniform != 0, f);
+    node_apply_acc<T, K_MM, 3>(L, GrainAcc<T, kNC>{d.gid, P<T>::tab(d), P<T>::sc(d)}, u, i, j, k, d.bc, P<T>::inv(d),
+                               d.uniform != 0, f);
 }
 
 // One weighted-Jacobi sweep out = in + omega (rhs - A in) / D; in < 0: out = omega rhs / D.
-template <typename T, int GF>
+template <typename T>
 __global__ __launch_bounds__(TB) void kb_smooth(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, int blk0,
                                                 int rhs, int in, int out, T omega) {
     const int blk = blk0 + blockIdx.x;
@@ -165,7 +161,7 @@ __global__ __launch_bounds__(TB) void kb_smooth(const BoxDesc* __restrict__ D, c
     }
     const T* xi = (const T*)d.f[in];
     T f[3];
-    box_apply<T, GF>(d, L, xi, i, j, k, f);
+    box_apply<T>(d, L, xi, i, j, k, f);
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
         const long long q = p * L.cs + o;
@@ -174,7 +170,7 @@ __global__ __launch_bounds__(TB) void kb_smooth(const BoxDesc* __restrict__ D, c
 }
 
 // r = b - A x (cycle precision).
-template <typename T, int GF>
+template <typename T>
 __global__ __launch_bounds__(TB) void kb_resid(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, int blk0,
                                                int x, int b, int r) {
     const int blk = blk0 + blockIdx.x;
@@ -186,7 +182,7 @@ __global__ __launch_bounds__(TB) void kb_resid(const BoxDesc* __restrict__ D, co
     const T* bb = (const T*)d.f[b];
     T* rr = (T*)d.f[r];
     T f[3];
-    box_apply<T, GF>(d, L, (const T*)d.f[x], i, j, k, f);
+    box_apply<T>(d, L, (const T*)d.f[x], i, j, k, f);
 #pragma unroll
     for (int p = 0; p < 3; ++p) rr[p * L.cs + o] = bb[p * L.cs + o] - f[p];
 }
@@ -236,7 +232,7 @@ __global__ __launch_bounds__(TB) void kb_resid64(const BoxDesc* __restrict__ D, 
     double m = 0.0;
     if (box_node(d, L, blockIdx.x, i, j, k, o)) {
         double f[3];
-        box_apply<double, 1>(d, L, d.u64, i, j, k, f);
+        box_apply<double>(d, L, d.u64, i, j, k, f);
         const double s = *d.scale;
         T* rs = (T*)d.f[res];
 #pragma unroll
@@ -576,7 +572,7 @@ __global__ __launch_bounds__(TB) void kbb_av(const BoxDesc* __restrict__ D, cons
         const T* rh = (const T*)d.bv[BV_RH];
         T* out = (T*)d.bv[BV_V];
         T f[3];
-        box_apply<T, -1>(d, L, in, i, j, k, f);
+        box_apply<T>(d, L, in, i, j, k, f);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const long long e = c * L.cs + o;
@@ -653,7 +649,7 @@ __global__ __launch_bounds__(TB) void kbb_at(const BoxDesc* __restrict__ D, cons
         const T* sv = (const T*)d.bv[BV_S];
         T* out = (T*)d.bv[BV_T];
         T f[3];
-        box_apply<T, -1>(d, L, in, i, j, k, f);
+        box_apply<T>(d, L, in, i, j, k, f);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const long long e = c * L.cs + o;
@@ -734,7 +730,7 @@ __global__ __launch_bounds__(TB) void kb_apply(const BoxDesc* __restrict__ D, co
     long long o;
     if (!box_node(d, L, blockIdx.x, i, j, k, o)) return;
     T f[3];
-    box_apply<T, 1>(d, L, (const T*)d.f[src], i, j, k, f);
+    box_apply<T>(d, L, (const T*)d.f[src], i, j, k, f);
     T* out = (T*)d.f[dst];
 #pragma unroll
     for (int p = 0; p < 3; ++p) out[p * L.cs + o] = f[p];
@@ -747,7 +743,7 @@ __global__ __launch_bounds__(TB) void kb_apply64(const BoxDesc* __restrict__ D, 
     long long o;
     if (!box_node(d, L, blockIdx.x, i, j, k, o)) return;
     double f[3];
-    box_apply<double, 1>(d, L, d.u64, i, j, k, f);
+    box_apply<double>(d, L, d.u64, i, j, k, f);
 #pragma unroll
     for (int p = 0; p < 3; ++p) d.r64[p * L.cs + o] = f[p];
 }
@@ -925,11 +921,11 @@ __global__ __launch_bounds__(64) void kb_fold_m(const BoxDesc* __restrict__ D, c
     }
 }
 
-// ------------------------------------------------------------------ set-up
-// Stiffness of a coarse level: the full-weighting average of the fine level's
-// (Elastic.cpp:722-809), from the grain form or from planes; launched over the coarse list.
-__global__ __launch_bounds__(TB) void kb_coef(const BoxDesc* __restrict__ Dc, const BoxDesc* __restrict__ Df,
-                                              const int* __restrict__ bmap) {
+// Coarse level of a member in grain form: each coarse node takes the grain id
+// (and factor) of the coincident fine node, as a grain-form single box
+// coarsens (Elastic.cpp:759-803 averages per-node stiffness instead).
+__global__ __launch_bounds__(TB) void kb_inject(const BoxDesc* __restrict__ Dc, const BoxDesc* __restrict__ Df,
+                                               const int* __restrict__ bmap) {
     const int box = bmap[blockIdx.x];
     const BoxDesc& c = Dc[box];
     const BoxDesc& f = Df[box];
@@ -937,28 +933,10 @@ __global__ __launch_bounds__(TB) void kb_coef(const BoxDesc* __restrict__ Dc, co
     int I, J, K;
     long long oc;
     if (!box_node(c, Lc, blockIdx.x, I, J, K, oc)) return;
-    double* out = (double*)c.C64;
-    const long long px = Lf.px, pl = Lf.pl, of = 2LL * I + px * (2LL * J) + pl * (2LL * K);
-    if (f.gid) {
-        const GrainAcc<double, kNC> A{f.gid, f.tab64, f.sc64};
-        for (int q = 0; q < kNC; ++q)
-            out[q * Lc.cs + oc] = restrict_node<double, 3>(
-                Lc, I, J, K, [&A, q, of, px, pl](int a, int b, int c) { return A.rec(of + a + px * b + pl * c)[q]; });
-    } else {
-        for (int q = 0; q < kNC; ++q) {
-            const double* F = f.C64 + q * Lf.cs + of;
-            out[q * Lc.cs + oc] =
-                restrict_node<double, 3>(Lc, I, J, K, [F, px, pl](int a, int b, int c) { return F[a + px * b + pl * c]; });
-        }
-    }
-}
-
-// C32 = (float) C64 on levels stored as planes.
-__global__ __launch_bounds__(TB) void kb_planes32(const BoxDesc* __restrict__ D, const int* __restrict__ bmap) {
-    const BoxDesc& d = D[bmap[blockIdx.x]];
-    const int q = ((int)blockIdx.x - d.tile0) * TB + (int)threadIdx.x;
-    if (q >= d.nodes || d.gid || !d.C32) return;
-    for (int c = 0; c < kNC; ++c) ((float*)d.C32)[c * d.L.cs + q] = (float)d.C64[c * d.L.cs + q];
+    const long long of = 2LL * I + Lf.px * (2LL * J) + Lf.pl * (2LL * K);
+    ((unsigned short*)c.gid)[oc] = f.gid[of];
+    if (c.sc64) ((double*)c.sc64)[oc] = f.sc64[of];
+    if (c.sc32) ((float*)c.sc32)[oc] = f.sc32[of];
 }
 
 // Jacobi diagonal (Elastic.cpp:264-352) from the FP64 stiffness, stored in the cycle precision.
@@ -969,8 +947,7 @@ __global__ __launch_bounds__(TB) void kb_diag(const BoxDesc* __restrict__ D, con
     long long o;
     if (!box_node(d, L, blockIdx.x, i, j, k, o)) return;
     double dg[3];
-    if (d.gid) node_diag_acc<K_MM, 3>(L, GrainAcc<double, kNC>{d.gid, d.tab64, d.sc64}, i, j, k, d.bc, d.h, dg);
-    else node_diag_acc<K_MM, 3>(L, PlaneAcc<double>{d.C64, L.cs}, i, j, k, d.bc, d.h, dg);
+    node_diag_acc<K_MM, 3>(L, GrainAcc<double, kNC>{d.gid, d.tab64, d.sc64}, i, j, k, d.bc, d.h, dg);
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
         if (d.d64) d.d64[p * L.cs + o] = dg[p];
@@ -996,7 +973,7 @@ __global__ __launch_bounds__(TB) void kb_probe(const BoxDesc* __restrict__ D, in
     const int nxy = L.nx * L.ny;
     const int k = t / nxy, j = (t - k * nxy) / L.nx, i = t - k * nxy - j * L.nx;
     double f[3];
-    box_apply<double, -1>(d, L, d.E + (long long)q * d.nunk, i, j, k, f);
+    box_apply<double>(d, L, d.E + (long long)q * d.nunk, i, j, k, f);
     const long long w = 2LL * d.nunk;
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
@@ -1100,8 +1077,9 @@ struct RLev {
     int n = 0;
     int nodes = 0;
     double h = 1;
-    double* C64 = nullptr;
-    float* C32 = nullptr;
+    unsigned short* gid = nullptr;  // levels >= 1: grain ids (and factors) injected from the finer level
+    double* sc64 = nullptr;
+    float* sc32 = nullptr;
     double* d64 = nullptr;
     float* d32 = nullptr;
     void* f[F_N] = {};
@@ -1250,8 +1228,11 @@ size_t layout_member(ela_rve* r, char* base) {
             r->b64 = (double*)take(3 * N * 8);
             r->r64 = (double*)take(3 * N * 8);
         } else {
-            v.C64 = (double*)take(kNC * N * 8);
-            if (mixed) v.C32 = (float*)take(kNC * N * 4);
+            v.gid = (unsigned short*)take(N * 2);
+            if (r->factor) {
+                v.sc64 = (double*)take(N * 8);
+                if (mixed) v.sc32 = (float*)take(N * 4);
+            }
         }
         if (mixed) v.d32 = (float*)take(3 * N * 4);
         else v.d64 = (double*)take(3 * N * 8);
@@ -1279,19 +1260,20 @@ BoxDesc desc_of(const ela_rve* r, int l) {
     d.h[0] = d.h[1] = d.h[2] = v.h;
     d.bc = r->bc;
     d.uniform = r->uniform;
+    d.ngr = r->ngrains;
+    d.tab64 = r->tab64;
+    d.tab32 = r->tab32;
     if (l == 0) {
-        d.ngr = r->ngrains;
         d.gid = r->gid;
-        d.tab64 = r->tab64;
-        d.tab32 = r->tab32;
         d.sc64 = r->sc64;
         d.sc32 = r->sc32;
         d.u64 = r->u64;
         d.b64 = r->b64;
         d.r64 = r->r64;
     } else {
-        d.C64 = v.C64;
-        d.C32 = v.C32;
+        d.gid = v.gid;
+        d.sc64 = v.sc64;
+        d.sc32 = v.sc32;
     }
     d.d64 = v.d64;
     d.d32 = v.d32;
@@ -1408,7 +1390,7 @@ int build_lists(ela_batch* B) {
         }
         B->cnt[g] = (int)lb.descs.size();
         B->tiles[g] = lb.tile.empty() ? std::vector<int>{0} : lb.tile;
-        march_number(lb, g > 0 ? B->cnt[g - 1] : 0, B->cnt[g], B->kz, bm, bs, bf, B->mlev[g]);
+        march_number(lb, 0, B->cnt[g], B->kz, bm, bs, bf, B->mlev[g]);
         pack_list(B, lb, &B->off_desc[g], &B->off_bmap[g]);
         B->mlev[g].off_bm = pack_ints(B, bm);
         B->mlev[g].off_bs = pack_ints(B, bs);
@@ -1481,10 +1463,6 @@ long long nodes_of(const ela_batch* B, int g, int count) {
     return s;
 }
 
-// Boxes [0, count) of level g: planes [0, s), grain form [s, count), where s
-// counts the members that already exist one level up (their finest is above g).
-int split_of(const ela_batch* B, int g, int count) { return std::min(count, g > 0 ? B->cnt[g - 1] : 0); }
-
 bool use_march(const ela_batch* B, const ela_batch::March& mi) {
     return B->march && mi.nm > 0 && mi.maxng <= kMarchMaxGrains && mi.fits;
 }
@@ -1513,17 +1491,11 @@ void smooth(ela_batch* B, int g, int count, int rhs, int in, int out) {
         e = (int)B->ev_launch.size() * 2;
         (void)hipEventRecord(B->ev[e], B->stream);
     }
-    const int s = split_of(B, g, count), b0 = nblocks(B, g, s);
-    if (b0)
-        hipLaunchKernelGGL((kb_smooth<T, 0>), dim3(b0), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), 0, rhs, in, out,
+    if (in >= 0 && count == B->cnt[g] && use_march(B, B->mlev[g]))
+        march_launch<T, 0, T>(B, LD(B, g), B->mlev[g], MOps{rhs, in, out, 2.0 / 3.0, nullptr, 0, B->xcd});
+    else
+        hipLaunchKernelGGL((kb_smooth<T>), dim3(nb), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), 0, rhs, in, out,
                            (T)(2.0 / 3.0));
-    if (nb > b0) {
-        if (in >= 0 && count == B->cnt[g] && use_march(B, B->mlev[g]))
-            march_launch<T, 0, T>(B, LD(B, g), B->mlev[g], MOps{rhs, in, out, 2.0 / 3.0, nullptr, 0, B->xcd});
-        else
-            hipLaunchKernelGGL((kb_smooth<T, 1>), dim3(nb - b0), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), b0, rhs,
-                               in, out, (T)(2.0 / 3.0));
-    }
     if (e >= 0) {
         (void)hipEventRecord(B->ev[e + 1], B->stream);
         B->ev_launch.push_back({e, nodes_of(B, g, count)});
@@ -1542,15 +1514,11 @@ void fsmooth(ela_batch* B, int g, int count, int rhs, bool zero_start) {
 template <typename T>
 void resid(ela_batch* B, int g, int count, int x, int b, int r) {
     const int nb = nblocks(B, g, count);
-    const int s = split_of(B, g, count), b0 = nblocks(B, g, s);
-    if (b0) hipLaunchKernelGGL((kb_resid<T, 0>), dim3(b0), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), 0, x, b, r);
-    if (nb > b0) {
-        if (count == B->cnt[g] && use_march(B, B->mlev[g]))
-            march_launch<T, 1, T>(B, LD(B, g), B->mlev[g], MOps{b, x, r, 0.0, nullptr, 0, B->xcd});
-        else
-            hipLaunchKernelGGL((kb_resid<T, 1>), dim3(nb - b0), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), b0, x, b,
-                               r);
-    }
+    if (!nb) return;
+    if (count == B->cnt[g] && use_march(B, B->mlev[g]))
+        march_launch<T, 1, T>(B, LD(B, g), B->mlev[g], MOps{b, x, r, 0.0, nullptr, 0, B->xcd});
+    else
+        hipLaunchKernelGGL((kb_resid<T>), dim3(nb), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), 0, x, b, r);
 }
 
 template <typename T>
@@ -1845,7 +1813,7 @@ int check_rve(const ela_rve* r) {
     return ELA_OK;
 }
 
-// Batched set-up of the members: coarse stiffness, diagonals, coarsest inverses.
+// Batched set-up of the members: coarse grain ids and factors, diagonals, coarsest inverses.
 int prepare(ela_batch* B, int count, ela_rve* const* rves) {
     std::vector<ela_rve*> set(rves, rves + count);
     for (auto* r : set) {
@@ -1907,15 +1875,13 @@ int prepare(ela_batch* B, int count, ela_rve* const* rves) {
         auto D = [&](int g) { return (const BoxDesc*)(dl + od[g]); };
         auto M = [&](int g) { return (const int*)(dl + om[g]); };
         auto nb = [&](int g, int c) { return L[g].tile[c]; };
-        // coarse stiffness (level by level, each member's next level from its previous one)
+        // coarse grain ids and factors (level by level, each member's next level from its previous one)
         for (int g = 0; g + 1 < G; ++g) {
             const int c = (int)L[g].descs.size();
-            hipLaunchKernelGGL(kb_coef, dim3(nb(g + 1, c)), dim3(TB), 0, B->stream, D(g + 1), D(g), M(g + 1));
+            hipLaunchKernelGGL(kb_inject, dim3(nb(g + 1, c)), dim3(TB), 0, B->stream, D(g + 1), D(g), M(g + 1));
         }
         for (int g = 0; g < G; ++g) {
             const int c = (int)L[g].descs.size();
-            if (B->mode == ELA_SOLVE_MIXED)
-                hipLaunchKernelGGL(kb_planes32, dim3(nb(g, c)), dim3(TB), 0, B->stream, D(g), M(g));
             hipLaunchKernelGGL(kb_diag, dim3(nb(g, c)), dim3(TB), 0, B->stream, D(g), M(g), B->dflag);
         }
         // coarsest inverse: probe A, Gauss-Jordan on [A | I], keep A^-1

@@ -590,7 +590,9 @@ int ela_rve_download(const ela_rve* r, int field, double* host);
 int ela_rve_grain_ids(const ela_rve* r, uint16_t* host);
 
 /* prepareForSolve for a set of members (src/Operator/Operator.cpp:445-452):
- * coarse stiffness, diagonals and the dense coarsest inverses, batched. */
+ * coarse stiffness (grain ids and factors injected level by level, as a
+ * grain-form hierarchy coarsens), diagonals and the dense coarsest inverses,
+ * batched. */
 int ela_batch_prepare(ela_batch* b, int count, ela_rve* const* rves);
 
 /* Fapply (src/Operator/Elastic.cpp:151-260) on the finest level of each
