@@ -140,6 +140,34 @@ def test_batched_solve_matches_oracle(oracle):
     b.close()
 
 
+def test_batch_member_runs_the_iteration_of_the_grain_form_single_box():
+    """A batch member keeps grain-form stiffness on every level (coarse grain ids injected at set-up), the
+    operators of a grain-form single-box hierarchy with the same coarsening (coarse_min 2): in FP64 both take
+    the same cycles and reach the same solution to 1e-12 relative (the plain FP64 baseline of the batched
+    RVE workload relies on it).  Displacement on every face, the member's affine boundary values."""
+    for seed, n, ng in ((91, 32, 12), (92, 24, 7)):
+        p = problem(seed, n, ng)
+        b = E.Batch(E.SOLVE_FP64)
+        r = make(b, p)
+        b.prepare([r])
+        st_b = b.solve([r], tol_rel=1e-10, max_cycles=80)[0]
+        ub = r.download(E.RVE.U)
+        bh = r.download(E.RVE.B)
+        b.close()
+        op = E.Elastic((n, n, n), E.MM, mode=E.SOLVE_FP64, coarse_min=2)
+        op.SetBC(np.zeros((26, 3), np.int32), np.zeros((26, 3)))
+        op.SetModelVoronoiCubic(p["seeds"], K.VORONOI_Q, p["R"].reshape(-1, 9), K.C11, K.C12, K.C44)
+        op.SetStiffnessMode(1)
+        op.SetKrylov(0)
+        op.prepareForSolve()
+        us, st_s = E.Linear(op).solve(np.zeros_like(bh), bh, tol_rel=1e-10, tol_abs=0.0, mode=E.SOLVE_FP64, max_iter=80)
+        op.close()
+        print("n=%d batch cycles %d single-box cycles %d" % (n, st_b["iters"], st_s["iters"]))
+        assert st_b["converged"] and st_s["converged"]
+        assert st_b["iters"] == st_s["iters"]
+        assert np.abs(ub - us).max() <= 1e-12 * np.abs(us).max()
+
+
 def test_fp64_batch_matches_mixed():
     sol = {}
     for mode in (E.SOLVE_FP64, E.SOLVE_MIXED):
