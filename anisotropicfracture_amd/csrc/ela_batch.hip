@@ -998,6 +998,36 @@ __global__ __launch_bounds__(256) void kb_gj_eliminate(const BoxDesc* __restrict
     gj_eliminate_row(d.W, d.nunk, c, blockIdx.x);
 }
 
+// Each member's Gauss-Jordan on its [A | I] in one launch a column
+// (gj_column: the pivot/eliminate pairs' pivots and bits), its scratch after
+// W: pivots (n doubles), two permutations (2n ints), two candidate columns (2n
+// doubles).  Grid (maxnunk rows, boxes); block r updates row r of its member.
+__global__ __launch_bounds__(256) void kb_gj_col(const BoxDesc* __restrict__ D, int c, int* flag) {
+    __shared__ double sval[256];
+    __shared__ int sidx[256];
+    const BoxDesc& d = D[blockIdx.y];
+    const int n = d.nunk;
+    if (c >= n || (int)blockIdx.x >= n) return;
+    double* dpiv = d.W + 2LL * n * n;
+    int* perm = (int*)(dpiv + n);
+    double* cand = dpiv + 2 * (size_t)n;
+    gj_column(d.W, n, c, c ? perm + (size_t)((c - 1) & 1) * n : nullptr, perm + (size_t)(c & 1) * n, dpiv,
+              c ? cand + (size_t)((c - 1) & 1) * n : nullptr, cand + (size_t)(c & 1) * n, flag, blockIdx.x, sval, sidx);
+}
+
+// After kb_gj_col's last column: row i of A^-1 (the right half of W row
+// perm[i], the last pivot row over its pivot) into M.
+__global__ __launch_bounds__(256) void kb_gj_final(const BoxDesc* __restrict__ D) {
+    const BoxDesc& d = D[blockIdx.y];
+    const int n = d.nunk, i = blockIdx.x;
+    if (i >= n) return;
+    const double* dpiv = d.W + 2LL * n * n;
+    const int* perm = (const int*)(dpiv + n) + (size_t)((n - 1) & 1) * n;
+    const double* src = d.W + (long long)perm[i] * 2 * n + n;
+    const double dl = dpiv[n - 1];
+    for (int q = threadIdx.x; q < n; q += blockDim.x) d.M[(long long)i * n + q] = i == n - 1 ? src[q] / dl : src[q];
+}
+
 // M = right half of W.  Grid (maxnunk rows, boxes).
 __global__ __launch_bounds__(256) void kb_gj_extract(const BoxDesc* __restrict__ D) {
     const BoxDesc& d = D[blockIdx.y];
@@ -1164,6 +1194,7 @@ struct ela_batch {
     // ELA_OPT_GRAPH: the F-cycle's launches captured into a HIP graph, re-captured
     // when the launch lists change (list_epoch), replayed otherwise
     bool graph_on = false;
+    bool gj_one = true;  // ELA_OPT_GJ: the members' coarsest Gauss-Jordan in one launch a column
     long long list_epoch = 0;
     hipGraphExec_t gexec = nullptr;
     long long gexec_epoch = -1;
@@ -1872,7 +1903,10 @@ int prepare(ela_batch* B, int count, ela_rve* const* rves) {
     sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
     size_t ws = 0;
     for (int s : sizes) ws += up256((size_t)s * s * 8);
-    for (auto* r : set) ws += up256((size_t)r->nunk * 2 * r->nunk * 8);
+    // [A | I], then the one-launch-a-column scratch: pivots (n doubles), two permutations (2n ints) and two
+    // candidate columns (2n doubles)
+    auto wsize = [](int n) { return up256(((size_t)n * 2 * n + 4 * (size_t)n) * 8); };
+    for (auto* r : set) ws += wsize(r->nunk);
     char* wbase = nullptr;
     ELA_HIPCHK(hipMalloc((void**)&wbase, std::max<size_t>(ws, 256)));
     ELA_HIPCHK(hipMemsetAsync(wbase, 0, std::max<size_t>(ws, 256), B->stream));
@@ -1889,7 +1923,7 @@ int prepare(ela_batch* B, int count, ela_rve* const* rves) {
         const int q = (int)(std::lower_bound(sizes.begin(), sizes.end(), d.nunk) - sizes.begin());
         d.E = eye[q];
         d.W = (double*)(wbase + woff);
-        woff += up256((size_t)d.nunk * 2 * d.nunk * 8);
+        woff += wsize(d.nunk);
     }
     // Pack and upload the lists.
     std::vector<char> save;
@@ -1926,11 +1960,17 @@ int prepare(ela_batch* B, int count, ela_rve* const* rves) {
         const int nbk = (maxnodes + TB - 1) / TB;
         if (maxn > 0) {  // (members with a BiCGStab coarsest level have nunk = 0 and are skipped)
             hipLaunchKernelGGL(kb_probe, dim3(maxn * nbk, nbox), dim3(TB), 0, B->stream, D(G - 1), nbk);
-            for (int c = 0; c < maxn; ++c) {
-                hipLaunchKernelGGL(kb_gj_pivot, dim3(nbox), dim3(1024), 0, B->stream, D(G - 1), c, B->dflag);
-                hipLaunchKernelGGL(kb_gj_eliminate, dim3(maxn, nbox), dim3(256), 0, B->stream, D(G - 1), c);
+            if (B->gj_one) {
+                for (int c = 0; c < maxn; ++c)
+                    hipLaunchKernelGGL(kb_gj_col, dim3(maxn, nbox), dim3(256), 0, B->stream, D(G - 1), c, B->dflag);
+                hipLaunchKernelGGL(kb_gj_final, dim3(maxn, nbox), dim3(256), 0, B->stream, D(G - 1));
+            } else {
+                for (int c = 0; c < maxn; ++c) {
+                    hipLaunchKernelGGL(kb_gj_pivot, dim3(nbox), dim3(1024), 0, B->stream, D(G - 1), c, B->dflag);
+                    hipLaunchKernelGGL(kb_gj_eliminate, dim3(maxn, nbox), dim3(256), 0, B->stream, D(G - 1), c);
+                }
+                hipLaunchKernelGGL(kb_gj_extract, dim3(maxn, nbox), dim3(256), 0, B->stream, D(G - 1));
             }
-            hipLaunchKernelGGL(kb_gj_extract, dim3(maxn, nbox), dim3(256), 0, B->stream, D(G - 1));
         }
         int flag = 0;
         (void)hipMemcpyAsync(&flag, B->dflag, sizeof flag, hipMemcpyDeviceToHost, B->stream);
@@ -2563,6 +2603,8 @@ int ela_batch_set_option(ela_batch* B, int option, int value) {
         B->xcd = value != 0;
     } else if (option == ELA_OPT_GRAPH) {
         B->graph_on = value != 0;
+    } else if (option == ELA_OPT_GJ) {
+        B->gj_one = value != 0;
     } else {
         return fail(ELA_ERR_BADARG, "unknown option");
     }

@@ -664,4 +664,76 @@ __device__ __forceinline__ void gj_eliminate_row(double* __restrict__ M, int n, 
     for (int q = threadIdx.x; q < w; q += blockDim.x) M[(long long)r * w + q] -= m * M[(long long)c * w + q];
 }
 
+// Column c of Gauss-Jordan on [A | I] (n rows of 2n) by the block that owns
+// matrix row r, with the pivot search in every block: the pivots of the
+// pivot/eliminate step pair (the largest |M[r][c]| over the rows not yet
+// pivots, ties to the first in the swapped order) and the same operations on
+// every element they keep, so the same bits.  The row swaps are a permutation
+// (perm_in -> perm_out, written by row 0's block; perm_in null: the
+// identity); the pivot row is divided by its pivot where it is read and, in
+// the matrix, by its owner one column later (dpiv[c] keeps each pivot); the
+// left half is updated right of the pivot column only (the columns left of it
+// are never read again), so no element one block writes is read by another
+// in the same step.  cand_in holds |M[row][c]| by matrix row (null: read
+// column c itself), and each row's owner writes |M[r][c+1]| after its update
+// into cand_out, so the next column's search reads n contiguous values
+// instead of a strided column.  256 threads; sval, sidx: 256 entries of LDS.
+__device__ __forceinline__ void gj_column(double* __restrict__ M, int n, int c, const int* __restrict__ perm_in,
+                                          int* __restrict__ perm_out, double* __restrict__ dpiv,
+                                          const double* __restrict__ cand_in, double* __restrict__ cand_out,
+                                          int* flag, int r, double* sval, int* sidx) {
+    const int tid = threadIdx.x;
+    const long long w = 2LL * n;
+    auto pin = [&](int i) { return perm_in ? perm_in[i] : i; };
+    double best = -1.0;
+    int bi = c;
+    for (int i = c + tid; i < n; i += 256) {
+        const int pr = pin(i);
+        const double a = cand_in ? cand_in[pr] : fabs(M[(long long)pr * w + c]);
+        if (a > best) { best = a; bi = i; }
+    }
+    sval[tid] = best;
+    sidx[tid] = bi;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (tid < st && (sval[tid + st] > sval[tid] || (sval[tid + st] == sval[tid] && sidx[tid + st] < sidx[tid]))) {
+            sval[tid] = sval[tid + st];
+            sidx[tid] = sidx[tid + st];
+        }
+        __syncthreads();
+    }
+    const int pi = sidx[0];
+    const int prow = pin(pi);
+    const double d = M[(long long)prow * w + c];
+    if (r == 0) {
+        for (int i = tid; i < n; i += 256) perm_out[i] = i == c ? prow : (i == pi ? pin(c) : pin(i));
+        if (tid == 0) {
+            dpiv[c] = d;
+            if (!(sval[0] > 0.0)) *flag = 1;
+        }
+    }
+    if (r == prow) return;
+    double* Mr = M + (long long)r * w;
+    const double* P = M + (long long)prow * w;
+    const int nl = n - 1 - c, nq = nl + n;  // columns kept: c+1 .. n-1 and n .. 2n-1
+    if (c > 0 && r == pin(c - 1)) {  // last column's pivot row: its division, then this column's elimination
+        const double dp = dpiv[c - 1];
+        const double m = Mr[c] / dp;
+        for (int t = tid; t < nq; t += 256) {
+            const int q = t < nl ? c + 1 + t : n + (t - nl);
+            const double s = Mr[q] / dp;
+            Mr[q] = m == 0.0 ? s : s - m * (P[q] / d);
+        }
+    } else {
+        const double m = Mr[c];
+        if (m != 0.0)
+            for (int t = tid; t < nq; t += 256) {
+                const int q = t < nl ? c + 1 + t : n + (t - nl);
+                Mr[q] -= m * (P[q] / d);
+            }
+    }
+    // thread 0 updated column c+1 of this row itself (t = 0)
+    if (tid == 0 && cand_out && nl > 0) cand_out[r] = fabs(Mr[c + 1]);
+}
+
 }  // namespace ela

@@ -508,58 +508,12 @@ __global__ __launch_bounds__(256) void k_gj_eliminate(double* __restrict__ M, in
 // element that one block writes is read by another in the same launch.  Block
 // r updates matrix row r.
 __global__ __launch_bounds__(256) void k_gj_col(double* __restrict__ M, int n, int c, const int* __restrict__ perm_in,
-                                                int* __restrict__ perm_out, double* __restrict__ dpiv, int* flag) {
+                                                int* __restrict__ perm_out, double* __restrict__ dpiv,
+                                                const double* __restrict__ cand_in, double* __restrict__ cand_out,
+                                                int* flag) {
     __shared__ double sval[256];
     __shared__ int sidx[256];
-    const int tid = threadIdx.x, r = blockIdx.x;
-    const long long w = 2LL * n;
-    auto pin = [&](int i) { return perm_in ? perm_in[i] : i; };
-    double best = -1.0;
-    int bi = c;
-    for (int i = c + tid; i < n; i += 256) {
-        const double a = fabs(M[(long long)pin(i) * w + c]);
-        if (a > best) { best = a; bi = i; }
-    }
-    sval[tid] = best;
-    sidx[tid] = bi;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-        if (tid < st && (sval[tid + st] > sval[tid] || (sval[tid + st] == sval[tid] && sidx[tid + st] < sidx[tid]))) {
-            sval[tid] = sval[tid + st];
-            sidx[tid] = sidx[tid + st];
-        }
-        __syncthreads();
-    }
-    const int pi = sidx[0];
-    const int prow = pin(pi);
-    const double d = M[(long long)prow * w + c];
-    if (r == 0) {
-        for (int i = tid; i < n; i += 256) perm_out[i] = i == c ? prow : (i == pi ? pin(c) : pin(i));
-        if (tid == 0) {
-            dpiv[c] = d;
-            if (!(sval[0] > 0.0)) *flag = 1;
-        }
-    }
-    if (r == prow) return;
-    double* Mr = M + (long long)r * w;
-    const double* P = M + (long long)prow * w;
-    const int nl = n - 1 - c, nq = nl + n;  // columns kept: c+1 .. n-1 and n .. 2n-1
-    if (c > 0 && r == pin(c - 1)) {  // last column's pivot row: its division, then this column's elimination
-        const double dp = dpiv[c - 1];
-        const double m = Mr[c] / dp;
-        for (int t = tid; t < nq; t += 256) {
-            const int q = t < nl ? c + 1 + t : n + (t - nl);
-            const double s = Mr[q] / dp;
-            Mr[q] = m == 0.0 ? s : s - m * (P[q] / d);
-        }
-    } else {
-        const double m = Mr[c];
-        if (m == 0.0) return;
-        for (int t = tid; t < nq; t += 256) {
-            const int q = t < nl ? c + 1 + t : n + (t - nl);
-            Mr[q] -= m * (P[q] / d);
-        }
-    }
+    gj_column(M, n, c, perm_in, perm_out, dpiv, cand_in, cand_out, flag, blockIdx.x, sval, sidx);
 }
 
 // After k_gj_col's last column: row i of A^-1 is the right half of matrix row
@@ -1554,10 +1508,11 @@ void launch_gauss_jordan(double* M, int n, int* flag, hipStream_t s) {
     hipLaunchKernelGGL(k_gj_left, dim3(n), dim3(256), 0, s, M, n);
 }
 
-void launch_gauss_jordan_col(double* M, int n, int* flag, int* perm, double* dpiv, hipStream_t s) {
+void launch_gauss_jordan_col(double* M, int n, int* flag, int* perm, double* dpiv, double* cand, hipStream_t s) {
     for (int c = 0; c < n; ++c)
         hipLaunchKernelGGL(k_gj_col, dim3(n), dim3(256), 0, s, M, n, c, c ? perm + (size_t)((c - 1) & 1) * n : nullptr,
-                           perm + (size_t)(c & 1) * n, dpiv, flag);
+                           perm + (size_t)(c & 1) * n, dpiv, c ? cand + (size_t)((c - 1) & 1) * n : nullptr,
+                           cand + (size_t)(c & 1) * n, flag);
     hipLaunchKernelGGL(k_gj_final, dim3(n), dim3(256), 0, s, M, n, perm + (size_t)((n - 1) & 1) * n, dpiv);
 }
 

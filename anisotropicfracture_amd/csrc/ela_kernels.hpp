@@ -97,10 +97,10 @@ void launch_probe_all(const Lay& L, int kind, int nodes, const double* C, double
 void launch_identity_right(double* M, int n, hipStream_t s);
 void launch_gauss_jordan(double* M, int n, int* flag, hipStream_t s);
 // Gauss-Jordan of [A | I] in one launch a column (k_gj_col): the same pivots
-// and bits as launch_gauss_jordan's two launches a column.  perm: 2n ints and
-// dpiv: n doubles of scratch.  Both leave row r of A^-1 in the left half of
-// row r.
-void launch_gauss_jordan_col(double* M, int n, int* flag, int* perm, double* dpiv, hipStream_t s);
+// and bits as launch_gauss_jordan's two launches a column.  Scratch: perm 2n
+// ints, dpiv n doubles, cand 2n doubles.  Both leave row r of A^-1 in the
+// left half of row r.
+void launch_gauss_jordan_col(double* M, int n, int* flag, int* perm, double* dpiv, double* cand, hipStream_t s);
 
 template <typename T>
 void launch_bottom(const Lay& L, int nodes, int dim, const double* M, const T* b, T* x, hipStream_t s);

@@ -1772,9 +1772,10 @@ int ela_prepare(ela_hier* h) {
     launch_identity_right(h->M, n, h->stream);
     int gj = 0;
     HIPCHK(hipMemsetAsync(h->dflag, 0, sizeof(int), h->stream));
-    double* G = nullptr;  // one-launch-a-column scratch: pivots (n doubles), two permutations (2n ints)
-    if (h->gj_one && halloc_t(h, 2 * (size_t)n, &G) == ELA_OK)
-        launch_gauss_jordan_col(h->M, n, h->dflag, (int*)(G + n), G, h->stream);
+    // one-launch-a-column scratch: pivots (n doubles), two permutations (2n ints), two candidate columns (2n doubles)
+    double* G = nullptr;
+    if (h->gj_one && halloc_t(h, 4 * (size_t)n, &G) == ELA_OK)
+        launch_gauss_jordan_col(h->M, n, h->dflag, (int*)(G + n), G, G + 2 * (size_t)n, h->stream);
     else
         launch_gauss_jordan(h->M, n, h->dflag, h->stream);
     HIPCHK(hipMemcpyAsync(&gj, h->dflag, sizeof gj, hipMemcpyDeviceToHost, h->stream));

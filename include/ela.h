@@ -698,7 +698,9 @@ int ela_server_stop(ela_server* s, int drain);
  * and replays it while the active set is unchanged; a change of the set (a
  * member admitted or retired) re-captures.  Cycles with a BiCGStab coarsest
  * level (their host reads the members' states) and timed cycles run on the
- * stream.  Same kernels in the same order: the same bits.  Default 0. */
+ * stream.  Same kernels in the same order: the same bits.  Default 0.
+ * ELA_OPT_GJ: the members' coarsest Gauss-Jordan in one launch a column (1,
+ * default) or in pivot/eliminate launch pairs (0); the same bits. */
 int ela_batch_set_option(ela_batch* b, int option, int value);
 /* Captures and replays of the batch's F-cycle graph since its creation (no
  * reference counterpart). */

@@ -412,6 +412,26 @@ def test_batch_autotune_keeps_the_bits():
         assert np.array_equal(a, c)
 
 
+@pytest.mark.parametrize("mode", [E.SOLVE_FP64, E.SOLVE_MIXED])
+def test_batched_gauss_jordan_one_launch_a_column_gives_the_launch_pair_bits(mode):
+    """ELA_OPT_GJ on a batch: the members' coarsest inverses by Gauss-Jordan in one launch a column (the pivot
+    search in every block over a compact candidate column, the row swaps as a permutation) or in pivot/eliminate
+    launch pairs give the same bits, so the batched solves do.  Members whose dense coarsest levels differ in
+    size (192, 648 and 1536 unknowns: 24, 40 and 28 cells a side), one with a crack factor."""
+    probs = [problem(81, 24, 6), problem(82, 40, 9, crack=True), problem(83, 28, 7)]
+    sols = []
+    for gj in (0, 1):
+        b = E.Batch(mode)
+        b.set_option(E.OPT_GJ, gj)
+        rs = [make(b, p) for p in probs]
+        b.prepare(rs)
+        b.solve(rs, tol_rel=1e-10, max_cycles=80)
+        sols.append([r.download(E.RVE.U) for r in rs])
+        b.close()
+    for a, c in zip(*sols):
+        assert np.array_equal(a, c)
+
+
 def test_mixed_batch_smooths_traction_boundaries_like_fp64():
     """The FP32 levels hold the inverse Jacobi diagonal: the marching launches' shell rows must multiply by it
     (a division there scaled the update of every traction boundary node by D^2).  With traction faces the
