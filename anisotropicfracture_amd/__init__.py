@@ -33,6 +33,7 @@ F64, F32 = 0, 1
 SOLVE_FP64, SOLVE_MIXED, ALLOC_BOTH = 0, 1, 2
 CYCLE_F, CYCLE_V = 0, 1
 OPT_MARCH, OPT_KZ, OPT_XCD, OPT_MASK, OPT_GRAPH, OPT_FOLD, OPT_ROWS, OPT_PREFETCH, OPT_GJ = 0, 1, 2, 3, 4, 5, 6, 7, 8
+OPT_GRAIN_DIAG = 9
 
 ERRORS = {1: "BADARG", 2: "OOM", 3: "HIP", 4: "NONCONVERGED", 5: "NAN", 6: "STATE"}
 
@@ -468,7 +469,9 @@ class Elastic:
         Krylov step writes the FP64 vector itself, default on; the same bits either way) or OPT_ROWS (rows per
         thread of level 0's FP32 marching tiles, 1 or 2; the same bits either way) or OPT_PREFETCH (planes a marching
         tile loads ahead, 1 or 2; the same bits either way) or OPT_GJ (the dense coarsest inverse by Gauss-Jordan in
-        one launch a column, default 1, or two; the same bits either way)."""
+        one launch a column, default 1, or two; the same bits either way) or OPT_GRAIN_DIAG (marching tiles of a
+        grain-form level take the interior Jacobi diagonal from the node's grain, default 1, or from the stored field;
+        the same bits either way)."""
         _check(lib().ela_set_option(self.ptr, option, int(value)))
 
     def autotune(self, nrep=10):

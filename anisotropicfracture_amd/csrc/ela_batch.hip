@@ -795,6 +795,7 @@ __device__ __forceinline__ GrainBox<T> grain_of(const BoxDesc& d, const MOps& op
     g.pf = 1;
     g.nt = false;
     g.xface = true;  // the x faces run in face patches (kf_grain)
+    g.dgt = nullptr;
     g.in = MODE == 2 ? (const T*)d.u64 : (const T*)d.f[op.in];
     g.rhs = MODE == 2 ? (const T*)d.b64 : (const T*)d.f[op.rhs];
     g.out = d.f[op.out];

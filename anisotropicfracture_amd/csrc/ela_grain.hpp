@@ -139,6 +139,14 @@ struct GrainBox {
     void* out;
     double* out2;
     const T* dg;
+    // Grain form without factors: the level's per-grain interior Jacobi
+    // diagonal (3 values a grain, as the smoothers hold it: FP32 the inverse),
+    // or null.  An interior node's diagonal is sum_b C_{p b p b} (-2 / h_b^2)
+    // of its own record (the first- and mixed-derivative stencils weigh the
+    // node itself by 0), so the tiles, whose nodes are all interior, take it
+    // from LDS beside the table instead of streaming dg; the shell and face
+    // nodes read dg.
+    const T* dgt;
     const T* hold;  // MODE 4
     const double* scale;
     T omega;
@@ -150,7 +158,8 @@ __device__ __forceinline__ auto box_acc(const GrainBox<T>& g) {
     else return PlaneAcc<T>{g.C, g.L.cs};
 }
 
-// The epilogue's operands of node o: rhs (MODES 0-2) and the diagonal (MODE 0).
+// The epilogue's operands of node o: rhs (MODES 0-2) and the diagonal (MODES 0
+// and 4, unless ldg is false: the caller takes it from the grain's record).
 // With g.nt (levels far larger than the caches) nothing reads them again
 // before they are evicted, so they are loaded non-temporally: they then leave
 // more of the L2 to the lines of u and of the grain ids that neighbouring
@@ -159,16 +168,16 @@ __device__ __forceinline__ auto box_acc(const GrainBox<T>& g) {
 // the caches between its sweeps, and the hint made its sweep slower (59 -> 67
 // us), so smaller levels load normally.
 template <typename T, int MODE>
-__device__ __forceinline__ void epilogue_load(const GrainBox<T>& g, long long o, T rb[3], T db[3]) {
+__device__ __forceinline__ void epilogue_load(const GrainBox<T>& g, long long o, T rb[3], T db[3], bool ldg = true) {
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
         const long long q = p * g.L.cs + o;
         if (g.nt) {
             rb[p] = MODE != 3 ? __builtin_nontemporal_load(g.rhs + q) : T(0);
-            db[p] = (MODE == 0 || MODE == 4) ? __builtin_nontemporal_load(g.dg + q) : T(1);
+            db[p] = (MODE == 0 || MODE == 4) && ldg ? __builtin_nontemporal_load(g.dg + q) : T(1);
         } else {
             rb[p] = MODE != 3 ? g.rhs[q] : T(0);
-            db[p] = (MODE == 0 || MODE == 4) ? g.dg[q] : T(1);
+            db[p] = (MODE == 0 || MODE == 4) && ldg ? g.dg[q] : T(1);
         }
     }
 }
@@ -238,8 +247,15 @@ __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchL
     const int tx = tid % MTX, ty = tid / MTX;
     const int i = i0 + tx;
     const int kb = 1 + bz * g.kz, ke = min(kb + g.kz, L.nz - 1);
-    if constexpr (GR)
+    // the interior diagonal from the grains' records (GrainBox::dgt), staged
+    // after the table; a box with factors reads dg
+    const bool tdg = GR && (MODE == 0 || MODE == 4) && g.dgt != nullptr && !(SC && g.sc != nullptr);
+    const T* sdg = GR ? stab + g.ngr * kGrainNC : nullptr;
+    if constexpr (GR) {
         for (int q = tid; q < g.ngr * kGrainNC; q += MTX * MTY) stab[q] = g.tab[q];
+        if (tdg)
+            for (int q = tid; q < g.ngr * 3; q += MTX * MTY) stab[g.ngr * kGrainNC + q] = g.dgt[q];
+    }
 
     // Plane loads.  Every thread reads its own RY nodes (i, j0 + RY ty + y) of
     // the tile for the three components, at 32-bit offsets from the plane's
@@ -366,11 +382,11 @@ __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchL
     if constexpr (PF == 2) {
         if (kb + 1 < ke) load(kb + 2, A);
         if (act[0]) {
-            epilogue_load<T, MODE>(g, ob0 + L.pl * kb, rb0, db0);
+            epilogue_load<T, MODE>(g, ob0 + L.pl * kb, rb0, db0, !tdg);
             rec_load(ob0 + L.pl * kb, cr, mk);
         }
         if constexpr (RY == 2)
-            if (act[RY - 1]) epilogue_load<T, MODE>(g, ob1 + L.pl * kb, rb1, db1);
+            if (act[RY - 1]) epilogue_load<T, MODE>(g, ob1 + L.pl * kb, rb1, db1, !tdg);
     }
     __syncthreads();
     double m = 0.0;
@@ -381,21 +397,21 @@ __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchL
             // the epilogue's loads are issued before the stencils, so their
             // latency overlaps them, as the next plane's loads do
             if (act[0]) {
-                epilogue_load<T, MODE>(g, o0, rb0, db0);
+                epilogue_load<T, MODE>(g, o0, rb0, db0, !tdg);
                 rec_load(o0, cr, mk);
             }
             if constexpr (RY == 2)
-                if (act[RY - 1]) epilogue_load<T, MODE>(g, o1, rb1, db1);
+                if (act[RY - 1]) epilogue_load<T, MODE>(g, o1, rb1, db1, !tdg);
             if (more) load(k + 2, A);
         } else {
             if (k + 2 < ke) load(k + 3, B);
             if (more) {
                 if (act[0]) {
-                    epilogue_load<T, MODE>(g, o0 + L.pl, nb0, nd0);
+                    epilogue_load<T, MODE>(g, o0 + L.pl, nb0, nd0, !tdg);
                     rec_load(o0 + L.pl, ncr, nmk);
                 }
                 if constexpr (RY == 2)
-                    if (act[RY - 1]) epilogue_load<T, MODE>(g, o1 + L.pl, nb1, nd1);
+                    if (act[RY - 1]) epilogue_load<T, MODE>(g, o1 + L.pl, nb1, nd1, !tdg);
             }
         }
         auto node = [&](int y, long long o, const T* rb, const T* db) {
@@ -410,6 +426,11 @@ __device__ __forceinline__ double march_tile(const GrainBox<T>& g, int t, MarchL
             }
 #pragma unroll
             for (int p = 0; p < 3; ++p) xc[p] = sh.su[3 * (s1 * PL) + tb[y] + p * PL];
+            if (tdg) {
+                const int gc = (int)sh.sg[s1 * PL + tb[y]] * 3;
+#pragma unroll
+                for (int p = 0; p < 3; ++p) d3[p] = sdg[gc + p];
+            }
             const double mm = grain_epilogue<T, MODE, TO>(g, o, xc, f, r3, d3);
             m = (mm > m || mm != mm) ? mm : m;
         };

@@ -1948,7 +1948,7 @@ void launch_grain(GrainBox<T> g, int kz, double* partials, unsigned long long* n
     g.kz = kz;
     const int nm = ntx * nty * ntz, nshp = (nsh + 7) / 8 * 8;
     const int nfp = g.xface ? (grain_face_patches(g.L) + 7) / 8 * 8 : 0;
-    const size_t lds = (size_t)g.ngr * kGrainNC * sizeof(T);
+    const size_t lds = (size_t)g.ngr * (kGrainNC + (g.dgt ? 3 : 0)) * sizeof(T);
     const dim3 grid(nshp + nfp + nm), block(MTX * MTY);
     constexpr int R2 = sizeof(T) == 4 ? 2 : 1;
     auto go = [&](auto sc, auto ry, auto pf) {

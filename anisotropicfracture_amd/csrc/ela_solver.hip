@@ -184,6 +184,7 @@ struct ela_hier {
     unsigned short* gid = nullptr;
     double *tab64 = nullptr, *dgt64 = nullptr;
     float *tab32 = nullptr, *dgt32 = nullptr;  // dgt32: inverse of the per-grain interior diagonal
+    // (dgt64, dgt32: one table of 3 values a grain for each MG level, at 3 ngrains l for level l)
     int ngrains = 0;
     bool has_table = false, use_table = false;
     // z-marching kernels for the grain-form level 0 (planes per tile), and the
@@ -194,6 +195,7 @@ struct ela_hier {
     int ry = 1;       // rows per thread of level 0's marching tiles (FP32 sweeps)
     int pf = 1;       // planes loaded ahead by the marching tiles (ELA_OPT_PREFETCH)
     bool gj_one = true;  // Gauss-Jordan of the dense coarsest operator in one launch a column (ELA_OPT_GJ)
+    bool tile_dgt = true;  // marching tiles take the interior diagonal per grain (ELA_OPT_GRAIN_DIAG)
     BCTab* dbc = nullptr;
     // dw = DDW : (grad u - F0) of Newton's right-hand side (dim*dim FP64 planes, allocated on first use)
     double* newton_dw = nullptr;
@@ -487,7 +489,8 @@ std::vector<unsigned long long> cycle_key(ela_hier* h, int kind) {
     unsigned long long om;
     std::memcpy(&om, &h->omega, sizeof om);
     k.push_back(om);
-    k.push_back(((unsigned long long)h->uniform << 8) | (unsigned long long)h->use_table);
+    k.push_back(((unsigned long long)h->tile_dgt << 16) | ((unsigned long long)h->uniform << 8) |
+                (unsigned long long)h->use_table);
     put(h->stream);
     put(h->M);
     put(h->partials);
@@ -934,6 +937,14 @@ hipStream_t strm(ela_hier* h, void* s) { return s ? (hipStream_t)s : h->stream; 
 
 template <typename T>
 TabDesc<T> tab_of(ela_hier* h);
+// Per-grain interior Jacobi diagonal of level v (the smoothers' form).
+template <typename T>
+const T* level_dgt(const ela_hier* h, const Level& v) {
+    const T* d;
+    if constexpr (sizeof(T) == 8) d = h->dgt64;
+    else d = h->dgt32;
+    return d ? d + (size_t)3 * h->ngrains * (size_t)(&v - &h->lev[0]) : nullptr;
+}
 template <>
 TabDesc<double> tab_of<double>(ela_hier* h) { return {h->gid, h->tab64, h->dgt64}; }
 template <>
@@ -955,7 +966,7 @@ template <typename T>
 TabDesc<T> coarse_tab(ela_hier* h, const Level& v) {
     TabDesc<T> t = tab_of<T>(h);
     t.gid = v.gid;
-    t.dgt = nullptr;  // the per-grain interior diagonal is level 0's
+    t.dgt = h->tile_dgt ? level_dgt<T>(h, v) : nullptr;
     return t;
 }
 // The grain form holds on every level of a grain-form hierarchy: the coarse
@@ -1056,6 +1067,7 @@ GrainBox<T> grain_box(ela_hier* h, const Level& v) {
     g.gid = level_gid(h, v);
     g.tab = tab_of<T>(h).tab;
     g.sc = nullptr;
+    g.dgt = h->tile_dgt ? level_dgt<T>(h, v) : nullptr;
     g.ngr = h->ngrains;
     g.xcd = h->xcd;
     g.ry = &v == &h->lev[0] ? h->ry : 1;
@@ -1531,8 +1543,9 @@ int ela_set_model_voronoi_cubic(ela_hier* h, int nseeds, const int64_t* seeds, i
     for (size_t l = 1; l < h->lev.size() && !rc; ++l) rc = halloc_t(h, (size_t)h->lev[l].L.cs, &h->lev[l].gid);
     if (!rc) rc = halloc_t(h, (size_t)21 * nseeds, &h->tab64);
     if (!rc) rc = halloc_t(h, (size_t)21 * nseeds, &h->tab32);
-    if (!rc) rc = halloc_t(h, (size_t)3 * nseeds, &h->dgt64);
-    if (!rc) rc = halloc_t(h, (size_t)3 * nseeds, &h->dgt32);
+    const size_t ndg = (size_t)3 * nseeds * h->lev.size();
+    if (!rc) rc = halloc_t(h, ndg, &h->dgt64);
+    if (!rc) rc = halloc_t(h, ndg, &h->dgt32);
     hipError_t e = hipSuccess;
     if (!rc) {
         e = hipMemcpyAsync(dseeds, seeds, sizeof(long long) * 3 * nseeds, hipMemcpyHostToDevice, h->stream);
@@ -1540,7 +1553,9 @@ int ela_set_model_voronoi_cubic(ela_hier* h, int nseeds, const int64_t* seeds, i
         if (e == hipSuccess) {
             launch_cubic_table(nseeds, (const double*)dR, C11, C12, C44, h->tab64, h->stream);
             launch_convert<double, float>(21LL * nseeds, h->tab64, h->tab32, h->stream);
-            launch_grain_diag(nseeds, h->tab64, v.h, h->dgt64, h->dgt32, h->stream);
+            for (size_t l = 0; l < h->lev.size(); ++l)  // the coarse levels' records are the grains' (injection)
+                launch_grain_diag(nseeds, h->tab64, h->lev[l].h, h->dgt64 + 3 * nseeds * l,
+                                  h->dgt32 + 3 * nseeds * l, h->stream);
             launch_voronoi_cubic(v.L, nseeds, (const long long*)dseeds, q, h->tab64, v.C64, h->gid, h->stream);
             for (size_t l = 1; l < h->lev.size(); ++l)
                 launch_inject_level<unsigned short>(h->lev[l].L, h->lev[l - 1].L, 1, level_gid(h, h->lev[l - 1]),
@@ -2164,6 +2179,8 @@ int ela_set_option(ela_hier* h, int option, int value) {
     } else if (option == ELA_OPT_GJ) {
         h->gj_one = value != 0;
         h->prepared = false;
+    } else if (option == ELA_OPT_GRAIN_DIAG) {
+        h->tile_dgt = value != 0;
     } else {
         return fail(ELA_ERR_BADARG, "unknown option");
     }

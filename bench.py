@@ -472,8 +472,11 @@ def pmc_traffic(fetch_csv, write_csv, n, tile, out_path):
     import re
     nodes = (n + 1) ** 3
     probe = float(1 << 30)
-    # (pattern on the kernel name, description, algorithmic bytes per node); level 0 only (grid size filter)
-    kinds = [(r"k_grain_sweep<float, 0,", "k_grain_sweep<float,0> (level-0 Jacobi sweep, grain form: shell + marching tiles)", 4 * 12 + 2),
+    # (pattern on the kernel name, description, algorithmic bytes per node); level 0 only (grid size filter).
+    # The Jacobi sweep reads x, b and the grain id and writes x; the diagonal of an interior node comes from
+    # its grain (LDS), the shell's nodes read theirs (12 B more on (n+1)^3 - (n-1)^3 nodes).
+    jac_node = 4 * 9 + 2 + 12.0 * ((n + 1) ** 3 - (n - 1) ** 3) / nodes
+    kinds = [(r"k_grain_sweep<float, 0,", "k_grain_sweep<float,0> (level-0 Jacobi sweep, grain form: shell + marching tiles)", jac_node),
              (r"k_grain_sweep<double, 3,", "k_grain_sweep<double,3> (level-0 Fapply FP64, grain form)", 8 * 6 + 2),
              (r"k_grain_sweep<float, 3,", "k_grain_sweep<float,3> (level-0 Fapply FP32, grain form)", 4 * 6 + 2),
              (r"k_fapply<double, 1, 3, ela::(Masked)?PlaneAcc", "k_fapply<double,MM,3,planes> (level-0 Fapply FP64, per-node planes)", 8 * 27),
@@ -723,9 +726,12 @@ def main():
         bytes_fa64 = 8 * (3 + 3 + nc) * nodes
         bytes_fa32 = 4 * (3 + 3 + nc) * nodes
         # dominant kernel of the mixed solve: level-0 FP32 Jacobi sweep vs level-0 FP64 residual.
-        # Jacobi bytes per node: x in, b, diagonal, x out (3 comps each) + the stiffness: 21 planes, or a
-        # 2-byte grain id (the grain records sit in LDS / L1, read once per block)
-        bytes_jac32 = (4 * (4 * 3 + nc) if not args.table else 4 * 12 + 2) * nodes
+        # Jacobi bytes per node: x in, b, diagonal, x out (3 comps each) + the stiffness: 21 planes; or in
+        # grain form x in, b, x out + a 2-byte grain id (the grain records and an interior node's diagonal,
+        # which depends on its own record only, sit in LDS, read once per block), the diagonal field read
+        # on the shell's nodes only
+        shell = nodes - (n - 1) ** 3
+        bytes_jac32 = (4 * (4 * 3 + nc)) * nodes if not args.table else (4 * 9 + 2) * nodes + 12 * shell
         # x, b, C in FP64 + scaled FP32 rhs out (plain MG) / A z: z, C in, w out (FGMRES)
         cbytes = 8 * nc if not args.table else 2
         bytes_res64 = ((8 * 6 + cbytes + 4 * 3) if not args.krylov else 8 * 6 + cbytes) * nodes

@@ -360,7 +360,14 @@ int ela_set_krylov(ela_hier* h, int restart);
  *                by Gauss-Jordan in one launch a column (every block finds
  *                the pivot itself; the row swaps are kept as a permutation);
  *                0: a pivot launch and an elimination launch a column.  Same
- *                pivots and bits either way.  Hierarchies only. */
+ *                pivots and bits either way.  Hierarchies only.
+ * ELA_OPT_GRAIN_DIAG 1 (default): the marching tiles of a grain-form level,
+ *                and the grain-table node kernels of its coarse levels, take
+ *                an interior node's Jacobi diagonal from its grain (3 values a
+ *                grain and level; the tiles stage them in LDS with the table:
+ *                sum_b C_{p b p b} (-2 / h_b^2) depends on the node's own
+ *                record only) instead of reading the stored diagonal field;
+ *                0: they read the field.  Same bits.  Hierarchies only. */
 enum {
     ELA_OPT_MARCH = 0,
     ELA_OPT_KZ = 1,
@@ -370,7 +377,8 @@ enum {
     ELA_OPT_FOLD = 5,
     ELA_OPT_ROWS = 6,
     ELA_OPT_PREFETCH = 7,
-    ELA_OPT_GJ = 8
+    ELA_OPT_GJ = 8,
+    ELA_OPT_GRAIN_DIAG = 9
 };
 int ela_set_option(ela_hier* h, int option, int value);
 

@@ -306,12 +306,13 @@ def test_grain_table_solve_matches_oracle(oracle, mode, krylov):
 @pytest.mark.parametrize("krylov", [0, 16])
 def test_grain_marching_kernels_give_the_node_kernel_bits(mode, krylov):
     """Level 0 in grain form through the LDS z-marching kernels (one and two rows per thread, one row with
-    two planes loaded ahead) and through the one-thread-per-node kernels: the same arithmetic, so the same
-    solve and the same Fapply bit for bit (a 40^3 box has partial tiles in x, y, z)."""
+    two planes loaded ahead; the tiles' interior Jacobi diagonal from the grains' records, and from the
+    stored field with OPT_GRAIN_DIAG 0) and through the one-thread-per-node kernels: the same arithmetic,
+    so the same solve and the same Fapply bit for bit (a 40^3 box has partial tiles in x, y, z)."""
     cfg = K.polycrystal(40, grains=14, seed=5)
     u1 = H.random_field(np.random.default_rng(8), cfg["n"], 3)
     sols, iters, fa = [], [], []
-    for march, rows, pf in ((0, 1, 1), (1, 1, 1), (1, 2, 1), (1, 1, 2)):
+    for march, rows, pf, gd in ((0, 1, 1, 1), (1, 1, 1, 1), (1, 2, 1, 1), (1, 1, 2, 1), (1, 1, 1, 0)):
         op = ela.Elastic(cfg["n"], MM, mode=ela.ALLOC_BOTH)
         op.SetModelVoronoiCubic(cfg["seeds"], K.VORONOI_Q, cfg["R"], cfg["C11"], cfg["C12"], cfg["C44"])
         op.SetBC(cfg["bc_type"], cfg["bc_value"])
@@ -320,6 +321,7 @@ def test_grain_marching_kernels_give_the_node_kernel_bits(mode, krylov):
         op.set_option(ela.OPT_MARCH, march)
         op.set_option(ela.OPT_ROWS, rows)
         op.set_option(ela.OPT_PREFETCH, pf)
+        op.set_option(ela.OPT_GRAIN_DIAG, gd)
         op.prepareForSolve()
         fa.append((op.Fapply(u1, prec=ela.F64), op.Fapply(u1, prec=ela.F32)))
         b = op.bc_init()
@@ -328,10 +330,33 @@ def test_grain_marching_kernels_give_the_node_kernel_bits(mode, krylov):
         sols.append(u)
         iters.append(st["iters"])
         op.close()
-    for v in (1, 2, 3):
+    for v in (1, 2, 3, 4):
         assert iters[0] == iters[v]
         assert np.array_equal(sols[0], sols[v])
         assert np.array_equal(fa[0][0], fa[v][0]) and np.array_equal(fa[0][1], fa[v][1])
+
+
+@pytest.mark.parametrize("mode", [ela.SOLVE_FP64, ela.SOLVE_MIXED])
+def test_per_grain_diagonal_keeps_the_bits_on_coarse_marching_levels(mode):
+    """A 256^3 polycrystal marches on level 1 (129^3) too, with its own spacing: the tiles' interior Jacobi
+    diagonal per grain and level (OPT_GRAIN_DIAG 1) gives the bits of the stored diagonal field (0),
+    cycle for cycle, over ten FGMRES iterations."""
+    cfg = K.polycrystal(256, grains=64, seed=2)
+    out = []
+    for gd in (0, 1):
+        op = ela.Elastic(cfg["n"], MM, mode=ela.ALLOC_BOTH)
+        op.SetModelVoronoiCubic(cfg["seeds"], K.VORONOI_Q, cfg["R"], cfg["C11"], cfg["C12"], cfg["C44"])
+        op.SetBC(cfg["bc_type"], cfg["bc_value"])
+        op.SetStiffnessMode(1)
+        op.set_option(ela.OPT_GRAIN_DIAG, gd)
+        op.prepareForSolve()
+        b = op.bc_init()
+        u, st = ela.Linear(op).solve(np.zeros_like(b), b, tol_rel=1e-30, tol_abs=0.0, mode=mode, max_iter=10,
+                                   raise_on_fail=False)
+        out.append((u, st["res"]))
+        op.close()
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][0], out[1][0])
 
 
 @pytest.mark.parametrize("mode", [ela.SOLVE_FP64, ela.SOLVE_MIXED])
