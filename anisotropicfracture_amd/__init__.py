@@ -1091,7 +1091,9 @@ class Batch:
         """ela_batch_set_option: OPT_MARCH (LDS z-marching kernels on the grain-form levels, default on),
         OPT_KZ (planes per marching tile, default 16), OPT_XCD (tile order), OPT_GRAPH (each step's F-cycle
         replayed from a captured HIP graph while the active set is unchanged, default off) or OPT_GJ (the members'
-        coarsest Gauss-Jordan in one launch a column, default 1, or two; the same bits)."""
+        coarsest Gauss-Jordan in one launch a column, default 1, or two; the same bits) or OPT_GRAIN_DIAG (the
+        marching tiles of members without factors take the interior Jacobi diagonal per grain, default 1, or read
+        the stored field; the same bits)."""
         _check(lib().ela_batch_set_option(self.ptr, option, int(value)))
 
     def graph_stats(self):

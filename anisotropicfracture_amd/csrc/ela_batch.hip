@@ -90,6 +90,9 @@ struct BoxDesc {
     const float* sc32;
     double* d64;  // Jacobi diagonal in the cycle precision (one of d64 / d32)
     float* d32;   // FP32: its inverse
+    // members without factors: the level's per-grain interior Jacobi diagonal (3 values a grain, the cycle
+    // precision, FP32 the inverse), which the marching tiles read instead of d64 / d32; else null
+    void* dgt;
     void* f[F_N];  // cycle fields, cycle precision
     double* u64;   // finest level: solution, right-hand side, residual / test output
     double* b64;
@@ -110,6 +113,7 @@ struct P<double> {
     __device__ static const double* tab(const BoxDesc& d) { return d.tab64; }
     __device__ static const double* sc(const BoxDesc& d) { return d.sc64; }
     __device__ static const double* dg(const BoxDesc& d) { return d.d64; }
+    __device__ static const double* dgt(const BoxDesc& d) { return (const double*)d.dgt; }
     __device__ static const Inv<double>& inv(const BoxDesc& d) { return d.i64; }
 };
 template <>
@@ -117,6 +121,7 @@ struct P<float> {
     __device__ static const float* tab(const BoxDesc& d) { return d.tab32; }
     __device__ static const float* sc(const BoxDesc& d) { return d.sc32; }
     __device__ static const float* dg(const BoxDesc& d) { return d.d32; }
+    __device__ static const float* dgt(const BoxDesc& d) { return (const float*)d.dgt; }
     __device__ static const Inv<float>& inv(const BoxDesc& d) { return d.i32; }
 };
 
@@ -774,6 +779,7 @@ struct MOps {
     double* partials;
     int mtot;  // marching blocks of the launch
     int xcd;   // XCD-contiguous tile order (else hardware order)
+    int gdiag = 0;  // Jacobi sweeps: the tiles take the interior diagonal per grain (BoxDesc::dgt)
 };
 
 template <typename T, int MODE>
@@ -795,7 +801,7 @@ __device__ __forceinline__ GrainBox<T> grain_of(const BoxDesc& d, const MOps& op
     g.pf = 1;
     g.nt = false;
     g.xface = true;  // the x faces run in face patches (kf_grain)
-    g.dgt = nullptr;
+    g.dgt = MODE == 0 && op.gdiag ? P<T>::dgt(d) : nullptr;  // a sweep of the cycle precision
     g.in = MODE == 2 ? (const T*)d.u64 : (const T*)d.f[op.in];
     g.rhs = MODE == 2 ? (const T*)d.b64 : (const T*)d.f[op.rhs];
     g.out = d.f[op.out];
@@ -939,6 +945,21 @@ __global__ __launch_bounds__(TB) void kb_inject(const BoxDesc* __restrict__ Dc, 
     ((unsigned short*)c.gid)[oc] = f.gid[of];
     if (c.sc64) ((double*)c.sc64)[oc] = f.sc64[of];
     if (c.sc32) ((float*)c.sc32)[oc] = f.sc32[of];
+}
+
+// Per-grain interior Jacobi diagonal of each box of the list that keeps one
+// (BoxDesc::dgt; grain_diag, as the single box's tables), in the cycle
+// precision.  Grid (boxes), one thread a grain.
+__global__ __launch_bounds__(256) void kb_grain_diag(const BoxDesc* __restrict__ D, int mixed) {
+    const BoxDesc& d = D[blockIdx.x];
+    if (!d.dgt) return;
+    for (int gi = threadIdx.x; gi < d.ngr; gi += blockDim.x)
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const double dg = grain_diag(d.tab64 + (long long)kNC * gi, d.h, p);
+            if (mixed) store_jacobi_diag((float*)d.dgt + 3 * gi + p, dg);  // the inverse
+            else store_jacobi_diag((double*)d.dgt + 3 * gi + p, dg);
+        }
 }
 
 // Jacobi diagonal (Elastic.cpp:264-352) from the FP64 stiffness, stored in the cycle precision.
@@ -1114,6 +1135,7 @@ struct RLev {
     float* sc32 = nullptr;
     double* d64 = nullptr;
     float* d32 = nullptr;
+    void* dgt = nullptr;  // members without factors: per-grain interior Jacobi diagonal (BoxDesc::dgt)
     void* f[F_N] = {};
 };
 
@@ -1196,6 +1218,7 @@ struct ela_batch {
     // when the launch lists change (list_epoch), replayed otherwise
     bool graph_on = false;
     bool gj_one = true;  // ELA_OPT_GJ: the members' coarsest Gauss-Jordan in one launch a column
+    bool gdiag = true;   // ELA_OPT_GRAIN_DIAG: the tiles' interior Jacobi diagonal per grain
     long long list_epoch = 0;
     hipGraphExec_t gexec = nullptr;
     long long gexec_epoch = -1;
@@ -1272,6 +1295,7 @@ size_t layout_member(ela_rve* r, char* base) {
         }
         if (mixed) v.d32 = (float*)take(3 * N * 4);
         else v.d64 = (double*)take(3 * N * 8);
+        v.dgt = r->factor ? nullptr : take((size_t)r->max_grains * 3 * ts);
         for (int q = 0; q < F_N; ++q) v.f[q] = take(3 * N * ts);
     }
     r->M = (double*)take((size_t)r->nunk * r->nunk * 8);
@@ -1313,6 +1337,7 @@ BoxDesc desc_of(const ela_rve* r, int l) {
     }
     d.d64 = v.d64;
     d.d32 = v.d32;
+    d.dgt = v.dgt;
     for (int q = 0; q < F_N; ++q) d.f[q] = v.f[q];
     if (l == r->depth - 1) {
         d.nunk = r->nunk;
@@ -1533,7 +1558,7 @@ bool march_on(const ela_batch* B, int g, int count) {
 
 template <typename T, int MODE, typename TO>
 void march_launch(ela_batch* B, const BoxDesc* D, const ela_batch::March& mi, const MOps& op) {
-    const size_t lds = (size_t)mi.maxng * kNC * sizeof(T);
+    const size_t lds = (size_t)mi.maxng * (kNC + 3) * sizeof(T);  // the records, then the per-grain diagonal
     const int nsp = (mi.ns + 7) / 8 * 8, nfp = (mi.nf + 7) / 8 * 8;
     const int* bm = (const int*)(B->dlists + mi.off_bm);
     const int* bs = (const int*)(B->dlists + mi.off_bs);
@@ -1556,7 +1581,7 @@ void smooth(ela_batch* B, int g, int count, int rhs, int in, int out) {
         (void)hipEventRecord(B->ev[e], B->stream);
     }
     if (in >= 0 && march_on(B, g, count))
-        march_launch<T, 0, T>(B, LD(B, g), march_prefix(B, g, count), MOps{rhs, in, out, 2.0 / 3.0, nullptr, 0, B->xcd});
+        march_launch<T, 0, T>(B, LD(B, g), march_prefix(B, g, count), MOps{rhs, in, out, 2.0 / 3.0, nullptr, 0, B->xcd, B->gdiag ? 1 : 0});
     else
         hipLaunchKernelGGL((kb_smooth<T>), dim3(nb), dim3(TB), 0, B->stream, LD(B, g), LM(B, g), 0, rhs, in, out,
                            (T)(2.0 / 3.0));
@@ -1950,6 +1975,7 @@ int prepare(ela_batch* B, int count, ela_rve* const* rves) {
         for (int g = 0; g < G; ++g) {
             const int c = (int)L[g].descs.size();
             hipLaunchKernelGGL(kb_diag, dim3(nb(g, c)), dim3(TB), 0, B->stream, D(g), M(g), B->dflag);
+            hipLaunchKernelGGL(kb_grain_diag, dim3(c), dim3(64), 0, B->stream, D(g), B->mode == ELA_SOLVE_MIXED ? 1 : 0);
         }
         // coarsest inverse: probe A, Gauss-Jordan on [A | I], keep A^-1
         const int nbox = (int)L[G - 1].descs.size();
@@ -2606,6 +2632,8 @@ int ela_batch_set_option(ela_batch* B, int option, int value) {
         B->graph_on = value != 0;
     } else if (option == ELA_OPT_GJ) {
         B->gj_one = value != 0;
+    } else if (option == ELA_OPT_GRAIN_DIAG) {
+        B->gdiag = value != 0;
     } else {
         return fail(ELA_ERR_BADARG, "unknown option");
     }

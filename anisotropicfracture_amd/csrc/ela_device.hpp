@@ -557,6 +557,15 @@ __device__ __forceinline__ void node_diag_acc(const Lay& L, const Acc& A, int i,
     }
 }
 
+// The interior diagonal of component p from a major/minor record r alone
+// (node_diag_acc's interior branch, the same operations): a grain's, per level.
+__device__ __forceinline__ double grain_diag(const double* r, const double h[3], int p) {
+    double acc = 0;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) acc += r[mm_slot<3>(p, b, p, b)] * (-2.0 / h[b] / h[b]);
+    return acc;
+}
+
 template <int K, int DIM>
 __device__ __forceinline__ void node_diag(const Lay& L, const double* __restrict__ C, int i, int j, int k,
                                           const BCTab& bc, const double h[3], double dg[DIM]) {

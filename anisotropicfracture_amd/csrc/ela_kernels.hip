@@ -2054,8 +2054,7 @@ __global__ void k_grain_diag(int ng, const double* __restrict__ tab, double hx, 
     if (gi >= ng) return;
     const double h[3] = {hx, hy, hz};
     for (int p = 0; p < 3; ++p) {
-        double acc = 0;
-        for (int b = 0; b < 3; ++b) acc += tab[21 * gi + mm_slot<3>(p, b, p, b)] * (-2.0 / h[b] / h[b]);
+        const double acc = grain_diag(tab + 21 * gi, h, p);
         if (d64) d64[3 * gi + p] = acc;
         if (d32) store_jacobi_diag(d32 + 3 * gi + p, acc);  // the inverse
     }

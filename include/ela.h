@@ -708,7 +708,10 @@ int ela_server_stop(ela_server* s, int drain);
  * level (their host reads the members' states) and timed cycles run on the
  * stream.  Same kernels in the same order: the same bits.  Default 0.
  * ELA_OPT_GJ: the members' coarsest Gauss-Jordan in one launch a column (1,
- * default) or in pivot/eliminate launch pairs (0); the same bits. */
+ * default) or in pivot/eliminate launch pairs (0); the same bits.
+ * ELA_OPT_GRAIN_DIAG: the marching tiles of members without factors take an
+ * interior node's Jacobi diagonal from its grain (1, default) or read the
+ * stored diagonal field (0); the same bits. */
 int ela_batch_set_option(ela_batch* b, int option, int value);
 /* Captures and replays of the batch's F-cycle graph since its creation (no
  * reference counterpart). */

@@ -432,6 +432,26 @@ def test_batched_gauss_jordan_one_launch_a_column_gives_the_launch_pair_bits(mod
         assert np.array_equal(a, c)
 
 
+@pytest.mark.parametrize("mode", [E.SOLVE_FP64, E.SOLVE_MIXED])
+def test_batched_per_grain_diagonal_keeps_the_bits(mode):
+    """ELA_OPT_GRAIN_DIAG on a batch: the marching tiles of members without factors take an interior node's
+    Jacobi diagonal from its grain (per member and level), or read the stored diagonal field; the solves
+    agree bit for bit.  A member with a crack factor (which reads the field either way) rides along."""
+    probs = [problem(84, 32, 6), problem(85, 40, 9, crack=True), problem(86, 48, 7)]
+    sols = []
+    for gd in (0, 1):
+        b = E.Batch(mode)
+        b.set_option(E.OPT_GRAIN_DIAG, gd)
+        rs = [make(b, p) for p in probs]
+        b.prepare(rs)
+        sts = b.solve(rs, tol_rel=1e-10, max_cycles=80)
+        sols.append([(r.download(E.RVE.U), st["iters"]) for r, st in zip(rs, sts)])
+        b.close()
+    for (a, ca), (c, cc) in zip(*sols):
+        assert ca == cc
+        assert np.array_equal(a, c)
+
+
 def test_mixed_batch_smooths_traction_boundaries_like_fp64():
     """The FP32 levels hold the inverse Jacobi diagonal: the marching launches' shell rows must multiply by it
     (a division there scaled the update of every traction boundary node by D^2).  With traction faces the
