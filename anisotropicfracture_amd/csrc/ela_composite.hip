@@ -708,9 +708,9 @@ int solve(ela_amr* a, double tol_rel, double tol_abs, int max_iter, ela_stats* s
                 double* const norm2 = a->kdev + 2 * (kMaxKrylov + 2);  // beyond the coefficients
                 if (!(hn2 > kReorth2 * wn0 * wn0)) {
                     if (b32) {
-                        launch_maxpy32(w, Vl32, j + 1, a->kdev, -1.0, n, s);
+                        launch_maxpy32(w, Vl32, j + 1, a->kdev, coef_ones(), -1.0, n, s);
                         launch_mdot_self(w, Vl32, j + 1, n, a->kpart, a->kdev, s);
-                        launch_maxpy_norm32(w, Vl32, j + 1, a->kdev, -1.0, n, a->kpart, norm2, s);
+                        launch_maxpy_norm32(w, Vl32, j + 1, a->kdev, coef_ones(), -1.0, n, a->kpart, norm2, s);
                     } else {
                         launch_maxpy(w, Vl, j + 1, a->kdev, -1.0, n, s);
                         launch_mdot(w, Vl, j + 1, n, a->kpart, a->kdev, s);
@@ -718,7 +718,7 @@ int solve(ela_amr* a, double tol_rel, double tol_abs, int max_iter, ela_stats* s
                     }
                     h2 = host_vec(a, j + 1);
                 } else {
-                    if (b32) launch_maxpy_norm32(w, Vl32, j + 1, a->kdev, -1.0, n, a->kpart, norm2, s);
+                    if (b32) launch_maxpy_norm32(w, Vl32, j + 1, a->kdev, coef_ones(), -1.0, n, a->kpart, norm2, s);
                     else launch_maxpy_norm(w, Vl, j + 1, a->kdev, -1.0, n, a->kpart, norm2, s);
                 }
                 double hn2_dev = 0.0;

@@ -871,12 +871,14 @@ __global__ __launch_bounds__(256) void k_mdot_final(const double* __restrict__ p
     if (threadIdx.x == 0) out[blockIdx.x] = sm[0];
 }
 
-// y += sign * sum_v coef[v] X_v; NORM: also the block partials of |y|^2 (the
-// new y), reduced as k_mdot reduces (x, x): partials[block].
-template <typename TB, int V, bool NORM>
+// y += sign * sum_v coef[v] mul[v] X_v; NORM: also the block partials of |y|^2
+// (the new y), reduced as k_mdot reduces (x, x): partials[block].  OUTF: the
+// new y goes to o1 and o2 (may be null) as (float) (osc y) instead of into y.
+template <typename TB, int V, bool NORM, bool OUTF = false>
 __global__ __launch_bounds__(FB) void k_maxpy(double* __restrict__ y, VecListT<TB> X, int nv,
-                                              const double* __restrict__ coef, double sign, long long n,
-                                              double* __restrict__ partials) {
+                                              const double* __restrict__ coef, CoefMul mul, double sign, long long n,
+                                              double* __restrict__ partials, double osc, float* __restrict__ o1,
+                                              float* __restrict__ o2) {
     double acc = 0.0;
     auto step = [&](long long q, auto vw) {
         constexpr int W = decltype(vw)::value;
@@ -893,7 +895,7 @@ __global__ __launch_bounds__(FB) void k_maxpy(double* __restrict__ y, VecListT<T
 #pragma unroll
             for (int v = 0; v < CH; ++v)
                 if (v0 + v < nv) {
-                    const double c = coef[v0 + v];
+                    const double c = coef[v0 + v] * mul.f[v0 + v];
 #pragma unroll
                     for (int e = 0; e < W; ++e) a[e] += c * (double)xq[v][e];
                 }
@@ -903,7 +905,15 @@ __global__ __launch_bounds__(FB) void k_maxpy(double* __restrict__ y, VecListT<T
             yq[e] = yq[e] + sign * a[e];
             if (NORM) acc += yq[e] * yq[e];
         }
-        stv<W>(y, q, yq);
+        if constexpr (OUTF) {
+            float f[W];
+#pragma unroll
+            for (int e = 0; e < W; ++e) f[e] = (float)(osc * yq[e]);
+            stv<W>(o1, q, f);
+            if (o2) stv<W>(o2, q, f);
+        } else {
+            stv<W>(y, q, yq);
+        }
     };
     ELA_FLAT_LOOP(V, n, q) step(q, std::integral_constant<int, V>{});
     ELA_FLAT_TAIL(V, n, q) step(q, std::integral_constant<int, 1>{});
@@ -1004,34 +1014,43 @@ void launch_mdot_self(const double* x, const VecList32& Y, int nv, long long n, 
 }
 
 namespace {
-template <typename TB, bool NORM>
-void maxpy_any(double* y, const VecListT<TB>& X, int nv, const double* coef, double sign, long long n,
-               double* partials, double* norm2, hipStream_t s) {
-    if (vec_ok(y, X, nv))
-        hipLaunchKernelGGL((k_maxpy<TB, 2, NORM>), dim3(FG), dim3(FB), 0, s, y, X, nv, coef, sign, n, partials);
+template <typename TB, bool NORM, bool OUTF = false>
+void maxpy_any(double* y, const VecListT<TB>& X, int nv, const double* coef, const CoefMul& mul, double sign,
+               long long n, double* partials, double* norm2, hipStream_t s, double osc = 1.0, float* o1 = nullptr,
+               float* o2 = nullptr) {
+    if (vec_ok(y, X, nv) && vec_ok({o1, o2}))
+        hipLaunchKernelGGL((k_maxpy<TB, 2, NORM, OUTF>), dim3(FG), dim3(FB), 0, s, y, X, nv, coef, mul, sign, n,
+                           partials, osc, o1, o2);
     else
-        hipLaunchKernelGGL((k_maxpy<TB, 1, NORM>), dim3(FG), dim3(FB), 0, s, y, X, nv, coef, sign, n, partials);
+        hipLaunchKernelGGL((k_maxpy<TB, 1, NORM, OUTF>), dim3(FG), dim3(FB), 0, s, y, X, nv, coef, mul, sign, n,
+                           partials, osc, o1, o2);
     if (NORM) hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(256), 0, s, partials, norm2);
 }
 }  // namespace
 
 void launch_maxpy(double* y, const VecList& X, int nv, const double* coef, double sign, long long n, hipStream_t s) {
-    maxpy_any<double, false>(y, X, nv, coef, sign, n, nullptr, nullptr, s);
+    maxpy_any<double, false>(y, X, nv, coef, coef_ones(), sign, n, nullptr, nullptr, s);
 }
 
 void launch_maxpy_norm(double* y, const VecList& X, int nv, const double* coef, double sign, long long n,
                        double* partials, double* norm2, hipStream_t s) {
-    maxpy_any<double, true>(y, X, nv, coef, sign, n, partials, norm2, s);
+    maxpy_any<double, true>(y, X, nv, coef, coef_ones(), sign, n, partials, norm2, s);
 }
 
-void launch_maxpy32(double* y, const VecList32& X, int nv, const double* coef, double sign, long long n,
-                    hipStream_t s) {
-    maxpy_any<float, false>(y, X, nv, coef, sign, n, nullptr, nullptr, s);
+void launch_maxpy32(double* y, const VecList32& X, int nv, const double* coef, const CoefMul& mul, double sign,
+                    long long n, hipStream_t s) {
+    maxpy_any<float, false>(y, X, nv, coef, mul, sign, n, nullptr, nullptr, s);
 }
 
-void launch_maxpy_norm32(double* y, const VecList32& X, int nv, const double* coef, double sign, long long n,
-                         double* partials, double* norm2, hipStream_t s) {
-    maxpy_any<float, true>(y, X, nv, coef, sign, n, partials, norm2, s);
+void launch_maxpy_norm32(double* y, const VecList32& X, int nv, const double* coef, const CoefMul& mul,
+                         double sign, long long n, double* partials, double* norm2, hipStream_t s) {
+    maxpy_any<float, true>(y, X, nv, coef, mul, sign, n, partials, norm2, s);
+}
+
+void launch_maxpy_norm32_to2f(const double* y, const VecList32& X, int nv, const double* coef, const CoefMul& mul,
+                              double sign, long long n, double osc, float* o1, float* o2, double* partials,
+                              double* norm2, hipStream_t s) {
+    maxpy_any<float, true, true>(const_cast<double*>(y), X, nv, coef, mul, sign, n, partials, norm2, s, osc, o1, o2);
 }
 
 void launch_scale_flat(double* y, const double* x, double a, long long n, hipStream_t s) {

@@ -197,6 +197,17 @@ struct VecListT {
 };
 using VecList = VecListT<double>;
 using VecList32 = VecListT<float>;
+// Per-vector multipliers of an update's coefficients (coef[v] * f[v]): an FP32
+// Krylov basis may hold vector v scaled by s_v, whose coefficient is then
+// taken times 1 / s_v^2.  A multiplier of 1 leaves the coefficient's bits.
+struct CoefMul {
+    double f[kMaxKrylov];
+};
+inline CoefMul coef_ones() {
+    CoefMul c;
+    for (double& x : c.f) x = 1.0;
+    return c;
+}
 // out[v] = (x, Y_v) for v < nv (deterministic two-pass reduction; partials: mdot_partials(nv) doubles).
 void launch_mdot(const double* x, const VecList& Y, int nv, long long n, double* partials, double* out,
                  hipStream_t s);
@@ -211,11 +222,18 @@ void launch_maxpy(double* y, const VecList& X, int nv, const double* coef, doubl
 // order launch_mdot sums (y, y) (partials: mdot_partials(1) doubles).
 void launch_maxpy_norm(double* y, const VecList& X, int nv, const double* coef, double sign, long long n,
                        double* partials, double* norm2, hipStream_t s);
-// The same with an FP32 list (FP64 arithmetic).
-void launch_maxpy32(double* y, const VecList32& X, int nv, const double* coef, double sign, long long n,
-                    hipStream_t s);
-void launch_maxpy_norm32(double* y, const VecList32& X, int nv, const double* coef, double sign, long long n,
-                         double* partials, double* norm2, hipStream_t s);
+// The same with an FP32 list (FP64 arithmetic), the coefficients times mul.
+void launch_maxpy32(double* y, const VecList32& X, int nv, const double* coef, const CoefMul& mul, double sign,
+                    long long n, hipStream_t s);
+void launch_maxpy_norm32(double* y, const VecList32& X, int nv, const double* coef, const CoefMul& mul,
+                         double sign, long long n, double* partials, double* norm2, hipStream_t s);
+// launch_maxpy_norm32 whose FP64 result r is not stored back into y: o1 and o2
+// (o2 may be null) receive (float) (osc r), osc a power of two (exact scaling),
+// and *norm2 = |r|^2 (the next FP32 Krylov vector and the MG cycle's
+// right-hand side in one pass, the normalization left to the caller).
+void launch_maxpy_norm32_to2f(const double* y, const VecList32& X, int nv, const double* coef, const CoefMul& mul,
+                              double sign, long long n, double osc, float* o1, float* o2, double* partials,
+                              double* norm2, hipStream_t s);
 // y = a x
 void launch_scale_flat(double* y, const double* x, double a, long long n, hipStream_t s);
 // y = a x and yo = (TO) y, flat over n elements (yo: a field of the same layout)

@@ -743,7 +743,14 @@ int krylov_alloc(ela_hier* h, bool b32) {
 // the dot products, the updates and the solution stay FP64): a basis vector
 // rounded to FP32 perturbs the Arnoldi relation by ~1e-7 of the restart's
 // residual, well below the reduction one restart asks for, and the vectors
-// are the MG cycle's FP32 right-hand sides as they stand.
+// are the MG cycle's FP32 right-hand sides as they stand.  The Gram-Schmidt
+// update writes the new FP32 vector (and the next cycle's right-hand side)
+// itself, before its norm is known: scaled by a power of two from the norm's
+// estimate |w|^2 - |h|^2, so it is stored as s v with v of unit norm and s
+// (near 1) kept on the host.  Its dot products are then divided by s and its
+// update coefficients by s^2; the cycle's z = M(s v) is a preconditioned
+// direction like any other (the flexible variant takes any), and A z enters
+// the Hessenberg column as computed.
 template <typename T>
 int solve_fgmres(ela_hier* h, const double* b, double* u, double tol_rel, double tol_abs, int max_iter,
                  ela_stats* st) {
@@ -772,6 +779,7 @@ int solve_fgmres(ela_hier* h, const double* b, double* u, double tol_rel, double
     int total = 0, status = ELA_OK, restarts = 0;
     double best = r0;
     std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1);
+    std::vector<double> sv(m + 1, 1.0);  // FP32 basis: stored vector q = sv[q] x the unit vector
     while (rn > target) {
         if (total >= max_iter) {
             status = fail(ELA_ERR_NONCONVERGED, "max_iter reached at " + std::to_string(rn / r0) + " relative");
@@ -786,6 +794,7 @@ int solve_fgmres(ela_hier* h, const double* b, double* u, double tol_rel, double
         // V0 and, in the same pass, the first cycle's right-hand side (T) V0
         if constexpr (B32) launch_scale_to2f(h->kr, 1.0 / beta, h->V32[0], *lb<T>(v0).res, n, s);
         else launch_scale_flat_to<T>(h->V[0], h->kr, 1.0 / beta, *lb<T>(v0).res, n, s);
+        sv[0] = 1.0;
         std::fill(g.begin(), g.end(), 0.0);
         g[0] = beta;
         int jend = 0;
@@ -822,24 +831,52 @@ int solve_fgmres(ela_hier* h, const double* b, double* u, double tol_rel, double
             std::vector<double> h1 = host_vec(j + 2);
             const double wn0 = std::sqrt(h1[j + 1]);
             h1.resize(j + 1);
+            // the FP32 basis' scaled vectors: unit-vector dot products, and the
+            // update's coefficient multipliers 1 / s^2 (1 for a unit vector)
+            CoefMul mul = coef_ones();
+            if constexpr (B32)
+                for (int q = 0; q <= j; ++q) {
+                    h1[q] /= sv[q];
+                    mul.f[q] = 1.0 / (sv[q] * sv[q]);
+                }
             std::vector<double> h2(j + 1, 0.0);
             double hn2 = wn0 * wn0;
             for (int q = 0; q <= j; ++q) hn2 -= h1[q] * h1[q];
             double* const norm2 = h->kdev + 2 * (m + 2);  // beyond the coefficients the update reads
+            // the FP32 vector's power-of-two scale, from the estimate of its norm
+            // (the squared norm of w after the first pass with a second pass)
+            auto pow2_inv = [](double e2, double fallback2) {
+                const double e = std::sqrt(e2 > 0 ? e2 : fallback2);
+                return e > 0 && std::isfinite(e) ? std::exp2(-std::floor(std::log2(e))) : 1.0;
+            };
+            double osc = 1.0;
+            float* const vnext = B32 ? h->V32[j + 1] : nullptr;
             if (!(hn2 > kReorth2 * wn0 * wn0)) {
                 if constexpr (B32) {
-                    launch_maxpy32(w, Vl32, j + 1, h->kdev, -1.0, n, s);
+                    launch_maxpy32(w, Vl32, j + 1, h->kdev, mul, -1.0, n, s);
                     launch_mdot_self(w, Vl32, j + 1, n, h->kpart, h->kdev, s);
-                    launch_maxpy_norm32(w, Vl32, j + 1, h->kdev, -1.0, n, h->kpart, norm2, s);
+                    std::vector<double> r2 = host_vec(j + 2);
+                    double e2 = r2[j + 1];
+                    for (int q = 0; q <= j; ++q) e2 -= (r2[q] / sv[q]) * (r2[q] / sv[q]);
+                    osc = pow2_inv(e2, r2[j + 1]);
+                    launch_maxpy_norm32_to2f(w, Vl32, j + 1, h->kdev, mul, -1.0, n, osc, vnext, *lb<T>(v0).res,
+                                             h->kpart, norm2, s);
+                    h2 = host_vec(j + 1);
+                    for (int q = 0; q <= j; ++q) h2[q] /= sv[q];
                 } else {
                     launch_maxpy(w, Vl, j + 1, h->kdev, -1.0, n, s);
                     launch_mdot(w, Vl, j + 1, n, h->kpart, h->kdev, s);
                     launch_maxpy_norm(w, Vl, j + 1, h->kdev, -1.0, n, h->kpart, norm2, s);
+                    h2 = host_vec(j + 1);
                 }
-                h2 = host_vec(j + 1);
             } else {
-                if constexpr (B32) launch_maxpy_norm32(w, Vl32, j + 1, h->kdev, -1.0, n, h->kpart, norm2, s);
-                else launch_maxpy_norm(w, Vl, j + 1, h->kdev, -1.0, n, h->kpart, norm2, s);
+                if constexpr (B32) {
+                    osc = pow2_inv(hn2, wn0 * wn0);
+                    launch_maxpy_norm32_to2f(w, Vl32, j + 1, h->kdev, mul, -1.0, n, osc, vnext, *lb<T>(v0).res,
+                                             h->kpart, norm2, s);
+                } else {
+                    launch_maxpy_norm(w, Vl, j + 1, h->kdev, -1.0, n, h->kpart, norm2, s);
+                }
             }
             double hn2_dev = 0.0;
             HIPCHK(hipMemcpyAsync(&hn2_dev, norm2, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -848,11 +885,10 @@ int solve_fgmres(ela_hier* h, const double* b, double* u, double tol_rel, double
             ++total;
             for (int q = 0; q <= j; ++q) H[(size_t)q * m + j] = h1[q] + h2[q];
             H[(size_t)(j + 1) * m + j] = hn;
-            // v_{j+1}, and the next cycle's right-hand side (T) v_{j+1} in the same pass
-            if (hn > 0) {
-                if constexpr (B32) launch_scale_to2f(w, 1.0 / hn, h->V32[j + 1], *lb<T>(v0).res, n, s);
-                else launch_scale_flat_to<T>(h->V[j + 1], w, 1.0 / hn, *lb<T>(v0).res, n, s);
-            }
+            // v_{j+1}: the FP32 basis holds it already as osc hn v_{j+1} (and the
+            // next cycle's right-hand side with it); FP64 normalizes it here
+            if constexpr (B32) sv[j + 1] = osc * hn;
+            else if (hn > 0) launch_scale_flat_to<T>(h->V[j + 1], w, 1.0 / hn, *lb<T>(v0).res, n, s);
             // Givens rotations
             for (int q = 0; q < j; ++q) {
                 const double a = H[(size_t)q * m + j], c = H[(size_t)(q + 1) * m + j];
