@@ -659,6 +659,7 @@ int solve(ela_amr* a, double tol_rel, double tol_abs, int max_iter, ela_stats* s
     if (m > 0)
         if (int rc = krylov_alloc(a)) return rc;
     std::vector<double> H((size_t)(m + 1) * std::max(m, 1)), cs(m), sn(m), g(m + 1);
+    std::vector<double> sv(m + 1, 1.0);  // FP32 basis: stored vector q = sv[q] x the unit vector
     while (rn > target) {
         if (total >= max_iter) {
             status = afail(ELA_ERR_NONCONVERGED, "max_iter reached at " + std::to_string(rn / r0) + " relative");
@@ -679,6 +680,7 @@ int solve(ela_amr* a, double tol_rel, double tol_abs, int max_iter, ela_stats* s
             if (!(beta > 0)) break;
             if (b32) launch_scale_to2f(a->r, 1.0 / beta, a->V32[0], nullptr, n, s);
             else launch_scale_flat(a->V[0], a->r, 1.0 / beta, n, s);
+            sv[0] = 1.0;
             std::fill(g.begin(), g.end(), 0.0);
             g[0] = beta;
             int jend = 0;
@@ -703,23 +705,47 @@ int solve(ela_amr* a, double tol_rel, double tol_abs, int max_iter, ela_stats* s
                 std::vector<double> h1 = host_vec(a, j + 2), h2(j + 1, 0.0);
                 const double wn0 = std::sqrt(h1[j + 1]);
                 h1.resize(j + 1);
+                // the FP32 basis holds s v (the single box's solve_fgmres): unit-vector dots, coefficients / s^2
+                CoefMul mul = coef_ones();
+                if (b32)
+                    for (int q = 0; q <= j; ++q) {
+                        h1[q] /= sv[q];
+                        mul.f[q] = 1.0 / (sv[q] * sv[q]);
+                    }
                 double hn2 = wn0 * wn0;
                 for (int q = 0; q <= j; ++q) hn2 -= h1[q] * h1[q];
                 double* const norm2 = a->kdev + 2 * (kMaxKrylov + 2);  // beyond the coefficients
+                auto pow2_inv = [](double e2, double fallback2) {
+                    const double e = std::sqrt(e2 > 0 ? e2 : fallback2);
+                    return e > 0 && std::isfinite(e) ? std::exp2(-std::floor(std::log2(e))) : 1.0;
+                };
+                double osc = 1.0;
                 if (!(hn2 > kReorth2 * wn0 * wn0)) {
                     if (b32) {
-                        launch_maxpy32(w, Vl32, j + 1, a->kdev, coef_ones(), -1.0, n, s);
+                        launch_maxpy32(w, Vl32, j + 1, a->kdev, mul, -1.0, n, s);
                         launch_mdot_self(w, Vl32, j + 1, n, a->kpart, a->kdev, s);
-                        launch_maxpy_norm32(w, Vl32, j + 1, a->kdev, coef_ones(), -1.0, n, a->kpart, norm2, s);
+                        std::vector<double> r2 = host_vec(a, j + 2);
+                        double e2 = r2[j + 1];
+                        for (int q = 0; q <= j; ++q) e2 -= (r2[q] / sv[q]) * (r2[q] / sv[q]);
+                        osc = pow2_inv(e2, r2[j + 1]);
+                        launch_maxpy_norm32_to2f(w, Vl32, j + 1, a->kdev, mul, -1.0, n, osc, a->V32[j + 1], nullptr,
+                                                 a->kpart, norm2, s);
+                        h2 = host_vec(a, j + 1);
+                        for (int q = 0; q <= j; ++q) h2[q] /= sv[q];
                     } else {
                         launch_maxpy(w, Vl, j + 1, a->kdev, -1.0, n, s);
                         launch_mdot(w, Vl, j + 1, n, a->kpart, a->kdev, s);
                         launch_maxpy_norm(w, Vl, j + 1, a->kdev, -1.0, n, a->kpart, norm2, s);
+                        h2 = host_vec(a, j + 1);
                     }
-                    h2 = host_vec(a, j + 1);
                 } else {
-                    if (b32) launch_maxpy_norm32(w, Vl32, j + 1, a->kdev, coef_ones(), -1.0, n, a->kpart, norm2, s);
-                    else launch_maxpy_norm(w, Vl, j + 1, a->kdev, -1.0, n, a->kpart, norm2, s);
+                    if (b32) {
+                        osc = pow2_inv(hn2, wn0 * wn0);
+                        launch_maxpy_norm32_to2f(w, Vl32, j + 1, a->kdev, mul, -1.0, n, osc, a->V32[j + 1], nullptr,
+                                                 a->kpart, norm2, s);
+                    } else {
+                        launch_maxpy_norm(w, Vl, j + 1, a->kdev, -1.0, n, a->kpart, norm2, s);
+                    }
                 }
                 double hn2_dev = 0.0;
                 ELA_HIPCHK(hipMemcpyAsync(&hn2_dev, norm2, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -728,10 +754,8 @@ int solve(ela_amr* a, double tol_rel, double tol_abs, int max_iter, ela_stats* s
                 ++total;
                 for (int q = 0; q <= j; ++q) H[(size_t)q * m + j] = h1[q] + h2[q];
                 H[(size_t)(j + 1) * m + j] = hn;
-                if (hn > 0) {
-                    if (b32) launch_scale_to2f(w, 1.0 / hn, a->V32[j + 1], nullptr, n, s);
-                    else launch_scale_flat(a->V[j + 1], a->V[j + 1], 1.0 / hn, n, s);
-                }
+                if (b32) sv[j + 1] = osc * hn;  // the update wrote osc hn v_{j+1}
+                else if (hn > 0) launch_scale_flat(a->V[j + 1], a->V[j + 1], 1.0 / hn, n, s);
                 for (int q = 0; q < j; ++q) {
                     const double x = H[(size_t)q * m + j], y = H[(size_t)(q + 1) * m + j];
                     H[(size_t)q * m + j] = cs[q] * x + sn[q] * y;
