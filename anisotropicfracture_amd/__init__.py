@@ -33,7 +33,7 @@ F64, F32 = 0, 1
 SOLVE_FP64, SOLVE_MIXED, ALLOC_BOTH = 0, 1, 2
 CYCLE_F, CYCLE_V = 0, 1
 OPT_MARCH, OPT_KZ, OPT_XCD, OPT_MASK, OPT_GRAPH, OPT_FOLD, OPT_ROWS, OPT_PREFETCH, OPT_GJ = 0, 1, 2, 3, 4, 5, 6, 7, 8
-OPT_GRAIN_DIAG = 9
+OPT_GRAIN_DIAG, OPT_PROLONG = 9, 10
 
 ERRORS = {1: "BADARG", 2: "OOM", 3: "HIP", 4: "NONCONVERGED", 5: "NAN", 6: "STATE"}
 
@@ -1093,7 +1093,8 @@ class Batch:
         replayed from a captured HIP graph while the active set is unchanged, default off) or OPT_GJ (the members'
         coarsest Gauss-Jordan in one launch a column, default 1, or two; the same bits) or OPT_GRAIN_DIAG (the
         marching tiles of members without factors take the interior Jacobi diagonal per grain, default 1, or read
-        the stored field; the same bits)."""
+        the stored field; the same bits) or OPT_PROLONG (prolongation with one thread per coarse node, default 1, or
+        per fine node; the same bits)."""
         _check(lib().ela_batch_set_option(self.ptr, option, int(value)))
 
     def graph_stats(self):

@@ -327,6 +327,40 @@ __global__ __launch_bounds__(TB) void kb_prolong(const BoxDesc* __restrict__ Df,
     }
 }
 
+// The same with one thread per coarse node (I, J, K) of the coarse level's
+// block list (bmap): its up to 8 fine nodes (2I + a, 2J + b, 2K + c), each
+// prolong_node's expression from the thread's coarse corner (the bits of
+// kb_prolong), so each coarse value is loaded once for 8 fine nodes.
+template <typename T>
+__global__ __launch_bounds__(TB) void kb_prolong_oct(const BoxDesc* __restrict__ Df, const BoxDesc* __restrict__ Dc,
+                                                     const int* __restrict__ bmap, int src, int dst, int add) {
+    const int box = bmap[blockIdx.x];
+    const BoxDesc& f = Df[box];
+    const BoxDesc& c = Dc[box];
+    const Lay Lf = f.L, Lc = c.L;
+    int I, J, K;
+    long long oc;
+    if (!box_node(c, Lc, blockIdx.x, I, J, K, oc)) return;
+    const T* C = (const T*)c.f[src];
+    T* out = (T*)f.f[dst];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const T* cp = C + p * Lc.cs + oc;
+#pragma unroll
+        for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+            for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+                for (int di = 0; di < 2; ++di) {
+                    const int i = 2 * I + di, j = 2 * J + dj, k = 2 * K + dk;
+                    if (i >= Lf.nx || j >= Lf.ny || k >= Lf.nz) continue;
+                    const T v = prolong_node<T>(Lc, i, j, k, cp);
+                    T* y = out + p * Lf.cs + i + Lf.px * j + Lf.pl * k;
+                    *y = add ? (*y + v) : v;
+                }
+    }
+}
+
 // f[y] += f[x]
 template <typename T>
 __global__ __launch_bounds__(TB) void kb_axpy(const BoxDesc* __restrict__ D, const int* __restrict__ bmap, int x, int y) {
@@ -1219,6 +1253,7 @@ struct ela_batch {
     bool graph_on = false;
     bool gj_one = true;  // ELA_OPT_GJ: the members' coarsest Gauss-Jordan in one launch a column
     bool gdiag = true;   // ELA_OPT_GRAIN_DIAG: the tiles' interior Jacobi diagonal per grain
+    bool prolong_oct = true;  // prolongation with one thread per coarse node (ELA_OPT_PROLONG)
     long long list_epoch = 0;
     hipGraphExec_t gexec = nullptr;
     long long gexec_epoch = -1;
@@ -1618,8 +1653,18 @@ void restrict_to(ela_batch* B, int g, int count, int src, int dst) {  // level g
                            src, dst);
 }
 
+// Prolongation onto level g: one thread per coarse node (kb_prolong_oct) over
+// the coarse list's blocks of the same members (measured on MI355X, 512 RVEs
+// batched: mixed 350 -> 364 solves/s, FP64 216 -> 220), or one per fine node
+// (ELA_OPT_PROLONG 0).
 template <typename T>
 void prolong_to(ela_batch* B, int g, int count, int src, int dst, int add) {  // level g + 1 -> g
+    const int nbc = nblocks(B, g + 1, count);
+    if (B->prolong_oct && nbc) {
+        hipLaunchKernelGGL((kb_prolong_oct<T>), dim3(nbc), dim3(TB), 0, B->stream, LD(B, g), LD(B, g + 1),
+                           LM(B, g + 1), src, dst, add);
+        return;
+    }
     const int nb = nblocks(B, g, count);
     if (nb)
         hipLaunchKernelGGL((kb_prolong<T>), dim3(nb), dim3(TB), 0, B->stream, LD(B, g), LD(B, g + 1), LM(B, g), src,
@@ -2634,6 +2679,8 @@ int ela_batch_set_option(ela_batch* B, int option, int value) {
         B->gj_one = value != 0;
     } else if (option == ELA_OPT_GRAIN_DIAG) {
         B->gdiag = value != 0;
+    } else if (option == ELA_OPT_PROLONG) {
+        B->prolong_oct = value != 0;
     } else {
         return fail(ELA_ERR_BADARG, "unknown option");
     }

@@ -378,7 +378,8 @@ enum {
     ELA_OPT_ROWS = 6,
     ELA_OPT_PREFETCH = 7,
     ELA_OPT_GJ = 8,
-    ELA_OPT_GRAIN_DIAG = 9
+    ELA_OPT_GRAIN_DIAG = 9,
+    ELA_OPT_PROLONG = 10
 };
 int ela_set_option(ela_hier* h, int option, int value);
 
@@ -711,7 +712,10 @@ int ela_server_stop(ela_server* s, int drain);
  * default) or in pivot/eliminate launch pairs (0); the same bits.
  * ELA_OPT_GRAIN_DIAG: the marching tiles of members without factors take an
  * interior node's Jacobi diagonal from its grain (1, default) or read the
- * stored diagonal field (0); the same bits. */
+ * stored diagonal field (0); the same bits.
+ * ELA_OPT_PROLONG: prolongation with one thread per coarse node, writing its
+ * up to 8 fine nodes (1, default), or one thread per fine node (0); the same
+ * bits.  Batches only. */
 int ela_batch_set_option(ela_batch* b, int option, int value);
 /* Captures and replays of the batch's F-cycle graph since its creation (no
  * reference counterpart). */

@@ -434,14 +434,16 @@ def test_batched_gauss_jordan_one_launch_a_column_gives_the_launch_pair_bits(mod
 
 @pytest.mark.parametrize("mode", [E.SOLVE_FP64, E.SOLVE_MIXED])
 def test_batched_per_grain_diagonal_keeps_the_bits(mode):
-    """ELA_OPT_GRAIN_DIAG on a batch: the marching tiles of members without factors take an interior node's
-    Jacobi diagonal from its grain (per member and level), or read the stored diagonal field; the solves
-    agree bit for bit.  A member with a crack factor (which reads the field either way) rides along."""
+    """ELA_OPT_GRAIN_DIAG and ELA_OPT_PROLONG on a batch: the marching tiles of members without factors take an
+    interior node's Jacobi diagonal from its grain (per member and level), or read the stored diagonal field;
+    the prolongation runs one thread per coarse node (its 8 fine nodes) or one per fine node.  The solves agree
+    bit for bit.  A member with a crack factor (which reads the field either way) rides along."""
     probs = [problem(84, 32, 6), problem(85, 40, 9, crack=True), problem(86, 48, 7)]
     sols = []
     for gd in (0, 1):
         b = E.Batch(mode)
         b.set_option(E.OPT_GRAIN_DIAG, gd)
+        b.set_option(E.OPT_PROLONG, gd)
         rs = [make(b, p) for p in probs]
         b.prepare(rs)
         sts = b.solve(rs, tol_rel=1e-10, max_cycles=80)
