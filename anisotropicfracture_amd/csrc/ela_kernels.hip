@@ -237,6 +237,25 @@ __global__ __launch_bounds__(256) void k_jacobi0(Lay L, const T* __restrict__ dg
     }
 }
 
+// The same on a grain-form level: an interior node's diagonal from its
+// grain's entry of dgt (3 values a grain, the value the stored field holds
+// there: the same bits), the boundary nodes' from dg.
+template <typename T>
+__global__ __launch_bounds__(256) void k_jacobi0_grain(Lay L, const unsigned short* __restrict__ gid,
+                                                       const T* __restrict__ dgt, const T* __restrict__ dg,
+                                                       const T* __restrict__ b, T* __restrict__ xout, T omega) {
+    int i, j, k;
+    if (!node_index(L, i, j, k)) return;
+    const long long o = (long long)i + L.px * j + L.pl * k;
+    const bool interior = i > 0 && i < L.nx - 1 && j > 0 && j < L.ny - 1 && k > 0 && k < L.nz - 1;
+    const int g3 = interior ? 3 * (int)gid[o] : 0;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const long long q = p * L.cs + o;
+        xout[q] = jacobi_first(omega, b[q], interior ? dgt[g3 + p] : dg[q]);
+    }
+}
+
 template <int K, int DIM>
 __global__ __launch_bounds__(256) void k_diag(Lay L, const double* __restrict__ C, double* __restrict__ d64,
                                               float* __restrict__ d32, BCTab bc, double hx, double hy, double hz,
@@ -1403,8 +1422,12 @@ void launch_node_list(const Lay& L, const unsigned char* sm, int* cnt, long long
 }
 
 template <typename T>
-void launch_jacobi0(const Lay& L, const T* dg, const T* b, T* xout, double omega, hipStream_t s) {
-    if (L.dim == 3) hipLaunchKernelGGL((k_jacobi0<T, 3>), node_grid(L), dim3(BX, BY), 0, s, L, dg, b, xout, (T)omega);
+void launch_jacobi0(const Lay& L, const T* dg, const T* b, T* xout, double omega, hipStream_t s,
+                    const unsigned short* gid, const T* dgt) {
+    if (L.dim == 3 && gid && dgt)
+        hipLaunchKernelGGL((k_jacobi0_grain<T>), node_grid(L), dim3(BX, BY), 0, s, L, gid, dgt, dg, b, xout,
+                           (T)omega);
+    else if (L.dim == 3) hipLaunchKernelGGL((k_jacobi0<T, 3>), node_grid(L), dim3(BX, BY), 0, s, L, dg, b, xout, (T)omega);
     else hipLaunchKernelGGL((k_jacobi0<T, 2>), node_grid(L), dim3(BX, BY), 0, s, L, dg, b, xout, (T)omega);
 }
 
@@ -2120,8 +2143,10 @@ template void launch_jacobi<double>(const Lay&, int, const double*, const double
                                     double*, const BCTab&, const double*, int, double, hipStream_t, const PlaneOpt&);
 template void launch_jacobi<float>(const Lay&, int, const float*, const float*, const float*, const float*, float*,
                                    const BCTab&, const double*, int, double, hipStream_t, const PlaneOpt&);
-template void launch_jacobi0<double>(const Lay&, const double*, const double*, double*, double, hipStream_t);
-template void launch_jacobi0<float>(const Lay&, const float*, const float*, float*, double, hipStream_t);
+template void launch_jacobi0<double>(const Lay&, const double*, const double*, double*, double, hipStream_t,
+                                     const unsigned short*, const double*);
+template void launch_jacobi0<float>(const Lay&, const float*, const float*, float*, double, hipStream_t,
+                                    const unsigned short*, const float*);
 template void launch_restrict<double>(const Lay&, const Lay&, int, const double*, double*, hipStream_t);
 template void launch_restrict<float>(const Lay&, const Lay&, int, const float*, float*, hipStream_t);
 template void launch_prolong<double>(const Lay&, const Lay&, int, const double*, double*, int, hipStream_t);

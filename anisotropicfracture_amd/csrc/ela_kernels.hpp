@@ -57,7 +57,8 @@ void launch_node_list(const Lay& L, const unsigned char* sm, int* cnt, long long
 void launch_barrier_probe(int blocks, unsigned* cnt, int* err, int rounds, hipStream_t s);
 
 template <typename T>
-void launch_jacobi0(const Lay& L, const T* dg, const T* b, T* xout, double omega, hipStream_t s);
+void launch_jacobi0(const Lay& L, const T* dg, const T* b, T* xout, double omega, hipStream_t s,
+                    const unsigned short* gid = nullptr, const T* dgt = nullptr);
 
 void launch_diag(const Lay& L, int kind, const double* C, double* d64, float* d32, const BCTab& bc,
                  const double h[3], unsigned long long* flag, hipStream_t s);

@@ -270,6 +270,10 @@ template <typename T>
 void grain_jacobi_to64(ela_hier* h, Level& v, const T* dg, const T* xin, const T* b, const T* hold, double* out64,
                        hipStream_t s);
 bool grain_on(ela_hier* h, const Level& v);
+bool grain_levels(const ela_hier* h);
+const unsigned short* level_gid(const ela_hier* h, const Level& v);
+template <typename T>
+const T* level_dgt(const ela_hier* h, const Level& v);
 // part: scratch for the per-block maxima of the norm (null: the hierarchy's own).
 template <typename T, typename TO>
 void A_residual(ela_hier* h, Level& v, const T* C, const T* x, const T* b, T* r, TO* rs, const double* scale,
@@ -299,7 +303,12 @@ void smooth_call(ela_hier* h, int l, bool zero_start, double* out64 = nullptr) {
     Level& v = h->lev[l];
     LB<T> b = lb<T>(v);
     const bool t0 = (l == 0);
-    if (zero_start) launch_jacobi0<T>(v.L, b.dg, *b.res, *b.tmp, h->omega, h->stream);
+    if (zero_start) {
+        // a grain-form level's interior diagonal per grain (the field's values there)
+        const bool pg = grain_levels(h) && h->tile_dgt && h->dim == 3;
+        launch_jacobi0<T>(v.L, b.dg, *b.res, *b.tmp, h->omega, h->stream, pg ? level_gid(h, v) : nullptr,
+                          pg ? level_dgt<T>(h, v) : nullptr);
+    }
     else {
         const int e = t0 ? ev_begin(h) : -1;
         A_jacobi<T>(h, v, b.C, b.dg, *b.cor, *b.res, *b.tmp, h->stream);

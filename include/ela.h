@@ -362,7 +362,8 @@ int ela_set_krylov(ela_hier* h, int restart);
  *                0: a pivot launch and an elimination launch a column.  Same
  *                pivots and bits either way.  Hierarchies only.
  * ELA_OPT_GRAIN_DIAG 1 (default): the marching tiles of a grain-form level,
- *                and the grain-table node kernels of its coarse levels, take
+ *                the grain-table node kernels of its coarse levels and the
+ *                first Jacobi sweep from zero on every level take
  *                an interior node's Jacobi diagonal from its grain (3 values a
  *                grain and level; the tiles stage them in LDS with the table:
  *                sum_b C_{p b p b} (-2 / h_b^2) depends on the node's own
